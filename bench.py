@@ -1,0 +1,119 @@
+"""Coupled fluid-filament step on one MI355X: MLUPS of the 1024^2 plain case.
+
+A step is one full coupled step (steric FFT field, 8 PBD cycles on 256 x 64
+filaments, 5 immersed-boundary iterations with the K = 4 stencil, fused D2Q9
+TRT + Guo collide-stream) over the device-resident 1024^2 lattice, replayed
+from a HIP graph.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line.  Also reports the fused LBM kernel's roofline fraction
+(algorithmic bytes / HIP-event duration) and the CPU restatement's rate on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from dataclasses import replace
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "coupled-step MLUPS (% HBM peak); batched evals/s + p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+LBM_BYTES_PER_CELL = 80  # read f 36 + write f 36 + write u* 8
+
+
+def cpu_baseline(wl, steps):
+    """The numpy FP32 restatement timed on this host over `steps` steps (after
+    one warm-up step); returns MLUPS."""
+    import numpy as np
+    from oracle import step_ref as R
+    from tests.helpers import oracle_params, oracle_state
+    p = oracle_params(wl)
+    s = oracle_state(wl)
+    s = R.step(replace(s, bending_rest_angles=wl.rest_angles_at(0)), wl.dt, p)
+    t0 = time.perf_counter()
+    for t in range(1, steps + 1):
+        s = R.step(replace(s, bending_rest_angles=wl.rest_angles_at(t)), wl.dt, p)
+    dt = time.perf_counter() - t0
+    h, w = wl.grid_shape
+    return h * w * steps / dt / 1e6, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=100)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "
+                                   "part of it (instances shard by whole lattice, see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from microcosmos_amd import PBD_SCHEME, Rollout
+    from microcosmos_amd import workloads as W
+    from tests.helpers import device_state
+
+    dev = torch.device("cuda", 0)
+    wl = W.plain_1024()
+    nodes, edges, fields = device_state(wl, dev)
+    cfg = replace(PBD_SCHEME, **wl.config_overrides)
+    ro = Rollout(nodes, edges, fields, cfg, wl.dt,
+                 actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
+    use_graph = not args.no_graph
+    ro.run(args.warmup, use_graph=use_graph)
+    torch.cuda.synchronize()
+    print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ro.run(args.steps, use_graph=use_graph)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    finite = bool(torch.isfinite(ro.f_grid).all().item() and torch.isfinite(ro.position).all().item())
+    h, w = wl.grid_shape
+    ms = elapsed / args.steps * 1e3
+    mlups = h * w * args.steps / elapsed / 1e6
+    # roofline of the dominant kernel (fused LBM collide-stream), HIP events
+    lbm_ms, step_ms_ev = ro.profile(args.profile_steps)
+    torch.cuda.synchronize()
+    lbm_bytes = LBM_BYTES_PER_CELL * h * w
+    achieved = lbm_bytes / (lbm_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_lbm", "avg_kernel_ms": round(lbm_ms, 5),
+                "bytes_per_launch": lbm_bytes}
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        v, t = cpu_baseline(wl, args.cpu_steps)
+        cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same 1024^2 256x64 workload after one "
+                         f"warm-up step, numpy FP32 restatement ({t:.1f} s)",
+               "host_cpus": len(os.sched_getaffinity(0))}
+    out = {"metric": METRIC, "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": wl.name, "grid": [h, w], "filaments": 256,
+                      "nodes_per_filament": 64, "pbd_cycles": 8, "ibm_kernel_size": 4,
+                      "ibm_iterations": 5, "steric_sigma": 2.0, "tau_plus": 0.6,
+                      "lambda_trt": 0.1875, "hip_graph": use_graph, "finite": finite},
+           "roofline": roofline, "cpu_baseline": cpu,
+           "coupled_steps_per_s": round(args.steps / elapsed, 2),
+           "step_ms_events": round(step_ms_ev, 5)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,174 @@
+/*
+ * microcosmos_hip.h -- C ABI of the MI355X (gfx950) coupled fluid-filament step.
+ *
+ * Drop-in boundary for the reference's step API (ProjectProgramAMark/microcosmos):
+ * plain pointers and sizes, no framework types.  Device pointers are HIP device
+ * memory; `stream` is a hipStream_t passed as void*.  Every function returns
+ * an mc_status; mc_last_error() gives a message for the calling thread.
+ *
+ * Array layouts are the reference's pytrees (structs/nodes.py:5-15,
+ * structs/edges.py:6-21, structs/fields.py:5-16), all float32 unless noted:
+ *   position, velocity, debug_vector   (N, 2)  x, y interleaved
+ *   pairs                              (E, 2)  int32 (src, tgt)
+ *   theta, rest_lengths                (E,)
+ *   bending_pairs                      (B, 2)  int32 (edge_in, edge_out)
+ *   bending_rest_angles, stiffness     (B,)
+ *   steric                             (H, W)          Fields.grid_shape = (H, W)
+ *   fluid_velocity                     (2, Hf, Wf)     lattice shape = f_grid.shape[1:]
+ *   f_grid                             (9, Hf, Wf)     D2Q9 order of fluid.py:15-18
+ */
+#ifndef MICROCOSMOS_HIP_H
+#define MICROCOSMOS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mc_status {
+  MC_OK = 0,
+  MC_ERR_BAD_SHAPE = 1,       /* inconsistent or unsupported sizes */
+  MC_ERR_OOM = 2,             /* device allocation failed */
+  MC_ERR_NON_FINITE = 3,      /* reserved: non-finite input detected */
+  MC_ERR_UNSUPPORTED = 4,     /* option not implemented on the device path */
+  MC_ERR_INVALID = 5,         /* null or aliasing argument that is not allowed */
+  MC_ERR_HIP = 6              /* HIP runtime error (message has the detail) */
+} mc_status;
+
+typedef enum mc_solver { MC_SOLVER_PBD = 0, MC_SOLVER_COSSERAT = 1 } mc_solver;
+
+/* Every numeric field of ConstraintSolverConfig (solver/config.py:9-71), in
+ * double precision: constants derived from them (omega+, 1/dt^2, 2 sigma^2, ...)
+ * are computed in double and rounded to float32 the way the reference's
+ * Python-float arithmetic is. */
+typedef struct mc_params {
+  int32_t solver_type;            /* mc_solver (config.py:28) */
+  int32_t cycles_per_step;        /* config.py:15 */
+  double stiffness_stretch;       /* config.py:16 */
+  double stiffness_shear;         /* config.py:17 */
+  double damping;                 /* config.py:20 */
+  int32_t enable_fluid;           /* config.py:23 */
+  double viscosity;               /* config.py:24 */
+  double max_fluid_velocity;      /* config.py:25 */
+  double config_dt;               /* config.py:32 (Cosserat inertia) */
+  int32_t ibm_iterations;         /* config.py:35 */
+  double synthetic_node_width;    /* config.py:36 (only 0.0 supported) */
+  int32_t ibm_kernel_size;        /* config.py:37, 1..6 */
+  double ibm_relaxation;          /* config.py:38 */
+  double node_mass;               /* config.py:45 */
+  double lambda_trt;              /* config.py:49 */
+  int32_t enable_force_clamp;     /* config.py:59 */
+  int32_t enable_steric;          /* config.py:63 */
+  double steric_strength;         /* config.py:64 */
+  double steric_sigma;            /* config.py:65 */
+  int32_t steric_neighbor_skip;   /* config.py:68 */
+  double steric_scatter_value;    /* config.py:71 */
+} mc_params;
+
+/* Device pointers of one state.  `in` arrays are read, `out` arrays written.
+ * Node and edge arrays may alias between in and out; f_grid may not. */
+typedef struct mc_state {
+  float* position;
+  float* velocity;
+  float* debug_vector;
+  float* theta;
+  float* steric;
+  float* fluid_velocity;
+  float* f_grid;
+} mc_state;
+
+/* Per-step edge parameters (device pointers, read only). */
+typedef struct mc_edge_params {
+  const float* rest_lengths;          /* (E,) */
+  const float* bending_rest_angles;   /* (B,) */
+  const float* bending_stiffness;     /* (B,) */
+} mc_edge_params;
+
+/* Optional in-kernel actuation of the bending rest angles for rollouts:
+ * rest_b(t) = amp_b * sin(omega_b * t + phase_b), evaluated in double, with
+ * t = t0 + step index.  This is the caller's per-step rewrite of
+ * bending_rest_angles (gym/base.py:169-173) moved onto the device. */
+typedef struct mc_actuation {
+  const float* amp;
+  const float* omega;
+  const float* phase;
+  int64_t t0;
+} mc_actuation;
+
+typedef struct mc_plan mc_plan;
+
+/* Library version string. */
+const char* mc_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char* mc_last_error(void);
+
+/* Builds the device plan for one topology and grid (like an FFT plan): the
+ * per-component rod layout (graph.py:156-401 topology, host arrays), FFT
+ * twiddles, and all workspace.  grid_h/grid_w = Fields.grid_shape; fluid_h/
+ * fluid_w = f_grid.shape[1:]; max_kernel_size bounds ibm_kernel_size.
+ * Replaces the jit trace of simulate.step (simulate.py:37-38) for a shape. */
+mc_status mc_plan_create(mc_plan** plan, int32_t grid_h, int32_t grid_w, int32_t fluid_h,
+                         int32_t fluid_w, int32_t num_nodes, const int32_t* pairs_host,
+                         int32_t num_edges, const int32_t* bending_pairs_host, int32_t num_hinges,
+                         int32_t max_kernel_size);
+mc_status mc_plan_destroy(mc_plan* plan);
+/* Device bytes held by the plan (topology + workspace). */
+mc_status mc_plan_workspace_bytes(const mc_plan* plan, size_t* bytes);
+
+/* One coupled step: simulate.step (simulate.py:37-100).
+ * in -> out with dt; f_grid of out must differ from in. */
+mc_status mc_step(mc_plan* plan, const mc_params* params, const mc_state* in, const mc_state* out,
+                  const mc_edge_params* edges, double dt, void* stream);
+
+/* num_steps coupled steps in place on `state` (f_grid ping-pongs through the
+ * plan's second lattice buffer and ends in state->f_grid).  simulate.py:15-35.
+ * act may be NULL (then edges->bending_rest_angles is used every step).
+ * record_position (num_steps, N, 2) and record_fluid_velocity
+ * (num_steps, 2, Hf, Wf) may be NULL; when given they receive the per-step
+ * trajectory that simulate() returns.  use_graph != 0 captures one step in a
+ * HIP graph and replays it. */
+mc_status mc_rollout(mc_plan* plan, const mc_params* params, const mc_state* state,
+                     const mc_edge_params* edges, const mc_actuation* act, double dt,
+                     int64_t num_steps, float* record_position, float* record_fluid_velocity,
+                     int32_t use_graph, void* stream);
+
+/* Measurement: num_steps coupled steps in place (no graph, same kernels as
+ * mc_rollout) with HIP events on `stream` around every fused LBM
+ * collide-stream launch and every step; returns the average durations in ms.
+ * No counterpart in the reference (its only timer is utils.py:53-59). */
+mc_status mc_profile_step(mc_plan* plan, const mc_params* params, const mc_state* state,
+                          const mc_edge_params* edges, const mc_actuation* act, double dt,
+                          int64_t num_steps, double* lbm_ms_avg, double* step_ms_avg, void* stream);
+
+/* Sub-operators of the step, each a drop-in for one reference function. */
+
+/* utils.py:35-49 gaussian_filter on an (H, W) field (H = grid_h, W = grid_w). */
+mc_status mc_gaussian_filter(mc_plan* plan, const float* data, float* out, double sigma,
+                             void* stream);
+
+/* forces.py:41-81 compute_field_steric_force_corrected: steric (H, W) field and
+ * per-node force (N, 2) at `position` (N, 2).  density_out (H, W) may be NULL
+ * (the CIC deposit before filtering, forces.py:112-115). */
+mc_status mc_steric_field(mc_plan* plan, const mc_params* params, const float* position,
+                          float* steric_out, float* force_out, float* density_out, void* stream);
+
+/* fluid.py:202-386 immersed_boundary_interaction: returns node velocity,
+ * fluid_velocity (u*) and streamed f.  pos_out receives the wrapped positions
+ * (simulate.py:97); it may be NULL. */
+mc_status mc_ib_lbm_step(mc_plan* plan, const mc_params* params, const float* position,
+                         const float* velocity, const float* f_in, float* velocity_out,
+                         float* fluid_velocity_out, float* f_out, float* pos_out, double dt,
+                         void* stream);
+
+/* cycles x (pbd_rod_constraint | stable_cosserat_constraint) (pbd.py:14-91,
+ * stable_cosserat.py:13-119) on positions / thetas, in place.  `y` is the
+ * Cosserat inertial target (debug_vector), may be NULL for PBD. */
+mc_status mc_rod_solve(mc_plan* plan, const mc_params* params, float* position, float* theta,
+                       const float* y, const mc_edge_params* edges, int32_t cycles, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MICROCOSMOS_HIP_H */

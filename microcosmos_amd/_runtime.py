@@ -1,0 +1,195 @@
+"""Host runtime around the C ABI: plan cache, parameter packing, pointers.
+
+Plans (device topology layout + workspace, like FFT plans) are cached per
+(device, grid shape, lattice shape, node count, topology digest, max kernel
+size).  The topology digest is memoised on the identity and version counter
+of the ``pairs`` / ``bending_pairs`` tensors, so a rollout that keeps its
+Edges tensors never synchronises with the device to look a plan up.
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import threading
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import McActuation, McEdgeParams, McParams, McState, check
+
+KMAX = 6  # largest ibm_kernel_size the device path supports
+
+
+def stream_handle(device: Optional[torch.device] = None) -> int:
+    s = torch.cuda.current_stream(device)
+    return int(s.cuda_stream)
+
+
+def ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else int(t.data_ptr())
+
+
+def as_f32(t: torch.Tensor, device) -> torch.Tensor:
+    t = torch.as_tensor(t)
+    if t.device != device or t.dtype != torch.float32 or not t.is_contiguous():
+        t = t.to(device=device, dtype=torch.float32).contiguous()
+    return t
+
+
+def params_from_config(cfg) -> McParams:
+    """ConstraintSolverConfig -> mc_params (solver/config.py:9-71)."""
+    st = getattr(cfg, "solver_type", "pbd")
+    if st not in ("pbd", "cosserat"):
+        raise ValueError(f"solver_type {st!r}")
+    from .solver.pbd import pbd_rod_constraint
+    from .solver.stable_cosserat import stable_cosserat_constraint
+    bc = getattr(cfg, "bending_constraint", None)
+    if bc is not None and bc not in (pbd_rod_constraint, stable_cosserat_constraint):
+        raise NotImplementedError(
+            "a custom bending_constraint callable cannot run inside the fused device step; "
+            "use pbd_rod_constraint or stable_cosserat_constraint")
+    solver = 1 if st == "cosserat" else 0
+    p = McParams()
+    p.solver_type = solver
+    p.cycles_per_step = int(cfg.cycles_per_step)
+    p.stiffness_stretch = float(cfg.stiffness_stretch)
+    p.stiffness_shear = float(cfg.stiffness_shear)
+    p.damping = float(cfg.damping)
+    p.enable_fluid = int(bool(cfg.enable_fluid))
+    p.viscosity = float(cfg.viscosity)
+    p.max_fluid_velocity = float(cfg.max_fluid_velocity)
+    p.config_dt = float(getattr(cfg, "dt", 0.05))
+    p.ibm_iterations = int(getattr(cfg, "ibm_iterations", 5))
+    p.synthetic_node_width = float(getattr(cfg, "synthetic_node_width", 0.0))
+    p.ibm_kernel_size = int(getattr(cfg, "ibm_kernel_size", 3))
+    p.ibm_relaxation = float(getattr(cfg, "ibm_relaxation", 0.3))
+    p.node_mass = float(getattr(cfg, "node_mass", 1.0))
+    p.lambda_trt = float(getattr(cfg, "lambda_trt", 0.25))
+    p.enable_force_clamp = int(bool(getattr(cfg, "enable_force_clamp", False)))
+    p.enable_steric = int(bool(getattr(cfg, "enable_steric", False)))
+    p.steric_strength = float(getattr(cfg, "steric_strength", 1.0))
+    p.steric_sigma = float(getattr(cfg, "steric_sigma", 1.5))
+    p.steric_neighbor_skip = int(getattr(cfg, "steric_neighbor_skip", 5))
+    p.steric_scatter_value = float(getattr(cfg, "steric_scatter_value", 1.0))
+    return p
+
+
+class Plan:
+    """Owns one mc_plan handle."""
+
+    def __init__(self, grid_shape, fluid_shape, num_nodes, pairs: np.ndarray,
+                 bending_pairs: np.ndarray, kmax: int = KMAX):
+        lib = _lib.load()
+        self.grid_shape = tuple(int(v) for v in grid_shape)
+        self.fluid_shape = tuple(int(v) for v in fluid_shape)
+        self.num_nodes = int(num_nodes)
+        self.pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        self.bending_pairs = np.ascontiguousarray(bending_pairs, dtype=np.int32).reshape(-1, 2)
+        h = ctypes.c_void_p()
+        check(lib.mc_plan_create(
+            ctypes.byref(h), self.grid_shape[0], self.grid_shape[1], self.fluid_shape[0],
+            self.fluid_shape[1], self.num_nodes,
+            self.pairs.ctypes.data if self.pairs.size else None, self.pairs.shape[0],
+            self.bending_pairs.ctypes.data if self.bending_pairs.size else None,
+            self.bending_pairs.shape[0], int(kmax)))
+        self.handle = h
+        self._lib = lib
+
+    @property
+    def num_edges(self):
+        return self.pairs.shape[0]
+
+    @property
+    def num_hinges(self):
+        return self.bending_pairs.shape[0]
+
+    def workspace_bytes(self) -> int:
+        b = ctypes.c_size_t()
+        check(self._lib.mc_plan_workspace_bytes(self.handle, ctypes.byref(b)))
+        return int(b.value)
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.mc_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_lock = threading.Lock()
+_plans: dict = {}
+_topo_memo: dict = {}
+
+
+def _host_topology(t) -> tuple:
+    """(numpy int32 array, digest) of a topology array, memoised per tensor version."""
+    if isinstance(t, torch.Tensor):
+        key = (id(t), t._version, t.data_ptr(), tuple(t.shape))
+        hit = _topo_memo.get(key)
+        if hit is not None and hit[0] is t:
+            return hit[1], hit[2]
+        a = t.detach().to("cpu").numpy().astype(np.int32, copy=False)
+        a = np.ascontiguousarray(a.reshape(-1, 2))
+        d = hashlib.blake2b(a.tobytes(), digest_size=16).hexdigest() + str(a.shape)
+        if len(_topo_memo) > 256:
+            _topo_memo.clear()
+        _topo_memo[key] = (t, a, d)
+        return a, d
+    a = np.ascontiguousarray(np.asarray(t, dtype=np.int32).reshape(-1, 2))
+    return a, hashlib.blake2b(a.tobytes(), digest_size=16).hexdigest() + str(a.shape)
+
+
+def plan_for(num_nodes: int, pairs, bending_pairs, grid_shape, fluid_shape, device) -> Plan:
+    pa, pd = _host_topology(pairs)
+    ba, bd = _host_topology(bending_pairs)
+    dev = torch.device(device)
+    key = (dev.index if dev.index is not None else torch.cuda.current_device(),
+           tuple(grid_shape), tuple(fluid_shape), int(num_nodes), pd, bd)
+    with _lock:
+        p = _plans.get(key)
+        if p is None:
+            with torch.cuda.device(dev):
+                p = Plan(grid_shape, fluid_shape, num_nodes, pa, ba)
+            if len(_plans) > 64:
+                _plans.clear()
+            _plans[key] = p
+    return p
+
+
+def clear_plans():
+    with _lock:
+        _plans.clear()
+        _topo_memo.clear()
+
+
+def edge_params(edges, device):
+    rl = as_f32(edges.rest_lengths, device)
+    ra = as_f32(edges.bending_rest_angles, device)
+    kb = as_f32(edges.bending_stiffness, device)
+    ep = McEdgeParams(ptr(rl), ptr(ra), ptr(kb))
+    return ep, (rl, ra, kb)
+
+
+def rod_iterations(carry, config, grid_shape, cycles: int, solver: str):
+    """`cycles` rod-constraint iterations on (nodes, edges) via mc_rod_solve."""
+    from dataclasses import replace
+    nodes, edges = carry
+    dev = nodes.position.device
+    lib = _lib.load()
+    N = nodes.position.shape[0]
+    plan = plan_for(N, edges.pairs, edges.bending_pairs, grid_shape, grid_shape, dev)
+    p = params_from_config(config)
+    p.solver_type = 1 if solver == "cosserat" else 0
+    pos = as_f32(nodes.position, dev).clone()
+    th = as_f32(edges.theta, dev).clone()
+    y = as_f32(nodes.debug_vector, dev) if solver == "cosserat" else None
+    ep, keep = edge_params(edges, dev)
+    check(lib.mc_rod_solve(plan.handle, ctypes.byref(p), ptr(pos), ptr(th), ptr(y),
+                           ctypes.byref(ep), int(cycles), stream_handle(dev)))
+    return replace(nodes, position=pos), replace(edges, theta=th)

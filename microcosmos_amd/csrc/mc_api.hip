@@ -1,0 +1,736 @@
+// C ABI of the coupled fluid-filament step: plan construction (topology
+// layout, FFT tables, workspace) and the stream-ordered step / rollout /
+// sub-operator entry points declared in include/microcosmos_hip.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/microcosmos_hip.h"
+#include "mc_internal.h"
+
+#define MC_VERSION "microcosmos-hip 0.1.0 (gfx950)"
+
+using namespace mc;
+
+namespace {
+
+thread_local std::string g_err;
+
+mc_status fail(mc_status st, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+mc_status fail(mc_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return st;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(MC_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                  __FILE__, __LINE__);                                                 \
+  } while (0)
+
+// Radix schedule: 4s first, then 2, 3, 5, then remaining primes <= 16.
+bool factor(int n, std::vector<int>& out) {
+  out.clear();
+  if (n < 1) return false;
+  while (n % 4 == 0) { out.push_back(4); n /= 4; }
+  while (n % 2 == 0) { out.push_back(2); n /= 2; }
+  for (int p = 3; p <= 16; p += 2) {
+    while (n % p == 0) { out.push_back(p); n /= p; }
+  }
+  return n == 1;
+}
+
+}  // namespace
+
+struct mc_plan {
+  int dev = 0;
+  int H = 0, W = 0, Hf = 0, Wf = 0, N = 0, E = 0, B = 0, kmax = 0;
+  int lds_max = 65536;
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+  // rod
+  RodLayoutDev L{};
+  size_t rod_lds_bytes = 0;
+  int rod_use_lds = 1;
+  float* rod_gscratch = nullptr;
+  // steric
+  float* pred_w = nullptr;
+  int* cic_cell = nullptr;
+  float* cic_w = nullptr;
+  int *row_cnt = nullptr, *row_off = nullptr, *row_fill = nullptr, *row_active = nullptr;
+  int *row_nactive = nullptr, *row_list = nullptr, *row_sort = nullptr;
+  float2* spec = nullptr;
+  float2 *tw_w = nullptr, *tw_h = nullptr;
+  int *radix_w = nullptr, *radix_h = nullptr;
+  int nrad_w = 0, nrad_h = 0;
+  bool fft_ok = false;
+  std::string fft_why;
+  // immersed boundary + LBM
+  IBWork ib{};
+  float* f_tmp = nullptr;
+  float* node_tmp = nullptr;  // (N, 2) scratch
+  long long* tick = nullptr;
+  hipStream_t cap_stream = nullptr;
+
+  template <class T>
+  mc_status alloc(T** p, size_t count, const char* what) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(MC_ERR_OOM, "hipMalloc(%s, %zu B): %s", what, count * sizeof(T), hipGetErrorString(e));
+    e = hipMemset(q, 0, count * sizeof(T));
+    if (e != hipSuccess) return fail(MC_ERR_HIP, "hipMemset(%s): %s", what, hipGetErrorString(e));
+    allocs.push_back(q);
+    bytes += count * sizeof(T);
+    *p = static_cast<T*>(q);
+    return MC_OK;
+  }
+};
+
+namespace {
+
+#define MCCHK(expr)                 \
+  do {                              \
+    mc_status s_ = (expr);          \
+    if (s_ != MC_OK) return s_;     \
+  } while (0)
+
+template <class T>
+mc_status upload(mc_plan* p, T** dst, const std::vector<T>& v, const char* what) {
+  MCCHK(p->alloc(dst, v.size(), what));
+  if (!v.empty()) HIPCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return MC_OK;
+}
+
+// Connected components of the edge graph (union by smaller root id), then
+// the component-grouped layout with ordered incidence lists.
+mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
+  const int N = p->N, E = p->E, B = p->B;
+  for (int e = 0; e < E; ++e)
+    for (int k = 0; k < 2; ++k)
+      if (pairs[2 * e + k] < 0 || pairs[2 * e + k] >= N)
+        return fail(MC_ERR_BAD_SHAPE, "edge %d endpoint %d out of range [0,%d)", e, pairs[2 * e + k], N);
+  for (int b = 0; b < B; ++b)
+    for (int k = 0; k < 2; ++k)
+      if (bp[2 * b + k] < 0 || bp[2 * b + k] >= E)
+        return fail(MC_ERR_BAD_SHAPE, "bending pair %d edge %d out of range [0,%d)", b, bp[2 * b + k], E);
+  std::vector<int> parent(N);
+  for (int i = 0; i < N; ++i) parent[i] = i;
+  auto find = [&](int a) {
+    int r = a;
+    while (parent[r] != r) r = parent[r];
+    while (parent[a] != r) { int nx = parent[a]; parent[a] = r; a = nx; }
+    return r;
+  };
+  for (int e = 0; e < E; ++e) {
+    int a = find(pairs[2 * e]), b = find(pairs[2 * e + 1]);
+    if (a != b) { if (a < b) parent[b] = a; else parent[a] = b; }
+  }
+  std::vector<int> root(N), comp_of_node(N);
+  for (int i = 0; i < N; ++i) root[i] = find(i);
+  // component index = rank of the root (the smallest node id of the component)
+  std::vector<int> comp_id(N, -1);
+  int C = 0;
+  for (int i = 0; i < N; ++i)
+    if (root[i] == i) comp_id[i] = C++;
+  for (int i = 0; i < N; ++i) comp_of_node[i] = comp_id[root[i]];
+  std::vector<int> comp_of_edge(E), comp_of_hinge(B);
+  for (int e = 0; e < E; ++e) comp_of_edge[e] = comp_of_node[pairs[2 * e]];
+  for (int b = 0; b < B; ++b) {
+    comp_of_hinge[b] = comp_of_edge[bp[2 * b]];
+    if (comp_of_edge[bp[2 * b + 1]] != comp_of_hinge[b])
+      return fail(MC_ERR_BAD_SHAPE, "bending pair %d joins edges of different components", b);
+  }
+  auto group = [](int n, int C, const std::vector<int>& comp, std::vector<int>& off,
+                  std::vector<int>& order, std::vector<int>& local) {
+    off.assign(C + 1, 0);
+    for (int i = 0; i < n; ++i) off[comp[i] + 1]++;
+    for (int c = 0; c < C; ++c) off[c + 1] += off[c];
+    order.assign(n, 0);
+    local.assign(n, 0);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int i = 0; i < n; ++i) {  // stable: ascending ids within a component
+      const int pos = fill[comp[i]]++;
+      order[pos] = i;
+      local[i] = pos - off[comp[i]];
+    }
+  };
+  std::vector<int> noff, norder, nlocal, eoff, eorder, elocal, hoff, horder, hlocal;
+  group(N, C, comp_of_node, noff, norder, nlocal);
+  group(E, C, comp_of_edge, eoff, eorder, elocal);
+  group(B, C, comp_of_hinge, hoff, horder, hlocal);
+  std::vector<int> esrc(E), etgt(E), hin(B), hout(B);
+  for (int q = 0; q < E; ++q) {
+    const int e = eorder[q];
+    esrc[q] = nlocal[pairs[2 * e]];
+    etgt[q] = nlocal[pairs[2 * e + 1]];
+  }
+  for (int q = 0; q < B; ++q) {
+    const int b = horder[q];
+    hin[q] = elocal[bp[2 * b]];
+    hout[q] = elocal[bp[2 * b + 1]];
+  }
+  // node incidences: src adds in edge order, then tgt adds in edge order
+  std::vector<std::vector<int>> ninc(N);
+  for (int e = 0; e < E; ++e) ninc[pairs[2 * e]].push_back((elocal[e] << 1) | 0);
+  for (int e = 0; e < E; ++e) ninc[pairs[2 * e + 1]].push_back((elocal[e] << 1) | 1);
+  std::vector<int> ninc_off(1, 0), ninc_flat;
+  for (int q = 0; q < N; ++q) {
+    for (int v : ninc[norder[q]]) ninc_flat.push_back(v);
+    ninc_off.push_back((int)ninc_flat.size());
+  }
+  // edge incidences: e_in adds in hinge order, then e_out adds in hinge order
+  std::vector<std::vector<int>> einc(E);
+  for (int b = 0; b < B; ++b) einc[bp[2 * b]].push_back((hlocal[b] << 1) | 0);
+  for (int b = 0; b < B; ++b) einc[bp[2 * b + 1]].push_back((hlocal[b] << 1) | 1);
+  std::vector<int> einc_off(1, 0), einc_flat;
+  for (int q = 0; q < E; ++q) {
+    for (int v : einc[eorder[q]]) einc_flat.push_back(v);
+    einc_off.push_back((int)einc_flat.size());
+  }
+  // LDS budget: the largest component
+  size_t maxf = 0;
+  for (int c = 0; c < C; ++c) {
+    const size_t f = rod_scratch_floats(noff[c + 1] - noff[c], eoff[c + 1] - eoff[c], hoff[c + 1] - hoff[c]);
+    maxf = std::max(maxf, f);
+  }
+  p->rod_lds_bytes = maxf * sizeof(float);
+  p->rod_use_lds = p->rod_lds_bytes <= (size_t)p->lds_max ? 1 : 0;
+  if (!p->rod_use_lds) MCCHK(p->alloc(&p->rod_gscratch, rod_scratch_floats(N, E, B), "rod scratch"));
+  int *a0, *a1, *a2, *a3, *a4, *a5, *a6, *a7, *a8, *a9, *a10, *a11, *a12, *a13;
+  MCCHK(upload(p, &a0, noff, "comp_node_off"));
+  MCCHK(upload(p, &a1, norder, "comp_nodes"));
+  MCCHK(upload(p, &a2, eoff, "comp_edge_off"));
+  MCCHK(upload(p, &a3, eorder, "comp_edges"));
+  MCCHK(upload(p, &a4, hoff, "comp_hinge_off"));
+  MCCHK(upload(p, &a5, horder, "comp_hinges"));
+  MCCHK(upload(p, &a6, esrc, "edge_src"));
+  MCCHK(upload(p, &a7, etgt, "edge_tgt"));
+  MCCHK(upload(p, &a8, hin, "hinge_in"));
+  MCCHK(upload(p, &a9, hout, "hinge_out"));
+  MCCHK(upload(p, &a10, ninc_off, "node_inc_off"));
+  MCCHK(upload(p, &a11, ninc_flat, "node_inc"));
+  MCCHK(upload(p, &a12, einc_off, "edge_inc_off"));
+  MCCHK(upload(p, &a13, einc_flat, "edge_inc"));
+  p->L = RodLayoutDev{C, a0, a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11, a12, a13};
+  return MC_OK;
+}
+
+mc_status build_fft(mc_plan* p) {
+  std::vector<int> rw, rh;
+  if (!factor(p->W, rw) || !factor(p->H, rh)) {
+    p->fft_why = "steric FFT needs grid sides whose prime factors are all <= 16";
+    return MC_OK;
+  }
+  const size_t row_lds = (size_t)2 * std::max(p->W, p->H) * sizeof(float2) + 16384;
+  if (row_lds > (size_t)p->lds_max || 2 * (size_t)p->H * sizeof(float2) > (size_t)p->lds_max - 1024) {
+    p->fft_why = "steric FFT row/column does not fit in LDS";
+    return MC_OK;
+  }
+  auto table = [](int n) {
+    std::vector<float2> t(n);
+    for (int k = 0; k < n; ++k) {
+      const double a = -2.0 * M_PI * (double)k / (double)n;
+      t[k] = make_float2((float)cos(a), (float)sin(a));
+    }
+    return t;
+  };
+  MCCHK(upload(p, &p->tw_w, table(p->W), "twiddles W"));
+  MCCHK(upload(p, &p->tw_h, table(p->H), "twiddles H"));
+  MCCHK(upload(p, &p->radix_w, rw, "radix W"));
+  MCCHK(upload(p, &p->radix_h, rh, "radix H"));
+  p->nrad_w = (int)rw.size();
+  p->nrad_h = (int)rh.size();
+  MCCHK(p->alloc(&p->spec, (size_t)p->H * p->W, "spectrum"));
+  p->fft_ok = true;
+  return MC_OK;
+}
+
+mc_status check_params(const mc_plan* p, const mc_params* q, bool need_fluid, bool need_steric) {
+  if (!q) return fail(MC_ERR_INVALID, "params is NULL");
+  if (q->solver_type != MC_SOLVER_PBD && q->solver_type != MC_SOLVER_COSSERAT)
+    return fail(MC_ERR_UNSUPPORTED, "solver_type %d", q->solver_type);
+  if (q->cycles_per_step < 0) return fail(MC_ERR_BAD_SHAPE, "cycles_per_step < 0");
+  if (need_fluid && q->enable_fluid) {
+    if (q->synthetic_node_width > 0.0)
+      return fail(MC_ERR_UNSUPPORTED, "synthetic_node_width > 0 is not implemented on the device path");
+    if (q->ibm_kernel_size < 1 || q->ibm_kernel_size > IB_KMAX || q->ibm_kernel_size > p->kmax)
+      return fail(MC_ERR_UNSUPPORTED, "ibm_kernel_size %d (plan supports 1..%d)", q->ibm_kernel_size, p->kmax);
+    if (q->ibm_kernel_size > p->Hf || q->ibm_kernel_size > p->Wf)
+      return fail(MC_ERR_BAD_SHAPE, "ibm_kernel_size %d exceeds the lattice %dx%d", q->ibm_kernel_size, p->Hf, p->Wf);
+    if (q->ibm_iterations < 0) return fail(MC_ERR_BAD_SHAPE, "ibm_iterations < 0");
+  }
+  if (need_steric && q->enable_steric) {
+    if (!p->fft_ok) return fail(MC_ERR_UNSUPPORTED, "%s (grid %dx%d)", p->fft_why.c_str(), p->H, p->W);
+    if (q->steric_neighbor_skip < 0) return fail(MC_ERR_BAD_SHAPE, "steric_neighbor_skip < 0");
+  }
+  return MC_OK;
+}
+
+Consts derive(const mc_plan* p, const mc_params* q, double dt) {
+  Consts c;
+  memset(&c, 0, sizeof(c));
+  c.dt = (float)dt;
+  c.damping = (float)q->damping;
+  c.enable_fluid = q->enable_fluid != 0;
+  c.enable_steric = q->enable_steric != 0;
+  c.dom_h = p->H;
+  c.dom_w = p->W;
+  c.dom_hf = (float)p->H;
+  c.dom_wf = (float)p->W;
+  const double s = q->steric_sigma;
+  c.steric_strength = (float)q->steric_strength;
+  c.steric_sv = (float)q->steric_scatter_value;
+  c.steric_2s2 = (float)(2.0 * s * s);                       // forces.py:76
+  c.steric_norm = (float)(2.0 * M_PI * s * s);               // forces.py:76
+  c.steric_svs = (float)(q->steric_scatter_value / (s * s)); // forces.py:77
+  c.fft_coef = (float)(-2.0 * (M_PI * M_PI) * (s * s));      // utils.py:47
+  c.steric_nskip = q->steric_neighbor_skip;
+  c.solver = q->solver_type;
+  c.cycles = q->cycles_per_step;
+  c.k_st = (float)q->stiffness_stretch;
+  c.k_sh = (float)q->stiffness_shear;
+  c.cos_w = (float)(1.0 / (q->config_dt * q->config_dt));   // stable_cosserat.py:186
+  c.K = q->ibm_kernel_size;
+  c.half = (c.K - 1) / 2;                                    // fluid.py:48
+  c.half_width = (float)(c.K / 2.0);                         // fluid.py:49
+  c.iters = q->ibm_iterations;
+  c.force_clamp = q->enable_force_clamp != 0;
+  c.relax = (float)q->ibm_relaxation;
+  // fluid.py:225-234 (scale_x uses grid_shape[0], as the reference does)
+  const double sx = (double)p->Wf / (double)p->H;
+  const double sy = (double)p->Hf / (double)p->W;
+  const double mean = 0.5 * (sx + sy);
+  c.umax = (float)(q->max_fluid_velocity * mean);
+  c.scale_x = (float)sx;
+  c.scale_y = (float)sy;
+  c.sdt_x = c.scale_x * c.dt;
+  c.sdt_y = c.scale_y * c.dt;
+  c.node_mass = (float)q->node_mass;
+  // fluid.py:155-161,342-344, float32 arithmetic after the double 1/(3 nu + 0.5)
+  const float eps = 1.1920928955078125e-07f;
+  float op = (float)(1.0 / (3.0 * q->viscosity + 0.5));
+  op = op < 0.0f ? 0.0f : (op > 1.9f ? 1.9f : op);
+  const float tpo = 1.0f / std::max(op, eps) - 0.5f;
+  const float om = 1.0f / ((float)q->lambda_trt / std::max(tpo, eps) + 0.5f);
+  c.omega_p = op;
+  c.omega_m = om;
+  c.cp = 1.0f - 0.5f * op;
+  c.cm = 1.0f - 0.5f * om;
+  c.fl_h = p->Hf;
+  c.fl_w = p->Wf;
+  return c;
+}
+
+// Steric pipeline on `pos` (+ optional velocity prediction): fills p->pred_w,
+// writes the filtered field to steric_out.
+mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float* vel,
+                     float* steric_out, float* density_out, hipStream_t s) {
+  HIPCHK(launch_predict(c, p->N, pos, vel, p->pred_w, p->cic_cell, p->cic_w, p->row_cnt, s));
+  if (!c.enable_steric) return MC_OK;
+  HIPCHK(launch_bin_scan(p->H, p->row_cnt, p->row_off, p->row_fill, p->row_active, p->row_nactive, s));
+  HIPCHK(launch_row_fill(c, p->N, p->cic_cell, p->row_off, p->row_fill, p->row_list, s));
+  HIPCHK(launch_deposit_rows_fft(c, p->N, p->row_off, p->row_list, p->cic_cell, p->cic_w, p->spec,
+                                 p->tw_w, p->radix_w, p->nrad_w, density_out, p->row_sort, s));
+  HIPCHK(launch_fft_cols_filter(p->H, p->W, p->spec, p->tw_h, p->radix_h, p->nrad_h, c.fft_coef,
+                                p->lds_max - 1024, s));
+  HIPCHK(launch_ifft_rows_real(p->H, p->W, p->spec, steric_out, p->tw_w, p->radix_w, p->nrad_w, s));
+  return MC_OK;
+}
+
+mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc_state* in,
+                       const mc_state* out, const mc_edge_params* ed, const Actuation* act,
+                       hipStream_t s, hipEvent_t ev_lbm0 = nullptr, hipEvent_t ev_lbm1 = nullptr) {
+  MCCHK(run_steric(p, c, in->position, in->velocity, out->steric, nullptr, s));
+  if (!c.enable_steric && out->steric != in->steric)
+    HIPCHK(hipMemcpyAsync(out->steric, in->steric, (size_t)p->H * p->W * sizeof(float),
+                          hipMemcpyDeviceToDevice, s));
+  HIPCHK(launch_rod(c, p->L, p->N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 0,
+                    in->position, in->velocity, nullptr, in->theta, ed->rest_lengths,
+                    ed->bending_rest_angles, ed->bending_stiffness, act, out->steric, p->pred_w,
+                    out->position, out->velocity, out->debug_vector, out->theta, s));
+  const size_t hwf = (size_t)p->Hf * p->Wf;
+  if (c.enable_fluid) {
+    HIPCHK(launch_ib(c, p->ib, in->f_grid, out->position, out->velocity, out->position, s));
+    if (ev_lbm0) HIPCHK(hipEventRecord(ev_lbm0, s));
+    HIPCHK(launch_lbm(c, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
+                      p->ib.mask_words_per_row, p->ib.force, s));
+    if (ev_lbm1) HIPCHK(hipEventRecord(ev_lbm1, s));
+  } else {
+    if (out->f_grid != in->f_grid)
+      HIPCHK(hipMemcpyAsync(out->f_grid, in->f_grid, 9 * hwf * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (out->fluid_velocity != in->fluid_velocity)
+      HIPCHK(hipMemcpyAsync(out->fluid_velocity, in->fluid_velocity, 2 * hwf * sizeof(float),
+                            hipMemcpyDeviceToDevice, s));
+  }
+  (void)q;
+  return MC_OK;
+}
+
+mc_status check_state(const mc_plan* p, const mc_state* st, const char* which) {
+  if (!st) return fail(MC_ERR_INVALID, "%s state is NULL", which);
+  if (p->N > 0 && (!st->position || !st->velocity || !st->debug_vector))
+    return fail(MC_ERR_INVALID, "%s node arrays are NULL", which);
+  if (p->E > 0 && !st->theta) return fail(MC_ERR_INVALID, "%s theta is NULL", which);
+  if (!st->steric || !st->fluid_velocity || !st->f_grid)
+    return fail(MC_ERR_INVALID, "%s field arrays are NULL", which);
+  return MC_OK;
+}
+
+mc_status check_edges(const mc_plan* p, const mc_edge_params* ed) {
+  if (!ed) return fail(MC_ERR_INVALID, "edge params are NULL");
+  if (p->E > 0 && !ed->rest_lengths) return fail(MC_ERR_INVALID, "rest_lengths is NULL");
+  if (p->B > 0 && (!ed->bending_rest_angles || !ed->bending_stiffness))
+    return fail(MC_ERR_INVALID, "bending arrays are NULL");
+  return MC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mc_version(void) { return MC_VERSION; }
+const char* mc_last_error(void) { return g_err.c_str(); }
+
+mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t fluid_h,
+                         int32_t fluid_w, int32_t num_nodes, const int32_t* pairs_host,
+                         int32_t num_edges, const int32_t* bending_pairs_host, int32_t num_hinges,
+                         int32_t max_kernel_size) {
+  g_err.clear();
+  if (!out) return fail(MC_ERR_INVALID, "plan pointer is NULL");
+  *out = nullptr;
+  if (grid_h < 2 || grid_w < 2 || fluid_h < 2 || fluid_w < 2)
+    return fail(MC_ERR_BAD_SHAPE, "grid %dx%d / lattice %dx%d must be at least 2x2", grid_h, grid_w, fluid_h, fluid_w);
+  if (num_nodes < 0 || num_edges < 0 || num_hinges < 0)
+    return fail(MC_ERR_BAD_SHAPE, "negative count");
+  if ((num_edges > 0 && !pairs_host) || (num_hinges > 0 && !bending_pairs_host))
+    return fail(MC_ERR_INVALID, "topology arrays are NULL");
+  if (max_kernel_size < 1 || max_kernel_size > IB_KMAX)
+    return fail(MC_ERR_UNSUPPORTED, "max_kernel_size %d (1..%d)", max_kernel_size, IB_KMAX);
+  mc_plan* p = new mc_plan();
+  HIPCHK(hipGetDevice(&p->dev));
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->dev) == hipSuccess && lds > 0)
+    p->lds_max = std::min(lds, 65536);
+  p->H = grid_h;
+  p->W = grid_w;
+  p->Hf = fluid_h;
+  p->Wf = fluid_w;
+  p->N = num_nodes;
+  p->E = num_edges;
+  p->B = num_hinges;
+  p->kmax = max_kernel_size;
+  auto bail = [&](mc_status st) {
+    std::string keep = g_err;
+    mc_plan_destroy(p);
+    g_err = keep;
+    return st;
+  };
+  mc_status st;
+  if ((st = build_layout(p, pairs_host, bending_pairs_host)) != MC_OK) return bail(st);
+  if ((st = build_fft(p)) != MC_OK) return bail(st);
+  const int N = p->N;
+  const size_t hw = (size_t)p->H * p->W;
+  const size_t hwf = (size_t)p->Hf * p->Wf;
+  if ((st = p->alloc(&p->pred_w, 2 * (size_t)N, "pred")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->node_tmp, 2 * (size_t)N, "node tmp")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->cic_cell, 4 * (size_t)N, "cic cell")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->cic_w, 4 * (size_t)N, "cic w")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_cnt, (size_t)p->H, "row cnt")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_off, (size_t)p->H + 1, "row off")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_fill, (size_t)p->H, "row fill")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_active, (size_t)p->H, "row active")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_nactive, 1, "row n")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_list, 2 * (size_t)N, "row list")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_sort, 2 * (size_t)N, "row sort")) != MC_OK) return bail(st);
+  // immersed boundary
+  IBWork& w = p->ib;
+  w.N = N;
+  w.tiles_x = (p->Wf + IB_TILE - 1) / IB_TILE;
+  w.tiles_y = (p->Hf + IB_TILE - 1) / IB_TILE;
+  w.T = w.tiles_x * w.tiles_y;
+  w.mask_words_per_row = (p->Wf + 31) / 32;
+  const size_t taps = (size_t)max_kernel_size * max_kernel_size;
+  // a node's stencil touches at most min(K, 3) tiles per axis
+  const size_t tpn = (size_t)std::min(max_kernel_size, 3) * std::min(max_kernel_size, 3);
+  if ((st = p->alloc(&w.st_sx, (size_t)N, "st sx")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.st_sy, (size_t)N, "st sy")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.st_w, taps * N, "st w")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.vib, 2 * (size_t)N, "vib")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.corr, 2 * (size_t)N, "corr")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.tile_cnt, (size_t)w.T, "tile cnt")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.tile_off, (size_t)w.T + 1, "tile off")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.tile_fill, (size_t)w.T, "tile fill")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.tile_active, (size_t)w.T, "tile active")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.n_active, 1, "n active")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.tile_list, tpn * N, "tile list")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.sort_scratch, tpn * N, "tile sort")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.rho, hwf, "rho U")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.u0, 2 * hwf, "u U")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.ucorr, 2 * hwf, "ucorr U")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.inv, hwf, "inv U")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.force, 2 * hwf, "force U")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.mask, (size_t)w.mask_words_per_row * p->Hf, "U mask")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->f_tmp, 9 * hwf, "f ping-pong")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->tick, 1, "tick")) != MC_OK) return bail(st);
+  (void)hw;
+  *out = p;
+  return MC_OK;
+}
+
+mc_status mc_plan_destroy(mc_plan* p) {
+  if (!p) return MC_OK;
+  for (void* q : p->allocs) (void)hipFree(q);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  delete p;
+  return MC_OK;
+}
+
+mc_status mc_plan_workspace_bytes(const mc_plan* p, size_t* bytes) {
+  if (!p || !bytes) return fail(MC_ERR_INVALID, "NULL argument");
+  *bytes = p->bytes;
+  return MC_OK;
+}
+
+mc_status mc_step(mc_plan* p, const mc_params* q, const mc_state* in, const mc_state* out,
+                  const mc_edge_params* ed, double dt, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  MCCHK(check_params(p, q, true, true));
+  MCCHK(check_state(p, in, "input"));
+  MCCHK(check_state(p, out, "output"));
+  MCCHK(check_edges(p, ed));
+  if (q->enable_fluid && in->f_grid == out->f_grid)
+    return fail(MC_ERR_INVALID, "f_grid of out must differ from in (streaming is out of place)");
+  if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  const Consts c = derive(p, q, dt);
+  return enqueue_step(p, q, c, in, out, ed, nullptr, s);
+}
+
+mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const mc_edge_params* ed,
+                     const mc_actuation* act, double dt, int64_t num_steps, float* rec_pos,
+                     float* rec_vel, int32_t use_graph, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  MCCHK(check_params(p, q, true, true));
+  MCCHK(check_state(p, st, "state"));
+  MCCHK(check_edges(p, ed));
+  if (num_steps < 0) return fail(MC_ERR_BAD_SHAPE, "num_steps < 0");
+  if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
+  if (act && p->B > 0 && (!act->amp || !act->omega || !act->phase))
+    return fail(MC_ERR_INVALID, "actuation arrays are NULL");
+  if (num_steps == 0) return MC_OK;
+  hipStream_t us = (hipStream_t)stream;
+  const Consts c = derive(p, q, dt);
+  const size_t hwf = (size_t)p->Hf * p->Wf;
+  Actuation A{};
+  const Actuation* ap = nullptr;
+  if (act) {
+    A = Actuation{act->amp, act->omega, act->phase, p->tick};
+    ap = &A;
+    long long t0 = (long long)act->t0;
+    HIPCHK(hipMemcpyAsync(p->tick, &t0, sizeof(t0), hipMemcpyHostToDevice, us));
+    HIPCHK(hipStreamSynchronize(us));  // t0 lives on the host stack
+  }
+  // ping-pong: even steps read state->f_grid, odd steps read f_tmp
+  mc_state sa = *st, sb = *st;
+  sb.f_grid = p->f_tmp;
+  auto one = [&](int parity, hipStream_t s) -> mc_status {
+    const mc_state* in = parity ? &sb : &sa;
+    const mc_state* out = parity ? &sa : &sb;
+    MCCHK(enqueue_step(p, q, c, in, out, ed, ap, s));
+    if (ap) HIPCHK(launch_tick(p->tick, s));
+    return MC_OK;
+  };
+  auto record = [&](int64_t k, hipStream_t s) -> mc_status {
+    if (rec_pos && p->N > 0)
+      HIPCHK(hipMemcpyAsync(rec_pos + (size_t)k * 2 * p->N, st->position, 2 * (size_t)p->N * sizeof(float),
+                            hipMemcpyDeviceToDevice, s));
+    if (rec_vel) {
+      // after step k the u* field is in the output state of that step; both
+      // parities share st->fluid_velocity
+      HIPCHK(hipMemcpyAsync(rec_vel + (size_t)k * 2 * hwf, st->fluid_velocity, 2 * hwf * sizeof(float),
+                            hipMemcpyDeviceToDevice, s));
+    }
+    return MC_OK;
+  };
+  const bool recording = rec_pos || rec_vel;
+  if (!use_graph) {
+    for (int64_t k = 0; k < num_steps; ++k) {
+      MCCHK(one((int)(k & 1), us));
+      MCCHK(record(k, us));
+    }
+  } else {
+    if (!p->cap_stream) HIPCHK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+    hipStream_t cs = p->cap_stream;
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev, us));
+    HIPCHK(hipStreamWaitEvent(cs, ev, 0));
+    // graphs: one per parity when recording, else one pair graph
+    hipGraphExec_t ge[2] = {nullptr, nullptr};
+    const int ngraph = recording ? 2 : 1;
+    for (int gi = 0; gi < ngraph; ++gi) {
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      mc_status r = one(recording ? gi : 0, cs);
+      if (r == MC_OK && !recording) r = one(1, cs);
+      hipError_t ec = hipStreamEndCapture(cs, &g);
+      if (r != MC_OK) return r;
+      if (ec != hipSuccess) return fail(MC_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
+      HIPCHK(hipGraphInstantiate(&ge[gi], g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(g));
+    }
+    if (recording) {
+      for (int64_t k = 0; k < num_steps; ++k) {
+        HIPCHK(hipGraphLaunch(ge[k & 1], cs));
+        MCCHK(record(k, cs));
+      }
+    } else {
+      for (int64_t k = 0; k + 1 < num_steps; k += 2) HIPCHK(hipGraphLaunch(ge[0], cs));
+      if (num_steps & 1) MCCHK(one(0, cs));
+    }
+    HIPCHK(hipEventRecord(ev, cs));
+    HIPCHK(hipStreamWaitEvent(us, ev, 0));
+    HIPCHK(hipStreamSynchronize(cs));
+    for (int gi = 0; gi < ngraph; ++gi) HIPCHK(hipGraphExecDestroy(ge[gi]));
+    HIPCHK(hipEventDestroy(ev));
+  }
+  if (num_steps & 1) {
+    // final lattice is in f_tmp; bring it back into the caller's buffer
+    if (q->enable_fluid)
+      HIPCHK(hipMemcpyAsync(st->f_grid, p->f_tmp, 9 * hwf * sizeof(float), hipMemcpyDeviceToDevice, us));
+  }
+  return MC_OK;
+}
+
+mc_status mc_profile_step(mc_plan* p, const mc_params* q, const mc_state* st,
+                          const mc_edge_params* ed, const mc_actuation* act, double dt,
+                          int64_t num_steps, double* lbm_ms_avg, double* step_ms_avg, void* stream) {
+  g_err.clear();
+  if (!p || !lbm_ms_avg || !step_ms_avg) return fail(MC_ERR_INVALID, "NULL argument");
+  MCCHK(check_params(p, q, true, true));
+  MCCHK(check_state(p, st, "state"));
+  MCCHK(check_edges(p, ed));
+  if (num_steps < 1) return fail(MC_ERR_BAD_SHAPE, "num_steps < 1");
+  if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  const Consts c = derive(p, q, dt);
+  Actuation A{};
+  const Actuation* ap = nullptr;
+  if (act) {
+    A = Actuation{act->amp, act->omega, act->phase, p->tick};
+    ap = &A;
+    long long t0 = (long long)act->t0;
+    HIPCHK(hipMemcpyAsync(p->tick, &t0, sizeof(t0), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  mc_state sa = *st, sb = *st;
+  sb.f_grid = p->f_tmp;
+  std::vector<hipEvent_t> ev(3 * (size_t)num_steps + 1);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(ev[0], s));
+  for (int64_t k = 0; k < num_steps; ++k) {
+    const mc_state* in = (k & 1) ? &sb : &sa;
+    const mc_state* out = (k & 1) ? &sa : &sb;
+    MCCHK(enqueue_step(p, q, c, in, out, ed, ap, s, ev[3 * k + 1], ev[3 * k + 2]));
+    if (ap) HIPCHK(launch_tick(p->tick, s));
+    HIPCHK(hipEventRecord(ev[3 * k + 3], s));
+  }
+  HIPCHK(hipEventSynchronize(ev[3 * num_steps]));
+  double lbm = 0.0;
+  for (int64_t k = 0; k < num_steps; ++k) {
+    float ms = 0.0f;
+    if (q->enable_fluid) {
+      HIPCHK(hipEventElapsedTime(&ms, ev[3 * k + 1], ev[3 * k + 2]));
+      lbm += ms;
+    }
+  }
+  float tot = 0.0f;
+  HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[3 * num_steps]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if ((num_steps & 1) && q->enable_fluid)
+    HIPCHK(hipMemcpyAsync(st->f_grid, p->f_tmp, 9 * (size_t)p->Hf * p->Wf * sizeof(float),
+                          hipMemcpyDeviceToDevice, s));
+  *lbm_ms_avg = lbm / (double)num_steps;
+  *step_ms_avg = (double)tot / (double)num_steps;
+  return MC_OK;
+}
+
+mc_status mc_gaussian_filter(mc_plan* p, const float* data, float* out, double sigma, void* stream) {
+  g_err.clear();
+  if (!p || !data || !out) return fail(MC_ERR_INVALID, "NULL argument");
+  if (!p->fft_ok) return fail(MC_ERR_UNSUPPORTED, "%s (grid %dx%d)", p->fft_why.c_str(), p->H, p->W);
+  hipStream_t s = (hipStream_t)stream;
+  const float coef = (float)(-2.0 * (M_PI * M_PI) * (sigma * sigma));
+  HIPCHK(launch_fft_rows(p->H, p->W, data, p->spec, p->tw_w, p->radix_w, p->nrad_w, s));
+  HIPCHK(launch_fft_cols_filter(p->H, p->W, p->spec, p->tw_h, p->radix_h, p->nrad_h, coef, p->lds_max - 1024, s));
+  HIPCHK(launch_ifft_rows_real(p->H, p->W, p->spec, out, p->tw_w, p->radix_w, p->nrad_w, s));
+  return MC_OK;
+}
+
+mc_status mc_steric_field(mc_plan* p, const mc_params* q, const float* position, float* steric_out,
+                          float* force_out, float* density_out, void* stream) {
+  g_err.clear();
+  if (!p || !q || (!position && p->N > 0) || !steric_out) return fail(MC_ERR_INVALID, "NULL argument");
+  mc_params qq = *q;
+  qq.enable_steric = 1;
+  MCCHK(check_params(p, &qq, false, true));
+  hipStream_t s = (hipStream_t)stream;
+  const Consts c = derive(p, &qq, 1.0);
+  MCCHK(run_steric(p, c, position, nullptr, steric_out, density_out, s));
+  if (force_out) HIPCHK(launch_steric_nodes(c, p->N, steric_out, p->pred_w, force_out, s));
+  return MC_OK;
+}
+
+mc_status mc_ib_lbm_step(mc_plan* p, const mc_params* q, const float* position, const float* velocity,
+                         const float* f_in, float* velocity_out, float* fluid_velocity_out, float* f_out,
+                         float* pos_out, double dt, void* stream) {
+  g_err.clear();
+  if (!p || !f_in || !f_out || !fluid_velocity_out) return fail(MC_ERR_INVALID, "NULL argument");
+  if (p->N > 0 && (!position || !velocity || !velocity_out)) return fail(MC_ERR_INVALID, "NULL node argument");
+  if (f_in == f_out) return fail(MC_ERR_INVALID, "f_out must differ from f_in");
+  mc_params qq = *q;
+  qq.enable_fluid = 1;
+  MCCHK(check_params(p, &qq, true, false));
+  if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  const Consts c = derive(p, &qq, dt);
+  if (p->N > 0 && velocity_out != velocity)
+    HIPCHK(hipMemcpyAsync(velocity_out, velocity, 2 * (size_t)p->N * sizeof(float), hipMemcpyDeviceToDevice, s));
+  HIPCHK(launch_ib(c, p->ib, f_in, position, velocity_out, pos_out ? pos_out : p->node_tmp, s));
+  HIPCHK(launch_lbm(c, f_in, f_out, fluid_velocity_out, p->ib.mask, p->ib.mask_words_per_row, p->ib.force, s));
+  return MC_OK;
+}
+
+mc_status mc_rod_solve(mc_plan* p, const mc_params* q, float* position, float* theta, const float* y,
+                       const mc_edge_params* ed, int32_t cycles, void* stream) {
+  g_err.clear();
+  if (!p || (!position && p->N > 0) || (!theta && p->E > 0)) return fail(MC_ERR_INVALID, "NULL argument");
+  MCCHK(check_params(p, q, false, false));
+  MCCHK(check_edges(p, ed));
+  if (cycles < 0) return fail(MC_ERR_BAD_SHAPE, "cycles < 0");
+  hipStream_t s = (hipStream_t)stream;
+  Consts c = derive(p, q, 1.0);
+  c.cycles = cycles;
+  HIPCHK(launch_rod(c, p->L, p->N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 1, position,
+                    nullptr, y, theta, ed->rest_lengths, ed->bending_rest_angles, ed->bending_stiffness,
+                    nullptr, nullptr, nullptr, position, nullptr, nullptr, theta, s));
+  return MC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// Deterministic binning primitives shared by the immersed-boundary and steric
+// kernels: a single-workgroup exclusive scan with active-bin compaction, and
+// an in-workgroup sort of unique ids (bitonic in LDS, rank-merge in global
+// memory beyond the LDS capacity).  Bins are filled with integer atomics (order
+// unspecified); sorting each bin's unique node ids restores the reference's
+// update order before any floating-point accumulation happens.
+#pragma once
+#include "mc_common.h"
+#include "mc_internal.h"
+
+namespace mc {
+
+// Inclusive scan of v across a workgroup of up to 1024 threads (wave64).
+// `wsum` must hold blockDim.x / 64 ints of LDS.  Returns the inclusive prefix;
+// *total receives the workgroup sum.
+__device__ __forceinline__ int block_inclusive_scan(int v, int* wsum, int* total) {
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  if (lane == 63) wsum[wid] = v;
+  __syncthreads();
+  if (wid == 0) {
+    int s = (lane < nw) ? wsum[lane] : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      int t = __shfl_up(s, d, 64);
+      if (lane >= d) s += t;
+    }
+    if (lane < nw) wsum[lane] = s;
+  }
+  __syncthreads();
+  int pre = (wid > 0) ? wsum[wid - 1] : 0;
+  *total = wsum[nw - 1];
+  __syncthreads();
+  return v + pre;
+}
+
+// Number of elements of sorted array a[0..n) that are < v.
+__device__ __forceinline__ int lower_bound_i(const int* a, int n, int v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Sort m unique ids in place (ascending).  All threads of the workgroup call it.
+// lds: SORT_LDS_CAP ints; gtmp: m ints of global scratch.
+__device__ inline void sort_unique_ids(int* list, int m, int* lds, int* gtmp) {
+  if (m <= 1) return;
+  if (m <= SORT_LDS_CAP) {
+    int P = 1;
+    while (P < m) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) lds[i] = (i < m) ? list[i] : 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+          int ixj = i ^ j;
+          if (ixj > i) {
+            int a = lds[i], b = lds[ixj];
+            bool up = ((i & k) == 0);
+            if ((a > b) == up) { lds[i] = b; lds[ixj] = a; }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < m; i += blockDim.x) list[i] = lds[i];
+    __syncthreads();
+    return;
+  }
+  // bottom-up merge by rank in the partner run (ids are unique)
+  int* src = list;
+  int* dst = gtmp;
+  for (int width = 1; width < m; width <<= 1) {
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      int start = (i / (2 * width)) * (2 * width);
+      int mid = min(start + width, m);
+      int end = min(start + 2 * width, m);
+      int v = src[i];
+      int pos;
+      if (i < mid) pos = (i - start) + lower_bound_i(src + mid, end - mid, v);
+      else pos = (i - mid) + lower_bound_i(src + start, mid - start, v);
+      dst[start + pos] = v;
+    }
+    __threadfence_block();
+    __syncthreads();
+    int* t = src; src = dst; dst = t;
+  }
+  if (src != list) {
+    for (int i = threadIdx.x; i < m; i += blockDim.x) list[i] = src[i];
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+}  // namespace mc

@@ -1,0 +1,330 @@
+// Rod solve of one coupled step (simulate.py:41-84 with pbd.py:14-91 or
+// stable_cosserat.py:13-119): one workgroup per connected component of the
+// edge graph (a filament), all constraint cycles in one launch.
+//
+// Per component the node positions, edge angles and the per-edge / per-hinge
+// intermediates live in LDS (global scratch when a component exceeds the LDS
+// budget).  The reference's Jacobi scatters become ordered gathers through the
+// incidence lists built on the host (src adds then tgt adds for nodes; e_in
+// adds then e_out adds for edges), so every accumulation is the reference's
+// left fold, with no atomics and no colouring needed: Jacobi reads only the
+// previous iterate, and each node / edge is owned by one thread.
+//
+// The kernel also performs the step's node prologue -- damping, inertial
+// prediction, steric sampling (forces.py:20-81) and the steric push -- and
+// its epilogue, v = (x - x_prev) / dt.
+#include "mc_common.h"
+#include "mc_internal.h"
+
+namespace mc {
+namespace {
+
+struct RodArgs {
+  Consts c;
+  RodLayoutDev L;
+  int N;
+  int mode;
+  int use_lds;
+  float* gscratch;
+  const float *pos_in, *vel_in, *y_in, *theta_in, *rest_len, *rest_ang, *kb;
+  Actuation act;
+  int has_act;
+  const float* steric;
+  const float* pred_w;
+  float *pos_out, *vel_out, *debug_out, *theta_out;
+};
+
+__device__ __forceinline__ float sample_bilinear_wrap(const float* V, int H, int W, float yy, float xx) {
+  const float ly = floorf(yy), lx = floorf(xx);
+  const float wyh = yy - ly, wxh = xx - lx;
+  const float wyl = 1.0f - wyh, wxl = 1.0f - wxh;
+  const int iy = f2i(ly), ix = f2i(lx);
+  const int y0 = imod(iy, H), y1 = imod(iy + 1, H);
+  const int x0 = imod(ix, W), x1 = imod(ix + 1, W);
+  float out = (wyl * wxl) * V[(size_t)y0 * W + x0];
+  out = out + (wyl * wxh) * V[(size_t)y0 * W + x1];
+  out = out + (wyh * wxl) * V[(size_t)y1 * W + x0];
+  out = out + (wyh * wxh) * V[(size_t)y1 * W + x1];
+  return out;
+}
+
+// forces.py:20-81 for one node: -grad V by central differences minus the
+// analytic (2n+1)-window self/neighbour contribution over the flat node index.
+__device__ __forceinline__ void steric_force(const Consts& c, const float* V, const float* pred_w, int N,
+                                             int g, float& fx, float& fy) {
+  const int H = c.dom_h, W = c.dom_w;
+  const float px = pred_w[2 * g], py = pred_w[2 * g + 1];
+  const float gx = (sample_bilinear_wrap(V, H, W, py, px + 0.5f) -
+                    sample_bilinear_wrap(V, H, W, py, px - 0.5f)) / 1.0f;
+  const float gy = (sample_bilinear_wrap(V, H, W, py + 0.5f, px) -
+                    sample_bilinear_wrap(V, H, W, py - 0.5f, px)) / 1.0f;
+  float ax = 0.0f, ay = 0.0f;
+  const int nsk = c.steric_nskip;
+  for (int k = -nsk; k <= nsk; ++k) {
+    const int j = g + k;
+    const float valid = (j >= 0 && j < N) ? 1.0f : 0.0f;
+    const int js = j < 0 ? 0 : (j > N - 1 ? N - 1 : j);
+    const float rx = min_image(px, pred_w[2 * js], c.dom_wf);
+    const float ry = min_image(py, pred_w[2 * js + 1], c.dom_hf);
+    const float d2 = rx * rx + ry * ry;
+    const float Kv = dexp(-d2 / c.steric_2s2) / c.steric_norm;
+    const float kv = Kv * valid;
+    ax = ax + kv * rx;
+    ay = ay + kv * ry;
+  }
+  fx = -gx - c.steric_svs * ax;
+  fy = -gy - c.steric_svs * ay;
+}
+
+__device__ __forceinline__ float wrap_angle(float d) {
+  return fmodf_floor(d + MC_PI_F, MC_2PI_F) - MC_PI_F;
+}
+
+__global__ void __launch_bounds__(256) k_rod(RodArgs a) {
+  extern __shared__ float smem[];
+  const Consts& c = a.c;
+  const RodLayoutDev& L = a.L;
+  const int comp = blockIdx.x;
+  const int n0 = L.comp_node_off[comp], nn = L.comp_node_off[comp + 1] - n0;
+  const int e0 = L.comp_edge_off[comp], ne = L.comp_edge_off[comp + 1] - e0;
+  const int b0 = L.comp_hinge_off[comp], nb = L.comp_hinge_off[comp + 1] - b0;
+  // scratch layout (floats): x[2nn] y[2nn] th[ne] l[ne] F[2ne] He[ne] corr[nb] rest[nb] kb[nb]
+  float* base = a.use_lds ? smem : (a.gscratch + (size_t)2 * 2 * n0 + (size_t)5 * e0 + (size_t)3 * b0);
+  float* X = base;
+  float* Y = X + 2 * nn;
+  float* TH = Y + 2 * nn;
+  float* Le = TH + ne;
+  float* Fe = Le + ne;
+  float* He = Fe + 2 * ne;
+  float* CO = He + ne;
+  float* RS = CO + nb;
+  float* KB = RS + nb;
+  const bool is_cos = c.solver == 1;
+
+  // prologue: damping, prediction, steric push (simulate.py:41-72)
+  for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+    const int g = L.comp_nodes[n0 + i];
+    if (a.mode == 1) {
+      const float px = a.pos_in[2 * g], py = a.pos_in[2 * g + 1];
+      X[2 * i] = px;
+      X[2 * i + 1] = py;
+      Y[2 * i] = a.y_in ? a.y_in[2 * g] : px;
+      Y[2 * i + 1] = a.y_in ? a.y_in[2 * g + 1] : py;
+      continue;
+    }
+    const float vx = a.vel_in[2 * g] * c.damping, vy = a.vel_in[2 * g + 1] * c.damping;
+    float px = a.pos_in[2 * g] + vx * c.dt, py = a.pos_in[2 * g + 1] + vy * c.dt;
+    if (c.enable_steric) {
+      float fx, fy;
+      steric_force(c, a.steric, a.pred_w, a.N, g, fx, fy);
+      px = px + (fx * c.steric_strength) * c.dt;
+      py = py + (fy * c.steric_strength) * c.dt;
+    }
+    X[2 * i] = px;
+    X[2 * i + 1] = py;
+    Y[2 * i] = px;
+    Y[2 * i + 1] = py;
+    a.debug_out[2 * g] = is_cos ? px : 0.0f;
+    a.debug_out[2 * g + 1] = is_cos ? py : 0.0f;
+  }
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    const int ge = L.comp_edges[e0 + e];
+    TH[e] = a.theta_in[ge];
+    Le[e] = a.rest_len[ge];
+  }
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const int gb = L.comp_hinges[b0 + b];
+    if (a.has_act) {
+      const double arg = (double)a.act.omega[gb] * (double)(*a.act.t) + (double)a.act.phase[gb];
+      RS[b] = (float)((double)a.act.amp[gb] * sin(arg));
+    } else {
+      RS[b] = a.rest_ang[gb];
+    }
+    KB[b] = a.kb[gb];
+  }
+  __syncthreads();
+
+  const float Wd = c.dom_wf, Hd = c.dom_hf;
+  for (int cyc = 0; cyc < c.cycles; ++cyc) {
+    // theta pass: hinge corrections (pbd.py:32-40)
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      const float tin = TH[L.hinge_in[b0 + b]];
+      const float tout = TH[L.hinge_out[b0 + b]];
+      const float diff = wrap_angle(tout - tin);
+      CO[b] = ((diff - RS[b]) * KB[b]) * 0.5f;
+    }
+    __syncthreads();
+    // per edge ordered gather of +-corr (pbd.py:44-49 / stable_cosserat.py:158-171)
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      const int i0 = L.edge_inc_off[e0 + e], i1 = L.edge_inc_off[e0 + e + 1];
+      float d = 0.0f;
+      for (int q = i0; q < i1; ++q) {
+        const int code = L.edge_inc[q];
+        const float v = CO[code >> 1];
+        d = d + ((code & 1) ? -v : v);
+      }
+      const float deg = fmax_nan((float)(i1 - i0), 1.0f);
+      if (!is_cos) {
+        TH[e] = TH[e] + (d / deg) * 1.8f;
+      } else {
+        float th = TH[e] + d / deg;
+        const int s = L.edge_src[e0 + e], t = L.edge_tgt[e0 + e];
+        const float sx = min_image(X[2 * t], X[2 * s], Wd);
+        const float sy = min_image(X[2 * t + 1], X[2 * s + 1], Hd);
+        const float alpha = datan2(sy, sx);
+        const float pull = wrap_angle(alpha - th);
+        th = th + pull * 0.05f;
+        TH[e] = th;
+      }
+    }
+    __syncthreads();
+    // per edge forces (pbd.py:60-81 / stable_cosserat.py:185-195)
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      const int s = L.edge_src[e0 + e], t = L.edge_tgt[e0 + e];
+      const float th = TH[e], l = Le[e];
+      const float dx = dcos(th), dy = dsin(th);
+      const float rx = min_image(X[2 * t], X[2 * s], Wd);
+      const float ry = min_image(X[2 * t + 1], X[2 * s + 1], Hd);
+      if (!is_cos) {
+        const float tx = dx * l, ty = dy * l;
+        const float len = sqrtf(rx * rx + ry * ry);
+        const float md = fmax_nan(len, 1e-8f);
+        const float cdx = rx / md, cdy = ry / md;
+        const float stx = cdx * (len - l), sty = cdy * (len - l);
+        const float shx = (rx - tx) - stx, shy = (ry - ty) - sty;
+        Fe[2 * e] = ((stx + shx * c.k_sh) * c.k_st) * 0.5f;
+        Fe[2 * e + 1] = ((sty + shy * c.k_sh) * c.k_st) * 0.5f;
+      } else {
+        const float cx = rx / l - dx, cy = ry / l - dy;
+        const float kl = c.k_st / l;
+        Fe[2 * e] = kl * cx;
+        Fe[2 * e + 1] = kl * cy;
+        He[e] = c.k_st / (l * l);
+      }
+    }
+    __syncthreads();
+    // per node ordered gather (pbd.py:84-89 / stable_cosserat.py:197-207)
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+      const int i0 = L.node_inc_off[n0 + i], i1 = L.node_inc_off[n0 + i + 1];
+      if (!is_cos) {
+        float ux = 0.0f, uy = 0.0f;
+        for (int q = i0; q < i1; ++q) {
+          const int code = L.node_inc[q];
+          const int e = code >> 1;
+          const float fx = Fe[2 * e], fy = Fe[2 * e + 1];
+          ux = ux + ((code & 1) ? -fx : fx);
+          uy = uy + ((code & 1) ? -fy : fy);
+        }
+        const float deg = fmax_nan((float)(i1 - i0), 1.0f);
+        X[2 * i] = X[2 * i] + ux / deg;
+        X[2 * i + 1] = X[2 * i + 1] + uy / deg;
+      } else {
+        const float w = c.cos_w;
+        const float xx = X[2 * i], xy = X[2 * i + 1];
+        float fx = w * (Y[2 * i] - xx), fy = w * (Y[2 * i + 1] - xy);
+        float h = w;
+        for (int q = i0; q < i1; ++q) {
+          const int code = L.node_inc[q];
+          const int e = code >> 1;
+          const float gx = Fe[2 * e], gy = Fe[2 * e + 1];
+          fx = fx + ((code & 1) ? -gx : gx);
+          fy = fy + ((code & 1) ? -gy : gy);
+        }
+        // H: src adds then tgt adds, all +H_edge (same incidence order)
+        for (int q = i0; q < i1; ++q) h = h + He[L.node_inc[q] >> 1];
+        X[2 * i] = xx + fx / h;
+        X[2 * i + 1] = xy + fy / h;
+      }
+    }
+    __syncthreads();
+  }
+
+  // epilogue: v = (x - x_prev) / dt (simulate.py:83-84); unwrapped x feeds the
+  // immersed boundary, or is wrapped here when the fluid is off.
+  for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+    const int g = L.comp_nodes[n0 + i];
+    const float xx = X[2 * i], xy = X[2 * i + 1];
+    if (a.mode == 1) {
+      a.pos_out[2 * g] = xx;
+      a.pos_out[2 * g + 1] = xy;
+      continue;
+    }
+    const float qx = a.pos_in[2 * g], qy = a.pos_in[2 * g + 1];
+    a.vel_out[2 * g] = (xx - qx) / c.dt;
+    a.vel_out[2 * g + 1] = (xy - qy) / c.dt;
+    if (c.enable_fluid) {
+      a.pos_out[2 * g] = xx;
+      a.pos_out[2 * g + 1] = xy;
+    } else {
+      a.pos_out[2 * g] = fmodf_floor(xx, Wd);
+      a.pos_out[2 * g + 1] = fmodf_floor(xy, Hd);
+    }
+  }
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) a.theta_out[L.comp_edges[e0 + e]] = TH[e];
+}
+
+}  // namespace
+
+size_t rod_scratch_floats(int nn, int ne, int nb) {
+  return (size_t)4 * nn + (size_t)5 * ne + (size_t)3 * nb;
+}
+
+__global__ void k_steric_nodes(Consts c, int N, const float* __restrict__ steric,
+                               const float* __restrict__ pred_w, float* __restrict__ force_out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float fx, fy;
+  steric_force(c, steric, pred_w, N, n, fx, fy);
+  force_out[2 * n] = fx;
+  force_out[2 * n + 1] = fy;
+}
+
+__global__ void k_tick(long long* t) { *t = *t + 1; }
+
+hipError_t launch_steric_nodes(const Consts& c, int N, const float* steric, const float* pred_w,
+                               float* force_out, hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_steric_nodes, dim3((N + 255) / 256), dim3(256), 0, s, c, N, steric, pred_w,
+                     force_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_tick(long long* t, hipStream_t s) {
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_rod(const Consts& c, const RodLayoutDev& L, int N, size_t lds_bytes, int use_lds,
+                      float* gscratch, int mode, const float* pos_in, const float* vel_in,
+                      const float* y_in, const float* theta_in, const float* rest_len,
+                      const float* rest_ang, const float* kb, const Actuation* act,
+                      const float* steric, const float* pred_w, float* pos_out, float* vel_out,
+                      float* debug_out, float* theta_out, hipStream_t s) {
+  if (L.num_comp == 0) return hipSuccess;
+  RodArgs a;
+  a.c = c;
+  a.L = L;
+  a.N = N;
+  a.mode = mode;
+  a.use_lds = use_lds;
+  a.gscratch = gscratch;
+  a.pos_in = pos_in;
+  a.vel_in = vel_in;
+  a.y_in = y_in;
+  a.theta_in = theta_in;
+  a.rest_len = rest_len;
+  a.rest_ang = rest_ang;
+  a.kb = kb;
+  a.has_act = act != nullptr;
+  if (act) a.act = *act; else a.act = Actuation{nullptr, nullptr, nullptr, nullptr};
+  a.steric = steric;
+  a.pred_w = pred_w;
+  a.pos_out = pos_out;
+  a.vel_out = vel_out;
+  a.debug_out = debug_out;
+  a.theta_out = theta_out;
+  hipLaunchKernelGGL(k_rod, dim3(L.num_comp), dim3(64), use_lds ? lds_bytes : 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace mc

@@ -1,0 +1,168 @@
+"""step / simulate (simulate.py:15-100) on the MI355X device path.
+
+``step(nodes, edges, fields, dt, solver_config)`` keeps the reference's
+signature and functional semantics (new output arrays; inputs untouched) and
+runs the whole coupled step as one stream-ordered sequence of HIP kernels:
+steric field -> rod solve -> immersed boundary -> fused LBM collide-stream.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import replace
+
+import torch
+
+from . import _lib
+from ._lib import McActuation, McState, check
+from ._runtime import as_f32, edge_params, params_from_config, plan_for, ptr, stream_handle
+from .structs import Edges, Fields, Nodes
+
+
+def _device_of(nodes, fields):
+    dev = fields.f_grid.device if fields.f_grid is not None else nodes.position.device
+    if dev.type != "cuda":
+        raise RuntimeError("microcosmos_amd runs on the GPU only: move the state to a cuda device")
+    return dev
+
+
+def step(nodes: Nodes, edges: Edges, fields: Fields, dt: float, solver_config):
+    """One coupled step (simulate.py:37-100)."""
+    dev = _device_of(nodes, fields)
+    lib = _lib.load()
+    params = params_from_config(solver_config)
+    H, W = fields.grid_shape
+    f_in = as_f32(fields.f_grid, dev)
+    Hf, Wf = f_in.shape[1], f_in.shape[2]
+    N = nodes.position.shape[0]
+    plan = plan_for(N, edges.pairs, edges.bending_pairs, (H, W), (Hf, Wf), dev)
+    pos = as_f32(nodes.position, dev)
+    vel = as_f32(nodes.velocity, dev)
+    dbg = as_f32(nodes.debug_vector, dev)
+    th = as_f32(edges.theta, dev)
+    st = as_f32(fields.steric, dev)
+    fv = as_f32(fields.fluid_velocity, dev)
+    o_pos, o_vel, o_dbg, o_th = (torch.empty_like(pos), torch.empty_like(vel),
+                                 torch.empty_like(dbg), torch.empty_like(th))
+    o_st = torch.empty_like(st) if params.enable_steric else st
+    if params.enable_fluid:
+        o_fv, o_f = torch.empty_like(fv), torch.empty_like(f_in)
+    else:
+        o_fv, o_f = fv, f_in
+    sin = McState(ptr(pos), ptr(vel), ptr(dbg), ptr(th), ptr(st), ptr(fv), ptr(f_in))
+    sout = McState(ptr(o_pos), ptr(o_vel), ptr(o_dbg), ptr(o_th), ptr(o_st), ptr(o_fv), ptr(o_f))
+    ep, keep = edge_params(edges, dev)
+    check(lib.mc_step(plan.handle, ctypes.byref(params), ctypes.byref(sin), ctypes.byref(sout),
+                      ctypes.byref(ep), float(dt), stream_handle(dev)))
+    new_nodes = Nodes(position=o_pos, velocity=o_vel, color_bend=nodes.color_bend,
+                      debug_vector=o_dbg)
+    new_edges = replace(edges, theta=o_th)
+    new_fields = replace(fields, steric=o_st, fluid_velocity=o_fv, f_grid=o_f)
+    return new_nodes, new_edges, new_fields
+
+
+def simulate(nodes: Nodes, edges: Edges, fields: Fields, dt: float, num_steps: int,
+             solver_config):
+    """simulate.py:15-35 -- returns (nodes_timeseries, fields_timeseries), each a
+    pytree of arrays stacked over steps; f_grid is dropped from the history."""
+    H, W = fields.grid_shape
+    if fields.f_grid is not None and tuple(fields.f_grid.shape[1:]) != (H, W):
+        raise NotImplementedError("simulate() with a lattice resolution different from "
+                                  "grid_shape (linear resize of the recorded velocity)")
+    pos, vel, dbg, cb, st, fv = [], [], [], [], [], []
+    for _ in range(int(num_steps)):
+        nodes, edges, fields = step(nodes, edges, fields, dt, solver_config)
+        pos.append(nodes.position)
+        vel.append(nodes.velocity)
+        dbg.append(nodes.debug_vector)
+        cb.append(nodes.color_bend)
+        st.append(fields.steric)
+        fv.append(fields.fluid_velocity)
+    nts = Nodes(position=torch.stack(pos), velocity=torch.stack(vel),
+                color_bend=torch.stack([torch.as_tensor(c) for c in cb]),
+                debug_vector=torch.stack(dbg))
+    fts = replace(fields, steric=torch.stack(st), fluid_velocity=torch.stack(fv), f_grid=None)
+    return nts, fts
+
+
+class Rollout:
+    """Device-resident, in-place rollout state for long runs (mc_rollout).
+
+    Holds one copy of every state array on the device and advances it with
+    graph-captured steps; optional in-kernel actuation of the rest angles.
+    """
+
+    def __init__(self, nodes: Nodes, edges: Edges, fields: Fields, solver_config, dt: float,
+                 actuation=None):
+        self.dev = _device_of(nodes, fields)
+        self.params = params_from_config(solver_config)
+        self.dt = float(dt)
+        d = self.dev
+        self.position = as_f32(nodes.position, d).clone()
+        self.velocity = as_f32(nodes.velocity, d).clone()
+        self.debug_vector = as_f32(nodes.debug_vector, d).clone()
+        self.color_bend = nodes.color_bend
+        self.theta = as_f32(edges.theta, d).clone()
+        self.steric = as_f32(fields.steric, d).clone()
+        self.fluid_velocity = as_f32(fields.fluid_velocity, d).clone()
+        self.f_grid = as_f32(fields.f_grid, d).clone()
+        self.edges = edges
+        self.fields = fields
+        self.grid_shape = tuple(fields.grid_shape)
+        self.plan = plan_for(self.position.shape[0], edges.pairs, edges.bending_pairs,
+                             self.grid_shape, tuple(self.f_grid.shape[1:]), d)
+        self.ep, self._keep = edge_params(edges, d)
+        self.t = 0
+        self.act = None
+        if actuation is not None:
+            amp, omega, phase = (as_f32(a, d) for a in actuation)
+            self._act_keep = (amp, omega, phase)
+            self.act = (amp, omega, phase)
+
+    def state(self) -> McState:
+        return McState(ptr(self.position), ptr(self.velocity), ptr(self.debug_vector),
+                       ptr(self.theta), ptr(self.steric), ptr(self.fluid_velocity),
+                       ptr(self.f_grid))
+
+    def run(self, num_steps: int, use_graph: bool = True, record_position=None,
+            record_fluid_velocity=None):
+        lib = _lib.load()
+        st = self.state()
+        act = None
+        if self.act is not None:
+            a, o, p = self.act
+            act = McActuation(ptr(a), ptr(o), ptr(p), int(self.t))
+        check(lib.mc_rollout(self.plan.handle, ctypes.byref(self.params), ctypes.byref(st),
+                             ctypes.byref(self.ep), ctypes.byref(act) if act else None,
+                             self.dt, int(num_steps), ptr(record_position),
+                             ptr(record_fluid_velocity), int(bool(use_graph)),
+                             stream_handle(self.dev)))
+        self.t += int(num_steps)
+
+    def profile(self, num_steps: int):
+        """num_steps steps with HIP events around each fused LBM launch:
+        returns (average LBM kernel ms, average step ms)."""
+        lib = _lib.load()
+        st = self.state()
+        act = None
+        if self.act is not None:
+            a, o, p = self.act
+            act = McActuation(ptr(a), ptr(o), ptr(p), int(self.t))
+        lbm = ctypes.c_double()
+        stp = ctypes.c_double()
+        check(lib.mc_profile_step(self.plan.handle, ctypes.byref(self.params), ctypes.byref(st),
+                                  ctypes.byref(self.ep), ctypes.byref(act) if act else None,
+                                  self.dt, int(num_steps), ctypes.byref(lbm), ctypes.byref(stp),
+                                  stream_handle(self.dev)))
+        self.t += int(num_steps)
+        return lbm.value, stp.value
+
+    def nodes(self) -> Nodes:
+        return Nodes(position=self.position, velocity=self.velocity, color_bend=self.color_bend,
+                     debug_vector=self.debug_vector)
+
+    def edges_now(self) -> Edges:
+        return replace(self.edges, theta=self.theta)
+
+    def fields_now(self) -> Fields:
+        return replace(self.fields, steric=self.steric, fluid_velocity=self.fluid_velocity,
+                       f_grid=self.f_grid)

@@ -1,0 +1,734 @@
+"""CPU restatement (NumPy) of the reference's coupled fluid-filament time step.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: it is imported by
+``tests/``, by ``__graft_entry__.smoke()`` and by the ``cpu_baseline`` leg of
+``bench.py``, and nowhere else.  The product path (``microcosmos_amd``) never
+imports, calls or falls back to it.
+
+It restates, operation for operation and in the same floating-point evaluation
+order, the JAX code of ``ProjectProgramAMark/microcosmos`` for the path
+
+    simulate.step -> steric FFT field -> PBD / Cosserat rod projection
+                  -> immersed boundary (IB) + D2Q9 TRT lattice Boltzmann
+
+Every function cites the reference file:line it follows.  The module is
+dtype-generic: run it on float32 arrays for the parity oracle and on float64
+arrays for the error reference (the reference itself never enables x64).
+
+Evaluation-order conventions (chosen to be reproducible on the GPU):
+  * every ``.at[].add`` scatter is an in-order fold from zero (``np.add.at``
+    applies updates in index order);
+  * every small reduction (9 populations, 16 stencil taps, 2n+1 steric window)
+    is a left fold in index order -- numpy's pairwise ``sum`` is never used;
+  * in float32 runs, cos/sin/atan2/exp are evaluated in float64 and rounded to
+    float32 (a correctly rounded result); the device does the same.  XLA-CPU's
+    own float32 polynomials are not reproduced (they differ by <= ~1 ulp);
+  * the FFT is numpy's pocketfft in the input precision (complex64 for float32
+    input), standing in for jnp.fft (ducc); their rounding is not reproduced.
+
+Parity status per function (how each is pinned; see tests/test_oracle_*.py):
+  displacement, periodic_boundary, index_from_position  -- reference KATs
+      (tests/test_utils.py:25-56)
+  gaussian_filter        -- reference peak/symmetry KAT (tests/test_utils.py:59-68),
+                            float64 brute-force circular convolution, mass
+  update_steric_potential -- reference positivity KAT (tests/test_forces.py:7-32),
+                            brute-force CIC, mass = N * scatter_value
+  force_interp           -- scipy.ndimage.map_coordinates(order=1, mode="grid-wrap"),
+                            exact gradient of a linear field
+  steric window          -- closed form for two isolated nodes
+  pbd_rod_constraint     -- straight rest chain fixed point, single-edge closed form,
+                            hand-derived one-hinge theta update
+  stable_cosserat        -- straight rest chain fixed point, single-edge closed form,
+                            reference invariants (tests/test_stable_cosserat.py:81-154)
+  stencil (IB)           -- brute-force integer taps, partition of unity,
+                            constant interpolation, spread/interp adjointness
+  LBM                    -- feq moments (float64), mass conservation, uniform fixed
+                            point, exact streaming permutation, Guo momentum input,
+                            Taylor-Green viscous decay exp(-2 nu k^2 t)
+  step                   -- reference worm behaviour (tests/test_worm.py:15-152)
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field, replace
+from typing import Optional
+
+import numpy as np
+
+# utils.py:9 -- float32 machine epsilon, used as a numpy float32 scalar so that
+# float32 arrays stay float32 and float64 arrays stay float64.
+EPSIL = np.float32(np.finfo(np.float32).eps)
+
+# fluid.py:15-23 -- D2Q9 lattice, direction order and weights.
+LBM_VELOCITIES = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1],
+                           [1, 1], [-1, 1], [-1, -1], [1, -1]])
+LBM_WEIGHTS = np.array([4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 36, 1 / 36])
+OPP_INDICES = np.array([0, 3, 4, 1, 2, 7, 8, 5, 6])
+
+
+# --------------------------------------------------------------------------
+# Parameters and state
+# --------------------------------------------------------------------------
+@dataclass(frozen=True)
+class Params:
+    """Numeric fields of ConstraintSolverConfig (solver/config.py:9-71).
+
+    ``config_dt`` is ConstraintSolverConfig.dt (the Cosserat inertia step);
+    the step's own ``dt`` is passed to ``step`` separately, as in the reference.
+    """
+    solver_type: str = "pbd"
+    cycles_per_step: int = 20
+    stiffness_stretch: float = 0.6
+    stiffness_shear: float = 1.0
+    damping: float = 1.0
+    enable_fluid: bool = True
+    viscosity: float = 0.01
+    max_fluid_velocity: float = 0.2
+    config_dt: float = 0.05
+    ibm_iterations: int = 5
+    synthetic_node_width: float = 0.0
+    ibm_kernel_size: int = 3
+    ibm_relaxation: float = 0.3
+    node_mass: float = 1.0
+    lambda_trt: float = 0.25
+    enable_force_clamp: bool = False
+    enable_steric: bool = False
+    steric_strength: float = 1.0
+    steric_sigma: float = 1.5
+    steric_neighbor_skip: int = 5
+    steric_scatter_value: float = 1.0
+
+
+# solver/config.py:74-97
+PBD_SCHEME = Params(solver_type="pbd", cycles_per_step=20, stiffness_stretch=0.6,
+                    stiffness_shear=1.0, damping=1.0, enable_fluid=True, viscosity=0.01,
+                    max_fluid_velocity=0.2, ibm_iterations=5, ibm_kernel_size=2)
+# solver/config.py:100-119
+PBD_SCHEME_NO_FLUID = Params(solver_type="pbd", cycles_per_step=10, stiffness_stretch=0.7,
+                             stiffness_shear=0.7, damping=0.0, enable_fluid=False,
+                             viscosity=0.1, max_fluid_velocity=0.1)
+# solver/config.py:126-144
+COSSERAT_SCHEME_NO_FLUID = Params(solver_type="cosserat", cycles_per_step=20,
+                                  stiffness_stretch=50.0, stiffness_shear=50.0, damping=0.0,
+                                  config_dt=0.05, enable_fluid=False, viscosity=0.1,
+                                  max_fluid_velocity=0.1, steric_strength=100.0)
+# solver/config.py:152-171
+COSSERAT_SCHEME = Params(solver_type="cosserat", cycles_per_step=20, stiffness_stretch=50.0,
+                         stiffness_shear=50.0, damping=1.0, config_dt=0.05, enable_fluid=True,
+                         viscosity=0.01, max_fluid_velocity=0.2, ibm_iterations=5,
+                         ibm_kernel_size=2)
+
+
+@dataclass
+class State:
+    """Nodes + Edges + Fields of the reference (structs/nodes.py:5-15,
+    structs/edges.py:6-21, structs/fields.py:5-16), as numpy arrays."""
+    position: np.ndarray          # (N, 2) x, y
+    velocity: np.ndarray          # (N, 2)
+    debug_vector: np.ndarray      # (N, 2)
+    pairs: np.ndarray             # (E, 2) int32 (src, tgt)
+    theta: np.ndarray             # (E,)
+    rest_lengths: np.ndarray      # (E,)
+    bending_pairs: np.ndarray     # (B, 2) int32 (edge_in, edge_out)
+    bending_rest_angles: np.ndarray  # (B,)
+    bending_stiffness: np.ndarray    # (B,)
+    grid_shape: tuple             # (H, W) of the node domain
+    steric: np.ndarray            # (H, W)
+    fluid_velocity: np.ndarray    # (2, Hf, Wf)
+    f_grid: np.ndarray            # (9, Hf, Wf)
+    color_bend: Optional[np.ndarray] = None
+
+    def copy(self) -> "State":
+        return replace(self, **{k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v)
+                                for k, v in self.__dict__.items()})
+
+    def astype(self, dtype) -> "State":
+        fl = ("position", "velocity", "debug_vector", "theta", "rest_lengths",
+              "bending_rest_angles", "bending_stiffness", "steric", "fluid_velocity", "f_grid")
+        return replace(self, **{k: np.asarray(getattr(self, k), dtype=dtype) for k in fl})
+
+
+# --------------------------------------------------------------------------
+# Elementwise helpers
+# --------------------------------------------------------------------------
+def _tf(fn, x):
+    """Transcendental in the array's precision; float32 via a float64 evaluation."""
+    x = np.asarray(x)
+    if x.dtype == np.float32:
+        return fn(x.astype(np.float64)).astype(np.float32)
+    return fn(x)
+
+
+def _tf2(fn, a, b):
+    a = np.asarray(a)
+    if a.dtype == np.float32:
+        return fn(a.astype(np.float64), np.asarray(b).astype(np.float64)).astype(np.float32)
+    return fn(a, b)
+
+
+def _size_xy(grid_shape, dtype):
+    """utils.py:21,32 -- jnp.flip(jnp.asarray((H, W))) -> (W, H) as floats."""
+    h, w = grid_shape
+    return np.array([w, h], dtype=dtype)
+
+
+def displacement(grid_shape, x, y):
+    """utils.py:18-23 -- minimum-image displacement mod(x - y + S/2, S) - S/2."""
+    s = _size_xy(grid_shape, np.asarray(x).dtype)
+    half = s / 2.0
+    return np.mod(x - y + half, s) - half
+
+
+def periodic_boundary(grid_shape, position):
+    """utils.py:29-33 -- floor-mod of (x, y) by (W, H)."""
+    return np.mod(position, _size_xy(grid_shape, position.dtype))
+
+
+def index_from_position(position):
+    """utils.py:25-27 -- (iy, ix) = flip(floor(pos)).T as integers."""
+    return np.flip(np.floor(position).astype(np.int64), axis=-1).T
+
+
+def fftfreq32(n, dtype):
+    """jnp.fft.fftfreq(n) as JAX computes it: integer k cast to float, divided by n."""
+    k = np.concatenate([np.arange(0, (n + 1) // 2), np.arange(-(n // 2), 0)]).astype(dtype)
+    return k / np.asarray(n, dtype=dtype)
+
+
+def gaussian_kernel_f(rows, cols, sigma, dtype):
+    """utils.py:41-47 -- exp(-2 pi^2 sigma^2 (fr^2 + fc^2)) on the fftfreq grid."""
+    r = fftfreq32(rows, dtype)
+    c = fftfreq32(cols, dtype)
+    r, c = np.meshgrid(r, c, indexing="ij")
+    coef = -2 * np.pi ** 2 * sigma ** 2           # Python double, weak scalar
+    return _tf(np.exp, coef * (r ** 2 + c ** 2))
+
+
+def gaussian_filter(data, sigma):
+    """utils.py:35-49 -- Re(ifft2(fft2(data) * kernel_f))."""
+    rows, cols = data.shape
+    kf = gaussian_kernel_f(rows, cols, sigma, data.dtype)
+    out = np.fft.ifft2(np.fft.fft2(data) * kf).real
+    return out.astype(data.dtype)
+
+
+# --------------------------------------------------------------------------
+# Steric field (forces.py)
+# --------------------------------------------------------------------------
+def update_steric_potential(position, grid_shape, sigma, scatter_value):
+    """forces.py:84-118 -- CIC deposit (corner-major, node-minor) + gaussian_filter."""
+    dtype = position.dtype
+    h, w = grid_shape
+    steric = np.zeros((h, w), dtype=dtype)
+    pos = periodic_boundary(grid_shape, position)
+    x = pos[..., 0]
+    y = pos[..., 1]
+    fx = np.floor(x)
+    fy = np.floor(y)
+    x0 = np.mod(fx.astype(np.int32), w)
+    y0 = np.mod(fy.astype(np.int32), h)
+    x1 = np.mod(x0 + 1, w)
+    y1 = np.mod(y0 + 1, h)
+    dx = x - fx
+    dy = y - fy
+    w00 = (1.0 - dx) * (1.0 - dy) * scatter_value
+    w10 = dx * (1.0 - dy) * scatter_value
+    w01 = (1.0 - dx) * dy * scatter_value
+    w11 = dx * dy * scatter_value
+    np.add.at(steric, (y0, x0), w00)
+    np.add.at(steric, (y0, x1), w10)
+    np.add.at(steric, (y1, x0), w01)
+    np.add.at(steric, (y1, x1), w11)
+    return steric, gaussian_filter(steric, sigma)
+
+
+def sample_bilinear_wrap(V, yy, xx):
+    """jax.scipy.ndimage.map_coordinates(V, [yy, xx], order=1, mode="wrap").
+
+    Linear weights (1 - t, t) per axis, corners visited in itertools.product
+    order (y_lo,x_lo), (y_lo,x_hi), (y_hi,x_lo), (y_hi,x_hi), each weighted by
+    wy * wx and summed left to right; indices wrap by floor-mod.
+    """
+    h, w = V.shape
+    ly = np.floor(yy)
+    wy_hi = yy - ly
+    wy_lo = 1 - wy_hi
+    iy = ly.astype(np.int32)
+    lx = np.floor(xx)
+    wx_hi = xx - lx
+    wx_lo = 1 - wx_hi
+    ix = lx.astype(np.int32)
+    out = None
+    for jy, wy in ((iy, wy_lo), (iy + 1, wy_hi)):
+        for jx, wx in ((ix, wx_lo), (ix + 1, wx_hi)):
+            c = (wy * wx) * V[np.mod(jy, h), np.mod(jx, w)]
+            out = c if out is None else out + c
+    return out
+
+
+def force_interp(position, potential, fd_eps=0.5):
+    """forces.py:20-38 -- -grad V by central differences of bilinear samples."""
+    y = position[..., 1]
+    x = position[..., 0]
+    gx = (sample_bilinear_wrap(potential, y, x + fd_eps)
+          - sample_bilinear_wrap(potential, y, x - fd_eps)) / (2.0 * fd_eps)
+    gy = (sample_bilinear_wrap(potential, y + fd_eps, x)
+          - sample_bilinear_wrap(potential, y - fd_eps, x)) / (2.0 * fd_eps)
+    return -np.stack([gx, gy], axis=-1)
+
+
+def steric_window_subtraction(pos, grid_shape, sigma, neighbor_skip, scatter_value):
+    """forces.py:66-81 -- analytic (2n+1)-window contribution over the flat node
+    index (the window crosses filament boundaries, as in the reference)."""
+    dtype = pos.dtype
+    n = pos.shape[0]
+    offsets = np.arange(-neighbor_skip, neighbor_skip + 1)
+    j_idx = np.arange(n)[:, None] + offsets[None, :]
+    valid = ((j_idx >= 0) & (j_idx < n)).astype(dtype)
+    j_safe = np.clip(j_idx, 0, n - 1)
+    r = displacement(grid_shape, pos[:, None, :], pos[j_safe])
+    d2 = r[..., 0] * r[..., 0] + r[..., 1] * r[..., 1]
+    K = _tf(np.exp, -d2 / (2.0 * sigma * sigma)) / (2.0 * np.pi * sigma * sigma)
+    terms = (K * valid)[..., None] * r
+    acc = np.zeros((n, 2), dtype=dtype)
+    for k in range(terms.shape[1]):
+        acc = acc + terms[:, k, :]
+    return (scatter_value / (sigma * sigma)) * acc
+
+
+def compute_field_steric_force_corrected(position, grid_shape, sigma, neighbor_skip,
+                                         scatter_value=1.0):
+    """forces.py:41-81 -- returns (density_grid, steric_field, f_steric)."""
+    density, steric = update_steric_potential(position, grid_shape, sigma, scatter_value)
+    pos = periodic_boundary(grid_shape, position)
+    raw = force_interp(pos, steric)
+    corr = steric_window_subtraction(pos, grid_shape, sigma, neighbor_skip, scatter_value)
+    return density, steric, raw - corr
+
+
+# --------------------------------------------------------------------------
+# Rod solvers (solver/pbd.py, solver/stable_cosserat.py)
+# --------------------------------------------------------------------------
+def _edge_bending_degree(n_edges, bp, dtype):
+    deg = np.zeros(n_edges, dtype=dtype)
+    np.add.at(deg, bp[:, 0], 1.0)
+    np.add.at(deg, bp[:, 1], 1.0)
+    return np.maximum(deg, 1.0)
+
+
+def _theta_bending_delta(theta, bp, rest, kb):
+    """pbd.py:23-47 (without the final division) -- scatter of the wrapped
+    hinge correction, e_in gets +c, e_out gets -c, in hinge order."""
+    t_in = theta[bp[:, 0]]
+    t_out = theta[bp[:, 1]]
+    diff = t_out - t_in
+    diff = np.mod(diff + np.pi, 2 * np.pi) - np.pi
+    corr = (diff - rest) * kb * 0.5
+    delta = np.zeros_like(theta)
+    np.add.at(delta, bp[:, 0], corr)
+    np.add.at(delta, bp[:, 1], -corr)
+    return delta
+
+
+def pbd_rod_constraint(s: State, p: Params):
+    """pbd.py:14-91 -- one Jacobi iteration: theta pass (SOR 1.8), then the
+    stretch/shear position pass.  Updates s.theta and s.position in place."""
+    dtype = s.position.dtype
+    src = s.pairs[:, 0]
+    tgt = s.pairs[:, 1]
+    bp = s.bending_pairs
+    deg_e = _edge_bending_degree(s.theta.shape[0], bp, dtype)
+    delta = _theta_bending_delta(s.theta, bp, s.bending_rest_angles, s.bending_stiffness)
+    delta = delta / deg_e * 1.8
+    s.theta = s.theta + delta
+
+    n = s.position.shape[0]
+    deg_n = np.zeros(n, dtype=dtype)
+    np.add.at(deg_n, src, 1.0)
+    np.add.at(deg_n, tgt, 1.0)
+    deg_n = np.maximum(deg_n, 1.0)
+    dirs = np.stack([_tf(np.cos, s.theta), _tf(np.sin, s.theta)], axis=-1)
+    target = dirs * s.rest_lengths[:, None]
+    cur = displacement(s.grid_shape, s.position[tgt], s.position[src])
+    length = np.sqrt(cur[:, 0:1] * cur[:, 0:1] + cur[:, 1:2] * cur[:, 1:2])
+    cdir = cur / np.maximum(length, 1e-8)
+    stretch = cdir * (length - s.rest_lengths[:, None])
+    shear = (cur - target) - stretch
+    forces = (stretch + shear * p.stiffness_shear) * p.stiffness_stretch * 0.5
+    disp = np.zeros_like(s.position)
+    np.add.at(disp, src, forces)
+    np.add.at(disp, tgt, -forces)
+    disp = disp / deg_n[:, None]
+    s.position = s.position + disp
+
+
+def stable_cosserat_constraint(s: State, p: Params):
+    """stable_cosserat.py:13-119 -- theta bending pass + 0.05 shear pull, then the
+    implicit per-vertex update with inertia w = 1/config.dt^2 toward y = debug_vector."""
+    dtype = s.position.dtype
+    x = s.position
+    y = s.debug_vector
+    src = s.pairs[:, 0]
+    tgt = s.pairs[:, 1]
+    l = s.rest_lengths
+    bp = s.bending_pairs
+    deg_e = _edge_bending_degree(s.theta.shape[0], bp, dtype)
+    delta = _theta_bending_delta(s.theta, bp, s.bending_rest_angles, s.bending_stiffness)
+    delta = delta / deg_e
+    theta_new = s.theta + delta
+    seg = displacement(s.grid_shape, x[tgt], x[src])
+    alpha = _tf2(np.arctan2, seg[:, 1], seg[:, 0])
+    pull = np.mod(alpha - theta_new + np.pi, 2 * np.pi) - np.pi
+    theta_new = theta_new + pull * 0.05
+    s.theta = theta_new
+
+    k_s = p.stiffness_stretch
+    w = 1.0 / (p.config_dt ** 2)
+    d = np.stack([_tf(np.cos, theta_new), _tf(np.sin, theta_new)], axis=-1)
+    r = displacement(s.grid_shape, x[tgt], x[src])
+    c = r / l[:, None] - d
+    f_edge = (k_s / l)[:, None] * c
+    h_edge = k_s / (l ** 2)
+    f_node = w * (y - x)
+    h_node = np.full(x.shape[0], w, dtype=dtype)
+    np.add.at(f_node, src, f_edge)
+    np.add.at(f_node, tgt, -f_edge)
+    np.add.at(h_node, src, h_edge)
+    np.add.at(h_node, tgt, h_edge)
+    s.position = x + f_node / h_node[:, None]
+
+
+# --------------------------------------------------------------------------
+# Immersed boundary + LBM (solver/fluid.py)
+# --------------------------------------------------------------------------
+def kernel_1d_peskin(r, h):
+    """fluid.py:31-38 -- 0.25 (1 + cos(pi |r|/h / 2)) for |r|/h <= 2."""
+    r_abs = np.abs(r) / h
+    return np.where(r_abs <= 2.0, 0.25 * (1.0 + _tf(np.cos, np.pi * r_abs / 2.0)), 0.0).astype(r.dtype)
+
+
+def ib_stencil(grid_pos, h, w, kernel_size):
+    """fluid.py:40-87 -- per-node flat int32 cell indices and normalised weights,
+    shape (N, K*K).  Tap t has dx = t // K (outer) and dy = t % K (inner): the
+    first output of meshgrid(offsets, offsets) is assigned to dy (fluid.py:54)."""
+    dtype = grid_pos.dtype
+    k = kernel_size
+    half = (k - 1) // 2
+    half_width = k / 2.0
+    x = grid_pos[:, 0:1]
+    y = grid_pos[:, 1:2]
+    start_x = np.floor(x) - half
+    start_y = np.floor(y) - half
+    offsets = np.arange(float(k)).astype(dtype)
+    dy, dx = np.meshgrid(offsets, offsets)
+    nx = start_x + dx.ravel()[None, :]
+    ny = start_y + dy.ravel()[None, :]
+    wx = kernel_1d_peskin(nx - x, half_width)
+    wy = kernel_1d_peskin(ny - y, half_width)
+    weights = wx * wy
+    total = np.zeros_like(weights[:, 0])
+    for t in range(k * k):
+        total = total + weights[:, t]
+    weights = weights / (total + EPSIL)[:, None]
+    idx_x = np.mod(nx.astype(np.int32), w)
+    idx_y = np.mod(ny.astype(np.int32), h)
+    flat = (idx_y * w + idx_x).astype(np.int32)
+    return flat, weights
+
+
+def ib_interpolate(flat, weights, field2):
+    """fluid.py:90-97 -- per node sum_t w_t * field[:, flat_t], left fold over taps."""
+    vflat = field2.reshape(2, -1)
+    acc = np.zeros((flat.shape[0], 2), dtype=field2.dtype)
+    for t in range(flat.shape[1]):
+        acc = acc + vflat[:, flat[:, t]].T * weights[:, t:t + 1]
+    return acc
+
+
+def ib_spread(flat, weights, force, h, w):
+    """fluid.py:109-120 -- in-order scatter of force * weight (node-major, tap-minor)."""
+    nst = flat.shape[1]
+    idx = flat.reshape(-1)
+    wexp = weights.reshape(-1)
+    out = np.zeros((2, h * w), dtype=force.dtype)
+    np.add.at(out[0], idx, np.repeat(force[:, 0], nst) * wexp)
+    np.add.at(out[1], idx, np.repeat(force[:, 1], nst) * wexp)
+    return out.reshape(2, h, w)
+
+
+def ib_kernel_weight(flat, weights, h, w):
+    """fluid.py:123-133 -- in-order scatter of the weights."""
+    out = np.zeros(h * w, dtype=weights.dtype)
+    np.add.at(out, flat.reshape(-1), weights.reshape(-1))
+    return out.reshape(h, w)
+
+
+def macroscopic(f):
+    """fluid.py:136-142 -- rho = max(sum f, 0.1), u = (sum e f) / rho; left folds."""
+    rho = f[0]
+    for i in range(1, 9):
+        rho = rho + f[i]
+    rho = np.maximum(rho, 0.1)
+    mom = np.zeros((2,) + f.shape[1:], dtype=f.dtype)
+    for i in range(9):
+        for a in range(2):
+            e = int(LBM_VELOCITIES[i, a])
+            if e != 0:
+                mom[a] = mom[a] + e * f[i]
+    return rho, mom / rho
+
+
+def equilibrium(rho, u):
+    """fluid.py:145-151 -- rho w_i (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2)."""
+    dtype = rho.dtype
+    usq = u[0] * u[0] + u[1] * u[1]
+    out = np.empty((9,) + rho.shape, dtype=dtype)
+    for i in range(9):
+        eu = _edot(i, u)
+        out[i] = rho * dtype.type(LBM_WEIGHTS[i]) * (1.0 + 3.0 * eu + 4.5 * (eu * eu) - 1.5 * usq)
+    return out
+
+
+def _edot(i, v):
+    """e_i . v for a (2, ...) field; products with 0/+-1 are exact, two-term sum."""
+    ex, ey = (int(c) for c in LBM_VELOCITIES[i])
+    return ex * v[0] + ey * v[1]
+
+
+def _clamp_mag(u, umax):
+    """fluid.py:272-274 -- u * min(1, umax / (sqrt(|u|^2 + eps) + eps))."""
+    mag = np.sqrt(u[0] * u[0] + u[1] * u[1] + EPSIL)
+    return u * np.minimum(1.0, umax / (mag + EPSIL))[None]
+
+
+def trt_omegas(viscosity, lambda_trt, dtype):
+    """fluid.py:155-161,342-344 -- omega+ from nu (double, then clipped in the
+    array precision); omega- from the magic parameter in the array precision."""
+    op = np.clip(np.asarray(1.0 / (3.0 * viscosity + 0.5), dtype=dtype), 0.0, 1.9).astype(dtype)
+    tpo = 1.0 / np.maximum(op, EPSIL) - 0.5
+    om = 1.0 / (lambda_trt / np.maximum(tpo, EPSIL) + 0.5)
+    return dtype.type(op), dtype.type(om)
+
+
+def collide_trt_guo(f, rho, u, force, omega_plus, omega_minus):
+    """fluid.py:164-199,346-350 -- TRT collision about feq(rho, u*) plus the TRT
+    Guo source; u* = u + 0.5 F / rho; floor at EPSIL."""
+    u_star = u + 0.5 * force / rho
+    feq = equilibrium(rho, u_star)
+    dtype = f.dtype
+    raw = np.empty_like(f)
+    udf = u_star[0] * force[0] + u_star[1] * force[1]
+    for i in range(9):
+        eu = _edot(i, u_star)
+        ef = _edot(i, force)
+        raw[i] = dtype.type(LBM_WEIGHTS[i]) * (3.0 * (ef - udf) + 9.0 * eu * ef)
+    out = np.empty_like(f)
+    cp = 1.0 - 0.5 * omega_plus
+    cm = 1.0 - 0.5 * omega_minus
+    for i in range(9):
+        o = OPP_INDICES[i]
+        coll = (f[i] - omega_plus * (0.5 * (f[i] + f[o]) - 0.5 * (feq[i] + feq[o]))
+                - omega_minus * (0.5 * (f[i] - f[o]) - 0.5 * (feq[i] - feq[o])))
+        src = cp * (0.5 * (raw[i] + raw[o])) + cm * (0.5 * (raw[i] - raw[o]))
+        out[i] = coll + src
+    return np.maximum(out, EPSIL), u_star
+
+
+def stream(f_post):
+    """fluid.py:352-357 -- f_new[i][y, x] = f_post[i][y - e_iy, x - e_ix] (periodic)."""
+    out = np.empty_like(f_post)
+    for i in range(9):
+        ex, ey = LBM_VELOCITIES[i]
+        out[i] = np.roll(f_post[i], shift=(int(ey), int(ex)), axis=(0, 1))
+    return np.maximum(out, EPSIL)
+
+
+@dataclass
+class IBTrace:
+    """Intermediates of one immersed_boundary_interaction call (for tests)."""
+    flat: np.ndarray = None
+    weights: np.ndarray = None
+    kernel_weight: np.ndarray = None
+    u_corrected: np.ndarray = None
+    force: np.ndarray = None
+    rho: np.ndarray = None
+    velocity: np.ndarray = None
+    node_force: np.ndarray = None
+
+
+def immersed_boundary_interaction(position, velocity, grid_shape, f_grid, dt, p: Params,
+                                  trace: Optional[IBTrace] = None):
+    """fluid.py:202-386 -- IB multi-direct forcing + TRT/Guo collide + stream + drag.
+
+    Returns (new node velocity, fluid_velocity = u*, f_new).
+    """
+    if p.synthetic_node_width > 0.0:
+        raise NotImplementedError("synthetic_node_width > 0 (fluid.py:246-263)")
+    dtype = f_grid.dtype
+    h, w = f_grid.shape[1], f_grid.shape[2]
+    scale_x = w / grid_shape[0]
+    scale_y = h / grid_shape[1]
+    scale = np.array([scale_x, scale_y], dtype=dtype)
+    mean_scale = 0.5 * (scale_x + scale_y)
+    umax = p.max_fluid_velocity * mean_scale
+    ibm_vel = velocity * scale * dt
+    grid_pos = position * scale
+
+    f = np.maximum(f_grid, EPSIL)
+    rho, u = macroscopic(f)
+    u = _clamp_mag(u, umax)
+
+    flat, wts = ib_stencil(grid_pos, h, w, p.ibm_kernel_size)
+    kw = ib_kernel_weight(flat, wts, h, w)
+    inv_kw = 1.0 / np.maximum(kw[None], 1.0)
+
+    u_corr = u
+    for _ in range(p.ibm_iterations):
+        un = ib_interpolate(flat, wts, u_corr)
+        c = (ibm_vel - un) * p.ibm_relaxation
+        mag = np.sqrt(c[:, 0:1] * c[:, 0:1] + c[:, 1:2] * c[:, 1:2] + EPSIL)
+        c = c * np.minimum(1.0, umax / (mag + EPSIL))
+        delta = ib_spread(flat, wts, c, h, w) * inv_kw
+        u_corr = u_corr + delta
+
+    u_corr = _clamp_mag(u_corr, umax)
+    force = (u_corr - u) * rho
+    if p.enable_force_clamp:
+        force = _clamp_mag(force, umax)
+
+    op, om = trt_omegas(p.viscosity, p.lambda_trt, dtype)
+    f_post, u_star = collide_trt_guo(f, rho, u, force, op, om)
+    f_new = stream(f_post)
+
+    fn = ib_interpolate(flat, wts, force)
+    dv = (fn / p.node_mass) / (scale * dt)
+    if trace is not None:
+        trace.flat, trace.weights, trace.kernel_weight = flat, wts, kw
+        trace.u_corrected, trace.force, trace.rho, trace.velocity = u_corr, force, rho, u
+        trace.node_force = fn
+    return velocity - dv, u_star, f_new
+
+
+# --------------------------------------------------------------------------
+# The step (simulate.py)
+# --------------------------------------------------------------------------
+def step(s: State, dt: float, p: Params) -> State:
+    """simulate.py:37-100 -- one coupled step; returns a new State."""
+    s = s.copy()
+    s.debug_vector = np.zeros_like(s.position)
+    prev = s.position
+    s.velocity = s.velocity * p.damping
+    pred = prev + s.velocity * dt
+    if p.enable_steric:
+        _, s.steric, fst = compute_field_steric_force_corrected(
+            pred, s.grid_shape, p.steric_sigma, p.steric_neighbor_skip, p.steric_scatter_value)
+        pred = pred + fst * p.steric_strength * dt
+    s.position = pred
+    if p.solver_type == "cosserat":
+        s.debug_vector = pred
+    fn = stable_cosserat_constraint if p.solver_type == "cosserat" else pbd_rod_constraint
+    for _ in range(p.cycles_per_step):
+        fn(s, p)
+    s.velocity = (s.position - prev) / dt
+    if p.enable_fluid:
+        s.velocity, s.fluid_velocity, s.f_grid = immersed_boundary_interaction(
+            s.position, s.velocity, s.grid_shape, s.f_grid, dt, p)
+    s.position = periodic_boundary(s.grid_shape, s.position)
+    return s
+
+
+def simulate(s: State, dt: float, num_steps: int, p: Params, actuation=None):
+    """simulate.py:15-35 -- returns (final state, per-step positions, per-step u*).
+
+    ``actuation(t, state) -> bending_rest_angles`` (optional) is applied before
+    each step, the way the gym/worm callers set rest angles (gym/base.py:169-173).
+    """
+    pos_ts, vel_ts = [], []
+    for t in range(num_steps):
+        if actuation is not None:
+            s = replace(s, bending_rest_angles=actuation(t, s))
+        s = step(s, dt, p)
+        pos_ts.append(s.position)
+        vel_ts.append(s.fluid_velocity)
+    return s, np.stack(pos_ts), np.stack(vel_ts)
+
+
+# --------------------------------------------------------------------------
+# Topology and initial state (graph.py, env.py)
+# --------------------------------------------------------------------------
+def make_edges(num_nodes, graph="loop", num_instances=1):
+    """graph.py:156-259 -- concatenated per-instance edge lists + departure angles."""
+    per = num_nodes // num_instances
+    rem = num_nodes % num_instances
+    if per < 1:
+        raise ValueError("num_nodes must be at least num_instances")
+    edges, deps = [], []
+    start = 0
+    for i in range(num_instances):
+        size = per + (1 if i < rem else 0)
+        if graph == "loop":
+            src = np.arange(size) + start
+            e = np.stack([src, np.roll(src, -1)], axis=1)
+            d = np.zeros(size)
+        elif graph == "line":
+            src = np.arange(size - 1) + start
+            e = np.stack([src, src + 1], axis=1)
+            d = np.zeros(size - 1)
+        elif graph == "tree":
+            trunk = int(size * 0.6)
+            remn = size - trunk
+            b1 = remn // 2
+            b2 = remn - b1
+            t0, b1s = start, start + trunk
+            b2s = b1s + b1
+            junction = t0 + trunk // 2
+            el, dl = [], []
+            for k in range(trunk - 1):
+                el.append((t0 + k, t0 + k + 1)); dl.append(0.0)
+            el.append((junction, b1s)); dl.append(np.pi / 2.0)
+            for k in range(b1 - 1):
+                el.append((b1s + k, b1s + k + 1)); dl.append(np.pi / 2.0)
+            el.append((junction, b2s)); dl.append(-np.pi / 2.0)
+            for k in range(b2 - 1):
+                el.append((b2s + k, b2s + k + 1)); dl.append(-np.pi / 2.0)
+            e = np.array(el)
+            d = np.array(dl)
+        else:
+            raise ValueError(f"Unknown graph type: {graph}")
+        edges.append(e)
+        deps.append(d)
+        start += size
+    return (np.concatenate(edges).astype(np.int32),
+            np.concatenate(deps).astype(np.float32))
+
+
+def compute_bending_pairs(edge_pairs, departure_angles, num_nodes):
+    """graph.py:347-401 -- per node (ascending), every incoming x outgoing edge
+    pair in edge order; rest angle = dep[e_out] - dep[e_in]."""
+    ep = np.asarray(edge_pairs)
+    dep = np.asarray(departure_angles)
+    inc = [[] for _ in range(num_nodes)]
+    out = [[] for _ in range(num_nodes)]
+    for e in range(ep.shape[0]):
+        out[int(ep[e, 0])].append(e)
+        inc[int(ep[e, 1])].append(e)
+    bp, rest = [], []
+    for v in range(num_nodes):
+        for ei in inc[v]:
+            for eo in out[v]:
+                bp.append((ei, eo))
+                rest.append(float(dep[eo] - dep[ei]))
+    if not bp:
+        return np.zeros((0, 2), np.int32), np.zeros(0, np.float32)
+    return np.array(bp, np.int32), np.array(rest, np.float32)
+
+
+def initialize_f_from_velocity(velocity_field, density=1.0):
+    """graph.py:533-540 -- feq(density, u) with float32 lattice weights."""
+    dtype = velocity_field.dtype
+    u = velocity_field
+    usq = u[0] * u[0] + u[1] * u[1]
+    out = np.empty((9,) + u.shape[1:], dtype=dtype)
+    for i in range(9):
+        eu = _edot(i, u)
+        out[i] = dtype.type(density * LBM_WEIGHTS[i]) * (1.0 + 3.0 * eu + 4.5 * (eu * eu) - 1.5 * usq)
+    return out

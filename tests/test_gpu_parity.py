@@ -1,0 +1,192 @@
+"""Parity of the HIP path with the CPU restatement, stage by stage and for the
+whole coupled step, on seeded inputs.  Every call goes through the C ABI.
+
+Tolerances:
+  * integer work and in-order folds: bit-exact (CIC density, kernel weights);
+  * single operator calls whose only rounding difference is the FFT or a
+    double-evaluated transcendental: relative L2 <= 1e-6;
+  * the coupled step: relative L2 <= 1e-5 after 100 steps (the FP32 tolerance
+    the path is specified to), stated per result together with the FP32-vs-FP64
+    oracle drift of the same run.
+"""
+from __future__ import annotations
+
+import sys
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from oracle import step_ref as R
+from microcosmos_amd import workloads as W
+
+from .helpers import (device_config, device_state, max_ulp, oracle_params, oracle_state,
+                      pos_rel_l2, rel_l2, run_oracle)
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda", 0)
+
+
+def _np(t):
+    return t.detach().to("cpu").numpy()
+
+
+@pytest.mark.parametrize("shape,sigma", [((48, 48), 2.0), ((64, 96), 1.5), ((10, 10), 1.0),
+                                         ((128, 128), 2.0), ((50, 50), 2.0)])
+def test_gaussian_filter(dev, shape, sigma):
+    _say(f"gaussian_filter shape={shape} sigma={sigma}")
+    from microcosmos_amd.utils import gaussian_filter
+    rng = np.random.default_rng(3)
+    d = rng.random(shape).astype(np.float32)
+    got = _np(gaussian_filter(torch.as_tensor(d, device=dev), sigma))
+    exp64 = R.gaussian_filter(d.astype(np.float64), sigma)
+    exp32 = R.gaussian_filter(d, sigma)
+    e = rel_l2(got, exp64)
+    assert e <= 1e-6, (e, rel_l2(exp32, exp64))
+
+
+def test_gaussian_filter_reference_kat(dev):
+    """tests/test_utils.py:59-68: delta at (5,5) on 10x10 keeps its peak and symmetry."""
+    _say("gaussian_filter KAT 10x10")
+    from microcosmos_amd.utils import gaussian_filter
+    d = torch.zeros((10, 10), device=dev)
+    d[5, 5] = 1.0
+    f = _np(gaussian_filter(d, 1.0))
+    assert int(np.argmax(f)) == 55
+    assert np.allclose(f[5, 4], f[5, 6]) and np.allclose(f[4, 5], f[6, 5])
+
+
+@pytest.mark.parametrize("case", ["small48", "plain128"])
+def test_steric_field(dev, case):
+    _say(f"steric field {case}")
+    from microcosmos_amd.forces import compute_field_steric_force_corrected, deposit_density
+    wl = W.small() if case == "small48" else W.plain_128()
+    nodes, edges, fields = device_state(wl, dev)
+    dens = _np(deposit_density(nodes.position, wl.grid_shape, 1.0))
+    exp_dens, exp_st, exp_f = R.compute_field_steric_force_corrected(
+        wl.position, wl.grid_shape, 2.0, 5, 1.0)
+    # CIC deposit: integer cells + in-order fold -> bit-exact
+    assert np.array_equal(dens, exp_dens), max_ulp(dens, exp_dens)
+    f2, force = compute_field_steric_force_corrected(nodes, fields, 2.0, 5, 1.0)
+    _, st64, f64 = R.compute_field_steric_force_corrected(
+        wl.position.astype(np.float64), wl.grid_shape, 2.0, 5, 1.0)
+    assert rel_l2(_np(f2.steric), st64) <= 1e-6
+    assert rel_l2(_np(force), f64) <= 1e-5
+    assert rel_l2(_np(force), exp_f) <= 1e-5
+
+
+@pytest.mark.parametrize("solver", ["pbd", "cosserat"])
+def test_rod_solve(dev, solver):
+    _say(f"rod solve {solver}")
+    from microcosmos_amd._runtime import rod_iterations
+    wl = W.small(n_fil=3, n_nodes=20, seed=11)
+    nodes, edges, fields = device_state(wl, dev)
+    rng = np.random.default_rng(5)
+    y = wl.position + rng.normal(0, 0.2, wl.position.shape).astype(np.float32)
+    rest = (rng.normal(0, 0.3, wl.bending_rest_angles.shape)).astype(np.float32)
+    base = R.COSSERAT_SCHEME if solver == "cosserat" else R.PBD_SCHEME
+    p = replace(base, cycles_per_step=6)
+    s = oracle_state(wl)
+    s = replace(s, debug_vector=y, bending_rest_angles=rest)
+    fn = R.stable_cosserat_constraint if solver == "cosserat" else R.pbd_rod_constraint
+    for _ in range(6):
+        fn(s, p)
+    from microcosmos_amd import COSSERAT_SCHEME, PBD_SCHEME
+    cfg = COSSERAT_SCHEME if solver == "cosserat" else PBD_SCHEME
+    nodes = replace(nodes, debug_vector=torch.as_tensor(y, device=dev))
+    edges = replace(edges, bending_rest_angles=torch.as_tensor(rest, device=dev))
+    n2, e2 = rod_iterations((nodes, edges), cfg, wl.grid_shape, 6, solver)
+    gp, gt = _np(n2.position), _np(e2.theta)
+    assert max_ulp(gp, s.position) <= 2, max_ulp(gp, s.position)
+    assert max_ulp(gt, s.theta) <= 2, max_ulp(gt, s.theta)
+
+
+@pytest.mark.parametrize("K", [4, 2, 3])
+def test_ib_lbm_single_call(dev, K):
+    _say(f"IB+LBM single call K={K}")
+    from microcosmos_amd.solver.fluid import immersed_boundary_interaction
+    wl = W.small(grid=(48, 48), n_fil=3, n_nodes=24, seed=13)
+    rng = np.random.default_rng(21)
+    vel = rng.normal(0, 0.05, wl.position.shape).astype(np.float32)
+    wl.velocity = vel
+    p = oracle_params(wl, ibm_kernel_size=K)
+    tr = R.IBTrace()
+    v_exp, u_exp, f_exp = R.immersed_boundary_interaction(wl.position, vel, wl.grid_shape,
+                                                          wl.f_grid, 1.0, p, trace=tr)
+    nodes, edges, fields = device_state(wl, dev)
+    cfg = device_config(wl, ibm_kernel_size=K)
+    n2, fl2, f2 = immersed_boundary_interaction(nodes, edges, fields, fields.f_grid, 1.0, cfg)
+    gv, gu, gf = _np(n2.velocity), _np(fl2.fluid_velocity), _np(f2)
+    assert max_ulp(gf, f_exp) <= 4, max_ulp(gf, f_exp)
+    assert max_ulp(gu, u_exp) <= 4, max_ulp(gu, u_exp)
+    assert max_ulp(gv, v_exp) <= 4, max_ulp(gv, v_exp)
+
+
+def _compare_states(tag, g, o, o64):
+    gpos = _np(g[0].position)
+    gu = _np(g[2].fluid_velocity)
+    gf = _np(g[2].f_grid)
+    gth = _np(g[1].theta)
+    r = {
+        "pos": pos_rel_l2(o.grid_shape, gpos, o.position),
+        "u": rel_l2(gu, o.fluid_velocity),
+        "f": rel_l2(gf, o.f_grid),
+        "theta": rel_l2(gth, o.theta),
+        "steric": rel_l2(_np(g[2].steric), o.steric),
+    }
+    r64 = {"pos": pos_rel_l2(o.grid_shape, gpos, o64.position),
+           "u": rel_l2(gu, o64.fluid_velocity), "f": rel_l2(gf, o64.f_grid)}
+    d64 = {"pos": pos_rel_l2(o.grid_shape, o.position, o64.position),
+           "u": rel_l2(o.fluid_velocity, o64.fluid_velocity), "f": rel_l2(o.f_grid, o64.f_grid)}
+    _say(f"{tag}: gpu-vs-fp32 {r}  gpu-vs-fp64 {r64}  fp32-vs-fp64 {d64}")
+    return r
+
+
+@pytest.mark.parametrize("steps", [1, 10])
+def test_step_parity_small(dev, steps):
+    _say(f"step parity small steps={steps}")
+    from microcosmos_amd import step
+    wl = W.small()
+    p = oracle_params(wl)
+    o = run_oracle(wl, steps, p)
+    o64 = run_oracle(wl, steps, p, dtype=np.float64)
+    nodes, edges, fields = device_state(wl, dev)
+    cfg = device_config(wl)
+    for t in range(steps):
+        edges = replace(edges, bending_rest_angles=torch.as_tensor(wl.rest_angles_at(t), device=dev))
+        nodes, edges, fields = step(nodes, edges, fields, wl.dt, cfg)
+    r = _compare_states(f"small {steps} steps", (nodes, edges, fields), o, o64)
+    for k in ("pos", "u", "f", "theta"):
+        assert r[k] <= 1e-5, (k, r)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_rollout_parity_100(dev, use_graph):
+    _say(f"rollout parity 128^2 100 steps graph={use_graph}")
+    from microcosmos_amd import Rollout
+    wl = W.plain_128()
+    p = oracle_params(wl)
+    o = run_oracle(wl, 100, p)
+    o64 = run_oracle(wl, 100, p, dtype=np.float64)
+    nodes, edges, fields = device_state(wl, dev)
+    cfg = device_config(wl)
+    ro = Rollout(nodes, edges, fields, cfg, wl.dt,
+                 actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
+    ro.run(100, use_graph=use_graph)
+    torch.cuda.synchronize()
+    r = _compare_states("plain128 100 steps", (ro.nodes(), ro.edges_now(), ro.fields_now()), o, o64)
+    for k in ("pos", "u", "f", "theta"):
+        assert r[k] <= 1e-5, (k, r)
