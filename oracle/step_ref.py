@@ -23,8 +23,10 @@ Evaluation-order conventions (chosen to be reproducible on the GPU):
   * in float32 runs, cos/sin/atan2/exp are evaluated in float64 and rounded to
     float32 (a correctly rounded result); the device does the same.  XLA-CPU's
     own float32 polynomials are not reproduced (they differ by <= ~1 ulp);
-  * the FFT is numpy's pocketfft in the input precision (complex64 for float32
-    input), standing in for jnp.fft (ducc); their rounding is not reproduced.
+  * the 2-D FFT is restated as the device's Stockham radix-4/2/3 network
+    (rows, then columns, per-axis inverse scaling); it is pinned against
+    numpy's pocketfft in float64 and a brute-force DFT.  jnp.fft (ducc, FP32)
+    rounds differently; that difference is not reproduced.
 
 Parity status per function (how each is pinned; see tests/test_oracle_*.py):
   displacement, periodic_boundary, index_from_position  -- reference KATs
@@ -203,12 +205,122 @@ def gaussian_kernel_f(rows, cols, sigma, dtype):
     return _tf(np.exp, coef * (r ** 2 + c ** 2))
 
 
-def gaussian_filter(data, sigma):
-    """utils.py:35-49 -- Re(ifft2(fft2(data) * kernel_f))."""
+def gaussian_filter_pocketfft(data, sigma):
+    """utils.py:35-49 with numpy's FFT (used to pin the Stockham restatement)."""
     rows, cols = data.shape
     kf = gaussian_kernel_f(rows, cols, sigma, data.dtype)
     out = np.fft.ifft2(np.fft.fft2(data) * kf).real
     return out.astype(data.dtype)
+
+
+def fft_radices(n):
+    """Radix schedule of the device FFT: 4s, then 2, then odd factors <= 15."""
+    out = []
+    while n % 4 == 0:
+        out.append(4); n //= 4
+    while n % 2 == 0:
+        out.append(2); n //= 2
+    for p in range(3, 17, 2):
+        while n % p == 0:
+            out.append(p); n //= p
+    if n != 1:
+        raise ValueError("FFT length with a prime factor > 16")
+    return out
+
+
+def _twiddles(n, dtype):
+    a = -2.0 * np.pi * np.arange(n, dtype=np.float64) / n
+    return np.cos(a).astype(dtype), np.sin(a).astype(dtype)
+
+
+def stockham_fft(re, im, inverse=False):
+    """Unnormalised DFT along the last axis by the Stockham autosort network
+    (radix 4 / 2 / 3 / generic), one rounding per real operation, in the order
+    the device kernel performs them.  Batched over leading axes."""
+    dtype = re.dtype
+    n = re.shape[-1]
+    twr, twi = _twiddles(n, dtype)
+    if inverse:
+        twi = -twi
+    Ns = 1
+    for R in fft_radices(n):
+        nb = n // R
+        tstride = n // (Ns * R)
+        j = np.arange(nb)
+        k = j % Ns
+        vr, vi = [], []
+        for r in range(R):
+            xr = re[..., j + r * nb]
+            xi = im[..., j + r * nb]
+            if r > 0:
+                m = k > 0
+                wr = np.where(m, twr[(r * k * tstride) % n], 1).astype(dtype)
+                wi = np.where(m, twi[(r * k * tstride) % n], 0).astype(dtype)
+                yr = xr * wr - xi * wi
+                yi = xr * wi + xi * wr
+                xr = np.where(m, yr, xr)
+                xi = np.where(m, yi, xi)
+            vr.append(xr)
+            vi.append(xi)
+        if R == 2:
+            Vr = [vr[0] + vr[1], vr[0] - vr[1]]
+            Vi = [vi[0] + vi[1], vi[0] - vi[1]]
+        elif R == 4:
+            a0r, a0i = vr[0] + vr[2], vi[0] + vi[2]
+            a1r, a1i = vr[0] - vr[2], vi[0] - vi[2]
+            a2r, a2i = vr[1] + vr[3], vi[1] + vi[3]
+            dr, di = vr[1] - vr[3], vi[1] - vi[3]
+            a3r, a3i = (-di, dr) if inverse else (di, -dr)
+            Vr = [a0r + a2r, a1r + a3r, a0r - a2r, a1r - a3r]
+            Vi = [a0i + a2i, a1i + a3i, a0i - a2i, a1i - a3i]
+        elif R == 3:
+            s3 = dtype.type(0.866025403784438647)
+            t1r, t1i = vr[1] + vr[2], vi[1] + vi[2]
+            t2r, t2i = vr[1] - vr[2], vi[1] - vi[2]
+            mr = vr[0] - dtype.type(0.5) * t1r
+            mi = vi[0] - dtype.type(0.5) * t1i
+            str_, sti = (-(s3 * t2i), s3 * t2r) if inverse else (s3 * t2i, -(s3 * t2r))
+            Vr = [vr[0] + t1r, mr + str_, mr - str_]
+            Vi = [vi[0] + t1i, mi + sti, mi - sti]
+        else:
+            Vr, Vi = [], []
+            for q in range(R):
+                ar, ai = vr[0], vi[0]
+                for r in range(1, R):
+                    idx = ((r * q) % R) * nb
+                    wr, wi = twr[idx], twi[idx]
+                    ar = ar + (vr[r] * wr - vi[r] * wi)
+                    ai = ai + (vr[r] * wi + vi[r] * wr)
+                Vr.append(ar)
+                Vi.append(ai)
+        dst = (j // Ns) * Ns * R + k
+        ore = np.empty_like(re)
+        oim = np.empty_like(im)
+        for r in range(R):
+            ore[..., dst + r * Ns] = Vr[r]
+            oim[..., dst + r * Ns] = Vi[r]
+        re, im = ore, oim
+        Ns *= R
+    return re, im
+
+
+def gaussian_filter(data, sigma):
+    """utils.py:35-49 -- Re(ifft2(fft2(data) * kernel_f)), the 2-D DFTs done as
+    rows (x) forward, columns (y) forward, transfer function, columns inverse
+    scaled by 1/H, rows inverse scaled by 1/W (the device's order)."""
+    dtype = data.dtype
+    rows, cols = data.shape
+    kf = gaussian_kernel_f(rows, cols, sigma, dtype)
+    re, im = stockham_fft(data.astype(dtype), np.zeros_like(data, dtype=dtype))
+    re, im = stockham_fft(re.T.copy(), im.T.copy())           # columns, as rows of the transpose
+    kt = kf.T
+    re, im = re * kt, im * kt
+    re, im = stockham_fft(re, im, inverse=True)
+    sh = dtype.type(1.0 / rows)
+    re, im = (re * sh).T.copy(), (im * sh).T.copy()
+    re, im = stockham_fft(re, im, inverse=True)
+    sw = dtype.type(1.0 / cols)
+    return re * sw
 
 
 # --------------------------------------------------------------------------

@@ -52,10 +52,14 @@ def test_gaussian_filter(dev, shape, sigma):
     rng = np.random.default_rng(3)
     d = rng.random(shape).astype(np.float32)
     got = _np(gaussian_filter(torch.as_tensor(d, device=dev), sigma))
-    exp64 = R.gaussian_filter(d.astype(np.float64), sigma)
+    exp64 = R.gaussian_filter_pocketfft(d.astype(np.float64), sigma)
     exp32 = R.gaussian_filter(d, sigma)
+    # same Stockham network, same roundings as the restatement
+    assert rel_l2(got, exp32) <= 1e-7, (rel_l2(got, exp32), max_ulp(got, exp32))
+    # FP32 error against the float64 DFT, stated
     e = rel_l2(got, exp64)
-    assert e <= 1e-6, (e, rel_l2(exp32, exp64))
+    _say(f"  fp32 vs fp64 rel-L2 {e:.2e}")
+    assert e <= 1e-6
 
 
 def test_gaussian_filter_reference_kat(dev):
@@ -83,9 +87,14 @@ def test_steric_field(dev, case):
     f2, force = compute_field_steric_force_corrected(nodes, fields, 2.0, 5, 1.0)
     _, st64, f64 = R.compute_field_steric_force_corrected(
         wl.position.astype(np.float64), wl.grid_shape, 2.0, 5, 1.0)
+    assert rel_l2(_np(f2.steric), exp_st) <= 1e-7
+    assert rel_l2(_np(force), exp_f) <= 1e-6, max_ulp(_np(force), exp_f)
+    # the steric force is a difference of nearly equal samples: its FP32 error
+    # against float64 is stated and bounded by the restatement's own
+    e_gpu, e_o32 = rel_l2(_np(force), f64), rel_l2(exp_f, f64)
+    _say(f"  force rel-L2 vs fp64: device {e_gpu:.2e}, fp32 restatement {e_o32:.2e}")
+    assert e_gpu <= 1.5 * e_o32 + 1e-7
     assert rel_l2(_np(f2.steric), st64) <= 1e-6
-    assert rel_l2(_np(force), f64) <= 1e-5
-    assert rel_l2(_np(force), exp_f) <= 1e-5
 
 
 @pytest.mark.parametrize("solver", ["pbd", "cosserat"])
