@@ -53,7 +53,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 9 x 1024 x 1024 float32 device copy (37.7 MB read + "
+                         "37.7 MB written per launch) to calibrate HBM byte counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.rand((9, 1024, 1024), device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_launch": src.numel() * 4,
+                          "bytes_written_per_launch": src.numel() * 4, "launches": 20}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "
                                    "part of it (instances shard by whole lattice, see DESIGN.md)",

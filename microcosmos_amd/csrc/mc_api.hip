@@ -40,13 +40,13 @@ mc_status fail(mc_status st, const char* fmt, ...) {
                   __FILE__, __LINE__);                                                 \
   } while (0)
 
-// Radix schedule: 4s first, then 2, 3, 5, then remaining primes <= 16.
+// Radix schedule: 4s first, then 2, 3, 5, 7, 11, 13.
 bool factor(int n, std::vector<int>& out) {
   out.clear();
   if (n < 1) return false;
   while (n % 4 == 0) { out.push_back(4); n /= 4; }
   while (n % 2 == 0) { out.push_back(2); n /= 2; }
-  for (int p = 3; p <= 16; p += 2) {
+  for (int p = 3; p <= 13; p += 2) {
     while (n % p == 0) { out.push_back(p); n /= p; }
   }
   return n == 1;
@@ -71,6 +71,7 @@ struct mc_plan {
   float* cic_w = nullptr;
   int *row_cnt = nullptr, *row_off = nullptr, *row_fill = nullptr, *row_active = nullptr;
   int *row_nactive = nullptr, *row_list = nullptr, *row_sort = nullptr;
+  unsigned long long* row_keys = nullptr;
   float2* spec = nullptr;
   float2 *tw_w = nullptr, *tw_h = nullptr;
   int *radix_w = nullptr, *radix_h = nullptr;
@@ -82,7 +83,10 @@ struct mc_plan {
   float* f_tmp = nullptr;
   float* node_tmp = nullptr;  // (N, 2) scratch
   long long* tick = nullptr;
+  float* kf = nullptr;          // Gaussian transfer function table (H, W)
+  double kf_sigma = -1.0;       // sigma the table holds (-1: none)
   hipStream_t cap_stream = nullptr;
+  hipStream_t side_stream = nullptr;
 
   template <class T>
   mc_status alloc(T** p, size_t count, const char* what) {
@@ -202,12 +206,15 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
     einc_off.push_back((int)einc_flat.size());
   }
   // LDS budget: the largest component
-  size_t maxf = 0;
+  size_t maxb = 0;
   for (int c = 0; c < C; ++c) {
-    const size_t f = rod_scratch_floats(noff[c + 1] - noff[c], eoff[c + 1] - eoff[c], hoff[c + 1] - hoff[c]);
-    maxf = std::max(maxf, f);
+    const int nni = ninc_off[noff[c + 1]] - ninc_off[noff[c]];
+    const int nei = einc_off[eoff[c + 1]] - einc_off[eoff[c]];
+    const size_t b = rod_lds_bytes(noff[c + 1] - noff[c], eoff[c + 1] - eoff[c],
+                                   hoff[c + 1] - hoff[c], nni, nei);
+    maxb = std::max(maxb, b);
   }
-  p->rod_lds_bytes = maxf * sizeof(float);
+  p->rod_lds_bytes = maxb;
   p->rod_use_lds = p->rod_lds_bytes <= (size_t)p->lds_max ? 1 : 0;
   if (!p->rod_use_lds) MCCHK(p->alloc(&p->rod_gscratch, rod_scratch_floats(N, E, B), "rod scratch"));
   int *a0, *a1, *a2, *a3, *a4, *a5, *a6, *a7, *a8, *a9, *a10, *a11, *a12, *a13;
@@ -232,11 +239,11 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
 mc_status build_fft(mc_plan* p) {
   std::vector<int> rw, rh;
   if (!factor(p->W, rw) || !factor(p->H, rh)) {
-    p->fft_why = "steric FFT needs grid sides whose prime factors are all <= 16";
+    p->fft_why = "steric FFT needs grid sides whose prime factors are all <= 13";
     return MC_OK;
   }
-  const size_t row_lds = (size_t)2 * std::max(p->W, p->H) * sizeof(float2) + 16384;
-  if (row_lds > (size_t)p->lds_max || 2 * (size_t)p->H * sizeof(float2) > (size_t)p->lds_max - 1024) {
+  const size_t row_lds = (size_t)3 * std::max(p->W, p->H) * sizeof(float2) + 49152;
+  if (row_lds > (size_t)p->lds_max || 3 * (size_t)p->H * sizeof(float2) > (size_t)p->lds_max - 1024) {
     p->fft_why = "steric FFT row/column does not fit in LDS";
     return MC_OK;
   }
@@ -255,6 +262,7 @@ mc_status build_fft(mc_plan* p) {
   p->nrad_w = (int)rw.size();
   p->nrad_h = (int)rh.size();
   MCCHK(p->alloc(&p->spec, (size_t)p->H * p->W, "spectrum"));
+  MCCHK(p->alloc(&p->kf, (size_t)p->H * p->W, "transfer function"));
   p->fft_ok = true;
   return MC_OK;
 }
@@ -335,6 +343,16 @@ Consts derive(const mc_plan* p, const mc_params* q, double dt) {
   return c;
 }
 
+// Tabulates the Gaussian transfer function for `sigma` (utils.py:41-47) when
+// the plan does not hold it yet.  Called before any stream capture.
+mc_status ensure_kf(mc_plan* p, double sigma, hipStream_t s) {
+  if (!p->fft_ok || p->kf_sigma == sigma) return MC_OK;
+  const float coef = (float)(-2.0 * (M_PI * M_PI) * (sigma * sigma));  // utils.py:47
+  HIPCHK(launch_gauss_table(p->H, p->W, coef, p->kf, s));
+  p->kf_sigma = sigma;
+  return MC_OK;
+}
+
 // Steric pipeline on `pos` (+ optional velocity prediction): fills p->pred_w,
 // writes the filtered field to steric_out.
 mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float* vel,
@@ -344,16 +362,29 @@ mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float*
   HIPCHK(launch_bin_scan(p->H, p->row_cnt, p->row_off, p->row_fill, p->row_active, p->row_nactive, s));
   HIPCHK(launch_row_fill(c, p->N, p->cic_cell, p->row_off, p->row_fill, p->row_list, s));
   HIPCHK(launch_deposit_rows_fft(c, p->N, p->row_off, p->row_list, p->cic_cell, p->cic_w, p->spec,
-                                 p->tw_w, p->radix_w, p->nrad_w, density_out, p->row_sort, s));
-  HIPCHK(launch_fft_cols_filter(p->H, p->W, p->spec, p->tw_h, p->radix_h, p->nrad_h, c.fft_coef,
+                                 p->tw_w, p->radix_w, p->nrad_w, density_out, p->row_sort,
+                                 p->row_keys, s));
+  HIPCHK(launch_fft_cols_filter(p->H, p->W, p->spec, p->tw_h, p->radix_h, p->nrad_h, p->kf,
                                 p->lds_max - 1024, s));
   HIPCHK(launch_ifft_rows_real(p->H, p->W, p->spec, steric_out, p->tw_w, p->radix_w, p->nrad_w, s));
   return MC_OK;
 }
 
+// Streams and events of one enqueued step.  With a side stream the fused LBM
+// pass of this step runs there, concurrently with the drag of this step and
+// the steric / rod / IB binning of the next step; only the next step's IB
+// solve (which reads the new lattice) waits for it.
+struct StepSync {
+  hipStream_t b = nullptr;          // side stream for the LBM pass (nullptr: main stream)
+  hipEvent_t ev_force = nullptr;    // main -> side: IB force of this step is ready
+  hipEvent_t ev_lbm = nullptr;      // side -> main: this step's lattice is written
+  hipEvent_t wait_lbm = nullptr;    // previous step's ev_lbm (nullptr: nothing pending)
+  hipEvent_t t0 = nullptr, t1 = nullptr;  // optional timing around the LBM launch
+};
+
 mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc_state* in,
                        const mc_state* out, const mc_edge_params* ed, const Actuation* act,
-                       hipStream_t s, hipEvent_t ev_lbm0 = nullptr, hipEvent_t ev_lbm1 = nullptr) {
+                       hipStream_t s, const StepSync& sy = StepSync()) {
   MCCHK(run_steric(p, c, in->position, in->velocity, out->steric, nullptr, s));
   if (!c.enable_steric && out->steric != in->steric)
     HIPCHK(hipMemcpyAsync(out->steric, in->steric, (size_t)p->H * p->W * sizeof(float),
@@ -363,13 +394,25 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
                     ed->bending_rest_angles, ed->bending_stiffness, act, out->steric, p->pred_w,
                     out->position, out->velocity, out->debug_vector, out->theta, s));
   const size_t hwf = (size_t)p->Hf * p->Wf;
+  long long* tick = act ? p->tick : nullptr;
   if (c.enable_fluid) {
-    HIPCHK(launch_ib(c, p->ib, in->f_grid, out->position, out->velocity, out->position, s));
-    if (ev_lbm0) HIPCHK(hipEventRecord(ev_lbm0, s));
+    HIPCHK(launch_ib_bin(c, p->ib, out->position, out->velocity, s));
+    if (sy.wait_lbm) HIPCHK(hipStreamWaitEvent(s, sy.wait_lbm, 0));
+    HIPCHK(launch_ib_solve(c, p->ib, in->f_grid, s));
+    hipStream_t ls = s;
+    if (sy.b) {
+      HIPCHK(hipEventRecord(sy.ev_force, s));
+      HIPCHK(hipStreamWaitEvent(sy.b, sy.ev_force, 0));
+      ls = sy.b;
+    }
+    if (sy.t0) HIPCHK(hipEventRecord(sy.t0, ls));
     HIPCHK(launch_lbm(c, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
-                      p->ib.mask_words_per_row, p->ib.force, s));
-    if (ev_lbm1) HIPCHK(hipEventRecord(ev_lbm1, s));
+                      p->ib.mask_words_per_row, p->ib.force, ls));
+    if (sy.t1) HIPCHK(hipEventRecord(sy.t1, ls));
+    if (sy.b) HIPCHK(hipEventRecord(sy.ev_lbm, sy.b));
+    HIPCHK(launch_ib_drag(c, p->ib, out->position, out->velocity, out->position, tick, s));
   } else {
+    if (tick) HIPCHK(launch_tick(tick, s));
     if (out->f_grid != in->f_grid)
       HIPCHK(hipMemcpyAsync(out->f_grid, in->f_grid, 9 * hwf * sizeof(float), hipMemcpyDeviceToDevice, s));
     if (out->fluid_velocity != in->fluid_velocity)
@@ -424,7 +467,7 @@ mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t 
   HIPCHK(hipGetDevice(&p->dev));
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->dev) == hipSuccess && lds > 0)
-    p->lds_max = std::min(lds, 65536);
+    p->lds_max = lds;
   p->H = grid_h;
   p->W = grid_w;
   p->Hf = fluid_h;
@@ -456,6 +499,7 @@ mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t 
   if ((st = p->alloc(&p->row_nactive, 1, "row n")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_list, 2 * (size_t)N, "row list")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_sort, 2 * (size_t)N, "row sort")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_keys, 8 * (size_t)N, "row keys")) != MC_OK) return bail(st);
   // immersed boundary
   IBWork& w = p->ib;
   w.N = N;
@@ -484,6 +528,8 @@ mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t 
   if ((st = p->alloc(&w.inv, hwf, "inv U")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.force, 2 * hwf, "force U")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.mask, (size_t)w.mask_words_per_row * p->Hf, "U mask")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.cell_span, hwf, "U spans")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.entries, taps * tpn * N, "U entries")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->f_tmp, 9 * hwf, "f ping-pong")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->tick, 1, "tick")) != MC_OK) return bail(st);
   (void)hw;
@@ -495,6 +541,7 @@ mc_status mc_plan_destroy(mc_plan* p) {
   if (!p) return MC_OK;
   for (void* q : p->allocs) (void)hipFree(q);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  if (p->side_stream) (void)hipStreamDestroy(p->side_stream);
   delete p;
   return MC_OK;
 }
@@ -518,6 +565,7 @@ mc_status mc_step(mc_plan* p, const mc_params* q, const mc_state* in, const mc_s
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
   hipStream_t s = (hipStream_t)stream;
   const Consts c = derive(p, q, dt);
+  if (q->enable_steric) MCCHK(ensure_kf(p, q->steric_sigma, s));
   return enqueue_step(p, q, c, in, out, ed, nullptr, s);
 }
 
@@ -537,6 +585,7 @@ mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const m
   hipStream_t us = (hipStream_t)stream;
   const Consts c = derive(p, q, dt);
   const size_t hwf = (size_t)p->Hf * p->Wf;
+  if (q->enable_steric) MCCHK(ensure_kf(p, q->steric_sigma, us));
   Actuation A{};
   const Actuation* ap = nullptr;
   if (act) {
@@ -549,66 +598,84 @@ mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const m
   // ping-pong: even steps read state->f_grid, odd steps read f_tmp
   mc_state sa = *st, sb = *st;
   sb.f_grid = p->f_tmp;
-  auto one = [&](int parity, hipStream_t s) -> mc_status {
-    const mc_state* in = parity ? &sb : &sa;
-    const mc_state* out = parity ? &sa : &sb;
-    MCCHK(enqueue_step(p, q, c, in, out, ed, ap, s));
-    if (ap) HIPCHK(launch_tick(p->tick, s));
-    return MC_OK;
-  };
   auto record = [&](int64_t k, hipStream_t s) -> mc_status {
     if (rec_pos && p->N > 0)
       HIPCHK(hipMemcpyAsync(rec_pos + (size_t)k * 2 * p->N, st->position, 2 * (size_t)p->N * sizeof(float),
                             hipMemcpyDeviceToDevice, s));
-    if (rec_vel) {
-      // after step k the u* field is in the output state of that step; both
-      // parities share st->fluid_velocity
+    if (rec_vel)  // both parities write u* into st->fluid_velocity
       HIPCHK(hipMemcpyAsync(rec_vel + (size_t)k * 2 * hwf, st->fluid_velocity, 2 * hwf * sizeof(float),
                             hipMemcpyDeviceToDevice, s));
-    }
     return MC_OK;
   };
   const bool recording = rec_pos || rec_vel;
-  if (!use_graph) {
+  if (!use_graph || recording) {
+    // step by step on the caller's stream (records between steps)
     for (int64_t k = 0; k < num_steps; ++k) {
-      MCCHK(one((int)(k & 1), us));
+      const int par = (int)(k & 1);
+      MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, us));
       MCCHK(record(k, us));
     }
   } else {
     if (!p->cap_stream) HIPCHK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
-    hipStream_t cs = p->cap_stream;
-    hipEvent_t ev;
-    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    HIPCHK(hipEventRecord(ev, us));
-    HIPCHK(hipStreamWaitEvent(cs, ev, 0));
-    // graphs: one per parity when recording, else one pair graph
-    hipGraphExec_t ge[2] = {nullptr, nullptr};
-    const int ngraph = recording ? 2 : 1;
-    for (int gi = 0; gi < ngraph; ++gi) {
+    if (!p->side_stream) HIPCHK(hipStreamCreateWithFlags(&p->side_stream, hipStreamNonBlocking));
+    hipStream_t cs = p->cap_stream, bs = p->side_stream;
+    // one graph holds S steps (S even keeps the lattice parity); inside it
+    // the LBM pass of step k overlaps step k+1 up to its IB solve
+    const int64_t S = num_steps >= 8 ? 8 : (num_steps >= 2 ? 2 : 0);
+    std::vector<hipEvent_t> evs;
+    auto mkev = [&](hipEvent_t* e) -> mc_status {
+      HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+      evs.push_back(*e);
+      return MC_OK;
+    };
+    hipEvent_t ev_in;
+    MCCHK(mkev(&ev_in));
+    HIPCHK(hipEventRecord(ev_in, us));
+    HIPCHK(hipStreamWaitEvent(cs, ev_in, 0));
+    hipGraphExec_t ge = nullptr;
+    mc_status r = MC_OK;
+    if (S > 0) {
+      std::vector<hipEvent_t> evf(S), evl(S);
+      for (int64_t k = 0; k < S && r == MC_OK; ++k) {
+        r = mkev(&evf[k]);
+        if (r == MC_OK) r = mkev(&evl[k]);
+      }
+      if (r != MC_OK) return r;
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-      mc_status r = one(recording ? gi : 0, cs);
-      if (r == MC_OK && !recording) r = one(1, cs);
+      for (int64_t k = 0; k < S && r == MC_OK; ++k) {
+        const int par = (int)(k & 1);
+        StepSync sy;
+        if (q->enable_fluid) {
+          sy.b = bs;
+          sy.ev_force = evf[k];
+          sy.ev_lbm = evl[k];
+          sy.wait_lbm = k > 0 ? evl[k - 1] : nullptr;
+        }
+        r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, sy);
+      }
+      if (r == MC_OK && q->enable_fluid) {
+        hipError_t e = hipStreamWaitEvent(cs, evl[S - 1], 0);  // join the side stream
+        if (e != hipSuccess) r = fail(MC_ERR_HIP, "join: %s", hipGetErrorString(e));
+      }
       hipError_t ec = hipStreamEndCapture(cs, &g);
       if (r != MC_OK) return r;
       if (ec != hipSuccess) return fail(MC_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
-      HIPCHK(hipGraphInstantiate(&ge[gi], g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
       HIPCHK(hipGraphDestroy(g));
+      for (int64_t k = 0; k + S <= num_steps; k += S) HIPCHK(hipGraphLaunch(ge, cs));
     }
-    if (recording) {
-      for (int64_t k = 0; k < num_steps; ++k) {
-        HIPCHK(hipGraphLaunch(ge[k & 1], cs));
-        MCCHK(record(k, cs));
-      }
-    } else {
-      for (int64_t k = 0; k + 1 < num_steps; k += 2) HIPCHK(hipGraphLaunch(ge[0], cs));
-      if (num_steps & 1) MCCHK(one(0, cs));
+    // remaining steps (fewer than S) without a graph, on the capture stream
+    const int64_t done = S > 0 ? (num_steps / S) * S : 0;
+    for (int64_t k = done; k < num_steps; ++k) {
+      const int par = (int)(k & 1);
+      MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs));
     }
-    HIPCHK(hipEventRecord(ev, cs));
-    HIPCHK(hipStreamWaitEvent(us, ev, 0));
+    HIPCHK(hipEventRecord(ev_in, cs));
+    HIPCHK(hipStreamWaitEvent(us, ev_in, 0));
     HIPCHK(hipStreamSynchronize(cs));
-    for (int gi = 0; gi < ngraph; ++gi) HIPCHK(hipGraphExecDestroy(ge[gi]));
-    HIPCHK(hipEventDestroy(ev));
+    if (ge) HIPCHK(hipGraphExecDestroy(ge));
+    for (auto e : evs) (void)hipEventDestroy(e);
   }
   if (num_steps & 1) {
     // final lattice is in f_tmp; bring it back into the caller's buffer
@@ -630,6 +697,7 @@ mc_status mc_profile_step(mc_plan* p, const mc_params* q, const mc_state* st,
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
   hipStream_t s = (hipStream_t)stream;
   const Consts c = derive(p, q, dt);
+  if (q->enable_steric) MCCHK(ensure_kf(p, q->steric_sigma, s));
   Actuation A{};
   const Actuation* ap = nullptr;
   if (act) {
@@ -647,8 +715,10 @@ mc_status mc_profile_step(mc_plan* p, const mc_params* q, const mc_state* st,
   for (int64_t k = 0; k < num_steps; ++k) {
     const mc_state* in = (k & 1) ? &sb : &sa;
     const mc_state* out = (k & 1) ? &sa : &sb;
-    MCCHK(enqueue_step(p, q, c, in, out, ed, ap, s, ev[3 * k + 1], ev[3 * k + 2]));
-    if (ap) HIPCHK(launch_tick(p->tick, s));
+    StepSync sy;
+    sy.t0 = ev[3 * k + 1];
+    sy.t1 = ev[3 * k + 2];
+    MCCHK(enqueue_step(p, q, c, in, out, ed, ap, s, sy));
     HIPCHK(hipEventRecord(ev[3 * k + 3], s));
   }
   HIPCHK(hipEventSynchronize(ev[3 * num_steps]));
@@ -676,9 +746,10 @@ mc_status mc_gaussian_filter(mc_plan* p, const float* data, float* out, double s
   if (!p || !data || !out) return fail(MC_ERR_INVALID, "NULL argument");
   if (!p->fft_ok) return fail(MC_ERR_UNSUPPORTED, "%s (grid %dx%d)", p->fft_why.c_str(), p->H, p->W);
   hipStream_t s = (hipStream_t)stream;
-  const float coef = (float)(-2.0 * (M_PI * M_PI) * (sigma * sigma));
+  MCCHK(ensure_kf(p, sigma, s));
   HIPCHK(launch_fft_rows(p->H, p->W, data, p->spec, p->tw_w, p->radix_w, p->nrad_w, s));
-  HIPCHK(launch_fft_cols_filter(p->H, p->W, p->spec, p->tw_h, p->radix_h, p->nrad_h, coef, p->lds_max - 1024, s));
+  HIPCHK(launch_fft_cols_filter(p->H, p->W, p->spec, p->tw_h, p->radix_h, p->nrad_h, p->kf,
+                                p->lds_max - 1024, s));
   HIPCHK(launch_ifft_rows_real(p->H, p->W, p->spec, out, p->tw_w, p->radix_w, p->nrad_w, s));
   return MC_OK;
 }
@@ -692,6 +763,7 @@ mc_status mc_steric_field(mc_plan* p, const mc_params* q, const float* position,
   MCCHK(check_params(p, &qq, false, true));
   hipStream_t s = (hipStream_t)stream;
   const Consts c = derive(p, &qq, 1.0);
+  MCCHK(ensure_kf(p, qq.steric_sigma, s));
   MCCHK(run_steric(p, c, position, nullptr, steric_out, density_out, s));
   if (force_out) HIPCHK(launch_steric_nodes(c, p->N, steric_out, p->pred_w, force_out, s));
   return MC_OK;
@@ -712,8 +784,10 @@ mc_status mc_ib_lbm_step(mc_plan* p, const mc_params* q, const float* position, 
   const Consts c = derive(p, &qq, dt);
   if (p->N > 0 && velocity_out != velocity)
     HIPCHK(hipMemcpyAsync(velocity_out, velocity, 2 * (size_t)p->N * sizeof(float), hipMemcpyDeviceToDevice, s));
-  HIPCHK(launch_ib(c, p->ib, f_in, position, velocity_out, pos_out ? pos_out : p->node_tmp, s));
+  HIPCHK(launch_ib_bin(c, p->ib, position, velocity_out, s));
+  HIPCHK(launch_ib_solve(c, p->ib, f_in, s));
   HIPCHK(launch_lbm(c, f_in, f_out, fluid_velocity_out, p->ib.mask, p->ib.mask_words_per_row, p->ib.force, s));
+  HIPCHK(launch_ib_drag(c, p->ib, position, velocity_out, pos_out ? pos_out : p->node_tmp, nullptr, s));
   return MC_OK;
 }
 

@@ -41,7 +41,8 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int* wsum, int* total
 }
 
 // Number of elements of sorted array a[0..n) that are < v.
-__device__ __forceinline__ int lower_bound_i(const int* a, int n, int v) {
+template <class T>
+__device__ __forceinline__ int lower_bound_t(const T* a, int n, T v) {
   int lo = 0, hi = n;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
@@ -50,21 +51,22 @@ __device__ __forceinline__ int lower_bound_i(const int* a, int n, int v) {
   return lo;
 }
 
-// Sort m unique ids in place (ascending).  All threads of the workgroup call it.
-// lds: SORT_LDS_CAP ints; gtmp: m ints of global scratch.
-__device__ inline void sort_unique_ids(int* list, int m, int* lds, int* gtmp) {
+// Sort m unique keys in place (ascending).  All threads of the workgroup call
+// it.  lds: `cap` keys of LDS; gtmp: m keys of global scratch (used beyond cap).
+template <class T>
+__device__ inline void sort_unique(T* list, int m, T* lds, int cap, T* gtmp, T pad) {
   if (m <= 1) return;
-  if (m <= SORT_LDS_CAP) {
+  if (m <= cap) {
     int P = 1;
     while (P < m) P <<= 1;
-    for (int i = threadIdx.x; i < P; i += blockDim.x) lds[i] = (i < m) ? list[i] : 0x7fffffff;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) lds[i] = (i < m) ? list[i] : pad;
     __syncthreads();
     for (int k = 2; k <= P; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
         for (int i = threadIdx.x; i < P; i += blockDim.x) {
           int ixj = i ^ j;
           if (ixj > i) {
-            int a = lds[i], b = lds[ixj];
+            T a = lds[i], b = lds[ixj];
             bool up = ((i & k) == 0);
             if ((a > b) == up) { lds[i] = b; lds[ixj] = a; }
           }
@@ -76,29 +78,33 @@ __device__ inline void sort_unique_ids(int* list, int m, int* lds, int* gtmp) {
     __syncthreads();
     return;
   }
-  // bottom-up merge by rank in the partner run (ids are unique)
-  int* src = list;
-  int* dst = gtmp;
+  // bottom-up merge by rank in the partner run (keys are unique)
+  T* src = list;
+  T* dst = gtmp;
   for (int width = 1; width < m; width <<= 1) {
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
       int start = (i / (2 * width)) * (2 * width);
       int mid = min(start + width, m);
       int end = min(start + 2 * width, m);
-      int v = src[i];
+      T v = src[i];
       int pos;
-      if (i < mid) pos = (i - start) + lower_bound_i(src + mid, end - mid, v);
-      else pos = (i - mid) + lower_bound_i(src + start, mid - start, v);
+      if (i < mid) pos = (i - start) + lower_bound_t(src + mid, end - mid, v);
+      else pos = (i - mid) + lower_bound_t(src + start, mid - start, v);
       dst[start + pos] = v;
     }
     __threadfence_block();
     __syncthreads();
-    int* t = src; src = dst; dst = t;
+    T* t = src; src = dst; dst = t;
   }
   if (src != list) {
     for (int i = threadIdx.x; i < m; i += blockDim.x) list[i] = src[i];
     __threadfence_block();
     __syncthreads();
   }
+}
+
+__device__ inline void sort_unique_ids(int* list, int m, int* lds, int* gtmp) {
+  sort_unique<int>(list, m, lds, SORT_LDS_CAP, gtmp, 0x7fffffff);
 }
 
 }  // namespace mc

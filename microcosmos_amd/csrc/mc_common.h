@@ -15,12 +15,13 @@
 
 namespace mc {
 
-// numpy / jnp.maximum semantics: NaN propagates.
-__device__ __forceinline__ float fmax_nan(float a, float b) { return (a > b || a != a) ? a : b; }
-__device__ __forceinline__ float fmin_nan(float a, float b) { return (a < b || a != a) ? a : b; }
+// numpy / jnp.maximum and minimum semantics (NaN propagates) where the second
+// operand is a finite constant: one compare and one select each.
+__device__ __forceinline__ float fmax_nan(float a, float c) { return (a < c) ? c : a; }
+__device__ __forceinline__ float fmin_nan(float c, float a) { return (a >= c) ? c : a; }
 
 // Floor-mod of floats (jnp.mod / np.mod): fmod, then add y when the signs differ.
-__device__ __forceinline__ float fmodf_floor(float x, float y) {
+__device__ __forceinline__ float fmodf_floor_slow(float x, float y) {
   float r = fmodf(x, y);
   if (r != 0.0f) {
     if ((r < 0.0f) != (y < 0.0f)) r = r + y;
@@ -29,6 +30,27 @@ __device__ __forceinline__ float fmodf_floor(float x, float y) {
   }
   return r;
 }
+
+// Same result for a positive modulus y, without fmodf on the common ranges:
+// for x in [0, y) fmod is x; for x in [y, 2y) it is x - y (exact, Sterbenz);
+// for x in [-y, 0) fmod is x and the sign fix-up adds y (one rounding, as in
+// np.mod).  A zero result is +0, as np.mod returns.
+__device__ __forceinline__ float fmodf_floor(float x, float y) {
+  float r;
+  if (x >= 0.0f) {
+    if (x < y) r = x;
+    else if (x < 2.0f * y) r = x - y;
+    else return fmodf_floor_slow(x, y);
+  } else if (x >= -y) {
+    r = x + y;
+  } else {
+    return fmodf_floor_slow(x, y);
+  }
+  return r == 0.0f ? 0.0f : r;
+}
+
+// Floor-mod of an int already known to lie in [-m, 2m).
+__device__ __forceinline__ int wrap1(int a, int m) { return a < 0 ? a + m : (a >= m ? a - m : a); }
 
 // Floor-mod of ints (Python/numpy semantics for a positive modulus).
 __device__ __forceinline__ int imod(int a, int m) {

@@ -61,11 +61,13 @@ hipError_t launch_row_fill(const Consts& c, int N, const int* cic_cell, const in
 hipError_t launch_deposit_rows_fft(const Consts& c, int N, const int* row_off, int* row_list,
                                    const int* cic_cell, const float* cic_w, float2* spec,
                                    const float2* tw_w, const int* radix_w, int nrad_w,
-                                   float* density_out, int* sort_scratch, hipStream_t s);
+                                   float* density_out, int* sort_scratch,
+                                   unsigned long long* key_scratch, hipStream_t s);
 hipError_t launch_fft_rows(int H, int W, const float* real_in, float2* spec,
                            const float2* tw_w, const int* radix_w, int nrad_w, hipStream_t s);
+hipError_t launch_gauss_table(int H, int W, float coef, float* kf, hipStream_t s);
 hipError_t launch_fft_cols_filter(int H, int W, float2* spec, const float2* tw_h,
-                                  const int* radix_h, int nrad_h, float coef, int lds_bytes_max,
+                                  const int* radix_h, int nrad_h, const float* kf, int lds_bytes_max,
                                   hipStream_t s);
 hipError_t launch_ifft_rows_real(int H, int W, const float2* spec, float* out,
                                  const float2* tw_w, const int* radix_w, int nrad_w,
@@ -84,6 +86,7 @@ hipError_t launch_steric_nodes(const Consts& c, int N, const float* steric, cons
                                float* force_out, hipStream_t s);
 hipError_t launch_tick(long long* t, hipStream_t s);
 size_t rod_scratch_floats(int nn, int ne, int nb);
+size_t rod_lds_bytes(int nn, int ne, int nb, int nni, int nei);
 // immersed boundary
 struct IBWork {
   int N, T, tiles_x, tiles_y;
@@ -96,12 +99,20 @@ struct IBWork {
   float *rho, *u0, *ucorr, *inv, *force;   // dense (H*W) / (2*H*W) on the fluid grid
   uint32_t* mask;              // per fluid row ceil(W/32) words
   int mask_words_per_row;
+  int2* cell_span;             // per covered cell: (first entry, entry count)
+  int2* entries;               // ordered (node, weight bits) lists of covered cells
 };
-hipError_t launch_ib(const Consts& c, IBWork& w, const float* f_in, const float* pos,
-                     float* vel_io, float* pos_out, hipStream_t s);
+// Immersed boundary in two parts: the f-independent binning (stencil, tile
+// scan, fill), then the support / iterations / force (reads f_in).  The drag
+// and final wrap run separately so the LBM pass can overlap them; `tick`
+// (may be NULL) is advanced once by the drag launch.
+hipError_t launch_ib_bin(const Consts& c, IBWork& w, const float* pos, float* vel_io,
+                         hipStream_t s);
+hipError_t launch_ib_solve(const Consts& c, IBWork& w, const float* f_in, hipStream_t s);
+hipError_t launch_ib_drag(const Consts& c, IBWork& w, const float* pos, float* vel_io,
+                          float* pos_out, long long* tick, hipStream_t s);
 // LBM
 hipError_t launch_lbm(const Consts& c, const float* f_in, float* f_out, float* ustar,
-                      uint32_t* mask, int mask_words_per_row, const float* force,
-                      hipStream_t s);
+                      uint32_t* mask, int mask_words_per_row, const float* force, hipStream_t s);
 
 }  // namespace mc

@@ -1,15 +1,15 @@
 // Fused D2Q9 TRT + Guo collide-and-stream pass (fluid.py:268-357).
 //
-// One workgroup owns whole lattice rows: it reads the nine populations of each
-// cell once (structure-of-arrays, one coalesced 256 B segment per wavefront per
+// One thread per cell: it reads the nine populations of its cell once
+// (structure-of-arrays, one coalesced 256 B segment per wavefront per
 // population), recomputes density and velocity, takes the immersed-boundary
 // force from the sparse support U (or the clamp-only force off U), collides,
 // and pushes the nine post-collision values to their streamed destinations.
-// Because a workgroup owns entire rows, every destination row segment of every
-// population is written by exactly one workgroup (the +-1 x shift wraps inside
-// the row), so no cache line is shared between workgroups on different XCDs.
-// The reference's state contract stores post-stream populations, so a push of
-// locally collided values is the streaming pass with no redundant collision.
+// Workgroups are mapped so each XCD owns a contiguous band of rows: the +-1
+// row and column shifts of the push then land in lines written by the same
+// XCD, except at band edges.  The reference's state contract stores
+// post-stream populations, so a push of locally collided values is the
+// streaming pass with no redundant collision.
 //
 // HBM traffic per cell: 36 B read + 36 B written (f) + 8 B written (u*)
 // (+ 1/8 B of the U bitmask, + 8 B only on the ~6 % of cells in U).
@@ -46,94 +46,124 @@ __device__ __forceinline__ float edot(int i, float vx, float vy) {
 __global__ void __launch_bounds__(256) k_lbm(const float* __restrict__ fin, float* __restrict__ fout,
                                              float* __restrict__ ustar, uint32_t* __restrict__ mask,
                                              int mwpr, const float* __restrict__ force, int H,
-                                             int W, int rows_per_block, float umax, float op,
-                                             float om, float cp, float cm, int force_clamp) {
+                                             int W, int nxb, float umax, float op, float om,
+                                             float cp, float cm, int force_clamp) {
+  // XCD-aware block order: workgroup i runs on XCD i % 8, so give each XCD a
+  // contiguous band of rows (its L2 then holds the rows it writes into).
+  int b = blockIdx.x;
+  const int G = gridDim.x;
+  if ((G & 7) == 0) b = (b & 7) * (G >> 3) + (b >> 3);
+  const int y = b / nxb;
+  const int x = (b - y * nxb) * blockDim.x + threadIdx.x;
+  if (x >= W) return;
   const size_t HW = (size_t)H * W;
-  const int y0 = blockIdx.x * rows_per_block;
-  for (int yy = 0; yy < rows_per_block; ++yy) {
-    const int y = y0 + yy;
-    if (y >= H) break;
-    const int yp = (y + 1 == H) ? 0 : y + 1;
-    const int ym = (y == 0) ? H - 1 : y - 1;
-    for (int x = threadIdx.x; x < W; x += blockDim.x) {
-      const size_t cell = (size_t)y * W + x;
-      float f[9];
+  const int yp = (y + 1 == H) ? 0 : y + 1;
+  const int ym = (y == 0) ? H - 1 : y - 1;
+  const size_t cell = (size_t)y * W + x;
+  float f[9];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) f[i] = fmax_nan(fin[i * HW + cell], MC_EPSIL);
-      float rho, ux, uy;
-      lbm_macro(f, umax, rho, ux, uy);
-      uint32_t* mw = mask + (size_t)y * mwpr + (x >> 5);
-      const uint32_t m = *mw;
-      float Fx, Fy;
-      if ((m >> (x & 31)) & 1u) {
-        Fx = force[cell];
-        Fy = force[HW + cell];
-      } else {
-        // u_corrected == u off the support: post-clamp and force (fluid.py:327-339)
-        float mag = sqrtf((ux * ux + uy * uy) + MC_EPSIL);
-        float cl = fmin_nan(1.0f, umax / (mag + MC_EPSIL));
-        float cx = ux * cl, cy = uy * cl;
-        Fx = (cx - ux) * rho;
-        Fy = (cy - uy) * rho;
-        if (force_clamp) {
-          float fm = sqrtf((Fx * Fx + Fy * Fy) + MC_EPSIL);
-          float fc = fmin_nan(1.0f, umax / (fm + MC_EPSIL));
-          Fx = Fx * fc;
-          Fy = Fy * fc;
-        }
-      }
-      if ((x & 31) == 0) *mw = 0u;  // consume the U bitmask word (every lane has read it)
-      // u* = u + 0.5 F / rho (fluid.py:346)
-      const float usx = ux + (0.5f * Fx) / rho;
-      const float usy = uy + (0.5f * Fy) / rho;
-      const float usq = usx * usx + usy * usy;
-      const float udf = usx * Fx + usy * Fy;
-      float feq[9], raw[9];
+  for (int i = 0; i < 9; ++i) f[i] = fmax_nan(fin[i * HW + cell], MC_EPSIL);
+  uint32_t* mw = mask + (size_t)y * mwpr + (x >> 5);
+  const uint32_t m = *mw;
+  // density and clamped velocity (fluid.py:136-142,268-274); `cl1` tells
+  // whether the clamp changed u
+  float r = f[0];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const float eu = edot(i, usx, usy);
-        const float ef = edot(i, Fx, Fy);
-        feq[i] = (rho * lw(i)) * (((1.0f + 3.0f * eu) + 4.5f * (eu * eu)) - 1.5f * usq);
-        raw[i] = lw(i) * (3.0f * (ef - udf) + (9.0f * eu) * ef);
-      }
-      float post[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const int o = OPP[i];
-        const float coll = (f[i] - op * (0.5f * (f[i] + f[o]) - 0.5f * (feq[i] + feq[o]))) -
-                           om * (0.5f * (f[i] - f[o]) - 0.5f * (feq[i] - feq[o]));
-        const float src = cp * (0.5f * (raw[i] + raw[o])) + cm * (0.5f * (raw[i] - raw[o]));
-        post[i] = fmax_nan(coll + src, MC_EPSIL);
-      }
-      const int xp = (x + 1 == W) ? 0 : x + 1;
-      const int xm = (x == 0) ? W - 1 : x - 1;
-      const size_t ry = (size_t)y * W, ryp = (size_t)yp * W, rym = (size_t)ym * W;
-      *(fout + 0 * HW + ry + x) = post[0];
-      *(fout + 1 * HW + ry + xp) = post[1];
-      *(fout + 2 * HW + ryp + x) = post[2];
-      *(fout + 3 * HW + ry + xm) = post[3];
-      *(fout + 4 * HW + rym + x) = post[4];
-      *(fout + 5 * HW + ryp + xp) = post[5];
-      *(fout + 6 * HW + ryp + xm) = post[6];
-      *(fout + 7 * HW + rym + xm) = post[7];
-      *(fout + 8 * HW + rym + xp) = post[8];
-      *(ustar + cell) = usx;
-      *(ustar + HW + cell) = usy;
+  for (int i = 1; i < 9; ++i) r = r + f[i];
+  const float rho = fmax_nan(r, 0.1f);
+  float ux = (((((f[1] - f[3]) + f[5]) - f[6]) - f[7]) + f[8]) / rho;
+  float uy = (((((f[2] - f[4]) + f[5]) + f[6]) - f[7]) - f[8]) / rho;
+  const float mag1 = sqrtf((ux * ux + uy * uy) + MC_EPSIL);
+  const float cl1 = fmin_nan(1.0f, umax / (mag1 + MC_EPSIL));
+  ux = ux * cl1;
+  uy = uy * cl1;
+  float Fx, Fy;
+  bool zero_force = false;
+  if ((m >> (x & 31)) & 1u) {
+    Fx = force[cell];
+    Fy = force[HW + cell];
+  } else if (cl1 == 1.0f) {
+    // Off the support u_corrected == u; when the first clamp left u unchanged
+    // the post-clamp sees the same magnitude, so its factor is 1 too and
+    // F = (u - u) rho = +0 exactly (fluid.py:327-332).
+    Fx = 0.0f;
+    Fy = 0.0f;
+    zero_force = true;
+  } else {
+    const float mag = sqrtf((ux * ux + uy * uy) + MC_EPSIL);
+    const float cl = fmin_nan(1.0f, umax / (mag + MC_EPSIL));
+    const float cx = ux * cl, cy = uy * cl;
+    Fx = (cx - ux) * rho;
+    Fy = (cy - uy) * rho;
+    if (force_clamp) {
+      const float fm = sqrtf((Fx * Fx + Fy * Fy) + MC_EPSIL);
+      const float fc = fmin_nan(1.0f, umax / (fm + MC_EPSIL));
+      Fx = Fx * fc;
+      Fy = Fy * fc;
     }
   }
+  if ((x & 31) == 0) *mw = 0u;  // consume the U bitmask word (every lane has read it)
+  // u* = u + 0.5 F / rho (fluid.py:346); with F = +0 the quotient is +0
+  const float usx = zero_force ? ux + 0.0f : ux + (0.5f * Fx) / rho;
+  const float usy = zero_force ? uy + 0.0f : uy + (0.5f * Fy) / rho;
+  const float usq = usx * usx + usy * usy;
+  float feq[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const float eu = edot(i, usx, usy);
+    feq[i] = (rho * lw(i)) * (((1.0f + 3.0f * eu) + 4.5f * (eu * eu)) - 1.5f * usq);
+  }
+  float post[9];
+  if (zero_force) {
+    // the TRT Guo source of a zero force is a signed zero: adding it is exact
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const int o = OPP[i];
+      const float coll = (f[i] - op * (0.5f * (f[i] + f[o]) - 0.5f * (feq[i] + feq[o]))) -
+                         om * (0.5f * (f[i] - f[o]) - 0.5f * (feq[i] - feq[o]));
+      post[i] = fmax_nan(coll, MC_EPSIL);
+    }
+  } else {
+    const float udf = usx * Fx + usy * Fy;
+    float raw[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const float eu = edot(i, usx, usy);
+      const float ef = edot(i, Fx, Fy);
+      raw[i] = lw(i) * (3.0f * (ef - udf) + (9.0f * eu) * ef);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const int o = OPP[i];
+      const float coll = (f[i] - op * (0.5f * (f[i] + f[o]) - 0.5f * (feq[i] + feq[o]))) -
+                         om * (0.5f * (f[i] - f[o]) - 0.5f * (feq[i] - feq[o]));
+      const float src = cp * (0.5f * (raw[i] + raw[o])) + cm * (0.5f * (raw[i] - raw[o]));
+      post[i] = fmax_nan(coll + src, MC_EPSIL);
+    }
+  }
+  const int xp = (x + 1 == W) ? 0 : x + 1;
+  const int xm = (x == 0) ? W - 1 : x - 1;
+  const size_t ry = (size_t)y * W, ryp = (size_t)yp * W, rym = (size_t)ym * W;
+  fout[0 * HW + ry + x] = post[0];
+  fout[1 * HW + ry + xp] = post[1];
+  fout[2 * HW + ryp + x] = post[2];
+  fout[3 * HW + ry + xm] = post[3];
+  fout[4 * HW + rym + x] = post[4];
+  fout[5 * HW + ryp + xp] = post[5];
+  fout[6 * HW + ryp + xm] = post[6];
+  fout[7 * HW + rym + xm] = post[7];
+  fout[8 * HW + rym + xp] = post[8];
+  ustar[cell] = usx;
+  ustar[HW + cell] = usy;
 }
 
 hipError_t launch_lbm(const Consts& c, const float* f_in, float* f_out, float* ustar,
                       uint32_t* mask, int mwpr, const float* force, hipStream_t s) {
   const int H = c.fl_h, W = c.fl_w;
-  int block = 256;
-  if (W < 256) block = ((W + 63) / 64) * 64;
-  // at least ~1024 cells per workgroup
-  int rpb = 1;
-  while (rpb * W < 1024 && rpb < H) rpb *= 2;
-  const int grid = (H + rpb - 1) / rpb;
-  hipLaunchKernelGGL(k_lbm, dim3(grid), dim3(block), 0, s, f_in, f_out, ustar, mask, mwpr, force,
-                     H, W, rpb, c.umax, c.omega_p, c.omega_m, c.cp, c.cm, c.force_clamp);
+  const int block = W >= 256 ? 256 : ((W + 63) / 64) * 64;
+  const int nxb = (W + block - 1) / block;
+  hipLaunchKernelGGL(k_lbm, dim3(H * nxb), dim3(block), 0, s, f_in, f_out, ustar, mask, mwpr, force,
+                     H, W, nxb, c.umax, c.omega_p, c.omega_m, c.cp, c.cm, c.force_clamp);
   return hipGetLastError();
 }
 

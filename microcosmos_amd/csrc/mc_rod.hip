@@ -88,8 +88,10 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   const int n0 = L.comp_node_off[comp], nn = L.comp_node_off[comp + 1] - n0;
   const int e0 = L.comp_edge_off[comp], ne = L.comp_edge_off[comp + 1] - e0;
   const int b0 = L.comp_hinge_off[comp], nb = L.comp_hinge_off[comp + 1] - b0;
-  // scratch layout (floats): x[2nn] y[2nn] th[ne] l[ne] F[2ne] He[ne] corr[nb] rest[nb] kb[nb]
-  float* base = a.use_lds ? smem : (a.gscratch + (size_t)2 * 2 * n0 + (size_t)5 * e0 + (size_t)3 * b0);
+  const int ni0 = L.node_inc_off[n0], nni = L.node_inc_off[n0 + nn] - ni0;
+  const int ei0 = L.edge_inc_off[e0], nei = L.edge_inc_off[e0 + ne] - ei0;
+  // float scratch: x[2nn] y[2nn] th[ne] l[ne] F[2ne] He[ne] corr[nb] rest[nb] kb[nb]
+  float* base = a.use_lds ? smem : (a.gscratch + (size_t)4 * n0 + (size_t)5 * e0 + (size_t)3 * b0);
   float* X = base;
   float* Y = X + 2 * nn;
   float* TH = Y + 2 * nn;
@@ -99,6 +101,44 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   float* CO = He + ne;
   float* RS = CO + nb;
   float* KB = RS + nb;
+  // topology: staged into LDS after the floats (component-relative offsets),
+  // or read in place from global memory for components beyond the LDS budget
+  const int *hin, *hout, *esrc, *etgt, *einc_off, *einc, *ninc_off, *ninc;
+  int eoff_base, noff_base;
+  if (a.use_lds) {
+    int* ib = reinterpret_cast<int*>(KB + nb);
+    int* s_hin = ib;
+    int* s_hout = s_hin + nb;
+    int* s_esrc = s_hout + nb;
+    int* s_etgt = s_esrc + ne;
+    int* s_eoff = s_etgt + ne;
+    int* s_einc = s_eoff + ne + 1;
+    int* s_noff = s_einc + nei;
+    int* s_ninc = s_noff + nn + 1;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      s_hin[b] = L.hinge_in[b0 + b];
+      s_hout[b] = L.hinge_out[b0 + b];
+    }
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      s_esrc[e] = L.edge_src[e0 + e];
+      s_etgt[e] = L.edge_tgt[e0 + e];
+    }
+    for (int e = threadIdx.x; e <= ne; e += blockDim.x) s_eoff[e] = L.edge_inc_off[e0 + e] - ei0;
+    for (int q = threadIdx.x; q < nei; q += blockDim.x) s_einc[q] = L.edge_inc[ei0 + q];
+    for (int i = threadIdx.x; i <= nn; i += blockDim.x) s_noff[i] = L.node_inc_off[n0 + i] - ni0;
+    for (int q = threadIdx.x; q < nni; q += blockDim.x) s_ninc[q] = L.node_inc[ni0 + q];
+    hin = s_hin; hout = s_hout; esrc = s_esrc; etgt = s_etgt;
+    einc_off = s_eoff; einc = s_einc; ninc_off = s_noff; ninc = s_ninc;
+    eoff_base = 0;
+    noff_base = 0;
+  } else {
+    hin = L.hinge_in + b0; hout = L.hinge_out + b0; esrc = L.edge_src + e0; etgt = L.edge_tgt + e0;
+    einc_off = L.edge_inc_off + e0; einc = L.edge_inc; ninc_off = L.node_inc_off + n0; ninc = L.node_inc;
+    eoff_base = 0;
+    noff_base = 0;
+  }
+  (void)eoff_base;
+  (void)noff_base;
   const bool is_cos = c.solver == 1;
 
   // prologue: damping, prediction, steric push (simulate.py:41-72)
@@ -148,18 +188,17 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   for (int cyc = 0; cyc < c.cycles; ++cyc) {
     // theta pass: hinge corrections (pbd.py:32-40)
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-      const float tin = TH[L.hinge_in[b0 + b]];
-      const float tout = TH[L.hinge_out[b0 + b]];
-      const float diff = wrap_angle(tout - tin);
+      const float diff = wrap_angle(TH[hout[b]] - TH[hin[b]]);
       CO[b] = ((diff - RS[b]) * KB[b]) * 0.5f;
     }
     __syncthreads();
     // per edge ordered gather of +-corr (pbd.py:44-49 / stable_cosserat.py:158-171)
     for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-      const int i0 = L.edge_inc_off[e0 + e], i1 = L.edge_inc_off[e0 + e + 1];
+      const int i0 = einc_off[e], i1 = einc_off[e + 1];
+      const int* inc = a.use_lds ? einc : (einc + 0);
       float d = 0.0f;
       for (int q = i0; q < i1; ++q) {
-        const int code = L.edge_inc[q];
+        const int code = a.use_lds ? inc[q] : inc[q + 0];
         const float v = CO[code >> 1];
         d = d + ((code & 1) ? -v : v);
       }
@@ -168,9 +207,9 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
         TH[e] = TH[e] + (d / deg) * 1.8f;
       } else {
         float th = TH[e] + d / deg;
-        const int s = L.edge_src[e0 + e], t = L.edge_tgt[e0 + e];
-        const float sx = min_image(X[2 * t], X[2 * s], Wd);
-        const float sy = min_image(X[2 * t + 1], X[2 * s + 1], Hd);
+        const int sn = esrc[e], tn = etgt[e];
+        const float sx = min_image(X[2 * tn], X[2 * sn], Wd);
+        const float sy = min_image(X[2 * tn + 1], X[2 * sn + 1], Hd);
         const float alpha = datan2(sy, sx);
         const float pull = wrap_angle(alpha - th);
         th = th + pull * 0.05f;
@@ -180,11 +219,13 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
     __syncthreads();
     // per edge forces (pbd.py:60-81 / stable_cosserat.py:185-195)
     for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-      const int s = L.edge_src[e0 + e], t = L.edge_tgt[e0 + e];
+      const int sn = esrc[e], tn = etgt[e];
       const float th = TH[e], l = Le[e];
-      const float dx = dcos(th), dy = dsin(th);
-      const float rx = min_image(X[2 * t], X[2 * s], Wd);
-      const float ry = min_image(X[2 * t + 1], X[2 * s + 1], Hd);
+      double sd, cd;
+      sincos((double)th, &sd, &cd);
+      const float dx = (float)cd, dy = (float)sd;
+      const float rx = min_image(X[2 * tn], X[2 * sn], Wd);
+      const float ry = min_image(X[2 * tn + 1], X[2 * sn + 1], Hd);
       if (!is_cos) {
         const float tx = dx * l, ty = dy * l;
         const float len = sqrtf(rx * rx + ry * ry);
@@ -205,11 +246,11 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
     __syncthreads();
     // per node ordered gather (pbd.py:84-89 / stable_cosserat.py:197-207)
     for (int i = threadIdx.x; i < nn; i += blockDim.x) {
-      const int i0 = L.node_inc_off[n0 + i], i1 = L.node_inc_off[n0 + i + 1];
+      const int i0 = ninc_off[i], i1 = ninc_off[i + 1];
       if (!is_cos) {
         float ux = 0.0f, uy = 0.0f;
         for (int q = i0; q < i1; ++q) {
-          const int code = L.node_inc[q];
+          const int code = ninc[q];
           const int e = code >> 1;
           const float fx = Fe[2 * e], fy = Fe[2 * e + 1];
           ux = ux + ((code & 1) ? -fx : fx);
@@ -224,14 +265,14 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
         float fx = w * (Y[2 * i] - xx), fy = w * (Y[2 * i + 1] - xy);
         float h = w;
         for (int q = i0; q < i1; ++q) {
-          const int code = L.node_inc[q];
+          const int code = ninc[q];
           const int e = code >> 1;
           const float gx = Fe[2 * e], gy = Fe[2 * e + 1];
           fx = fx + ((code & 1) ? -gx : gx);
           fy = fy + ((code & 1) ? -gy : gy);
         }
         // H: src adds then tgt adds, all +H_edge (same incidence order)
-        for (int q = i0; q < i1; ++q) h = h + He[L.node_inc[q] >> 1];
+        for (int q = i0; q < i1; ++q) h = h + He[ninc[q] >> 1];
         X[2 * i] = xx + fx / h;
         X[2 * i + 1] = xy + fy / h;
       }
@@ -267,6 +308,11 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
 
 size_t rod_scratch_floats(int nn, int ne, int nb) {
   return (size_t)4 * nn + (size_t)5 * ne + (size_t)3 * nb;
+}
+
+size_t rod_lds_bytes(int nn, int ne, int nb, int nni, int nei) {
+  const size_t ints = (size_t)2 * nb + (size_t)3 * ne + 1 + nei + (size_t)nn + 1 + nni;
+  return rod_scratch_floats(nn, ne, nb) * sizeof(float) + ints * sizeof(int);
 }
 
 __global__ void k_steric_nodes(Consts c, int N, const float* __restrict__ steric,

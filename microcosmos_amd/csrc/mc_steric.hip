@@ -22,75 +22,130 @@
 namespace mc {
 namespace {
 
-constexpr int MAX_RADIX = 16;
-
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
-// Batched Stockham FFT over `batch` contiguous length-n sequences in LDS.
-// Returns the buffer that holds the result (a or b).
-__device__ float2* fft_lds(float2* a, float2* b, int n, int batch, const int* radix, int nrad,
-                           const float2* __restrict__ tw, bool inverse) {
+// Radix-R butterfly on registers (compile-time R keeps v[] in VGPRs).
+template <int R>
+__device__ __forceinline__ void butterfly(float2 (&v)[R], float2 (&V)[R], bool inverse,
+                                          const float2* __restrict__ tw, int nb) {
+  if constexpr (R == 2) {
+    V[0] = cadd(v[0], v[1]);
+    V[1] = csub(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const float2 a2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+    // forward: -i * d ; inverse: +i * d
+    const float2 a3 = inverse ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+    V[0] = cadd(a0, a2);
+    V[2] = csub(a0, a2);
+    V[1] = cadd(a1, a3);
+    V[3] = csub(a1, a3);
+  } else if constexpr (R == 3) {
+    const float s3 = 0.866025403784438647f;  // sin(2 pi / 3)
+    const float2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    const float2 m = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+    // forward: -i s t2 ; inverse: +i s t2
+    const float2 st = inverse ? make_float2(-s3 * t2.y, s3 * t2.x) : make_float2(s3 * t2.y, -s3 * t2.x);
+    V[0] = cadd(v[0], t1);
+    V[1] = cadd(m, st);
+    V[2] = csub(m, st);
+  } else {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[((r * q) % R) * nb];
+        if (inverse) w.y = -w.y;
+        acc = cadd(acc, cmul(v[r], w));
+      }
+      V[q] = acc;
+    }
+  }
+}
+
+// One Stockham stage of radix R over `batch` contiguous length-n sequences.
+template <int R>
+__device__ __forceinline__ void fft_stage(const float2* src, float2* dst, int n, int stride, int batch,
+                                          int Ns, const float2* __restrict__ tw, bool inverse) {
+  const int nb = n / R;
+  const int tstride = n / (Ns * R);
+  for (int idx = threadIdx.x; idx < batch * nb; idx += blockDim.x) {
+    const int col = idx / nb;
+    const int j = idx - col * nb;
+    const int base = col * stride;
+    const int k = j % Ns;
+    float2 v[R], V[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      float2 x = src[base + j + r * nb];
+      if (r > 0 && k > 0) {
+        float2 w = tw[r * k * tstride];
+        if (inverse) w.y = -w.y;
+        x = cmul(x, w);
+      }
+      v[r] = x;
+    }
+    butterfly<R>(v, V, inverse, tw, nb);
+    const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[base + idxD + r * Ns] = V[r];
+  }
+}
+
+// Radix 5 / 7 / 11 / 13 stage without register arrays: each output re-reads and
+// re-twiddles its inputs (same roundings as fft_stage, fewer VGPRs; these
+// radices only occur for unusual grid sides).
+__device__ __forceinline__ void fft_stage_generic(const float2* src, float2* dst, int n, int stride,
+                                                  int batch, int Ns, int R,
+                                                  const float2* __restrict__ tw, bool inverse) {
+  const int nb = n / R;
+  const int tstride = n / (Ns * R);
+  for (int idx = threadIdx.x; idx < batch * nb; idx += blockDim.x) {
+    const int col = idx / nb;
+    const int j = idx - col * nb;
+    const int base = col * stride;
+    const int k = j % Ns;
+    const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll 1
+    for (int q = 0; q < R; ++q) {
+      float2 acc = src[base + j];
+#pragma unroll 1
+      for (int r = 1; r < R; ++r) {
+        float2 x = src[base + j + r * nb];
+        if (k > 0) {
+          float2 w = tw[r * k * tstride];
+          if (inverse) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        float2 w = tw[((r * q) % R) * nb];
+        if (inverse) w.y = -w.y;
+        acc = cadd(acc, cmul(x, w));
+      }
+      dst[base + idxD + q * Ns] = acc;
+    }
+  }
+}
+
+// Batched Stockham FFT over `batch` length-n sequences in LDS, sequence c at
+// offset c * stride.  Radices come from the host schedule (4s, then 2, then
+// odd primes <= 13).  Returns the buffer that holds the result (a or b).
+__device__ float2* fft_lds(float2* a, float2* b, int n, int stride, int batch, const int* radix,
+                           int nrad, const float2* __restrict__ tw, bool inverse) {
   float2* src = a;
   float2* dst = b;
   int Ns = 1;
   for (int s = 0; s < nrad; ++s) {
     const int R = radix[s];
-    const int nb = n / R;
-    const int tstride = n / (Ns * R);
-    for (int idx = threadIdx.x; idx < batch * nb; idx += blockDim.x) {
-      const int col = idx / nb;
-      const int j = idx - col * nb;
-      const int base = col * n;
-      const int k = j % Ns;
-      float2 v[MAX_RADIX];
-      for (int r = 0; r < R; ++r) {
-        float2 x = src[base + j + r * nb];
-        if (r > 0 && k > 0) {
-          float2 w = tw[r * k * tstride];
-          if (inverse) w.y = -w.y;
-          x = cmul(x, w);
-        }
-        v[r] = x;
-      }
-      float2 V[MAX_RADIX];
-      if (R == 2) {
-        V[0] = cadd(v[0], v[1]);
-        V[1] = csub(v[0], v[1]);
-      } else if (R == 4) {
-        const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-        const float2 a2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
-        // forward: -i * d ; inverse: +i * d
-        const float2 a3 = inverse ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
-        V[0] = cadd(a0, a2);
-        V[2] = csub(a0, a2);
-        V[1] = cadd(a1, a3);
-        V[3] = csub(a1, a3);
-      } else if (R == 3) {
-        const float s3 = 0.866025403784438647f;  // sin(2 pi / 3)
-        const float2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-        const float2 m = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
-        // forward: -i s t2 ; inverse: +i s t2
-        const float2 st = inverse ? make_float2(-s3 * t2.y, s3 * t2.x) : make_float2(s3 * t2.y, -s3 * t2.x);
-        V[0] = cadd(v[0], t1);
-        V[1] = cadd(m, st);
-        V[2] = csub(m, st);
-      } else {
-        for (int q = 0; q < R; ++q) {
-          float2 acc = v[0];
-          for (int r = 1; r < R; ++r) {
-            float2 w = tw[((r * q) % R) * nb];
-            if (inverse) w.y = -w.y;
-            acc = cadd(acc, cmul(v[r], w));
-          }
-          V[q] = acc;
-        }
-      }
-      const int idxD = (j / Ns) * Ns * R + k;
-      for (int r = 0; r < R; ++r) dst[base + idxD + r * Ns] = V[r];
+    switch (R) {
+      case 4: fft_stage<4>(src, dst, n, stride, batch, Ns, tw, inverse); break;
+      case 2: fft_stage<2>(src, dst, n, stride, batch, Ns, tw, inverse); break;
+      case 3: fft_stage<3>(src, dst, n, stride, batch, Ns, tw, inverse); break;
+      default: fft_stage_generic(src, dst, n, stride, batch, Ns, R, tw, inverse); break;
     }
     __syncthreads();
     float2* t = src;
@@ -147,74 +202,92 @@ __global__ void k_row_fill(int N, const int* __restrict__ cic_cell, const int* _
   }
 }
 
-constexpr int DEP_CH = 256;
+constexpr int DEP_KEYS_LDS = 512;    // 64-bit keys of one row's CIC entries held in LDS
+constexpr int DEP_IDS_LDS = 1024;    // node ids of one row sorted in LDS
 
+__device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ tw, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = tw[i];
+}
+
+// One workgroup per row y: fold the row's CIC entries cell by cell in the
+// reference's order (all w00 by node, then w10, w01, w11 -- forces.py:112-115),
+// then the forward FFT of the row.  Each node binned to the row contributes
+// exactly two entries (corners q = 0, 1 if y == y0; q = 2, 3 if y == y1).
+// Sorting the entries by (x, q, node rank) puts each cell's contributions
+// contiguously and in reference order; a segment head thread folds them.
 __global__ void __launch_bounds__(256) k_deposit_rows_fft(
     int H, int W, const int* __restrict__ row_off, int* __restrict__ row_list,
-    int* __restrict__ sort_tmp, const int* __restrict__ cic_cell, const float* __restrict__ cic_w,
-    float2* __restrict__ spec, const float2* __restrict__ tw, const int* __restrict__ radix_g,
-    int nrad, float* __restrict__ density_out) {
+    int* __restrict__ sort_tmp, unsigned long long* __restrict__ key_tmp, size_t key_half,
+    const int* __restrict__ cic_cell, const float* __restrict__ cic_w, float2* __restrict__ spec,
+    const float2* __restrict__ tw_g, const int* __restrict__ radix_g, int nrad,
+    float* __restrict__ density_out) {
   extern __shared__ float2 lds[];
-  __shared__ int sort_lds[SORT_LDS_CAP];
-  __shared__ int s_x[DEP_CH][2];
-  __shared__ int s_y[DEP_CH][2];
-  __shared__ float s_w[DEP_CH][4];
+  __shared__ int sort_lds[DEP_IDS_LDS];
+  __shared__ unsigned long long keys_lds[DEP_KEYS_LDS];
+  __shared__ unsigned long long sort_keys[DEP_KEYS_LDS];
   __shared__ int s_radix[32];
   float2* a = lds;
   float2* b = lds + W;
+  float2* tw = lds + 2 * W;
   if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
+  load_twiddles(tw, tw_g, W);
   const int y = blockIdx.x;
   const int off = row_off[y];
   const int m = row_off[y + 1] - off;
-  sort_unique_ids(row_list + off, m, sort_lds, sort_tmp + off);
+  int* nodes = row_list + off;
+  sort_unique<int>(nodes, m, sort_lds, DEP_IDS_LDS, sort_tmp + off, 0x7fffffff);
   for (int x = threadIdx.x; x < W; x += blockDim.x) a[x] = make_float2(0.0f, 0.0f);
-  __syncthreads();
-  // fold: corner-major (00, 10, 01, 11), node-minor, per cell
-  for (int q = 0; q < 4; ++q) {
-    const int xsel = q & 1;         // 0: x0, 1: x1
-    const int ysel = q >> 1;        // 0: y0, 1: y1
-    for (int j0 = 0; j0 < m; j0 += DEP_CH) {
-      const int cnt = min(DEP_CH, m - j0);
-      for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
-        const int n = row_list[off + j0 + j];
-        s_x[j][0] = cic_cell[4 * n + 0];
-        s_x[j][1] = cic_cell[4 * n + 1];
-        s_y[j][0] = cic_cell[4 * n + 2];
-        s_y[j][1] = cic_cell[4 * n + 3];
-        s_w[j][0] = cic_w[4 * n + 0];
-        s_w[j][1] = cic_w[4 * n + 1];
-        s_w[j][2] = cic_w[4 * n + 2];
-        s_w[j][3] = cic_w[4 * n + 3];
-      }
-      __syncthreads();
-      for (int x = threadIdx.x; x < W; x += blockDim.x) {
-        float acc = a[x].x;
-        for (int j = 0; j < cnt; ++j)
-          if (s_y[j][ysel] == y && s_x[j][xsel] == x) acc = acc + s_w[j][q];
-        a[x].x = acc;
-      }
-      __syncthreads();
-    }
+  const int ne = 2 * m;
+  unsigned long long* keys = (ne <= DEP_KEYS_LDS) ? keys_lds : (key_tmp + 2 * (size_t)off);
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    const int n = nodes[j];
+    const unsigned long long x0 = (unsigned)cic_cell[4 * n + 0], x1 = (unsigned)cic_cell[4 * n + 1];
+    const unsigned long long qb = (cic_cell[4 * n + 2] == y) ? 0ull : 2ull;
+    keys[2 * j] = (x0 << 34) | (qb << 32) | (unsigned)j;
+    keys[2 * j + 1] = (x1 << 34) | ((qb + 1ull) << 32) | (unsigned)j;
   }
+  __threadfence_block();
+  __syncthreads();
+  // beyond the LDS capacity the keys live in key_tmp[2 off ..) and the merge
+  // sort ping-pongs through the second half of key_tmp (key_half entries on)
+  sort_unique<unsigned long long>(keys, ne, sort_keys, DEP_KEYS_LDS,
+                                  key_tmp + key_half + 2 * (size_t)off, ~0ull);
+  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+    const unsigned long long k = keys[i];
+    const int x = (int)(k >> 34);
+    if (i > 0 && (int)(keys[i - 1] >> 34) == x) continue;
+    float acc = 0.0f;
+    for (int e = i; e < ne; ++e) {
+      const unsigned long long ke = keys[e];
+      if ((int)(ke >> 34) != x) break;
+      const int q = (int)((ke >> 32) & 3ull);
+      const int n = nodes[(int)(ke & 0xffffffffull)];
+      acc = acc + cic_w[4 * n + q];
+    }
+    a[x].x = acc;
+  }
+  __syncthreads();
   if (density_out)
     for (int x = threadIdx.x; x < W; x += blockDim.x) density_out[(size_t)y * W + x] = a[x].x;
-  float2* r = fft_lds(a, b, W, 1, s_radix, nrad, tw, false);
+  float2* r = fft_lds(a, b, W, W, 1, s_radix, nrad, tw, false);
   for (int x = threadIdx.x; x < W; x += blockDim.x) spec[(size_t)y * W + x] = r[x];
 }
 
 __global__ void __launch_bounds__(256) k_fft_rows(int H, int W, const float* __restrict__ in,
                                                   float2* __restrict__ spec,
-                                                  const float2* __restrict__ tw,
+                                                  const float2* __restrict__ tw_g,
                                                   const int* __restrict__ radix_g, int nrad) {
   extern __shared__ float2 lds[];
   __shared__ int s_radix[32];
   float2* a = lds;
   float2* b = lds + W;
+  float2* tw = lds + 2 * W;
   if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
+  load_twiddles(tw, tw_g, W);
   const int y = blockIdx.x;
   for (int x = threadIdx.x; x < W; x += blockDim.x) a[x] = make_float2(in[(size_t)y * W + x], 0.0f);
   __syncthreads();
-  float2* r = fft_lds(a, b, W, 1, s_radix, nrad, tw, false);
+  float2* r = fft_lds(a, b, W, W, 1, s_radix, nrad, tw, false);
   for (int x = threadIdx.x; x < W; x += blockDim.x) spec[(size_t)y * W + x] = r[x];
 }
 
@@ -224,16 +297,31 @@ __device__ __forceinline__ float fftfreq(int k, int n) {
   return (float)kk / (float)n;
 }
 
+// Gaussian transfer function exp(coef (fr^2 + fc^2)) on the fftfreq grid
+// (utils.py:41-47), tabulated once per sigma.
+__global__ void k_gauss_table(int H, int W, float coef, float* __restrict__ kf) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)H * W) return;
+  const int row = (int)(i / W), col = (int)(i - (size_t)row * W);
+  const float fr = fftfreq(row, H), fc = fftfreq(col, W);
+  kf[i] = dexp(coef * (fr * fr + fc * fc));
+}
+
 __global__ void __launch_bounds__(256) k_fft_cols_filter(int H, int W, int cs,
                                                          float2* __restrict__ spec,
-                                                         const float2* __restrict__ tw,
+                                                         const float2* __restrict__ tw_g,
                                                          const int* __restrict__ radix_g,
-                                                         int nrad, float coef) {
+                                                         int nrad, const float* __restrict__ kft) {
   extern __shared__ float2 lds[];
   __shared__ int s_radix[32];
+  // columns are stored with a padded stride so the transposing loads and
+  // stores spread over the LDS banks
+  const int cst = H + 1;
   float2* a = lds;
-  float2* b = lds + (size_t)cs * H;
+  float2* b = lds + (size_t)cs * cst;
+  float2* tw = lds + (size_t)2 * cs * cst;
   if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
+  load_twiddles(tw, tw_g, H);
   // XCD-aware strip order: consecutive workgroups land on different XCDs, so
   // give the 8 workgroups of one dispatch round strips far apart and the
   // workgroups of one XCD neighbouring strips (they share cache lines).
@@ -244,45 +332,67 @@ __global__ void __launch_bounds__(256) k_fft_cols_filter(int H, int W, int cs,
   const int c0 = strip * cs;
   const int ncol = min(cs, W - c0);
   if (c0 >= W) return;
-  for (int i = threadIdx.x; i < ncol * H; i += blockDim.x) {
-    const int row = i / ncol, col = i - row * ncol;
-    a[col * H + row] = spec[(size_t)row * W + c0 + col];
+  // issue all strided loads of this thread before any LDS store (the loads
+  // are independent; a load-store loop would serialise their latencies)
+  {
+    constexpr int U = 16;
+    const int total = ncol * H;
+    for (int i0 = 0; i0 < total; i0 += U * blockDim.x) {
+      float2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * blockDim.x + threadIdx.x;
+        if (i < total) {
+          const int row = i / ncol, col = i - row * ncol;
+          v[u] = spec[(size_t)row * W + c0 + col];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * blockDim.x + threadIdx.x;
+        if (i < total) {
+          const int row = i / ncol, col = i - row * ncol;
+          a[col * cst + row] = v[u];
+        }
+      }
+    }
   }
   __syncthreads();
-  float2* r = fft_lds(a, b, H, ncol, s_radix, nrad, tw, false);
+  float2* r = fft_lds(a, b, H, cst, ncol, s_radix, nrad, tw, false);
   float2* o = (r == a) ? b : a;
   for (int i = threadIdx.x; i < ncol * H; i += blockDim.x) {
     const int col = i / H, row = i - col * H;
-    const float fr = fftfreq(row, H), fc = fftfreq(c0 + col, W);
-    const float kf = dexp(coef * (fr * fr + fc * fc));
-    float2 v = r[i];
+    const float kf = kft[(size_t)row * W + c0 + col];
+    float2 v = r[col * cst + row];
     v.x = v.x * kf;
     v.y = v.y * kf;
-    r[i] = v;
+    r[col * cst + row] = v;
   }
   __syncthreads();
-  float2* res = fft_lds(r, o, H, ncol, s_radix, nrad, tw, true);
+  float2* res = fft_lds(r, o, H, cst, ncol, s_radix, nrad, tw, true);
   const float sc = (float)(1.0 / (double)H);
   for (int i = threadIdx.x; i < ncol * H; i += blockDim.x) {
     const int row = i / ncol, col = i - row * ncol;
-    float2 v = res[col * H + row];
+    float2 v = res[col * cst + row];
     spec[(size_t)row * W + c0 + col] = make_float2(v.x * sc, v.y * sc);
   }
 }
 
 __global__ void __launch_bounds__(256) k_ifft_rows_real(int H, int W, const float2* __restrict__ spec,
                                                         float* __restrict__ out,
-                                                        const float2* __restrict__ tw,
+                                                        const float2* __restrict__ tw_g,
                                                         const int* __restrict__ radix_g, int nrad) {
   extern __shared__ float2 lds[];
   __shared__ int s_radix[32];
   float2* a = lds;
   float2* b = lds + W;
+  float2* tw = lds + 2 * W;
   if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
+  load_twiddles(tw, tw_g, W);
   const int y = blockIdx.x;
   for (int x = threadIdx.x; x < W; x += blockDim.x) a[x] = spec[(size_t)y * W + x];
   __syncthreads();
-  float2* r = fft_lds(a, b, W, 1, s_radix, nrad, tw, true);
+  float2* r = fft_lds(a, b, W, W, 1, s_radix, nrad, tw, true);
   const float sc = (float)(1.0 / (double)W);
   for (int x = threadIdx.x; x < W; x += blockDim.x) out[(size_t)y * W + x] = r[x].x * sc;
 }
@@ -308,37 +418,48 @@ hipError_t launch_row_fill(const Consts& c, int N, const int* cic_cell, const in
 hipError_t launch_deposit_rows_fft(const Consts& c, int N, const int* row_off, int* row_list,
                                    const int* cic_cell, const float* cic_w, float2* spec,
                                    const float2* tw_w, const int* radix_w, int nrad_w,
-                                   float* density_out, int* sort_scratch, hipStream_t s) {
+                                   float* density_out, int* sort_scratch,
+                                   unsigned long long* key_scratch, hipStream_t s) {
   const int H = c.dom_h, W = c.dom_w;
-  const size_t lds = (size_t)2 * W * sizeof(float2);
+  const size_t lds = (size_t)3 * W * sizeof(float2);
   hipLaunchKernelGGL(k_deposit_rows_fft, dim3(H), dim3(256), lds, s, H, W, row_off, row_list,
-                     sort_scratch, cic_cell, cic_w, spec, tw_w, radix_w, nrad_w, density_out);
+                     sort_scratch, key_scratch, (size_t)4 * N, cic_cell, cic_w, spec, tw_w, radix_w,
+                     nrad_w, density_out);
   return hipGetLastError();
 }
 
 hipError_t launch_fft_rows(int H, int W, const float* real_in, float2* spec, const float2* tw_w,
                            const int* radix_w, int nrad_w, hipStream_t s) {
-  const size_t lds = (size_t)2 * W * sizeof(float2);
+  const size_t lds = (size_t)3 * W * sizeof(float2);
   hipLaunchKernelGGL(k_fft_rows, dim3(H), dim3(256), lds, s, H, W, real_in, spec, tw_w, radix_w,
                      nrad_w);
   return hipGetLastError();
 }
 
+hipError_t launch_gauss_table(int H, int W, float coef, float* kf, hipStream_t s) {
+  const size_t n = (size_t)H * W;
+  hipLaunchKernelGGL(k_gauss_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, W, coef, kf);
+  return hipGetLastError();
+}
+
 hipError_t launch_fft_cols_filter(int H, int W, float2* spec, const float2* tw_h,
-                                  const int* radix_h, int nrad_h, float coef, int lds_bytes_max,
+                                  const int* radix_h, int nrad_h, const float* kf, int lds_bytes_max,
                                   hipStream_t s) {
+  // widest strip that fits, but keep at least one workgroup per CU
   int cs = 16;
-  while (cs > 1 && (size_t)2 * cs * H * sizeof(float2) > (size_t)lds_bytes_max) cs >>= 1;
-  const size_t lds = (size_t)2 * cs * H * sizeof(float2);
+  while (cs > 1 && (((size_t)2 * cs * (H + 1) + H) * sizeof(float2) > (size_t)lds_bytes_max ||
+                    (W + cs - 1) / cs < 256))
+    cs >>= 1;
+  const size_t lds = ((size_t)2 * cs * (H + 1) + H) * sizeof(float2);
   const int nstrips = (W + cs - 1) / cs;
   hipLaunchKernelGGL(k_fft_cols_filter, dim3(nstrips), dim3(256), lds, s, H, W, cs, spec, tw_h,
-                     radix_h, nrad_h, coef);
+                     radix_h, nrad_h, kf);
   return hipGetLastError();
 }
 
 hipError_t launch_ifft_rows_real(int H, int W, const float2* spec, float* out, const float2* tw_w,
                                  const int* radix_w, int nrad_w, hipStream_t s) {
-  const size_t lds = (size_t)2 * W * sizeof(float2);
+  const size_t lds = (size_t)3 * W * sizeof(float2);
   hipLaunchKernelGGL(k_ifft_rows_real, dim3(H), dim3(256), lds, s, H, W, spec, out, tw_w, radix_w,
                      nrad_w);
   return hipGetLastError();

@@ -214,17 +214,17 @@ def gaussian_filter_pocketfft(data, sigma):
 
 
 def fft_radices(n):
-    """Radix schedule of the device FFT: 4s, then 2, then odd factors <= 15."""
+    """Radix schedule of the device FFT: 4s, then 2, then odd factors <= 13."""
     out = []
     while n % 4 == 0:
         out.append(4); n //= 4
     while n % 2 == 0:
         out.append(2); n //= 2
-    for p in range(3, 17, 2):
+    for p in range(3, 15, 2):
         while n % p == 0:
             out.append(p); n //= p
     if n != 1:
-        raise ValueError("FFT length with a prime factor > 16")
+        raise ValueError("FFT length with a prime factor > 13")
     return out
 
 
