@@ -242,8 +242,9 @@ mc_status build_fft(mc_plan* p) {
     p->fft_why = "steric FFT needs grid sides whose prime factors are all <= 13";
     return MC_OK;
   }
-  const size_t row_lds = (size_t)3 * std::max(p->W, p->H) * sizeof(float2) + 49152;
-  if (row_lds > (size_t)p->lds_max || 3 * (size_t)p->H * sizeof(float2) > (size_t)p->lds_max - 1024) {
+  const int TR = fft_rows_per_group(p->H, p->W);
+  const size_t row_lds = ((size_t)2 * TR * p->W + p->W) * sizeof(float2) + 16384;
+  if (row_lds > (size_t)p->lds_max || 3 * (size_t)p->H * sizeof(float2) + 256 > (size_t)p->lds_max) {
     p->fft_why = "steric FFT row/column does not fit in LDS";
     return MC_OK;
   }
@@ -361,12 +362,10 @@ mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float*
   if (!c.enable_steric) return MC_OK;
   HIPCHK(launch_bin_scan(p->H, p->row_cnt, p->row_off, p->row_fill, p->row_active, p->row_nactive, s));
   HIPCHK(launch_row_fill(c, p->N, p->cic_cell, p->row_off, p->row_fill, p->row_list, s));
-  HIPCHK(launch_deposit_rows_fft(c, p->N, p->row_off, p->row_list, p->cic_cell, p->cic_w, p->spec,
-                                 p->tw_w, p->radix_w, p->nrad_w, density_out, p->row_sort,
-                                 p->row_keys, s));
-  HIPCHK(launch_fft_cols_filter(p->H, p->W, p->spec, p->tw_h, p->radix_h, p->nrad_h, p->kf,
-                                p->lds_max - 1024, s));
-  HIPCHK(launch_ifft_rows_real(p->H, p->W, p->spec, steric_out, p->tw_w, p->radix_w, p->nrad_w, s));
+  HIPCHK(launch_steric_fft_T(p->H, p->W, nullptr, p->row_off, p->row_list, p->row_sort, p->row_keys,
+                             (size_t)4 * p->N, p->cic_cell, p->cic_w, p->spec, p->tw_w, p->radix_w,
+                             p->nrad_w, p->tw_h, p->radix_h, p->nrad_h, p->kf, density_out,
+                             steric_out, s));
   return MC_OK;
 }
 
@@ -747,10 +746,9 @@ mc_status mc_gaussian_filter(mc_plan* p, const float* data, float* out, double s
   if (!p->fft_ok) return fail(MC_ERR_UNSUPPORTED, "%s (grid %dx%d)", p->fft_why.c_str(), p->H, p->W);
   hipStream_t s = (hipStream_t)stream;
   MCCHK(ensure_kf(p, sigma, s));
-  HIPCHK(launch_fft_rows(p->H, p->W, data, p->spec, p->tw_w, p->radix_w, p->nrad_w, s));
-  HIPCHK(launch_fft_cols_filter(p->H, p->W, p->spec, p->tw_h, p->radix_h, p->nrad_h, p->kf,
-                                p->lds_max - 1024, s));
-  HIPCHK(launch_ifft_rows_real(p->H, p->W, p->spec, out, p->tw_w, p->radix_w, p->nrad_w, s));
+  HIPCHK(launch_steric_fft_T(p->H, p->W, data, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
+                             p->spec, p->tw_w, p->radix_w, p->nrad_w, p->tw_h, p->radix_h,
+                             p->nrad_h, p->kf, nullptr, out, s));
   return MC_OK;
 }
 

@@ -66,6 +66,15 @@ hipError_t launch_deposit_rows_fft(const Consts& c, int N, const int* row_off, i
 hipError_t launch_fft_rows(int H, int W, const float* real_in, float2* spec,
                            const float2* tw_w, const int* radix_w, int nrad_w, hipStream_t s);
 hipError_t launch_gauss_table(int H, int W, float coef, float* kf, hipStream_t s);
+// Whole steric filter: forward rows (CIC deposit when real_in is NULL),
+// transfer function on the transposed spectrum, inverse rows into `out`.
+int fft_rows_per_group(int H, int W);
+hipError_t launch_steric_fft_T(int H, int W, const float* real_in, const int* row_off, int* row_list,
+                               int* sort_scratch, unsigned long long* key_scratch, size_t key_half,
+                               const int* cic_cell, const float* cic_w, float2* specT,
+                               const float2* tw_w, const int* radix_w, int nrad_w,
+                               const float2* tw_h, const int* radix_h, int nrad_h, const float* kfT,
+                               float* density_out, float* out, hipStream_t s);
 hipError_t launch_fft_cols_filter(int H, int W, float2* spec, const float2* tw_h,
                                   const int* radix_h, int nrad_h, const float* kf, int lds_bytes_max,
                                   hipStream_t s);

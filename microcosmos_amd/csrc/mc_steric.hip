@@ -397,6 +397,169 @@ __global__ void __launch_bounds__(256) k_ifft_rows_real(int H, int W, const floa
   for (int x = threadIdx.x; x < W; x += blockDim.x) out[(size_t)y * W + x] = r[x].x * sc;
 }
 
+// ---------------------------------------------------------------------------
+// Transposed 2-D FFT: the row passes handle TR consecutive rows per workgroup
+// and exchange the spectrum with the column pass in transposed layout
+// specT[k * H + y], in contiguous TR-element segments; the column pass then
+// reads whole contiguous rows of specT.  Same butterflies, same roundings as
+// the untransposed passes.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int xcd_group(int b, int G) {
+  // workgroup i runs on XCD i % 8: give each XCD a contiguous range of groups
+  return (G & 7) == 0 ? (b & 7) * (G >> 3) + (b >> 3) : b;
+}
+
+// CIC fold of row y into dst[0..W) (see k_deposit_rows_fft for the order)
+__device__ void deposit_row(int y, int W, const int* __restrict__ row_off, int* __restrict__ row_list,
+                            int* __restrict__ sort_tmp, unsigned long long* __restrict__ key_tmp,
+                            size_t key_half, const int* __restrict__ cic_cell,
+                            const float* __restrict__ cic_w, float2* dst, int* sort_lds,
+                            unsigned long long* keys_lds, unsigned long long* sort_keys) {
+  const int off = row_off[y];
+  const int m = row_off[y + 1] - off;
+  int* nodes = row_list + off;
+  sort_unique<int>(nodes, m, sort_lds, DEP_IDS_LDS, sort_tmp + off, 0x7fffffff);
+  for (int x = threadIdx.x; x < W; x += blockDim.x) dst[x] = make_float2(0.0f, 0.0f);
+  const int ne = 2 * m;
+  unsigned long long* keys = (ne <= DEP_KEYS_LDS) ? keys_lds : (key_tmp + 2 * (size_t)off);
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    const int n = nodes[j];
+    const unsigned long long x0 = (unsigned)cic_cell[4 * n + 0], x1 = (unsigned)cic_cell[4 * n + 1];
+    const unsigned long long qb = (cic_cell[4 * n + 2] == y) ? 0ull : 2ull;
+    keys[2 * j] = (x0 << 34) | (qb << 32) | (unsigned)j;
+    keys[2 * j + 1] = (x1 << 34) | ((qb + 1ull) << 32) | (unsigned)j;
+  }
+  __threadfence_block();
+  __syncthreads();
+  sort_unique<unsigned long long>(keys, ne, sort_keys, DEP_KEYS_LDS, key_tmp + key_half + 2 * (size_t)off,
+                                  ~0ull);
+  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+    const unsigned long long k = keys[i];
+    const int x = (int)(k >> 34);
+    if (i > 0 && (int)(keys[i - 1] >> 34) == x) continue;
+    float acc = 0.0f;
+    for (int e = i; e < ne; ++e) {
+      const unsigned long long ke = keys[e];
+      if ((int)(ke >> 34) != x) break;
+      const int q = (int)((ke >> 32) & 3ull);
+      const int n = nodes[(int)(ke & 0xffffffffull)];
+      acc = acc + cic_w[4 * n + q];
+    }
+    dst[x].x = acc;
+  }
+  __syncthreads();
+}
+
+// Forward row FFT of TR rows (from the CIC deposit when `real_in` is NULL,
+// else from a real (H, W) array), written transposed to specT.
+template <int TR>
+__global__ void __launch_bounds__(256) k_rows_fwd_T(
+    int H, int W, const float* __restrict__ real_in, const int* __restrict__ row_off,
+    int* __restrict__ row_list, int* __restrict__ sort_tmp, unsigned long long* __restrict__ key_tmp,
+    size_t key_half, const int* __restrict__ cic_cell, const float* __restrict__ cic_w,
+    float2* __restrict__ specT, const float2* __restrict__ tw_g, const int* __restrict__ radix_g,
+    int nrad, float* __restrict__ density_out) {
+  extern __shared__ float2 lds[];
+  __shared__ int sort_lds[DEP_IDS_LDS];
+  __shared__ unsigned long long keys_lds[DEP_KEYS_LDS];
+  __shared__ unsigned long long sort_keys[DEP_KEYS_LDS];
+  __shared__ int s_radix[32];
+  float2* a = lds;
+  float2* b = lds + (size_t)TR * W;
+  float2* tw = lds + (size_t)2 * TR * W;
+  if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
+  load_twiddles(tw, tw_g, W);
+  const int y0 = xcd_group(blockIdx.x, gridDim.x) * TR;
+  const int nr = min(TR, H - y0);
+  if (real_in) {
+    for (int i = threadIdx.x; i < nr * W; i += blockDim.x)
+      a[i] = make_float2(real_in[(size_t)y0 * W + i], 0.0f);
+    __syncthreads();
+  } else {
+    for (int r = 0; r < nr; ++r)
+      deposit_row(y0 + r, W, row_off, row_list, sort_tmp, key_tmp, key_half, cic_cell, cic_w,
+                  a + (size_t)r * W, sort_lds, keys_lds, sort_keys);
+    if (density_out)
+      for (int i = threadIdx.x; i < nr * W; i += blockDim.x) density_out[(size_t)y0 * W + i] = a[i].x;
+  }
+  float2* res = fft_lds(a, b, W, W, nr, s_radix, nrad, tw, false);
+  // transposed store: consecutive threads write consecutive rows of one column
+  for (int i = threadIdx.x; i < nr * W; i += blockDim.x) {
+    const int k = i / nr, r = i - k * nr;
+    specT[(size_t)k * H + y0 + r] = res[(size_t)r * W + k];
+  }
+}
+
+// Column pass on the transposed spectrum: one column k per workgroup, forward
+// FFT along y, transfer function, inverse FFT scaled by 1/H.
+__global__ void __launch_bounds__(256) k_cols_T(int H, int W, float2* __restrict__ specT,
+                                                const float2* __restrict__ tw_g,
+                                                const int* __restrict__ radix_g, int nrad,
+                                                const float* __restrict__ kfT) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_radix[32];
+  float2* a = lds;
+  float2* b = lds + H;
+  float2* tw = lds + 2 * H;
+  if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
+  load_twiddles(tw, tw_g, H);
+  const int k = blockIdx.x;
+  float2* col = specT + (size_t)k * H;
+  for (int y = threadIdx.x; y < H; y += blockDim.x) a[y] = col[y];
+  __syncthreads();
+  float2* r = fft_lds(a, b, H, H, 1, s_radix, nrad, tw, false);
+  float2* o = (r == a) ? b : a;
+  const float* kc = kfT + (size_t)k * H;
+  for (int y = threadIdx.x; y < H; y += blockDim.x) {
+    const float kf = kc[y];
+    float2 v = r[y];
+    v.x = v.x * kf;
+    v.y = v.y * kf;
+    r[y] = v;
+  }
+  __syncthreads();
+  float2* res = fft_lds(r, o, H, H, 1, s_radix, nrad, tw, true);
+  const float sc = (float)(1.0 / (double)H);
+  for (int y = threadIdx.x; y < H; y += blockDim.x) {
+    const float2 v = res[y];
+    col[y] = make_float2(v.x * sc, v.y * sc);
+  }
+}
+
+// Inverse row FFT of TR rows read transposed from specT; real part scaled by 1/W.
+template <int TR>
+__global__ void __launch_bounds__(256) k_rows_inv_T(int H, int W, const float2* __restrict__ specT,
+                                                    float* __restrict__ out,
+                                                    const float2* __restrict__ tw_g,
+                                                    const int* __restrict__ radix_g, int nrad) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_radix[32];
+  float2* a = lds;
+  float2* b = lds + (size_t)TR * W;
+  float2* tw = lds + (size_t)2 * TR * W;
+  if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
+  load_twiddles(tw, tw_g, W);
+  const int y0 = xcd_group(blockIdx.x, gridDim.x) * TR;
+  const int nr = min(TR, H - y0);
+  for (int i = threadIdx.x; i < nr * W; i += blockDim.x) {
+    const int k = i / nr, r = i - k * nr;
+    a[(size_t)r * W + k] = specT[(size_t)k * H + y0 + r];
+  }
+  __syncthreads();
+  float2* res = fft_lds(a, b, W, W, nr, s_radix, nrad, tw, true);
+  const float sc = (float)(1.0 / (double)W);
+  for (int i = threadIdx.x; i < nr * W; i += blockDim.x) out[(size_t)y0 * W + i] = res[i].x * sc;
+}
+
+// Transposed transfer function kfT[k * H + y] = exp(coef (fr(y)^2 + fc(k)^2)).
+__global__ void k_gauss_table_T(int H, int W, float coef, float* __restrict__ kfT) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)H * W) return;
+  const int k = (int)(i / H), y = (int)(i - (size_t)k * H);
+  const float fr = fftfreq(y, H), fc = fftfreq(k, W);
+  kfT[i] = dexp(coef * (fr * fr + fc * fc));
+}
+
 }  // namespace
 
 hipError_t launch_predict(const Consts& c, int N, const float* pos, const float* vel, float* pred_w,
@@ -438,7 +601,48 @@ hipError_t launch_fft_rows(int H, int W, const float* real_in, float2* spec, con
 
 hipError_t launch_gauss_table(int H, int W, float coef, float* kf, hipStream_t s) {
   const size_t n = (size_t)H * W;
-  hipLaunchKernelGGL(k_gauss_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, W, coef, kf);
+  hipLaunchKernelGGL(k_gauss_table_T, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, W, coef,
+                     kf);
+  return hipGetLastError();
+}
+
+int fft_rows_per_group(int H, int W) {
+  // One row per workgroup: the FFT passes are latency-bound, so parallelism
+  // across rows beats longer transposed segments.  The XCD-grouped mapping
+  // keeps the rows that share a specT cache line on one XCD, whose L2 merges
+  // the 8-byte transposed writes before they reach HBM.
+  (void)H;
+  (void)W;
+  return 1;
+}
+
+hipError_t launch_steric_fft_T(int H, int W, const float* real_in, const int* row_off, int* row_list,
+                               int* sort_scratch, unsigned long long* key_scratch, size_t key_half,
+                               const int* cic_cell, const float* cic_w, float2* specT,
+                               const float2* tw_w, const int* radix_w, int nrad_w,
+                               const float2* tw_h, const int* radix_h, int nrad_h, const float* kfT,
+                               float* density_out, float* out, hipStream_t s) {
+  const int TR = fft_rows_per_group(H, W);
+  const int G = (H + TR - 1) / TR;
+  const size_t lds_rows = ((size_t)2 * TR * W + W) * sizeof(float2);
+  const size_t lds_cols = (size_t)3 * H * sizeof(float2);
+#define MC_ROWS(T)                                                                                  \
+  case T:                                                                                           \
+    hipLaunchKernelGGL(k_rows_fwd_T<T>, dim3(G), dim3(256), lds_rows, s, H, W, real_in, row_off,   \
+                       row_list, sort_scratch, key_scratch, key_half, cic_cell, cic_w, specT, tw_w, \
+                       radix_w, nrad_w, density_out);                                               \
+    break;
+  switch (TR) { MC_ROWS(4) MC_ROWS(2) default: MC_ROWS(1) }
+#undef MC_ROWS
+  hipLaunchKernelGGL(k_cols_T, dim3(W), dim3(256), lds_cols, s, H, W, specT, tw_h, radix_h, nrad_h,
+                     kfT);
+#define MC_IROWS(T)                                                                                 \
+  case T:                                                                                           \
+    hipLaunchKernelGGL(k_rows_inv_T<T>, dim3(G), dim3(256), lds_rows, s, H, W, specT, out, tw_w,   \
+                       radix_w, nrad_w);                                                            \
+    break;
+  switch (TR) { MC_IROWS(4) MC_IROWS(2) default: MC_IROWS(1) }
+#undef MC_IROWS
   return hipGetLastError();
 }
 
