@@ -62,6 +62,7 @@ struct mc_plan {
   size_t bytes = 0;
   // rod
   RodLayoutDev L{};
+  int* comp_of_node = nullptr;  // component of every node
   size_t rod_lds_bytes = 0;
   int rod_use_lds = 1;
   float* rod_gscratch = nullptr;
@@ -232,6 +233,7 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
   MCCHK(upload(p, &a11, ninc_flat, "node_inc"));
   MCCHK(upload(p, &a12, einc_off, "edge_inc_off"));
   MCCHK(upload(p, &a13, einc_flat, "edge_inc"));
+  MCCHK(upload(p, &p->comp_of_node, comp_of_node, "comp_of_node"));
   p->L = RodLayoutDev{C, a0, a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11, a12, a13};
   return MC_OK;
 }
@@ -370,9 +372,9 @@ mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float*
 }
 
 // Streams and events of one enqueued step.  With a side stream the fused LBM
-// pass of this step runs there, concurrently with the drag of this step and
-// the steric / rod / IB binning of the next step; only the next step's IB
-// solve (which reads the new lattice) waits for it.
+// pass of this step runs there, concurrently with the steric and rod work of
+// the next step; only the next step's IB (which reads the new lattice and
+// rewrites the force and support mask) waits for it.
 struct StepSync {
   hipStream_t b = nullptr;          // side stream for the LBM pass (nullptr: main stream)
   hipEvent_t ev_force = nullptr;    // main -> side: IB force of this step is ready
@@ -395,9 +397,8 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
   const size_t hwf = (size_t)p->Hf * p->Wf;
   long long* tick = act ? p->tick : nullptr;
   if (c.enable_fluid) {
-    HIPCHK(launch_ib_bin(c, p->ib, out->position, out->velocity, s));
     if (sy.wait_lbm) HIPCHK(hipStreamWaitEvent(s, sy.wait_lbm, 0));
-    HIPCHK(launch_ib_solve(c, p->ib, in->f_grid, s));
+    HIPCHK(launch_ib(c, p->ib, in->f_grid, out->position, out->velocity, out->position, tick, s));
     hipStream_t ls = s;
     if (sy.b) {
       HIPCHK(hipEventRecord(sy.ev_force, s));
@@ -409,7 +410,6 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
                       p->ib.mask_words_per_row, p->ib.force, ls));
     if (sy.t1) HIPCHK(hipEventRecord(sy.t1, ls));
     if (sy.b) HIPCHK(hipEventRecord(sy.ev_lbm, sy.b));
-    HIPCHK(launch_ib_drag(c, p->ib, out->position, out->velocity, out->position, tick, s));
   } else {
     if (tick) HIPCHK(launch_tick(tick, s));
     if (out->f_grid != in->f_grid)
@@ -502,33 +502,43 @@ mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t 
   // immersed boundary
   IBWork& w = p->ib;
   w.N = N;
-  w.tiles_x = (p->Wf + IB_TILE - 1) / IB_TILE;
-  w.tiles_y = (p->Hf + IB_TILE - 1) / IB_TILE;
-  w.T = w.tiles_x * w.tiles_y;
+  w.C = p->L.num_comp;
+  w.comp_of_node = p->comp_of_node;
+  w.comp_node_off = p->L.comp_node_off;
+  w.comp_nodes = p->L.comp_nodes;
   w.mask_words_per_row = (p->Wf + 31) / 32;
+  w.cl_bpn = ib_scratch_bytes_per_node(max_kernel_size);
+  w.lds_bytes = ib_lds_bytes(p->lds_max);
+  w.dirty = false;
   const size_t taps = (size_t)max_kernel_size * max_kernel_size;
-  // a node's stencil touches at most min(K, 3) tiles per axis
-  const size_t tpn = (size_t)std::min(max_kernel_size, 3) * std::min(max_kernel_size, 3);
+  const size_t C = (size_t)w.C;
   if ((st = p->alloc(&w.st_sx, (size_t)N, "st sx")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.st_sy, (size_t)N, "st sy")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.st_w, taps * N, "st w")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.st_won, (size_t)N, "st won")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.vib, 2 * (size_t)N, "vib")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.corr, 2 * (size_t)N, "corr")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.tile_cnt, (size_t)w.T, "tile cnt")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.tile_off, (size_t)w.T + 1, "tile off")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.tile_fill, (size_t)w.T, "tile fill")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.tile_active, (size_t)w.T, "tile active")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.n_active, 1, "n active")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.tile_list, tpn * N, "tile list")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.sort_scratch, tpn * N, "tile sort")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.rho, hwf, "rho U")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.u0, 2 * hwf, "u U")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.ucorr, 2 * hwf, "ucorr U")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.inv, hwf, "inv U")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.owner, hwf, "cell owner")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.cell_seg, hwf, "cell seg")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.links, taps * N, "cluster links")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.nlinks, 1, "cluster link count")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.label, C, "cluster label")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.cl_cnt, C, "cluster nodes")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.cl_node_off, C, "cluster node off")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.cl_mem_cnt, C, "cluster members")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.cl_mem_off, C, "cluster member off")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.cl_members, C, "cluster member list")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.cl_scratch, w.cl_bpn * N, "cluster scratch")) != MC_OK) return bail(st);
+  {
+    std::vector<int> ident(C);
+    for (size_t i = 0; i < C; ++i) ident[i] = (int)i;
+    if ((st = upload(p, &w.uf_parent, ident, "union-find")) != MC_OK) return bail(st);
+    if (hipMemset(w.owner, 0xff, hwf * sizeof(int)) != hipSuccess)
+      return bail(fail(MC_ERR_HIP, "hipMemset(cell owner) failed"));
+  }
+  if (ib_prepare(p->lds_max) != hipSuccess)
+    return bail(fail(MC_ERR_HIP, "cluster solve: %zu B of dynamic LDS refused", w.lds_bytes));
   if ((st = p->alloc(&w.force, 2 * hwf, "force U")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.mask, (size_t)w.mask_words_per_row * p->Hf, "U mask")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.cell_span, hwf, "U spans")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.entries, taps * tpn * N, "U entries")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->f_tmp, 9 * hwf, "f ping-pong")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->tick, 1, "tick")) != MC_OK) return bail(st);
   (void)hw;
@@ -782,10 +792,8 @@ mc_status mc_ib_lbm_step(mc_plan* p, const mc_params* q, const float* position, 
   const Consts c = derive(p, &qq, dt);
   if (p->N > 0 && velocity_out != velocity)
     HIPCHK(hipMemcpyAsync(velocity_out, velocity, 2 * (size_t)p->N * sizeof(float), hipMemcpyDeviceToDevice, s));
-  HIPCHK(launch_ib_bin(c, p->ib, position, velocity_out, s));
-  HIPCHK(launch_ib_solve(c, p->ib, f_in, s));
+  HIPCHK(launch_ib(c, p->ib, f_in, position, velocity_out, pos_out ? pos_out : p->node_tmp, nullptr, s));
   HIPCHK(launch_lbm(c, f_in, f_out, fluid_velocity_out, p->ib.mask, p->ib.mask_words_per_row, p->ib.force, s));
-  HIPCHK(launch_ib_drag(c, p->ib, position, velocity_out, pos_out ? pos_out : p->node_tmp, nullptr, s));
   return MC_OK;
 }
 

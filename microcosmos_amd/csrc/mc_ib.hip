@@ -1,23 +1,34 @@
-// Sparse immersed-boundary transfer (fluid.py:202-384) with deterministic,
-// atomic-free floating-point accumulation.
+// Sparse immersed-boundary transfer (fluid.py:202-384), solved per interaction
+// cluster with deterministic, atomic-free floating-point accumulation.
+//
+// Two nodes interact through the IB only when their K x K stencils share a
+// lattice cell.  Filaments (connected components of the edge graph) whose
+// stencils share cells are merged into clusters; no cell is touched by two
+// clusters, so one workgroup can run a whole cluster's IB -- kernel-weight
+// fold, every iteration, post-clamp, force and node drag -- with workgroup
+// barriers only.  Three launches per step:
+//   1. stencil: per node, wrapped start cell, normalised K*K weights
+//      (fluid.py:40-87) and v_ib; per tap, a compare-and-swap that makes the
+//      first arriving component the cell's owner.  The node records which of
+//      its taps won; a tap that finds another component's claim emits a link;
+//   2. cluster build (one workgroup): union-find over the links (in LDS),
+//      cluster labels, node counts, offsets and member lists;
+//   3. cluster solve (one workgroup per cluster): the cluster's nodes ranked
+//      by id, its covered cells ("segments", one per won tap), each cell's
+//      entry list in ascending node id, then the reference's arithmetic.
 //
 // The reference scatters node-major, tap-minor (fluid.py:111-118,130-132).  A
-// node's K x K stencil touches each cell at most once (grid >= K per axis), so
-// the value a cell accumulates is a left fold over the covering nodes in
-// ascending node order.  The device reproduces that fold exactly:
-//   1. stencil: per node, wrapped start cell and normalised K*K weights
-//      (fluid.py:40-87); count node-tile incidences (integer atomics);
-//   2. scan: tile offsets and the list of active tiles;
-//   3. fill: node ids into their tiles' bins (order unspecified);
-//   4. tile init: sort each bin ascending, then every cell thread folds the
-//      kernel weights of its covering nodes in that order; covered cells (the
-//      support U) get rho, u, u_corrected = u, 1/max(kw, 1) and a U bit;
-//   5. per IB iteration: node gather (interpolation + clamped correction),
-//      then the per-cell ordered fold of the spread (fluid.py:292-316);
-//      the last cell pass also post-clamps and forms F = (u_c - u) rho;
-//   6. drag: node gather of F (fluid.py:364-382) and the final position wrap
-//      (simulate.py:97).
-// Only U (about 6 % of cells at 16k nodes on 1024^2) is ever read or written.
+// node's stencil covers each cell at most once (lattice >= K per axis), so a
+// cell's sum is a left fold over its covering nodes in ascending node id; a
+// node's gather is a left fold over its taps in tap order (fluid.py:90-97).
+// Both folds are reproduced exactly.  Segment numbering comes from LDS
+// atomics and is arbitrary; no floating-point result depends on it.
+//
+// Idle state between calls: owner == -1 on every lattice cell and no pending
+// links.  The solve resets the owners of the cells it covered, the build
+// clears the link count.
+#include <algorithm>
+
 #include "mc_bin.h"
 #include "mc_common.h"
 #include "mc_internal.h"
@@ -25,39 +36,66 @@
 namespace mc {
 namespace {
 
-constexpr int CH = 128;  // nodes staged in LDS per chunk
-
 __device__ __forceinline__ float peskin(float r, float h) {
   float ra = fabsf(r) / h;
   return (ra <= 2.0f) ? 0.25f * (1.0f + dcos((MC_PI_F * ra) / 2.0f)) : 0.0f;
 }
 
-// Distinct tile indices covered along one axis by cells s..s+K-1 (mod n).
-__device__ __forceinline__ int axis_tiles(int s, int K, int n, int* out) {
-  int cnt = 0;
-  for (int d = 0; d < K; ++d) {
-    int t = wrap1(s + d, n) / IB_TILE;
-    bool seen = false;
-    for (int q = 0; q < cnt; ++q) seen = seen || (out[q] == t);
-    if (!seen) out[cnt++] = t;
+// Relaxed load at device scope: reads a value produced by atomics at L2
+// rather than a possibly stale L1 line.
+__device__ __forceinline__ int ld_l2(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Union-find on a generic pointer (LDS or global memory).
+__device__ __forceinline__ int uf_find(int* p, int x) {
+  while (true) {
+    const int q = __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (q == x) return x;
+    x = q;
   }
-  return cnt;
+}
+
+// Link the larger root under the smaller one, so a tree's root is its
+// smallest component id; a failed CAS means the root moved: retry.
+__device__ __forceinline__ void uf_union(int* p, int a, int b) {
+  while (true) {
+    a = uf_find(p, a);
+    b = uf_find(p, b);
+    if (a == b) return;
+    if (a < b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(&p[a], a, b) == a) return;
+  }
 }
 
 template <int K>
-__global__ void k_ib_stencil(Consts c, int N, const float* __restrict__ pos,
-                             const float* __restrict__ vel, int* __restrict__ st_sx,
-                             int* __restrict__ st_sy, float* __restrict__ st_w,
-                             float* __restrict__ vib, int* __restrict__ tile_cnt, int tiles_x) {
+__global__ void __launch_bounds__(64) k_ib_stencil(
+    Consts c, int N, const float* __restrict__ pos, const float* __restrict__ vel,
+    const int* __restrict__ comp_of_node, int* __restrict__ st_sx, int* __restrict__ st_sy,
+    float* __restrict__ st_w, unsigned long long* __restrict__ st_won, float* __restrict__ vib,
+    int* __restrict__ owner, int2* __restrict__ links, int* __restrict__ nlinks) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   constexpr int K2 = K * K;
+  const int W = c.fl_w, H = c.fl_h;
   const float gx = pos[2 * n] * c.scale_x;
   const float gy = pos[2 * n + 1] * c.scale_y;
   vib[2 * n] = (vel[2 * n] * c.scale_x) * c.dt;
   vib[2 * n + 1] = (vel[2 * n + 1] * c.scale_y) * c.dt;
   const float sxf = floorf(gx) - (float)c.half;
   const float syf = floorf(gy) - (float)c.half;
+  const int sx = imod(f2i(sxf), W);
+  const int sy = imod(f2i(syf), H);
+  const int comp = comp_of_node[n];
+  // claims first, so their latency overlaps the weight arithmetic
+  int prev[K2];
+#pragma unroll
+  for (int t = 0; t < K2; ++t)
+    prev[t] = atomicCAS(&owner[wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W)], -1, comp);
   // 1-D weights: tap t uses wx[t / K] and wy[t % K] (fluid.py:53-69)
   float wx[K], wy[K];
 #pragma unroll
@@ -71,277 +109,716 @@ __global__ void k_ib_stencil(Consts c, int N, const float* __restrict__ pos,
   const float den = total + MC_EPSIL;
 #pragma unroll
   for (int t = 0; t < K2; ++t) st_w[(size_t)n * K2 + t] = (wx[t / K] * wy[t % K]) / den;
-  const int sx = imod(f2i(sxf), c.fl_w);
-  const int sy = imod(f2i(syf), c.fl_h);
   st_sx[n] = sx;
   st_sy[n] = sy;
-  int tx[K], ty[K];
-  const int ntx = axis_tiles(sx, K, c.fl_w, tx);
-  const int nty = axis_tiles(sy, K, c.fl_h, ty);
-  for (int a = 0; a < nty; ++a)
-    for (int b = 0; b < ntx; ++b) atomicAdd(&tile_cnt[ty[a] * tiles_x + tx[b]], 1);
+  unsigned long long won = 0;
+  int last = -1;
+#pragma unroll
+  for (int t = 0; t < K2; ++t) {
+    const int q = prev[t];
+    if (q == -1) won |= 1ull << t;
+    // a cell claimed by another component links the two (consecutive
+    // repeats dropped; at most one link per tap)
+    if (q != -1 && q != comp && q != last) {
+      last = q;
+      links[atomicAdd(nlinks, 1)] = make_int2(q, comp);
+    }
+  }
+  st_won[n] = won;
 }
 
-__global__ void k_ib_fill(Consts c, int N, const int* __restrict__ st_sx,
-                          const int* __restrict__ st_sy, const int* __restrict__ tile_off,
-                          int* __restrict__ tile_fill, int* __restrict__ tile_list, int tiles_x) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  int tx[IB_KMAX], ty[IB_KMAX];
-  const int ntx = axis_tiles(st_sx[n], c.K, c.fl_w, tx);
-  const int nty = axis_tiles(st_sy[n], c.K, c.fl_h, ty);
-  for (int a = 0; a < nty; ++a)
-    for (int b = 0; b < ntx; ++b) {
-      const int t = ty[a] * tiles_x + tx[b];
-      const int p = atomicAdd(&tile_fill[t], 1);
-      tile_list[tile_off[t] + p] = n;
-    }
-}
-
-struct TileStage {
-  int id[CH];
-  int sx[CH];
-  int sy[CH];
-  float cx[CH];
-  float cy[CH];
-  float w[CH * IB_TAPS_MAX];
-};
-
-// Stage nodes list[j0 .. j0+cnt) (cnt <= CH <= blockDim) into LDS: one node
-// per thread, all of its global loads issued together.
-__device__ __forceinline__ void stage_nodes(TileStage& S, const int* list, int j0, int cnt, int K2,
-                                            const int* st_sx, const int* st_sy, const float* st_w,
-                                            const float* corr) {
-  const int j = threadIdx.x;
-  if (j < cnt) {
-    const int n = list[j0 + j];
-    S.id[j] = n;
-    S.sx[j] = st_sx[n];
-    S.sy[j] = st_sy[n];
-    if (corr) {
-      S.cx[j] = corr[2 * n];
-      S.cy[j] = corr[2 * n + 1];
-    }
-    const float* wn = st_w + (size_t)n * K2;
-    float wv[IB_TAPS_MAX];
-#pragma unroll
-    for (int t = 0; t < IB_TAPS_MAX; ++t)
-      if (t < K2) wv[t] = wn[t];
-#pragma unroll
-    for (int t = 0; t < IB_TAPS_MAX; ++t)
-      if (t < K2) S.w[j * K2 + t] = wv[t];
+// Single workgroup.  Unions the linked components (union-find in LDS when the
+// components fit, else in global memory), then labels every component with
+// its cluster root (the smallest component id of the cluster), counts and
+// lists each cluster's members.  Also advances the actuation step counter:
+// the rod kernel of this step, which read it, has completed.
+__global__ void __launch_bounds__(1024) k_ib_cluster_build(
+    int C, int lds_ints, const int* __restrict__ comp_node_off, const int2* __restrict__ links,
+    int* __restrict__ nlinks, int* __restrict__ gparent, int* __restrict__ label,
+    int* __restrict__ cl_cnt, int* __restrict__ cl_node_off, int* __restrict__ cl_mem_cnt,
+    int* __restrict__ cl_mem_off, int* __restrict__ cl_members, long long* __restrict__ tick) {
+  extern __shared__ __align__(16) int sm[];
+  __shared__ int wsum[16];
+  __shared__ int wsum2[16];
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const bool lds = 3 * (long long)C <= lds_ints;
+  int* par = lds ? sm : gparent;  // global: singletons between calls
+  int* cnt = lds ? sm + C : cl_cnt;
+  int* mcnt = lds ? sm + 2 * C : cl_mem_cnt;
+  if (tick && tid == 0) *tick = *tick + 1;
+  for (int c = tid; c < C; c += bs) {
+    if (lds) par[c] = c;
+    cnt[c] = 0;
+    mcnt[c] = 0;
   }
   __syncthreads();
+  const int nl = ld_l2(nlinks);
+  for (int i = tid; i < nl; i += bs) {
+    const int2 l = links[i];
+    uf_union(par, l.x, l.y);
+  }
+  __syncthreads();
+  for (int c = tid; c < C; c += bs) label[c] = uf_find(par, c);
+  __syncthreads();
+  for (int c = tid; c < C; c += bs) {
+    if (!lds) par[c] = c;
+    const int r = label[c];
+    atomicAdd(&cnt[r], comp_node_off[c + 1] - comp_node_off[c]);
+    atomicAdd(&mcnt[r], 1);
+  }
+  __syncthreads();
+  int* fill = lds ? par : cl_mem_cnt;  // fill cursors
+  int carry = 0, carry2 = 0;
+  for (int base = 0; base < C; base += bs) {
+    const int c = base + tid;
+    const bool root = c < C && label[c] == c;
+    // global counters were written by atomics (performed at L2): read there
+    const int v = root ? (lds ? cnt[c] : ld_l2(&cnt[c])) : 0;
+    const int v2 = root ? (lds ? mcnt[c] : ld_l2(&mcnt[c])) : 0;
+    int tot, tot2;
+    const int inc = block_inclusive_scan(v, wsum, &tot);
+    const int inc2 = block_inclusive_scan(v2, wsum2, &tot2);
+    if (c < C) {
+      cl_node_off[c] = carry + inc - v;
+      cl_mem_off[c] = carry2 + inc2 - v2;
+      if (lds) {
+        cl_cnt[c] = v;
+        cl_mem_cnt[c] = v2;
+      }
+      fill[c] = 0;  // global: counts back up to the member count
+    }
+    carry += tot;
+    carry2 += tot2;
+  }
+  __syncthreads();
+  for (int c = tid; c < C; c += bs) {
+    const int r = label[c];
+    cl_members[cl_mem_off[r] + atomicAdd(&fill[r], 1)] = c;
+  }
+  if (tid == 0) *nlinks = 0;
 }
 
-// Tap index of cell (cx, cy) in the stencil starting at (sx, sy), or -1.
-__device__ __forceinline__ int tap_of(int cx, int cy, int sx, int sy, int K, int W, int H) {
-  const int dx = wrap1(cx - sx, W);
-  const int dy = wrap1(cy - sy, H);
-  return (dx < K && dy < K) ? dx * K + dy : -1;
+__device__ __forceinline__ size_t al16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+struct ClusterArgs {
+  Consts c;
+  const int *comp_node_off, *comp_nodes, *label, *cl_cnt, *cl_node_off, *cl_mem_cnt, *cl_mem_off,
+      *cl_members;
+  const int *st_sx, *st_sy;
+  const unsigned long long* st_won;
+  const float *st_w, *vib, *f_in, *pos;
+  float *vel_io, *pos_out, *force;
+  int *cell_seg, *owner;
+  uint32_t* mask;
+  int mwpr;
+  unsigned char* gscratch;
+  size_t bpn;        // global scratch bytes per node
+  size_t lds_bytes;  // dynamic LDS of the launch (a multiple of 16)
+};
+
+// Arrays of one cluster solve.  e = r * K2 + t indexes tap t of the node of
+// rank r; s indexes segments.  Placed in LDS when they fit, else in the
+// cluster's global scratch; scell / srho / su0 always live in global scratch.
+struct CLayout {
+  int *nodes, *sstart, *eidx, *tapseg, *tmp;
+  float *wt, *sinv;
+  float2 *corr, *vib, *suc;
+};
+
+struct GExtra {
+  int* scell;
+  float* srho;
+  float2* su0;
+};
+
+// Global scratch of a cluster, sized for ns <= ne.
+__device__ __forceinline__ void glayout(unsigned char* base, int nq, int ne, CLayout& L, GExtra& X) {
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    unsigned char* p = base + off;
+    off += al16(b);
+    return p;
+  };
+  X.scell = (int*)take(4 * (size_t)ne);
+  X.srho = (float*)take(4 * (size_t)ne);
+  X.su0 = (float2*)take(8 * (size_t)ne);
+  L.nodes = (int*)take(4 * (size_t)nq);
+  L.tmp = (int*)take(4 * (size_t)nq);
+  L.sstart = (int*)take(4 * ((size_t)ne + 1));
+  L.eidx = (int*)take(4 * (size_t)ne);
+  L.tapseg = (int*)take(4 * (size_t)ne);
+  L.wt = (float*)take(4 * (size_t)ne);
+  L.corr = (float2*)take(8 * (size_t)nq);
+  L.vib = (float2*)take(8 * (size_t)nq);
+  L.sinv = (float*)take(4 * (size_t)ne);
+  L.suc = (float2*)take(8 * (size_t)ne);
 }
 
-__global__ void __launch_bounds__(256) k_ib_tile_init(
-    Consts c, const int* __restrict__ tile_off, const int* __restrict__ active,
-    const int* __restrict__ n_active, int* __restrict__ tile_list, int* __restrict__ sort_tmp,
-    const int* __restrict__ st_sx, const int* __restrict__ st_sy, const float* __restrict__ st_w,
-    const float* __restrict__ f_in, float* __restrict__ rho_s, float* __restrict__ u0,
-    float* __restrict__ ucorr, float* __restrict__ inv, uint32_t* __restrict__ mask, int mwpr,
-    int tiles_x, int2* __restrict__ cell_span, int2* __restrict__ entries) {
-  __shared__ TileStage S;
-  __shared__ int sort_lds[SORT_LDS_CAP];
-  __shared__ int wsum[16];
-  const int W = c.fl_w, H = c.fl_h, K = c.K, K2 = K * K;
+// The same arrays in LDS for ns segments; returns the bytes used.
+__device__ __forceinline__ size_t llayout(unsigned char* base, int nq, int ne, int ns, CLayout& L) {
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    unsigned char* p = base + off;
+    off += al16(b);
+    return p;
+  };
+  L.nodes = (int*)take(4 * (size_t)nq);
+  L.sstart = (int*)take(4 * ((size_t)ns + 1));
+  L.eidx = (int*)take(4 * (size_t)ne);
+  L.tapseg = (int*)take(4 * (size_t)ne);
+  L.wt = (float*)take(4 * (size_t)ne);
+  L.corr = (float2*)take(8 * (size_t)nq);
+  L.vib = (float2*)take(8 * (size_t)nq);
+  L.sinv = (float*)take(4 * (size_t)ns);
+  L.suc = (float2*)take(8 * (size_t)ns);
+  L.tmp = nullptr;
+  return off;
+}
+
+// Workgroup size of the cluster solve: the unrolled K*K tap arrays of K >= 5
+// need more than the 128 VGPRs a 1024-thread workgroup leaves per lane.
+template <int K>
+constexpr int cluster_threads() { return K <= 4 ? 1024 : 512; }
+
+constexpr int SPT = 4;  // segments per thread whose loads are issued together
+
+// Small-cluster solve: at most one node per thread (thread r holds the node
+// of rank r, its taps' segments and weights in registers) and every array in
+// LDS.  Layout from `base`: sstart[ne+1] eidx[ne] wt[ne] corr[nq], then per
+// segment scell sinv srho suc su0.
+__device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns) {
+  return al16(4 * ((size_t)ne + 1)) + 2 * al16(4 * (size_t)ne) + al16(8 * (size_t)nq) +
+         3 * al16(4 * (size_t)ns) + 2 * al16(8 * (size_t)ns);
+}
+
+template <int K>
+__device__ __forceinline__ void cluster_small(const ClusterArgs& a, unsigned char* base,
+                                              const int* nodes, int nq, int ns, int* wsum,
+                                              int* s_seg) {
+  constexpr int K2 = K * K;
+  const Consts& c = a.c;
+  const int ne = nq * K2;
+  const int W = c.fl_w, H = c.fl_h;
   const size_t HW = (size_t)H * W;
-  const int na = *n_active;
-  for (int b = blockIdx.x; b < na; b += gridDim.x) {
-    const int t = active[b];
-    const int off = tile_off[t];
-    const int m = tile_off[t + 1] - off;
-    sort_unique_ids(tile_list + off, m, sort_lds, sort_tmp + off);
-    const int cx = (t % tiles_x) * IB_TILE + (threadIdx.x & (IB_TILE - 1));
-    const int cy = (t / tiles_x) * IB_TILE + (threadIdx.x / IB_TILE);
-    const bool valid = cx < W && cy < H;
-    // pass 1: per cell, fold of the covering nodes' weights and their count
-    float kw = 0.0f;
-    int cnt = 0;
-    for (int j0 = 0; j0 < m; j0 += CH) {
-      const int nc = min(CH, m - j0);
-      stage_nodes(S, tile_list + off, j0, nc, K2, st_sx, st_sy, st_w, nullptr);
-      if (valid) {
-        for (int j = 0; j < nc; ++j) {
-          const int tap = tap_of(cx, cy, S.sx[j], S.sy[j], K, W, H);
-          if (tap >= 0) {
-            ++cnt;
-            kw = kw + S.w[j * K2 + tap];
-          }
-        }
-      }
-      __syncthreads();
-    }
-    // the tile's entry region holds at most K*K entries per node-tile incidence
-    int tot;
-    const int incl = block_inclusive_scan(cnt, wsum, &tot);
-    const int start = K2 * off + (incl - cnt);
-    // pass 2: the ordered (node, weight) list of every covered cell
-    if (m > CH) {
-      int e = start;
-      for (int j0 = 0; j0 < m; j0 += CH) {
-        const int nc = min(CH, m - j0);
-        stage_nodes(S, tile_list + off, j0, nc, K2, st_sx, st_sy, st_w, nullptr);
-        if (valid && cnt > 0) {
-          for (int j = 0; j < nc; ++j) {
-            const int tap = tap_of(cx, cy, S.sx[j], S.sy[j], K, W, H);
-            if (tap >= 0) entries[e++] = make_int2(S.id[j], __float_as_int(S.w[j * K2 + tap]));
-          }
-        }
-        __syncthreads();
-      }
-    } else if (valid && cnt > 0) {
-      // the single chunk is still staged
-      int e = start;
-      for (int j = 0; j < m; ++j) {
-        const int tap = tap_of(cx, cy, S.sx[j], S.sy[j], K, W, H);
-        if (tap >= 0) entries[e++] = make_int2(S.id[j], __float_as_int(S.w[j * K2 + tap]));
-      }
-    }
-    if (valid && cnt > 0) {
-      const size_t cell = (size_t)cy * W + cx;
-      cell_span[cell] = make_int2(start, cnt);
-      float f[9];
+  const int tid = threadIdx.x, bs = blockDim.x;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    unsigned char* p = base + off;
+    off += al16(b);
+    return p;
+  };
+  int* sstart = (int*)take(4 * ((size_t)ne + 1));
+  int* eidx = (int*)take(4 * (size_t)ne);
+  float* wt = (float*)take(4 * (size_t)ne);
+  float2* corr = (float2*)take(8 * (size_t)nq);
+  int* scell = (int*)take(4 * (size_t)ns);
+  float* sinv = (float*)take(4 * (size_t)ns);
+  float* srho = (float*)take(4 * (size_t)ns);
+  float2* suc = (float2*)take(8 * (size_t)ns);
+  float2* su0 = (float2*)take(8 * (size_t)ns);
+
+  const bool mine = tid < nq;
+  const int r = tid;
+  int n = 0, g[K2], seg[K2], sub[K2];
+  float w[K2];
+  float2 vb = make_float2(0.0f, 0.0f);
+  unsigned long long won = 0;
+  if (mine) {
+    n = nodes[r];
+    const int sx = a.st_sx[n], sy = a.st_sy[n];
+    won = a.st_won[n];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) f[i] = fmax_nan(f_in[i * HW + cell], MC_EPSIL);
+    for (int t = 0; t < K2; ++t) w[t] = a.st_w[(size_t)n * K2 + t];
+    vb = make_float2(a.vib[2 * n], a.vib[2 * n + 1]);
+#pragma unroll
+    for (int t = 0; t < K2; ++t) g[t] = wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W);
+  }
+  for (int s = tid; s <= ns; s += bs) sstart[s] = 0;
+  // won taps claim segments (arbitrary numbering); their cells return to idle
+  if (mine) {
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      if ((won >> t) & 1ull) {
+        const int sg = atomicAdd(s_seg, 1);
+        a.cell_seg[g[t]] = sg;
+        scell[sg] = g[t];
+        a.owner[g[t]] = -1;
+      }
+    }
+  }
+  __syncthreads();
+  // every tap's segment and its arrival index there
+  if (mine) {
+#pragma unroll
+    for (int t = 0; t < K2; ++t) seg[t] = a.cell_seg[g[t]];
+#pragma unroll
+    for (int t = 0; t < K2; ++t) sub[t] = atomicAdd(&sstart[seg[t]], 1);
+  }
+  __syncthreads();
+  {
+    int carry = 0;
+    for (int b0 = 0; b0 < ns; b0 += bs) {
+      const int s = b0 + tid;
+      const int v = s < ns ? sstart[s] : 0;
+      int tot;
+      const int inc = block_inclusive_scan(v, wsum, &tot);
+      if (s < ns) sstart[s] = carry + inc - v;
+      carry += tot;
+    }
+    if (tid == 0) sstart[ns] = carry;  // == ne
+  }
+  __syncthreads();
+  if (mine) {
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      eidx[sstart[seg[t]] + sub[t]] = r * K2 + t;
+      wt[r * K2 + t] = w[t];
+    }
+  }
+  __syncthreads();
+  // each segment's entries ascending (= ascending node id); kernel-weight
+  // fold, 1/max(kw, 1), rho and clamped u (fluid.py:123-133,268-290)
+  for (int s0 = 0; s0 < ns; s0 += SPT * bs) {
+    float f[SPT][9];
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+      const int s = s0 + q * bs + tid;
+      if (s < ns) {
+        const size_t cell = (size_t)scell[s];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f[q][i] = a.f_in[i * HW + cell];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+      const int s = s0 + q * bs + tid;
+      if (s >= ns) continue;
+      const int lo = sstart[s], hi = sstart[s + 1];
+      for (int j = lo + 1; j < hi; ++j) {
+        const int v = eidx[j];
+        int k = j - 1;
+        while (k >= lo && eidx[k] > v) {
+          eidx[k + 1] = eidx[k];
+          --k;
+        }
+        eidx[k + 1] = v;
+      }
+      float kw = 0.0f;
+      for (int j = lo; j < hi; ++j) kw = kw + wt[eidx[j]];
+      float fq[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) fq[i] = fmax_nan(f[q][i], MC_EPSIL);
       float rho, ux, uy;
-      lbm_macro(f, c.umax, rho, ux, uy);
-      rho_s[cell] = rho;
-      u0[cell] = ux;
-      u0[HW + cell] = uy;
-      ucorr[cell] = ux;
-      ucorr[HW + cell] = uy;
-      inv[cell] = 1.0f / fmax_nan(kw, 1.0f);
-      atomicOr(&mask[(size_t)cy * mwpr + (cx >> 5)], 1u << (cx & 31));
+      lbm_macro(fq, c.umax, rho, ux, uy);
+      sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
+      srho[s] = rho;
+      su0[s] = make_float2(ux, uy);
+      suc[s] = make_float2(ux, uy);
+    }
+  }
+  __syncthreads();
+  // multi-direct forcing iterations (fluid.py:292-316)
+  for (int it = 0; it < c.iters; ++it) {
+    if (mine) {
+      float2 u[K2];
+#pragma unroll
+      for (int t = 0; t < K2; ++t) u[t] = suc[seg[t]];
+      float ux = 0.0f, uy = 0.0f;
+#pragma unroll
+      for (int t = 0; t < K2; ++t) {
+        ux = ux + u[t].x * w[t];
+        uy = uy + u[t].y * w[t];
+      }
+      const float cx = (vb.x - ux) * c.relax;
+      const float cy = (vb.y - uy) * c.relax;
+      const float mag = sqrtf((cx * cx + cy * cy) + MC_EPSIL);
+      const float cf = fmin_nan(1.0f, c.umax / (mag + MC_EPSIL));
+      corr[r] = make_float2(cx * cf, cy * cf);
+    }
+    __syncthreads();
+    for (int s = tid; s < ns; s += bs) {
+      const int lo = sstart[s], hi = sstart[s + 1];
+      // the first four entries' loads are issued together
+      int e4[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) e4[k] = lo + k < hi ? eidx[lo + k] : 0;
+      float w4[4];
+      float2 c4[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        w4[k] = wt[e4[k]];
+        c4[k] = corr[e4[k] / K2];
+      }
+      float ax = 0.0f, ay = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (lo + k < hi) {
+          ax = ax + c4[k].x * w4[k];
+          ay = ay + c4[k].y * w4[k];
+        }
+      }
+      for (int j = lo + 4; j < hi; ++j) {
+        const int e = eidx[j];
+        const float ww = wt[e];
+        const float2 cc = corr[e / K2];
+        ax = ax + cc.x * ww;
+        ay = ay + cc.y * ww;
+      }
+      const float iv = sinv[s];
+      const float2 u = suc[s];
+      suc[s] = make_float2(u.x + ax * iv, u.y + ay * iv);
     }
     __syncthreads();
   }
+  // post-clamp and force (fluid.py:327-339); F replaces u_corrected
+  for (int s = tid; s < ns; s += bs) {
+    const int cell = scell[s];
+    const float rho = srho[s];
+    const float2 u0 = su0[s];
+    float ucx = suc[s].x, ucy = suc[s].y;
+    const float mag = sqrtf((ucx * ucx + ucy * ucy) + MC_EPSIL);
+    const float cl = fmin_nan(1.0f, c.umax / (mag + MC_EPSIL));
+    ucx = ucx * cl;
+    ucy = ucy * cl;
+    float fx = (ucx - u0.x) * rho;
+    float fy = (ucy - u0.y) * rho;
+    if (c.force_clamp) {
+      const float fm = sqrtf((fx * fx + fy * fy) + MC_EPSIL);
+      const float fc = fmin_nan(1.0f, c.umax / (fm + MC_EPSIL));
+      fx = fx * fc;
+      fy = fy * fc;
+    }
+    suc[s] = make_float2(fx, fy);
+    a.force[cell] = fx;
+    a.force[HW + cell] = fy;
+    const int cy = cell / W, cx = cell - cy * W;
+    atomicOr(&a.mask[(size_t)cy * a.mwpr + (cx >> 5)], 1u << (cx & 31));
+  }
+  __syncthreads();
+  // node drag (fluid.py:364-382) and the final wrap (simulate.py:97)
+  if (mine) {
+    float2 F[K2];
+#pragma unroll
+    for (int t = 0; t < K2; ++t) F[t] = suc[seg[t]];
+    float fx = 0.0f, fy = 0.0f;
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      fx = fx + F[t].x * w[t];
+      fy = fy + F[t].y * w[t];
+    }
+    const float px = a.pos[2 * n], py = a.pos[2 * n + 1];
+    const float vx = a.vel_io[2 * n], vy = a.vel_io[2 * n + 1];
+    const float dvx = (fx / c.node_mass) / c.sdt_x;
+    const float dvy = (fy / c.node_mass) / c.sdt_y;
+    a.vel_io[2 * n] = vx - dvx;
+    a.vel_io[2 * n + 1] = vy - dvy;
+    a.pos_out[2 * n] = fmodf_floor(px, c.dom_wf);
+    a.pos_out[2 * n + 1] = fmodf_floor(py, c.dom_hf);
+  }
 }
 
-// Node phase: gather of a (2, H, W) field over the stencil, left fold over taps.
+// General solve: any cluster size, arrays in LDS when they fit, else in the
+// cluster's global scratch.  `nodes` holds the node ids in ascending order.
 template <int K>
-__device__ __forceinline__ void gather2(const Consts& c, int n, const int* st_sx, const int* st_sy,
-                                        const float* st_w, const float* fld, float& ax, float& ay) {
+__device__ __forceinline__ void cluster_general(const ClusterArgs& a, unsigned char* smem,
+                                                size_t avail, const CLayout& G, const GExtra& X,
+                                                const int* nodes, int nq, int ns, int* wsum,
+                                                int* s_seg) {
   constexpr int K2 = K * K;
+  const Consts& c = a.c;
+  const int ne = nq * K2;
   const int W = c.fl_w, H = c.fl_h;
   const size_t HW = (size_t)H * W;
-  const int sx = st_sx[n], sy = st_sy[n];
-  int row[K], col[K];
+  const int tid = threadIdx.x, bs = blockDim.x;
+  CLayout L;
+  const size_t lbytes = llayout(smem, nq, ne, ns, L);
+  const bool in_lds = lbytes <= avail;
+  if (!in_lds) L = G;
+  for (int i = tid; i < nq; i += bs) L.nodes[i] = nodes[i];
+  for (int s = tid; s <= ns; s += bs) L.sstart[s] = 0;
+  __syncthreads();
+  // won taps claim segments (arbitrary numbering); their cells return to idle
+  for (int r = tid; r < nq; r += bs) {
+    const int n = L.nodes[r];
+    const unsigned long long won = a.st_won[n];
+    const int sx = a.st_sx[n], sy = a.st_sy[n];
 #pragma unroll
-  for (int d = 0; d < K; ++d) {
-    col[d] = wrap1(sx + d, W);
-    row[d] = wrap1(sy + d, H) * W;
+    for (int t = 0; t < K2; ++t) {
+      if ((won >> t) & 1ull) {
+        const int g = wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W);
+        const int seg = atomicAdd(s_seg, 1);
+        a.cell_seg[g] = seg;
+        X.scell[seg] = g;
+        a.owner[g] = -1;
+      }
+    }
   }
-  ax = 0.0f;
-  ay = 0.0f;
+  __syncthreads();
+  // every tap's segment and its arrival index there (LDS or L2 atomics)
+  for (int r = tid; r < nq; r += bs) {
+    const int n = L.nodes[r];
+    const int sx = a.st_sx[n], sy = a.st_sy[n];
+    int seg[K2];
 #pragma unroll
-  for (int t = 0; t < K2; ++t) {
-    const size_t cell = (size_t)row[t % K] + col[t / K];
-    const float w = st_w[(size_t)n * K2 + t];
-    ax = ax + fld[cell] * w;
-    ay = ay + fld[HW + cell] * w;
+    for (int t = 0; t < K2; ++t) seg[t] = a.cell_seg[wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W)];
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      L.tapseg[r * K2 + t] = seg[t];
+      L.eidx[r * K2 + t] = atomicAdd(&L.sstart[seg[t]], 1);  // parked arrival index
+    }
   }
-}
-
-template <int K>
-__global__ void k_ib_interp(Consts c, int N, const int* __restrict__ st_sx,
-                            const int* __restrict__ st_sy, const float* __restrict__ st_w,
-                            const float* __restrict__ vib, const float* __restrict__ ucorr,
-                            float* __restrict__ corr) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  float ux, uy;
-  gather2<K>(c, n, st_sx, st_sy, st_w, ucorr, ux, uy);
-  float cx = (vib[2 * n] - ux) * c.relax;
-  float cy = (vib[2 * n + 1] - uy) * c.relax;
-  const float mag = sqrtf((cx * cx + cy * cy) + MC_EPSIL);
-  const float cf = fmin_nan(1.0f, c.umax / (mag + MC_EPSIL));
-  corr[2 * n] = cx * cf;
-  corr[2 * n + 1] = cy * cf;
-}
-
-// Cell phase: every covered cell folds c_n * w over its ordered entry list
-// (fluid.py:109-120), scales by 1/max(kw, 1) and adds to u_corrected
-// (fluid.py:307,310); the last pass also post-clamps and forms the force.
-__global__ void __launch_bounds__(256) k_ib_spread(
-    Consts c, const int* __restrict__ active, const int* __restrict__ n_active,
-    const int2* __restrict__ cell_span, const int2* __restrict__ entries,
-    const float* __restrict__ corr, const float* __restrict__ rho_s, const float* __restrict__ u0,
-    float* __restrict__ ucorr, const float* __restrict__ inv, const uint32_t* __restrict__ mask,
-    int mwpr, float* __restrict__ force, int do_spread, int do_final, int tiles_x) {
-  const int W = c.fl_w, H = c.fl_h;
-  const size_t HW = (size_t)H * W;
-  const int na = *n_active;
-  for (int b = blockIdx.x; b < na; b += gridDim.x) {
-    const int t = active[b];
-    const int cx = (t % tiles_x) * IB_TILE + (threadIdx.x & (IB_TILE - 1));
-    const int cy = (t / tiles_x) * IB_TILE + (threadIdx.x / IB_TILE);
-    if (cx >= W || cy >= H) continue;
-    if (!((mask[(size_t)cy * mwpr + (cx >> 5)] >> (cx & 31)) & 1u)) continue;
-    const size_t cell = (size_t)cy * W + cx;
-    float ucx = ucorr[cell], ucy = ucorr[HW + cell];
-    if (do_spread) {
-      const int2 sp = cell_span[cell];
+  __syncthreads();
+  {
+    int carry = 0;
+    for (int b0 = 0; b0 < ns; b0 += bs) {
+      const int s = b0 + tid;
+      const int v = s < ns ? (in_lds ? L.sstart[s] : ld_l2(&L.sstart[s])) : 0;
+      int tot;
+      const int inc = block_inclusive_scan(v, wsum, &tot);
+      if (s < ns) L.sstart[s] = carry + inc - v;
+      carry += tot;
+    }
+    if (tid == 0) L.sstart[ns] = carry;  // == ne
+  }
+  __syncthreads();
+  // parked arrival indices become entry positions (all read before any
+  // entry lands), then entries go to their segments in arrival order
+  for (int e = tid; e < ne; e += bs) L.wt[e] = __int_as_float(L.sstart[L.tapseg[e]] + L.eidx[e]);
+  __syncthreads();
+  for (int r = tid; r < nq; r += bs) {
+    const int n = L.nodes[r];
+    float w[K2];
+#pragma unroll
+    for (int t = 0; t < K2; ++t) w[t] = a.st_w[(size_t)n * K2 + t];
+    const float2 vb = make_float2(a.vib[2 * n], a.vib[2 * n + 1]);
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      L.eidx[__float_as_int(L.wt[r * K2 + t])] = r * K2 + t;
+      L.wt[r * K2 + t] = w[t];
+    }
+    L.vib[r] = vb;
+  }
+  __syncthreads();
+  for (int s = tid; s < ns; s += bs) {
+    const size_t cell = (size_t)X.scell[s];
+    float f[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = a.f_in[i * HW + cell];
+    const int lo = L.sstart[s], hi = L.sstart[s + 1];
+    for (int j = lo + 1; j < hi; ++j) {
+      const int v = L.eidx[j];
+      int k = j - 1;
+      while (k >= lo && L.eidx[k] > v) {
+        L.eidx[k + 1] = L.eidx[k];
+        --k;
+      }
+      L.eidx[k + 1] = v;
+    }
+    float kw = 0.0f;
+    for (int j = lo; j < hi; ++j) kw = kw + L.wt[L.eidx[j]];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = fmax_nan(f[i], MC_EPSIL);
+    float rho, ux, uy;
+    lbm_macro(f, c.umax, rho, ux, uy);
+    L.sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
+    X.srho[s] = rho;
+    X.su0[s] = make_float2(ux, uy);
+    L.suc[s] = make_float2(ux, uy);
+  }
+  __syncthreads();
+  for (int it = 0; it < c.iters; ++it) {
+    for (int r = tid; r < nq; r += bs) {
+      float ux = 0.0f, uy = 0.0f;
+#pragma unroll
+      for (int t = 0; t < K2; ++t) {
+        const float w = L.wt[r * K2 + t];
+        const float2 u = L.suc[L.tapseg[r * K2 + t]];
+        ux = ux + u.x * w;
+        uy = uy + u.y * w;
+      }
+      const float2 vb = L.vib[r];
+      const float cx = (vb.x - ux) * c.relax;
+      const float cy = (vb.y - uy) * c.relax;
+      const float mag = sqrtf((cx * cx + cy * cy) + MC_EPSIL);
+      const float cf = fmin_nan(1.0f, c.umax / (mag + MC_EPSIL));
+      L.corr[r] = make_float2(cx * cf, cy * cf);
+    }
+    __syncthreads();
+    for (int s = tid; s < ns; s += bs) {
       float ax = 0.0f, ay = 0.0f;
-      for (int e = sp.x; e < sp.x + sp.y; ++e) {
-        const int2 en = entries[e];
-        const float w = __int_as_float(en.y);
-        ax = ax + corr[2 * en.x] * w;
-        ay = ay + corr[2 * en.x + 1] * w;
+      const int hi = L.sstart[s + 1];
+      for (int j = L.sstart[s]; j < hi; ++j) {
+        const int e = L.eidx[j];
+        const float w = L.wt[e];
+        const float2 cc = L.corr[e / K2];
+        ax = ax + cc.x * w;
+        ay = ay + cc.y * w;
       }
-      const float iv = inv[cell];
-      ucx = ucx + ax * iv;
-      ucy = ucy + ay * iv;
-      ucorr[cell] = ucx;
-      ucorr[HW + cell] = ucy;
+      const float iv = L.sinv[s];
+      const float2 u = L.suc[s];
+      L.suc[s] = make_float2(u.x + ax * iv, u.y + ay * iv);
     }
-    if (do_final) {
-      const float mag = sqrtf((ucx * ucx + ucy * ucy) + MC_EPSIL);
-      const float cl = fmin_nan(1.0f, c.umax / (mag + MC_EPSIL));
-      ucx = ucx * cl;
-      ucy = ucy * cl;
-      const float rho = rho_s[cell];
-      float fx = (ucx - u0[cell]) * rho;
-      float fy = (ucy - u0[HW + cell]) * rho;
-      if (c.force_clamp) {
-        const float fm = sqrtf((fx * fx + fy * fy) + MC_EPSIL);
-        const float fc = fmin_nan(1.0f, c.umax / (fm + MC_EPSIL));
-        fx = fx * fc;
-        fy = fy * fc;
-      }
-      force[cell] = fx;
-      force[HW + cell] = fy;
+    __syncthreads();
+  }
+  for (int s = tid; s < ns; s += bs) {
+    const int cell = X.scell[s];
+    const float rho = X.srho[s];
+    const float2 u0 = X.su0[s];
+    float ucx = L.suc[s].x, ucy = L.suc[s].y;
+    const float mag = sqrtf((ucx * ucx + ucy * ucy) + MC_EPSIL);
+    const float cl = fmin_nan(1.0f, c.umax / (mag + MC_EPSIL));
+    ucx = ucx * cl;
+    ucy = ucy * cl;
+    float fx = (ucx - u0.x) * rho;
+    float fy = (ucy - u0.y) * rho;
+    if (c.force_clamp) {
+      const float fm = sqrtf((fx * fx + fy * fy) + MC_EPSIL);
+      const float fc = fmin_nan(1.0f, c.umax / (fm + MC_EPSIL));
+      fx = fx * fc;
+      fy = fy * fc;
     }
+    L.suc[s] = make_float2(fx, fy);
+    a.force[cell] = fx;
+    a.force[HW + cell] = fy;
+    const int cy = cell / W, cx = cell - cy * W;
+    atomicOr(&a.mask[(size_t)cy * a.mwpr + (cx >> 5)], 1u << (cx & 31));
+  }
+  __syncthreads();
+  for (int r = tid; r < nq; r += bs) {
+    const int n = L.nodes[r];
+    float fx = 0.0f, fy = 0.0f;
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      const float w = L.wt[r * K2 + t];
+      const float2 F = L.suc[L.tapseg[r * K2 + t]];
+      fx = fx + F.x * w;
+      fy = fy + F.y * w;
+    }
+    const float dvx = (fx / c.node_mass) / c.sdt_x;
+    const float dvy = (fy / c.node_mass) / c.sdt_y;
+    a.vel_io[2 * n] = a.vel_io[2 * n] - dvx;
+    a.vel_io[2 * n + 1] = a.vel_io[2 * n + 1] - dvy;
+    a.pos_out[2 * n] = fmodf_floor(a.pos[2 * n], c.dom_wf);
+    a.pos_out[2 * n + 1] = fmodf_floor(a.pos[2 * n + 1], c.dom_hf);
   }
 }
 
+// Every thread of a cluster is in this workgroup: the workgroup barrier (with
+// its workgroup-scope fence) orders both the LDS and the global-memory phases.
 template <int K>
-__global__ void k_ib_drag(Consts c, int N, const int* __restrict__ st_sx,
-                          const int* __restrict__ st_sy, const float* __restrict__ st_w,
-                          const float* __restrict__ force, const float* __restrict__ pos,
-                          float* __restrict__ vel_io, float* __restrict__ pos_out,
-                          long long* __restrict__ tick) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  // advance the actuation step counter (the rod kernel of this step, which
-  // read it, has completed before this kernel starts)
-  if (tick && n == 0) *tick = *tick + 1;
-  if (n >= N) return;
-  float fx, fy;
-  gather2<K>(c, n, st_sx, st_sy, st_w, force, fx, fy);
-  const float dvx = (fx / c.node_mass) / c.sdt_x;
-  const float dvy = (fy / c.node_mass) / c.sdt_y;
-  vel_io[2 * n] = vel_io[2 * n] - dvx;
-  vel_io[2 * n + 1] = vel_io[2 * n + 1] - dvy;
-  const float px = pos[2 * n], py = pos[2 * n + 1];
-  pos_out[2 * n] = fmodf_floor(px, c.dom_wf);
-  pos_out[2 * n + 1] = fmodf_floor(py, c.dom_hf);
+__global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int wsum[16];
+  __shared__ int s_ns, s_seg;
+  constexpr int K2 = K * K;
+  const int root = blockIdx.x;
+  if (a.label[root] != root) return;
+  const int nq = a.cl_cnt[root];
+  const int ne = nq * K2;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  CLayout G;
+  GExtra X;
+  glayout(a.gscratch + (size_t)a.cl_node_off[root] * a.bpn, nq, ne, G, X);
+  if (tid == 0) s_seg = 0;
+
+  // The cluster's node ids, ascending (the rank order of every fold), at the
+  // top of the LDS when they fit there (twice: member order and ranked),
+  // else in global scratch.
+  const size_t in_bytes = al16(4 * (size_t)nq);
+  const bool stage_lds = 8 * in_bytes <= a.lds_bytes;
+  int* nodes_in = stage_lds ? reinterpret_cast<int*>(smem + a.lds_bytes - in_bytes) : G.tmp;
+  int* ranked = stage_lds ? reinterpret_cast<int*>(smem + a.lds_bytes - 2 * in_bytes) : G.nodes;
+  const size_t avail = stage_lds ? a.lds_bytes - 2 * in_bytes : a.lds_bytes;
+  {
+    // members ordered by their smallest node id, so the gathered list is
+    // already ascending unless components interleave
+    const int m0 = a.cl_mem_off[root], nm = a.cl_mem_cnt[root];
+    const bool m_lds = 12 * ((size_t)nm + 1) <= avail;
+    int* mfirst = m_lds ? reinterpret_cast<int*>(smem) : G.eidx;
+    int* msort = m_lds ? mfirst + (nm + 1) : G.tapseg;
+    int* moff = m_lds ? msort + (nm + 1) : G.sstart;
+    for (int i = tid; i < nm; i += bs) {
+      const int cc = a.cl_members[m0 + i];
+      mfirst[i] = a.comp_nodes[a.comp_node_off[cc]];  // the smallest id of the component
+    }
+    __syncthreads();
+    for (int i = tid; i < nm; i += bs) {
+      const int v = mfirst[i];
+      int rank = 0;
+      for (int j = 0; j < nm; ++j) rank += mfirst[j] < v ? 1 : 0;
+      msort[rank] = a.cl_members[m0 + i];
+    }
+    __syncthreads();
+    int carry = 0;
+    for (int b0 = 0; b0 < nm; b0 += bs) {
+      const int i = b0 + tid;
+      int cnt = 0;
+      if (i < nm) {
+        const int cc = msort[i];
+        cnt = a.comp_node_off[cc + 1] - a.comp_node_off[cc];
+      }
+      int tot;
+      const int inc = block_inclusive_scan(cnt, wsum, &tot);
+      if (i < nm) moff[i] = carry + inc - cnt;
+      carry += tot;
+    }
+    __syncthreads();
+    for (int i = tid; i < nq; i += bs) {
+      int lo = 0, hi = nm - 1;  // last member whose offset is <= i
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (moff[mid] <= i) lo = mid; else hi = mid - 1;
+      }
+      const int cc = msort[lo];
+      nodes_in[i] = a.comp_nodes[a.comp_node_off[cc] + (i - moff[lo])];
+    }
+    __syncthreads();
+  }
+  int unsorted = 0;
+  int won = 0;
+  for (int i = tid; i < nq; i += bs) {
+    if (i + 1 < nq && nodes_in[i] > nodes_in[i + 1]) unsorted = 1;
+    won += __popcll(a.st_won[nodes_in[i]]);
+  }
+  int tot;
+  block_inclusive_scan(won, wsum, &tot);
+  if (tid == 0) s_ns = tot;  // segments = taps that won their cell
+  const int* nodes = nodes_in;
+  if (__syncthreads_or(unsorted)) {
+    if (stage_lds) {
+      // ranks by counting (ids are unique), four ids per LDS read
+      for (int i = tid; i < nq; i += bs) {
+        const int v = nodes_in[i];
+        int rank = 0;
+        int j = 0;
+        for (; j + 4 <= nq; j += 4) {
+          const int4 q = *reinterpret_cast<const int4*>(nodes_in + j);
+          rank += (q.x < v) + (q.y < v) + (q.z < v) + (q.w < v);
+        }
+        for (; j < nq; ++j) rank += nodes_in[j] < v ? 1 : 0;
+        ranked[rank] = v;
+      }
+    } else {
+      for (int i = tid; i < nq; i += bs) ranked[i] = nodes_in[i];
+      __syncthreads();
+      sort_unique<int>(ranked, nq, reinterpret_cast<int*>(smem), 0, G.eidx, 0x7fffffff);
+    }
+    nodes = ranked;
+  }
+  __syncthreads();
+  const int ns = s_ns;
+  if (nq <= bs && stage_lds && small_bytes(nq, ne, ns) <= avail)
+    cluster_small<K>(a, smem, nodes, nq, ns, wsum, &s_seg);
+  else
+    cluster_general<K>(a, smem, avail, G, X, nodes, nq, ns, wsum, &s_seg);
+}
+
+__global__ void k_ib_reset(int HW, int C, int* __restrict__ owner, int* __restrict__ parent,
+                           int* __restrict__ nlinks) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *nlinks = 0;
+  if (i < HW) owner[i] = -1;
+  if (i < C) parent[i] = i;
 }
 
 __global__ void __launch_bounds__(1024) k_bin_scan(int T, int* __restrict__ cnt,
@@ -382,75 +859,109 @@ hipError_t launch_bin_scan(int T, const int* cnt, int* off, int* fill, int* acti
   return hipGetLastError();
 }
 
-#define MC_KSWITCH(LAUNCH)                          \
-  switch (c.K) {                                    \
-    case 1: LAUNCH(1); break;                       \
-    case 2: LAUNCH(2); break;                       \
-    case 3: LAUNCH(3); break;                       \
-    case 4: LAUNCH(4); break;                       \
-    case 5: LAUNCH(5); break;                       \
-    default: LAUNCH(6); break;                      \
-  }
-
-hipError_t launch_ib_bin(const Consts& c, IBWork& w, const float* pos, float* vel_io,
-                         hipStream_t s) {
-  const int N = w.N;
-  if (N == 0) return hipSuccess;
-  const int nb = (N + 255) / 256;
-#define L_STENCIL(KK)                                                                            \
-  hipLaunchKernelGGL(k_ib_stencil<KK>, dim3(nb), dim3(256), 0, s, c, N, pos, vel_io, w.st_sx,    \
-                     w.st_sy, w.st_w, w.vib, w.tile_cnt, w.tiles_x)
-  MC_KSWITCH(L_STENCIL);
-#undef L_STENCIL
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, w.T, w.tile_cnt, w.tile_off,
-                     w.tile_fill, w.tile_active, w.n_active);
-  hipLaunchKernelGGL(k_ib_fill, dim3(nb), dim3(256), 0, s, c, N, w.st_sx, w.st_sy, w.tile_off,
-                     w.tile_fill, w.tile_list, w.tiles_x);
-  return hipGetLastError();
+size_t ib_scratch_bytes_per_node(int kmax) {
+  // glayout(nq, nq * K2): nq * (28 + 40 K2) + 4 + 13 arrays x 15 B of padding, nq >= 1
+  const size_t K2 = (size_t)kmax * kmax;
+  return 28 + 40 * K2 + 4 + 16 * 16;
 }
 
-hipError_t launch_ib_solve(const Consts& c, IBWork& w, const float* f_in, hipStream_t s) {
-  const int N = w.N;
-  // no nodes: no support; the LBM pass takes the clamp-only force everywhere
-  if (N == 0) return hipSuccess;
-  const int nb = (N + 255) / 256;
-  const int tg = w.T;  // one workgroup per tile: inactive ones exit at once
-  hipLaunchKernelGGL(k_ib_tile_init, dim3(tg), dim3(256), 0, s, c, w.tile_off, w.tile_active,
-                     w.n_active, w.tile_list, w.sort_scratch, w.st_sx, w.st_sy, w.st_w, f_in, w.rho,
-                     w.u0, w.ucorr, w.inv, w.mask, w.mask_words_per_row, w.tiles_x, w.cell_span,
-                     w.entries);
-  if (c.iters == 0) {
-    hipLaunchKernelGGL(k_ib_spread, dim3(tg), dim3(256), 0, s, c, w.tile_active, w.n_active,
-                       w.cell_span, w.entries, w.corr, w.rho, w.u0, w.ucorr, w.inv, w.mask,
-                       w.mask_words_per_row, w.force, 0, 1, w.tiles_x);
-  }
-#define L_INTERP(KK)                                                                             \
-  hipLaunchKernelGGL(k_ib_interp<KK>, dim3(nb), dim3(256), 0, s, c, N, w.st_sx, w.st_sy, w.st_w, \
-                     w.vib, w.ucorr, w.corr)
-  for (int it = 0; it < c.iters; ++it) {
-    MC_KSWITCH(L_INTERP);
-    hipLaunchKernelGGL(k_ib_spread, dim3(tg), dim3(256), 0, s, c, w.tile_active, w.n_active,
-                       w.cell_span, w.entries, w.corr, w.rho, w.u0, w.ucorr, w.inv, w.mask,
-                       w.mask_words_per_row, w.force, 1, it == c.iters - 1 ? 1 : 0, w.tiles_x);
-  }
-#undef L_INTERP
-  return hipGetLastError();
+size_t ib_lds_bytes(int lds_max) {
+  // dynamic LDS of the cluster kernels: the device maximum less a reserve for
+  // their static LDS (scan partials, the workgroup reduction's 256 B), a
+  // multiple of 16
+  return ((size_t)lds_max - 1024) & ~(size_t)15;
 }
 
-hipError_t launch_ib_drag(const Consts& c, IBWork& w, const float* pos, float* vel_io,
-                          float* pos_out, long long* tick, hipStream_t s) {
-  const int N = w.N;
+hipError_t ib_prepare(int lds_max) {
+  const int lds = (int)ib_lds_bytes(lds_max);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ib_cluster_build),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+#define SET_LDS(KK)                                                                     \
+  if (e == hipSuccess)                                                                  \
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ib_cluster<KK>),           \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  SET_LDS(1) SET_LDS(2) SET_LDS(3) SET_LDS(4) SET_LDS(5) SET_LDS(6)
+#undef SET_LDS
+  return e;
+}
+
+#define MC_KSWITCH(LAUNCH)     \
+  switch (c.K) {               \
+    case 1: LAUNCH(1); break;  \
+    case 2: LAUNCH(2); break;  \
+    case 3: LAUNCH(3); break;  \
+    case 4: LAUNCH(4); break;  \
+    case 5: LAUNCH(5); break;  \
+    default: LAUNCH(6); break; \
+  }
+
+hipError_t launch_ib(const Consts& c, IBWork& w, const float* f_in, const float* pos,
+                     float* vel_io, float* pos_out, long long* tick, hipStream_t s) {
+  const int N = w.N, C = w.C;
   if (N == 0) {
+    // no nodes: no support; the LBM pass takes the clamp-only force everywhere
     if (tick) return launch_tick(tick, s);
     return hipSuccess;
   }
-  const int nb = (N + 255) / 256;
-#define L_DRAG(KK)                                                                               \
-  hipLaunchKernelGGL(k_ib_drag<KK>, dim3(nb), dim3(256), 0, s, c, N, w.st_sx, w.st_sy, w.st_w,   \
-                     w.force, pos, vel_io, pos_out, tick)
-  MC_KSWITCH(L_DRAG);
-#undef L_DRAG
-  return hipGetLastError();
+  const int HW = c.fl_h * c.fl_w;
+  if (w.dirty) {
+    // an earlier call stopped between its launches: restore the idle state
+    const int n = std::max(HW, C);
+    hipLaunchKernelGGL(k_ib_reset, dim3((n + 255) / 256), dim3(256), 0, s, HW, C, w.owner,
+                       w.uf_parent, w.nlinks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  w.dirty = true;
+  const int nb = (N + 63) / 64;
+#define L_STENCIL(KK)                                                                            \
+  hipLaunchKernelGGL(k_ib_stencil<KK>, dim3(nb), dim3(64), 0, s, c, N, pos, vel_io,              \
+                     w.comp_of_node, w.st_sx, w.st_sy, w.st_w, w.st_won, w.vib, w.owner, w.links, \
+                     w.nlinks)
+  MC_KSWITCH(L_STENCIL);
+#undef L_STENCIL
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ib_cluster_build, dim3(1), dim3(1024), w.lds_bytes, s, C,
+                     (int)(w.lds_bytes / sizeof(int)), w.comp_node_off, w.links, w.nlinks,
+                     w.uf_parent, w.label, w.cl_cnt, w.cl_node_off, w.cl_mem_cnt, w.cl_mem_off,
+                     w.cl_members, tick);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  ClusterArgs a;
+  a.c = c;
+  a.comp_node_off = w.comp_node_off;
+  a.comp_nodes = w.comp_nodes;
+  a.label = w.label;
+  a.cl_cnt = w.cl_cnt;
+  a.cl_node_off = w.cl_node_off;
+  a.cl_mem_cnt = w.cl_mem_cnt;
+  a.cl_mem_off = w.cl_mem_off;
+  a.cl_members = w.cl_members;
+  a.st_sx = w.st_sx;
+  a.st_sy = w.st_sy;
+  a.st_won = w.st_won;
+  a.st_w = w.st_w;
+  a.vib = w.vib;
+  a.f_in = f_in;
+  a.pos = pos;
+  a.vel_io = vel_io;
+  a.pos_out = pos_out;
+  a.force = w.force;
+  a.cell_seg = w.cell_seg;
+  a.owner = w.owner;
+  a.mask = w.mask;
+  a.mwpr = w.mask_words_per_row;
+  a.gscratch = w.cl_scratch;
+  a.bpn = w.cl_bpn;
+  a.lds_bytes = w.lds_bytes;
+#define L_CLUSTER(KK) \
+  hipLaunchKernelGGL(k_ib_cluster<KK>, dim3(C), dim3(cluster_threads<KK>()), w.lds_bytes, s, a)
+  MC_KSWITCH(L_CLUSTER);
+#undef L_CLUSTER
+  e = hipGetLastError();
+  if (e == hipSuccess) w.dirty = false;
+  return e;
 }
 
 }  // namespace mc

@@ -7,11 +7,7 @@
 
 namespace mc {
 
-// Tile edge (cells) of the immersed-boundary cell phase: 16 x 16 cells, one
-// thread per cell, one workgroup per tile.
-constexpr int IB_TILE = 16;
 constexpr int IB_KMAX = 6;                 // largest supported ibm_kernel_size
-constexpr int IB_TAPS_MAX = IB_KMAX * IB_KMAX;
 constexpr int SORT_LDS_CAP = 2048;         // bitonic sort capacity in LDS (ids)
 
 // Float constants derived from ConstraintSolverConfig exactly as the reference
@@ -96,30 +92,36 @@ hipError_t launch_steric_nodes(const Consts& c, int N, const float* steric, cons
 hipError_t launch_tick(long long* t, hipStream_t s);
 size_t rod_scratch_floats(int nn, int ne, int nb);
 size_t rod_lds_bytes(int nn, int ne, int nb, int nni, int nei);
-// immersed boundary
+// immersed boundary (solved per interaction cluster)
 struct IBWork {
-  int N, T, tiles_x, tiles_y;
+  int N, C;                    // nodes, filament components
+  const int *comp_of_node, *comp_node_off, *comp_nodes;  // component layout (rod layout)
   int *st_sx, *st_sy;          // per node wrapped stencil start cell
   float *st_w;                 // per node K*K weights
+  unsigned long long* st_won;  // per node: bit t set when tap t claimed its cell
   float *vib;                  // per node (2) IB target velocity
-  float *corr;                 // per node (2) correction of the current iteration
-  int *tile_cnt, *tile_off, *tile_fill, *tile_active, *n_active, *tile_list;
-  int *sort_scratch;
-  float *rho, *u0, *ucorr, *inv, *force;   // dense (H*W) / (2*H*W) on the fluid grid
-  uint32_t* mask;              // per fluid row ceil(W/32) words
+  int *owner, *cell_seg;       // per lattice cell: claiming component (-1 idle), segment
+  int2* links;                 // component pairs whose stencils share a cell
+  int* nlinks;                 // number of links (0 between calls)
+  int *uf_parent, *label;      // per component: union-find parent (global fallback), cluster root
+  int *cl_cnt, *cl_node_off, *cl_mem_cnt, *cl_mem_off, *cl_members;  // per cluster root
+  unsigned char* cl_scratch;   // cl_bpn bytes per node, cluster-major
+  size_t cl_bpn;
+  size_t lds_bytes;            // dynamic LDS of the cluster solve
+  bool dirty;                  // a call stopped between launches: idle state to restore
+  float *force;                // (2, H, W) IB force, valid on the support U
+  uint32_t* mask;              // per lattice row ceil(W/32) words: U bits
   int mask_words_per_row;
-  int2* cell_span;             // per covered cell: (first entry, entry count)
-  int2* entries;               // ordered (node, weight bits) lists of covered cells
 };
-// Immersed boundary in two parts: the f-independent binning (stencil, tile
-// scan, fill), then the support / iterations / force (reads f_in).  The drag
-// and final wrap run separately so the LBM pass can overlap them; `tick`
-// (may be NULL) is advanced once by the drag launch.
-hipError_t launch_ib_bin(const Consts& c, IBWork& w, const float* pos, float* vel_io,
-                         hipStream_t s);
-hipError_t launch_ib_solve(const Consts& c, IBWork& w, const float* f_in, hipStream_t s);
-hipError_t launch_ib_drag(const Consts& c, IBWork& w, const float* pos, float* vel_io,
-                          float* pos_out, long long* tick, hipStream_t s);
+size_t ib_scratch_bytes_per_node(int kmax);
+size_t ib_lds_bytes(int lds_max);
+hipError_t ib_prepare(int lds_max);
+// Whole IB of a step: stencil + cluster links, cluster build, cluster solve
+// (support, iterations, force into w.force / w.mask, node drag into vel_io,
+// final wrap of pos into pos_out).  Only the solve reads f_in.  `tick` (may be
+// NULL) is advanced once.
+hipError_t launch_ib(const Consts& c, IBWork& w, const float* f_in, const float* pos,
+                     float* vel_io, float* pos_out, long long* tick, hipStream_t s);
 // LBM
 hipError_t launch_lbm(const Consts& c, const float* f_in, float* f_out, float* ustar,
                       uint32_t* mask, int mask_words_per_row, const float* force, hipStream_t s);
