@@ -168,6 +168,31 @@ mc_status mc_ib_lbm_step(mc_plan* plan, const mc_params* params, const float* po
 mc_status mc_rod_solve(mc_plan* plan, const mc_params* params, float* position, float* theta,
                        const float* y, const mc_edge_params* edges, int32_t cycles, void* stream);
 
+/* Swimmer fitness (the caller of the step).  Centre of mass of node groups
+ * [group_offsets[g], group_offsets[g+1]) (device int32, n_groups + 1):
+ * periodic != 0 -> reference-node minimum-image mean, wrapped
+ * (experiments/swim_sweep/swimmers/swim_base.py:167-177 _periodic_com);
+ * periodic == 0 -> plain mean (examples/01_es_locomotion.ipynb
+ * rollout_fitness, position[s].mean(axis=0)).  com_out is (n_groups, 2). */
+mc_status mc_center_of_mass(const float* position, int32_t n_groups, const int32_t* group_offsets,
+                            int32_t grid_h, int32_t grid_w, int32_t periodic, float* com_out,
+                            void* stream);
+
+/* Per-step displacement tracking on a plan (swim_base.py:179-209
+ * _init_displacement / _update_displacement): once enabled, every step that
+ * mc_step / mc_rollout enqueues ends with, per group, com = periodic centre
+ * of the new positions; count += 1; if count > start_step then
+ * cumulative += displacement(com, prev); prev = com.  group_offsets_host
+ * (n_groups + 1) are host int32 node offsets; n_groups = 0 turns tracking
+ * off.  No host synchronisation per step. */
+mc_status mc_track_enable(mc_plan* plan, int32_t n_groups, const int32_t* group_offsets_host,
+                          int32_t start_step);
+/* _init_displacement: prev = centre of `position`, cumulative = 0, count = 0. */
+mc_status mc_track_reset(mc_plan* plan, const float* position, void* stream);
+/* Copies cumulative displacement and the last centre, (n_groups, 2) each,
+ * into device buffers (either may be NULL), stream-ordered. */
+mc_status mc_track_read(mc_plan* plan, float* cumulative_out, float* com_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,11 @@ SIGNATURES = {
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "mc_rod_solve": (c_int32, [c_void_p, POINTER(McParams), c_void_p, c_void_p, c_void_p,
                                POINTER(McEdgeParams), c_int32, c_void_p]),
+    "mc_center_of_mass": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32,
+                                    c_void_p, c_void_p]),
+    "mc_track_enable": (c_int32, [c_void_p, c_int32, c_void_p, c_int32]),
+    "mc_track_reset": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "mc_track_read": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -88,6 +88,7 @@ struct mc_plan {
   double kf_sigma = -1.0;       // sigma the table holds (-1: none)
   hipStream_t cap_stream = nullptr;
   hipStream_t side_stream = nullptr;
+  TrackWork track{};            // displacement tracking (off: n_groups == 0)
 
   template <class T>
   mc_status alloc(T** p, size_t count, const char* what) {
@@ -418,6 +419,8 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
       HIPCHK(hipMemcpyAsync(out->fluid_velocity, in->fluid_velocity, 2 * hwf * sizeof(float),
                             hipMemcpyDeviceToDevice, s));
   }
+  if (p->track.n_groups > 0 && p->N > 0)
+    HIPCHK(launch_track(p->track, out->position, p->H, p->W, 1, s));
   (void)q;
   return MC_OK;
 }
@@ -810,6 +813,67 @@ mc_status mc_rod_solve(mc_plan* p, const mc_params* q, float* position, float* t
   HIPCHK(launch_rod(c, p->L, p->N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 1, position,
                     nullptr, y, theta, ed->rest_lengths, ed->bending_rest_angles, ed->bending_stiffness,
                     nullptr, nullptr, nullptr, position, nullptr, nullptr, theta, s));
+  return MC_OK;
+}
+
+mc_status mc_center_of_mass(const float* position, int32_t n_groups, const int32_t* group_offsets,
+                            int32_t grid_h, int32_t grid_w, int32_t periodic, float* com_out,
+                            void* stream) {
+  g_err.clear();
+  if (n_groups < 0) return fail(MC_ERR_BAD_SHAPE, "n_groups < 0");
+  if (n_groups == 0) return MC_OK;
+  if (!position || !group_offsets || !com_out) return fail(MC_ERR_INVALID, "NULL argument");
+  if (grid_h < 1 || grid_w < 1) return fail(MC_ERR_BAD_SHAPE, "grid %dx%d", grid_h, grid_w);
+  HIPCHK(launch_com(n_groups, group_offsets, position, grid_h, grid_w, periodic ? 1 : 0, com_out,
+                    (hipStream_t)stream));
+  return MC_OK;
+}
+
+mc_status mc_track_enable(mc_plan* p, int32_t n_groups, const int32_t* group_offsets_host,
+                          int32_t start_step) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  if (n_groups < 0) return fail(MC_ERR_BAD_SHAPE, "n_groups < 0");
+  if (n_groups == 0) {
+    p->track.n_groups = 0;
+    return MC_OK;
+  }
+  if (!group_offsets_host) return fail(MC_ERR_INVALID, "group_offsets is NULL");
+  if (group_offsets_host[0] < 0 || group_offsets_host[n_groups] > p->N)
+    return fail(MC_ERR_BAD_SHAPE, "group offsets outside [0, %d]", p->N);
+  for (int g = 0; g < n_groups; ++g)
+    if (group_offsets_host[g + 1] <= group_offsets_host[g])
+      return fail(MC_ERR_BAD_SHAPE, "group %d is empty or offsets descend", g);
+  TrackWork t{};
+  t.n_groups = n_groups;
+  t.start_step = start_step;
+  std::vector<int> off(group_offsets_host, group_offsets_host + n_groups + 1);
+  MCCHK(upload(p, &t.off, off, "track offsets"));
+  MCCHK(p->alloc(&t.prev, 2 * (size_t)n_groups, "track prev"));
+  MCCHK(p->alloc(&t.cum, 2 * (size_t)n_groups, "track cumulative"));
+  MCCHK(p->alloc(&t.count, (size_t)n_groups, "track count"));
+  p->track = t;
+  return MC_OK;
+}
+
+mc_status mc_track_reset(mc_plan* p, const float* position, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  if (p->track.n_groups == 0) return fail(MC_ERR_INVALID, "tracking is not enabled");
+  if (!position) return fail(MC_ERR_INVALID, "position is NULL");
+  HIPCHK(launch_track(p->track, position, p->H, p->W, 0, (hipStream_t)stream));
+  return MC_OK;
+}
+
+mc_status mc_track_read(mc_plan* p, float* cumulative_out, float* com_out, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  if (p->track.n_groups == 0) return fail(MC_ERR_INVALID, "tracking is not enabled");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t b = 2 * (size_t)p->track.n_groups * sizeof(float);
+  if (cumulative_out)
+    HIPCHK(hipMemcpyAsync(cumulative_out, p->track.cum, b, hipMemcpyDeviceToDevice, s));
+  if (com_out) HIPCHK(hipMemcpyAsync(com_out, p->track.prev, b, hipMemcpyDeviceToDevice, s));
   return MC_OK;
 }
 

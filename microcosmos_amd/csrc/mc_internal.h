@@ -122,6 +122,19 @@ hipError_t ib_prepare(int lds_max);
 // NULL) is advanced once.
 hipError_t launch_ib(const Consts& c, IBWork& w, const float* f_in, const float* pos,
                      float* vel_io, float* pos_out, long long* tick, hipStream_t s);
+// swimmer centre of mass / displacement tracking
+struct TrackWork {
+  int n_groups;       // 0: tracking off
+  int start_step;     // steps up to this one do not accumulate
+  int* off;           // (n_groups + 1) node-range offsets
+  float *prev, *cum;  // (n_groups, 2) previous centre, cumulative displacement
+  long long* count;   // (n_groups) steps since the reset
+};
+hipError_t launch_com(int n_groups, const int* off, const float* pos, int H, int W, int periodic,
+                      float* com, hipStream_t s);
+// mode 0: reset (prev = centre, cumulative = 0); mode 1: one step's update
+hipError_t launch_track(const TrackWork& t, const float* pos, int H, int W, int mode,
+                        hipStream_t s);
 // LBM
 hipError_t launch_lbm(const Consts& c, const float* f_in, float* f_out, float* ustar,
                       uint32_t* mask, int mask_words_per_row, const float* force, hipStream_t s);

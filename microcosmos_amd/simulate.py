@@ -14,7 +14,7 @@ import torch
 
 from . import _lib
 from ._lib import McActuation, McState, check
-from ._runtime import as_f32, edge_params, params_from_config, plan_for, ptr, stream_handle
+from ._runtime import Plan, as_f32, edge_params, params_from_config, plan_for, ptr, stream_handle
 from .structs import Edges, Fields, Nodes
 
 
@@ -155,6 +155,21 @@ class Rollout:
                                   stream_handle(self.dev)))
         self.t += int(num_steps)
         return lbm.value, stp.value
+
+    def track(self, groups=None, start_step: int = 0):
+        """Displacement tracking of swim_base.py:179-209 over this rollout's
+        steps, on the device and inside the step graphs.  groups: node
+        slices / offsets (default: all nodes).  The rollout switches to a
+        private plan so that other callers of the same topology are not
+        tracked.  Returns the DisplacementTracker (reset to the current
+        positions)."""
+        from .fitness import DisplacementTracker
+        self.plan = Plan(self.grid_shape, tuple(self.f_grid.shape[1:]), self.position.shape[0],
+                         self.plan.pairs, self.plan.bending_pairs)
+        self.tracker = DisplacementTracker(self.plan, self.grid_shape, groups,
+                                           start_step=start_step)
+        self.tracker.init(self.position)
+        return self.tracker
 
     def nodes(self) -> Nodes:
         return Nodes(position=self.position, velocity=self.velocity, color_bend=self.color_bend,

@@ -48,6 +48,9 @@ Parity status per function (how each is pinned; see tests/test_oracle_*.py):
                             point, exact streaming permutation, Guo momentum input,
                             Taylor-Green viscous decay exp(-2 nu k^2 t)
   step                   -- reference worm behaviour (tests/test_worm.py:15-152)
+  group_com / swim tracker -- float64 mean, translation by a lattice period,
+                            a blob straddling the boundary, brute-force
+                            per-step cumulative sum
 """
 from __future__ import annotations
 
@@ -844,3 +847,66 @@ def initialize_f_from_velocity(velocity_field, density=1.0):
         eu = _edot(i, u)
         out[i] = dtype.type(density * LBM_WEIGHTS[i]) * (1.0 + 3.0 * eu + 4.5 * (eu * eu) - 1.5 * usq)
     return out
+
+
+# --------------------------------------------------------------------------
+# Swimmer fitness (the caller of the step)
+# --------------------------------------------------------------------------
+COM_LANES = 256  # the device reduction width
+
+
+def _lane_tree_mean(d):
+    """Mean of d (n, 2) in the device's order: lane i folds rows i, i + 256,
+    ... in order from 0, then the 256 lane sums are halved pairwise
+    (p[i] += p[i + s], s = 128 .. 1), then divided by n."""
+    n = d.shape[0]
+    dt = d.dtype
+    part = np.zeros((COM_LANES, 2), dt)
+    for k in range(0, n, COM_LANES):
+        blk = d[k:k + COM_LANES]
+        part[:blk.shape[0]] = part[:blk.shape[0]] + blk
+    s = COM_LANES // 2
+    while s > 0:
+        part[:s] = part[:s] + part[s:2 * s]
+        s //= 2
+    return part[0] / dt.type(n)
+
+
+def group_com(position, grid_shape, offsets=None, periodic=True):
+    """Centre of each node group [off[g], off[g+1]).
+
+    periodic=True: swim_base.py:167-177 ``_periodic_com`` -- displacements of
+    every node from the group's first node (utils.py:18-23), their mean, added
+    to the first node and wrapped (utils.py:29-33).  periodic=False: the plain
+    mean of 01_es_locomotion.ipynb ``rollout_fitness``.  The mean's summation
+    order is the device's (``_lane_tree_mean``); jnp.mean's XLA order is not
+    reproduced (it differs in the last bits only)."""
+    pos = np.asarray(position)
+    off = [0, pos.shape[0]] if offsets is None else list(offsets)
+    out = np.zeros((len(off) - 1, 2), pos.dtype)
+    for g in range(len(off) - 1):
+        p = pos[off[g]:off[g + 1]]
+        if periodic:
+            ref = p[0]
+            m = _lane_tree_mean(displacement(grid_shape, p, ref))
+            out[g] = periodic_boundary(grid_shape, ref + m)
+        else:
+            out[g] = _lane_tree_mean(p)
+    return out
+
+
+def swim_track(positions, grid_shape, offsets=None, start_step=0):
+    """swim_base.py:179-209 over a trajectory: positions[0] initialises
+    (``_init_displacement``), each later frame is one ``_update_displacement``.
+    Returns the cumulative displacement (G, 2) after every update."""
+    prev = group_com(positions[0], grid_shape, offsets)
+    cum = np.zeros_like(prev)
+    hist = []
+    for k, pos in enumerate(positions[1:], start=1):
+        com = group_com(pos, grid_shape, offsets)
+        delta = displacement(grid_shape, com, prev)
+        prev = com
+        if k > start_step:
+            cum = cum + delta
+        hist.append(cum.copy())
+    return np.asarray(hist)
