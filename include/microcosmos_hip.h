@@ -113,6 +113,26 @@ mc_status mc_plan_create(mc_plan** plan, int32_t grid_h, int32_t grid_w, int32_t
                          int32_t fluid_w, int32_t num_nodes, const int32_t* pairs_host,
                          int32_t num_edges, const int32_t* bending_pairs_host, int32_t num_hinges,
                          int32_t max_kernel_size);
+/* Batched plan: num_instances independent simulations (each its own periodic
+ * grid, lattice and filaments -- one reference `simulate` each) stepped
+ * together, every kernel launch covering all instances.  Instance i has
+ * instance_shapes[4i..4i+3] = (grid_h, grid_w, fluid_h, fluid_w) and nodes
+ * [node_offsets[i], node_offsets[i+1]); pairs / bending pairs use global node
+ * / edge ids and never join two instances.  State buffers are the
+ * concatenation of the per-instance arrays in instance order: positions and
+ * velocities (N, 2); steric (sum of grid_h*grid_w); fluid_velocity and f_grid
+ * instance by instance as (2 | 9, fluid_h, fluid_w) blocks.  The steric
+ * window (forces.py:66-70) runs over the instance's own nodes.  A plan made by
+ * mc_plan_create is the one-instance case.  Replaces the reference's jax.vmap
+ * over rollouts (examples/01_es_locomotion.ipynb), for ragged shapes. */
+mc_status mc_batch_plan_create(mc_plan** plan, int32_t num_instances,
+                               const int32_t* instance_shapes, const int32_t* node_offsets,
+                               const int32_t* pairs_host, int32_t num_edges,
+                               const int32_t* bending_pairs_host, int32_t num_hinges,
+                               int32_t max_kernel_size);
+/* Number of instances; offsets (may be NULL) receives per instance
+ * (grid cell offset, lattice cell offset, first node) as int64 triples. */
+mc_status mc_plan_instances(const mc_plan* plan, int32_t* num_instances, int64_t* offsets);
 mc_status mc_plan_destroy(mc_plan* plan);
 /* Device bytes held by the plan (topology + workspace). */
 mc_status mc_plan_workspace_bytes(const mc_plan* plan, size_t* bytes);
@@ -133,6 +153,16 @@ mc_status mc_rollout(mc_plan* plan, const mc_params* params, const mc_state* sta
                      const mc_edge_params* edges, const mc_actuation* act, double dt,
                      int64_t num_steps, float* record_position, float* record_fluid_velocity,
                      int32_t use_graph, void* stream);
+
+/* ES evaluation of every instance of a (batched) plan: num_steps coupled
+ * steps in place, graph-replayed, with the in-kernel actuation `act` (may be
+ * NULL).  displacement_out (num_instances, 2), device, may be NULL: the
+ * cumulative minimum-image displacement of each instance's periodic centre
+ * of mass over the rollout (swim_base.py:167-209), tracked per step on the
+ * device.  No host synchronisation per step. */
+mc_status mc_batch_rollout(mc_plan* plan, const mc_params* params, const mc_state* state,
+                           const mc_edge_params* edges, const mc_actuation* act, double dt,
+                           int64_t num_steps, float* displacement_out, void* stream);
 
 /* Measurement: num_steps coupled steps in place (no graph, same kernels as
  * mc_rollout) with HIP events on `stream` around every fused LBM

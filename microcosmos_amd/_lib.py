@@ -75,6 +75,12 @@ SIGNATURES = {
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "mc_rod_solve": (c_int32, [c_void_p, POINTER(McParams), c_void_p, c_void_p, c_void_p,
                                POINTER(McEdgeParams), c_int32, c_void_p]),
+    "mc_batch_plan_create": (c_int32, [POINTER(c_void_p), c_int32, c_void_p, c_void_p, c_void_p,
+                                       c_int32, c_void_p, c_int32, c_int32]),
+    "mc_plan_instances": (c_int32, [c_void_p, POINTER(c_int32), c_void_p]),
+    "mc_batch_rollout": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
+                                   POINTER(McEdgeParams), POINTER(McActuation), c_double, c_int64,
+                                   c_void_p, c_void_p]),
     "mc_center_of_mass": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32,
                                     c_void_p, c_void_p]),
     "mc_track_enable": (c_int32, [c_void_p, c_int32, c_void_p, c_int32]),

@@ -54,10 +54,25 @@ bool factor(int n, std::vector<int>& out) {
 
 }  // namespace
 
+struct InstHost {
+  int H, W, Hf, Wf, node0, nodes;
+};
+
 struct mc_plan {
   int dev = 0;
-  int H = 0, W = 0, Hf = 0, Wf = 0, N = 0, E = 0, B = 0, kmax = 0;
+  int N = 0, E = 0, B = 0, kmax = 0;
   int lds_max = 65536;
+  // instances
+  int n_inst = 0;
+  std::vector<InstHost> inst;
+  std::vector<InstGeo> geo_h;        // host copy of the geometry table
+  std::vector<int> node_inst_h;      // instance of every node
+  InstGeo* geo_d = nullptr;
+  double geo_mfv = 0.0;              // max_fluid_velocity the device table holds
+  bool geo_umax_set = false;         // u_max of the device table is valid
+  long long grid_cells = 0, lat_cells = 0, mask_words = 0;
+  int rows = 0, cols = 0, lbm_blocks = 0, max_w = 0, max_h = 0, min_hf = 0, min_wf = 0;
+  BatchDev bd{};
   std::vector<void*> allocs;
   size_t bytes = 0;
   // rod
@@ -74,9 +89,8 @@ struct mc_plan {
   int *row_nactive = nullptr, *row_list = nullptr, *row_sort = nullptr;
   unsigned long long* row_keys = nullptr;
   float2* spec = nullptr;
-  float2 *tw_w = nullptr, *tw_h = nullptr;
-  int *radix_w = nullptr, *radix_h = nullptr;
-  int nrad_w = 0, nrad_h = 0;
+  std::vector<int2> shapes;     // distinct (H, W) steric grids, each with a kf table
+  std::vector<long long> shape_kf_off;
   bool fft_ok = false;
   std::string fft_why;
   // immersed boundary + LBM
@@ -84,11 +98,12 @@ struct mc_plan {
   float* f_tmp = nullptr;
   float* node_tmp = nullptr;  // (N, 2) scratch
   long long* tick = nullptr;
-  float* kf = nullptr;          // Gaussian transfer function table (H, W)
+  float* kf = nullptr;          // Gaussian transfer functions, one table per distinct shape
   double kf_sigma = -1.0;       // sigma the table holds (-1: none)
   hipStream_t cap_stream = nullptr;
   hipStream_t side_stream = nullptr;
   TrackWork track{};            // displacement tracking (off: n_groups == 0)
+  TrackWork inst_track{};       // per-instance tracker of mc_batch_rollout
 
   template <class T>
   mc_status alloc(T** p, size_t count, const char* what) {
@@ -141,6 +156,9 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
     while (parent[a] != r) { int nx = parent[a]; parent[a] = r; a = nx; }
     return r;
   };
+  for (int e = 0; e < E; ++e)
+    if (p->node_inst_h[pairs[2 * e]] != p->node_inst_h[pairs[2 * e + 1]])
+      return fail(MC_ERR_BAD_SHAPE, "edge %d joins nodes of different instances", e);
   for (int e = 0; e < E; ++e) {
     int a = find(pairs[2 * e]), b = find(pairs[2 * e + 1]);
     if (a != b) { if (a < b) parent[b] = a; else parent[a] = b; }
@@ -235,38 +253,150 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
   MCCHK(upload(p, &a12, einc_off, "edge_inc_off"));
   MCCHK(upload(p, &a13, einc_flat, "edge_inc"));
   MCCHK(upload(p, &p->comp_of_node, comp_of_node, "comp_of_node"));
+  std::vector<int> comp_inst(C);
+  for (int c = 0; c < C; ++c) comp_inst[c] = p->node_inst_h[norder[noff[c]]];
+  int* ci_d;
+  MCCHK(upload(p, &ci_d, comp_inst, "comp_inst"));
+  p->bd.comp_inst = ci_d;
   p->L = RodLayoutDev{C, a0, a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11, a12, a13};
   return MC_OK;
 }
 
-mc_status build_fft(mc_plan* p) {
-  std::vector<int> rw, rh;
-  if (!factor(p->W, rw) || !factor(p->H, rh)) {
-    p->fft_why = "steric FFT needs grid sides whose prime factors are all <= 13";
-    return MC_OK;
+// Instance tables: node / row / column / LBM-workgroup ownership and the
+// concatenation offsets of every per-instance array.
+mc_status build_instances(mc_plan* p) {
+  const int n = p->n_inst;
+  p->geo_h.assign(n, InstGeo{});
+  p->node_inst_h.assign(p->N, 0);
+  std::vector<int> row_inst, col_inst, lbm_blk(n + 1, 0);
+  long long go = 0, fo = 0, mo = 0;
+  int r0 = 0, c0 = 0, b0 = 0;
+  p->min_hf = 1 << 30;
+  p->min_wf = 1 << 30;
+  for (int i = 0; i < n; ++i) {
+    const InstHost& h = p->inst[i];
+    InstGeo& g = p->geo_h[i];
+    g.H = h.H;
+    g.W = h.W;
+    g.Hf = h.Hf;
+    g.Wf = h.Wf;
+    g.dom_hf = (float)h.H;
+    g.dom_wf = (float)h.W;
+    // fluid.py:225-234 (scale_x uses grid_shape[0], as the reference does)
+    g.scale_x = (float)((double)h.Wf / (double)h.H);
+    g.scale_y = (float)((double)h.Hf / (double)h.W);
+    g.grid_off = go;
+    g.fl_off = fo;
+    g.mask_off = mo;
+    g.row0 = r0;
+    g.col0 = c0;
+    g.lbm_blk0 = b0;
+    g.node0 = h.node0;
+    g.nodes = h.nodes;
+    const long long hwf = (long long)h.Hf * h.Wf;
+    go += (long long)h.H * h.W;
+    fo += hwf;
+    mo += (hwf + 31) / 32;
+    for (int y = 0; y < h.H; ++y) row_inst.push_back(i);
+    for (int x = 0; x < h.W; ++x) col_inst.push_back(i);
+    r0 += h.H;
+    c0 += h.W;
+    b0 += (int)((hwf + LBM_THREADS - 1) / LBM_THREADS);
+    lbm_blk[i + 1] = b0;
+    for (int k = 0; k < h.nodes; ++k) p->node_inst_h[h.node0 + k] = i;
+    p->max_w = std::max(p->max_w, h.W);
+    p->max_h = std::max(p->max_h, h.H);
+    p->min_hf = std::min(p->min_hf, h.Hf);
+    p->min_wf = std::min(p->min_wf, h.Wf);
   }
-  const int TR = fft_rows_per_group(p->H, p->W);
-  const size_t row_lds = ((size_t)2 * TR * p->W + p->W) * sizeof(float2) + 16384;
-  if (row_lds > (size_t)p->lds_max || 3 * (size_t)p->H * sizeof(float2) + 256 > (size_t)p->lds_max) {
+  p->grid_cells = go;
+  p->lat_cells = fo;
+  p->mask_words = mo;
+  p->rows = r0;
+  p->cols = c0;
+  p->lbm_blocks = b0;
+  int *ni, *ri, *cc, *lb;
+  MCCHK(upload(p, &ni, p->node_inst_h, "node_inst"));
+  MCCHK(upload(p, &ri, row_inst, "row_inst"));
+  MCCHK(upload(p, &cc, col_inst, "col_inst"));
+  MCCHK(upload(p, &lb, lbm_blk, "lbm blocks"));
+  MCCHK(p->alloc(&p->geo_d, (size_t)n, "instance geometry"));
+  BatchDev& b = p->bd;
+  b.n_inst = n;
+  b.geo = p->geo_d;
+  b.node_inst = ni;
+  b.row_inst = ri;
+  b.col_inst = cc;
+  b.lbm_blk = lb;
+  b.rows = p->rows;
+  b.cols = p->cols;
+  b.lbm_blocks = p->lbm_blocks;
+  b.max_w = p->max_w;
+  b.max_h = p->max_h;
+  return MC_OK;
+}
+
+// Stockham plans for every distinct side length, twiddles, and one transfer-
+// function table per distinct (H, W) shape.  The geometry table is uploaded
+// by ensure_geo.
+mc_status build_fft(mc_plan* p) {
+  std::vector<int> lens;
+  for (const InstHost& h : p->inst) {
+    lens.push_back(h.W);
+    lens.push_back(h.H);
+  }
+  std::sort(lens.begin(), lens.end());
+  lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
+  std::vector<FftDev> plans;
+  std::vector<float2> tw;
+  for (int n : lens) {
+    std::vector<int> r;
+    if (!factor(n, r) || (int)r.size() > FFT_MAX_RADICES) {
+      p->fft_why = "steric FFT needs grid sides whose prime factors are all <= 13";
+      return MC_OK;
+    }
+    FftDev f{};
+    f.n = n;
+    f.tw_off = (int)tw.size();
+    f.nrad = (int)r.size();
+    for (size_t k = 0; k < r.size(); ++k) f.radix[k] = r[k];
+    for (int k = 0; k < n; ++k) {
+      const double a = -2.0 * M_PI * (double)k / (double)n;
+      tw.push_back(make_float2((float)cos(a), (float)sin(a)));
+    }
+    plans.push_back(f);
+  }
+  // row pass: 3 W complex + the deposit's static LDS (~12.3 KB); column pass: 3 H complex
+  if (fft_rows_lds_bytes(p->max_w) + 16384 > (size_t)p->lds_max ||
+      fft_cols_lds_bytes(p->max_h) + 256 > (size_t)p->lds_max) {
     p->fft_why = "steric FFT row/column does not fit in LDS";
     return MC_OK;
   }
-  auto table = [](int n) {
-    std::vector<float2> t(n);
-    for (int k = 0; k < n; ++k) {
-      const double a = -2.0 * M_PI * (double)k / (double)n;
-      t[k] = make_float2((float)cos(a), (float)sin(a));
-    }
-    return t;
+  auto plan_of = [&](int n) {
+    return (int)(std::lower_bound(lens.begin(), lens.end(), n) - lens.begin());
   };
-  MCCHK(upload(p, &p->tw_w, table(p->W), "twiddles W"));
-  MCCHK(upload(p, &p->tw_h, table(p->H), "twiddles H"));
-  MCCHK(upload(p, &p->radix_w, rw, "radix W"));
-  MCCHK(upload(p, &p->radix_h, rh, "radix H"));
-  p->nrad_w = (int)rw.size();
-  p->nrad_h = (int)rh.size();
-  MCCHK(p->alloc(&p->spec, (size_t)p->H * p->W, "spectrum"));
-  MCCHK(p->alloc(&p->kf, (size_t)p->H * p->W, "transfer function"));
+  long long kf_cells = 0;
+  for (size_t i = 0; i < p->inst.size(); ++i) {
+    InstGeo& g = p->geo_h[i];
+    g.fft_w = plan_of(g.W);
+    g.fft_h = plan_of(g.H);
+    size_t sh = 0;
+    while (sh < p->shapes.size() && !(p->shapes[sh].x == g.H && p->shapes[sh].y == g.W)) ++sh;
+    if (sh == p->shapes.size()) {
+      p->shapes.push_back(make_int2(g.H, g.W));
+      p->shape_kf_off.push_back(kf_cells);
+      kf_cells += (long long)g.H * g.W;
+    }
+    g.kf_off = p->shape_kf_off[sh];
+  }
+  FftDev* fd;
+  float2* twd;
+  MCCHK(upload(p, &fd, plans, "fft plans"));
+  MCCHK(upload(p, &twd, tw, "twiddles"));
+  p->bd.fft = fd;
+  p->bd.tw = twd;
+  MCCHK(p->alloc(&p->spec, (size_t)p->grid_cells, "spectrum"));
+  MCCHK(p->alloc(&p->kf, (size_t)kf_cells, "transfer functions"));
   p->fft_ok = true;
   return MC_OK;
 }
@@ -281,12 +411,13 @@ mc_status check_params(const mc_plan* p, const mc_params* q, bool need_fluid, bo
       return fail(MC_ERR_UNSUPPORTED, "synthetic_node_width > 0 is not implemented on the device path");
     if (q->ibm_kernel_size < 1 || q->ibm_kernel_size > IB_KMAX || q->ibm_kernel_size > p->kmax)
       return fail(MC_ERR_UNSUPPORTED, "ibm_kernel_size %d (plan supports 1..%d)", q->ibm_kernel_size, p->kmax);
-    if (q->ibm_kernel_size > p->Hf || q->ibm_kernel_size > p->Wf)
-      return fail(MC_ERR_BAD_SHAPE, "ibm_kernel_size %d exceeds the lattice %dx%d", q->ibm_kernel_size, p->Hf, p->Wf);
+    if (q->ibm_kernel_size > p->min_hf || q->ibm_kernel_size > p->min_wf)
+      return fail(MC_ERR_BAD_SHAPE, "ibm_kernel_size %d exceeds a lattice side (smallest %dx%d)",
+                  q->ibm_kernel_size, p->min_hf, p->min_wf);
     if (q->ibm_iterations < 0) return fail(MC_ERR_BAD_SHAPE, "ibm_iterations < 0");
   }
   if (need_steric && q->enable_steric) {
-    if (!p->fft_ok) return fail(MC_ERR_UNSUPPORTED, "%s (grid %dx%d)", p->fft_why.c_str(), p->H, p->W);
+    if (!p->fft_ok) return fail(MC_ERR_UNSUPPORTED, "%s", p->fft_why.c_str());
     if (q->steric_neighbor_skip < 0) return fail(MC_ERR_BAD_SHAPE, "steric_neighbor_skip < 0");
   }
   return MC_OK;
@@ -299,10 +430,6 @@ Consts derive(const mc_plan* p, const mc_params* q, double dt) {
   c.damping = (float)q->damping;
   c.enable_fluid = q->enable_fluid != 0;
   c.enable_steric = q->enable_steric != 0;
-  c.dom_h = p->H;
-  c.dom_w = p->W;
-  c.dom_hf = (float)p->H;
-  c.dom_wf = (float)p->W;
   const double s = q->steric_sigma;
   c.steric_strength = (float)q->steric_strength;
   c.steric_sv = (float)q->steric_scatter_value;
@@ -322,15 +449,6 @@ Consts derive(const mc_plan* p, const mc_params* q, double dt) {
   c.iters = q->ibm_iterations;
   c.force_clamp = q->enable_force_clamp != 0;
   c.relax = (float)q->ibm_relaxation;
-  // fluid.py:225-234 (scale_x uses grid_shape[0], as the reference does)
-  const double sx = (double)p->Wf / (double)p->H;
-  const double sy = (double)p->Hf / (double)p->W;
-  const double mean = 0.5 * (sx + sy);
-  c.umax = (float)(q->max_fluid_velocity * mean);
-  c.scale_x = (float)sx;
-  c.scale_y = (float)sy;
-  c.sdt_x = c.scale_x * c.dt;
-  c.sdt_y = c.scale_y * c.dt;
   c.node_mass = (float)q->node_mass;
   // fluid.py:155-161,342-344, float32 arithmetic after the double 1/(3 nu + 0.5)
   const float eps = 1.1920928955078125e-07f;
@@ -342,9 +460,26 @@ Consts derive(const mc_plan* p, const mc_params* q, double dt) {
   c.omega_m = om;
   c.cp = 1.0f - 0.5f * op;
   c.cm = 1.0f - 0.5f * om;
-  c.fl_h = p->Hf;
-  c.fl_w = p->Wf;
+  (void)p;
   return c;
+}
+
+// Per-instance u_max = max_fluid_velocity * mean scale (fluid.py:225-234, in
+// double, then float32) into the device geometry table.  Called before any
+// stream capture; uploads only when max_fluid_velocity changes.
+mc_status ensure_geo(mc_plan* p, const mc_params* q) {
+  if (p->geo_umax_set && p->geo_mfv == q->max_fluid_velocity) return MC_OK;
+  for (size_t i = 0; i < p->inst.size(); ++i) {
+    const InstHost& h = p->inst[i];
+    const double sx = (double)h.Wf / (double)h.H;
+    const double sy = (double)h.Hf / (double)h.W;
+    p->geo_h[i].umax = (float)(q->max_fluid_velocity * (0.5 * (sx + sy)));
+  }
+  HIPCHK(hipMemcpy(p->geo_d, p->geo_h.data(), p->geo_h.size() * sizeof(InstGeo),
+                   hipMemcpyHostToDevice));
+  p->geo_mfv = q->max_fluid_velocity;
+  p->geo_umax_set = true;
+  return MC_OK;
 }
 
 // Tabulates the Gaussian transfer function for `sigma` (utils.py:41-47) when
@@ -352,7 +487,8 @@ Consts derive(const mc_plan* p, const mc_params* q, double dt) {
 mc_status ensure_kf(mc_plan* p, double sigma, hipStream_t s) {
   if (!p->fft_ok || p->kf_sigma == sigma) return MC_OK;
   const float coef = (float)(-2.0 * (M_PI * M_PI) * (sigma * sigma));  // utils.py:47
-  HIPCHK(launch_gauss_table(p->H, p->W, coef, p->kf, s));
+  for (size_t i = 0; i < p->shapes.size(); ++i)
+    HIPCHK(launch_gauss_table(p->shapes[i].x, p->shapes[i].y, coef, p->kf + p->shape_kf_off[i], s));
   p->kf_sigma = sigma;
   return MC_OK;
 }
@@ -361,13 +497,12 @@ mc_status ensure_kf(mc_plan* p, double sigma, hipStream_t s) {
 // writes the filtered field to steric_out.
 mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float* vel,
                      float* steric_out, float* density_out, hipStream_t s) {
-  HIPCHK(launch_predict(c, p->N, pos, vel, p->pred_w, p->cic_cell, p->cic_w, p->row_cnt, s));
+  HIPCHK(launch_predict(c, p->bd, p->N, pos, vel, p->pred_w, p->cic_cell, p->cic_w, p->row_cnt, s));
   if (!c.enable_steric) return MC_OK;
-  HIPCHK(launch_bin_scan(p->H, p->row_cnt, p->row_off, p->row_fill, p->row_active, p->row_nactive, s));
+  HIPCHK(launch_bin_scan(p->rows, p->row_cnt, p->row_off, p->row_fill, p->row_active, p->row_nactive, s));
   HIPCHK(launch_row_fill(c, p->N, p->cic_cell, p->row_off, p->row_fill, p->row_list, s));
-  HIPCHK(launch_steric_fft_T(p->H, p->W, nullptr, p->row_off, p->row_list, p->row_sort, p->row_keys,
-                             (size_t)4 * p->N, p->cic_cell, p->cic_w, p->spec, p->tw_w, p->radix_w,
-                             p->nrad_w, p->tw_h, p->radix_h, p->nrad_h, p->kf, density_out,
+  HIPCHK(launch_steric_fft_T(p->bd, nullptr, p->row_off, p->row_list, p->row_sort, p->row_keys,
+                             (size_t)4 * p->N, p->cic_cell, p->cic_w, p->spec, p->kf, density_out,
                              steric_out, s));
   return MC_OK;
 }
@@ -389,17 +524,18 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
                        hipStream_t s, const StepSync& sy = StepSync()) {
   MCCHK(run_steric(p, c, in->position, in->velocity, out->steric, nullptr, s));
   if (!c.enable_steric && out->steric != in->steric)
-    HIPCHK(hipMemcpyAsync(out->steric, in->steric, (size_t)p->H * p->W * sizeof(float),
+    HIPCHK(hipMemcpyAsync(out->steric, in->steric, (size_t)p->grid_cells * sizeof(float),
                           hipMemcpyDeviceToDevice, s));
-  HIPCHK(launch_rod(c, p->L, p->N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 0,
+  HIPCHK(launch_rod(c, p->bd, p->L, p->N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 0,
                     in->position, in->velocity, nullptr, in->theta, ed->rest_lengths,
                     ed->bending_rest_angles, ed->bending_stiffness, act, out->steric, p->pred_w,
                     out->position, out->velocity, out->debug_vector, out->theta, s));
-  const size_t hwf = (size_t)p->Hf * p->Wf;
+  const size_t hwf = (size_t)p->lat_cells;
   long long* tick = act ? p->tick : nullptr;
   if (c.enable_fluid) {
     if (sy.wait_lbm) HIPCHK(hipStreamWaitEvent(s, sy.wait_lbm, 0));
-    HIPCHK(launch_ib(c, p->ib, in->f_grid, out->position, out->velocity, out->position, tick, s));
+    HIPCHK(launch_ib(c, p->bd, p->ib, in->f_grid, out->position, out->velocity, out->position, tick,
+                     s));
     hipStream_t ls = s;
     if (sy.b) {
       HIPCHK(hipEventRecord(sy.ev_force, s));
@@ -407,8 +543,8 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
       ls = sy.b;
     }
     if (sy.t0) HIPCHK(hipEventRecord(sy.t0, ls));
-    HIPCHK(launch_lbm(c, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
-                      p->ib.mask_words_per_row, p->ib.force, ls));
+    HIPCHK(launch_lbm(c, p->bd, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
+                      p->ib.force, ls));
     if (sy.t1) HIPCHK(hipEventRecord(sy.t1, ls));
     if (sy.b) HIPCHK(hipEventRecord(sy.ev_lbm, sy.b));
   } else {
@@ -420,7 +556,7 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
                             hipMemcpyDeviceToDevice, s));
   }
   if (p->track.n_groups > 0 && p->N > 0)
-    HIPCHK(launch_track(p->track, out->position, p->H, p->W, 1, s));
+    HIPCHK(launch_track(p->track, out->position, 1, s));
   (void)q;
   return MC_OK;
 }
@@ -450,17 +586,28 @@ extern "C" {
 const char* mc_version(void) { return MC_VERSION; }
 const char* mc_last_error(void) { return g_err.c_str(); }
 
-mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t fluid_h,
-                         int32_t fluid_w, int32_t num_nodes, const int32_t* pairs_host,
-                         int32_t num_edges, const int32_t* bending_pairs_host, int32_t num_hinges,
-                         int32_t max_kernel_size) {
-  g_err.clear();
+}  // extern "C"
+
+namespace {
+
+mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const int32_t* node_off,
+                      const int32_t* pairs_host, int32_t num_edges,
+                      const int32_t* bending_pairs_host, int32_t num_hinges,
+                      int32_t max_kernel_size) {
   if (!out) return fail(MC_ERR_INVALID, "plan pointer is NULL");
   *out = nullptr;
-  if (grid_h < 2 || grid_w < 2 || fluid_h < 2 || fluid_w < 2)
-    return fail(MC_ERR_BAD_SHAPE, "grid %dx%d / lattice %dx%d must be at least 2x2", grid_h, grid_w, fluid_h, fluid_w);
-  if (num_nodes < 0 || num_edges < 0 || num_hinges < 0)
-    return fail(MC_ERR_BAD_SHAPE, "negative count");
+  if (n_inst < 1) return fail(MC_ERR_BAD_SHAPE, "num_instances %d < 1", n_inst);
+  if (!shape || !node_off) return fail(MC_ERR_INVALID, "instance arrays are NULL");
+  for (int i = 0; i < n_inst; ++i) {
+    const int* sh = shape + 4 * i;
+    if (sh[0] < 2 || sh[1] < 2 || sh[2] < 2 || sh[3] < 2)
+      return fail(MC_ERR_BAD_SHAPE, "instance %d: grid %dx%d / lattice %dx%d must be at least 2x2",
+                  i, sh[0], sh[1], sh[2], sh[3]);
+    if (node_off[i + 1] < node_off[i])
+      return fail(MC_ERR_BAD_SHAPE, "node offsets descend at instance %d", i);
+  }
+  if (node_off[0] != 0) return fail(MC_ERR_BAD_SHAPE, "node offsets must start at 0");
+  if (num_edges < 0 || num_hinges < 0) return fail(MC_ERR_BAD_SHAPE, "negative count");
   if ((num_edges > 0 && !pairs_host) || (num_hinges > 0 && !bending_pairs_host))
     return fail(MC_ERR_INVALID, "topology arrays are NULL");
   if (max_kernel_size < 1 || max_kernel_size > IB_KMAX)
@@ -470,11 +617,11 @@ mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t 
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->dev) == hipSuccess && lds > 0)
     p->lds_max = lds;
-  p->H = grid_h;
-  p->W = grid_w;
-  p->Hf = fluid_h;
-  p->Wf = fluid_w;
-  p->N = num_nodes;
+  p->n_inst = n_inst;
+  for (int i = 0; i < n_inst; ++i)
+    p->inst.push_back(InstHost{shape[4 * i], shape[4 * i + 1], shape[4 * i + 2], shape[4 * i + 3],
+                               node_off[i], node_off[i + 1] - node_off[i]});
+  p->N = node_off[n_inst];
   p->E = num_edges;
   p->B = num_hinges;
   p->kmax = max_kernel_size;
@@ -485,19 +632,24 @@ mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t 
     return st;
   };
   mc_status st;
+  if ((st = build_instances(p)) != MC_OK) return bail(st);
   if ((st = build_layout(p, pairs_host, bending_pairs_host)) != MC_OK) return bail(st);
   if ((st = build_fft(p)) != MC_OK) return bail(st);
+  // geometry without u_max (set by ensure_geo from the parameters of a call)
+  if (hipMemcpy(p->geo_d, p->geo_h.data(), p->geo_h.size() * sizeof(InstGeo),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(MC_ERR_HIP, "upload of the instance geometry failed"));
   const int N = p->N;
-  const size_t hw = (size_t)p->H * p->W;
-  const size_t hwf = (size_t)p->Hf * p->Wf;
+  const size_t hwf = (size_t)p->lat_cells;
+  const size_t R = (size_t)p->rows;
   if ((st = p->alloc(&p->pred_w, 2 * (size_t)N, "pred")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->node_tmp, 2 * (size_t)N, "node tmp")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->cic_cell, 4 * (size_t)N, "cic cell")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->cic_w, 4 * (size_t)N, "cic w")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->row_cnt, (size_t)p->H, "row cnt")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->row_off, (size_t)p->H + 1, "row off")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->row_fill, (size_t)p->H, "row fill")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->row_active, (size_t)p->H, "row active")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_cnt, R, "row cnt")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_off, R + 1, "row off")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_fill, R, "row fill")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->row_active, R, "row active")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_nactive, 1, "row n")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_list, 2 * (size_t)N, "row list")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_sort, 2 * (size_t)N, "row sort")) != MC_OK) return bail(st);
@@ -509,7 +661,7 @@ mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t 
   w.comp_of_node = p->comp_of_node;
   w.comp_node_off = p->L.comp_node_off;
   w.comp_nodes = p->L.comp_nodes;
-  w.mask_words_per_row = (p->Wf + 31) / 32;
+  w.cells = p->lat_cells;
   w.cl_bpn = ib_scratch_bytes_per_node(max_kernel_size);
   w.lds_bytes = ib_lds_bytes(p->lds_max);
   w.dirty = false;
@@ -541,12 +693,37 @@ mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t 
   if (ib_prepare(p->lds_max) != hipSuccess)
     return bail(fail(MC_ERR_HIP, "cluster solve: %zu B of dynamic LDS refused", w.lds_bytes));
   if ((st = p->alloc(&w.force, 2 * hwf, "force U")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.mask, (size_t)w.mask_words_per_row * p->Hf, "U mask")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.mask, (size_t)p->mask_words, "U mask")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->f_tmp, 9 * hwf, "f ping-pong")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->tick, 1, "tick")) != MC_OK) return bail(st);
-  (void)hw;
   *out = p;
   return MC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t fluid_h,
+                         int32_t fluid_w, int32_t num_nodes, const int32_t* pairs_host,
+                         int32_t num_edges, const int32_t* bending_pairs_host, int32_t num_hinges,
+                         int32_t max_kernel_size) {
+  g_err.clear();
+  if (!out) return fail(MC_ERR_INVALID, "plan pointer is NULL");
+  if (num_nodes < 0) return fail(MC_ERR_BAD_SHAPE, "negative count");
+  const int32_t shape[4] = {grid_h, grid_w, fluid_h, fluid_w};
+  const int32_t off[2] = {0, num_nodes};
+  return create_plan(out, 1, shape, off, pairs_host, num_edges, bending_pairs_host, num_hinges,
+                     max_kernel_size);
+}
+
+mc_status mc_batch_plan_create(mc_plan** out, int32_t num_instances, const int32_t* instance_shapes,
+                               const int32_t* node_offsets, const int32_t* pairs_host,
+                               int32_t num_edges, const int32_t* bending_pairs_host,
+                               int32_t num_hinges, int32_t max_kernel_size) {
+  g_err.clear();
+  return create_plan(out, num_instances, instance_shapes, node_offsets, pairs_host, num_edges,
+                     bending_pairs_host, num_hinges, max_kernel_size);
 }
 
 mc_status mc_plan_destroy(mc_plan* p) {
@@ -577,6 +754,7 @@ mc_status mc_step(mc_plan* p, const mc_params* q, const mc_state* in, const mc_s
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
   hipStream_t s = (hipStream_t)stream;
   const Consts c = derive(p, q, dt);
+  MCCHK(ensure_geo(p, q));
   if (q->enable_steric) MCCHK(ensure_kf(p, q->steric_sigma, s));
   return enqueue_step(p, q, c, in, out, ed, nullptr, s);
 }
@@ -596,7 +774,8 @@ mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const m
   if (num_steps == 0) return MC_OK;
   hipStream_t us = (hipStream_t)stream;
   const Consts c = derive(p, q, dt);
-  const size_t hwf = (size_t)p->Hf * p->Wf;
+  const size_t hwf = (size_t)p->lat_cells;
+  MCCHK(ensure_geo(p, q));
   if (q->enable_steric) MCCHK(ensure_kf(p, q->steric_sigma, us));
   Actuation A{};
   const Actuation* ap = nullptr;
@@ -709,6 +888,7 @@ mc_status mc_profile_step(mc_plan* p, const mc_params* q, const mc_state* st,
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
   hipStream_t s = (hipStream_t)stream;
   const Consts c = derive(p, q, dt);
+  MCCHK(ensure_geo(p, q));
   if (q->enable_steric) MCCHK(ensure_kf(p, q->steric_sigma, s));
   Actuation A{};
   const Actuation* ap = nullptr;
@@ -746,7 +926,7 @@ mc_status mc_profile_step(mc_plan* p, const mc_params* q, const mc_state* st,
   HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[3 * num_steps]));
   for (auto& e : ev) (void)hipEventDestroy(e);
   if ((num_steps & 1) && q->enable_fluid)
-    HIPCHK(hipMemcpyAsync(st->f_grid, p->f_tmp, 9 * (size_t)p->Hf * p->Wf * sizeof(float),
+    HIPCHK(hipMemcpyAsync(st->f_grid, p->f_tmp, 9 * (size_t)p->lat_cells * sizeof(float),
                           hipMemcpyDeviceToDevice, s));
   *lbm_ms_avg = lbm / (double)num_steps;
   *step_ms_avg = (double)tot / (double)num_steps;
@@ -756,12 +936,11 @@ mc_status mc_profile_step(mc_plan* p, const mc_params* q, const mc_state* st,
 mc_status mc_gaussian_filter(mc_plan* p, const float* data, float* out, double sigma, void* stream) {
   g_err.clear();
   if (!p || !data || !out) return fail(MC_ERR_INVALID, "NULL argument");
-  if (!p->fft_ok) return fail(MC_ERR_UNSUPPORTED, "%s (grid %dx%d)", p->fft_why.c_str(), p->H, p->W);
+  if (!p->fft_ok) return fail(MC_ERR_UNSUPPORTED, "%s", p->fft_why.c_str());
   hipStream_t s = (hipStream_t)stream;
   MCCHK(ensure_kf(p, sigma, s));
-  HIPCHK(launch_steric_fft_T(p->H, p->W, data, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
-                             p->spec, p->tw_w, p->radix_w, p->nrad_w, p->tw_h, p->radix_h,
-                             p->nrad_h, p->kf, nullptr, out, s));
+  HIPCHK(launch_steric_fft_T(p->bd, data, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
+                             p->spec, p->kf, nullptr, out, s));
   return MC_OK;
 }
 
@@ -774,9 +953,10 @@ mc_status mc_steric_field(mc_plan* p, const mc_params* q, const float* position,
   MCCHK(check_params(p, &qq, false, true));
   hipStream_t s = (hipStream_t)stream;
   const Consts c = derive(p, &qq, 1.0);
+  MCCHK(ensure_geo(p, &qq));
   MCCHK(ensure_kf(p, qq.steric_sigma, s));
   MCCHK(run_steric(p, c, position, nullptr, steric_out, density_out, s));
-  if (force_out) HIPCHK(launch_steric_nodes(c, p->N, steric_out, p->pred_w, force_out, s));
+  if (force_out) HIPCHK(launch_steric_nodes(c, p->bd, p->N, steric_out, p->pred_w, force_out, s));
   return MC_OK;
 }
 
@@ -793,10 +973,12 @@ mc_status mc_ib_lbm_step(mc_plan* p, const mc_params* q, const float* position, 
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
   hipStream_t s = (hipStream_t)stream;
   const Consts c = derive(p, &qq, dt);
+  MCCHK(ensure_geo(p, &qq));
   if (p->N > 0 && velocity_out != velocity)
     HIPCHK(hipMemcpyAsync(velocity_out, velocity, 2 * (size_t)p->N * sizeof(float), hipMemcpyDeviceToDevice, s));
-  HIPCHK(launch_ib(c, p->ib, f_in, position, velocity_out, pos_out ? pos_out : p->node_tmp, nullptr, s));
-  HIPCHK(launch_lbm(c, f_in, f_out, fluid_velocity_out, p->ib.mask, p->ib.mask_words_per_row, p->ib.force, s));
+  HIPCHK(launch_ib(c, p->bd, p->ib, f_in, position, velocity_out, pos_out ? pos_out : p->node_tmp,
+                   nullptr, s));
+  HIPCHK(launch_lbm(c, p->bd, f_in, f_out, fluid_velocity_out, p->ib.mask, p->ib.force, s));
   return MC_OK;
 }
 
@@ -810,7 +992,7 @@ mc_status mc_rod_solve(mc_plan* p, const mc_params* q, float* position, float* t
   hipStream_t s = (hipStream_t)stream;
   Consts c = derive(p, q, 1.0);
   c.cycles = cycles;
-  HIPCHK(launch_rod(c, p->L, p->N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 1, position,
+  HIPCHK(launch_rod(c, p->bd, p->L, p->N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 1, position,
                     nullptr, y, theta, ed->rest_lengths, ed->bending_rest_angles, ed->bending_stiffness,
                     nullptr, nullptr, nullptr, position, nullptr, nullptr, theta, s));
   return MC_OK;
@@ -844,11 +1026,19 @@ mc_status mc_track_enable(mc_plan* p, int32_t n_groups, const int32_t* group_off
   for (int g = 0; g < n_groups; ++g)
     if (group_offsets_host[g + 1] <= group_offsets_host[g])
       return fail(MC_ERR_BAD_SHAPE, "group %d is empty or offsets descend", g);
+  std::vector<int2> dims(n_groups);
+  for (int g = 0; g < n_groups; ++g) {
+    const int i0 = p->node_inst_h[group_offsets_host[g]];
+    if (p->node_inst_h[group_offsets_host[g + 1] - 1] != i0)
+      return fail(MC_ERR_BAD_SHAPE, "group %d spans two instances", g);
+    dims[g] = make_int2(p->inst[i0].H, p->inst[i0].W);
+  }
   TrackWork t{};
   t.n_groups = n_groups;
   t.start_step = start_step;
   std::vector<int> off(group_offsets_host, group_offsets_host + n_groups + 1);
   MCCHK(upload(p, &t.off, off, "track offsets"));
+  MCCHK(upload(p, &t.dims, dims, "track dims"));
   MCCHK(p->alloc(&t.prev, 2 * (size_t)n_groups, "track prev"));
   MCCHK(p->alloc(&t.cum, 2 * (size_t)n_groups, "track cumulative"));
   MCCHK(p->alloc(&t.count, (size_t)n_groups, "track count"));
@@ -861,7 +1051,7 @@ mc_status mc_track_reset(mc_plan* p, const float* position, void* stream) {
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   if (p->track.n_groups == 0) return fail(MC_ERR_INVALID, "tracking is not enabled");
   if (!position) return fail(MC_ERR_INVALID, "position is NULL");
-  HIPCHK(launch_track(p->track, position, p->H, p->W, 0, (hipStream_t)stream));
+  HIPCHK(launch_track(p->track, position, 0, (hipStream_t)stream));
   return MC_OK;
 }
 
@@ -874,6 +1064,62 @@ mc_status mc_track_read(mc_plan* p, float* cumulative_out, float* com_out, void*
   if (cumulative_out)
     HIPCHK(hipMemcpyAsync(cumulative_out, p->track.cum, b, hipMemcpyDeviceToDevice, s));
   if (com_out) HIPCHK(hipMemcpyAsync(com_out, p->track.prev, b, hipMemcpyDeviceToDevice, s));
+  return MC_OK;
+}
+
+mc_status mc_batch_rollout(mc_plan* p, const mc_params* q, const mc_state* st,
+                           const mc_edge_params* ed, const mc_actuation* act, double dt,
+                           int64_t num_steps, float* displacement_out, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  if (displacement_out) {
+    if (!st || (p->N > 0 && !st->position)) return fail(MC_ERR_INVALID, "state is NULL");
+    for (int i = 0; i < p->n_inst; ++i)
+      if (p->inst[i].nodes == 0)
+        return fail(MC_ERR_BAD_SHAPE, "instance %d has no nodes to track", i);
+    if (p->inst_track.n_groups == 0) {
+      // one tracker group per instance, allocated on first use
+      std::vector<int> off(1, 0);
+      std::vector<int2> dims;
+      for (const InstHost& h : p->inst) {
+        off.push_back(h.node0 + h.nodes);
+        dims.push_back(make_int2(h.H, h.W));
+      }
+      TrackWork t{};
+      t.n_groups = p->n_inst;
+      t.start_step = 0;
+      MCCHK(upload(p, &t.off, off, "instance offsets"));
+      MCCHK(upload(p, &t.dims, dims, "instance dims"));
+      MCCHK(p->alloc(&t.prev, 2 * (size_t)p->n_inst, "instance prev"));
+      MCCHK(p->alloc(&t.cum, 2 * (size_t)p->n_inst, "instance cumulative"));
+      MCCHK(p->alloc(&t.count, (size_t)p->n_inst, "instance count"));
+      p->inst_track = t;
+    }
+    HIPCHK(launch_track(p->inst_track, st->position, 0, (hipStream_t)stream));
+  }
+  const TrackWork keep = p->track;
+  if (displacement_out) p->track = p->inst_track;
+  const mc_status r = mc_rollout(p, q, st, ed, act, dt, num_steps, nullptr, nullptr, 1, stream);
+  p->track = keep;
+  if (r != MC_OK) return r;
+  if (displacement_out)
+    HIPCHK(hipMemcpyAsync(displacement_out, p->inst_track.cum,
+                          2 * (size_t)p->n_inst * sizeof(float), hipMemcpyDeviceToDevice,
+                          (hipStream_t)stream));
+  return MC_OK;
+}
+
+mc_status mc_plan_instances(const mc_plan* p, int32_t* num_instances, int64_t* offsets) {
+  g_err.clear();
+  if (!p || !num_instances) return fail(MC_ERR_INVALID, "NULL argument");
+  *num_instances = p->n_inst;
+  if (offsets)
+    for (int i = 0; i < p->n_inst; ++i) {
+      const InstGeo& g = p->geo_h[i];
+      offsets[3 * i] = g.grid_off;
+      offsets[3 * i + 1] = g.fl_off;
+      offsets[3 * i + 2] = g.node0;
+    }
   return MC_OK;
 }
 

@@ -74,18 +74,19 @@ __device__ __forceinline__ void uf_union(int* p, int a, int b) {
 
 template <int K>
 __global__ void __launch_bounds__(64) k_ib_stencil(
-    Consts c, int N, const float* __restrict__ pos, const float* __restrict__ vel,
+    Consts c, BatchDev bd, int N, const float* __restrict__ pos, const float* __restrict__ vel,
     const int* __restrict__ comp_of_node, int* __restrict__ st_sx, int* __restrict__ st_sy,
     float* __restrict__ st_w, unsigned long long* __restrict__ st_won, float* __restrict__ vib,
     int* __restrict__ owner, int2* __restrict__ links, int* __restrict__ nlinks) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   constexpr int K2 = K * K;
-  const int W = c.fl_w, H = c.fl_h;
-  const float gx = pos[2 * n] * c.scale_x;
-  const float gy = pos[2 * n + 1] * c.scale_y;
-  vib[2 * n] = (vel[2 * n] * c.scale_x) * c.dt;
-  vib[2 * n + 1] = (vel[2 * n + 1] * c.scale_y) * c.dt;
+  const InstGeo& gi = bd.geo[bd.node_inst[n]];
+  const int W = gi.Wf, H = gi.Hf;
+  const float gx = pos[2 * n] * gi.scale_x;
+  const float gy = pos[2 * n + 1] * gi.scale_y;
+  vib[2 * n] = (vel[2 * n] * gi.scale_x) * c.dt;
+  vib[2 * n + 1] = (vel[2 * n + 1] * gi.scale_y) * c.dt;
   const float sxf = floorf(gx) - (float)c.half;
   const float syf = floorf(gy) - (float)c.half;
   const int sx = imod(f2i(sxf), W);
@@ -95,7 +96,8 @@ __global__ void __launch_bounds__(64) k_ib_stencil(
   int prev[K2];
 #pragma unroll
   for (int t = 0; t < K2; ++t)
-    prev[t] = atomicCAS(&owner[wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W)], -1, comp);
+    prev[t] = atomicCAS(&owner[gi.fl_off + wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W)], -1,
+                        comp);
   // 1-D weights: tap t uses wx[t / K] and wy[t % K] (fluid.py:53-69)
   float wx[K], wy[K];
 #pragma unroll
@@ -210,7 +212,7 @@ struct ClusterArgs {
   float *vel_io, *pos_out, *force;
   int *cell_seg, *owner;
   uint32_t* mask;
-  int mwpr;
+  BatchDev bd;
   unsigned char* gscratch;
   size_t bpn;        // global scratch bytes per node
   size_t lds_bytes;  // dynamic LDS of the launch (a multiple of 16)
@@ -292,14 +294,21 @@ __device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns) {
 }
 
 template <int K>
-__device__ __forceinline__ void cluster_small(const ClusterArgs& a, unsigned char* base,
+__device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGeo& gi,
+                                              unsigned char* base,
                                               const int* nodes, int nq, int ns, int* wsum,
                                               int* s_seg) {
   constexpr int K2 = K * K;
   const Consts& c = a.c;
   const int ne = nq * K2;
-  const int W = c.fl_w, H = c.fl_h;
+  const int W = gi.Wf, H = gi.Hf;
   const size_t HW = (size_t)H * W;
+  const float umax = gi.umax;
+  const float* fin = a.f_in + 9 * gi.fl_off;
+  float* frc = a.force + 2 * gi.fl_off;
+  uint32_t* msk = a.mask + gi.mask_off;
+  int* cseg = a.cell_seg + gi.fl_off;
+  int* own = a.owner + gi.fl_off;
   const int tid = threadIdx.x, bs = blockDim.x;
   size_t off = 0;
   auto take = [&](size_t b) {
@@ -340,9 +349,9 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, unsigned cha
     for (int t = 0; t < K2; ++t) {
       if ((won >> t) & 1ull) {
         const int sg = atomicAdd(s_seg, 1);
-        a.cell_seg[g[t]] = sg;
+        cseg[g[t]] = sg;
         scell[sg] = g[t];
-        a.owner[g[t]] = -1;
+        own[g[t]] = -1;
       }
     }
   }
@@ -350,7 +359,7 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, unsigned cha
   // every tap's segment and its arrival index there
   if (mine) {
 #pragma unroll
-    for (int t = 0; t < K2; ++t) seg[t] = a.cell_seg[g[t]];
+    for (int t = 0; t < K2; ++t) seg[t] = cseg[g[t]];
 #pragma unroll
     for (int t = 0; t < K2; ++t) sub[t] = atomicAdd(&sstart[seg[t]], 1);
   }
@@ -386,7 +395,7 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, unsigned cha
       if (s < ns) {
         const size_t cell = (size_t)scell[s];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) f[q][i] = a.f_in[i * HW + cell];
+        for (int i = 0; i < 9; ++i) f[q][i] = fin[i * HW + cell];
       }
     }
 #pragma unroll
@@ -409,7 +418,7 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, unsigned cha
 #pragma unroll
       for (int i = 0; i < 9; ++i) fq[i] = fmax_nan(f[q][i], MC_EPSIL);
       float rho, ux, uy;
-      lbm_macro(fq, c.umax, rho, ux, uy);
+      lbm_macro(fq, umax, rho, ux, uy);
       sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
       srho[s] = rho;
       su0[s] = make_float2(ux, uy);
@@ -432,7 +441,7 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, unsigned cha
       const float cx = (vb.x - ux) * c.relax;
       const float cy = (vb.y - uy) * c.relax;
       const float mag = sqrtf((cx * cx + cy * cy) + MC_EPSIL);
-      const float cf = fmin_nan(1.0f, c.umax / (mag + MC_EPSIL));
+      const float cf = fmin_nan(1.0f, umax / (mag + MC_EPSIL));
       corr[r] = make_float2(cx * cf, cy * cf);
     }
     __syncthreads();
@@ -477,22 +486,21 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, unsigned cha
     const float2 u0 = su0[s];
     float ucx = suc[s].x, ucy = suc[s].y;
     const float mag = sqrtf((ucx * ucx + ucy * ucy) + MC_EPSIL);
-    const float cl = fmin_nan(1.0f, c.umax / (mag + MC_EPSIL));
+    const float cl = fmin_nan(1.0f, umax / (mag + MC_EPSIL));
     ucx = ucx * cl;
     ucy = ucy * cl;
     float fx = (ucx - u0.x) * rho;
     float fy = (ucy - u0.y) * rho;
     if (c.force_clamp) {
       const float fm = sqrtf((fx * fx + fy * fy) + MC_EPSIL);
-      const float fc = fmin_nan(1.0f, c.umax / (fm + MC_EPSIL));
+      const float fc = fmin_nan(1.0f, umax / (fm + MC_EPSIL));
       fx = fx * fc;
       fy = fy * fc;
     }
     suc[s] = make_float2(fx, fy);
-    a.force[cell] = fx;
-    a.force[HW + cell] = fy;
-    const int cy = cell / W, cx = cell - cy * W;
-    atomicOr(&a.mask[(size_t)cy * a.mwpr + (cx >> 5)], 1u << (cx & 31));
+    frc[cell] = fx;
+    frc[HW + cell] = fy;
+    atomicOr(&msk[cell >> 5], 1u << (cell & 31));
   }
   __syncthreads();
   // node drag (fluid.py:364-382) and the final wrap (simulate.py:97)
@@ -508,27 +516,34 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, unsigned cha
     }
     const float px = a.pos[2 * n], py = a.pos[2 * n + 1];
     const float vx = a.vel_io[2 * n], vy = a.vel_io[2 * n + 1];
-    const float dvx = (fx / c.node_mass) / c.sdt_x;
-    const float dvy = (fy / c.node_mass) / c.sdt_y;
+    const float dvx = (fx / c.node_mass) / (gi.scale_x * c.dt);
+    const float dvy = (fy / c.node_mass) / (gi.scale_y * c.dt);
     a.vel_io[2 * n] = vx - dvx;
     a.vel_io[2 * n + 1] = vy - dvy;
-    a.pos_out[2 * n] = fmodf_floor(px, c.dom_wf);
-    a.pos_out[2 * n + 1] = fmodf_floor(py, c.dom_hf);
+    a.pos_out[2 * n] = fmodf_floor(px, gi.dom_wf);
+    a.pos_out[2 * n + 1] = fmodf_floor(py, gi.dom_hf);
   }
 }
 
 // General solve: any cluster size, arrays in LDS when they fit, else in the
 // cluster's global scratch.  `nodes` holds the node ids in ascending order.
 template <int K>
-__device__ __forceinline__ void cluster_general(const ClusterArgs& a, unsigned char* smem,
+__device__ __forceinline__ void cluster_general(const ClusterArgs& a, const InstGeo& gi,
+                                                unsigned char* smem,
                                                 size_t avail, const CLayout& G, const GExtra& X,
                                                 const int* nodes, int nq, int ns, int* wsum,
                                                 int* s_seg) {
   constexpr int K2 = K * K;
   const Consts& c = a.c;
   const int ne = nq * K2;
-  const int W = c.fl_w, H = c.fl_h;
+  const int W = gi.Wf, H = gi.Hf;
   const size_t HW = (size_t)H * W;
+  const float umax = gi.umax;
+  const float* fin = a.f_in + 9 * gi.fl_off;
+  float* frc = a.force + 2 * gi.fl_off;
+  uint32_t* msk = a.mask + gi.mask_off;
+  int* cseg = a.cell_seg + gi.fl_off;
+  int* own = a.owner + gi.fl_off;
   const int tid = threadIdx.x, bs = blockDim.x;
   CLayout L;
   const size_t lbytes = llayout(smem, nq, ne, ns, L);
@@ -547,9 +562,9 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, unsigned c
       if ((won >> t) & 1ull) {
         const int g = wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W);
         const int seg = atomicAdd(s_seg, 1);
-        a.cell_seg[g] = seg;
+        cseg[g] = seg;
         X.scell[seg] = g;
-        a.owner[g] = -1;
+        own[g] = -1;
       }
     }
   }
@@ -560,7 +575,7 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, unsigned c
     const int sx = a.st_sx[n], sy = a.st_sy[n];
     int seg[K2];
 #pragma unroll
-    for (int t = 0; t < K2; ++t) seg[t] = a.cell_seg[wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W)];
+    for (int t = 0; t < K2; ++t) seg[t] = cseg[wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W)];
 #pragma unroll
     for (int t = 0; t < K2; ++t) {
       L.tapseg[r * K2 + t] = seg[t];
@@ -603,7 +618,7 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, unsigned c
     const size_t cell = (size_t)X.scell[s];
     float f[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) f[i] = a.f_in[i * HW + cell];
+    for (int i = 0; i < 9; ++i) f[i] = fin[i * HW + cell];
     const int lo = L.sstart[s], hi = L.sstart[s + 1];
     for (int j = lo + 1; j < hi; ++j) {
       const int v = L.eidx[j];
@@ -619,7 +634,7 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, unsigned c
 #pragma unroll
     for (int i = 0; i < 9; ++i) f[i] = fmax_nan(f[i], MC_EPSIL);
     float rho, ux, uy;
-    lbm_macro(f, c.umax, rho, ux, uy);
+    lbm_macro(f, umax, rho, ux, uy);
     L.sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
     X.srho[s] = rho;
     X.su0[s] = make_float2(ux, uy);
@@ -640,7 +655,7 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, unsigned c
       const float cx = (vb.x - ux) * c.relax;
       const float cy = (vb.y - uy) * c.relax;
       const float mag = sqrtf((cx * cx + cy * cy) + MC_EPSIL);
-      const float cf = fmin_nan(1.0f, c.umax / (mag + MC_EPSIL));
+      const float cf = fmin_nan(1.0f, umax / (mag + MC_EPSIL));
       L.corr[r] = make_float2(cx * cf, cy * cf);
     }
     __syncthreads();
@@ -666,22 +681,21 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, unsigned c
     const float2 u0 = X.su0[s];
     float ucx = L.suc[s].x, ucy = L.suc[s].y;
     const float mag = sqrtf((ucx * ucx + ucy * ucy) + MC_EPSIL);
-    const float cl = fmin_nan(1.0f, c.umax / (mag + MC_EPSIL));
+    const float cl = fmin_nan(1.0f, umax / (mag + MC_EPSIL));
     ucx = ucx * cl;
     ucy = ucy * cl;
     float fx = (ucx - u0.x) * rho;
     float fy = (ucy - u0.y) * rho;
     if (c.force_clamp) {
       const float fm = sqrtf((fx * fx + fy * fy) + MC_EPSIL);
-      const float fc = fmin_nan(1.0f, c.umax / (fm + MC_EPSIL));
+      const float fc = fmin_nan(1.0f, umax / (fm + MC_EPSIL));
       fx = fx * fc;
       fy = fy * fc;
     }
     L.suc[s] = make_float2(fx, fy);
-    a.force[cell] = fx;
-    a.force[HW + cell] = fy;
-    const int cy = cell / W, cx = cell - cy * W;
-    atomicOr(&a.mask[(size_t)cy * a.mwpr + (cx >> 5)], 1u << (cx & 31));
+    frc[cell] = fx;
+    frc[HW + cell] = fy;
+    atomicOr(&msk[cell >> 5], 1u << (cell & 31));
   }
   __syncthreads();
   for (int r = tid; r < nq; r += bs) {
@@ -694,12 +708,12 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, unsigned c
       fx = fx + F.x * w;
       fy = fy + F.y * w;
     }
-    const float dvx = (fx / c.node_mass) / c.sdt_x;
-    const float dvy = (fy / c.node_mass) / c.sdt_y;
+    const float dvx = (fx / c.node_mass) / (gi.scale_x * c.dt);
+    const float dvy = (fy / c.node_mass) / (gi.scale_y * c.dt);
     a.vel_io[2 * n] = a.vel_io[2 * n] - dvx;
     a.vel_io[2 * n + 1] = a.vel_io[2 * n + 1] - dvy;
-    a.pos_out[2 * n] = fmodf_floor(a.pos[2 * n], c.dom_wf);
-    a.pos_out[2 * n + 1] = fmodf_floor(a.pos[2 * n + 1], c.dom_hf);
+    a.pos_out[2 * n] = fmodf_floor(a.pos[2 * n], gi.dom_wf);
+    a.pos_out[2 * n + 1] = fmodf_floor(a.pos[2 * n + 1], gi.dom_hf);
   }
 }
 
@@ -713,6 +727,7 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
   constexpr int K2 = K * K;
   const int root = blockIdx.x;
   if (a.label[root] != root) return;
+  const InstGeo& gi = a.bd.geo[a.bd.comp_inst[root]];  // a cluster never spans instances
   const int nq = a.cl_cnt[root];
   const int ne = nq * K2;
   const int tid = threadIdx.x, bs = blockDim.x;
@@ -808,9 +823,9 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
   __syncthreads();
   const int ns = s_ns;
   if (nq <= bs && stage_lds && small_bytes(nq, ne, ns) <= avail)
-    cluster_small<K>(a, smem, nodes, nq, ns, wsum, &s_seg);
+    cluster_small<K>(a, gi, smem, nodes, nq, ns, wsum, &s_seg);
   else
-    cluster_general<K>(a, smem, avail, G, X, nodes, nq, ns, wsum, &s_seg);
+    cluster_general<K>(a, gi, smem, avail, G, X, nodes, nq, ns, wsum, &s_seg);
 }
 
 __global__ void k_ib_reset(int HW, int C, int* __restrict__ owner, int* __restrict__ parent,
@@ -895,19 +910,20 @@ hipError_t ib_prepare(int lds_max) {
     default: LAUNCH(6); break; \
   }
 
-hipError_t launch_ib(const Consts& c, IBWork& w, const float* f_in, const float* pos,
-                     float* vel_io, float* pos_out, long long* tick, hipStream_t s) {
+hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float* f_in,
+                     const float* pos, float* vel_io, float* pos_out, long long* tick,
+                     hipStream_t s) {
   const int N = w.N, C = w.C;
   if (N == 0) {
     // no nodes: no support; the LBM pass takes the clamp-only force everywhere
     if (tick) return launch_tick(tick, s);
     return hipSuccess;
   }
-  const int HW = c.fl_h * c.fl_w;
+  const long long HW = w.cells;
   if (w.dirty) {
     // an earlier call stopped between its launches: restore the idle state
-    const int n = std::max(HW, C);
-    hipLaunchKernelGGL(k_ib_reset, dim3((n + 255) / 256), dim3(256), 0, s, HW, C, w.owner,
+    const int n = (int)std::max(HW, (long long)C);
+    hipLaunchKernelGGL(k_ib_reset, dim3((n + 255) / 256), dim3(256), 0, s, (int)HW, C, w.owner,
                        w.uf_parent, w.nlinks);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -915,7 +931,7 @@ hipError_t launch_ib(const Consts& c, IBWork& w, const float* f_in, const float*
   w.dirty = true;
   const int nb = (N + 63) / 64;
 #define L_STENCIL(KK)                                                                            \
-  hipLaunchKernelGGL(k_ib_stencil<KK>, dim3(nb), dim3(64), 0, s, c, N, pos, vel_io,              \
+  hipLaunchKernelGGL(k_ib_stencil<KK>, dim3(nb), dim3(64), 0, s, c, b, N, pos, vel_io,           \
                      w.comp_of_node, w.st_sx, w.st_sy, w.st_w, w.st_won, w.vib, w.owner, w.links, \
                      w.nlinks)
   MC_KSWITCH(L_STENCIL);
@@ -951,7 +967,7 @@ hipError_t launch_ib(const Consts& c, IBWork& w, const float* f_in, const float*
   a.cell_seg = w.cell_seg;
   a.owner = w.owner;
   a.mask = w.mask;
-  a.mwpr = w.mask_words_per_row;
+  a.bd = b;
   a.gscratch = w.cl_scratch;
   a.bpn = w.cl_bpn;
   a.lds_bytes = w.lds_bytes;

@@ -9,15 +9,15 @@ namespace mc {
 
 constexpr int IB_KMAX = 6;                 // largest supported ibm_kernel_size
 constexpr int SORT_LDS_CAP = 2048;         // bitonic sort capacity in LDS (ids)
+constexpr int FFT_MAX_RADICES = 16;        // radix stages of one FFT length
 
 // Float constants derived from ConstraintSolverConfig exactly as the reference
-// derives them (Python double arithmetic, then a float32 weak scalar).
+// derives them (Python double arithmetic, then a float32 weak scalar).  Shared
+// by every instance of a plan; per-instance geometry is in InstGeo.
 struct Consts {
   // step (simulate.py:37-100)
   float dt, damping;
   int enable_fluid, enable_steric;
-  int dom_h, dom_w;           // Fields.grid_shape
-  float dom_hf, dom_wf;       // as floats
   // steric (forces.py)
   float steric_strength, steric_sv, steric_2s2, steric_norm, steric_svs, fft_coef;
   int steric_nskip;
@@ -27,10 +27,64 @@ struct Consts {
   float k_st, k_sh, cos_w;
   // IB + LBM (fluid.py)
   int K, half, iters, force_clamp;
-  float half_width, relax, umax, scale_x, scale_y, sdt_x, sdt_y, node_mass;
+  float half_width, relax, node_mass;
   float omega_p, omega_m, cp, cm;
-  int fl_h, fl_w;             // f_grid spatial shape
 };
+
+// One independent instance (lattice + filaments) of a plan.  A plan holds
+// n_inst >= 1 instances; every per-instance array is the concatenation of the
+// instances' arrays in instance order:
+//   (H, W) grids (steric, density) start at grid_off;
+//   a k-plane lattice array (f: 9, u* and force: 2) starts at k * fl_off,
+//   each plane Hf * Wf cells;
+//   nodes [node0, node0 + nodes).
+struct InstGeo {
+  int H, W, Hf, Wf;           // Fields.grid_shape, f_grid.shape[1:]
+  float dom_hf, dom_wf;       // H, W as floats (periodic_boundary sizes)
+  float scale_x, scale_y;     // fluid.py:225-234 (scale_x uses grid_shape[0])
+  float umax;                 // max_fluid_velocity * mean scale (params-dependent)
+  long long grid_off;         // first cell of the (H, W) grids
+  long long fl_off;           // first cell of the lattice
+  long long mask_off;         // first U-mask word (ceil(Hf * Wf / 32) words, flat cell index)
+  int row0, col0;             // first global steric row / column
+  int lbm_blk0;               // first LBM workgroup
+  int node0, nodes;           // node range
+  int fft_w, fft_h;           // FFT plan of length W / of length H
+  long long kf_off;           // transposed transfer-function table of the (H, W) shape
+};
+
+// Stockham plan of one FFT length: radix schedule and twiddle table offset.
+struct FftDev {
+  int n, tw_off, nrad;
+  int radix[FFT_MAX_RADICES];
+};
+
+// Device view of a plan's instances.
+struct BatchDev {
+  int n_inst;
+  const InstGeo* geo;
+  const int* node_inst;       // (N) instance of every node
+  const int* comp_inst;       // (C) instance of every filament component
+  const int* row_inst;        // (rows) instance of every global steric row
+  const int* col_inst;        // (cols) instance of every global steric column
+  const int* lbm_blk;         // (n_inst + 1) LBM workgroup offsets
+  const FftDev* fft;
+  const float2* tw;           // concatenated twiddle tables
+  int rows, cols, lbm_blocks; // totals over instances
+  int max_w, max_h;           // largest steric grid sides
+};
+
+constexpr int LBM_THREADS = 256;  // lattice cells per LBM workgroup
+
+// Instance of item i, given per-instance offsets off[0..n] (off[0] = 0).
+__device__ __forceinline__ int find_inst(const int* off, int n, int i) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
 
 struct RodLayoutDev {
   int num_comp;
@@ -47,48 +101,36 @@ struct Actuation {           // rest_b(t) = amp_b * sin(omega_b * t + phase_b)
 
 // ---- launchers (stream-ordered, no host synchronisation) ----
 // steric
-hipError_t launch_predict(const Consts& c, int N, const float* pos, const float* vel,
-                          float* pred_w, int* cic_cell, float* cic_w, int* row_cnt,
-                          hipStream_t s);
+hipError_t launch_predict(const Consts& c, const BatchDev& b, int N, const float* pos,
+                          const float* vel, float* pred_w, int* cic_cell, float* cic_w,
+                          int* row_cnt, hipStream_t s);
 hipError_t launch_bin_scan(int T, const int* cnt, int* off, int* fill, int* active,
                            int* n_active, hipStream_t s);
 hipError_t launch_row_fill(const Consts& c, int N, const int* cic_cell, const int* row_off,
                            int* row_fill, int* row_list, hipStream_t s);
-hipError_t launch_deposit_rows_fft(const Consts& c, int N, const int* row_off, int* row_list,
-                                   const int* cic_cell, const float* cic_w, float2* spec,
-                                   const float2* tw_w, const int* radix_w, int nrad_w,
-                                   float* density_out, int* sort_scratch,
-                                   unsigned long long* key_scratch, hipStream_t s);
-hipError_t launch_fft_rows(int H, int W, const float* real_in, float2* spec,
-                           const float2* tw_w, const int* radix_w, int nrad_w, hipStream_t s);
-hipError_t launch_gauss_table(int H, int W, float coef, float* kf, hipStream_t s);
-// Whole steric filter: forward rows (CIC deposit when real_in is NULL),
-// transfer function on the transposed spectrum, inverse rows into `out`.
-int fft_rows_per_group(int H, int W);
-hipError_t launch_steric_fft_T(int H, int W, const float* real_in, const int* row_off, int* row_list,
-                               int* sort_scratch, unsigned long long* key_scratch, size_t key_half,
-                               const int* cic_cell, const float* cic_w, float2* specT,
-                               const float2* tw_w, const int* radix_w, int nrad_w,
-                               const float2* tw_h, const int* radix_h, int nrad_h, const float* kfT,
-                               float* density_out, float* out, hipStream_t s);
-hipError_t launch_fft_cols_filter(int H, int W, float2* spec, const float2* tw_h,
-                                  const int* radix_h, int nrad_h, const float* kf, int lds_bytes_max,
-                                  hipStream_t s);
-hipError_t launch_ifft_rows_real(int H, int W, const float2* spec, float* out,
-                                 const float2* tw_w, const int* radix_w, int nrad_w,
-                                 hipStream_t s);
+hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t s);
+// Whole steric filter over every instance: forward rows (CIC deposit when
+// real_in is NULL), transfer function on the transposed spectrum, inverse
+// rows into `out`.
+hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const int* row_off,
+                               int* row_list, int* sort_scratch, unsigned long long* key_scratch,
+                               size_t key_half, const int* cic_cell, const float* cic_w,
+                               float2* specT, const float* kfT, float* density_out, float* out,
+                               hipStream_t s);
+size_t fft_rows_lds_bytes(int max_w);
+size_t fft_cols_lds_bytes(int max_h);
 // rod
 // mode 0: full node phase of a step (prologue + constraint cycles + velocity)
 // mode 1: constraint cycles only, positions in place, y = Cosserat target
-hipError_t launch_rod(const Consts& c, const RodLayoutDev& L, int N, size_t lds_bytes,
-                      int use_lds, float* gscratch, int mode, const float* pos_in,
-                      const float* vel_in, const float* y_in, const float* theta_in,
-                      const float* rest_len, const float* rest_ang, const float* kb,
-                      const Actuation* act, const float* steric, const float* pred_w,
-                      float* pos_out, float* vel_out, float* debug_out, float* theta_out,
-                      hipStream_t s);
-hipError_t launch_steric_nodes(const Consts& c, int N, const float* steric, const float* pred_w,
-                               float* force_out, hipStream_t s);
+hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L, int N,
+                      size_t lds_bytes, int use_lds, float* gscratch, int mode,
+                      const float* pos_in, const float* vel_in, const float* y_in,
+                      const float* theta_in, const float* rest_len, const float* rest_ang,
+                      const float* kb, const Actuation* act, const float* steric,
+                      const float* pred_w, float* pos_out, float* vel_out, float* debug_out,
+                      float* theta_out, hipStream_t s);
+hipError_t launch_steric_nodes(const Consts& c, const BatchDev& b, int N, const float* steric,
+                               const float* pred_w, float* force_out, hipStream_t s);
 hipError_t launch_tick(long long* t, hipStream_t s);
 size_t rod_scratch_floats(int nn, int ne, int nb);
 size_t rod_lds_bytes(int nn, int ne, int nb, int nni, int nei);
@@ -109,9 +151,9 @@ struct IBWork {
   size_t cl_bpn;
   size_t lds_bytes;            // dynamic LDS of the cluster solve
   bool dirty;                  // a call stopped between launches: idle state to restore
-  float *force;                // (2, H, W) IB force, valid on the support U
-  uint32_t* mask;              // per lattice row ceil(W/32) words: U bits
-  int mask_words_per_row;
+  long long cells;             // lattice cells over all instances
+  float *force;                // (2, Hf, Wf) per instance: IB force, valid on the support U
+  uint32_t* mask;              // U bits by flat lattice cell index, per instance
 };
 size_t ib_scratch_bytes_per_node(int kmax);
 size_t ib_lds_bytes(int lds_max);
@@ -120,23 +162,24 @@ hipError_t ib_prepare(int lds_max);
 // (support, iterations, force into w.force / w.mask, node drag into vel_io,
 // final wrap of pos into pos_out).  Only the solve reads f_in.  `tick` (may be
 // NULL) is advanced once.
-hipError_t launch_ib(const Consts& c, IBWork& w, const float* f_in, const float* pos,
-                     float* vel_io, float* pos_out, long long* tick, hipStream_t s);
+hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float* f_in,
+                     const float* pos, float* vel_io, float* pos_out, long long* tick,
+                     hipStream_t s);
 // swimmer centre of mass / displacement tracking
 struct TrackWork {
   int n_groups;       // 0: tracking off
   int start_step;     // steps up to this one do not accumulate
   int* off;           // (n_groups + 1) node-range offsets
+  int2* dims;         // (n_groups) grid (H, W) of each group
   float *prev, *cum;  // (n_groups, 2) previous centre, cumulative displacement
   long long* count;   // (n_groups) steps since the reset
 };
 hipError_t launch_com(int n_groups, const int* off, const float* pos, int H, int W, int periodic,
                       float* com, hipStream_t s);
 // mode 0: reset (prev = centre, cumulative = 0); mode 1: one step's update
-hipError_t launch_track(const TrackWork& t, const float* pos, int H, int W, int mode,
-                        hipStream_t s);
+hipError_t launch_track(const TrackWork& t, const float* pos, int mode, hipStream_t s);
 // LBM
-hipError_t launch_lbm(const Consts& c, const float* f_in, float* f_out, float* ustar,
-                      uint32_t* mask, int mask_words_per_row, const float* force, hipStream_t s);
+hipError_t launch_lbm(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
+                      float* ustar, uint32_t* mask, const float* force, hipStream_t s);
 
 }  // namespace mc

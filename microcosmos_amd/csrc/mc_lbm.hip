@@ -43,27 +43,42 @@ __device__ __forceinline__ float edot(int i, float vx, float vy) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) k_lbm(const float* __restrict__ fin, float* __restrict__ fout,
-                                             float* __restrict__ ustar, uint32_t* __restrict__ mask,
-                                             int mwpr, const float* __restrict__ force, int H,
-                                             int W, int nxb, float umax, float op, float om,
-                                             float cp, float cm, int force_clamp) {
+__global__ void __launch_bounds__(LBM_THREADS) k_lbm(BatchDev bd, const float* __restrict__ fin_all,
+                                                      float* __restrict__ fout_all,
+                                                      float* __restrict__ ustar_all,
+                                                      uint32_t* __restrict__ mask_all,
+                                                      const float* __restrict__ force_all,
+                                                      float op, float om, float cp, float cm,
+                                                      int force_clamp) {
   // XCD-aware block order: workgroup i runs on XCD i % 8, so give each XCD a
-  // contiguous band of rows (its L2 then holds the rows it writes into).
+  // contiguous band of cells (its L2 then holds the rows it writes into).
   int b = blockIdx.x;
   const int G = gridDim.x;
   if ((G & 7) == 0) b = (b & 7) * (G >> 3) + (b >> 3);
-  const int y = b / nxb;
-  const int x = (b - y * nxb) * blockDim.x + threadIdx.x;
-  if (x >= W) return;
+  const int inst = find_inst(bd.lbm_blk, bd.n_inst, b);
+  const InstGeo& g = bd.geo[inst];
+  const int H = g.Hf, W = g.Wf;
   const size_t HW = (size_t)H * W;
+  // LBM_THREADS consecutive cells of the instance's lattice per workgroup
+  const int ci = (b - g.lbm_blk0) * LBM_THREADS + threadIdx.x;
+  if (ci >= (int)HW) return;
+  const int y = ci / W;
+  const int x = ci - y * W;
+  const float umax = g.umax;
+  const float* fin = fin_all + 9 * g.fl_off;
+  float* fout = fout_all + 9 * g.fl_off;
+  float* ustar = ustar_all + 2 * g.fl_off;
+  const float* force = force_all + 2 * g.fl_off;
+  uint32_t* mask = mask_all + g.mask_off;
   const int yp = (y + 1 == H) ? 0 : y + 1;
   const int ym = (y == 0) ? H - 1 : y - 1;
   const size_t cell = (size_t)y * W + x;
   float f[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) f[i] = fmax_nan(fin[i * HW + cell], MC_EPSIL);
-  uint32_t* mw = mask + (size_t)y * mwpr + (x >> 5);
+  // U bits are indexed by the flat cell index: a 32-cell word lies inside one
+  // wavefront, which reads it before its first lane clears it
+  uint32_t* mw = mask + (ci >> 5);
   const uint32_t m = *mw;
   // density and clamped velocity (fluid.py:136-142,268-274); `cl1` tells
   // whether the clamp changed u
@@ -79,7 +94,7 @@ __global__ void __launch_bounds__(256) k_lbm(const float* __restrict__ fin, floa
   uy = uy * cl1;
   float Fx, Fy;
   bool zero_force = false;
-  if ((m >> (x & 31)) & 1u) {
+  if ((m >> (ci & 31)) & 1u) {
     Fx = force[cell];
     Fy = force[HW + cell];
   } else if (cl1 == 1.0f) {
@@ -102,7 +117,7 @@ __global__ void __launch_bounds__(256) k_lbm(const float* __restrict__ fin, floa
       Fy = Fy * fc;
     }
   }
-  if ((x & 31) == 0) *mw = 0u;  // consume the U bitmask word (every lane has read it)
+  if ((ci & 31) == 0) *mw = 0u;  // consume the U bitmask word (every lane has read it)
   // u* = u + 0.5 F / rho (fluid.py:346); with F = +0 the quotient is +0
   const float usx = zero_force ? ux + 0.0f : ux + (0.5f * Fx) / rho;
   const float usy = zero_force ? uy + 0.0f : uy + (0.5f * Fy) / rho;
@@ -157,13 +172,10 @@ __global__ void __launch_bounds__(256) k_lbm(const float* __restrict__ fin, floa
   ustar[HW + cell] = usy;
 }
 
-hipError_t launch_lbm(const Consts& c, const float* f_in, float* f_out, float* ustar,
-                      uint32_t* mask, int mwpr, const float* force, hipStream_t s) {
-  const int H = c.fl_h, W = c.fl_w;
-  const int block = W >= 256 ? 256 : ((W + 63) / 64) * 64;
-  const int nxb = (W + block - 1) / block;
-  hipLaunchKernelGGL(k_lbm, dim3(H * nxb), dim3(block), 0, s, f_in, f_out, ustar, mask, mwpr, force,
-                     H, W, nxb, c.umax, c.omega_p, c.omega_m, c.cp, c.cm, c.force_clamp);
+hipError_t launch_lbm(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
+                      float* ustar, uint32_t* mask, const float* force, hipStream_t s) {
+  hipLaunchKernelGGL(k_lbm, dim3(b.lbm_blocks), dim3(LBM_THREADS), 0, s, b, f_in, f_out, ustar,
+                     mask, force, c.omega_p, c.omega_m, c.cp, c.cm, c.force_clamp);
   return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@ namespace {
 
 struct RodArgs {
   Consts c;
+  BatchDev bd;
   RodLayoutDev L;
   int N;
   int mode;
@@ -49,10 +50,13 @@ __device__ __forceinline__ float sample_bilinear_wrap(const float* V, int H, int
 }
 
 // forces.py:20-81 for one node: -grad V by central differences minus the
-// analytic (2n+1)-window self/neighbour contribution over the flat node index.
-__device__ __forceinline__ void steric_force(const Consts& c, const float* V, const float* pred_w, int N,
-                                             int g, float& fx, float& fy) {
-  const int H = c.dom_h, W = c.dom_w;
+// analytic (2n+1)-window self/neighbour contribution over the flat node index
+// (within the node's instance: an instance is one reference simulation).
+__device__ __forceinline__ void steric_force(const Consts& c, const InstGeo& gi, const float* V_all,
+                                             const float* pred_w, int g, float& fx, float& fy) {
+  const int H = gi.H, W = gi.W;
+  const float* V = V_all + gi.grid_off;
+  const int lo = gi.node0, hi = gi.node0 + gi.nodes;
   const float px = pred_w[2 * g], py = pred_w[2 * g + 1];
   const float gx = (sample_bilinear_wrap(V, H, W, py, px + 0.5f) -
                     sample_bilinear_wrap(V, H, W, py, px - 0.5f)) / 1.0f;
@@ -62,10 +66,10 @@ __device__ __forceinline__ void steric_force(const Consts& c, const float* V, co
   const int nsk = c.steric_nskip;
   for (int k = -nsk; k <= nsk; ++k) {
     const int j = g + k;
-    const float valid = (j >= 0 && j < N) ? 1.0f : 0.0f;
-    const int js = j < 0 ? 0 : (j > N - 1 ? N - 1 : j);
-    const float rx = min_image(px, pred_w[2 * js], c.dom_wf);
-    const float ry = min_image(py, pred_w[2 * js + 1], c.dom_hf);
+    const float valid = (j >= lo && j < hi) ? 1.0f : 0.0f;
+    const int js = j < lo ? lo : (j > hi - 1 ? hi - 1 : j);
+    const float rx = min_image(px, pred_w[2 * js], gi.dom_wf);
+    const float ry = min_image(py, pred_w[2 * js + 1], gi.dom_hf);
     const float d2 = rx * rx + ry * ry;
     const float Kv = dexp(-d2 / c.steric_2s2) / c.steric_norm;
     const float kv = Kv * valid;
@@ -85,6 +89,7 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   const Consts& c = a.c;
   const RodLayoutDev& L = a.L;
   const int comp = blockIdx.x;
+  const InstGeo& gi = a.bd.geo[a.bd.comp_inst[comp]];
   const int n0 = L.comp_node_off[comp], nn = L.comp_node_off[comp + 1] - n0;
   const int e0 = L.comp_edge_off[comp], ne = L.comp_edge_off[comp + 1] - e0;
   const int b0 = L.comp_hinge_off[comp], nb = L.comp_hinge_off[comp + 1] - b0;
@@ -156,7 +161,7 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
     float px = a.pos_in[2 * g] + vx * c.dt, py = a.pos_in[2 * g + 1] + vy * c.dt;
     if (c.enable_steric) {
       float fx, fy;
-      steric_force(c, a.steric, a.pred_w, a.N, g, fx, fy);
+      steric_force(c, gi, a.steric, a.pred_w, g, fx, fy);
       px = px + (fx * c.steric_strength) * c.dt;
       py = py + (fy * c.steric_strength) * c.dt;
     }
@@ -184,7 +189,7 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   }
   __syncthreads();
 
-  const float Wd = c.dom_wf, Hd = c.dom_hf;
+  const float Wd = gi.dom_wf, Hd = gi.dom_hf;
   for (int cyc = 0; cyc < c.cycles; ++cyc) {
     // theta pass: hinge corrections (pbd.py:32-40)
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -315,23 +320,23 @@ size_t rod_lds_bytes(int nn, int ne, int nb, int nni, int nei) {
   return rod_scratch_floats(nn, ne, nb) * sizeof(float) + ints * sizeof(int);
 }
 
-__global__ void k_steric_nodes(Consts c, int N, const float* __restrict__ steric,
+__global__ void k_steric_nodes(Consts c, BatchDev bd, int N, const float* __restrict__ steric,
                                const float* __restrict__ pred_w, float* __restrict__ force_out) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   float fx, fy;
-  steric_force(c, steric, pred_w, N, n, fx, fy);
+  steric_force(c, bd.geo[bd.node_inst[n]], steric, pred_w, n, fx, fy);
   force_out[2 * n] = fx;
   force_out[2 * n + 1] = fy;
 }
 
 __global__ void k_tick(long long* t) { *t = *t + 1; }
 
-hipError_t launch_steric_nodes(const Consts& c, int N, const float* steric, const float* pred_w,
-                               float* force_out, hipStream_t s) {
+hipError_t launch_steric_nodes(const Consts& c, const BatchDev& b, int N, const float* steric,
+                               const float* pred_w, float* force_out, hipStream_t s) {
   if (N == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_steric_nodes, dim3((N + 255) / 256), dim3(256), 0, s, c, N, steric, pred_w,
-                     force_out);
+  hipLaunchKernelGGL(k_steric_nodes, dim3((N + 255) / 256), dim3(256), 0, s, c, b, N, steric,
+                     pred_w, force_out);
   return hipGetLastError();
 }
 
@@ -340,8 +345,8 @@ hipError_t launch_tick(long long* t, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_rod(const Consts& c, const RodLayoutDev& L, int N, size_t lds_bytes, int use_lds,
-                      float* gscratch, int mode, const float* pos_in, const float* vel_in,
+hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L, int N,
+                      size_t lds_bytes, int use_lds, float* gscratch, int mode, const float* pos_in, const float* vel_in,
                       const float* y_in, const float* theta_in, const float* rest_len,
                       const float* rest_ang, const float* kb, const Actuation* act,
                       const float* steric, const float* pred_w, float* pos_out, float* vel_out,
@@ -349,6 +354,7 @@ hipError_t launch_rod(const Consts& c, const RodLayoutDev& L, int N, size_t lds_
   if (L.num_comp == 0) return hipSuccess;
   RodArgs a;
   a.c = c;
+  a.bd = b;
   a.L = L;
   a.N = N;
   a.mode = mode;

@@ -156,37 +156,40 @@ __device__ float2* fft_lds(float2* a, float2* b, int n, int stride, int batch, c
   return src;
 }
 
-__global__ void k_predict(Consts c, int N, const float* __restrict__ pos, const float* __restrict__ vel,
-                          float* __restrict__ pred_w, int* __restrict__ cic_cell,
-                          float* __restrict__ cic_w, int* __restrict__ row_cnt) {
+__global__ void k_predict(Consts c, BatchDev bd, int N, const float* __restrict__ pos,
+                          const float* __restrict__ vel, float* __restrict__ pred_w,
+                          int* __restrict__ cic_cell, float* __restrict__ cic_w,
+                          int* __restrict__ row_cnt) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
+  const InstGeo& gi = bd.geo[bd.node_inst[n]];
   float px = pos[2 * n], py = pos[2 * n + 1];
   if (vel) {  // inertial prediction (simulate.py:46-50); without vel: positions as given
     const float vx = vel[2 * n] * c.damping, vy = vel[2 * n + 1] * c.damping;
     px = px + vx * c.dt;
     py = py + vy * c.dt;
   }
-  const float x = fmodf_floor(px, c.dom_wf), y = fmodf_floor(py, c.dom_hf);
+  const float x = fmodf_floor(px, gi.dom_wf), y = fmodf_floor(py, gi.dom_hf);
   pred_w[2 * n] = x;
   pred_w[2 * n + 1] = y;
   if (!c.enable_steric) return;
-  const int W = c.dom_w, H = c.dom_h;
+  const int W = gi.W, H = gi.H;
   const float fx = floorf(x), fy = floorf(y);
   const int x0 = imod(f2i(fx), W), y0 = imod(f2i(fy), H);
   const int x1 = imod(x0 + 1, W), y1 = imod(y0 + 1, H);
   const float dx = x - fx, dy = y - fy;
   const float sv = c.steric_sv;
+  // columns within the instance, rows as global steric rows
   cic_cell[4 * n + 0] = x0;
   cic_cell[4 * n + 1] = x1;
-  cic_cell[4 * n + 2] = y0;
-  cic_cell[4 * n + 3] = y1;
+  cic_cell[4 * n + 2] = gi.row0 + y0;
+  cic_cell[4 * n + 3] = gi.row0 + y1;
   cic_w[4 * n + 0] = ((1.0f - dx) * (1.0f - dy)) * sv;
   cic_w[4 * n + 1] = (dx * (1.0f - dy)) * sv;
   cic_w[4 * n + 2] = ((1.0f - dx) * dy) * sv;
   cic_w[4 * n + 3] = (dx * dy) * sv;
-  atomicAdd(&row_cnt[y0], 1);
-  if (y1 != y0) atomicAdd(&row_cnt[y1], 1);
+  atomicAdd(&row_cnt[gi.row0 + y0], 1);
+  if (y1 != y0) atomicAdd(&row_cnt[gi.row0 + y1], 1);
 }
 
 __global__ void k_row_fill(int N, const int* __restrict__ cic_cell, const int* __restrict__ row_off,
@@ -209,214 +212,25 @@ __device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restr
   for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = tw[i];
 }
 
-// One workgroup per row y: fold the row's CIC entries cell by cell in the
-// reference's order (all w00 by node, then w10, w01, w11 -- forces.py:112-115),
-// then the forward FFT of the row.  Each node binned to the row contributes
-// exactly two entries (corners q = 0, 1 if y == y0; q = 2, 3 if y == y1).
-// Sorting the entries by (x, q, node rank) puts each cell's contributions
-// contiguously and in reference order; a segment head thread folds them.
-__global__ void __launch_bounds__(256) k_deposit_rows_fft(
-    int H, int W, const int* __restrict__ row_off, int* __restrict__ row_list,
-    int* __restrict__ sort_tmp, unsigned long long* __restrict__ key_tmp, size_t key_half,
-    const int* __restrict__ cic_cell, const float* __restrict__ cic_w, float2* __restrict__ spec,
-    const float2* __restrict__ tw_g, const int* __restrict__ radix_g, int nrad,
-    float* __restrict__ density_out) {
-  extern __shared__ float2 lds[];
-  __shared__ int sort_lds[DEP_IDS_LDS];
-  __shared__ unsigned long long keys_lds[DEP_KEYS_LDS];
-  __shared__ unsigned long long sort_keys[DEP_KEYS_LDS];
-  __shared__ int s_radix[32];
-  float2* a = lds;
-  float2* b = lds + W;
-  float2* tw = lds + 2 * W;
-  if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
-  load_twiddles(tw, tw_g, W);
-  const int y = blockIdx.x;
-  const int off = row_off[y];
-  const int m = row_off[y + 1] - off;
-  int* nodes = row_list + off;
-  sort_unique<int>(nodes, m, sort_lds, DEP_IDS_LDS, sort_tmp + off, 0x7fffffff);
-  for (int x = threadIdx.x; x < W; x += blockDim.x) a[x] = make_float2(0.0f, 0.0f);
-  const int ne = 2 * m;
-  unsigned long long* keys = (ne <= DEP_KEYS_LDS) ? keys_lds : (key_tmp + 2 * (size_t)off);
-  for (int j = threadIdx.x; j < m; j += blockDim.x) {
-    const int n = nodes[j];
-    const unsigned long long x0 = (unsigned)cic_cell[4 * n + 0], x1 = (unsigned)cic_cell[4 * n + 1];
-    const unsigned long long qb = (cic_cell[4 * n + 2] == y) ? 0ull : 2ull;
-    keys[2 * j] = (x0 << 34) | (qb << 32) | (unsigned)j;
-    keys[2 * j + 1] = (x1 << 34) | ((qb + 1ull) << 32) | (unsigned)j;
-  }
-  __threadfence_block();
-  __syncthreads();
-  // beyond the LDS capacity the keys live in key_tmp[2 off ..) and the merge
-  // sort ping-pongs through the second half of key_tmp (key_half entries on)
-  sort_unique<unsigned long long>(keys, ne, sort_keys, DEP_KEYS_LDS,
-                                  key_tmp + key_half + 2 * (size_t)off, ~0ull);
-  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
-    const unsigned long long k = keys[i];
-    const int x = (int)(k >> 34);
-    if (i > 0 && (int)(keys[i - 1] >> 34) == x) continue;
-    float acc = 0.0f;
-    for (int e = i; e < ne; ++e) {
-      const unsigned long long ke = keys[e];
-      if ((int)(ke >> 34) != x) break;
-      const int q = (int)((ke >> 32) & 3ull);
-      const int n = nodes[(int)(ke & 0xffffffffull)];
-      acc = acc + cic_w[4 * n + q];
-    }
-    a[x].x = acc;
-  }
-  __syncthreads();
-  if (density_out)
-    for (int x = threadIdx.x; x < W; x += blockDim.x) density_out[(size_t)y * W + x] = a[x].x;
-  float2* r = fft_lds(a, b, W, W, 1, s_radix, nrad, tw, false);
-  for (int x = threadIdx.x; x < W; x += blockDim.x) spec[(size_t)y * W + x] = r[x];
-}
-
-__global__ void __launch_bounds__(256) k_fft_rows(int H, int W, const float* __restrict__ in,
-                                                  float2* __restrict__ spec,
-                                                  const float2* __restrict__ tw_g,
-                                                  const int* __restrict__ radix_g, int nrad) {
-  extern __shared__ float2 lds[];
-  __shared__ int s_radix[32];
-  float2* a = lds;
-  float2* b = lds + W;
-  float2* tw = lds + 2 * W;
-  if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
-  load_twiddles(tw, tw_g, W);
-  const int y = blockIdx.x;
-  for (int x = threadIdx.x; x < W; x += blockDim.x) a[x] = make_float2(in[(size_t)y * W + x], 0.0f);
-  __syncthreads();
-  float2* r = fft_lds(a, b, W, W, 1, s_radix, nrad, tw, false);
-  for (int x = threadIdx.x; x < W; x += blockDim.x) spec[(size_t)y * W + x] = r[x];
-}
-
-// fftfreq(n)[k] as JAX computes it: float(k or k - n) / float(n)
-__device__ __forceinline__ float fftfreq(int k, int n) {
-  const int kk = (k < (n + 1) / 2) ? k : k - n;
-  return (float)kk / (float)n;
-}
-
-// Gaussian transfer function exp(coef (fr^2 + fc^2)) on the fftfreq grid
-// (utils.py:41-47), tabulated once per sigma.
-__global__ void k_gauss_table(int H, int W, float coef, float* __restrict__ kf) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)H * W) return;
-  const int row = (int)(i / W), col = (int)(i - (size_t)row * W);
-  const float fr = fftfreq(row, H), fc = fftfreq(col, W);
-  kf[i] = dexp(coef * (fr * fr + fc * fc));
-}
-
-__global__ void __launch_bounds__(256) k_fft_cols_filter(int H, int W, int cs,
-                                                         float2* __restrict__ spec,
-                                                         const float2* __restrict__ tw_g,
-                                                         const int* __restrict__ radix_g,
-                                                         int nrad, const float* __restrict__ kft) {
-  extern __shared__ float2 lds[];
-  __shared__ int s_radix[32];
-  // columns are stored with a padded stride so the transposing loads and
-  // stores spread over the LDS banks
-  const int cst = H + 1;
-  float2* a = lds;
-  float2* b = lds + (size_t)cs * cst;
-  float2* tw = lds + (size_t)2 * cs * cst;
-  if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
-  load_twiddles(tw, tw_g, H);
-  // XCD-aware strip order: consecutive workgroups land on different XCDs, so
-  // give the 8 workgroups of one dispatch round strips far apart and the
-  // workgroups of one XCD neighbouring strips (they share cache lines).
-  const int nstrips = (W + cs - 1) / cs;
-  const int g = gridDim.x;
-  int strip = blockIdx.x;
-  if (g % 8 == 0) strip = (blockIdx.x % 8) * (g / 8) + blockIdx.x / 8;
-  const int c0 = strip * cs;
-  const int ncol = min(cs, W - c0);
-  if (c0 >= W) return;
-  // issue all strided loads of this thread before any LDS store (the loads
-  // are independent; a load-store loop would serialise their latencies)
-  {
-    constexpr int U = 16;
-    const int total = ncol * H;
-    for (int i0 = 0; i0 < total; i0 += U * blockDim.x) {
-      float2 v[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = i0 + u * blockDim.x + threadIdx.x;
-        if (i < total) {
-          const int row = i / ncol, col = i - row * ncol;
-          v[u] = spec[(size_t)row * W + c0 + col];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = i0 + u * blockDim.x + threadIdx.x;
-        if (i < total) {
-          const int row = i / ncol, col = i - row * ncol;
-          a[col * cst + row] = v[u];
-        }
-      }
-    }
-  }
-  __syncthreads();
-  float2* r = fft_lds(a, b, H, cst, ncol, s_radix, nrad, tw, false);
-  float2* o = (r == a) ? b : a;
-  for (int i = threadIdx.x; i < ncol * H; i += blockDim.x) {
-    const int col = i / H, row = i - col * H;
-    const float kf = kft[(size_t)row * W + c0 + col];
-    float2 v = r[col * cst + row];
-    v.x = v.x * kf;
-    v.y = v.y * kf;
-    r[col * cst + row] = v;
-  }
-  __syncthreads();
-  float2* res = fft_lds(r, o, H, cst, ncol, s_radix, nrad, tw, true);
-  const float sc = (float)(1.0 / (double)H);
-  for (int i = threadIdx.x; i < ncol * H; i += blockDim.x) {
-    const int row = i / ncol, col = i - row * ncol;
-    float2 v = res[col * cst + row];
-    spec[(size_t)row * W + c0 + col] = make_float2(v.x * sc, v.y * sc);
-  }
-}
-
-__global__ void __launch_bounds__(256) k_ifft_rows_real(int H, int W, const float2* __restrict__ spec,
-                                                        float* __restrict__ out,
-                                                        const float2* __restrict__ tw_g,
-                                                        const int* __restrict__ radix_g, int nrad) {
-  extern __shared__ float2 lds[];
-  __shared__ int s_radix[32];
-  float2* a = lds;
-  float2* b = lds + W;
-  float2* tw = lds + 2 * W;
-  if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
-  load_twiddles(tw, tw_g, W);
-  const int y = blockIdx.x;
-  for (int x = threadIdx.x; x < W; x += blockDim.x) a[x] = spec[(size_t)y * W + x];
-  __syncthreads();
-  float2* r = fft_lds(a, b, W, W, 1, s_radix, nrad, tw, true);
-  const float sc = (float)(1.0 / (double)W);
-  for (int x = threadIdx.x; x < W; x += blockDim.x) out[(size_t)y * W + x] = r[x].x * sc;
-}
-
-// ---------------------------------------------------------------------------
-// Transposed 2-D FFT: the row passes handle TR consecutive rows per workgroup
-// and exchange the spectrum with the column pass in transposed layout
-// specT[k * H + y], in contiguous TR-element segments; the column pass then
-// reads whole contiguous rows of specT.  Same butterflies, same roundings as
-// the untransposed passes.
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ int xcd_group(int b, int G) {
   // workgroup i runs on XCD i % 8: give each XCD a contiguous range of groups
   return (G & 7) == 0 ? (b & 7) * (G >> 3) + (b >> 3) : b;
 }
 
-// CIC fold of row y into dst[0..W) (see k_deposit_rows_fft for the order)
-__device__ void deposit_row(int y, int W, const int* __restrict__ row_off, int* __restrict__ row_list,
-                            int* __restrict__ sort_tmp, unsigned long long* __restrict__ key_tmp,
-                            size_t key_half, const int* __restrict__ cic_cell,
-                            const float* __restrict__ cic_w, float2* dst, int* sort_lds,
-                            unsigned long long* keys_lds, unsigned long long* sort_keys) {
-  const int off = row_off[y];
-  const int m = row_off[y + 1] - off;
+// CIC deposit of global steric row `row` into dst[0..W): each node binned to
+// the row contributes two entries (corners q = 0, 1 if the row is its y0
+// row, q = 2, 3 if its y1 row).  Sorting the entries by (x, q, node rank)
+// puts each cell's contributions contiguously and in the reference's order
+// (all w00 by node, then w10, w01, w11 -- forces.py:112-115); a segment head
+// thread folds them.
+__device__ void deposit_row(int row, int W, const int* __restrict__ row_off,
+                            int* __restrict__ row_list, int* __restrict__ sort_tmp,
+                            unsigned long long* __restrict__ key_tmp, size_t key_half,
+                            const int* __restrict__ cic_cell, const float* __restrict__ cic_w,
+                            float2* dst, int* sort_lds, unsigned long long* keys_lds,
+                            unsigned long long* sort_keys) {
+  const int off = row_off[row];
+  const int m = row_off[row + 1] - off;
   int* nodes = row_list + off;
   sort_unique<int>(nodes, m, sort_lds, DEP_IDS_LDS, sort_tmp + off, 0x7fffffff);
   for (int x = threadIdx.x; x < W; x += blockDim.x) dst[x] = make_float2(0.0f, 0.0f);
@@ -425,7 +239,7 @@ __device__ void deposit_row(int y, int W, const int* __restrict__ row_off, int* 
   for (int j = threadIdx.x; j < m; j += blockDim.x) {
     const int n = nodes[j];
     const unsigned long long x0 = (unsigned)cic_cell[4 * n + 0], x1 = (unsigned)cic_cell[4 * n + 1];
-    const unsigned long long qb = (cic_cell[4 * n + 2] == y) ? 0ull : 2ull;
+    const unsigned long long qb = (cic_cell[4 * n + 2] == row) ? 0ull : 2ull;
     keys[2 * j] = (x0 << 34) | (qb << 32) | (unsigned)j;
     keys[2 * j + 1] = (x1 << 34) | ((qb + 1ull) << 32) | (unsigned)j;
   }
@@ -450,66 +264,81 @@ __device__ void deposit_row(int y, int W, const int* __restrict__ row_off, int* 
   __syncthreads();
 }
 
-// Forward row FFT of TR rows (from the CIC deposit when `real_in` is NULL,
-// else from a real (H, W) array), written transposed to specT.
-template <int TR>
+__device__ __forceinline__ float fftfreq(int k, int n) {
+  // numpy / jnp.fft.fftfreq in float32: k / n for k < (n + 1) / 2, else (k - n) / n
+  const int kk = k < (n + 1) / 2 ? k : k - n;
+  return (float)kk / (float)n;
+}
+
+__device__ __forceinline__ void load_plan(const FftDev& p, int* s_radix) {
+  if (threadIdx.x < p.nrad) s_radix[threadIdx.x] = p.radix[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// 2-D FFT over every instance, exchanging the spectrum between the row and
+// column passes in transposed layout specT[k * H + y] (per instance, at the
+// instance's grid offset): the column pass then reads whole contiguous rows.
+// One global steric row per workgroup in the row passes, one global column per
+// workgroup in the column pass.
+// ---------------------------------------------------------------------------
+
+// Forward row FFT of one row (from the CIC deposit when `real_in` is NULL,
+// else from the real grid), written transposed to specT.
 __global__ void __launch_bounds__(256) k_rows_fwd_T(
-    int H, int W, const float* __restrict__ real_in, const int* __restrict__ row_off,
+    BatchDev bd, const float* __restrict__ real_in, const int* __restrict__ row_off,
     int* __restrict__ row_list, int* __restrict__ sort_tmp, unsigned long long* __restrict__ key_tmp,
     size_t key_half, const int* __restrict__ cic_cell, const float* __restrict__ cic_w,
-    float2* __restrict__ specT, const float2* __restrict__ tw_g, const int* __restrict__ radix_g,
-    int nrad, float* __restrict__ density_out) {
+    float2* __restrict__ specT, float* __restrict__ density_out) {
   extern __shared__ float2 lds[];
   __shared__ int sort_lds[DEP_IDS_LDS];
   __shared__ unsigned long long keys_lds[DEP_KEYS_LDS];
   __shared__ unsigned long long sort_keys[DEP_KEYS_LDS];
-  __shared__ int s_radix[32];
+  __shared__ int s_radix[FFT_MAX_RADICES];
+  const int row = xcd_group(blockIdx.x, gridDim.x);
+  const InstGeo& gi = bd.geo[bd.row_inst[row]];
+  const int W = gi.W, H = gi.H, y = row - gi.row0;
+  const FftDev& fp = bd.fft[gi.fft_w];
   float2* a = lds;
-  float2* b = lds + (size_t)TR * W;
-  float2* tw = lds + (size_t)2 * TR * W;
-  if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
-  load_twiddles(tw, tw_g, W);
-  const int y0 = xcd_group(blockIdx.x, gridDim.x) * TR;
-  const int nr = min(TR, H - y0);
+  float2* b = lds + W;
+  float2* tw = lds + 2 * W;
+  load_plan(fp, s_radix);
+  load_twiddles(tw, bd.tw + fp.tw_off, W);
+  const size_t g0 = (size_t)gi.grid_off + (size_t)y * W;
   if (real_in) {
-    for (int i = threadIdx.x; i < nr * W; i += blockDim.x)
-      a[i] = make_float2(real_in[(size_t)y0 * W + i], 0.0f);
+    for (int i = threadIdx.x; i < W; i += blockDim.x) a[i] = make_float2(real_in[g0 + i], 0.0f);
     __syncthreads();
   } else {
-    for (int r = 0; r < nr; ++r)
-      deposit_row(y0 + r, W, row_off, row_list, sort_tmp, key_tmp, key_half, cic_cell, cic_w,
-                  a + (size_t)r * W, sort_lds, keys_lds, sort_keys);
+    deposit_row(row, W, row_off, row_list, sort_tmp, key_tmp, key_half, cic_cell, cic_w, a,
+                sort_lds, keys_lds, sort_keys);
     if (density_out)
-      for (int i = threadIdx.x; i < nr * W; i += blockDim.x) density_out[(size_t)y0 * W + i] = a[i].x;
+      for (int i = threadIdx.x; i < W; i += blockDim.x) density_out[g0 + i] = a[i].x;
   }
-  float2* res = fft_lds(a, b, W, W, nr, s_radix, nrad, tw, false);
-  // transposed store: consecutive threads write consecutive rows of one column
-  for (int i = threadIdx.x; i < nr * W; i += blockDim.x) {
-    const int k = i / nr, r = i - k * nr;
-    specT[(size_t)k * H + y0 + r] = res[(size_t)r * W + k];
-  }
+  float2* res = fft_lds(a, b, W, W, 1, s_radix, fp.nrad, tw, false);
+  float2* sp = specT + gi.grid_off;
+  for (int k = threadIdx.x; k < W; k += blockDim.x) sp[(size_t)k * H + y] = res[k];
 }
 
-// Column pass on the transposed spectrum: one column k per workgroup, forward
-// FFT along y, transfer function, inverse FFT scaled by 1/H.
-__global__ void __launch_bounds__(256) k_cols_T(int H, int W, float2* __restrict__ specT,
-                                                const float2* __restrict__ tw_g,
-                                                const int* __restrict__ radix_g, int nrad,
+// Column pass on the transposed spectrum: forward FFT along y, transfer
+// function, inverse FFT scaled by 1/H.
+__global__ void __launch_bounds__(256) k_cols_T(BatchDev bd, float2* __restrict__ specT,
                                                 const float* __restrict__ kfT) {
   extern __shared__ float2 lds[];
-  __shared__ int s_radix[32];
+  __shared__ int s_radix[FFT_MAX_RADICES];
+  const int colg = blockIdx.x;
+  const InstGeo& gi = bd.geo[bd.col_inst[colg]];
+  const int H = gi.H, k = colg - gi.col0;
+  const FftDev& fp = bd.fft[gi.fft_h];
   float2* a = lds;
   float2* b = lds + H;
   float2* tw = lds + 2 * H;
-  if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
-  load_twiddles(tw, tw_g, H);
-  const int k = blockIdx.x;
-  float2* col = specT + (size_t)k * H;
+  load_plan(fp, s_radix);
+  load_twiddles(tw, bd.tw + fp.tw_off, H);
+  float2* col = specT + gi.grid_off + (size_t)k * H;
   for (int y = threadIdx.x; y < H; y += blockDim.x) a[y] = col[y];
   __syncthreads();
-  float2* r = fft_lds(a, b, H, H, 1, s_radix, nrad, tw, false);
+  float2* r = fft_lds(a, b, H, H, 1, s_radix, fp.nrad, tw, false);
   float2* o = (r == a) ? b : a;
-  const float* kc = kfT + (size_t)k * H;
+  const float* kc = kfT + gi.kf_off + (size_t)k * H;
   for (int y = threadIdx.x; y < H; y += blockDim.x) {
     const float kf = kc[y];
     float2 v = r[y];
@@ -518,7 +347,7 @@ __global__ void __launch_bounds__(256) k_cols_T(int H, int W, float2* __restrict
     r[y] = v;
   }
   __syncthreads();
-  float2* res = fft_lds(r, o, H, H, 1, s_radix, nrad, tw, true);
+  float2* res = fft_lds(r, o, H, H, 1, s_radix, fp.nrad, tw, true);
   const float sc = (float)(1.0 / (double)H);
   for (int y = threadIdx.x; y < H; y += blockDim.x) {
     const float2 v = res[y];
@@ -526,29 +355,27 @@ __global__ void __launch_bounds__(256) k_cols_T(int H, int W, float2* __restrict
   }
 }
 
-// Inverse row FFT of TR rows read transposed from specT; real part scaled by 1/W.
-template <int TR>
-__global__ void __launch_bounds__(256) k_rows_inv_T(int H, int W, const float2* __restrict__ specT,
-                                                    float* __restrict__ out,
-                                                    const float2* __restrict__ tw_g,
-                                                    const int* __restrict__ radix_g, int nrad) {
+// Inverse row FFT of one row read transposed from specT; real part scaled by 1/W.
+__global__ void __launch_bounds__(256) k_rows_inv_T(BatchDev bd, const float2* __restrict__ specT,
+                                                    float* __restrict__ out) {
   extern __shared__ float2 lds[];
-  __shared__ int s_radix[32];
+  __shared__ int s_radix[FFT_MAX_RADICES];
+  const int row = xcd_group(blockIdx.x, gridDim.x);
+  const InstGeo& gi = bd.geo[bd.row_inst[row]];
+  const int W = gi.W, H = gi.H, y = row - gi.row0;
+  const FftDev& fp = bd.fft[gi.fft_w];
   float2* a = lds;
-  float2* b = lds + (size_t)TR * W;
-  float2* tw = lds + (size_t)2 * TR * W;
-  if (threadIdx.x < nrad) s_radix[threadIdx.x] = radix_g[threadIdx.x];
-  load_twiddles(tw, tw_g, W);
-  const int y0 = xcd_group(blockIdx.x, gridDim.x) * TR;
-  const int nr = min(TR, H - y0);
-  for (int i = threadIdx.x; i < nr * W; i += blockDim.x) {
-    const int k = i / nr, r = i - k * nr;
-    a[(size_t)r * W + k] = specT[(size_t)k * H + y0 + r];
-  }
+  float2* b = lds + W;
+  float2* tw = lds + 2 * W;
+  load_plan(fp, s_radix);
+  load_twiddles(tw, bd.tw + fp.tw_off, W);
+  const float2* sp = specT + gi.grid_off;
+  for (int k = threadIdx.x; k < W; k += blockDim.x) a[k] = sp[(size_t)k * H + y];
   __syncthreads();
-  float2* res = fft_lds(a, b, W, W, nr, s_radix, nrad, tw, true);
+  float2* res = fft_lds(a, b, W, W, 1, s_radix, fp.nrad, tw, true);
   const float sc = (float)(1.0 / (double)W);
-  for (int i = threadIdx.x; i < nr * W; i += blockDim.x) out[(size_t)y0 * W + i] = res[i].x * sc;
+  float* o = out + gi.grid_off + (size_t)y * W;
+  for (int i = threadIdx.x; i < W; i += blockDim.x) o[i] = res[i].x * sc;
 }
 
 // Transposed transfer function kfT[k * H + y] = exp(coef (fr(y)^2 + fc(k)^2)).
@@ -562,110 +389,46 @@ __global__ void k_gauss_table_T(int H, int W, float coef, float* __restrict__ kf
 
 }  // namespace
 
-hipError_t launch_predict(const Consts& c, int N, const float* pos, const float* vel, float* pred_w,
-                          int* cic_cell, float* cic_w, int* row_cnt, hipStream_t s) {
+hipError_t launch_predict(const Consts& c, const BatchDev& b, int N, const float* pos,
+                          const float* vel, float* pred_w, int* cic_cell, float* cic_w,
+                          int* row_cnt, hipStream_t s) {
   if (N == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_predict, dim3((N + 255) / 256), dim3(256), 0, s, c, N, pos, vel, pred_w,
+  hipLaunchKernelGGL(k_predict, dim3((N + 255) / 256), dim3(256), 0, s, c, b, N, pos, vel, pred_w,
                      cic_cell, cic_w, row_cnt);
   return hipGetLastError();
 }
 
 hipError_t launch_row_fill(const Consts& c, int N, const int* cic_cell, const int* row_off,
                            int* row_fill, int* row_list, hipStream_t s) {
+  (void)c;
   if (N == 0) return hipSuccess;
   hipLaunchKernelGGL(k_row_fill, dim3((N + 255) / 256), dim3(256), 0, s, N, cic_cell, row_off,
                      row_fill, row_list);
   return hipGetLastError();
 }
 
-hipError_t launch_deposit_rows_fft(const Consts& c, int N, const int* row_off, int* row_list,
-                                   const int* cic_cell, const float* cic_w, float2* spec,
-                                   const float2* tw_w, const int* radix_w, int nrad_w,
-                                   float* density_out, int* sort_scratch,
-                                   unsigned long long* key_scratch, hipStream_t s) {
-  const int H = c.dom_h, W = c.dom_w;
-  const size_t lds = (size_t)3 * W * sizeof(float2);
-  hipLaunchKernelGGL(k_deposit_rows_fft, dim3(H), dim3(256), lds, s, H, W, row_off, row_list,
-                     sort_scratch, key_scratch, (size_t)4 * N, cic_cell, cic_w, spec, tw_w, radix_w,
-                     nrad_w, density_out);
-  return hipGetLastError();
-}
-
-hipError_t launch_fft_rows(int H, int W, const float* real_in, float2* spec, const float2* tw_w,
-                           const int* radix_w, int nrad_w, hipStream_t s) {
-  const size_t lds = (size_t)3 * W * sizeof(float2);
-  hipLaunchKernelGGL(k_fft_rows, dim3(H), dim3(256), lds, s, H, W, real_in, spec, tw_w, radix_w,
-                     nrad_w);
-  return hipGetLastError();
-}
-
-hipError_t launch_gauss_table(int H, int W, float coef, float* kf, hipStream_t s) {
+hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t s) {
   const size_t n = (size_t)H * W;
   hipLaunchKernelGGL(k_gauss_table_T, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, W, coef,
-                     kf);
-  return hipGetLastError();
-}
-
-int fft_rows_per_group(int H, int W) {
-  // One row per workgroup: the FFT passes are latency-bound, so parallelism
-  // across rows beats longer transposed segments.  The XCD-grouped mapping
-  // keeps the rows that share a specT cache line on one XCD, whose L2 merges
-  // the 8-byte transposed writes before they reach HBM.
-  (void)H;
-  (void)W;
-  return 1;
-}
-
-hipError_t launch_steric_fft_T(int H, int W, const float* real_in, const int* row_off, int* row_list,
-                               int* sort_scratch, unsigned long long* key_scratch, size_t key_half,
-                               const int* cic_cell, const float* cic_w, float2* specT,
-                               const float2* tw_w, const int* radix_w, int nrad_w,
-                               const float2* tw_h, const int* radix_h, int nrad_h, const float* kfT,
-                               float* density_out, float* out, hipStream_t s) {
-  const int TR = fft_rows_per_group(H, W);
-  const int G = (H + TR - 1) / TR;
-  const size_t lds_rows = ((size_t)2 * TR * W + W) * sizeof(float2);
-  const size_t lds_cols = (size_t)3 * H * sizeof(float2);
-#define MC_ROWS(T)                                                                                  \
-  case T:                                                                                           \
-    hipLaunchKernelGGL(k_rows_fwd_T<T>, dim3(G), dim3(256), lds_rows, s, H, W, real_in, row_off,   \
-                       row_list, sort_scratch, key_scratch, key_half, cic_cell, cic_w, specT, tw_w, \
-                       radix_w, nrad_w, density_out);                                               \
-    break;
-  switch (TR) { MC_ROWS(4) MC_ROWS(2) default: MC_ROWS(1) }
-#undef MC_ROWS
-  hipLaunchKernelGGL(k_cols_T, dim3(W), dim3(256), lds_cols, s, H, W, specT, tw_h, radix_h, nrad_h,
                      kfT);
-#define MC_IROWS(T)                                                                                 \
-  case T:                                                                                           \
-    hipLaunchKernelGGL(k_rows_inv_T<T>, dim3(G), dim3(256), lds_rows, s, H, W, specT, out, tw_w,   \
-                       radix_w, nrad_w);                                                            \
-    break;
-  switch (TR) { MC_IROWS(4) MC_IROWS(2) default: MC_IROWS(1) }
-#undef MC_IROWS
   return hipGetLastError();
 }
 
-hipError_t launch_fft_cols_filter(int H, int W, float2* spec, const float2* tw_h,
-                                  const int* radix_h, int nrad_h, const float* kf, int lds_bytes_max,
-                                  hipStream_t s) {
-  // widest strip that fits, but keep at least one workgroup per CU
-  int cs = 16;
-  while (cs > 1 && (((size_t)2 * cs * (H + 1) + H) * sizeof(float2) > (size_t)lds_bytes_max ||
-                    (W + cs - 1) / cs < 256))
-    cs >>= 1;
-  const size_t lds = ((size_t)2 * cs * (H + 1) + H) * sizeof(float2);
-  const int nstrips = (W + cs - 1) / cs;
-  hipLaunchKernelGGL(k_fft_cols_filter, dim3(nstrips), dim3(256), lds, s, H, W, cs, spec, tw_h,
-                     radix_h, nrad_h, kf);
-  return hipGetLastError();
-}
+size_t fft_rows_lds_bytes(int max_w) { return (size_t)3 * max_w * sizeof(float2); }
+size_t fft_cols_lds_bytes(int max_h) { return (size_t)3 * max_h * sizeof(float2); }
 
-hipError_t launch_ifft_rows_real(int H, int W, const float2* spec, float* out, const float2* tw_w,
-                                 const int* radix_w, int nrad_w, hipStream_t s) {
-  const size_t lds = (size_t)3 * W * sizeof(float2);
-  hipLaunchKernelGGL(k_ifft_rows_real, dim3(H), dim3(256), lds, s, H, W, spec, out, tw_w, radix_w,
-                     nrad_w);
+hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const int* row_off,
+                               int* row_list, int* sort_scratch, unsigned long long* key_scratch,
+                               size_t key_half, const int* cic_cell, const float* cic_w,
+                               float2* specT, const float* kfT, float* density_out, float* out,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_rows_fwd_T, dim3(b.rows), dim3(256), fft_rows_lds_bytes(b.max_w), s, b,
+                     real_in, row_off, row_list, sort_scratch, key_scratch, key_half, cic_cell,
+                     cic_w, specT, density_out);
+  hipLaunchKernelGGL(k_cols_T, dim3(b.cols), dim3(256), fft_cols_lds_bytes(b.max_h), s, b, specT,
+                     kfT);
+  hipLaunchKernelGGL(k_rows_inv_T, dim3(b.rows), dim3(256), fft_rows_lds_bytes(b.max_w), s, b,
+                     specT, out);
   return hipGetLastError();
 }
 

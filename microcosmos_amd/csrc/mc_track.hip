@@ -75,13 +75,15 @@ __global__ void __launch_bounds__(COM_THREADS) k_com(const float* __restrict__ p
 // start_step, adds the minimum-image step displacement of the centre to
 // cumulative; prev = centre.
 __global__ void __launch_bounds__(COM_THREADS) k_track(const float* __restrict__ pos,
-                                                       const int* __restrict__ off, float W,
-                                                       float H, int mode, int start_step,
+                                                       const int* __restrict__ off,
+                                                       const int2* __restrict__ dims, int mode,
+                                                       int start_step,
                                                        float* __restrict__ prev,
                                                        float* __restrict__ cum,
                                                        long long* __restrict__ count) {
   __shared__ float sx[COM_THREADS], sy[COM_THREADS];
   const int g = blockIdx.x;
+  const float H = (float)dims[g].x, W = (float)dims[g].y;
   float cx, cy;
   group_com(pos, off[g], off[g + 1], W, H, 1, sx, sy, cx, cy);
   if (threadIdx.x != 0) return;
@@ -111,11 +113,10 @@ hipError_t launch_com(int n_groups, const int* off, const float* pos, int H, int
   return hipGetLastError();
 }
 
-hipError_t launch_track(const TrackWork& t, const float* pos, int H, int W, int mode,
-                        hipStream_t s) {
+hipError_t launch_track(const TrackWork& t, const float* pos, int mode, hipStream_t s) {
   if (t.n_groups <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_track, dim3(t.n_groups), dim3(COM_THREADS), 0, s, pos, t.off, (float)W,
-                     (float)H, mode, t.start_step, t.prev, t.cum, t.count);
+  hipLaunchKernelGGL(k_track, dim3(t.n_groups), dim3(COM_THREADS), 0, s, pos, t.off, t.dims, mode,
+                     t.start_step, t.prev, t.cum, t.count);
   return hipGetLastError();
 }
 

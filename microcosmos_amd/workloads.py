@@ -82,12 +82,14 @@ def _min_image(grid_shape, a, b):
 
 def filaments_workload(name, grid, n_fil, n_nodes, seed, *, straight_center=False,
                        jitter=0.05, spacing=1.0, amp=0.3, period=200.0, kb=0.5,
-                       u0_amp=1e-3, steps=1000, overrides=None, node_counts=None):
+                       u0_amp=1e-3, steps=1000, overrides=None, node_counts=None,
+                       amps=None, periods=None):
     """``n_fil`` line filaments of ``n_nodes`` nodes on a periodic ``grid``.
 
     straight_center: one filament centred horizontally (the 128^2 plain case);
     otherwise start points are uniform on the grid, headings uniform in
-    [0, 2 pi), actuation phases uniform in [0, 2 pi).
+    [0, 2 pi), actuation phases uniform in [0, 2 pi).  amps / periods: per
+    filament actuation amplitude and period (default: amp, period for all).
     """
     rng = np.random.default_rng(seed)
     h, w = grid
@@ -121,8 +123,10 @@ def filaments_workload(name, grid, n_fil, n_nodes, seed, *, straight_center=Fals
     for f, n in enumerate(counts):
         nb = max(n - 2, 0)
         b = np.arange(nb, dtype=np.float64)
-        amp_b.append(np.full(nb, amp))
-        om_b.append(np.full(nb, 2 * np.pi / period))
+        a_f = amp if amps is None else float(amps[f])
+        t_f = period if periods is None else float(periods[f])
+        amp_b.append(np.full(nb, a_f))
+        om_b.append(np.full(nb, 2 * np.pi / t_f))
         ph_b.append(phases[f] - 2 * np.pi * b / max(nb, 1))
         start_b += nb
     act_amp = np.concatenate(amp_b).astype(np.float32)
@@ -169,3 +173,25 @@ def small(grid=(48, 48), n_fil=2, n_nodes=16, seed=7, cycles=4, tau_plus=0.8, st
     return filaments_workload(f"small-{grid[0]}x{grid[1]}-{n_fil}x{n_nodes}", grid, n_fil,
                               n_nodes, seed, straight_center=straight_center, steps=steps,
                               overrides=_lbm_overrides(tau_plus, cycles), **kw)
+
+
+def es_population(n=512, seed=2, sides=(128, 192, 256), max_filaments=16, min_nodes=16,
+                  max_nodes=64, steps=500):
+    """Batched ES population: ``n`` independent instances, each with a grid
+    side drawn from ``sides``, 1..max_filaments filaments of
+    min_nodes..max_nodes nodes, per-filament actuation amplitude U(0, 0.5) and
+    period U(100, 400); fluid, IB and steric settings of the 128^2 plain case
+    (tau+ 0.8, Lambda 3/16, K = 4, 4 PBD cycles, sigma 2)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        side = int(rng.choice(sides))
+        nf = int(rng.integers(1, max_filaments + 1))
+        counts = rng.integers(min_nodes, max_nodes + 1, size=nf).tolist()
+        amps = rng.uniform(0.0, 0.5, size=nf)
+        periods = rng.uniform(100.0, 400.0, size=nf)
+        out.append(filaments_workload(
+            f"es-{i}-{side}x{side}-{nf}f", (side, side), nf, 0, int(rng.integers(1 << 30)),
+            node_counts=counts, amps=amps, periods=periods, steps=steps,
+            overrides=_lbm_overrides(0.8, 4)))
+    return out
