@@ -6,8 +6,11 @@ TRT + Guo collide-stream) over the device-resident 1024^2 lattice, replayed
 from a HIP graph.  Inputs are resident in HBM before the timed region.
 
 Prints one JSON line.  Also reports the fused LBM kernel's roofline fraction
-(algorithmic bytes / HIP-event duration) and the CPU restatement's rate on a
-bounded sample of the same workload.
+(algorithmic bytes / HIP-event duration), the CPU restatement's rate on a
+bounded sample of the same workload, and the batched ES evaluation rate: a
+seeded population of ragged instances (grid sides 128/192/256, 1-16
+filaments of 16-64 nodes) rolled out together by mc_batch_rollout, against
+the same rollouts run one instance at a time on the plain GPU path.
 """
 from __future__ import annotations
 
@@ -44,6 +47,60 @@ def cpu_baseline(wl, steps):
     return h * w * steps / dt / 1e6, dt
 
 
+def batched_es(n_inst, steps, plain_sample, dev):
+    """Evaluations per second of a batched population vs the plain loop."""
+    import torch
+    from microcosmos_amd import PBD_SCHEME, Rollout
+    from microcosmos_amd import workloads as W
+    from microcosmos_amd.batch import Batch
+    from tests.helpers import device_state
+    pop = W.es_population(n_inst, seed=2, steps=steps)
+    cfg = replace(PBD_SCHEME, **pop[0].config_overrides)
+
+    def act(w):
+        return tuple(torch.as_tensor(a, device=dev) for a in (w.act_amp, w.act_omega, w.act_phase))
+
+    b = Batch([device_state(w, dev) for w in pop], cfg, pop[0].dt,
+              actuation=[act(w) for w in pop])
+    b.run(16)
+    torch.cuda.synchronize()
+    print(f"[bench] batched warmup done ({n_inst} instances)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    disp = b.run(steps)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    finite = bool(torch.isfinite(disp).all().item())
+    cells = sum(w.grid_shape[0] * w.grid_shape[1] for w in pop)
+    # the plain per-instance loop on a sample of the same instances
+    sample = pop[:plain_sample]
+    for w in sample[:1]:  # warm the single-instance path
+        n, e, f = device_state(w, dev)
+        Rollout(n, e, f, cfg, w.dt, actuation=(w.act_amp, w.act_omega, w.act_phase)).run(16)
+    torch.cuda.synchronize()
+    ros = []
+    for w in sample:
+        n, e, f = device_state(w, dev)
+        ro = Rollout(n, e, f, cfg, w.dt, actuation=(w.act_amp, w.act_omega, w.act_phase))
+        ro.track()
+        ros.append(ro)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ro in ros:
+        ro.run(steps)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    print(f"[bench] batched {tb:.3f} s, plain sample {tp:.3f} s", file=sys.stderr, flush=True)
+    ev_b = n_inst / tb
+    ev_p = len(sample) / tp
+    return {"workload": f"es-population-{n_inst}", "instances": n_inst, "rollout_steps": steps,
+            "lattice_cells": cells, "evals_per_s": round(ev_b, 2),
+            "instance_steps_per_s": round(n_inst * steps / tb, 1),
+            "mlups": round(cells * steps / tb / 1e6, 1),
+            "plain_loop": {"sample_instances": len(sample), "evals_per_s": round(ev_p, 2)},
+            "vs_plain_loop": round(ev_b / ev_p, 2), "finite": finite,
+            "p99_latency": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +110,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 9 x 1024 x 1024 float32 device copy (37.7 MB read + "
                          "37.7 MB written per launch) to calibrate HBM byte counters")
@@ -69,7 +130,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "
-                                   "part of it (instances shard by whole lattice, see DESIGN.md)",
+                                   "part of it (instances would shard by whole lattice)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
@@ -106,6 +167,9 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_lbm", "avg_kernel_ms": round(lbm_ms, 5),
                 "bytes_per_launch": lbm_bytes}
+    batched = None
+    if not args.no_batch:
+        batched = batched_es(args.batch_instances, args.batch_steps, args.plain_sample, dev)
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
@@ -122,7 +186,7 @@ def main():
                       "nodes_per_filament": 64, "pbd_cycles": 8, "ibm_kernel_size": 4,
                       "ibm_iterations": 5, "steric_sigma": 2.0, "tau_plus": 0.6,
                       "lambda_trt": 0.1875, "hip_graph": use_graph, "finite": finite},
-           "roofline": roofline, "cpu_baseline": cpu,
+           "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
            "coupled_steps_per_s": round(args.steps / elapsed, 2),
            "step_ms_events": round(step_ms_ev, 5)}
     print(json.dumps(out), flush=True)

@@ -294,6 +294,10 @@ mc_status build_instances(mc_plan* p) {
     g.node0 = h.node0;
     g.nodes = h.nodes;
     const long long hwf = (long long)h.Hf * h.Wf;
+    // floor(c / Wf) == (c * ceil(2^40 / Wf)) >> 40 for every c < hwf when hwf * Wf < 2^40
+    g.wdiv = (unsigned long long)hwf * (unsigned long long)h.Wf < (1ull << 40)
+                 ? ((1ull << 40) + (unsigned long long)h.Wf - 1) / (unsigned long long)h.Wf
+                 : 0ull;
     go += (long long)h.H * h.W;
     fo += hwf;
     mo += (hwf + 31) / 32;

@@ -51,6 +51,7 @@ struct InstGeo {
   int node0, nodes;           // node range
   int fft_w, fft_h;           // FFT plan of length W / of length H
   long long kf_off;           // transposed transfer-function table of the (H, W) shape
+  unsigned long long wdiv;    // ceil(2^40 / Wf): y = (cell * wdiv) >> 40 (0: divide)
 };
 
 // Stockham plan of one FFT length: radix schedule and twiddle table offset.

@@ -62,7 +62,9 @@ __global__ void __launch_bounds__(LBM_THREADS) k_lbm(BatchDev bd, const float* _
   // LBM_THREADS consecutive cells of the instance's lattice per workgroup
   const int ci = (b - g.lbm_blk0) * LBM_THREADS + threadIdx.x;
   if (ci >= (int)HW) return;
-  const int y = ci / W;
+  // row of the cell: exact reciprocal multiply when cells * Wf < 2^40 (set on
+  // the host), else a division
+  const int y = g.wdiv ? (int)(((unsigned long long)ci * g.wdiv) >> 40) : ci / W;
   const int x = ci - y * W;
   const float umax = g.umax;
   const float* fin = fin_all + 9 * g.fl_off;
