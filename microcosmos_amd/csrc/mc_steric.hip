@@ -219,10 +219,13 @@ __device__ __forceinline__ int xcd_group(int b, int G) {
 
 // CIC deposit of global steric row `row` into dst[0..W): each node binned to
 // the row contributes two entries (corners q = 0, 1 if the row is its y0
-// row, q = 2, 3 if its y1 row).  Sorting the entries by (x, q, node rank)
+// row, q = 2, 3 if its y1 row).  Ordering the entries by (x, q, node rank)
 // puts each cell's contributions contiguously and in the reference's order
 // (all w00 by node, then w10, w01, w11 -- forces.py:112-115); a segment head
-// thread folds them.
+// thread folds them.  Rows with at most DEP_FAST nodes order node ids and
+// entry keys by counting ranks in LDS (keys are unique); longer rows sort.
+constexpr int DEP_FAST = 256;
+
 __device__ void deposit_row(int row, int W, const int* __restrict__ row_off,
                             int* __restrict__ row_list, int* __restrict__ sort_tmp,
                             unsigned long long* __restrict__ key_tmp, size_t key_half,
@@ -231,22 +234,58 @@ __device__ void deposit_row(int row, int W, const int* __restrict__ row_off,
                             unsigned long long* sort_keys) {
   const int off = row_off[row];
   const int m = row_off[row + 1] - off;
-  int* nodes = row_list + off;
-  sort_unique<int>(nodes, m, sort_lds, DEP_IDS_LDS, sort_tmp + off, 0x7fffffff);
-  for (int x = threadIdx.x; x < W; x += blockDim.x) dst[x] = make_float2(0.0f, 0.0f);
   const int ne = 2 * m;
-  unsigned long long* keys = (ne <= DEP_KEYS_LDS) ? keys_lds : (key_tmp + 2 * (size_t)off);
-  for (int j = threadIdx.x; j < m; j += blockDim.x) {
-    const int n = nodes[j];
-    const unsigned long long x0 = (unsigned)cic_cell[4 * n + 0], x1 = (unsigned)cic_cell[4 * n + 1];
-    const unsigned long long qb = (cic_cell[4 * n + 2] == row) ? 0ull : 2ull;
-    keys[2 * j] = (x0 << 34) | (qb << 32) | (unsigned)j;
-    keys[2 * j + 1] = (x1 << 34) | ((qb + 1ull) << 32) | (unsigned)j;
+  for (int x = threadIdx.x; x < W; x += blockDim.x) dst[x] = make_float2(0.0f, 0.0f);
+  const int* nodes;
+  const unsigned long long* keys;
+  if (m <= DEP_FAST) {
+    // ids into LDS, ranks by counting, keys in rank order, key ranks by counting
+    int* ids = sort_lds;               // [0, m): arrival order
+    int* sorted = sort_lds + DEP_FAST;  // [0, m): ascending
+    for (int j = threadIdx.x; j < m; j += blockDim.x) ids[j] = row_list[off + j];
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+      const int v = ids[j];
+      int r = 0;
+      for (int k = 0; k < m; ++k) r += ids[k] < v ? 1 : 0;
+      sorted[r] = v;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+      const int n = sorted[j];
+      const unsigned long long x0 = (unsigned)cic_cell[4 * n + 0], x1 = (unsigned)cic_cell[4 * n + 1];
+      const unsigned long long qb = (cic_cell[4 * n + 2] == row) ? 0ull : 2ull;
+      sort_keys[2 * j] = (x0 << 34) | (qb << 32) | (unsigned)j;
+      sort_keys[2 * j + 1] = (x1 << 34) | ((qb + 1ull) << 32) | (unsigned)j;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+      const unsigned long long v = sort_keys[i];
+      int r = 0;
+      for (int k = 0; k < ne; ++k) r += sort_keys[k] < v ? 1 : 0;
+      keys_lds[r] = v;
+    }
+    __syncthreads();
+    nodes = sorted;
+    keys = keys_lds;
+  } else {
+    int* nl = row_list + off;
+    sort_unique<int>(nl, m, sort_lds, DEP_IDS_LDS, sort_tmp + off, 0x7fffffff);
+    unsigned long long* kk = (ne <= DEP_KEYS_LDS) ? keys_lds : (key_tmp + 2 * (size_t)off);
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+      const int n = nl[j];
+      const unsigned long long x0 = (unsigned)cic_cell[4 * n + 0], x1 = (unsigned)cic_cell[4 * n + 1];
+      const unsigned long long qb = (cic_cell[4 * n + 2] == row) ? 0ull : 2ull;
+      kk[2 * j] = (x0 << 34) | (qb << 32) | (unsigned)j;
+      kk[2 * j + 1] = (x1 << 34) | ((qb + 1ull) << 32) | (unsigned)j;
+    }
+    __threadfence_block();
+    __syncthreads();
+    sort_unique<unsigned long long>(kk, ne, sort_keys, DEP_KEYS_LDS,
+                                    key_tmp + key_half + 2 * (size_t)off, ~0ull);
+    nodes = nl;
+    keys = kk;
   }
-  __threadfence_block();
-  __syncthreads();
-  sort_unique<unsigned long long>(keys, ne, sort_keys, DEP_KEYS_LDS, key_tmp + key_half + 2 * (size_t)off,
-                                  ~0ull);
   for (int i = threadIdx.x; i < ne; i += blockDim.x) {
     const unsigned long long k = keys[i];
     const int x = (int)(k >> 34);
