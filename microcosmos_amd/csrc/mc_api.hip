@@ -675,6 +675,8 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
   if ((st = p->alloc(&w.st_sy, (size_t)N, "st sy")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.st_w, taps * N, "st w")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.st_won, (size_t)N, "st won")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.st_rho, taps * N, "st rho")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.st_u, taps * N, "st u")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.vib, 2 * (size_t)N, "vib")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.owner, hwf, "cell owner")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.cell_seg, hwf, "cell seg")) != MC_OK) return bail(st);

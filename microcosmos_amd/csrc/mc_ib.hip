@@ -72,61 +72,89 @@ __device__ __forceinline__ void uf_union(int* p, int a, int b) {
   }
 }
 
+// Lanes per node in the stencil kernel: one lane per tap, the node's group a
+// power of two within a wavefront.
 template <int K>
-__global__ void __launch_bounds__(64) k_ib_stencil(
+constexpr int stencil_lanes() {
+  return K * K <= 1 ? 1 : (K * K <= 4 ? 4 : (K * K <= 16 ? 16 : (K * K <= 32 ? 32 : 64)));
+}
+
+// One lane per (node, tap).  Every lane of a node group evaluates the node's
+// geometry; lane t owns tap t (dx = t / K, dy = t % K): its weight, the claim
+// of its cell, a link when another component holds the cell, and -- when the
+// claim wins -- rho and the clamped u of the cell from f_in
+// (fluid.py:268-276), which the cluster solve takes as the cell's state.
+template <int K>
+__global__ void __launch_bounds__(256) k_ib_stencil(
     Consts c, BatchDev bd, int N, const float* __restrict__ pos, const float* __restrict__ vel,
-    const int* __restrict__ comp_of_node, int* __restrict__ st_sx, int* __restrict__ st_sy,
-    float* __restrict__ st_w, unsigned long long* __restrict__ st_won, float* __restrict__ vib,
-    int* __restrict__ owner, int2* __restrict__ links, int* __restrict__ nlinks) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+    const float* __restrict__ f_all, const int* __restrict__ comp_of_node,
+    int* __restrict__ st_sx, int* __restrict__ st_sy, float* __restrict__ st_w,
+    unsigned long long* __restrict__ st_won, float* __restrict__ st_rho,
+    float2* __restrict__ st_u, float* __restrict__ vib, int* __restrict__ owner,
+    int2* __restrict__ links, int* __restrict__ nlinks) {
   constexpr int K2 = K * K;
-  const InstGeo& gi = bd.geo[bd.node_inst[n]];
-  const int W = gi.Wf, H = gi.Hf;
-  const float gx = pos[2 * n] * gi.scale_x;
-  const float gy = pos[2 * n + 1] * gi.scale_y;
-  vib[2 * n] = (vel[2 * n] * gi.scale_x) * c.dt;
-  vib[2 * n + 1] = (vel[2 * n + 1] * gi.scale_y) * c.dt;
-  const float sxf = floorf(gx) - (float)c.half;
-  const float syf = floorf(gy) - (float)c.half;
-  const int sx = imod(f2i(sxf), W);
-  const int sy = imod(f2i(syf), H);
-  const int comp = comp_of_node[n];
-  // claims first, so their latency overlaps the weight arithmetic
-  int prev[K2];
-#pragma unroll
-  for (int t = 0; t < K2; ++t)
-    prev[t] = atomicCAS(&owner[gi.fl_off + wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W)], -1,
-                        comp);
-  // 1-D weights: tap t uses wx[t / K] and wy[t % K] (fluid.py:53-69)
-  float wx[K], wy[K];
-#pragma unroll
-  for (int d = 0; d < K; ++d) {
-    wx[d] = peskin((sxf + (float)d) - gx, c.half_width);
-    wy[d] = peskin((syf + (float)d) - gy, c.half_width);
-  }
-  float total = 0.0f;
-#pragma unroll
-  for (int t = 0; t < K2; ++t) total = total + wx[t / K] * wy[t % K];
-  const float den = total + MC_EPSIL;
-#pragma unroll
-  for (int t = 0; t < K2; ++t) st_w[(size_t)n * K2 + t] = (wx[t / K] * wy[t % K]) / den;
-  st_sx[n] = sx;
-  st_sy[n] = sy;
-  unsigned long long won = 0;
-  int last = -1;
-#pragma unroll
-  for (int t = 0; t < K2; ++t) {
-    const int q = prev[t];
-    if (q == -1) won |= 1ull << t;
-    // a cell claimed by another component links the two (consecutive
-    // repeats dropped; at most one link per tap)
-    if (q != -1 && q != comp && q != last) {
-      last = q;
-      links[atomicAdd(nlinks, 1)] = make_int2(q, comp);
+  constexpr int L = stencil_lanes<K>();
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = gid / L, t = gid - n * L;
+  const int lane = threadIdx.x & 63, base = lane - t;  // the group's first lane
+  const bool node_ok = n < N;
+  const bool tap = node_ok && t < K2;
+  float prod = 0.0f;
+  int prev = -1, comp = -1;
+  int g = 0;
+  float sxf = 0.0f, syf = 0.0f, gx = 0.0f, gy = 0.0f;
+  const InstGeo* gi = nullptr;
+  if (node_ok) {
+    gi = &bd.geo[bd.node_inst[n]];
+    const int W = gi->Wf, H = gi->Hf;
+    gx = pos[2 * n] * gi->scale_x;
+    gy = pos[2 * n + 1] * gi->scale_y;
+    sxf = floorf(gx) - (float)c.half;
+    syf = floorf(gy) - (float)c.half;
+    const int sx = imod(f2i(sxf), W);
+    const int sy = imod(f2i(syf), H);
+    comp = comp_of_node[n];
+    if (tap) {
+      const int dx = t / K, dy = t - dx * K;
+      g = wrap1(sy + dy, H) * W + wrap1(sx + dx, W);
+      prev = atomicCAS(&owner[gi->fl_off + g], -1, comp);
+      // 1-D weights: tap t uses wx[t / K] and wy[t % K] (fluid.py:53-69)
+      const float wxv = peskin((sxf + (float)dx) - gx, c.half_width);
+      const float wyv = peskin((syf + (float)dy) - gy, c.half_width);
+      prod = wxv * wyv;
+    }
+    if (t == 0) {
+      st_sx[n] = sx;
+      st_sy[n] = sy;
+      vib[2 * n] = (vel[2 * n] * gi->scale_x) * c.dt;
+      vib[2 * n + 1] = (vel[2 * n + 1] * gi->scale_y) * c.dt;
     }
   }
-  st_won[n] = won;
+  // the normalising sum over taps 0..K2-1 in order (every lane of the group)
+  float total = 0.0f;
+#pragma unroll
+  for (int k = 0; k < K2; ++k) total = total + __shfl(prod, base + k, 64);
+  const unsigned long long won_lanes = __ballot(tap && prev == -1);
+  const int prev_left = __shfl(prev, t > 0 ? lane - 1 : lane, 64);
+  if (!tap) return;
+  const float den = total + MC_EPSIL;
+  st_w[(size_t)n * K2 + t] = prod / den;
+  if (t == 0) st_won[n] = (won_lanes >> base) & (K2 == 64 ? ~0ull : ((1ull << K2) - 1ull));
+  // a cell claimed by another component links the two (a repeat of the left
+  // neighbour tap's claimant is dropped; at most one link per tap)
+  if (prev != -1 && prev != comp && (t == 0 || prev != prev_left))
+    links[atomicAdd(nlinks, 1)] = make_int2(prev, comp);
+  if (prev == -1) {
+    const size_t HW = (size_t)gi->Hf * gi->Wf;
+    const float* fin = f_all + 9 * gi->fl_off;
+    float f[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = fmax_nan(fin[i * HW + g], MC_EPSIL);
+    float rho, ux, uy;
+    lbm_macro(f, gi->umax, rho, ux, uy);
+    st_rho[(size_t)n * K2 + t] = rho;
+    st_u[(size_t)n * K2 + t] = make_float2(ux, uy);
+  }
 }
 
 // Single workgroup.  Unions the linked components (union-find in LDS when the
@@ -208,7 +236,8 @@ struct ClusterArgs {
       *cl_members;
   const int *st_sx, *st_sy;
   const unsigned long long* st_won;
-  const float *st_w, *vib, *f_in, *pos;
+  const float *st_w, *st_rho, *vib, *f_in, *pos;
+  const float2* st_u;
   float *vel_io, *pos_out, *force;
   int *cell_seg, *owner;
   uint32_t* mask;
@@ -286,25 +315,31 @@ constexpr int SPT = 4;  // segments per thread whose loads are issued together
 
 // Small-cluster solve: at most one node per thread (thread r holds the node
 // of rank r, its taps' segments and weights in registers) and every array in
-// LDS.  Layout from `base`: sstart[ne+1] eidx[ne] wt[ne] corr[nq], then per
-// segment scell sinv srho suc su0.
+// LDS.  Entries are (entry id e = r * K2 + t, weight) pairs, grouped by
+// segment and sorted by e, i.e. by node id.  Layout from `base`:
+// sstart[ns+1] ew[ne] corr[nq] scell[ns] sinv[ns] srho[ns] suc[ns] su0[ns].
 __device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns) {
-  return al16(4 * ((size_t)ne + 1)) + 2 * al16(4 * (size_t)ne) + al16(8 * (size_t)nq) +
+  return al16(4 * ((size_t)ns + 1)) + al16(8 * (size_t)ne) + al16(8 * (size_t)nq) +
          3 * al16(4 * (size_t)ns) + 2 * al16(8 * (size_t)ns);
+}
+
+__device__ __forceinline__ void cswap_key(float2& a, float2& b) {
+  const bool sw = __float_as_int(a.x) > __float_as_int(b.x);
+  const float2 lo = sw ? b : a, hi = sw ? a : b;
+  a = lo;
+  b = hi;
 }
 
 template <int K>
 __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGeo& gi,
-                                              unsigned char* base,
-                                              const int* nodes, int nq, int ns, int* wsum,
-                                              int* s_seg) {
+                                              unsigned char* base, const int* nodes, int nq,
+                                              int ns, int* wsum, int* s_seg) {
   constexpr int K2 = K * K;
   const Consts& c = a.c;
   const int ne = nq * K2;
   const int W = gi.Wf, H = gi.Hf;
   const size_t HW = (size_t)H * W;
   const float umax = gi.umax;
-  const float* fin = a.f_in + 9 * gi.fl_off;
   float* frc = a.force + 2 * gi.fl_off;
   uint32_t* msk = a.mask + gi.mask_off;
   int* cseg = a.cell_seg + gi.fl_off;
@@ -316,9 +351,8 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
     off += al16(b);
     return p;
   };
-  int* sstart = (int*)take(4 * ((size_t)ne + 1));
-  int* eidx = (int*)take(4 * (size_t)ne);
-  float* wt = (float*)take(4 * (size_t)ne);
+  int* sstart = (int*)take(4 * ((size_t)ns + 1));
+  float2* ew = (float2*)take(8 * (size_t)ne);
   float2* corr = (float2*)take(8 * (size_t)nq);
   int* scell = (int*)take(4 * (size_t)ns);
   float* sinv = (float*)take(4 * (size_t)ns);
@@ -332,6 +366,7 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
   float w[K2];
   float2 vb = make_float2(0.0f, 0.0f);
   unsigned long long won = 0;
+  for (int s = tid; s <= ns; s += bs) sstart[s] = 0;
   if (mine) {
     n = nodes[r];
     const int sx = a.st_sx[n], sy = a.st_sy[n];
@@ -341,17 +376,27 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
     vb = make_float2(a.vib[2 * n], a.vib[2 * n + 1]);
 #pragma unroll
     for (int t = 0; t < K2; ++t) g[t] = wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W);
-  }
-  for (int s = tid; s <= ns; s += bs) sstart[s] = 0;
-  // won taps claim segments (arbitrary numbering); their cells return to idle
-  if (mine) {
+    // won taps claim segments (arbitrary numbering), return their cell to
+    // idle, and carry the cell's rho and clamped u from the stencil pass
+    float rw[K2];
+    float2 uw[K2];
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      if ((won >> t) & 1ull) {
+        rw[t] = a.st_rho[(size_t)n * K2 + t];
+        uw[t] = a.st_u[(size_t)n * K2 + t];
+      }
+    }
 #pragma unroll
     for (int t = 0; t < K2; ++t) {
       if ((won >> t) & 1ull) {
         const int sg = atomicAdd(s_seg, 1);
         cseg[g[t]] = sg;
-        scell[sg] = g[t];
         own[g[t]] = -1;
+        scell[sg] = g[t];
+        srho[sg] = rw[t];
+        su0[sg] = uw[t];
+        suc[sg] = uw[t];
       }
     }
   }
@@ -379,51 +424,43 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
   __syncthreads();
   if (mine) {
 #pragma unroll
-    for (int t = 0; t < K2; ++t) {
-      eidx[sstart[seg[t]] + sub[t]] = r * K2 + t;
-      wt[r * K2 + t] = w[t];
-    }
+    for (int t = 0; t < K2; ++t)
+      ew[sstart[seg[t]] + sub[t]] = make_float2(__int_as_float(r * K2 + t), w[t]);
   }
   __syncthreads();
-  // each segment's entries ascending (= ascending node id); kernel-weight
-  // fold, 1/max(kw, 1), rho and clamped u (fluid.py:123-133,268-290)
-  for (int s0 = 0; s0 < ns; s0 += SPT * bs) {
-    float f[SPT][9];
+  // each segment's entries ascending by entry id (= node id); the kernel-
+  // weight fold in that order and 1/max(kw, 1) (fluid.py:123-133,289-290)
+  for (int s = tid; s < ns; s += bs) {
+    const int lo = sstart[s], hi = sstart[s + 1], len = hi - lo;
+    float kw = 0.0f;
+    if (len <= 8) {
+      float2 v[8];
 #pragma unroll
-    for (int q = 0; q < SPT; ++q) {
-      const int s = s0 + q * bs + tid;
-      if (s < ns) {
-        const size_t cell = (size_t)scell[s];
+      for (int k = 0; k < 8; ++k)
+        v[k] = k < len ? ew[lo + k] : make_float2(__int_as_float(0x7fffffff), 0.0f);
 #pragma unroll
-        for (int i = 0; i < 9; ++i) f[q][i] = fin[i * HW + cell];
-      }
-    }
+      for (int i = 0; i < 7; ++i)
 #pragma unroll
-    for (int q = 0; q < SPT; ++q) {
-      const int s = s0 + q * bs + tid;
-      if (s >= ns) continue;
-      const int lo = sstart[s], hi = sstart[s + 1];
+        for (int k = 0; k < 7 - i; ++k) cswap_key(v[k], v[k + 1]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < len) {
+          ew[lo + k] = v[k];
+          kw = kw + v[k].y;
+        }
+    } else {
       for (int j = lo + 1; j < hi; ++j) {
-        const int v = eidx[j];
+        const float2 v = ew[j];
         int k = j - 1;
-        while (k >= lo && eidx[k] > v) {
-          eidx[k + 1] = eidx[k];
+        while (k >= lo && __float_as_int(ew[k].x) > __float_as_int(v.x)) {
+          ew[k + 1] = ew[k];
           --k;
         }
-        eidx[k + 1] = v;
+        ew[k + 1] = v;
       }
-      float kw = 0.0f;
-      for (int j = lo; j < hi; ++j) kw = kw + wt[eidx[j]];
-      float fq[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) fq[i] = fmax_nan(f[q][i], MC_EPSIL);
-      float rho, ux, uy;
-      lbm_macro(fq, umax, rho, ux, uy);
-      sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
-      srho[s] = rho;
-      su0[s] = make_float2(ux, uy);
-      suc[s] = make_float2(ux, uy);
+      for (int j = lo; j < hi; ++j) kw = kw + ew[j].y;
     }
+    sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
   }
   __syncthreads();
   // multi-direct forcing iterations (fluid.py:292-316)
@@ -448,30 +485,25 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
     for (int s = tid; s < ns; s += bs) {
       const int lo = sstart[s], hi = sstart[s + 1];
       // the first four entries' loads are issued together
-      int e4[4];
+      float2 e4[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) e4[k] = lo + k < hi ? eidx[lo + k] : 0;
-      float w4[4];
+      for (int k = 0; k < 4; ++k) e4[k] = lo + k < hi ? ew[lo + k] : make_float2(0.0f, 0.0f);
       float2 c4[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        w4[k] = wt[e4[k]];
-        c4[k] = corr[e4[k] / K2];
-      }
+      for (int k = 0; k < 4; ++k) c4[k] = corr[__float_as_int(e4[k].x) / K2];
       float ax = 0.0f, ay = 0.0f;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (lo + k < hi) {
-          ax = ax + c4[k].x * w4[k];
-          ay = ay + c4[k].y * w4[k];
+          ax = ax + c4[k].x * e4[k].y;
+          ay = ay + c4[k].y * e4[k].y;
         }
       }
       for (int j = lo + 4; j < hi; ++j) {
-        const int e = eidx[j];
-        const float ww = wt[e];
-        const float2 cc = corr[e / K2];
-        ax = ax + cc.x * ww;
-        ay = ay + cc.y * ww;
+        const float2 e = ew[j];
+        const float2 cc = corr[__float_as_int(e.x) / K2];
+        ax = ax + cc.x * e.y;
+        ay = ay + cc.y * e.y;
       }
       const float iv = sinv[s];
       const float2 u = suc[s];
@@ -539,7 +571,6 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
   const int W = gi.Wf, H = gi.Hf;
   const size_t HW = (size_t)H * W;
   const float umax = gi.umax;
-  const float* fin = a.f_in + 9 * gi.fl_off;
   float* frc = a.force + 2 * gi.fl_off;
   uint32_t* msk = a.mask + gi.mask_off;
   int* cseg = a.cell_seg + gi.fl_off;
@@ -565,6 +596,11 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
         cseg[g] = seg;
         X.scell[seg] = g;
         own[g] = -1;
+        // the cell's rho and clamped u from the stencil pass
+        const float2 u = a.st_u[(size_t)n * K2 + t];
+        X.srho[seg] = a.st_rho[(size_t)n * K2 + t];
+        X.su0[seg] = u;
+        L.suc[seg] = u;
       }
     }
   }
@@ -615,10 +651,6 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
   }
   __syncthreads();
   for (int s = tid; s < ns; s += bs) {
-    const size_t cell = (size_t)X.scell[s];
-    float f[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) f[i] = fin[i * HW + cell];
     const int lo = L.sstart[s], hi = L.sstart[s + 1];
     for (int j = lo + 1; j < hi; ++j) {
       const int v = L.eidx[j];
@@ -631,14 +663,7 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
     }
     float kw = 0.0f;
     for (int j = lo; j < hi; ++j) kw = kw + L.wt[L.eidx[j]];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) f[i] = fmax_nan(f[i], MC_EPSIL);
-    float rho, ux, uy;
-    lbm_macro(f, umax, rho, ux, uy);
     L.sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
-    X.srho[s] = rho;
-    X.su0[s] = make_float2(ux, uy);
-    L.suc[s] = make_float2(ux, uy);
   }
   __syncthreads();
   for (int it = 0; it < c.iters; ++it) {
@@ -929,11 +954,11 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
     if (e != hipSuccess) return e;
   }
   w.dirty = true;
-  const int nb = (N + 63) / 64;
 #define L_STENCIL(KK)                                                                            \
-  hipLaunchKernelGGL(k_ib_stencil<KK>, dim3(nb), dim3(64), 0, s, c, b, N, pos, vel_io,           \
-                     w.comp_of_node, w.st_sx, w.st_sy, w.st_w, w.st_won, w.vib, w.owner, w.links, \
-                     w.nlinks)
+  hipLaunchKernelGGL(k_ib_stencil<KK>,                                                           \
+                     dim3((unsigned)(((long long)N * stencil_lanes<KK>() + 255) / 256)), dim3(256), \
+                     0, s, c, b, N, pos, vel_io, f_in, w.comp_of_node, w.st_sx, w.st_sy, w.st_w,  \
+                     w.st_won, w.st_rho, w.st_u, w.vib, w.owner, w.links, w.nlinks)
   MC_KSWITCH(L_STENCIL);
 #undef L_STENCIL
   hipError_t e = hipGetLastError();
@@ -957,6 +982,8 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   a.st_sx = w.st_sx;
   a.st_sy = w.st_sy;
   a.st_won = w.st_won;
+  a.st_rho = w.st_rho;
+  a.st_u = w.st_u;
   a.st_w = w.st_w;
   a.vib = w.vib;
   a.f_in = f_in;

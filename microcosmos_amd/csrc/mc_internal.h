@@ -142,6 +142,8 @@ struct IBWork {
   int *st_sx, *st_sy;          // per node wrapped stencil start cell
   float *st_w;                 // per node K*K weights
   unsigned long long* st_won;  // per node: bit t set when tap t claimed its cell
+  float* st_rho;               // per node tap that claimed its cell: rho of the cell
+  float2* st_u;                // ... and its clamped u (fluid.py:268-276)
   float *vib;                  // per node (2) IB target velocity
   int *owner, *cell_seg;       // per lattice cell: claiming component (-1 idle), segment
   int2* links;                 // component pairs whose stencils share a cell
