@@ -237,6 +237,33 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
   p->rod_lds_bytes = maxb;
   p->rod_use_lds = p->rod_lds_bytes <= (size_t)p->lds_max ? 1 : 0;
   if (!p->rod_use_lds) MCCHK(p->alloc(&p->rod_gscratch, rod_scratch_floats(N, E, B), "rod scratch"));
+  // per-component topology blocks and contiguous id ranges
+  std::vector<int> blob, blob_off(1, 0), nfirst(C), efirst(C), hfirst(C);
+  auto first_of = [](const std::vector<int>& off, const std::vector<int>& order, int c) {
+    for (int k = off[c]; k < off[c + 1]; ++k)
+      if (order[k] != order[off[c]] + (k - off[c])) return -1;
+    return off[c + 1] > off[c] ? order[off[c]] : 0;
+  };
+  for (int c = 0; c < C; ++c) {
+    for (int q = hoff[c]; q < hoff[c + 1]; ++q) blob.push_back(hin[q]);
+    for (int q = hoff[c]; q < hoff[c + 1]; ++q) blob.push_back(hout[q]);
+    for (int q = eoff[c]; q < eoff[c + 1]; ++q) blob.push_back(esrc[q]);
+    for (int q = eoff[c]; q < eoff[c + 1]; ++q) blob.push_back(etgt[q]);
+    for (int q = eoff[c]; q <= eoff[c + 1]; ++q) blob.push_back(einc_off[q] - einc_off[eoff[c]]);
+    for (int q = einc_off[eoff[c]]; q < einc_off[eoff[c + 1]]; ++q) blob.push_back(einc_flat[q]);
+    for (int q = noff[c]; q <= noff[c + 1]; ++q) blob.push_back(ninc_off[q] - ninc_off[noff[c]]);
+    for (int q = ninc_off[noff[c]]; q < ninc_off[noff[c + 1]]; ++q) blob.push_back(ninc_flat[q]);
+    blob_off.push_back((int)blob.size());
+    nfirst[c] = first_of(noff, norder, c);
+    efirst[c] = first_of(eoff, eorder, c);
+    hfirst[c] = first_of(hoff, horder, c);
+  }
+  int *bl, *blo, *nf, *ef, *hf;
+  MCCHK(upload(p, &bl, blob, "topology blocks"));
+  MCCHK(upload(p, &blo, blob_off, "topology block offsets"));
+  MCCHK(upload(p, &nf, nfirst, "node ranges"));
+  MCCHK(upload(p, &ef, efirst, "edge ranges"));
+  MCCHK(upload(p, &hf, hfirst, "hinge ranges"));
   int *a0, *a1, *a2, *a3, *a4, *a5, *a6, *a7, *a8, *a9, *a10, *a11, *a12, *a13;
   MCCHK(upload(p, &a0, noff, "comp_node_off"));
   MCCHK(upload(p, &a1, norder, "comp_nodes"));
@@ -258,7 +285,8 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
   int* ci_d;
   MCCHK(upload(p, &ci_d, comp_inst, "comp_inst"));
   p->bd.comp_inst = ci_d;
-  p->L = RodLayoutDev{C, a0, a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11, a12, a13};
+  p->L = RodLayoutDev{C, a0, a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11, a12, a13,
+                      bl, blo, nf, ef, hf};
   return MC_OK;
 }
 

@@ -381,18 +381,15 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
     float rw[K2];
     float2 uw[K2];
 #pragma unroll
-    for (int t = 0; t < K2; ++t) {
-      if ((won >> t) & 1ull) {
-        rw[t] = a.st_rho[(size_t)n * K2 + t];
-        uw[t] = a.st_u[(size_t)n * K2 + t];
-      }
+    for (int t = 0; t < K2; ++t) {  // all taps: no wait for `won`
+      rw[t] = a.st_rho[(size_t)n * K2 + t];
+      uw[t] = a.st_u[(size_t)n * K2 + t];
     }
 #pragma unroll
     for (int t = 0; t < K2; ++t) {
       if ((won >> t) & 1ull) {
         const int sg = atomicAdd(s_seg, 1);
         cseg[g[t]] = sg;
-        own[g[t]] = -1;
         scell[sg] = g[t];
         srho[sg] = rw[t];
         su0[sg] = uw[t];
@@ -530,11 +527,18 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
       fy = fy * fc;
     }
     suc[s] = make_float2(fx, fy);
-    frc[cell] = fx;
-    frc[HW + cell] = fy;
-    atomicOr(&msk[cell >> 5], 1u << (cell & 31));
   }
   __syncthreads();
+  // global results nothing in this workgroup reads again: force and support
+  // bits for the LBM pass, the idle state of the covered cells
+  for (int s = tid; s < ns; s += bs) {
+    const int cell = scell[s];
+    const float2 F = suc[s];
+    frc[cell] = F.x;
+    frc[HW + cell] = F.y;
+    atomicOr(&msk[cell >> 5], 1u << (cell & 31));
+    own[cell] = -1;
+  }
   // node drag (fluid.py:364-382) and the final wrap (simulate.py:97)
   if (mine) {
     float2 F[K2];

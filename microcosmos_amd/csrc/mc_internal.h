@@ -93,6 +93,11 @@ struct RodLayoutDev {
   const int *comp_hinge_off, *comp_hinges;
   const int *edge_src, *edge_tgt, *hinge_in, *hinge_out;
   const int *node_inc_off, *node_inc, *edge_inc_off, *edge_inc;
+  // per component: the topology as one block (component-relative) in the LDS
+  // order hin[nb] hout[nb] esrc[ne] etgt[ne] eoff[ne+1] einc[nei] noff[nn+1]
+  // ninc[nni], and the first node / edge / hinge id when the component's ids
+  // are one contiguous range (-1 otherwise)
+  const int *blob, *blob_off, *node_first, *edge_first, *hinge_first;
 };
 
 struct Actuation {           // rest_b(t) = amp_b * sin(omega_b * t + phase_b)

@@ -106,85 +106,142 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   float* CO = He + ne;
   float* RS = CO + nb;
   float* KB = RS + nb;
-  // topology: staged into LDS after the floats (component-relative offsets),
-  // or read in place from global memory for components beyond the LDS budget
+  const bool is_cos = c.solver == 1;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  // ids of this thread's first node / edge / hinge (no load for components
+  // whose ids are one contiguous range)
+  const int nf = L.node_first[comp], ef = L.edge_first[comp], hf = L.hinge_first[comp];
+  auto node_id = [&](int i) { return nf >= 0 ? nf + i : L.comp_nodes[n0 + i]; };
+  auto edge_id = [&](int e) { return ef >= 0 ? ef + e : L.comp_edges[e0 + e]; };
+  auto hinge_id = [&](int b) { return hf >= 0 ? hf + b : L.comp_hinges[b0 + b]; };
+
+  // Every global load that depends on nothing else is issued first, then
+  // the topology copy, then the dependent steric samples.
+  const double tact = a.has_act ? (double)(*a.act.t) : 0.0;
+  int g = -1;
+  float px = 0.0f, py = 0.0f, vx = 0.0f, vy = 0.0f, yx = 0.0f, yy = 0.0f;
+  if (tid < nn) {
+    g = node_id(tid);
+    px = a.pos_in[2 * g];
+    py = a.pos_in[2 * g + 1];
+    if (a.mode == 1) {
+      yx = a.y_in ? a.y_in[2 * g] : px;
+      yy = a.y_in ? a.y_in[2 * g + 1] : py;
+    } else {
+      vx = a.vel_in[2 * g];
+      vy = a.vel_in[2 * g + 1];
+    }
+  }
+  float th0 = 0.0f, le0 = 0.0f;
+  if (tid < ne) {
+    const int ge = edge_id(tid);
+    th0 = a.theta_in[ge];
+    le0 = a.rest_len[ge];
+  }
+  float am = 0.0f, om = 0.0f, ph = 0.0f, rs0 = 0.0f, kb0 = 0.0f;
+  if (tid < nb) {
+    const int gb = hinge_id(tid);
+    if (a.has_act) {
+      am = a.act.amp[gb];
+      om = a.act.omega[gb];
+      ph = a.act.phase[gb];
+    } else {
+      rs0 = a.rest_ang[gb];
+    }
+    kb0 = a.kb[gb];
+  }
+  // topology: one contiguous block per component, staged into LDS after the
+  // floats (component-relative offsets), or read in place from global memory
+  // for components beyond the LDS budget
   const int *hin, *hout, *esrc, *etgt, *einc_off, *einc, *ninc_off, *ninc;
-  int eoff_base, noff_base;
   if (a.use_lds) {
     int* ib = reinterpret_cast<int*>(KB + nb);
-    int* s_hin = ib;
-    int* s_hout = s_hin + nb;
-    int* s_esrc = s_hout + nb;
-    int* s_etgt = s_esrc + ne;
-    int* s_eoff = s_etgt + ne;
-    int* s_einc = s_eoff + ne + 1;
-    int* s_noff = s_einc + nei;
-    int* s_ninc = s_noff + nn + 1;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-      s_hin[b] = L.hinge_in[b0 + b];
-      s_hout[b] = L.hinge_out[b0 + b];
+    const int* blob = L.blob + L.blob_off[comp];
+    const int len = L.blob_off[comp + 1] - L.blob_off[comp];
+    for (int q0 = 0; q0 < len; q0 += 4 * bs) {
+      int v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int q = q0 + k * bs + tid;
+        v[k] = q < len ? blob[q] : 0;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int q = q0 + k * bs + tid;
+        if (q < len) ib[q] = v[k];
+      }
     }
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-      s_esrc[e] = L.edge_src[e0 + e];
-      s_etgt[e] = L.edge_tgt[e0 + e];
-    }
-    for (int e = threadIdx.x; e <= ne; e += blockDim.x) s_eoff[e] = L.edge_inc_off[e0 + e] - ei0;
-    for (int q = threadIdx.x; q < nei; q += blockDim.x) s_einc[q] = L.edge_inc[ei0 + q];
-    for (int i = threadIdx.x; i <= nn; i += blockDim.x) s_noff[i] = L.node_inc_off[n0 + i] - ni0;
-    for (int q = threadIdx.x; q < nni; q += blockDim.x) s_ninc[q] = L.node_inc[ni0 + q];
-    hin = s_hin; hout = s_hout; esrc = s_esrc; etgt = s_etgt;
-    einc_off = s_eoff; einc = s_einc; ninc_off = s_noff; ninc = s_ninc;
-    eoff_base = 0;
-    noff_base = 0;
+    hin = ib;
+    hout = hin + nb;
+    esrc = hout + nb;
+    etgt = esrc + ne;
+    einc_off = etgt + ne;
+    einc = einc_off + ne + 1;
+    ninc_off = einc + nei;
+    ninc = ninc_off + nn + 1;
   } else {
     hin = L.hinge_in + b0; hout = L.hinge_out + b0; esrc = L.edge_src + e0; etgt = L.edge_tgt + e0;
     einc_off = L.edge_inc_off + e0; einc = L.edge_inc; ninc_off = L.node_inc_off + n0; ninc = L.node_inc;
-    eoff_base = 0;
-    noff_base = 0;
   }
-  (void)eoff_base;
-  (void)noff_base;
-  const bool is_cos = c.solver == 1;
 
   // prologue: damping, prediction, steric push (simulate.py:41-72)
-  for (int i = threadIdx.x; i < nn; i += blockDim.x) {
-    const int g = L.comp_nodes[n0 + i];
+  auto node_prologue = [&](int i, int gg, float qx, float qy, float ux, float uy, float zx,
+                           float zy) {
     if (a.mode == 1) {
-      const float px = a.pos_in[2 * g], py = a.pos_in[2 * g + 1];
-      X[2 * i] = px;
-      X[2 * i + 1] = py;
-      Y[2 * i] = a.y_in ? a.y_in[2 * g] : px;
-      Y[2 * i + 1] = a.y_in ? a.y_in[2 * g + 1] : py;
-      continue;
+      X[2 * i] = qx;
+      X[2 * i + 1] = qy;
+      Y[2 * i] = zx;
+      Y[2 * i + 1] = zy;
+      return;
     }
-    const float vx = a.vel_in[2 * g] * c.damping, vy = a.vel_in[2 * g + 1] * c.damping;
-    float px = a.pos_in[2 * g] + vx * c.dt, py = a.pos_in[2 * g + 1] + vy * c.dt;
+    const float dvx = ux * c.damping, dvy = uy * c.damping;
+    float x = qx + dvx * c.dt, y = qy + dvy * c.dt;
     if (c.enable_steric) {
       float fx, fy;
-      steric_force(c, gi, a.steric, a.pred_w, g, fx, fy);
-      px = px + (fx * c.steric_strength) * c.dt;
-      py = py + (fy * c.steric_strength) * c.dt;
+      steric_force(c, gi, a.steric, a.pred_w, gg, fx, fy);
+      x = x + (fx * c.steric_strength) * c.dt;
+      y = y + (fy * c.steric_strength) * c.dt;
     }
-    X[2 * i] = px;
-    X[2 * i + 1] = py;
-    Y[2 * i] = px;
-    Y[2 * i + 1] = py;
-    a.debug_out[2 * g] = is_cos ? px : 0.0f;
-    a.debug_out[2 * g + 1] = is_cos ? py : 0.0f;
+    X[2 * i] = x;
+    X[2 * i + 1] = y;
+    Y[2 * i] = x;
+    Y[2 * i + 1] = y;
+    a.debug_out[2 * gg] = is_cos ? x : 0.0f;
+    a.debug_out[2 * gg + 1] = is_cos ? y : 0.0f;
+  };
+  auto hinge_rest = [&](float amp_, float om_, float ph_, float rs_) {
+    if (!a.has_act) return rs_;
+    const double arg = (double)om_ * tact + (double)ph_;
+    return (float)((double)amp_ * sin(arg));
+  };
+  if (tid < nn) node_prologue(tid, g, px, py, vx, vy, yx, yy);
+  if (tid < ne) {
+    TH[tid] = th0;
+    Le[tid] = le0;
   }
-  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-    const int ge = L.comp_edges[e0 + e];
+  if (tid < nb) {
+    RS[tid] = hinge_rest(am, om, ph, rs0);
+    KB[tid] = kb0;
+  }
+  // components with more nodes / edges / hinges than threads
+  for (int i = tid + bs; i < nn; i += bs) {
+    const int gg = node_id(i);
+    const float qx = a.pos_in[2 * gg], qy = a.pos_in[2 * gg + 1];
+    if (a.mode == 1)
+      node_prologue(i, gg, qx, qy, 0.0f, 0.0f, a.y_in ? a.y_in[2 * gg] : qx,
+                    a.y_in ? a.y_in[2 * gg + 1] : qy);
+    else
+      node_prologue(i, gg, qx, qy, a.vel_in[2 * gg], a.vel_in[2 * gg + 1], 0.0f, 0.0f);
+  }
+  for (int e = tid + bs; e < ne; e += bs) {
+    const int ge = edge_id(e);
     TH[e] = a.theta_in[ge];
     Le[e] = a.rest_len[ge];
   }
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    const int gb = L.comp_hinges[b0 + b];
-    if (a.has_act) {
-      const double arg = (double)a.act.omega[gb] * (double)(*a.act.t) + (double)a.act.phase[gb];
-      RS[b] = (float)((double)a.act.amp[gb] * sin(arg));
-    } else {
-      RS[b] = a.rest_ang[gb];
-    }
+  for (int b = tid + bs; b < nb; b += bs) {
+    const int gb = hinge_id(b);
+    RS[b] = a.has_act ? hinge_rest(a.act.amp[gb], a.act.omega[gb], a.act.phase[gb], 0.0f)
+                      : a.rest_ang[gb];
     KB[b] = a.kb[gb];
   }
   __syncthreads();
