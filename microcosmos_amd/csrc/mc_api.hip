@@ -258,6 +258,26 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
     efirst[c] = first_of(eoff, eorder, c);
     hfirst[c] = first_of(hoff, horder, c);
   }
+  std::vector<CompMeta> meta(C);
+  for (int c = 0; c < C; ++c) {
+    CompMeta& m = meta[c];
+    m.n0 = noff[c];
+    m.nn = noff[c + 1] - noff[c];
+    m.e0 = eoff[c];
+    m.ne = eoff[c + 1] - eoff[c];
+    m.b0 = hoff[c];
+    m.nb = hoff[c + 1] - hoff[c];
+    m.nni = ninc_off[noff[c + 1]] - ninc_off[noff[c]];
+    m.nei = einc_off[eoff[c + 1]] - einc_off[eoff[c]];
+    m.nf = nfirst[c];
+    m.ef = efirst[c];
+    m.hf = hfirst[c];
+    m.blob_off = blob_off[c];
+    m.inst = p->node_inst_h[norder[noff[c]]];
+    m.pad0 = m.pad1 = m.pad2 = 0;
+  }
+  CompMeta* md;
+  MCCHK(upload(p, &md, meta, "component records"));
   int *bl, *blo, *nf, *ef, *hf;
   MCCHK(upload(p, &bl, blob, "topology blocks"));
   MCCHK(upload(p, &blo, blob_off, "topology block offsets"));
@@ -286,7 +306,7 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
   MCCHK(upload(p, &ci_d, comp_inst, "comp_inst"));
   p->bd.comp_inst = ci_d;
   p->L = RodLayoutDev{C, a0, a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11, a12, a13,
-                      bl, blo, nf, ef, hf};
+                      bl, blo, nf, ef, hf, md};
   return MC_OK;
 }
 

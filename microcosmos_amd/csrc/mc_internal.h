@@ -98,6 +98,14 @@ struct RodLayoutDev {
   // ninc[nni], and the first node / edge / hinge id when the component's ids
   // are one contiguous range (-1 otherwise)
   const int *blob, *blob_off, *node_first, *edge_first, *hinge_first;
+  const struct CompMeta* meta;  // the per-component scalars in one record
+};
+
+// Per-component scalars of the rod layout, read by one 64-byte load.
+struct CompMeta {
+  int n0, nn, e0, ne, b0, nb, nni, nei;  // node / edge / hinge ranges, incidence counts
+  int nf, ef, hf;                        // first ids of contiguous ranges (-1: listed)
+  int blob_off, inst, pad0, pad1, pad2;
 };
 
 struct Actuation {           // rest_b(t) = amp_b * sin(omega_b * t + phase_b)

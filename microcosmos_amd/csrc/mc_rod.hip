@@ -89,12 +89,10 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   const Consts& c = a.c;
   const RodLayoutDev& L = a.L;
   const int comp = blockIdx.x;
-  const InstGeo& gi = a.bd.geo[a.bd.comp_inst[comp]];
-  const int n0 = L.comp_node_off[comp], nn = L.comp_node_off[comp + 1] - n0;
-  const int e0 = L.comp_edge_off[comp], ne = L.comp_edge_off[comp + 1] - e0;
-  const int b0 = L.comp_hinge_off[comp], nb = L.comp_hinge_off[comp + 1] - b0;
-  const int ni0 = L.node_inc_off[n0], nni = L.node_inc_off[n0 + nn] - ni0;
-  const int ei0 = L.edge_inc_off[e0], nei = L.edge_inc_off[e0 + ne] - ei0;
+  const CompMeta m = L.meta[comp];
+  const InstGeo& gi = a.bd.geo[m.inst];
+  const int n0 = m.n0, nn = m.nn, e0 = m.e0, ne = m.ne, b0 = m.b0, nb = m.nb;
+  const int nni = m.nni, nei = m.nei;
   // float scratch: x[2nn] y[2nn] th[ne] l[ne] F[2ne] He[ne] corr[nb] rest[nb] kb[nb]
   float* base = a.use_lds ? smem : (a.gscratch + (size_t)4 * n0 + (size_t)5 * e0 + (size_t)3 * b0);
   float* X = base;
@@ -110,7 +108,7 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   const int tid = threadIdx.x, bs = blockDim.x;
   // ids of this thread's first node / edge / hinge (no load for components
   // whose ids are one contiguous range)
-  const int nf = L.node_first[comp], ef = L.edge_first[comp], hf = L.hinge_first[comp];
+  const int nf = m.nf, ef = m.ef, hf = m.hf;
   auto node_id = [&](int i) { return nf >= 0 ? nf + i : L.comp_nodes[n0 + i]; };
   auto edge_id = [&](int e) { return ef >= 0 ? ef + e : L.comp_edges[e0 + e]; };
   auto hinge_id = [&](int b) { return hf >= 0 ? hf + b : L.comp_hinges[b0 + b]; };
@@ -156,8 +154,8 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   const int *hin, *hout, *esrc, *etgt, *einc_off, *einc, *ninc_off, *ninc;
   if (a.use_lds) {
     int* ib = reinterpret_cast<int*>(KB + nb);
-    const int* blob = L.blob + L.blob_off[comp];
-    const int len = L.blob_off[comp + 1] - L.blob_off[comp];
+    const int* blob = L.blob + m.blob_off;
+    const int len = 2 * nb + 3 * ne + 1 + nei + nn + 1 + nni;
     for (int q0 = 0; q0 < len; q0 += 4 * bs) {
       int v[4];
 #pragma unroll
@@ -254,40 +252,35 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
       CO[b] = ((diff - RS[b]) * KB[b]) * 0.5f;
     }
     __syncthreads();
-    // per edge ordered gather of +-corr (pbd.py:44-49 / stable_cosserat.py:158-171)
+    // per edge: ordered gather of +-corr and the angle update (pbd.py:44-49 /
+    // stable_cosserat.py:158-171), then the edge force with the new angle
+    // (pbd.py:60-81 / stable_cosserat.py:185-195) -- one thread owns the edge
     for (int e = threadIdx.x; e < ne; e += blockDim.x) {
       const int i0 = einc_off[e], i1 = einc_off[e + 1];
-      const int* inc = a.use_lds ? einc : (einc + 0);
       float d = 0.0f;
       for (int q = i0; q < i1; ++q) {
-        const int code = a.use_lds ? inc[q] : inc[q + 0];
+        const int code = einc[q];
         const float v = CO[code >> 1];
         d = d + ((code & 1) ? -v : v);
       }
       const float deg = fmax_nan((float)(i1 - i0), 1.0f);
+      const int sn = esrc[e], tn = etgt[e];
+      const float rx = min_image(X[2 * tn], X[2 * sn], Wd);
+      const float ry = min_image(X[2 * tn + 1], X[2 * sn + 1], Hd);
+      float th;
       if (!is_cos) {
-        TH[e] = TH[e] + (d / deg) * 1.8f;
+        th = TH[e] + (d / deg) * 1.8f;
       } else {
-        float th = TH[e] + d / deg;
-        const int sn = esrc[e], tn = etgt[e];
-        const float sx = min_image(X[2 * tn], X[2 * sn], Wd);
-        const float sy = min_image(X[2 * tn + 1], X[2 * sn + 1], Hd);
-        const float alpha = datan2(sy, sx);
+        th = TH[e] + d / deg;
+        const float alpha = datan2(ry, rx);
         const float pull = wrap_angle(alpha - th);
         th = th + pull * 0.05f;
-        TH[e] = th;
       }
-    }
-    __syncthreads();
-    // per edge forces (pbd.py:60-81 / stable_cosserat.py:185-195)
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-      const int sn = esrc[e], tn = etgt[e];
-      const float th = TH[e], l = Le[e];
+      TH[e] = th;
+      const float l = Le[e];
       double sd, cd;
       sincos((double)th, &sd, &cd);
       const float dx = (float)cd, dy = (float)sd;
-      const float rx = min_image(X[2 * tn], X[2 * sn], Wd);
-      const float ry = min_image(X[2 * tn + 1], X[2 * sn + 1], Hd);
       if (!is_cos) {
         const float tx = dx * l, ty = dy * l;
         const float len = sqrtf(rx * rx + ry * ry);
