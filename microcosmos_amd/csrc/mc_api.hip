@@ -85,8 +85,8 @@ struct mc_plan {
   float* pred_w = nullptr;
   int* cic_cell = nullptr;
   float* cic_w = nullptr;
-  int *row_cnt = nullptr, *row_off = nullptr, *row_fill = nullptr, *row_active = nullptr;
-  int *row_nactive = nullptr, *row_list = nullptr, *row_sort = nullptr;
+  int *row_cnt = nullptr, *row_off = nullptr, *row_fill = nullptr;
+  int *row_list = nullptr, *row_sort = nullptr;
   unsigned long long* row_keys = nullptr;
   float2* spec = nullptr;
   std::vector<int2> shapes;     // distinct (H, W) steric grids, each with a kf table
@@ -551,7 +551,7 @@ mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float*
                      float* steric_out, float* density_out, hipStream_t s) {
   HIPCHK(launch_predict(c, p->bd, p->N, pos, vel, p->pred_w, p->cic_cell, p->cic_w, p->row_cnt, s));
   if (!c.enable_steric) return MC_OK;
-  HIPCHK(launch_bin_scan(p->rows, p->row_cnt, p->row_off, p->row_fill, p->row_active, p->row_nactive, s));
+  HIPCHK(launch_bin_scan(p->rows, p->row_cnt, p->row_off, p->row_fill, s));
   HIPCHK(launch_row_fill(c, p->N, p->cic_cell, p->row_off, p->row_fill, p->row_list, s));
   HIPCHK(launch_steric_fft_T(p->bd, nullptr, p->row_off, p->row_list, p->row_sort, p->row_keys,
                              (size_t)4 * p->N, p->cic_cell, p->cic_w, p->spec, p->kf, density_out,
@@ -701,8 +701,6 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
   if ((st = p->alloc(&p->row_cnt, R, "row cnt")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_off, R + 1, "row off")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_fill, R, "row fill")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->row_active, R, "row active")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->row_nactive, 1, "row n")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_list, 2 * (size_t)N, "row list")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_sort, 2 * (size_t)N, "row sort")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_keys, 8 * (size_t)N, "row keys")) != MC_OK) return bail(st);

@@ -118,8 +118,9 @@ struct Actuation {           // rest_b(t) = amp_b * sin(omega_b * t + phase_b)
 hipError_t launch_predict(const Consts& c, const BatchDev& b, int N, const float* pos,
                           const float* vel, float* pred_w, int* cic_cell, float* cic_w,
                           int* row_cnt, hipStream_t s);
-hipError_t launch_bin_scan(int T, const int* cnt, int* off, int* fill, int* active,
-                           int* n_active, hipStream_t s);
+// exclusive scan of T counts into off[0..T]; zeroes fill[0..T) and cnt[0..T)
+// (cnt, off and fill 16-byte aligned)
+hipError_t launch_bin_scan(int T, int* cnt, int* off, int* fill, hipStream_t s);
 hipError_t launch_row_fill(const Consts& c, int N, const int* cic_cell, const int* row_off,
                            int* row_fill, int* row_list, hipStream_t s);
 hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t s);

@@ -456,17 +456,25 @@ hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t 
 size_t fft_rows_lds_bytes(int max_w) { return (size_t)3 * max_w * sizeof(float2); }
 size_t fft_cols_lds_bytes(int max_h) { return (size_t)3 * max_h * sizeof(float2); }
 
+// Workgroup size of an FFT pass: one radix-4 butterfly per thread and stage
+// for the longest sequence, between one and four wavefronts.
+static int fft_threads(int n) {
+  const int t = ((n / 4 + 63) / 64) * 64;
+  return t < 64 ? 64 : (t > 256 ? 256 : t);
+}
+
 hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const int* row_off,
                                int* row_list, int* sort_scratch, unsigned long long* key_scratch,
                                size_t key_half, const int* cic_cell, const float* cic_w,
                                float2* specT, const float* kfT, float* density_out, float* out,
                                hipStream_t s) {
-  hipLaunchKernelGGL(k_rows_fwd_T, dim3(b.rows), dim3(256), fft_rows_lds_bytes(b.max_w), s, b,
+  const int tr = fft_threads(b.max_w), tc = fft_threads(b.max_h);
+  hipLaunchKernelGGL(k_rows_fwd_T, dim3(b.rows), dim3(tr), fft_rows_lds_bytes(b.max_w), s, b,
                      real_in, row_off, row_list, sort_scratch, key_scratch, key_half, cic_cell,
                      cic_w, specT, density_out);
-  hipLaunchKernelGGL(k_cols_T, dim3(b.cols), dim3(256), fft_cols_lds_bytes(b.max_h), s, b, specT,
+  hipLaunchKernelGGL(k_cols_T, dim3(b.cols), dim3(tc), fft_cols_lds_bytes(b.max_h), s, b, specT,
                      kfT);
-  hipLaunchKernelGGL(k_rows_inv_T, dim3(b.rows), dim3(256), fft_rows_lds_bytes(b.max_w), s, b,
+  hipLaunchKernelGGL(k_rows_inv_T, dim3(b.rows), dim3(tr), fft_rows_lds_bytes(b.max_w), s, b,
                      specT, out);
   return hipGetLastError();
 }
