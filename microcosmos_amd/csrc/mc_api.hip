@@ -418,8 +418,8 @@ mc_status build_fft(mc_plan* p) {
     }
     plans.push_back(f);
   }
-  // row pass: 3 W complex + the deposit's static LDS (~12.3 KB); column pass: 3 H complex
-  if (fft_rows_lds_bytes(p->max_w) + 16384 > (size_t)p->lds_max ||
+  // row pass: W complex + max(2 W complex, deposit scratch); column pass: 3 H complex
+  if (fft_rows_lds_bytes(p->max_w) + 256 > (size_t)p->lds_max ||
       fft_cols_lds_bytes(p->max_h) + 256 > (size_t)p->lds_max) {
     p->fft_why = "steric FFT row/column does not fit in LDS";
     return MC_OK;

@@ -222,71 +222,76 @@ __device__ __forceinline__ int xcd_group(int b, int G) {
 // row, q = 2, 3 if its y1 row).  Ordering the entries by (x, q, node rank)
 // puts each cell's contributions contiguously and in the reference's order
 // (all w00 by node, then w10, w01, w11 -- forces.py:112-115); a segment head
-// thread folds them.  Rows with at most DEP_FAST nodes order node ids and
-// entry keys by counting ranks in LDS (keys are unique); longer rows sort.
-constexpr int DEP_FAST = 256;
+// thread folds them from zero.  Rows of up to `cap` nodes order node ids and
+// entry keys by counting ranks in `scratch` (DEP_BYTES_PER_NODE bytes of LDS
+// per node, aliasing the FFT's second buffer and twiddles); longer rows
+// merge-sort in global scratch.
+constexpr int DEP_BYTES_PER_NODE = 36;  // sorted id + 2 entry keys + 2 ranked keys
+constexpr int DEP_CAP_MAX = 256;
+constexpr int DEP_CAP_MIN = 64;
 
-__device__ void deposit_row(int row, int W, const int* __restrict__ row_off,
+__device__ void deposit_row(int row, int W, int cap, const int* __restrict__ row_off,
                             int* __restrict__ row_list, int* __restrict__ sort_tmp,
                             unsigned long long* __restrict__ key_tmp, size_t key_half,
                             const int* __restrict__ cic_cell, const float* __restrict__ cic_w,
-                            float2* dst, int* sort_lds, unsigned long long* keys_lds,
-                            unsigned long long* sort_keys) {
+                            float2* __restrict__ dst, char* __restrict__ scratch) {
+  const int B = blockDim.x, tid = threadIdx.x;
   const int off = row_off[row];
   const int m = row_off[row + 1] - off;
   const int ne = 2 * m;
-  for (int x = threadIdx.x; x < W; x += blockDim.x) dst[x] = make_float2(0.0f, 0.0f);
+  for (int x = tid; x < W; x += B) dst[x] = make_float2(0.0f, 0.0f);
+  if (m == 0) return;
   const int* nodes;
   const unsigned long long* keys;
-  if (m <= DEP_FAST) {
-    // ids into LDS, ranks by counting, keys in rank order, key ranks by counting
-    int* ids = sort_lds;               // [0, m): arrival order
-    int* sorted = sort_lds + DEP_FAST;  // [0, m): ascending
-    for (int j = threadIdx.x; j < m; j += blockDim.x) ids[j] = row_list[off + j];
+  if (m <= cap) {
+    int* sorted = reinterpret_cast<int*>(scratch);                             // [cap]
+    unsigned long long* k0 = reinterpret_cast<unsigned long long*>(sorted + cap);  // [2 cap]
+    unsigned long long* k1 = k0 + 2 * cap;                                      // [2 cap]
+    int* ids = reinterpret_cast<int*>(k0);  // arrival order, dead once ranked
+    for (int j = tid; j < m; j += B) ids[j] = row_list[off + j];
     __syncthreads();
-    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    for (int j = tid; j < m; j += B) {
       const int v = ids[j];
       int r = 0;
       for (int k = 0; k < m; ++k) r += ids[k] < v ? 1 : 0;
       sorted[r] = v;
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    for (int j = tid; j < m; j += B) {
       const int n = sorted[j];
-      const unsigned long long x0 = (unsigned)cic_cell[4 * n + 0], x1 = (unsigned)cic_cell[4 * n + 1];
-      const unsigned long long qb = (cic_cell[4 * n + 2] == row) ? 0ull : 2ull;
-      sort_keys[2 * j] = (x0 << 34) | (qb << 32) | (unsigned)j;
-      sort_keys[2 * j + 1] = (x1 << 34) | ((qb + 1ull) << 32) | (unsigned)j;
+      const int4 cc = *reinterpret_cast<const int4*>(cic_cell + 4 * (size_t)n);
+      const unsigned long long qb = (cc.z == row) ? 0ull : 2ull;
+      k0[2 * j] = ((unsigned long long)(unsigned)cc.x << 34) | (qb << 32) | (unsigned)j;
+      k0[2 * j + 1] = ((unsigned long long)(unsigned)cc.y << 34) | ((qb + 1ull) << 32) | (unsigned)j;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < ne; i += blockDim.x) {
-      const unsigned long long v = sort_keys[i];
+    for (int i = tid; i < ne; i += B) {
+      const unsigned long long v = k0[i];
       int r = 0;
-      for (int k = 0; k < ne; ++k) r += sort_keys[k] < v ? 1 : 0;
-      keys_lds[r] = v;
+      for (int k = 0; k < ne; ++k) r += k0[k] < v ? 1 : 0;
+      k1[r] = v;
     }
     __syncthreads();
     nodes = sorted;
-    keys = keys_lds;
+    keys = k1;
   } else {
     int* nl = row_list + off;
-    sort_unique<int>(nl, m, sort_lds, DEP_IDS_LDS, sort_tmp + off, 0x7fffffff);
-    unsigned long long* kk = (ne <= DEP_KEYS_LDS) ? keys_lds : (key_tmp + 2 * (size_t)off);
-    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    sort_unique<int>(nl, m, nullptr, 0, sort_tmp + off, 0x7fffffff);
+    unsigned long long* kk = key_tmp + 2 * (size_t)off;
+    for (int j = tid; j < m; j += B) {
       const int n = nl[j];
-      const unsigned long long x0 = (unsigned)cic_cell[4 * n + 0], x1 = (unsigned)cic_cell[4 * n + 1];
-      const unsigned long long qb = (cic_cell[4 * n + 2] == row) ? 0ull : 2ull;
-      kk[2 * j] = (x0 << 34) | (qb << 32) | (unsigned)j;
-      kk[2 * j + 1] = (x1 << 34) | ((qb + 1ull) << 32) | (unsigned)j;
+      const int4 cc = *reinterpret_cast<const int4*>(cic_cell + 4 * (size_t)n);
+      const unsigned long long qb = (cc.z == row) ? 0ull : 2ull;
+      kk[2 * j] = ((unsigned long long)(unsigned)cc.x << 34) | (qb << 32) | (unsigned)j;
+      kk[2 * j + 1] = ((unsigned long long)(unsigned)cc.y << 34) | ((qb + 1ull) << 32) | (unsigned)j;
     }
     __threadfence_block();
     __syncthreads();
-    sort_unique<unsigned long long>(kk, ne, sort_keys, DEP_KEYS_LDS,
-                                    key_tmp + key_half + 2 * (size_t)off, ~0ull);
+    sort_unique<unsigned long long>(kk, ne, nullptr, 0, key_tmp + key_half + 2 * (size_t)off, ~0ull);
     nodes = nl;
     keys = kk;
   }
-  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+  for (int i = tid; i < ne; i += B) {
     const unsigned long long k = keys[i];
     const int x = (int)(k >> 34);
     if (i > 0 && (int)(keys[i - 1] >> 34) == x) continue;
@@ -324,14 +329,11 @@ __device__ __forceinline__ void load_plan(const FftDev& p, int* s_radix) {
 // Forward row FFT of one row (from the CIC deposit when `real_in` is NULL,
 // else from the real grid), written transposed to specT.
 __global__ void __launch_bounds__(256) k_rows_fwd_T(
-    BatchDev bd, const float* __restrict__ real_in, const int* __restrict__ row_off,
+    BatchDev bd, const float* __restrict__ real_in, int dep_cap, const int* __restrict__ row_off,
     int* __restrict__ row_list, int* __restrict__ sort_tmp, unsigned long long* __restrict__ key_tmp,
     size_t key_half, const int* __restrict__ cic_cell, const float* __restrict__ cic_w,
     float2* __restrict__ specT, float* __restrict__ density_out) {
   extern __shared__ float2 lds[];
-  __shared__ int sort_lds[DEP_IDS_LDS];
-  __shared__ unsigned long long keys_lds[DEP_KEYS_LDS];
-  __shared__ unsigned long long sort_keys[DEP_KEYS_LDS];
   __shared__ int s_radix[FFT_MAX_RADICES];
   const int row = xcd_group(blockIdx.x, gridDim.x);
   const InstGeo& gi = bd.geo[bd.row_inst[row]];
@@ -339,19 +341,19 @@ __global__ void __launch_bounds__(256) k_rows_fwd_T(
   const FftDev& fp = bd.fft[gi.fft_w];
   float2* a = lds;
   float2* b = lds + W;
-  float2* tw = lds + 2 * W;
+  float2* tw = lds + 2 * W;  // b and tw alias the deposit scratch
   load_plan(fp, s_radix);
-  load_twiddles(tw, bd.tw + fp.tw_off, W);
   const size_t g0 = (size_t)gi.grid_off + (size_t)y * W;
   if (real_in) {
     for (int i = threadIdx.x; i < W; i += blockDim.x) a[i] = make_float2(real_in[g0 + i], 0.0f);
-    __syncthreads();
   } else {
-    deposit_row(row, W, row_off, row_list, sort_tmp, key_tmp, key_half, cic_cell, cic_w, a,
-                sort_lds, keys_lds, sort_keys);
+    deposit_row(row, W, dep_cap, row_off, row_list, sort_tmp, key_tmp, key_half, cic_cell, cic_w,
+                a, reinterpret_cast<char*>(lds + W));
     if (density_out)
       for (int i = threadIdx.x; i < W; i += blockDim.x) density_out[g0 + i] = a[i].x;
   }
+  load_twiddles(tw, bd.tw + fp.tw_off, W);
+  __syncthreads();
   float2* res = fft_lds(a, b, W, W, 1, s_radix, fp.nrad, tw, false);
   float2* sp = specT + gi.grid_off;
   for (int k = threadIdx.x; k < W; k += blockDim.x) sp[(size_t)k * H + y] = res[k];
@@ -453,15 +455,19 @@ hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t 
   return hipGetLastError();
 }
 
-size_t fft_rows_lds_bytes(int max_w) { return (size_t)3 * max_w * sizeof(float2); }
-size_t fft_cols_lds_bytes(int max_h) { return (size_t)3 * max_h * sizeof(float2); }
+// Workgroup size of an FFT pass: a power of two from one to four wavefronts,
+// about one radix-4 butterfly per thread and stage for the longest sequence.
+static int fft_threads(int n) { return n <= 256 ? 64 : (n <= 512 ? 128 : 256); }
 
-// Workgroup size of an FFT pass: one radix-4 butterfly per thread and stage
-// for the longest sequence, between one and four wavefronts.
-static int fft_threads(int n) {
-  const int t = ((n / 4 + 63) / 64) * 64;
-  return t < 64 ? 64 : (t > 256 ? 256 : t);
+// LDS of the forward row pass: the row, then the larger of (second buffer +
+// twiddles) and the deposit scratch of DEP_CAP_MIN nodes.
+static size_t rows_region_bytes(int max_w) {
+  const size_t fft = (size_t)2 * max_w * sizeof(float2);
+  const size_t dep = (size_t)DEP_CAP_MIN * DEP_BYTES_PER_NODE;
+  return fft > dep ? fft : dep;
 }
+size_t fft_rows_lds_bytes(int max_w) { return (size_t)max_w * sizeof(float2) + rows_region_bytes(max_w); }
+size_t fft_cols_lds_bytes(int max_h) { return (size_t)3 * max_h * sizeof(float2); }
 
 hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const int* row_off,
                                int* row_list, int* sort_scratch, unsigned long long* key_scratch,
@@ -469,13 +475,16 @@ hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const in
                                float2* specT, const float* kfT, float* density_out, float* out,
                                hipStream_t s) {
   const int tr = fft_threads(b.max_w), tc = fft_threads(b.max_h);
+  // rows of any instance (W <= max_w) have at least this region after the row
+  int cap = (int)(rows_region_bytes(b.max_w) / DEP_BYTES_PER_NODE) & ~1;
+  if (cap > DEP_CAP_MAX) cap = DEP_CAP_MAX;
   hipLaunchKernelGGL(k_rows_fwd_T, dim3(b.rows), dim3(tr), fft_rows_lds_bytes(b.max_w), s, b,
-                     real_in, row_off, row_list, sort_scratch, key_scratch, key_half, cic_cell,
+                     real_in, cap, row_off, row_list, sort_scratch, key_scratch, key_half, cic_cell,
                      cic_w, specT, density_out);
   hipLaunchKernelGGL(k_cols_T, dim3(b.cols), dim3(tc), fft_cols_lds_bytes(b.max_h), s, b, specT,
                      kfT);
-  hipLaunchKernelGGL(k_rows_inv_T, dim3(b.rows), dim3(tr), fft_rows_lds_bytes(b.max_w), s, b,
-                     specT, out);
+  hipLaunchKernelGGL(k_rows_inv_T, dim3(b.rows), dim3(tr), (size_t)3 * b.max_w * sizeof(float2), s,
+                     b, specT, out);
   return hipGetLastError();
 }
 

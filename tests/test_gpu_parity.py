@@ -192,11 +192,13 @@ def _compare_states(tag, g, o, o64):
         "f": rel_l2(gf, o.f_grid),
         "theta": rel_l2(gth, o.theta),
         "steric": rel_l2(_np(g[2].steric), o.steric),
+        "rho": rel_l2(gf.astype(np.float64).sum(axis=0), o.f_grid.astype(np.float64).sum(axis=0)),
     }
     r64 = {"pos": pos_rel_l2(o.grid_shape, gpos, o64.position),
-           "u": rel_l2(gu, o64.fluid_velocity), "f": rel_l2(gf, o64.f_grid)}
+           "u": rel_l2(gu, o64.fluid_velocity), "f": rel_l2(gf, o64.f_grid)} if o64 else None
     d64 = {"pos": pos_rel_l2(o.grid_shape, o.position, o64.position),
-           "u": rel_l2(o.fluid_velocity, o64.fluid_velocity), "f": rel_l2(o.f_grid, o64.f_grid)}
+           "u": rel_l2(o.fluid_velocity, o64.fluid_velocity),
+           "f": rel_l2(o.f_grid, o64.f_grid)} if o64 else None
     _say(f"{tag}: gpu-vs-fp32 {r}  gpu-vs-fp64 {r64}  fp32-vs-fp64 {d64}")
     return r
 
@@ -236,3 +238,74 @@ def test_rollout_parity_100(dev, use_graph):
     r = _compare_states("plain128 100 steps", (ro.nodes(), ro.edges_now(), ro.fields_now()), o, o64)
     for k in ("pos", "u", "f", "theta"):
         assert r[k] <= 1e-5, (k, r)
+
+
+def test_rollout_parity_1000(dev):
+    """1000 steps of the 128^2 plain case: density, velocity, positions within
+    relative L2 1e-3 of the float32 restatement (and 1e-5 at step 100 on the
+    way, from the same run)."""
+    _say("rollout parity 128^2 1000 steps")
+    from microcosmos_amd import Rollout
+    wl = W.plain_128()
+    p = oracle_params(wl)
+    nodes, edges, fields = device_state(wl, dev)
+    cfg = device_config(wl)
+    ro = Rollout(nodes, edges, fields, cfg, wl.dt,
+                 actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
+    s = oracle_state(wl)
+    s64 = oracle_state(wl, np.float64)
+    done = 0
+    for stop, tol in ((100, 1e-5), (1000, 1e-3)):
+        ro.run(stop - done)
+        torch.cuda.synchronize()
+        for t in range(done, stop):
+            s = R.step(replace(s, bending_rest_angles=wl.rest_angles_at(t)), wl.dt, p)
+            s64 = R.step(replace(s64, bending_rest_angles=wl.rest_angles_at(t).astype(np.float64)),
+                         wl.dt, p)
+        done = stop
+        r = _compare_states(f"plain128 {stop} steps", (ro.nodes(), ro.edges_now(), ro.fields_now()),
+                            s, s64)
+        for k in ("pos", "u", "rho", "f"):
+            assert r[k] <= tol, (stop, k, r)
+
+
+def test_plain_1024_two_steps(dev):
+    """The full 1024^2, 256 x 64 workload against the float32 restatement over
+    two coupled steps (graph-replayed rollout)."""
+    _say("plain 1024^2 two steps vs oracle")
+    from microcosmos_amd import Rollout
+    wl = W.plain_1024()
+    p = oracle_params(wl)
+    o = run_oracle(wl, 2, p)
+    nodes, edges, fields = device_state(wl, dev)
+    ro = Rollout(nodes, edges, fields, device_config(wl), wl.dt,
+                 actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
+    ro.run(2)
+    torch.cuda.synchronize()
+    r = _compare_states("plain1024 2 steps", (ro.nodes(), ro.edges_now(), ro.fields_now()), o, None)
+    for k in ("pos", "u", "rho", "f", "theta", "steric"):
+        assert r[k] <= 1e-5, (k, r)
+
+
+def test_plain_1024_deterministic(dev):
+    """Two 50-step rollouts of the 1024^2 workload from the same state give
+    bit-identical lattices, velocities and node states: no result depends on
+    the order in which workgroups or atomics run."""
+    _say("plain 1024^2 determinism 2 x 50 steps")
+    from microcosmos_amd import Rollout
+    wl = W.plain_1024()
+    outs = []
+    for _ in range(2):
+        nodes, edges, fields = device_state(wl, dev)
+        ro = Rollout(nodes, edges, fields, device_config(wl), wl.dt,
+                     actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
+        ro.run(50)
+        torch.cuda.synchronize()
+        fl = ro.fields_now()
+        outs.append([_np(x).copy() for x in (ro.nodes().position, ro.nodes().velocity,
+                                             ro.edges_now().theta, fl.f_grid, fl.fluid_velocity,
+                                             fl.steric)])
+        del ro
+    for a, b in zip(*outs):
+        assert np.isfinite(a).all()
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
