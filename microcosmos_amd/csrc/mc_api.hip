@@ -80,13 +80,14 @@ struct mc_plan {
   int* comp_of_node = nullptr;  // component of every node
   size_t rod_lds_bytes = 0;
   int rod_use_lds = 1;
+  bool rod_fuse = false;        // rollouts may run an LBM pass inside the rod launch
   float* rod_gscratch = nullptr;
   // steric
   float* pred_w = nullptr;
   int* cic_cell = nullptr;
   float* cic_w = nullptr;
-  int *row_cnt = nullptr, *row_off = nullptr, *row_fill = nullptr;
-  int *row_list = nullptr, *row_sort = nullptr;
+  int2* comp_rows = nullptr;    // per component: steric row band
+  int *dep_arena = nullptr, *dep_top = nullptr, *row_sort = nullptr;
   unsigned long long* row_keys = nullptr;
   float2* spec = nullptr;
   std::vector<int2> shapes;     // distinct (H, W) steric grids, each with a kf table
@@ -101,7 +102,6 @@ struct mc_plan {
   float* kf = nullptr;          // Gaussian transfer functions, one table per distinct shape
   double kf_sigma = -1.0;       // sigma the table holds (-1: none)
   hipStream_t cap_stream = nullptr;
-  hipStream_t side_stream = nullptr;
   TrackWork track{};            // displacement tracking (off: n_groups == 0)
   TrackWork inst_track{};       // per-instance tracker of mc_batch_rollout
 
@@ -235,8 +235,11 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
     maxb = std::max(maxb, b);
   }
   p->rod_lds_bytes = maxb;
-  p->rod_use_lds = p->rod_lds_bytes <= (size_t)p->lds_max ? 1 : 0;
+  p->rod_use_lds = p->rod_lds_bytes + 1024 <= (size_t)p->lds_max ? 1 : 0;
   if (!p->rod_use_lds) MCCHK(p->alloc(&p->rod_gscratch, rod_scratch_floats(N, E, B), "rod scratch"));
+  // the fused launch gives every workgroup the rod's LDS: keep it small enough
+  // for the LBM workgroups to reach about 20 wavefronts per CU
+  p->rod_fuse = !p->rod_use_lds || p->rod_lds_bytes <= 8192;
   // per-component topology blocks and contiguous id ranges
   std::vector<int> blob, blob_off(1, 0), nfirst(C), efirst(C), hfirst(C);
   auto first_of = [](const std::vector<int>& off, const std::vector<int>& order, int c) {
@@ -302,6 +305,15 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
   MCCHK(upload(p, &p->comp_of_node, comp_of_node, "comp_of_node"));
   std::vector<int> comp_inst(C);
   for (int c = 0; c < C; ++c) comp_inst[c] = p->node_inst_h[norder[noff[c]]];
+  // components are numbered by their smallest node and instances own
+  // contiguous node ranges, so each instance's components are contiguous
+  for (int i = 0, c = 0; i < p->n_inst; ++i) {
+    p->geo_h[i].comp0 = c;
+    while (c < C && comp_inst[c] == i) ++c;
+    p->geo_h[i].ncomp = c - p->geo_h[i].comp0;
+    if (i + 1 == p->n_inst && c != C)
+      return fail(MC_ERR_BAD_SHAPE, "filaments are not grouped by instance");
+  }
   int* ci_d;
   MCCHK(upload(p, &ci_d, comp_inst, "comp_inst"));
   p->bd.comp_inst = ci_d;
@@ -385,6 +397,8 @@ mc_status build_instances(mc_plan* p) {
   b.lbm_blocks = p->lbm_blocks;
   b.max_w = p->max_w;
   b.max_h = p->max_h;
+  b.max_nodes = 0;
+  for (const InstHost& h : p->inst) b.max_nodes = std::max(b.max_nodes, h.nodes);
   return MC_OK;
 }
 
@@ -549,31 +563,38 @@ mc_status ensure_kf(mc_plan* p, double sigma, hipStream_t s) {
 // writes the filtered field to steric_out.
 mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float* vel,
                      float* steric_out, float* density_out, hipStream_t s) {
-  HIPCHK(launch_predict(c, p->bd, p->N, pos, vel, p->pred_w, p->cic_cell, p->cic_w, p->row_cnt, s));
+  HIPCHK(launch_predict(c, p->bd, p->L, pos, vel, p->pred_w, p->cic_cell, p->cic_w, p->comp_rows,
+                        p->dep_top, s));
   if (!c.enable_steric) return MC_OK;
-  HIPCHK(launch_bin_scan(p->rows, p->row_cnt, p->row_off, p->row_fill, s));
-  HIPCHK(launch_row_fill(c, p->N, p->cic_cell, p->row_off, p->row_fill, p->row_list, s));
-  HIPCHK(launch_steric_fft_T(p->bd, nullptr, p->row_off, p->row_list, p->row_sort, p->row_keys,
-                             (size_t)4 * p->N, p->cic_cell, p->cic_w, p->spec, p->kf, density_out,
-                             steric_out, s));
+  StericDep d;
+  d.L = p->L;
+  d.comp_rows = p->comp_rows;
+  d.cic_cell = p->cic_cell;
+  d.cic_w = p->cic_w;
+  d.arena = p->dep_arena;
+  d.arena_top = p->dep_top;
+  d.sort_tmp = p->row_sort;
+  d.key_tmp = p->row_keys;
+  d.key_half = (size_t)4 * p->N;
+  HIPCHK(launch_steric_fft_T(p->bd, nullptr, &d, p->spec, p->kf, density_out, steric_out, s));
   return MC_OK;
 }
 
-// Streams and events of one enqueued step.  With a side stream the fused LBM
-// pass of this step runs there, concurrently with the steric and rod work of
-// the next step; only the next step's IB (which reads the new lattice and
-// rewrites the force and support mask) waits for it.
+// Optional timing events around the LBM launch of one step.
 struct StepSync {
-  hipStream_t b = nullptr;          // side stream for the LBM pass (nullptr: main stream)
-  hipEvent_t ev_force = nullptr;    // main -> side: IB force of this step is ready
-  hipEvent_t ev_lbm = nullptr;      // side -> main: this step's lattice is written
-  hipEvent_t wait_lbm = nullptr;    // previous step's ev_lbm (nullptr: nothing pending)
-  hipEvent_t t0 = nullptr, t1 = nullptr;  // optional timing around the LBM launch
+  hipEvent_t t0 = nullptr, t1 = nullptr;
 };
 
+// Enqueues one coupled step on `s`.  The LBM pass of the step may be deferred
+// (defer_lbm != NULL: its arguments are returned instead of launched); the
+// next step then runs it inside its rod launch (fuse_lbm), where it fills the
+// CUs the filaments leave idle.  That is valid because the rod and steric
+// work of the next step neither read the lattice nor the IB force, and its IB
+// (which reads the new lattice) is enqueued after the rod launch.
 mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc_state* in,
                        const mc_state* out, const mc_edge_params* ed, const Actuation* act,
-                       hipStream_t s, const StepSync& sy = StepSync()) {
+                       hipStream_t s, const StepSync& sy = StepSync(),
+                       const LbmArgs* fuse_lbm = nullptr, LbmArgs* defer_lbm = nullptr) {
   MCCHK(run_steric(p, c, in->position, in->velocity, out->steric, nullptr, s));
   if (!c.enable_steric && out->steric != in->steric)
     HIPCHK(hipMemcpyAsync(out->steric, in->steric, (size_t)p->grid_cells * sizeof(float),
@@ -581,24 +602,21 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
   HIPCHK(launch_rod(c, p->bd, p->L, p->N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 0,
                     in->position, in->velocity, nullptr, in->theta, ed->rest_lengths,
                     ed->bending_rest_angles, ed->bending_stiffness, act, out->steric, p->pred_w,
-                    out->position, out->velocity, out->debug_vector, out->theta, s));
+                    out->position, out->velocity, out->debug_vector, out->theta, fuse_lbm, s));
   const size_t hwf = (size_t)p->lat_cells;
   long long* tick = act ? p->tick : nullptr;
   if (c.enable_fluid) {
-    if (sy.wait_lbm) HIPCHK(hipStreamWaitEvent(s, sy.wait_lbm, 0));
     HIPCHK(launch_ib(c, p->bd, p->ib, in->f_grid, out->position, out->velocity, out->position, tick,
                      s));
-    hipStream_t ls = s;
-    if (sy.b) {
-      HIPCHK(hipEventRecord(sy.ev_force, s));
-      HIPCHK(hipStreamWaitEvent(sy.b, sy.ev_force, 0));
-      ls = sy.b;
+    if (defer_lbm) {
+      *defer_lbm = make_lbm_args(c, p->bd, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
+                                 p->ib.force);
+    } else {
+      if (sy.t0) HIPCHK(hipEventRecord(sy.t0, s));
+      HIPCHK(launch_lbm(c, p->bd, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
+                        p->ib.force, s));
+      if (sy.t1) HIPCHK(hipEventRecord(sy.t1, s));
     }
-    if (sy.t0) HIPCHK(hipEventRecord(sy.t0, ls));
-    HIPCHK(launch_lbm(c, p->bd, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
-                      p->ib.force, ls));
-    if (sy.t1) HIPCHK(hipEventRecord(sy.t1, ls));
-    if (sy.b) HIPCHK(hipEventRecord(sy.ev_lbm, sy.b));
   } else {
     if (tick) HIPCHK(launch_tick(tick, s));
     if (out->f_grid != in->f_grid)
@@ -693,15 +711,13 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
     return bail(fail(MC_ERR_HIP, "upload of the instance geometry failed"));
   const int N = p->N;
   const size_t hwf = (size_t)p->lat_cells;
-  const size_t R = (size_t)p->rows;
   if ((st = p->alloc(&p->pred_w, 2 * (size_t)N, "pred")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->node_tmp, 2 * (size_t)N, "node tmp")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->cic_cell, 4 * (size_t)N, "cic cell")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->cic_w, 4 * (size_t)N, "cic w")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->row_cnt, R, "row cnt")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->row_off, R + 1, "row off")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->row_fill, R, "row fill")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->row_list, 2 * (size_t)N, "row list")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->comp_rows, (size_t)p->L.num_comp, "comp rows")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->dep_arena, 2 * (size_t)N, "deposit arena")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->dep_top, 1, "deposit arena top")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_sort, 2 * (size_t)N, "row sort")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_keys, 8 * (size_t)N, "row keys")) != MC_OK) return bail(st);
   // immersed boundary
@@ -744,6 +760,10 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
   }
   if (ib_prepare(p->lds_max) != hipSuccess)
     return bail(fail(MC_ERR_HIP, "cluster solve: %zu B of dynamic LDS refused", w.lds_bytes));
+  if (rod_prepare(p->lds_max) != hipSuccess)
+    return bail(fail(MC_ERR_HIP, "rod kernels: %d B of dynamic LDS refused", p->lds_max - 1024));
+  if (steric_prepare(p->lds_max) != hipSuccess)
+    return bail(fail(MC_ERR_HIP, "steric kernels: %d B of dynamic LDS refused", p->lds_max - 1024));
   if ((st = p->alloc(&w.force, 2 * hwf, "force U")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.mask, (size_t)p->mask_words, "U mask")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->f_tmp, 9 * hwf, "f ping-pong")) != MC_OK) return bail(st);
@@ -782,7 +802,6 @@ mc_status mc_plan_destroy(mc_plan* p) {
   if (!p) return MC_OK;
   for (void* q : p->allocs) (void)hipFree(q);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
-  if (p->side_stream) (void)hipStreamDestroy(p->side_stream);
   delete p;
   return MC_OK;
 }
@@ -860,10 +879,10 @@ mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const m
     }
   } else {
     if (!p->cap_stream) HIPCHK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
-    if (!p->side_stream) HIPCHK(hipStreamCreateWithFlags(&p->side_stream, hipStreamNonBlocking));
-    hipStream_t cs = p->cap_stream, bs = p->side_stream;
+    hipStream_t cs = p->cap_stream;
     // one graph holds S steps (S even keeps the lattice parity); inside it
-    // the LBM pass of step k overlaps step k+1 up to its IB solve
+    // the LBM pass of step k runs inside step k+1's rod launch, and the last
+    // step's LBM pass runs on its own
     const int64_t S = num_steps >= 8 ? 8 : (num_steps >= 2 ? 2 : 0);
     std::vector<hipEvent_t> evs;
     auto mkev = [&](hipEvent_t* e) -> mc_status {
@@ -878,28 +897,16 @@ mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const m
     hipGraphExec_t ge = nullptr;
     mc_status r = MC_OK;
     if (S > 0) {
-      std::vector<hipEvent_t> evf(S), evl(S);
-      for (int64_t k = 0; k < S && r == MC_OK; ++k) {
-        r = mkev(&evf[k]);
-        if (r == MC_OK) r = mkev(&evl[k]);
-      }
-      if (r != MC_OK) return r;
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      LbmArgs pend[2];
+      const LbmArgs* fuse = nullptr;
       for (int64_t k = 0; k < S && r == MC_OK; ++k) {
         const int par = (int)(k & 1);
-        StepSync sy;
-        if (q->enable_fluid) {
-          sy.b = bs;
-          sy.ev_force = evf[k];
-          sy.ev_lbm = evl[k];
-          sy.wait_lbm = k > 0 ? evl[k - 1] : nullptr;
-        }
-        r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, sy);
-      }
-      if (r == MC_OK && q->enable_fluid) {
-        hipError_t e = hipStreamWaitEvent(cs, evl[S - 1], 0);  // join the side stream
-        if (e != hipSuccess) r = fail(MC_ERR_HIP, "join: %s", hipGetErrorString(e));
+        LbmArgs* defer = (q->enable_fluid && p->rod_fuse && k + 1 < S) ? &pend[par] : nullptr;
+        r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, StepSync(), fuse,
+                         defer);
+        fuse = defer;
       }
       hipError_t ec = hipStreamEndCapture(cs, &g);
       if (r != MC_OK) return r;
@@ -991,8 +998,7 @@ mc_status mc_gaussian_filter(mc_plan* p, const float* data, float* out, double s
   if (!p->fft_ok) return fail(MC_ERR_UNSUPPORTED, "%s", p->fft_why.c_str());
   hipStream_t s = (hipStream_t)stream;
   MCCHK(ensure_kf(p, sigma, s));
-  HIPCHK(launch_steric_fft_T(p->bd, data, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
-                             p->spec, p->kf, nullptr, out, s));
+  HIPCHK(launch_steric_fft_T(p->bd, data, nullptr, p->spec, p->kf, nullptr, out, s));
   return MC_OK;
 }
 
@@ -1046,7 +1052,7 @@ mc_status mc_rod_solve(mc_plan* p, const mc_params* q, float* position, float* t
   c.cycles = cycles;
   HIPCHK(launch_rod(c, p->bd, p->L, p->N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 1, position,
                     nullptr, y, theta, ed->rest_lengths, ed->bending_rest_angles, ed->bending_stiffness,
-                    nullptr, nullptr, nullptr, position, nullptr, nullptr, theta, s));
+                    nullptr, nullptr, nullptr, position, nullptr, nullptr, theta, nullptr, s));
   return MC_OK;
 }
 

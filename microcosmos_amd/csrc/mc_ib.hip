@@ -865,69 +865,7 @@ __global__ void k_ib_reset(int HW, int C, int* __restrict__ owner, int* __restri
   if (i < C) parent[i] = i;
 }
 
-// Exclusive scan of T bin counts (one workgroup): offsets off[0..T], zeroed
-// fill cursors, and the counts cleared for the next use.  Each pass covers
-// 8 consecutive bins per thread (two 16-byte loads), so the loop runs
-// T / 8192 times.
-__global__ void __launch_bounds__(1024) k_bin_scan(int T, int* __restrict__ cnt,
-                                                   int* __restrict__ off, int* __restrict__ fill) {
-  __shared__ int wsum[16];
-  constexpr int IT = 8;
-  const int tile = blockDim.x * IT;
-  int carry = 0;
-  for (int base = 0; base < T; base += tile) {
-    const int i0 = base + threadIdx.x * IT;
-    const bool full = i0 + IT <= T;
-    int v[IT];
-    if (full) {
-      const int4 a = *reinterpret_cast<const int4*>(cnt + i0);
-      const int4 b = *reinterpret_cast<const int4*>(cnt + i0 + 4);
-      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-      v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    } else {
-#pragma unroll
-      for (int k = 0; k < IT; ++k) v[k] = i0 + k < T ? cnt[i0 + k] : 0;
-    }
-    int sum = 0;
-#pragma unroll
-    for (int k = 0; k < IT; ++k) sum += v[k];
-    int tot;
-    const int inc = block_inclusive_scan(sum, wsum, &tot);
-    int run = carry + inc - sum;
-    int o[IT];
-#pragma unroll
-    for (int k = 0; k < IT; ++k) {
-      o[k] = run;
-      run += v[k];
-    }
-    if (full) {
-      const int4 z = make_int4(0, 0, 0, 0);
-      *reinterpret_cast<int4*>(off + i0) = make_int4(o[0], o[1], o[2], o[3]);
-      *reinterpret_cast<int4*>(off + i0 + 4) = make_int4(o[4], o[5], o[6], o[7]);
-      *reinterpret_cast<int4*>(fill + i0) = z;
-      *reinterpret_cast<int4*>(fill + i0 + 4) = z;
-      *reinterpret_cast<int4*>(cnt + i0) = z;
-      *reinterpret_cast<int4*>(cnt + i0 + 4) = z;
-    } else {
-#pragma unroll
-      for (int k = 0; k < IT; ++k)
-        if (i0 + k < T) {
-          off[i0 + k] = o[k];
-          fill[i0 + k] = 0;
-          cnt[i0 + k] = 0;
-        }
-    }
-    carry += tot;
-  }
-  if (threadIdx.x == 0) off[T] = carry;
-}
-
 }  // namespace
-
-hipError_t launch_bin_scan(int T, int* cnt, int* off, int* fill, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, T, cnt, off, fill);
-  return hipGetLastError();
-}
 
 size_t ib_scratch_bytes_per_node(int kmax) {
   // glayout(nq, nq * K2): nq * (28 + 40 K2) + 4 + 13 arrays x 15 B of padding, nq >= 1

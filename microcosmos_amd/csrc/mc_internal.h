@@ -49,6 +49,7 @@ struct InstGeo {
   int row0, col0;             // first global steric row / column
   int lbm_blk0;               // first LBM workgroup
   int node0, nodes;           // node range
+  int comp0, ncomp;           // filament (connected component) range
   int fft_w, fft_h;           // FFT plan of length W / of length H
   long long kf_off;           // transposed transfer-function table of the (H, W) shape
   unsigned long long wdiv;    // ceil(2^40 / Wf): y = (cell * wdiv) >> 40 (0: divide)
@@ -73,6 +74,7 @@ struct BatchDev {
   const float2* tw;           // concatenated twiddle tables
   int rows, cols, lbm_blocks; // totals over instances
   int max_w, max_h;           // largest steric grid sides
+  int max_nodes;              // largest instance node count
 };
 
 constexpr int LBM_THREADS = 256;  // lattice cells per LBM workgroup
@@ -113,23 +115,43 @@ struct Actuation {           // rest_b(t) = amp_b * sin(omega_b * t + phase_b)
   const long long* t;        // device step counter (advanced by launch_tick)
 };
 
+// Arguments of one LBM pass (fused collide-stream of every instance).
+struct LbmArgs {
+  BatchDev bd;
+  const float* fin;
+  float *fout, *ustar;
+  uint32_t* mask;
+  const float* force;
+  float op, om, cp, cm;
+  int force_clamp;
+};
+
 // ---- launchers (stream-ordered, no host synchronisation) ----
 // steric
-hipError_t launch_predict(const Consts& c, const BatchDev& b, int N, const float* pos,
-                          const float* vel, float* pred_w, int* cic_cell, float* cic_w,
-                          int* row_cnt, hipStream_t s);
-// exclusive scan of T counts into off[0..T]; zeroes fill[0..T) and cnt[0..T)
-// (cnt, off and fill 16-byte aligned)
-hipError_t launch_bin_scan(int T, int* cnt, int* off, int* fill, hipStream_t s);
-hipError_t launch_row_fill(const Consts& c, int N, const int* cic_cell, const int* row_off,
-                           int* row_fill, int* row_list, hipStream_t s);
+// Per node: wrapped prediction, CIC cell and weights; per filament: the band
+// of steric rows its CIC entries touch (comp_rows); clears *arena_top.
+hipError_t launch_predict(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
+                          const float* pos, const float* vel, float* pred_w, int* cic_cell,
+                          float* cic_w, int2* comp_rows, int* arena_top, hipStream_t s);
+// Raises the dynamic LDS limit of the steric kernels to lds_max - 1024 bytes.
+hipError_t steric_prepare(int lds_max);
 hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t s);
-// Whole steric filter over every instance: forward rows (CIC deposit when
-// real_in is NULL), transfer function on the transposed spectrum, inverse
-// rows into `out`.
-hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const int* row_off,
-                               int* row_list, int* sort_scratch, unsigned long long* key_scratch,
-                               size_t key_half, const int* cic_cell, const float* cic_w,
+// Inputs of the CIC deposit (launch_predict's outputs and scratch).
+struct StericDep {
+  RodLayoutDev L;
+  const int2* comp_rows;
+  const int* cic_cell;
+  const float* cic_w;
+  int* arena;                   // 2N ints
+  int* arena_top;               // 1 int, cleared by launch_predict
+  int* sort_tmp;                // 2N ints
+  unsigned long long* key_tmp;  // 8N
+  size_t key_half;              // 4N
+};
+// Whole steric filter over every instance: forward rows (CIC deposit from
+// `dep` when real_in is NULL), transfer function on the transposed spectrum,
+// inverse rows into `out`.
+hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const StericDep* dep,
                                float2* specT, const float* kfT, float* density_out, float* out,
                                hipStream_t s);
 size_t fft_rows_lds_bytes(int max_w);
@@ -137,13 +159,17 @@ size_t fft_cols_lds_bytes(int max_h);
 // rod
 // mode 0: full node phase of a step (prologue + constraint cycles + velocity)
 // mode 1: constraint cycles only, positions in place, y = Cosserat target
+// lbm (may be NULL): an LBM pass independent of this rod solve, run by extra
+// workgroups of the same launch (it fills the CUs the filaments leave idle)
 hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L, int N,
                       size_t lds_bytes, int use_lds, float* gscratch, int mode,
                       const float* pos_in, const float* vel_in, const float* y_in,
                       const float* theta_in, const float* rest_len, const float* rest_ang,
                       const float* kb, const Actuation* act, const float* steric,
                       const float* pred_w, float* pos_out, float* vel_out, float* debug_out,
-                      float* theta_out, hipStream_t s);
+                      float* theta_out, const LbmArgs* lbm, hipStream_t s);
+// Raises the dynamic LDS limit of the rod kernels to lds_max - 1024 bytes.
+hipError_t rod_prepare(int lds_max);
 hipError_t launch_steric_nodes(const Consts& c, const BatchDev& b, int N, const float* steric,
                                const float* pred_w, float* force_out, hipStream_t s);
 hipError_t launch_tick(long long* t, hipStream_t s);
@@ -196,6 +222,8 @@ hipError_t launch_com(int n_groups, const int* off, const float* pos, int H, int
 // mode 0: reset (prev = centre, cumulative = 0); mode 1: one step's update
 hipError_t launch_track(const TrackWork& t, const float* pos, int mode, hipStream_t s);
 // LBM
+LbmArgs make_lbm_args(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
+                      float* ustar, uint32_t* mask, const float* force);
 hipError_t launch_lbm(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
                       float* ustar, uint32_t* mask, const float* force, hipStream_t s);
 

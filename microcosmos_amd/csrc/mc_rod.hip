@@ -15,6 +15,7 @@
 // its epilogue, v = (x - x_prev) / dt.
 #include "mc_common.h"
 #include "mc_internal.h"
+#include "mc_lbm.h"
 
 namespace mc {
 namespace {
@@ -84,11 +85,9 @@ __device__ __forceinline__ float wrap_angle(float d) {
   return fmodf_floor(d + MC_PI_F, MC_2PI_F) - MC_PI_F;
 }
 
-__global__ void __launch_bounds__(256) k_rod(RodArgs a) {
-  extern __shared__ float smem[];
+__device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float* smem) {
   const Consts& c = a.c;
   const RodLayoutDev& L = a.L;
-  const int comp = blockIdx.x;
   const CompMeta m = L.meta[comp];
   const InstGeo& gi = a.bd.geo[m.inst];
   const int n0 = m.n0, nn = m.nn, e0 = m.e0, ne = m.ne, b0 = m.b0, nb = m.nb;
@@ -359,6 +358,29 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   for (int e = threadIdx.x; e < ne; e += blockDim.x) a.theta_out[L.comp_edges[e0 + e]] = TH[e];
 }
 
+__global__ void __launch_bounds__(256) k_rod(RodArgs a) {
+  extern __shared__ float smem[];
+  rod_component(a, blockIdx.x, smem);
+}
+
+// Rod components in workgroups [0, rod_blocks), then an LBM pass in 64-cell
+// workgroups (a quarter of an LBM_THREADS block each, one wavefront, so the
+// U-mask words stay within a wavefront).  rod_blocks is a multiple of 8:
+// workgroup i runs on XCD i % 8, and the LBM part keeps the standalone
+// kernel's XCD-banded order.
+__global__ void __launch_bounds__(256) k_rod_lbm(RodArgs a, LbmArgs l, int rod_blocks) {
+  extern __shared__ float smem[];
+  const int bid = blockIdx.x;
+  if (bid < rod_blocks) {
+    if (bid < a.L.num_comp) rod_component(a, bid, smem);
+    return;
+  }
+  int v = bid - rod_blocks;
+  const int G = gridDim.x - rod_blocks;
+  if ((G & 7) == 0) v = (v & 7) * (G >> 3) + (v >> 3);
+  lbm_cell(l, v >> 2, (v & 3) * 64 + threadIdx.x);
+}
+
 }  // namespace
 
 size_t rod_scratch_floats(int nn, int ne, int nb) {
@@ -395,13 +417,24 @@ hipError_t launch_tick(long long* t, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t rod_prepare(int lds_max) {
+  // components beyond 64 KB of LDS (about 2000 nodes) need the raised limit
+  const int lds = lds_max - 1024;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rod),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rod_lbm),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  return e;
+}
+
 hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L, int N,
                       size_t lds_bytes, int use_lds, float* gscratch, int mode, const float* pos_in, const float* vel_in,
                       const float* y_in, const float* theta_in, const float* rest_len,
                       const float* rest_ang, const float* kb, const Actuation* act,
                       const float* steric, const float* pred_w, float* pos_out, float* vel_out,
-                      float* debug_out, float* theta_out, hipStream_t s) {
-  if (L.num_comp == 0) return hipSuccess;
+                      float* debug_out, float* theta_out, const LbmArgs* lbm, hipStream_t s) {
+  if (L.num_comp == 0 && !lbm) return hipSuccess;
   RodArgs a;
   a.c = c;
   a.bd = b;
@@ -425,7 +458,13 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
   a.vel_out = vel_out;
   a.debug_out = debug_out;
   a.theta_out = theta_out;
-  hipLaunchKernelGGL(k_rod, dim3(L.num_comp), dim3(64), use_lds ? lds_bytes : 0, s, a);
+  const size_t lds = use_lds ? lds_bytes : 0;
+  if (lbm) {
+    const int rb = (L.num_comp + 7) & ~7;
+    hipLaunchKernelGGL(k_rod_lbm, dim3(rb + 4 * b.lbm_blocks), dim3(64), lds, s, a, *lbm, rb);
+  } else {
+    hipLaunchKernelGGL(k_rod, dim3(L.num_comp), dim3(64), lds, s, a);
+  }
   return hipGetLastError();
 }
 

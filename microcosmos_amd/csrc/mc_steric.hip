@@ -156,57 +156,83 @@ __device__ float2* fft_lds(float2* a, float2* b, int n, int stride, int batch, c
   return src;
 }
 
-__global__ void k_predict(Consts c, BatchDev bd, int N, const float* __restrict__ pos,
-                          const float* __restrict__ vel, float* __restrict__ pred_w,
-                          int* __restrict__ cic_cell, float* __restrict__ cic_w,
-                          int* __restrict__ row_cnt) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  const InstGeo& gi = bd.geo[bd.node_inst[n]];
+// Wrapped inertial prediction of node n (simulate.py:46-50, forces.py:95);
+// without vel, the positions as given.
+__device__ __forceinline__ void predict_xy(const Consts& c, const InstGeo& gi,
+                                           const float* __restrict__ pos,
+                                           const float* __restrict__ vel, int n, float& x,
+                                           float& y) {
   float px = pos[2 * n], py = pos[2 * n + 1];
-  if (vel) {  // inertial prediction (simulate.py:46-50); without vel: positions as given
+  if (vel) {
     const float vx = vel[2 * n] * c.damping, vy = vel[2 * n + 1] * c.damping;
     px = px + vx * c.dt;
     py = py + vy * c.dt;
   }
-  const float x = fmodf_floor(px, gi.dom_wf), y = fmodf_floor(py, gi.dom_hf);
-  pred_w[2 * n] = x;
-  pred_w[2 * n + 1] = y;
-  if (!c.enable_steric) return;
-  const int W = gi.W, H = gi.H;
-  const float fx = floorf(x), fy = floorf(y);
-  const int x0 = imod(f2i(fx), W), y0 = imod(f2i(fy), H);
-  const int x1 = imod(x0 + 1, W), y1 = imod(y0 + 1, H);
-  const float dx = x - fx, dy = y - fy;
-  const float sv = c.steric_sv;
-  // columns within the instance, rows as global steric rows
-  cic_cell[4 * n + 0] = x0;
-  cic_cell[4 * n + 1] = x1;
-  cic_cell[4 * n + 2] = gi.row0 + y0;
-  cic_cell[4 * n + 3] = gi.row0 + y1;
-  cic_w[4 * n + 0] = ((1.0f - dx) * (1.0f - dy)) * sv;
-  cic_w[4 * n + 1] = (dx * (1.0f - dy)) * sv;
-  cic_w[4 * n + 2] = ((1.0f - dx) * dy) * sv;
-  cic_w[4 * n + 3] = (dx * dy) * sv;
-  atomicAdd(&row_cnt[gi.row0 + y0], 1);
-  if (y1 != y0) atomicAdd(&row_cnt[gi.row0 + y1], 1);
+  x = fmodf_floor(px, gi.dom_wf);
+  y = fmodf_floor(py, gi.dom_hf);
 }
 
-__global__ void k_row_fill(int N, const int* __restrict__ cic_cell, const int* __restrict__ row_off,
-                           int* __restrict__ row_fill, int* __restrict__ row_list) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  const int y0 = cic_cell[4 * n + 2], y1 = cic_cell[4 * n + 3];
-  int p = atomicAdd(&row_fill[y0], 1);
-  row_list[row_off[y0] + p] = n;
-  if (y1 != y0) {
-    p = atomicAdd(&row_fill[y1], 1);
-    row_list[row_off[y1] + p] = n;
+// One wavefront per filament (connected component of the edge graph): each
+// node's wrapped prediction, CIC cell and weights (forces.py:95-110), and the
+// band of instance rows the component's CIC entries touch, rows
+// start .. start + len - 1 (mod H).  A node's row y0 is the first node's row
+// plus a minimum-image offset d in [-H/2, H/2); the band runs from min d to
+// max d + 1, so it holds every y0 and y1 = y0 + 1 of the component whatever
+// its shape (a loose band only costs the row pass extra checks).  Block 0
+// also clears the deposit's overflow arena counter.
+__global__ void __launch_bounds__(64) k_predict(Consts c, BatchDev bd, RodLayoutDev L,
+                                                const float* __restrict__ pos,
+                                                const float* __restrict__ vel,
+                                                float* __restrict__ pred_w,
+                                                int* __restrict__ cic_cell,
+                                                float* __restrict__ cic_w,
+                                                int2* __restrict__ comp_rows,
+                                                int* __restrict__ arena_top) {
+  const int comp = blockIdx.x, lane = threadIdx.x;
+  if (comp == 0 && lane == 0) *arena_top = 0;
+  const CompMeta m = L.meta[comp];
+  const InstGeo& gi = bd.geo[m.inst];
+  const int H = gi.H, W = gi.W;
+  auto node_id = [&](int i) { return m.nf >= 0 ? m.nf + i : L.comp_nodes[m.n0 + i]; };
+  int yref = 0;
+  if (c.enable_steric) {
+    float x, y;
+    predict_xy(c, gi, pos, vel, node_id(0), x, y);
+    yref = imod(f2i(floorf(y)), H);
+  }
+  int lo = 0x7fffffff, hi = -0x7fffffff;
+  for (int i = lane; i < m.nn; i += 64) {
+    const int n = node_id(i);
+    float x, y;
+    predict_xy(c, gi, pos, vel, n, x, y);
+    pred_w[2 * n] = x;
+    pred_w[2 * n + 1] = y;
+    if (!c.enable_steric) continue;
+    const float fx = floorf(x), fy = floorf(y);
+    const int x0 = imod(f2i(fx), W), y0 = imod(f2i(fy), H);
+    const int x1 = imod(x0 + 1, W), y1 = imod(y0 + 1, H);
+    const float dx = x - fx, dy = y - fy;
+    const float sv = c.steric_sv;
+    // columns within the instance, rows as global steric rows
+    *reinterpret_cast<int4*>(cic_cell + 4 * (size_t)n) =
+        make_int4(x0, x1, gi.row0 + y0, gi.row0 + y1);
+    *reinterpret_cast<float4*>(cic_w + 4 * (size_t)n) =
+        make_float4(((1.0f - dx) * (1.0f - dy)) * sv, (dx * (1.0f - dy)) * sv,
+                    ((1.0f - dx) * dy) * sv, (dx * dy) * sv);
+    const int d = imod(y0 - yref + H / 2, H) - H / 2;
+    lo = min(lo, d);
+    hi = max(hi, d);
+  }
+  if (!c.enable_steric) return;
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o, 64));
+    hi = max(hi, __shfl_xor(hi, o, 64));
+  }
+  if (lane == 0) {
+    const int len = hi + 2 - lo;  // rows lo .. hi + 1 relative to yref
+    comp_rows[comp] = len >= H ? make_int2(0, H) : make_int2(imod(yref + lo, H), len);
   }
 }
-
-constexpr int DEP_KEYS_LDS = 512;    // 64-bit keys of one row's CIC entries held in LDS
-constexpr int DEP_IDS_LDS = 1024;    // node ids of one row sorted in LDS
 
 __device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ tw, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = tw[i];
@@ -217,39 +243,103 @@ __device__ __forceinline__ int xcd_group(int b, int G) {
   return (G & 7) == 0 ? (b & 7) * (G >> 3) + (b >> 3) : b;
 }
 
-// CIC deposit of global steric row `row` into dst[0..W): each node binned to
-// the row contributes two entries (corners q = 0, 1 if the row is its y0
-// row, q = 2, 3 if its y1 row).  Ordering the entries by (x, q, node rank)
-// puts each cell's contributions contiguously and in the reference's order
-// (all w00 by node, then w10, w01, w11 -- forces.py:112-115); a segment head
-// thread folds them from zero.  Rows of up to `cap` nodes order node ids and
-// entry keys by counting ranks in `scratch` (DEP_BYTES_PER_NODE bytes of LDS
-// per node, aliasing the FFT's second buffer and twiddles); longer rows
-// merge-sort in global scratch.
+// CIC deposit of global steric row `row` into dst[0..W): each node whose
+// CIC rows include the row contributes two entries (corners q = 0, 1 if the
+// row is its y0 row, q = 2, 3 if its y1 row).  Ordering the entries by (x, q,
+// node rank) puts each cell's contributions contiguously and in the
+// reference's order (all w00 by node, then w10, w01, w11 --
+// forces.py:112-115); a segment head thread folds them from zero.  Rows of up
+// to `cap` nodes order node ids and entry keys by counting ranks in `scratch`
+// (DEP_BYTES_PER_NODE bytes of LDS per node, aliasing the FFT's second buffer
+// and twiddles); longer rows take a slice of the global arena and merge-sort
+// in global scratch.
 constexpr int DEP_BYTES_PER_NODE = 36;  // sorted id + 2 entry keys + 2 ranked keys
 constexpr int DEP_CAP_MAX = 256;
 constexpr int DEP_CAP_MIN = 64;
 
-__device__ void deposit_row(int row, int W, int cap, const int* __restrict__ row_off,
-                            int* __restrict__ row_list, int* __restrict__ sort_tmp,
-                            unsigned long long* __restrict__ key_tmp, size_t key_half,
-                            const int* __restrict__ cic_cell, const float* __restrict__ cic_w,
-                            float2* __restrict__ dst, char* __restrict__ scratch) {
-  const int B = blockDim.x, tid = threadIdx.x;
-  const int off = row_off[row];
-  const int m = row_off[row + 1] - off;
-  const int ne = 2 * m;
+struct DepArgs {
+  RodLayoutDev L;
+  const int2* comp_rows;        // per component: first row and row count of its band
+  const int* cic_cell;
+  const float* cic_w;
+  int* arena;                   // 2N ids for rows beyond cap
+  int* arena_top;               // arena slots taken this step
+  int* sort_tmp;                // 2N ints
+  unsigned long long* key_tmp;  // 8N keys: entries at 2 aoff, merge scratch at key_half + 2 aoff
+  size_t key_half;
+  int cap;                      // LDS capacity in nodes (16 cap >= 12 blockDim)
+};
+
+// Ids of the nodes whose CIC rows include global row `row`, each once: the
+// instance's components whose band holds the row are compacted per chunk of
+// blockDim components (workgroup scans), then their nodes are checked.  Match
+// p goes to out[p] when p < lim.  `cand` holds 3 blockDim ints of LDS.
+// Returns the number of matches (the same in every thread).
+__device__ int collect_row_nodes(const DepArgs& a, const InstGeo& gi, int row, int* out, int lim,
+                                 int* cand, int* wsum, int* s_cnt) {
+  const int B = blockDim.x, tid = threadIdx.x, H = gi.H, y = row - gi.row0;
+  int* c_first = cand;        // first node id of a contiguous component, else -1
+  int* c_list = cand + B;     // offset of its node list
+  int* c_off = cand + 2 * B;  // prefix of node counts over the chunk's candidates
+  if (tid == 0) *s_cnt = 0;
+  __syncthreads();
+  for (int base = 0; base < gi.ncomp; base += B) {
+    int is = 0, nn = 0, first = 0, list = 0;
+    if (base + tid < gi.ncomp) {
+      const int k = gi.comp0 + base + tid;
+      const int2 r = a.comp_rows[k];
+      if (imod(y - r.x, H) < r.y) {
+        const CompMeta& m = a.L.meta[k];
+        is = 1;
+        nn = m.nn;
+        first = m.nf;
+        list = m.n0;
+      }
+    }
+    int ccount, ntot;
+    const int cpos = block_inclusive_scan(is, wsum, &ccount) - is;
+    const int noff = block_inclusive_scan(nn, wsum, &ntot) - nn;
+    if (is) {
+      c_first[cpos] = first;
+      c_list[cpos] = list;
+      c_off[cpos] = noff;
+    }
+    __syncthreads();
+    for (int f = tid; f < ntot; f += B) {
+      int lo = 0, hi = ccount - 1;  // the last candidate with c_off <= f
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (c_off[mid] <= f) lo = mid; else hi = mid - 1;
+      }
+      const int local = f - c_off[lo];
+      const int n = c_first[lo] >= 0 ? c_first[lo] + local : a.L.comp_nodes[c_list[lo] + local];
+      const int2 yy = *reinterpret_cast<const int2*>(a.cic_cell + 4 * (size_t)n + 2);
+      if (yy.x == row || yy.y == row) {
+        const int p = atomicAdd(s_cnt, 1);
+        if (p < lim) out[p] = n;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+  return *s_cnt;
+}
+
+__device__ void deposit_row(const DepArgs& a, const InstGeo& gi, int row, float2* __restrict__ dst,
+                            char* __restrict__ scratch, int* wsum, int* s_cnt) {
+  const int B = blockDim.x, tid = threadIdx.x, W = gi.W, cap = a.cap;
   for (int x = tid; x < W; x += B) dst[x] = make_float2(0.0f, 0.0f);
+  int* sorted = reinterpret_cast<int*>(scratch);                                 // [cap]
+  unsigned long long* k0 = reinterpret_cast<unsigned long long*>(sorted + cap);  // [2 cap]
+  unsigned long long* k1 = k0 + 2 * cap;                                         // [2 cap]
+  int* ids = reinterpret_cast<int*>(k0);   // arrival order, dead once ranked
+  int* cand = reinterpret_cast<int*>(k1);  // candidate components, dead once collected
+  const int m = collect_row_nodes(a, gi, row, ids, cap, cand, wsum, s_cnt);
   if (m == 0) return;
+  const int ne = 2 * m;
   const int* nodes;
   const unsigned long long* keys;
   if (m <= cap) {
-    int* sorted = reinterpret_cast<int*>(scratch);                             // [cap]
-    unsigned long long* k0 = reinterpret_cast<unsigned long long*>(sorted + cap);  // [2 cap]
-    unsigned long long* k1 = k0 + 2 * cap;                                      // [2 cap]
-    int* ids = reinterpret_cast<int*>(k0);  // arrival order, dead once ranked
-    for (int j = tid; j < m; j += B) ids[j] = row_list[off + j];
-    __syncthreads();
     for (int j = tid; j < m; j += B) {
       const int v = ids[j];
       int r = 0;
@@ -259,7 +349,7 @@ __device__ void deposit_row(int row, int W, int cap, const int* __restrict__ row
     __syncthreads();
     for (int j = tid; j < m; j += B) {
       const int n = sorted[j];
-      const int4 cc = *reinterpret_cast<const int4*>(cic_cell + 4 * (size_t)n);
+      const int4 cc = *reinterpret_cast<const int4*>(a.cic_cell + 4 * (size_t)n);
       const unsigned long long qb = (cc.z == row) ? 0ull : 2ull;
       k0[2 * j] = ((unsigned long long)(unsigned)cc.x << 34) | (qb << 32) | (unsigned)j;
       k0[2 * j + 1] = ((unsigned long long)(unsigned)cc.y << 34) | ((qb + 1ull) << 32) | (unsigned)j;
@@ -275,19 +365,24 @@ __device__ void deposit_row(int row, int W, int cap, const int* __restrict__ row
     nodes = sorted;
     keys = k1;
   } else {
-    int* nl = row_list + off;
-    sort_unique<int>(nl, m, nullptr, 0, sort_tmp + off, 0x7fffffff);
-    unsigned long long* kk = key_tmp + 2 * (size_t)off;
+    if (tid == 0) s_cnt[1] = atomicAdd(a.arena_top, m);
+    __syncthreads();
+    const int aoff = s_cnt[1];
+    int* nl = a.arena + aoff;
+    collect_row_nodes(a, gi, row, nl, m, cand, wsum, s_cnt);
+    sort_unique<int>(nl, m, nullptr, 0, a.sort_tmp + aoff, 0x7fffffff);
+    unsigned long long* kk = a.key_tmp + 2 * (size_t)aoff;
     for (int j = tid; j < m; j += B) {
       const int n = nl[j];
-      const int4 cc = *reinterpret_cast<const int4*>(cic_cell + 4 * (size_t)n);
+      const int4 cc = *reinterpret_cast<const int4*>(a.cic_cell + 4 * (size_t)n);
       const unsigned long long qb = (cc.z == row) ? 0ull : 2ull;
       kk[2 * j] = ((unsigned long long)(unsigned)cc.x << 34) | (qb << 32) | (unsigned)j;
       kk[2 * j + 1] = ((unsigned long long)(unsigned)cc.y << 34) | ((qb + 1ull) << 32) | (unsigned)j;
     }
     __threadfence_block();
     __syncthreads();
-    sort_unique<unsigned long long>(kk, ne, nullptr, 0, key_tmp + key_half + 2 * (size_t)off, ~0ull);
+    sort_unique<unsigned long long>(kk, ne, nullptr, 0, a.key_tmp + a.key_half + 2 * (size_t)aoff,
+                                    ~0ull);
     nodes = nl;
     keys = kk;
   }
@@ -301,7 +396,7 @@ __device__ void deposit_row(int row, int W, int cap, const int* __restrict__ row
       if ((int)(ke >> 34) != x) break;
       const int q = (int)((ke >> 32) & 3ull);
       const int n = nodes[(int)(ke & 0xffffffffull)];
-      acc = acc + cic_w[4 * n + q];
+      acc = acc + a.cic_w[4 * n + q];
     }
     dst[x].x = acc;
   }
@@ -328,13 +423,13 @@ __device__ __forceinline__ void load_plan(const FftDev& p, int* s_radix) {
 
 // Forward row FFT of one row (from the CIC deposit when `real_in` is NULL,
 // else from the real grid), written transposed to specT.
-__global__ void __launch_bounds__(256) k_rows_fwd_T(
-    BatchDev bd, const float* __restrict__ real_in, int dep_cap, const int* __restrict__ row_off,
-    int* __restrict__ row_list, int* __restrict__ sort_tmp, unsigned long long* __restrict__ key_tmp,
-    size_t key_half, const int* __restrict__ cic_cell, const float* __restrict__ cic_w,
-    float2* __restrict__ specT, float* __restrict__ density_out) {
+__global__ void __launch_bounds__(256) k_rows_fwd_T(BatchDev bd, const float* __restrict__ real_in,
+                                                    DepArgs dep, float2* __restrict__ specT,
+                                                    float* __restrict__ density_out) {
   extern __shared__ float2 lds[];
   __shared__ int s_radix[FFT_MAX_RADICES];
+  __shared__ int wsum[16];
+  __shared__ int s_cnt[2];
   const int row = xcd_group(blockIdx.x, gridDim.x);
   const InstGeo& gi = bd.geo[bd.row_inst[row]];
   const int W = gi.W, H = gi.H, y = row - gi.row0;
@@ -347,8 +442,7 @@ __global__ void __launch_bounds__(256) k_rows_fwd_T(
   if (real_in) {
     for (int i = threadIdx.x; i < W; i += blockDim.x) a[i] = make_float2(real_in[g0 + i], 0.0f);
   } else {
-    deposit_row(row, W, dep_cap, row_off, row_list, sort_tmp, key_tmp, key_half, cic_cell, cic_w,
-                a, reinterpret_cast<char*>(lds + W));
+    deposit_row(dep, gi, row, a, reinterpret_cast<char*>(lds + W), wsum, s_cnt);
     if (density_out)
       for (int i = threadIdx.x; i < W; i += blockDim.x) density_out[g0 + i] = a[i].x;
   }
@@ -430,22 +524,26 @@ __global__ void k_gauss_table_T(int H, int W, float coef, float* __restrict__ kf
 
 }  // namespace
 
-hipError_t launch_predict(const Consts& c, const BatchDev& b, int N, const float* pos,
-                          const float* vel, float* pred_w, int* cic_cell, float* cic_w,
-                          int* row_cnt, hipStream_t s) {
-  if (N == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_predict, dim3((N + 255) / 256), dim3(256), 0, s, c, b, N, pos, vel, pred_w,
-                     cic_cell, cic_w, row_cnt);
+hipError_t launch_predict(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
+                          const float* pos, const float* vel, float* pred_w, int* cic_cell,
+                          float* cic_w, int2* comp_rows, int* arena_top, hipStream_t s) {
+  if (L.num_comp == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_predict, dim3(L.num_comp), dim3(64), 0, s, c, b, L, pos, vel, pred_w,
+                     cic_cell, cic_w, comp_rows, arena_top);
   return hipGetLastError();
 }
 
-hipError_t launch_row_fill(const Consts& c, int N, const int* cic_cell, const int* row_off,
-                           int* row_fill, int* row_list, hipStream_t s) {
-  (void)c;
-  if (N == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_row_fill, dim3((N + 255) / 256), dim3(256), 0, s, N, cic_cell, row_off,
-                     row_fill, row_list);
-  return hipGetLastError();
+hipError_t steric_prepare(int lds_max) {
+  // the FFT passes of long sides use more than 64 KB of LDS
+  const int lds = lds_max - 1024;
+  const void* fns[] = {reinterpret_cast<const void*>(&k_rows_fwd_T),
+                       reinterpret_cast<const void*>(&k_cols_T),
+                       reinterpret_cast<const void*>(&k_rows_inv_T)};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t s) {
@@ -460,27 +558,46 @@ hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t 
 static int fft_threads(int n) { return n <= 256 ? 64 : (n <= 512 ? 128 : 256); }
 
 // LDS of the forward row pass: the row, then the larger of (second buffer +
-// twiddles) and the deposit scratch of DEP_CAP_MIN nodes.
-static size_t rows_region_bytes(int max_w) {
+// twiddles) and the deposit scratch of `cap` nodes.
+static size_t rows_region_bytes(int max_w, int cap) {
   const size_t fft = (size_t)2 * max_w * sizeof(float2);
-  const size_t dep = (size_t)DEP_CAP_MIN * DEP_BYTES_PER_NODE;
+  const size_t dep = (size_t)cap * DEP_BYTES_PER_NODE;
   return fft > dep ? fft : dep;
 }
-size_t fft_rows_lds_bytes(int max_w) { return (size_t)max_w * sizeof(float2) + rows_region_bytes(max_w); }
+// Smallest deposit capacity: DEP_CAP_MIN, and room for the candidate lists
+// (3 ints per thread in the 16 cap bytes of the ranked keys).
+static int dep_cap_floor(int threads) {
+  const int c = (3 * threads + 3) / 4;
+  return ((c > DEP_CAP_MIN ? c : DEP_CAP_MIN) + 1) & ~1;
+}
+size_t fft_rows_lds_bytes(int max_w) {
+  return (size_t)max_w * sizeof(float2) + rows_region_bytes(max_w, dep_cap_floor(fft_threads(max_w)));
+}
 size_t fft_cols_lds_bytes(int max_h) { return (size_t)3 * max_h * sizeof(float2); }
 
-hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const int* row_off,
-                               int* row_list, int* sort_scratch, unsigned long long* key_scratch,
-                               size_t key_half, const int* cic_cell, const float* cic_w,
+hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const StericDep* d,
                                float2* specT, const float* kfT, float* density_out, float* out,
                                hipStream_t s) {
   const int tr = fft_threads(b.max_w), tc = fft_threads(b.max_h);
+  const int floor_cap = dep_cap_floor(tr);
   // rows of any instance (W <= max_w) have at least this region after the row
-  int cap = (int)(rows_region_bytes(b.max_w) / DEP_BYTES_PER_NODE) & ~1;
-  if (cap > DEP_CAP_MAX) cap = DEP_CAP_MAX;
+  int cap = (int)(rows_region_bytes(b.max_w, floor_cap) / DEP_BYTES_PER_NODE) & ~1;
+  if (cap > DEP_CAP_MAX) cap = DEP_CAP_MAX > floor_cap ? DEP_CAP_MAX : floor_cap;
+  DepArgs dep{};
+  if (d) {
+    dep.L = d->L;
+    dep.comp_rows = d->comp_rows;
+    dep.cic_cell = d->cic_cell;
+    dep.cic_w = d->cic_w;
+    dep.arena = d->arena;
+    dep.arena_top = d->arena_top;
+    dep.sort_tmp = d->sort_tmp;
+    dep.key_tmp = d->key_tmp;
+    dep.key_half = d->key_half;
+  }
+  dep.cap = cap;
   hipLaunchKernelGGL(k_rows_fwd_T, dim3(b.rows), dim3(tr), fft_rows_lds_bytes(b.max_w), s, b,
-                     real_in, cap, row_off, row_list, sort_scratch, key_scratch, key_half, cic_cell,
-                     cic_w, specT, density_out);
+                     real_in, dep, specT, density_out);
   hipLaunchKernelGGL(k_cols_T, dim3(b.cols), dim3(tc), fft_cols_lds_bytes(b.max_h), s, b, specT,
                      kfT);
   hipLaunchKernelGGL(k_rows_inv_T, dim3(b.rows), dim3(tr), (size_t)3 * b.max_w * sizeof(float2), s,
