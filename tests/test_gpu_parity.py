@@ -97,6 +97,27 @@ def test_steric_field(dev, case):
     assert rel_l2(_np(f2.steric), st64) <= 1e-6
 
 
+@pytest.mark.parametrize("shape", [(48, 48), (48, 80)])
+def test_cic_deposit_dense_rows(dev, shape):
+    """Rows holding more nodes than the row pass keeps in LDS (here 300 and
+    120 nodes against 64) take the global overflow path; a stack of nodes on
+    one cell checks the fold order of equal keys.  Bit-exact against the
+    restatement."""
+    _say(f"cic deposit dense rows {shape}")
+    from microcosmos_amd.forces import deposit_density
+    H, W = shape
+    rng = np.random.default_rng(21)
+    a = np.stack([rng.uniform(0, W, 300), rng.uniform(10.0, 11.0, 300)], 1)
+    b = np.stack([rng.uniform(0, W, 120), rng.uniform(30.2, 30.9, 120)], 1)
+    c = np.tile([[7.25, 40.5]], (40, 1)) + rng.normal(0, 1e-3, (40, 2))
+    d = rng.uniform(-W, 2 * W, (60, 2))  # wrapped on the device
+    pos = np.concatenate([a, b, c, d]).astype(np.float32)
+    rng.shuffle(pos)
+    dens = _np(deposit_density(torch.as_tensor(pos, device=dev), shape, 0.7))
+    exp_dens = R.compute_field_steric_force_corrected(pos, shape, 2.0, 5, 0.7)[0]
+    assert np.array_equal(dens, exp_dens), max_ulp(dens, exp_dens)
+
+
 @pytest.mark.parametrize("solver", ["pbd", "cosserat"])
 def test_rod_solve(dev, solver):
     _say(f"rod solve {solver}")
@@ -218,6 +239,33 @@ def test_step_parity_small(dev, steps):
         nodes, edges, fields = step(nodes, edges, fields, wl.dt, cfg)
     r = _compare_states(f"small {steps} steps", (nodes, edges, fields), o, o64)
     for k in ("pos", "u", "f", "theta"):
+        assert r[k] <= 1e-5, (k, r)
+
+
+def test_rollout_permuted_node_ids(dev):
+    """Filaments whose node ids are not contiguous ranges (the small case with
+    its node numbering shuffled) take the listed-id paths of every kernel;
+    12 graph-replayed steps (one 8-step graph and 4 single steps) against
+    the restatement on the same numbering."""
+    _say("rollout with shuffled node ids, 12 steps")
+    from microcosmos_amd import Rollout
+    wl = W.small()
+    rng = np.random.default_rng(8)
+    order = rng.permutation(wl.num_nodes)  # new node k is old node order[k]
+    new_id = np.argsort(order).astype(np.int32)
+    wl = replace(wl, position=wl.position[order].copy(), velocity=wl.velocity[order].copy(),
+                 pairs=new_id[wl.pairs],
+                 color_bend=None if wl.color_bend is None else wl.color_bend[order].copy())
+    p = oracle_params(wl)
+    o = run_oracle(wl, 12, p)
+    nodes, edges, fields = device_state(wl, dev)
+    ro = Rollout(nodes, edges, fields, device_config(wl), wl.dt,
+                 actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
+    ro.run(12)
+    torch.cuda.synchronize()
+    r = _compare_states("shuffled ids 12 steps", (ro.nodes(), ro.edges_now(), ro.fields_now()), o,
+                        None)
+    for k in ("pos", "u", "f", "theta", "steric"):
         assert r[k] <= 1e-5, (k, r)
 
 
