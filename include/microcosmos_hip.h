@@ -147,8 +147,10 @@ mc_status mc_step(mc_plan* plan, const mc_params* params, const mc_state* in, co
  * act may be NULL (then edges->bending_rest_angles is used every step).
  * record_position (num_steps, N, 2) and record_fluid_velocity
  * (num_steps, 2, Hf, Wf) may be NULL; when given they receive the per-step
- * trajectory that simulate() returns.  use_graph != 0 captures one step in a
- * HIP graph and replays it. */
+ * trajectory that simulate() returns.  use_graph != 0 (and no recording)
+ * captures 8 steps in one HIP graph and replays it: within the graph each
+ * step's LBM collide-stream runs inside the next step's rod launch, so every
+ * kernel is enqueued on one stream; remaining steps run unfused. */
 mc_status mc_rollout(mc_plan* plan, const mc_params* params, const mc_state* state,
                      const mc_edge_params* edges, const mc_actuation* act, double dt,
                      int64_t num_steps, float* record_position, float* record_fluid_velocity,
@@ -164,10 +166,11 @@ mc_status mc_batch_rollout(mc_plan* plan, const mc_params* params, const mc_stat
                            const mc_edge_params* edges, const mc_actuation* act, double dt,
                            int64_t num_steps, float* displacement_out, void* stream);
 
-/* Measurement: num_steps coupled steps in place (no graph, same kernels as
- * mc_rollout) with HIP events on `stream` around every fused LBM
- * collide-stream launch and every step; returns the average durations in ms.
- * No counterpart in the reference (its only timer is utils.py:53-59). */
+/* Measurement: num_steps coupled steps in place (no graph, the LBM
+ * collide-stream as its own launch) with HIP events carried by every LBM
+ * dispatch and recorded after every step on `stream`; returns the average
+ * LBM kernel duration and step duration in ms.  No counterpart in the
+ * reference (its only timer is utils.py:53-59). */
 mc_status mc_profile_step(mc_plan* plan, const mc_params* params, const mc_state* state,
                           const mc_edge_params* edges, const mc_actuation* act, double dt,
                           int64_t num_steps, double* lbm_ms_avg, double* step_ms_avg, void* stream);

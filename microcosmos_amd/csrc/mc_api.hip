@@ -612,10 +612,8 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
       *defer_lbm = make_lbm_args(c, p->bd, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
                                  p->ib.force);
     } else {
-      if (sy.t0) HIPCHK(hipEventRecord(sy.t0, s));
       HIPCHK(launch_lbm(c, p->bd, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
-                        p->ib.force, s));
-      if (sy.t1) HIPCHK(hipEventRecord(sy.t1, s));
+                        p->ib.force, s, sy.t0, sy.t1));
     }
   } else {
     if (tick) HIPCHK(launch_tick(tick, s));

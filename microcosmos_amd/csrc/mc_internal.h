@@ -224,7 +224,9 @@ hipError_t launch_track(const TrackWork& t, const float* pos, int mode, hipStrea
 // LBM
 LbmArgs make_lbm_args(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
                       float* ustar, uint32_t* mask, const float* force);
+// start / stop (optional): events recorded with the dispatch, timing the kernel
 hipError_t launch_lbm(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
-                      float* ustar, uint32_t* mask, const float* force, hipStream_t s);
+                      float* ustar, uint32_t* mask, const float* force, hipStream_t s,
+                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 }  // namespace mc

@@ -1,5 +1,7 @@
 // Standalone launch of the fused D2Q9 TRT + Guo collide-and-stream pass
 // (fluid.py:268-357); the per-cell update is in mc_lbm.h.
+#include <hip/hip_ext.h>
+
 #include "mc_lbm.h"
 
 namespace mc {
@@ -34,9 +36,15 @@ LbmArgs make_lbm_args(const Consts& c, const BatchDev& b, const float* f_in, flo
 }
 
 hipError_t launch_lbm(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
-                      float* ustar, uint32_t* mask, const float* force, hipStream_t s) {
-  hipLaunchKernelGGL(k_lbm, dim3(b.lbm_blocks), dim3(LBM_THREADS), 0, s,
-                     make_lbm_args(c, b, f_in, f_out, ustar, mask, force));
+                      float* ustar, uint32_t* mask, const float* force, hipStream_t s,
+                      hipEvent_t start, hipEvent_t stop) {
+  const LbmArgs a = make_lbm_args(c, b, f_in, f_out, ustar, mask, force);
+  if (start && stop) {
+    // events carried by the dispatch itself: they bracket the kernel's execution
+    hipExtLaunchKernelGGL(k_lbm, dim3(b.lbm_blocks), dim3(LBM_THREADS), 0, s, start, stop, 0, a);
+  } else {
+    hipLaunchKernelGGL(k_lbm, dim3(b.lbm_blocks), dim3(LBM_THREADS), 0, s, a);
+  }
   return hipGetLastError();
 }
 
