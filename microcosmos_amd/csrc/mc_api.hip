@@ -430,6 +430,26 @@ mc_status build_fft(mc_plan* p) {
       const double a = -2.0 * M_PI * (double)k / (double)n;
       tw.push_back(make_float2((float)cos(a), (float)sin(a)));
     }
+    // the same values per stage, contiguous in the butterfly index k: for
+    // radix R after Ns points, tw[r k n/(Ns R)] at (r-1) Ns + k, then for
+    // R > 4 the R-th roots tw[m n/R]
+    f.stw_off = (int)tw.size();
+    int Ns = 1;
+    for (int R : r) {
+      const int tstride = n / (Ns * R), nb = n / R;
+      for (int q = 1; q < R; ++q)
+        for (int k = 0; k < Ns; ++k) {
+          const float2 v = tw[f.tw_off + q * k * tstride];
+          tw.push_back(v);
+        }
+      if (R > 4)
+        for (int m = 0; m < R; ++m) {
+          const float2 v = tw[f.tw_off + m * nb];
+          tw.push_back(v);
+        }
+      Ns *= R;
+    }
+    f.stw_len = (int)tw.size() - f.stw_off;
     plans.push_back(f);
   }
   // row pass: W complex + max(2 W complex, deposit scratch); column pass: 3 H complex
@@ -725,6 +745,7 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
   w.comp_of_node = p->comp_of_node;
   w.comp_node_off = p->L.comp_node_off;
   w.comp_nodes = p->L.comp_nodes;
+  w.meta = p->L.meta;
   w.cells = p->lat_cells;
   w.cl_bpn = ib_scratch_bytes_per_node(max_kernel_size);
   w.lds_bytes = ib_lds_bytes(p->lds_max);

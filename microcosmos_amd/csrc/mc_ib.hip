@@ -234,6 +234,7 @@ struct ClusterArgs {
   Consts c;
   const int *comp_node_off, *comp_nodes, *label, *cl_cnt, *cl_node_off, *cl_mem_cnt, *cl_mem_off,
       *cl_members;
+  const CompMeta* meta;
   const int *st_sx, *st_sy;
   const unsigned long long* st_won;
   const float *st_w, *st_rho, *vib, *f_in, *pos;
@@ -323,6 +324,39 @@ __device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns) {
          3 * al16(4 * (size_t)ns) + 2 * al16(8 * (size_t)ns);
 }
 
+// The K2 per-tap values of one node, 16 bytes per load when the rows allow.
+template <int K2>
+__device__ __forceinline__ void load_taps(const float* __restrict__ p, float (&v)[K2]) {
+  if constexpr (K2 % 4 == 0) {
+#pragma unroll
+    for (int t = 0; t < K2; t += 4) {
+      const float4 q = *reinterpret_cast<const float4*>(p + t);
+      v[t] = q.x;
+      v[t + 1] = q.y;
+      v[t + 2] = q.z;
+      v[t + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < K2; ++t) v[t] = p[t];
+  }
+}
+
+template <int K2>
+__device__ __forceinline__ void load_taps2(const float2* __restrict__ p, float2 (&v)[K2]) {
+  if constexpr (K2 % 2 == 0) {
+#pragma unroll
+    for (int t = 0; t < K2; t += 2) {
+      const float4 q = *reinterpret_cast<const float4*>(p + t);
+      v[t] = make_float2(q.x, q.y);
+      v[t + 1] = make_float2(q.z, q.w);
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < K2; ++t) v[t] = p[t];
+  }
+}
+
 __device__ __forceinline__ void cswap_key(float2& a, float2& b) {
   const bool sw = __float_as_int(a.x) > __float_as_int(b.x);
   const float2 lo = sw ? b : a, hi = sw ? a : b;
@@ -365,26 +399,26 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
   int n = 0, g[K2], seg[K2], sub[K2];
   float w[K2];
   float2 vb = make_float2(0.0f, 0.0f);
+  float2 p0 = make_float2(0.0f, 0.0f), v0 = make_float2(0.0f, 0.0f);
   unsigned long long won = 0;
   for (int s = tid; s <= ns; s += bs) sstart[s] = 0;
   if (mine) {
     n = nodes[r];
     const int sx = a.st_sx[n], sy = a.st_sy[n];
     won = a.st_won[n];
-#pragma unroll
-    for (int t = 0; t < K2; ++t) w[t] = a.st_w[(size_t)n * K2 + t];
-    vb = make_float2(a.vib[2 * n], a.vib[2 * n + 1]);
+    load_taps<K2>(a.st_w + (size_t)n * K2, w);
+    vb = *reinterpret_cast<const float2*>(a.vib + 2 * n);
+    // the drag's inputs, read now (only this thread writes them, at the end)
+    p0 = *reinterpret_cast<const float2*>(a.pos + 2 * n);
+    v0 = *reinterpret_cast<const float2*>(a.vel_io + 2 * n);
 #pragma unroll
     for (int t = 0; t < K2; ++t) g[t] = wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W);
     // won taps claim segments (arbitrary numbering), return their cell to
     // idle, and carry the cell's rho and clamped u from the stencil pass
     float rw[K2];
     float2 uw[K2];
-#pragma unroll
-    for (int t = 0; t < K2; ++t) {  // all taps: no wait for `won`
-      rw[t] = a.st_rho[(size_t)n * K2 + t];
-      uw[t] = a.st_u[(size_t)n * K2 + t];
-    }
+    load_taps<K2>(a.st_rho + (size_t)n * K2, rw);  // all taps: no wait for `won`
+    load_taps2<K2>(a.st_u + (size_t)n * K2, uw);
 #pragma unroll
     for (int t = 0; t < K2; ++t) {
       if ((won >> t) & 1ull) {
@@ -550,14 +584,12 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
       fx = fx + F[t].x * w[t];
       fy = fy + F[t].y * w[t];
     }
-    const float px = a.pos[2 * n], py = a.pos[2 * n + 1];
-    const float vx = a.vel_io[2 * n], vy = a.vel_io[2 * n + 1];
     const float dvx = (fx / c.node_mass) / (gi.scale_x * c.dt);
     const float dvy = (fy / c.node_mass) / (gi.scale_y * c.dt);
-    a.vel_io[2 * n] = vx - dvx;
-    a.vel_io[2 * n + 1] = vy - dvy;
-    a.pos_out[2 * n] = fmodf_floor(px, gi.dom_wf);
-    a.pos_out[2 * n + 1] = fmodf_floor(py, gi.dom_hf);
+    a.vel_io[2 * n] = v0.x - dvx;
+    a.vel_io[2 * n + 1] = v0.y - dvy;
+    a.pos_out[2 * n] = fmodf_floor(p0.x, gi.dom_wf);
+    a.pos_out[2 * n + 1] = fmodf_floor(p0.y, gi.dom_hf);
   }
 }
 
@@ -775,32 +807,36 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
   const size_t avail = stage_lds ? a.lds_bytes - 2 * in_bytes : a.lds_bytes;
   {
     // members ordered by their smallest node id, so the gathered list is
-    // already ascending unless components interleave
+    // already ascending unless components interleave; one record load per
+    // member (node count, first id of a contiguous id range, list offset)
     const int m0 = a.cl_mem_off[root], nm = a.cl_mem_cnt[root];
-    const bool m_lds = 12 * ((size_t)nm + 1) <= avail;
+    const bool m_lds = 24 * ((size_t)nm + 1) <= avail;
     int* mfirst = m_lds ? reinterpret_cast<int*>(smem) : G.eidx;
     int* msort = m_lds ? mfirst + (nm + 1) : G.tapseg;
     int* moff = m_lds ? msort + (nm + 1) : G.sstart;
+    int* mnn = m_lds ? moff + (nm + 1) : reinterpret_cast<int*>(G.wt);
+    int* mnf = m_lds ? mnn + (nm + 1) : reinterpret_cast<int*>(G.sinv);
+    int* mn0 = m_lds ? mnf + (nm + 1) : reinterpret_cast<int*>(G.corr);
     for (int i = tid; i < nm; i += bs) {
-      const int cc = a.cl_members[m0 + i];
-      mfirst[i] = a.comp_nodes[a.comp_node_off[cc]];  // the smallest id of the component
+      const CompMeta m = a.meta[a.cl_members[m0 + i]];
+      // a component's node list is ascending: its first entry is the smallest id
+      mfirst[i] = m.nf >= 0 ? m.nf : a.comp_nodes[m.n0];
+      mnn[i] = m.nn;
+      mnf[i] = m.nf;
+      mn0[i] = m.n0;
     }
     __syncthreads();
     for (int i = tid; i < nm; i += bs) {
       const int v = mfirst[i];
       int rank = 0;
       for (int j = 0; j < nm; ++j) rank += mfirst[j] < v ? 1 : 0;
-      msort[rank] = a.cl_members[m0 + i];
+      msort[rank] = i;
     }
     __syncthreads();
     int carry = 0;
     for (int b0 = 0; b0 < nm; b0 += bs) {
       const int i = b0 + tid;
-      int cnt = 0;
-      if (i < nm) {
-        const int cc = msort[i];
-        cnt = a.comp_node_off[cc + 1] - a.comp_node_off[cc];
-      }
+      const int cnt = i < nm ? mnn[msort[i]] : 0;
       int tot;
       const int inc = block_inclusive_scan(cnt, wsum, &tot);
       if (i < nm) moff[i] = carry + inc - cnt;
@@ -813,8 +849,8 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
         const int mid = (lo + hi + 1) >> 1;
         if (moff[mid] <= i) lo = mid; else hi = mid - 1;
       }
-      const int cc = msort[lo];
-      nodes_in[i] = a.comp_nodes[a.comp_node_off[cc] + (i - moff[lo])];
+      const int k = msort[lo], local = i - moff[lo];
+      nodes_in[i] = mnf[k] >= 0 ? mnf[k] + local : a.comp_nodes[mn0[k] + local];
     }
     __syncthreads();
   }
@@ -941,6 +977,7 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   a.c = c;
   a.comp_node_off = w.comp_node_off;
   a.comp_nodes = w.comp_nodes;
+  a.meta = w.meta;
   a.label = w.label;
   a.cl_cnt = w.cl_cnt;
   a.cl_node_off = w.cl_node_off;

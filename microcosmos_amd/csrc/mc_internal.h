@@ -55,9 +55,11 @@ struct InstGeo {
   unsigned long long wdiv;    // ceil(2^40 / Wf): y = (cell * wdiv) >> 40 (0: divide)
 };
 
-// Stockham plan of one FFT length: radix schedule and twiddle table offset.
+// Stockham plan of one FFT length: radix schedule, twiddle table offset, and
+// the offset / length of the same twiddles laid out per stage.
 struct FftDev {
   int n, tw_off, nrad;
+  int stw_off, stw_len;
   int radix[FFT_MAX_RADICES];
 };
 
@@ -179,6 +181,7 @@ size_t rod_lds_bytes(int nn, int ne, int nb, int nni, int nei);
 struct IBWork {
   int N, C;                    // nodes, filament components
   const int *comp_of_node, *comp_node_off, *comp_nodes;  // component layout (rod layout)
+  const struct CompMeta* meta;                           // per-component record (rod layout)
   int *st_sx, *st_sy;          // per node wrapped stencil start cell
   float *st_w;                 // per node K*K weights
   unsigned long long* st_won;  // per node: bit t set when tap t claimed its cell

@@ -156,6 +156,153 @@ __device__ float2* fft_lds(float2* a, float2* b, int n, int stride, int batch, c
   return src;
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time Stockham networks for the common lengths.  The same radix
+// schedule, twiddle values and operation order as fft_lds (so the same
+// roundings), with every index known at compile time, the twiddles read per
+// stage from a contiguous table, and the intermediate buffers padded by one
+// element per 16 against LDS bank conflicts.  Input and result are unpadded.
+// ---------------------------------------------------------------------------
+constexpr int ct_radix(int n, int s) {
+  int i = 0;
+  while (n % 4 == 0) {
+    if (i++ == s) return 4;
+    n /= 4;
+  }
+  while (n % 2 == 0) {
+    if (i++ == s) return 2;
+    n /= 2;
+  }
+  for (int p = 3; p <= 13; p += 2)
+    while (n % p == 0) {
+      if (i++ == s) return p;
+      n /= p;
+    }
+  return 0;
+}
+constexpr int ct_nstages(int n) {
+  int c = 0;
+  while (ct_radix(n, c) != 0) ++c;
+  return c;
+}
+constexpr int ct_ns(int n, int s) {
+  int ns = 1;
+  for (int i = 0; i < s; ++i) ns *= ct_radix(n, i);
+  return ns;
+}
+constexpr int ct_twoff(int n, int s) {
+  int off = 0;
+  for (int i = 0; i < s; ++i) {
+    const int R = ct_radix(n, i);
+    off += (R - 1) * ct_ns(n, i) + (R > 4 ? R : 0);
+  }
+  return off;
+}
+// Elements of a padded buffer of length n.
+__host__ __device__ constexpr int ct_buf(int n) { return n + (n >> 4) + 1; }
+template <bool P>
+__device__ __forceinline__ int ct_pad(int i) { return P ? i + (i >> 4) : i; }
+
+template <int R, bool INV>
+__device__ __forceinline__ void ct_butterfly(float2 (&v)[R], float2 (&V)[R],
+                                             const float2* __restrict__ roots) {
+  if constexpr (R == 2 || R == 3 || R == 4) {
+    butterfly<R>(v, V, INV, nullptr, 0);
+  } else {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = roots[(r * q) % R];
+        if (INV) w.y = -w.y;
+        acc = cadd(acc, cmul(v[r], w));
+      }
+      V[q] = acc;
+    }
+  }
+}
+
+template <int N, int S, bool INV>
+__device__ __forceinline__ void ct_stage(const float2* __restrict__ src, float2* __restrict__ dst,
+                                         const float2* __restrict__ stw) {
+  constexpr int R = ct_radix(N, S), Ns = ct_ns(N, S), nb = N / R;
+  constexpr bool pin = S > 0, pout = S + 1 < ct_nstages(N);
+  const float2* tws = stw + ct_twoff(N, S);
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+    const int k = j % Ns;
+    float2 v[R], V[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      float2 x = src[ct_pad<pin>(j + r * nb)];
+      if (r > 0 && k > 0) {
+        float2 w = tws[(r - 1) * Ns + k];
+        if (INV) w.y = -w.y;
+        x = cmul(x, w);
+      }
+      v[r] = x;
+    }
+    ct_butterfly<R, INV>(v, V, tws + (R - 1) * Ns);
+    const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[ct_pad<pout>(idxD + r * Ns)] = V[r];
+  }
+}
+
+template <int N, int S, bool INV>
+__device__ __forceinline__ float2* ct_run(float2* src, float2* dst, const float2* stw) {
+  if constexpr (S == ct_nstages(N)) {
+    return src;
+  } else {
+    ct_stage<N, S, INV>(src, dst, stw);
+    __syncthreads();
+    return ct_run<N, S + 1, INV>(dst, src, stw);
+  }
+}
+
+// Lengths with a compile-time network.
+__device__ __forceinline__ bool ct_supported(int n) {
+  switch (n) {
+    case 64: case 128: case 192: case 256: case 384: case 512: case 768: case 1024: case 2048:
+      return true;
+    default:
+      return false;
+  }
+}
+
+// Twiddles of length n into LDS: the per-stage table for a compile-time
+// network, else the plain table.
+__device__ __forceinline__ void load_fft_twiddles(float2* dst, const float2* __restrict__ tw,
+                                                  const FftDev& fp) {
+  const bool ct = ct_supported(fp.n);
+  const float2* src = tw + (ct ? fp.stw_off : fp.tw_off);
+  const int len = ct ? fp.stw_len : fp.n;
+  for (int i = threadIdx.x; i < len; i += blockDim.x) dst[i] = src[i];
+}
+
+// FFT of length n on a (unpadded; a and b hold ct_buf(n) elements each).
+// Returns the buffer holding the unpadded result.
+__device__ __forceinline__ float2* fft_any(float2* a, float2* b, int n, const int* radix, int nrad,
+                           const float2* __restrict__ tw, bool inverse) {
+#define MC_CT_CASE(NN)                                                              \
+  case NN:                                                                          \
+    return inverse ? ct_run<NN, 0, true>(a, b, tw) : ct_run<NN, 0, false>(a, b, tw);
+  switch (n) {
+    MC_CT_CASE(64)
+    MC_CT_CASE(128)
+    MC_CT_CASE(192)
+    MC_CT_CASE(256)
+    MC_CT_CASE(384)
+    MC_CT_CASE(512)
+    MC_CT_CASE(768)
+    MC_CT_CASE(1024)
+    MC_CT_CASE(2048)
+    default:
+      return fft_lds(a, b, n, n, 1, radix, nrad, tw, inverse);
+  }
+#undef MC_CT_CASE
+}
+
 // Wrapped inertial prediction of node n (simulate.py:46-50, forces.py:95);
 // without vel, the positions as given.
 __device__ __forceinline__ void predict_xy(const Consts& c, const InstGeo& gi,
@@ -232,10 +379,6 @@ __global__ void __launch_bounds__(64) k_predict(Consts c, BatchDev bd, RodLayout
     const int len = hi + 2 - lo;  // rows lo .. hi + 1 relative to yref
     comp_rows[comp] = len >= H ? make_int2(0, H) : make_int2(imod(yref + lo, H), len);
   }
-}
-
-__device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ tw, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = tw[i];
 }
 
 __device__ __forceinline__ int xcd_group(int b, int G) {
@@ -435,20 +578,20 @@ __global__ void __launch_bounds__(256) k_rows_fwd_T(BatchDev bd, const float* __
   const int W = gi.W, H = gi.H, y = row - gi.row0;
   const FftDev& fp = bd.fft[gi.fft_w];
   float2* a = lds;
-  float2* b = lds + W;
-  float2* tw = lds + 2 * W;  // b and tw alias the deposit scratch
+  float2* b = lds + ct_buf(W);
+  float2* tw = lds + 2 * ct_buf(W);  // b and tw alias the deposit scratch
   load_plan(fp, s_radix);
   const size_t g0 = (size_t)gi.grid_off + (size_t)y * W;
   if (real_in) {
     for (int i = threadIdx.x; i < W; i += blockDim.x) a[i] = make_float2(real_in[g0 + i], 0.0f);
   } else {
-    deposit_row(dep, gi, row, a, reinterpret_cast<char*>(lds + W), wsum, s_cnt);
+    deposit_row(dep, gi, row, a, reinterpret_cast<char*>(b), wsum, s_cnt);
     if (density_out)
       for (int i = threadIdx.x; i < W; i += blockDim.x) density_out[g0 + i] = a[i].x;
   }
-  load_twiddles(tw, bd.tw + fp.tw_off, W);
+  load_fft_twiddles(tw, bd.tw, fp);
   __syncthreads();
-  float2* res = fft_lds(a, b, W, W, 1, s_radix, fp.nrad, tw, false);
+  float2* res = fft_any(a, b, W, s_radix, fp.nrad, tw, false);
   float2* sp = specT + gi.grid_off;
   for (int k = threadIdx.x; k < W; k += blockDim.x) sp[(size_t)k * H + y] = res[k];
 }
@@ -464,14 +607,14 @@ __global__ void __launch_bounds__(256) k_cols_T(BatchDev bd, float2* __restrict_
   const int H = gi.H, k = colg - gi.col0;
   const FftDev& fp = bd.fft[gi.fft_h];
   float2* a = lds;
-  float2* b = lds + H;
-  float2* tw = lds + 2 * H;
+  float2* b = lds + ct_buf(H);
+  float2* tw = lds + 2 * ct_buf(H);
   load_plan(fp, s_radix);
-  load_twiddles(tw, bd.tw + fp.tw_off, H);
+  load_fft_twiddles(tw, bd.tw, fp);
   float2* col = specT + gi.grid_off + (size_t)k * H;
   for (int y = threadIdx.x; y < H; y += blockDim.x) a[y] = col[y];
   __syncthreads();
-  float2* r = fft_lds(a, b, H, H, 1, s_radix, fp.nrad, tw, false);
+  float2* r = fft_any(a, b, H, s_radix, fp.nrad, tw, false);
   float2* o = (r == a) ? b : a;
   const float* kc = kfT + gi.kf_off + (size_t)k * H;
   for (int y = threadIdx.x; y < H; y += blockDim.x) {
@@ -482,7 +625,7 @@ __global__ void __launch_bounds__(256) k_cols_T(BatchDev bd, float2* __restrict_
     r[y] = v;
   }
   __syncthreads();
-  float2* res = fft_lds(r, o, H, H, 1, s_radix, fp.nrad, tw, true);
+  float2* res = fft_any(r, o, H, s_radix, fp.nrad, tw, true);
   const float sc = (float)(1.0 / (double)H);
   for (int y = threadIdx.x; y < H; y += blockDim.x) {
     const float2 v = res[y];
@@ -500,14 +643,14 @@ __global__ void __launch_bounds__(256) k_rows_inv_T(BatchDev bd, const float2* _
   const int W = gi.W, H = gi.H, y = row - gi.row0;
   const FftDev& fp = bd.fft[gi.fft_w];
   float2* a = lds;
-  float2* b = lds + W;
-  float2* tw = lds + 2 * W;
+  float2* b = lds + ct_buf(W);
+  float2* tw = lds + 2 * ct_buf(W);
   load_plan(fp, s_radix);
-  load_twiddles(tw, bd.tw + fp.tw_off, W);
+  load_fft_twiddles(tw, bd.tw, fp);
   const float2* sp = specT + gi.grid_off;
   for (int k = threadIdx.x; k < W; k += blockDim.x) a[k] = sp[(size_t)k * H + y];
   __syncthreads();
-  float2* res = fft_lds(a, b, W, W, 1, s_radix, fp.nrad, tw, true);
+  float2* res = fft_any(a, b, W, s_radix, fp.nrad, tw, true);
   const float sc = (float)(1.0 / (double)W);
   float* o = out + gi.grid_off + (size_t)y * W;
   for (int i = threadIdx.x; i < W; i += blockDim.x) o[i] = res[i].x * sc;
@@ -557,10 +700,19 @@ hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t 
 // about one radix-4 butterfly per thread and stage for the longest sequence.
 static int fft_threads(int n) { return n <= 256 ? 64 : (n <= 512 ? 128 : 256); }
 
+// Twiddle elements an FFT pass of length n keeps in LDS: the plain table (n)
+// or the per-stage table (n - 1 plus R roots per stage of radix R > 4).
+static int fft_tw_elems(int n) { return n + 64; }
+
+// LDS of one FFT pass of length n: two padded buffers and the twiddles.
+static size_t fft_pass_bytes(int n) {
+  return ((size_t)2 * ct_buf(n) + fft_tw_elems(n)) * sizeof(float2);
+}
+
 // LDS of the forward row pass: the row, then the larger of (second buffer +
 // twiddles) and the deposit scratch of `cap` nodes.
 static size_t rows_region_bytes(int max_w, int cap) {
-  const size_t fft = (size_t)2 * max_w * sizeof(float2);
+  const size_t fft = fft_pass_bytes(max_w) - (size_t)ct_buf(max_w) * sizeof(float2);
   const size_t dep = (size_t)cap * DEP_BYTES_PER_NODE;
   return fft > dep ? fft : dep;
 }
@@ -571,9 +723,10 @@ static int dep_cap_floor(int threads) {
   return ((c > DEP_CAP_MIN ? c : DEP_CAP_MIN) + 1) & ~1;
 }
 size_t fft_rows_lds_bytes(int max_w) {
-  return (size_t)max_w * sizeof(float2) + rows_region_bytes(max_w, dep_cap_floor(fft_threads(max_w)));
+  return (size_t)ct_buf(max_w) * sizeof(float2) +
+         rows_region_bytes(max_w, dep_cap_floor(fft_threads(max_w)));
 }
-size_t fft_cols_lds_bytes(int max_h) { return (size_t)3 * max_h * sizeof(float2); }
+size_t fft_cols_lds_bytes(int max_h) { return fft_pass_bytes(max_h); }
 
 hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const StericDep* d,
                                float2* specT, const float* kfT, float* density_out, float* out,
@@ -600,8 +753,8 @@ hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const St
                      real_in, dep, specT, density_out);
   hipLaunchKernelGGL(k_cols_T, dim3(b.cols), dim3(tc), fft_cols_lds_bytes(b.max_h), s, b, specT,
                      kfT);
-  hipLaunchKernelGGL(k_rows_inv_T, dim3(b.rows), dim3(tr), (size_t)3 * b.max_w * sizeof(float2), s,
-                     b, specT, out);
+  hipLaunchKernelGGL(k_rows_inv_T, dim3(b.rows), dim3(tr), fft_pass_bytes(b.max_w), s, b, specT,
+                     out);
   return hipGetLastError();
 }
 
