@@ -86,7 +86,7 @@ struct mc_plan {
   float* pred_w = nullptr;
   int* cic_cell = nullptr;
   float* cic_w = nullptr;
-  int2* comp_rows = nullptr;    // per component: steric row band
+  int4* comp_rows = nullptr;    // per component: steric row band, node count, ids
   int *dep_arena = nullptr, *dep_top = nullptr, *row_sort = nullptr;
   unsigned long long* row_keys = nullptr;
   float2* spec = nullptr;

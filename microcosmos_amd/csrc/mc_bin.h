@@ -40,6 +40,48 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int* wsum, int* total
   return v + pre;
 }
 
+// Inclusive scans of two values at once across a workgroup of up to 1024
+// threads; `wsum` holds 32 ints of LDS.  Returns both prefixes; *total
+// receives both workgroup sums.
+__device__ __forceinline__ int2 block_inclusive_scan2(int2 v, int* wsum, int2* total) {
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int tx = __shfl_up(v.x, d, 64), ty = __shfl_up(v.y, d, 64);
+    if (lane >= d) {
+      v.x += tx;
+      v.y += ty;
+    }
+  }
+  if (lane == 63) {
+    wsum[wid] = v.x;
+    wsum[16 + wid] = v.y;
+  }
+  __syncthreads();
+  if (wid == 0) {
+    int sx = (lane < nw) ? wsum[lane] : 0, sy = (lane < nw) ? wsum[16 + lane] : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int tx = __shfl_up(sx, d, 64), ty = __shfl_up(sy, d, 64);
+      if (lane >= d) {
+        sx += tx;
+        sy += ty;
+      }
+    }
+    if (lane < nw) {
+      wsum[lane] = sx;
+      wsum[16 + lane] = sy;
+    }
+  }
+  __syncthreads();
+  const int px = (wid > 0) ? wsum[wid - 1] : 0, py = (wid > 0) ? wsum[16 + wid - 1] : 0;
+  *total = make_int2(wsum[nw - 1], wsum[16 + nw - 1]);
+  __syncthreads();
+  return make_int2(v.x + px, v.y + py);
+}
+
 // Number of elements of sorted array a[0..n) that are < v.
 template <class T>
 __device__ __forceinline__ int lower_bound_t(const T* a, int n, T v) {

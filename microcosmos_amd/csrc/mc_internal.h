@@ -134,14 +134,14 @@ struct LbmArgs {
 // of steric rows its CIC entries touch (comp_rows); clears *arena_top.
 hipError_t launch_predict(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
                           const float* pos, const float* vel, float* pred_w, int* cic_cell,
-                          float* cic_w, int2* comp_rows, int* arena_top, hipStream_t s);
+                          float* cic_w, int4* comp_rows, int* arena_top, hipStream_t s);
 // Raises the dynamic LDS limit of the steric kernels to lds_max - 1024 bytes.
 hipError_t steric_prepare(int lds_max);
 hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t s);
 // Inputs of the CIC deposit (launch_predict's outputs and scratch).
 struct StericDep {
   RodLayoutDev L;
-  const int2* comp_rows;
+  const int4* comp_rows;
   const int* cic_cell;
   const float* cic_w;
   int* arena;                   // 2N ints

@@ -280,6 +280,37 @@ __device__ __forceinline__ void load_fft_twiddles(float2* dst, const float2* __r
   for (int i = threadIdx.x; i < len; i += blockDim.x) dst[i] = src[i];
 }
 
+// The same table in two halves: its first TW_REGS * blockDim elements loaded
+// into registers (issued early, so the loads overlap other work), then
+// stored to LDS together with the rest.
+constexpr int TW_REGS = 4;
+struct TwPrefetch {
+  float2 v[TW_REGS];
+};
+__device__ __forceinline__ void twiddles_fetch(TwPrefetch& t, const float2* __restrict__ tw,
+                                               const FftDev& fp) {
+  const bool ct = ct_supported(fp.n);
+  const float2* src = tw + (ct ? fp.stw_off : fp.tw_off);
+  const int len = ct ? fp.stw_len : fp.n;
+#pragma unroll
+  for (int q = 0; q < TW_REGS; ++q) {
+    const int i = threadIdx.x + q * blockDim.x;
+    t.v[q] = i < len ? src[i] : make_float2(0.0f, 0.0f);
+  }
+}
+__device__ __forceinline__ void twiddles_store(const TwPrefetch& t, float2* dst,
+                                               const float2* __restrict__ tw, const FftDev& fp) {
+  const bool ct = ct_supported(fp.n);
+  const float2* src = tw + (ct ? fp.stw_off : fp.tw_off);
+  const int len = ct ? fp.stw_len : fp.n;
+#pragma unroll
+  for (int q = 0; q < TW_REGS; ++q) {
+    const int i = threadIdx.x + q * blockDim.x;
+    if (i < len) dst[i] = t.v[q];
+  }
+  for (int i = threadIdx.x + TW_REGS * blockDim.x; i < len; i += blockDim.x) dst[i] = src[i];
+}
+
 // FFT of length n on a (unpadded; a and b hold ct_buf(n) elements each).
 // Returns the buffer holding the unpadded result.
 __device__ __forceinline__ float2* fft_any(float2* a, float2* b, int n, const int* radix, int nrad,
@@ -322,7 +353,7 @@ __device__ __forceinline__ void predict_xy(const Consts& c, const InstGeo& gi,
 // One wavefront per filament (connected component of the edge graph): each
 // node's wrapped prediction, CIC cell and weights (forces.py:95-110), and the
 // band of instance rows the component's CIC entries touch, rows
-// start .. start + len - 1 (mod H).  A node's row y0 is the first node's row
+// start .. start + len - 1 (mod H), with the component's node count and ids.  A node's row y0 is the first node's row
 // plus a minimum-image offset d in [-H/2, H/2); the band runs from min d to
 // max d + 1, so it holds every y0 and y1 = y0 + 1 of the component whatever
 // its shape (a loose band only costs the row pass extra checks).  Block 0
@@ -333,7 +364,7 @@ __global__ void __launch_bounds__(64) k_predict(Consts c, BatchDev bd, RodLayout
                                                 float* __restrict__ pred_w,
                                                 int* __restrict__ cic_cell,
                                                 float* __restrict__ cic_w,
-                                                int2* __restrict__ comp_rows,
+                                                int4* __restrict__ comp_rows,
                                                 int* __restrict__ arena_top) {
   const int comp = blockIdx.x, lane = threadIdx.x;
   if (comp == 0 && lane == 0) *arena_top = 0;
@@ -377,7 +408,11 @@ __global__ void __launch_bounds__(64) k_predict(Consts c, BatchDev bd, RodLayout
   }
   if (lane == 0) {
     const int len = hi + 2 - lo;  // rows lo .. hi + 1 relative to yref
-    comp_rows[comp] = len >= H ? make_int2(0, H) : make_int2(imod(yref + lo, H), len);
+    // with the component's node count and first id (contiguous ids) or the
+    // complement of its node-list offset, for the row pass
+    const int first = m.nf >= 0 ? m.nf : ~m.n0;
+    comp_rows[comp] = len >= H ? make_int4(0, H, m.nn, first)
+                               : make_int4(imod(yref + lo, H), len, m.nn, first);
   }
 }
 
@@ -392,17 +427,19 @@ __device__ __forceinline__ int xcd_group(int b, int G) {
 // node rank) puts each cell's contributions contiguously and in the
 // reference's order (all w00 by node, then w10, w01, w11 --
 // forces.py:112-115); a segment head thread folds them from zero.  Rows of up
-// to `cap` nodes order node ids and entry keys by counting ranks in `scratch`
-// (DEP_BYTES_PER_NODE bytes of LDS per node, aliasing the FFT's second buffer
-// and twiddles); longer rows take a slice of the global arena and merge-sort
-// in global scratch.
-constexpr int DEP_BYTES_PER_NODE = 36;  // sorted id + 2 entry keys + 2 ranked keys
+// to `cap` nodes keep each node's id, cells and weights in LDS (one global
+// round trip) and order node ids and entry keys by counting ranks in
+// `scratch` (DEP_BYTES_PER_NODE bytes of LDS per node, aliasing the FFT's
+// second buffer); longer rows take a slice of the global arena and
+// merge-sort in global scratch.
+constexpr int DEP_BYTES_PER_NODE = 52;  // rank, 2 entry keys, 2 ranked keys, cells, weights
 constexpr int DEP_CAP_MAX = 256;
 constexpr int DEP_CAP_MIN = 64;
 
 struct DepArgs {
   RodLayoutDev L;
-  const int2* comp_rows;        // per component: first row and row count of its band
+  const int4* comp_rows;        // per component: band (first row, rows), node count, first id
+                                // (contiguous ids) or ~(node-list offset)
   const int* cic_cell;
   const float* cic_w;
   int* arena;                   // 2N ids for rows beyond cap
@@ -410,42 +447,40 @@ struct DepArgs {
   int* sort_tmp;                // 2N ints
   unsigned long long* key_tmp;  // 8N keys: entries at 2 aoff, merge scratch at key_half + 2 aoff
   size_t key_half;
-  int cap;                      // LDS capacity in nodes (16 cap >= 12 blockDim)
+  int cap;                      // LDS capacity in nodes (a multiple of 4, 16 cap >= 8 blockDim)
+  int b_off, tw_off;            // LDS offsets (float2) of the scratch / second buffer and twiddles
 };
 
 // Ids of the nodes whose CIC rows include global row `row`, each once: the
 // instance's components whose band holds the row are compacted per chunk of
-// blockDim components (workgroup scans), then their nodes are checked.  Match
-// p goes to out[p] when p < lim.  `cand` holds 3 blockDim ints of LDS.
+// blockDim components (one workgroup scan), then their nodes are checked.
+// Match p goes to out[p] when p < lim, with (when rx is not NULL) its cells
+// rx[p] = (x0 | role << 30, x1), role 0 for the y0 row, and the role's two
+// weights rw[p].  `cand` holds 2 blockDim ints of LDS, `wsum` 32 ints.
 // Returns the number of matches (the same in every thread).
 __device__ int collect_row_nodes(const DepArgs& a, const InstGeo& gi, int row, int* out, int lim,
-                                 int* cand, int* wsum, int* s_cnt) {
+                                 int2* rx, float2* rw, int* cand, int* wsum, int* s_cnt) {
   const int B = blockDim.x, tid = threadIdx.x, H = gi.H, y = row - gi.row0;
-  int* c_first = cand;        // first node id of a contiguous component, else -1
-  int* c_list = cand + B;     // offset of its node list
-  int* c_off = cand + 2 * B;  // prefix of node counts over the chunk's candidates
+  int* c_first = cand;    // first node id (contiguous ids) or ~(node-list offset)
+  int* c_off = cand + B;  // prefix of node counts over the chunk's candidates
   if (tid == 0) *s_cnt = 0;
   __syncthreads();
   for (int base = 0; base < gi.ncomp; base += B) {
-    int is = 0, nn = 0, first = 0, list = 0;
+    int is = 0, nn = 0, first = 0;
     if (base + tid < gi.ncomp) {
-      const int k = gi.comp0 + base + tid;
-      const int2 r = a.comp_rows[k];
+      const int4 r = a.comp_rows[gi.comp0 + base + tid];
       if (imod(y - r.x, H) < r.y) {
-        const CompMeta& m = a.L.meta[k];
         is = 1;
-        nn = m.nn;
-        first = m.nf;
-        list = m.n0;
+        nn = r.z;
+        first = r.w;
       }
     }
-    int ccount, ntot;
-    const int cpos = block_inclusive_scan(is, wsum, &ccount) - is;
-    const int noff = block_inclusive_scan(nn, wsum, &ntot) - nn;
+    int2 tot;
+    const int2 inc = block_inclusive_scan2(make_int2(is, nn), wsum, &tot);
+    const int ccount = tot.x, ntot = tot.y;
     if (is) {
-      c_first[cpos] = first;
-      c_list[cpos] = list;
-      c_off[cpos] = noff;
+      c_first[inc.x - 1] = first;
+      c_off[inc.x - 1] = inc.y - nn;
     }
     __syncthreads();
     for (int f = tid; f < ntot; f += B) {
@@ -454,12 +489,20 @@ __device__ int collect_row_nodes(const DepArgs& a, const InstGeo& gi, int row, i
         const int mid = (lo + hi + 1) >> 1;
         if (c_off[mid] <= f) lo = mid; else hi = mid - 1;
       }
-      const int local = f - c_off[lo];
-      const int n = c_first[lo] >= 0 ? c_first[lo] + local : a.L.comp_nodes[c_list[lo] + local];
-      const int2 yy = *reinterpret_cast<const int2*>(a.cic_cell + 4 * (size_t)n + 2);
-      if (yy.x == row || yy.y == row) {
+      const int local = f - c_off[lo], fc = c_first[lo];
+      const int n = fc >= 0 ? fc + local : a.L.comp_nodes[~fc + local];
+      const int4 cc = *reinterpret_cast<const int4*>(a.cic_cell + 4 * (size_t)n);
+      const float4 ww = *reinterpret_cast<const float4*>(a.cic_w + 4 * (size_t)n);
+      if (cc.z == row || cc.w == row) {
         const int p = atomicAdd(s_cnt, 1);
-        if (p < lim) out[p] = n;
+        if (p < lim) {
+          out[p] = n;
+          if (rx) {
+            const int role = cc.z == row ? 0 : 1;
+            rx[p] = make_int2(cc.x | (role << 30), cc.y);
+            rw[p] = role ? make_float2(ww.z, ww.w) : make_float2(ww.x, ww.y);
+          }
+        }
       }
     }
     __threadfence_block();
@@ -475,27 +518,28 @@ __device__ void deposit_row(const DepArgs& a, const InstGeo& gi, int row, float2
   int* sorted = reinterpret_cast<int*>(scratch);                                 // [cap]
   unsigned long long* k0 = reinterpret_cast<unsigned long long*>(sorted + cap);  // [2 cap]
   unsigned long long* k1 = k0 + 2 * cap;                                         // [2 cap]
+  int2* rx = reinterpret_cast<int2*>(k1 + 2 * cap);                              // [cap]
+  float2* rw = reinterpret_cast<float2*>(rx + cap);                              // [cap]
   int* ids = reinterpret_cast<int*>(k0);   // arrival order, dead once ranked
-  int* cand = reinterpret_cast<int*>(k1);  // candidate components, dead once collected
-  const int m = collect_row_nodes(a, gi, row, ids, cap, cand, wsum, s_cnt);
+  int* cand = reinterpret_cast<int*>(k1);  // candidate components (2 B ints), dead once collected
+  const int m = collect_row_nodes(a, gi, row, ids, cap, rx, rw, cand, wsum, s_cnt);
   if (m == 0) return;
   const int ne = 2 * m;
-  const int* nodes;
-  const unsigned long long* keys;
   if (m <= cap) {
+    // sorted[rank] = arrival index of the node of that rank
     for (int j = tid; j < m; j += B) {
       const int v = ids[j];
       int r = 0;
       for (int k = 0; k < m; ++k) r += ids[k] < v ? 1 : 0;
-      sorted[r] = v;
+      sorted[r] = j;
     }
     __syncthreads();
     for (int j = tid; j < m; j += B) {
-      const int n = sorted[j];
-      const int4 cc = *reinterpret_cast<const int4*>(a.cic_cell + 4 * (size_t)n);
-      const unsigned long long qb = (cc.z == row) ? 0ull : 2ull;
-      k0[2 * j] = ((unsigned long long)(unsigned)cc.x << 34) | (qb << 32) | (unsigned)j;
-      k0[2 * j + 1] = ((unsigned long long)(unsigned)cc.y << 34) | ((qb + 1ull) << 32) | (unsigned)j;
+      const int2 c = rx[sorted[j]];
+      const unsigned long long qb = (unsigned long long)(c.x >> 30) << 1;
+      const unsigned long long x0 = (unsigned)(c.x & 0x3fffffff), x1 = (unsigned)c.y;
+      k0[2 * j] = (x0 << 34) | (qb << 32) | (unsigned)j;
+      k0[2 * j + 1] = (x1 << 34) | ((qb + 1ull) << 32) | (unsigned)j;
     }
     __syncthreads();
     for (int i = tid; i < ne; i += B) {
@@ -505,40 +549,51 @@ __device__ void deposit_row(const DepArgs& a, const InstGeo& gi, int row, float2
       k1[r] = v;
     }
     __syncthreads();
-    nodes = sorted;
-    keys = k1;
-  } else {
-    if (tid == 0) s_cnt[1] = atomicAdd(a.arena_top, m);
-    __syncthreads();
-    const int aoff = s_cnt[1];
-    int* nl = a.arena + aoff;
-    collect_row_nodes(a, gi, row, nl, m, cand, wsum, s_cnt);
-    sort_unique<int>(nl, m, nullptr, 0, a.sort_tmp + aoff, 0x7fffffff);
-    unsigned long long* kk = a.key_tmp + 2 * (size_t)aoff;
-    for (int j = tid; j < m; j += B) {
-      const int n = nl[j];
-      const int4 cc = *reinterpret_cast<const int4*>(a.cic_cell + 4 * (size_t)n);
-      const unsigned long long qb = (cc.z == row) ? 0ull : 2ull;
-      kk[2 * j] = ((unsigned long long)(unsigned)cc.x << 34) | (qb << 32) | (unsigned)j;
-      kk[2 * j + 1] = ((unsigned long long)(unsigned)cc.y << 34) | ((qb + 1ull) << 32) | (unsigned)j;
+    for (int i = tid; i < ne; i += B) {
+      const unsigned long long k = k1[i];
+      const int x = (int)(k >> 34);
+      if (i > 0 && (int)(k1[i - 1] >> 34) == x) continue;
+      float acc = 0.0f;
+      for (int e = i; e < ne; ++e) {
+        const unsigned long long ke = k1[e];
+        if ((int)(ke >> 34) != x) break;
+        const float2 w2 = rw[sorted[(int)(ke & 0xffffffffull)]];
+        acc = acc + (((ke >> 32) & 1ull) ? w2.y : w2.x);
+      }
+      dst[x].x = acc;
     }
-    __threadfence_block();
     __syncthreads();
-    sort_unique<unsigned long long>(kk, ne, nullptr, 0, a.key_tmp + a.key_half + 2 * (size_t)aoff,
-                                    ~0ull);
-    nodes = nl;
-    keys = kk;
+    return;
   }
+  // rows beyond cap: ids in a slice of the arena, sorts and keys in global scratch
+  if (tid == 0) s_cnt[1] = atomicAdd(a.arena_top, m);
+  __syncthreads();
+  const int aoff = s_cnt[1];
+  int* nl = a.arena + aoff;
+  collect_row_nodes(a, gi, row, nl, m, nullptr, nullptr, cand, wsum, s_cnt);
+  sort_unique<int>(nl, m, nullptr, 0, a.sort_tmp + aoff, 0x7fffffff);
+  unsigned long long* kk = a.key_tmp + 2 * (size_t)aoff;
+  for (int j = tid; j < m; j += B) {
+    const int n = nl[j];
+    const int4 cc = *reinterpret_cast<const int4*>(a.cic_cell + 4 * (size_t)n);
+    const unsigned long long qb = (cc.z == row) ? 0ull : 2ull;
+    kk[2 * j] = ((unsigned long long)(unsigned)cc.x << 34) | (qb << 32) | (unsigned)j;
+    kk[2 * j + 1] = ((unsigned long long)(unsigned)cc.y << 34) | ((qb + 1ull) << 32) | (unsigned)j;
+  }
+  __threadfence_block();
+  __syncthreads();
+  sort_unique<unsigned long long>(kk, ne, nullptr, 0, a.key_tmp + a.key_half + 2 * (size_t)aoff,
+                                  ~0ull);
   for (int i = tid; i < ne; i += B) {
-    const unsigned long long k = keys[i];
+    const unsigned long long k = kk[i];
     const int x = (int)(k >> 34);
-    if (i > 0 && (int)(keys[i - 1] >> 34) == x) continue;
+    if (i > 0 && (int)(kk[i - 1] >> 34) == x) continue;
     float acc = 0.0f;
     for (int e = i; e < ne; ++e) {
-      const unsigned long long ke = keys[e];
+      const unsigned long long ke = kk[e];
       if ((int)(ke >> 34) != x) break;
       const int q = (int)((ke >> 32) & 3ull);
-      const int n = nodes[(int)(ke & 0xffffffffull)];
+      const int n = nl[(int)(ke & 0xffffffffull)];
       acc = acc + a.cic_w[4 * n + q];
     }
     dst[x].x = acc;
@@ -571,16 +626,18 @@ __global__ void __launch_bounds__(256) k_rows_fwd_T(BatchDev bd, const float* __
                                                     float* __restrict__ density_out) {
   extern __shared__ float2 lds[];
   __shared__ int s_radix[FFT_MAX_RADICES];
-  __shared__ int wsum[16];
+  __shared__ int wsum[32];
   __shared__ int s_cnt[2];
   const int row = xcd_group(blockIdx.x, gridDim.x);
   const InstGeo& gi = bd.geo[bd.row_inst[row]];
   const int W = gi.W, H = gi.H, y = row - gi.row0;
   const FftDev& fp = bd.fft[gi.fft_w];
   float2* a = lds;
-  float2* b = lds + ct_buf(W);
-  float2* tw = lds + 2 * ct_buf(W);  // b and tw alias the deposit scratch
+  float2* b = lds + dep.b_off;    // the second FFT buffer aliases the deposit scratch
+  float2* tw = lds + dep.tw_off;
   load_plan(fp, s_radix);
+  TwPrefetch twp;
+  twiddles_fetch(twp, bd.tw, fp);  // in flight during the deposit
   const size_t g0 = (size_t)gi.grid_off + (size_t)y * W;
   if (real_in) {
     for (int i = threadIdx.x; i < W; i += blockDim.x) a[i] = make_float2(real_in[g0 + i], 0.0f);
@@ -589,7 +646,7 @@ __global__ void __launch_bounds__(256) k_rows_fwd_T(BatchDev bd, const float* __
     if (density_out)
       for (int i = threadIdx.x; i < W; i += blockDim.x) density_out[g0 + i] = a[i].x;
   }
-  load_fft_twiddles(tw, bd.tw, fp);
+  twiddles_store(twp, tw, bd.tw, fp);
   __syncthreads();
   float2* res = fft_any(a, b, W, s_radix, fp.nrad, tw, false);
   float2* sp = specT + gi.grid_off;
@@ -669,7 +726,7 @@ __global__ void k_gauss_table_T(int H, int W, float coef, float* __restrict__ kf
 
 hipError_t launch_predict(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
                           const float* pos, const float* vel, float* pred_w, int* cic_cell,
-                          float* cic_w, int2* comp_rows, int* arena_top, hipStream_t s) {
+                          float* cic_w, int4* comp_rows, int* arena_top, hipStream_t s) {
   if (L.num_comp == 0) return hipSuccess;
   hipLaunchKernelGGL(k_predict, dim3(L.num_comp), dim3(64), 0, s, c, b, L, pos, vel, pred_w,
                      cic_cell, cic_w, comp_rows, arena_top);
@@ -709,33 +766,39 @@ static size_t fft_pass_bytes(int n) {
   return ((size_t)2 * ct_buf(n) + fft_tw_elems(n)) * sizeof(float2);
 }
 
-// LDS of the forward row pass: the row, then the larger of (second buffer +
-// twiddles) and the deposit scratch of `cap` nodes.
-static size_t rows_region_bytes(int max_w, int cap) {
-  const size_t fft = fft_pass_bytes(max_w) - (size_t)ct_buf(max_w) * sizeof(float2);
-  const size_t dep = (size_t)cap * DEP_BYTES_PER_NODE;
-  return fft > dep ? fft : dep;
-}
 // Smallest deposit capacity: DEP_CAP_MIN, and room for the candidate lists
-// (3 ints per thread in the 16 cap bytes of the ranked keys).
+// (2 ints per thread in the 16 cap bytes of the ranked keys).
 static int dep_cap_floor(int threads) {
-  const int c = (3 * threads + 3) / 4;
-  return ((c > DEP_CAP_MIN ? c : DEP_CAP_MIN) + 1) & ~1;
+  const int c = threads / 2;
+  return ((c > DEP_CAP_MIN ? c : DEP_CAP_MIN) + 3) & ~3;
 }
-size_t fft_rows_lds_bytes(int max_w) {
-  return (size_t)ct_buf(max_w) * sizeof(float2) +
-         rows_region_bytes(max_w, dep_cap_floor(fft_threads(max_w)));
+
+// LDS layout of the forward row pass (float2 units): the row, the second
+// buffer (which the deposit scratch of `cap` nodes aliases), the twiddles.
+struct RowsLayout {
+  int b_off, tw_off, total, cap;
+};
+static RowsLayout rows_layout(int max_w) {
+  RowsLayout L;
+  const int floor_cap = dep_cap_floor(fft_threads(max_w));
+  L.b_off = (ct_buf(max_w) + 1) & ~1;  // 16-byte aligned scratch
+  int bsz = (ct_buf(max_w) + 1) & ~1;
+  const int need = (floor_cap * DEP_BYTES_PER_NODE + 15) / 16 * 2;
+  if (bsz < need) bsz = need;
+  L.cap = (bsz * (int)sizeof(float2) / DEP_BYTES_PER_NODE) & ~3;
+  if (L.cap > DEP_CAP_MAX) L.cap = DEP_CAP_MAX;
+  L.tw_off = L.b_off + bsz;
+  L.total = L.tw_off + fft_tw_elems(max_w);
+  return L;
 }
+size_t fft_rows_lds_bytes(int max_w) { return (size_t)rows_layout(max_w).total * sizeof(float2); }
 size_t fft_cols_lds_bytes(int max_h) { return fft_pass_bytes(max_h); }
 
 hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const StericDep* d,
                                float2* specT, const float* kfT, float* density_out, float* out,
                                hipStream_t s) {
   const int tr = fft_threads(b.max_w), tc = fft_threads(b.max_h);
-  const int floor_cap = dep_cap_floor(tr);
-  // rows of any instance (W <= max_w) have at least this region after the row
-  int cap = (int)(rows_region_bytes(b.max_w, floor_cap) / DEP_BYTES_PER_NODE) & ~1;
-  if (cap > DEP_CAP_MAX) cap = DEP_CAP_MAX > floor_cap ? DEP_CAP_MAX : floor_cap;
+  const RowsLayout rl = rows_layout(b.max_w);
   DepArgs dep{};
   if (d) {
     dep.L = d->L;
@@ -748,7 +811,9 @@ hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const St
     dep.key_tmp = d->key_tmp;
     dep.key_half = d->key_half;
   }
-  dep.cap = cap;
+  dep.cap = rl.cap;
+  dep.b_off = rl.b_off;
+  dep.tw_off = rl.tw_off;
   hipLaunchKernelGGL(k_rows_fwd_T, dim3(b.rows), dim3(tr), fft_rows_lds_bytes(b.max_w), s, b,
                      real_in, dep, specT, density_out);
   hipLaunchKernelGGL(k_cols_T, dim3(b.cols), dim3(tc), fft_cols_lds_bytes(b.max_h), s, b, specT,
