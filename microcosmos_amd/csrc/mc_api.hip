@@ -256,6 +256,7 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
     for (int q = einc_off[eoff[c]]; q < einc_off[eoff[c + 1]]; ++q) blob.push_back(einc_flat[q]);
     for (int q = noff[c]; q <= noff[c + 1]; ++q) blob.push_back(ninc_off[q] - ninc_off[noff[c]]);
     for (int q = ninc_off[noff[c]]; q < ninc_off[noff[c + 1]]; ++q) blob.push_back(ninc_flat[q]);
+    while (blob.size() % 4) blob.push_back(0);  // blocks start on 16 bytes
     blob_off.push_back((int)blob.size());
     nfirst[c] = first_of(noff, norder, c);
     efirst[c] = first_of(eoff, eorder, c);

@@ -53,12 +53,16 @@ __device__ __forceinline__ float sample_bilinear_wrap(const float* V, int H, int
 // forces.py:20-81 for one node: -grad V by central differences minus the
 // analytic (2n+1)-window self/neighbour contribution over the flat node index
 // (within the node's instance: an instance is one reference simulation).
+// pw (may be NULL): the predicted positions of nodes pw0 .. in LDS, covering
+// every window node of g after clamping to the instance.
 __device__ __forceinline__ void steric_force(const Consts& c, const InstGeo& gi, const float* V_all,
-                                             const float* pred_w, int g, float& fx, float& fy) {
+                                             const float* pred_w, int g, float& fx, float& fy,
+                                             const float2* pw = nullptr, int pw0 = 0) {
   const int H = gi.H, W = gi.W;
   const float* V = V_all + gi.grid_off;
   const int lo = gi.node0, hi = gi.node0 + gi.nodes;
-  const float px = pred_w[2 * g], py = pred_w[2 * g + 1];
+  const float2 pg = *reinterpret_cast<const float2*>(pred_w + 2 * g);
+  const float px = pg.x, py = pg.y;
   const float gx = (sample_bilinear_wrap(V, H, W, py, px + 0.5f) -
                     sample_bilinear_wrap(V, H, W, py, px - 0.5f)) / 1.0f;
   const float gy = (sample_bilinear_wrap(V, H, W, py + 0.5f, px) -
@@ -69,8 +73,9 @@ __device__ __forceinline__ void steric_force(const Consts& c, const InstGeo& gi,
     const int j = g + k;
     const float valid = (j >= lo && j < hi) ? 1.0f : 0.0f;
     const int js = j < lo ? lo : (j > hi - 1 ? hi - 1 : j);
-    const float rx = min_image(px, pred_w[2 * js], gi.dom_wf);
-    const float ry = min_image(py, pred_w[2 * js + 1], gi.dom_hf);
+    const float2 q = pw ? pw[js - pw0] : *reinterpret_cast<const float2*>(pred_w + 2 * js);
+    const float rx = min_image(px, q.x, gi.dom_wf);
+    const float ry = min_image(py, q.y, gi.dom_hf);
     const float d2 = rx * rx + ry * ry;
     const float Kv = dexp(-d2 / c.steric_2s2) / c.steric_norm;
     const float kv = Kv * valid;
@@ -80,6 +85,8 @@ __device__ __forceinline__ void steric_force(const Consts& c, const InstGeo& gi,
   fx = -gx - c.steric_svs * ax;
   fy = -gy - c.steric_svs * ay;
 }
+
+constexpr int ROD_WIN_MAX = 8;  // steric windows up to this half-width are staged in LDS
 
 __device__ __forceinline__ float wrap_angle(float d) {
   return fmodf_floor(d + MC_PI_F, MC_2PI_F) - MC_PI_F;
@@ -92,8 +99,11 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
   const InstGeo& gi = a.bd.geo[m.inst];
   const int n0 = m.n0, nn = m.nn, e0 = m.e0, ne = m.ne, b0 = m.b0, nb = m.nb;
   const int nni = m.nni, nei = m.nei;
+  // LDS: the steric window's predicted positions pw[nn + 2 ROD_WIN_MAX], then
   // float scratch: x[2nn] y[2nn] th[ne] l[ne] F[2ne] He[ne] corr[nb] rest[nb] kb[nb]
-  float* base = a.use_lds ? smem : (a.gscratch + (size_t)4 * n0 + (size_t)5 * e0 + (size_t)3 * b0);
+  float2* pw = reinterpret_cast<float2*>(smem);
+  float* base = a.use_lds ? smem + 2 * (nn + 2 * ROD_WIN_MAX)
+                          : (a.gscratch + (size_t)4 * n0 + (size_t)5 * e0 + (size_t)3 * b0);
   float* X = base;
   float* Y = X + 2 * nn;
   float* TH = Y + 2 * nn;
@@ -113,7 +123,22 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
   auto hinge_id = [&](int b) { return hf >= 0 ? hf + b : L.comp_hinges[b0 + b]; };
 
   // Every global load that depends on nothing else is issued first, then
-  // the topology copy, then the dependent steric samples.
+  // the topology copy, then the dependent steric samples.  A contiguous
+  // component stages the predicted positions of its steric window (its nodes
+  // and nsk on either side, within the instance) in LDS: one coalesced load
+  // instead of 2 nsk + 1 dependent ones per node.
+  const int nsk = c.steric_nskip;
+  const bool stage_pw = a.use_lds && a.mode == 0 && c.enable_steric && nf >= 0 &&
+                        nsk <= ROD_WIN_MAX && nn + 2 * nsk <= 2 * bs;
+  const int pw0 = nf - nsk;
+  float2 pwv[2];  // entries tid and tid + bs, stored after the other loads are issued
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int i = tid + h * bs, j = pw0 + i;
+    pwv[h] = (stage_pw && i < nn + 2 * nsk && j >= gi.node0 && j < gi.node0 + gi.nodes)
+                 ? *reinterpret_cast<const float2*>(a.pred_w + 2 * (size_t)j)
+                 : make_float2(0.0f, 0.0f);
+  }
   const double tact = a.has_act ? (double)(*a.act.t) : 0.0;
   int g = -1;
   float px = 0.0f, py = 0.0f, vx = 0.0f, vy = 0.0f, yx = 0.0f, yy = 0.0f;
@@ -153,19 +178,24 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
   const int *hin, *hout, *esrc, *etgt, *einc_off, *einc, *ninc_off, *ninc;
   if (a.use_lds) {
     int* ib = reinterpret_cast<int*>(KB + nb);
-    const int* blob = L.blob + m.blob_off;
+    // blocks start on 16 bytes: 16-byte loads, all in flight before the stores
+    const int4* blob4 = reinterpret_cast<const int4*>(L.blob + m.blob_off);
     const int len = 2 * nb + 3 * ne + 1 + nei + nn + 1 + nni;
-    for (int q0 = 0; q0 < len; q0 += 4 * bs) {
-      int v[4];
+    const int len4 = (len + 3) >> 2;
+    for (int q0 = 0; q0 < len4; q0 += 4 * bs) {
+      int4 v[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int q = q0 + k * bs + tid;
-        v[k] = q < len ? blob[q] : 0;
+        v[k] = q < len4 ? blob4[q] : make_int4(0, 0, 0, 0);
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const int q = q0 + k * bs + tid;
-        if (q < len) ib[q] = v[k];
+        const int q = 4 * (q0 + k * bs + tid);
+        if (q < len) ib[q] = v[k].x;
+        if (q + 1 < len) ib[q + 1] = v[k].y;
+        if (q + 2 < len) ib[q + 2] = v[k].z;
+        if (q + 3 < len) ib[q + 3] = v[k].w;
       }
     }
     hin = ib;
@@ -195,7 +225,7 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
     float x = qx + dvx * c.dt, y = qy + dvy * c.dt;
     if (c.enable_steric) {
       float fx, fy;
-      steric_force(c, gi, a.steric, a.pred_w, gg, fx, fy);
+      steric_force(c, gi, a.steric, a.pred_w, gg, fx, fy, stage_pw ? pw : nullptr, pw0);
       x = x + (fx * c.steric_strength) * c.dt;
       y = y + (fy * c.steric_strength) * c.dt;
     }
@@ -211,6 +241,12 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
     const double arg = (double)om_ * tact + (double)ph_;
     return (float)((double)amp_ * sin(arg));
   };
+  if (stage_pw) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      if (tid + h * bs < nn + 2 * nsk) pw[tid + h * bs] = pwv[h];
+    __syncthreads();  // the staged window is complete
+  }
   if (tid < nn) node_prologue(tid, g, px, py, vx, vy, yx, yy);
   if (tid < ne) {
     TH[tid] = th0;
@@ -389,7 +425,8 @@ size_t rod_scratch_floats(int nn, int ne, int nb) {
 
 size_t rod_lds_bytes(int nn, int ne, int nb, int nni, int nei) {
   const size_t ints = (size_t)2 * nb + (size_t)3 * ne + 1 + nei + (size_t)nn + 1 + nni;
-  return rod_scratch_floats(nn, ne, nb) * sizeof(float) + ints * sizeof(int);
+  return ((size_t)nn + 2 * ROD_WIN_MAX) * sizeof(float2) +
+         rod_scratch_floats(nn, ne, nb) * sizeof(float) + ints * sizeof(int);
 }
 
 __global__ void k_steric_nodes(Consts c, BatchDev bd, int N, const float* __restrict__ steric,
