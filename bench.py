@@ -101,6 +101,80 @@ def batched_es(n_inst, steps, plain_sample, dev):
             "p99_latency": None}
 
 
+def serving(n_req, dev, lanes, chunk, max_batch, plain=True):
+    """Served trace (Poisson arrivals of mixed rollout requests) through the
+    continuous-batching server vs the same requests one at a time on the
+    plain GPU path (per request: plan, graph-replayed rollout, displacement)."""
+    import numpy as np
+    import torch
+    from microcosmos_amd import PBD_SCHEME, Rollout
+    from microcosmos_amd import workloads as W
+    from microcosmos_amd.serve import Request, Server, fifo_report, latency_report
+    from tests.helpers import device_state
+    trace = W.serving_trace(n_req, seed=4)
+    cfg = replace(PBD_SCHEME, **trace[0].workload.config_overrides)
+    dt = trace[0].workload.dt
+
+    def requests(items):
+        out = []
+        for it in items:
+            wl = it.workload
+            n, e, f = device_state(wl, dev)
+            e = replace(e, pairs=wl.pairs, bending_pairs=wl.bending_pairs)
+            act = tuple(torch.as_tensor(a, device=dev)
+                        for a in (wl.act_amp, wl.act_omega, wl.act_phase))
+            out.append(Request(nodes=n, edges=e, fields=f, steps=it.steps, actuation=act,
+                               arrival=it.arrival))
+        return out
+
+    warm = W.serving_trace(3, seed=99, max_filaments=4, max_steps=300)
+    Server(cfg, dt, lanes=lanes, chunk=chunk, max_batch=max_batch).run(requests(warm))
+    torch.cuda.synchronize()
+    reqs = requests(trace)
+    torch.cuda.synchronize()
+    print(f"[bench] serving {n_req} requests ...", file=sys.stderr, flush=True)
+    srv = Server(cfg, dt, lanes=lanes, chunk=chunk, max_batch=max_batch)
+    done = srv.run(reqs, realtime=True)
+    torch.cuda.synchronize()
+    rep = latency_report(done)
+    finite = all(bool(torch.isfinite(r.result[0].position).all().item()) for r in done)
+    cells = sum(it.workload.grid_shape[0] * it.workload.grid_shape[1] * it.steps for it in trace)
+    out = {"workload": f"serving-trace-{n_req}", "arrival_rate_per_s": 200.0,
+           "grid_sides": [128, 256, 512, 1024], "filaments": [1, 256], "nodes_per_filament": [16, 64],
+           "rollout_steps": [200, 2000], "lanes": lanes, "chunk_steps": chunk,
+           "max_batch": max_batch, "lattice_cell_updates": cells,
+           "gradient_requests": sum(it.gradient for it in trace),
+           "gradient": "not computed: gradient requests run their forward rollout only",
+           "p50_latency_s": round(rep["p50_latency_s"], 4),
+           "p99_latency_s": round(rep["p99_latency_s"], 4),
+           "mean_queue_wait_s": round(rep["mean_queue_wait_s"], 4),
+           "rollouts_per_s": round(rep["rollouts_per_s"], 3),
+           "mlups": round(cells / rep["makespan_s"] / 1e6, 1),
+           "batch_reforms": sum(l.reforms for l in srv.lanes),
+           "peak_admitted_bytes": srv.peak_used, "finite": finite}
+    print(f"[bench] served: {rep}", file=sys.stderr, flush=True)
+    if plain:
+        service = []
+        for it in trace:
+            wl = it.workload
+            n, e, f = device_state(wl, dev)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ro = Rollout(n, e, f, cfg, dt, actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
+            ro.track()
+            ro.run(it.steps)
+            torch.cuda.synchronize()
+            service.append(time.perf_counter() - t0)
+            del ro
+        fifo = fifo_report(np.array([it.arrival for it in trace]), np.array(service))
+        print(f"[bench] one by one: {fifo}", file=sys.stderr, flush=True)
+        out["plain_one_by_one"] = {k: round(v, 4) if isinstance(v, float) else v
+                                   for k, v in fifo.items()}
+        out["vs_plain_rollouts_per_s"] = round(rep["rollouts_per_s"] / fifo["rollouts_per_s"], 2)
+        out["vs_plain_p99"] = round(fifo["p99_latency_s"] / rep["p99_latency_s"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +188,14 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-lanes", type=int, default=2)
+    ap.add_argument("--serve-chunk", type=int, default=64)
+    ap.add_argument("--serve-max-batch", type=int, default=32)
+    ap.add_argument("--serve-no-plain", action="store_true")
+    ap.add_argument("--only-serve", action="store_true",
+                    help="run only the serving leg and print its JSON")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 9 x 1024 x 1024 float32 device copy (37.7 MB read + "
                          "37.7 MB written per launch) to calibrate HBM byte counters")
@@ -140,6 +222,10 @@ def main():
     from tests.helpers import device_state
 
     dev = torch.device("cuda", 0)
+    if args.only_serve:
+        print(json.dumps(serving(args.serve_requests, dev, args.serve_lanes, args.serve_chunk,
+                                 args.serve_max_batch, plain=not args.serve_no_plain)), flush=True)
+        return
     wl = W.plain_1024()
     nodes, edges, fields = device_state(wl, dev)
     cfg = replace(PBD_SCHEME, **wl.config_overrides)
@@ -170,6 +256,12 @@ def main():
     batched = None
     if not args.no_batch:
         batched = batched_es(args.batch_instances, args.batch_steps, args.plain_sample, dev)
+    served = None
+    if not args.no_serve:
+        served = serving(args.serve_requests, dev, args.serve_lanes, args.serve_chunk,
+                         args.serve_max_batch, plain=not args.serve_no_plain)
+        if batched is not None:
+            batched["p99_latency"] = served["p99_latency_s"]
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
@@ -186,7 +278,7 @@ def main():
                       "nodes_per_filament": 64, "pbd_cycles": 8, "ibm_kernel_size": 4,
                       "ibm_iterations": 5, "steric_sigma": 2.0, "tau_plus": 0.6,
                       "lambda_trt": 0.1875, "hip_graph": use_graph, "finite": finite},
-           "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
+           "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "serving": served,
            "coupled_steps_per_s": round(args.steps / elapsed, 2),
            "step_ms_events": round(step_ms_ev, 5)}
     print(json.dumps(out), flush=True)

@@ -133,7 +133,15 @@ mc_status mc_batch_plan_create(mc_plan** plan, int32_t num_instances,
 /* Number of instances; offsets (may be NULL) receives per instance
  * (grid cell offset, lattice cell offset, first node) as int64 triples. */
 mc_status mc_plan_instances(const mc_plan* plan, int32_t* num_instances, int64_t* offsets);
+/* Waits for all work on the device, then frees the plan. */
 mc_status mc_plan_destroy(mc_plan* plan);
+/* Frees the plan once the work enqueued so far on `stream` has finished,
+ * without synchronising the rest of the device: every call that used the plan
+ * must have been enqueued on `stream` (or have completed).  Plan memory comes
+ * from a per-device stream-ordered pool and returns to it.  No counterpart in
+ * the reference (jit traces are freed by the JAX runtime); it lets a request
+ * server retire plans while other requests keep running. */
+mc_status mc_plan_destroy_after(mc_plan* plan, void* stream);
 /* Device bytes held by the plan (topology + workspace). */
 mc_status mc_plan_workspace_bytes(const mc_plan* plan, size_t* bytes);
 
@@ -150,7 +158,9 @@ mc_status mc_step(mc_plan* plan, const mc_params* params, const mc_state* in, co
  * trajectory that simulate() returns.  use_graph != 0 (and no recording)
  * captures 8 steps in one HIP graph and replays it: within the graph each
  * step's LBM collide-stream runs inside the next step's rod launch, so every
- * kernel is enqueued on one stream; remaining steps run unfused. */
+ * kernel is enqueued on one stream; remaining steps run unfused.  The graph
+ * stays instantiated in the plan and later calls with the same arguments
+ * replay it; the call returns without waiting for the steps. */
 mc_status mc_rollout(mc_plan* plan, const mc_params* params, const mc_state* state,
                      const mc_edge_params* edges, const mc_actuation* act, double dt,
                      int64_t num_steps, float* record_position, float* record_fluid_velocity,
@@ -165,6 +175,24 @@ mc_status mc_rollout(mc_plan* plan, const mc_params* params, const mc_state* sta
 mc_status mc_batch_rollout(mc_plan* plan, const mc_params* params, const mc_state* state,
                            const mc_edge_params* edges, const mc_actuation* act, double dt,
                            int64_t num_steps, float* displacement_out, void* stream);
+
+/* Continuous batching: mc_batch_rollout for instances that each run on their
+ * own clock.  Instance i's actuation uses t = act->t0 + instance_t0[i] + step
+ * index (instance_t0: host, num_instances entries, may be NULL), so
+ * instances that joined the batch at different points of their own rollouts
+ * keep their own actuation phase.  displacement_io (num_instances, 2),
+ * device, may be NULL: on entry each instance's cumulative centre-of-mass
+ * displacement so far, on return that value advanced over these steps
+ * (swim_base.py:179-209) -- the same sum, in the same order, as one
+ * uninterrupted rollout, so a rollout split over any sequence of calls and
+ * batches gives bit-identical results.  Stream-ordered: returns without
+ * waiting for the steps (the S-step graph is instantiated once and replayed
+ * by later calls with the same arguments).  Replaces the per-request jit
+ * calls of the reference's ES callers (gym/base.py:163-196). */
+mc_status mc_batch_advance(mc_plan* plan, const mc_params* params, const mc_state* state,
+                           const mc_edge_params* edges, const mc_actuation* act,
+                           const int64_t* instance_t0, double dt, int64_t num_steps,
+                           float* displacement_io, void* stream);
 
 /* Measurement: num_steps coupled steps in place (no graph, the LBM
  * collide-stream as its own launch) with HIP events carried by every LBM

@@ -59,6 +59,7 @@ SIGNATURES = {
     "mc_plan_create": (c_int32, [POINTER(c_void_p), c_int32, c_int32, c_int32, c_int32, c_int32,
                                  c_void_p, c_int32, c_void_p, c_int32, c_int32]),
     "mc_plan_destroy": (c_int32, [c_void_p]),
+    "mc_plan_destroy_after": (c_int32, [c_void_p, c_void_p]),
     "mc_plan_workspace_bytes": (c_int32, [c_void_p, POINTER(c_size_t)]),
     "mc_step": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState), POINTER(McState),
                           POINTER(McEdgeParams), c_double, c_void_p]),
@@ -81,6 +82,9 @@ SIGNATURES = {
     "mc_batch_rollout": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
                                    POINTER(McEdgeParams), POINTER(McActuation), c_double, c_int64,
                                    c_void_p, c_void_p]),
+    "mc_batch_advance": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
+                                   POINTER(McEdgeParams), POINTER(McActuation), c_void_p,
+                                   c_double, c_int64, c_void_p, c_void_p]),
     "mc_center_of_mass": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32,
                                     c_void_p, c_void_p]),
     "mc_track_enable": (c_int32, [c_void_p, c_int32, c_void_p, c_int32]),

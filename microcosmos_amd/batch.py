@@ -23,6 +23,14 @@ from ._runtime import KMAX, _host_topology, as_f32, params_from_config, ptr, str
 from .structs import Edges, Fields, Nodes
 
 
+class _nullctx:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *a):
+        return False
+
+
 class BatchPlan:
     """Owns one batched mc_plan handle."""
 
@@ -46,6 +54,18 @@ class BatchPlan:
     def num_nodes(self):
         return int(self.node_offsets[-1])
 
+    def workspace_bytes(self) -> int:
+        b = ctypes.c_size_t()
+        check(self._lib.mc_plan_workspace_bytes(self.handle, ctypes.byref(b)))
+        return int(b.value)
+
+    def release(self, stream: int):
+        """Frees the plan after the work enqueued on ``stream`` (a hipStream_t
+        handle) has finished, without synchronising other streams."""
+        if getattr(self, "handle", None) and self.handle.value:
+            check(self._lib.mc_plan_destroy_after(self.handle, stream))
+            self.handle = ctypes.c_void_p()
+
     def __del__(self):
         if getattr(self, "handle", None) and self.handle.value:
             self._lib.mc_plan_destroy(self.handle)
@@ -61,11 +81,17 @@ class Batch:
     rest_b(t) = amp sin(omega t + phase), applied in-kernel.
     """
 
-    def __init__(self, instances, solver_config, dt: float, actuation=None, device=None):
+    def __init__(self, instances, solver_config, dt: float, actuation=None, device=None,
+                 stream=None):
         if not instances:
             raise ValueError("empty batch")
         dev = torch.device(device) if device is not None else instances[0][0].position.device
         self.dev = dev
+        self.stream = stream
+        with torch.cuda.stream(stream) if stream is not None else _nullctx():
+            self._build(instances, solver_config, dt, actuation, dev)
+
+    def _build(self, instances, solver_config, dt, actuation, dev):
         self.params = params_from_config(solver_config)
         self.dt = float(dt)
         self.n = len(instances)
@@ -108,6 +134,9 @@ class Batch:
             self.act = tuple(cat([a[k] for a in actuation]) for k in range(3))
         self.t = 0
 
+    def _stream(self) -> int:
+        return int(self.stream.cuda_stream) if self.stream is not None else stream_handle(self.dev)
+
     def _state(self):
         return McState(ptr(self.position), ptr(self.velocity), ptr(self.debug_vector),
                        ptr(self.theta), ptr(self.steric), ptr(self.fluid_velocity),
@@ -127,9 +156,39 @@ class Batch:
         disp = torch.empty((self.n, 2), dtype=torch.float32, device=self.dev) if track else None
         check(lib.mc_batch_rollout(self.plan.handle, ctypes.byref(self.params), ctypes.byref(st),
                                    ctypes.byref(ep), ctypes.byref(act) if act else None, self.dt,
-                                   int(num_steps), ptr(disp), stream_handle(self.dev)))
+                                   int(num_steps), ptr(disp), self._stream()))
         self.t += int(num_steps)
         return disp
+
+    def advance(self, num_steps: int, instance_t0, displacement: torch.Tensor):
+        """num_steps steps of every instance, instance i at its own clock
+        (actuation time instance_t0[i] + steps this batch has run), adding
+        each instance's centre-of-mass displacement over these steps to
+        ``displacement`` (n, 2) in place (mc_batch_advance).  Stream-ordered:
+        returns without waiting."""
+        lib = _lib.load()
+        st = self._state()
+        ep = _lib.McEdgeParams(ptr(self.rest_lengths), ptr(self.rest_angles), ptr(self.stiffness))
+        act = None
+        t0 = np.ascontiguousarray(np.asarray(instance_t0, dtype=np.int64).reshape(-1))
+        if t0.shape[0] != self.n:
+            raise ValueError(f"instance_t0 has {t0.shape[0]} entries for {self.n} instances")
+        if displacement.shape != (self.n, 2) or displacement.dtype != torch.float32:
+            raise ValueError("displacement must be a float32 (n, 2) device tensor")
+        if self.act is not None:
+            a, o, p = self.act
+            act = McActuation(ptr(a), ptr(o), ptr(p), int(self.t))
+        check(lib.mc_batch_advance(self.plan.handle, ctypes.byref(self.params), ctypes.byref(st),
+                                   ctypes.byref(ep), ctypes.byref(act) if act else None,
+                                   t0.ctypes.data, self.dt, int(num_steps), ptr(displacement),
+                                   self._stream()))
+        self.t += int(num_steps)
+
+    def close(self):
+        """Releases the plan once this batch's stream has finished its work."""
+        if self.plan is not None:
+            self.plan.release(self._stream())
+            self.plan = None
 
     def instance(self, i: int):
         """(Nodes, Edges, Fields) views of instance i's current state."""

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -58,6 +59,65 @@ struct InstHost {
   int H, W, Hf, Wf, node0, nodes;
 };
 
+namespace {
+
+// Plan memory comes from one stream-ordered pool per device (release
+// threshold at maximum, so freed blocks stay cached for the next plan) and
+// is allocated, zero-filled and released on a library-owned non-blocking
+// stream.  Releasing a plan therefore never synchronises the device the way
+// hipFree does, which matters for a server that retires and builds plans
+// while other requests keep running.  When the pool cannot be created,
+// plans fall back to hipMalloc / hipFree.
+struct DevicePool {
+  bool tried = false;
+  hipMemPool_t pool = nullptr;
+  hipStream_t stream = nullptr;
+};
+std::mutex g_pool_mu;
+DevicePool g_pools[64];
+
+DevicePool* device_pool(int dev) {
+  if (dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  DevicePool& d = g_pools[dev];
+  if (!d.tried) {
+    d.tried = true;
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t pool = nullptr;
+    hipStream_t s = nullptr;
+    if (hipMemPoolCreate(&pool, &props) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess &&
+          hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess) {
+        d.pool = pool;
+        d.stream = s;
+      } else {
+        (void)hipMemPoolDestroy(pool);
+      }
+    }
+    (void)hipGetLastError();
+  }
+  return d.pool ? &d : nullptr;
+}
+
+}  // namespace
+
+// Key of the cached rollout graph: everything the captured kernels take.
+struct GraphKey {
+  mc_params q;
+  double dt;
+  mc_state sa, sb;
+  mc_edge_params ed;
+  Actuation act;
+  int has_act;
+  int64_t steps;
+  TrackWork track;
+};
+
 struct mc_plan {
   int dev = 0;
   int N = 0, E = 0, B = 0, kmax = 0;
@@ -74,6 +134,7 @@ struct mc_plan {
   int rows = 0, cols = 0, lbm_blocks = 0, max_w = 0, max_h = 0, min_hf = 0, min_wf = 0;
   BatchDev bd{};
   std::vector<void*> allocs;
+  bool pooled = false;          // allocs come from the device pool
   size_t bytes = 0;
   // rod
   RodLayoutDev L{};
@@ -102,6 +163,14 @@ struct mc_plan {
   float* kf = nullptr;          // Gaussian transfer functions, one table per distinct shape
   double kf_sigma = -1.0;       // sigma the table holds (-1: none)
   hipStream_t cap_stream = nullptr;
+  // the instantiated S-step rollout graph of the last mc_rollout call, reused
+  // while its key matches; `graph_done` follows its last launch
+  GraphKey graph_key;
+  hipGraphExec_t graph_exec = nullptr;
+  hipEvent_t graph_done = nullptr;
+  // per-instance actuation time offsets (mc_batch_advance), host mirror
+  long long* t_inst = nullptr;
+  std::vector<long long> t_inst_h;
   TrackWork track{};            // displacement tracking (off: n_groups == 0)
   TrackWork inst_track{};       // per-instance tracker of mc_batch_rollout
 
@@ -110,11 +179,25 @@ struct mc_plan {
     *p = nullptr;
     if (count == 0) count = 1;
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, count * sizeof(T));
-    if (e != hipSuccess) return fail(MC_ERR_OOM, "hipMalloc(%s, %zu B): %s", what, count * sizeof(T), hipGetErrorString(e));
-    e = hipMemset(q, 0, count * sizeof(T));
-    if (e != hipSuccess) return fail(MC_ERR_HIP, "hipMemset(%s): %s", what, hipGetErrorString(e));
-    allocs.push_back(q);
+    const size_t nb = count * sizeof(T);
+    DevicePool* dp = device_pool(dev);
+    if (allocs.empty()) pooled = dp != nullptr;
+    if (pooled) {
+      // zero-filled and complete before return: later synchronous uploads on
+      // the null stream are ordered after the fill
+      hipError_t e = hipMallocFromPoolAsync(&q, nb, dp->pool, dp->stream);
+      if (e != hipSuccess) return fail(MC_ERR_OOM, "pool allocation (%s, %zu B): %s", what, nb, hipGetErrorString(e));
+      allocs.push_back(q);
+      e = hipMemsetAsync(q, 0, nb, dp->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(dp->stream);
+      if (e != hipSuccess) return fail(MC_ERR_HIP, "zero fill (%s): %s", what, hipGetErrorString(e));
+    } else {
+      hipError_t e = hipMalloc(&q, nb);
+      if (e != hipSuccess) return fail(MC_ERR_OOM, "hipMalloc(%s, %zu B): %s", what, nb, hipGetErrorString(e));
+      allocs.push_back(q);
+      e = hipMemset(q, 0, nb);
+      if (e != hipSuccess) return fail(MC_ERR_HIP, "hipMemset(%s): %s", what, hipGetErrorString(e));
+    }
     bytes += count * sizeof(T);
     *p = static_cast<T*>(q);
     return MC_OK;
@@ -128,6 +211,20 @@ namespace {
     mc_status s_ = (expr);          \
     if (s_ != MC_OK) return s_;     \
   } while (0)
+
+// Frees everything the plan holds; no work that uses it may be pending.
+void release_plan(mc_plan* p) {
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph_done) (void)hipEventDestroy(p->graph_done);
+  DevicePool* dp = p->pooled ? device_pool(p->dev) : nullptr;
+  for (void* q : p->allocs) {
+    if (dp) (void)hipFreeAsync(q, dp->stream);
+    else (void)hipFree(q);
+  }
+  if (dp) (void)hipStreamSynchronize(dp->stream);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  delete p;
+}
 
 template <class T>
 mc_status upload(mc_plan* p, T** dst, const std::vector<T>& v, const char* what) {
@@ -716,7 +813,7 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
   p->kmax = max_kernel_size;
   auto bail = [&](mc_status st) {
     std::string keep = g_err;
-    mc_plan_destroy(p);
+    release_plan(p);
     g_err = keep;
     return st;
   };
@@ -820,9 +917,24 @@ mc_status mc_batch_plan_create(mc_plan** out, int32_t num_instances, const int32
 
 mc_status mc_plan_destroy(mc_plan* p) {
   if (!p) return MC_OK;
-  for (void* q : p->allocs) (void)hipFree(q);
-  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
-  delete p;
+  // work using the plan may still be queued on any stream
+  (void)hipDeviceSynchronize();
+  release_plan(p);
+  return MC_OK;
+}
+
+mc_status mc_plan_destroy_after(mc_plan* p, void* stream) {
+  g_err.clear();
+  if (!p) return MC_OK;
+  // everything that used the plan was enqueued on `stream`: wait for that
+  // stream only, then hand the memory back to the pool
+  hipEvent_t ev;
+  HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  hipError_t e = hipEventRecord(ev, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ev);
+  (void)hipEventDestroy(ev);
+  if (e != hipSuccess) return fail(MC_ERR_HIP, "waiting for the plan's stream: %s", hipGetErrorString(e));
+  release_plan(p);
   return MC_OK;
 }
 
@@ -850,10 +962,24 @@ mc_status mc_step(mc_plan* p, const mc_params* q, const mc_state* in, const mc_s
   return enqueue_step(p, q, c, in, out, ed, nullptr, s);
 }
 
-mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const mc_edge_params* ed,
-                     const mc_actuation* act, double dt, int64_t num_steps, float* rec_pos,
-                     float* rec_vel, int32_t use_graph, void* stream) {
-  g_err.clear();
+}  // extern "C"
+
+namespace {
+
+// Drops the cached rollout graph after its last launch has finished.
+mc_status drop_graph(mc_plan* p) {
+  if (!p->graph_exec) return MC_OK;
+  if (p->graph_done) HIPCHK(hipEventSynchronize(p->graph_done));
+  HIPCHK(hipGraphExecDestroy(p->graph_exec));
+  p->graph_exec = nullptr;
+  return MC_OK;
+}
+
+// mc_rollout with optional per-instance actuation offsets t_inst (device).
+mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const mc_edge_params* ed,
+                       const mc_actuation* act, const long long* t_inst, double dt,
+                       int64_t num_steps, float* rec_pos, float* rec_vel, int32_t use_graph,
+                       void* stream) {
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   MCCHK(check_params(p, q, true, true));
   MCCHK(check_state(p, st, "state"));
@@ -871,11 +997,9 @@ mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const m
   Actuation A{};
   const Actuation* ap = nullptr;
   if (act) {
-    A = Actuation{act->amp, act->omega, act->phase, p->tick};
+    A = Actuation{act->amp, act->omega, act->phase, p->tick, t_inst};
     ap = &A;
-    long long t0 = (long long)act->t0;
-    HIPCHK(hipMemcpyAsync(p->tick, &t0, sizeof(t0), hipMemcpyHostToDevice, us));
-    HIPCHK(hipStreamSynchronize(us));  // t0 lives on the host stack
+    HIPCHK(launch_set_tick(p->tick, (long long)act->t0, us));
   }
   // ping-pong: even steps read state->f_grid, odd steps read f_tmp
   mc_state sa = *st, sb = *st;
@@ -898,54 +1022,60 @@ mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const m
       MCCHK(record(k, us));
     }
   } else {
-    if (!p->cap_stream) HIPCHK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
-    hipStream_t cs = p->cap_stream;
     // one graph holds S steps (S even keeps the lattice parity); inside it
     // the LBM pass of step k runs inside step k+1's rod launch, and the last
-    // step's LBM pass runs on its own
+    // step's LBM pass runs on its own.  The instantiated graph is kept and
+    // replayed by later calls with the same arguments; it is captured on a
+    // private stream and launched on the caller's.
     const int64_t S = num_steps >= 8 ? 8 : (num_steps >= 2 ? 2 : 0);
-    std::vector<hipEvent_t> evs;
-    auto mkev = [&](hipEvent_t* e) -> mc_status {
-      HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
-      evs.push_back(*e);
-      return MC_OK;
-    };
-    hipEvent_t ev_in;
-    MCCHK(mkev(&ev_in));
-    HIPCHK(hipEventRecord(ev_in, us));
-    HIPCHK(hipStreamWaitEvent(cs, ev_in, 0));
-    hipGraphExec_t ge = nullptr;
-    mc_status r = MC_OK;
     if (S > 0) {
-      hipGraph_t g;
-      HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-      LbmArgs pend[2];
-      const LbmArgs* fuse = nullptr;
-      for (int64_t k = 0; k < S && r == MC_OK; ++k) {
-        const int par = (int)(k & 1);
-        LbmArgs* defer = (q->enable_fluid && p->rod_fuse && k + 1 < S) ? &pend[par] : nullptr;
-        r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, StepSync(), fuse,
-                         defer);
-        fuse = defer;
+      GraphKey key;
+      memset(&key, 0, sizeof(key));
+      key.q = *q;
+      key.dt = dt;
+      key.sa = sa;
+      key.sb = sb;
+      key.ed = *ed;
+      if (ap) key.act = A;
+      key.has_act = ap != nullptr;
+      key.steps = S;
+      key.track = p->track;
+      if (p->graph_exec && memcmp(&key, &p->graph_key, sizeof(key)) != 0) MCCHK(drop_graph(p));
+      if (!p->graph_exec) {
+        if (!p->cap_stream) HIPCHK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        if (!p->graph_done) HIPCHK(hipEventCreateWithFlags(&p->graph_done, hipEventDisableTiming));
+        hipStream_t cs = p->cap_stream;
+        hipGraph_t g;
+        mc_status r = MC_OK;
+        HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        LbmArgs pend[2];
+        const LbmArgs* fuse = nullptr;
+        for (int64_t k = 0; k < S && r == MC_OK; ++k) {
+          const int par = (int)(k & 1);
+          LbmArgs* defer = (q->enable_fluid && p->rod_fuse && k + 1 < S) ? &pend[par] : nullptr;
+          r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, StepSync(), fuse,
+                           defer);
+          fuse = defer;
+        }
+        hipError_t ec = hipStreamEndCapture(cs, &g);
+        if (r != MC_OK) return r;
+        if (ec != hipSuccess) return fail(MC_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
+        hipGraphExec_t ge = nullptr;
+        ec = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ec != hipSuccess) return fail(MC_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ec));
+        p->graph_exec = ge;
+        p->graph_key = key;
       }
-      hipError_t ec = hipStreamEndCapture(cs, &g);
-      if (r != MC_OK) return r;
-      if (ec != hipSuccess) return fail(MC_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
-      HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-      HIPCHK(hipGraphDestroy(g));
-      for (int64_t k = 0; k + S <= num_steps; k += S) HIPCHK(hipGraphLaunch(ge, cs));
+      for (int64_t k = 0; k + S <= num_steps; k += S) HIPCHK(hipGraphLaunch(p->graph_exec, us));
+      HIPCHK(hipEventRecord(p->graph_done, us));
     }
-    // remaining steps (fewer than S) without a graph, on the capture stream
+    // remaining steps (fewer than S) without a graph
     const int64_t done = S > 0 ? (num_steps / S) * S : 0;
     for (int64_t k = done; k < num_steps; ++k) {
       const int par = (int)(k & 1);
-      MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs));
+      MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, us));
     }
-    HIPCHK(hipEventRecord(ev_in, cs));
-    HIPCHK(hipStreamWaitEvent(us, ev_in, 0));
-    HIPCHK(hipStreamSynchronize(cs));
-    if (ge) HIPCHK(hipGraphExecDestroy(ge));
-    for (auto e : evs) (void)hipEventDestroy(e);
   }
   if (num_steps & 1) {
     // final lattice is in f_tmp; bring it back into the caller's buffer
@@ -953,6 +1083,18 @@ mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const m
       HIPCHK(hipMemcpyAsync(st->f_grid, p->f_tmp, 9 * hwf * sizeof(float), hipMemcpyDeviceToDevice, us));
   }
   return MC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mc_status mc_rollout(mc_plan* p, const mc_params* q, const mc_state* st, const mc_edge_params* ed,
+                     const mc_actuation* act, double dt, int64_t num_steps, float* rec_pos,
+                     float* rec_vel, int32_t use_graph, void* stream) {
+  g_err.clear();
+  return rollout_impl(p, q, st, ed, act, nullptr, dt, num_steps, rec_pos, rec_vel, use_graph,
+                      stream);
 }
 
 mc_status mc_profile_step(mc_plan* p, const mc_params* q, const mc_state* st,
@@ -972,7 +1114,7 @@ mc_status mc_profile_step(mc_plan* p, const mc_params* q, const mc_state* st,
   Actuation A{};
   const Actuation* ap = nullptr;
   if (act) {
-    A = Actuation{act->amp, act->omega, act->phase, p->tick};
+    A = Actuation{act->amp, act->omega, act->phase, p->tick, nullptr};
     ap = &A;
     long long t0 = (long long)act->t0;
     HIPCHK(hipMemcpyAsync(p->tick, &t0, sizeof(t0), hipMemcpyHostToDevice, s));
@@ -1145,46 +1287,95 @@ mc_status mc_track_read(mc_plan* p, float* cumulative_out, float* com_out, void*
   return MC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// One tracker group per instance (mc_batch_rollout / mc_batch_advance),
+// allocated on first use.
+mc_status ensure_inst_track(mc_plan* p) {
+  if (p->inst_track.n_groups != 0) return MC_OK;
+  for (int i = 0; i < p->n_inst; ++i)
+    if (p->inst[i].nodes == 0) return fail(MC_ERR_BAD_SHAPE, "instance %d has no nodes to track", i);
+  std::vector<int> off(1, 0);
+  std::vector<int2> dims;
+  for (const InstHost& h : p->inst) {
+    off.push_back(h.node0 + h.nodes);
+    dims.push_back(make_int2(h.H, h.W));
+  }
+  TrackWork t{};
+  t.n_groups = p->n_inst;
+  t.start_step = 0;
+  MCCHK(upload(p, &t.off, off, "instance offsets"));
+  MCCHK(upload(p, &t.dims, dims, "instance dims"));
+  MCCHK(p->alloc(&t.prev, 2 * (size_t)p->n_inst, "instance prev"));
+  MCCHK(p->alloc(&t.cum, 2 * (size_t)p->n_inst, "instance cumulative"));
+  MCCHK(p->alloc(&t.count, (size_t)p->n_inst, "instance count"));
+  p->inst_track = t;
+  return MC_OK;
+}
+
+// Rollout of every instance with the per-instance tracker swapped in.
+// mode 0 starts the displacement from zero, mode 2 from `disp_in` (device).
+mc_status tracked_rollout(mc_plan* p, const mc_params* q, const mc_state* st,
+                          const mc_edge_params* ed, const mc_actuation* act,
+                          const long long* t_inst, double dt, int64_t num_steps,
+                          const float* disp_in, float* disp_out, hipStream_t s) {
+  if (disp_out) {
+    if (!st || (p->N > 0 && !st->position)) return fail(MC_ERR_INVALID, "state is NULL");
+    MCCHK(ensure_inst_track(p));
+    if (disp_in)
+      HIPCHK(hipMemcpyAsync(p->inst_track.cum, disp_in, 2 * (size_t)p->n_inst * sizeof(float),
+                            hipMemcpyDeviceToDevice, s));
+    HIPCHK(launch_track(p->inst_track, st->position, disp_in ? 2 : 0, s));
+  }
+  const TrackWork keep = p->track;
+  if (disp_out) p->track = p->inst_track;
+  const mc_status r = rollout_impl(p, q, st, ed, act, t_inst, dt, num_steps, nullptr, nullptr, 1, s);
+  p->track = keep;
+  if (r != MC_OK) return r;
+  if (disp_out)
+    HIPCHK(hipMemcpyAsync(disp_out, p->inst_track.cum, 2 * (size_t)p->n_inst * sizeof(float),
+                          hipMemcpyDeviceToDevice, s));
+  return MC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 mc_status mc_batch_rollout(mc_plan* p, const mc_params* q, const mc_state* st,
                            const mc_edge_params* ed, const mc_actuation* act, double dt,
                            int64_t num_steps, float* displacement_out, void* stream) {
   g_err.clear();
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
-  if (displacement_out) {
-    if (!st || (p->N > 0 && !st->position)) return fail(MC_ERR_INVALID, "state is NULL");
-    for (int i = 0; i < p->n_inst; ++i)
-      if (p->inst[i].nodes == 0)
-        return fail(MC_ERR_BAD_SHAPE, "instance %d has no nodes to track", i);
-    if (p->inst_track.n_groups == 0) {
-      // one tracker group per instance, allocated on first use
-      std::vector<int> off(1, 0);
-      std::vector<int2> dims;
-      for (const InstHost& h : p->inst) {
-        off.push_back(h.node0 + h.nodes);
-        dims.push_back(make_int2(h.H, h.W));
-      }
-      TrackWork t{};
-      t.n_groups = p->n_inst;
-      t.start_step = 0;
-      MCCHK(upload(p, &t.off, off, "instance offsets"));
-      MCCHK(upload(p, &t.dims, dims, "instance dims"));
-      MCCHK(p->alloc(&t.prev, 2 * (size_t)p->n_inst, "instance prev"));
-      MCCHK(p->alloc(&t.cum, 2 * (size_t)p->n_inst, "instance cumulative"));
-      MCCHK(p->alloc(&t.count, (size_t)p->n_inst, "instance count"));
-      p->inst_track = t;
+  return tracked_rollout(p, q, st, ed, act, nullptr, dt, num_steps, nullptr, displacement_out,
+                         (hipStream_t)stream);
+}
+
+mc_status mc_batch_advance(mc_plan* p, const mc_params* q, const mc_state* st,
+                           const mc_edge_params* ed, const mc_actuation* act,
+                           const int64_t* instance_t0, double dt, int64_t num_steps,
+                           float* displacement_io, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  const long long* t_inst = nullptr;
+  if (act && instance_t0) {
+    if (!p->t_inst) MCCHK(p->alloc(&p->t_inst, (size_t)p->n_inst, "instance time offsets"));
+    bool same = p->t_inst_h.size() == (size_t)p->n_inst;
+    for (int i = 0; same && i < p->n_inst; ++i) same = p->t_inst_h[i] == (long long)instance_t0[i];
+    if (!same) {
+      // earlier launches on this stream may still read the old offsets
+      HIPCHK(hipStreamSynchronize(s));
+      p->t_inst_h.assign(instance_t0, instance_t0 + p->n_inst);
+      HIPCHK(hipMemcpy(p->t_inst, p->t_inst_h.data(), (size_t)p->n_inst * sizeof(long long),
+                       hipMemcpyHostToDevice));
     }
-    HIPCHK(launch_track(p->inst_track, st->position, 0, (hipStream_t)stream));
+    t_inst = p->t_inst;
   }
-  const TrackWork keep = p->track;
-  if (displacement_out) p->track = p->inst_track;
-  const mc_status r = mc_rollout(p, q, st, ed, act, dt, num_steps, nullptr, nullptr, 1, stream);
-  p->track = keep;
-  if (r != MC_OK) return r;
-  if (displacement_out)
-    HIPCHK(hipMemcpyAsync(displacement_out, p->inst_track.cum,
-                          2 * (size_t)p->n_inst * sizeof(float), hipMemcpyDeviceToDevice,
-                          (hipStream_t)stream));
-  return MC_OK;
+  return tracked_rollout(p, q, st, ed, act, t_inst, dt, num_steps, displacement_io,
+                         displacement_io, s);
 }
 
 mc_status mc_plan_instances(const mc_plan* p, int32_t* num_instances, int64_t* offsets) {

@@ -115,6 +115,7 @@ struct CompMeta {
 struct Actuation {           // rest_b(t) = amp_b * sin(omega_b * t + phase_b)
   const float *amp, *omega, *phase;
   const long long* t;        // device step counter (advanced by launch_tick)
+  const long long* t_inst;   // per-instance offset added to *t (NULL: none)
 };
 
 // Arguments of one LBM pass (fused collide-stream of every instance).
@@ -175,6 +176,7 @@ hipError_t rod_prepare(int lds_max);
 hipError_t launch_steric_nodes(const Consts& c, const BatchDev& b, int N, const float* steric,
                                const float* pred_w, float* force_out, hipStream_t s);
 hipError_t launch_tick(long long* t, hipStream_t s);
+hipError_t launch_set_tick(long long* t, long long value, hipStream_t s);
 size_t rod_scratch_floats(int nn, int ne, int nb);
 size_t rod_lds_bytes(int nn, int ne, int nb, int nni, int nei);
 // immersed boundary (solved per interaction cluster)
@@ -222,7 +224,8 @@ struct TrackWork {
 };
 hipError_t launch_com(int n_groups, const int* off, const float* pos, int H, int W, int periodic,
                       float* com, hipStream_t s);
-// mode 0: reset (prev = centre, cumulative = 0); mode 1: one step's update
+// mode 0: reset (prev = centre, cumulative = 0); mode 1: one step's update;
+// mode 2: resume (prev = centre, step count 0, cumulative kept)
 hipError_t launch_track(const TrackWork& t, const float* pos, int mode, hipStream_t s);
 // LBM
 LbmArgs make_lbm_args(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,

@@ -139,7 +139,8 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
                  ? *reinterpret_cast<const float2*>(a.pred_w + 2 * (size_t)j)
                  : make_float2(0.0f, 0.0f);
   }
-  const double tact = a.has_act ? (double)(*a.act.t) : 0.0;
+  const double tact =
+      a.has_act ? (double)(*a.act.t + (a.act.t_inst ? a.act.t_inst[m.inst] : 0LL)) : 0.0;
   int g = -1;
   float px = 0.0f, py = 0.0f, vx = 0.0f, vy = 0.0f, yx = 0.0f, yy = 0.0f;
   if (tid < nn) {
@@ -440,6 +441,7 @@ __global__ void k_steric_nodes(Consts c, BatchDev bd, int N, const float* __rest
 }
 
 __global__ void k_tick(long long* t) { *t = *t + 1; }
+__global__ void k_set_tick(long long* t, long long v) { *t = v; }
 
 hipError_t launch_steric_nodes(const Consts& c, const BatchDev& b, int N, const float* steric,
                                const float* pred_w, float* force_out, hipStream_t s) {
@@ -451,6 +453,11 @@ hipError_t launch_steric_nodes(const Consts& c, const BatchDev& b, int N, const 
 
 hipError_t launch_tick(long long* t, hipStream_t s) {
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_tick(long long* t, long long value, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_tick, dim3(1), dim3(1), 0, s, t, value);
   return hipGetLastError();
 }
 
@@ -488,7 +495,7 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
   a.rest_ang = rest_ang;
   a.kb = kb;
   a.has_act = act != nullptr;
-  if (act) a.act = *act; else a.act = Actuation{nullptr, nullptr, nullptr, nullptr};
+  if (act) a.act = *act; else a.act = Actuation{nullptr, nullptr, nullptr, nullptr, nullptr};
   a.steric = steric;
   a.pred_w = pred_w;
   a.pos_out = pos_out;

@@ -71,7 +71,8 @@ __global__ void __launch_bounds__(COM_THREADS) k_com(const float* __restrict__ p
 }
 
 // swim_base.py:179-209: reset (mode 0) sets prev = centre, cumulative = 0 and
-// the group's step count to 0; update (mode 1) advances the count and, past
+// the group's step count to 0; resume (mode 2) does the same but keeps
+// cumulative; update (mode 1) advances the count and, past
 // start_step, adds the minimum-image step displacement of the centre to
 // cumulative; prev = centre.
 __global__ void __launch_bounds__(COM_THREADS) k_track(const float* __restrict__ pos,
@@ -87,9 +88,11 @@ __global__ void __launch_bounds__(COM_THREADS) k_track(const float* __restrict__
   float cx, cy;
   group_com(pos, off[g], off[g + 1], W, H, 1, sx, sy, cx, cy);
   if (threadIdx.x != 0) return;
-  if (mode == 0) {
-    cum[2 * g] = 0.0f;
-    cum[2 * g + 1] = 0.0f;
+  if (mode == 0 || mode == 2) {
+    if (mode == 0) {
+      cum[2 * g] = 0.0f;
+      cum[2 * g + 1] = 0.0f;
+    }
     count[g] = 0;
   } else {
     const long long k = count[g] + 1;  // this step's number since the reset
