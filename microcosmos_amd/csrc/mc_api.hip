@@ -4,6 +4,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -881,6 +882,25 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
     return bail(fail(MC_ERR_HIP, "rod kernels: %d B of dynamic LDS refused", p->lds_max - 1024));
   if (steric_prepare(p->lds_max) != hipSuccess)
     return bail(fail(MC_ERR_HIP, "steric kernels: %d B of dynamic LDS refused", p->lds_max - 1024));
+  // the wide solve exists when an instance can hold a cluster of more than
+  // IB_WIDE_MIN nodes
+  w.wide_wg = 0;
+  for (const InstHost& h : p->inst)
+    if (h.nodes > IB_WIDE_MIN) w.wide_wg = IB_WIDE_WG;
+  if (w.wide_wg > 0) {
+    // MC_IB_WIDE_WG (1..IB_WIDE_WG) overrides the workgroup count, for tuning
+    const char* env = getenv("MC_IB_WIDE_WG");
+    const int v = env ? atoi(env) : 0;
+    if (v >= 1 && v <= IB_WIDE_WG) w.wide_wg = v;
+  }
+  if (w.wide_wg > 0) {
+    unsigned char* ws = nullptr;
+    if ((st = p->alloc(&ws, ib_wide_scratch_bytes(N, max_kernel_size), "wide scratch")) != MC_OK)
+      return bail(st);
+    w.wide_scratch = ws;
+    if ((st = p->alloc(&w.wide_count, hwf, "wide cell counters")) != MC_OK) return bail(st);
+    if ((st = p->alloc(&w.wide_ctl, 1, "wide control")) != MC_OK) return bail(st);
+  }
   if ((st = p->alloc(&w.force, 2 * hwf, "force U")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.mask, (size_t)p->mask_words, "U mask")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->f_tmp, 9 * hwf, "f ping-pong")) != MC_OK) return bail(st);

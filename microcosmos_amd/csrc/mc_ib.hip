@@ -72,6 +72,26 @@ __device__ __forceinline__ void uf_union(int* p, int a, int b) {
   }
 }
 
+// Wavefront-aggregated atomic add: one device-scope atomic per wavefront
+// instead of one per lane (a single counter hit by every lane of a large
+// launch serialises).  Every lane of the wavefront must call it; returns this
+// lane's offset, lanes in lane order.
+__device__ __forceinline__ int wave_fetch_add(int* ctr, int v) {
+  const int lane = threadIdx.x & 63;
+  int inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += u;
+  }
+  const int total = __shfl(inc, 63, 64);
+  int base = 0;
+  if (lane == 63 && total > 0)
+    base = __hip_atomic_fetch_add(ctr, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  base = __shfl(base, 63, 64);
+  return base + inc - v;
+}
+
 // Lanes per node in the stencil kernel: one lane per tap, the node's group a
 // power of two within a wavefront.
 template <int K>
@@ -136,14 +156,15 @@ __global__ void __launch_bounds__(256) k_ib_stencil(
   for (int k = 0; k < K2; ++k) total = total + __shfl(prod, base + k, 64);
   const unsigned long long won_lanes = __ballot(tap && prev == -1);
   const int prev_left = __shfl(prev, t > 0 ? lane - 1 : lane, 64);
+  // a cell claimed by another component links the two (a repeat of the left
+  // neighbour tap's claimant is dropped; at most one link per tap)
+  const bool emit = tap && prev != -1 && prev != comp && (t == 0 || prev != prev_left);
+  const int li = wave_fetch_add(nlinks, emit ? 1 : 0);
   if (!tap) return;
   const float den = total + MC_EPSIL;
   st_w[(size_t)n * K2 + t] = prod / den;
   if (t == 0) st_won[n] = (won_lanes >> base) & (K2 == 64 ? ~0ull : ((1ull << K2) - 1ull));
-  // a cell claimed by another component links the two (a repeat of the left
-  // neighbour tap's claimant is dropped; at most one link per tap)
-  if (prev != -1 && prev != comp && (t == 0 || prev != prev_left))
-    links[atomicAdd(nlinks, 1)] = make_int2(prev, comp);
+  if (emit) links[li] = make_int2(prev, comp);
   if (prev == -1) {
     const size_t HW = (size_t)gi->Hf * gi->Wf;
     const float* fin = f_all + 9 * gi->fl_off;
@@ -166,7 +187,8 @@ __global__ void __launch_bounds__(1024) k_ib_cluster_build(
     int C, int lds_ints, const int* __restrict__ comp_node_off, const int2* __restrict__ links,
     int* __restrict__ nlinks, int* __restrict__ gparent, int* __restrict__ label,
     int* __restrict__ cl_cnt, int* __restrict__ cl_node_off, int* __restrict__ cl_mem_cnt,
-    int* __restrict__ cl_mem_off, int* __restrict__ cl_members, long long* __restrict__ tick) {
+    int* __restrict__ cl_mem_off, int* __restrict__ cl_members, long long* __restrict__ tick,
+    WideCtl* __restrict__ wctl) {
   extern __shared__ __align__(16) int sm[];
   __shared__ int wsum[16];
   __shared__ int wsum2[16];
@@ -176,6 +198,13 @@ __global__ void __launch_bounds__(1024) k_ib_cluster_build(
   int* cnt = lds ? sm + C : cl_cnt;
   int* mcnt = lds ? sm + 2 * C : cl_mem_cnt;
   if (tick && tid == 0) *tick = *tick + 1;
+  if (wctl && tid == 0) {
+    // atomic stores: the counters are later updated by device-scope atomics
+    __hip_atomic_store(&wctl->bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&wctl->wide, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&wctl->nseg, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&wctl->nq, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   for (int c = tid; c < C; c += bs) {
     if (lds) par[c] = c;
     cnt[c] = 0;
@@ -208,6 +237,7 @@ __global__ void __launch_bounds__(1024) k_ib_cluster_build(
     int tot, tot2;
     const int inc = block_inclusive_scan(v, wsum, &tot);
     const int inc2 = block_inclusive_scan(v2, wsum2, &tot2);
+    if (wctl && root && v > IB_WIDE_MIN) atomicAdd(&wctl->wide, 1);
     if (c < C) {
       cl_node_off[c] = carry + inc - v;
       cl_mem_off[c] = carry2 + inc2 - v2;
@@ -246,6 +276,12 @@ struct ClusterArgs {
   unsigned char* gscratch;
   size_t bpn;        // global scratch bytes per node
   size_t lds_bytes;  // dynamic LDS of the launch (a multiple of 16)
+  // wide solve
+  int N, C, wide_wg;
+  const int* comp_of_node;
+  unsigned char* wscratch;
+  int* wcount;  // per lattice cell entry counter (0 between calls)
+  WideCtl* wctl;
 };
 
 // Arrays of one cluster solve.  e = r * K2 + t indexes tap t of the node of
@@ -778,6 +814,380 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
   }
 }
 
+// ---- wide solve ----------------------------------------------------------
+// Clusters of more than IB_WIDE_MIN nodes are solved together, as one
+// problem, by the last wide_wg workgroups of the cluster launch.  Clusters
+// share no cell, so merging them changes no fold: a cell's sum is still over
+// its covering nodes in ascending node id and a node's gather over its taps
+// in tap order.  Entries are identified by e = n * K2 + t with the global
+// node id n, so ascending e is ascending node id within a cell.
+//
+// Phases (grid-strided loops, separated by a grid barrier):
+//   1. per node of a wide cluster: list it; per tap, an arrival index from a
+//      per-cell counter; won taps claim a segment (the cell's rho / u);
+//   2. per node: every tap's segment; a grid scan of the segments' entry
+//      counts gives their entry ranges;
+//   3. per node: entries placed in arrival order; the first gather
+//      (fluid.py:292-305) -> correction;
+//   4. per entry: its rank among its segment's entries (a count of smaller
+//      entry ids, all loads independent) -> the segment's entries ascending;
+//   5. per segment: kernel-weight fold, 1/max(kw, 1) and the first spread;
+//      then per iteration a node gather and a segment spread;
+//   6. per segment: post-clamp and force; per node: drag and wrap.
+//
+// The grid barrier: after a workgroup barrier, one thread per workgroup
+// issues an agent-scope release fence, adds to the barrier counter, waits
+// for all wide_wg arrivals and issues an agent-scope acquire fence; a second
+// workgroup barrier releases the other threads.  The workgroups are
+// co-resident: there are at most 64 of them, each needing one CU, the device
+// has 256 CUs, and no workgroup of the launch waits on anything else.  The
+// wait is bounded nevertheless: after ~2^22 polls it records the timeout in
+// WideCtl::err and proceeds, so the kernel always ends.
+
+// Scratch of the wide solve for N nodes and ne = N * K2 entries.
+struct WLayout {
+  int *list, *arr, *tapseg, *eplace, *esort, *scell, *sstart;
+  float *srho, *sinv;
+  float2 *su0, *suc, *corr;
+};
+
+__device__ __forceinline__ void wlayout(unsigned char* base, int N, int ne, WLayout& L) {
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    unsigned char* p = base + off;
+    off += al16(b);
+    return p;
+  };
+  L.list = (int*)take(4 * (size_t)N);
+  L.corr = (float2*)take(8 * (size_t)N);
+  L.arr = (int*)take(4 * (size_t)ne);
+  L.tapseg = (int*)take(4 * (size_t)ne);
+  L.eplace = (int*)take(4 * (size_t)ne);
+  L.esort = (int*)take(4 * (size_t)ne);
+  L.scell = (int*)take(4 * (size_t)ne);
+  L.sstart = (int*)take(4 * ((size_t)ne + 1));
+  L.srho = (float*)take(4 * (size_t)ne);
+  L.sinv = (float*)take(4 * (size_t)ne);
+  L.su0 = (float2*)take(8 * (size_t)ne);
+  L.suc = (float2*)take(8 * (size_t)ne);
+}
+
+__device__ __forceinline__ void grid_sync(WideCtl* ctl, unsigned int target) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(&ctl->bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int polls = 0;
+    while (__hip_atomic_load(&ctl->bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++polls > (1 << 22)) {
+        __hip_atomic_store(&ctl->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int ld_agent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive scan of val(i), i in [0, m), across the wide workgroups (each
+// scans a contiguous chunk, then adds the partial sums of the chunks before
+// it); out may alias the storage val reads (each index is read by the thread
+// that writes it).  Returns the total to every thread.  Ends with a barrier.
+template <class F>
+__device__ __forceinline__ int grid_scan(int m, F val, int* out, WideCtl* ctl, int h, int H,
+                                         unsigned int& target, int* wsum, int* s_off) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int lo = (int)(((long long)m * h) / H), hi = (int)(((long long)m * (h + 1)) / H);
+  int carry = 0;
+  for (int b0 = lo; b0 < hi; b0 += bs) {
+    const int i = b0 + tid;
+    const int v = i < hi ? val(i) : 0;
+    int tot;
+    const int inc = block_inclusive_scan(v, wsum, &tot);
+    if (i < hi) out[i] = carry + inc - v;
+    carry += tot;
+  }
+  if (tid == 0) __hip_atomic_store(&ctl->bsum[h], carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  target += H;
+  grid_sync(ctl, target);
+  if (tid == 0) {
+    int off = 0, total = 0;
+    for (int j = 0; j < H; ++j) {
+      const int v = ld_agent(&ctl->bsum[j]);
+      if (j < h) off += v;
+      total += v;
+    }
+    s_off[0] = off;
+    s_off[1] = total;
+  }
+  __syncthreads();
+  const int off = s_off[0], total = s_off[1];
+  if (off != 0)
+    for (int i = lo + tid; i < hi; i += bs) out[i] += off;
+  target += H;
+  grid_sync(ctl, target);
+  return total;
+}
+
+template <int K>
+__device__ __forceinline__ long long wide_cell(const InstGeo& gi, int sx, int sy, int t) {
+  return gi.fl_off + (long long)wrap1(sy + t % K, gi.Hf) * gi.Wf + wrap1(sx + t / K, gi.Wf);
+}
+
+// Spread of one segment: the fold over its entries (ascending entry id) of
+// corr[node] * weight (fluid.py:109-120), then u += sum * inv.
+template <int K2>
+__device__ __forceinline__ void wide_spread(const int* __restrict__ es, int lo, int hi,
+                                            const float* __restrict__ w,
+                                            const float2* __restrict__ corr, float iv,
+                                            float2& u) {
+  float ax = 0.0f, ay = 0.0f;
+  int j = lo;
+  for (; j + 4 <= hi; j += 4) {
+    int e[4];
+    float wv[4];
+    float2 cv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) e[k] = es[j + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      wv[k] = w[e[k]];
+      cv[k] = corr[e[k] / K2];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ax = ax + cv[k].x * wv[k];
+      ay = ay + cv[k].y * wv[k];
+    }
+  }
+  for (; j < hi; ++j) {
+    const int e = es[j];
+    const float wv = w[e];
+    const float2 cc = corr[e / K2];
+    ax = ax + cc.x * wv;
+    ay = ay + cc.y * wv;
+  }
+  u = make_float2(u.x + ax * iv, u.y + ay * iv);
+}
+
+// Gather and correction of one node (fluid.py:90-97, 295-305).
+template <int K2>
+__device__ __forceinline__ float2 wide_corr(const Consts& c, const float* __restrict__ w,
+                                            const int* __restrict__ seg,
+                                            const float2* __restrict__ suc, float2 vb,
+                                            float umax) {
+  float2 u[K2];
+#pragma unroll
+  for (int t = 0; t < K2; ++t) u[t] = suc[seg[t]];
+  float ux = 0.0f, uy = 0.0f;
+#pragma unroll
+  for (int t = 0; t < K2; ++t) {
+    ux = ux + u[t].x * w[t];
+    uy = uy + u[t].y * w[t];
+  }
+  const float cx = (vb.x - ux) * c.relax;
+  const float cy = (vb.y - uy) * c.relax;
+  const float mag = sqrtf((cx * cx + cy * cy) + MC_EPSIL);
+  const float cf = fmin_nan(1.0f, umax / (mag + MC_EPSIL));
+  return make_float2(cx * cf, cy * cf);
+}
+
+template <int K>
+__device__ void cluster_wide(const ClusterArgs& a, int h, int H, int* wsum, int* s_off) {
+  constexpr int K2 = K * K;
+  const Consts& c = a.c;
+  WideCtl* ctl = a.wctl;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int gt = h * bs + tid, gs = H * bs;
+  unsigned int target = 0;
+  auto gsync = [&]() {
+    target += H;
+    grid_sync(ctl, target);
+  };
+  WLayout L;
+  wlayout(a.wscratch, a.N, a.N * K2, L);
+  const float* wts = a.st_w;  // entry e = n * K2 + t has weight st_w[e]
+  // 1. list the wide clusters' nodes; arrival index of every tap at its
+  // cell; won taps claim segments (arbitrary numbering) and return their
+  // cells to idle
+  // (the wavefront walks the nodes together: list slots and segment ids come
+  // from wavefront-aggregated counters)
+  for (int nb = h * bs + (tid & ~63); nb < a.N; nb += gs) {
+    const int n = nb + (tid & 63);
+    const bool wide = n < a.N && a.cl_cnt[a.label[a.comp_of_node[n]]] > IB_WIDE_MIN;
+    const unsigned long long won = wide ? a.st_won[n] : 0ull;
+    const int slot = wave_fetch_add(&ctl->nq, wide ? 1 : 0);
+    int seg = wave_fetch_add(&ctl->nseg, __popcll(won));
+    if (!wide) continue;
+    L.list[slot] = n;
+    const InstGeo& gi = a.bd.geo[a.bd.node_inst[n]];
+    const int sx = a.st_sx[n], sy = a.st_sy[n];
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      const long long cell = wide_cell<K>(gi, sx, sy, t);
+      L.arr[n * K2 + t] = __hip_atomic_fetch_add(&a.wcount[cell], 1, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+      if ((won >> t) & 1ull) {
+        a.cell_seg[cell] = seg;
+        L.scell[seg] = (int)cell;  // global cell index
+        a.owner[cell] = -1;
+        const float2 u = a.st_u[(size_t)n * K2 + t];
+        L.srho[seg] = a.st_rho[(size_t)n * K2 + t];
+        L.su0[seg] = u;
+        L.suc[seg] = u;
+        ++seg;
+      }
+    }
+  }
+  gsync();
+  const int nq = ld_agent(&ctl->nq);
+  const int ns = ld_agent(&ctl->nseg);
+  // 2. every tap's segment; entry ranges of the segments
+  for (int r = gt; r < nq; r += gs) {
+    const int n = L.list[r];
+    const InstGeo& gi = a.bd.geo[a.bd.node_inst[n]];
+    const int sx = a.st_sx[n], sy = a.st_sy[n];
+    int seg[K2];
+#pragma unroll
+    for (int t = 0; t < K2; ++t) seg[t] = a.cell_seg[wide_cell<K>(gi, sx, sy, t)];
+#pragma unroll
+    for (int t = 0; t < K2; ++t) L.tapseg[n * K2 + t] = seg[t];
+  }
+  const int* wc = a.wcount;
+  const int* sc = L.scell;
+  const int tot = grid_scan(ns, [&](int s) { return ld_agent(&wc[sc[s]]); }, L.sstart, ctl, h, H,
+                            target, wsum, s_off);
+  if (gt == 0) L.sstart[ns] = tot;
+  // 3. entries in arrival order; the cell counters return to zero; first
+  // gather and correction
+  for (int r = gt; r < nq; r += gs) {
+    const int n = L.list[r];
+    const InstGeo& gi = a.bd.geo[a.bd.node_inst[n]];
+    const unsigned long long won = a.st_won[n];
+    const int sx = a.st_sx[n], sy = a.st_sy[n];
+    int seg[K2];
+    float w[K2];
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      seg[t] = L.tapseg[n * K2 + t];
+      w[t] = wts[n * K2 + t];
+    }
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      L.eplace[L.sstart[seg[t]] + L.arr[n * K2 + t]] = n * K2 + t;
+      if ((won >> t) & 1ull) a.wcount[wide_cell<K>(gi, sx, sy, t)] = 0;
+    }
+    if (c.iters > 0)
+      L.corr[n] = wide_corr<K2>(c, w, seg, L.suc, make_float2(a.vib[2 * n], a.vib[2 * n + 1]), gi.umax);
+  }
+  gsync();
+  // 4. each entry's rank among its segment's entries -> ascending entry ids
+  for (int q = gt; q < nq * K2; q += gs) {
+    const int r = q / K2, t = q - r * K2;
+    const int e = L.list[r] * K2 + t;
+    const int sg = L.tapseg[e];
+    const int lo = L.sstart[sg], hi = L.sstart[sg + 1];
+    int rank = 0;
+    for (int j = lo; j < hi; ++j) rank += L.eplace[j] < e ? 1 : 0;
+    L.esort[lo + rank] = e;
+  }
+  gsync();
+  // 5. kernel-weight fold and 1/max(kw, 1) (fluid.py:123-133,289-290), the
+  // first spread; then the remaining iterations (fluid.py:292-316)
+  for (int s = gt; s < ns; s += gs) {
+    const int lo = L.sstart[s], hi = L.sstart[s + 1];
+    float kw = 0.0f;
+    for (int j = lo; j < hi; ++j) kw = kw + wts[L.esort[j]];
+    const float iv = 1.0f / fmax_nan(kw, 1.0f);
+    L.sinv[s] = iv;
+    if (c.iters > 0) {
+      float2 u = L.suc[s];
+      wide_spread<K2>(L.esort, lo, hi, wts, L.corr, iv, u);
+      L.suc[s] = u;
+    }
+  }
+  gsync();
+  for (int it = 1; it < c.iters; ++it) {
+    for (int r = gt; r < nq; r += gs) {
+      const int n = L.list[r];
+      int seg[K2];
+      float w[K2];
+#pragma unroll
+      for (int t = 0; t < K2; ++t) {
+        seg[t] = L.tapseg[n * K2 + t];
+        w[t] = wts[n * K2 + t];
+      }
+      L.corr[n] = wide_corr<K2>(c, w, seg, L.suc, make_float2(a.vib[2 * n], a.vib[2 * n + 1]),
+                                a.bd.geo[a.bd.node_inst[n]].umax);
+    }
+    gsync();
+    for (int s = gt; s < ns; s += gs) {
+      float2 u = L.suc[s];
+      wide_spread<K2>(L.esort, L.sstart[s], L.sstart[s + 1], wts, L.corr, L.sinv[s], u);
+      L.suc[s] = u;
+    }
+    gsync();
+  }
+  // 6. post-clamp and force (fluid.py:327-339); F replaces u_corrected
+  for (int s = gt; s < ns; s += gs) {
+    // the segment's instance: that of any covering node (its first entry)
+    const InstGeo& gi = a.bd.geo[a.bd.node_inst[L.esort[L.sstart[s]] / K2]];
+    const float umax = gi.umax;
+    const size_t HW = (size_t)gi.Hf * gi.Wf;
+    const int cell = (int)(L.scell[s] - gi.fl_off);
+    const float rho = L.srho[s];
+    const float2 u0 = L.su0[s];
+    float ucx = L.suc[s].x, ucy = L.suc[s].y;
+    const float mag = sqrtf((ucx * ucx + ucy * ucy) + MC_EPSIL);
+    const float cl = fmin_nan(1.0f, umax / (mag + MC_EPSIL));
+    ucx = ucx * cl;
+    ucy = ucy * cl;
+    float fx = (ucx - u0.x) * rho;
+    float fy = (ucy - u0.y) * rho;
+    if (c.force_clamp) {
+      const float fm = sqrtf((fx * fx + fy * fy) + MC_EPSIL);
+      const float fc = fmin_nan(1.0f, umax / (fm + MC_EPSIL));
+      fx = fx * fc;
+      fy = fy * fc;
+    }
+    L.suc[s] = make_float2(fx, fy);
+    float* frc = a.force + 2 * gi.fl_off;
+    frc[cell] = fx;
+    frc[HW + cell] = fy;
+    atomicOr(&a.mask[gi.mask_off + (cell >> 5)], 1u << (cell & 31));
+  }
+  gsync();
+  // node drag (fluid.py:364-382) and the final wrap (simulate.py:97)
+  for (int r = gt; r < nq; r += gs) {
+    const int n = L.list[r];
+    const InstGeo& gi = a.bd.geo[a.bd.node_inst[n]];
+    float2 F[K2];
+    float w[K2];
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      F[t] = L.suc[L.tapseg[n * K2 + t]];
+      w[t] = wts[n * K2 + t];
+    }
+    float fx = 0.0f, fy = 0.0f;
+#pragma unroll
+    for (int t = 0; t < K2; ++t) {
+      fx = fx + F[t].x * w[t];
+      fy = fy + F[t].y * w[t];
+    }
+    const float dvx = (fx / c.node_mass) / (gi.scale_x * c.dt);
+    const float dvy = (fy / c.node_mass) / (gi.scale_y * c.dt);
+    a.vel_io[2 * n] = a.vel_io[2 * n] - dvx;
+    a.vel_io[2 * n + 1] = a.vel_io[2 * n + 1] - dvy;
+    a.pos_out[2 * n] = fmodf_floor(a.pos[2 * n], gi.dom_wf);
+    a.pos_out[2 * n + 1] = fmodf_floor(a.pos[2 * n + 1], gi.dom_hf);
+  }
+}
+
 // Every thread of a cluster is in this workgroup: the workgroup barrier (with
 // its workgroup-scope fence) orders both the LDS and the global-memory phases.
 template <int K>
@@ -785,11 +1195,19 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int wsum[16];
   __shared__ int s_ns, s_seg;
+  __shared__ int s_off[2];
   constexpr int K2 = K * K;
   const int root = blockIdx.x;
+  if (root >= a.C) {
+    // one of the wide solve's workgroups
+    if (__hip_atomic_load(&a.wctl->wide, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    cluster_wide<K>(a, root - a.C, a.wide_wg, wsum, &s_off[0]);
+    return;
+  }
   if (a.label[root] != root) return;
   const InstGeo& gi = a.bd.geo[a.bd.comp_inst[root]];  // a cluster never spans instances
   const int nq = a.cl_cnt[root];
+  if (a.wide_wg > 0 && nq > IB_WIDE_MIN) return;  // solved by the wide workgroups
   const int ne = nq * K2;
   const int tid = threadIdx.x, bs = blockDim.x;
   CLayout G;
@@ -909,6 +1327,12 @@ size_t ib_scratch_bytes_per_node(int kmax) {
   return 28 + 40 * K2 + 4 + 16 * 16;
 }
 
+size_t ib_wide_scratch_bytes(int N, int kmax) {
+  // wlayout(N, N * K2): 12 B per node, 48 B per entry (+1 int), 12 arrays of padding
+  const size_t K2 = (size_t)kmax * kmax;
+  return 12 * (size_t)N + 48 * K2 * (size_t)N + 4 + 12 * 16;
+}
+
 size_t ib_lds_bytes(int lds_max) {
   // dynamic LDS of the cluster kernels: the device maximum less a reserve for
   // their static LDS (scan partials, the workgroup reduction's 256 B), a
@@ -970,7 +1394,7 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   hipLaunchKernelGGL(k_ib_cluster_build, dim3(1), dim3(1024), w.lds_bytes, s, C,
                      (int)(w.lds_bytes / sizeof(int)), w.comp_node_off, w.links, w.nlinks,
                      w.uf_parent, w.label, w.cl_cnt, w.cl_node_off, w.cl_mem_cnt, w.cl_mem_off,
-                     w.cl_members, tick);
+                     w.cl_members, tick, w.wide_wg > 0 ? w.wide_ctl : nullptr);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   ClusterArgs a;
@@ -1003,8 +1427,16 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   a.gscratch = w.cl_scratch;
   a.bpn = w.cl_bpn;
   a.lds_bytes = w.lds_bytes;
-#define L_CLUSTER(KK) \
-  hipLaunchKernelGGL(k_ib_cluster<KK>, dim3(C), dim3(cluster_threads<KK>()), w.lds_bytes, s, a)
+  a.N = N;
+  a.C = C;
+  a.wide_wg = w.wide_wg;
+  a.comp_of_node = w.comp_of_node;
+  a.wscratch = w.wide_scratch;
+  a.wcount = w.wide_count;
+  a.wctl = w.wide_ctl;
+#define L_CLUSTER(KK)                                                                           \
+  hipLaunchKernelGGL(k_ib_cluster<KK>, dim3(C + w.wide_wg), dim3(cluster_threads<KK>()), w.lds_bytes, \
+                     s, a)
   MC_KSWITCH(L_CLUSTER);
 #undef L_CLUSTER
   e = hipGetLastError();

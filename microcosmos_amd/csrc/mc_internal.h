@@ -10,6 +10,8 @@ namespace mc {
 constexpr int IB_KMAX = 6;                 // largest supported ibm_kernel_size
 constexpr int SORT_LDS_CAP = 2048;         // bitonic sort capacity in LDS (ids)
 constexpr int FFT_MAX_RADICES = 16;        // radix stages of one FFT length
+constexpr int IB_WIDE_MIN = 1024;          // clusters with more nodes take the wide solve
+constexpr int IB_WIDE_WG = 64;             // workgroups of the wide solve
 
 // Float constants derived from ConstraintSolverConfig exactly as the reference
 // derives them (Python double arithmetic, then a float32 weak scalar).  Shared
@@ -202,7 +204,23 @@ struct IBWork {
   long long cells;             // lattice cells over all instances
   float *force;                // (2, Hf, Wf) per instance: IB force, valid on the support U
   uint32_t* mask;              // U bits by flat lattice cell index, per instance
+  // wide clusters (more than IB_WIDE_MIN nodes) are solved together by
+  // wide_wg extra workgroups of the cluster launch (0: no instance is large
+  // enough to form one)
+  int wide_wg;
+  unsigned char* wide_scratch; // ib_wide_scratch_bytes(N, kmax)
+  int* wide_count;             // per lattice cell entry counter (0 between calls)
+  struct WideCtl* wide_ctl;
 };
+// Control block of the wide solve: grid-barrier counter, wide-cluster count,
+// segment and node counters, a flag set if a barrier wait ever timed out, and the
+// per-workgroup partial sums of the grid scans.
+struct WideCtl {
+  unsigned int bar;
+  int wide, nseg, nq, err;
+  int bsum[IB_WIDE_WG];
+};
+size_t ib_wide_scratch_bytes(int N, int kmax);
 size_t ib_scratch_bytes_per_node(int kmax);
 size_t ib_lds_bytes(int lds_max);
 hipError_t ib_prepare(int lds_max);

@@ -174,6 +174,33 @@ def test_ib_lbm_crowded_cluster(dev):
     _ib_case(dev, W.small(grid=(64, 64), n_fil=24, n_nodes=40, seed=3), 4)
 
 
+@pytest.mark.parametrize("K", [4, 6])
+def test_ib_lbm_wide_cluster(dev, K):
+    # 1920 nodes on 64^2 chain into one cluster of more than 1024 nodes: it
+    # is solved by the wide workgroups with grid barriers
+    _say(f"IB+LBM single call, one 1920-node cluster (wide solve), K={K}")
+    _ib_case(dev, W.small(grid=(64, 64), n_fil=48, n_nodes=40, seed=3), K)
+
+
+def test_step_parity_wide_cluster(dev):
+    """Ten coupled steps of a crowded 64^2 case whose filaments form one wide
+    cluster: bit-identical to the float32 restatement."""
+    _say("10 coupled steps, 64^2 with 48 x 40 nodes (wide cluster)")
+    wl = W.small(grid=(64, 64), n_fil=48, n_nodes=40, seed=3)
+    p = oracle_params(wl)
+    o = run_oracle(wl, 10, p)
+    from microcosmos_amd import Rollout
+    nodes, edges, fields = device_state(wl, dev)
+    ro = Rollout(nodes, edges, fields, device_config(wl), wl.dt,
+                 actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
+    ro.run(10, use_graph=True)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(_np(ro.position), o.position)
+    np.testing.assert_array_equal(_np(ro.f_grid), o.f_grid)
+    np.testing.assert_array_equal(_np(ro.fluid_velocity), o.fluid_velocity)
+    np.testing.assert_array_equal(_np(ro.velocity), o.velocity)
+
+
 def test_ib_lbm_singletons(dev):
     # filaments of 1 and 2 nodes (components without edges or hinges)
     _say("IB+LBM single call, singleton components")
