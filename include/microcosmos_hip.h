@@ -194,6 +194,33 @@ mc_status mc_batch_advance(mc_plan* plan, const mc_params* params, const mc_stat
                            const int64_t* instance_t0, double dt, int64_t num_steps,
                            float* displacement_io, void* stream);
 
+/* Differentiation of the rod-only step (PBD constraint, enable_fluid = 0,
+ * enable_steric = 0): the configuration the reference differentiates with
+ * jax.value_and_grad through lax.scan of simulate.step
+ * (experiments/filament_folding/experiment.py:985-1018).
+ *
+ * mc_rod_adjoint_forward runs num_steps steps in place on `state` (as
+ * mc_rollout without actuation: the rest angles edges->bending_rest_angles
+ * are the parameters, constant over the rollout) and keeps a checkpoint of
+ * (position, velocity, theta) every checkpoint_every steps in the plan.
+ *
+ * mc_rod_adjoint_backward then takes position_bar = dL/d position after the
+ * last step (N, 2) and velocity_bar = dL/d velocity (may be NULL: zero), and
+ * writes rest_angle_bar (B) = dL/d bending_rest_angles, plus (each may be
+ * NULL) the gradient with respect to the initial position, velocity and
+ * theta.  Between checkpoints the steps are recomputed; each step's reverse
+ * pass is one hand-written adjoint kernel (one workgroup per filament).
+ * params, dt and the edge arrays must be those of the recorded rollout.
+ * Minimum-image, angle and periodic wraps are differentiated as identity. */
+mc_status mc_rod_adjoint_forward(mc_plan* plan, const mc_params* params, const mc_state* state,
+                                 const mc_edge_params* edges, double dt, int64_t num_steps,
+                                 int32_t checkpoint_every, void* stream);
+mc_status mc_rod_adjoint_backward(mc_plan* plan, const mc_params* params,
+                                  const mc_edge_params* edges, double dt,
+                                  const float* position_bar, const float* velocity_bar,
+                                  float* rest_angle_bar, float* position0_bar,
+                                  float* velocity0_bar, float* theta0_bar, void* stream);
+
 /* Measurement: num_steps coupled steps in place (no graph, the LBM
  * collide-stream as its own launch) with HIP events carried by every LBM
  * dispatch and recorded after every step on `stream`; returns the average

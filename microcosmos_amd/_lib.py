@@ -85,6 +85,12 @@ SIGNATURES = {
     "mc_batch_advance": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
                                    POINTER(McEdgeParams), POINTER(McActuation), c_void_p,
                                    c_double, c_int64, c_void_p, c_void_p]),
+    "mc_rod_adjoint_forward": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
+                                         POINTER(McEdgeParams), c_double, c_int64, c_int32,
+                                         c_void_p]),
+    "mc_rod_adjoint_backward": (c_int32, [c_void_p, POINTER(McParams), POINTER(McEdgeParams),
+                                          c_double, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p]),
     "mc_center_of_mass": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32,
                                     c_void_p, c_void_p]),
     "mc_track_enable": (c_int32, [c_void_p, c_int32, c_void_p, c_int32]),

@@ -172,6 +172,20 @@ struct mc_plan {
   // per-instance actuation time offsets (mc_batch_advance), host mirror
   long long* t_inst = nullptr;
   std::vector<long long> t_inst_h;
+  // rod adjoint: largest component, checkpoints of the last recorded
+  // rollout, the segment tape and the adjoint / working state
+  int comp_max_nn = 0, comp_max_ne = 0, comp_max_nb = 0;
+  float* adj_ckpt = nullptr;
+  size_t adj_ckpt_cap = 0;
+  float* adj_tape = nullptr;
+  size_t adj_tape_cap = 0;
+  float* adj_work = nullptr;
+  bool adj_ok = false;
+  int64_t adj_steps = 0;
+  int adj_every = 0;
+  double adj_dt = 0.0;
+  mc_params adj_q;
+  mc_edge_params adj_ed;
   TrackWork track{};            // displacement tracking (off: n_groups == 0)
   TrackWork inst_track{};       // per-instance tracker of mc_batch_rollout
 
@@ -326,6 +340,9 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
   // LDS budget: the largest component
   size_t maxb = 0;
   for (int c = 0; c < C; ++c) {
+    p->comp_max_nn = std::max(p->comp_max_nn, noff[c + 1] - noff[c]);
+    p->comp_max_ne = std::max(p->comp_max_ne, eoff[c + 1] - eoff[c]);
+    p->comp_max_nb = std::max(p->comp_max_nb, hoff[c + 1] - hoff[c]);
     const int nni = ninc_off[noff[c + 1]] - ninc_off[noff[c]];
     const int nei = einc_off[eoff[c + 1]] - einc_off[eoff[c]];
     const size_t b = rod_lds_bytes(noff[c + 1] - noff[c], eoff[c + 1] - eoff[c],
@@ -1396,6 +1413,166 @@ mc_status mc_batch_advance(mc_plan* p, const mc_params* q, const mc_state* st,
   }
   return tracked_rollout(p, q, st, ed, act, t_inst, dt, num_steps, displacement_io,
                          displacement_io, s);
+}
+
+}  // extern "C"
+
+namespace {
+
+mc_status check_rod_adjoint(const mc_params* q) {
+  if (!q) return fail(MC_ERR_INVALID, "params is NULL");
+  if (q->solver_type != MC_SOLVER_PBD || q->enable_fluid || q->enable_steric)
+    return fail(MC_ERR_UNSUPPORTED,
+                "the rod adjoint covers the PBD step with enable_fluid = enable_steric = 0");
+  return MC_OK;
+}
+
+// Copies position, velocity, theta of `from` into the packed slot `to`
+// (2N + 2N + E floats), or back when `unpack`.
+mc_status pack_state(const mc_plan* p, float* slot, float* pos, float* vel, float* th, bool unpack,
+                     hipStream_t s) {
+  const size_t n2 = 2 * (size_t)p->N * sizeof(float), ne = (size_t)p->E * sizeof(float);
+  auto cp = [&](float* dst, float* src, size_t b) -> mc_status {
+    if (b) HIPCHK(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, s));
+    return MC_OK;
+  };
+  float* sp = slot;
+  float* sv = slot + 2 * (size_t)p->N;
+  float* st = slot + 4 * (size_t)p->N;
+  if (unpack) {
+    MCCHK(cp(pos, sp, n2));
+    MCCHK(cp(vel, sv, n2));
+    MCCHK(cp(th, st, ne));
+  } else {
+    MCCHK(cp(sp, pos, n2));
+    MCCHK(cp(sv, vel, n2));
+    MCCHK(cp(st, th, ne));
+  }
+  return MC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state* st,
+                                 const mc_edge_params* ed, double dt, int64_t num_steps,
+                                 int32_t checkpoint_every, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  MCCHK(check_rod_adjoint(q));
+  MCCHK(check_params(p, q, false, false));
+  MCCHK(check_state(p, st, "state"));
+  MCCHK(check_edges(p, ed));
+  if (num_steps < 1) return fail(MC_ERR_BAD_SHAPE, "num_steps < 1");
+  if (checkpoint_every < 1) return fail(MC_ERR_BAD_SHAPE, "checkpoint_every < 1");
+  if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  p->adj_ok = false;
+  const size_t S = 4 * (size_t)p->N + (size_t)p->E;
+  const int64_t nck = (num_steps + checkpoint_every - 1) / checkpoint_every;
+  if (p->adj_ckpt_cap < (size_t)nck * S) {
+    // earlier work on the plan may still read the old buffer: it stays
+    // allocated until the plan is destroyed
+    MCCHK(p->alloc(&p->adj_ckpt, (size_t)nck * S, "adjoint checkpoints"));
+    p->adj_ckpt_cap = (size_t)nck * S;
+  }
+  const Consts c = derive(p, q, dt);
+  MCCHK(ensure_geo(p, q));
+  mc_state io = *st;
+  for (int64_t t = 0; t < num_steps; ++t) {
+    if (t % checkpoint_every == 0)
+      MCCHK(pack_state(p, p->adj_ckpt + (size_t)(t / checkpoint_every) * S, io.position,
+                       io.velocity, io.theta, false, s));
+    MCCHK(enqueue_step(p, q, c, &io, &io, ed, nullptr, s));
+  }
+  p->adj_ok = true;
+  p->adj_steps = num_steps;
+  p->adj_every = checkpoint_every;
+  p->adj_dt = dt;
+  p->adj_q = *q;
+  p->adj_ed = *ed;
+  return MC_OK;
+}
+
+mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params* ed,
+                                  double dt, const float* position_bar, const float* velocity_bar,
+                                  float* rest_angle_bar, float* position0_bar,
+                                  float* velocity0_bar, float* theta0_bar, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  MCCHK(check_rod_adjoint(q));
+  if (!p->adj_ok) return fail(MC_ERR_INVALID, "no rollout recorded by mc_rod_adjoint_forward");
+  if (!ed || memcmp(q, &p->adj_q, sizeof(mc_params)) != 0 || dt != p->adj_dt ||
+      memcmp(ed, &p->adj_ed, sizeof(mc_edge_params)) != 0)
+    return fail(MC_ERR_INVALID, "params, dt and edge arrays must be those of the recorded rollout");
+  if (!position_bar || (p->B > 0 && !rest_angle_bar))
+    return fail(MC_ERR_INVALID, "position_bar / rest_angle_bar is NULL");
+  const size_t lds = rod_adjoint_lds_bytes(q->cycles_per_step, p->comp_max_nn, p->comp_max_ne,
+                                           p->comp_max_nb);
+  if (lds + 1024 > (size_t)p->lds_max)
+    return fail(MC_ERR_UNSUPPORTED, "a filament of %d nodes with %d cycles needs %zu B of LDS for "
+                "the adjoint (device: %d B)", p->comp_max_nn, q->cycles_per_step, lds, p->lds_max);
+  hipStream_t s = (hipStream_t)stream;
+  const int N = p->N, E = p->E, B = p->B;
+  const size_t S = 4 * (size_t)N + (size_t)E;
+  const int every = p->adj_every;
+  const int64_t T = p->adj_steps;
+  if (p->adj_tape_cap < (size_t)every * S) {
+    MCCHK(p->alloc(&p->adj_tape, (size_t)every * S, "adjoint tape"));
+    p->adj_tape_cap = (size_t)every * S;
+  }
+  if (!p->adj_work) MCCHK(p->alloc(&p->adj_work, 2 * S + 2 * (size_t)N, "adjoint state"));
+  float* xb = p->adj_work;
+  float* vb = xb + 2 * (size_t)N;
+  float* thb = vb + 2 * (size_t)N;
+  float* wpos = p->adj_work + S;
+  float* wvel = wpos + 2 * (size_t)N;
+  float* wth = wvel + 2 * (size_t)N;
+  float* wdbg = wth + E;
+  const size_t n2 = 2 * (size_t)N * sizeof(float);
+  HIPCHK(hipMemcpyAsync(xb, position_bar, n2, hipMemcpyDeviceToDevice, s));
+  if (velocity_bar) HIPCHK(hipMemcpyAsync(vb, velocity_bar, n2, hipMemcpyDeviceToDevice, s));
+  else HIPCHK(hipMemsetAsync(vb, 0, n2, s));
+  if (E) HIPCHK(hipMemsetAsync(thb, 0, (size_t)E * sizeof(float), s));
+  if (B) HIPCHK(hipMemsetAsync(rest_angle_bar, 0, (size_t)B * sizeof(float), s));
+  const Consts c = derive(p, q, dt);
+  MCCHK(ensure_geo(p, q));
+  // working state for the recomputation (lattice arrays unused: fluid off)
+  mc_state w{wpos, wvel, wdbg, wth, p->f_tmp, p->f_tmp, p->f_tmp};
+  RodAdjArgs a{};
+  a.c = c;
+  a.bd = p->bd;
+  a.L = p->L;
+  a.rest_len = ed->rest_lengths;
+  a.rest_ang = ed->bending_rest_angles;
+  a.kb = ed->bending_stiffness;
+  a.xb = xb;
+  a.vb = vb;
+  a.thb = thb;
+  a.rest_bar = rest_angle_bar;
+  const int64_t nck = (T + every - 1) / every;
+  for (int64_t seg = nck - 1; seg >= 0; --seg) {
+    const int64_t t0 = seg * every;
+    const int len = (int)std::min<int64_t>(every, T - t0);
+    MCCHK(pack_state(p, p->adj_ckpt + (size_t)seg * S, wpos, wvel, wth, true, s));
+    for (int k = 0; k < len; ++k) {
+      MCCHK(pack_state(p, p->adj_tape + (size_t)k * S, wpos, wvel, wth, false, s));
+      if (k + 1 < len) MCCHK(enqueue_step(p, q, c, &w, &w, ed, nullptr, s));
+    }
+    for (int k = len - 1; k >= 0; --k) {
+      float* slot = p->adj_tape + (size_t)k * S;
+      a.pos = slot;
+      a.vel = slot + 2 * (size_t)N;
+      a.theta = slot + 4 * (size_t)N;
+      HIPCHK(launch_rod_adjoint(a, lds, s));
+    }
+  }
+  if (position0_bar) HIPCHK(hipMemcpyAsync(position0_bar, xb, n2, hipMemcpyDeviceToDevice, s));
+  if (velocity0_bar) HIPCHK(hipMemcpyAsync(velocity0_bar, vb, n2, hipMemcpyDeviceToDevice, s));
+  if (theta0_bar && E)
+    HIPCHK(hipMemcpyAsync(theta0_bar, thb, (size_t)E * sizeof(float), hipMemcpyDeviceToDevice, s));
+  return MC_OK;
 }
 
 mc_status mc_plan_instances(const mc_plan* p, int32_t* num_instances, int64_t* offsets) {

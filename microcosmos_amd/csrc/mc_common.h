@@ -78,6 +78,11 @@ __device__ __forceinline__ float min_image(float a, float b, float s) {
   return fmodf_floor(a - b + half, s) - half;
 }
 
+// pbd.py:38 angle wrap (d + pi) mod 2 pi - pi, floor-mod.
+__device__ __forceinline__ float wrap_angle(float d) {
+  return fmodf_floor(d + MC_PI_F, MC_2PI_F) - MC_PI_F;
+}
+
 // D2Q9 lattice (fluid.py:15-23).
 __constant__ constexpr int EX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 __constant__ constexpr int EY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};

@@ -69,9 +69,11 @@ __device__ __forceinline__ void lbm_cell(const LbmArgs& a, int b, int t) {
   const int yp = (y + 1 == H) ? 0 : y + 1;
   const int ym = (y == 0) ? H - 1 : y - 1;
   const size_t cell = (size_t)y * W + x;
+  // the populations are read once: non-temporal loads keep them from
+  // displacing the lines the pushes write into
   float f[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) f[i] = fmax_nan(fin[i * HW + cell], MC_EPSIL);
+  for (int i = 0; i < 9; ++i) f[i] = fmax_nan(__builtin_nontemporal_load(&fin[i * HW + cell]), MC_EPSIL);
   // U bits are indexed by the flat cell index: a 32-cell word lies inside one
   // wavefront, which reads it before its first lane clears it
   uint32_t* mw = mask + (ci >> 5);

@@ -88,10 +88,6 @@ __device__ __forceinline__ void steric_force(const Consts& c, const InstGeo& gi,
 
 constexpr int ROD_WIN_MAX = 8;  // steric windows up to this half-width are staged in LDS
 
-__device__ __forceinline__ float wrap_angle(float d) {
-  return fmodf_floor(d + MC_PI_F, MC_2PI_F) - MC_PI_F;
-}
-
 __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float* smem) {
   const Consts& c = a.c;
   const RodLayoutDev& L = a.L;

@@ -1,0 +1,137 @@
+"""Gradients of rod rollouts through hand-written adjoint kernels
+(mc_rod_adjoint_forward / mc_rod_adjoint_backward).
+
+The reference differentiates its rollouts with ``jax.value_and_grad``
+through ``lax.scan`` of ``simulate.step``
+(experiments/filament_folding/experiment.py:985-1018), storing every step.
+Here the forward rollout keeps a checkpoint every ``checkpoint_every`` steps
+and the reverse pass recomputes each segment before running the per-step
+adjoint kernel backwards through it, so the device memory of a gradient is
+(checkpoints + one segment) x state instead of the whole trajectory.
+
+Scope: the rod-only step (PBD constraint, ``enable_fluid=False``,
+``enable_steric=False``) with the bending rest angles as the parameters.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import replace
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import McState, check
+from ._runtime import Plan, as_f32, edge_params, params_from_config, ptr, stream_handle
+from .structs import Edges, Fields, Nodes
+
+
+def hinge_center_nodes(pairs, bending_pairs) -> np.ndarray:
+    """Node at the centre of every hinge: the target node of its incoming edge
+    (examples/01_es_locomotion.ipynb ``bending_center_nodes``)."""
+    pa = np.asarray(pairs).reshape(-1, 2)
+    bp = np.asarray(bending_pairs).reshape(-1, 2)
+    return pa[bp[:, 0], 1].astype(np.int64)
+
+
+def node_gradient(rest_angle_grad: torch.Tensor, centers, num_nodes: int) -> torch.Tensor:
+    """Gradient with respect to per-node rest curvatures kappa_n when hinge b
+    uses kappa at its centre node (rest_b = kappa[center_b])."""
+    idx = torch.as_tensor(np.asarray(centers), device=rest_angle_grad.device)
+    out = torch.zeros(num_nodes, dtype=rest_angle_grad.dtype, device=rest_angle_grad.device)
+    return out.index_add_(0, idx, rest_angle_grad)
+
+
+def min_image(grid_shape, a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """utils.py:18-23 minimum-image displacement a - b on the (H, W) torus."""
+    h, w = grid_shape
+    s = torch.tensor([float(w), float(h)], dtype=a.dtype, device=a.device)
+    return torch.remainder(a - b + s / 2, s) - s / 2
+
+
+class RodAdjoint:
+    """A rollout of the rod-only step that can be differentiated.
+
+    ``forward(num_steps)`` advances a private copy of the state and records
+    checkpoints; ``backward(position_bar)`` returns d loss / d rest angles
+    (and the gradient with respect to the initial state) for a loss of the
+    final positions (and velocities).
+    """
+
+    def __init__(self, nodes: Nodes, edges: Edges, fields: Fields, solver_config, dt: float):
+        if solver_config.enable_fluid or getattr(solver_config, "enable_steric", False) or \
+                getattr(solver_config, "solver_type", "pbd") != "pbd":
+            raise NotImplementedError("rod adjoint: PBD constraint with enable_fluid=False and "
+                                      "enable_steric=False")
+        d = nodes.position.device
+        if d.type != "cuda":
+            raise RuntimeError("microcosmos_amd runs on the GPU only")
+        self.dev = d
+        self.params = params_from_config(solver_config)
+        self.dt = float(dt)
+        self.position = as_f32(nodes.position, d).clone()
+        self.velocity = as_f32(nodes.velocity, d).clone()
+        self.debug_vector = as_f32(nodes.debug_vector, d).clone()
+        self.theta = as_f32(edges.theta, d).clone()
+        self.grid_shape = tuple(fields.grid_shape)
+        self.steric = as_f32(fields.steric, d)
+        self.fluid_velocity = as_f32(fields.fluid_velocity, d)
+        self.f_grid = as_f32(fields.f_grid, d)
+        from ._runtime import _host_topology
+        pa = _host_topology(edges.pairs)[0]
+        bp = _host_topology(edges.bending_pairs)[0]
+        # a private plan: it holds this rollout's checkpoints
+        self.plan = Plan(self.grid_shape, tuple(self.f_grid.shape[1:]), self.position.shape[0],
+                         pa, bp)
+        self.ep, self._keep = edge_params(edges, d)
+        self.num_edges = pa.shape[0]
+        self.num_hinges = bp.shape[0]
+        self.steps = 0
+
+    def _state(self) -> McState:
+        return McState(ptr(self.position), ptr(self.velocity), ptr(self.debug_vector),
+                       ptr(self.theta), ptr(self.steric), ptr(self.fluid_velocity),
+                       ptr(self.f_grid))
+
+    def forward(self, num_steps: int, checkpoint_every: int = 50) -> torch.Tensor:
+        lib = _lib.load()
+        st = self._state()
+        check(lib.mc_rod_adjoint_forward(self.plan.handle, ctypes.byref(self.params),
+                                         ctypes.byref(st), ctypes.byref(self.ep), self.dt,
+                                         int(num_steps), int(checkpoint_every),
+                                         stream_handle(self.dev)))
+        self.steps = int(num_steps)
+        return self.position
+
+    def backward(self, position_bar: torch.Tensor, velocity_bar: torch.Tensor = None):
+        """Returns (rest_angle_grad (B,), position0_grad (N, 2),
+        velocity0_grad (N, 2), theta0_grad (E,))."""
+        lib = _lib.load()
+        d = self.dev
+        pb = as_f32(position_bar, d)
+        vb = as_f32(velocity_bar, d) if velocity_bar is not None else None
+        n = self.position.shape[0]
+        rg = torch.empty(self.num_hinges, dtype=torch.float32, device=d)
+        g_x = torch.empty((n, 2), dtype=torch.float32, device=d)
+        g_v = torch.empty((n, 2), dtype=torch.float32, device=d)
+        g_t = torch.empty(self.num_edges, dtype=torch.float32, device=d)
+        check(lib.mc_rod_adjoint_backward(self.plan.handle, ctypes.byref(self.params),
+                                          ctypes.byref(self.ep), self.dt, ptr(pb), ptr(vb),
+                                          ptr(rg), ptr(g_x), ptr(g_v), ptr(g_t),
+                                          stream_handle(d)))
+        return rg, g_x, g_v, g_t
+
+
+def rollout_value_and_grad(nodes: Nodes, edges: Edges, fields: Fields, solver_config, dt: float,
+                           num_steps: int, target: torch.Tensor, checkpoint_every: int = 50):
+    """Folding loss after ``num_steps`` steps -- the mean over nodes of the
+    squared minimum-image distance to ``target`` (N, 2) -- and its gradient
+    with respect to the (time-constant) bending rest angles."""
+    ad = RodAdjoint(nodes, edges, fields, solver_config, dt)
+    x = ad.forward(num_steps, checkpoint_every)
+    tgt = as_f32(target, ad.dev)
+    dvec = min_image(ad.grid_shape, x, tgt)
+    n = x.shape[0]
+    loss = (dvec * dvec).sum() / n
+    g = ad.backward(2.0 * dvec / n)[0]
+    return loss, g

@@ -233,3 +233,33 @@ def serving_trace(n=48, seed=4, rate=200.0, sides=(128, 256, 512, 1024), max_fil
             overrides=_lbm_overrides(0.8, 4))
         out.append(ServeItem(arrival=t, workload=wl, steps=steps, gradient=grad))
     return out
+
+
+def folding(seed=3, grid=(256, 256), n_fil=8, n_nodes=48, steps=500, modes=4, amplitude=5.0):
+    """Differentiable folding: ``n_fil`` straight filaments of ``n_nodes``
+    nodes (uniform start points and headings) on a ``grid`` node domain, the
+    rod-only step with the reference's PBD_SCHEME_NO_FLUID parameters and the
+    steric term off, and a seeded smooth target curve per filament: the
+    straight rest shape displaced along its normal by a sum of ``modes``
+    Fourier modes sin(pi k s + phi_k), k = 1..modes, s in [0, 1], whose
+    absolute coefficients sum to ``amplitude`` lattice units.
+    Returns (workload, target (N, 2) float32)."""
+    wl = filaments_workload(f"folding-{grid[0]}x{grid[1]}-{n_fil}x{n_nodes}", grid, n_fil,
+                            n_nodes, seed, jitter=0.0, amp=0.0, steps=steps,
+                            overrides=dict(enable_steric=False))
+    rng = np.random.default_rng(seed + 1000)
+    h, w = grid
+    tgt = []
+    for f in range(n_fil):
+        p = wl.position[f * n_nodes:(f + 1) * n_nodes].astype(np.float64)
+        d = _min_image(grid, p[-1:], p[:1])[0].astype(np.float64)
+        t = d / max(np.linalg.norm(d), 1e-12)
+        nrm = np.array([-t[1], t[0]])
+        coef = rng.normal(size=modes)
+        coef *= amplitude / np.abs(coef).sum()
+        phi = rng.uniform(0, 2 * np.pi, size=modes)
+        s = np.arange(n_nodes) / max(n_nodes - 1, 1)
+        off = sum(coef[k] * np.sin(np.pi * (k + 1) * s + phi[k]) for k in range(modes))
+        q = p + off[:, None] * nrm[None, :]
+        tgt.append(np.mod(q, np.array([w, h], np.float64)))
+    return wl, np.concatenate(tgt).astype(np.float32)

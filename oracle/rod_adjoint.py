@@ -1,0 +1,166 @@
+"""Reverse-mode derivative of the rod-only coupled step (CPU restatement).
+
+TEST INFRASTRUCTURE ONLY, like ``step_ref``: imported by ``tests/`` as the
+checker of the device adjoint, never by the product path.
+
+Scope: ``step`` (simulate.py:37-100) with ``enable_fluid = False``,
+``enable_steric = False`` and the PBD constraint (pbd.py:14-91), the
+configuration of the reference's differentiable folding runs without the
+steric term (experiments/filament_folding/experiment.py:52-53, 985-1018 take
+``jax.value_and_grad`` through ``lax.scan`` of this step).  The derivative is
+written out by hand from the forward equations of ``step_ref``:
+
+forward, one step (x, v, theta; rest angles rho):
+    p0 = x + (v * d) * dt                                    simulate.py:46-50
+    cycle c:  theta pass (pbd.py:23-49)
+                  diff_b = wrap(theta[e_out] - theta[e_in])
+                  corr_b = (diff_b - rho_b) * kb_b * 0.5
+                  theta' = theta + omega / deg_e * (sum_in corr - sum_out corr)
+              position pass (pbd.py:55-89), per edge (s, g):
+                  r = min_image(p[g] - p[s]),  L = |r|,  tv = l0 (cos th', sin th')
+                  F = k_st/2 * ((1 - k_sh) * r (L - l0) / L + k_sh * (r - tv))
+                  p'[n] = p[n] + (sum_{s=n} F - sum_{g=n} F) / deg_n
+    x_new = wrap(p_C),  v_new = (p_C - x) / dt                 simulate.py:83-98
+
+reverse (adjoints xb, vb, thb of x_new, v_new, theta_new):
+    pb = xb + vb / dt;  xb_direct = -vb / dt
+    cycle c (last to first):
+      Fb_e = pb[s] / deg_s - pb[g] / deg_g
+      rb_e = J_e Fb_e,  J = k_st/2 ((1 - k_sh)(I - l0/L (I - rhat rhat^T)) + k_sh I)
+      pb[g] += rb_e;  pb[s] -= rb_e
+      thb'_e += -k_st/2 k_sh l0 (-sin th' Fb_x + cos th' Fb_y)
+      corrb_b = omega (thb'[e_in] / deg_in - thb'[e_out] / deg_out)
+      rhob_b -= corrb_b kb_b / 2;  diffb_b = corrb_b kb_b / 2
+      thb[e_out] += diffb_b;  thb[e_in] -= diffb_b
+    xb_prev = pb + xb_direct;  vb_prev = pb * d * dt
+
+The minimum image, the angle wrap and the final periodic wrap have derivative
+1 almost everywhere; their jumps are not differentiated (as in the reference's
+``jax.grad``, whose ``%`` derivative is 1).
+
+Pinned by central finite differences of the float64 forward (``step_ref.step``)
+in ``tests/test_rod_adjoint_oracle.py``.
+"""
+from __future__ import annotations
+
+from dataclasses import replace
+
+import numpy as np
+
+from . import step_ref as R
+
+OMEGA = 1.8  # pbd.py:43
+
+
+def _degrees(s: R.State):
+    src, tgt = s.pairs[:, 0], s.pairs[:, 1]
+    n = s.position.shape[0]
+    deg_n = np.zeros(n)
+    np.add.at(deg_n, src, 1.0)
+    np.add.at(deg_n, tgt, 1.0)
+    deg_n = np.maximum(deg_n, 1.0)
+    deg_e = np.asarray(R._edge_bending_degree(s.theta.shape[0], s.bending_pairs, np.float64))
+    return deg_n, deg_e
+
+
+def _check(p: R.Params):
+    if p.enable_fluid or p.enable_steric or p.solver_type != "pbd":
+        raise NotImplementedError("the rod adjoint covers the PBD step without fluid and steric")
+
+
+def step_tape(s: R.State, dt: float, p: R.Params):
+    """Forward step that also returns the per-cycle positions (before each
+    position pass) and angles (after each angle pass)."""
+    _check(p)
+    s = s.copy()
+    prev = s.position
+    s.velocity = s.velocity * p.damping
+    s.position = prev + s.velocity * dt
+    ps, ths = [], []
+    for _ in range(p.cycles_per_step):
+        t = s.copy()
+        deg_e = R._edge_bending_degree(t.theta.shape[0], t.bending_pairs, t.theta.dtype)
+        delta = R._theta_bending_delta(t.theta, t.bending_pairs, t.bending_rest_angles,
+                                       t.bending_stiffness)
+        ths.append(t.theta + delta / deg_e * OMEGA)
+        ps.append(s.position.copy())
+        R.pbd_rod_constraint(s, p)
+    s.velocity = (s.position - prev) / dt
+    s.position = R.periodic_boundary(s.grid_shape, s.position)
+    return s, ps, ths
+
+
+def step_vjp(s: R.State, dt: float, p: R.Params, xb, vb, thb):
+    """Adjoint of one step at state ``s``: returns (xb_prev, vb_prev,
+    thb_prev, rhob) given the adjoints of the step's outputs."""
+    _, ps, ths = step_tape(s, dt, p)
+    deg_n, deg_e = _degrees(s)
+    src, tgt = s.pairs[:, 0], s.pairs[:, 1]
+    bin_, bout = s.bending_pairs[:, 0], s.bending_pairs[:, 1]
+    l0 = np.asarray(s.rest_lengths, np.float64)
+    kb = np.asarray(s.bending_stiffness, np.float64)
+    kst, ksh = float(p.stiffness_stretch), float(p.stiffness_shear)
+    xb = np.asarray(xb, np.float64)
+    vb = np.asarray(vb, np.float64)
+    pb = xb + vb / dt
+    xb_direct = -vb / dt
+    thb = np.array(thb, np.float64, copy=True)
+    rhob = np.zeros(s.bending_pairs.shape[0])
+    for c in reversed(range(p.cycles_per_step)):
+        pos = np.asarray(ps[c], np.float64)
+        th = np.asarray(ths[c], np.float64)
+        # position pass
+        Fb = pb[src] / deg_n[src, None] - pb[tgt] / deg_n[tgt, None]
+        r = np.asarray(R.displacement(s.grid_shape, pos[tgt], pos[src]), np.float64)
+        L = np.sqrt((r * r).sum(axis=1))
+        Ls = np.maximum(L, 1e-8)
+        rh = r / Ls[:, None]
+        a = (1.0 - l0 / Ls)[:, None]
+        # J Fb with J = k_st/2 ((1 - k_sh)(a I + l0/L rh rh^T) + k_sh I)
+        proj = (rh * Fb).sum(axis=1)[:, None] * rh
+        rb = 0.5 * kst * ((1.0 - ksh) * (a * Fb + (l0 / Ls)[:, None] * proj) + ksh * Fb)
+        np.add.at(pb, tgt, rb)
+        np.add.at(pb, src, -rb)
+        thb = thb + (-0.5 * kst * ksh * l0) * (-np.sin(th) * Fb[:, 0] + np.cos(th) * Fb[:, 1])
+        # angle pass
+        corrb = OMEGA * (thb[bin_] / deg_e[bin_] - thb[bout] / deg_e[bout])
+        diffb = corrb * kb * 0.5
+        rhob = rhob - diffb
+        np.add.at(thb, bout, diffb)
+        np.add.at(thb, bin_, -diffb)
+    return pb + xb_direct, pb * float(p.damping) * dt, thb, rhob
+
+
+def msd_loss(position, target, grid_shape):
+    """Folding loss: mean over nodes of the squared minimum-image distance to
+    the target curve (experiments/filament_folding/experiment.py:57-165 uses
+    point-set distances of this kind); returns (loss, d loss / d position)."""
+    d = np.asarray(R.displacement(grid_shape, np.asarray(position, np.float64),
+                                  np.asarray(target, np.float64)), np.float64)
+    n = d.shape[0]
+    return float((d * d).sum() / n), 2.0 * d / n
+
+
+def rollout_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, target):
+    """Loss after ``num_steps`` steps and its gradient with respect to the
+    (time-constant) bending rest angles, by storing every step's state."""
+    states = [s0.astype(np.float64)]
+    for _ in range(num_steps):
+        states.append(R.step(states[-1], dt, p))
+    loss, xb = msd_loss(states[-1].position, target, s0.grid_shape)
+    vb = np.zeros_like(xb)
+    thb = np.zeros(s0.theta.shape[0])
+    rhob = np.zeros(s0.bending_pairs.shape[0])
+    for t in reversed(range(num_steps)):
+        xb, vb, thb, rb = step_vjp(states[t], dt, p, xb, vb, thb)
+        rhob = rhob + rb
+    return loss, rhob
+
+
+def rollout_loss(s0: R.State, dt: float, p: R.Params, num_steps: int, target, rest=None):
+    s = s0.astype(np.float64)
+    if rest is not None:
+        s = replace(s, bending_rest_angles=np.asarray(rest, np.float64))
+    for _ in range(num_steps):
+        s = R.step(s, dt, p)
+    return msd_loss(s.position, target, s0.grid_shape)[0]

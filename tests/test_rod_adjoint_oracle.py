@@ -1,0 +1,91 @@
+"""The rod adjoint restatement (oracle/rod_adjoint.py) against central finite
+differences of the float64 forward restatement.  CPU only."""
+from __future__ import annotations
+
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from oracle import rod_adjoint as A
+from oracle import step_ref as R
+from microcosmos_amd import workloads as W
+
+from .helpers import oracle_state
+
+
+def _case(damping=0.5, cycles=4, seed=11):
+    wl = W.small(grid=(32, 32), n_fil=2, n_nodes=12, seed=seed)
+    rng = np.random.default_rng(seed)
+    s = oracle_state(wl, np.float64)
+    s = replace(s, bending_rest_angles=rng.normal(0.0, 0.2, s.bending_pairs.shape[0]),
+                velocity=rng.normal(0.0, 0.05, s.position.shape))
+    p = replace(R.PBD_SCHEME_NO_FLUID, cycles_per_step=cycles, damping=damping,
+                enable_steric=False, stiffness_stretch=0.6, stiffness_shear=0.8)
+    target = s.position + rng.normal(0.0, 1.0, s.position.shape)
+    return s, p, target
+
+
+def _fd(f, x, h):
+    g = np.zeros_like(x)
+    for i in range(x.size):
+        e = np.zeros_like(x)
+        e.flat[i] = h
+        g.flat[i] = (f(x + e) - f(x - e)) / (2 * h)
+    return g
+
+
+def test_rollout_rest_angle_gradient_matches_fd():
+    s, p, target = _case()
+    steps = 6
+    loss, g = A.rollout_grad(s, 1.0, p, steps, target)
+    fd = _fd(lambda r: A.rollout_loss(s, 1.0, p, steps, target, rest=r), s.bending_rest_angles, 1e-6)
+    assert np.isfinite(loss)
+    assert np.max(np.abs(g - fd)) <= 1e-6 * max(1.0, np.max(np.abs(fd))), (g, fd)
+    cos = g @ fd / (np.linalg.norm(g) * np.linalg.norm(fd))
+    assert cos > 0.999999
+
+
+@pytest.mark.parametrize("which", ["position", "velocity", "theta"])
+def test_step_vjp_directional_derivatives(which):
+    """<vjp(w), u> equals the finite-difference derivative of <w, step(s)> in
+    direction u, for the step inputs x, v and theta."""
+    s, p, _ = _case(damping=0.7, cycles=3, seed=5)
+    rng = np.random.default_rng(3)
+    wx = rng.normal(size=s.position.shape)
+    wv = rng.normal(size=s.position.shape)
+    wt = rng.normal(size=s.theta.shape)
+    xb, vb, tb, _ = A.step_vjp(s, 1.0, p, wx, wv, wt)
+    u = rng.normal(size=getattr(s, which).shape)
+    adj = {"position": xb, "velocity": vb, "theta": tb}[which]
+
+    def f(eps):
+        t = replace(s, **{which: getattr(s, which) + eps * u})
+        o = R.step(t, 1.0, p)
+        # unwrap the output positions around the unperturbed run
+        return (np.sum(wx * o.position) + np.sum(wv * o.velocity) + np.sum(wt * o.theta))
+
+    base = R.step(s, 1.0, p)
+    h = 1e-6
+
+    def f_unwrapped(eps):
+        t = replace(s, **{which: getattr(s, which) + eps * u})
+        o = R.step(t, 1.0, p)
+        d = R.displacement(s.grid_shape, o.position, base.position)
+        return (np.sum(wx * (base.position + d)) + np.sum(wv * o.velocity) + np.sum(wt * o.theta))
+
+    fd = (f_unwrapped(h) - f_unwrapped(-h)) / (2 * h)
+    assert abs(np.sum(adj * u) - fd) <= 1e-6 * max(1.0, abs(fd)), (np.sum(adj * u), fd)
+
+
+def test_zero_damping_cuts_the_velocity_path():
+    s, p, target = _case(damping=0.0)
+    xb, vb, tb, rb = A.step_vjp(s, 1.0, p, np.ones_like(s.position), np.ones_like(s.position),
+                                np.zeros_like(s.theta))
+    assert np.all(vb == 0.0)
+
+
+def test_rejects_fluid_and_steric():
+    s, p, target = _case()
+    with pytest.raises(NotImplementedError):
+        A.step_vjp(s, 1.0, replace(p, enable_steric=True), s.position, s.position, s.theta)
