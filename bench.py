@@ -121,6 +121,10 @@ def serving(n_req, dev, lanes, chunk, max_batch, plain=True):
             wl = it.workload
             n, e, f = device_state(wl, dev)
             e = replace(e, pairs=wl.pairs, bending_pairs=wl.bending_pairs)
+            if it.gradient:
+                out.append(Request(nodes=n, edges=e, fields=f, steps=it.steps, arrival=it.arrival,
+                                   gradient=True, target=torch.as_tensor(it.target, device=dev)))
+                continue
             act = tuple(torch.as_tensor(a, device=dev)
                         for a in (wl.act_amp, wl.act_omega, wl.act_phase))
             out.append(Request(nodes=n, edges=e, fields=f, steps=it.steps, actuation=act,
@@ -137,14 +141,17 @@ def serving(n_req, dev, lanes, chunk, max_batch, plain=True):
     done = srv.run(reqs, realtime=True)
     torch.cuda.synchronize()
     rep = latency_report(done)
-    finite = all(bool(torch.isfinite(r.result[0].position).all().item()) for r in done)
-    cells = sum(it.workload.grid_shape[0] * it.workload.grid_shape[1] * it.steps for it in trace)
+    finite = all(bool(torch.isfinite(r.rest_angle_grad if r.gradient else r.result[0].position)
+                      .all().item()) for r in done)
+    cells = sum(it.workload.grid_shape[0] * it.workload.grid_shape[1] * it.steps for it in trace
+                if not it.gradient)
     out = {"workload": f"serving-trace-{n_req}", "arrival_rate_per_s": 200.0,
            "grid_sides": [128, 256, 512, 1024], "filaments": [1, 256], "nodes_per_filament": [16, 64],
            "rollout_steps": [200, 2000], "lanes": lanes, "chunk_steps": chunk,
            "max_batch": max_batch, "lattice_cell_updates": cells,
            "gradient_requests": sum(it.gradient for it in trace),
-           "gradient": "not computed: gradient requests run their forward rollout only",
+           "gradient": "folding gradient (256^2, 8 x 48 nodes, 500 steps, checkpoints every 50) "
+                       "through the rod adjoint kernels",
            "p50_latency_s": round(rep["p50_latency_s"], 4),
            "p99_latency_s": round(rep["p99_latency_s"], 4),
            "mean_queue_wait_s": round(rep["mean_queue_wait_s"], 4),
@@ -154,18 +161,24 @@ def serving(n_req, dev, lanes, chunk, max_batch, plain=True):
            "peak_admitted_bytes": srv.peak_used, "finite": finite}
     print(f"[bench] served: {rep}", file=sys.stderr, flush=True)
     if plain:
+        from microcosmos_amd import PBD_SCHEME_NO_FLUID
+        from microcosmos_amd.grad import rollout_value_and_grad
+        gcfg = replace(PBD_SCHEME_NO_FLUID, enable_steric=False)
         service = []
         for it in trace:
             wl = it.workload
             n, e, f = device_state(wl, dev)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            ro = Rollout(n, e, f, cfg, dt, actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
-            ro.track()
-            ro.run(it.steps)
+            if it.gradient:
+                rollout_value_and_grad(n, e, f, gcfg, dt, it.steps,
+                                       torch.as_tensor(it.target, device=dev), 50)
+            else:
+                ro = Rollout(n, e, f, cfg, dt, actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
+                ro.track()
+                ro.run(it.steps)
             torch.cuda.synchronize()
             service.append(time.perf_counter() - t0)
-            del ro
         fifo = fifo_report(np.array([it.arrival for it in trace]), np.array(service))
         print(f"[bench] one by one: {fifo}", file=sys.stderr, flush=True)
         out["plain_one_by_one"] = {k: round(v, 4) if isinstance(v, float) else v
@@ -173,6 +186,48 @@ def serving(n_req, dev, lanes, chunk, max_batch, plain=True):
         out["vs_plain_rollouts_per_s"] = round(rep["rollouts_per_s"] / fifo["rollouts_per_s"], 2)
         out["vs_plain_p99"] = round(fifo["p99_latency_s"] / rep["p99_latency_s"], 2)
     return out
+
+
+def gradient_leg(dev, reps=5):
+    """The folding gradient: loss of the final positions after 500 steps of
+    the rod-only step (256^2, 8 filaments x 48 nodes) and its gradient with
+    respect to the 368 hinge rest angles (384 per-node curvatures), checkpoints
+    every 50 steps, forward and adjoint kernels on the device."""
+    import torch
+    from microcosmos_amd import PBD_SCHEME_NO_FLUID
+    from microcosmos_amd import workloads as W
+    from microcosmos_amd.grad import RodAdjoint, min_image
+    from tests.helpers import device_state
+    wl, tgt = W.folding()
+    cfg = replace(PBD_SCHEME_NO_FLUID, enable_steric=False)
+    target = torch.as_tensor(tgt, device=dev)
+
+    def one():
+        n, e, f = device_state(wl, dev)
+        ad = RodAdjoint(n, e, f, cfg, wl.dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = ad.forward(500, 50)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        d = min_image(ad.grid_shape, x, target)
+        g = ad.backward(2.0 * d / x.shape[0])[0]
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, float(torch.isfinite(g).all().item())
+
+    one()
+    fw, bw = [], []
+    for _ in range(reps):
+        a, b, ok = one()
+        fw.append(a)
+        bw.append(b)
+    import numpy as np
+    f_ms, b_ms = float(np.median(fw)) * 1e3, float(np.median(bw)) * 1e3
+    return {"workload": wl.name, "steps": 500, "checkpoint_every": 50, "parameters": 384,
+            "hinges": int(wl.bending_pairs.shape[0]), "forward_ms": round(f_ms, 3),
+            "backward_ms": round(b_ms, 3), "gradients_per_s": round(1e3 / (f_ms + b_ms), 2),
+            "finite": bool(ok)}
 
 
 def main():
@@ -196,6 +251,7 @@ def main():
     ap.add_argument("--serve-no-plain", action="store_true")
     ap.add_argument("--only-serve", action="store_true",
                     help="run only the serving leg and print its JSON")
+    ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 9 x 1024 x 1024 float32 device copy (37.7 MB read + "
                          "37.7 MB written per launch) to calibrate HBM byte counters")
@@ -256,6 +312,10 @@ def main():
     batched = None
     if not args.no_batch:
         batched = batched_es(args.batch_instances, args.batch_steps, args.plain_sample, dev)
+    grad = None
+    if not args.no_grad:
+        grad = gradient_leg(dev)
+        print(f"[bench] gradient {grad}", file=sys.stderr, flush=True)
     served = None
     if not args.no_serve:
         served = serving(args.serve_requests, dev, args.serve_lanes, args.serve_chunk,
@@ -279,6 +339,7 @@ def main():
                       "ibm_iterations": 5, "steric_sigma": 2.0, "tau_plus": 0.6,
                       "lambda_trt": 0.1875, "hip_graph": use_graph, "finite": finite},
            "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "serving": served,
+           "gradient": grad,
            "coupled_steps_per_s": round(args.steps / elapsed, 2),
            "step_ms_events": round(step_ms_ev, 5)}
     print(json.dumps(out), flush=True)

@@ -115,6 +115,13 @@ class Plan:
             self._lib.mc_plan_destroy(self.handle)
             self.handle = ctypes.c_void_p()
 
+    def release(self, stream: int):
+        """Frees the plan once the work enqueued on ``stream`` (a hipStream_t
+        handle) has finished, without synchronising other streams."""
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            check(self._lib.mc_plan_destroy_after(self.handle, stream))
+            self.handle = ctypes.c_void_p()
+
     def __del__(self):
         try:
             self.close()

@@ -24,6 +24,10 @@ the device until it completes:
   wherever its chunks fell.
 * **Overlap.**  While one lane re-forms its batch on the host, the other
   lanes keep the device busy, and the lanes' kernels run concurrently.
+* **Gradient requests** (the folding gradient of the reference's
+  differentiable experiments, experiments/filament_folding/experiment.py
+  :985-1018) run on a lane of their own, one at a time: a checkpointed
+  forward rollout and the hand-written adjoint (``grad.RodAdjoint``).
 
 Latency of a request = completion (observed by the server) - arrival.
 """
@@ -67,6 +71,12 @@ class Request:
     actuation: Optional[tuple] = None
     arrival: float = 0.0
     rid: int = -1
+    # gradient request: d loss / d bending rest angles of the folding loss
+    # (mean squared minimum-image distance of the final positions to target)
+    gradient: bool = False
+    target: Optional[torch.Tensor] = None
+    loss: Optional[torch.Tensor] = None
+    rest_angle_grad: Optional[torch.Tensor] = None
     # filled in by the server
     footprint: int = 0
     admitted: Optional[float] = None
@@ -84,6 +94,14 @@ class _Slot:
     req: Request
     done: int = 0            # steps completed
     joined_at: int = 0       # steps completed when it joined the lane's batch
+
+
+@dataclass
+class _GradLane:
+    stream: torch.cuda.Stream
+    req: Optional[Request] = None
+    adj: Optional[object] = None
+    event: Optional[torch.cuda.Event] = None
 
 
 @dataclass
@@ -110,7 +128,8 @@ class Server:
     """
 
     def __init__(self, solver_config, dt: float, hbm_budget: Optional[int] = None,
-                 lanes: int = 2, chunk: int = 64, max_batch: int = 64, device=None):
+                 lanes: int = 2, chunk: int = 64, max_batch: int = 64, device=None,
+                 gradient_config=None, checkpoint_every: int = 50):
         self.dev = torch.device(device) if device is not None else torch.device(
             "cuda", torch.cuda.current_device())
         if self.dev.type != "cuda":
@@ -126,6 +145,13 @@ class Server:
         if lanes < 1 or self.chunk < 1 or self.max_batch < 1:
             raise ValueError("lanes, chunk and max_batch must be positive")
         self.lanes = [_Lane(stream=torch.cuda.Stream(device=self.dev)) for _ in range(lanes)]
+        if gradient_config is None:
+            from .solver.config import PBD_SCHEME_NO_FLUID
+            gradient_config = replace(PBD_SCHEME_NO_FLUID, enable_steric=False)
+        self.gradient_config = gradient_config
+        self.checkpoint_every = int(checkpoint_every)
+        self.glane = _GradLane(stream=torch.cuda.Stream(device=self.dev))
+        self.gwaiting: List[Request] = []
         self.waiting: List[Request] = []
         self.completed: List[Request] = []
         self.used = 0
@@ -146,25 +172,32 @@ class Server:
                                         int(req.edges.bending_pairs.shape[0]))
         if req.footprint > self.budget:
             raise ValueError(f"request needs ~{req.footprint} B, more than the budget {self.budget} B")
+        if req.gradient:
+            # checkpoints and one recomputed segment of the rollout on top
+            per_state = 4 * (4 * n + int(req.edges.pairs.shape[0]))
+            req.footprint += per_state * (req.steps // self.checkpoint_every + 2 +
+                                          self.checkpoint_every)
+            if req.target is None:
+                raise ValueError("a gradient request needs a target curve")
         if req.rid < 0:
             req.rid = self._next_id
         self._next_id += 1
-        self.waiting.append(req)
+        (self.gwaiting if req.gradient else self.waiting).append(req)
         return req
 
     @property
     def active(self) -> int:
-        return sum(len(l.slots) for l in self.lanes)
+        return sum(len(l.slots) for l in self.lanes) + (1 if self.glane.req is not None else 0)
 
     def idle(self) -> bool:
-        return not self.waiting and self.active == 0
+        return not self.waiting and not self.gwaiting and self.active == 0
 
     # ---- scheduling -------------------------------------------------------
     def poll(self) -> List[Request]:
         """One scheduling pass: harvest lanes whose chunk finished, retire
         completed requests, admit waiting ones, launch the next chunks.
         Returns the requests completed in this pass."""
-        done: List[Request] = []
+        done: List[Request] = self._poll_gradient()
         free = [l for l in self.lanes if l.event is None or l.event.query()]
         if not free:
             return done
@@ -179,6 +212,40 @@ class Server:
             done.extend(self._reform(lane, share))
             self._launch(lane)
         return done
+
+    def _poll_gradient(self) -> List[Request]:
+        """The gradient lane: retire the finished request, start the next."""
+        g = self.glane
+        out: List[Request] = []
+        if g.req is not None:
+            if not g.event.query():
+                return out
+            r = g.req
+            r.finished = self.clock()
+            g.adj.plan.release(int(g.stream.cuda_stream))
+            self.used -= r.footprint
+            self.completed.append(r)
+            out.append(r)
+            g.req, g.adj, g.event = None, None, None
+        if not self.gwaiting or self.used + self.gwaiting[0].footprint > self.budget:
+            return out
+        from .grad import RodAdjoint, min_image
+        r = self.gwaiting.pop(0)
+        self.used += r.footprint
+        self.peak_used = max(self.peak_used, self.used)
+        r.admitted = self.clock()
+        with torch.cuda.stream(g.stream):
+            g.stream.wait_stream(torch.cuda.current_stream(self.dev))
+            adj = RodAdjoint(r.nodes, r.edges, r.fields, self.gradient_config, self.dt)
+            x = adj.forward(r.steps, self.checkpoint_every)
+            d = min_image(adj.grid_shape, x, r.target)
+            n = x.shape[0]
+            r.loss = (d * d).sum() / n
+            r.rest_angle_grad = adj.backward(2.0 * d / n)[0]
+            ev = torch.cuda.Event()
+            ev.record(g.stream)
+        g.req, g.adj, g.event = r, adj, ev
+        return out
 
     def _admit(self, lane: _Lane, share: int) -> List[Request]:
         out = []

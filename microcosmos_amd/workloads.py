@@ -200,21 +200,26 @@ def es_population(n=512, seed=2, sides=(128, 192, 256), max_filaments=16, min_no
 @dataclass
 class ServeItem:
     """One request of a serving trace: arrival time (s), workload, steps and
-    whether the request also asks for a gradient."""
+    whether it is a gradient request (then ``target`` is its folding target
+    curve and the workload a folding workload)."""
     arrival: float
     workload: Workload
     steps: int
     gradient: bool
+    target: Optional[np.ndarray] = None
 
 
 def serving_trace(n=48, seed=4, rate=200.0, sides=(128, 256, 512, 1024), max_filaments=256,
                   min_nodes=16, max_nodes=64, min_steps=200, max_steps=2000,
                   gradient_fraction=0.2):
-    """Poisson(rate) arrivals of ``n`` rollout requests: grid side uniform over
-    ``sides``, 1..max_filaments filaments of min_nodes..max_nodes nodes,
-    rollouts of min_steps..max_steps steps, a fraction asking for a gradient.
-    Controllers are per-filament actuation amplitudes U(0, 0.5) and periods
-    U(100, 400); fluid, IB and steric settings of the 128^2 plain case."""
+    """Poisson(rate) arrivals of ``n`` requests.  A fraction
+    ``gradient_fraction`` are gradient requests: the folding gradient of
+    ``folding()`` (256^2, 8 filaments x 48 nodes, 500 steps) with a seeded
+    target.  The others are rollouts: grid side uniform over ``sides``,
+    1..max_filaments filaments of min_nodes..max_nodes nodes, rollouts of
+    min_steps..max_steps steps, per-filament actuation amplitudes U(0, 0.5)
+    and periods U(100, 400), fluid, IB and steric settings of the 128^2 plain
+    case."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -227,11 +232,17 @@ def serving_trace(n=48, seed=4, rate=200.0, sides=(128, 256, 512, 1024), max_fil
         periods = rng.uniform(100.0, 400.0, size=nf)
         steps = int(rng.integers(min_steps, max_steps + 1))
         grad = bool(rng.uniform() < gradient_fraction)
+        wseed = int(rng.integers(1 << 30))
+        if grad:
+            wl, tgt = folding(seed=wseed % 100000)
+            out.append(ServeItem(arrival=t, workload=wl, steps=wl.steps, gradient=True,
+                                 target=tgt))
+            continue
         wl = filaments_workload(
-            f"req-{i}-{side}x{side}-{nf}f", (side, side), nf, 0, int(rng.integers(1 << 30)),
+            f"req-{i}-{side}x{side}-{nf}f", (side, side), nf, 0, wseed,
             node_counts=counts, amps=amps, periods=periods, steps=steps,
             overrides=_lbm_overrides(0.8, 4))
-        out.append(ServeItem(arrival=t, workload=wl, steps=steps, gradient=grad))
+        out.append(ServeItem(arrival=t, workload=wl, steps=steps, gradient=False))
     return out
 
 

@@ -159,3 +159,38 @@ def test_server_single_request_alone(dev):
     ro, dref = _alone(wl, 30, cfg, dev)
     n, e, f = r.result
     _assert_same("single", n, e, f, r.displacement, ro, dref)
+
+
+def test_server_gradient_requests_equal_alone(dev):
+    """Gradient requests (folding gradients) served next to rollouts give
+    the same loss and gradient as computed alone."""
+    _say("server: 2 gradient requests among 3 rollouts")
+    from microcosmos_amd import PBD_SCHEME_NO_FLUID
+    from microcosmos_amd.grad import rollout_value_and_grad
+    from microcosmos_amd.serve import Request, Server
+    wls = _mix()[:3]
+    cfg = device_config(wls[0])
+    reqs = _requests(dev, [(0.0, wls[0], 20), (0.001, wls[1], 14), (0.002, wls[2], 9)])
+    grads = []
+    for k, seed in enumerate((5, 6)):
+        wl, tgt = W.folding(seed=seed, grid=(64, 64), n_fil=3, n_nodes=20, steps=60)
+        nodes, edges, fields = device_state(wl, dev)
+        edges = replace(edges, pairs=wl.pairs.copy(), bending_pairs=wl.bending_pairs.copy())
+        grads.append((wl, tgt))
+        reqs.append(Request(nodes=nodes, edges=edges, fields=fields, steps=60, arrival=0.0005 * k,
+                            gradient=True, target=torch.as_tensor(tgt, device=dev)))
+    srv = Server(cfg, 1.0, lanes=2, chunk=8, device=dev, checkpoint_every=16)
+    srv.run(reqs, realtime=True)
+    torch.cuda.synchronize()
+    gcfg = replace(PBD_SCHEME_NO_FLUID, enable_steric=False)
+    for r, (wl, tgt) in zip(reqs[3:], grads):
+        nodes, edges, fields = device_state(wl, dev)
+        loss, g = rollout_value_and_grad(nodes, edges, fields, gcfg, 1.0, 60,
+                                         torch.as_tensor(tgt, device=dev), checkpoint_every=16)
+        torch.cuda.synchronize()
+        assert float(r.loss) == float(loss)
+        np.testing.assert_array_equal(_np(r.rest_angle_grad), _np(g))
+    for r, wl, k in zip(reqs[:3], wls, (20, 14, 9)):
+        ro, dref = _alone(wl, k, cfg, dev)
+        n, e, f = r.result
+        _assert_same(f"request {r.rid}", n, e, f, r.displacement, ro, dref)
