@@ -208,8 +208,11 @@ mc_status mc_batch_advance(mc_plan* plan, const mc_params* params, const mc_stat
  * last step (N, 2) and velocity_bar = dL/d velocity (may be NULL: zero), and
  * writes rest_angle_bar (B) = dL/d bending_rest_angles, plus (each may be
  * NULL) the gradient with respect to the initial position, velocity and
- * theta.  Between checkpoints the steps are recomputed; each step's reverse
- * pass is one hand-written adjoint kernel (one workgroup per filament).
+ * theta.  Without fluid and steric term the filaments are independent: one
+ * workgroup per filament runs a whole checkpoint segment per launch, forward
+ * (bit-identical to mc_rollout) or reverse (the segment recomputed from its
+ * checkpoint, then the hand-written adjoint of each step backwards).  Every
+ * filament must fit the LDS budget (MC_ERR_UNSUPPORTED otherwise).
  * params, dt and the edge arrays must be those of the recorded rollout.
  * Minimum-image, angle and periodic wraps are differentiated as identity. */
 mc_status mc_rod_adjoint_forward(mc_plan* plan, const mc_params* params, const mc_state* state,

@@ -3,11 +3,17 @@
 // (pbd.py:14-91).  The reference differentiates this step with jax.grad
 // through lax.scan (experiments/filament_folding/experiment.py:985-1018).
 //
-// One workgroup per filament (connected component), like the forward rod
-// kernel.  The workgroup first recomputes the step's constraint cycles from
-// the step's input state, with the forward's arithmetic, keeping every
-// cycle's positions (before the position pass) and angles (after the angle
-// pass) in LDS; then it runs the cycles backwards:
+// Without the fluid and the steric term the filaments do not interact, so one
+// workgroup (one wavefront: its barriers are cheap) per filament (connected
+// component) runs a whole checkpoint segment in one launch, its state in LDS:
+//   * forward segment: K steps of the rod-only step with the forward rod
+//     kernel's arithmetic (bit-identical results), the segment's start state
+//     written to the checkpoint slot;
+//   * reverse segment: the K steps recomputed from the checkpoint (each
+//     step's start state kept in the filament's part of a global tape), then
+//     step by step backwards: the step's constraint cycles are recomputed in
+//     LDS, keeping every cycle's positions (before the position pass) and
+//     angles (after the angle pass), and run backwards:
 //
 //   position pass, per edge (s, g):  Fb = pb[s] / deg_s - pb[g] / deg_g
 //       rb = J Fb,  J = k_st/2 ((1 - k_sh)((1 - l0/L) I + l0/L rhat rhat^T) + k_sh I)
@@ -25,69 +31,84 @@
 namespace mc {
 namespace {
 
-__global__ void __launch_bounds__(256) k_rod_adjoint(RodAdjArgs a) {
-  extern __shared__ float sm[];
-  const Consts& c = a.c;
-  const RodLayoutDev& L = a.L;
-  const int comp = blockIdx.x;
-  const CompMeta m = L.meta[comp];
-  const InstGeo& gi = a.bd.geo[m.inst];
-  const int n0 = m.n0, nn = m.nn, e0 = m.e0, ne = m.ne, b0 = m.b0, nb = m.nb;
-  const int C = c.cycles;
-  const int tid = threadIdx.x, bs = blockDim.x;
-  float* Xs = sm;                    // C x 2nn: positions before each position pass
-  float* THs = Xs + (size_t)C * 2 * nn;  // C x ne: angles after each angle pass
-  float* X = THs + (size_t)C * ne;   // 2nn
-  float* TH = X + 2 * nn;            // ne
-  float* CO = TH + ne;               // nb
-  float* FE = CO + nb;               // 2ne: edge forces, then edge adjoints rb
-  float* XB = FE + 2 * ne;           // 2nn
-  float* THB = XB + 2 * nn;          // ne
-  float* DB = THB + ne;              // nb
-  float* RB = DB + nb;               // nb
-  const int* esrc = L.edge_src + e0;
-  const int* etgt = L.edge_tgt + e0;
-  const int* hin = L.hinge_in + b0;
-  const int* hout = L.hinge_out + b0;
-  const int* einc_off = L.edge_inc_off + e0;
-  const int* ninc_off = L.node_inc_off + n0;
-  const float Wd = gi.dom_wf, Hd = gi.dom_hf;
+// Per-component view of the rod layout and the LDS arrays of a segment.
+struct SegCtx {
+  int nn, ne, nb, n0, e0, b0, C;
+  const int *esrc, *etgt, *hin, *hout, *einc_off, *ninc_off, *edge_inc, *node_inc;
+  const int *nodes, *edges, *hinges;   // component order -> global ids
+  float Wd, Hd;
+  // LDS
+  float *X, *V, *TH, *P, *CO, *FE, *RS, *KB, *LE;
+  float *Xs, *THs;                     // C x 2nn, C x ne (reverse only)
+  float *XB, *VB, *THB, *DB, *RB;      // adjoints (reverse only)
+};
 
-  // prologue (simulate.py:46-50) and the step's angles
-  for (int i = tid; i < nn; i += bs) {
-    const int g = L.comp_nodes[n0 + i];
-    const float dvx = a.vel[2 * g] * c.damping, dvy = a.vel[2 * g + 1] * c.damping;
-    X[2 * i] = a.pos[2 * g] + dvx * c.dt;
-    X[2 * i + 1] = a.pos[2 * g + 1] + dvy * c.dt;
+__device__ __forceinline__ void seg_init(const RodAdjArgs& a, SegCtx& s, float* sm, bool reverse) {
+  const RodLayoutDev& L = a.L;
+  const CompMeta m = L.meta[blockIdx.x];
+  const InstGeo& gi = a.bd.geo[m.inst];
+  s.nn = m.nn; s.ne = m.ne; s.nb = m.nb; s.n0 = m.n0; s.e0 = m.e0; s.b0 = m.b0;
+  s.C = a.c.cycles;
+  s.esrc = L.edge_src + m.e0; s.etgt = L.edge_tgt + m.e0;
+  s.hin = L.hinge_in + m.b0; s.hout = L.hinge_out + m.b0;
+  s.einc_off = L.edge_inc_off + m.e0; s.ninc_off = L.node_inc_off + m.n0;
+  s.edge_inc = L.edge_inc; s.node_inc = L.node_inc;
+  s.nodes = L.comp_nodes + m.n0; s.edges = L.comp_edges + m.e0; s.hinges = L.comp_hinges + m.b0;
+  s.Wd = gi.dom_wf; s.Hd = gi.dom_hf;
+  const int nn = s.nn, ne = s.ne, nb = s.nb;
+  float* q = sm;
+  auto take = [&](size_t n) { float* r = q; q += n; return r; };
+  s.X = take(2 * nn); s.V = take(2 * nn); s.P = take(2 * nn); s.TH = take(ne);
+  s.CO = take(nb); s.FE = take(2 * ne); s.RS = take(nb); s.KB = take(nb); s.LE = take(ne);
+  if (reverse) {
+    s.Xs = take((size_t)s.C * 2 * nn); s.THs = take((size_t)s.C * ne);
+    s.XB = take(2 * nn); s.VB = take(2 * nn); s.THB = take(ne); s.DB = take(nb); s.RB = take(nb);
   }
-  for (int e = tid; e < ne; e += bs) TH[e] = a.theta[L.comp_edges[e0 + e]];
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    s.RS[b] = a.rest_ang[s.hinges[b]];
+    s.KB[b] = a.kb[s.hinges[b]];
+  }
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) s.LE[e] = a.rest_len[s.edges[e]];
+}
+
+// One rod-only step on the LDS state (X, V, TH) -> (X, V, TH), with the
+// arithmetic of the forward rod kernel (PBD branch, steric and fluid off):
+// prologue (simulate.py:46-50), the cycles (pbd.py:14-91), velocity and the
+// final wrap (simulate.py:83-98).  With `keep`, every cycle's positions
+// before the position pass and angles after the angle pass go to Xs / THs.
+__device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, bool keep) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int nn = s.nn, ne = s.ne, nb = s.nb;
+  for (int i = tid; i < nn; i += bs) {
+    const float dvx = s.V[2 * i] * c.damping, dvy = s.V[2 * i + 1] * c.damping;
+    s.P[2 * i] = s.X[2 * i] + dvx * c.dt;
+    s.P[2 * i + 1] = s.X[2 * i + 1] + dvy * c.dt;
+  }
   __syncthreads();
-  // forward cycles, as the rod kernel computes them (PBD branch)
-  for (int cyc = 0; cyc < C; ++cyc) {
-    float* Xc = Xs + (size_t)cyc * 2 * nn;
+  for (int cyc = 0; cyc < s.C; ++cyc) {
     for (int b = tid; b < nb; b += bs) {
-      const int gb = L.comp_hinges[b0 + b];
-      const float diff = wrap_angle(TH[hout[b]] - TH[hin[b]]);
-      CO[b] = ((diff - a.rest_ang[gb]) * a.kb[gb]) * 0.5f;
+      const float diff = wrap_angle(s.TH[s.hout[b]] - s.TH[s.hin[b]]);
+      s.CO[b] = ((diff - s.RS[b]) * s.KB[b]) * 0.5f;
     }
-    for (int i = tid; i < 2 * nn; i += bs) Xc[i] = X[i];
+    if (keep)
+      for (int i = tid; i < 2 * nn; i += bs) s.Xs[(size_t)cyc * 2 * nn + i] = s.P[i];
     __syncthreads();
     for (int e = tid; e < ne; e += bs) {
-      const int i0 = einc_off[e], i1 = einc_off[e + 1];
+      const int i0 = s.einc_off[e], i1 = s.einc_off[e + 1];
       float d = 0.0f;
       for (int q = i0; q < i1; ++q) {
-        const int code = L.edge_inc[q];
-        const float v = CO[code >> 1];
+        const int code = s.edge_inc[q];
+        const float v = s.CO[code >> 1];
         d = d + ((code & 1) ? -v : v);
       }
       const float deg = fmax_nan((float)(i1 - i0), 1.0f);
-      const float th = TH[e] + (d / deg) * 1.8f;
-      TH[e] = th;
-      THs[(size_t)cyc * ne + e] = th;
-      const int sn = esrc[e], tn = etgt[e];
-      const float rx = min_image(X[2 * tn], X[2 * sn], Wd);
-      const float ry = min_image(X[2 * tn + 1], X[2 * sn + 1], Hd);
-      const float l = a.rest_len[L.comp_edges[e0 + e]];
+      const float th = s.TH[e] + (d / deg) * 1.8f;
+      s.TH[e] = th;
+      if (keep) s.THs[(size_t)cyc * ne + e] = th;
+      const int sn = s.esrc[e], tn = s.etgt[e];
+      const float rx = min_image(s.P[2 * tn], s.P[2 * sn], s.Wd);
+      const float ry = min_image(s.P[2 * tn + 1], s.P[2 * sn + 1], s.Hd);
+      const float l = s.LE[e];
       double sd, cd;
       sincos((double)th, &sd, &cd);
       const float tx = (float)cd * l, ty = (float)sd * l;
@@ -96,126 +117,232 @@ __global__ void __launch_bounds__(256) k_rod_adjoint(RodAdjArgs a) {
       const float cdx = rx / md, cdy = ry / md;
       const float stx = cdx * (len - l), sty = cdy * (len - l);
       const float shx = (rx - tx) - stx, shy = (ry - ty) - sty;
-      FE[2 * e] = ((stx + shx * c.k_sh) * c.k_st) * 0.5f;
-      FE[2 * e + 1] = ((sty + shy * c.k_sh) * c.k_st) * 0.5f;
+      s.FE[2 * e] = ((stx + shx * c.k_sh) * c.k_st) * 0.5f;
+      s.FE[2 * e + 1] = ((sty + shy * c.k_sh) * c.k_st) * 0.5f;
     }
     __syncthreads();
     for (int i = tid; i < nn; i += bs) {
-      const int i0 = ninc_off[i], i1 = ninc_off[i + 1];
+      const int i0 = s.ninc_off[i], i1 = s.ninc_off[i + 1];
       float ux = 0.0f, uy = 0.0f;
       for (int q = i0; q < i1; ++q) {
-        const int code = L.node_inc[q];
+        const int code = s.node_inc[q];
         const int e = code >> 1;
-        ux = ux + ((code & 1) ? -FE[2 * e] : FE[2 * e]);
-        uy = uy + ((code & 1) ? -FE[2 * e + 1] : FE[2 * e + 1]);
+        ux = ux + ((code & 1) ? -s.FE[2 * e] : s.FE[2 * e]);
+        uy = uy + ((code & 1) ? -s.FE[2 * e + 1] : s.FE[2 * e + 1]);
       }
       const float deg = fmax_nan((float)(i1 - i0), 1.0f);
-      X[2 * i] = X[2 * i] + ux / deg;
-      X[2 * i + 1] = X[2 * i + 1] + uy / deg;
+      s.P[2 * i] = s.P[2 * i] + ux / deg;
+      s.P[2 * i + 1] = s.P[2 * i + 1] + uy / deg;
     }
     __syncthreads();
   }
-  // adjoint of the epilogue: pb = xb + vb / dt (the direct -vb / dt term is
-  // added at the end)
   for (int i = tid; i < nn; i += bs) {
-    const int g = L.comp_nodes[n0 + i];
-    XB[2 * i] = a.xb[2 * g] + a.vb[2 * g] / c.dt;
-    XB[2 * i + 1] = a.xb[2 * g + 1] + a.vb[2 * g + 1] / c.dt;
+    const float xx = s.P[2 * i], xy = s.P[2 * i + 1];
+    s.V[2 * i] = (xx - s.X[2 * i]) / c.dt;
+    s.V[2 * i + 1] = (xy - s.X[2 * i + 1]) / c.dt;
+    s.X[2 * i] = fmodf_floor(xx, s.Wd);
+    s.X[2 * i + 1] = fmodf_floor(xy, s.Hd);
   }
-  for (int e = tid; e < ne; e += bs) THB[e] = a.thb[L.comp_edges[e0 + e]];
-  for (int b = tid; b < nb; b += bs) RB[b] = 0.0f;
+  __syncthreads();
+}
+
+// Packed state slot: position (2N), velocity (2N), theta (E) by global ids.
+__device__ __forceinline__ void seg_load(const SegCtx& s, const float* pos, const float* vel,
+                                         const float* th) {
+  for (int i = threadIdx.x; i < s.nn; i += blockDim.x) {
+    const int g = s.nodes[i];
+    s.X[2 * i] = pos[2 * g];
+    s.X[2 * i + 1] = pos[2 * g + 1];
+    s.V[2 * i] = vel[2 * g];
+    s.V[2 * i + 1] = vel[2 * g + 1];
+  }
+  for (int e = threadIdx.x; e < s.ne; e += blockDim.x) s.TH[e] = th[s.edges[e]];
+  __syncthreads();
+}
+
+__device__ __forceinline__ void seg_store(const SegCtx& s, float* pos, float* vel, float* th) {
+  for (int i = threadIdx.x; i < s.nn; i += blockDim.x) {
+    const int g = s.nodes[i];
+    pos[2 * g] = s.X[2 * i];
+    pos[2 * g + 1] = s.X[2 * i + 1];
+    vel[2 * g] = s.V[2 * i];
+    vel[2 * g + 1] = s.V[2 * i + 1];
+  }
+  for (int e = threadIdx.x; e < s.ne; e += blockDim.x) th[s.edges[e]] = s.TH[e];
+}
+
+// Forward segment: the state (a.pos / a.vel / a.theta, in place) is written
+// to the checkpoint slot, then advanced `steps` steps; debug_vector is zero
+// (PBD).
+__global__ void __launch_bounds__(64) k_rod_fwd_segment(RodAdjArgs a, int steps, float* ck,
+                                                         float* dbg) {
+  extern __shared__ float sm[];
+  SegCtx s;
+  seg_init(a, s, sm, false);
+  float* pos = const_cast<float*>(a.pos);
+  float* vel = const_cast<float*>(a.vel);
+  float* th = const_cast<float*>(a.theta);
+  seg_load(s, pos, vel, th);
+  const size_t N2 = 2 * (size_t)a.N;
+  seg_store(s, ck, ck + N2, ck + 2 * N2);
+  for (int k = 0; k < steps; ++k) seg_step(a.c, s, false);
+  seg_store(s, pos, vel, th);
+  for (int i = threadIdx.x; i < s.nn; i += blockDim.x) {
+    const int g = s.nodes[i];
+    dbg[2 * g] = 0.0f;
+    dbg[2 * g + 1] = 0.0f;
+  }
+}
+
+// Reverse segment from checkpoint slot `ck` over `steps` steps; adjoints
+// (a.xb / a.vb / a.thb) and the rest-angle adjoint (a.rest_bar) updated in place.
+__global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps, const float* ck,
+                                                         float* tape) {
+  extern __shared__ float sm[];
+  SegCtx s;
+  seg_init(a, s, sm, true);
+  const Consts& c = a.c;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int nn = s.nn, ne = s.ne, nb = s.nb;
+  const size_t N2 = 2 * (size_t)a.N;
+  // the filament's part of the tape: steps x (4 nn + ne)
+  const size_t per = 4 * (size_t)nn + ne;
+  float* tp = tape + (size_t)steps * (4 * (size_t)s.n0 + s.e0);
+  seg_load(s, ck, ck + N2, ck + 2 * N2);
+  for (int k = 0; k < steps; ++k) {
+    float* slot = tp + (size_t)k * per;
+    for (int i = tid; i < 2 * nn; i += bs) {
+      slot[i] = s.X[i];
+      slot[2 * nn + i] = s.V[i];
+    }
+    for (int e = tid; e < ne; e += bs) slot[4 * nn + e] = s.TH[e];
+    if (k + 1 < steps) seg_step(c, s, false);
+  }
+  for (int i = tid; i < nn; i += bs) {
+    const int g = s.nodes[i];
+    s.XB[2 * i] = a.xb[2 * g];
+    s.XB[2 * i + 1] = a.xb[2 * g + 1];
+    s.VB[2 * i] = a.vb[2 * g];
+    s.VB[2 * i + 1] = a.vb[2 * g + 1];
+  }
+  for (int e = tid; e < ne; e += bs) s.THB[e] = a.thb[s.edges[e]];
+  // the rest-angle adjoint continues the running sum, so the summation order
+  // does not depend on where the checkpoints fall
+  for (int b = tid; b < nb; b += bs) s.RB[b] = a.rest_bar[s.hinges[b]];
   __syncthreads();
   const float kst = c.k_st, ksh = c.k_sh;
-  for (int cyc = C - 1; cyc >= 0; --cyc) {
-    const float* Xc = Xs + (size_t)cyc * 2 * nn;
-    const float* THc = THs + (size_t)cyc * ne;
-    // position pass: per edge
-    for (int e = tid; e < ne; e += bs) {
-      const int sn = esrc[e], tn = etgt[e];
-      const float ds = fmax_nan((float)(ninc_off[sn + 1] - ninc_off[sn]), 1.0f);
-      const float dg = fmax_nan((float)(ninc_off[tn + 1] - ninc_off[tn]), 1.0f);
-      const float fbx = XB[2 * sn] / ds - XB[2 * tn] / dg;
-      const float fby = XB[2 * sn + 1] / ds - XB[2 * tn + 1] / dg;
-      const float rx = min_image(Xc[2 * tn], Xc[2 * sn], Wd);
-      const float ry = min_image(Xc[2 * tn + 1], Xc[2 * sn + 1], Hd);
-      const float l = a.rest_len[L.comp_edges[e0 + e]];
-      const float len = sqrtf(rx * rx + ry * ry);
-      const float ls = fmax_nan(len, 1e-8f);
-      const float hx = rx / ls, hy = ry / ls;
-      const float al = 1.0f - l / ls, bl = l / ls;
-      const float pr = hx * fbx + hy * fby;
-      const float sx = al * fbx + bl * pr * hx, sy = al * fby + bl * pr * hy;
-      FE[2 * e] = 0.5f * kst * ((1.0f - ksh) * sx + ksh * fbx);
-      FE[2 * e + 1] = 0.5f * kst * ((1.0f - ksh) * sy + ksh * fby);
-      double sd, cd;
-      sincos((double)THc[e], &sd, &cd);
-      THB[e] = THB[e] + (-0.5f * kst * ksh * l) * (-(float)sd * fbx + (float)cd * fby);
+  for (int k = steps - 1; k >= 0; --k) {
+    const float* slot = tp + (size_t)k * per;
+    for (int i = tid; i < 2 * nn; i += bs) {
+      s.X[i] = slot[i];
+      s.V[i] = slot[2 * nn + i];
     }
+    for (int e = tid; e < ne; e += bs) s.TH[e] = slot[4 * nn + e];
     __syncthreads();
-    // per node: gather of its edges' rb (+ at the edge's target, - at its source)
-    for (int i = tid; i < nn; i += bs) {
-      const int i0 = ninc_off[i], i1 = ninc_off[i + 1];
-      float ux = 0.0f, uy = 0.0f;
-      for (int q = i0; q < i1; ++q) {
-        const int code = L.node_inc[q];
-        const int e = code >> 1;
-        ux = ux + ((code & 1) ? FE[2 * e] : -FE[2 * e]);
-        uy = uy + ((code & 1) ? FE[2 * e + 1] : -FE[2 * e + 1]);
-      }
-      XB[2 * i] = XB[2 * i] + ux;
-      XB[2 * i + 1] = XB[2 * i + 1] + uy;
-    }
-    // angle pass: per hinge
-    for (int b = tid; b < nb; b += bs) {
-      const int ei = hin[b], eo = hout[b];
-      const float di = fmax_nan((float)(einc_off[ei + 1] - einc_off[ei]), 1.0f);
-      const float dout = fmax_nan((float)(einc_off[eo + 1] - einc_off[eo]), 1.0f);
-      const float corrb = 1.8f * (THB[ei] / di - THB[eo] / dout);
-      const float db = corrb * a.kb[L.comp_hinges[b0 + b]] * 0.5f;
-      DB[b] = db;
-      RB[b] = RB[b] - db;
-    }
+    seg_step(c, s, true);
+    // adjoint of the epilogue: pb = xb + vb / dt
+    for (int i = tid; i < 2 * nn; i += bs) s.XB[i] = s.XB[i] + s.VB[i] / c.dt;
     __syncthreads();
-    for (int e = tid; e < ne; e += bs) {
-      const int i0 = einc_off[e], i1 = einc_off[e + 1];
-      float d = 0.0f;
-      for (int q = i0; q < i1; ++q) {
-        const int code = L.edge_inc[q];
-        d = d + ((code & 1) ? DB[code >> 1] : -DB[code >> 1]);
+    for (int cyc = s.C - 1; cyc >= 0; --cyc) {
+      const float* Xc = s.Xs + (size_t)cyc * 2 * nn;
+      const float* THc = s.THs + (size_t)cyc * ne;
+      for (int e = tid; e < ne; e += bs) {
+        const int sn = s.esrc[e], tn = s.etgt[e];
+        const float ds = fmax_nan((float)(s.ninc_off[sn + 1] - s.ninc_off[sn]), 1.0f);
+        const float dg = fmax_nan((float)(s.ninc_off[tn + 1] - s.ninc_off[tn]), 1.0f);
+        const float fbx = s.XB[2 * sn] / ds - s.XB[2 * tn] / dg;
+        const float fby = s.XB[2 * sn + 1] / ds - s.XB[2 * tn + 1] / dg;
+        const float rx = min_image(Xc[2 * tn], Xc[2 * sn], s.Wd);
+        const float ry = min_image(Xc[2 * tn + 1], Xc[2 * sn + 1], s.Hd);
+        const float l = s.LE[e];
+        const float len = sqrtf(rx * rx + ry * ry);
+        const float ls = fmax_nan(len, 1e-8f);
+        const float hx = rx / ls, hy = ry / ls;
+        const float al = 1.0f - l / ls, bl = l / ls;
+        const float pr = hx * fbx + hy * fby;
+        const float sx = al * fbx + bl * pr * hx, sy = al * fby + bl * pr * hy;
+        s.FE[2 * e] = 0.5f * kst * ((1.0f - ksh) * sx + ksh * fbx);
+        s.FE[2 * e + 1] = 0.5f * kst * ((1.0f - ksh) * sy + ksh * fby);
+        double sd, cd;
+        sincos((double)THc[e], &sd, &cd);
+        s.THB[e] = s.THB[e] + (-0.5f * kst * ksh * l) * (-(float)sd * fbx + (float)cd * fby);
       }
-      THB[e] = THB[e] + d;
+      __syncthreads();
+      // per node: + rb at the edge's target, - rb at its source
+      for (int i = tid; i < nn; i += bs) {
+        const int i0 = s.ninc_off[i], i1 = s.ninc_off[i + 1];
+        float ux = 0.0f, uy = 0.0f;
+        for (int q = i0; q < i1; ++q) {
+          const int code = s.node_inc[q];
+          const int e = code >> 1;
+          ux = ux + ((code & 1) ? s.FE[2 * e] : -s.FE[2 * e]);
+          uy = uy + ((code & 1) ? s.FE[2 * e + 1] : -s.FE[2 * e + 1]);
+        }
+        s.XB[2 * i] = s.XB[2 * i] + ux;
+        s.XB[2 * i + 1] = s.XB[2 * i + 1] + uy;
+      }
+      // angle pass, per hinge
+      for (int b = tid; b < nb; b += bs) {
+        const int ei = s.hin[b], eo = s.hout[b];
+        const float di = fmax_nan((float)(s.einc_off[ei + 1] - s.einc_off[ei]), 1.0f);
+        const float dout = fmax_nan((float)(s.einc_off[eo + 1] - s.einc_off[eo]), 1.0f);
+        const float corrb = 1.8f * (s.THB[ei] / di - s.THB[eo] / dout);
+        const float db = corrb * s.KB[b] * 0.5f;
+        s.DB[b] = db;
+        s.RB[b] = s.RB[b] - db;
+      }
+      __syncthreads();
+      for (int e = tid; e < ne; e += bs) {
+        const int i0 = s.einc_off[e], i1 = s.einc_off[e + 1];
+        float d = 0.0f;
+        for (int q = i0; q < i1; ++q) {
+          const int code = s.edge_inc[q];
+          d = d + ((code & 1) ? s.DB[code >> 1] : -s.DB[code >> 1]);
+        }
+        s.THB[e] = s.THB[e] + d;
+      }
+      __syncthreads();
+    }
+    // adjoint of the prologue: xb = pb - vb / dt, vb = pb d dt
+    for (int i = tid; i < 2 * nn; i += bs) {
+      const float pb = s.XB[i];
+      s.XB[i] = pb - s.VB[i] / c.dt;
+      s.VB[i] = (pb * c.damping) * c.dt;
     }
     __syncthreads();
   }
-  // adjoint of the prologue
   for (int i = tid; i < nn; i += bs) {
-    const int g = L.comp_nodes[n0 + i];
-    const float vbx = a.vb[2 * g], vby = a.vb[2 * g + 1];
-    a.xb[2 * g] = XB[2 * i] - vbx / c.dt;
-    a.xb[2 * g + 1] = XB[2 * i + 1] - vby / c.dt;
-    a.vb[2 * g] = (XB[2 * i] * c.damping) * c.dt;
-    a.vb[2 * g + 1] = (XB[2 * i + 1] * c.damping) * c.dt;
+    const int g = s.nodes[i];
+    a.xb[2 * g] = s.XB[2 * i];
+    a.xb[2 * g + 1] = s.XB[2 * i + 1];
+    a.vb[2 * g] = s.VB[2 * i];
+    a.vb[2 * g + 1] = s.VB[2 * i + 1];
   }
-  for (int e = tid; e < ne; e += bs) a.thb[L.comp_edges[e0 + e]] = THB[e];
-  for (int b = tid; b < nb; b += bs) {
-    const int gb = L.comp_hinges[b0 + b];
-    a.rest_bar[gb] = a.rest_bar[gb] + RB[b];
-  }
+  for (int e = tid; e < ne; e += bs) a.thb[s.edges[e]] = s.THB[e];
+  for (int b = tid; b < nb; b += bs) a.rest_bar[s.hinges[b]] = s.RB[b];
 }
 
 }  // namespace
 
-size_t rod_adjoint_lds_bytes(int cycles, int nn, int ne, int nb) {
-  return sizeof(float) * ((size_t)cycles * (2 * (size_t)nn + ne) + 4 * (size_t)nn + 4 * (size_t)ne +
-                          3 * (size_t)nb);
+size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse) {
+  size_t f = 6 * (size_t)nn + 4 * (size_t)ne + 3 * (size_t)nb;
+  if (reverse) f += (size_t)cycles * (2 * (size_t)nn + ne) + 4 * (size_t)nn + ne + 2 * (size_t)nb;
+  return f * sizeof(float);
 }
 
-hipError_t launch_rod_adjoint(const RodAdjArgs& a, size_t lds_bytes, hipStream_t s) {
+hipError_t launch_rod_segment(const RodAdjArgs& a, bool reverse, int steps, float* ck,
+                              float* tape_or_dbg, size_t lds_bytes, hipStream_t s) {
   if (a.L.num_comp == 0) return hipSuccess;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rod_adjoint),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  const void* fn = reverse ? reinterpret_cast<const void*>(&k_rod_adj_segment)
+                           : reinterpret_cast<const void*>(&k_rod_fwd_segment);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_rod_adjoint, dim3(a.L.num_comp), dim3(256), lds_bytes, s, a);
+  if (reverse)
+    hipLaunchKernelGGL(k_rod_adj_segment, dim3(a.L.num_comp), dim3(64), lds_bytes, s, a, steps,
+                       (const float*)ck, tape_or_dbg);
+  else
+    hipLaunchKernelGGL(k_rod_fwd_segment, dim3(a.L.num_comp), dim3(64), lds_bytes, s, a, steps,
+                       ck, tape_or_dbg);
   return hipGetLastError();
 }
 

@@ -1427,30 +1427,6 @@ mc_status check_rod_adjoint(const mc_params* q) {
   return MC_OK;
 }
 
-// Copies position, velocity, theta of `from` into the packed slot `to`
-// (2N + 2N + E floats), or back when `unpack`.
-mc_status pack_state(const mc_plan* p, float* slot, float* pos, float* vel, float* th, bool unpack,
-                     hipStream_t s) {
-  const size_t n2 = 2 * (size_t)p->N * sizeof(float), ne = (size_t)p->E * sizeof(float);
-  auto cp = [&](float* dst, float* src, size_t b) -> mc_status {
-    if (b) HIPCHK(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, s));
-    return MC_OK;
-  };
-  float* sp = slot;
-  float* sv = slot + 2 * (size_t)p->N;
-  float* st = slot + 4 * (size_t)p->N;
-  if (unpack) {
-    MCCHK(cp(pos, sp, n2));
-    MCCHK(cp(vel, sv, n2));
-    MCCHK(cp(th, st, ne));
-  } else {
-    MCCHK(cp(sp, pos, n2));
-    MCCHK(cp(sv, vel, n2));
-    MCCHK(cp(st, th, ne));
-  }
-  return MC_OK;
-}
-
 }  // namespace
 
 extern "C" {
@@ -1467,6 +1443,11 @@ mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state*
   if (num_steps < 1) return fail(MC_ERR_BAD_SHAPE, "num_steps < 1");
   if (checkpoint_every < 1) return fail(MC_ERR_BAD_SHAPE, "checkpoint_every < 1");
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
+  const size_t lds = rod_segment_lds_bytes(q->cycles_per_step, p->comp_max_nn, p->comp_max_ne,
+                                           p->comp_max_nb, true);
+  if (lds + 1024 > (size_t)p->lds_max)
+    return fail(MC_ERR_UNSUPPORTED, "a filament of %d nodes with %d cycles needs %zu B of LDS for "
+                "the adjoint (device: %d B)", p->comp_max_nn, q->cycles_per_step, lds, p->lds_max);
   hipStream_t s = (hipStream_t)stream;
   p->adj_ok = false;
   const size_t S = 4 * (size_t)p->N + (size_t)p->E;
@@ -1477,14 +1458,23 @@ mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state*
     MCCHK(p->alloc(&p->adj_ckpt, (size_t)nck * S, "adjoint checkpoints"));
     p->adj_ckpt_cap = (size_t)nck * S;
   }
-  const Consts c = derive(p, q, dt);
-  MCCHK(ensure_geo(p, q));
-  mc_state io = *st;
-  for (int64_t t = 0; t < num_steps; ++t) {
-    if (t % checkpoint_every == 0)
-      MCCHK(pack_state(p, p->adj_ckpt + (size_t)(t / checkpoint_every) * S, io.position,
-                       io.velocity, io.theta, false, s));
-    MCCHK(enqueue_step(p, q, c, &io, &io, ed, nullptr, s));
+  RodAdjArgs a{};
+  a.c = derive(p, q, dt);
+  a.bd = p->bd;
+  a.L = p->L;
+  a.N = p->N;
+  a.pos = st->position;
+  a.vel = st->velocity;
+  a.theta = st->theta;
+  a.rest_len = ed->rest_lengths;
+  a.rest_ang = ed->bending_rest_angles;
+  a.kb = ed->bending_stiffness;
+  const size_t lds_f = rod_segment_lds_bytes(q->cycles_per_step, p->comp_max_nn, p->comp_max_ne,
+                                             p->comp_max_nb, false);
+  for (int64_t seg = 0; seg < nck; ++seg) {
+    const int len = (int)std::min<int64_t>(checkpoint_every, num_steps - seg * checkpoint_every);
+    HIPCHK(launch_rod_segment(a, false, len, p->adj_ckpt + (size_t)seg * S, st->debug_vector,
+                              lds_f, s));
   }
   p->adj_ok = true;
   p->adj_steps = num_steps;
@@ -1508,11 +1498,8 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
     return fail(MC_ERR_INVALID, "params, dt and edge arrays must be those of the recorded rollout");
   if (!position_bar || (p->B > 0 && !rest_angle_bar))
     return fail(MC_ERR_INVALID, "position_bar / rest_angle_bar is NULL");
-  const size_t lds = rod_adjoint_lds_bytes(q->cycles_per_step, p->comp_max_nn, p->comp_max_ne,
-                                           p->comp_max_nb);
-  if (lds + 1024 > (size_t)p->lds_max)
-    return fail(MC_ERR_UNSUPPORTED, "a filament of %d nodes with %d cycles needs %zu B of LDS for "
-                "the adjoint (device: %d B)", p->comp_max_nn, q->cycles_per_step, lds, p->lds_max);
+  const size_t lds = rod_segment_lds_bytes(q->cycles_per_step, p->comp_max_nn, p->comp_max_ne,
+                                           p->comp_max_nb, true);
   hipStream_t s = (hipStream_t)stream;
   const int N = p->N, E = p->E, B = p->B;
   const size_t S = 4 * (size_t)N + (size_t)E;
@@ -1522,28 +1509,21 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
     MCCHK(p->alloc(&p->adj_tape, (size_t)every * S, "adjoint tape"));
     p->adj_tape_cap = (size_t)every * S;
   }
-  if (!p->adj_work) MCCHK(p->alloc(&p->adj_work, 2 * S + 2 * (size_t)N, "adjoint state"));
+  if (!p->adj_work) MCCHK(p->alloc(&p->adj_work, S, "adjoint state"));
   float* xb = p->adj_work;
   float* vb = xb + 2 * (size_t)N;
   float* thb = vb + 2 * (size_t)N;
-  float* wpos = p->adj_work + S;
-  float* wvel = wpos + 2 * (size_t)N;
-  float* wth = wvel + 2 * (size_t)N;
-  float* wdbg = wth + E;
   const size_t n2 = 2 * (size_t)N * sizeof(float);
   HIPCHK(hipMemcpyAsync(xb, position_bar, n2, hipMemcpyDeviceToDevice, s));
   if (velocity_bar) HIPCHK(hipMemcpyAsync(vb, velocity_bar, n2, hipMemcpyDeviceToDevice, s));
   else HIPCHK(hipMemsetAsync(vb, 0, n2, s));
   if (E) HIPCHK(hipMemsetAsync(thb, 0, (size_t)E * sizeof(float), s));
   if (B) HIPCHK(hipMemsetAsync(rest_angle_bar, 0, (size_t)B * sizeof(float), s));
-  const Consts c = derive(p, q, dt);
-  MCCHK(ensure_geo(p, q));
-  // working state for the recomputation (lattice arrays unused: fluid off)
-  mc_state w{wpos, wvel, wdbg, wth, p->f_tmp, p->f_tmp, p->f_tmp};
   RodAdjArgs a{};
-  a.c = c;
+  a.c = derive(p, q, dt);
   a.bd = p->bd;
   a.L = p->L;
+  a.N = N;
   a.rest_len = ed->rest_lengths;
   a.rest_ang = ed->bending_rest_angles;
   a.kb = ed->bending_stiffness;
@@ -1553,20 +1533,8 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
   a.rest_bar = rest_angle_bar;
   const int64_t nck = (T + every - 1) / every;
   for (int64_t seg = nck - 1; seg >= 0; --seg) {
-    const int64_t t0 = seg * every;
-    const int len = (int)std::min<int64_t>(every, T - t0);
-    MCCHK(pack_state(p, p->adj_ckpt + (size_t)seg * S, wpos, wvel, wth, true, s));
-    for (int k = 0; k < len; ++k) {
-      MCCHK(pack_state(p, p->adj_tape + (size_t)k * S, wpos, wvel, wth, false, s));
-      if (k + 1 < len) MCCHK(enqueue_step(p, q, c, &w, &w, ed, nullptr, s));
-    }
-    for (int k = len - 1; k >= 0; --k) {
-      float* slot = p->adj_tape + (size_t)k * S;
-      a.pos = slot;
-      a.vel = slot + 2 * (size_t)N;
-      a.theta = slot + 4 * (size_t)N;
-      HIPCHK(launch_rod_adjoint(a, lds, s));
-    }
+    const int len = (int)std::min<int64_t>(every, T - seg * every);
+    HIPCHK(launch_rod_segment(a, true, len, p->adj_ckpt + (size_t)seg * S, p->adj_tape, lds, s));
   }
   if (position0_bar) HIPCHK(hipMemcpyAsync(position0_bar, xb, n2, hipMemcpyDeviceToDevice, s));
   if (velocity0_bar) HIPCHK(hipMemcpyAsync(velocity0_bar, vb, n2, hipMemcpyDeviceToDevice, s));

@@ -181,19 +181,25 @@ hipError_t launch_tick(long long* t, hipStream_t s);
 hipError_t launch_set_tick(long long* t, long long value, hipStream_t s);
 size_t rod_scratch_floats(int nn, int ne, int nb);
 size_t rod_lds_bytes(int nn, int ne, int nb, int nni, int nei);
-// rod adjoint (rod-only PBD step): one step's reverse pass per component,
-// adjoints updated in place, rest-angle adjoint accumulated
+// rod-only PBD step, one workgroup per filament running a whole checkpoint
+// segment: forward (state advanced in place, start state to the checkpoint
+// slot) or reverse (recompute from the checkpoint into the tape, then the
+// adjoint, updated in place, rest-angle adjoint accumulated)
 struct RodAdjArgs {
   Consts c;
   BatchDev bd;
   RodLayoutDev L;
-  const float *pos, *vel, *theta;              // the step's input state
+  int N;
+  const float *pos, *vel, *theta;              // forward: the state (in place)
   const float *rest_len, *rest_ang, *kb;
-  float *xb, *vb, *thb;                        // adjoints: step outputs in, inputs out
+  float *xb, *vb, *thb;                        // adjoints: segment end in, start out
   float* rest_bar;                             // (B) += d loss / d rest angles
 };
-size_t rod_adjoint_lds_bytes(int cycles, int nn, int ne, int nb);
-hipError_t launch_rod_adjoint(const RodAdjArgs& a, size_t lds_bytes, hipStream_t s);
+size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse);
+// forward: ck = checkpoint slot written, tape_or_dbg = debug_vector (N, 2);
+// reverse: ck = checkpoint slot read, tape_or_dbg = tape (steps x (4N + E))
+hipError_t launch_rod_segment(const RodAdjArgs& a, bool reverse, int steps, float* ck,
+                              float* tape_or_dbg, size_t lds_bytes, hipStream_t s);
 // immersed boundary (solved per interaction cluster)
 struct IBWork {
   int N, C;                    // nodes, filament components
