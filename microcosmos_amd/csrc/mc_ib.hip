@@ -816,7 +816,8 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
 
 // ---- wide solve ----------------------------------------------------------
 // Clusters of more than IB_WIDE_MIN nodes are solved together, as one
-// problem, by the last wide_wg workgroups of the cluster launch.  Clusters
+// problem, by the wide_wg workgroups of the launch after the cluster solve
+// (the cluster solve skips them).  Clusters
 // share no cell, so merging them changes no fold: a cell's sum is still over
 // its covering nodes in ascending node id and a node's gather over its taps
 // in tap order.  Entries are identified by e = n * K2 + t with the global
@@ -839,8 +840,8 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
 // issues an agent-scope release fence, adds to the barrier counter, waits
 // for all wide_wg arrivals and issues an agent-scope acquire fence; a second
 // workgroup barrier releases the other threads.  The workgroups are
-// co-resident: there are at most 64 of them, each needing one CU, the device
-// has 256 CUs, and no workgroup of the launch waits on anything else.  The
+// co-resident: there are at most 64 of them, each needing at most one CU, and
+// the device has 256 CUs.  The
 // wait is bounded nevertheless: after ~2^22 polls it records the timeout in
 // WideCtl::err and proceeds, so the kernel always ends.
 
@@ -998,7 +999,8 @@ __device__ __forceinline__ float2 wide_corr(const Consts& c, const float* __rest
 }
 
 template <int K>
-__device__ void cluster_wide(const ClusterArgs& a, int h, int H, int* wsum, int* s_off) {
+__device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H, int* wsum,
+                                             int* s_off) {
   constexpr int K2 = K * K;
   const Consts& c = a.c;
   WideCtl* ctl = a.wctl;
@@ -1195,15 +1197,8 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int wsum[16];
   __shared__ int s_ns, s_seg;
-  __shared__ int s_off[2];
   constexpr int K2 = K * K;
   const int root = blockIdx.x;
-  if (root >= a.C) {
-    // one of the wide solve's workgroups
-    if (__hip_atomic_load(&a.wctl->wide, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
-    cluster_wide<K>(a, root - a.C, a.wide_wg, wsum, &s_off[0]);
-    return;
-  }
   if (a.label[root] != root) return;
   const InstGeo& gi = a.bd.geo[a.bd.comp_inst[root]];  // a cluster never spans instances
   const int nq = a.cl_cnt[root];
@@ -1309,6 +1304,17 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
     cluster_small<K>(a, gi, smem, nodes, nq, ns, wsum, &s_seg);
   else
     cluster_general<K>(a, gi, smem, avail, G, X, nodes, nq, ns, wsum, &s_seg);
+}
+
+// The wide solve: wide_wg workgroups launched after the cluster solve (a
+// kernel of its own, so that its register demand does not reach the cluster
+// kernel); they return at once when no cluster is wide.
+template <int K>
+__global__ void __launch_bounds__(cluster_threads<K>()) k_ib_wide(ClusterArgs a) {
+  __shared__ int wsum[16];
+  __shared__ int s_off[2];
+  if (__hip_atomic_load(&a.wctl->wide, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+  cluster_wide<K>(a, blockIdx.x, gridDim.x, wsum, s_off);
 }
 
 __global__ void k_ib_reset(int HW, int C, int* __restrict__ owner, int* __restrict__ parent,
@@ -1434,12 +1440,18 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   a.wscratch = w.wide_scratch;
   a.wcount = w.wide_count;
   a.wctl = w.wide_ctl;
-#define L_CLUSTER(KK)                                                                           \
-  hipLaunchKernelGGL(k_ib_cluster<KK>, dim3(C + w.wide_wg), dim3(cluster_threads<KK>()), w.lds_bytes, \
-                     s, a)
+#define L_CLUSTER(KK) \
+  hipLaunchKernelGGL(k_ib_cluster<KK>, dim3(C), dim3(cluster_threads<KK>()), w.lds_bytes, s, a)
   MC_KSWITCH(L_CLUSTER);
 #undef L_CLUSTER
   e = hipGetLastError();
+  if (e == hipSuccess && w.wide_wg > 0) {
+#define L_WIDE(KK) \
+  hipLaunchKernelGGL(k_ib_wide<KK>, dim3(w.wide_wg), dim3(cluster_threads<KK>()), 0, s, a)
+    MC_KSWITCH(L_WIDE);
+#undef L_WIDE
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) w.dirty = false;
   return e;
 }

@@ -223,9 +223,9 @@ struct IBWork {
   long long cells;             // lattice cells over all instances
   float *force;                // (2, Hf, Wf) per instance: IB force, valid on the support U
   uint32_t* mask;              // U bits by flat lattice cell index, per instance
-  // wide clusters (more than IB_WIDE_MIN nodes) are solved together by
-  // wide_wg extra workgroups of the cluster launch (0: no instance is large
-  // enough to form one)
+  // wide clusters (more than IB_WIDE_MIN nodes) are solved together by a
+  // launch of wide_wg workgroups after the cluster solve (0: no instance is
+  // large enough to form one)
   int wide_wg;
   unsigned char* wide_scratch; // ib_wide_scratch_bytes(N, kmax)
   int* wide_count;             // per lattice cell entry counter (0 between calls)
