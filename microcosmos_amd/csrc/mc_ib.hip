@@ -840,7 +840,7 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
 // issues an agent-scope release fence, adds to the barrier counter, waits
 // for all wide_wg arrivals and issues an agent-scope acquire fence; a second
 // workgroup barrier releases the other threads.  The workgroups are
-// co-resident: there are at most 64 of them, each needing at most one CU, and
+// co-resident: there are at most 32 of them, each needing at most one CU, and
 // the device has 256 CUs.  The
 // wait is bounded nevertheless: after ~2^22 polls it records the timeout in
 // WideCtl::err and proceeds, so the kernel always ends.
@@ -1309,8 +1309,10 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
 // The wide solve: wide_wg workgroups launched after the cluster solve (a
 // kernel of its own, so that its register demand does not reach the cluster
 // kernel); they return at once when no cluster is wide.
+constexpr int WIDE_THREADS = 256;  // small workgroups: a launch with nothing to do ends quickly
+
 template <int K>
-__global__ void __launch_bounds__(cluster_threads<K>()) k_ib_wide(ClusterArgs a) {
+__global__ void __launch_bounds__(WIDE_THREADS) k_ib_wide(ClusterArgs a) {
   __shared__ int wsum[16];
   __shared__ int s_off[2];
   if (__hip_atomic_load(&a.wctl->wide, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
@@ -1447,7 +1449,7 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   e = hipGetLastError();
   if (e == hipSuccess && w.wide_wg > 0) {
 #define L_WIDE(KK) \
-  hipLaunchKernelGGL(k_ib_wide<KK>, dim3(w.wide_wg), dim3(cluster_threads<KK>()), 0, s, a)
+  hipLaunchKernelGGL(k_ib_wide<KK>, dim3(w.wide_wg), dim3(WIDE_THREADS), 0, s, a)
     MC_KSWITCH(L_WIDE);
 #undef L_WIDE
     e = hipGetLastError();

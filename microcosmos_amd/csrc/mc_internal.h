@@ -11,7 +11,7 @@ constexpr int IB_KMAX = 6;                 // largest supported ibm_kernel_size
 constexpr int SORT_LDS_CAP = 2048;         // bitonic sort capacity in LDS (ids)
 constexpr int FFT_MAX_RADICES = 16;        // radix stages of one FFT length
 constexpr int IB_WIDE_MIN = 1024;          // clusters with more nodes take the wide solve
-constexpr int IB_WIDE_WG = 64;             // workgroups of the wide solve
+constexpr int IB_WIDE_WG = 32;             // workgroups of the wide solve
 
 // Float constants derived from ConstraintSolverConfig exactly as the reference
 // derives them (Python double arithmetic, then a float32 weak scalar).  Shared
