@@ -112,8 +112,9 @@ def serving(n_req, dev, lanes, chunk, max_batch, plain=True):
     from microcosmos_amd.serve import Request, Server, fifo_report, latency_report
     from tests.helpers import device_state
     trace = W.serving_trace(n_req, seed=4)
-    cfg = replace(PBD_SCHEME, **trace[0].workload.config_overrides)
-    dt = trace[0].workload.dt
+    first = next(it for it in trace if not it.gradient)  # the rollout requests' settings
+    cfg = replace(PBD_SCHEME, **first.workload.config_overrides)
+    dt = first.workload.dt
 
     def requests(items):
         out = []
