@@ -202,6 +202,7 @@ __global__ void __launch_bounds__(1024) k_ib_cluster_build(
     // atomic stores: the counters are later updated by device-scope atomics
     __hip_atomic_store(&wctl->bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&wctl->wide, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&wctl->wide_nodes, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&wctl->nseg, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&wctl->nq, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -237,7 +238,10 @@ __global__ void __launch_bounds__(1024) k_ib_cluster_build(
     int tot, tot2;
     const int inc = block_inclusive_scan(v, wsum, &tot);
     const int inc2 = block_inclusive_scan(v2, wsum2, &tot2);
-    if (wctl && root && v > IB_WIDE_MIN) atomicAdd(&wctl->wide, 1);
+    if (wctl && root && v > IB_WIDE_MIN) {
+      atomicAdd(&wctl->wide, 1);
+      atomicAdd(&wctl->wide_nodes, v);
+    }
     if (c < C) {
       cl_node_off[c] = carry + inc - v;
       cl_mem_off[c] = carry2 + inc2 - v2;
@@ -840,8 +844,8 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
 // issues an agent-scope release fence, adds to the barrier counter, waits
 // for all wide_wg arrivals and issues an agent-scope acquire fence; a second
 // workgroup barrier releases the other threads.  The workgroups are
-// co-resident: there are at most 32 of them, each needing at most one CU, and
-// the device has 256 CUs.  The
+// co-resident: at most 256 workgroups of 256 threads (77 VGPRs, no dynamic
+// LDS) take part, several fit on one CU, and the device has 256 CUs.  The
 // wait is bounded nevertheless: after ~2^22 polls it records the timeout in
 // WideCtl::err and proceeds, so the kernel always ends.
 
@@ -1311,12 +1315,18 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
 // kernel); they return at once when no cluster is wide.
 constexpr int WIDE_THREADS = 256;  // small workgroups: a launch with nothing to do ends quickly
 
+// The number of participating workgroups follows the problem: one per
+// IB_WIDE_NODES_PER_WG wide nodes (the others return at once), so small
+// problems pay for few barrier arrivals and large ones get the whole launch.
 template <int K>
 __global__ void __launch_bounds__(WIDE_THREADS) k_ib_wide(ClusterArgs a) {
   __shared__ int wsum[16];
   __shared__ int s_off[2];
-  if (__hip_atomic_load(&a.wctl->wide, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
-  cluster_wide<K>(a, blockIdx.x, gridDim.x, wsum, s_off);
+  const int nodes = __hip_atomic_load(&a.wctl->wide_nodes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (nodes == 0) return;
+  const int H = min((int)gridDim.x, max(1, (nodes + IB_WIDE_NODES_PER_WG - 1) / IB_WIDE_NODES_PER_WG));
+  if ((int)blockIdx.x >= H) return;
+  cluster_wide<K>(a, blockIdx.x, H, wsum, s_off);
 }
 
 __global__ void k_ib_reset(int HW, int C, int* __restrict__ owner, int* __restrict__ parent,

@@ -11,7 +11,8 @@ constexpr int IB_KMAX = 6;                 // largest supported ibm_kernel_size
 constexpr int SORT_LDS_CAP = 2048;         // bitonic sort capacity in LDS (ids)
 constexpr int FFT_MAX_RADICES = 16;        // radix stages of one FFT length
 constexpr int IB_WIDE_MIN = 1024;          // clusters with more nodes take the wide solve
-constexpr int IB_WIDE_WG = 32;             // workgroups of the wide solve
+constexpr int IB_WIDE_WG = 256;            // workgroups of the wide solve launch (at most)
+constexpr int IB_WIDE_NODES_PER_WG = 192;  // wide nodes per participating workgroup
 
 // Float constants derived from ConstraintSolverConfig exactly as the reference
 // derives them (Python double arithmetic, then a float32 weak scalar).  Shared
@@ -231,12 +232,13 @@ struct IBWork {
   int* wide_count;             // per lattice cell entry counter (0 between calls)
   struct WideCtl* wide_ctl;
 };
-// Control block of the wide solve: grid-barrier counter, wide-cluster count,
-// segment and node counters, a flag set if a barrier wait ever timed out, and the
+// Control block of the wide solve: grid-barrier counter, wide-cluster and
+// wide-node counts, segment and node counters, a flag set if a barrier wait
+// ever timed out, and the
 // per-workgroup partial sums of the grid scans.
 struct WideCtl {
   unsigned int bar;
-  int wide, nseg, nq, err;
+  int wide, wide_nodes, nseg, nq, err;
   int bsum[IB_WIDE_WG];
 };
 size_t ib_wide_scratch_bytes(int N, int kmax);
