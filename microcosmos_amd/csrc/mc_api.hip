@@ -902,10 +902,18 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
   // the wide solve exists when an instance can hold a cluster of more than
   // IB_WIDE_MIN nodes
   w.wide_wg = 0;
+  w.wide_min = IB_WIDE_MIN;
+  {
+    // MC_IB_WIDE_MIN overrides the cluster size above which the wide solve
+    // takes over (64..IB_CLUSTER_MAX), for tuning
+    const char* env = getenv("MC_IB_WIDE_MIN");
+    const int v = env ? atoi(env) : 0;
+    if (v >= 64 && v <= IB_CLUSTER_MAX) w.wide_min = v;
+  }
   for (const InstHost& h : p->inst)
-    if (h.nodes > IB_WIDE_MIN) w.wide_wg = IB_WIDE_WG;
+    if (h.nodes > w.wide_min) w.wide_wg = IB_WIDE_WG;
   if (w.wide_wg > 0) {
-    // MC_IB_WIDE_WG (1..IB_WIDE_WG) overrides the workgroup count, for tuning
+    // MC_IB_WIDE_WG (1..IB_WIDE_WG) caps the workgroup count, for tuning
     const char* env = getenv("MC_IB_WIDE_WG");
     const int v = env ? atoi(env) : 0;
     if (v >= 1 && v <= IB_WIDE_WG) w.wide_wg = v;

@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(1024) k_ib_cluster_build(
     int* __restrict__ nlinks, int* __restrict__ gparent, int* __restrict__ label,
     int* __restrict__ cl_cnt, int* __restrict__ cl_node_off, int* __restrict__ cl_mem_cnt,
     int* __restrict__ cl_mem_off, int* __restrict__ cl_members, long long* __restrict__ tick,
-    WideCtl* __restrict__ wctl) {
+    WideCtl* __restrict__ wctl, int wide_min) {
   extern __shared__ __align__(16) int sm[];
   __shared__ int wsum[16];
   __shared__ int wsum2[16];
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(1024) k_ib_cluster_build(
     int tot, tot2;
     const int inc = block_inclusive_scan(v, wsum, &tot);
     const int inc2 = block_inclusive_scan(v2, wsum2, &tot2);
-    if (wctl && root && v > IB_WIDE_MIN) {
+    if (wctl && root && v > wide_min) {
       atomicAdd(&wctl->wide, 1);
       atomicAdd(&wctl->wide_nodes, v);
     }
@@ -281,7 +281,7 @@ struct ClusterArgs {
   size_t bpn;        // global scratch bytes per node
   size_t lds_bytes;  // dynamic LDS of the launch (a multiple of 16)
   // wide solve
-  int N, C, wide_wg;
+  int N, C, wide_wg, wide_min;
   const int* comp_of_node;
   unsigned char* wscratch;
   int* wcount;  // per lattice cell entry counter (0 between calls)
@@ -1025,7 +1025,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
   // from wavefront-aggregated counters)
   for (int nb = h * bs + (tid & ~63); nb < a.N; nb += gs) {
     const int n = nb + (tid & 63);
-    const bool wide = n < a.N && a.cl_cnt[a.label[a.comp_of_node[n]]] > IB_WIDE_MIN;
+    const bool wide = n < a.N && a.cl_cnt[a.label[a.comp_of_node[n]]] > a.wide_min;
     const unsigned long long won = wide ? a.st_won[n] : 0ull;
     const int slot = wave_fetch_add(&ctl->nq, wide ? 1 : 0);
     int seg = wave_fetch_add(&ctl->nseg, __popcll(won));
@@ -1206,7 +1206,7 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
   if (a.label[root] != root) return;
   const InstGeo& gi = a.bd.geo[a.bd.comp_inst[root]];  // a cluster never spans instances
   const int nq = a.cl_cnt[root];
-  if (a.wide_wg > 0 && nq > IB_WIDE_MIN) return;  // solved by the wide workgroups
+  if (a.wide_wg > 0 && nq > a.wide_min) return;  // solved by the wide workgroups
   const int ne = nq * K2;
   const int tid = threadIdx.x, bs = blockDim.x;
   CLayout G;
@@ -1412,7 +1412,7 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   hipLaunchKernelGGL(k_ib_cluster_build, dim3(1), dim3(1024), w.lds_bytes, s, C,
                      (int)(w.lds_bytes / sizeof(int)), w.comp_node_off, w.links, w.nlinks,
                      w.uf_parent, w.label, w.cl_cnt, w.cl_node_off, w.cl_mem_cnt, w.cl_mem_off,
-                     w.cl_members, tick, w.wide_wg > 0 ? w.wide_ctl : nullptr);
+                     w.cl_members, tick, w.wide_wg > 0 ? w.wide_ctl : nullptr, w.wide_min);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   ClusterArgs a;
@@ -1448,6 +1448,7 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   a.N = N;
   a.C = C;
   a.wide_wg = w.wide_wg;
+  a.wide_min = w.wide_min;
   a.comp_of_node = w.comp_of_node;
   a.wscratch = w.wide_scratch;
   a.wcount = w.wide_count;

@@ -10,7 +10,8 @@ namespace mc {
 constexpr int IB_KMAX = 6;                 // largest supported ibm_kernel_size
 constexpr int SORT_LDS_CAP = 2048;         // bitonic sort capacity in LDS (ids)
 constexpr int FFT_MAX_RADICES = 16;        // radix stages of one FFT length
-constexpr int IB_WIDE_MIN = 1024;          // clusters with more nodes take the wide solve
+constexpr int IB_WIDE_MIN = 512;           // clusters with more nodes take the wide solve
+constexpr int IB_CLUSTER_MAX = 1024;       // largest cluster the per-cluster solve can take
 constexpr int IB_WIDE_WG = 256;            // workgroups of the wide solve launch (at most)
 constexpr int IB_WIDE_NODES_PER_WG = 192;  // wide nodes per participating workgroup
 
@@ -228,6 +229,7 @@ struct IBWork {
   // launch of wide_wg workgroups after the cluster solve (0: no instance is
   // large enough to form one)
   int wide_wg;
+  int wide_min;                // clusters with more nodes take the wide solve
   unsigned char* wide_scratch; // ib_wide_scratch_bytes(N, kmax)
   int* wide_count;             // per lattice cell entry counter (0 between calls)
   struct WideCtl* wide_ctl;

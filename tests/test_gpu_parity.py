@@ -168,15 +168,16 @@ def test_ib_lbm_kernel_sizes(dev, K):
 
 
 def test_ib_lbm_crowded_cluster(dev):
-    # 960 nodes on 64^2: the stencils chain every filament into one cluster
-    # whose working set exceeds the LDS, so the solve runs from global memory
-    _say("IB+LBM single call, one 960-node cluster")
-    _ib_case(dev, W.small(grid=(64, 64), n_fil=24, n_nodes=40, seed=3), 4)
+    # 480 nodes on 40^2: the stencils chain every filament into one cluster
+    # whose working set exceeds the LDS (but below the wide-solve size), so
+    # the per-cluster solve runs from global memory
+    _say("IB+LBM single call, one 480-node cluster")
+    _ib_case(dev, W.small(grid=(40, 40), n_fil=12, n_nodes=40, seed=3), 4)
 
 
 @pytest.mark.parametrize("K", [4, 6])
 def test_ib_lbm_wide_cluster(dev, K):
-    # 1920 nodes on 64^2 chain into one cluster of more than 1024 nodes: it
+    # 1920 nodes on 64^2 chain into one cluster of more than 512 nodes: it
     # is solved by the wide workgroups with grid barriers
     _say(f"IB+LBM single call, one 1920-node cluster (wide solve), K={K}")
     _ib_case(dev, W.small(grid=(64, 64), n_fil=48, n_nodes=40, seed=3), K)
