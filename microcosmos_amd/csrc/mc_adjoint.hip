@@ -9,11 +9,10 @@
 //   * forward segment: K steps of the rod-only step with the forward rod
 //     kernel's arithmetic (bit-identical results), the segment's start state
 //     written to the checkpoint slot;
-//   * reverse segment: the K steps recomputed from the checkpoint (each
-//     step's start state kept in the filament's part of a global tape), then
-//     step by step backwards: the step's constraint cycles are recomputed in
-//     LDS, keeping every cycle's positions (before the position pass) and
-//     angles (after the angle pass), and run backwards:
+//   * reverse segment: the K steps recomputed once from the checkpoint, each
+//     cycle's per-edge minimum-image vector and director (cos, sin) written
+//     to the filament's part of a global tape, then the steps' cycles run
+//     backwards from the tape (no trigonometry or recomputation there):
 //
 //   position pass, per edge (s, g):  Fb = pb[s] / deg_s - pb[g] / deg_g
 //       rb = J Fb,  J = k_st/2 ((1 - k_sh)((1 - l0/L) I + l0/L rhat rhat^T) + k_sh I)
@@ -39,7 +38,6 @@ struct SegCtx {
   float Wd, Hd;
   // LDS
   float *X, *V, *TH, *P, *CO, *FE, *RS, *KB, *LE;
-  float *Xs, *THs;                     // C x 2nn, C x ne (reverse only)
   float *XB, *VB, *THB, *DB, *RB;      // adjoints (reverse only)
 };
 
@@ -61,7 +59,6 @@ __device__ __forceinline__ void seg_init(const RodAdjArgs& a, SegCtx& s, float* 
   s.X = take(2 * nn); s.V = take(2 * nn); s.P = take(2 * nn); s.TH = take(ne);
   s.CO = take(nb); s.FE = take(2 * ne); s.RS = take(nb); s.KB = take(nb); s.LE = take(ne);
   if (reverse) {
-    s.Xs = take((size_t)s.C * 2 * nn); s.THs = take((size_t)s.C * ne);
     s.XB = take(2 * nn); s.VB = take(2 * nn); s.THB = take(ne); s.DB = take(nb); s.RB = take(nb);
   }
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -74,9 +71,10 @@ __device__ __forceinline__ void seg_init(const RodAdjArgs& a, SegCtx& s, float* 
 // One rod-only step on the LDS state (X, V, TH) -> (X, V, TH), with the
 // arithmetic of the forward rod kernel (PBD branch, steric and fluid off):
 // prologue (simulate.py:46-50), the cycles (pbd.py:14-91), velocity and the
-// final wrap (simulate.py:83-98).  With `keep`, every cycle's positions
-// before the position pass and angles after the angle pass go to Xs / THs.
-__device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, bool keep) {
+// final wrap (simulate.py:83-98).  With `rec`, every cycle's per-edge
+// minimum-image vector and the direction (cos, sin) of the updated angle --
+// all the adjoint of the position pass needs -- go to rec[cyc * ne + e].
+__device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, float4* rec) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int nn = s.nn, ne = s.ne, nb = s.nb;
   for (int i = tid; i < nn; i += bs) {
@@ -90,8 +88,6 @@ __device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, bool keep) 
       const float diff = wrap_angle(s.TH[s.hout[b]] - s.TH[s.hin[b]]);
       s.CO[b] = ((diff - s.RS[b]) * s.KB[b]) * 0.5f;
     }
-    if (keep)
-      for (int i = tid; i < 2 * nn; i += bs) s.Xs[(size_t)cyc * 2 * nn + i] = s.P[i];
     __syncthreads();
     for (int e = tid; e < ne; e += bs) {
       const int i0 = s.einc_off[e], i1 = s.einc_off[e + 1];
@@ -104,13 +100,13 @@ __device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, bool keep) 
       const float deg = fmax_nan((float)(i1 - i0), 1.0f);
       const float th = s.TH[e] + (d / deg) * 1.8f;
       s.TH[e] = th;
-      if (keep) s.THs[(size_t)cyc * ne + e] = th;
       const int sn = s.esrc[e], tn = s.etgt[e];
       const float rx = min_image(s.P[2 * tn], s.P[2 * sn], s.Wd);
       const float ry = min_image(s.P[2 * tn + 1], s.P[2 * sn + 1], s.Hd);
       const float l = s.LE[e];
       double sd, cd;
       sincos((double)th, &sd, &cd);
+      if (rec) rec[(size_t)cyc * ne + e] = make_float4(rx, ry, (float)cd, (float)sd);
       const float tx = (float)cd * l, ty = (float)sd * l;
       const float len = sqrtf(rx * rx + ry * ry);
       const float md = fmax_nan(len, 1e-8f);
@@ -185,7 +181,7 @@ __global__ void __launch_bounds__(64) k_rod_fwd_segment(RodAdjArgs a, int steps,
   seg_load(s, pos, vel, th);
   const size_t N2 = 2 * (size_t)a.N;
   seg_store(s, ck, ck + N2, ck + 2 * N2);
-  for (int k = 0; k < steps; ++k) seg_step(a.c, s, false);
+  for (int k = 0; k < steps; ++k) seg_step(a.c, s, nullptr);
   seg_store(s, pos, vel, th);
   for (int i = threadIdx.x; i < s.nn; i += blockDim.x) {
     const int g = s.nodes[i];
@@ -194,8 +190,11 @@ __global__ void __launch_bounds__(64) k_rod_fwd_segment(RodAdjArgs a, int steps,
   }
 }
 
-// Reverse segment from checkpoint slot `ck` over `steps` steps; adjoints
-// (a.xb / a.vb / a.thb) and the rest-angle adjoint (a.rest_bar) updated in place.
+// Reverse segment from checkpoint slot `ck` over `steps` steps: one forward
+// pass records each cycle's per-edge vector and direction in the filament's
+// part of the tape (steps x C x ne float4), then the step adjoints run
+// backwards from the tape; adjoints (a.xb / a.vb / a.thb) and the rest-angle
+// adjoint (a.rest_bar) are updated in place.
 __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps, const float* ck,
                                                          float* tape) {
   extern __shared__ float sm[];
@@ -203,21 +202,11 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
   seg_init(a, s, sm, true);
   const Consts& c = a.c;
   const int tid = threadIdx.x, bs = blockDim.x;
-  const int nn = s.nn, ne = s.ne, nb = s.nb;
+  const int nn = s.nn, ne = s.ne, nb = s.nb, C = s.C;
   const size_t N2 = 2 * (size_t)a.N;
-  // the filament's part of the tape: steps x (4 nn + ne)
-  const size_t per = 4 * (size_t)nn + ne;
-  float* tp = tape + (size_t)steps * (4 * (size_t)s.n0 + s.e0);
+  float4* tp = reinterpret_cast<float4*>(tape) + (size_t)steps * C * s.e0;
   seg_load(s, ck, ck + N2, ck + 2 * N2);
-  for (int k = 0; k < steps; ++k) {
-    float* slot = tp + (size_t)k * per;
-    for (int i = tid; i < 2 * nn; i += bs) {
-      slot[i] = s.X[i];
-      slot[2 * nn + i] = s.V[i];
-    }
-    for (int e = tid; e < ne; e += bs) slot[4 * nn + e] = s.TH[e];
-    if (k + 1 < steps) seg_step(c, s, false);
-  }
+  for (int k = 0; k < steps; ++k) seg_step(c, s, tp + (size_t)k * C * ne);
   for (int i = tid; i < nn; i += bs) {
     const int g = s.nodes[i];
     s.XB[2 * i] = a.xb[2 * g];
@@ -232,28 +221,19 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
   __syncthreads();
   const float kst = c.k_st, ksh = c.k_sh;
   for (int k = steps - 1; k >= 0; --k) {
-    const float* slot = tp + (size_t)k * per;
-    for (int i = tid; i < 2 * nn; i += bs) {
-      s.X[i] = slot[i];
-      s.V[i] = slot[2 * nn + i];
-    }
-    for (int e = tid; e < ne; e += bs) s.TH[e] = slot[4 * nn + e];
-    __syncthreads();
-    seg_step(c, s, true);
     // adjoint of the epilogue: pb = xb + vb / dt
     for (int i = tid; i < 2 * nn; i += bs) s.XB[i] = s.XB[i] + s.VB[i] / c.dt;
     __syncthreads();
-    for (int cyc = s.C - 1; cyc >= 0; --cyc) {
-      const float* Xc = s.Xs + (size_t)cyc * 2 * nn;
-      const float* THc = s.THs + (size_t)cyc * ne;
+    for (int cyc = C - 1; cyc >= 0; --cyc) {
+      const float4* rc = tp + ((size_t)k * C + cyc) * ne;
       for (int e = tid; e < ne; e += bs) {
+        const float4 q = rc[e];  // (rx, ry, cos th', sin th')
         const int sn = s.esrc[e], tn = s.etgt[e];
         const float ds = fmax_nan((float)(s.ninc_off[sn + 1] - s.ninc_off[sn]), 1.0f);
         const float dg = fmax_nan((float)(s.ninc_off[tn + 1] - s.ninc_off[tn]), 1.0f);
         const float fbx = s.XB[2 * sn] / ds - s.XB[2 * tn] / dg;
         const float fby = s.XB[2 * sn + 1] / ds - s.XB[2 * tn + 1] / dg;
-        const float rx = min_image(Xc[2 * tn], Xc[2 * sn], s.Wd);
-        const float ry = min_image(Xc[2 * tn + 1], Xc[2 * sn + 1], s.Hd);
+        const float rx = q.x, ry = q.y;
         const float l = s.LE[e];
         const float len = sqrtf(rx * rx + ry * ry);
         const float ls = fmax_nan(len, 1e-8f);
@@ -263,9 +243,7 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
         const float sx = al * fbx + bl * pr * hx, sy = al * fby + bl * pr * hy;
         s.FE[2 * e] = 0.5f * kst * ((1.0f - ksh) * sx + ksh * fbx);
         s.FE[2 * e + 1] = 0.5f * kst * ((1.0f - ksh) * sy + ksh * fby);
-        double sd, cd;
-        sincos((double)THc[e], &sd, &cd);
-        s.THB[e] = s.THB[e] + (-0.5f * kst * ksh * l) * (-(float)sd * fbx + (float)cd * fby);
+        s.THB[e] = s.THB[e] + (-0.5f * kst * ksh * l) * (-q.w * fbx + q.z * fby);
       }
       __syncthreads();
       // per node: + rb at the edge's target, - rb at its source
@@ -325,8 +303,9 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
 }  // namespace
 
 size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse) {
+  (void)cycles;
   size_t f = 6 * (size_t)nn + 4 * (size_t)ne + 3 * (size_t)nb;
-  if (reverse) f += (size_t)cycles * (2 * (size_t)nn + ne) + 4 * (size_t)nn + ne + 2 * (size_t)nb;
+  if (reverse) f += 4 * (size_t)nn + ne + 2 * (size_t)nb;
   return f * sizeof(float);
 }
 

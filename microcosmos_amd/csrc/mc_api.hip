@@ -1513,9 +1513,10 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
   const size_t S = 4 * (size_t)N + (size_t)E;
   const int every = p->adj_every;
   const int64_t T = p->adj_steps;
-  if (p->adj_tape_cap < (size_t)every * S) {
-    MCCHK(p->alloc(&p->adj_tape, (size_t)every * S, "adjoint tape"));
-    p->adj_tape_cap = (size_t)every * S;
+  const size_t tape = 4 * (size_t)every * (size_t)std::max(q->cycles_per_step, 1) * (size_t)E;
+  if (p->adj_tape_cap < tape) {
+    MCCHK(p->alloc(&p->adj_tape, tape, "adjoint tape"));
+    p->adj_tape_cap = tape;
   }
   if (!p->adj_work) MCCHK(p->alloc(&p->adj_work, S, "adjoint state"));
   float* xb = p->adj_work;

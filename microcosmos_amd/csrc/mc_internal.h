@@ -199,7 +199,7 @@ struct RodAdjArgs {
 };
 size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse);
 // forward: ck = checkpoint slot written, tape_or_dbg = debug_vector (N, 2);
-// reverse: ck = checkpoint slot read, tape_or_dbg = tape (steps x (4N + E))
+// reverse: ck = checkpoint slot read, tape_or_dbg = tape (steps x cycles x E float4)
 hipError_t launch_rod_segment(const RodAdjArgs& a, bool reverse, int steps, float* ck,
                               float* tape_or_dbg, size_t lds_bytes, hipStream_t s);
 // immersed boundary (solved per interaction cluster)
