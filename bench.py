@@ -189,18 +189,19 @@ def serving(n_req, dev, lanes, chunk, max_batch, plain=True):
     return out
 
 
-def gradient_leg(dev, reps=5):
+def gradient_leg(dev, reps=5, scheme="pbd"):
     """The folding gradient: loss of the final positions after 500 steps of
     the rod-only step (256^2, 8 filaments x 48 nodes) and its gradient with
     respect to the 368 hinge rest angles (384 per-node curvatures), checkpoints
     every 50 steps, forward and adjoint kernels on the device."""
     import torch
-    from microcosmos_amd import PBD_SCHEME_NO_FLUID
+    from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID, PBD_SCHEME_NO_FLUID
     from microcosmos_amd import workloads as W
     from microcosmos_amd.grad import RodAdjoint, min_image
     from tests.helpers import device_state
     wl, tgt = W.folding()
-    cfg = replace(PBD_SCHEME_NO_FLUID, enable_steric=False)
+    base = COSSERAT_SCHEME_NO_FLUID if scheme == "cosserat" else PBD_SCHEME_NO_FLUID
+    cfg = replace(base, enable_steric=False)
     target = torch.as_tensor(tgt, device=dev)
 
     def one():
@@ -225,7 +226,8 @@ def gradient_leg(dev, reps=5):
         bw.append(b)
     import numpy as np
     f_ms, b_ms = float(np.median(fw)) * 1e3, float(np.median(bw)) * 1e3
-    return {"workload": wl.name, "steps": 500, "checkpoint_every": 50, "parameters": 384,
+    return {"workload": wl.name, "scheme": scheme, "cycles": cfg.cycles_per_step, "steps": 500,
+            "checkpoint_every": 50, "parameters": 384,
             "hinges": int(wl.bending_pairs.shape[0]), "forward_ms": round(f_ms, 3),
             "backward_ms": round(b_ms, 3), "gradients_per_s": round(1e3 / (f_ms + b_ms), 2),
             "finite": bool(ok)}
@@ -316,6 +318,7 @@ def main():
     grad = None
     if not args.no_grad:
         grad = gradient_leg(dev)
+        grad["stable_cosserat"] = gradient_leg(dev, scheme="cosserat")
         print(f"[bench] gradient {grad}", file=sys.stderr, flush=True)
     served = None
     if not args.no_serve:

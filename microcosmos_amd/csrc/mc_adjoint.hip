@@ -38,7 +38,8 @@ struct SegCtx {
   float Wd, Hd;
   // LDS
   float *X, *V, *TH, *P, *CO, *FE, *RS, *KB, *LE;
-  float *XB, *VB, *THB, *DB, *RB;      // adjoints (reverse only)
+  float *Y, *HN;                       // Cosserat: inertial target, per-node diagonal h
+  float *XB, *VB, *THB, *DB, *RB, *YB; // adjoints (reverse only)
 };
 
 __device__ __forceinline__ void seg_init(const RodAdjArgs& a, SegCtx& s, float* sm, bool reverse) {
@@ -58,14 +59,28 @@ __device__ __forceinline__ void seg_init(const RodAdjArgs& a, SegCtx& s, float* 
   auto take = [&](size_t n) { float* r = q; q += n; return r; };
   s.X = take(2 * nn); s.V = take(2 * nn); s.P = take(2 * nn); s.TH = take(ne);
   s.CO = take(nb); s.FE = take(2 * ne); s.RS = take(nb); s.KB = take(nb); s.LE = take(ne);
+  s.Y = take(2 * nn); s.HN = take(nn);
   if (reverse) {
     s.XB = take(2 * nn); s.VB = take(2 * nn); s.THB = take(ne); s.DB = take(nb); s.RB = take(nb);
+    s.YB = take(2 * nn);
   }
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
     s.RS[b] = a.rest_ang[s.hinges[b]];
     s.KB[b] = a.kb[s.hinges[b]];
   }
   for (int e = threadIdx.x; e < ne; e += blockDim.x) s.LE[e] = a.rest_len[s.edges[e]];
+  __syncthreads();
+  // Cosserat diagonal h_n = w + sum of k_st / l^2 over the node's edges, in
+  // the forward's incidence order (stable_cosserat.py:197-207)
+  if (a.c.solver == 1)
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+      float h = a.c.cos_w;
+      for (int q = s.ninc_off[i]; q < s.ninc_off[i + 1]; ++q) {
+        const float l = s.LE[s.node_inc[q] >> 1];
+        h = h + a.c.k_st / (l * l);
+      }
+      s.HN[i] = h;
+    }
 }
 
 // One rod-only step on the LDS state (X, V, TH) -> (X, V, TH), with the
@@ -77,10 +92,13 @@ __device__ __forceinline__ void seg_init(const RodAdjArgs& a, SegCtx& s, float* 
 __device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, float4* rec) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int nn = s.nn, ne = s.ne, nb = s.nb;
+  const bool is_cos = c.solver == 1;
   for (int i = tid; i < nn; i += bs) {
     const float dvx = s.V[2 * i] * c.damping, dvy = s.V[2 * i + 1] * c.damping;
     s.P[2 * i] = s.X[2 * i] + dvx * c.dt;
     s.P[2 * i + 1] = s.X[2 * i + 1] + dvy * c.dt;
+    s.Y[2 * i] = s.P[2 * i];
+    s.Y[2 * i + 1] = s.P[2 * i + 1];
   }
   __syncthreads();
   for (int cyc = 0; cyc < s.C; ++cyc) {
@@ -98,37 +116,67 @@ __device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, float4* rec
         d = d + ((code & 1) ? -v : v);
       }
       const float deg = fmax_nan((float)(i1 - i0), 1.0f);
-      const float th = s.TH[e] + (d / deg) * 1.8f;
-      s.TH[e] = th;
       const int sn = s.esrc[e], tn = s.etgt[e];
       const float rx = min_image(s.P[2 * tn], s.P[2 * sn], s.Wd);
       const float ry = min_image(s.P[2 * tn + 1], s.P[2 * sn + 1], s.Hd);
+      float th;
+      if (!is_cos) {
+        th = s.TH[e] + (d / deg) * 1.8f;
+      } else {
+        // stable_cosserat.py:55-80: angle pass without SOR, then the 0.05 pull
+        th = s.TH[e] + d / deg;
+        const float alpha = datan2(ry, rx);
+        const float pull = wrap_angle(alpha - th);
+        th = th + pull * 0.05f;
+      }
+      s.TH[e] = th;
       const float l = s.LE[e];
       double sd, cd;
       sincos((double)th, &sd, &cd);
       if (rec) rec[(size_t)cyc * ne + e] = make_float4(rx, ry, (float)cd, (float)sd);
-      const float tx = (float)cd * l, ty = (float)sd * l;
-      const float len = sqrtf(rx * rx + ry * ry);
-      const float md = fmax_nan(len, 1e-8f);
-      const float cdx = rx / md, cdy = ry / md;
-      const float stx = cdx * (len - l), sty = cdy * (len - l);
-      const float shx = (rx - tx) - stx, shy = (ry - ty) - sty;
-      s.FE[2 * e] = ((stx + shx * c.k_sh) * c.k_st) * 0.5f;
-      s.FE[2 * e + 1] = ((sty + shy * c.k_sh) * c.k_st) * 0.5f;
+      if (!is_cos) {
+        const float tx = (float)cd * l, ty = (float)sd * l;
+        const float len = sqrtf(rx * rx + ry * ry);
+        const float md = fmax_nan(len, 1e-8f);
+        const float cdx = rx / md, cdy = ry / md;
+        const float stx = cdx * (len - l), sty = cdy * (len - l);
+        const float shx = (rx - tx) - stx, shy = (ry - ty) - sty;
+        s.FE[2 * e] = ((stx + shx * c.k_sh) * c.k_st) * 0.5f;
+        s.FE[2 * e + 1] = ((sty + shy * c.k_sh) * c.k_st) * 0.5f;
+      } else {
+        const float cx = rx / l - (float)cd, cy = ry / l - (float)sd;
+        const float kl = c.k_st / l;
+        s.FE[2 * e] = kl * cx;
+        s.FE[2 * e + 1] = kl * cy;
+      }
     }
     __syncthreads();
     for (int i = tid; i < nn; i += bs) {
       const int i0 = s.ninc_off[i], i1 = s.ninc_off[i + 1];
-      float ux = 0.0f, uy = 0.0f;
-      for (int q = i0; q < i1; ++q) {
-        const int code = s.node_inc[q];
-        const int e = code >> 1;
-        ux = ux + ((code & 1) ? -s.FE[2 * e] : s.FE[2 * e]);
-        uy = uy + ((code & 1) ? -s.FE[2 * e + 1] : s.FE[2 * e + 1]);
+      if (!is_cos) {
+        float ux = 0.0f, uy = 0.0f;
+        for (int q = i0; q < i1; ++q) {
+          const int code = s.node_inc[q];
+          const int e = code >> 1;
+          ux = ux + ((code & 1) ? -s.FE[2 * e] : s.FE[2 * e]);
+          uy = uy + ((code & 1) ? -s.FE[2 * e + 1] : s.FE[2 * e + 1]);
+        }
+        const float deg = fmax_nan((float)(i1 - i0), 1.0f);
+        s.P[2 * i] = s.P[2 * i] + ux / deg;
+        s.P[2 * i + 1] = s.P[2 * i + 1] + uy / deg;
+      } else {
+        const float w = c.cos_w;
+        const float xx = s.P[2 * i], xy = s.P[2 * i + 1];
+        float fx = w * (s.Y[2 * i] - xx), fy = w * (s.Y[2 * i + 1] - xy);
+        for (int q = i0; q < i1; ++q) {
+          const int code = s.node_inc[q];
+          const int e = code >> 1;
+          fx = fx + ((code & 1) ? -s.FE[2 * e] : s.FE[2 * e]);
+          fy = fy + ((code & 1) ? -s.FE[2 * e + 1] : s.FE[2 * e + 1]);
+        }
+        s.P[2 * i] = xx + fx / s.HN[i];
+        s.P[2 * i + 1] = xy + fy / s.HN[i];
       }
-      const float deg = fmax_nan((float)(i1 - i0), 1.0f);
-      s.P[2 * i] = s.P[2 * i] + ux / deg;
-      s.P[2 * i + 1] = s.P[2 * i + 1] + uy / deg;
     }
     __syncthreads();
   }
@@ -183,10 +231,77 @@ __global__ void __launch_bounds__(64) k_rod_fwd_segment(RodAdjArgs a, int steps,
   seg_store(s, ck, ck + N2, ck + 2 * N2);
   for (int k = 0; k < steps; ++k) seg_step(a.c, s, nullptr);
   seg_store(s, pos, vel, th);
+  const bool is_cos = a.c.solver == 1;
   for (int i = threadIdx.x; i < s.nn; i += blockDim.x) {
     const int g = s.nodes[i];
-    dbg[2 * g] = 0.0f;
-    dbg[2 * g + 1] = 0.0f;
+    dbg[2 * g] = (is_cos && steps > 0) ? s.Y[2 * i] : 0.0f;
+    dbg[2 * g + 1] = (is_cos && steps > 0) ? s.Y[2 * i + 1] : 0.0f;
+  }
+}
+
+// Reverse of one step's stable Cosserat cycles (stable_cosserat.py:13-119)
+// from its tape rec[cyc * ne + e] = (r, cos th2, sin th2):
+//   Fb = pb[s] / h_s - pb[g] / h_g
+//   rb = k / l^2 Fb + 0.05 th2b (-r_y, r_x) / |r|^2   (the pull's atan2)
+//   th2b = thb - k / l (-sin th2 Fb_x + cos th2 Fb_y),  th1b = 0.95 th2b
+//   pb' = pb (1 - w / h) + sum (+rb at the target, -rb at the source);
+//   yb += pb w / h;  corrb = th1b[e_in] / deg_in - th1b[e_out] / deg_out
+__device__ __forceinline__ void cosserat_cycles_adjoint(const Consts& c, SegCtx& s,
+                                                        const float4* rec) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int nn = s.nn, ne = s.ne, nb = s.nb;
+  const float k = c.k_st, w = c.cos_w;
+  for (int cyc = s.C - 1; cyc >= 0; --cyc) {
+    const float4* rc = rec + (size_t)cyc * ne;
+    for (int e = tid; e < ne; e += bs) {
+      const float4 q = rc[e];
+      const int sn = s.esrc[e], tn = s.etgt[e];
+      const float fbx = s.XB[2 * sn] / s.HN[sn] - s.XB[2 * tn] / s.HN[tn];
+      const float fby = s.XB[2 * sn + 1] / s.HN[sn] - s.XB[2 * tn + 1] / s.HN[tn];
+      const float l = s.LE[e];
+      const float th2b = s.THB[e] + (-(k / l)) * (-q.w * fbx + q.z * fby);
+      const float ab = 0.05f * th2b / (q.x * q.x + q.y * q.y);
+      s.FE[2 * e] = (k / (l * l)) * fbx - ab * q.y;
+      s.FE[2 * e + 1] = (k / (l * l)) * fby + ab * q.x;
+      s.THB[e] = 0.95f * th2b;
+    }
+    __syncthreads();
+    for (int i = tid; i < nn; i += bs) {
+      const int i0 = s.ninc_off[i], i1 = s.ninc_off[i + 1];
+      float ux = 0.0f, uy = 0.0f;
+      for (int q = i0; q < i1; ++q) {
+        const int code = s.node_inc[q];
+        const int e = code >> 1;
+        ux = ux + ((code & 1) ? s.FE[2 * e] : -s.FE[2 * e]);
+        uy = uy + ((code & 1) ? s.FE[2 * e + 1] : -s.FE[2 * e + 1]);
+      }
+      const float r = w / s.HN[i];
+      const float pbx = s.XB[2 * i], pby = s.XB[2 * i + 1];
+      s.YB[2 * i] = s.YB[2 * i] + pbx * r;
+      s.YB[2 * i + 1] = s.YB[2 * i + 1] + pby * r;
+      s.XB[2 * i] = pbx * (1.0f - r) + ux;
+      s.XB[2 * i + 1] = pby * (1.0f - r) + uy;
+    }
+    for (int b = tid; b < nb; b += bs) {
+      const int ei = s.hin[b], eo = s.hout[b];
+      const float di = fmax_nan((float)(s.einc_off[ei + 1] - s.einc_off[ei]), 1.0f);
+      const float dout = fmax_nan((float)(s.einc_off[eo + 1] - s.einc_off[eo]), 1.0f);
+      const float corrb = s.THB[ei] / di - s.THB[eo] / dout;
+      const float db = corrb * s.KB[b] * 0.5f;
+      s.DB[b] = db;
+      s.RB[b] = s.RB[b] - db;
+    }
+    __syncthreads();
+    for (int e = tid; e < ne; e += bs) {
+      const int i0 = s.einc_off[e], i1 = s.einc_off[e + 1];
+      float d = 0.0f;
+      for (int q = i0; q < i1; ++q) {
+        const int code = s.edge_inc[q];
+        d = d + ((code & 1) ? s.DB[code >> 1] : -s.DB[code >> 1]);
+      }
+      s.THB[e] = s.THB[e] + d;
+    }
+    __syncthreads();
   }
 }
 
@@ -220,10 +335,17 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
   for (int b = tid; b < nb; b += bs) s.RB[b] = a.rest_bar[s.hinges[b]];
   __syncthreads();
   const float kst = c.k_st, ksh = c.k_sh;
+  const bool is_cos = c.solver == 1;
   for (int k = steps - 1; k >= 0; --k) {
     // adjoint of the epilogue: pb = xb + vb / dt
-    for (int i = tid; i < 2 * nn; i += bs) s.XB[i] = s.XB[i] + s.VB[i] / c.dt;
+    for (int i = tid; i < 2 * nn; i += bs) {
+      s.XB[i] = s.XB[i] + s.VB[i] / c.dt;
+      s.YB[i] = 0.0f;
+    }
     __syncthreads();
+    if (is_cos) {
+      cosserat_cycles_adjoint(c, s, tp + (size_t)k * C * ne);
+    } else
     for (int cyc = C - 1; cyc >= 0; --cyc) {
       const float4* rc = tp + ((size_t)k * C + cyc) * ne;
       for (int e = tid; e < ne; e += bs) {
@@ -281,9 +403,10 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
       }
       __syncthreads();
     }
-    // adjoint of the prologue: xb = pb - vb / dt, vb = pb d dt
+    // adjoint of the prologue: xb = pb - vb / dt, vb = pb d dt (pb includes
+    // the Cosserat inertial target's share, y = the prediction)
     for (int i = tid; i < 2 * nn; i += bs) {
-      const float pb = s.XB[i];
+      const float pb = s.XB[i] + s.YB[i];
       s.XB[i] = pb - s.VB[i] / c.dt;
       s.VB[i] = (pb * c.damping) * c.dt;
     }
@@ -304,8 +427,8 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
 
 size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse) {
   (void)cycles;
-  size_t f = 6 * (size_t)nn + 4 * (size_t)ne + 3 * (size_t)nb;
-  if (reverse) f += 4 * (size_t)nn + ne + 2 * (size_t)nb;
+  size_t f = 9 * (size_t)nn + 4 * (size_t)ne + 3 * (size_t)nb;
+  if (reverse) f += 6 * (size_t)nn + ne + 2 * (size_t)nb;
   return f * sizeof(float);
 }
 

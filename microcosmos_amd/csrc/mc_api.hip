@@ -1429,9 +1429,10 @@ namespace {
 
 mc_status check_rod_adjoint(const mc_params* q) {
   if (!q) return fail(MC_ERR_INVALID, "params is NULL");
-  if (q->solver_type != MC_SOLVER_PBD || q->enable_fluid || q->enable_steric)
+  if ((q->solver_type != MC_SOLVER_PBD && q->solver_type != MC_SOLVER_COSSERAT) ||
+      q->enable_fluid || q->enable_steric)
     return fail(MC_ERR_UNSUPPORTED,
-                "the rod adjoint covers the PBD step with enable_fluid = enable_steric = 0");
+                "the rod adjoint covers the rod-only step with enable_fluid = enable_steric = 0");
   return MC_OK;
 }
 

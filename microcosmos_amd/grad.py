@@ -9,8 +9,9 @@ and the reverse pass recomputes each segment before running the per-step
 adjoint kernel backwards through it, so the device memory of a gradient is
 (checkpoints + one segment) x state instead of the whole trajectory.
 
-Scope: the rod-only step (PBD constraint, ``enable_fluid=False``,
-``enable_steric=False``) with the bending rest angles as the parameters.
+Scope: the rod-only step (PBD or stable Cosserat constraint,
+``enable_fluid=False``, ``enable_steric=False``) with the bending rest angles
+as the parameters.
 """
 from __future__ import annotations
 
@@ -60,9 +61,9 @@ class RodAdjoint:
 
     def __init__(self, nodes: Nodes, edges: Edges, fields: Fields, solver_config, dt: float):
         if solver_config.enable_fluid or getattr(solver_config, "enable_steric", False) or \
-                getattr(solver_config, "solver_type", "pbd") != "pbd":
-            raise NotImplementedError("rod adjoint: PBD constraint with enable_fluid=False and "
-                                      "enable_steric=False")
+                getattr(solver_config, "solver_type", "pbd") not in ("pbd", "cosserat"):
+            raise NotImplementedError("rod adjoint: PBD or stable Cosserat constraint with "
+                                      "enable_fluid=False and enable_steric=False")
         d = nodes.position.device
         if d.type != "cuda":
             raise RuntimeError("microcosmos_amd runs on the GPU only")

@@ -4,8 +4,9 @@ TEST INFRASTRUCTURE ONLY, like ``step_ref``: imported by ``tests/`` as the
 checker of the device adjoint, never by the product path.
 
 Scope: ``step`` (simulate.py:37-100) with ``enable_fluid = False``,
-``enable_steric = False`` and the PBD constraint (pbd.py:14-91), the
-configuration of the reference's differentiable folding runs without the
+``enable_steric = False`` and the PBD constraint (pbd.py:14-91) or the stable
+Cosserat constraint (stable_cosserat.py:13-119, see ``_cosserat_cycles_vjp``),
+the configurations of the reference's differentiable folding runs without the
 steric term (experiments/filament_folding/experiment.py:52-53, 985-1018 take
 ``jax.value_and_grad`` through ``lax.scan`` of this step).  The derivative is
 written out by hand from the forward equations of ``step_ref``:
@@ -64,8 +65,9 @@ def _degrees(s: R.State):
 
 
 def _check(p: R.Params):
-    if p.enable_fluid or p.enable_steric or p.solver_type != "pbd":
-        raise NotImplementedError("the rod adjoint covers the PBD step without fluid and steric")
+    if p.enable_fluid or p.enable_steric or p.solver_type not in ("pbd", "cosserat"):
+        raise NotImplementedError("the rod adjoint covers the rod-only step (PBD or stable "
+                                  "Cosserat) without fluid and steric")
 
 
 def step_tape(s: R.State, dt: float, p: R.Params):
@@ -76,18 +78,66 @@ def step_tape(s: R.State, dt: float, p: R.Params):
     prev = s.position
     s.velocity = s.velocity * p.damping
     s.position = prev + s.velocity * dt
+    cos = p.solver_type == "cosserat"
+    if cos:
+        s.debug_vector = s.position
     ps, ths = [], []
     for _ in range(p.cycles_per_step):
-        t = s.copy()
-        deg_e = R._edge_bending_degree(t.theta.shape[0], t.bending_pairs, t.theta.dtype)
-        delta = R._theta_bending_delta(t.theta, t.bending_pairs, t.bending_rest_angles,
-                                       t.bending_stiffness)
-        ths.append(t.theta + delta / deg_e * OMEGA)
         ps.append(s.position.copy())
-        R.pbd_rod_constraint(s, p)
+        if cos:
+            R.stable_cosserat_constraint(s, p)
+            ths.append(s.theta.copy())
+        else:
+            t = s.copy()
+            deg_e = R._edge_bending_degree(t.theta.shape[0], t.bending_pairs, t.theta.dtype)
+            delta = R._theta_bending_delta(t.theta, t.bending_pairs, t.bending_rest_angles,
+                                           t.bending_stiffness)
+            ths.append(t.theta + delta / deg_e * OMEGA)
+            R.pbd_rod_constraint(s, p)
     s.velocity = (s.position - prev) / dt
     s.position = R.periodic_boundary(s.grid_shape, s.position)
     return s, ps, ths
+
+
+def _cosserat_cycles_vjp(s, p, ps, ths, pb, thb, rhob, deg_e):
+    """Reverse of the stable Cosserat cycles (stable_cosserat.py:13-119):
+      th1 = theta + delta / deg_e;  th2 = th1 + 0.05 wrap(atan2(r) - th1)
+      F_e = k/l (r / l - (cos th2, sin th2));  h_n = w + sum k / l^2
+      x' = x + (w (y - x) + sum_src F - sum_tgt F) / h
+    Returns (pb, thb, yb, rhob)."""
+    src, tgt = s.pairs[:, 0], s.pairs[:, 1]
+    bin_, bout = s.bending_pairs[:, 0], s.bending_pairs[:, 1]
+    l = np.asarray(s.rest_lengths, np.float64)
+    kb = np.asarray(s.bending_stiffness, np.float64)
+    k = float(p.stiffness_stretch)
+    w = 1.0 / (float(p.config_dt) ** 2)
+    h = np.full(s.position.shape[0], w)
+    np.add.at(h, src, k / l ** 2)
+    np.add.at(h, tgt, k / l ** 2)
+    yb = np.zeros_like(pb)
+    for c in reversed(range(p.cycles_per_step)):
+        pos = np.asarray(ps[c], np.float64)
+        th2 = np.asarray(ths[c], np.float64)
+        Fb = pb[src] / h[src, None] - pb[tgt] / h[tgt, None]
+        yb = yb + pb * (w / h)[:, None]
+        pnew = pb * (1.0 - w / h)[:, None]
+        rb = (k / l ** 2)[:, None] * Fb
+        th2b = thb + (-(k / l)) * (-np.sin(th2) * Fb[:, 0] + np.cos(th2) * Fb[:, 1])
+        th1b = 0.95 * th2b
+        ab = 0.05 * th2b
+        r = np.asarray(R.displacement(s.grid_shape, pos[tgt], pos[src]), np.float64)
+        L2 = (r * r).sum(axis=1)
+        rb = rb + (ab / L2)[:, None] * np.stack([-r[:, 1], r[:, 0]], axis=1)
+        np.add.at(pnew, tgt, rb)
+        np.add.at(pnew, src, -rb)
+        corrb = th1b[bin_] / deg_e[bin_] - th1b[bout] / deg_e[bout]
+        diffb = corrb * kb * 0.5
+        rhob = rhob - diffb
+        thb = th1b.copy()
+        np.add.at(thb, bout, diffb)
+        np.add.at(thb, bin_, -diffb)
+        pb = pnew
+    return pb, thb, yb, rhob
 
 
 def step_vjp(s: R.State, dt: float, p: R.Params, xb, vb, thb):
@@ -106,6 +156,10 @@ def step_vjp(s: R.State, dt: float, p: R.Params, xb, vb, thb):
     xb_direct = -vb / dt
     thb = np.array(thb, np.float64, copy=True)
     rhob = np.zeros(s.bending_pairs.shape[0])
+    if p.solver_type == "cosserat":
+        pb, thb, yb, rhob = _cosserat_cycles_vjp(s, p, ps, ths, pb, thb, rhob, deg_e)
+        pb = pb + yb  # y is the prologue's prediction (simulate.py:71-72)
+        return pb + xb_direct, pb * float(p.damping) * dt, thb, rhob
     for c in reversed(range(p.cycles_per_step)):
         pos = np.asarray(ps[c], np.float64)
         th = np.asarray(ths[c], np.float64)

@@ -32,38 +32,53 @@ def dev():
     return torch.device("cuda", 0)
 
 
-def _small(seed=11):
+def _small(seed=11, solver="pbd"):
     wl = W.small(grid=(32, 32), n_fil=2, n_nodes=12, seed=seed)
     rng = np.random.default_rng(seed)
     wl.bending_rest_angles = rng.normal(0.0, 0.2, wl.bending_pairs.shape[0]).astype(np.float32)
     wl.velocity = rng.normal(0.0, 0.05, wl.position.shape).astype(np.float32)
     target = (wl.position + rng.normal(0.0, 1.0, wl.position.shape)).astype(np.float32)
-    over = dict(cycles_per_step=4, damping=0.5, stiffness_stretch=0.6, stiffness_shear=0.8,
-                enable_steric=False)
-    return wl, target, over
+    if solver == "cosserat":
+        over = dict(cycles_per_step=4, damping=0.5, stiffness_stretch=5.0, enable_steric=False)
+    else:
+        over = dict(cycles_per_step=4, damping=0.5, stiffness_stretch=0.6, stiffness_shear=0.8,
+                    enable_steric=False)
+    return wl, target, over, solver
 
 
-def _device_grad(wl, target, over, steps, every, dev):
-    from microcosmos_amd import PBD_SCHEME_NO_FLUID
+def _base(solver):
+    from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID, PBD_SCHEME_NO_FLUID
+    return COSSERAT_SCHEME_NO_FLUID if solver == "cosserat" else PBD_SCHEME_NO_FLUID
+
+
+def _obase(solver):
+    return R.COSSERAT_SCHEME_NO_FLUID if solver == "cosserat" else R.PBD_SCHEME_NO_FLUID
+
+
+def _device_grad(wl, target, over, steps, every, dev, solver="pbd"):
     from microcosmos_amd.grad import rollout_value_and_grad
     nodes, edges, fields = device_state(wl, dev)
-    cfg = replace(PBD_SCHEME_NO_FLUID, **over)
+    cfg = replace(_base(solver), **over)
     loss, g = rollout_value_and_grad(nodes, edges, fields, cfg, wl.dt, steps,
                                      torch.as_tensor(target, device=dev), checkpoint_every=every)
     torch.cuda.synchronize()
     return float(loss.item()), g.cpu().numpy().astype(np.float64)
 
 
-def _oracle(wl, over):
+def _oracle(wl, over, solver="pbd"):
     s = oracle_state(wl, np.float64)
-    return s, replace(R.PBD_SCHEME_NO_FLUID, **over)
+    p = replace(_obase(solver), **over)
+    if solver == "cosserat":
+        p = replace(p, config_dt=0.05)
+    return s, p
 
 
-def test_adjoint_matches_restatement(dev):
-    _say("rod adjoint, 2 x 12 nodes, 20 steps, checkpoints every 7")
-    wl, target, over = _small()
-    loss, g = _device_grad(wl, target, over, 20, 7, dev)
-    s, p = _oracle(wl, over)
+@pytest.mark.parametrize("solver", ["pbd", "cosserat"])
+def test_adjoint_matches_restatement(dev, solver):
+    _say(f"rod adjoint ({solver}), 2 x 12 nodes, 20 steps, checkpoints every 7")
+    wl, target, over, solver = _small(solver=solver)
+    loss, g = _device_grad(wl, target, over, 20, 7, dev, solver)
+    s, p = _oracle(wl, over, solver)
     lo, go = A.rollout_grad(s, wl.dt, p, 20, target)
     assert abs(loss - lo) <= 1e-5 * abs(lo)
     rel = np.linalg.norm(g - go) / np.linalg.norm(go)
@@ -72,28 +87,29 @@ def test_adjoint_matches_restatement(dev):
     assert rel <= 1e-4 and cos >= 0.99999
 
 
-def test_adjoint_forward_equals_rollout(dev):
+@pytest.mark.parametrize("solver", ["pbd", "cosserat"])
+def test_adjoint_forward_equals_rollout(dev, solver):
     """The forward segments of the adjoint reproduce the plain rollout bit for
     bit (rod-only step)."""
-    _say("rod adjoint forward vs mc_rollout, 23 steps")
-    from microcosmos_amd import PBD_SCHEME_NO_FLUID, Rollout
+    _say(f"rod adjoint forward ({solver}) vs mc_rollout, 23 steps")
+    from microcosmos_amd import Rollout
     from microcosmos_amd.grad import RodAdjoint
-    wl, target, over = _small(seed=8)
-    cfg = replace(PBD_SCHEME_NO_FLUID, **over)
+    wl, target, over, solver = _small(seed=8, solver=solver)
+    cfg = replace(_base(solver), **over)
     nodes, edges, fields = device_state(wl, dev)
     ad = RodAdjoint(nodes, edges, fields, cfg, wl.dt)
     ad.forward(23, 5)
     ro = Rollout(nodes, edges, fields, cfg, wl.dt)
     ro.run(23, use_graph=False)
     torch.cuda.synchronize()
-    for name in ("position", "velocity", "theta"):
+    for name in ("position", "velocity", "theta", "debug_vector"):
         np.testing.assert_array_equal(getattr(ad, name).cpu().numpy(),
                                       getattr(ro, name).cpu().numpy(), err_msg=name)
 
 
 def test_checkpoint_interval_does_not_change_the_gradient(dev):
     _say("rod adjoint, checkpoints every 1, 7 and 20 steps")
-    wl, target, over = _small(seed=4)
+    wl, target, over, _ = _small(seed=4)
     gs = [_device_grad(wl, target, over, 20, k, dev)[1] for k in (1, 7, 20)]
     np.testing.assert_array_equal(gs[0], gs[1])
     np.testing.assert_array_equal(gs[0], gs[2])
@@ -127,3 +143,26 @@ def test_folding_gradient_vs_finite_differences(dev):
         fd = (fp - fm) / (2 * h)
         _say(f"direction {k}: adjoint {gk @ u:.6e}  central FD {fd:.6e}")
         assert abs(gk @ u - fd) <= 1e-3 * abs(fd) + 1e-6
+
+
+def test_folding_gradient_cosserat(dev):
+    """The folding experiment's default scheme (stable_cosserat,
+    experiments/conf/experiment/filament_folding.yaml:71) without the steric
+    term: 8 x 48 nodes, 200 steps, checkpoints every 50; cosine against the
+    restatement and a central difference of the float64 forward."""
+    _say("folding (stable Cosserat): 8 x 48 nodes, 200 steps")
+    wl, target = W.folding(steps=200)
+    over = dict(enable_steric=False)
+    loss, g = _device_grad(wl, target, over, 200, 50, dev, "cosserat")
+    s, p = _oracle(wl, over, "cosserat")
+    lo, go = A.rollout_grad(s, wl.dt, p, 200, target)
+    cos = g @ go / (np.linalg.norm(g) * np.linalg.norm(go))
+    _say(f"loss {loss:.6f} (restatement {lo:.6f}); cosine {cos:.7f}")
+    assert cos >= 0.999
+    u = np.random.default_rng(9).normal(size=go.shape[0])
+    h = 1e-4
+    fp = A.rollout_loss(s, wl.dt, p, 200, target, rest=s.bending_rest_angles + h * u)
+    fm = A.rollout_loss(s, wl.dt, p, 200, target, rest=s.bending_rest_angles - h * u)
+    fd = (fp - fm) / (2 * h)
+    _say(f"adjoint {g @ u:.6e}  central FD {fd:.6e}")
+    assert abs(g @ u - fd) <= 1e-3 * abs(fd) + 1e-6

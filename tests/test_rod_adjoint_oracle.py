@@ -14,14 +14,18 @@ from microcosmos_amd import workloads as W
 from .helpers import oracle_state
 
 
-def _case(damping=0.5, cycles=4, seed=11):
+def _case(damping=0.5, cycles=4, seed=11, solver="pbd"):
     wl = W.small(grid=(32, 32), n_fil=2, n_nodes=12, seed=seed)
     rng = np.random.default_rng(seed)
     s = oracle_state(wl, np.float64)
     s = replace(s, bending_rest_angles=rng.normal(0.0, 0.2, s.bending_pairs.shape[0]),
                 velocity=rng.normal(0.0, 0.05, s.position.shape))
-    p = replace(R.PBD_SCHEME_NO_FLUID, cycles_per_step=cycles, damping=damping,
-                enable_steric=False, stiffness_stretch=0.6, stiffness_shear=0.8)
+    if solver == "cosserat":
+        p = replace(R.COSSERAT_SCHEME_NO_FLUID, cycles_per_step=cycles, damping=damping,
+                    enable_steric=False, stiffness_stretch=5.0, config_dt=0.2)
+    else:
+        p = replace(R.PBD_SCHEME_NO_FLUID, cycles_per_step=cycles, damping=damping,
+                    enable_steric=False, stiffness_stretch=0.6, stiffness_shear=0.8)
     target = s.position + rng.normal(0.0, 1.0, s.position.shape)
     return s, p, target
 
@@ -35,8 +39,9 @@ def _fd(f, x, h):
     return g
 
 
-def test_rollout_rest_angle_gradient_matches_fd():
-    s, p, target = _case()
+@pytest.mark.parametrize("solver", ["pbd", "cosserat"])
+def test_rollout_rest_angle_gradient_matches_fd(solver):
+    s, p, target = _case(solver=solver)
     steps = 6
     loss, g = A.rollout_grad(s, 1.0, p, steps, target)
     fd = _fd(lambda r: A.rollout_loss(s, 1.0, p, steps, target, rest=r), s.bending_rest_angles, 1e-6)
@@ -46,11 +51,12 @@ def test_rollout_rest_angle_gradient_matches_fd():
     assert cos > 0.999999
 
 
+@pytest.mark.parametrize("solver", ["pbd", "cosserat"])
 @pytest.mark.parametrize("which", ["position", "velocity", "theta"])
-def test_step_vjp_directional_derivatives(which):
+def test_step_vjp_directional_derivatives(which, solver):
     """<vjp(w), u> equals the finite-difference derivative of <w, step(s)> in
     direction u, for the step inputs x, v and theta."""
-    s, p, _ = _case(damping=0.7, cycles=3, seed=5)
+    s, p, _ = _case(damping=0.7, cycles=3, seed=5, solver=solver)
     rng = np.random.default_rng(3)
     wx = rng.normal(size=s.position.shape)
     wv = rng.normal(size=s.position.shape)
@@ -58,12 +64,6 @@ def test_step_vjp_directional_derivatives(which):
     xb, vb, tb, _ = A.step_vjp(s, 1.0, p, wx, wv, wt)
     u = rng.normal(size=getattr(s, which).shape)
     adj = {"position": xb, "velocity": vb, "theta": tb}[which]
-
-    def f(eps):
-        t = replace(s, **{which: getattr(s, which) + eps * u})
-        o = R.step(t, 1.0, p)
-        # unwrap the output positions around the unperturbed run
-        return (np.sum(wx * o.position) + np.sum(wv * o.velocity) + np.sum(wt * o.theta))
 
     base = R.step(s, 1.0, p)
     h = 1e-6
