@@ -190,26 +190,37 @@ def serving(n_req, dev, lanes, chunk, max_batch, plain=True):
 
 
 def gradient_leg(dev, reps=5, scheme="pbd"):
-    """The folding gradient: loss of the final positions after 500 steps of
-    the rod-only step (256^2, 8 filaments x 48 nodes) and its gradient with
-    respect to the 368 hinge rest angles (384 per-node curvatures), checkpoints
-    every 50 steps, forward and adjoint kernels on the device."""
+    """The folding gradient: loss of the final positions after a rollout on
+    the folding workload (256^2, 8 filaments x 48 nodes) and its gradient with
+    respect to the 368 hinge rest angles (384 per-node curvatures), forward
+    and adjoint kernels on the device.  scheme "pbd" / "cosserat": the
+    rod-only step, 500 steps at dt 1, checkpoints every 50; "folding": the
+    folding experiment's default physics (stable Cosserat with the steric
+    term, dt 0.02, experiments/conf/experiment/filament_folding.yaml:53-90),
+    150 steps, every step kept."""
     import torch
     from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID, PBD_SCHEME_NO_FLUID
     from microcosmos_amd import workloads as W
     from microcosmos_amd.grad import RodAdjoint, min_image
     from tests.helpers import device_state
-    wl, tgt = W.folding()
-    base = COSSERAT_SCHEME_NO_FLUID if scheme == "cosserat" else PBD_SCHEME_NO_FLUID
-    cfg = replace(base, enable_steric=False)
+    if scheme == "folding":
+        steps, every, dt = W.FOLDING_STEPS, 1, W.FOLDING_DT
+        wl, tgt = W.folding(steps=steps)
+        cfg = replace(COSSERAT_SCHEME_NO_FLUID, dt=dt, **W.FOLDING_COSSERAT)
+    else:
+        steps, every = 500, 50
+        wl, tgt = W.folding()
+        dt = wl.dt
+        base = COSSERAT_SCHEME_NO_FLUID if scheme == "cosserat" else PBD_SCHEME_NO_FLUID
+        cfg = replace(base, enable_steric=False)
     target = torch.as_tensor(tgt, device=dev)
 
     def one():
         n, e, f = device_state(wl, dev)
-        ad = RodAdjoint(n, e, f, cfg, wl.dt)
+        ad = RodAdjoint(n, e, f, cfg, dt)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        x = ad.forward(500, 50)
+        x = ad.forward(steps, every)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         d = min_image(ad.grid_shape, x, target)
@@ -226,8 +237,9 @@ def gradient_leg(dev, reps=5, scheme="pbd"):
         bw.append(b)
     import numpy as np
     f_ms, b_ms = float(np.median(fw)) * 1e3, float(np.median(bw)) * 1e3
-    return {"workload": wl.name, "scheme": scheme, "cycles": cfg.cycles_per_step, "steps": 500,
-            "checkpoint_every": 50, "parameters": 384,
+    return {"workload": wl.name, "scheme": scheme, "cycles": cfg.cycles_per_step, "steps": steps,
+            "dt": dt, "steric": bool(cfg.enable_steric), "checkpoint_every": every,
+            "parameters": 384,
             "hinges": int(wl.bending_pairs.shape[0]), "forward_ms": round(f_ms, 3),
             "backward_ms": round(b_ms, 3), "gradients_per_s": round(1e3 / (f_ms + b_ms), 2),
             "finite": bool(ok)}
@@ -319,6 +331,7 @@ def main():
     if not args.no_grad:
         grad = gradient_leg(dev)
         grad["stable_cosserat"] = gradient_leg(dev, scheme="cosserat")
+        grad["folding_physics"] = gradient_leg(dev, scheme="folding")
         print(f"[bench] gradient {grad}", file=sys.stderr, flush=True)
     served = None
     if not args.no_serve:

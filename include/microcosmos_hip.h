@@ -208,11 +208,19 @@ mc_status mc_batch_advance(mc_plan* plan, const mc_params* params, const mc_stat
  * last step (N, 2) and velocity_bar = dL/d velocity (may be NULL: zero), and
  * writes rest_angle_bar (B) = dL/d bending_rest_angles, plus (each may be
  * NULL) the gradient with respect to the initial position, velocity and
- * theta.  Without fluid and steric term the filaments are independent: one
- * workgroup per filament runs a whole checkpoint segment per launch, forward
- * (bit-identical to mc_rollout) or reverse (the segment recomputed from its
- * checkpoint, then the hand-written adjoint of each step backwards).  Every
- * filament must fit the LDS budget (MC_ERR_UNSUPPORTED otherwise).
+ * theta.  params->enable_fluid must be 0; the PBD and stable Cosserat
+ * constraints are covered, with or without the steric term.  Without it the
+ * filaments are independent: one workgroup per filament runs a whole
+ * checkpoint segment per launch, forward (bit-identical to mc_rollout) or
+ * reverse (the segment recomputed from its checkpoint, then the hand-written
+ * adjoint of each step backwards).  With it the filaments couple through the
+ * field every step: every step's state is kept (checkpoint_every is taken as
+ * 1; memory (4 N + E) floats per step), the forward runs the coupled step's
+ * steric pipeline each step (bit-identical to mc_rollout, field included),
+ * and each reverse step recomputes its field before the adjoint of the
+ * constraint, the steric force (deposit, Gaussian filter, samples, window)
+ * and the prediction.  Every filament must fit the LDS budget
+ * (MC_ERR_UNSUPPORTED otherwise).
  * params, dt and the edge arrays must be those of the recorded rollout.
  * Minimum-image, angle and periodic wraps are differentiated as identity. */
 mc_status mc_rod_adjoint_forward(mc_plan* plan, const mc_params* params, const mc_state* state,

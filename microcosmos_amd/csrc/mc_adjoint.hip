@@ -1,7 +1,16 @@
-// Adjoint (reverse-mode derivative) of the rod-only coupled step: simulate.py
-// :37-100 with the fluid and the steric term off and the PBD constraint
-// (pbd.py:14-91).  The reference differentiates this step with jax.grad
-// through lax.scan (experiments/filament_folding/experiment.py:985-1018).
+// Adjoint (reverse-mode derivative) of the coupled step without the fluid:
+// simulate.py:37-100 with the PBD (pbd.py:14-91) or stable Cosserat
+// (stable_cosserat.py:13-119) constraint, with or without the steric term.
+// The reference differentiates this step with jax.grad through lax.scan
+// (experiments/filament_folding/experiment.py:985-1018).
+//
+// With the steric term every filament feels every other through the field,
+// so each step is its own segment: the forward runs the coupled step's
+// steric pipeline (prediction, deposit, filter, per-node force) and then a
+// one-step segment that adds the force to the prediction; the reverse
+// recomputes that step's field and force from the kept state, runs the
+// one-step segment adjoint up to d loss / d (pushed prediction), and
+// launch_steric_adjoint finishes the step (see the steric section below).
 //
 // Without the fluid and the steric term the filaments do not interact, so one
 // workgroup (one wavefront: its barriers are cheap) per filament (connected
@@ -89,16 +98,23 @@ __device__ __forceinline__ void seg_init(const RodAdjArgs& a, SegCtx& s, float* 
 // final wrap (simulate.py:83-98).  With `rec`, every cycle's per-edge
 // minimum-image vector and the direction (cos, sin) of the updated angle --
 // all the adjoint of the position pass needs -- go to rec[cyc * ne + e].
-__device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, float4* rec) {
+__device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, float4* rec,
+                                         const float* fst) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int nn = s.nn, ne = s.ne, nb = s.nb;
   const bool is_cos = c.solver == 1;
   for (int i = tid; i < nn; i += bs) {
     const float dvx = s.V[2 * i] * c.damping, dvy = s.V[2 * i + 1] * c.damping;
-    s.P[2 * i] = s.X[2 * i] + dvx * c.dt;
-    s.P[2 * i + 1] = s.X[2 * i + 1] + dvy * c.dt;
-    s.Y[2 * i] = s.P[2 * i];
-    s.Y[2 * i + 1] = s.P[2 * i + 1];
+    float x = s.X[2 * i] + dvx * c.dt, y = s.X[2 * i + 1] + dvy * c.dt;
+    if (fst) {  // the steric push (simulate.py:52-63)
+      const int g = s.nodes[i];
+      x = x + (fst[2 * g] * c.steric_strength) * c.dt;
+      y = y + (fst[2 * g + 1] * c.steric_strength) * c.dt;
+    }
+    s.P[2 * i] = x;
+    s.P[2 * i + 1] = y;
+    s.Y[2 * i] = x;
+    s.Y[2 * i + 1] = y;
   }
   __syncthreads();
   for (int cyc = 0; cyc < s.C; ++cyc) {
@@ -229,7 +245,7 @@ __global__ void __launch_bounds__(64) k_rod_fwd_segment(RodAdjArgs a, int steps,
   seg_load(s, pos, vel, th);
   const size_t N2 = 2 * (size_t)a.N;
   seg_store(s, ck, ck + N2, ck + 2 * N2);
-  for (int k = 0; k < steps; ++k) seg_step(a.c, s, nullptr);
+  for (int k = 0; k < steps; ++k) seg_step(a.c, s, nullptr, a.fst);
   seg_store(s, pos, vel, th);
   const bool is_cos = a.c.solver == 1;
   for (int i = threadIdx.x; i < s.nn; i += blockDim.x) {
@@ -321,7 +337,7 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
   const size_t N2 = 2 * (size_t)a.N;
   float4* tp = reinterpret_cast<float4*>(tape) + (size_t)steps * C * s.e0;
   seg_load(s, ck, ck + N2, ck + 2 * N2);
-  for (int k = 0; k < steps; ++k) seg_step(c, s, tp + (size_t)k * C * ne);
+  for (int k = 0; k < steps; ++k) seg_step(c, s, tp + (size_t)k * C * ne, a.fst);
   for (int i = tid; i < nn; i += bs) {
     const int g = s.nodes[i];
     s.XB[2 * i] = a.xb[2 * g];
@@ -404,26 +420,229 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
       __syncthreads();
     }
     // adjoint of the prologue: xb = pb - vb / dt, vb = pb d dt (pb includes
-    // the Cosserat inertial target's share, y = the prediction)
-    for (int i = tid; i < 2 * nn; i += bs) {
-      const float pb = s.XB[i] + s.YB[i];
-      s.XB[i] = pb - s.VB[i] / c.dt;
-      s.VB[i] = (pb * c.damping) * c.dt;
+    // the Cosserat inertial target's share, y = the prediction).  With the
+    // steric term, pb (of the pushed prediction) goes to a.pb1 instead.
+    if (a.pb1) {
+      for (int i = tid; i < nn; i += bs) {
+        const int g = s.nodes[i];
+        a.pb1[2 * g] = s.XB[2 * i] + s.YB[2 * i];
+        a.pb1[2 * g + 1] = s.XB[2 * i + 1] + s.YB[2 * i + 1];
+      }
+    } else {
+      for (int i = tid; i < 2 * nn; i += bs) {
+        const float pb = s.XB[i] + s.YB[i];
+        s.XB[i] = pb - s.VB[i] / c.dt;
+        s.VB[i] = (pb * c.damping) * c.dt;
+      }
     }
     __syncthreads();
   }
-  for (int i = tid; i < nn; i += bs) {
-    const int g = s.nodes[i];
-    a.xb[2 * g] = s.XB[2 * i];
-    a.xb[2 * g + 1] = s.XB[2 * i + 1];
-    a.vb[2 * g] = s.VB[2 * i];
-    a.vb[2 * g + 1] = s.VB[2 * i + 1];
-  }
+  if (!a.pb1)
+    for (int i = tid; i < nn; i += bs) {
+      const int g = s.nodes[i];
+      a.xb[2 * g] = s.XB[2 * i];
+      a.xb[2 * g + 1] = s.XB[2 * i + 1];
+      a.vb[2 * g] = s.VB[2 * i];
+      a.vb[2 * g + 1] = s.VB[2 * i + 1];
+    }
   for (int e = tid; e < ne; e += bs) a.thb[s.edges[e]] = s.THB[e];
   for (int b = tid; b < nb; b += bs) a.rest_bar[s.hinges[b]] = s.RB[b];
 }
 
+// ---- steric term ----
+// f = -grad_FD V(p) - c sum_j K(r_pj) r_pj with V = G * D(p) (forces.py:20-118):
+// G the Gaussian filter (symmetric, so its adjoint is G itself), D the CIC
+// deposit, the gradient by central differences of bilinear samples at
+// p +- 0.5 e_x, p +- 0.5 e_y, c = sv / sigma^2.  Given g = pb1 strength dt
+// per node, the adjoint f^T g has three parts:
+//   field samples through V:  Vbar = sum_i sum_samples -sgn g_comp w_corner
+//   (a deposit of the samples' bilinear weights), Dbar = G Vbar, and per node
+//   sv sum_corners Dbar dw_corner/dp;
+//   field samples directly:   -J_i^T g_i with J the 2x2 derivative of the
+//   central differences (bilinear slopes of V);
+//   window term:              c sum_j K (I - r r^T / sigma^2)(g_j - g_i).
+
+constexpr int BAR_THREADS = 256;
+constexpr int BAR_LIST = 1024;  // staged nodes per flush
+
+__device__ __forceinline__ bool touches_row(float py, int H, int y) {
+  const int a = f2i(floorf(py - 0.5f)), b = f2i(floorf(py)), d = f2i(floorf(py + 0.5f));
+  return imod(a, H) == y || imod(a + 1, H) == y || imod(b, H) == y || imod(b + 1, H) == y ||
+         imod(d, H) == y || imod(d + 1, H) == y;
+}
+
+// acc += q * (bilinear weight of cell (y, x) in the sample at (yy, xx))
+__device__ __forceinline__ float bar_sample(float acc, float q, float yy, float xx, int H, int W,
+                                            int y, int x) {
+  const float ly = floorf(yy), lx = floorf(xx);
+  const float wyh = yy - ly, wxh = xx - lx;
+  const float wyl = 1.0f - wyh, wxl = 1.0f - wxh;
+  const int iy = f2i(ly), ix = f2i(lx);
+  const int y0 = imod(iy, H), y1 = imod(iy + 1, H);
+  const int x0 = imod(ix, W), x1 = imod(ix + 1, W);
+  if (y0 == y && x0 == x) acc = acc + q * (wyl * wxl);
+  if (y0 == y && x1 == x) acc = acc + q * (wyl * wxh);
+  if (y1 == y && x0 == x) acc = acc + q * (wyh * wxl);
+  if (y1 == y && x1 == x) acc = acc + q * (wyh * wxh);
+  return acc;
+}
+
+// One workgroup per steric row: the nodes whose samples reach the row are
+// staged in node order, then each thread gathers its columns' Vbar from them
+// in that order (deterministic, no atomics).
+__global__ void __launch_bounds__(BAR_THREADS) k_steric_bar_rows(Consts c, BatchDev bd,
+    const float* __restrict__ pred_w, const float* __restrict__ pb1, float* __restrict__ vbar) {
+  __shared__ float4 lst[BAR_LIST];  // (px, py, gx, gy)
+  __shared__ int wcnt[BAR_THREADS / 64];
+  __shared__ int ltop;
+  const int r = blockIdx.x;
+  const InstGeo gi = bd.geo[bd.row_inst[r]];
+  const int y = r - gi.row0, H = gi.H, W = gi.W;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const float sdt = c.steric_strength * c.dt;
+  float* out = vbar + gi.grid_off + (size_t)y * W;
+  for (int x0 = 0; x0 < W; x0 += BAR_THREADS) {
+    const int x = x0 + tid;
+    float acc = 0.0f;
+    if (tid == 0) ltop = 0;
+    __syncthreads();
+    auto flush = [&]() {
+      const int n = ltop;
+      if (x < W)
+        for (int k = 0; k < n; ++k) {
+          const float4 q = lst[k];
+          acc = bar_sample(acc, -q.z, q.y, q.x + 0.5f, H, W, y, x);
+          acc = bar_sample(acc, q.z, q.y, q.x - 0.5f, H, W, y, x);
+          acc = bar_sample(acc, -q.w, q.y + 0.5f, q.x, H, W, y, x);
+          acc = bar_sample(acc, q.w, q.y - 0.5f, q.x, H, W, y, x);
+        }
+      __syncthreads();
+      if (tid == 0) ltop = 0;
+      __syncthreads();
+    };
+    for (int i0 = 0; i0 < gi.nodes; i0 += BAR_THREADS) {
+      const int i = i0 + tid;
+      float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      bool hit = false;
+      if (i < gi.nodes) {
+        const int g = gi.node0 + i;
+        const float2 p = *reinterpret_cast<const float2*>(pred_w + 2 * g);
+        hit = touches_row(p.y, H, y);
+        e = make_float4(p.x, p.y, pb1[2 * g] * sdt, pb1[2 * g + 1] * sdt);
+      }
+      const unsigned long long m = __ballot(hit);
+      if (lane == 0) wcnt[wv] = __popcll(m);
+      __syncthreads();
+      int base = ltop;
+      for (int w = 0; w < wv; ++w) base += wcnt[w];
+      if (hit) lst[base + __popcll(m & ((1ull << lane) - 1ull))] = e;
+      __syncthreads();
+      if (tid == 0) {
+        int t = ltop;
+        for (int w = 0; w < BAR_THREADS / 64; ++w) t += wcnt[w];
+        ltop = t;
+      }
+      __syncthreads();
+      if (ltop > BAR_LIST - BAR_THREADS) flush();
+    }
+    flush();
+    if (x < W) out[x] = acc;
+    __syncthreads();
+  }
+}
+
+// Bilinear slopes (d/dx, d/dy) of the wrapped sample of V at (yy, xx).
+__device__ __forceinline__ void sample_slopes(const float* V, int H, int W, float yy, float xx,
+                                              float& sx, float& sy) {
+  const float ly = floorf(yy), lx = floorf(xx);
+  const float wyh = yy - ly, wxh = xx - lx;
+  const float wyl = 1.0f - wyh, wxl = 1.0f - wxh;
+  const int iy = f2i(ly), ix = f2i(lx);
+  const int y0 = imod(iy, H), y1 = imod(iy + 1, H);
+  const int x0 = imod(ix, W), x1 = imod(ix + 1, W);
+  const float v00 = V[(size_t)y0 * W + x0], v01 = V[(size_t)y0 * W + x1];
+  const float v10 = V[(size_t)y1 * W + x0], v11 = V[(size_t)y1 * W + x1];
+  sx = wyl * (v01 - v00) + wyh * (v11 - v10);
+  sy = wxl * (v10 - v00) + wxh * (v11 - v01);
+}
+
+// Per node: the direct sample term, the deposit term (from Dbar = G Vbar) and
+// the window term; then the prologue adjoint xb = pb0 - vb / dt, vb = pb0 d dt.
+__global__ void k_steric_adj_nodes(Consts c, BatchDev bd, int N, const float* __restrict__ pred_w,
+                                   const float* __restrict__ V_all, const float* __restrict__ D_all,
+                                   const float* __restrict__ pb1, float* __restrict__ xb,
+                                   float* __restrict__ vb) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const InstGeo gi = bd.geo[bd.node_inst[n]];
+  const int H = gi.H, W = gi.W;
+  const float* V = V_all + gi.grid_off;
+  const float* D = D_all + gi.grid_off;
+  const float sdt = c.steric_strength * c.dt;
+  const float px = pred_w[2 * n], py = pred_w[2 * n + 1];
+  const float gx = pb1[2 * n] * sdt, gy = pb1[2 * n + 1] * sdt;
+  float ox = 0.0f, oy = 0.0f;
+  // direct: gx_f = S(y, x+.5) - S(y, x-.5), gy_f = S(y+.5, x) - S(y-.5, x)
+  float ax, ay, bx, by;
+  sample_slopes(V, H, W, py, px + 0.5f, ax, ay);
+  sample_slopes(V, H, W, py, px - 0.5f, bx, by);
+  const float j00 = ax - bx, j01 = ay - by;
+  sample_slopes(V, H, W, py + 0.5f, px, ax, ay);
+  sample_slopes(V, H, W, py - 0.5f, px, bx, by);
+  const float j10 = ax - bx, j11 = ay - by;
+  ox = ox - (j00 * gx + j10 * gy);
+  oy = oy - (j01 * gx + j11 * gy);
+  // deposit: sv sum_c Dbar[c] dw_c / dp
+  const float fx = floorf(px), fy = floorf(py);
+  const float dx = px - fx, dy = py - fy;
+  const int x0 = imod(f2i(fx), W), y0 = imod(f2i(fy), H);
+  const int x1 = imod(x0 + 1, W), y1 = imod(y0 + 1, H);
+  const float d00 = D[(size_t)y0 * W + x0], d10 = D[(size_t)y0 * W + x1];
+  const float d01 = D[(size_t)y1 * W + x0], d11 = D[(size_t)y1 * W + x1];
+  ox = ox + c.steric_sv * ((1.0f - dy) * (d10 - d00) + dy * (d11 - d01));
+  oy = oy + c.steric_sv * ((1.0f - dx) * (d01 - d00) + dx * (d11 - d10));
+  // window (flat node index within the instance)
+  const int lo = gi.node0, hi = gi.node0 + gi.nodes;
+  const float inv_s2 = 2.0f / c.steric_2s2;
+  float wx = 0.0f, wy = 0.0f;
+  for (int k = -c.steric_nskip; k <= c.steric_nskip; ++k) {
+    const int j = n + k;
+    if (j < lo || j >= hi) continue;
+    const float rx = min_image(px, pred_w[2 * j], gi.dom_wf);
+    const float ry = min_image(py, pred_w[2 * j + 1], gi.dom_hf);
+    const float Kv = dexp(-(rx * rx + ry * ry) / c.steric_2s2) / c.steric_norm;
+    const float dgx = pb1[2 * j] * sdt - gx, dgy = pb1[2 * j + 1] * sdt - gy;
+    const float rd = (rx * dgx + ry * dgy) * inv_s2;
+    wx = wx + Kv * (dgx - rx * rd);
+    wy = wy + Kv * (dgy - ry * rd);
+  }
+  ox = ox + c.steric_svs * wx;
+  oy = oy + c.steric_svs * wy;
+  const float p0x = pb1[2 * n] + ox, p0y = pb1[2 * n + 1] + oy;
+  const float vbx = vb[2 * n], vby = vb[2 * n + 1];
+  xb[2 * n] = p0x - vbx / c.dt;
+  xb[2 * n + 1] = p0y - vby / c.dt;
+  vb[2 * n] = (p0x * c.damping) * c.dt;
+  vb[2 * n + 1] = (p0y * c.damping) * c.dt;
+}
+
 }  // namespace
+
+hipError_t launch_steric_adjoint(const Consts& c, const BatchDev& b, int N, const float* pred_w,
+                                 const float* V, const float* pb1, float* vbar, float* dbar,
+                                 float2* specT, const float* kfT, float* xb, float* vb,
+                                 hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_steric_bar_rows, dim3(b.rows), dim3(BAR_THREADS), 0, s, c, b, pred_w, pb1,
+                     vbar);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = launch_steric_fft_T(b, vbar, nullptr, specT, kfT, nullptr, dbar, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_steric_adj_nodes, dim3((N + 255) / 256), dim3(256), 0, s, c, b, N, pred_w,
+                     V, dbar, pb1, xb, vb);
+  return hipGetLastError();
+}
 
 size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse) {
   (void)cycles;

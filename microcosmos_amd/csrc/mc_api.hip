@@ -180,6 +180,7 @@ struct mc_plan {
   float* adj_tape = nullptr;
   size_t adj_tape_cap = 0;
   float* adj_work = nullptr;
+  float* adj_steric = nullptr;  // steric term: force (2N), pb1 (2N), V / Vbar / Dbar grids
   bool adj_ok = false;
   int64_t adj_steps = 0;
   int adj_every = 0;
@@ -1430,9 +1431,17 @@ namespace {
 mc_status check_rod_adjoint(const mc_params* q) {
   if (!q) return fail(MC_ERR_INVALID, "params is NULL");
   if ((q->solver_type != MC_SOLVER_PBD && q->solver_type != MC_SOLVER_COSSERAT) ||
-      q->enable_fluid || q->enable_steric)
+      q->enable_fluid)
     return fail(MC_ERR_UNSUPPORTED,
-                "the rod adjoint covers the rod-only step with enable_fluid = enable_steric = 0");
+                "the rod adjoint covers the step without the fluid (enable_fluid = 0)");
+  return MC_OK;
+}
+
+// Steric-term scratch of the adjoint: per-node force and pb1, three grids.
+mc_status ensure_adj_steric(mc_plan* p) {
+  if (!p->adj_steric)
+    MCCHK(p->alloc(&p->adj_steric, 4 * (size_t)p->N + 3 * (size_t)p->grid_cells,
+                   "adjoint steric scratch"));
   return MC_OK;
 }
 
@@ -1446,12 +1455,15 @@ mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state*
   g_err.clear();
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   MCCHK(check_rod_adjoint(q));
-  MCCHK(check_params(p, q, false, false));
+  MCCHK(check_params(p, q, false, true));
   MCCHK(check_state(p, st, "state"));
   MCCHK(check_edges(p, ed));
   if (num_steps < 1) return fail(MC_ERR_BAD_SHAPE, "num_steps < 1");
   if (checkpoint_every < 1) return fail(MC_ERR_BAD_SHAPE, "checkpoint_every < 1");
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
+  // the steric term couples all filaments every step: every step's state is kept
+  const bool steric = q->enable_steric != 0;
+  if (steric) checkpoint_every = 1;
   const size_t lds = rod_segment_lds_bytes(q->cycles_per_step, p->comp_max_nn, p->comp_max_ne,
                                            p->comp_max_nb, true);
   if (lds + 1024 > (size_t)p->lds_max)
@@ -1480,8 +1492,19 @@ mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state*
   a.kb = ed->bending_stiffness;
   const size_t lds_f = rod_segment_lds_bytes(q->cycles_per_step, p->comp_max_nn, p->comp_max_ne,
                                              p->comp_max_nb, false);
+  if (steric) {
+    MCCHK(ensure_geo(p, q));
+    MCCHK(ensure_kf(p, q->steric_sigma, s));
+    MCCHK(ensure_adj_steric(p));
+    a.fst = p->adj_steric;
+  }
   for (int64_t seg = 0; seg < nck; ++seg) {
     const int len = (int)std::min<int64_t>(checkpoint_every, num_steps - seg * checkpoint_every);
+    if (steric) {
+      // the step's prediction, field and per-node force, as the coupled step
+      MCCHK(run_steric(p, a.c, st->position, st->velocity, st->steric, nullptr, s));
+      HIPCHK(launch_steric_nodes(a.c, p->bd, p->N, st->steric, p->pred_w, p->adj_steric, s));
+    }
     HIPCHK(launch_rod_segment(a, false, len, p->adj_ckpt + (size_t)seg * S, st->debug_vector,
                               lds_f, s));
   }
@@ -1542,9 +1565,30 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
   a.thb = thb;
   a.rest_bar = rest_angle_bar;
   const int64_t nck = (T + every - 1) / every;
+  const bool steric = q->enable_steric != 0;
+  float *fst = nullptr, *pb1 = nullptr, *vg = nullptr, *vbar = nullptr, *dbar = nullptr;
+  if (steric) {
+    MCCHK(ensure_adj_steric(p));
+    fst = p->adj_steric;
+    pb1 = fst + 2 * (size_t)N;
+    vg = pb1 + 2 * (size_t)N;
+    vbar = vg + p->grid_cells;
+    dbar = vbar + p->grid_cells;
+    a.fst = fst;
+    a.pb1 = pb1;
+  }
   for (int64_t seg = nck - 1; seg >= 0; --seg) {
     const int len = (int)std::min<int64_t>(every, T - seg * every);
-    HIPCHK(launch_rod_segment(a, true, len, p->adj_ckpt + (size_t)seg * S, p->adj_tape, lds, s));
+    const float* ck = p->adj_ckpt + (size_t)seg * S;
+    if (steric) {
+      // recompute the step's prediction, field and force from its state
+      MCCHK(run_steric(p, a.c, ck, ck + 2 * (size_t)N, vg, nullptr, s));
+      HIPCHK(launch_steric_nodes(a.c, p->bd, N, vg, p->pred_w, fst, s));
+    }
+    HIPCHK(launch_rod_segment(a, true, len, const_cast<float*>(ck), p->adj_tape, lds, s));
+    if (steric)
+      HIPCHK(launch_steric_adjoint(a.c, p->bd, N, p->pred_w, vg, pb1, vbar, dbar, p->spec, p->kf,
+                                   xb, vb, s));
   }
   if (position0_bar) HIPCHK(hipMemcpyAsync(position0_bar, xb, n2, hipMemcpyDeviceToDevice, s));
   if (velocity0_bar) HIPCHK(hipMemcpyAsync(velocity0_bar, vb, n2, hipMemcpyDeviceToDevice, s));

@@ -196,10 +196,25 @@ struct RodAdjArgs {
   const float *rest_len, *rest_ang, *kb;
   float *xb, *vb, *thb;                        // adjoints: segment end in, start out
   float* rest_bar;                             // (B) += d loss / d rest angles
+  // steric term (one-step segments only): fst = the step's steric force per
+  // node, added to the prediction; pb1 (reverse) = output of d loss / d the
+  // pushed prediction, whose steric and prologue adjoint launch_steric_adjoint
+  // finishes (a.xb / a.vb are then left to it)
+  const float* fst;
+  float* pb1;
 };
 size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse);
 // forward: ck = checkpoint slot written, tape_or_dbg = debug_vector (N, 2);
 // reverse: ck = checkpoint slot read, tape_or_dbg = tape (steps x cycles x E float4)
+// Adjoint of the steric push p1 = p0 + f(p0) strength dt (forces.py:41-118)
+// and of the prediction p0 = x + (v d) dt, given pb1 = d loss / d p1, the
+// wrapped prediction pred_w and its filtered field V: writes xb = d loss / d x
+// and vb = d loss / d v (vb holds d loss / d v_new on entry).  vbar and dbar
+// are grid-sized scratch.
+hipError_t launch_steric_adjoint(const Consts& c, const BatchDev& b, int N, const float* pred_w,
+                                 const float* V, const float* pb1, float* vbar, float* dbar,
+                                 float2* specT, const float* kfT, float* xb, float* vb,
+                                 hipStream_t s);
 hipError_t launch_rod_segment(const RodAdjArgs& a, bool reverse, int steps, float* ck,
                               float* tape_or_dbg, size_t lds_bytes, hipStream_t s);
 // immersed boundary (solved per interaction cluster)

@@ -9,9 +9,11 @@ and the reverse pass recomputes each segment before running the per-step
 adjoint kernel backwards through it, so the device memory of a gradient is
 (checkpoints + one segment) x state instead of the whole trajectory.
 
-Scope: the rod-only step (PBD or stable Cosserat constraint,
-``enable_fluid=False``, ``enable_steric=False``) with the bending rest angles
-as the parameters.
+Scope: the step without the fluid (PBD or stable Cosserat constraint,
+``enable_fluid=False``, with or without the steric term) with the bending
+rest angles as the parameters.  The steric term couples every filament each
+step, so with it every step's state is kept (``checkpoint_every`` is 1) and
+each reverse step recomputes that step's steric field.
 """
 from __future__ import annotations
 
@@ -60,10 +62,10 @@ class RodAdjoint:
     """
 
     def __init__(self, nodes: Nodes, edges: Edges, fields: Fields, solver_config, dt: float):
-        if solver_config.enable_fluid or getattr(solver_config, "enable_steric", False) or \
+        if solver_config.enable_fluid or \
                 getattr(solver_config, "solver_type", "pbd") not in ("pbd", "cosserat"):
             raise NotImplementedError("rod adjoint: PBD or stable Cosserat constraint with "
-                                      "enable_fluid=False and enable_steric=False")
+                                      "enable_fluid=False")
         d = nodes.position.device
         if d.type != "cuda":
             raise RuntimeError("microcosmos_amd runs on the GPU only")
@@ -75,7 +77,7 @@ class RodAdjoint:
         self.debug_vector = as_f32(nodes.debug_vector, d).clone()
         self.theta = as_f32(edges.theta, d).clone()
         self.grid_shape = tuple(fields.grid_shape)
-        self.steric = as_f32(fields.steric, d)
+        self.steric = as_f32(fields.steric, d).clone()
         self.fluid_velocity = as_f32(fields.fluid_velocity, d)
         self.f_grid = as_f32(fields.f_grid, d)
         from ._runtime import _host_topology

@@ -246,6 +246,17 @@ def serving_trace(n=48, seed=4, rate=200.0, sides=(128, 256, 512, 1024), max_fil
     return out
 
 
+# The folding experiment's default physics (experiments/conf/experiment/
+# filament_folding.yaml:53-90, assembled by experiment.py:925-941): stable
+# Cosserat with the steric term at dt 0.02, 150 simulated steps.  `dt` of the
+# Cosserat scheme is its inertia time step and must equal the step's dt.
+FOLDING_DT = 0.02
+FOLDING_STEPS = 150
+FOLDING_COSSERAT = dict(cycles_per_step=10, stiffness_stretch=55000.0, stiffness_shear=1.0,
+                        damping=0.98, enable_steric=True, steric_strength=1300.0,
+                        steric_sigma=2.0, steric_scatter_value=1.0, steric_neighbor_skip=3)
+
+
 def folding(seed=3, grid=(256, 256), n_fil=8, n_nodes=48, steps=500, modes=4, amplitude=5.0):
     """Differentiable folding: ``n_fil`` straight filaments of ``n_nodes``
     nodes (uniform start points and headings) on a ``grid`` node domain, the

@@ -3,13 +3,15 @@
 TEST INFRASTRUCTURE ONLY, like ``step_ref``: imported by ``tests/`` as the
 checker of the device adjoint, never by the product path.
 
-Scope: ``step`` (simulate.py:37-100) with ``enable_fluid = False``,
-``enable_steric = False`` and the PBD constraint (pbd.py:14-91) or the stable
-Cosserat constraint (stable_cosserat.py:13-119, see ``_cosserat_cycles_vjp``),
-the configurations of the reference's differentiable folding runs without the
-steric term (experiments/filament_folding/experiment.py:52-53, 985-1018 take
-``jax.value_and_grad`` through ``lax.scan`` of this step).  The derivative is
-written out by hand from the forward equations of ``step_ref``:
+Scope: ``step`` (simulate.py:37-100) with ``enable_fluid = False``, the PBD
+constraint (pbd.py:14-91) or the stable Cosserat constraint
+(stable_cosserat.py:13-119, see ``_cosserat_cycles_vjp``), with or without
+the steric term (forces.py:20-118, see ``steric_vjp``): the configurations of
+the reference's differentiable folding runs
+(experiments/filament_folding/experiment.py:52-53, 925-941, 985-1018 take
+``jax.value_and_grad`` through ``lax.scan`` of this step; its default physics
+is stable Cosserat with the steric term).  The derivative is written out by
+hand from the forward equations of ``step_ref``:
 
 forward, one step (x, v, theta; rest angles rho):
     p0 = x + (v * d) * dt                                    simulate.py:46-50
@@ -33,14 +35,18 @@ reverse (adjoints xb, vb, thb of x_new, v_new, theta_new):
       corrb_b = omega (thb'[e_in] / deg_in - thb'[e_out] / deg_out)
       rhob_b -= corrb_b kb_b / 2;  diffb_b = corrb_b kb_b / 2
       thb[e_out] += diffb_b;  thb[e_in] -= diffb_b
-    xb_prev = pb + xb_direct;  vb_prev = pb * d * dt
+    steric push p1 = p0 + f(p0) strength dt:  pb0 = pb1 + (df/dp0)^T pb1 strength dt
+    xb_prev = pb0 + xb_direct;  vb_prev = pb0 * d * dt
 
 The minimum image, the angle wrap and the final periodic wrap have derivative
 1 almost everywhere; their jumps are not differentiated (as in the reference's
 ``jax.grad``, whose ``%`` derivative is 1).
 
 Pinned by central finite differences of the float64 forward (``step_ref.step``)
-in ``tests/test_rod_adjoint_oracle.py``.
+in ``tests/test_rod_adjoint_oracle.py``: per step (directional derivatives
+for x, v and theta, both solvers, steric on and off), the steric force's VJP
+alone, and whole rollouts (rest-angle gradients), including the folding
+experiment's default physics.
 """
 from __future__ import annotations
 
@@ -65,9 +71,77 @@ def _degrees(s: R.State):
 
 
 def _check(p: R.Params):
-    if p.enable_fluid or p.enable_steric or p.solver_type not in ("pbd", "cosserat"):
+    if p.enable_fluid or p.solver_type not in ("pbd", "cosserat"):
         raise NotImplementedError("the rod adjoint covers the rod-only step (PBD or stable "
-                                  "Cosserat) without fluid and steric")
+                                  "Cosserat, with or without the steric term) without fluid")
+
+
+def _sample_w(V, yy, xx):
+    """Corners, weights and the x / y derivatives of the bilinear sample
+    sample_bilinear_wrap(V, yy, xx) (forces.py:20-38 via map_coordinates)."""
+    h, w = V.shape
+    ly, lx = np.floor(yy), np.floor(xx)
+    wy1, wx1 = yy - ly, xx - lx
+    wy0, wx0 = 1.0 - wy1, 1.0 - wx1
+    iy, ix = ly.astype(np.int64), lx.astype(np.int64)
+    y0, y1 = np.mod(iy, h), np.mod(iy + 1, h)
+    x0, x1 = np.mod(ix, w), np.mod(ix + 1, w)
+    corners = [(y0, x0, wy0 * wx0), (y0, x1, wy0 * wx1), (y1, x0, wy1 * wx0), (y1, x1, wy1 * wx1)]
+    dsx = wy0 * (V[y0, x1] - V[y0, x0]) + wy1 * (V[y1, x1] - V[y1, x0])
+    dsy = wx0 * (V[y1, x0] - V[y0, x0]) + wx1 * (V[y1, x1] - V[y0, x1])
+    return corners, dsx, dsy
+
+
+def steric_vjp(pos0, grid_shape, p: R.Params, gbar):
+    """(d f / d pos0)^T gbar for f = compute_field_steric_force_corrected(pos0)
+    (forces.py:41-118): f = -grad_FD V(pos) - c sum_j K(r_ij) r_ij with
+    V = G * D(pos), D the CIC deposit, G the Gaussian filter (a symmetric
+    operator, so its adjoint is itself), c = sv / sigma^2 and
+    d(K r)/dr = K (I - r r^T / sigma^2)."""
+    sigma, nsk, sv = float(p.steric_sigma), int(p.steric_neighbor_skip), float(p.steric_scatter_value)
+    pos = np.asarray(R.periodic_boundary(grid_shape, np.asarray(pos0, np.float64)), np.float64)
+    g = np.asarray(gbar, np.float64)
+    n = pos.shape[0]
+    _, V = R.update_steric_potential(pos, grid_shape, sigma, sv)
+    V = np.asarray(V, np.float64)
+    out = np.zeros((n, 2))
+    # window term: out_m += c sum_j J(r_mj) (g_j - g_m)
+    c = sv / (sigma * sigma)
+    for k in range(-nsk, nsk + 1):
+        j = np.arange(n) + k
+        valid = (j >= 0) & (j < n)
+        js = np.clip(j, 0, n - 1)
+        r = np.asarray(R.displacement(grid_shape, pos, pos[js]), np.float64)
+        d2 = (r * r).sum(axis=1)
+        K = np.exp(-d2 / (2 * sigma * sigma)) / (2 * np.pi * sigma * sigma)
+        dg = g[js] - g
+        rd = (r * dg).sum(axis=1)
+        out += (c * valid * K)[:, None] * (dg - r * (rd / (sigma * sigma))[:, None])
+    # field term f = -(gx, gy), gx = S(y, x+e) - S(y, x-e), gy = S(y+e, x) - S(y-e, x)
+    e = 0.5
+    x, y = pos[:, 0], pos[:, 1]
+    samples = [(y, x + e, 0, 1.0), (y, x - e, 0, -1.0), (y + e, x, 1, 1.0), (y - e, x, 1, -1.0)]
+    Vbar = np.zeros_like(V)
+    jac = np.zeros((n, 2, 2))  # d(gx, gy)/d(x, y)
+    for yy, xx, comp, sgn in samples:
+        corners, dsx, dsy = _sample_w(V, yy, xx)
+        jac[:, comp, 0] += sgn * dsx / (2 * e)
+        jac[:, comp, 1] += sgn * dsy / (2 * e)
+        for cy, cx, wgt in corners:
+            np.add.at(Vbar, (cy, cx), -g[:, comp] * sgn * wgt / (2 * e))
+    out[:, 0] += -(jac[:, 0, 0] * g[:, 0] + jac[:, 1, 0] * g[:, 1])
+    out[:, 1] += -(jac[:, 0, 1] * g[:, 0] + jac[:, 1, 1] * g[:, 1])
+    Dbar = np.asarray(R.gaussian_filter(Vbar, sigma), np.float64)
+    h, w = grid_shape
+    fx, fy = np.floor(x), np.floor(y)
+    dx, dy = x - fx, y - fy
+    x0 = np.mod(fx.astype(np.int64), w)
+    y0 = np.mod(fy.astype(np.int64), h)
+    x1, y1 = np.mod(x0 + 1, w), np.mod(y0 + 1, h)
+    d00, d10, d01, d11 = Dbar[y0, x0], Dbar[y0, x1], Dbar[y1, x0], Dbar[y1, x1]
+    out[:, 0] += sv * (-(1 - dy) * d00 + (1 - dy) * d10 - dy * d01 + dy * d11)
+    out[:, 1] += sv * (-(1 - dx) * d00 - dx * d10 + (1 - dx) * d01 + dx * d11)
+    return out
 
 
 def step_tape(s: R.State, dt: float, p: R.Params):
@@ -78,6 +152,11 @@ def step_tape(s: R.State, dt: float, p: R.Params):
     prev = s.position
     s.velocity = s.velocity * p.damping
     s.position = prev + s.velocity * dt
+    if p.enable_steric:
+        _, s.steric, fst = R.compute_field_steric_force_corrected(
+            s.position, s.grid_shape, p.steric_sigma, p.steric_neighbor_skip,
+            p.steric_scatter_value)
+        s.position = s.position + fst * p.steric_strength * dt
     cos = p.solver_type == "cosserat"
     if cos:
         s.debug_vector = s.position
@@ -140,6 +219,15 @@ def _cosserat_cycles_vjp(s, p, ps, ths, pb, thb, rhob, deg_e):
     return pb, thb, yb, rhob
 
 
+def _prologue_vjp(s, dt, p: R.Params, pb):
+    """Adjoint of the steric push (simulate.py:52-63) on the prediction
+    pred0 = x + (v d) dt:  pred1 = pred0 + f(pred0) strength dt."""
+    if not p.enable_steric:
+        return pb
+    pred0 = s.position + (s.velocity * p.damping) * dt
+    return pb + steric_vjp(pred0, s.grid_shape, p, pb * float(p.steric_strength) * dt)
+
+
 def step_vjp(s: R.State, dt: float, p: R.Params, xb, vb, thb):
     """Adjoint of one step at state ``s``: returns (xb_prev, vb_prev,
     thb_prev, rhob) given the adjoints of the step's outputs."""
@@ -159,6 +247,7 @@ def step_vjp(s: R.State, dt: float, p: R.Params, xb, vb, thb):
     if p.solver_type == "cosserat":
         pb, thb, yb, rhob = _cosserat_cycles_vjp(s, p, ps, ths, pb, thb, rhob, deg_e)
         pb = pb + yb  # y is the prologue's prediction (simulate.py:71-72)
+        pb = _prologue_vjp(s, dt, p, pb)
         return pb + xb_direct, pb * float(p.damping) * dt, thb, rhob
     for c in reversed(range(p.cycles_per_step)):
         pos = np.asarray(ps[c], np.float64)
@@ -182,6 +271,7 @@ def step_vjp(s: R.State, dt: float, p: R.Params, xb, vb, thb):
         rhob = rhob - diffb
         np.add.at(thb, bout, diffb)
         np.add.at(thb, bin_, -diffb)
+    pb = _prologue_vjp(s, dt, p, pb)
     return pb + xb_direct, pb * float(p.damping) * dt, thb, rhob
 
 
@@ -195,10 +285,15 @@ def msd_loss(position, target, grid_shape):
     return float((d * d).sum() / n), 2.0 * d / n
 
 
-def rollout_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, target):
+def rollout_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, target,
+                 dtype=np.float64):
     """Loss after ``num_steps`` steps and its gradient with respect to the
-    (time-constant) bending rest angles, by storing every step's state."""
-    states = [s0.astype(np.float64)]
+    (time-constant) bending rest angles, by storing every step's state.
+    ``dtype`` is the precision of the forward trajectory (and of each step's
+    recomputed tape); the adjoint arithmetic is float64 either way.  A float32
+    trajectory is the device's own (bit-identical steps), which separates the
+    adjoint's accuracy from the divergence of float32 and float64 rollouts."""
+    states = [s0.astype(dtype)]
     for _ in range(num_steps):
         states.append(R.step(states[-1], dt, p))
     loss, xb = msd_loss(states[-1].position, target, s0.grid_shape)

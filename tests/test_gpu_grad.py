@@ -32,17 +32,20 @@ def dev():
     return torch.device("cuda", 0)
 
 
-def _small(seed=11, solver="pbd"):
+def _small(seed=11, solver="pbd", steric=False):
     wl = W.small(grid=(32, 32), n_fil=2, n_nodes=12, seed=seed)
     rng = np.random.default_rng(seed)
     wl.bending_rest_angles = rng.normal(0.0, 0.2, wl.bending_pairs.shape[0]).astype(np.float32)
     wl.velocity = rng.normal(0.0, 0.05, wl.position.shape).astype(np.float32)
     target = (wl.position + rng.normal(0.0, 1.0, wl.position.shape)).astype(np.float32)
     if solver == "cosserat":
-        over = dict(cycles_per_step=4, damping=0.5, stiffness_stretch=5.0, enable_steric=False)
+        # steric strength 10: the scheme's 100 at dt 1 makes this small case
+        # chaotic (float32 and float64 trajectories part within 20 steps)
+        over = dict(cycles_per_step=4, damping=0.5, stiffness_stretch=5.0, enable_steric=steric,
+                    steric_strength=10.0)
     else:
         over = dict(cycles_per_step=4, damping=0.5, stiffness_stretch=0.6, stiffness_shear=0.8,
-                    enable_steric=False)
+                    enable_steric=steric)
     return wl, target, over, solver
 
 
@@ -73,10 +76,11 @@ def _oracle(wl, over, solver="pbd"):
     return s, p
 
 
+@pytest.mark.parametrize("steric", [False, True])
 @pytest.mark.parametrize("solver", ["pbd", "cosserat"])
-def test_adjoint_matches_restatement(dev, solver):
-    _say(f"rod adjoint ({solver}), 2 x 12 nodes, 20 steps, checkpoints every 7")
-    wl, target, over, solver = _small(solver=solver)
+def test_adjoint_matches_restatement(dev, solver, steric):
+    _say(f"rod adjoint ({solver}, steric {steric}), 2 x 12 nodes, 20 steps, checkpoints every 7")
+    wl, target, over, solver = _small(solver=solver, steric=steric)
     loss, g = _device_grad(wl, target, over, 20, 7, dev, solver)
     s, p = _oracle(wl, over, solver)
     lo, go = A.rollout_grad(s, wl.dt, p, 20, target)
@@ -84,17 +88,22 @@ def test_adjoint_matches_restatement(dev, solver):
     rel = np.linalg.norm(g - go) / np.linalg.norm(go)
     cos = g @ go / (np.linalg.norm(g) * np.linalg.norm(go))
     _say(f"rel-L2 {rel:.3e} cosine {cos:.9f}")
-    assert rel <= 1e-4 and cos >= 0.99999
+    # Cosserat with the steric term is ill-conditioned here: perturbing the
+    # initial positions by 1e-7 (relative, float32 rounding) moves the float64
+    # gradient by 5e-4 to 7.5e-4, so float32 cannot agree more closely
+    tol = 3e-3 if (steric and solver == "cosserat") else 1e-4
+    assert rel <= tol and cos >= 0.99999
 
 
+@pytest.mark.parametrize("steric", [False, True])
 @pytest.mark.parametrize("solver", ["pbd", "cosserat"])
-def test_adjoint_forward_equals_rollout(dev, solver):
+def test_adjoint_forward_equals_rollout(dev, solver, steric):
     """The forward segments of the adjoint reproduce the plain rollout bit for
-    bit (rod-only step)."""
-    _say(f"rod adjoint forward ({solver}) vs mc_rollout, 23 steps")
+    bit (step without the fluid; with the steric term, the field too)."""
+    _say(f"rod adjoint forward ({solver}, steric {steric}) vs mc_rollout, 23 steps")
     from microcosmos_amd import Rollout
     from microcosmos_amd.grad import RodAdjoint
-    wl, target, over, solver = _small(seed=8, solver=solver)
+    wl, target, over, solver = _small(seed=8, solver=solver, steric=steric)
     cfg = replace(_base(solver), **over)
     nodes, edges, fields = device_state(wl, dev)
     ad = RodAdjoint(nodes, edges, fields, cfg, wl.dt)
@@ -102,7 +111,8 @@ def test_adjoint_forward_equals_rollout(dev, solver):
     ro = Rollout(nodes, edges, fields, cfg, wl.dt)
     ro.run(23, use_graph=False)
     torch.cuda.synchronize()
-    for name in ("position", "velocity", "theta", "debug_vector"):
+    names = ("position", "velocity", "theta", "debug_vector") + (("steric",) if steric else ())
+    for name in names:
         np.testing.assert_array_equal(getattr(ad, name).cpu().numpy(),
                                       getattr(ro, name).cpu().numpy(), err_msg=name)
 
@@ -166,3 +176,46 @@ def test_folding_gradient_cosserat(dev):
     fd = (fp - fm) / (2 * h)
     _say(f"adjoint {g @ u:.6e}  central FD {fd:.6e}")
     assert abs(g @ u - fd) <= 1e-3 * abs(fd) + 1e-6
+
+
+def test_folding_gradient_reference_physics(dev):
+    """The folding experiment's default physics -- stable Cosserat with the
+    steric term, dt 0.02 (experiments/conf/experiment/filament_folding.yaml
+    :53-90) -- on the 8 x 48-node folding workload, 75 steps.
+
+    This rollout is sensitive: the float32 and float64 trajectories part by
+    1e-4 in the loss and 1.2 % in the gradient.  So the device gradient is
+    held tightly against the restatement's adjoint along the float32
+    trajectory (the device's own: its steps are bit-identical to the float32
+    restatement), loosely (cosine) against the float64 one, and the float64
+    restatement against a central difference (h = 1e-6) of its forward."""
+    _say("folding (stable Cosserat + steric, dt 0.02): 8 x 48 nodes, 75 steps")
+    from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID
+    from microcosmos_amd.grad import rollout_value_and_grad
+    steps = 75
+    wl, target = W.folding(steps=steps)
+    nodes, edges, fields = device_state(wl, dev)
+    cfg = replace(COSSERAT_SCHEME_NO_FLUID, dt=W.FOLDING_DT, **W.FOLDING_COSSERAT)
+    loss, g = rollout_value_and_grad(nodes, edges, fields, cfg, W.FOLDING_DT, steps,
+                                     torch.as_tensor(target, device=dev))
+    torch.cuda.synchronize()
+    loss, g = float(loss.item()), g.cpu().numpy().astype(np.float64)
+    s = oracle_state(wl, np.float64)
+    p = replace(R.COSSERAT_SCHEME_NO_FLUID, config_dt=W.FOLDING_DT, **W.FOLDING_COSSERAT)
+    l32, g32 = A.rollout_grad(s, W.FOLDING_DT, p, steps, target, dtype=np.float32)
+    rel = np.linalg.norm(g - g32) / np.linalg.norm(g32)
+    _say(f"loss {loss:.7f} (float32-trajectory restatement {l32:.7f}); rel-L2 {rel:.3e}")
+    assert abs(loss - l32) <= 1e-6 * abs(l32)
+    assert rel <= 1e-3
+    lo, go = A.rollout_grad(s, W.FOLDING_DT, p, steps, target)
+    cos = g @ go / (np.linalg.norm(g) * np.linalg.norm(go))
+    _say(f"float64 restatement: loss {lo:.7f}, cosine {cos:.7f}")
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    assert cos >= 0.999
+    u = np.random.default_rng(9).normal(size=go.shape[0])
+    h = 1e-6
+    fp = A.rollout_loss(s, W.FOLDING_DT, p, steps, target, rest=s.bending_rest_angles + h * u)
+    fm = A.rollout_loss(s, W.FOLDING_DT, p, steps, target, rest=s.bending_rest_angles - h * u)
+    fd = (fp - fm) / (2 * h)
+    _say(f"float64 restatement {go @ u:.7e}  central FD {fd:.7e}  device {g @ u:.7e}")
+    assert abs(go @ u - fd) <= 1e-4 * abs(fd)

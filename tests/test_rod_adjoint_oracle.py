@@ -14,7 +14,7 @@ from microcosmos_amd import workloads as W
 from .helpers import oracle_state
 
 
-def _case(damping=0.5, cycles=4, seed=11, solver="pbd"):
+def _case(damping=0.5, cycles=4, seed=11, solver="pbd", steric=False):
     wl = W.small(grid=(32, 32), n_fil=2, n_nodes=12, seed=seed)
     rng = np.random.default_rng(seed)
     s = oracle_state(wl, np.float64)
@@ -22,10 +22,10 @@ def _case(damping=0.5, cycles=4, seed=11, solver="pbd"):
                 velocity=rng.normal(0.0, 0.05, s.position.shape))
     if solver == "cosserat":
         p = replace(R.COSSERAT_SCHEME_NO_FLUID, cycles_per_step=cycles, damping=damping,
-                    enable_steric=False, stiffness_stretch=5.0, config_dt=0.2)
+                    enable_steric=steric, stiffness_stretch=5.0, config_dt=0.2)
     else:
         p = replace(R.PBD_SCHEME_NO_FLUID, cycles_per_step=cycles, damping=damping,
-                    enable_steric=False, stiffness_stretch=0.6, stiffness_shear=0.8)
+                    enable_steric=steric, stiffness_stretch=0.6, stiffness_shear=0.8)
     target = s.position + rng.normal(0.0, 1.0, s.position.shape)
     return s, p, target
 
@@ -39,9 +39,10 @@ def _fd(f, x, h):
     return g
 
 
+@pytest.mark.parametrize("steric", [False, True])
 @pytest.mark.parametrize("solver", ["pbd", "cosserat"])
-def test_rollout_rest_angle_gradient_matches_fd(solver):
-    s, p, target = _case(solver=solver)
+def test_rollout_rest_angle_gradient_matches_fd(solver, steric):
+    s, p, target = _case(solver=solver, steric=steric)
     steps = 6
     loss, g = A.rollout_grad(s, 1.0, p, steps, target)
     fd = _fd(lambda r: A.rollout_loss(s, 1.0, p, steps, target, rest=r), s.bending_rest_angles, 1e-6)
@@ -51,12 +52,13 @@ def test_rollout_rest_angle_gradient_matches_fd(solver):
     assert cos > 0.999999
 
 
+@pytest.mark.parametrize("steric", [False, True])
 @pytest.mark.parametrize("solver", ["pbd", "cosserat"])
 @pytest.mark.parametrize("which", ["position", "velocity", "theta"])
-def test_step_vjp_directional_derivatives(which, solver):
+def test_step_vjp_directional_derivatives(which, solver, steric):
     """<vjp(w), u> equals the finite-difference derivative of <w, step(s)> in
     direction u, for the step inputs x, v and theta."""
-    s, p, _ = _case(damping=0.7, cycles=3, seed=5, solver=solver)
+    s, p, _ = _case(damping=0.7, cycles=3, seed=5, solver=solver, steric=steric)
     rng = np.random.default_rng(3)
     wx = rng.normal(size=s.position.shape)
     wv = rng.normal(size=s.position.shape)
@@ -85,7 +87,45 @@ def test_zero_damping_cuts_the_velocity_path():
     assert np.all(vb == 0.0)
 
 
-def test_rejects_fluid_and_steric():
+def test_rejects_fluid():
     s, p, target = _case()
     with pytest.raises(NotImplementedError):
-        A.step_vjp(s, 1.0, replace(p, enable_steric=True), s.position, s.position, s.theta)
+        A.step_vjp(s, 1.0, replace(p, enable_fluid=True), s.position, s.position, s.theta)
+
+
+def test_steric_vjp_directional_derivative():
+    """The steric force's VJP (window term, bilinear-sample term and the
+    deposit -> Gaussian filter -> sample path) against central differences of
+    compute_field_steric_force_corrected."""
+    s, p, _ = _case(seed=7, steric=True)
+    rng = np.random.default_rng(9)
+    g = rng.normal(size=s.position.shape)
+    u = rng.normal(size=s.position.shape)
+    out = A.steric_vjp(s.position, s.grid_shape, p, g)
+
+    def f(eps):
+        return np.sum(g * R.compute_field_steric_force_corrected(
+            s.position + eps * u, s.grid_shape, p.steric_sigma, p.steric_neighbor_skip,
+            p.steric_scatter_value)[2])
+
+    h = 1e-6
+    fd = (f(h) - f(-h)) / (2 * h)
+    assert abs(np.sum(out * u) - fd) <= 1e-7 * max(1.0, abs(fd)), (np.sum(out * u), fd)
+
+
+def test_folding_configuration_gradient_matches_fd():
+    """The folding experiment's default physics (stable Cosserat, steric term,
+    dt 0.02) on the 8 x 48-node folding workload, 30 steps: the rest-angle
+    gradient along a random direction against a central difference with
+    h = 1e-6 (CIC and bilinear kinks make larger steps and longer horizons
+    cross pieces of the piecewise-smooth loss)."""
+    wl, target = W.folding(steps=30)
+    s = oracle_state(wl, np.float64)
+    p = replace(R.COSSERAT_SCHEME_NO_FLUID, config_dt=W.FOLDING_DT, **W.FOLDING_COSSERAT)
+    loss, g = A.rollout_grad(s, W.FOLDING_DT, p, 30, target)
+    u = np.random.default_rng(9).normal(size=g.shape[0])
+    h = 1e-6
+    fp = A.rollout_loss(s, W.FOLDING_DT, p, 30, target, rest=s.bending_rest_angles + h * u)
+    fm = A.rollout_loss(s, W.FOLDING_DT, p, 30, target, rest=s.bending_rest_angles - h * u)
+    fd = (fp - fm) / (2 * h)
+    assert abs(g @ u - fd) <= 1e-5 * abs(fd), (g @ u, fd)
