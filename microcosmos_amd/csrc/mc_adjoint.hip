@@ -10,7 +10,8 @@
 // one-step segment that adds the force to the prediction; the reverse
 // recomputes that step's field and force from the kept state, runs the
 // one-step segment adjoint up to d loss / d (pushed prediction), and
-// launch_steric_adjoint finishes the step (see the steric section below).
+// launch_steric_bar, the Gaussian filter and launch_steric_adj_nodes finish
+// the step (see the steric section below).
 //
 // Without the fluid and the steric term the filaments do not interact, so one
 // workgroup (one wavefront: its barriers are cheap) per filament (connected
@@ -628,17 +629,18 @@ __global__ void k_steric_adj_nodes(Consts c, BatchDev bd, int N, const float* __
 
 }  // namespace
 
-hipError_t launch_steric_adjoint(const Consts& c, const BatchDev& b, int N, const float* pred_w,
-                                 const float* V, const float* pb1, float* vbar, float* dbar,
-                                 float2* specT, const float* kfT, float* xb, float* vb,
-                                 hipStream_t s) {
-  if (N == 0) return hipSuccess;
+hipError_t launch_steric_bar(const Consts& c, const BatchDev& b, const float* pred_w,
+                             const float* pb1, float* vbar, hipStream_t s) {
+  if (b.rows == 0) return hipSuccess;
   hipLaunchKernelGGL(k_steric_bar_rows, dim3(b.rows), dim3(BAR_THREADS), 0, s, c, b, pred_w, pb1,
                      vbar);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  e = launch_steric_fft_T(b, vbar, nullptr, specT, kfT, nullptr, dbar, s);
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_steric_adj_nodes(const Consts& c, const BatchDev& b, int N, const float* pred_w,
+                                   const float* V, const float* dbar, const float* pb1, float* xb,
+                                   float* vb, hipStream_t s) {
+  if (N == 0) return hipSuccess;
   hipLaunchKernelGGL(k_steric_adj_nodes, dim3((N + 255) / 256), dim3(256), 0, s, c, b, N, pred_w,
                      V, dbar, pb1, xb, vb);
   return hipGetLastError();

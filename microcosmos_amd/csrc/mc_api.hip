@@ -163,6 +163,8 @@ struct mc_plan {
   long long* tick = nullptr;
   float* kf = nullptr;          // Gaussian transfer functions, one table per distinct shape
   double kf_sigma = -1.0;       // sigma the table holds (-1: none)
+  bool filter_direct = false;   // some shape's table holds direct-filter taps
+  bool filter_fft = false;      // some shape's table holds a transfer function
   hipStream_t cap_stream = nullptr;
   // the instantiated S-step rollout graph of the last mc_rollout call, reused
   // while its key matches; `graph_done` follows its last launch
@@ -685,15 +687,101 @@ mc_status ensure_geo(mc_plan* p, const mc_params* q) {
   return MC_OK;
 }
 
-// Tabulates the Gaussian transfer function for `sigma` (utils.py:41-47) when
-// the plan does not hold it yet.  Called before any stream capture.
+// The periodic 1-D kernel of the Gaussian filter along an axis of length n:
+// h[m] = (1/n) sum_k exp(-2 pi^2 s^2 f_k^2) cos(2 pi k m / n) (the inverse DFT
+// of one factor of the transfer function, utils.py:41-47), k summed in order,
+// in double.  Keeps offsets lo .. lo + count - 1: the smallest radius whose
+// tail holds at most 2^-27 of the kernel's |mass|, or all n offsets once when
+// the kernel wraps.  False when more than FILTER_TAPS_MAX taps are needed.
+static bool gauss_taps(int n, double sigma, int* lo, std::vector<float>* taps) {
+  std::vector<double> T(n), C(n), h(n, 0.0);
+  for (int k = 0; k < n; ++k) {
+    const double f = (double)(k < (n + 1) / 2 ? k : k - n) / (double)n;
+    T[k] = exp((-2.0 * M_PI * M_PI * sigma * sigma) * (f * f));
+    C[k] = cos(2.0 * M_PI * (double)k / (double)n);
+  }
+  for (int k = 0; k < n; ++k)
+    for (int m = 0; m < n; ++m) h[m] = h[m] + T[k] * C[(int)(((long long)k * m) % n)];
+  double total = 0.0;
+  for (int m = 0; m < n; ++m) {
+    h[m] = h[m] / (double)n;
+    total += fabs(h[m]);
+  }
+  int R = 0;
+  for (;; ++R) {
+    if (R >= n / 2) break;
+    double tail = 0.0;
+    for (int m = 0; m < n; ++m)
+      if (std::min(m, n - m) > R) tail += fabs(h[m]);
+    if (tail <= std::ldexp(1.0, -27) * total) break;
+  }
+  int cnt;
+  if (2 * R + 1 >= n) {
+    *lo = -((n - 1) / 2);
+    cnt = n;
+  } else {
+    *lo = -R;
+    cnt = 2 * R + 1;
+  }
+  if (cnt > FILTER_TAPS_MAX) return false;
+  taps->resize(cnt);
+  for (int t = 0; t < cnt; ++t) (*taps)[t] = (float)h[(((*lo + t) % n) + n) % n];
+  return true;
+}
+
+// Tabulates the Gaussian filter for `sigma` when the plan does not hold it
+// yet, per steric grid shape: the direct filter's taps when both of the
+// shape's kernels are short, else the transfer function of the FFT path
+// (utils.py:41-47).  The kernels tell the two apart by the table's first word:
+// lo_x <= 0 as an int, or exp(0) = 1.0f.  Called before any stream capture.
 mc_status ensure_kf(mc_plan* p, double sigma, hipStream_t s) {
   if (!p->fft_ok || p->kf_sigma == sigma) return MC_OK;
   const float coef = (float)(-2.0 * (M_PI * M_PI) * (sigma * sigma));  // utils.py:47
-  for (size_t i = 0; i < p->shapes.size(); ++i)
-    HIPCHK(launch_gauss_table(p->shapes[i].x, p->shapes[i].y, coef, p->kf + p->shape_kf_off[i], s));
+  bool any_direct = false, any_fft = false;
+  std::vector<float> tab;
+  for (size_t i = 0; i < p->shapes.size(); ++i) {
+    const int H = p->shapes[i].x, W = p->shapes[i].y;
+    int lx, ly;
+    std::vector<float> tx, ty;
+    const bool direct = gauss_taps(W, sigma, &lx, &tx) && gauss_taps(H, sigma, &ly, &ty) &&
+                        4 + tx.size() + ty.size() <= (size_t)H * W;
+    if (!direct) {
+      any_fft = true;
+      HIPCHK(launch_gauss_table(H, W, coef, p->kf + p->shape_kf_off[i], s));
+      continue;
+    }
+    any_direct = true;
+    const int hd[4] = {lx, (int)tx.size(), ly, (int)ty.size()};
+    tab.assign(4, 0.0f);
+    memcpy(tab.data(), hd, sizeof(hd));
+    tab.insert(tab.end(), tx.begin(), tx.end());
+    tab.insert(tab.end(), ty.begin(), ty.end());
+    // ordered after every earlier use of the table on `s`; the wait keeps the
+    // host staging alive until the copy is done
+    HIPCHK(hipMemcpyAsync(p->kf + p->shape_kf_off[i], tab.data(), tab.size() * sizeof(float),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  p->filter_direct = any_direct;
+  p->filter_fft = any_fft;
   p->kf_sigma = sigma;
   return MC_OK;
+}
+
+// The Gaussian filter of the plan's current tables over every instance: of
+// real_in, or of the CIC deposit described by `dep` (also written to
+// density_out when not NULL).  Each path's kernels skip the other's shapes;
+// the direct path's row-pass scratch is the spectrum buffer, free by then.
+hipError_t run_filter(mc_plan* p, const float* real_in, const StericDep* dep, float* density_out,
+                      float* out, hipStream_t s) {
+  if (p->filter_direct) {
+    const hipError_t e = launch_steric_conv(p->bd, real_in, dep, reinterpret_cast<float*>(p->spec),
+                                            p->kf, density_out, out, s);
+    if (e != hipSuccess) return e;
+  }
+  if (p->filter_fft)
+    return launch_steric_fft_T(p->bd, real_in, dep, p->spec, p->kf, density_out, out, s);
+  return hipSuccess;
 }
 
 // Steric pipeline on `pos` (+ optional velocity prediction): fills p->pred_w,
@@ -713,7 +801,7 @@ mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float*
   d.sort_tmp = p->row_sort;
   d.key_tmp = p->row_keys;
   d.key_half = (size_t)4 * p->N;
-  HIPCHK(launch_steric_fft_T(p->bd, nullptr, &d, p->spec, p->kf, density_out, steric_out, s));
+  HIPCHK(run_filter(p, nullptr, &d, density_out, steric_out, s));
   return MC_OK;
 }
 
@@ -1206,7 +1294,7 @@ mc_status mc_gaussian_filter(mc_plan* p, const float* data, float* out, double s
   if (!p->fft_ok) return fail(MC_ERR_UNSUPPORTED, "%s", p->fft_why.c_str());
   hipStream_t s = (hipStream_t)stream;
   MCCHK(ensure_kf(p, sigma, s));
-  HIPCHK(launch_steric_fft_T(p->bd, data, nullptr, p->spec, p->kf, nullptr, out, s));
+  HIPCHK(run_filter(p, data, nullptr, nullptr, out, s));
   return MC_OK;
 }
 
@@ -1586,9 +1674,11 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
       HIPCHK(launch_steric_nodes(a.c, p->bd, N, vg, p->pred_w, fst, s));
     }
     HIPCHK(launch_rod_segment(a, true, len, const_cast<float*>(ck), p->adj_tape, lds, s));
-    if (steric)
-      HIPCHK(launch_steric_adjoint(a.c, p->bd, N, p->pred_w, vg, pb1, vbar, dbar, p->spec, p->kf,
-                                   xb, vb, s));
+    if (steric) {
+      HIPCHK(launch_steric_bar(a.c, p->bd, p->pred_w, pb1, vbar, s));
+      HIPCHK(run_filter(p, vbar, nullptr, nullptr, dbar, s));
+      HIPCHK(launch_steric_adj_nodes(a.c, p->bd, N, p->pred_w, vg, dbar, pb1, xb, vb, s));
+    }
   }
   if (position0_bar) HIPCHK(hipMemcpyAsync(position0_bar, xb, n2, hipMemcpyDeviceToDevice, s));
   if (velocity0_bar) HIPCHK(hipMemcpyAsync(velocity0_bar, vb, n2, hipMemcpyDeviceToDevice, s));

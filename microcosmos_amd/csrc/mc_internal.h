@@ -155,12 +155,20 @@ struct StericDep {
   unsigned long long* key_tmp;  // 8N
   size_t key_half;              // 4N
 };
-// Whole steric filter over every instance: forward rows (CIC deposit from
-// `dep` when real_in is NULL), transfer function on the transposed spectrum,
-// inverse rows into `out`.
+// Steric filter (FFT path) over every instance whose table holds a transfer
+// function: forward rows (CIC deposit from `dep` when real_in is NULL),
+// transfer function on the transposed spectrum, inverse rows into `out`.
 hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const StericDep* dep,
                                float2* specT, const float* kfT, float* density_out, float* out,
                                hipStream_t s);
+// Steric filter (direct path) over every instance whose table holds taps
+// (both axes' kernels at most FILTER_TAPS_MAX taps): the table is
+// (int lo_x, n_x, lo_y, n_y), the x taps, the y taps; tmp is grid-sized
+// scratch.
+constexpr int FILTER_TAPS_MAX = 129;
+hipError_t launch_steric_conv(const BatchDev& b, const float* real_in, const StericDep* dep,
+                              float* tmp, const float* taps, float* density_out, float* out,
+                              hipStream_t s);
 size_t fft_rows_lds_bytes(int max_w);
 size_t fft_cols_lds_bytes(int max_h);
 // rod
@@ -198,8 +206,8 @@ struct RodAdjArgs {
   float* rest_bar;                             // (B) += d loss / d rest angles
   // steric term (one-step segments only): fst = the step's steric force per
   // node, added to the prediction; pb1 (reverse) = output of d loss / d the
-  // pushed prediction, whose steric and prologue adjoint launch_steric_adjoint
-  // finishes (a.xb / a.vb are then left to it)
+  // pushed prediction, whose steric and prologue adjoint launch_steric_bar,
+  // the filter and launch_steric_adj_nodes finish (a.xb / a.vb are left to them)
   const float* fst;
   float* pb1;
 };
@@ -207,14 +215,17 @@ size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse);
 // forward: ck = checkpoint slot written, tape_or_dbg = debug_vector (N, 2);
 // reverse: ck = checkpoint slot read, tape_or_dbg = tape (steps x cycles x E float4)
 // Adjoint of the steric push p1 = p0 + f(p0) strength dt (forces.py:41-118)
-// and of the prediction p0 = x + (v d) dt, given pb1 = d loss / d p1, the
-// wrapped prediction pred_w and its filtered field V: writes xb = d loss / d x
-// and vb = d loss / d v (vb holds d loss / d v_new on entry).  vbar and dbar
-// are grid-sized scratch.
-hipError_t launch_steric_adjoint(const Consts& c, const BatchDev& b, int N, const float* pred_w,
-                                 const float* V, const float* pb1, float* vbar, float* dbar,
-                                 float2* specT, const float* kfT, float* xb, float* vb,
-                                 hipStream_t s);
+// and of the prediction p0 = x + (v d) dt, given pb1 = d loss / d p1 and the
+// wrapped prediction pred_w, in three launches: launch_steric_bar writes the
+// field adjoint vbar (grid-sized) of the force's samples; the caller filters
+// it (the Gaussian filter is its own adjoint) into dbar; then
+// launch_steric_adj_nodes, with the step's field V, writes xb = d loss / d x
+// and vb = d loss / d v (vb holds d loss / d v_new on entry).
+hipError_t launch_steric_bar(const Consts& c, const BatchDev& b, const float* pred_w,
+                             const float* pb1, float* vbar, hipStream_t s);
+hipError_t launch_steric_adj_nodes(const Consts& c, const BatchDev& b, int N, const float* pred_w,
+                                   const float* V, const float* dbar, const float* pb1, float* xb,
+                                   float* vb, hipStream_t s);
 hipError_t launch_rod_segment(const RodAdjArgs& a, bool reverse, int steps, float* ck,
                               float* tape_or_dbg, size_t lds_bytes, hipStream_t s);
 // immersed boundary (solved per interaction cluster)

@@ -619,10 +619,17 @@ __device__ __forceinline__ void load_plan(const FftDev& p, int* s_radix) {
 // workgroup in the column pass.
 // ---------------------------------------------------------------------------
 
+// The shape's table holds the direct filter's taps (its first word, lo_x <= 0
+// as an int) rather than a transfer function (first word exp(0) = 1.0f).
+__device__ __forceinline__ bool filter_direct(const float* table, const InstGeo& gi) {
+  return reinterpret_cast<const int*>(table + gi.kf_off)[0] <= 0;
+}
+
 // Forward row FFT of one row (from the CIC deposit when `real_in` is NULL,
 // else from the real grid), written transposed to specT.
 __global__ void __launch_bounds__(256) k_rows_fwd_T(BatchDev bd, const float* __restrict__ real_in,
                                                     DepArgs dep, float2* __restrict__ specT,
+                                                    const float* __restrict__ kfT,
                                                     float* __restrict__ density_out) {
   extern __shared__ float2 lds[];
   __shared__ int s_radix[FFT_MAX_RADICES];
@@ -630,6 +637,7 @@ __global__ void __launch_bounds__(256) k_rows_fwd_T(BatchDev bd, const float* __
   __shared__ int s_cnt[2];
   const int row = xcd_group(blockIdx.x, gridDim.x);
   const InstGeo& gi = bd.geo[bd.row_inst[row]];
+  if (filter_direct(kfT, gi)) return;
   const int W = gi.W, H = gi.H, y = row - gi.row0;
   const FftDev& fp = bd.fft[gi.fft_w];
   float2* a = lds;
@@ -661,6 +669,7 @@ __global__ void __launch_bounds__(256) k_cols_T(BatchDev bd, float2* __restrict_
   __shared__ int s_radix[FFT_MAX_RADICES];
   const int colg = blockIdx.x;
   const InstGeo& gi = bd.geo[bd.col_inst[colg]];
+  if (filter_direct(kfT, gi)) return;
   const int H = gi.H, k = colg - gi.col0;
   const FftDev& fp = bd.fft[gi.fft_h];
   float2* a = lds;
@@ -692,11 +701,13 @@ __global__ void __launch_bounds__(256) k_cols_T(BatchDev bd, float2* __restrict_
 
 // Inverse row FFT of one row read transposed from specT; real part scaled by 1/W.
 __global__ void __launch_bounds__(256) k_rows_inv_T(BatchDev bd, const float2* __restrict__ specT,
+                                                    const float* __restrict__ kfT,
                                                     float* __restrict__ out) {
   extern __shared__ float2 lds[];
   __shared__ int s_radix[FFT_MAX_RADICES];
   const int row = xcd_group(blockIdx.x, gridDim.x);
   const InstGeo& gi = bd.geo[bd.row_inst[row]];
+  if (filter_direct(kfT, gi)) return;
   const int W = gi.W, H = gi.H, y = row - gi.row0;
   const FftDev& fp = bd.fft[gi.fft_w];
   float2* a = lds;
@@ -711,6 +722,114 @@ __global__ void __launch_bounds__(256) k_rows_inv_T(BatchDev bd, const float2* _
   const float sc = (float)(1.0 / (double)W);
   float* o = out + gi.grid_off + (size_t)y * W;
   for (int i = threadIdx.x; i < W; i += blockDim.x) o[i] = res[i].x * sc;
+}
+
+// ---------------------------------------------------------------------------
+// Direct filter.  The transfer function exp(-2 pi^2 s^2 (fr^2 + fc^2)) is the
+// product of one factor per axis, so utils.py:35-49's filter is the circular
+// convolution with h_y (x) h_x, h = the inverse DFT of one factor.  When both
+// kernels are short (their tails below 2^-27 of their mass), the filter runs
+// as a row pass (fused with the row's CIC deposit) and a column pass, each
+// output summed tap by tap from the first offset; the shape's table then
+// holds (int lo_x, n_x, lo_y, n_y) and the taps of x, then of y.
+// ---------------------------------------------------------------------------
+
+// Row pass: the row (deposited, or read from real_in) in LDS, convolved with
+// h_x, to tmp; the deposit also goes to density_out.
+__global__ void __launch_bounds__(256) k_rows_conv(BatchDev bd, const float* __restrict__ real_in,
+                                                   DepArgs dep, const float* __restrict__ taps,
+                                                   float* __restrict__ tmp,
+                                                   float* __restrict__ density_out) {
+  extern __shared__ float2 lds[];
+  __shared__ int wsum[32];
+  __shared__ int s_cnt[2];
+  const int row = xcd_group(blockIdx.x, gridDim.x);
+  const InstGeo& gi = bd.geo[bd.row_inst[row]];
+  if (!filter_direct(taps, gi)) return;
+  const int W = gi.W, y = row - gi.row0;
+  float2* a = lds;
+  float2* b = lds + dep.b_off;
+  float* tw = reinterpret_cast<float*>(lds + dep.tw_off);
+  const int* hd = reinterpret_cast<const int*>(taps + gi.kf_off);
+  const int lo = hd[0], nt = hd[1];
+  const float* tx = taps + gi.kf_off + 4;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) tw[t] = tx[t];
+  const size_t g0 = (size_t)gi.grid_off + (size_t)y * W;
+  if (real_in) {
+    for (int i = threadIdx.x; i < W; i += blockDim.x) a[i] = make_float2(real_in[g0 + i], 0.0f);
+  } else {
+    deposit_row(dep, gi, row, a, reinterpret_cast<char*>(b), wsum, s_cnt);
+    if (density_out)
+      for (int i = threadIdx.x; i < W; i += blockDim.x) density_out[g0 + i] = a[i].x;
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < W; x += blockDim.x) {
+    float acc = 0.0f;
+    int i = x + lo;
+    if (i < 0) i += W;
+    for (int t = 0; t < nt; ++t) {
+      acc = acc + tw[t] * a[i].x;
+      i = (i + 1 == W) ? 0 : i + 1;
+    }
+    tmp[g0 + x] = acc;
+  }
+}
+
+// Column pass: one output row per workgroup; each thread takes COLS_COLS
+// columns at once and loads COLS_BATCH input rows of each before using them
+// (COLS_COLS x COLS_BATCH loads in flight); every output sums its taps in
+// order.
+constexpr int COLS_COLS = 4;
+constexpr int COLS_BATCH = 8;
+__global__ void __launch_bounds__(256) k_cols_conv(BatchDev bd, const float* __restrict__ taps,
+                                                   const float* __restrict__ tmp,
+                                                   float* __restrict__ out) {
+  __shared__ float ty[FILTER_TAPS_MAX];
+  __shared__ int roff[FILTER_TAPS_MAX];
+  const int row = xcd_group(blockIdx.x, gridDim.x);
+  const InstGeo& gi = bd.geo[bd.row_inst[row]];
+  if (!filter_direct(taps, gi)) return;
+  const int W = gi.W, H = gi.H, y = row - gi.row0;
+  const int* hd = reinterpret_cast<const int*>(taps + gi.kf_off);
+  const int nx = hd[1], lo = hd[2], nt = hd[3];
+  int y0 = y + lo;
+  if (y0 < 0) y0 += H;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+    ty[t] = taps[gi.kf_off + 4 + nx + t];
+    int yy = y0 + t;
+    if (yy >= H) yy -= H;
+    roff[t] = yy * W;
+  }
+  __syncthreads();
+  const float* src = tmp + gi.grid_off;
+  const int B = blockDim.x;
+  for (int xb = threadIdx.x; xb < W; xb += COLS_COLS * B) {
+    float acc[COLS_COLS];
+#pragma unroll
+    for (int c = 0; c < COLS_COLS; ++c) acc[c] = 0.0f;
+    for (int t0 = 0; t0 < nt; t0 += COLS_BATCH) {
+      float v[COLS_COLS][COLS_BATCH];
+#pragma unroll
+      for (int k = 0; k < COLS_BATCH; ++k)
+#pragma unroll
+        for (int c = 0; c < COLS_COLS; ++c) {
+          const int x = xb + c * B;
+          v[c][k] = (t0 + k < nt && x < W) ? src[(size_t)roff[t0 + k] + x] : 0.0f;
+        }
+#pragma unroll
+      for (int k = 0; k < COLS_BATCH; ++k)
+        if (t0 + k < nt) {
+          const float w = ty[t0 + k];
+#pragma unroll
+          for (int c = 0; c < COLS_COLS; ++c) acc[c] = acc[c] + w * v[c][k];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < COLS_COLS; ++c) {
+      const int x = xb + c * B;
+      if (x < W) out[gi.grid_off + (size_t)y * W + x] = acc[c];
+    }
+  }
 }
 
 // Transposed transfer function kfT[k * H + y] = exp(coef (fr(y)^2 + fc(k)^2)).
@@ -738,7 +857,8 @@ hipError_t steric_prepare(int lds_max) {
   const int lds = lds_max - 1024;
   const void* fns[] = {reinterpret_cast<const void*>(&k_rows_fwd_T),
                        reinterpret_cast<const void*>(&k_cols_T),
-                       reinterpret_cast<const void*>(&k_rows_inv_T)};
+                       reinterpret_cast<const void*>(&k_rows_inv_T),
+                       reinterpret_cast<const void*>(&k_rows_conv)};
   for (const void* f : fns) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
@@ -815,11 +935,37 @@ hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const St
   dep.b_off = rl.b_off;
   dep.tw_off = rl.tw_off;
   hipLaunchKernelGGL(k_rows_fwd_T, dim3(b.rows), dim3(tr), fft_rows_lds_bytes(b.max_w), s, b,
-                     real_in, dep, specT, density_out);
+                     real_in, dep, specT, kfT, density_out);
   hipLaunchKernelGGL(k_cols_T, dim3(b.cols), dim3(tc), fft_cols_lds_bytes(b.max_h), s, b, specT,
                      kfT);
   hipLaunchKernelGGL(k_rows_inv_T, dim3(b.rows), dim3(tr), fft_pass_bytes(b.max_w), s, b, specT,
-                     out);
+                     kfT, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_steric_conv(const BatchDev& b, const float* real_in, const StericDep* d,
+                              float* tmp, const float* taps, float* density_out, float* out,
+                              hipStream_t s) {
+  const int tr = fft_threads(b.max_w);
+  const RowsLayout rl = rows_layout(b.max_w);
+  DepArgs dep{};
+  if (d) {
+    dep.L = d->L;
+    dep.comp_rows = d->comp_rows;
+    dep.cic_cell = d->cic_cell;
+    dep.cic_w = d->cic_w;
+    dep.arena = d->arena;
+    dep.arena_top = d->arena_top;
+    dep.sort_tmp = d->sort_tmp;
+    dep.key_tmp = d->key_tmp;
+    dep.key_half = d->key_half;
+  }
+  dep.cap = rl.cap;
+  dep.b_off = rl.b_off;
+  dep.tw_off = rl.tw_off;
+  hipLaunchKernelGGL(k_rows_conv, dim3(b.rows), dim3(tr), fft_rows_lds_bytes(b.max_w), s, b,
+                     real_in, dep, taps, tmp, density_out);
+  hipLaunchKernelGGL(k_cols_conv, dim3(b.rows), dim3(256), 0, s, b, taps, tmp, out);
   return hipGetLastError();
 }
 

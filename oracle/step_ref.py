@@ -307,7 +307,77 @@ def stockham_fft(re, im, inverse=False):
     return re, im
 
 
+CONV_TAIL = 2.0 ** -27   # dropped kernel mass, relative to the kernel's total
+CONV_TAPS_MAX = 129      # longest tap list of the direct filter (radius 64)
+
+
+def gauss_taps(n, sigma):
+    """The exact periodic 1-D kernel of utils.py:35-49's filter along an axis
+    of length n: h[m] = IDFT(exp(-2 pi^2 sigma^2 f_k^2))[m] (the 2-D transfer
+    function is the product of two such factors, so the filter is the
+    circular convolution with h_y (x) h_x), in float64, k summed in order.
+    Returns (lo, taps): taps[t] weights offset lo + t.  The radius R is the
+    smallest with the mass of |h| beyond it at most CONV_TAIL of the total; a
+    kernel that wraps (2 R + 1 >= n) keeps all n offsets once each.  None
+    when more than CONV_TAPS_MAX taps are needed."""
+    k = np.arange(n)
+    f = np.where(k < (n + 1) // 2, k, k - n).astype(np.float64) / n
+    T = np.exp((-2.0 * np.pi * np.pi * sigma * sigma) * (f * f))
+    C = np.cos(2.0 * np.pi * np.arange(n, dtype=np.float64) / n)
+    m = np.arange(n)
+    h = np.zeros(n)
+    for kk in range(n):
+        h = h + T[kk] * C[(kk * m) % n]
+    h = h / n
+    a = np.abs(h)
+    d = np.minimum(m, n - m)
+    total = a.sum()
+    R = 0
+    while R < n // 2 and a[d > R].sum() > CONV_TAIL * total:
+        R += 1
+    if 2 * R + 1 >= n:
+        lo, cnt = -((n - 1) // 2), n
+    else:
+        lo, cnt = -R, 2 * R + 1
+    if cnt > CONV_TAPS_MAX:
+        return None
+    return lo, h[(lo + np.arange(cnt)) % n]
+
+
+def gaussian_filter_direct(data, sigma, dtype=None):
+    """The device's direct filter: circular convolution with the truncated
+    kernels of ``gauss_taps``, rows (x) first, then columns (y), each output
+    summed tap by tap from lo upward, rounded in ``dtype`` (default: the
+    data's).  None when an axis needs more than CONV_TAPS_MAX taps."""
+    dtype = np.dtype(dtype or data.dtype)
+    H, W = data.shape
+    tx, ty = gauss_taps(W, sigma), gauss_taps(H, sigma)
+    if tx is None or ty is None:
+        return None
+    src = np.asarray(data, dtype)
+    acc = np.zeros_like(src)
+    for t, w in enumerate(tx[1].astype(dtype)):
+        acc = acc + w * np.roll(src, -(tx[0] + t), axis=1)
+    src, acc = acc, np.zeros_like(acc)
+    for t, w in enumerate(ty[1].astype(dtype)):
+        acc = acc + w * np.roll(src, -(ty[0] + t), axis=0)
+    return acc
+
+
 def gaussian_filter(data, sigma):
+    """utils.py:35-49 -- Re(ifft2(fft2(data) * kernel_f)).
+
+    float32 (the device's precision): the device's direct filter
+    (``gaussian_filter_direct``) when both axes' kernels fit CONV_TAPS_MAX
+    taps, else the device's FFT; float64: the FFT, exact to rounding."""
+    if np.asarray(data).dtype == np.float32:
+        out = gaussian_filter_direct(data, sigma)
+        if out is not None:
+            return out
+    return gaussian_filter_fft(data, sigma)
+
+
+def gaussian_filter_fft(data, sigma):
     """utils.py:35-49 -- Re(ifft2(fft2(data) * kernel_f)), the 2-D DFTs done as
     rows (x) forward, columns (y) forward, transfer function, columns inverse
     scaled by 1/H, rows inverse scaled by 1/W (the device's order)."""

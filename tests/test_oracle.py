@@ -112,6 +112,49 @@ def test_gaussian_filter_restatement_matches_pocketfft(shape):
     assert np.linalg.norm(a32 - b) / np.linalg.norm(b) < 1e-6
 
 
+@pytest.mark.parametrize("n,sigma", [(1024, 2.0), (256, 5.0), (48, 1.5), (10, 1.0), (33, 2.0)])
+def test_gauss_taps_are_the_idft_kernel(n, sigma):
+    """The direct filter's taps are the exact periodic kernel (brute-force
+    inverse DFT of one transfer-function factor) on the kept offsets, the
+    dropped tail holds at most CONV_TAIL of the mass, and the kernel is
+    symmetric (the transfer function is even)."""
+    k = np.arange(n)
+    f = np.where(k < (n + 1) // 2, k, k - n) / n
+    g = np.exp(-2 * np.pi ** 2 * sigma ** 2 * f ** 2)
+    h = (g[None, :] * np.cos(2 * np.pi * np.outer(k, k) / n)).sum(1) / n
+    lo, taps = R.gauss_taps(n, sigma)
+    off = lo + np.arange(taps.size)
+    assert np.max(np.abs(taps - h[off % n])) < 1e-13  # cos of large arguments in the brute force
+    kept = np.zeros(n, bool)
+    kept[off % n] = True
+    assert np.abs(h[~kept]).sum() <= R.CONV_TAIL * np.abs(h).sum()
+    assert taps.size == n or np.allclose(taps, taps[::-1], rtol=0, atol=1e-15)
+
+
+def test_gauss_taps_long_kernel_falls_back():
+    """sigma 1.5 on 512: the transfer function is cut at Nyquist at 1.5e-5,
+    so the periodic kernel rings over the whole axis: FFT path."""
+    assert R.gauss_taps(512, 1.5) is None
+    d = np.random.default_rng(2).random((512, 512)).astype(np.float32)
+    np.testing.assert_array_equal(R.gaussian_filter(d, 1.5), R.gaussian_filter_fft(d, 1.5))
+
+
+@pytest.mark.parametrize("shape,sigma", [((48, 48), 2.0), ((64, 96), 1.5), ((10, 10), 1.0),
+                                         ((256, 128), 5.0)])
+def test_direct_filter_matches_pocketfft(shape, sigma):
+    """The direct filter against numpy's FFT filter: in float64 arithmetic to
+    the dropped tail (2 CONV_TAIL of the data's scale), in float32 to float32
+    rounding."""
+    rng = np.random.default_rng(5)
+    d = rng.random(shape)
+    b = R.gaussian_filter_pocketfft(d, sigma)
+    a64 = R.gaussian_filter_direct(d, sigma)
+    assert np.max(np.abs(a64 - b)) <= 2 * R.CONV_TAIL * np.abs(d).max() + 1e-14
+    a32 = R.gaussian_filter_direct(d.astype(np.float32), sigma)
+    assert a32.dtype == np.float32
+    assert np.linalg.norm(a32 - b) / np.linalg.norm(b) < 1e-6
+
+
 def test_fftfreq_matches_numpy():
     for n in (8, 9, 10, 128, 192):
         assert np.allclose(R.fftfreq32(n, np.float64), np.fft.fftfreq(n))
