@@ -267,6 +267,8 @@ def main():
     ap.add_argument("--only-serve", action="store_true",
                     help="run only the serving leg and print its JSON")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--only-grad", choices=["pbd", "cosserat", "folding"],
+                    help="run only the gradient leg of this scheme and print its JSON")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 9 x 1024 x 1024 float32 device copy (37.7 MB read + "
                          "37.7 MB written per launch) to calibrate HBM byte counters")
@@ -293,6 +295,9 @@ def main():
     from tests.helpers import device_state
 
     dev = torch.device("cuda", 0)
+    if args.only_grad:
+        print(json.dumps(gradient_leg(dev, scheme=args.only_grad)), flush=True)
+        return
     if args.only_serve:
         print(json.dumps(serving(args.serve_requests, dev, args.serve_lanes, args.serve_chunk,
                                  args.serve_max_batch, plain=not args.serve_no_plain)), flush=True)

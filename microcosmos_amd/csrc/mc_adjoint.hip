@@ -466,10 +466,19 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
 constexpr int BAR_THREADS = 256;
 constexpr int BAR_LIST = 1024;  // staged nodes per flush
 
+// v mod n for v in [-n, 2n) -- the cells of samples within one cell of a
+// wrapped position (in [0, n], rounding included) -- without the integer
+// divisions of imod; clamped so that no input (a NaN or saturated position)
+// indexes outside [0, n).
+__device__ __forceinline__ int wrap1(int v, int n) {
+  v = v < 0 ? v + n : (v >= n ? v - n : v);
+  return v < 0 ? 0 : (v >= n ? n - 1 : v);
+}
+
 __device__ __forceinline__ bool touches_row(float py, int H, int y) {
   const int a = f2i(floorf(py - 0.5f)), b = f2i(floorf(py)), d = f2i(floorf(py + 0.5f));
-  return imod(a, H) == y || imod(a + 1, H) == y || imod(b, H) == y || imod(b + 1, H) == y ||
-         imod(d, H) == y || imod(d + 1, H) == y;
+  return wrap1(a, H) == y || wrap1(a + 1, H) == y || wrap1(b, H) == y || wrap1(b + 1, H) == y ||
+         wrap1(d, H) == y || wrap1(d + 1, H) == y;
 }
 
 // acc += q * (bilinear weight of cell (y, x) in the sample at (yy, xx))
@@ -479,8 +488,8 @@ __device__ __forceinline__ float bar_sample(float acc, float q, float yy, float 
   const float wyh = yy - ly, wxh = xx - lx;
   const float wyl = 1.0f - wyh, wxl = 1.0f - wxh;
   const int iy = f2i(ly), ix = f2i(lx);
-  const int y0 = imod(iy, H), y1 = imod(iy + 1, H);
-  const int x0 = imod(ix, W), x1 = imod(ix + 1, W);
+  const int y0 = wrap1(iy, H), y1 = wrap1(iy + 1, H);
+  const int x0 = wrap1(ix, W), x1 = wrap1(ix + 1, W);
   if (y0 == y && x0 == x) acc = acc + q * (wyl * wxl);
   if (y0 == y && x1 == x) acc = acc + q * (wyl * wxh);
   if (y1 == y && x0 == x) acc = acc + q * (wyh * wxl);
@@ -507,46 +516,53 @@ __global__ void __launch_bounds__(BAR_THREADS) k_steric_bar_rows(Consts c, Batch
     float acc = 0.0f;
     if (tid == 0) ltop = 0;
     __syncthreads();
-    auto flush = [&]() {
-      const int n = ltop;
-      if (x < W)
-        for (int k = 0; k < n; ++k) {
-          const float4 q = lst[k];
-          acc = bar_sample(acc, -q.z, q.y, q.x + 0.5f, H, W, y, x);
-          acc = bar_sample(acc, q.z, q.y, q.x - 0.5f, H, W, y, x);
-          acc = bar_sample(acc, -q.w, q.y + 0.5f, q.x, H, W, y, x);
-          acc = bar_sample(acc, q.w, q.y - 0.5f, q.x, H, W, y, x);
+    for (int i0 = 0;; i0 += BAR_THREADS) {
+      const bool more = i0 < gi.nodes;
+      if (more) {
+        const int i = i0 + tid;
+        float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool hit = false;
+        if (i < gi.nodes) {
+          const int g = gi.node0 + i;
+          const float2 p = *reinterpret_cast<const float2*>(pred_w + 2 * g);
+          hit = touches_row(p.y, H, y);
+          e = make_float4(p.x, p.y, pb1[2 * g] * sdt, pb1[2 * g + 1] * sdt);
         }
-      __syncthreads();
-      if (tid == 0) ltop = 0;
-      __syncthreads();
-    };
-    for (int i0 = 0; i0 < gi.nodes; i0 += BAR_THREADS) {
-      const int i = i0 + tid;
-      float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      bool hit = false;
-      if (i < gi.nodes) {
-        const int g = gi.node0 + i;
-        const float2 p = *reinterpret_cast<const float2*>(pred_w + 2 * g);
-        hit = touches_row(p.y, H, y);
-        e = make_float4(p.x, p.y, pb1[2 * g] * sdt, pb1[2 * g + 1] * sdt);
+        const unsigned long long m = __ballot(hit);
+        if (lane == 0) wcnt[wv] = __popcll(m);
+        __syncthreads();
+        int base = ltop;
+        for (int w = 0; w < wv; ++w) base += wcnt[w];
+        if (hit) lst[base + __popcll(m & ((1ull << lane) - 1ull))] = e;
+        __syncthreads();
+        if (tid == 0) {
+          int t = ltop;
+          for (int w = 0; w < BAR_THREADS / 64; ++w) t += wcnt[w];
+          ltop = t;
+        }
+        __syncthreads();
       }
-      const unsigned long long m = __ballot(hit);
-      if (lane == 0) wcnt[wv] = __popcll(m);
-      __syncthreads();
-      int base = ltop;
-      for (int w = 0; w < wv; ++w) base += wcnt[w];
-      if (hit) lst[base + __popcll(m & ((1ull << lane) - 1ull))] = e;
-      __syncthreads();
-      if (tid == 0) {
-        int t = ltop;
-        for (int w = 0; w < BAR_THREADS / 64; ++w) t += wcnt[w];
-        ltop = t;
+      // gather from the staged nodes when the list is nearly full or complete
+      if (!more || ltop > BAR_LIST - BAR_THREADS) {
+        const int n = ltop;
+        if (x < W)
+          for (int k = 0; k < n; ++k) {
+            const float4 q = lst[k];
+            // the samples' cells lie in columns floor(px) - 1 .. floor(px) + 2
+            int d = x - f2i(floorf(q.x)) + 1;
+            d = d < 0 ? d + W : (d >= W ? d - W : d);
+            if (d > 3) continue;
+            acc = bar_sample(acc, -q.z, q.y, q.x + 0.5f, H, W, y, x);
+            acc = bar_sample(acc, q.z, q.y, q.x - 0.5f, H, W, y, x);
+            acc = bar_sample(acc, -q.w, q.y + 0.5f, q.x, H, W, y, x);
+            acc = bar_sample(acc, q.w, q.y - 0.5f, q.x, H, W, y, x);
+          }
+        __syncthreads();
+        if (tid == 0) ltop = 0;
+        __syncthreads();
       }
-      __syncthreads();
-      if (ltop > BAR_LIST - BAR_THREADS) flush();
+      if (!more) break;
     }
-    flush();
     if (x < W) out[x] = acc;
     __syncthreads();
   }
@@ -559,8 +575,8 @@ __device__ __forceinline__ void sample_slopes(const float* V, int H, int W, floa
   const float wyh = yy - ly, wxh = xx - lx;
   const float wyl = 1.0f - wyh, wxl = 1.0f - wxh;
   const int iy = f2i(ly), ix = f2i(lx);
-  const int y0 = imod(iy, H), y1 = imod(iy + 1, H);
-  const int x0 = imod(ix, W), x1 = imod(ix + 1, W);
+  const int y0 = wrap1(iy, H), y1 = wrap1(iy + 1, H);
+  const int x0 = wrap1(ix, W), x1 = wrap1(ix + 1, W);
   const float v00 = V[(size_t)y0 * W + x0], v01 = V[(size_t)y0 * W + x1];
   const float v10 = V[(size_t)y1 * W + x0], v11 = V[(size_t)y1 * W + x1];
   sx = wyl * (v01 - v00) + wyh * (v11 - v10);
@@ -596,8 +612,8 @@ __global__ void k_steric_adj_nodes(Consts c, BatchDev bd, int N, const float* __
   // deposit: sv sum_c Dbar[c] dw_c / dp
   const float fx = floorf(px), fy = floorf(py);
   const float dx = px - fx, dy = py - fy;
-  const int x0 = imod(f2i(fx), W), y0 = imod(f2i(fy), H);
-  const int x1 = imod(x0 + 1, W), y1 = imod(y0 + 1, H);
+  const int x0 = wrap1(f2i(fx), W), y0 = wrap1(f2i(fy), H);
+  const int x1 = wrap1(x0 + 1, W), y1 = wrap1(y0 + 1, H);
   const float d00 = D[(size_t)y0 * W + x0], d10 = D[(size_t)y0 * W + x1];
   const float d01 = D[(size_t)y1 * W + x0], d11 = D[(size_t)y1 * W + x1];
   ox = ox + c.steric_sv * ((1.0f - dy) * (d10 - d00) + dy * (d11 - d01));
