@@ -91,3 +91,35 @@ def test_batch_instance_matches_oracle(dev):
         assert rel_l2(_np(bf.f_grid), o.f_grid) <= 1e-5
         assert rel_l2(_np(bf.fluid_velocity), o.fluid_velocity) <= 1e-5
         assert rel_l2(_np(be.theta), o.theta) <= 1e-5
+
+
+def test_batch_mixed_filter_paths(dev):
+    """sigma 1.5: the 64^2 and 24x36 instances take the direct filter (their
+    periodic kernels fit the tap limit), the 256^2 instance the FFT; in one
+    batch each kernel skips the other path's instances, and every instance
+    stays bit-identical to itself alone (steric field included)."""
+    _say("batch with direct-filter and FFT instances (sigma 1.5) vs each alone, 6 steps")
+    from dataclasses import replace
+    from microcosmos_amd import Rollout
+    from microcosmos_amd.batch import Batch
+    assert R.gauss_taps(256, 1.5) is None and R.gauss_taps(64, 1.5) is not None
+    wls = [W.small(grid=(64, 64), n_fil=2, n_nodes=16, seed=3),
+           W.small(grid=(256, 256), n_fil=3, n_nodes=24, seed=4),
+           W.small(grid=(24, 36), n_fil=1, n_nodes=30, seed=9)]
+    cfg = replace(device_config(wls[0]), steric_sigma=1.5)
+    steps = 6
+    b = Batch([device_state(w, dev) for w in wls], cfg, wls[0].dt,
+              actuation=[_act(w, dev) for w in wls])
+    b.run(steps, track=False)
+    torch.cuda.synchronize()
+    for i, w in enumerate(wls):
+        nodes, edges, fields = device_state(w, dev)
+        ro = Rollout(nodes, edges, fields, cfg, w.dt,
+                     actuation=(w.act_amp, w.act_omega, w.act_phase))
+        ro.run(steps, use_graph=False)
+        torch.cuda.synchronize()
+        bn, be, bf = b.instance(i)
+        np.testing.assert_array_equal(_np(bf.steric), _np(ro.steric), err_msg=f"inst {i} steric")
+        np.testing.assert_array_equal(_np(bn.position), _np(ro.position), err_msg=f"inst {i} pos")
+        np.testing.assert_array_equal(_np(bf.f_grid), _np(ro.f_grid), err_msg=f"inst {i} f")
+        assert np.abs(_np(bf.steric)).max() > 0.0
