@@ -18,6 +18,7 @@ each reverse step recomputes that step's steric field.
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import replace
 
 import numpy as np
@@ -138,3 +139,71 @@ def rollout_value_and_grad(nodes: Nodes, edges: Edges, fields: Fields, solver_co
     loss = (dvec * dvec).sum() / n
     g = ad.backward(2.0 * dvec / n)[0]
     return loss, g
+
+
+def folding_schedule(step: int, lr: float = 0.06, lr_min: float = 3e-5, warmup_steps: int = 15,
+                     total_steps: int = 150) -> float:
+    """Learning rate of the folding experiment without curriculum stages
+    (experiments/filament_folding/experiment.py:340-395): a linear warmup
+    from lr / 100 to lr over ``warmup_steps``, then a cosine decay from lr to
+    lr_min over the remaining steps (optax linear_schedule, cosine_decay_schedule
+    with alpha = lr_min / lr, joined at the warmup boundary).  Defaults from
+    experiments/conf/experiment/filament_folding.yaml:59-66."""
+    if step < warmup_steps:
+        return lr * 0.01 + (lr - lr * 0.01) * (step / warmup_steps)
+    decay = max(total_steps - warmup_steps, 1)
+    t = min(step - warmup_steps, decay)
+    alpha = lr_min / lr
+    return lr * ((1.0 - alpha) * 0.5 * (1.0 + math.cos(math.pi * t / decay)) + alpha)
+
+
+class FoldingOptimizer:
+    """The folding experiment's update (experiment.py:397-400, 1021-1023):
+    gradient clipping by global norm, then Adam (b1 0.9, b2 0.999, eps 1e-8)
+    scaled by ``folding_schedule`` at the update count.  Parameters and
+    gradients are device tensors; the update is elementwise."""
+
+    def __init__(self, lr: float = 0.06, lr_min: float = 3e-5, warmup_steps: int = 15,
+                 total_steps: int = 150, grad_clip: float = 1.0, b1: float = 0.9,
+                 b2: float = 0.999, eps: float = 1e-8):
+        self.lr, self.lr_min, self.warmup, self.total = lr, lr_min, warmup_steps, total_steps
+        self.clip, self.b1, self.b2, self.eps = grad_clip, b1, b2, eps
+        self.count = 0
+        self.mu = None
+        self.nu = None
+
+    def update(self, params: torch.Tensor, grad: torch.Tensor) -> torch.Tensor:
+        g = grad.to(torch.float32)
+        norm = torch.sqrt((g * g).sum())
+        g = torch.where(norm < self.clip, g, g / norm * self.clip)
+        if self.mu is None:
+            self.mu = torch.zeros_like(g)
+            self.nu = torch.zeros_like(g)
+        self.mu = self.b1 * self.mu + (1.0 - self.b1) * g
+        self.nu = self.b2 * self.nu + (1.0 - self.b2) * (g * g)
+        t = self.count + 1
+        mu_hat = self.mu / (1.0 - self.b1 ** t)
+        nu_hat = self.nu / (1.0 - self.b2 ** t)
+        lr = folding_schedule(self.count, self.lr, self.lr_min, self.warmup, self.total)
+        self.count += 1
+        return params - lr * mu_hat / (torch.sqrt(nu_hat) + self.eps)
+
+
+def optimize_folding(nodes: Nodes, edges: Edges, fields: Fields, solver_config, dt: float,
+                     num_steps: int, target: torch.Tensor, iterations: int,
+                     optimizer: FoldingOptimizer = None, checkpoint_every: int = 50):
+    """The folding experiment's optimisation loop over the bending rest angles
+    (experiment.py:985-1030 without curriculum, noise or visualisation): each
+    iteration rolls the rod out from the same initial state, takes the loss of
+    the final positions and its gradient through the adjoint kernels, and
+    applies ``optimizer``.  Returns (rest angles, per-iteration losses)."""
+    opt = optimizer or FoldingOptimizer(total_steps=iterations)
+    rest = as_f32(edges.bending_rest_angles, nodes.position.device).clone()
+    losses = []
+    for _ in range(iterations):
+        e = replace(edges, bending_rest_angles=rest)
+        loss, g = rollout_value_and_grad(nodes, e, fields, solver_config, dt, num_steps, target,
+                                         checkpoint_every)
+        losses.append(float(loss.item()))
+        rest = opt.update(rest, g)
+    return rest, losses

@@ -108,3 +108,60 @@ def test_bench_rejects_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_folding_schedule_closed_form():
+    """Warmup from lr/100 to lr, then cosine to lr_min (optax linear_schedule
+    and cosine_decay_schedule joined at the warmup boundary)."""
+    import math
+    from microcosmos_amd.grad import folding_schedule
+    lr, lo, wu, tot = 0.06, 3e-5, 15, 150
+    assert folding_schedule(0, lr, lo, wu, tot) == pytest.approx(lr * 0.01)
+    assert folding_schedule(5, lr, lo, wu, tot) == pytest.approx(lr * 0.01 + (lr - lr * 0.01) / 3)
+    assert folding_schedule(wu, lr, lo, wu, tot) == pytest.approx(lr)
+    mid = wu + (tot - wu) // 2
+    t = (mid - wu) / (tot - wu)
+    assert folding_schedule(mid, lr, lo, wu, tot) == pytest.approx(
+        lr * ((1 - lo / lr) * 0.5 * (1 + math.cos(math.pi * t)) + lo / lr))
+    assert folding_schedule(tot, lr, lo, wu, tot) == pytest.approx(lo)
+    assert folding_schedule(tot + 50, lr, lo, wu, tot) == pytest.approx(lo)
+
+
+def test_folding_optimizer_matches_adam_with_clipping():
+    """Three updates against a float64 restatement of optax's
+    chain(clip_by_global_norm, adam(schedule)): the first gradient is clipped,
+    the later ones are not."""
+    import torch
+    from microcosmos_amd.grad import FoldingOptimizer, folding_schedule
+    rng = np.random.default_rng(0)
+    p0 = rng.normal(size=12)
+    grads = [rng.normal(size=12) * 3.0, rng.normal(size=12) * 0.1, rng.normal(size=12) * 0.05]
+    opt = FoldingOptimizer(lr=0.06, lr_min=3e-5, warmup_steps=2, total_steps=10, grad_clip=1.0)
+    p = torch.tensor(p0, dtype=torch.float32)
+    mu = np.zeros(12)
+    nu = np.zeros(12)
+    q = p0.copy()
+    for k, g in enumerate(grads):
+        p = opt.update(p, torch.tensor(g, dtype=torch.float32))
+        n = np.sqrt((g * g).sum())
+        gc = g if n < 1.0 else g / n
+        mu = 0.9 * mu + 0.1 * gc
+        nu = 0.999 * nu + 0.001 * gc * gc
+        mh = mu / (1 - 0.9 ** (k + 1))
+        nh = nu / (1 - 0.999 ** (k + 1))
+        q = q - folding_schedule(k, 0.06, 3e-5, 2, 10) * mh / (np.sqrt(nh) + 1e-8)
+        np.testing.assert_allclose(p.numpy(), q, rtol=1e-5, atol=1e-6)
+
+
+def test_folding_optimizer_first_step_is_sign_times_lr():
+    """Adam's first bias-corrected step is lr * g / (|g| + eps) per component
+    (sign times lr up to eps), whatever the gradient's scale: clipping only
+    rescales g (here by its norm, 40.1, so the 1e-3 entry feels eps)."""
+    import torch
+    from microcosmos_amd.grad import FoldingOptimizer
+    g = torch.tensor([3.0, -0.5, 1e-3, -40.0])
+    opt = FoldingOptimizer(lr=0.06, warmup_steps=0, total_steps=10)
+    p = opt.update(torch.zeros(4), g)
+    gc = g.numpy().astype(np.float64) / np.linalg.norm(g.numpy().astype(np.float64))
+    np.testing.assert_allclose(p.numpy(), -0.06 * gc / (np.abs(gc) + 1e-8), rtol=1e-5)
+    assert np.all(np.abs(np.abs(p.numpy()) - 0.06) < 3e-5)
