@@ -243,7 +243,12 @@ mc_status mc_profile_step(mc_plan* plan, const mc_params* params, const mc_state
 
 /* Sub-operators of the step, each a drop-in for one reference function. */
 
-/* utils.py:35-49 gaussian_filter on an (H, W) field (H = grid_h, W = grid_w). */
+/* utils.py:35-49 gaussian_filter on an (H, W) field (H = grid_h, W = grid_w).
+ * The filter is the circular convolution with the exact periodic kernel of
+ * each axis when both kernels fit 129 taps (their dropped tails below 2^-27
+ * of their mass; sigma 2: 25 taps), else an FFT.  MC_ERR_UNSUPPORTED when the
+ * kernel is long and a side has a prime factor above 13 (no FFT plan); the
+ * same holds for every entry point with the steric term on. */
 mc_status mc_gaussian_filter(mc_plan* plan, const float* data, float* out, double sigma,
                              void* stream);
 

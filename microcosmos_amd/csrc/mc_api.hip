@@ -154,6 +154,7 @@ struct mc_plan {
   float2* spec = nullptr;
   std::vector<int2> shapes;     // distinct (H, W) steric grids, each with a kf table
   std::vector<long long> shape_kf_off;
+  std::vector<bool> shape_fft;  // both sides of the shape have FFT plans
   bool fft_ok = false;
   std::string fft_why;
   // immersed boundary + LBM
@@ -534,11 +535,15 @@ mc_status build_fft(mc_plan* p) {
   lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
   std::vector<FftDev> plans;
   std::vector<float2> tw;
+  std::vector<int> no_fft;  // sides without an FFT plan: the direct filter only
   for (int n : lens) {
     std::vector<int> r;
     if (!factor(n, r) || (int)r.size() > FFT_MAX_RADICES) {
-      p->fft_why = "steric FFT needs grid sides whose prime factors are all <= 13";
-      return MC_OK;
+      FftDev f{};
+      f.n = n;
+      plans.push_back(f);
+      no_fft.push_back(n);
+      continue;
     }
     FftDev f{};
     f.n = n;
@@ -590,6 +595,9 @@ mc_status build_fft(mc_plan* p) {
     if (sh == p->shapes.size()) {
       p->shapes.push_back(make_int2(g.H, g.W));
       p->shape_kf_off.push_back(kf_cells);
+      p->shape_fft.push_back(
+          !std::binary_search(no_fft.begin(), no_fft.end(), g.W) &&
+          !std::binary_search(no_fft.begin(), no_fft.end(), g.H));
       kf_cells += (long long)g.H * g.W;
     }
     g.kf_off = p->shape_kf_off[sh];
@@ -746,6 +754,12 @@ mc_status ensure_kf(mc_plan* p, double sigma, hipStream_t s) {
     const bool direct = gauss_taps(W, sigma, &lx, &tx) && gauss_taps(H, sigma, &ly, &ty) &&
                         4 + tx.size() + ty.size() <= (size_t)H * W;
     if (!direct) {
+      if (!p->shape_fft[i]) {
+        p->kf_sigma = -1.0;  // the tables are no longer those of any sigma
+        return fail(MC_ERR_UNSUPPORTED,
+                    "steric filter: sigma %g on a %dx%d grid needs more than %d taps and an FFT, "
+                    "whose sides must have prime factors <= 13", sigma, H, W, FILTER_TAPS_MAX);
+      }
       any_fft = true;
       HIPCHK(launch_gauss_table(H, W, coef, p->kf + p->shape_kf_off[i], s));
       continue;

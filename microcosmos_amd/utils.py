@@ -1,6 +1,6 @@
 """Geometry and filter primitives (utils.py:9-49).
 
-``gaussian_filter`` runs the device FFT; the small geometry helpers are
+``gaussian_filter`` runs the device filter; the small geometry helpers are
 elementwise torch expressions of the reference formulas, for callers that set
 up states (they are not on the step path, which does this arithmetic inside
 the HIP kernels).
@@ -46,7 +46,9 @@ def norm(v, axis=-1, keepdims=True):
 
 
 def gaussian_filter(data, sigma):
-    """utils.py:35-49 -- periodic Gaussian filter via the device LDS FFT."""
+    """utils.py:35-49 -- periodic Gaussian filter on the device: the direct
+    separable convolution when the periodic kernel is short, else the LDS
+    FFT."""
     dev = data.device
     lib = _lib.load()
     d = as_f32(data, dev)

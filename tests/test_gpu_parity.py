@@ -47,10 +47,11 @@ def _np(t):
 @pytest.mark.parametrize("shape,sigma", [((48, 48), 2.0), ((64, 96), 1.5), ((10, 10), 1.0),
                                          ((128, 128), 2.0), ((50, 50), 2.0),
                                          ((1024, 1024), 2.0), ((256, 128), 5.0),
-                                         ((256, 256), 1.5)])
+                                         ((256, 256), 1.5), ((34, 51), 2.0), ((262, 131), 2.0)])
 def test_gaussian_filter(dev, shape, sigma):
-    """Direct filter (short kernels) or FFT (sigma 1.5 on 256^2), each against
-    its restatement and against numpy's float64 FFT filter."""
+    """Direct filter (short kernels; also sides with prime factors above 13,
+    which have no FFT plan) or FFT (sigma 1.5 on 256^2), each against its
+    restatement and against numpy's float64 FFT filter."""
     _say(f"gaussian_filter shape={shape} sigma={sigma}")
     from microcosmos_amd.utils import gaussian_filter
     rng = np.random.default_rng(3)
@@ -64,6 +65,21 @@ def test_gaussian_filter(dev, shape, sigma):
     e = rel_l2(got, exp64)
     _say(f"  fp32 vs fp64 rel-L2 {e:.2e}")
     assert e <= 1e-6
+
+
+def test_gaussian_filter_needs_fft_side(dev):
+    """A kernel too long for the direct filter on a side without an FFT plan
+    (262 = 2 x 131, sigma 1.5) is refused with a clear error."""
+    _say("gaussian_filter 262x262 sigma 1.5: refused")
+    from microcosmos_amd.utils import gaussian_filter
+    assert R.gauss_taps(262, 1.5) is None
+    d = torch.zeros((262, 262), device=dev)
+    with pytest.raises(Exception, match="prime factors"):
+        gaussian_filter(d, 1.5)
+    # the same plan shape still filters with a short kernel
+    d[3, 5] = 1.0
+    f = _np(gaussian_filter(d, 2.0))
+    np.testing.assert_array_equal(f, R.gaussian_filter(_np(d), 2.0))
 
 
 def test_gaussian_filter_reference_kat(dev):
