@@ -232,6 +232,26 @@ mc_status mc_rod_adjoint_backward(mc_plan* plan, const mc_params* params,
                                   float* rest_angle_bar, float* position0_bar,
                                   float* velocity0_bar, float* theta0_bar, void* stream);
 
+/* Folding losses of the filament-folding experiment between particles (n, 2)
+ * and targets (m, 2), float32 device arrays, and their gradient with respect
+ * to the particles (grad (n, 2), may be NULL); loss is one device float:
+ *   MC_LOSS_CHAMFER     experiments/filament_folding/experiment.py:57-63
+ *   MC_LOSS_DCD         density-aware Chamfer, experiment.py:105-128
+ *   MC_LOSS_SMOOTH_MAX  logsumexp(alpha sqrt(min d^2 + 1e-8)) / alpha, :139-149
+ * The derivative of a minimum over tied entries is split equally among them
+ * (jax.grad's rule); the DCD counts carry no derivative (stop_gradient).
+ * workspace: mc_point_loss_workspace(n, m) bytes of device memory.
+ * Stream-ordered; no host synchronisation. */
+typedef enum mc_loss_kind {
+  MC_LOSS_CHAMFER = 0,
+  MC_LOSS_DCD = 1,
+  MC_LOSS_SMOOTH_MAX = 2
+} mc_loss_kind;
+size_t mc_point_loss_workspace(int64_t n, int64_t m);
+mc_status mc_point_loss(int32_t kind, const float* particles, int64_t n, const float* targets,
+                        int64_t m, double alpha, float* loss, float* grad, void* workspace,
+                        void* stream);
+
 /* Measurement: num_steps coupled steps in place (no graph, the LBM
  * collide-stream as its own launch) with HIP events carried by every LBM
  * dispatch and recorded after every step on `stream`; returns the average

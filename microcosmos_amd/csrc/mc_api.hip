@@ -1701,6 +1701,26 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
   return MC_OK;
 }
 
+size_t mc_point_loss_workspace(int64_t n, int64_t m) {
+  return n < 0 || m < 0 ? 0 : point_loss_workspace(n, m);
+}
+
+mc_status mc_point_loss(int32_t kind, const float* particles, int64_t n, const float* targets,
+                        int64_t m, double alpha, float* loss, float* grad, void* workspace,
+                        void* stream) {
+  g_err.clear();
+  if (kind != MC_LOSS_CHAMFER && kind != MC_LOSS_DCD && kind != MC_LOSS_SMOOTH_MAX)
+    return fail(MC_ERR_INVALID, "unknown loss kind %d", kind);
+  if (n < 1 || m < 1 || n > (1 << 24) || m > (1 << 24))
+    return fail(MC_ERR_BAD_SHAPE, "particle and target counts must be in 1 .. 2^24");
+  if (!particles || !targets || !loss || !workspace) return fail(MC_ERR_INVALID, "NULL argument");
+  if (kind == MC_LOSS_SMOOTH_MAX && !(alpha > 0.0))
+    return fail(MC_ERR_BAD_SHAPE, "alpha must be positive");
+  HIPCHK(launch_point_loss(kind, particles, (int)n, targets, (int)m, (float)alpha, loss, grad,
+                           workspace, (hipStream_t)stream));
+  return MC_OK;
+}
+
 mc_status mc_plan_instances(const mc_plan* p, int32_t* num_instances, int64_t* offsets) {
   g_err.clear();
   if (!p || !num_instances) return fail(MC_ERR_INVALID, "NULL argument");

@@ -226,6 +226,12 @@ hipError_t launch_steric_bar(const Consts& c, const BatchDev& b, const float* pr
 hipError_t launch_steric_adj_nodes(const Consts& c, const BatchDev& b, int N, const float* pred_w,
                                    const float* V, const float* dbar, const float* pb1, float* xb,
                                    float* vb, hipStream_t s);
+// Folding losses (mc_losses.hip): kind = mc_loss_kind; the workspace holds
+// point_loss_workspace(n, m) bytes; grad may be NULL.
+size_t point_loss_workspace(long long n, long long m);
+hipError_t launch_point_loss(int kind, const float* particles, int n, const float* targets, int m,
+                             float alpha, float* loss, float* grad, void* workspace,
+                             hipStream_t s);
 hipError_t launch_rod_segment(const RodAdjArgs& a, bool reverse, int steps, float* ck,
                               float* tape_or_dbg, size_t lds_bytes, hipStream_t s);
 // immersed boundary (solved per interaction cluster)

@@ -1,0 +1,234 @@
+// Folding losses of the filament-folding experiment and their gradients with
+// respect to the particle positions (experiments/filament_folding/
+// experiment.py): Chamfer (:57-63), density-aware Chamfer (:105-128) and the
+// smooth maximum of the nearest-neighbour distances (:139-149).
+//
+// Three launches:
+//   k_nn         one thread per particle and per target: the nearest point of
+//                the other cloud (squared distance, first index, number of
+//                tied minima), a pass over the other cloud staged in LDS;
+//   k_loss       one workgroup: the density counts (DCD), the loss in a fixed
+//                order, and per particle the weight of its own term;
+//   k_loss_grad  one thread per particle: 2 w (p - t) over the targets tied at
+//                its minimum, plus the targets whose tied minima include it
+//                (jax's min rule: a tied minimum's derivative is split
+//                equally), in target order.
+// Squared distances are dx * dx + dy * dy in float32 (contraction off), the
+// same expression in every pass, so the tie tests are exact.
+#include "mc_common.h"
+#include "mc_internal.h"
+#include "../../include/microcosmos_hip.h"
+
+namespace mc {
+namespace {
+
+constexpr int NN_THREADS = 256;
+constexpr int LOSS_THREADS = 1024;
+
+struct LossWs {
+  float *dmin_p, *dmin_t, *cnt_p, *cnt_t, *wown, *scal;
+  int *arg_p, *tie_p, *arg_t, *tie_t;
+};
+
+__host__ __device__ inline LossWs loss_ws(void* base, long long n, long long m) {
+  LossWs w;
+  float* f = reinterpret_cast<float*>(base);
+  w.dmin_p = f;
+  w.cnt_t = w.dmin_p + n;
+  w.wown = w.cnt_t + n;
+  w.dmin_t = w.wown + n;
+  w.cnt_p = w.dmin_t + m;
+  w.scal = w.cnt_p + m;
+  int* q = reinterpret_cast<int*>(w.scal + 4);
+  w.arg_p = q;
+  w.tie_p = w.arg_p + n;
+  w.arg_t = w.tie_p + n;
+  w.tie_t = w.arg_t + m;
+  return w;
+}
+
+__device__ __forceinline__ float sqdist(float ax, float ay, float bx, float by) {
+  const float dx = ax - bx, dy = ay - by;
+  return dx * dx + dy * dy;
+}
+
+// blocks [0, bp): particles against the targets; [bp, grid): targets against
+// the particles.  The other cloud is read in LDS tiles of NN_THREADS points.
+__global__ void __launch_bounds__(NN_THREADS) k_nn(const float2* __restrict__ p, int n,
+                                                   const float2* __restrict__ t, int m, int bp,
+                                                   LossWs w) {
+  __shared__ float2 tile[NN_THREADS];
+  const bool rows = (int)blockIdx.x < bp;
+  const int i = (rows ? blockIdx.x : blockIdx.x - bp) * NN_THREADS + threadIdx.x;
+  const float2* self = rows ? p : t;
+  const float2* other = rows ? t : p;
+  const int ns = rows ? n : m, no = rows ? m : n;
+  const float2 a = i < ns ? self[i] : make_float2(0.0f, 0.0f);
+  float best = __builtin_inff();
+  int arg = 0, ties = 0;
+  for (int j0 = 0; j0 < no; j0 += NN_THREADS) {
+    __syncthreads();
+    if (j0 + (int)threadIdx.x < no) tile[threadIdx.x] = other[j0 + threadIdx.x];
+    __syncthreads();
+    const int cnt = min(NN_THREADS, no - j0);
+    for (int k = 0; k < cnt; ++k) {
+      const float2 b = tile[k];
+      const float d = rows ? sqdist(a.x, a.y, b.x, b.y) : sqdist(b.x, b.y, a.x, a.y);
+      if (d < best || (d != d && best == best)) {
+        best = d;
+        arg = j0 + k;
+        ties = 1;
+      } else if (d == best) {
+        ++ties;
+      }
+    }
+  }
+  if (i < ns) {
+    if (rows) {
+      w.dmin_p[i] = best;
+      w.arg_p[i] = arg;
+      w.tie_p[i] = ties;
+    } else {
+      w.dmin_t[i] = best;
+      w.arg_t[i] = arg;
+      w.tie_t[i] = ties;
+    }
+  }
+}
+
+// Fixed-order block sum of per-thread partials (stride-ordered, then a tree).
+__device__ float block_sum(float v, float* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = LOSS_THREADS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const float r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ float block_max(float v, float* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = LOSS_THREADS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax_nan(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const float r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(LOSS_THREADS) k_loss(int kind, int n, int m, float alpha,
+                                                       LossWs w, float* __restrict__ loss) {
+  __shared__ float red[LOSS_THREADS];
+  const int tid = threadIdx.x;
+  if (kind == MC_LOSS_DCD) {
+    // counts: particles per target (argmin rows), targets per particle
+    for (int j = tid; j < m; j += LOSS_THREADS) {
+      int c = 0;
+      for (int i = 0; i < n; ++i) c += w.arg_p[i] == j ? 1 : 0;
+      w.cnt_p[j] = (float)c;
+    }
+    for (int i = tid; i < n; i += LOSS_THREADS) {
+      int c = 0;
+      for (int j = 0; j < m; ++j) c += w.arg_t[j] == i ? 1 : 0;
+      w.cnt_t[i] = (float)c;
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+  if (kind == MC_LOSS_SMOOTH_MAX) {
+    float zm = -__builtin_inff();
+    for (int i = tid; i < n; i += LOSS_THREADS)
+      zm = fmax_nan(zm, alpha * sqrtf(w.dmin_p[i] + 1e-8f));
+    const float zmax = block_max(zm, red);
+    float s = 0.0f;
+    for (int i = tid; i < n; i += LOSS_THREADS)
+      s = s + expf(alpha * sqrtf(w.dmin_p[i] + 1e-8f) - zmax);
+    const float tot = block_sum(s, red);
+    for (int i = tid; i < n; i += LOSS_THREADS) {
+      const float r = sqrtf(w.dmin_p[i] + 1e-8f);
+      w.wown[i] = (expf(alpha * r - zmax) / tot) / (2.0f * r);
+    }
+    if (tid == 0) *loss = (zmax + logf(tot)) / alpha;
+    return;
+  }
+  const bool dcd = kind == MC_LOSS_DCD;
+  float sp = 0.0f, st = 0.0f;
+  for (int i = tid; i < n; i += LOSS_THREADS) {
+    const float c = dcd ? w.cnt_p[w.arg_p[i]] : 1.0f;
+    sp = sp + w.dmin_p[i] / c;
+    w.wown[i] = 1.0f / ((float)n * c);
+  }
+  for (int j = tid; j < m; j += LOSS_THREADS) {
+    const float c = dcd ? w.cnt_t[w.arg_t[j]] : 1.0f;
+    st = st + w.dmin_t[j] / c;
+  }
+  const float a = block_sum(sp, red), b = block_sum(st, red);
+  if (tid == 0) *loss = a / (float)n + b / (float)m;
+}
+
+__global__ void __launch_bounds__(NN_THREADS) k_loss_grad(int kind, const float2* __restrict__ p,
+                                                          int n, const float2* __restrict__ t,
+                                                          int m, LossWs w,
+                                                          float* __restrict__ grad) {
+  __shared__ float2 tile[NN_THREADS];
+  const int i = blockIdx.x * NN_THREADS + threadIdx.x;
+  const float2 a = i < n ? p[i] : make_float2(0.0f, 0.0f);
+  const float own = i < n ? w.wown[i] / (float)w.tie_p[i] : 0.0f;
+  const float best = i < n ? w.dmin_p[i] : 0.0f;
+  const bool targets = kind != MC_LOSS_SMOOTH_MAX;
+  const bool dcd = kind == MC_LOSS_DCD;
+  float gx = 0.0f, gy = 0.0f;
+  for (int j0 = 0; j0 < m; j0 += NN_THREADS) {
+    __syncthreads();
+    if (j0 + (int)threadIdx.x < m) tile[threadIdx.x] = t[j0 + threadIdx.x];
+    __syncthreads();
+    const int cnt = min(NN_THREADS, m - j0);
+    if (i >= n) continue;
+    for (int k = 0; k < cnt; ++k) {
+      const int j = j0 + k;
+      const float2 b = tile[k];
+      const float d = sqdist(a.x, a.y, b.x, b.y);
+      float wt = 0.0f;
+      if (d == best) wt = own;
+      if (targets && d == w.dmin_t[j]) {
+        const float c = dcd ? w.cnt_t[w.arg_t[j]] : 1.0f;
+        wt = wt + 1.0f / ((float)m * c) / (float)w.tie_t[j];
+      }
+      if (wt != 0.0f) {
+        gx = gx + wt * (a.x - b.x);
+        gy = gy + wt * (a.y - b.y);
+      }
+    }
+  }
+  if (i < n) {
+    grad[2 * i] = 2.0f * gx;
+    grad[2 * i + 1] = 2.0f * gy;
+  }
+}
+
+}  // namespace
+
+size_t point_loss_workspace(long long n, long long m) {
+  return (size_t)(5 * n + 4 * m + 4) * sizeof(float);
+}
+
+hipError_t launch_point_loss(int kind, const float* particles, int n, const float* targets, int m,
+                             float alpha, float* loss, float* grad, void* workspace,
+                             hipStream_t s) {
+  const LossWs w = loss_ws(workspace, n, m);
+  const int bp = (n + NN_THREADS - 1) / NN_THREADS, bt = (m + NN_THREADS - 1) / NN_THREADS;
+  const float2* p = reinterpret_cast<const float2*>(particles);
+  const float2* t = reinterpret_cast<const float2*>(targets);
+  hipLaunchKernelGGL(k_nn, dim3(bp + bt), dim3(NN_THREADS), 0, s, p, n, t, m, bp, w);
+  hipLaunchKernelGGL(k_loss, dim3(1), dim3(LOSS_THREADS), 0, s, kind, n, m, alpha, w, loss);
+  if (grad)
+    hipLaunchKernelGGL(k_loss_grad, dim3(bp), dim3(NN_THREADS), 0, s, kind, p, n, t, m, w, grad);
+  return hipGetLastError();
+}
+
+}  // namespace mc

@@ -127,13 +127,20 @@ class RodAdjoint:
 
 
 def rollout_value_and_grad(nodes: Nodes, edges: Edges, fields: Fields, solver_config, dt: float,
-                           num_steps: int, target: torch.Tensor, checkpoint_every: int = 50):
-    """Folding loss after ``num_steps`` steps -- the mean over nodes of the
-    squared minimum-image distance to ``target`` (N, 2) -- and its gradient
-    with respect to the (time-constant) bending rest angles."""
+                           num_steps: int, target: torch.Tensor, checkpoint_every: int = 50,
+                           loss_fn=None):
+    """Folding loss after ``num_steps`` steps and its gradient with respect
+    to the (time-constant) bending rest angles.  The loss is the mean over
+    nodes of the squared minimum-image distance to ``target`` (N, 2), or
+    ``loss_fn(positions, target) -> (loss, d loss / d positions)`` -- e.g.
+    ``losses.chamfer_loss`` or ``losses.dcd_loss`` against a target point
+    cloud (M, 2)."""
     ad = RodAdjoint(nodes, edges, fields, solver_config, dt)
     x = ad.forward(num_steps, checkpoint_every)
     tgt = as_f32(target, ad.dev)
+    if loss_fn is not None:
+        loss, gx = loss_fn(x, tgt)
+        return loss, ad.backward(gx)[0]
     dvec = min_image(ad.grid_shape, x, tgt)
     n = x.shape[0]
     loss = (dvec * dvec).sum() / n
@@ -191,19 +198,21 @@ class FoldingOptimizer:
 
 def optimize_folding(nodes: Nodes, edges: Edges, fields: Fields, solver_config, dt: float,
                      num_steps: int, target: torch.Tensor, iterations: int,
-                     optimizer: FoldingOptimizer = None, checkpoint_every: int = 50):
+                     optimizer: FoldingOptimizer = None, checkpoint_every: int = 50,
+                     loss_fn=None):
     """The folding experiment's optimisation loop over the bending rest angles
     (experiment.py:985-1030 without curriculum, noise or visualisation): each
     iteration rolls the rod out from the same initial state, takes the loss of
     the final positions and its gradient through the adjoint kernels, and
-    applies ``optimizer``.  Returns (rest angles, per-iteration losses)."""
+    applies ``optimizer``; ``loss_fn`` as in ``rollout_value_and_grad``.
+    Returns (rest angles, per-iteration losses)."""
     opt = optimizer or FoldingOptimizer(total_steps=iterations)
     rest = as_f32(edges.bending_rest_angles, nodes.position.device).clone()
     losses = []
     for _ in range(iterations):
         e = replace(edges, bending_rest_angles=rest)
         loss, g = rollout_value_and_grad(nodes, e, fields, solver_config, dt, num_steps, target,
-                                         checkpoint_every)
+                                         checkpoint_every, loss_fn)
         losses.append(float(loss.item()))
         rest = opt.update(rest, g)
     return rest, losses

@@ -1,0 +1,103 @@
+"""Folding losses of the filament-folding experiment on the device
+(experiments/filament_folding/experiment.py:57-165), each returning the loss
+and its gradient with respect to the particle positions -- the cotangent the
+rod adjoint (``grad.RodAdjoint.backward``) takes.
+
+The nearest-neighbour search, the density counts, the loss and the gradient
+run in HIP kernels (``mc_point_loss``); the PCA alignment's 2 x 2 principal
+axes and the rotation of the centred cloud are a handful of elementwise device
+operations around them.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _lib
+from ._lib import check
+from ._runtime import as_f32, ptr, stream_handle
+
+CHAMFER, DCD, SMOOTH_MAX = 0, 1, 2
+
+
+def _point_loss(kind: int, particles: torch.Tensor, targets: torch.Tensor, alpha: float = 0.0):
+    d = particles.device
+    if d.type != "cuda":
+        raise RuntimeError("microcosmos_amd runs on the GPU only")
+    lib = _lib.load()
+    p = as_f32(particles, d)
+    t = as_f32(targets, d)
+    n, m = p.shape[0], t.shape[0]
+    ws = torch.empty(int(lib.mc_point_loss_workspace(n, m)), dtype=torch.uint8, device=d)
+    loss = torch.empty((), dtype=torch.float32, device=d)
+    grad = torch.empty((n, 2), dtype=torch.float32, device=d)
+    check(lib.mc_point_loss(kind, ptr(p), n, ptr(t), m, float(alpha), ptr(loss), ptr(grad),
+                            ptr(ws), stream_handle(d)))
+    return loss, grad
+
+
+def chamfer_loss(particles: torch.Tensor, targets: torch.Tensor):
+    """experiment.py:57-63 -- mean_i min_j |p_i - t_j|^2 + mean_j min_i
+    |p_i - t_j|^2; returns (loss, d loss / d particles)."""
+    return _point_loss(CHAMFER, particles, targets)
+
+
+def dcd_loss(particles: torch.Tensor, targets: torch.Tensor):
+    """experiment.py:105-128 -- density-aware Chamfer: each nearest-neighbour
+    term divided by how many points chose the same neighbour."""
+    return _point_loss(DCD, particles, targets)
+
+
+def smooth_max_nn(particles: torch.Tensor, targets: torch.Tensor, alpha: float = 0.1):
+    """experiment.py:139-149 -- logsumexp(alpha sqrt(min_j d2 + 1e-8)) / alpha."""
+    return _point_loss(SMOOTH_MAX, particles, targets, alpha)
+
+
+def _principal_angle(x: torch.Tensor) -> torch.Tensor:
+    # direction of the eigenvector of the larger eigenvalue of the 2 x 2
+    # covariance + 1e-6 I (experiment.py:74-81), up to its sign: half the
+    # angle of (2 c_xy, c_xx - c_yy)
+    c = x - x.mean(dim=0)
+    cxx = (c[:, 0] * c[:, 0]).mean() + 1e-6
+    cyy = (c[:, 1] * c[:, 1]).mean() + 1e-6
+    cxy = (c[:, 0] * c[:, 1]).mean()
+    return 0.5 * torch.atan2(2.0 * cxy, cxx - cyy)
+
+
+def pca_rotation_angle(particles: torch.Tensor, targets: torch.Tensor) -> float:
+    """experiment.py:66-84: the targets' principal-axis angle minus the
+    particles' (each defined up to pi, which the callers' +-pi minimum
+    absorbs)."""
+    a = _principal_angle(as_f32(targets, targets.device).double()) - \
+        _principal_angle(as_f32(particles, particles.device).double())
+    return float(a.item())
+
+
+def pca_aligned(loss_fn, particles: torch.Tensor, targets: torch.Tensor, **kw):
+    """The *_pca_aligned wrappers (experiment.py:95-102, 131-136, 152-160):
+    the loss of the centred particles rotated by the principal-axis angle and
+    by the angle + pi against the centred targets, the smaller one kept; the
+    angle carries no gradient, the rotation and the centring do."""
+    p = as_f32(particles, particles.device)
+    t = as_f32(targets, targets.device)
+    a = pca_rotation_angle(p, t)
+    tc = t - t.mean(dim=0)
+    best = None
+    for ang in (a, a + math.pi):
+        co, si = math.cos(ang), math.sin(ang)
+        c = p - p.mean(dim=0)
+        q = torch.stack([c[:, 0] * co - c[:, 1] * si, c[:, 0] * si + c[:, 1] * co], dim=-1)
+        l, gq = loss_fn(q, tc, **kw)
+        gc = torch.stack([gq[:, 0] * co + gq[:, 1] * si, -gq[:, 0] * si + gq[:, 1] * co], dim=-1)
+        g = gc - gc.mean(dim=0)
+        if best is None:
+            best = (l, g)
+        else:
+            l0, g0 = best
+            lv0, lv1 = float(l0.item()), float(l.item())
+            if lv1 < lv0:
+                best = (l, g)
+            elif lv1 == lv0:
+                best = (l0, 0.5 * (g0 + g))
+    return best

@@ -286,17 +286,23 @@ def msd_loss(position, target, grid_shape):
 
 
 def rollout_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, target,
-                 dtype=np.float64):
+                 dtype=np.float64, loss_fn=None):
     """Loss after ``num_steps`` steps and its gradient with respect to the
     (time-constant) bending rest angles, by storing every step's state.
     ``dtype`` is the precision of the forward trajectory (and of each step's
     recomputed tape); the adjoint arithmetic is float64 either way.  A float32
     trajectory is the device's own (bit-identical steps), which separates the
-    adjoint's accuracy from the divergence of float32 and float64 rollouts."""
+    adjoint's accuracy from the divergence of float32 and float64 rollouts.
+    ``loss_fn(position) -> (loss, d loss / d position)`` replaces the mean
+    squared distance to ``target`` (e.g. the folding losses of
+    ``oracle.losses``)."""
     states = [s0.astype(dtype)]
     for _ in range(num_steps):
         states.append(R.step(states[-1], dt, p))
-    loss, xb = msd_loss(states[-1].position, target, s0.grid_shape)
+    if loss_fn is None:
+        loss, xb = msd_loss(states[-1].position, target, s0.grid_shape)
+    else:
+        loss, xb = loss_fn(np.asarray(states[-1].position, np.float64))
     vb = np.zeros_like(xb)
     thb = np.zeros(s0.theta.shape[0])
     rhob = np.zeros(s0.bending_pairs.shape[0])
