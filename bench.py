@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from dataclasses import replace
@@ -189,6 +190,21 @@ def serving(n_req, dev, lanes, chunk, max_batch, plain=True):
     return out
 
 
+def profiled_lbm_traffic(h, w):
+    """HBM bytes per `k_lbm` launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by its plain-copy calibration) of
+    this workload, or None when the summary is absent or for another lattice."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "coupled_step_1024x1024_256x64.md")
+    if (h, w) != (1024, 1024) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"`k_lbm`: ([0-9.]+) MB per launch", line)
+        if m:
+            return int(round(float(m.group(1)) * 1e6))
+    return None
+
+
 def gradient_leg(dev, reps=5, scheme="pbd"):
     """The folding gradient: loss of the final positions after a rollout on
     the folding workload (256^2, 8 filaments x 48 nodes) and its gradient with
@@ -326,7 +342,10 @@ def main():
     lbm_bytes = LBM_BYTES_PER_CELL * h * w
     achieved = lbm_bytes / (lbm_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_lbm_traffic(h, w),
+                "traffic_source": "profiles/coupled_step_1024x1024_256x64.md (rocprofv3 --pmc "
+                                  "FETCH_SIZE + WRITE_SIZE, plain-copy calibrated)",
                 "kernel": "k_lbm", "avg_kernel_ms": round(lbm_ms, 5),
                 "bytes_per_launch": lbm_bytes}
     batched = None
