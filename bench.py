@@ -261,6 +261,54 @@ def gradient_leg(dev, reps=5, scheme="pbd"):
             "finite": bool(ok)}
 
 
+def gradient_batch_leg(dev, batch=4, reps=5):
+    """The folding experiment's batched objective (experiment.py:915-923
+    vmaps the loss over optimization.batch_size = 4,
+    experiments/conf/experiment/filament_folding.yaml:67): ``batch`` folding
+    instances (seeds 3..) with the experiment's default physics, 150 steps,
+    one batched forward and one batched adjoint; gradients/s counts instance
+    gradients."""
+    import numpy as np
+    import torch
+    from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID
+    from microcosmos_amd import workloads as W
+    from microcosmos_amd.grad import BatchRodAdjoint, min_image
+    from tests.helpers import device_state
+    steps, dt = W.FOLDING_STEPS, W.FOLDING_DT
+    cfg = replace(COSSERAT_SCHEME_NO_FLUID, dt=dt, **W.FOLDING_COSSERAT)
+    wls = [W.folding(seed=3 + k, steps=steps) for k in range(batch)]
+    tg = [torch.as_tensor(t, device=dev) for _, t in wls]
+
+    def one():
+        ad = BatchRodAdjoint([device_state(wl, dev) for wl, _ in wls], cfg, dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xs = ad.forward(steps, 1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        bars = []
+        for (wl, _), x, t in zip(wls, xs, tg):
+            d = min_image(wl.grid_shape, x, t)
+            bars.append(2.0 * d / x.shape[0])
+        gs = ad.backward(bars)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, all(bool(torch.isfinite(g).all().item()) for g in gs)
+
+    one()
+    fw, bw = [], []
+    ok = True
+    for _ in range(reps):
+        a, b, o = one()
+        fw.append(a)
+        bw.append(b)
+        ok = ok and o
+    f_ms, b_ms = float(np.median(fw)) * 1e3, float(np.median(bw)) * 1e3
+    return {"workload": f"folding-256x256-8x48 x {batch}", "scheme": "folding", "batch": batch,
+            "steps": steps, "dt": dt, "forward_ms": round(f_ms, 3), "backward_ms": round(b_ms, 3),
+            "gradients_per_s": round(batch * 1e3 / (f_ms + b_ms), 2), "finite": bool(ok)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,7 +331,7 @@ def main():
     ap.add_argument("--only-serve", action="store_true",
                     help="run only the serving leg and print its JSON")
     ap.add_argument("--no-grad", action="store_true")
-    ap.add_argument("--only-grad", choices=["pbd", "cosserat", "folding"],
+    ap.add_argument("--only-grad", choices=["pbd", "cosserat", "folding", "folding-batch4"],
                     help="run only the gradient leg of this scheme and print its JSON")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 9 x 1024 x 1024 float32 device copy (37.7 MB read + "
@@ -312,7 +360,9 @@ def main():
 
     dev = torch.device("cuda", 0)
     if args.only_grad:
-        print(json.dumps(gradient_leg(dev, scheme=args.only_grad)), flush=True)
+        leg = gradient_batch_leg(dev, 4) if args.only_grad == "folding-batch4" else \
+            gradient_leg(dev, scheme=args.only_grad)
+        print(json.dumps(leg), flush=True)
         return
     if args.only_serve:
         print(json.dumps(serving(args.serve_requests, dev, args.serve_lanes, args.serve_chunk,
@@ -356,6 +406,7 @@ def main():
         grad = gradient_leg(dev)
         grad["stable_cosserat"] = gradient_leg(dev, scheme="cosserat")
         grad["folding_physics"] = gradient_leg(dev, scheme="folding")
+        grad["folding_physics_batch4"] = gradient_batch_leg(dev, 4)
         print(f"[bench] gradient {grad}", file=sys.stderr, flush=True)
     served = None
     if not args.no_serve:

@@ -216,3 +216,76 @@ def optimize_folding(nodes: Nodes, edges: Edges, fields: Fields, solver_config, 
         losses.append(float(loss.item()))
         rest = opt.update(rest, g)
     return rest, losses
+
+
+class BatchRodAdjoint:
+    """Rod adjoints of several independent instances in one batched plan
+    (the batch the folding experiment vmaps over, experiment.py:915-923): the
+    same forward and reverse launches carry every instance's filaments, so a
+    batch costs about as much time as one instance while the kernels stay
+    latency-bound.  ``instances`` is a list of (Nodes, Edges, Fields); every
+    instance's gradient equals the one it gets alone (the kernels never mix
+    instances)."""
+
+    def __init__(self, instances, solver_config, dt: float):
+        from .batch import Batch
+        if solver_config.enable_fluid or \
+                getattr(solver_config, "solver_type", "pbd") not in ("pbd", "cosserat"):
+            raise NotImplementedError("rod adjoint: PBD or stable Cosserat constraint with "
+                                      "enable_fluid=False")
+        self.batch = Batch(instances, solver_config, dt)
+        if self.batch.dev.type != "cuda":
+            raise RuntimeError("microcosmos_amd runs on the GPU only")
+        self.dt = float(dt)
+        self.steps = 0
+
+    def forward(self, num_steps: int, checkpoint_every: int = 50):
+        """Advances every instance ``num_steps`` steps; returns the list of
+        final (N_i, 2) positions."""
+        b = self.batch
+        lib = _lib.load()
+        st = b._state()
+        ep = _lib.McEdgeParams(ptr(b.rest_lengths), ptr(b.rest_angles), ptr(b.stiffness))
+        self._ep = ep
+        check(lib.mc_rod_adjoint_forward(b.plan.handle, ctypes.byref(b.params), ctypes.byref(st),
+                                         ctypes.byref(ep), self.dt, int(num_steps),
+                                         int(checkpoint_every), stream_handle(b.dev)))
+        self.steps = int(num_steps)
+        x = b.position.view(-1, 2)
+        return [x[int(b.node_off[i]):int(b.node_off[i + 1])] for i in range(b.n)]
+
+    def backward(self, position_bars):
+        """position_bars: per instance d loss_i / d final positions (N_i, 2).
+        Returns the per-instance rest-angle gradients (B_i,)."""
+        b = self.batch
+        lib = _lib.load()
+        pb = torch.cat([as_f32(g, b.dev).reshape(-1) for g in position_bars]).contiguous()
+        rg = torch.empty(int(b.hinge_off[-1]), dtype=torch.float32, device=b.dev)
+        check(lib.mc_rod_adjoint_backward(b.plan.handle, ctypes.byref(b.params),
+                                          ctypes.byref(self._ep), self.dt, ptr(pb), None, ptr(rg),
+                                          None, None, None, stream_handle(b.dev)))
+        return [rg[int(b.hinge_off[i]):int(b.hinge_off[i + 1])] for i in range(b.n)]
+
+
+def batch_value_and_grad(instances, solver_config, dt: float, num_steps: int, targets,
+                         checkpoint_every: int = 50, loss_fn=None):
+    """Per-instance folding losses and rest-angle gradients of a batch of
+    independent rollouts (one batched adjoint).  ``targets`` holds one target
+    per instance; ``loss_fn`` as in ``rollout_value_and_grad`` (default: the
+    mean squared minimum-image distance).  The folding experiment's batched
+    objective is the mean of the losses, whose gradient with respect to shared
+    rest angles is the mean of these gradients."""
+    ad = BatchRodAdjoint(instances, solver_config, dt)
+    xs = ad.forward(num_steps, checkpoint_every)
+    losses, bars = [], []
+    for (nodes, _, fields), x, t in zip(instances, xs, targets):
+        tgt = as_f32(t, x.device)
+        if loss_fn is not None:
+            loss, gx = loss_fn(x, tgt)
+        else:
+            dvec = min_image(tuple(fields.grid_shape), x, tgt)
+            n = x.shape[0]
+            loss, gx = (dvec * dvec).sum() / n, 2.0 * dvec / n
+        losses.append(loss)
+        bars.append(gx)
+    return losses, ad.backward(bars)
