@@ -99,7 +99,7 @@ class _Slot:
 @dataclass
 class _GradLane:
     stream: torch.cuda.Stream
-    req: Optional[Request] = None
+    reqs: List[Request] = field(default_factory=list)
     adj: Optional[object] = None
     event: Optional[torch.cuda.Event] = None
 
@@ -187,7 +187,7 @@ class Server:
 
     @property
     def active(self) -> int:
-        return sum(len(l.slots) for l in self.lanes) + (1 if self.glane.req is not None else 0)
+        return sum(len(l.slots) for l in self.lanes) + len(self.glane.reqs)
 
     def idle(self) -> bool:
         return not self.waiting and not self.gwaiting and self.active == 0
@@ -214,37 +214,51 @@ class Server:
         return done
 
     def _poll_gradient(self) -> List[Request]:
-        """The gradient lane: retire the finished request, start the next."""
+        """The gradient lane: retire the finished batch, start the next one --
+        the waiting gradient requests at the head of the queue with the same
+        step count (up to max_batch, within the budget), one batched adjoint
+        (each request's gradient is the one it gets alone)."""
         g = self.glane
         out: List[Request] = []
-        if g.req is not None:
+        if g.reqs:
             if not g.event.query():
                 return out
-            r = g.req
-            r.finished = self.clock()
-            g.adj.plan.release(int(g.stream.cuda_stream))
-            self.used -= r.footprint
-            self.completed.append(r)
-            out.append(r)
-            g.req, g.adj, g.event = None, None, None
-        if not self.gwaiting or self.used + self.gwaiting[0].footprint > self.budget:
+            now = self.clock()
+            for r in g.reqs:
+                r.finished = now
+                self.used -= r.footprint
+                self.completed.append(r)
+                out.append(r)
+            g.adj.batch.plan.release(int(g.stream.cuda_stream))
+            g.reqs, g.adj, g.event = [], None, None
+        take: List[Request] = []
+        while (self.gwaiting and len(take) < self.max_batch
+               and self.gwaiting[0].steps == (take[0].steps if take else self.gwaiting[0].steps)
+               and self.used + self.gwaiting[0].footprint <= self.budget):
+            r = self.gwaiting.pop(0)
+            self.used += r.footprint
+            self.peak_used = max(self.peak_used, self.used)
+            r.admitted = self.clock()
+            take.append(r)
+        if not take:
             return out
-        from .grad import RodAdjoint, min_image
-        r = self.gwaiting.pop(0)
-        self.used += r.footprint
-        self.peak_used = max(self.peak_used, self.used)
-        r.admitted = self.clock()
+        from .grad import BatchRodAdjoint, min_image
         with torch.cuda.stream(g.stream):
             g.stream.wait_stream(torch.cuda.current_stream(self.dev))
-            adj = RodAdjoint(r.nodes, r.edges, r.fields, self.gradient_config, self.dt)
-            x = adj.forward(r.steps, self.checkpoint_every)
-            d = min_image(adj.grid_shape, x, r.target)
-            n = x.shape[0]
-            r.loss = (d * d).sum() / n
-            r.rest_angle_grad = adj.backward(2.0 * d / n)[0]
+            adj = BatchRodAdjoint([(r.nodes, r.edges, r.fields) for r in take],
+                                  self.gradient_config, self.dt)
+            xs = adj.forward(take[0].steps, self.checkpoint_every)
+            bars = []
+            for r, x in zip(take, xs):
+                d = min_image(tuple(r.fields.grid_shape), x, r.target)
+                n = x.shape[0]
+                r.loss = (d * d).sum() / n
+                bars.append(2.0 * d / n)
+            for r, gr in zip(take, adj.backward(bars)):
+                r.rest_angle_grad = gr
             ev = torch.cuda.Event()
             ev.record(g.stream)
-        g.req, g.adj, g.event = r, adj, ev
+        g.reqs, g.adj, g.event = take, adj, ev
         return out
 
     def _admit(self, lane: _Lane, share: int) -> List[Request]:
