@@ -448,7 +448,7 @@ mc_status build_instances(mc_plan* p) {
   const int n = p->n_inst;
   p->geo_h.assign(n, InstGeo{});
   p->node_inst_h.assign(p->N, 0);
-  std::vector<int> row_inst, col_inst, lbm_blk(n + 1, 0);
+  std::vector<int> row_inst, col_inst, lbm_blk(n + 1, 0), ctile(n + 1, 0);
   long long go = 0, fo = 0, mo = 0;
   int r0 = 0, c0 = 0, b0 = 0;
   p->min_hf = 1 << 30;
@@ -487,6 +487,8 @@ mc_status build_instances(mc_plan* p) {
     c0 += h.W;
     b0 += (int)((hwf + LBM_THREADS - 1) / LBM_THREADS);
     lbm_blk[i + 1] = b0;
+    ctile[i + 1] = ctile[i] + ((h.H + FILTER_TILE_H - 1) / FILTER_TILE_H) *
+                                  ((h.W + FILTER_TILE_W - 1) / FILTER_TILE_W);
     for (int k = 0; k < h.nodes; ++k) p->node_inst_h[h.node0 + k] = i;
     p->max_w = std::max(p->max_w, h.W);
     p->max_h = std::max(p->max_h, h.H);
@@ -499,11 +501,12 @@ mc_status build_instances(mc_plan* p) {
   p->rows = r0;
   p->cols = c0;
   p->lbm_blocks = b0;
-  int *ni, *ri, *cc, *lb;
+  int *ni, *ri, *cc, *lb, *ct;
   MCCHK(upload(p, &ni, p->node_inst_h, "node_inst"));
   MCCHK(upload(p, &ri, row_inst, "row_inst"));
   MCCHK(upload(p, &cc, col_inst, "col_inst"));
   MCCHK(upload(p, &lb, lbm_blk, "lbm blocks"));
+  MCCHK(upload(p, &ct, ctile, "filter tiles"));
   MCCHK(p->alloc(&p->geo_d, (size_t)n, "instance geometry"));
   BatchDev& b = p->bd;
   b.n_inst = n;
@@ -512,6 +515,8 @@ mc_status build_instances(mc_plan* p) {
   b.row_inst = ri;
   b.col_inst = cc;
   b.lbm_blk = lb;
+  b.ctile = ct;
+  b.ctiles = ctile[n];
   b.rows = p->rows;
   b.cols = p->cols;
   b.lbm_blocks = p->lbm_blocks;

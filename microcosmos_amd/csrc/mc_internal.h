@@ -76,6 +76,8 @@ struct BatchDev {
   const int* row_inst;        // (rows) instance of every global steric row
   const int* col_inst;        // (cols) instance of every global steric column
   const int* lbm_blk;         // (n_inst + 1) LBM workgroup offsets
+  const int* ctile;           // (n_inst + 1) direct-filter column-pass tile offsets
+  int ctiles;                 // total column-pass tiles
   const FftDev* fft;
   const float2* tw;           // concatenated twiddle tables
   int rows, cols, lbm_blocks; // totals over instances
@@ -84,6 +86,8 @@ struct BatchDev {
 };
 
 constexpr int LBM_THREADS = 256;  // lattice cells per LBM workgroup
+// Output tile of the direct filter's column pass (columns x rows).
+constexpr int FILTER_TILE_W = 64, FILTER_TILE_H = 32;
 
 // Instance of item i, given per-instance offsets off[0..n] (off[0] = 0).
 __device__ __forceinline__ int find_inst(const int* off, int n, int i) {

@@ -775,60 +775,58 @@ __global__ void __launch_bounds__(256) k_rows_conv(BatchDev bd, const float* __r
   }
 }
 
-// Column pass: one output row per workgroup; each thread takes COLS_COLS
-// columns at once and loads COLS_BATCH input rows of each before using them
-// (COLS_COLS x COLS_BATCH loads in flight); every output sums its taps in
-// order.
-constexpr int COLS_COLS = 4;
-constexpr int COLS_BATCH = 8;
-__global__ void __launch_bounds__(256) k_cols_conv(BatchDev bd, const float* __restrict__ taps,
-                                                   const float* __restrict__ tmp,
-                                                   float* __restrict__ out) {
+// Column pass by tiles: a workgroup per FILTER_TILE_W x FILTER_TILE_H output
+// tile of one instance (a tile table maps workgroups to instances); the
+// tile's input rows (its rows plus the kernel's reach) are loaded into LDS
+// once, then each thread sums FILTER_TILE_H / 4 outputs of its column tap by
+// tap in order from the first offset; an output costs 1 + (n - 1) / 32 global
+// loads instead of n.
+constexpr int TILE_IN_MAX = FILTER_TILE_H + FILTER_TAPS_MAX - 1;
+__global__ void __launch_bounds__(256) k_cols_conv_tiled(BatchDev bd, const float* __restrict__ taps,
+                                                         const float* __restrict__ tmp,
+                                                         float* __restrict__ out) {
   __shared__ float ty[FILTER_TAPS_MAX];
-  __shared__ int roff[FILTER_TAPS_MAX];
-  const int row = xcd_group(blockIdx.x, gridDim.x);
-  const InstGeo& gi = bd.geo[bd.row_inst[row]];
+  __shared__ float tile[TILE_IN_MAX * FILTER_TILE_W];
+  const int b = xcd_group(blockIdx.x, gridDim.x);
+  const int inst = find_inst(bd.ctile, bd.n_inst, b);
+  const InstGeo& gi = bd.geo[inst];
   if (!filter_direct(taps, gi)) return;
-  const int W = gi.W, H = gi.H, y = row - gi.row0;
+  const int W = gi.W, H = gi.H;
+  const int txn = (W + FILTER_TILE_W - 1) / FILTER_TILE_W;
+  const int local = b - bd.ctile[inst];
+  const int ty0 = (local / txn) * FILTER_TILE_H, tx0 = (local % txn) * FILTER_TILE_W;
   const int* hd = reinterpret_cast<const int*>(taps + gi.kf_off);
   const int nx = hd[1], lo = hd[2], nt = hd[3];
-  int y0 = y + lo;
-  if (y0 < 0) y0 += H;
-  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-    ty[t] = taps[gi.kf_off + 4 + nx + t];
-    int yy = y0 + t;
-    if (yy >= H) yy -= H;
-    roff[t] = yy * W;
+  const int nin = FILTER_TILE_H + nt - 1;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) ty[t] = taps[gi.kf_off + 4 + nx + t];
+  const int c = threadIdx.x & (FILTER_TILE_W - 1), rq = threadIdx.x / FILTER_TILE_W;
+  const int x = tx0 + c;
+  const float* src = tmp + gi.grid_off;
+  int yb = ty0 + lo;  // > -H: lo > -H, ty0 >= 0
+  if (yb < 0) yb += H;
+  // the tile's input rows, 8 loads in flight per thread before their stores
+  constexpr int RS = 256 / FILTER_TILE_W, LB = 8;
+  for (int r0 = rq; r0 < nin; r0 += RS * LB) {
+    float v[LB];
+#pragma unroll
+    for (int k = 0; k < LB; ++k) {
+      const int r = r0 + k * RS;
+      int yy = yb + r;
+      while (yy >= H) yy -= H;
+      v[k] = (r < nin && x < W) ? src[(size_t)yy * W + x] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < LB; ++k)
+      if (r0 + k * RS < nin) tile[(r0 + k * RS) * FILTER_TILE_W + c] = v[k];
   }
   __syncthreads();
-  const float* src = tmp + gi.grid_off;
-  const int B = blockDim.x;
-  for (int xb = threadIdx.x; xb < W; xb += COLS_COLS * B) {
-    float acc[COLS_COLS];
-#pragma unroll
-    for (int c = 0; c < COLS_COLS; ++c) acc[c] = 0.0f;
-    for (int t0 = 0; t0 < nt; t0 += COLS_BATCH) {
-      float v[COLS_COLS][COLS_BATCH];
-#pragma unroll
-      for (int k = 0; k < COLS_BATCH; ++k)
-#pragma unroll
-        for (int c = 0; c < COLS_COLS; ++c) {
-          const int x = xb + c * B;
-          v[c][k] = (t0 + k < nt && x < W) ? src[(size_t)roff[t0 + k] + x] : 0.0f;
-        }
-#pragma unroll
-      for (int k = 0; k < COLS_BATCH; ++k)
-        if (t0 + k < nt) {
-          const float w = ty[t0 + k];
-#pragma unroll
-          for (int c = 0; c < COLS_COLS; ++c) acc[c] = acc[c] + w * v[c][k];
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < COLS_COLS; ++c) {
-      const int x = xb + c * B;
-      if (x < W) out[gi.grid_off + (size_t)y * W + x] = acc[c];
-    }
+  constexpr int ROWS = FILTER_TILE_H / (256 / FILTER_TILE_W);
+  for (int k = 0; k < ROWS; ++k) {
+    const int orow = rq * ROWS + k;
+    float acc = 0.0f;
+    for (int t = 0; t < nt; ++t) acc = acc + ty[t] * tile[(orow + t) * FILTER_TILE_W + c];
+    const int y = ty0 + orow;
+    if (x < W && y < H) out[gi.grid_off + (size_t)y * W + x] = acc;
   }
 }
 
@@ -965,7 +963,7 @@ hipError_t launch_steric_conv(const BatchDev& b, const float* real_in, const Ste
   dep.tw_off = rl.tw_off;
   hipLaunchKernelGGL(k_rows_conv, dim3(b.rows), dim3(tr), fft_rows_lds_bytes(b.max_w), s, b,
                      real_in, dep, taps, tmp, density_out);
-  hipLaunchKernelGGL(k_cols_conv, dim3(b.rows), dim3(256), 0, s, b, taps, tmp, out);
+  hipLaunchKernelGGL(k_cols_conv_tiled, dim3(b.ctiles), dim3(256), 0, s, b, taps, tmp, out);
   return hipGetLastError();
 }
 
