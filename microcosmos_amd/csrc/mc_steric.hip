@@ -483,24 +483,44 @@ __device__ int collect_row_nodes(const DepArgs& a, const InstGeo& gi, int row, i
       c_off[inc.x - 1] = inc.y - nn;
     }
     __syncthreads();
-    for (int f = tid; f < ntot; f += B) {
-      int lo = 0, hi = ccount - 1;  // the last candidate with c_off <= f
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (c_off[mid] <= f) lo = mid; else hi = mid - 1;
+    // CR candidates per thread at a time: their node loads are in flight together
+    constexpr int CR = 4;
+    for (int f0 = tid; f0 < ntot; f0 += CR * B) {
+      int nid[CR];
+#pragma unroll
+      for (int u = 0; u < CR; ++u) {
+        const int f = f0 + u * B;
+        nid[u] = -1;
+        if (f < ntot) {
+          int lo = 0, hi = ccount - 1;  // the last candidate with c_off <= f
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (c_off[mid] <= f) lo = mid; else hi = mid - 1;
+          }
+          const int local = f - c_off[lo], fc = c_first[lo];
+          nid[u] = fc >= 0 ? fc + local : a.L.comp_nodes[~fc + local];
+        }
       }
-      const int local = f - c_off[lo], fc = c_first[lo];
-      const int n = fc >= 0 ? fc + local : a.L.comp_nodes[~fc + local];
-      const int4 cc = *reinterpret_cast<const int4*>(a.cic_cell + 4 * (size_t)n);
-      const float4 ww = *reinterpret_cast<const float4*>(a.cic_w + 4 * (size_t)n);
-      if (cc.z == row || cc.w == row) {
+      int4 cc[CR];
+      float4 ww[CR];
+#pragma unroll
+      for (int u = 0; u < CR; ++u) {
+        const size_t q = 4 * (size_t)(nid[u] < 0 ? 0 : nid[u]);
+        cc[u] = nid[u] < 0 ? make_int4(-1, -1, -1, -1)
+                           : *reinterpret_cast<const int4*>(a.cic_cell + q);
+        ww[u] = nid[u] < 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
+                           : *reinterpret_cast<const float4*>(a.cic_w + q);
+      }
+#pragma unroll
+      for (int u = 0; u < CR; ++u) {
+        if (nid[u] < 0 || !(cc[u].z == row || cc[u].w == row)) continue;
         const int p = atomicAdd(s_cnt, 1);
         if (p < lim) {
-          out[p] = n;
+          out[p] = nid[u];
           if (rx) {
-            const int role = cc.z == row ? 0 : 1;
-            rx[p] = make_int2(cc.x | (role << 30), cc.y);
-            rw[p] = role ? make_float2(ww.z, ww.w) : make_float2(ww.x, ww.y);
+            const int role = cc[u].z == row ? 0 : 1;
+            rx[p] = make_int2(cc[u].x | (role << 30), cc[u].y);
+            rw[p] = role ? make_float2(ww[u].z, ww[u].w) : make_float2(ww[u].x, ww[u].y);
           }
         }
       }
