@@ -981,7 +981,9 @@ hipError_t launch_steric_conv(const BatchDev& b, const float* real_in, const Ste
   dep.cap = rl.cap;
   dep.b_off = rl.b_off;
   dep.tw_off = rl.tw_off;
-  hipLaunchKernelGGL(k_rows_conv, dim3(b.rows), dim3(tr), fft_rows_lds_bytes(b.max_w), s, b,
+  // the FFT pass's layout without its twiddles: the taps need FILTER_TAPS_MAX floats
+  const size_t lds = (size_t)(rl.tw_off + (FILTER_TAPS_MAX + 1) / 2) * sizeof(float2);
+  hipLaunchKernelGGL(k_rows_conv, dim3(b.rows), dim3(tr), lds, s, b,
                      real_in, dep, taps, tmp, density_out);
   hipLaunchKernelGGL(k_cols_conv_tiled, dim3(b.ctiles), dim3(256), 0, s, b, taps, tmp, out);
   return hipGetLastError();
