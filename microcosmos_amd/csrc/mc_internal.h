@@ -10,7 +10,7 @@ namespace mc {
 constexpr int IB_KMAX = 6;                 // largest supported ibm_kernel_size
 constexpr int SORT_LDS_CAP = 2048;         // bitonic sort capacity in LDS (ids)
 constexpr int FFT_MAX_RADICES = 16;        // radix stages of one FFT length
-constexpr int IB_WIDE_MIN = 512;           // clusters with more nodes take the wide solve
+constexpr int IB_WIDE_MIN = 768;           // clusters with more nodes take the wide solve
 constexpr int IB_CLUSTER_MAX = 1024;       // largest cluster the per-cluster solve can take
 constexpr int IB_WIDE_WG = 256;            // workgroups of the wide solve launch (at most)
 constexpr int IB_WIDE_NODES_PER_WG = 192;  // wide nodes per participating workgroup

@@ -197,7 +197,7 @@ def test_ib_lbm_crowded_cluster(dev):
 
 @pytest.mark.parametrize("K", [4, 6])
 def test_ib_lbm_wide_cluster(dev, K):
-    # 1920 nodes on 64^2 chain into one cluster of more than 512 nodes: it
+    # 1920 nodes on 64^2 chain into one cluster of more than 768 nodes: it
     # is solved by the wide workgroups with grid barriers
     _say(f"IB+LBM single call, one 1920-node cluster (wide solve), K={K}")
     _ib_case(dev, W.small(grid=(64, 64), n_fil=48, n_nodes=40, seed=3), K)
