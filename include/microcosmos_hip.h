@@ -232,6 +232,13 @@ mc_status mc_rod_adjoint_backward(mc_plan* plan, const mc_params* params,
                                   float* rest_angle_bar, float* position0_bar,
                                   float* velocity0_bar, float* theta0_bar, void* stream);
 
+/* The direct filter's taps along an axis of n cells for sigma (host only, no
+ * device work; no counterpart in the reference -- it exposes the plan's
+ * tabulation of utils.py:41-47's filter for checking): returns the tap count
+ * and writes the first offset to *lo and up to `capacity` float32 taps, 0
+ * when the axis needs the FFT (more than 129 taps), -1 on a bad argument. */
+int32_t mc_filter_taps(int32_t n, double sigma, int32_t* lo, float* taps, int32_t capacity);
+
 /* Folding losses of the filament-folding experiment between particles (n, 2)
  * and targets (m, 2), float32 device arrays, and their gradient with respect
  * to the particles (grad (n, 2), may be NULL); loss is one device float:

@@ -1706,6 +1706,20 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
   return MC_OK;
 }
 
+int32_t mc_filter_taps(int32_t n, double sigma, int32_t* lo, float* taps, int32_t capacity) {
+  g_err.clear();
+  if (n < 1 || !(sigma > 0.0) || !lo || (!taps && capacity > 0)) {
+    fail(MC_ERR_INVALID, "mc_filter_taps: bad argument");
+    return -1;
+  }
+  int l = 0;
+  std::vector<float> t;
+  if (!gauss_taps(n, sigma, &l, &t)) return 0;
+  *lo = l;
+  for (int i = 0; i < (int)t.size() && i < capacity; ++i) taps[i] = t[i];
+  return (int32_t)t.size();
+}
+
 size_t mc_point_loss_workspace(int64_t n, int64_t m) {
   return n < 0 || m < 0 ? 0 : point_loss_workspace(n, m);
 }

@@ -507,3 +507,23 @@ def test_plain_case_fp32_tracks_fp64():
     assert np.linalg.norm(a.f_grid - b.f_grid) / np.linalg.norm(b.f_grid) < 1e-6
     du = np.linalg.norm(a.fluid_velocity - b.fluid_velocity) / np.linalg.norm(b.fluid_velocity)
     assert du < 1e-3
+
+
+@pytest.mark.parametrize("n,sigma", [(1024, 2.0), (256, 2.0), (48, 2.0), (32, 1.5), (10, 1.0),
+                                     (96, 1.5), (262, 2.0), (256, 5.0), (512, 1.5), (4096, 2.0)])
+def test_library_filter_taps_equal_the_restatement(n, sigma):
+    """The plan's host tabulation of the direct filter (mc_filter_taps, no
+    device work) gives the restatement's offsets and float32 taps bit for bit,
+    and the same refusal where the FFT is needed."""
+    import ctypes
+    from microcosmos_amd import _lib
+    lib = _lib.load()
+    lo = ctypes.c_int32()
+    buf = (ctypes.c_float * 256)()
+    cnt = lib.mc_filter_taps(n, sigma, ctypes.byref(lo), buf, 256)
+    ref = R.gauss_taps(n, sigma)
+    if ref is None:
+        assert cnt == 0
+        return
+    assert cnt == ref[1].size and lo.value == ref[0]
+    np.testing.assert_array_equal(np.frombuffer(buf, np.float32)[:cnt], ref[1].astype(np.float32))
