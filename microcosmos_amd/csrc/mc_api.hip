@@ -184,6 +184,9 @@ struct mc_plan {
   size_t adj_tape_cap = 0;
   float* adj_work = nullptr;
   float* adj_steric = nullptr;  // steric term: force (2N), pb1 (2N), V / Vbar / Dbar grids
+  float* adj_stape = nullptr;   // steric term, per step: field, force, wrapped prediction
+  size_t adj_stape_cap = 0;
+  bool adj_stored = false;      // the last forward kept the steric tape
   bool adj_ok = false;
   int64_t adj_steps = 0;
   int adj_every = 0;
@@ -806,9 +809,10 @@ hipError_t run_filter(mc_plan* p, const float* real_in, const StericDep* dep, fl
 // Steric pipeline on `pos` (+ optional velocity prediction): fills p->pred_w,
 // writes the filtered field to steric_out.
 mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float* vel,
-                     float* steric_out, float* density_out, hipStream_t s) {
-  HIPCHK(launch_predict(c, p->bd, p->L, pos, vel, p->pred_w, p->cic_cell, p->cic_w, p->comp_rows,
-                        p->dep_top, s));
+                     float* steric_out, float* density_out, hipStream_t s,
+                     float* pred_out = nullptr) {
+  HIPCHK(launch_predict(c, p->bd, p->L, pos, vel, pred_out ? pred_out : p->pred_w, p->cic_cell,
+                        p->cic_w, p->comp_rows, p->dep_top, s));
   if (!c.enable_steric) return MC_OK;
   StericDep d;
   d.L = p->L;
@@ -1544,6 +1548,10 @@ mc_status check_rod_adjoint(const mc_params* q) {
   return MC_OK;
 }
 
+// Largest steric tape the adjoint keeps (field, force and prediction of every
+// step); longer or larger rollouts recompute them in the reverse pass.
+constexpr size_t ADJ_STERIC_TAPE_MAX = (size_t)2 << 30;
+
 // Steric-term scratch of the adjoint: per-node force and pb1, three grids.
 mc_status ensure_adj_steric(mc_plan* p) {
   if (!p->adj_steric)
@@ -1599,15 +1607,33 @@ mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state*
   a.kb = ed->bending_stiffness;
   const size_t lds_f = rod_segment_lds_bytes(q->cycles_per_step, p->comp_max_nn, p->comp_max_ne,
                                              p->comp_max_nb, false);
+  // steric tape slot per step: field (grid), force (2N), wrapped prediction (2N)
+  const size_t slot = (size_t)p->grid_cells + 4 * (size_t)p->N;
+  size_t tape_max = ADJ_STERIC_TAPE_MAX;
+  if (const char* env = getenv("MC_ADJ_STERIC_TAPE_MAX"))  // bytes; 0 always recomputes
+    tape_max = (size_t)strtoull(env, nullptr, 10);
+  const bool store = steric && (size_t)num_steps * slot * sizeof(float) <= tape_max;
   if (steric) {
     MCCHK(ensure_geo(p, q));
     MCCHK(ensure_kf(p, q->steric_sigma, s));
     MCCHK(ensure_adj_steric(p));
     a.fst = p->adj_steric;
+    if (store && p->adj_stape_cap < (size_t)num_steps * slot) {
+      MCCHK(p->alloc(&p->adj_stape, (size_t)num_steps * slot, "adjoint steric tape"));
+      p->adj_stape_cap = (size_t)num_steps * slot;
+    }
   }
   for (int64_t seg = 0; seg < nck; ++seg) {
     const int len = (int)std::min<int64_t>(checkpoint_every, num_steps - seg * checkpoint_every);
-    if (steric) {
+    if (store) {
+      // the step's field, force and prediction, kept for the reverse pass
+      float* vt = p->adj_stape + (size_t)seg * slot;
+      float* ft = vt + p->grid_cells;
+      float* pt = ft + 2 * (size_t)p->N;
+      MCCHK(run_steric(p, a.c, st->position, st->velocity, vt, nullptr, s, pt));
+      HIPCHK(launch_steric_nodes(a.c, p->bd, p->N, vt, pt, ft, s));
+      a.fst = ft;
+    } else if (steric) {
       // the step's prediction, field and per-node force, as the coupled step
       MCCHK(run_steric(p, a.c, st->position, st->velocity, st->steric, nullptr, s));
       HIPCHK(launch_steric_nodes(a.c, p->bd, p->N, st->steric, p->pred_w, p->adj_steric, s));
@@ -1615,6 +1641,10 @@ mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state*
     HIPCHK(launch_rod_segment(a, false, len, p->adj_ckpt + (size_t)seg * S, st->debug_vector,
                               lds_f, s));
   }
+  if (store)  // the state's field is the last step's, as after the coupled step
+    HIPCHK(hipMemcpyAsync(st->steric, p->adj_stape + (size_t)(nck - 1) * slot,
+                          (size_t)p->grid_cells * sizeof(float), hipMemcpyDeviceToDevice, s));
+  p->adj_stored = store;
   p->adj_ok = true;
   p->adj_steps = num_steps;
   p->adj_every = checkpoint_every;
@@ -1687,16 +1717,23 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
   for (int64_t seg = nck - 1; seg >= 0; --seg) {
     const int len = (int)std::min<int64_t>(every, T - seg * every);
     const float* ck = p->adj_ckpt + (size_t)seg * S;
-    if (steric) {
+    const float *vt = vg, *pt = p->pred_w;
+    if (steric && p->adj_stored) {
+      // the step's field, force and prediction from the forward's tape
+      const size_t slot = (size_t)p->grid_cells + 4 * (size_t)N;
+      vt = p->adj_stape + (size_t)seg * slot;
+      a.fst = vt + p->grid_cells;
+      pt = a.fst + 2 * (size_t)N;
+    } else if (steric) {
       // recompute the step's prediction, field and force from its state
       MCCHK(run_steric(p, a.c, ck, ck + 2 * (size_t)N, vg, nullptr, s));
       HIPCHK(launch_steric_nodes(a.c, p->bd, N, vg, p->pred_w, fst, s));
     }
     HIPCHK(launch_rod_segment(a, true, len, const_cast<float*>(ck), p->adj_tape, lds, s));
     if (steric) {
-      HIPCHK(launch_steric_bar(a.c, p->bd, p->pred_w, pb1, vbar, s));
+      HIPCHK(launch_steric_bar(a.c, p->bd, pt, pb1, vbar, s));
       HIPCHK(run_filter(p, vbar, nullptr, nullptr, dbar, s));
-      HIPCHK(launch_steric_adj_nodes(a.c, p->bd, N, p->pred_w, vg, dbar, pb1, xb, vb, s));
+      HIPCHK(launch_steric_adj_nodes(a.c, p->bd, N, pt, vt, dbar, pb1, xb, vb, s));
     }
   }
   if (position0_bar) HIPCHK(hipMemcpyAsync(position0_bar, xb, n2, hipMemcpyDeviceToDevice, s));

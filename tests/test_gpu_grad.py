@@ -267,3 +267,30 @@ def test_batched_adjoint_equals_each_alone(dev, physics):
         torch.cuda.synchronize()
         assert float(losses[i].item()) == float(lo.item()), i
         np.testing.assert_array_equal(grads[i].cpu().numpy(), go.cpu().numpy(), err_msg=f"inst {i}")
+
+
+def test_steric_tape_and_recompute_agree(dev, monkeypatch):
+    """With the steric term the forward keeps each step's field, force and
+    prediction for the reverse pass (below a size limit) or the reverse pass
+    recomputes them (MC_ADJ_STERIC_TAPE_MAX=0): the gradients are
+    bit-identical, and so is the final state."""
+    _say("steric tape vs recompute (folding physics, 30 steps)")
+    from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID
+    from microcosmos_amd.grad import RodAdjoint
+    wl, target = W.folding(steps=30)
+    cfg = replace(COSSERAT_SCHEME_NO_FLUID, dt=W.FOLDING_DT, **W.FOLDING_COSSERAT)
+    tg = torch.as_tensor(target, device=dev)
+    out = []
+    for limit in (None, "0"):
+        if limit is None:
+            monkeypatch.delenv("MC_ADJ_STERIC_TAPE_MAX", raising=False)
+        else:
+            monkeypatch.setenv("MC_ADJ_STERIC_TAPE_MAX", limit)
+        nodes, edges, fields = device_state(wl, dev)
+        ad = RodAdjoint(nodes, edges, fields, cfg, W.FOLDING_DT)
+        x = ad.forward(30, 1)
+        g = ad.backward(2.0 * (x - tg) / x.shape[0])[0]
+        torch.cuda.synchronize()
+        out.append((g.cpu().numpy(), x.cpu().numpy(), ad.steric.cpu().numpy()))
+    for a, b, name in zip(out[0], out[1], ("gradient", "position", "steric")):
+        np.testing.assert_array_equal(a, b, err_msg=name)
