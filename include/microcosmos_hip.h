@@ -216,8 +216,11 @@ mc_status mc_batch_advance(mc_plan* plan, const mc_params* params, const mc_stat
  * adjoint of each step backwards).  With it the filaments couple through the
  * field every step: every step's state is kept (checkpoint_every is taken as
  * 1; memory (4 N + E) floats per step), the forward runs the coupled step's
- * steric pipeline each step (bit-identical to mc_rollout, field included),
- * and each reverse step recomputes its field before the adjoint of the
+ * steric pipeline each step (bit-identical to mc_rollout, field included) and
+ * keeps each step's field, force and prediction (grid cells + 4 N floats per
+ * step, up to 2 GiB -- the environment variable MC_ADJ_STERIC_TAPE_MAX sets
+ * the limit in bytes; beyond it the reverse pass recomputes them,
+ * bit-identically), and each reverse step runs the adjoint of the
  * constraint, the steric force (deposit, Gaussian filter, samples, window)
  * and the prediction.  Every filament must fit the LDS budget
  * (MC_ERR_UNSUPPORTED otherwise).
