@@ -546,16 +546,24 @@ __global__ void __launch_bounds__(BAR_THREADS) k_steric_bar_rows(Consts c, Batch
       if (!more || ltop > BAR_LIST - BAR_THREADS) {
         const int n = ltop;
         if (x < W)
-          for (int k = 0; k < n; ++k) {
-            const float4 q = lst[k];
-            // the samples' cells lie in columns floor(px) - 1 .. floor(px) + 2
-            int d = x - f2i(floorf(q.x)) + 1;
-            d = d < 0 ? d + W : (d >= W ? d - W : d);
-            if (d > 3) continue;
-            acc = bar_sample(acc, -q.z, q.y, q.x + 0.5f, H, W, y, x);
-            acc = bar_sample(acc, q.z, q.y, q.x - 0.5f, H, W, y, x);
-            acc = bar_sample(acc, -q.w, q.y + 0.5f, q.x, H, W, y, x);
-            acc = bar_sample(acc, q.w, q.y - 0.5f, q.x, H, W, y, x);
+          for (int k0 = 0; k0 < n; k0 += 8) {
+            // eight staged nodes read ahead of their use (LDS latency overlaps)
+            float4 qv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) qv[u] = lst[min(k0 + u, n - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              if (k0 + u >= n) break;
+              const float4 q = qv[u];
+              // the samples' cells lie in columns floor(px) - 1 .. floor(px) + 2
+              int d = x - f2i(floorf(q.x)) + 1;
+              d = d < 0 ? d + W : (d >= W ? d - W : d);
+              if (d > 3) continue;
+              acc = bar_sample(acc, -q.z, q.y, q.x + 0.5f, H, W, y, x);
+              acc = bar_sample(acc, q.z, q.y, q.x - 0.5f, H, W, y, x);
+              acc = bar_sample(acc, -q.w, q.y + 0.5f, q.x, H, W, y, x);
+              acc = bar_sample(acc, q.w, q.y - 0.5f, q.x, H, W, y, x);
+            }
           }
         __syncthreads();
         if (tid == 0) ltop = 0;
