@@ -234,6 +234,25 @@ mc_status mc_rod_adjoint_backward(mc_plan* plan, const mc_params* params,
                                   const float* position_bar, const float* velocity_bar,
                                   float* rest_angle_bar, float* position0_bar,
                                   float* velocity0_bar, float* theta0_bar, void* stream);
+/* The recorded state after `step` steps (a multiple of the checkpoint
+ * interval below num_steps): position and velocity (N, 2), theta (E); each
+ * output may be NULL.  The frames of a multi-frame loss are read here. */
+mc_status mc_rod_adjoint_state(mc_plan* plan, int64_t step, float* position_out,
+                               float* velocity_out, float* theta_out, void* stream);
+/* The same with position cotangents on several frames -- a loss over the
+ * positions after frame_steps[f] steps, as the folding experiment's shape and
+ * stability terms over evenly spaced frames (experiment.py:166-330):
+ * frame_position_bars holds num_frames (N, 2) arrays (device); every frame
+ * step lies in 1 .. num_steps and is a multiple of the recorded checkpoint
+ * interval or num_steps itself (MC_ERR_BAD_SHAPE otherwise).  velocity_bar is
+ * the cotangent of the final velocity (may be NULL). */
+mc_status mc_rod_adjoint_backward_frames(mc_plan* plan, const mc_params* params,
+                                         const mc_edge_params* edges, double dt,
+                                         int32_t num_frames, const int64_t* frame_steps,
+                                         const float* frame_position_bars,
+                                         const float* velocity_bar, float* rest_angle_bar,
+                                         float* position0_bar, float* velocity0_bar,
+                                         float* theta0_bar, void* stream);
 
 /* The direct filter's taps along an axis of n cells for sigma (host only, no
  * device work; no counterpart in the reference -- it exposes the plan's

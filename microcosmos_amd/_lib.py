@@ -91,6 +91,12 @@ SIGNATURES = {
     "mc_rod_adjoint_backward": (c_int32, [c_void_p, POINTER(McParams), POINTER(McEdgeParams),
                                           c_double, c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p]),
+    "mc_rod_adjoint_state": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                       c_void_p]),
+    "mc_rod_adjoint_backward_frames": (c_int32, [c_void_p, POINTER(McParams),
+                                                 POINTER(McEdgeParams), c_double, c_int32,
+                                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                 c_void_p, c_void_p, c_void_p]),
     "mc_center_of_mass": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32,
                                     c_void_p, c_void_p]),
     "mc_track_enable": (c_int32, [c_void_p, c_int32, c_void_p, c_int32]),

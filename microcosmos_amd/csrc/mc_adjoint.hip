@@ -653,6 +653,19 @@ __global__ void k_steric_adj_nodes(Consts c, BatchDev bd, int N, const float* __
 
 }  // namespace
 
+namespace {
+__global__ void k_add(float* __restrict__ y, const float* __restrict__ x, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = y[i] + x[i];
+}
+}  // namespace
+
+hipError_t launch_add(float* y, const float* x, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, y, x, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_steric_bar(const Consts& c, const BatchDev& b, const float* pred_w,
                              const float* pb1, float* vbar, hipStream_t s) {
   if (b.rows == 0) return hipSuccess;

@@ -1654,19 +1654,32 @@ mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state*
   return MC_OK;
 }
 
-mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params* ed,
-                                  double dt, const float* position_bar, const float* velocity_bar,
-                                  float* rest_angle_bar, float* position0_bar,
-                                  float* velocity0_bar, float* theta0_bar, void* stream) {
-  g_err.clear();
+}  // extern "C"
+
+namespace {
+
+// The reverse pass with position cotangents at frames (step counts, each at
+// a checkpoint boundary or the last step): a frame's cotangent joins the
+// position adjoint when the reverse pass reaches the state after that step.
+mc_status adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params* ed, double dt,
+                           int nframes, const int64_t* frame_steps, const float* frame_bars,
+                           const float* velocity_bar, float* rest_angle_bar, float* position0_bar,
+                           float* velocity0_bar, float* theta0_bar, void* stream) {
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   MCCHK(check_rod_adjoint(q));
   if (!p->adj_ok) return fail(MC_ERR_INVALID, "no rollout recorded by mc_rod_adjoint_forward");
   if (!ed || memcmp(q, &p->adj_q, sizeof(mc_params)) != 0 || dt != p->adj_dt ||
       memcmp(ed, &p->adj_ed, sizeof(mc_edge_params)) != 0)
     return fail(MC_ERR_INVALID, "params, dt and edge arrays must be those of the recorded rollout");
-  if (!position_bar || (p->B > 0 && !rest_angle_bar))
-    return fail(MC_ERR_INVALID, "position_bar / rest_angle_bar is NULL");
+  if (nframes < 1 || !frame_steps || !frame_bars || (p->B > 0 && !rest_angle_bar))
+    return fail(MC_ERR_INVALID, "frames / position cotangents / rest_angle_bar missing");
+  for (int f = 0; f < nframes; ++f) {
+    const int64_t t = frame_steps[f];
+    if (t < 1 || t > p->adj_steps || (t != p->adj_steps && t % p->adj_every != 0))
+      return fail(MC_ERR_BAD_SHAPE, "frame step %lld: must be in 1..%lld and a multiple of the "
+                  "checkpoint interval %d (or the last step)", (long long)t,
+                  (long long)p->adj_steps, p->adj_every);
+  }
   const size_t lds = rod_segment_lds_bytes(q->cycles_per_step, p->comp_max_nn, p->comp_max_ne,
                                            p->comp_max_nb, true);
   hipStream_t s = (hipStream_t)stream;
@@ -1684,7 +1697,7 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
   float* vb = xb + 2 * (size_t)N;
   float* thb = vb + 2 * (size_t)N;
   const size_t n2 = 2 * (size_t)N * sizeof(float);
-  HIPCHK(hipMemcpyAsync(xb, position_bar, n2, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemsetAsync(xb, 0, n2, s));
   if (velocity_bar) HIPCHK(hipMemcpyAsync(vb, velocity_bar, n2, hipMemcpyDeviceToDevice, s));
   else HIPCHK(hipMemsetAsync(vb, 0, n2, s));
   if (E) HIPCHK(hipMemsetAsync(thb, 0, (size_t)E * sizeof(float), s));
@@ -1716,6 +1729,10 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
   }
   for (int64_t seg = nck - 1; seg >= 0; --seg) {
     const int len = (int)std::min<int64_t>(every, T - seg * every);
+    // the cotangents of the frames at the state this segment ends in
+    for (int f = 0; f < nframes; ++f)
+      if (frame_steps[f] == seg * every + len)
+        HIPCHK(launch_add(xb, frame_bars + (size_t)f * 2 * N, 2 * (size_t)N, s));
     const float* ck = p->adj_ckpt + (size_t)seg * S;
     const float *vt = vg, *pt = p->pred_w;
     if (steric && p->adj_stored) {
@@ -1742,6 +1759,55 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
     HIPCHK(hipMemcpyAsync(theta0_bar, thb, (size_t)E * sizeof(float), hipMemcpyDeviceToDevice, s));
   return MC_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params* ed,
+                                  double dt, const float* position_bar, const float* velocity_bar,
+                                  float* rest_angle_bar, float* position0_bar,
+                                  float* velocity0_bar, float* theta0_bar, void* stream) {
+  g_err.clear();
+  if (!position_bar) return fail(MC_ERR_INVALID, "position_bar is NULL");
+  const int64_t last = p ? p->adj_steps : 0;
+  return adjoint_backward(p, q, ed, dt, 1, &last, position_bar, velocity_bar, rest_angle_bar,
+                          position0_bar, velocity0_bar, theta0_bar, stream);
+}
+
+mc_status mc_rod_adjoint_state(mc_plan* p, int64_t step, float* position_out,
+                               float* velocity_out, float* theta_out, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  if (!p->adj_ok) return fail(MC_ERR_INVALID, "no rollout recorded by mc_rod_adjoint_forward");
+  if (step < 0 || step >= p->adj_steps || step % p->adj_every != 0)
+    return fail(MC_ERR_BAD_SHAPE, "step %lld: the recorded states are at multiples of %d below %lld",
+                (long long)step, p->adj_every, (long long)p->adj_steps);
+  hipStream_t s = (hipStream_t)stream;
+  const size_t S = 4 * (size_t)p->N + (size_t)p->E, n2 = 2 * (size_t)p->N * sizeof(float);
+  const float* ck = p->adj_ckpt + (size_t)(step / p->adj_every) * S;
+  if (position_out) HIPCHK(hipMemcpyAsync(position_out, ck, n2, hipMemcpyDeviceToDevice, s));
+  if (velocity_out)
+    HIPCHK(hipMemcpyAsync(velocity_out, ck + 2 * (size_t)p->N, n2, hipMemcpyDeviceToDevice, s));
+  if (theta_out && p->E)
+    HIPCHK(hipMemcpyAsync(theta_out, ck + 4 * (size_t)p->N, (size_t)p->E * sizeof(float),
+                          hipMemcpyDeviceToDevice, s));
+  return MC_OK;
+}
+
+mc_status mc_rod_adjoint_backward_frames(mc_plan* p, const mc_params* q,
+                                         const mc_edge_params* ed, double dt, int32_t num_frames,
+                                         const int64_t* frame_steps,
+                                         const float* frame_position_bars,
+                                         const float* velocity_bar, float* rest_angle_bar,
+                                         float* position0_bar, float* velocity0_bar,
+                                         float* theta0_bar, void* stream) {
+  g_err.clear();
+  return adjoint_backward(p, q, ed, dt, num_frames, frame_steps, frame_position_bars,
+                          velocity_bar, rest_angle_bar, position0_bar, velocity0_bar, theta0_bar,
+                          stream);
+}
+
 
 int32_t mc_filter_taps(int32_t n, double sigma, int32_t* lo, float* taps, int32_t capacity) {
   g_err.clear();

@@ -225,6 +225,8 @@ size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse);
 // it (the Gaussian filter is its own adjoint) into dbar; then
 // launch_steric_adj_nodes, with the step's field V, writes xb = d loss / d x
 // and vb = d loss / d v (vb holds d loss / d v_new on entry).
+// y += x over n floats.
+hipError_t launch_add(float* y, const float* x, size_t n, hipStream_t s);
 hipError_t launch_steric_bar(const Consts& c, const BatchDev& b, const float* pred_w,
                              const float* pb1, float* vbar, hipStream_t s);
 hipError_t launch_steric_adj_nodes(const Consts& c, const BatchDev& b, int N, const float* pred_w,

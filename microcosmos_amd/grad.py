@@ -97,15 +97,55 @@ class RodAdjoint:
                        ptr(self.theta), ptr(self.steric), ptr(self.fluid_velocity),
                        ptr(self.f_grid))
 
-    def forward(self, num_steps: int, checkpoint_every: int = 50) -> torch.Tensor:
+    def forward(self, num_steps: int, checkpoint_every: int = 50, frames=None) -> torch.Tensor:
+        """Runs the rollout; ``frames`` (step counts) are the intermediate
+        states a multi-frame loss will score: the checkpoint interval is
+        lowered to a common divisor so each frame ends a segment."""
         lib = _lib.load()
         st = self._state()
+        every = int(checkpoint_every)
+        for t in frames or ():
+            if not 1 <= int(t) <= int(num_steps):
+                raise ValueError(f"frame {t} outside 1..{num_steps}")
+            if int(t) != int(num_steps):
+                every = math.gcd(every, int(t))
         check(lib.mc_rod_adjoint_forward(self.plan.handle, ctypes.byref(self.params),
                                          ctypes.byref(st), ctypes.byref(self.ep), self.dt,
-                                         int(num_steps), int(checkpoint_every),
-                                         stream_handle(self.dev)))
+                                         int(num_steps), every, stream_handle(self.dev)))
         self.steps = int(num_steps)
         return self.position
+
+    def frame_position(self, step: int) -> torch.Tensor:
+        """Positions (N, 2) after ``step`` steps of the recorded rollout (a
+        frame given to ``forward``, or the last step)."""
+        if int(step) == self.steps:
+            return self.position
+        lib = _lib.load()
+        out = torch.empty_like(self.position)
+        check(lib.mc_rod_adjoint_state(self.plan.handle, int(step), ptr(out), None, None,
+                                       stream_handle(self.dev)))
+        return out
+
+    def backward_frames(self, frame_bars: dict, velocity_bar: torch.Tensor = None):
+        """Gradient of a loss over several frames: ``frame_bars`` maps a step
+        count (given to ``forward`` as a frame, or the last step) to d loss /
+        d positions after that many steps.  Returns (rest_angle_grad,
+        position0_grad, velocity0_grad, theta0_grad)."""
+        lib = _lib.load()
+        d = self.dev
+        steps = np.ascontiguousarray(sorted(frame_bars), dtype=np.int64)
+        bars = torch.stack([as_f32(frame_bars[int(t)], d).reshape(-1, 2) for t in steps]).contiguous()
+        vb = as_f32(velocity_bar, d) if velocity_bar is not None else None
+        n = self.position.shape[0]
+        rg = torch.empty(self.num_hinges, dtype=torch.float32, device=d)
+        g_x = torch.empty((n, 2), dtype=torch.float32, device=d)
+        g_v = torch.empty((n, 2), dtype=torch.float32, device=d)
+        g_t = torch.empty(self.num_edges, dtype=torch.float32, device=d)
+        check(lib.mc_rod_adjoint_backward_frames(
+            self.plan.handle, ctypes.byref(self.params), ctypes.byref(self.ep), self.dt,
+            int(steps.shape[0]), steps.ctypes.data, ptr(bars), ptr(vb), ptr(rg), ptr(g_x),
+            ptr(g_v), ptr(g_t), stream_handle(d)))
+        return rg, g_x, g_v, g_t
 
     def backward(self, position_bar: torch.Tensor, velocity_bar: torch.Tensor = None):
         """Returns (rest_angle_grad (B,), position0_grad (N, 2),

@@ -312,6 +312,47 @@ def rollout_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, target,
     return loss, rhob
 
 
+def rollout_frames_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, frame_losses,
+                        dtype=np.float64):
+    """Multi-frame objective: ``frame_losses`` maps a step count t (1 ..
+    num_steps) to ``loss_fn(position after t steps) -> (loss, d loss / d
+    position)``; returns (sum of the frame losses, its gradient with respect
+    to the rest angles).  The reverse pass adds each frame's cotangent to the
+    position adjoint when it reaches that frame."""
+    states = [s0.astype(dtype)]
+    for _ in range(num_steps):
+        states.append(R.step(states[-1], dt, p))
+    total = 0.0
+    bars = {}
+    for t, fn in frame_losses.items():
+        l, g = fn(np.asarray(states[t].position, np.float64))
+        total += l
+        bars[t] = g
+    xb = np.zeros((s0.position.shape[0], 2))
+    vb = np.zeros_like(xb)
+    thb = np.zeros(s0.theta.shape[0])
+    rhob = np.zeros(s0.bending_pairs.shape[0])
+    for t in reversed(range(num_steps)):
+        if t + 1 in bars:
+            xb = xb + bars[t + 1]
+        xb, vb, thb, rb = step_vjp(states[t], dt, p, xb, vb, thb)
+        rhob = rhob + rb
+    return total, rhob
+
+
+def rollout_frames_loss(s0: R.State, dt: float, p: R.Params, num_steps: int, frame_losses,
+                        rest=None):
+    s = s0.astype(np.float64)
+    if rest is not None:
+        s = replace(s, bending_rest_angles=np.asarray(rest, np.float64))
+    total = 0.0
+    for t in range(1, num_steps + 1):
+        s = R.step(s, dt, p)
+        if t in frame_losses:
+            total += frame_losses[t](np.asarray(s.position, np.float64))[0]
+    return total
+
+
 def rollout_loss(s0: R.State, dt: float, p: R.Params, num_steps: int, target, rest=None):
     s = s0.astype(np.float64)
     if rest is not None:

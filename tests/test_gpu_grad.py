@@ -294,3 +294,47 @@ def test_steric_tape_and_recompute_agree(dev, monkeypatch):
         out.append((g.cpu().numpy(), x.cpu().numpy(), ad.steric.cpu().numpy()))
     for a, b, name in zip(out[0], out[1], ("gradient", "position", "steric")):
         np.testing.assert_array_equal(a, b, err_msg=name)
+
+
+@pytest.mark.parametrize("physics", ["pbd", "folding"])
+def test_multi_frame_gradient_matches_restatement(dev, physics):
+    """A loss over frames after 10, 20 and 30 steps (each against its own
+    target, as the folding experiment scores evenly spaced frames): the
+    device's frame positions equal the restatement's float32 trajectory and
+    its gradient matches the restatement's multi-frame adjoint along it."""
+    _say(f"multi-frame gradient ({physics}): frames 10, 20, 30")
+    from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID, PBD_SCHEME_NO_FLUID
+    from microcosmos_amd.grad import RodAdjoint, min_image
+    wl, target = W.folding(steps=30)
+    if physics == "pbd":
+        cfg = replace(PBD_SCHEME_NO_FLUID, enable_steric=False)
+        p = replace(R.PBD_SCHEME_NO_FLUID, enable_steric=False)
+        dt = wl.dt
+    else:
+        cfg = replace(COSSERAT_SCHEME_NO_FLUID, dt=W.FOLDING_DT, **W.FOLDING_COSSERAT)
+        p = replace(R.COSSERAT_SCHEME_NO_FLUID, config_dt=W.FOLDING_DT, **W.FOLDING_COSSERAT)
+        dt = W.FOLDING_DT
+    rng = np.random.default_rng(5)
+    tgts = {t: (target + rng.normal(0.0, 0.5, target.shape)).astype(np.float32) for t in (10, 20, 30)}
+    nodes, edges, fields = device_state(wl, dev)
+    ad = RodAdjoint(nodes, edges, fields, cfg, dt)
+    ad.forward(30, 25, frames=[10, 20])
+    bars, xs = {}, {}
+    for t, tg in tgts.items():
+        x = ad.frame_position(t)
+        d = min_image(tuple(wl.grid_shape), x, torch.as_tensor(tg, device=dev))
+        bars[t] = 2.0 * d / x.shape[0]
+        xs[t] = x.cpu().numpy()
+    g = ad.backward_frames(bars)[0].cpu().numpy().astype(np.float64)
+    torch.cuda.synchronize()
+    s = oracle_state(wl, np.float64)
+    st = s.astype(np.float32)
+    for t in range(1, 31):
+        st = R.step(st, dt, p)
+        if t in xs:
+            np.testing.assert_array_equal(xs[t], st.position, err_msg=f"frame {t}")
+    frames = {t: (lambda pos, tg=tg: A.msd_loss(pos, tg, s.grid_shape)) for t, tg in tgts.items()}
+    lo, go = A.rollout_frames_grad(s, dt, p, 30, frames, dtype=np.float32)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    _say(f"rel-L2 {rel:.2e}")
+    assert rel <= 1e-4

@@ -129,3 +129,23 @@ def test_folding_configuration_gradient_matches_fd():
     fm = A.rollout_loss(s, W.FOLDING_DT, p, 30, target, rest=s.bending_rest_angles - h * u)
     fd = (fp - fm) / (2 * h)
     assert abs(g @ u - fd) <= 1e-5 * abs(fd), (g @ u, fd)
+
+
+@pytest.mark.parametrize("solver", ["pbd", "cosserat"])
+def test_multi_frame_gradient_matches_fd(solver):
+    """A loss over several frames (positions after 2, 4 and 6 steps, each
+    against its own target) -- the shape of the folding experiment's
+    objective, which scores evenly spaced frames: the adjoint with frame
+    cotangents against central differences."""
+    s, p, _ = _case(solver=solver, steric=True)
+    rng = np.random.default_rng(21)
+    targets = {t: s.position + rng.normal(0.0, 1.0, s.position.shape) for t in (2, 4, 6)}
+    frames = {t: (lambda pos, tg=tg: A.msd_loss(pos, tg, s.grid_shape)) for t, tg in targets.items()}
+    loss, g = A.rollout_frames_grad(s, 1.0, p, 6, frames)
+    fd = _fd(lambda r: A.rollout_frames_loss(s, 1.0, p, 6, frames, rest=r), s.bending_rest_angles,
+             1e-6)
+    assert np.max(np.abs(g - fd)) <= 1e-6 * max(1.0, np.max(np.abs(fd))), (g, fd)
+    # the single-frame case is the final-state gradient
+    l1, g1 = A.rollout_frames_grad(s, 1.0, p, 6, {6: frames[6]})
+    l2, g2 = A.rollout_grad(s, 1.0, p, 6, targets[6])
+    assert l1 == pytest.approx(l2) and np.allclose(g1, g2, rtol=1e-12, atol=1e-14)
