@@ -281,6 +281,22 @@ mc_status mc_point_loss(int32_t kind, const float* particles, int64_t n, const f
                         int64_t m, double alpha, float* loss, float* grad, void* workspace,
                         void* stream);
 
+/* Stability term of the folding loss (experiments/filament_folding/
+ * experiment.py:878-893, `_frame_stab` averaged over the stability frames):
+ *   loss = mean_s [ mean(d(frames[s], main)^2) + mean(d(frames[s], anchor)^2) ]
+ * with d the minimum-image displacement on the height x width torus
+ * (utils.py:18-23) and each mean over the (n, 2) array.  frames (num_frames,
+ * n, 2), main (n, 2) and anchor (n, 2) are float32 device arrays; anchor may
+ * be NULL (the reference's use_anchor false: that term is left out).  With
+ * frames_grad non-NULL, writes d loss / d frames (num_frames, n, 2), d loss /
+ * d main (n, 2) and, when anchor is given, d loss / d anchor (n, 2); a caller
+ * whose main or anchor frame is also a stability frame adds the gradients.
+ * loss is one device float.  Stream-ordered; no host synchronisation. */
+mc_status mc_frame_stability(const float* frames, int32_t num_frames, int64_t n,
+                             const float* main_pos, const float* anchor_pos, int32_t height,
+                             int32_t width, float* loss, float* frames_grad, float* main_grad,
+                             float* anchor_grad, void* stream);
+
 /* Measurement: num_steps coupled steps in place (no graph, the LBM
  * collide-stream as its own launch) with HIP events carried by every LBM
  * dispatch and recorded after every step on `stream`; returns the average

@@ -1843,6 +1843,23 @@ mc_status mc_point_loss(int32_t kind, const float* particles, int64_t n, const f
   return MC_OK;
 }
 
+mc_status mc_frame_stability(const float* frames, int32_t num_frames, int64_t n,
+                             const float* main_pos, const float* anchor_pos, int32_t height,
+                             int32_t width, float* loss, float* frames_grad, float* main_grad,
+                             float* anchor_grad, void* stream) {
+  g_err.clear();
+  if (num_frames < 1 || num_frames > 4096 || n < 1 || n > (1 << 24))
+    return fail(MC_ERR_BAD_SHAPE, "need 1 .. 4096 frames of 1 .. 2^24 nodes");
+  if (height < 1 || width < 1) return fail(MC_ERR_BAD_SHAPE, "grid must be at least 1 x 1");
+  if (!frames || !main_pos || !loss) return fail(MC_ERR_INVALID, "NULL argument");
+  if (frames_grad && (!main_grad || (anchor_pos && !anchor_grad)))
+    return fail(MC_ERR_INVALID, "frames_grad needs main_grad (and anchor_grad with an anchor)");
+  HIPCHK(launch_stability(frames, num_frames, (int)n, main_pos, anchor_pos, height, width, loss,
+                          frames_grad, frames_grad ? main_grad : nullptr,
+                          frames_grad ? anchor_grad : nullptr, (hipStream_t)stream));
+  return MC_OK;
+}
+
 mc_status mc_plan_instances(const mc_plan* p, int32_t* num_instances, int64_t* offsets) {
   g_err.clear();
   if (!p || !num_instances) return fail(MC_ERR_INVALID, "NULL argument");

@@ -238,6 +238,11 @@ size_t point_loss_workspace(long long n, long long m);
 hipError_t launch_point_loss(int kind, const float* particles, int n, const float* targets, int m,
                              float alpha, float* loss, float* grad, void* workspace,
                              hipStream_t s);
+// Stability term of the folding loss over nf frames (nf, n, 2) against the
+// main and (optional) anchor frames; g_frames NULL skips the gradients.
+hipError_t launch_stability(const float* frames, int nf, int n, const float* main_pos,
+                            const float* anchor_pos, int height, int width, float* loss,
+                            float* g_frames, float* g_main, float* g_anchor, hipStream_t s);
 hipError_t launch_rod_segment(const RodAdjArgs& a, bool reverse, int steps, float* ck,
                               float* tape_or_dbg, size_t lds_bytes, hipStream_t s);
 // immersed boundary (solved per interaction cluster)

@@ -1,7 +1,8 @@
 // Folding losses of the filament-folding experiment and their gradients with
 // respect to the particle positions (experiments/filament_folding/
 // experiment.py): Chamfer (:57-63), density-aware Chamfer (:105-128) and the
-// smooth maximum of the nearest-neighbour distances (:139-149).
+// smooth maximum of the nearest-neighbour distances (:139-149), and the
+// stability term over the frames after the shape is scored (:878-893).
 //
 // Three launches:
 //   k_nn         one thread per particle and per target: the nearest point of
@@ -211,7 +212,69 @@ __global__ void __launch_bounds__(NN_THREADS) k_loss_grad(int kind, const float2
   }
 }
 
+// experiment.py:878-893: per frame s, mean(d(x_s, main)^2) (+ the same
+// against the anchor frame), d the minimum-image displacement (utils.py:18-23)
+// over the (n, 2) array; the loss is the mean over the frames.  Each frame's
+// sums are fixed-order block sums, so the result does not depend on timing.
+// The gradient of the floor-mod is 1 with respect to its first argument
+// (jax's rem rule), so d/dx_s = 2 d / (2 n S) and the main and anchor frames
+// take minus the sum over s.
+__global__ void __launch_bounds__(LOSS_THREADS) k_stability(const float* __restrict__ frames,
+                                                            int nf, int n,
+                                                            const float* __restrict__ main_pos,
+                                                            const float* __restrict__ anchor_pos,
+                                                            float sx, float sy,
+                                                            float* __restrict__ loss,
+                                                            float* __restrict__ g_frames,
+                                                            float* __restrict__ g_main,
+                                                            float* __restrict__ g_anchor) {
+  __shared__ float red[LOSS_THREADS];
+  const int tid = threadIdx.x, ne = 2 * n;
+  const float inv = 1.0f / (float)ne, fs = (float)nf;
+  float total = 0.0f;
+  for (int s = 0; s < nf; ++s) {
+    const float* x = frames + (size_t)s * ne;
+    float pm = 0.0f, pa = 0.0f;
+    for (int e = tid; e < ne; e += LOSS_THREADS) {
+      const float L = (e & 1) ? sy : sx;
+      const float dm = min_image(x[e], main_pos[e], L);
+      pm = pm + dm * dm;
+      float ga = 0.0f;
+      if (anchor_pos) {
+        const float da = min_image(x[e], anchor_pos[e], L);
+        pa = pa + da * da;
+        ga = 2.0f * da * inv / fs;
+      }
+      if (g_frames) g_frames[(size_t)s * ne + e] = 2.0f * dm * inv / fs + ga;
+    }
+    float d = block_sum(pm, red) * inv;
+    if (anchor_pos) d = d + block_sum(pa, red) * inv;
+    total = total + d;
+  }
+  if (tid == 0) *loss = total / fs;
+  if (!g_frames) return;
+  for (int e = tid; e < ne; e += LOSS_THREADS) {
+    const float L = (e & 1) ? sy : sx;
+    float gm = 0.0f, ga = 0.0f;
+    for (int s = 0; s < nf; ++s) {
+      const float xv = frames[(size_t)s * ne + e];
+      gm = gm + 2.0f * min_image(xv, main_pos[e], L) * inv / fs;
+      if (anchor_pos) ga = ga + 2.0f * min_image(xv, anchor_pos[e], L) * inv / fs;
+    }
+    g_main[e] = -gm;
+    if (anchor_pos) g_anchor[e] = -ga;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_stability(const float* frames, int nf, int n, const float* main_pos,
+                            const float* anchor_pos, int height, int width, float* loss,
+                            float* g_frames, float* g_main, float* g_anchor, hipStream_t s) {
+  hipLaunchKernelGGL(k_stability, dim3(1), dim3(LOSS_THREADS), 0, s, frames, nf, n, main_pos,
+                     anchor_pos, (float)width, (float)height, loss, g_frames, g_main, g_anchor);
+  return hipGetLastError();
+}
 
 size_t point_loss_workspace(long long n, long long m) {
   return (size_t)(5 * n + 4 * m + 4) * sizeof(float);

@@ -101,3 +101,30 @@ def pca_aligned(loss_fn, particles: torch.Tensor, targets: torch.Tensor, **kw):
             elif lv1 == lv0:
                 best = (l0, 0.5 * (g0 + g))
     return best
+
+
+def frame_stability(grid_shape, frames: torch.Tensor, main: torch.Tensor,
+                    anchor: torch.Tensor = None):
+    """experiment.py:878-893: the stability term, mean over the frames
+    (S, n, 2) of mean(d(x_s, main)^2) + mean(d(x_s, anchor)^2), d the
+    minimum-image displacement on the (H, W) torus (mc_frame_stability).
+    ``anchor`` None leaves its term out.  Returns (loss, d/d frames,
+    d/d main, d/d anchor or None)."""
+    d = frames.device
+    if d.type != "cuda":
+        raise RuntimeError("microcosmos_amd runs on the GPU only")
+    lib = _lib.load()
+    fr = as_f32(frames, d)
+    s, n = fr.shape[0], fr.shape[1]
+    m = as_f32(main, d)
+    a = as_f32(anchor, d) if anchor is not None else None
+    if fr.shape != (s, n, 2) or m.shape != (n, 2) or (a is not None and a.shape != (n, 2)):
+        raise ValueError("frames must be (S, n, 2) and main / anchor (n, 2)")
+    h, w = grid_shape
+    loss = torch.empty((), dtype=torch.float32, device=d)
+    gf = torch.empty_like(fr)
+    gm = torch.empty_like(m)
+    ga = torch.empty_like(a) if a is not None else None
+    check(lib.mc_frame_stability(ptr(fr), s, n, ptr(m), ptr(a), int(h), int(w), ptr(loss),
+                                 ptr(gf), ptr(gm), ptr(ga), stream_handle(d)))
+    return loss, gf, gm, ga

@@ -12,12 +12,18 @@ experiments/filament_folding/experiment.py:
   _pca_rotation_angle:66-84   principal-axis angle difference (no gradient)
   _rotate_particles  :87-92   rotation of the centred particle cloud
   *_pca_aligned               min over the angle and the angle + pi, targets centred
+  _frame_stab        :878-893 mean min-image squared displacement of a frame
+                              from the main (and anchor) frame
+  loss_fn            :803-916 shape loss over the evaluation frames + weighted
+                              stability and smooth-max terms
 
 Gradients follow jax.grad's rules: the derivative of a min over ties is split
 equally among the tied entries (jax's reduce-min / minimum rule), argmin takes
 the first index, and stop_gradient quantities are constants.  Every function
 is float64.  Pinned by central finite differences and by closed forms in
-``tests/test_losses_oracle.py``.
+``tests/test_losses_oracle.py``; the frame indices of the folding objective
+are pinned there by hand-evaluated values of the reference's formulas at its
+configured sizes.
 """
 from __future__ import annotations
 
@@ -129,3 +135,100 @@ def pca_aligned(loss_fn, p, t, angle=None, **kw):
     if l1 < l0:
         return l1, g1
     return l0, 0.5 * (g0 + g1)
+
+
+def min_image(grid_shape, a, b):
+    """utils.py:18-23: floor-mod minimum-image displacement a - b, with the
+    size flipped to (W, H) for (x, y) positions."""
+    h, w = grid_shape
+    size = np.array([float(w), float(h)])
+    return np.mod(np.asarray(a, np.float64) - b + size / 2.0, size) - size / 2.0
+
+
+def frame_stability(grid_shape, frames, main, anchor=None):
+    """experiment.py:878-893: mean over the frames of mean(d(x_s, main)^2)
+    (+ mean(d(x_s, anchor)^2) when the anchor is used).  The floor-mod's
+    derivative is 1 in its first argument (jax's rem rule).  Returns (loss,
+    d/d frames (S, n, 2), d/d main, d/d anchor or None)."""
+    frames = np.asarray(frames, np.float64)
+    s = frames.shape[0]
+    k = frames[0].size
+    loss = 0.0
+    g = np.zeros_like(frames)
+    gm = np.zeros_like(frames[0])
+    ga = np.zeros_like(frames[0]) if anchor is not None else None
+    for i in range(s):
+        dm = min_image(grid_shape, frames[i], main)
+        loss += (dm * dm).mean()
+        g[i] += 2.0 * dm / (k * s)
+        gm -= 2.0 * dm / (k * s)
+        if anchor is not None:
+            da = min_image(grid_shape, frames[i], anchor)
+            loss += (da * da).mean()
+            g[i] += 2.0 * da / (k * s)
+            ga -= 2.0 * da / (k * s)
+    return loss / s, g, gm, ga
+
+
+def smoothness(angles):
+    """experiment.py:895-896: mean((a[1:] - a[:-1])^2) over the rest angles;
+    returns (loss, d loss / d angles)."""
+    a = np.asarray(angles, np.float64)
+    d = a[1:] - a[:-1]
+    g = np.zeros_like(a)
+    g[1:] += 2.0 * d / d.size
+    g[:-1] -= 2.0 * d / d.size
+    return float((d * d).mean()), g
+
+
+def _centred(loss_fn, p, t, **kw):
+    """The non-PCA branch (experiment.py:826-827): both clouds centred; the
+    gradient goes through the particles' centring."""
+    p = np.asarray(p, np.float64)
+    t = np.asarray(t, np.float64)
+    l, g = loss_fn(p - p.mean(axis=0), t - t.mean(axis=0), **kw)
+    return l, g - g.mean(axis=0)
+
+
+def folding_objective(positions, target, grid_shape, eval_steps, stab_steps, anchor_step,
+                      loss_mode="dcd", use_pca=True, stability_weight=1.0, max_dist_weight=1.0,
+                      alpha=0.1):
+    """experiment.py:803-916 for the point-cloud modes, as a function of the
+    frame positions: ``positions`` maps a step count to the positions after
+    that many steps (the reference's timeseries index + 1); ``eval_steps``
+    and ``stab_steps`` are the scored frames, the main frame is the last
+    evaluation frame and ``anchor_step`` the anchor (None: not used).  The
+    smoothness term depends on the rest angles only and is not part of this
+    function.  Returns (total, shape, stability, {step: d total / d
+    positions})."""
+    fn = chamfer if loss_mode == "pointcloud" else dcd
+    bars = {t: np.zeros_like(np.asarray(positions[t], np.float64)) for t in positions}
+    ne = len(eval_steps)
+
+    def frame(loss, p, **kw):
+        return pca_aligned(loss, p, target, **kw) if use_pca else _centred(loss, p, target, **kw)
+
+    shape = 0.0
+    for t in eval_steps:
+        l, g = frame(fn, positions[t])
+        shape += l / ne
+        bars[t] += g / ne
+    main = eval_steps[-1]
+    if stab_steps:
+        fr = np.stack([np.asarray(positions[t], np.float64) for t in stab_steps])
+        stab, g, gm, ga = frame_stability(grid_shape, fr, positions[main],
+                                          None if anchor_step is None else positions[anchor_step])
+        for i, t in enumerate(stab_steps):
+            bars[t] += stability_weight * g[i]
+        bars[main] += stability_weight * gm
+        if anchor_step is not None:
+            bars[anchor_step] += stability_weight * ga
+    else:
+        stab = 0.0   # the last frame is the main frame: zero displacement
+    md = 0.0
+    if max_dist_weight > 0.0:
+        for t in eval_steps:
+            l, g = frame(smooth_max_nn, positions[t], alpha=alpha)
+            md += l / ne
+            bars[t] += max_dist_weight * g / ne
+    return shape + stability_weight * stab + max_dist_weight * md, shape, stab, bars

@@ -54,6 +54,7 @@ from dataclasses import replace
 
 import numpy as np
 
+from . import losses as LS
 from . import step_ref as R
 
 OMEGA = 1.8  # pbd.py:43
@@ -319,15 +320,30 @@ def rollout_frames_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, fra
     position)``; returns (sum of the frame losses, its gradient with respect
     to the rest angles).  The reverse pass adds each frame's cotangent to the
     position adjoint when it reaches that frame."""
+    def objective(pos):
+        total, bars = 0.0, {}
+        for t, fn in frame_losses.items():
+            l, g = fn(pos[t])
+            total += l
+            bars[t] = g
+        return total, bars
+    total, rhob, _ = rollout_objective_grad(s0, dt, p, num_steps, list(frame_losses), objective,
+                                            dtype)
+    return total, rhob
+
+
+def rollout_objective_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, steps,
+                           objective, dtype=np.float64):
+    """An objective coupling several frames: ``objective({t: position after
+    t steps for t in steps})`` returns (loss, {t: d loss / d that position}),
+    as the folding loss's stability term compares frames with each other.
+    Returns (loss, its gradient with respect to the rest angles, the frame
+    positions)."""
     states = [s0.astype(dtype)]
     for _ in range(num_steps):
         states.append(R.step(states[-1], dt, p))
-    total = 0.0
-    bars = {}
-    for t, fn in frame_losses.items():
-        l, g = fn(np.asarray(states[t].position, np.float64))
-        total += l
-        bars[t] = g
+    pos = {int(t): np.asarray(states[int(t)].position, np.float64) for t in steps}
+    total, bars = objective(pos)
     xb = np.zeros((s0.position.shape[0], 2))
     vb = np.zeros_like(xb)
     thb = np.zeros(s0.theta.shape[0])
@@ -337,7 +353,36 @@ def rollout_frames_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, fra
             xb = xb + bars[t + 1]
         xb, vb, thb, rb = step_vjp(states[t], dt, p, xb, vb, thb)
         rhob = rhob + rb
-    return total, rhob
+    return total, rhob, pos
+
+
+def folding_raw_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, raw,
+                     curvature_scale: float, target, eval_steps, stab_steps, anchor_step,
+                     smoothness_weight: float = 0.0, rest=None, dtype=np.float64, **objective_kw):
+    """The folding experiment's loss for one initial state
+    (experiments/filament_folding/experiment.py:739-745, 803-916): rest
+    angles tanh(raw) * curvature_scale (or ``rest``, the same values computed
+    elsewhere, so a float32 rollout can follow another's rounding), a rollout
+    of ``num_steps`` steps scored by ``losses.folding_objective`` at the given
+    frames, plus the smoothness term.  Returns (total, d total / d raw, frame
+    positions)."""
+    raw = np.asarray(raw, np.float64)
+    th = np.tanh(raw)
+    r = th * curvature_scale if rest is None else np.asarray(rest, np.float64)
+    s = replace(s0, bending_rest_angles=r)
+    steps = sorted(set(eval_steps) | set(stab_steps) | ({anchor_step} - {None}))
+
+    def objective(pos):
+        total, _, _, bars = LS.folding_objective(pos, target, s0.grid_shape, eval_steps,
+                                                 stab_steps, anchor_step, **objective_kw)
+        return total, bars
+
+    total, g_rest, pos = rollout_objective_grad(s, dt, p, num_steps, steps, objective, dtype)
+    if smoothness_weight > 0.0:
+        l, g = LS.smoothness(r)
+        total += smoothness_weight * l
+        g_rest = g_rest + smoothness_weight * g
+    return total, g_rest * curvature_scale * (1.0 - th * th), pos
 
 
 def rollout_frames_loss(s0: R.State, dt: float, p: R.Params, num_steps: int, frame_losses,

@@ -338,3 +338,71 @@ def test_multi_frame_gradient_matches_restatement(dev, physics):
     rel = np.linalg.norm(g - go) / np.linalg.norm(go)
     _say(f"rel-L2 {rel:.2e}")
     assert rel <= 1e-4
+
+
+def _objective_case(physics):
+    from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID, PBD_SCHEME_NO_FLUID
+    wl, target = W.folding(steps=30)
+    if physics == "pbd":
+        cfg = replace(PBD_SCHEME_NO_FLUID, enable_steric=False)
+        p = replace(R.PBD_SCHEME_NO_FLUID, enable_steric=False)
+        dt = wl.dt
+    else:
+        cfg = replace(COSSERAT_SCHEME_NO_FLUID, dt=W.FOLDING_DT, **W.FOLDING_COSSERAT)
+        p = replace(R.COSSERAT_SCHEME_NO_FLUID, config_dt=W.FOLDING_DT, **W.FOLDING_COSSERAT)
+        dt = W.FOLDING_DT
+    rng = np.random.default_rng(6)
+    raw = rng.normal(0.0, 0.3, wl.bending_pairs.shape[0]).astype(np.float32)
+    cloud = (target[::3] + rng.normal(0.0, 0.5, target[::3].shape)).astype(np.float32)
+    return wl, cfg, p, dt, raw, cloud
+
+
+@pytest.mark.parametrize("physics", ["pbd", "folding"])
+def test_folding_objective_matches_restatement(dev, physics):
+    """The folding loss of one initial state over 20 + 10 steps (DCD,
+    PCA-aligned, over 3 evaluation frames; stability over 3 frames against
+    the main and anchor frames; smooth max; smoothness) and its raw-angle
+    gradient against the restatement along the float32 trajectory."""
+    _say(f"folding objective ({physics}): 20 + 10 steps")
+    from microcosmos_amd import folding as F
+    wl, cfg, p, dt, raw, cloud = _objective_case(physics)
+    lc = F.FoldingLossConfig(num_eval_frames=3, num_stability_frames=3,
+                             stability_anchor_offset=5, smoothness_weight=0.1)
+    nodes, edges, fields = device_state(wl, dev)
+    raw_d = torch.as_tensor(raw, device=dev)
+    total, shape, stab, g = F.folding_value_and_grad(
+        nodes, edges, fields, cfg, dt, 20, 10, raw_d, torch.as_tensor(cloud, device=dev), lc,
+        checkpoint_every=25)
+    rest = F.rest_angles(raw_d, lc).cpu().numpy()
+    g = g.cpu().numpy().astype(np.float64)
+    torch.cuda.synchronize()
+    ev = [t + 1 for t in F.eval_indices(20, 3)]
+    st = [t + 1 for t in F.stability_indices(20, 10, 3)]
+    an = F.anchor_index(20, 10, ev[-1] - 1, 5) + 1
+    assert (ev, st, an) == ([10, 15, 20], [21, 25, 30], 25)
+    s = oracle_state(wl, np.float64)
+    lo, go, _ = A.folding_raw_grad(s, dt, p, 30, raw, lc.curvature_scale, cloud, ev, st, an,
+                                   smoothness_weight=0.1, rest=rest, dtype=np.float32,
+                                   loss_mode="dcd", use_pca=True, stability_weight=1.0,
+                                   max_dist_weight=1.0, alpha=0.1)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    _say(f"loss {float(total.item()):.6f} vs {lo:.6f}; shape {float(shape.item()):.4f} "
+         f"stability {float(stab.item()):.4f}; gradient rel-L2 {rel:.2e}")
+    assert float(total.item()) == pytest.approx(lo, rel=1e-4)
+    assert float(stab.item()) > 0.0
+    assert rel <= 1e-4
+
+
+def test_folding_objective_optimisation_lowers_the_loss(dev):
+    _say("folding objective optimisation: 12 iterations, 20 + 10 steps")
+    from microcosmos_amd import folding as F
+    from microcosmos_amd.grad import FoldingOptimizer
+    wl, cfg, p, dt, raw, cloud = _objective_case("folding")
+    lc = F.FoldingLossConfig(num_eval_frames=3, num_stability_frames=3, stability_anchor_offset=5)
+    nodes, edges, fields = device_state(wl, dev)
+    opt = FoldingOptimizer(lr=0.05, warmup_steps=0, total_steps=12)
+    _, hist = F.optimize_folding_objective(nodes, edges, fields, cfg, dt, 20, 10,
+                                           torch.as_tensor(cloud, device=dev), 12, lc,
+                                           optimizer=opt)
+    _say(f"losses {hist[0]:.4f} -> {hist[-1]:.4f}")
+    assert np.all(np.isfinite(hist)) and hist[-1] < hist[0]

@@ -105,3 +105,32 @@ def test_chamfer_gradient_drives_the_rod_adjoint(dev):
     _say(f"chamfer rollout: loss {float(loss.item()):.6f} (restatement {lo:.6f}), rel-L2 {rel:.2e}")
     assert abs(float(loss.item()) - lo) <= 1e-5 * abs(lo)
     assert rel <= 1e-4
+
+
+@pytest.mark.parametrize("use_anchor", [False, True])
+@pytest.mark.parametrize("shape", [(4, 384), (1, 1), (9, 2500)])
+def test_frame_stability_matches_restatement(dev, use_anchor, shape):
+    """mc_frame_stability against the restatement, with frames that cross
+    the periodic boundary (minimum image) on a non-square grid."""
+    s, n = shape
+    _say(f"frame stability frames={s} n={n} anchor={use_anchor}")
+    from microcosmos_amd import losses as D
+    rng = np.random.default_rng(s * 1000 + n)
+    grid = (192, 256)
+    size = np.array([grid[1], grid[0]], np.float64)
+    main = (rng.uniform(0, 1, (n, 2)) * size).astype(np.float32)
+    fr = np.mod(main[None] + rng.normal(0, 4.0, (s, n, 2)), size).astype(np.float32)
+    anchor = np.mod(main + rng.normal(0, 3.0, (n, 2)), size).astype(np.float32) if use_anchor \
+        else None
+    t = lambda a: None if a is None else torch.as_tensor(a, device=dev)
+    l, gf, gm, ga = D.frame_stability(grid, t(fr), t(main), t(anchor))
+    torch.cuda.synchronize()
+    lo, gfo, gmo, gao = L.frame_stability(grid, fr.astype(np.float64), main.astype(np.float64),
+                                          None if anchor is None else anchor.astype(np.float64))
+    assert float(l.item()) == pytest.approx(lo, rel=1e-5)
+    assert _rel(gf.cpu().numpy(), gfo) <= 1e-5
+    assert _rel(gm.cpu().numpy(), gmo) <= 1e-5
+    if use_anchor:
+        assert _rel(ga.cpu().numpy(), gao) <= 1e-5
+    else:
+        assert ga is None

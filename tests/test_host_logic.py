@@ -165,3 +165,22 @@ def test_folding_optimizer_first_step_is_sign_times_lr():
     gc = g.numpy().astype(np.float64) / np.linalg.norm(g.numpy().astype(np.float64))
     np.testing.assert_allclose(p.numpy(), -0.06 * gc / (np.abs(gc) + 1e-8), rtol=1e-5)
     assert np.all(np.abs(np.abs(p.numpy()) - 0.06) < 3e-5)
+
+
+def test_folding_frame_indices():
+    """experiment.py:787-801, 872-877 evaluated by hand at the folding
+    configuration (num_steps 150, stability_steps 30, 4 evaluation and 4
+    stability frames, anchor offset 30): evaluation 74 + 75 i / 3, stability
+    150 + 29 i / 3 rounded, anchor min(149 + 30, 179)."""
+    from microcosmos_amd import folding as F
+    assert F.eval_indices(150, 4) == [74, 99, 124, 149]
+    assert F.eval_indices(150, 1) == [149]
+    assert F.eval_indices(9, 3) == [3, 6, 8]
+    assert F.stability_indices(150, 30, 4) == [150, 160, 169, 179]
+    assert F.stability_indices(150, 30, 1) == [179]
+    assert F.stability_indices(150, 0, 4) == []
+    assert F.anchor_index(150, 30, 149, 30) == 179
+    assert F.anchor_index(150, 10, 149, 30) == 159
+    assert F.anchor_index(150, 0, 149, 30) is None
+    cfg = F.FoldingLossConfig()
+    assert F.objective_steps(150, 30, cfg) == [75, 100, 125, 150, 151, 161, 170, 180]

@@ -99,3 +99,79 @@ def test_pca_angle_of_rotated_copy():
     a = L.pca_rotation_angle(p, t)
     assert np.isclose(np.mod(a - 0.7 + np.pi / 2, np.pi) - np.pi / 2, 0.0, atol=1e-9)
     assert L.pca_aligned(L.chamfer, p, t)[0] < 1e-12
+
+
+def _frames(seed, s=3, n=30, grid=(40, 50)):
+    rng = np.random.default_rng(seed)
+    h, w = grid
+    main = rng.uniform(0, 1, (n, 2)) * np.array([w, h])
+    fr = np.mod(main[None] + rng.normal(0, 3.0, (s, n, 2)), np.array([w, h]))
+    anchor = np.mod(main + rng.normal(0, 2.0, (n, 2)), np.array([w, h]))
+    return fr, main, anchor, grid
+
+
+@pytest.mark.parametrize("use_anchor", [False, True])
+def test_frame_stability_matches_fd(use_anchor):
+    fr, main, anchor, grid = _frames(7)
+    an = anchor if use_anchor else None
+    l, gf, gm, ga = L.frame_stability(grid, fr, main, an)
+    np.testing.assert_allclose(gf, _fd(lambda x: L.frame_stability(grid, x, main, an)[0], fr),
+                               atol=1e-7)
+    np.testing.assert_allclose(gm, _fd(lambda x: L.frame_stability(grid, fr, x, an)[0], main),
+                               atol=1e-7)
+    if use_anchor:
+        np.testing.assert_allclose(
+            ga, _fd(lambda x: L.frame_stability(grid, fr, main, x)[0], anchor), atol=1e-7)
+    else:
+        assert ga is None
+
+
+def test_frame_stability_closed_forms():
+    """Every frame shifted by (a, b) from the main frame: each term is
+    mean((a, b)^2) = (a^2 + b^2) / 2; a shift by whole periods is no
+    displacement (minimum image); the frame-wise mean divides by the number
+    of frames."""
+    _, main, _, grid = _frames(8)
+    h, w = grid
+    a, b = 1.5, -2.0
+    fr = np.stack([main + [a, b], main + [a, b]])
+    assert L.frame_stability(grid, fr, main)[0] == pytest.approx((a * a + b * b) / 2)
+    wrapped = np.stack([main + [w, -h], main + [2 * w, 0.0]])
+    assert L.frame_stability(grid, wrapped, main)[0] == pytest.approx(0.0, abs=1e-24)
+    two = np.stack([main + [a, 0.0], main])
+    assert L.frame_stability(grid, two, main)[0] == pytest.approx(a * a / 2 / 2)
+    # with the anchor equal to main the anchor term doubles the loss
+    assert L.frame_stability(grid, fr, main, main)[0] == pytest.approx(a * a + b * b)
+
+
+def test_smoothness_matches_fd_and_closed_form():
+    a = np.random.default_rng(9).normal(size=12)
+    l, g = L.smoothness(a)
+    np.testing.assert_allclose(g, _fd(lambda x: L.smoothness(x)[0], a), atol=1e-8)
+    assert L.smoothness(np.arange(5.0) * 0.3)[0] == pytest.approx(0.09)
+
+
+@pytest.mark.parametrize("mode", ["dcd", "pointcloud"])
+def test_folding_objective_matches_fd(mode):
+    """The frame bookkeeping of the objective (evaluation, stability, main
+    and anchor frames, weights) against finite differences in every frame's
+    positions, centred branch (the PCA angle is stop_gradient, pinned on its
+    own above)."""
+    rng = np.random.default_rng(10)
+    grid = (40, 50)
+    base = rng.uniform(10, 30, (20, 2))
+    pos = {t: base + rng.normal(0, 1.0, base.shape) for t in (3, 5, 6, 7, 8)}
+    tgt = rng.normal(0, 4.0, (25, 2))
+    kw = dict(loss_mode=mode, use_pca=False, stability_weight=0.7, max_dist_weight=0.4, alpha=0.5)
+    ev, st, an = [3, 5], [6, 7, 8], 7
+
+    def f(t, x):
+        q = dict(pos)
+        q[t] = x
+        return L.folding_objective(q, tgt, grid, ev, st, an, **kw)[0]
+
+    total, shape, stab, bars = L.folding_objective(pos, tgt, grid, ev, st, an, **kw)
+    assert total > shape > 0.0 and stab > 0.0
+    for t in pos:
+        np.testing.assert_allclose(bars[t], _fd(lambda x: f(t, x), pos[t]), atol=2e-6,
+                                   err_msg=f"frame {t}")

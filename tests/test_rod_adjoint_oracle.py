@@ -149,3 +149,22 @@ def test_multi_frame_gradient_matches_fd(solver):
     l1, g1 = A.rollout_frames_grad(s, 1.0, p, 6, {6: frames[6]})
     l2, g2 = A.rollout_grad(s, 1.0, p, 6, targets[6])
     assert l1 == pytest.approx(l2) and np.allclose(g1, g2, rtol=1e-12, atol=1e-14)
+
+
+def test_folding_raw_angle_gradient_matches_fd():
+    """The folding loss of one initial state (shape over evaluation frames,
+    stability against the main and anchor frames, smooth max, smoothness)
+    through tanh(raw) * curvature_scale: its raw-angle gradient against
+    central finite differences of the float64 rollout."""
+    s, p, _ = _case()
+    rng = np.random.default_rng(4)
+    raw = rng.normal(0.0, 0.4, s.bending_pairs.shape[0])
+    tgt = s.position[:16] + rng.normal(0.0, 1.5, (16, 2))
+    frames = dict(eval_steps=[4, 6], stab_steps=[7, 8, 10], anchor_step=9)
+    kw = dict(loss_mode="dcd", use_pca=False, stability_weight=0.8, max_dist_weight=0.5,
+              alpha=0.3, smoothness_weight=0.2)
+    total, g, _ = A.folding_raw_grad(s, 1.0, p, 10, raw, 0.6, tgt, **frames, **kw)
+    fd = _fd(lambda r: A.folding_raw_grad(s, 1.0, p, 10, r, 0.6, tgt, **frames, **kw)[0], raw,
+             1e-6)
+    assert np.isfinite(total) and np.max(np.abs(fd)) > 1e-3
+    np.testing.assert_allclose(g, fd, atol=1e-6 * max(1.0, np.max(np.abs(fd))))
