@@ -253,6 +253,21 @@ mc_status mc_rod_adjoint_backward_frames(mc_plan* plan, const mc_params* params,
                                          const float* velocity_bar, float* rest_angle_bar,
                                          float* position0_bar, float* velocity0_bar,
                                          float* theta0_bar, void* stream);
+/* As mc_rod_adjoint_backward_frames, and also d loss / d rest lengths (E,
+ * device, required): the folding experiment optimises the rest lengths with
+ * the rest angles (experiments/filament_folding/experiment.py:739-745, its
+ * rest_lengths_raw).  The rest lengths enter the position pass (PBD: the
+ * stretch and the shear target, pbd.py:55-89; stable Cosserat: the edge force
+ * and the diagonal h, stable_cosserat.py:94-117); with the stable Cosserat
+ * solver the reverse pass records each cycle's node update (checkpoint
+ * interval x cycles x N float2 of device memory, allocated on first use). */
+mc_status mc_rod_adjoint_backward_params(mc_plan* plan, const mc_params* params,
+                                         const mc_edge_params* edges, double dt,
+                                         int32_t num_frames, const int64_t* frame_steps,
+                                         const float* frame_position_bars,
+                                         const float* velocity_bar, float* rest_angle_bar,
+                                         float* rest_length_bar, float* position0_bar,
+                                         float* velocity0_bar, float* theta0_bar, void* stream);
 
 /* The direct filter's taps along an axis of n cells for sigma (host only, no
  * device work; no counterpart in the reference -- it exposes the plan's

@@ -97,6 +97,10 @@ SIGNATURES = {
                                                  POINTER(McEdgeParams), c_double, c_int32,
                                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                                  c_void_p, c_void_p, c_void_p]),
+    "mc_rod_adjoint_backward_params": (c_int32, [c_void_p, POINTER(McParams),
+                                                 POINTER(McEdgeParams), c_double, c_int32,
+                                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                 c_void_p, c_void_p, c_void_p, c_void_p]),
     "mc_center_of_mass": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32,
                                     c_void_p, c_void_p]),
     "mc_track_enable": (c_int32, [c_void_p, c_int32, c_void_p, c_int32]),

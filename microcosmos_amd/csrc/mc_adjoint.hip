@@ -50,6 +50,7 @@ struct SegCtx {
   float *X, *V, *TH, *P, *CO, *FE, *RS, *KB, *LE;
   float *Y, *HN;                       // Cosserat: inertial target, per-node diagonal h
   float *XB, *VB, *THB, *DB, *RB, *YB; // adjoints (reverse only)
+  float* LB;                           // rest-length adjoint (reverse only)
 };
 
 __device__ __forceinline__ void seg_init(const RodAdjArgs& a, SegCtx& s, float* sm, bool reverse) {
@@ -72,7 +73,7 @@ __device__ __forceinline__ void seg_init(const RodAdjArgs& a, SegCtx& s, float* 
   s.Y = take(2 * nn); s.HN = take(nn);
   if (reverse) {
     s.XB = take(2 * nn); s.VB = take(2 * nn); s.THB = take(ne); s.DB = take(nb); s.RB = take(nb);
-    s.YB = take(2 * nn);
+    s.YB = take(2 * nn); s.LB = take(ne);
   }
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
     s.RS[b] = a.rest_ang[s.hinges[b]];
@@ -99,8 +100,10 @@ __device__ __forceinline__ void seg_init(const RodAdjArgs& a, SegCtx& s, float* 
 // final wrap (simulate.py:83-98).  With `rec`, every cycle's per-edge
 // minimum-image vector and the direction (cos, sin) of the updated angle --
 // all the adjoint of the position pass needs -- go to rec[cyc * ne + e].
+// With `urec` (Cosserat), each cycle's node update (w (y - x) + sum F) / h
+// goes to urec[cyc * nn + i]: the rest-length adjoint's share through h.
 __device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, float4* rec,
-                                         const float* fst) {
+                                         const float* fst, float2* urec = nullptr) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int nn = s.nn, ne = s.ne, nb = s.nb;
   const bool is_cos = c.solver == 1;
@@ -193,6 +196,7 @@ __device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, float4* rec
         }
         s.P[2 * i] = xx + fx / s.HN[i];
         s.P[2 * i + 1] = xy + fy / s.HN[i];
+        if (urec) urec[(size_t)cyc * nn + i] = make_float2(fx / s.HN[i], fy / s.HN[i]);
       }
     }
     __syncthreads();
@@ -263,8 +267,12 @@ __global__ void __launch_bounds__(64) k_rod_fwd_segment(RodAdjArgs a, int steps,
 //   th2b = thb - k / l (-sin th2 Fb_x + cos th2 Fb_y),  th1b = 0.95 th2b
 //   pb' = pb (1 - w / h) + sum (+rb at the target, -rb at the source);
 //   yb += pb w / h;  corrb = th1b[e_in] / deg_in - th1b[e_out] / deg_out
+// With `urec` (the cycles' node updates u = (x' - x)), the rest-length
+// adjoint: lb += Fb . dF/dl + (hb_s + hb_g) dh/dl with dF/dl = -2 k r / l^3 +
+// k / l^2 (cos th2, sin th2), dh/dl = -2 k / l^3, hb_n = -(pb_n . u_n) / h_n.
 __device__ __forceinline__ void cosserat_cycles_adjoint(const Consts& c, SegCtx& s,
-                                                        const float4* rec) {
+                                                        const float4* rec,
+                                                        const float2* urec) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int nn = s.nn, ne = s.ne, nb = s.nb;
   const float k = c.k_st, w = c.cos_w;
@@ -276,6 +284,14 @@ __device__ __forceinline__ void cosserat_cycles_adjoint(const Consts& c, SegCtx&
       const float fbx = s.XB[2 * sn] / s.HN[sn] - s.XB[2 * tn] / s.HN[tn];
       const float fby = s.XB[2 * sn + 1] / s.HN[sn] - s.XB[2 * tn + 1] / s.HN[tn];
       const float l = s.LE[e];
+      if (urec) {
+        const float2 us = urec[(size_t)cyc * nn + sn], ut = urec[(size_t)cyc * nn + tn];
+        const float hb = -(s.XB[2 * sn] * us.x + s.XB[2 * sn + 1] * us.y) / s.HN[sn] -
+                         (s.XB[2 * tn] * ut.x + s.XB[2 * tn + 1] * ut.y) / s.HN[tn];
+        const float m3 = -2.0f * k / (l * l * l);
+        s.LB[e] = s.LB[e] + m3 * ((q.x * fbx + q.y * fby) + hb) +
+                  (k / (l * l)) * (q.z * fbx + q.w * fby);
+      }
       const float th2b = s.THB[e] + (-(k / l)) * (-q.w * fbx + q.z * fby);
       const float ab = 0.05f * th2b / (q.x * q.x + q.y * q.y);
       s.FE[2 * e] = (k / (l * l)) * fbx - ab * q.y;
@@ -337,8 +353,13 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
   const int nn = s.nn, ne = s.ne, nb = s.nb, C = s.C;
   const size_t N2 = 2 * (size_t)a.N;
   float4* tp = reinterpret_cast<float4*>(tape) + (size_t)steps * C * s.e0;
+  const bool lbar = a.len_bar != nullptr;
+  float2* up = (lbar && c.solver == 1)
+                   ? reinterpret_cast<float2*>(a.utape) + (size_t)steps * C * s.n0
+                   : nullptr;
   seg_load(s, ck, ck + N2, ck + 2 * N2);
-  for (int k = 0; k < steps; ++k) seg_step(c, s, tp + (size_t)k * C * ne, a.fst);
+  for (int k = 0; k < steps; ++k)
+    seg_step(c, s, tp + (size_t)k * C * ne, a.fst, up ? up + (size_t)k * C * nn : nullptr);
   for (int i = tid; i < nn; i += bs) {
     const int g = s.nodes[i];
     s.XB[2 * i] = a.xb[2 * g];
@@ -350,6 +371,7 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
   // the rest-angle adjoint continues the running sum, so the summation order
   // does not depend on where the checkpoints fall
   for (int b = tid; b < nb; b += bs) s.RB[b] = a.rest_bar[s.hinges[b]];
+  for (int e = tid; e < ne; e += bs) s.LB[e] = lbar ? a.len_bar[s.edges[e]] : 0.0f;
   __syncthreads();
   const float kst = c.k_st, ksh = c.k_sh;
   const bool is_cos = c.solver == 1;
@@ -361,7 +383,8 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
     }
     __syncthreads();
     if (is_cos) {
-      cosserat_cycles_adjoint(c, s, tp + (size_t)k * C * ne);
+      cosserat_cycles_adjoint(c, s, tp + (size_t)k * C * ne,
+                              up ? up + (size_t)k * C * nn : nullptr);
     } else
     for (int cyc = C - 1; cyc >= 0; --cyc) {
       const float4* rc = tp + ((size_t)k * C + cyc) * ne;
@@ -383,6 +406,9 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
         s.FE[2 * e] = 0.5f * kst * ((1.0f - ksh) * sx + ksh * fbx);
         s.FE[2 * e + 1] = 0.5f * kst * ((1.0f - ksh) * sy + ksh * fby);
         s.THB[e] = s.THB[e] + (-0.5f * kst * ksh * l) * (-q.w * fbx + q.z * fby);
+        // rest length: dF/dl0 = -k_st/2 ((1 - k_sh) rhat + k_sh (cos th', sin th'))
+        if (lbar)
+          s.LB[e] = s.LB[e] - 0.5f * kst * ((1.0f - ksh) * pr + ksh * (q.z * fbx + q.w * fby));
       }
       __syncthreads();
       // per node: + rb at the edge's target, - rb at its source
@@ -448,6 +474,8 @@ __global__ void __launch_bounds__(64) k_rod_adj_segment(RodAdjArgs a, int steps,
     }
   for (int e = tid; e < ne; e += bs) a.thb[s.edges[e]] = s.THB[e];
   for (int b = tid; b < nb; b += bs) a.rest_bar[s.hinges[b]] = s.RB[b];
+  if (lbar)
+    for (int e = tid; e < ne; e += bs) a.len_bar[s.edges[e]] = s.LB[e];
 }
 
 // ---- steric term ----
@@ -686,7 +714,7 @@ hipError_t launch_steric_adj_nodes(const Consts& c, const BatchDev& b, int N, co
 size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse) {
   (void)cycles;
   size_t f = 9 * (size_t)nn + 4 * (size_t)ne + 3 * (size_t)nb;
-  if (reverse) f += 6 * (size_t)nn + ne + 2 * (size_t)nb;
+  if (reverse) f += 6 * (size_t)nn + 2 * (size_t)ne + 2 * (size_t)nb;
   return f * sizeof(float);
 }
 

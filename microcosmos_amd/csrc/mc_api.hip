@@ -182,6 +182,8 @@ struct mc_plan {
   size_t adj_ckpt_cap = 0;
   float* adj_tape = nullptr;
   size_t adj_tape_cap = 0;
+  float* adj_utape = nullptr;   // Cosserat node updates for the rest-length adjoint
+  size_t adj_utape_cap = 0;
   float* adj_work = nullptr;
   float* adj_steric = nullptr;  // steric term: force (2N), pb1 (2N), V / Vbar / Dbar grids
   float* adj_stape = nullptr;   // steric term, per step: field, force, wrapped prediction
@@ -1663,8 +1665,9 @@ namespace {
 // position adjoint when the reverse pass reaches the state after that step.
 mc_status adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params* ed, double dt,
                            int nframes, const int64_t* frame_steps, const float* frame_bars,
-                           const float* velocity_bar, float* rest_angle_bar, float* position0_bar,
-                           float* velocity0_bar, float* theta0_bar, void* stream) {
+                           const float* velocity_bar, float* rest_angle_bar,
+                           float* rest_length_bar, float* position0_bar, float* velocity0_bar,
+                           float* theta0_bar, void* stream) {
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   MCCHK(check_rod_adjoint(q));
   if (!p->adj_ok) return fail(MC_ERR_INVALID, "no rollout recorded by mc_rod_adjoint_forward");
@@ -1692,6 +1695,11 @@ mc_status adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params*
     MCCHK(p->alloc(&p->adj_tape, tape, "adjoint tape"));
     p->adj_tape_cap = tape;
   }
+  const size_t utape = 2 * (size_t)every * (size_t)std::max(q->cycles_per_step, 1) * (size_t)N;
+  if (rest_length_bar && q->solver_type == MC_SOLVER_COSSERAT && p->adj_utape_cap < utape) {
+    MCCHK(p->alloc(&p->adj_utape, utape, "adjoint node-update tape"));
+    p->adj_utape_cap = utape;
+  }
   if (!p->adj_work) MCCHK(p->alloc(&p->adj_work, S, "adjoint state"));
   float* xb = p->adj_work;
   float* vb = xb + 2 * (size_t)N;
@@ -1702,6 +1710,8 @@ mc_status adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params*
   else HIPCHK(hipMemsetAsync(vb, 0, n2, s));
   if (E) HIPCHK(hipMemsetAsync(thb, 0, (size_t)E * sizeof(float), s));
   if (B) HIPCHK(hipMemsetAsync(rest_angle_bar, 0, (size_t)B * sizeof(float), s));
+  if (rest_length_bar && E)
+    HIPCHK(hipMemsetAsync(rest_length_bar, 0, (size_t)E * sizeof(float), s));
   RodAdjArgs a{};
   a.c = derive(p, q, dt);
   a.bd = p->bd;
@@ -1714,6 +1724,8 @@ mc_status adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params*
   a.vb = vb;
   a.thb = thb;
   a.rest_bar = rest_angle_bar;
+  a.len_bar = E ? rest_length_bar : nullptr;
+  a.utape = p->adj_utape;
   const int64_t nck = (T + every - 1) / every;
   const bool steric = q->enable_steric != 0;
   float *fst = nullptr, *pb1 = nullptr, *vg = nullptr, *vbar = nullptr, *dbar = nullptr;
@@ -1772,7 +1784,7 @@ mc_status mc_rod_adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_
   if (!position_bar) return fail(MC_ERR_INVALID, "position_bar is NULL");
   const int64_t last = p ? p->adj_steps : 0;
   return adjoint_backward(p, q, ed, dt, 1, &last, position_bar, velocity_bar, rest_angle_bar,
-                          position0_bar, velocity0_bar, theta0_bar, stream);
+                          nullptr, position0_bar, velocity0_bar, theta0_bar, stream);
 }
 
 mc_status mc_rod_adjoint_state(mc_plan* p, int64_t step, float* position_out,
@@ -1804,8 +1816,22 @@ mc_status mc_rod_adjoint_backward_frames(mc_plan* p, const mc_params* q,
                                          float* theta0_bar, void* stream) {
   g_err.clear();
   return adjoint_backward(p, q, ed, dt, num_frames, frame_steps, frame_position_bars,
-                          velocity_bar, rest_angle_bar, position0_bar, velocity0_bar, theta0_bar,
-                          stream);
+                          velocity_bar, rest_angle_bar, nullptr, position0_bar, velocity0_bar,
+                          theta0_bar, stream);
+}
+
+mc_status mc_rod_adjoint_backward_params(mc_plan* p, const mc_params* q,
+                                         const mc_edge_params* ed, double dt, int32_t num_frames,
+                                         const int64_t* frame_steps,
+                                         const float* frame_position_bars,
+                                         const float* velocity_bar, float* rest_angle_bar,
+                                         float* rest_length_bar, float* position0_bar,
+                                         float* velocity0_bar, float* theta0_bar, void* stream) {
+  g_err.clear();
+  if (p && p->E > 0 && !rest_length_bar) return fail(MC_ERR_INVALID, "rest_length_bar is NULL");
+  return adjoint_backward(p, q, ed, dt, num_frames, frame_steps, frame_position_bars,
+                          velocity_bar, rest_angle_bar, rest_length_bar, position0_bar,
+                          velocity0_bar, theta0_bar, stream);
 }
 
 

@@ -214,6 +214,11 @@ struct RodAdjArgs {
   // the filter and launch_steric_adj_nodes finish (a.xb / a.vb are left to them)
   const float* fst;
   float* pb1;
+  // rest-length adjoint (reverse, optional): len_bar (E) += d loss / d rest
+  // lengths; utape = the Cosserat cycles' node updates (steps x cycles x N
+  // float2) the reverse pass records for it
+  float* len_bar;
+  float* utape;
 };
 size_t rod_segment_lds_bytes(int cycles, int nn, int ne, int nb, bool reverse);
 // forward: ck = checkpoint slot written, tape_or_dbg = debug_vector (N, 2);

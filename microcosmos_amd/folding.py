@@ -19,9 +19,11 @@ indices: index t holds the state after t + 1 steps.
 
 The losses run in HIP kernels (``losses``); the frames are checkpoint
 boundaries of the adjoint's forward rollout and their cotangents enter the
-reverse pass there (``RodAdjoint.backward_frames``).  The rest lengths are
-inputs here (``rest_lengths`` gives the reference's parameterisation); the
-gradient is taken with respect to the raw rest angles.
+reverse pass there (``RodAdjoint.backward_frames``, ``backward_params``).
+The gradient is taken with respect to the raw rest angles and, when they are
+given, the raw rest lengths (distance_min + sigmoid(raw) (distance_max -
+distance_min), experiment.py:741-742); otherwise the edges' rest lengths are
+held fixed.
 """
 from __future__ import annotations
 
@@ -164,17 +166,23 @@ def objective_steps(num_steps: int, stability_steps: int, cfg: FoldingLossConfig
 def folding_value_and_grad(nodes: Nodes, edges: Edges, fields: Fields, solver_config, dt: float,
                            num_steps: int, stability_steps: int, raw_angles: torch.Tensor,
                            target: torch.Tensor, cfg: FoldingLossConfig = FoldingLossConfig(),
-                           checkpoint_every: int = 50):
+                           checkpoint_every: int = 50, raw_lengths: torch.Tensor = None):
     """One evaluation of the folding experiment's loss (experiment.py:803-916
     for one initial state) and its gradient with respect to the raw rest
     angles: rolls out num_steps + stability_steps steps from (nodes, edges,
     fields) with rest angles tanh(raw) * curvature_scale, scores the frames
     and runs the adjoint back through them.  Returns (total, shape,
-    stability, d total / d raw_angles)."""
+    stability, d total / d raw_angles); with ``raw_lengths`` (E,), the rest
+    lengths are ``rest_lengths(raw_lengths)`` and the tuple ends with
+    d total / d raw_lengths as well."""
     dev = nodes.position.device
     raw = as_f32(raw_angles, dev)
     rest = rest_angles(raw, cfg)
-    ad = RodAdjoint(nodes, replace(edges, bending_rest_angles=rest), fields, solver_config, dt)
+    e = replace(edges, bending_rest_angles=rest)
+    if raw_lengths is not None:
+        raw_l = as_f32(raw_lengths, dev)
+        e = replace(e, rest_lengths=rest_lengths(raw_l, cfg))
+    ad = RodAdjoint(nodes, e, fields, solver_config, dt)
     total_steps = int(num_steps) + int(stability_steps)
     steps = objective_steps(num_steps, stability_steps, cfg)
     ad.forward(total_steps, checkpoint_every, frames=steps)
@@ -182,7 +190,10 @@ def folding_value_and_grad(nodes: Nodes, edges: Edges, fields: Fields, solver_co
     tgt = as_f32(target, dev)
     total, shape, stab, bars = frame_objective(pos, tgt, ad.grid_shape, num_steps,
                                                stability_steps, cfg)
-    g_rest = ad.backward_frames(bars)[0]
+    if raw_lengths is None:
+        g_rest = ad.backward_frames(bars)[0]
+    else:
+        g_rest, g_len = ad.backward_params(bars)[:2]
     if cfg.smoothness_weight > 0.0 and rest.shape[0] > 1:
         # experiment.py:895-896: mean((a[1:] - a[:-1])^2)
         d = rest[1:] - rest[:-1]
@@ -192,28 +203,39 @@ def folding_value_and_grad(nodes: Nodes, edges: Edges, fields: Fields, solver_co
         g_rest[1:] += gd
         g_rest[:-1] -= gd
     th = torch.tanh(raw)
-    return total, shape, stab, g_rest * cfg.curvature_scale * (1.0 - th * th)
+    g_raw = g_rest * cfg.curvature_scale * (1.0 - th * th)
+    if raw_lengths is None:
+        return total, shape, stab, g_raw
+    sg = torch.sigmoid(raw_l)
+    return total, shape, stab, g_raw, g_len * (cfg.distance_max - cfg.distance_min) * sg * (1.0 - sg)
 
 
 def optimize_folding_objective(nodes: Nodes, edges: Edges, fields: Fields, solver_config,
                                dt: float, num_steps: int, stability_steps: int,
                                target: torch.Tensor, iterations: int,
                                cfg: FoldingLossConfig = FoldingLossConfig(), raw_angles=None,
-                               optimizer: FoldingOptimizer = None, checkpoint_every: int = 50):
-    """The folding experiment's loop over the raw rest angles
-    (experiment.py:985-1030 for one initial state, without curriculum
-    stages, noise or visualisation): clip, Adam and the warmup-cosine
-    schedule of ``FoldingOptimizer``.  Returns (raw angles, per-iteration
-    total losses)."""
+                               optimizer: FoldingOptimizer = None, checkpoint_every: int = 50,
+                               raw_lengths=None):
+    """The folding experiment's loop over its parameters
+    (experiment.py:397-403, 985-1030 for one initial state, without
+    curriculum stages, noise or visualisation): the raw rest angles and,
+    when ``raw_lengths`` is given (the reference starts both at zero), the
+    raw rest lengths, updated together by ``FoldingOptimizer`` (the global
+    norm is taken over both, as optax's over the parameter tree).  Returns
+    (raw angles, raw lengths or None, per-iteration total losses)."""
     dev = nodes.position.device
-    raw = (torch.zeros(edges.bending_rest_angles.shape[0], dtype=torch.float32, device=dev)
+    nb = edges.bending_rest_angles.shape[0]
+    raw = (torch.zeros(nb, dtype=torch.float32, device=dev)
            if raw_angles is None else as_f32(raw_angles, dev).clone())
+    if raw_lengths is not None:
+        raw = torch.cat([raw, as_f32(raw_lengths, dev)])
     opt = optimizer or FoldingOptimizer(total_steps=iterations)
     hist = []
     for _ in range(iterations):
-        total, _, _, g = folding_value_and_grad(nodes, edges, fields, solver_config, dt,
-                                                num_steps, stability_steps, raw, target, cfg,
-                                                checkpoint_every)
-        hist.append(float(total.item()))
+        out = folding_value_and_grad(nodes, edges, fields, solver_config, dt, num_steps,
+                                     stability_steps, raw[:nb], target, cfg, checkpoint_every,
+                                     None if raw_lengths is None else raw[nb:])
+        hist.append(float(out[0].item()))
+        g = out[3] if raw_lengths is None else torch.cat([out[3], out[4]])
         raw = opt.update(raw, g)
-    return raw, hist
+    return raw[:nb], (None if raw_lengths is None else raw[nb:]), hist

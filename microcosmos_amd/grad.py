@@ -11,7 +11,8 @@ adjoint kernel backwards through it, so the device memory of a gradient is
 
 Scope: the step without the fluid (PBD or stable Cosserat constraint,
 ``enable_fluid=False``, with or without the steric term) with the bending
-rest angles as the parameters.  The steric term couples every filament each
+rest angles (and, through ``backward_params``, the rest lengths) as the
+parameters.  The steric term couples every filament each
 step, so with it every step's state is kept (``checkpoint_every`` is 1) and
 each reverse step recomputes that step's steric field.
 """
@@ -146,6 +147,28 @@ class RodAdjoint:
             int(steps.shape[0]), steps.ctypes.data, ptr(bars), ptr(vb), ptr(rg), ptr(g_x),
             ptr(g_v), ptr(g_t), stream_handle(d)))
         return rg, g_x, g_v, g_t
+
+    def backward_params(self, frame_bars: dict, velocity_bar: torch.Tensor = None):
+        """``backward_frames`` that also returns the gradient with respect to
+        the rest lengths (mc_rod_adjoint_backward_params): (rest_angle_grad
+        (B,), rest_length_grad (E,), position0_grad, velocity0_grad,
+        theta0_grad)."""
+        lib = _lib.load()
+        d = self.dev
+        steps = np.ascontiguousarray(sorted(frame_bars), dtype=np.int64)
+        bars = torch.stack([as_f32(frame_bars[int(t)], d).reshape(-1, 2) for t in steps]).contiguous()
+        vb = as_f32(velocity_bar, d) if velocity_bar is not None else None
+        n = self.position.shape[0]
+        rg = torch.empty(self.num_hinges, dtype=torch.float32, device=d)
+        lg = torch.empty(self.num_edges, dtype=torch.float32, device=d)
+        g_x = torch.empty((n, 2), dtype=torch.float32, device=d)
+        g_v = torch.empty((n, 2), dtype=torch.float32, device=d)
+        g_t = torch.empty(self.num_edges, dtype=torch.float32, device=d)
+        check(lib.mc_rod_adjoint_backward_params(
+            self.plan.handle, ctypes.byref(self.params), ctypes.byref(self.ep), self.dt,
+            int(steps.shape[0]), steps.ctypes.data, ptr(bars), ptr(vb), ptr(rg), ptr(lg),
+            ptr(g_x), ptr(g_v), ptr(g_t), stream_handle(d)))
+        return rg, lg, g_x, g_v, g_t
 
     def backward(self, position_bar: torch.Tensor, velocity_bar: torch.Tensor = None):
         """Returns (rest_angle_grad (B,), position0_grad (N, 2),

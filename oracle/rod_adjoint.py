@@ -45,8 +45,8 @@ The minimum image, the angle wrap and the final periodic wrap have derivative
 Pinned by central finite differences of the float64 forward (``step_ref.step``)
 in ``tests/test_rod_adjoint_oracle.py``: per step (directional derivatives
 for x, v and theta, both solvers, steric on and off), the steric force's VJP
-alone, and whole rollouts (rest-angle gradients), including the folding
-experiment's default physics.
+alone, and whole rollouts (rest-angle and rest-length gradients), including
+the folding experiment's default physics and its objective over frames.
 """
 from __future__ import annotations
 
@@ -184,7 +184,9 @@ def _cosserat_cycles_vjp(s, p, ps, ths, pb, thb, rhob, deg_e):
       th1 = theta + delta / deg_e;  th2 = th1 + 0.05 wrap(atan2(r) - th1)
       F_e = k/l (r / l - (cos th2, sin th2));  h_n = w + sum k / l^2
       x' = x + (w (y - x) + sum_src F - sum_tgt F) / h
-    Returns (pb, thb, yb, rhob)."""
+    The rest length l enters F (dF/dl = -2 k r / l^3 + k / l^2 (cos, sin))
+    and h (dh_n/dl = -2 k / l^3 at both ends, dx'/dh = -(x' - x) / h).
+    Returns (pb, thb, yb, rhob, lb)."""
     src, tgt = s.pairs[:, 0], s.pairs[:, 1]
     bin_, bout = s.bending_pairs[:, 0], s.bending_pairs[:, 1]
     l = np.asarray(s.rest_lengths, np.float64)
@@ -195,10 +197,22 @@ def _cosserat_cycles_vjp(s, p, ps, ths, pb, thb, rhob, deg_e):
     np.add.at(h, src, k / l ** 2)
     np.add.at(h, tgt, k / l ** 2)
     yb = np.zeros_like(pb)
+    lb = np.zeros_like(l)
+    y = np.asarray(ps[0], np.float64)
     for c in reversed(range(p.cycles_per_step)):
         pos = np.asarray(ps[c], np.float64)
         th2 = np.asarray(ths[c], np.float64)
         Fb = pb[src] / h[src, None] - pb[tgt] / h[tgt, None]
+        # the cycle's update (x' - x) = num / h, from its input positions
+        rr = np.asarray(R.displacement(s.grid_shape, pos[tgt], pos[src]), np.float64)
+        d2 = np.stack([np.cos(th2), np.sin(th2)], axis=1)
+        F = (k / l)[:, None] * (rr / l[:, None] - d2)
+        num = w * (y - pos)
+        np.add.at(num, src, F)
+        np.add.at(num, tgt, -F)
+        hb = -(pb * num).sum(axis=1) / h ** 2
+        lb = lb + (-2.0 * k / l ** 3) * ((rr * Fb).sum(axis=1) + hb[src] + hb[tgt]) \
+            + (k / l ** 2) * (d2 * Fb).sum(axis=1)
         yb = yb + pb * (w / h)[:, None]
         pnew = pb * (1.0 - w / h)[:, None]
         rb = (k / l ** 2)[:, None] * Fb
@@ -217,7 +231,7 @@ def _cosserat_cycles_vjp(s, p, ps, ths, pb, thb, rhob, deg_e):
         np.add.at(thb, bout, diffb)
         np.add.at(thb, bin_, -diffb)
         pb = pnew
-    return pb, thb, yb, rhob
+    return pb, thb, yb, rhob, lb
 
 
 def _prologue_vjp(s, dt, p: R.Params, pb):
@@ -232,6 +246,13 @@ def _prologue_vjp(s, dt, p: R.Params, pb):
 def step_vjp(s: R.State, dt: float, p: R.Params, xb, vb, thb):
     """Adjoint of one step at state ``s``: returns (xb_prev, vb_prev,
     thb_prev, rhob) given the adjoints of the step's outputs."""
+    return step_vjp_params(s, dt, p, xb, vb, thb)[:4]
+
+
+def step_vjp_params(s: R.State, dt: float, p: R.Params, xb, vb, thb):
+    """``step_vjp`` plus the adjoint of the rest lengths: returns (xb_prev,
+    vb_prev, thb_prev, rhob, lb).  PBD: dF/dl0 = -k_st/2 ((1 - k_sh) rhat +
+    k_sh (cos th', sin th'))."""
     _, ps, ths = step_tape(s, dt, p)
     deg_n, deg_e = _degrees(s)
     src, tgt = s.pairs[:, 0], s.pairs[:, 1]
@@ -246,10 +267,11 @@ def step_vjp(s: R.State, dt: float, p: R.Params, xb, vb, thb):
     thb = np.array(thb, np.float64, copy=True)
     rhob = np.zeros(s.bending_pairs.shape[0])
     if p.solver_type == "cosserat":
-        pb, thb, yb, rhob = _cosserat_cycles_vjp(s, p, ps, ths, pb, thb, rhob, deg_e)
+        pb, thb, yb, rhob, lb = _cosserat_cycles_vjp(s, p, ps, ths, pb, thb, rhob, deg_e)
         pb = pb + yb  # y is the prologue's prediction (simulate.py:71-72)
         pb = _prologue_vjp(s, dt, p, pb)
-        return pb + xb_direct, pb * float(p.damping) * dt, thb, rhob
+        return pb + xb_direct, pb * float(p.damping) * dt, thb, rhob, lb
+    lb = np.zeros_like(l0)
     for c in reversed(range(p.cycles_per_step)):
         pos = np.asarray(ps[c], np.float64)
         th = np.asarray(ths[c], np.float64)
@@ -265,6 +287,8 @@ def step_vjp(s: R.State, dt: float, p: R.Params, xb, vb, thb):
         rb = 0.5 * kst * ((1.0 - ksh) * (a * Fb + (l0 / Ls)[:, None] * proj) + ksh * Fb)
         np.add.at(pb, tgt, rb)
         np.add.at(pb, src, -rb)
+        lb = lb - 0.5 * kst * ((1.0 - ksh) * (rh * Fb).sum(axis=1)
+                               + ksh * (np.cos(th) * Fb[:, 0] + np.sin(th) * Fb[:, 1]))
         thb = thb + (-0.5 * kst * ksh * l0) * (-np.sin(th) * Fb[:, 0] + np.cos(th) * Fb[:, 1])
         # angle pass
         corrb = OMEGA * (thb[bin_] / deg_e[bin_] - thb[bout] / deg_e[bout])
@@ -273,7 +297,7 @@ def step_vjp(s: R.State, dt: float, p: R.Params, xb, vb, thb):
         np.add.at(thb, bout, diffb)
         np.add.at(thb, bin_, -diffb)
     pb = _prologue_vjp(s, dt, p, pb)
-    return pb + xb_direct, pb * float(p.damping) * dt, thb, rhob
+    return pb + xb_direct, pb * float(p.damping) * dt, thb, rhob, lb
 
 
 def msd_loss(position, target, grid_shape):
@@ -333,12 +357,13 @@ def rollout_frames_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, fra
 
 
 def rollout_objective_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, steps,
-                           objective, dtype=np.float64):
+                           objective, dtype=np.float64, lengths=False):
     """An objective coupling several frames: ``objective({t: position after
     t steps for t in steps})`` returns (loss, {t: d loss / d that position}),
     as the folding loss's stability term compares frames with each other.
     Returns (loss, its gradient with respect to the rest angles, the frame
-    positions)."""
+    positions); with ``lengths``, (loss, rest-angle gradient, rest-length
+    gradient, frame positions)."""
     states = [s0.astype(dtype)]
     for _ in range(num_steps):
         states.append(R.step(states[-1], dt, p))
@@ -348,28 +373,41 @@ def rollout_objective_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, 
     vb = np.zeros_like(xb)
     thb = np.zeros(s0.theta.shape[0])
     rhob = np.zeros(s0.bending_pairs.shape[0])
+    lenb = np.zeros(s0.pairs.shape[0])
     for t in reversed(range(num_steps)):
         if t + 1 in bars:
             xb = xb + bars[t + 1]
-        xb, vb, thb, rb = step_vjp(states[t], dt, p, xb, vb, thb)
+        xb, vb, thb, rb, lb = step_vjp_params(states[t], dt, p, xb, vb, thb)
         rhob = rhob + rb
+        lenb = lenb + lb
+    if lengths:
+        return total, rhob, lenb, pos
     return total, rhob, pos
 
 
 def folding_raw_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, raw,
                      curvature_scale: float, target, eval_steps, stab_steps, anchor_step,
-                     smoothness_weight: float = 0.0, rest=None, dtype=np.float64, **objective_kw):
+                     smoothness_weight: float = 0.0, rest=None, dtype=np.float64,
+                     raw_lengths=None, distance_range=(0.1, 4.0), lengths=None, **objective_kw):
     """The folding experiment's loss for one initial state
     (experiments/filament_folding/experiment.py:739-745, 803-916): rest
     angles tanh(raw) * curvature_scale (or ``rest``, the same values computed
     elsewhere, so a float32 rollout can follow another's rounding), a rollout
     of ``num_steps`` steps scored by ``losses.folding_objective`` at the given
     frames, plus the smoothness term.  Returns (total, d total / d raw, frame
-    positions)."""
+    positions); with ``raw_lengths``, the rest lengths are lo + sigmoid(raw)
+    (hi - lo) over ``distance_range`` (experiment.py:741-742; or ``lengths``,
+    the same values computed elsewhere) and the result is (total, d / d raw,
+    d / d raw_lengths, frame positions)."""
     raw = np.asarray(raw, np.float64)
     th = np.tanh(raw)
     r = th * curvature_scale if rest is None else np.asarray(rest, np.float64)
     s = replace(s0, bending_rest_angles=r)
+    if raw_lengths is not None:
+        lo, hi = distance_range
+        sg = 1.0 / (1.0 + np.exp(-np.asarray(raw_lengths, np.float64)))
+        ln = lo + sg * (hi - lo) if lengths is None else np.asarray(lengths, np.float64)
+        s = replace(s, rest_lengths=ln)
     steps = sorted(set(eval_steps) | set(stab_steps) | ({anchor_step} - {None}))
 
     def objective(pos):
@@ -377,12 +415,16 @@ def folding_raw_grad(s0: R.State, dt: float, p: R.Params, num_steps: int, raw,
                                                  stab_steps, anchor_step, **objective_kw)
         return total, bars
 
-    total, g_rest, pos = rollout_objective_grad(s, dt, p, num_steps, steps, objective, dtype)
+    total, g_rest, g_len, pos = rollout_objective_grad(s, dt, p, num_steps, steps, objective,
+                                                       dtype, lengths=True)
     if smoothness_weight > 0.0:
         l, g = LS.smoothness(r)
         total += smoothness_weight * l
         g_rest = g_rest + smoothness_weight * g
-    return total, g_rest * curvature_scale * (1.0 - th * th), pos
+    g_raw = g_rest * curvature_scale * (1.0 - th * th)
+    if raw_lengths is None:
+        return total, g_raw, pos
+    return total, g_raw, g_len * (hi - lo) * sg * (1.0 - sg), pos
 
 
 def rollout_frames_loss(s0: R.State, dt: float, p: R.Params, num_steps: int, frame_losses,

@@ -401,8 +401,80 @@ def test_folding_objective_optimisation_lowers_the_loss(dev):
     lc = F.FoldingLossConfig(num_eval_frames=3, num_stability_frames=3, stability_anchor_offset=5)
     nodes, edges, fields = device_state(wl, dev)
     opt = FoldingOptimizer(lr=0.05, warmup_steps=0, total_steps=12)
-    _, hist = F.optimize_folding_objective(nodes, edges, fields, cfg, dt, 20, 10,
-                                           torch.as_tensor(cloud, device=dev), 12, lc,
-                                           optimizer=opt)
+    _, _, hist = F.optimize_folding_objective(nodes, edges, fields, cfg, dt, 20, 10,
+                                              torch.as_tensor(cloud, device=dev), 12, lc,
+                                              optimizer=opt)
     _say(f"losses {hist[0]:.4f} -> {hist[-1]:.4f}")
     assert np.all(np.isfinite(hist)) and hist[-1] < hist[0]
+
+
+@pytest.mark.parametrize("physics", ["pbd", "folding"])
+def test_rest_length_gradient_matches_restatement(dev, physics):
+    """mc_rod_adjoint_backward_params: the rest-length gradient of a loss over
+    frames after 10, 20 and 30 steps (rest lengths scaled by 0.8 .. 1.3)
+    against the restatement along the float32 trajectory; its rest-angle
+    gradient equals mc_rod_adjoint_backward_frames'."""
+    _say(f"rest-length gradient ({physics}): frames 10, 20, 30")
+    from microcosmos_amd.grad import RodAdjoint, min_image
+    wl, cfg, p, dt, _, _ = _objective_case(physics)
+    rng = np.random.default_rng(8)
+    wl.rest_lengths = (wl.rest_lengths * rng.uniform(0.8, 1.3, wl.rest_lengths.shape)).astype(
+        np.float32)
+    _, target = W.folding(steps=30)
+    tgts = {t: (target + rng.normal(0.0, 0.5, target.shape)).astype(np.float32) for t in (10, 20, 30)}
+    nodes, edges, fields = device_state(wl, dev)
+    ad = RodAdjoint(nodes, edges, fields, cfg, dt)
+    ad.forward(30, 25, frames=[10, 20])
+    bars = {}
+    for t, tg in tgts.items():
+        x = ad.frame_position(t)
+        d = min_image(tuple(wl.grid_shape), x, torch.as_tensor(tg, device=dev))
+        bars[t] = 2.0 * d / x.shape[0]
+    rg, lg = [v.cpu().numpy() for v in ad.backward_params(bars)[:2]]
+    rg_f = ad.backward_frames(bars)[0].cpu().numpy()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(rg, rg_f)
+    s = oracle_state(wl, np.float64)
+
+    def objective(pos):
+        out = {t: A.msd_loss(pos[t], tg, s.grid_shape) for t, tg in tgts.items()}
+        return sum(v[0] for v in out.values()), {t: v[1] for t, v in out.items()}
+
+    _, _, lo, _ = A.rollout_objective_grad(s, dt, p, 30, list(tgts), objective, dtype=np.float32,
+                                           lengths=True)
+    rel = np.linalg.norm(lg.astype(np.float64) - lo) / np.linalg.norm(lo)
+    _say(f"rest-length gradient rel-L2 {rel:.2e}")
+    assert rel <= 1e-4
+
+
+def test_folding_objective_with_rest_lengths(dev):
+    """The folding objective over the raw rest angles and the raw rest
+    lengths (the experiment's physics): both gradients against the
+    restatement along the float32 trajectory."""
+    _say("folding objective with rest lengths: 20 + 10 steps")
+    from microcosmos_amd import folding as F
+    wl, cfg, p, dt, raw, cloud = _objective_case("folding")
+    raw_l = np.random.default_rng(9).normal(-1.0, 0.3, wl.rest_lengths.shape).astype(np.float32)
+    lc = F.FoldingLossConfig(num_eval_frames=3, num_stability_frames=3,
+                             stability_anchor_offset=5)
+    nodes, edges, fields = device_state(wl, dev)
+    raw_d = torch.as_tensor(raw, device=dev)
+    raw_ld = torch.as_tensor(raw_l, device=dev)
+    total, _, _, g, gl = F.folding_value_and_grad(
+        nodes, edges, fields, cfg, dt, 20, 10, raw_d, torch.as_tensor(cloud, device=dev), lc,
+        checkpoint_every=25, raw_lengths=raw_ld)
+    rest = F.rest_angles(raw_d, lc).cpu().numpy()
+    lens = F.rest_lengths(raw_ld, lc).cpu().numpy()
+    g, gl = g.cpu().numpy().astype(np.float64), gl.cpu().numpy().astype(np.float64)
+    torch.cuda.synchronize()
+    s = oracle_state(wl, np.float64)
+    lo, go, glo, _ = A.folding_raw_grad(s, dt, p, 30, raw, lc.curvature_scale, cloud,
+                                        [10, 15, 20], [21, 25, 30], 25, rest=rest,
+                                        dtype=np.float32, raw_lengths=raw_l, lengths=lens,
+                                        loss_mode="dcd", use_pca=True, stability_weight=1.0,
+                                        max_dist_weight=1.0, alpha=0.1)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    rel_l = np.linalg.norm(gl - glo) / np.linalg.norm(glo)
+    _say(f"loss {float(total.item()):.6f} vs {lo:.6f}; rel-L2 angles {rel:.2e} lengths {rel_l:.2e}")
+    assert float(total.item()) == pytest.approx(lo, rel=1e-4)
+    assert rel <= 1e-4 and rel_l <= 1e-4

@@ -168,3 +168,47 @@ def test_folding_raw_angle_gradient_matches_fd():
              1e-6)
     assert np.isfinite(total) and np.max(np.abs(fd)) > 1e-3
     np.testing.assert_allclose(g, fd, atol=1e-6 * max(1.0, np.max(np.abs(fd))))
+
+
+@pytest.mark.parametrize("solver,steric", [("pbd", False), ("pbd", True), ("cosserat", False),
+                                           ("cosserat", True)])
+def test_rest_length_gradient_matches_fd(solver, steric):
+    """Gradient of a two-frame loss with respect to the rest lengths (the
+    folding experiment also optimises them, experiment.py:741-742) against
+    central finite differences of the float64 rollout."""
+    s, p, target = _case(solver=solver, steric=steric)
+    s = replace(s, rest_lengths=s.rest_lengths * np.random.default_rng(3).uniform(
+        0.8, 1.3, s.rest_lengths.shape))
+
+    def objective(pos):
+        l1, g1 = A.msd_loss(pos[4], target, s.grid_shape)
+        l2, g2 = A.msd_loss(pos[8], target, s.grid_shape)
+        return l1 + l2, {4: g1, 8: g2}
+
+    def loss(lengths):
+        q = replace(s, rest_lengths=lengths)
+        return A.rollout_objective_grad(q, 1.0, p, 8, [4, 8], objective)[0]
+
+    total, rg, lg, _ = A.rollout_objective_grad(s, 1.0, p, 8, [4, 8], objective, lengths=True)
+    fd = _fd(loss, np.asarray(s.rest_lengths, np.float64), 1e-6)
+    assert np.max(np.abs(fd)) > 1e-3
+    np.testing.assert_allclose(lg, fd, atol=2e-6 * max(1.0, np.max(np.abs(fd))))
+
+
+@pytest.mark.parametrize("solver", ["pbd", "cosserat"])
+def test_folding_raw_length_gradient_matches_fd(solver):
+    """The folding loss through the rest-length parameterisation lo +
+    sigmoid(raw) (hi - lo) (experiment.py:741-742): raw-length gradient
+    against central finite differences of the float64 rollout."""
+    s, p, _ = _case(solver=solver)
+    rng = np.random.default_rng(12)
+    raw = rng.normal(0.0, 0.4, s.bending_pairs.shape[0])
+    raw_l = rng.normal(-1.0, 0.3, s.pairs.shape[0])
+    tgt = s.position[:16] + rng.normal(0.0, 1.5, (16, 2))
+    kw = dict(eval_steps=[4, 6], stab_steps=[7, 8, 10], anchor_step=9, loss_mode="dcd",
+              use_pca=False, stability_weight=0.8, max_dist_weight=0.5, alpha=0.3)
+    total, g, gl, _ = A.folding_raw_grad(s, 1.0, p, 10, raw, 0.6, tgt, raw_lengths=raw_l, **kw)
+    fd = _fd(lambda r: A.folding_raw_grad(s, 1.0, p, 10, raw, 0.6, tgt, raw_lengths=r, **kw)[0],
+             raw_l, 1e-6)
+    assert np.max(np.abs(fd)) > 1e-3
+    np.testing.assert_allclose(gl, fd, atol=2e-6 * max(1.0, np.max(np.abs(fd))))
