@@ -261,6 +261,61 @@ def gradient_leg(dev, reps=5, scheme="pbd"):
             "finite": bool(ok)}
 
 
+def objective_leg(dev, batch=1, reps=3):
+    """The folding experiment's whole loss and its gradient
+    (experiments/filament_folding/experiment.py:803-923) with its default
+    settings: the default physics, 150 + 30 steps, DCD PCA-aligned over 4
+    evaluation frames, the stability term over 4 frames against the main and
+    anchor frames, the smooth max, and gradients with respect to the raw rest
+    angles (368) and raw rest lengths (376), both starting at zero as in the
+    reference; the target cloud is the folding workload's target curve (384
+    points).  ``batch`` initial states (seeds 3..) share the parameters and
+    go through one batched rollout and adjoint (optimization.batch_size = 4).
+    One evaluation = loss + gradient of the whole batch."""
+    import numpy as np
+    import torch
+    from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID
+    from microcosmos_amd import folding as F
+    from microcosmos_amd import workloads as W
+    from tests.helpers import device_state
+    dt = W.FOLDING_DT
+    cfg = replace(COSSERAT_SCHEME_NO_FLUID, dt=dt, **W.FOLDING_COSSERAT)
+    wls = [W.folding(seed=3 + k, steps=W.FOLDING_STEPS) for k in range(batch)]
+    cloud = torch.as_tensor(wls[0][1], device=dev)
+    lc = F.FoldingLossConfig()
+    nb = int(wls[0][0].bending_pairs.shape[0])
+    ne = int(wls[0][0].pairs.shape[0])
+    raw = torch.zeros(nb, dtype=torch.float32, device=dev)
+    raw_l = torch.zeros(ne, dtype=torch.float32, device=dev)
+
+    def one():
+        inst = [device_state(wl, dev) for wl, _ in wls]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if batch == 1:
+            out = F.folding_value_and_grad(*inst[0], cfg, dt, W.FOLDING_STEPS, 30, raw, cloud, lc,
+                                           raw_lengths=raw_l)
+        else:
+            out = F.batch_folding_value_and_grad(inst, cfg, dt, W.FOLDING_STEPS, 30, raw, cloud,
+                                                 lc, raw_lengths=raw_l)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, float(out[0].item()), \
+            bool(torch.isfinite(out[3]).all().item() and torch.isfinite(out[4]).all().item())
+
+    one()
+    ts = []
+    for _ in range(reps):
+        t, loss, ok = one()
+        ts.append(t)
+    ms = float(np.median(ts)) * 1e3
+    return {"workload": wls[0][0].name, "batch": batch, "steps": W.FOLDING_STEPS + 30,
+            "eval_frames": lc.num_eval_frames, "stability_frames": lc.num_stability_frames,
+            "loss_mode": lc.loss_mode, "parameters": nb + ne, "ms_per_evaluation": round(ms, 3),
+            "evaluations_per_s": round(1e3 / ms, 2),
+            "instance_gradients_per_s": round(batch * 1e3 / ms, 2), "loss": round(loss, 4),
+            "finite": ok}
+
+
 def gradient_batch_leg(dev, batch=4, reps=5):
     """The folding experiment's batched objective (experiment.py:915-923
     vmaps the loss over optimization.batch_size = 4,
@@ -331,7 +386,8 @@ def main():
     ap.add_argument("--only-serve", action="store_true",
                     help="run only the serving leg and print its JSON")
     ap.add_argument("--no-grad", action="store_true")
-    ap.add_argument("--only-grad", choices=["pbd", "cosserat", "folding", "folding-batch4"],
+    ap.add_argument("--only-grad", choices=["pbd", "cosserat", "folding", "folding-batch4",
+                                            "objective", "objective-batch4"],
                     help="run only the gradient leg of this scheme and print its JSON")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 9 x 1024 x 1024 float32 device copy (37.7 MB read + "
@@ -360,8 +416,12 @@ def main():
 
     dev = torch.device("cuda", 0)
     if args.only_grad:
-        leg = gradient_batch_leg(dev, 4) if args.only_grad == "folding-batch4" else \
-            gradient_leg(dev, scheme=args.only_grad)
+        if args.only_grad.startswith("objective"):
+            leg = objective_leg(dev, 4 if args.only_grad.endswith("batch4") else 1)
+        elif args.only_grad == "folding-batch4":
+            leg = gradient_batch_leg(dev, 4)
+        else:
+            leg = gradient_leg(dev, scheme=args.only_grad)
         print(json.dumps(leg), flush=True)
         return
     if args.only_serve:
@@ -407,6 +467,8 @@ def main():
         grad["stable_cosserat"] = gradient_leg(dev, scheme="cosserat")
         grad["folding_physics"] = gradient_leg(dev, scheme="folding")
         grad["folding_physics_batch4"] = gradient_batch_leg(dev, 4)
+        grad["folding_objective"] = objective_leg(dev, 1)
+        grad["folding_objective_batch4"] = objective_leg(dev, 4)
         print(f"[bench] gradient {grad}", file=sys.stderr, flush=True)
     served = None
     if not args.no_serve:

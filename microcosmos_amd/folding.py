@@ -210,6 +210,63 @@ def folding_value_and_grad(nodes: Nodes, edges: Edges, fields: Fields, solver_co
     return total, shape, stab, g_raw, g_len * (cfg.distance_max - cfg.distance_min) * sg * (1.0 - sg)
 
 
+def batch_folding_value_and_grad(instances, solver_config, dt: float, num_steps: int,
+                                 stability_steps: int, raw_angles: torch.Tensor,
+                                 target: torch.Tensor,
+                                 cfg: FoldingLossConfig = FoldingLossConfig(),
+                                 checkpoint_every: int = 50, raw_lengths: torch.Tensor = None):
+    """The experiment's batched loss (experiment.py:918-923: the mean over
+    ``optimization.batch_size`` initial states of the per-state loss, with
+    shared parameters): one batched rollout and adjoint
+    (``BatchRodAdjoint``) for every instance of ``instances`` ((Nodes, Edges,
+    Fields) of the same filament topology).  Returns (mean total, mean
+    shape, mean stability, d mean / d raw_angles[, d mean / d raw_lengths])
+    as ``folding_value_and_grad``."""
+    from .grad import BatchRodAdjoint
+    dev = instances[0][0].position.device
+    raw = as_f32(raw_angles, dev)
+    rest = rest_angles(raw, cfg)
+    raw_l = None if raw_lengths is None else as_f32(raw_lengths, dev)
+    inst = []
+    for nodes, edges, fields in instances:
+        e = replace(edges, bending_rest_angles=rest)
+        if raw_l is not None:
+            e = replace(e, rest_lengths=rest_lengths(raw_l, cfg))
+        inst.append((nodes, e, fields))
+    ad = BatchRodAdjoint(inst, solver_config, dt)
+    total_steps = int(num_steps) + int(stability_steps)
+    steps = objective_steps(num_steps, stability_steps, cfg)
+    ad.forward(total_steps, checkpoint_every, frames=steps)
+    pos = {t: ad.frame_positions(t) for t in steps}
+    tgt = as_f32(target, dev)
+    nb = len(instances)
+    outs = []
+    for i, (_, _, fields) in enumerate(inst):
+        outs.append(frame_objective({t: pos[t][i] for t in steps}, tgt,
+                                    tuple(fields.grid_shape), num_steps, stability_steps, cfg))
+    # the mean over the batch: each instance's cotangents scaled by 1 / nb
+    bars = {t: [o[3][t] / nb for o in outs] for t in steps}
+    grads = ad.backward_params(bars)
+    total = sum(o[0] for o in outs) / nb
+    shape = sum(o[1] for o in outs) / nb
+    stab = sum(o[2] for o in outs) / nb
+    g_rest = sum(g[0] for g in grads)
+    if cfg.smoothness_weight > 0.0 and rest.shape[0] > 1:
+        d = rest[1:] - rest[:-1]
+        total = total + cfg.smoothness_weight * (d * d).mean()
+        gd = cfg.smoothness_weight * 2.0 * d / d.shape[0]
+        g_rest = g_rest.clone()
+        g_rest[1:] += gd
+        g_rest[:-1] -= gd
+    th = torch.tanh(raw)
+    g_raw = g_rest * cfg.curvature_scale * (1.0 - th * th)
+    if raw_l is None:
+        return total, shape, stab, g_raw
+    sg = torch.sigmoid(raw_l)
+    g_len = sum(g[1] for g in grads)
+    return total, shape, stab, g_raw, g_len * (cfg.distance_max - cfg.distance_min) * sg * (1.0 - sg)
+
+
 def optimize_folding_objective(nodes: Nodes, edges: Edges, fields: Fields, solver_config,
                                dt: float, num_steps: int, stability_steps: int,
                                target: torch.Tensor, iterations: int,

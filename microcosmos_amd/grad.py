@@ -302,12 +302,17 @@ class BatchRodAdjoint:
         self.dt = float(dt)
         self.steps = 0
 
-    def forward(self, num_steps: int, checkpoint_every: int = 50):
+    def forward(self, num_steps: int, checkpoint_every: int = 50, frames=None):
         """Advances every instance ``num_steps`` steps; returns the list of
-        final (N_i, 2) positions."""
+        final (N_i, 2) positions.  ``frames`` as in ``RodAdjoint.forward``."""
         b = self.batch
         lib = _lib.load()
         st = b._state()
+        for t in frames or ():
+            if not 1 <= int(t) <= int(num_steps):
+                raise ValueError(f"frame {t} outside 1..{num_steps}")
+            if int(t) != int(num_steps):
+                checkpoint_every = math.gcd(int(checkpoint_every), int(t))
         ep = _lib.McEdgeParams(ptr(b.rest_lengths), ptr(b.rest_angles), ptr(b.stiffness))
         self._ep = ep
         check(lib.mc_rod_adjoint_forward(b.plan.handle, ctypes.byref(b.params), ctypes.byref(st),
@@ -328,6 +333,35 @@ class BatchRodAdjoint:
                                           ctypes.byref(self._ep), self.dt, ptr(pb), None, ptr(rg),
                                           None, None, None, stream_handle(b.dev)))
         return [rg[int(b.hinge_off[i]):int(b.hinge_off[i + 1])] for i in range(b.n)]
+
+    def frame_positions(self, step: int):
+        """Per instance, the positions (N_i, 2) after ``step`` steps of the
+        recorded rollout (a frame given to ``forward``, or the last step)."""
+        b = self.batch
+        if int(step) == self.steps:
+            x = b.position.view(-1, 2)
+        else:
+            x = torch.empty((int(b.node_off[-1]), 2), dtype=torch.float32, device=b.dev)
+            check(_lib.load().mc_rod_adjoint_state(b.plan.handle, int(step), ptr(x), None, None,
+                                                   stream_handle(b.dev)))
+        return [x[int(b.node_off[i]):int(b.node_off[i + 1])] for i in range(b.n)]
+
+    def backward_params(self, frame_bars: dict):
+        """``frame_bars`` maps a frame step to the per-instance cotangents
+        (list of (N_i, 2)).  Returns per instance (rest-angle gradient (B_i,),
+        rest-length gradient (E_i,))."""
+        b = self.batch
+        steps = np.ascontiguousarray(sorted(frame_bars), dtype=np.int64)
+        bars = torch.stack([torch.cat([as_f32(g, b.dev).reshape(-1) for g in frame_bars[int(t)]])
+                            for t in steps]).contiguous()
+        rg = torch.empty(int(b.hinge_off[-1]), dtype=torch.float32, device=b.dev)
+        lg = torch.empty(int(b.edge_off[-1]), dtype=torch.float32, device=b.dev)
+        check(_lib.load().mc_rod_adjoint_backward_params(
+            b.plan.handle, ctypes.byref(b.params), ctypes.byref(self._ep), self.dt,
+            int(steps.shape[0]), steps.ctypes.data, ptr(bars), None, ptr(rg), ptr(lg), None, None,
+            None, stream_handle(b.dev)))
+        return [(rg[int(b.hinge_off[i]):int(b.hinge_off[i + 1])],
+                 lg[int(b.edge_off[i]):int(b.edge_off[i + 1])]) for i in range(b.n)]
 
 
 def batch_value_and_grad(instances, solver_config, dt: float, num_steps: int, targets,

@@ -77,30 +77,27 @@ def pca_rotation_angle(particles: torch.Tensor, targets: torch.Tensor) -> float:
 def pca_aligned(loss_fn, particles: torch.Tensor, targets: torch.Tensor, **kw):
     """The *_pca_aligned wrappers (experiment.py:95-102, 131-136, 152-160):
     the loss of the centred particles rotated by the principal-axis angle and
-    by the angle + pi against the centred targets, the smaller one kept; the
-    angle carries no gradient, the rotation and the centring do."""
+    by the angle + pi against the centred targets, the smaller one kept (a
+    tie averages the two gradients, as jnp.minimum); the angle carries no
+    gradient, the rotation and the centring do.  Stays on the device: the
+    angle, its cosine and sine and the choice are tensors, so no step waits
+    for the host."""
     p = as_f32(particles, particles.device)
     t = as_f32(targets, targets.device)
-    a = pca_rotation_angle(p, t)
+    a = _principal_angle(t.double()) - _principal_angle(p.double())
     tc = t - t.mean(dim=0)
-    best = None
-    for ang in (a, a + math.pi):
-        co, si = math.cos(ang), math.sin(ang)
-        c = p - p.mean(dim=0)
+    c = p - p.mean(dim=0)
+    out = []
+    for k in (0, 1):
+        ang = a + k * math.pi
+        co, si = torch.cos(ang).float(), torch.sin(ang).float()
         q = torch.stack([c[:, 0] * co - c[:, 1] * si, c[:, 0] * si + c[:, 1] * co], dim=-1)
         l, gq = loss_fn(q, tc, **kw)
         gc = torch.stack([gq[:, 0] * co + gq[:, 1] * si, -gq[:, 0] * si + gq[:, 1] * co], dim=-1)
-        g = gc - gc.mean(dim=0)
-        if best is None:
-            best = (l, g)
-        else:
-            l0, g0 = best
-            lv0, lv1 = float(l0.item()), float(l.item())
-            if lv1 < lv0:
-                best = (l, g)
-            elif lv1 == lv0:
-                best = (l0, 0.5 * (g0 + g))
-    return best
+        out.append((l, gc - gc.mean(dim=0)))
+    (l0, g0), (l1, g1) = out
+    g = torch.where(l1 < l0, g1, torch.where(l0 < l1, g0, 0.5 * (g0 + g1)))
+    return torch.minimum(l0, l1), g
 
 
 def frame_stability(grid_shape, frames: torch.Tensor, main: torch.Tensor,

@@ -478,3 +478,33 @@ def test_folding_objective_with_rest_lengths(dev):
     _say(f"loss {float(total.item()):.6f} vs {lo:.6f}; rel-L2 angles {rel:.2e} lengths {rel_l:.2e}")
     assert float(total.item()) == pytest.approx(lo, rel=1e-4)
     assert rel <= 1e-4 and rel_l <= 1e-4
+
+
+def test_batched_folding_objective_equals_mean_of_each(dev):
+    """batch_folding_value_and_grad over 3 initial states (shared raw rest
+    angles and lengths, the experiment's physics): the mean of the losses
+    and gradients each state gets alone."""
+    _say("batched folding objective: 3 states, 20 + 10 steps")
+    from microcosmos_amd import folding as F
+    wl0, cfg, p, dt, raw, cloud = _objective_case("folding")
+    wls = [wl0] + [W.folding(seed=sd, steps=30)[0] for sd in (4, 5)]
+    raw_l = np.random.default_rng(9).normal(-1.0, 0.3, wl0.rest_lengths.shape).astype(np.float32)
+    lc = F.FoldingLossConfig(num_eval_frames=3, num_stability_frames=3,
+                             stability_anchor_offset=5, smoothness_weight=0.1)
+    raw_d = torch.as_tensor(raw, device=dev)
+    raw_ld = torch.as_tensor(raw_l, device=dev)
+    cloud_d = torch.as_tensor(cloud, device=dev)
+    insts = [device_state(w, dev) for w in wls]
+    tb, _, _, gb, glb = F.batch_folding_value_and_grad(insts, cfg, dt, 20, 10, raw_d, cloud_d, lc,
+                                                       checkpoint_every=25, raw_lengths=raw_ld)
+    singles = [F.folding_value_and_grad(*inst, cfg, dt, 20, 10, raw_d, cloud_d, lc,
+                                        checkpoint_every=25, raw_lengths=raw_ld) for inst in insts]
+    ts = np.mean([float(o[0].item()) for o in singles])
+    gs = np.mean([o[3].cpu().numpy() for o in singles], axis=0)
+    gls = np.mean([o[4].cpu().numpy() for o in singles], axis=0)
+    torch.cuda.synchronize()
+    rel = np.linalg.norm(gb.cpu().numpy() - gs) / np.linalg.norm(gs)
+    rel_l = np.linalg.norm(glb.cpu().numpy() - gls) / np.linalg.norm(gls)
+    _say(f"loss {float(tb.item()):.6f} vs {ts:.6f}; rel-L2 angles {rel:.2e} lengths {rel_l:.2e}")
+    assert float(tb.item()) == pytest.approx(ts, rel=1e-6)
+    assert rel <= 1e-5 and rel_l <= 1e-5
