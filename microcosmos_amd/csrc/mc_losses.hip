@@ -5,12 +5,16 @@
 // stability term over the frames after the shape is scored (:878-893).
 //
 // Three launches:
-//   k_nn         one thread per particle and per target: the nearest point of
+//   k_nn         NN_G lanes per particle and per target: the nearest point of
 //                the other cloud (squared distance, first index, number of
-//                tied minima), a pass over the other cloud staged in LDS;
+//                tied minima), each lane over every NN_G-th point of the
+//                other cloud staged in LDS, the lanes' results combined by
+//                shuffles (a combination that gives the sequential scan's
+//                answer: smaller wins, equal minima add their ties and keep
+//                the smaller index, the first NaN wins);
 //   k_loss       one workgroup: the density counts (DCD), the loss in a fixed
 //                order, and per particle the weight of its own term;
-//   k_loss_grad  one thread per particle: 2 w (p - t) over the targets tied at
+//   k_loss_grad  NN_G lanes per particle: 2 w (p - t) over the targets tied at
 //                its minimum, plus the targets whose tied minima include it
 //                (jax's min rule: a tied minimum's derivative is split
 //                equally), in target order.
@@ -24,6 +28,8 @@ namespace mc {
 namespace {
 
 constexpr int NN_THREADS = 256;
+constexpr int NN_G = 16;                    // lanes per point (a wave holds 4 points)
+constexpr int NN_PTS = NN_THREADS / NN_G;   // points per workgroup
 constexpr int LOSS_THREADS = 1024;
 
 struct LossWs {
@@ -53,14 +59,35 @@ __device__ __forceinline__ float sqdist(float ax, float ay, float bx, float by) 
   return dx * dx + dy * dy;
 }
 
+// Nearest-point partial results of two lanes (each over its own subset of
+// the other cloud), combined as one sequential scan over both subsets would
+// have left them: the first NaN wins (with one tie), a smaller minimum
+// wins, equal minima keep the smaller index and add their ties.
+__device__ __forceinline__ void nn_combine(float& b, int& a, int& t, float b1, int a1, int t1) {
+  const bool n0 = b != b, n1 = b1 != b1;
+  if (n0 && n1) {
+    a = min(a, a1);
+    t = 1;
+  } else if (n1 || (!n0 && b1 < b)) {
+    b = b1;
+    a = a1;
+    t = t1;
+  } else if (!n0 && b1 == b) {
+    a = min(a, a1);
+    t = t + t1;
+  }
+}
+
 // blocks [0, bp): particles against the targets; [bp, grid): targets against
-// the particles.  The other cloud is read in LDS tiles of NN_THREADS points.
+// the particles.  The other cloud is read in LDS tiles of NN_THREADS points;
+// lane r of a point's NN_G lanes takes tile entries r, r + NN_G, ...
 __global__ void __launch_bounds__(NN_THREADS) k_nn(const float2* __restrict__ p, int n,
                                                    const float2* __restrict__ t, int m, int bp,
                                                    LossWs w) {
   __shared__ float2 tile[NN_THREADS];
   const bool rows = (int)blockIdx.x < bp;
-  const int i = (rows ? blockIdx.x : blockIdx.x - bp) * NN_THREADS + threadIdx.x;
+  const int lane = threadIdx.x % NN_G;
+  const int i = (rows ? blockIdx.x : blockIdx.x - bp) * NN_PTS + threadIdx.x / NN_G;
   const float2* self = rows ? p : t;
   const float2* other = rows ? t : p;
   const int ns = rows ? n : m, no = rows ? m : n;
@@ -72,7 +99,7 @@ __global__ void __launch_bounds__(NN_THREADS) k_nn(const float2* __restrict__ p,
     if (j0 + (int)threadIdx.x < no) tile[threadIdx.x] = other[j0 + threadIdx.x];
     __syncthreads();
     const int cnt = min(NN_THREADS, no - j0);
-    for (int k = 0; k < cnt; ++k) {
+    for (int k = lane; k < cnt; k += NN_G) {
       const float2 b = tile[k];
       const float d = rows ? sqdist(a.x, a.y, b.x, b.y) : sqdist(b.x, b.y, a.x, a.y);
       if (d < best || (d != d && best == best)) {
@@ -84,7 +111,13 @@ __global__ void __launch_bounds__(NN_THREADS) k_nn(const float2* __restrict__ p,
       }
     }
   }
-  if (i < ns) {
+  for (int off = NN_G / 2; off > 0; off >>= 1) {
+    const float b1 = __shfl_xor(best, off, NN_G);
+    const int a1 = __shfl_xor(arg, off, NN_G);
+    const int t1 = __shfl_xor(ties, off, NN_G);
+    nn_combine(best, arg, ties, b1, a1, t1);
+  }
+  if (i < ns && lane == 0) {
     if (rows) {
       w.dmin_p[i] = best;
       w.arg_p[i] = arg;
@@ -177,7 +210,8 @@ __global__ void __launch_bounds__(NN_THREADS) k_loss_grad(int kind, const float2
                                                           int m, LossWs w,
                                                           float* __restrict__ grad) {
   __shared__ float2 tile[NN_THREADS];
-  const int i = blockIdx.x * NN_THREADS + threadIdx.x;
+  const int lane = threadIdx.x % NN_G;
+  const int i = blockIdx.x * NN_PTS + threadIdx.x / NN_G;
   const float2 a = i < n ? p[i] : make_float2(0.0f, 0.0f);
   const float own = i < n ? w.wown[i] / (float)w.tie_p[i] : 0.0f;
   const float best = i < n ? w.dmin_p[i] : 0.0f;
@@ -190,7 +224,7 @@ __global__ void __launch_bounds__(NN_THREADS) k_loss_grad(int kind, const float2
     __syncthreads();
     const int cnt = min(NN_THREADS, m - j0);
     if (i >= n) continue;
-    for (int k = 0; k < cnt; ++k) {
+    for (int k = lane; k < cnt; k += NN_G) {
       const int j = j0 + k;
       const float2 b = tile[k];
       const float d = sqdist(a.x, a.y, b.x, b.y);
@@ -206,7 +240,12 @@ __global__ void __launch_bounds__(NN_THREADS) k_loss_grad(int kind, const float2
       }
     }
   }
-  if (i < n) {
+  // the lanes' partial sums, in a fixed butterfly order
+  for (int off = NN_G / 2; off > 0; off >>= 1) {
+    gx = gx + __shfl_xor(gx, off, NN_G);
+    gy = gy + __shfl_xor(gy, off, NN_G);
+  }
+  if (i < n && lane == 0) {
     grad[2 * i] = 2.0f * gx;
     grad[2 * i + 1] = 2.0f * gy;
   }
@@ -284,7 +323,7 @@ hipError_t launch_point_loss(int kind, const float* particles, int n, const floa
                              float alpha, float* loss, float* grad, void* workspace,
                              hipStream_t s) {
   const LossWs w = loss_ws(workspace, n, m);
-  const int bp = (n + NN_THREADS - 1) / NN_THREADS, bt = (m + NN_THREADS - 1) / NN_THREADS;
+  const int bp = (n + NN_PTS - 1) / NN_PTS, bt = (m + NN_PTS - 1) / NN_PTS;
   const float2* p = reinterpret_cast<const float2*>(particles);
   const float2* t = reinterpret_cast<const float2*>(targets);
   hipLaunchKernelGGL(k_nn, dim3(bp + bt), dim3(NN_THREADS), 0, s, p, n, t, m, bp, w);
