@@ -29,23 +29,27 @@ sys.path.insert(0, str(ROOT))
 METRIC = "coupled-step MLUPS (% HBM peak); batched evals/s + p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
 LBM_BYTES_PER_CELL = 80  # read f 36 + write f 36 + write u* 8
+NODE_BYTES = 100  # per node and step: positions, velocities, debug_vector, theta, edge/hinge params
 
 
-def cpu_baseline(wl, steps):
-    """The numpy FP32 restatement timed on this host over `steps` steps (after
-    one warm-up step); returns MLUPS."""
+def cpu_baseline(wl, steps, runs=1):
+    """The numpy FP32 restatement timed on this host: `runs` runs of `steps`
+    steps, each after one warm-up step; returns (median MLUPS, list of run
+    seconds)."""
     import numpy as np
     from oracle import step_ref as R
     from tests.helpers import oracle_params, oracle_state
     p = oracle_params(wl)
-    s = oracle_state(wl)
-    s = R.step(replace(s, bending_rest_angles=wl.rest_angles_at(0)), wl.dt, p)
-    t0 = time.perf_counter()
-    for t in range(1, steps + 1):
-        s = R.step(replace(s, bending_rest_angles=wl.rest_angles_at(t)), wl.dt, p)
-    dt = time.perf_counter() - t0
+    secs = []
+    for _ in range(runs):
+        s = oracle_state(wl)
+        s = R.step(replace(s, bending_rest_angles=wl.rest_angles_at(0)), wl.dt, p)
+        t0 = time.perf_counter()
+        for t in range(1, steps + 1):
+            s = R.step(replace(s, bending_rest_angles=wl.rest_angles_at(t)), wl.dt, p)
+        secs.append(time.perf_counter() - t0)
     h, w = wl.grid_shape
-    return h * w * steps / dt / 1e6, dt
+    return h * w * steps / float(np.median(secs)) / 1e6, secs
 
 
 def batched_es(n_inst, steps, plain_sample, dev):
@@ -54,7 +58,7 @@ def batched_es(n_inst, steps, plain_sample, dev):
     from microcosmos_amd import PBD_SCHEME, Rollout
     from microcosmos_amd import workloads as W
     from microcosmos_amd.batch import Batch
-    from tests.helpers import device_state
+    from microcosmos_amd.workloads import device_state
     pop = W.es_population(n_inst, seed=2, steps=steps)
     cfg = replace(PBD_SCHEME, **pop[0].config_overrides)
 
@@ -72,32 +76,47 @@ def batched_es(n_inst, steps, plain_sample, dev):
     tb = time.perf_counter() - t0
     finite = bool(torch.isfinite(disp).all().item())
     cells = sum(w.grid_shape[0] * w.grid_shape[1] for w in pop)
-    # the plain per-instance loop on a sample of the same instances
-    sample = pop[:plain_sample]
-    for w in sample[:1]:  # warm the single-instance path
-        n, e, f = device_state(w, dev)
-        Rollout(n, e, f, cfg, w.dt, actuation=(w.act_amp, w.act_omega, w.act_phase)).run(16)
-    torch.cuda.synchronize()
+    # the plain per-instance loop on a sample of the same instances, stratified
+    # by grid side (the first instances of each side); every sample rollout
+    # gets its plan and graphs, and runs once untimed, before the clock starts
+    sides = sorted({w.grid_shape[0] for w in pop})
+    per = max(1, plain_sample // len(sides))
+    sample = [w for s in sides for w in [x for x in pop if x.grid_shape[0] == s][:per]]
     ros = []
     for w in sample:
         n, e, f = device_state(w, dev)
         ro = Rollout(n, e, f, cfg, w.dt, actuation=(w.act_amp, w.act_omega, w.act_phase))
-        ro.track()
+        tr = ro.track()
+        ro.run(steps)
+        tr.init(ro.position)
         ros.append(ro)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for ro in ros:
+    side_time = {s: [] for s in sides}
+    for w, ro in zip(sample, ros):
+        t0 = time.perf_counter()
         ro.run(steps)
-    torch.cuda.synchronize()
-    tp = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        side_time[w.grid_shape[0]].append(time.perf_counter() - t0)
+    tp = sum(sum(v) for v in side_time.values())
+    # the population's plain-loop time: each side's mean rollout time times
+    # the number of instances of that side
+    count = {s: sum(1 for w in pop if w.grid_shape[0] == s) for s in sides}
+    t_pop = sum(count[s] * sum(side_time[s]) / len(side_time[s]) for s in sides)
     print(f"[bench] batched {tb:.3f} s, plain sample {tp:.3f} s", file=sys.stderr, flush=True)
     ev_b = n_inst / tb
-    ev_p = len(sample) / tp
+    ev_p = n_inst / t_pop
     return {"workload": f"es-population-{n_inst}", "instances": n_inst, "rollout_steps": steps,
             "lattice_cells": cells, "evals_per_s": round(ev_b, 2),
             "instance_steps_per_s": round(n_inst * steps / tb, 1),
             "mlups": round(cells * steps / tb / 1e6, 1),
-            "plain_loop": {"sample_instances": len(sample), "evals_per_s": round(ev_p, 2)},
+            "plain_loop": {"sample_instances": len(sample),
+                           "sample": f"{per} instances of each side {sides}, each timed over "
+                                     f"{steps} graph-replayed steps after an untimed run of the "
+                                     f"same length; population time = sum over sides of "
+                                     f"count x mean",
+                           "mean_rollout_s": {str(s): round(sum(v) / len(v), 5)
+                                              for s, v in side_time.items()},
+                           "evals_per_s": round(ev_p, 2)},
             "vs_plain_loop": round(ev_b / ev_p, 2), "finite": finite,
             "p99_latency": None}
 
@@ -111,7 +130,7 @@ def serving(n_req, dev, lanes, chunk, max_batch, plain=True):
     from microcosmos_amd import PBD_SCHEME, Rollout
     from microcosmos_amd import workloads as W
     from microcosmos_amd.serve import Request, Server, fifo_report, latency_report
-    from tests.helpers import device_state
+    from microcosmos_amd.workloads import device_state
     trace = W.serving_trace(n_req, seed=4)
     first = next(it for it in trace if not it.gradient)  # the rollout requests' settings
     cfg = replace(PBD_SCHEME, **first.workload.config_overrides)
@@ -218,7 +237,7 @@ def gradient_leg(dev, reps=5, scheme="pbd"):
     from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID, PBD_SCHEME_NO_FLUID
     from microcosmos_amd import workloads as W
     from microcosmos_amd.grad import RodAdjoint, min_image
-    from tests.helpers import device_state
+    from microcosmos_amd.workloads import device_state
     if scheme == "folding":
         steps, every, dt = W.FOLDING_STEPS, 1, W.FOLDING_DT
         wl, tgt = W.folding(steps=steps)
@@ -264,8 +283,8 @@ def gradient_leg(dev, reps=5, scheme="pbd"):
 def objective_leg(dev, batch=1, reps=3):
     """The folding experiment's whole loss and its gradient
     (experiments/filament_folding/experiment.py:803-923) with its default
-    settings: the default physics, 150 + 30 steps, DCD PCA-aligned over 4
-    evaluation frames, the stability term over 4 frames against the main and
+    settings (filament_folding.yaml): the default physics, 150 + 30 steps, DCD
+    centred without PCA rotation over 4 evaluation frames, the stability term over 4 frames against the main and
     anchor frames, the smooth max, and gradients with respect to the raw rest
     angles (368) and raw rest lengths (376), both starting at zero as in the
     reference; the target cloud is the folding workload's target curve (384
@@ -277,7 +296,7 @@ def objective_leg(dev, batch=1, reps=3):
     from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID
     from microcosmos_amd import folding as F
     from microcosmos_amd import workloads as W
-    from tests.helpers import device_state
+    from microcosmos_amd.workloads import device_state
     dt = W.FOLDING_DT
     cfg = replace(COSSERAT_SCHEME_NO_FLUID, dt=dt, **W.FOLDING_COSSERAT)
     wls = [W.folding(seed=3 + k, steps=W.FOLDING_STEPS) for k in range(batch)]
@@ -310,7 +329,8 @@ def objective_leg(dev, batch=1, reps=3):
     ms = float(np.median(ts)) * 1e3
     return {"workload": wls[0][0].name, "batch": batch, "steps": W.FOLDING_STEPS + 30,
             "eval_frames": lc.num_eval_frames, "stability_frames": lc.num_stability_frames,
-            "loss_mode": lc.loss_mode, "parameters": nb + ne, "ms_per_evaluation": round(ms, 3),
+            "loss_mode": lc.loss_mode, "pca_rotation": lc.use_pca_rotation,
+            "parameters": nb + ne, "ms_per_evaluation": round(ms, 3),
             "evaluations_per_s": round(1e3 / ms, 2),
             "instance_gradients_per_s": round(batch * 1e3 / ms, 2), "loss": round(loss, 4),
             "finite": ok}
@@ -328,7 +348,7 @@ def gradient_batch_leg(dev, batch=4, reps=5):
     from microcosmos_amd import COSSERAT_SCHEME_NO_FLUID
     from microcosmos_amd import workloads as W
     from microcosmos_amd.grad import BatchRodAdjoint, min_image
-    from tests.helpers import device_state
+    from microcosmos_amd.workloads import device_state
     steps, dt = W.FOLDING_STEPS, W.FOLDING_DT
     cfg = replace(COSSERAT_SCHEME_NO_FLUID, dt=dt, **W.FOLDING_COSSERAT)
     wls = [W.folding(seed=3 + k, steps=steps) for k in range(batch)]
@@ -370,13 +390,15 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-c0-steps", type=int, default=1000,
+                    help="steps of the 128^2 plain case on the CPU restatement (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=500)
-    ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--plain-sample", type=int, default=12)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-lanes", type=int, default=2)
@@ -412,7 +434,7 @@ def main():
     import torch
     from microcosmos_amd import PBD_SCHEME, Rollout
     from microcosmos_amd import workloads as W
-    from tests.helpers import device_state
+    from microcosmos_amd.workloads import device_state
 
     dev = torch.device("cuda", 0)
     if args.only_grad:
@@ -435,12 +457,18 @@ def main():
                  actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
     use_graph = not args.no_graph
     ro.run(args.warmup, use_graph=use_graph)
+    # one untimed run of the timed length instantiates the graphs the timed
+    # run replays (its bulk graph and its remainder graph)
+    ro.run(args.steps, use_graph=use_graph)
     torch.cuda.synchronize()
-    print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
+    ro.check()
+    print(f"[bench] warmup {args.warmup} (+{args.steps} graph instantiation) done",
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     ro.run(args.steps, use_graph=use_graph)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    ro.check()
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
     finite = bool(torch.isfinite(ro.f_grid).all().item() and torch.isfinite(ro.position).all().item())
     h, w = wl.grid_shape
@@ -451,13 +479,21 @@ def main():
     torch.cuda.synchronize()
     lbm_bytes = LBM_BYTES_PER_CELL * h * w
     achieved = lbm_bytes / (lbm_ms * 1e-3) / 1e9
+    # the whole step against the same roof: (80 B per cell + 100 B per node)
+    # per step over the timed wall clock
+    step_bytes = LBM_BYTES_PER_CELL * h * w + NODE_BYTES * wl.num_nodes
+    step_gbs = step_bytes * args.steps / elapsed / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": profiled_lbm_traffic(h, w),
-                "traffic_source": "profiles/coupled_step_1024x1024_256x64.md (rocprofv3 --pmc "
+                "traffic_source": "committed profile, not this run: "
+                                  "profiles/coupled_step_1024x1024_256x64.md (rocprofv3 --pmc "
                                   "FETCH_SIZE + WRITE_SIZE, plain-copy calibrated)",
                 "kernel": "k_lbm", "avg_kernel_ms": round(lbm_ms, 5),
-                "bytes_per_launch": lbm_bytes}
+                "bytes_per_launch": lbm_bytes,
+                "step": {"bytes_per_step": step_bytes, "achieved": round(step_gbs, 1),
+                         "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                         "basis": "(80 B/cell + 100 B/node) x steps / timed wall clock"}}
     batched = None
     if not args.no_batch:
         batched = batched_es(args.batch_instances, args.batch_steps, args.plain_sample, dev)
@@ -479,11 +515,20 @@ def main():
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        v, t = cpu_baseline(wl, args.cpu_steps)
+        v, secs = cpu_baseline(wl, args.cpu_steps, runs=3)
         cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same 1024^2 256x64 workload after one "
-                         f"warm-up step, numpy FP32 restatement ({t:.1f} s)",
-               "host_cpus": len(os.sched_getaffinity(0))}
+               "sample": f"median of 3 runs of {args.cpu_steps} steps of the same 1024^2 256x64 "
+                         f"workload, each after one warm-up step, numpy FP32 restatement "
+                         f"(runs {', '.join(f'{t:.2f}' for t in secs)} s)",
+               "host_cpus": len(os.sched_getaffinity(0)),
+               "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+        if args.cpu_c0_steps > 0:
+            wl0 = W.plain_128()
+            v0, s0 = cpu_baseline(wl0, args.cpu_c0_steps, runs=1)
+            cpu["plain_128"] = {"value": round(v0, 3), "unit": "MLUPS", "steps": args.cpu_c0_steps,
+                                "seconds": round(s0[0], 2),
+                                "coupled_steps_per_s": round(args.cpu_c0_steps / s0[0], 2),
+                                "sample": "the 128^2 32-node plain case, numpy FP32 restatement"}
     out = {"metric": METRIC, "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",

@@ -39,12 +39,24 @@ typedef enum mc_status {
 
 typedef enum mc_solver { MC_SOLVER_PBD = 0, MC_SOLVER_COSSERAT = 1 } mc_solver;
 
+/* Which rod constraint the step runs (ConstraintSolverConfig.bending_constraint,
+ * config.py:12, called at simulate.py:74-77).  The reference runs whatever
+ * bending_constraint is and uses solver_type only to decide whether the
+ * inertial target y = debug_vector is set to the prediction (simulate.py:
+ * 71-72); a Cosserat constraint under solver_type "pbd" therefore sees y = 0.
+ * MC_CONSTRAINT_AUTO (0, what a zero-filled struct holds) follows solver_type. */
+typedef enum mc_constraint {
+  MC_CONSTRAINT_AUTO = 0,
+  MC_CONSTRAINT_PBD = 1,       /* pbd_rod_constraint (pbd.py:14-91) */
+  MC_CONSTRAINT_COSSERAT = 2   /* stable_cosserat_constraint (stable_cosserat.py:13-119) */
+} mc_constraint;
+
 /* Every numeric field of ConstraintSolverConfig (solver/config.py:9-71), in
  * double precision: constants derived from them (omega+, 1/dt^2, 2 sigma^2, ...)
  * are computed in double and rounded to float32 the way the reference's
  * Python-float arithmetic is. */
 typedef struct mc_params {
-  int32_t solver_type;            /* mc_solver (config.py:28) */
+  int32_t solver_type;            /* mc_solver (config.py:28): sets the Cosserat target */
   int32_t cycles_per_step;        /* config.py:15 */
   double stiffness_stretch;       /* config.py:16 */
   double stiffness_shear;         /* config.py:17 */
@@ -65,6 +77,7 @@ typedef struct mc_params {
   double steric_sigma;            /* config.py:65 */
   int32_t steric_neighbor_skip;   /* config.py:68 */
   double steric_scatter_value;    /* config.py:71 */
+  int32_t bending_constraint;     /* mc_constraint (config.py:12) */
 } mc_params;
 
 /* Device pointers of one state.  `in` arrays are read, `out` arrays written.
@@ -142,6 +155,19 @@ mc_status mc_plan_destroy(mc_plan* plan);
  * the reference (jit traces are freed by the JAX runtime); it lets a request
  * server retire plans while other requests keep running. */
 mc_status mc_plan_destroy_after(mc_plan* plan, void* stream);
+/* Waits for `stream`, then reports whether the plan's work so far completed
+ * as specified.  The wide immersed-boundary solve (clusters of more than 768
+ * nodes) synchronises its workgroups with grid barriers whose waits are
+ * bounded; if a wait gives up (its workgroups were not all resident, e.g.
+ * under heavy concurrent use of the device), the steps' node velocities and
+ * IB forces are invalid.  That is returned as MC_ERR_HIP, here and by the
+ * next mc_step / mc_rollout / mc_batch_rollout / mc_batch_advance /
+ * mc_ib_lbm_step / mc_track_read / mc_plan_destroy_after on the plan, once,
+ * and the plan's IB state is restored so later calls compute normally.  No
+ * counterpart in the reference (XLA has no device-side failure mode here).
+ * MC_IB_WIDE_POLL_LIMIT (environment, read at plan creation) sets the poll
+ * bound of those waits (default 2^22). */
+mc_status mc_plan_status(mc_plan* plan, void* stream);
 /* Device bytes held by the plan (topology + workspace). */
 mc_status mc_plan_workspace_bytes(const mc_plan* plan, size_t* bytes);
 

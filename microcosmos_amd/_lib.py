@@ -34,7 +34,8 @@ class McParams(ctypes.Structure):
                 ("node_mass", c_double), ("lambda_trt", c_double),
                 ("enable_force_clamp", c_int32), ("enable_steric", c_int32),
                 ("steric_strength", c_double), ("steric_sigma", c_double),
-                ("steric_neighbor_skip", c_int32), ("steric_scatter_value", c_double)]
+                ("steric_neighbor_skip", c_int32), ("steric_scatter_value", c_double),
+                ("bending_constraint", c_int32)]
 
 
 class McState(ctypes.Structure):
@@ -60,6 +61,7 @@ SIGNATURES = {
                                  c_void_p, c_int32, c_void_p, c_int32, c_int32]),
     "mc_plan_destroy": (c_int32, [c_void_p]),
     "mc_plan_destroy_after": (c_int32, [c_void_p, c_void_p]),
+    "mc_plan_status": (c_int32, [c_void_p, c_void_p]),
     "mc_plan_workspace_bytes": (c_int32, [c_void_p, POINTER(c_size_t)]),
     "mc_step": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState), POINTER(McState),
                           POINTER(McEdgeParams), c_double, c_void_p]),

@@ -38,8 +38,17 @@ def as_f32(t: torch.Tensor, device) -> torch.Tensor:
     return t
 
 
+CONSTRAINT_AUTO, CONSTRAINT_PBD, CONSTRAINT_COSSERAT = 0, 1, 2  # mc_constraint
+
+
 def params_from_config(cfg) -> McParams:
-    """ConstraintSolverConfig -> mc_params (solver/config.py:9-71)."""
+    """ConstraintSolverConfig -> mc_params (solver/config.py:9-71).
+
+    The rod kernel follows ``bending_constraint``, which the reference calls
+    whatever ``solver_type`` says (simulate.py:74-77); ``solver_type`` only
+    decides whether the Cosserat inertial target y = debug_vector is set to
+    the prediction (simulate.py:71-72), so a Cosserat constraint under
+    solver_type "pbd" sees y = 0, as it does in the reference."""
     st = getattr(cfg, "solver_type", "pbd")
     if st not in ("pbd", "cosserat"):
         raise ValueError(f"solver_type {st!r}")
@@ -50,9 +59,13 @@ def params_from_config(cfg) -> McParams:
         raise NotImplementedError(
             "a custom bending_constraint callable cannot run inside the fused device step; "
             "use pbd_rod_constraint or stable_cosserat_constraint")
-    solver = 1 if st == "cosserat" else 0
     p = McParams()
-    p.solver_type = solver
+    p.solver_type = 1 if st == "cosserat" else 0
+    if bc is None:
+        p.bending_constraint = CONSTRAINT_AUTO
+    else:
+        p.bending_constraint = CONSTRAINT_COSSERAT if bc is stable_cosserat_constraint \
+            else CONSTRAINT_PBD
     p.cycles_per_step = int(cfg.cycles_per_step)
     p.stiffness_stretch = float(cfg.stiffness_stretch)
     p.stiffness_shear = float(cfg.stiffness_shear)
@@ -109,6 +122,11 @@ class Plan:
         b = ctypes.c_size_t()
         check(self._lib.mc_plan_workspace_bytes(self.handle, ctypes.byref(b)))
         return int(b.value)
+
+    def status(self, stream: int):
+        """Waits for ``stream`` and raises MicrocosmosError if the plan's work
+        did not complete as specified (mc_plan_status)."""
+        check(self._lib.mc_plan_status(self.handle, stream))
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
@@ -193,6 +211,7 @@ def rod_iterations(carry, config, grid_shape, cycles: int, solver: str):
     plan = plan_for(N, edges.pairs, edges.bending_pairs, grid_shape, grid_shape, dev)
     p = params_from_config(config)
     p.solver_type = 1 if solver == "cosserat" else 0
+    p.bending_constraint = CONSTRAINT_COSSERAT if solver == "cosserat" else CONSTRAINT_PBD
     pos = as_f32(nodes.position, dev).clone()
     th = as_f32(edges.theta, dev).clone()
     y = as_f32(nodes.debug_vector, dev) if solver == "cosserat" else None

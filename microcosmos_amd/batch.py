@@ -59,6 +59,11 @@ class BatchPlan:
         check(self._lib.mc_plan_workspace_bytes(self.handle, ctypes.byref(b)))
         return int(b.value)
 
+    def status(self, stream: int):
+        """Waits for ``stream`` and raises MicrocosmosError if the plan's work
+        did not complete as specified (mc_plan_status)."""
+        check(self._lib.mc_plan_status(self.handle, stream))
+
     def release(self, stream: int):
         """Frees the plan after the work enqueued on ``stream`` (a hipStream_t
         handle) has finished, without synchronising other streams."""
@@ -183,6 +188,11 @@ class Batch:
                                    t0.ctypes.data, self.dt, int(num_steps), ptr(displacement),
                                    self._stream()))
         self.t += int(num_steps)
+
+    def check(self):
+        """Waits for this batch's stream and raises MicrocosmosError if its
+        steps did not complete as specified (mc_plan_status)."""
+        self.plan.status(self._stream())
 
     def close(self):
         """Releases the plan once this batch's stream has finished its work."""

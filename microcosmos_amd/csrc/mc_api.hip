@@ -168,9 +168,11 @@ struct mc_plan {
   bool filter_fft = false;      // some shape's table holds a transfer function
   hipStream_t cap_stream = nullptr;
   // the instantiated S-step rollout graph of the last mc_rollout call, reused
-  // while its key matches; `graph_done` follows its last launch
-  GraphKey graph_key;
-  hipGraphExec_t graph_exec = nullptr;
+  // while its key matches: slot 0 the ROLLOUT_GRAPH_STEPS-step graph
+  // replayed for the bulk of a rollout, slot 1 the graph of its remainder;
+  // `graph_done` follows the last launch of either
+  GraphKey graph_key[2];
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   hipEvent_t graph_done = nullptr;
   // per-instance actuation time offsets (mc_batch_advance), host mirror
   long long* t_inst = nullptr;
@@ -195,6 +197,7 @@ struct mc_plan {
   double adj_dt = 0.0;
   mc_params adj_q;
   mc_edge_params adj_ed;
+  int* host_err = nullptr;      // pinned: set by the wide IB solve when a barrier wait gave up
   TrackWork track{};            // displacement tracking (off: n_groups == 0)
   TrackWork inst_track{};       // per-instance tracker of mc_batch_rollout
 
@@ -238,7 +241,9 @@ namespace {
 
 // Frees everything the plan holds; no work that uses it may be pending.
 void release_plan(mc_plan* p) {
-  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->host_err) (void)hipHostFree(p->host_err);
+  for (hipGraphExec_t g : p->graph_exec)
+    if (g) (void)hipGraphExecDestroy(g);
   if (p->graph_done) (void)hipEventDestroy(p->graph_done);
   DevicePool* dp = p->pooled ? device_pool(p->dev) : nullptr;
   for (void* q : p->allocs) {
@@ -628,6 +633,8 @@ mc_status check_params(const mc_plan* p, const mc_params* q, bool need_fluid, bo
   if (!q) return fail(MC_ERR_INVALID, "params is NULL");
   if (q->solver_type != MC_SOLVER_PBD && q->solver_type != MC_SOLVER_COSSERAT)
     return fail(MC_ERR_UNSUPPORTED, "solver_type %d", q->solver_type);
+  if (q->bending_constraint < MC_CONSTRAINT_AUTO || q->bending_constraint > MC_CONSTRAINT_COSSERAT)
+    return fail(MC_ERR_UNSUPPORTED, "bending_constraint %d", q->bending_constraint);
   if (q->cycles_per_step < 0) return fail(MC_ERR_BAD_SHAPE, "cycles_per_step < 0");
   if (need_fluid && q->enable_fluid) {
     if (q->synthetic_node_width > 0.0)
@@ -661,7 +668,12 @@ Consts derive(const mc_plan* p, const mc_params* q, double dt) {
   c.steric_svs = (float)(q->steric_scatter_value / (s * s)); // forces.py:77
   c.fft_coef = (float)(-2.0 * (M_PI * M_PI) * (s * s));      // utils.py:47
   c.steric_nskip = q->steric_neighbor_skip;
-  c.solver = q->solver_type;
+  // the kernel follows bending_constraint (simulate.py:74-77); solver_type
+  // only decides whether the inertial target is set (simulate.py:71-72)
+  c.solver = q->bending_constraint == MC_CONSTRAINT_AUTO
+                 ? q->solver_type
+                 : (q->bending_constraint == MC_CONSTRAINT_COSSERAT ? 1 : 0);
+  c.cos_target = q->solver_type == MC_SOLVER_COSSERAT;
   c.cycles = q->cycles_per_step;
   c.k_st = (float)q->stiffness_stretch;
   c.k_sh = (float)q->stiffness_shear;
@@ -1039,6 +1051,25 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
     w.wide_scratch = ws;
     if ((st = p->alloc(&w.wide_count, hwf, "wide cell counters")) != MC_OK) return bail(st);
     if ((st = p->alloc(&w.wide_ctl, 1, "wide control")) != MC_OK) return bail(st);
+    // the barrier-timeout record: a pinned host flag the wide solve sets, and
+    // the poll bound of its barrier waits (MC_IB_WIDE_POLL_LIMIT overrides
+    // the default 2^22, e.g. to exercise the error path)
+    if (hipHostMalloc(reinterpret_cast<void**>(&p->host_err), sizeof(int)) != hipSuccess)
+      return bail(fail(MC_ERR_OOM, "pinned barrier-timeout flag"));
+    *p->host_err = 0;
+    int limit = 1 << 22;
+    if (const char* env = getenv("MC_IB_WIDE_POLL_LIMIT")) {
+      const long v = atol(env);
+      if (v >= 0 && v <= (1L << 30)) limit = (int)v;
+    }
+    int* dev_flag = nullptr;
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_flag), p->host_err, 0) != hipSuccess)
+      return bail(fail(MC_ERR_HIP, "device address of the barrier-timeout flag"));
+    WideCtl init{};
+    init.poll_limit = limit;
+    init.host_err = dev_flag;
+    if (hipMemcpy(w.wide_ctl, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(MC_ERR_HIP, "wide control upload"));
   }
   if ((st = p->alloc(&w.force, 2 * hwf, "force U")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.mask, (size_t)p->mask_words, "U mask")) != MC_OK) return bail(st);
@@ -1048,9 +1079,31 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
   return MC_OK;
 }
 
+// A wide IB solve of work enqueued earlier gave up at a grid barrier (its
+// workgroups were not all resident): report it once, and restore the idle IB
+// state on `s` ahead of the caller's next work (the device flag stays set,
+// and the wide solve does nothing but wrap positions, until then).
+mc_status check_async(mc_plan* p, hipStream_t s) {
+  if (!p->host_err || __atomic_load_n(p->host_err, __ATOMIC_ACQUIRE) == 0) return MC_OK;
+  __atomic_store_n(p->host_err, 0, __ATOMIC_RELEASE);
+  const hipError_t e = launch_ib_reset(p->ib, s);
+  if (e != hipSuccess) return fail(MC_ERR_HIP, "IB reset after a barrier timeout: %s", hipGetErrorString(e));
+  return fail(MC_ERR_HIP,
+              "immersed-boundary wide solve: a grid barrier wait gave up (its workgroups were not "
+              "all resident on the device); node velocities and IB forces of the steps enqueued "
+              "since the last successful call on this plan are invalid");
+}
+
 }  // namespace
 
 extern "C" {
+
+mc_status mc_plan_status(mc_plan* p, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  return check_async(p, (hipStream_t)stream);
+}
 
 mc_status mc_plan_create(mc_plan** out, int32_t grid_h, int32_t grid_w, int32_t fluid_h,
                          int32_t fluid_w, int32_t num_nodes, const int32_t* pairs_host,
@@ -1093,7 +1146,12 @@ mc_status mc_plan_destroy_after(mc_plan* p, void* stream) {
   if (e == hipSuccess) e = hipEventSynchronize(ev);
   (void)hipEventDestroy(ev);
   if (e != hipSuccess) return fail(MC_ERR_HIP, "waiting for the plan's stream: %s", hipGetErrorString(e));
+  const bool timed_out = p->host_err && __atomic_load_n(p->host_err, __ATOMIC_ACQUIRE) != 0;
   release_plan(p);
+  if (timed_out)
+    return fail(MC_ERR_HIP,
+                "immersed-boundary wide solve: a grid barrier wait gave up (its workgroups were "
+                "not all resident on the device); the plan's last results are invalid");
   return MC_OK;
 }
 
@@ -1115,6 +1173,7 @@ mc_status mc_step(mc_plan* p, const mc_params* q, const mc_state* in, const mc_s
     return fail(MC_ERR_INVALID, "f_grid of out must differ from in (streaming is out of place)");
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
   hipStream_t s = (hipStream_t)stream;
+  MCCHK(check_async(p, s));
   const Consts c = derive(p, q, dt);
   MCCHK(ensure_geo(p, q));
   if (q->enable_steric) MCCHK(ensure_kf(p, q->steric_sigma, s));
@@ -1126,11 +1185,11 @@ mc_status mc_step(mc_plan* p, const mc_params* q, const mc_state* in, const mc_s
 namespace {
 
 // Drops the cached rollout graph after its last launch has finished.
-mc_status drop_graph(mc_plan* p) {
-  if (!p->graph_exec) return MC_OK;
+mc_status drop_graph(mc_plan* p, int slot) {
+  if (!p->graph_exec[slot]) return MC_OK;
   if (p->graph_done) HIPCHK(hipEventSynchronize(p->graph_done));
-  HIPCHK(hipGraphExecDestroy(p->graph_exec));
-  p->graph_exec = nullptr;
+  HIPCHK(hipGraphExecDestroy(p->graph_exec[slot]));
+  p->graph_exec[slot] = nullptr;
   return MC_OK;
 }
 
@@ -1147,6 +1206,7 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
   if (act && p->B > 0 && (!act->amp || !act->omega || !act->phase))
     return fail(MC_ERR_INVALID, "actuation arrays are NULL");
+  MCCHK(check_async(p, (hipStream_t)stream));
   if (num_steps == 0) return MC_OK;
   hipStream_t us = (hipStream_t)stream;
   const Consts c = derive(p, q, dt);
@@ -1181,13 +1241,16 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
       MCCHK(record(k, us));
     }
   } else {
-    // one graph holds S steps (S even keeps the lattice parity); inside it
-    // the LBM pass of step k runs inside step k+1's rod launch, and the last
-    // step's LBM pass runs on its own.  The instantiated graph is kept and
-    // replayed by later calls with the same arguments; it is captured on a
-    // private stream and launched on the caller's.
-    const int64_t S = num_steps >= 8 ? 8 : (num_steps >= 2 ? 2 : 0);
-    if (S > 0) {
+    // The bulk of the rollout replays one graph of ROLLOUT_GRAPH_STEPS steps
+    // (even, so every replay starts on the same lattice buffer), the
+    // remainder one graph of its own length; inside a graph the LBM pass of
+    // step k runs inside step k+1's rod launch and the last step's LBM pass
+    // runs on its own.  Both instantiated graphs are kept and replayed by
+    // later calls with the same arguments; they are captured on a private
+    // stream and launched on the caller's.
+    const int64_t S = ROLLOUT_GRAPH_STEPS;
+    const int64_t bulk = (num_steps / S) * S, tail = num_steps - bulk;
+    auto graph_for = [&](int slot, int64_t len) -> mc_status {
       GraphKey key;
       memset(&key, 0, sizeof(key));
       key.q = *q;
@@ -1197,44 +1260,47 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
       key.ed = *ed;
       if (ap) key.act = A;
       key.has_act = ap != nullptr;
-      key.steps = S;
+      key.steps = len;
       key.track = p->track;
-      if (p->graph_exec && memcmp(&key, &p->graph_key, sizeof(key)) != 0) MCCHK(drop_graph(p));
-      if (!p->graph_exec) {
-        if (!p->cap_stream) HIPCHK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
-        if (!p->graph_done) HIPCHK(hipEventCreateWithFlags(&p->graph_done, hipEventDisableTiming));
-        hipStream_t cs = p->cap_stream;
-        hipGraph_t g;
-        mc_status r = MC_OK;
-        HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-        LbmArgs pend[2];
-        const LbmArgs* fuse = nullptr;
-        for (int64_t k = 0; k < S && r == MC_OK; ++k) {
-          const int par = (int)(k & 1);
-          LbmArgs* defer = (q->enable_fluid && p->rod_fuse && k + 1 < S) ? &pend[par] : nullptr;
-          r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, StepSync(), fuse,
-                           defer);
-          fuse = defer;
-        }
-        hipError_t ec = hipStreamEndCapture(cs, &g);
-        if (r != MC_OK) return r;
-        if (ec != hipSuccess) return fail(MC_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
-        hipGraphExec_t ge = nullptr;
-        ec = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ec != hipSuccess) return fail(MC_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ec));
-        p->graph_exec = ge;
-        p->graph_key = key;
+      if (p->graph_exec[slot] && memcmp(&key, &p->graph_key[slot], sizeof(key)) != 0)
+        MCCHK(drop_graph(p, slot));
+      if (p->graph_exec[slot]) return MC_OK;
+      if (!p->cap_stream) HIPCHK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+      if (!p->graph_done) HIPCHK(hipEventCreateWithFlags(&p->graph_done, hipEventDisableTiming));
+      hipStream_t cs = p->cap_stream;
+      hipGraph_t g;
+      mc_status r = MC_OK;
+      HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      LbmArgs pend[2];
+      const LbmArgs* fuse = nullptr;
+      for (int64_t k = 0; k < len && r == MC_OK; ++k) {
+        const int par = (int)(k & 1);
+        LbmArgs* defer = (q->enable_fluid && p->rod_fuse && k + 1 < len) ? &pend[par] : nullptr;
+        r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, StepSync(), fuse,
+                         defer);
+        fuse = defer;
       }
-      for (int64_t k = 0; k + S <= num_steps; k += S) HIPCHK(hipGraphLaunch(p->graph_exec, us));
-      HIPCHK(hipEventRecord(p->graph_done, us));
+      hipError_t ec = hipStreamEndCapture(cs, &g);
+      if (r != MC_OK) return r;
+      if (ec != hipSuccess) return fail(MC_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
+      hipGraphExec_t ge = nullptr;
+      ec = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ec != hipSuccess) return fail(MC_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ec));
+      p->graph_exec[slot] = ge;
+      p->graph_key[slot] = key;
+      return MC_OK;
+    };
+    if (bulk > 0) {
+      MCCHK(graph_for(0, S));
+      for (int64_t k = 0; k < bulk; k += S) HIPCHK(hipGraphLaunch(p->graph_exec[0], us));
     }
-    // remaining steps (fewer than S) without a graph
-    const int64_t done = S > 0 ? (num_steps / S) * S : 0;
-    for (int64_t k = done; k < num_steps; ++k) {
-      const int par = (int)(k & 1);
-      MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, us));
+    if (tail > 0) {
+      // the remainder starts on an even step (bulk is even): same parity as step 0
+      MCCHK(graph_for(1, tail));
+      HIPCHK(hipGraphLaunch(p->graph_exec[1], us));
     }
+    HIPCHK(hipEventRecord(p->graph_done, us));
   }
   if (num_steps & 1) {
     // final lattice is in f_tmp; bring it back into the caller's buffer
@@ -1351,6 +1417,7 @@ mc_status mc_ib_lbm_step(mc_plan* p, const mc_params* q, const float* position, 
   MCCHK(check_params(p, &qq, true, false));
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
   hipStream_t s = (hipStream_t)stream;
+  MCCHK(check_async(p, s));
   const Consts c = derive(p, &qq, dt);
   MCCHK(ensure_geo(p, &qq));
   if (p->N > 0 && velocity_out != velocity)
@@ -1439,6 +1506,7 @@ mc_status mc_track_read(mc_plan* p, float* cumulative_out, float* com_out, void*
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   if (p->track.n_groups == 0) return fail(MC_ERR_INVALID, "tracking is not enabled");
   hipStream_t s = (hipStream_t)stream;
+  MCCHK(check_async(p, s));
   const size_t b = 2 * (size_t)p->track.n_groups * sizeof(float);
   if (cumulative_out)
     HIPCHK(hipMemcpyAsync(cumulative_out, p->track.cum, b, hipMemcpyDeviceToDevice, s));
@@ -1547,6 +1615,13 @@ mc_status check_rod_adjoint(const mc_params* q) {
       q->enable_fluid)
     return fail(MC_ERR_UNSUPPORTED,
                 "the rod adjoint covers the step without the fluid (enable_fluid = 0)");
+  const int kern = q->bending_constraint == MC_CONSTRAINT_AUTO
+                       ? q->solver_type
+                       : (q->bending_constraint == MC_CONSTRAINT_COSSERAT ? 1 : 0);
+  if (kern != q->solver_type)
+    return fail(MC_ERR_UNSUPPORTED,
+                "the rod adjoint needs bending_constraint and solver_type to agree "
+                "(a Cosserat constraint with solver_type pbd, or the reverse)");
   return MC_OK;
 }
 

@@ -846,8 +846,10 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
 // workgroup barrier releases the other threads.  The workgroups are
 // co-resident: at most 256 workgroups of 256 threads (77 VGPRs, no dynamic
 // LDS) take part, several fit on one CU, and the device has 256 CUs.  The
-// wait is bounded nevertheless: after ~2^22 polls it records the timeout in
-// WideCtl::err and proceeds, so the kernel always ends.
+// wait is bounded nevertheless: after WideCtl::poll_limit polls (2^22 by
+// default) it records the timeout in WideCtl::err and in the host flag, and
+// the workgroup leaves the solve (it reads nothing another workgroup was to
+// write after the barrier), so every workgroup ends.
 
 // Scratch of the wide solve for N nodes and ne = N * K2 entries.
 struct WLayout {
@@ -877,22 +879,30 @@ __device__ __forceinline__ void wlayout(unsigned char* base, int N, int ne, WLay
   L.suc = (float2*)take(8 * (size_t)ne);
 }
 
-__device__ __forceinline__ void grid_sync(WideCtl* ctl, unsigned int target) {
+// Returns false (to every thread of the workgroup) when the wait gave up.
+__device__ __forceinline__ bool grid_sync(WideCtl* ctl, unsigned int target, int* s_ok) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_fetch_add(&ctl->bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int polls = 0;
+    const int limit = ctl->poll_limit;
+    int polls = 0, ok = 1;
     while (__hip_atomic_load(&ctl->bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++polls > (1 << 22)) {
+      if (polls >= limit) {
+        ok = 0;
         __hip_atomic_store(&ctl->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ctl->host_err)
+          __hip_atomic_store(ctl->host_err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         break;
       }
+      __builtin_amdgcn_s_sleep(1);
+      ++polls;
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *s_ok = ok;
   }
   __syncthreads();
+  return *s_ok != 0;
 }
 
 __device__ __forceinline__ int ld_agent(const int* p) {
@@ -905,7 +915,7 @@ __device__ __forceinline__ int ld_agent(const int* p) {
 // that writes it).  Returns the total to every thread.  Ends with a barrier.
 template <class F>
 __device__ __forceinline__ int grid_scan(int m, F val, int* out, WideCtl* ctl, int h, int H,
-                                         unsigned int& target, int* wsum, int* s_off) {
+                                         unsigned int& target, int* wsum, int* s_off, bool& ok) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int lo = (int)(((long long)m * h) / H), hi = (int)(((long long)m * (h + 1)) / H);
   int carry = 0;
@@ -919,7 +929,8 @@ __device__ __forceinline__ int grid_scan(int m, F val, int* out, WideCtl* ctl, i
   }
   if (tid == 0) __hip_atomic_store(&ctl->bsum[h], carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   target += H;
-  grid_sync(ctl, target);
+  ok = grid_sync(ctl, target, &s_off[2]);
+  if (!ok) return 0;
   if (tid == 0) {
     int off = 0, total = 0;
     for (int j = 0; j < H; ++j) {
@@ -935,7 +946,7 @@ __device__ __forceinline__ int grid_scan(int m, F val, int* out, WideCtl* ctl, i
   if (off != 0)
     for (int i = lo + tid; i < hi; i += bs) out[i] += off;
   target += H;
-  grid_sync(ctl, target);
+  ok = grid_sync(ctl, target, &s_off[2]);
   return total;
 }
 
@@ -1013,7 +1024,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
   unsigned int target = 0;
   auto gsync = [&]() {
     target += H;
-    grid_sync(ctl, target);
+    return grid_sync(ctl, target, &s_off[2]);
   };
   WLayout L;
   wlayout(a.wscratch, a.N, a.N * K2, L);
@@ -1050,7 +1061,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
       }
     }
   }
-  gsync();
+  if (!gsync()) return;
   const int nq = ld_agent(&ctl->nq);
   const int ns = ld_agent(&ctl->nseg);
   // 2. every tap's segment; entry ranges of the segments
@@ -1066,8 +1077,10 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
   }
   const int* wc = a.wcount;
   const int* sc = L.scell;
+  bool ok = true;
   const int tot = grid_scan(ns, [&](int s) { return ld_agent(&wc[sc[s]]); }, L.sstart, ctl, h, H,
-                            target, wsum, s_off);
+                            target, wsum, s_off, ok);
+  if (!ok) return;
   if (gt == 0) L.sstart[ns] = tot;
   // 3. entries in arrival order; the cell counters return to zero; first
   // gather and correction
@@ -1091,7 +1104,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
     if (c.iters > 0)
       L.corr[n] = wide_corr<K2>(c, w, seg, L.suc, make_float2(a.vib[2 * n], a.vib[2 * n + 1]), gi.umax);
   }
-  gsync();
+  if (!gsync()) return;
   // 4. each entry's rank among its segment's entries -> ascending entry ids
   for (int q = gt; q < nq * K2; q += gs) {
     const int r = q / K2, t = q - r * K2;
@@ -1102,7 +1115,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
     for (int j = lo; j < hi; ++j) rank += L.eplace[j] < e ? 1 : 0;
     L.esort[lo + rank] = e;
   }
-  gsync();
+  if (!gsync()) return;
   // 5. kernel-weight fold and 1/max(kw, 1) (fluid.py:123-133,289-290), the
   // first spread; then the remaining iterations (fluid.py:292-316)
   for (int s = gt; s < ns; s += gs) {
@@ -1117,7 +1130,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
       L.suc[s] = u;
     }
   }
-  gsync();
+  if (!gsync()) return;
   for (int it = 1; it < c.iters; ++it) {
     for (int r = gt; r < nq; r += gs) {
       const int n = L.list[r];
@@ -1131,13 +1144,13 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
       L.corr[n] = wide_corr<K2>(c, w, seg, L.suc, make_float2(a.vib[2 * n], a.vib[2 * n + 1]),
                                 a.bd.geo[a.bd.node_inst[n]].umax);
     }
-    gsync();
+    if (!gsync()) return;
     for (int s = gt; s < ns; s += gs) {
       float2 u = L.suc[s];
       wide_spread<K2>(L.esort, L.sstart[s], L.sstart[s + 1], wts, L.corr, L.sinv[s], u);
       L.suc[s] = u;
     }
-    gsync();
+    if (!gsync()) return;
   }
   // 6. post-clamp and force (fluid.py:327-339); F replaces u_corrected
   for (int s = gt; s < ns; s += gs) {
@@ -1167,7 +1180,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
     frc[HW + cell] = fy;
     atomicOr(&a.mask[gi.mask_off + (cell >> 5)], 1u << (cell & 31));
   }
-  gsync();
+  if (!gsync()) return;
   // node drag (fluid.py:364-382) and the final wrap (simulate.py:97)
   for (int r = gt; r < nq; r += gs) {
     const int n = L.list[r];
@@ -1321,20 +1334,39 @@ constexpr int WIDE_THREADS = 256;  // small workgroups: a launch with nothing to
 template <int K>
 __global__ void __launch_bounds__(WIDE_THREADS) k_ib_wide(ClusterArgs a) {
   __shared__ int wsum[16];
-  __shared__ int s_off[2];
+  __shared__ int s_off[3];
   const int nodes = __hip_atomic_load(&a.wctl->wide_nodes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (nodes == 0) return;
   const int H = min((int)gridDim.x, max(1, (nodes + IB_WIDE_NODES_PER_WG - 1) / IB_WIDE_NODES_PER_WG));
   if ((int)blockIdx.x >= H) return;
+  if (__hip_atomic_load(&a.wctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    // an earlier solve gave up at a barrier: its cell counters are not back
+    // at zero until launch_ib_reset, so only the final wrap of the wide
+    // clusters' nodes is done (simulate.py:97); the host reports the error
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += H * blockDim.x) {
+      if (a.cl_cnt[a.label[a.comp_of_node[n]]] <= a.wide_min) continue;
+      const InstGeo& gi = a.bd.geo[a.bd.node_inst[n]];
+      a.pos_out[2 * n] = fmodf_floor(a.pos[2 * n], gi.dom_wf);
+      a.pos_out[2 * n + 1] = fmodf_floor(a.pos[2 * n + 1], gi.dom_hf);
+    }
+    return;
+  }
   cluster_wide<K>(a, blockIdx.x, H, wsum, s_off);
 }
 
-__global__ void k_ib_reset(int HW, int C, int* __restrict__ owner, int* __restrict__ parent,
-                           int* __restrict__ nlinks) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *nlinks = 0;
-  if (i < HW) owner[i] = -1;
-  if (i < C) parent[i] = i;
+__global__ void k_ib_reset(long long HW, int C, int* __restrict__ owner, int* __restrict__ parent,
+                           int* __restrict__ nlinks, int* __restrict__ wcount,
+                           WideCtl* __restrict__ wctl) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    *nlinks = 0;
+    if (wctl) __hip_atomic_store(&wctl->err, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (i < HW) {
+    owner[i] = -1;
+    if (wcount) wcount[i] = 0;
+  }
+  if (i < C) parent[i] = (int)i;
 }
 
 }  // namespace
@@ -1381,6 +1413,14 @@ hipError_t ib_prepare(int lds_max) {
     default: LAUNCH(6); break; \
   }
 
+hipError_t launch_ib_reset(const IBWork& w, hipStream_t s) {
+  const long long n = std::max(w.cells, (long long)w.C);
+  hipLaunchKernelGGL(k_ib_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w.cells, w.C,
+                     w.owner, w.uf_parent, w.nlinks, w.wide_wg > 0 ? w.wide_count : nullptr,
+                     w.wide_wg > 0 ? w.wide_ctl : nullptr);
+  return hipGetLastError();
+}
+
 hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float* f_in,
                      const float* pos, float* vel_io, float* pos_out, long long* tick,
                      hipStream_t s) {
@@ -1390,13 +1430,9 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
     if (tick) return launch_tick(tick, s);
     return hipSuccess;
   }
-  const long long HW = w.cells;
   if (w.dirty) {
     // an earlier call stopped between its launches: restore the idle state
-    const int n = (int)std::max(HW, (long long)C);
-    hipLaunchKernelGGL(k_ib_reset, dim3((n + 255) / 256), dim3(256), 0, s, (int)HW, C, w.owner,
-                       w.uf_parent, w.nlinks);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_ib_reset(w, s);
     if (e != hipSuccess) return e;
   }
   w.dirty = true;

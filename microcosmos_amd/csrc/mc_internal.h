@@ -26,7 +26,8 @@ struct Consts {
   float steric_strength, steric_sv, steric_2s2, steric_norm, steric_svs, fft_coef;
   int steric_nskip;
   // rod (pbd.py / stable_cosserat.py)
-  int solver;                 // 0 pbd, 1 cosserat
+  int solver;                 // constraint kernel: 0 pbd, 1 cosserat (bending_constraint)
+  int cos_target;             // solver_type == cosserat: y = debug_vector = prediction
   int cycles;
   float k_st, k_sh, cos_w;
   // IB + LBM (fluid.py)
@@ -86,6 +87,7 @@ struct BatchDev {
 };
 
 constexpr int LBM_THREADS = 256;  // lattice cells per LBM workgroup
+constexpr int ROLLOUT_GRAPH_STEPS = 16;  // steps of the replayed rollout graph (even)
 // Output tile of the direct filter's column pass (columns x rows).
 constexpr int FILTER_TILE_W = 64, FILTER_TILE_H = 32;
 
@@ -283,12 +285,18 @@ struct IBWork {
   struct WideCtl* wide_ctl;
 };
 // Control block of the wide solve: grid-barrier counter, wide-cluster and
-// wide-node counts, segment and node counters, a flag set if a barrier wait
-// ever timed out, and the
-// per-workgroup partial sums of the grid scans.
+// wide-node counts, segment and node counters, the per-workgroup partial sums
+// of the grid scans, and the barrier-timeout record.  A barrier wait gives up
+// after poll_limit polls; it then sets err (sticky until launch_ib_reset
+// clears it) and *host_err (pinned host memory the host checks at its next
+// call), and the workgroup leaves the solve.  While err is set the wide solve
+// only wraps positions, so no later launch indexes through the counters the
+// abandoned solve left behind.
 struct WideCtl {
   unsigned int bar;
   int wide, wide_nodes, nseg, nq, err;
+  int poll_limit;
+  int* host_err;
   int bsum[IB_WIDE_WG];
 };
 size_t ib_wide_scratch_bytes(int N, int kmax);
@@ -302,6 +310,9 @@ hipError_t ib_prepare(int lds_max);
 hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float* f_in,
                      const float* pos, float* vel_io, float* pos_out, long long* tick,
                      hipStream_t s);
+// Restores the idle state between calls (cell owners, union-find parents, the
+// link count, the wide solve's cell counters) and clears WideCtl::err.
+hipError_t launch_ib_reset(const IBWork& w, hipStream_t s);
 // swimmer centre of mass / displacement tracking
 struct TrackWork {
   int n_groups;       // 0: tracking off

@@ -226,12 +226,15 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
       x = x + (fx * c.steric_strength) * c.dt;
       y = y + (fy * c.steric_strength) * c.dt;
     }
+    // the inertial target y = debug_vector is the prediction only under
+    // solver_type "cosserat" (simulate.py:41,71-72), else it stays zero
+    const float tx = c.cos_target ? x : 0.0f, ty = c.cos_target ? y : 0.0f;
     X[2 * i] = x;
     X[2 * i + 1] = y;
-    Y[2 * i] = x;
-    Y[2 * i + 1] = y;
-    a.debug_out[2 * gg] = is_cos ? x : 0.0f;
-    a.debug_out[2 * gg + 1] = is_cos ? y : 0.0f;
+    Y[2 * i] = tx;
+    Y[2 * i + 1] = ty;
+    a.debug_out[2 * gg] = tx;
+    a.debug_out[2 * gg + 1] = ty;
   };
   auto hinge_rest = [&](float amp_, float om_, float ph_, float rs_) {
     if (!a.has_act) return rs_;

@@ -40,9 +40,11 @@ from .structs import Edges, Fields, Nodes
 @dataclass(frozen=True)
 class FoldingLossConfig:
     """Loss settings of experiments/conf/experiment/filament_folding.yaml:36-48
-    (loss section) and :66 (optimization.stability_weight)."""
+    (loss section) and :66 (optimization.stability_weight).  The yaml turns
+    the PCA alignment off (centring only); use_pca_rotation=True selects the
+    PCA-aligned form of the losses."""
     loss_mode: str = "dcd"              # "dcd" or "pointcloud" (Chamfer)
-    use_pca_rotation: bool = True
+    use_pca_rotation: bool = False
     num_eval_frames: int = 4
     num_stability_frames: int = 4
     stability_anchor_offset: int = 30

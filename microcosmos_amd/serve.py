@@ -205,6 +205,10 @@ class Server:
         share = max(1, -(-len(self.waiting) // len(free)))
         for lane in free:
             if lane.event is not None:
+                # the chunk has finished: a barrier timeout of its wide IB
+                # solve is reported here, at the chunk boundary
+                if lane.batch is not None:
+                    lane.batch.check()
                 lane.event = None
                 for sl in lane.slots:
                     sl.done += lane.in_flight

@@ -171,6 +171,11 @@ class Rollout:
         self.tracker.init(self.position)
         return self.tracker
 
+    def check(self):
+        """Waits for the rollout's stream and raises MicrocosmosError if its
+        steps did not complete as specified (mc_plan_status)."""
+        self.plan.status(stream_handle(self.dev))
+
     def nodes(self) -> Nodes:
         return Nodes(position=self.position, velocity=self.velocity, color_bend=self.color_bend,
                      debug_vector=self.debug_vector)

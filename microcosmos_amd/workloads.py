@@ -285,3 +285,24 @@ def folding(seed=3, grid=(256, 256), n_fil=8, n_nodes=48, steps=500, modes=4, am
         q = p + off[:, None] * nrm[None, :]
         tgt.append(np.mod(q, np.array([w, h], np.float64)))
     return wl, np.concatenate(tgt).astype(np.float32)
+
+
+def device_state(wl: Workload, device="cuda"):
+    """The workload's initial state as (Nodes, Edges, Fields) of torch tensors
+    on ``device`` (float32, int32 topology)."""
+    import torch
+    from .structs import Edges, Fields, Nodes
+
+    def t(a, dt=torch.float32):
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+
+    nodes = Nodes(position=t(wl.position), velocity=t(wl.velocity),
+                  color_bend=None if wl.color_bend is None else t(wl.color_bend, torch.int32),
+                  debug_vector=t(np.zeros_like(wl.position)))
+    edges = Edges(pairs=t(wl.pairs, torch.int32), theta=t(wl.theta),
+                  rest_lengths=t(wl.rest_lengths), bending_pairs=t(wl.bending_pairs, torch.int32),
+                  bending_rest_angles=t(wl.bending_rest_angles),
+                  bending_stiffness=t(wl.bending_stiffness))
+    fields = Fields(grid_shape=tuple(wl.grid_shape), steric=t(wl.steric),
+                    fluid_velocity=t(wl.fluid_velocity), f_grid=t(wl.f_grid))
+    return nodes, edges, fields

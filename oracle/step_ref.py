@@ -44,6 +44,12 @@ Parity status per function (how each is pinned; see tests/test_oracle_*.py):
                             reference invariants (tests/test_stable_cosserat.py:81-154)
   stencil (IB)           -- brute-force integer taps, partition of unity,
                             constant interpolation, spread/interp adjointness
+  IB forcing loop, post-clamp, force, drag (immersed_boundary_interaction)
+                         -- closed forms at rest fluid (tests/test_oracle_ib.py):
+                            one node's deficit (1 - r S)^n, M coincident nodes'
+                            (1 - r S_M)^n through inv_kw, both clamps through the
+                            rank-one recurrence, mass / scale / dt / axis quirk,
+                            rho in the force, injected momentum
   LBM                    -- feq moments (float64), mass conservation, uniform fixed
                             point, exact streaming permutation, Guo momentum input,
                             Taylor-Green viscous decay exp(-2 nu k^2 t)
@@ -81,6 +87,9 @@ class Params:
     the step's own ``dt`` is passed to ``step`` separately, as in the reference.
     """
     solver_type: str = "pbd"
+    # ConstraintSolverConfig.bending_constraint as a name, "pbd" or
+    # "cosserat" (None: the one solver_type names)
+    bending_constraint: Optional[str] = None
     cycles_per_step: int = 20
     stiffness_stretch: float = 0.6
     stiffness_shear: float = 1.0
@@ -808,9 +817,12 @@ def step(s: State, dt: float, p: Params) -> State:
             pred, s.grid_shape, p.steric_sigma, p.steric_neighbor_skip, p.steric_scatter_value)
         pred = pred + fst * p.steric_strength * dt
     s.position = pred
+    # solver_type only sets the inertial target (simulate.py:71-72); the
+    # constraint run is bending_constraint (simulate.py:74-77)
     if p.solver_type == "cosserat":
         s.debug_vector = pred
-    fn = stable_cosserat_constraint if p.solver_type == "cosserat" else pbd_rod_constraint
+    bc = p.bending_constraint or p.solver_type
+    fn = stable_cosserat_constraint if bc == "cosserat" else pbd_rod_constraint
     for _ in range(p.cycles_per_step):
         fn(s, p)
     s.velocity = (s.position - prev) / dt

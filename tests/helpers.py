@@ -36,19 +36,8 @@ def run_oracle(wl, steps, params, dtype=np.float32, actuate=True, start=0):
 
 
 def device_state(wl, device="cuda"):
-    import torch
-    from microcosmos_amd import Edges, Fields, Nodes
-    t = lambda a, dt=torch.float32: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
-    nodes = Nodes(position=t(wl.position), velocity=t(wl.velocity),
-                  color_bend=t(wl.color_bend, torch.int32),
-                  debug_vector=t(np.zeros_like(wl.position)))
-    edges = Edges(pairs=t(wl.pairs, torch.int32), theta=t(wl.theta),
-                  rest_lengths=t(wl.rest_lengths), bending_pairs=t(wl.bending_pairs, torch.int32),
-                  bending_rest_angles=t(wl.bending_rest_angles),
-                  bending_stiffness=t(wl.bending_stiffness))
-    fields = Fields(grid_shape=tuple(wl.grid_shape), steric=t(wl.steric),
-                    fluid_velocity=t(wl.fluid_velocity), f_grid=t(wl.f_grid))
-    return nodes, edges, fields
+    from microcosmos_amd.workloads import device_state as _ds
+    return _ds(wl, device)
 
 
 def device_config(wl, base=None, **kw):

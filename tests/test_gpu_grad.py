@@ -366,7 +366,7 @@ def test_folding_objective_matches_restatement(dev, physics):
     _say(f"folding objective ({physics}): 20 + 10 steps")
     from microcosmos_amd import folding as F
     wl, cfg, p, dt, raw, cloud = _objective_case(physics)
-    lc = F.FoldingLossConfig(num_eval_frames=3, num_stability_frames=3,
+    lc = F.FoldingLossConfig(use_pca_rotation=True, num_eval_frames=3, num_stability_frames=3,
                              stability_anchor_offset=5, smoothness_weight=0.1)
     nodes, edges, fields = device_state(wl, dev)
     raw_d = torch.as_tensor(raw, device=dev)
@@ -398,7 +398,7 @@ def test_folding_objective_optimisation_lowers_the_loss(dev):
     from microcosmos_amd import folding as F
     from microcosmos_amd.grad import FoldingOptimizer
     wl, cfg, p, dt, raw, cloud = _objective_case("folding")
-    lc = F.FoldingLossConfig(num_eval_frames=3, num_stability_frames=3, stability_anchor_offset=5)
+    lc = F.FoldingLossConfig(use_pca_rotation=True, num_eval_frames=3, num_stability_frames=3, stability_anchor_offset=5)
     nodes, edges, fields = device_state(wl, dev)
     opt = FoldingOptimizer(lr=0.05, warmup_steps=0, total_steps=12)
     _, _, hist = F.optimize_folding_objective(nodes, edges, fields, cfg, dt, 20, 10,
@@ -455,7 +455,7 @@ def test_folding_objective_with_rest_lengths(dev):
     from microcosmos_amd import folding as F
     wl, cfg, p, dt, raw, cloud = _objective_case("folding")
     raw_l = np.random.default_rng(9).normal(-1.0, 0.3, wl.rest_lengths.shape).astype(np.float32)
-    lc = F.FoldingLossConfig(num_eval_frames=3, num_stability_frames=3,
+    lc = F.FoldingLossConfig(use_pca_rotation=True, num_eval_frames=3, num_stability_frames=3,
                              stability_anchor_offset=5)
     nodes, edges, fields = device_state(wl, dev)
     raw_d = torch.as_tensor(raw, device=dev)
@@ -489,7 +489,7 @@ def test_batched_folding_objective_equals_mean_of_each(dev):
     wl0, cfg, p, dt, raw, cloud = _objective_case("folding")
     wls = [wl0] + [W.folding(seed=sd, steps=30)[0] for sd in (4, 5)]
     raw_l = np.random.default_rng(9).normal(-1.0, 0.3, wl0.rest_lengths.shape).astype(np.float32)
-    lc = F.FoldingLossConfig(num_eval_frames=3, num_stability_frames=3,
+    lc = F.FoldingLossConfig(use_pca_rotation=True, num_eval_frames=3, num_stability_frames=3,
                              stability_anchor_offset=5, smoothness_weight=0.1)
     raw_d = torch.as_tensor(raw, device=dev)
     raw_ld = torch.as_tensor(raw_l, device=dev)

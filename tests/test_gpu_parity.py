@@ -290,6 +290,35 @@ def test_step_parity_small(dev, steps):
         assert r[k] <= 1e-5, (k, r)
 
 
+@pytest.mark.parametrize("bc,st", [("cosserat", "pbd"), ("pbd", "cosserat"), ("cosserat", "cosserat")])
+def test_step_follows_bending_constraint(dev, bc, st):
+    """The kernel follows bending_constraint; solver_type only sets the
+    inertial target (a Cosserat constraint under solver_type "pbd" pulls toward
+    y = 0), five steps with the fluid against the restatement."""
+    _say(f"step dispatch bending_constraint={bc} solver_type={st}")
+    from microcosmos_amd import step, pbd_rod_constraint, stable_cosserat_constraint
+    wl = W.small(seed=11)
+    base = R.COSSERAT_SCHEME if bc == "cosserat" else R.PBD_SCHEME
+    p = replace(oracle_params(wl, base, cycles_per_step=4), solver_type=st, bending_constraint=bc)
+    o = run_oracle(wl, 5, p)
+    from microcosmos_amd import COSSERAT_SCHEME, PBD_SCHEME
+    cfg = device_config(wl, COSSERAT_SCHEME if bc == "cosserat" else PBD_SCHEME, cycles_per_step=4)
+    cfg = replace(cfg, solver_type=st, bending_constraint=(
+        stable_cosserat_constraint if bc == "cosserat" else pbd_rod_constraint))
+    nodes, edges, fields = device_state(wl, dev)
+    for t in range(5):
+        edges = replace(edges, bending_rest_angles=torch.as_tensor(wl.rest_angles_at(t), device=dev))
+        nodes, edges, fields = step(nodes, edges, fields, wl.dt, cfg)
+    r = _compare_states(f"dispatch {bc}/{st}", (nodes, edges, fields), o, None)
+    for k in ("pos", "u", "f", "theta"):
+        assert r[k] <= 1e-5, (k, r)
+    dbg = _np(nodes.debug_vector)
+    if st == "cosserat":
+        assert rel_l2(dbg, o.debug_vector) <= 1e-5
+    else:
+        assert np.all(dbg == 0)
+
+
 def test_rollout_permuted_node_ids(dev):
     """Filaments whose node ids are not contiguous ranges (the small case with
     its node numbering shuffled) take the listed-id paths of every kernel;
