@@ -143,6 +143,7 @@ struct mc_plan {
   size_t rod_lds_bytes = 0;
   int rod_use_lds = 1;
   bool rod_fuse = false;        // rollouts may run an LBM pass inside the rod launch
+  bool all_chains = false;      // every component is a chain (rod: one-wavefront path only)
   float* rod_gscratch = nullptr;
   // steric
   float* pred_w = nullptr;
@@ -407,8 +408,19 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
     m.hf = hfirst[c];
     m.blob_off = blob_off[c];
     m.inst = p->node_inst_h[norder[noff[c]]];
-    m.pad0 = m.pad1 = m.pad2 = 0;
+    m.chain = m.pad1 = m.pad2 = 0;
+    // a chain: at most 64 nodes with contiguous ids, edge e = (e, e + 1) and
+    // hinge b = (b, b + 1) in component-local ids (the rod's one-wavefront path)
+    bool chain = m.nn >= 1 && m.nn <= 64 && m.nf >= 0 && m.ef >= 0 && m.hf >= 0 &&
+                 m.ne == m.nn - 1 && m.nb == std::max(m.ne - 1, 0);
+    for (int e = 0; chain && e < m.ne; ++e)
+      chain = esrc[m.e0 + e] == e && etgt[m.e0 + e] == e + 1;
+    for (int b = 0; chain && b < m.nb; ++b)
+      chain = hin[m.b0 + b] == b && hout[m.b0 + b] == b + 1;
+    m.chain = chain ? 1 : 0;
   }
+  p->all_chains = C > 0;
+  for (int c = 0; c < C; ++c) p->all_chains = p->all_chains && meta[c].chain;
   CompMeta* md;
   MCCHK(upload(p, &md, meta, "component records"));
   int *bl, *blo, *nf, *ef, *hf;
@@ -864,7 +876,8 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
   HIPCHK(launch_rod(c, p->bd, p->L, p->N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 0,
                     in->position, in->velocity, nullptr, in->theta, ed->rest_lengths,
                     ed->bending_rest_angles, ed->bending_stiffness, act, out->steric, p->pred_w,
-                    out->position, out->velocity, out->debug_vector, out->theta, fuse_lbm, s));
+                    out->position, out->velocity, out->debug_vector, out->theta, fuse_lbm, s,
+                    p->all_chains));
   const size_t hwf = (size_t)p->lat_cells;
   long long* tick = act ? p->tick : nullptr;
   if (c.enable_fluid) {
@@ -998,13 +1011,29 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
   if ((st = p->alloc(&w.st_sy, (size_t)N, "st sy")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.st_w, taps * N, "st w")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.st_won, (size_t)N, "st won")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.st_rho, taps * N, "st rho")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.st_u, taps * N, "st u")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.vib, 2 * (size_t)N, "vib")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.owner, hwf, "cell owner")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.cell_seg, hwf, "cell seg")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.links, taps * N, "cluster links")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.nlinks, 1, "cluster link count")) != MC_OK) return bail(st);
+  for (const InstHost& h : p->inst)
+    if (h.Hf > 32767 || h.Wf > 32767)
+      return bail(fail(MC_ERR_BAD_SHAPE, "lattice side above 32767 (%d x %d)", h.Hf, h.Wf));
+  // link buckets per instance when every instance's components fit the
+  // per-workgroup union-find
+  w.bucketed = true;
+  for (const InstGeo& gh : p->geo_h) w.bucketed = w.bucketed && gh.ncomp <= IB_UF_MAX;
+  w.nbuckets = w.bucketed ? p->n_inst : 1;
+  if ((st = p->alloc(&w.nlinks, (size_t)w.nbuckets, "cluster link counts")) != MC_OK) return bail(st);
+  {
+    std::vector<int> loff(p->n_inst);
+    long long acc = 0;
+    for (int i = 0; i < p->n_inst; ++i) {
+      loff[i] = (int)acc;
+      acc += (long long)p->inst[i].nodes * (long long)taps;
+    }
+    if ((st = upload(p, &w.link_off, loff, "link bucket offsets")) != MC_OK) return bail(st);
+  }
+  if ((st = p->alloc(&w.ictl, 1, "solve counters")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.label, C, "cluster label")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.cl_cnt, C, "cluster nodes")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.cl_node_off, C, "cluster node off")) != MC_OK) return bail(st);

@@ -30,16 +30,12 @@
 #include <algorithm>
 
 #include "mc_bin.h"
+#include "mc_claim.h"
 #include "mc_common.h"
 #include "mc_internal.h"
 
 namespace mc {
 namespace {
-
-__device__ __forceinline__ float peskin(float r, float h) {
-  float ra = fabsf(r) / h;
-  return (ra <= 2.0f) ? 0.25f * (1.0f + dcos((MC_PI_F * ra) / 2.0f)) : 0.0f;
-}
 
 // Relaxed load at device scope: reads a value produced by atomics at L2
 // rather than a possibly stale L1 line.
@@ -72,71 +68,56 @@ __device__ __forceinline__ void uf_union(int* p, int a, int b) {
   }
 }
 
-// Wavefront-aggregated atomic add: one device-scope atomic per wavefront
-// instead of one per lane (a single counter hit by every lane of a large
-// launch serialises).  Every lane of the wavefront must call it; returns this
-// lane's offset, lanes in lane order.
-__device__ __forceinline__ int wave_fetch_add(int* ctr, int v) {
-  const int lane = threadIdx.x & 63;
-  int inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int u = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += u;
-  }
-  const int total = __shfl(inc, 63, 64);
-  int base = 0;
-  if (lane == 63 && total > 0)
-    base = __hip_atomic_fetch_add(ctr, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  base = __shfl(base, 63, 64);
-  return base + inc - v;
-}
-
-// Lanes per node in the stencil kernel: one lane per tap, the node's group a
-// power of two within a wavefront.
-template <int K>
-constexpr int stencil_lanes() {
-  return K * K <= 1 ? 1 : (K * K <= 4 ? 4 : (K * K <= 16 ? 16 : (K * K <= 32 ? 32 : 64)));
-}
-
 // One lane per (node, tap).  Every lane of a node group evaluates the node's
 // geometry; lane t owns tap t (dx = t / K, dy = t % K): its weight, the claim
-// of its cell, a link when another component holds the cell, and -- when the
-// claim wins -- rho and the clamped u of the cell from f_in
-// (fluid.py:268-276), which the cluster solve takes as the cell's state.
+// of its cell (compare-and-swap: the first arriving component owns it; the
+// node records which of its taps won) and a link when another component holds
+// the cell.  Links go to the node's instance bucket (bucketed) or to one list.
+// Block 0 also clears the wide solve's counters and the solve's scratch
+// allocator (the previous step's launches have completed).
 template <int K>
-__global__ void __launch_bounds__(256) k_ib_stencil(
+__global__ void __launch_bounds__(256) k_ib_claim(
     Consts c, BatchDev bd, int N, const float* __restrict__ pos, const float* __restrict__ vel,
-    const float* __restrict__ f_all, const int* __restrict__ comp_of_node,
-    int* __restrict__ st_sx, int* __restrict__ st_sy, float* __restrict__ st_w,
-    unsigned long long* __restrict__ st_won, float* __restrict__ st_rho,
-    float2* __restrict__ st_u, float* __restrict__ vib, int* __restrict__ owner,
-    int2* __restrict__ links, int* __restrict__ nlinks) {
+    const int* __restrict__ comp_of_node, int* __restrict__ st_sx, int* __restrict__ st_sy,
+    float* __restrict__ st_w, unsigned long long* __restrict__ st_won, float* __restrict__ vib,
+    int* __restrict__ owner, int2* __restrict__ links, int* __restrict__ nlinks,
+    const int* __restrict__ link_off, int bucketed, WideCtl* __restrict__ wctl,
+    IbCtl* __restrict__ ictl) {
   constexpr int K2 = K * K;
   constexpr int L = stencil_lanes<K>();
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = gid / L, t = gid - n * L;
   const int lane = threadIdx.x & 63, base = lane - t;  // the group's first lane
+  if (gid == 0) {
+    // the previous solve launch has completed: its scratch is free
+    if (ictl) __hip_atomic_store(&ictl->scratch_top, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (wctl) {
+      // atomic stores: the counters are later updated by device-scope atomics
+      __hip_atomic_store(&wctl->bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&wctl->wide, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&wctl->wide_nodes, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&wctl->nseg, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&wctl->nq, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
   const bool node_ok = n < N;
   const bool tap = node_ok && t < K2;
   float prod = 0.0f;
-  int prev = -1, comp = -1;
-  int g = 0;
-  float sxf = 0.0f, syf = 0.0f, gx = 0.0f, gy = 0.0f;
-  const InstGeo* gi = nullptr;
+  int prev = -1, comp = -1, inst = 0;
   if (node_ok) {
-    gi = &bd.geo[bd.node_inst[n]];
+    inst = bd.node_inst[n];
+    const InstGeo* gi = &bd.geo[inst];
     const int W = gi->Wf, H = gi->Hf;
-    gx = pos[2 * n] * gi->scale_x;
-    gy = pos[2 * n + 1] * gi->scale_y;
-    sxf = floorf(gx) - (float)c.half;
-    syf = floorf(gy) - (float)c.half;
+    const float gx = pos[2 * n] * gi->scale_x;
+    const float gy = pos[2 * n + 1] * gi->scale_y;
+    const float sxf = floorf(gx) - (float)c.half;
+    const float syf = floorf(gy) - (float)c.half;
     const int sx = imod(f2i(sxf), W);
     const int sy = imod(f2i(syf), H);
     comp = comp_of_node[n];
     if (tap) {
       const int dx = t / K, dy = t - dx * K;
-      g = wrap1(sy + dy, H) * W + wrap1(sx + dx, W);
+      const int g = wrap1(sy + dy, H) * W + wrap1(sx + dx, W);
       prev = atomicCAS(&owner[gi->fl_off + g], -1, comp);
       // 1-D weights: tap t uses wx[t / K] and wy[t % K] (fluid.py:53-69)
       const float wxv = peskin((sxf + (float)dx) - gx, c.half_width);
@@ -159,23 +140,31 @@ __global__ void __launch_bounds__(256) k_ib_stencil(
   // a cell claimed by another component links the two (a repeat of the left
   // neighbour tap's claimant is dropped; at most one link per tap)
   const bool emit = tap && prev != -1 && prev != comp && (t == 0 || prev != prev_left);
-  const int li = wave_fetch_add(nlinks, emit ? 1 : 0);
+  // one counter atomic per wavefront when its nodes share a bucket
+  const int key = bucketed ? inst : 0;
+  const int k0 = __shfl(key, 0, 64);
+  int li;
+  if (__all(!emit || key == k0)) li = wave_fetch_add(&nlinks[k0], emit ? 1 : 0);
+  else li = emit ? atomicAdd(&nlinks[key], 1) : 0;
   if (!tap) return;
   const float den = total + MC_EPSIL;
   st_w[(size_t)n * K2 + t] = prod / den;
   if (t == 0) st_won[n] = (won_lanes >> base) & (K2 == 64 ? ~0ull : ((1ull << K2) - 1ull));
-  if (emit) links[li] = make_int2(prev, comp);
-  if (prev == -1) {
-    const size_t HW = (size_t)gi->Hf * gi->Wf;
-    const float* fin = f_all + 9 * gi->fl_off;
-    float f[9];
+  if (emit) links[(bucketed ? link_off[key] : 0) + li] = make_int2(prev, comp);
+}
+
+// Density and clamped velocity of lattice cell `cell` (flat index within its
+// instance) of f_all (fluid.py:268-276): the cell state the forcing starts from.
+__device__ __forceinline__ void cell_state(const float* __restrict__ f_all, const InstGeo& gi,
+                                           int cell, float& rho, float2& u) {
+  const size_t HW = (size_t)gi.Hf * gi.Wf;
+  const float* fin = f_all + 9 * gi.fl_off;
+  float f[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) f[i] = fmax_nan(fin[i * HW + g], MC_EPSIL);
-    float rho, ux, uy;
-    lbm_macro(f, gi->umax, rho, ux, uy);
-    st_rho[(size_t)n * K2 + t] = rho;
-    st_u[(size_t)n * K2 + t] = make_float2(ux, uy);
-  }
+  for (int i = 0; i < 9; ++i) f[i] = fmax_nan(fin[i * HW + cell], MC_EPSIL);
+  float ux, uy;
+  lbm_macro(f, gi.umax, rho, ux, uy);
+  u = make_float2(ux, uy);
 }
 
 // Single workgroup.  Unions the linked components (union-find in LDS when the
@@ -266,13 +255,12 @@ __device__ __forceinline__ size_t al16(size_t b) { return (b + 15) & ~(size_t)15
 
 struct ClusterArgs {
   Consts c;
-  const int *comp_node_off, *comp_nodes, *label, *cl_cnt, *cl_node_off, *cl_mem_cnt, *cl_mem_off,
-      *cl_members;
+  const int *comp_node_off, *comp_nodes, *cl_node_off, *cl_mem_cnt, *cl_mem_off, *cl_members;
+  int *label, *cl_cnt;  // per component: cluster root; per root: node count
   const CompMeta* meta;
   const int *st_sx, *st_sy;
   const unsigned long long* st_won;
-  const float *st_w, *st_rho, *vib, *f_in, *pos;
-  const float2* st_u;
+  const float *st_w, *vib, *f_in, *pos;
   float *vel_io, *pos_out, *force;
   int *cell_seg, *owner;
   uint32_t* mask;
@@ -280,6 +268,14 @@ struct ClusterArgs {
   unsigned char* gscratch;
   size_t bpn;        // global scratch bytes per node
   size_t lds_bytes;  // dynamic LDS of the launch (a multiple of 16)
+  // membership: per-instance link buckets and a union-find per workgroup
+  // (bucketed), else the cluster build's tables
+  int bucketed;
+  const int2* links;
+  int* nlinks;
+  const int* link_off;
+  IbCtl* ictl;
+  long long* tick;  // the actuation step counter, advanced once (may be NULL)
   // wide solve
   int N, C, wide_wg, wide_min;
   const int* comp_of_node;
@@ -354,16 +350,6 @@ constexpr int cluster_threads() { return K <= 4 ? 1024 : 512; }
 
 constexpr int SPT = 4;  // segments per thread whose loads are issued together
 
-// Small-cluster solve: at most one node per thread (thread r holds the node
-// of rank r, its taps' segments and weights in registers) and every array in
-// LDS.  Entries are (entry id e = r * K2 + t, weight) pairs, grouped by
-// segment and sorted by e, i.e. by node id.  Layout from `base`:
-// sstart[ns+1] ew[ne] corr[nq] scell[ns] sinv[ns] srho[ns] suc[ns] su0[ns].
-__device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns) {
-  return al16(4 * ((size_t)ns + 1)) + al16(8 * (size_t)ne) + al16(8 * (size_t)nq) +
-         3 * al16(4 * (size_t)ns) + 2 * al16(8 * (size_t)ns);
-}
-
 // The K2 per-tap values of one node, 16 bytes per load when the rows allow.
 template <int K2>
 __device__ __forceinline__ void load_taps(const float* __restrict__ p, float (&v)[K2]) {
@@ -382,32 +368,78 @@ __device__ __forceinline__ void load_taps(const float* __restrict__ p, float (&v
   }
 }
 
-template <int K2>
-__device__ __forceinline__ void load_taps2(const float2* __restrict__ p, float2 (&v)[K2]) {
-  if constexpr (K2 % 2 == 0) {
+// Ascending sort of M ints in registers (odd-even transposition: M rounds).
+template <int M>
+__device__ __forceinline__ void sort_regs(int (&v)[M]) {
 #pragma unroll
-    for (int t = 0; t < K2; t += 2) {
-      const float4 q = *reinterpret_cast<const float4*>(p + t);
-      v[t] = make_float2(q.x, q.y);
-      v[t + 1] = make_float2(q.z, q.w);
+  for (int r = 0; r < M; ++r)
+#pragma unroll
+    for (int i = r & 1; i + 1 < M; i += 2) {
+      const int lo = min(v[i], v[i + 1]), hi = max(v[i], v[i + 1]);
+      v[i] = lo;
+      v[i + 1] = hi;
     }
-  } else {
-#pragma unroll
-    for (int t = 0; t < K2; ++t) v[t] = p[t];
-  }
 }
 
-__device__ __forceinline__ void cswap_key(float2& a, float2& b) {
-  const bool sw = __float_as_int(a.x) > __float_as_int(b.x);
-  const float2 lo = sw ? b : a, hi = sw ? a : b;
-  a = lo;
-  b = hi;
+// Slot of lattice cell `cell` in a table of 2^bits slots (Fibonacci hashing).
+__device__ __forceinline__ int cell_slot(int cell, int bits) {
+  return (int)(((unsigned)cell * 2654435761u) >> (32 - bits));
 }
 
+// Table bits for ns segments: the smallest power of two >= 2 ns, at least 64.
+__device__ __forceinline__ int table_bits(int ns) {
+  int b = 6;
+  while ((1 << b) < 2 * ns) ++b;
+  return b;
+}
+
+// Node data of the small solve, loaded by the thread of the node's rank.
+template <int K2>
+struct NodeIn {
+  int n, sx, sy;
+  unsigned long long won;
+  float w[K2];
+  float2 vb, p0, v0;
+};
+
+template <int K2>
+__device__ __forceinline__ void load_node(const ClusterArgs& a, int n, NodeIn<K2>& d) {
+  d.n = n;
+  d.sx = a.st_sx[n];
+  d.sy = a.st_sy[n];
+  d.won = a.st_won[n];
+  load_taps<K2>(a.st_w + (size_t)n * K2, d.w);
+  d.vb = *reinterpret_cast<const float2*>(a.vib + 2 * n);
+  // the drag's inputs, read now (only this thread writes them, at the end)
+  d.p0 = *reinterpret_cast<const float2*>(a.pos + 2 * n);
+  d.v0 = *reinterpret_cast<const float2*>(a.vel_io + 2 * n);
+}
+
+// LDS bytes of the small solve: the cell -> segment table (2^bits keys and
+// values), per node its stencil start and won mask (16 B) and correction (8
+// B), per segment its entry start, cursor, cell, 1/kw, rho (4 B each) and u0,
+// u (8 B each), per entry its id, weight and segment (4 B each).
+__device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns, int bits) {
+  return 2 * al16(4 * ((size_t)1 << bits)) + al16(16 * (size_t)nq) + al16(8 * (size_t)nq) +
+         al16(4 * ((size_t)ns + 1)) + 4 * al16(4 * (size_t)ns) + 2 * al16(8 * (size_t)ns) +
+         3 * al16(4 * (size_t)ne);
+}
+
+// Small-cluster solve: one node per thread for the node phases (thread r
+// holds the node of rank r and its taps' segments in registers), one entry
+// (node, tap) per thread for the entry phases, every array in LDS.  The
+// segments are the cells the cluster's won taps claimed, one per cell, node
+// r's won taps numbered from seg0 (the exclusive prefix of the won counts) in
+// tap order; a cell -> segment hash table in LDS gives every tap its segment.
+// A segment's entries (entry id e = r K2 + t) are placed by arrival, then
+// sorted by e -- by node rank, i.e. ascending node id -- by the segment's own
+// thread, which folds the kernel weight in that order.  Global memory is
+// touched in two places only: the cells' populations (one gather, in flight
+// while the taps look their segments up) and the results at the end.
 template <int K>
-__device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGeo& gi,
-                                              unsigned char* base, const int* nodes, int nq,
-                                              int ns, int* wsum, int* s_seg) {
+__device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo& gi,
+                                            unsigned char* base, const NodeIn<K * K>& d, int nq,
+                                            int ns, int seg0, int bits, int* wsum) {
   constexpr int K2 = K * K;
   const Consts& c = a.c;
   const int ne = nq * K2;
@@ -416,68 +448,105 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
   const float umax = gi.umax;
   float* frc = a.force + 2 * gi.fl_off;
   uint32_t* msk = a.mask + gi.mask_off;
-  int* cseg = a.cell_seg + gi.fl_off;
   int* own = a.owner + gi.fl_off;
+  const float* fin = a.f_in + 9 * gi.fl_off;
   const int tid = threadIdx.x, bs = blockDim.x;
+  const int T = 1 << bits;
   size_t off = 0;
   auto take = [&](size_t b) {
     unsigned char* p = base + off;
     off += al16(b);
     return p;
   };
-  int* sstart = (int*)take(4 * ((size_t)ns + 1));
-  float2* ew = (float2*)take(8 * (size_t)ne);
+  int* hkey = (int*)take(4 * (size_t)T);
+  int* hval = (int*)take(4 * (size_t)T);
+  int4* nd = (int4*)take(16 * (size_t)nq);  // per node: sx | sy << 16, seg0, won (low, high)
   float2* corr = (float2*)take(8 * (size_t)nq);
+  int* sstart = (int*)take(4 * ((size_t)ns + 1));
+  int* scur = (int*)take(4 * (size_t)ns);
   int* scell = (int*)take(4 * (size_t)ns);
   float* sinv = (float*)take(4 * (size_t)ns);
   float* srho = (float*)take(4 * (size_t)ns);
-  float2* suc = (float2*)take(8 * (size_t)ns);
   float2* su0 = (float2*)take(8 * (size_t)ns);
+  float2* suc = (float2*)take(8 * (size_t)ns);
+  int* arr = (int*)take(4 * (size_t)ne);
+  float* wt = (float*)take(4 * (size_t)ne);
+  int* eseg = (int*)take(4 * (size_t)ne);
 
   const bool mine = tid < nq;
   const int r = tid;
-  int n = 0, g[K2], seg[K2], sub[K2];
-  float w[K2];
-  float2 vb = make_float2(0.0f, 0.0f);
-  float2 p0 = make_float2(0.0f, 0.0f), v0 = make_float2(0.0f, 0.0f);
-  unsigned long long won = 0;
+  for (int i = tid; i < T; i += bs) hkey[i] = -1;
   for (int s = tid; s <= ns; s += bs) sstart[s] = 0;
   if (mine) {
-    n = nodes[r];
-    const int sx = a.st_sx[n], sy = a.st_sy[n];
-    won = a.st_won[n];
-    load_taps<K2>(a.st_w + (size_t)n * K2, w);
-    vb = *reinterpret_cast<const float2*>(a.vib + 2 * n);
-    // the drag's inputs, read now (only this thread writes them, at the end)
-    p0 = *reinterpret_cast<const float2*>(a.pos + 2 * n);
-    v0 = *reinterpret_cast<const float2*>(a.vel_io + 2 * n);
+    nd[r] = make_int4(d.sx | (d.sy << 16), seg0, (int)(unsigned)d.won, (int)(unsigned)(d.won >> 32));
 #pragma unroll
-    for (int t = 0; t < K2; ++t) g[t] = wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W);
-    // won taps claim segments (arbitrary numbering), return their cell to
-    // idle, and carry the cell's rho and clamped u from the stencil pass
-    float rw[K2];
-    float2 uw[K2];
-    load_taps<K2>(a.st_rho + (size_t)n * K2, rw);  // all taps: no wait for `won`
-    load_taps2<K2>(a.st_u + (size_t)n * K2, uw);
-#pragma unroll
-    for (int t = 0; t < K2; ++t) {
-      if ((won >> t) & 1ull) {
-        const int sg = atomicAdd(s_seg, 1);
-        cseg[g[t]] = sg;
-        scell[sg] = g[t];
-        srho[sg] = rw[t];
-        su0[sg] = uw[t];
-        suc[sg] = uw[t];
-      }
+    for (int t = 0; t < K2; ++t) wt[r * K2 + t] = d.w[t];
+  }
+  __syncthreads();
+  auto ecell = [&](const int4& q, int t) {
+    const int sx = q.x & 0xffff, sy = (int)((unsigned)q.x >> 16);
+    return wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W);
+  };
+  // won taps (one entry per thread) enter their cell in the table: tap t of
+  // node r is segment seg0(r) + (won taps of r before t)
+  for (int e = tid; e < ne; e += bs) {
+    const int rr = e / K2, t = e - rr * K2;
+    const int4 q = nd[rr];
+    const unsigned long long wm = ((unsigned long long)(unsigned)q.w << 32) | (unsigned)q.z;
+    if ((wm >> t) & 1ull) {
+      const int g = ecell(q, t);
+      const int sg = q.y + __popcll(wm & ((1ull << t) - 1ull));
+      int h = cell_slot(g, bits);
+      for (int k = 0; k < T && atomicCAS(&hkey[h], -1, g) != -1; ++k) h = (h + 1) & (T - 1);
+      hval[h] = sg;
+      scell[sg] = g;
     }
   }
   __syncthreads();
-  // every tap's segment and its arrival index there
-  if (mine) {
+  // per segment (at most two per thread here, more in a loop): the cell's
+  // populations, in flight while the taps look their segments up
+  constexpr int SPT2 = 2;
+  float fp[SPT2][9];
 #pragma unroll
-    for (int t = 0; t < K2; ++t) seg[t] = cseg[g[t]];
+  for (int k = 0; k < SPT2; ++k) {
+    const int s = tid + k * bs;
+    const int cell = s < ns ? scell[s] : 0;
 #pragma unroll
-    for (int t = 0; t < K2; ++t) sub[t] = atomicAdd(&sstart[seg[t]], 1);
+    for (int i = 0; i < 9; ++i) fp[k][i] = s < ns ? fin[i * HW + cell] : 1.0f;
+  }
+  // every entry's segment (its cell's table entry; a cell is in the table
+  // exactly once) and the segments' entry counts
+  for (int e = tid; e < ne; e += bs) {
+    const int rr = e / K2, t = e - rr * K2;
+    const int g = ecell(nd[rr], t);
+    int h = cell_slot(g, bits);
+    for (int k = 0; k < T && hkey[h] != g; ++k) h = (h + 1) & (T - 1);
+    const int sg = hkey[h] == g ? hval[h] : 0;
+    eseg[e] = sg;
+    atomicAdd(&sstart[sg], 1);
+  }
+  // the cells' rho and clamped u (fluid.py:268-276)
+#pragma unroll
+  for (int k = 0; k < SPT2; ++k) {
+    const int s = tid + k * bs;
+    if (s < ns) {
+      float f[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) f[i] = fmax_nan(fp[k][i], MC_EPSIL);
+      float rho, ux, uy;
+      lbm_macro(f, umax, rho, ux, uy);
+      srho[s] = rho;
+      su0[s] = make_float2(ux, uy);
+      suc[s] = make_float2(ux, uy);
+    }
+  }
+  for (int s = tid + SPT2 * bs; s < ns; s += bs) {
+    float rho;
+    float2 u;
+    cell_state(a.f_in, gi, scell[s], rho, u);
+    srho[s] = rho;
+    su0[s] = u;
+    suc[s] = u;
   }
   __syncthreads();
   {
@@ -487,94 +556,147 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
       const int v = s < ns ? sstart[s] : 0;
       int tot;
       const int inc = block_inclusive_scan(v, wsum, &tot);
-      if (s < ns) sstart[s] = carry + inc - v;
+      if (s < ns) {
+        sstart[s] = carry + inc - v;
+        scur[s] = carry + inc - v;
+      }
       carry += tot;
     }
     if (tid == 0) sstart[ns] = carry;  // == ne
   }
   __syncthreads();
+  // entries to their segments (arrival order), and each node's segments
+  for (int e = tid; e < ne; e += bs) arr[atomicAdd(&scur[eseg[e]], 1)] = e;
+  int seg[K2];
   if (mine) {
 #pragma unroll
-    for (int t = 0; t < K2; ++t)
-      ew[sstart[seg[t]] + sub[t]] = make_float2(__int_as_float(r * K2 + t), w[t]);
+    for (int t = 0; t < K2; ++t) seg[t] = eseg[r * K2 + t];
   }
   __syncthreads();
-  // each segment's entries ascending by entry id (= node id); the kernel-
-  // weight fold in that order and 1/max(kw, 1) (fluid.py:123-133,289-290)
+  // each segment's entries ascending by entry id, then the kernel-weight fold
+  // in that order and 1/max(kw, 1) (fluid.py:123-133,289-290)
   for (int s = tid; s < ns; s += bs) {
     const int lo = sstart[s], hi = sstart[s + 1], len = hi - lo;
     float kw = 0.0f;
     if (len <= 8) {
-      float2 v[8];
+      int v[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        v[k] = k < len ? ew[lo + k] : make_float2(__int_as_float(0x7fffffff), 0.0f);
-#pragma unroll
-      for (int i = 0; i < 7; ++i)
-#pragma unroll
-        for (int k = 0; k < 7 - i; ++k) cswap_key(v[k], v[k + 1]);
+      for (int k = 0; k < 8; ++k) v[k] = k < len ? arr[lo + k] : 0x7fffffff;
+      sort_regs<8>(v);
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (k < len) {
-          ew[lo + k] = v[k];
-          kw = kw + v[k].y;
+          arr[lo + k] = v[k];
+          kw = kw + wt[v[k]];
+        }
+    } else if (len <= 16) {
+      int v[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = k < len ? arr[lo + k] : 0x7fffffff;
+      sort_regs<16>(v);
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if (k < len) {
+          arr[lo + k] = v[k];
+          kw = kw + wt[v[k]];
         }
     } else {
       for (int j = lo + 1; j < hi; ++j) {
-        const float2 v = ew[j];
+        const int v = arr[j];
         int k = j - 1;
-        while (k >= lo && __float_as_int(ew[k].x) > __float_as_int(v.x)) {
-          ew[k + 1] = ew[k];
+        while (k >= lo && arr[k] > v) {
+          arr[k + 1] = arr[k];
           --k;
         }
-        ew[k + 1] = v;
+        arr[k + 1] = v;
       }
-      for (int j = lo; j < hi; ++j) kw = kw + ew[j].y;
+      for (int j = lo; j < hi; ++j) kw = kw + wt[arr[j]];
     }
     sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
   }
   __syncthreads();
+  // a thread's first segment, when it has at most 8 entries, keeps their node
+  // ranks and weights in registers for the spreads
+  constexpr int CSEG = 1;
+  int cr[CSEG][8];
+  float cw[CSEG][8];
+  int clen[CSEG];
+#pragma unroll
+  for (int k = 0; k < CSEG; ++k) {
+    const int s = tid + k * bs;
+    const int lo = s < ns ? sstart[s] : 0, len = s < ns ? sstart[s + 1] - lo : 0;
+    clen[k] = len <= 8 ? len : -1;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = q < len && len <= 8 ? arr[lo + q] : 0;
+      cr[k][q] = e / K2;
+      cw[k][q] = wt[e];
+    }
+  }
   // multi-direct forcing iterations (fluid.py:292-316)
   for (int it = 0; it < c.iters; ++it) {
     if (mine) {
       float2 u[K2];
+      float w[K2];
 #pragma unroll
-      for (int t = 0; t < K2; ++t) u[t] = suc[seg[t]];
+      for (int t = 0; t < K2; ++t) {
+        u[t] = suc[seg[t]];
+        w[t] = wt[r * K2 + t];
+      }
       float ux = 0.0f, uy = 0.0f;
 #pragma unroll
       for (int t = 0; t < K2; ++t) {
         ux = ux + u[t].x * w[t];
         uy = uy + u[t].y * w[t];
       }
-      const float cx = (vb.x - ux) * c.relax;
-      const float cy = (vb.y - uy) * c.relax;
+      const float cx = (d.vb.x - ux) * c.relax;
+      const float cy = (d.vb.y - uy) * c.relax;
       const float mag = sqrtf((cx * cx + cy * cy) + MC_EPSIL);
       const float cf = fmin_nan(1.0f, umax / (mag + MC_EPSIL));
       corr[r] = make_float2(cx * cf, cy * cf);
     }
     __syncthreads();
-    for (int s = tid; s < ns; s += bs) {
-      const int lo = sstart[s], hi = sstart[s + 1];
-      // the first four entries' loads are issued together
-      float2 e4[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) e4[k] = lo + k < hi ? ew[lo + k] : make_float2(0.0f, 0.0f);
-      float2 c4[4];
+    for (int k = 0; k < CSEG; ++k) {
+      const int s = tid + k * bs;
+      if (s >= ns || clen[k] < 0) continue;
+      float2 cv[8];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) c4[k] = corr[__float_as_int(e4[k].x) / K2];
+      for (int q = 0; q < 8; ++q) cv[q] = corr[cr[k][q]];
       float ax = 0.0f, ay = 0.0f;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        if (lo + k < hi) {
-          ax = ax + c4[k].x * e4[k].y;
-          ay = ay + c4[k].y * e4[k].y;
+      for (int q = 0; q < 8; ++q)
+        if (q < clen[k]) {
+          ax = ax + cv[q].x * cw[k][q];
+          ay = ay + cv[q].y * cw[k][q];
         }
-      }
-      for (int j = lo + 4; j < hi; ++j) {
-        const float2 e = ew[j];
-        const float2 cc = corr[__float_as_int(e.x) / K2];
-        ax = ax + cc.x * e.y;
-        ay = ay + cc.y * e.y;
+      const float iv = sinv[s];
+      const float2 u = suc[s];
+      suc[s] = make_float2(u.x + ax * iv, u.y + ay * iv);
+    }
+    for (int s = tid; s < ns; s += bs) {
+      const int k = (s - tid) / bs;
+      if (k < CSEG && clen[k] >= 0) continue;  // done above
+      const int lo = sstart[s], hi = sstart[s + 1];
+      float ax = 0.0f, ay = 0.0f;
+      // eight entries' loads in flight at a time, summed in order
+      for (int j0 = lo; j0 < hi; j0 += 8) {
+        int e[8];
+        float wv[8];
+        float2 cv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) e[k] = j0 + k < hi ? arr[j0 + k] : 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          wv[k] = wt[e[k]];
+          cv[k] = corr[e[k] / K2];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (j0 + k < hi) {
+            ax = ax + cv[k].x * wv[k];
+            ay = ay + cv[k].y * wv[k];
+          }
       }
       const float iv = sinv[s];
       const float2 u = suc[s];
@@ -582,7 +704,8 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
     }
     __syncthreads();
   }
-  // post-clamp and force (fluid.py:327-339); F replaces u_corrected
+  // post-clamp and force (fluid.py:327-339); F replaces u_corrected; the force
+  // and support bits for the LBM pass; the covered cells return to idle
   for (int s = tid; s < ns; s += bs) {
     const int cell = scell[s];
     const float rho = srho[s];
@@ -601,23 +724,21 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
       fy = fy * fc;
     }
     suc[s] = make_float2(fx, fy);
-  }
-  __syncthreads();
-  // global results nothing in this workgroup reads again: force and support
-  // bits for the LBM pass, the idle state of the covered cells
-  for (int s = tid; s < ns; s += bs) {
-    const int cell = scell[s];
-    const float2 F = suc[s];
-    frc[cell] = F.x;
-    frc[HW + cell] = F.y;
+    frc[cell] = fx;
+    frc[HW + cell] = fy;
     atomicOr(&msk[cell >> 5], 1u << (cell & 31));
     own[cell] = -1;
   }
+  __syncthreads();
   // node drag (fluid.py:364-382) and the final wrap (simulate.py:97)
   if (mine) {
     float2 F[K2];
+    float w[K2];
 #pragma unroll
-    for (int t = 0; t < K2; ++t) F[t] = suc[seg[t]];
+    for (int t = 0; t < K2; ++t) {
+      F[t] = suc[seg[t]];
+      w[t] = wt[r * K2 + t];
+    }
     float fx = 0.0f, fy = 0.0f;
 #pragma unroll
     for (int t = 0; t < K2; ++t) {
@@ -626,10 +747,10 @@ __device__ __forceinline__ void cluster_small(const ClusterArgs& a, const InstGe
     }
     const float dvx = (fx / c.node_mass) / (gi.scale_x * c.dt);
     const float dvy = (fy / c.node_mass) / (gi.scale_y * c.dt);
-    a.vel_io[2 * n] = v0.x - dvx;
-    a.vel_io[2 * n + 1] = v0.y - dvy;
-    a.pos_out[2 * n] = fmodf_floor(p0.x, gi.dom_wf);
-    a.pos_out[2 * n + 1] = fmodf_floor(p0.y, gi.dom_hf);
+    const int n = d.n;
+    *reinterpret_cast<float2*>(a.vel_io + 2 * n) = make_float2(d.v0.x - dvx, d.v0.y - dvy);
+    *reinterpret_cast<float2*>(a.pos_out + 2 * n) =
+        make_float2(fmodf_floor(d.p0.x, gi.dom_wf), fmodf_floor(d.p0.y, gi.dom_hf));
   }
 }
 
@@ -672,9 +793,11 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
         cseg[g] = seg;
         X.scell[seg] = g;
         own[g] = -1;
-        // the cell's rho and clamped u from the stencil pass
-        const float2 u = a.st_u[(size_t)n * K2 + t];
-        X.srho[seg] = a.st_rho[(size_t)n * K2 + t];
+        // the cell's rho and clamped u (fluid.py:268-276)
+        float rho;
+        float2 u;
+        cell_state(a.f_in, gi, g, rho, u);
+        X.srho[seg] = rho;
         X.su0[seg] = u;
         L.suc[seg] = u;
       }
@@ -1053,8 +1176,10 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
         a.cell_seg[cell] = seg;
         L.scell[seg] = (int)cell;  // global cell index
         a.owner[cell] = -1;
-        const float2 u = a.st_u[(size_t)n * K2 + t];
-        L.srho[seg] = a.st_rho[(size_t)n * K2 + t];
+        float rho;
+        float2 u;
+        cell_state(a.f_in, gi, (int)(cell - gi.fl_off), rho, u);
+        L.srho[seg] = rho;
         L.su0[seg] = u;
         L.suc[seg] = u;
         ++seg;
@@ -1207,35 +1332,153 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
   }
 }
 
-// Every thread of a cluster is in this workgroup: the workgroup barrier (with
-// its workgroup-scope fence) orders both the LDS and the global-memory phases.
+// A solve workgroup has read its instance's link bucket: the last workgroup
+// of the launch returns every bucket's count to zero for the next step.
+__device__ __forceinline__ void links_done(const ClusterArgs& a) {
+  if (!a.bucketed || threadIdx.x != 0) return;
+  const int d = __hip_atomic_fetch_add(&a.ictl->done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (d == a.C - 1) {
+    for (int i = 0; i < a.bd.n_inst; ++i)
+      __hip_atomic_store(&a.nlinks[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&a.ictl->done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Bucketed membership, at the end of the cluster's solve: every member's
+// cluster root and the root's node count, which the wide solve reads.
+__device__ __forceinline__ void write_labels(const ClusterArgs& a, const InstGeo& gi, int root,
+                                             unsigned mbits, int nq) {
+  for (int k = 0; k < 32 && (mbits >> k); ++k)
+    if ((mbits >> k) & 1u) a.label[gi.comp0 + (int)threadIdx.x + k * (int)blockDim.x] = root;
+  if (threadIdx.x == 0) a.cl_cnt[root] = nq;
+}
+
+// One workgroup per filament component; the workgroup of a cluster's smallest
+// component solves the cluster (the others return).  Membership comes from a
+// union-find over the instance's link bucket in LDS (bucketed), else from the
+// cluster build's tables.  Every thread of a cluster is in this workgroup: the
+// workgroup barrier (with its workgroup-scope fence) orders both the LDS and
+// the global-memory phases.
 template <int K>
-__global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs a) {
+__global__ void __launch_bounds__(cluster_threads<K>()) k_ib_solve(ClusterArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int wsum[16];
-  __shared__ int s_ns, s_seg;
+  __shared__ int wsum[32];
+  __shared__ int s_ns, s_seg, s_base;
   constexpr int K2 = K * K;
   const int root = blockIdx.x;
-  if (a.label[root] != root) return;
-  const InstGeo& gi = a.bd.geo[a.bd.comp_inst[root]];  // a cluster never spans instances
-  const int nq = a.cl_cnt[root];
-  if (a.wide_wg > 0 && nq > a.wide_min) return;  // solved by the wide workgroups
-  const int ne = nq * K2;
   const int tid = threadIdx.x, bs = blockDim.x;
+  const int inst = a.bd.comp_inst[root];  // a cluster never spans instances
+  const InstGeo& gi = a.bd.geo[inst];
+  if (tid == 0) s_seg = 0;
+  // the rod launch of this step, which read the actuation counter, has completed
+  if (root == 0 && tid == 0 && a.tick) *a.tick = *a.tick + 1;
+  int nq = 0;
+  unsigned mbits = 0;  // bucketed: member components tid + k bs, bit k
+  // The cluster's node ids, ascending (the rank order of every fold), at the
+  // top of the LDS when they fit there (twice: member order and ranked).
+  // Bucketed membership keeps its tables at the bottom until the list is built.
+  size_t in_bytes = 0;
+  bool stage_lds = false;
+  int* nodes_in = nullptr;
+  int* ranked = nullptr;
   CLayout G;
   GExtra X;
-  glayout(a.gscratch + (size_t)a.cl_node_off[root] * a.bpn, nq, ne, G, X);
-  if (tid == 0) s_seg = 0;
-
-  // The cluster's node ids, ascending (the rank order of every fold), at the
-  // top of the LDS when they fit there (twice: member order and ranked),
-  // else in global scratch.
-  const size_t in_bytes = al16(4 * (size_t)nq);
-  const bool stage_lds = 8 * in_bytes <= a.lds_bytes;
-  int* nodes_in = stage_lds ? reinterpret_cast<int*>(smem + a.lds_bytes - in_bytes) : G.tmp;
-  int* ranked = stage_lds ? reinterpret_cast<int*>(smem + a.lds_bytes - 2 * in_bytes) : G.nodes;
-  const size_t avail = stage_lds ? a.lds_bytes - 2 * in_bytes : a.lds_bytes;
-  {
+  if (a.bucketed) {
+    const int c0 = gi.comp0, nc = gi.ncomp;
+    const int ncp = (nc + 3) & ~3;
+    int* mem = reinterpret_cast<int*>(smem);  // members, ascending
+    int* par = mem + ncp;                     // union-find forest
+    int* cnn = par + ncp;                     // per component: node count
+    int* cnf = cnn + ncp;                     // first id (contiguous) or ~(list offset)
+    // the instance's component records and its links, all loads in flight
+    const int nl = ld_l2(&a.nlinks[inst]);
+    const int2* lk = a.links + a.link_off[inst];
+    const int2 l0 = tid < nl ? lk[tid] : make_int2(c0, c0);
+    for (int i = tid; i < nc; i += bs) {
+      const CompMeta m = a.meta[c0 + i];
+      par[i] = i;
+      cnn[i] = m.nn;
+      cnf[i] = m.nf >= 0 ? m.nf : ~m.n0;
+    }
+    __syncthreads();
+    uf_union(par, l0.x - c0, l0.y - c0);
+    for (int i = tid + bs; i < nl; i += bs) {
+      const int2 l = lk[i];
+      uf_union(par, l.x - c0, l.y - c0);
+    }
+    __syncthreads();
+    const int me = root - c0;
+    if (uf_find(par, me) != me) {
+      links_done(a);
+      return;
+    }
+    // members (ascending component ids: ascending smallest node ids) and their
+    // node count; every member's label and the root's count for the wide solve
+    int carry = 0, nodes = 0;
+    for (int b0 = 0, k = 0; b0 < nc; b0 += bs, ++k) {
+      const int i = b0 + tid;
+      const bool in = i < nc && uf_find(par, i) == me;
+      const int2 v = make_int2(in ? 1 : 0, in ? cnn[i] : 0);
+      int2 tot;
+      const int2 inc = block_inclusive_scan2(v, wsum, &tot);
+      if (in) {
+        mem[carry + inc.x - 1] = i;
+        mbits |= 1u << k;
+      }
+      carry += tot.x;
+      nodes += tot.y;
+    }
+    const int nm = carry;
+    nq = nodes;
+    if (a.wide_wg > 0 && nq > a.wide_min) {
+      // solved by the wide workgroups
+      write_labels(a, gi, root, mbits, nq);
+      if (tid == 0) {
+        atomicAdd(&a.wctl->wide, 1);
+        atomicAdd(&a.wctl->wide_nodes, nq);
+      }
+      links_done(a);
+      return;
+    }
+    in_bytes = al16(4 * (size_t)nq);
+    stage_lds = 8 * in_bytes <= a.lds_bytes && 16 * (size_t)ncp + 2 * in_bytes <= a.lds_bytes;
+    if (!stage_lds && tid == 0) s_base = atomicAdd(&a.ictl->scratch_top, nq);
+    __syncthreads();
+    if (!stage_lds) glayout(a.gscratch + (size_t)s_base * a.bpn, nq, nq * K2, G, X);
+    nodes_in = stage_lds ? reinterpret_cast<int*>(smem + a.lds_bytes - in_bytes) : G.tmp;
+    ranked = stage_lds ? reinterpret_cast<int*>(smem + a.lds_bytes - 2 * in_bytes) : G.nodes;
+    // members in order: their node ranges concatenated (member k's first
+    // list position into par[k], then one node per thread)
+    carry = 0;
+    for (int b0 = 0; b0 < nm; b0 += bs) {
+      const int i = b0 + tid;
+      const int cnt = i < nm ? cnn[mem[i]] : 0;
+      int tot;
+      const int inc = block_inclusive_scan(cnt, wsum, &tot);
+      if (i < nm) par[i] = carry + inc - cnt;
+      carry += tot;
+    }
+    __syncthreads();
+    for (int i = tid; i < nq; i += bs) {
+      int lo = 0, hi = nm - 1;  // last member whose first position is <= i
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (par[mid] <= i) lo = mid; else hi = mid - 1;
+      }
+      const int f = cnf[mem[lo]], k = i - par[lo];
+      nodes_in[i] = f >= 0 ? f + k : a.comp_nodes[~f + k];
+    }
+    __syncthreads();
+  } else {
+    if (a.label[root] != root) return;
+    nq = a.cl_cnt[root];
+    if (a.wide_wg > 0 && nq > a.wide_min) return;  // solved by the wide workgroups
+    in_bytes = al16(4 * (size_t)nq);
+    stage_lds = 8 * in_bytes <= a.lds_bytes;
+    glayout(a.gscratch + (size_t)a.cl_node_off[root] * a.bpn, nq, nq * K2, G, X);
+    nodes_in = stage_lds ? reinterpret_cast<int*>(smem + a.lds_bytes - in_bytes) : G.tmp;
+    ranked = stage_lds ? reinterpret_cast<int*>(smem + a.lds_bytes - 2 * in_bytes) : G.nodes;
+    const size_t avail = stage_lds ? a.lds_bytes - 2 * in_bytes : a.lds_bytes;
     // members ordered by their smallest node id, so the gathered list is
     // already ascending unless components interleave; one record load per
     // member (node count, first id of a contiguous id range, list offset)
@@ -1284,15 +1527,10 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
     }
     __syncthreads();
   }
+  const int ne = nq * K2;
+  const size_t avail = stage_lds ? a.lds_bytes - 2 * in_bytes : a.lds_bytes;
   int unsorted = 0;
-  int won = 0;
-  for (int i = tid; i < nq; i += bs) {
-    if (i + 1 < nq && nodes_in[i] > nodes_in[i + 1]) unsorted = 1;
-    won += __popcll(a.st_won[nodes_in[i]]);
-  }
-  int tot;
-  block_inclusive_scan(won, wsum, &tot);
-  if (tid == 0) s_ns = tot;  // segments = taps that won their cell
+  for (int i = tid; i + 1 < nq; i += bs) unsorted |= nodes_in[i] > nodes_in[i + 1] ? 1 : 0;
   const int* nodes = nodes_in;
   if (__syncthreads_or(unsorted)) {
     if (stage_lds) {
@@ -1313,14 +1551,46 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_cluster(ClusterArgs
       __syncthreads();
       sort_unique<int>(ranked, nq, reinterpret_cast<int*>(smem), 0, G.eidx, 0x7fffffff);
     }
+    __syncthreads();
     nodes = ranked;
   }
+  if (nq <= bs && stage_lds) {
+    // the node of this thread's rank: all its data in one round trip; the
+    // segments are numbered by the exclusive prefix of the won counts
+    NodeIn<K2> d;
+    d.won = 0;
+    if (tid < nq) load_node<K2>(a, nodes[tid], d);
+    const int cnt = tid < nq ? __popcll(d.won) : 0;
+    int ns;
+    const int inc = block_inclusive_scan(cnt, wsum, &ns);
+    const int bits = table_bits(ns);
+    if (small_bytes(nq, ne, ns, bits) <= avail) {
+      solve_small<K>(a, gi, smem, d, nq, ns, inc - cnt, bits, wsum);
+      if (a.bucketed) {
+        write_labels(a, gi, root, mbits, nq);
+        links_done(a);
+      }
+      return;
+    }
+    if (tid == 0) s_ns = ns;
+  } else {
+    int won = 0;
+    for (int i = tid; i < nq; i += bs) won += __popcll(a.st_won[nodes[i]]);
+    int tot;
+    block_inclusive_scan(won, wsum, &tot);
+    if (tid == 0) s_ns = tot;  // segments = taps that won their cell
+  }
+  // beyond the small solve: the general solve in global scratch (a bucketed
+  // cluster whose list was staged in LDS takes its scratch now)
+  if (tid == 0 && a.bucketed && stage_lds) s_base = atomicAdd(&a.ictl->scratch_top, nq);
+  if (tid == 0 && !a.bucketed) s_base = a.cl_node_off[root];
   __syncthreads();
-  const int ns = s_ns;
-  if (nq <= bs && stage_lds && small_bytes(nq, ne, ns) <= avail)
-    cluster_small<K>(a, gi, smem, nodes, nq, ns, wsum, &s_seg);
-  else
-    cluster_general<K>(a, gi, smem, avail, G, X, nodes, nq, ns, wsum, &s_seg);
+  glayout(a.gscratch + (size_t)s_base * a.bpn, nq, ne, G, X);
+  cluster_general<K>(a, gi, smem, avail, G, X, nodes, nq, s_ns, wsum, &s_seg);
+  if (a.bucketed) {
+    write_labels(a, gi, root, mbits, nq);
+    links_done(a);
+  }
 }
 
 // The wide solve: wide_wg workgroups launched after the cluster solve (a
@@ -1355,11 +1625,13 @@ __global__ void __launch_bounds__(WIDE_THREADS) k_ib_wide(ClusterArgs a) {
 }
 
 __global__ void k_ib_reset(long long HW, int C, int* __restrict__ owner, int* __restrict__ parent,
-                           int* __restrict__ nlinks, int* __restrict__ wcount,
-                           WideCtl* __restrict__ wctl) {
+                           int* __restrict__ nlinks, int nbuckets, int* __restrict__ wcount,
+                           WideCtl* __restrict__ wctl, IbCtl* __restrict__ ictl) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbuckets) nlinks[i] = 0;
   if (i == 0) {
-    *nlinks = 0;
+    ictl->scratch_top = 0;
+    ictl->done = 0;
     if (wctl) __hip_atomic_store(&wctl->err, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (i < HW) {
@@ -1396,7 +1668,7 @@ hipError_t ib_prepare(int lds_max) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 #define SET_LDS(KK)                                                                     \
   if (e == hipSuccess)                                                                  \
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ib_cluster<KK>),           \
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ib_solve<KK>),             \
                             hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   SET_LDS(1) SET_LDS(2) SET_LDS(3) SET_LDS(4) SET_LDS(5) SET_LDS(6)
 #undef SET_LDS
@@ -1414,10 +1686,11 @@ hipError_t ib_prepare(int lds_max) {
   }
 
 hipError_t launch_ib_reset(const IBWork& w, hipStream_t s) {
-  const long long n = std::max(w.cells, (long long)w.C);
+  const long long n = std::max(std::max(w.cells, (long long)w.C), (long long)w.nbuckets);
   hipLaunchKernelGGL(k_ib_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w.cells, w.C,
-                     w.owner, w.uf_parent, w.nlinks, w.wide_wg > 0 ? w.wide_count : nullptr,
-                     w.wide_wg > 0 ? w.wide_ctl : nullptr);
+                     w.owner, w.uf_parent, w.nlinks, w.nbuckets,
+                     w.wide_wg > 0 ? w.wide_count : nullptr, w.wide_wg > 0 ? w.wide_ctl : nullptr,
+                     w.ictl);
   return hipGetLastError();
 }
 
@@ -1436,21 +1709,25 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
     if (e != hipSuccess) return e;
   }
   w.dirty = true;
-#define L_STENCIL(KK)                                                                            \
-  hipLaunchKernelGGL(k_ib_stencil<KK>,                                                           \
+  WideCtl* wctl = w.wide_wg > 0 ? w.wide_ctl : nullptr;
+#define L_CLAIM(KK)                                                                               \
+  hipLaunchKernelGGL(k_ib_claim<KK>,                                                              \
                      dim3((unsigned)(((long long)N * stencil_lanes<KK>() + 255) / 256)), dim3(256), \
-                     0, s, c, b, N, pos, vel_io, f_in, w.comp_of_node, w.st_sx, w.st_sy, w.st_w,  \
-                     w.st_won, w.st_rho, w.st_u, w.vib, w.owner, w.links, w.nlinks)
-  MC_KSWITCH(L_STENCIL);
-#undef L_STENCIL
+                     0, s, c, b, N, pos, vel_io, w.comp_of_node, w.st_sx, w.st_sy, w.st_w,         \
+                     w.st_won, w.vib, w.owner, w.links, w.nlinks, w.link_off, w.bucketed ? 1 : 0,  \
+                     wctl, w.ictl)
+  MC_KSWITCH(L_CLAIM);
+#undef L_CLAIM
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ib_cluster_build, dim3(1), dim3(1024), w.lds_bytes, s, C,
-                     (int)(w.lds_bytes / sizeof(int)), w.comp_node_off, w.links, w.nlinks,
-                     w.uf_parent, w.label, w.cl_cnt, w.cl_node_off, w.cl_mem_cnt, w.cl_mem_off,
-                     w.cl_members, tick, w.wide_wg > 0 ? w.wide_ctl : nullptr, w.wide_min);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (!w.bucketed) {
+    hipLaunchKernelGGL(k_ib_cluster_build, dim3(1), dim3(1024), w.lds_bytes, s, C,
+                       (int)(w.lds_bytes / sizeof(int)), w.comp_node_off, w.links, w.nlinks,
+                       w.uf_parent, w.label, w.cl_cnt, w.cl_node_off, w.cl_mem_cnt, w.cl_mem_off,
+                       w.cl_members, nullptr, wctl, w.wide_min);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   ClusterArgs a;
   a.c = c;
   a.comp_node_off = w.comp_node_off;
@@ -1465,8 +1742,6 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   a.st_sx = w.st_sx;
   a.st_sy = w.st_sy;
   a.st_won = w.st_won;
-  a.st_rho = w.st_rho;
-  a.st_u = w.st_u;
   a.st_w = w.st_w;
   a.vib = w.vib;
   a.f_in = f_in;
@@ -1481,6 +1756,11 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   a.gscratch = w.cl_scratch;
   a.bpn = w.cl_bpn;
   a.lds_bytes = w.lds_bytes;
+  a.bucketed = w.bucketed ? 1 : 0;
+  a.links = w.links;
+  a.nlinks = w.nlinks;
+  a.link_off = w.link_off;
+  a.ictl = w.ictl;
   a.N = N;
   a.C = C;
   a.wide_wg = w.wide_wg;
@@ -1489,8 +1769,9 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   a.wscratch = w.wide_scratch;
   a.wcount = w.wide_count;
   a.wctl = w.wide_ctl;
+  a.tick = tick;
 #define L_CLUSTER(KK) \
-  hipLaunchKernelGGL(k_ib_cluster<KK>, dim3(C), dim3(cluster_threads<KK>()), w.lds_bytes, s, a)
+  hipLaunchKernelGGL(k_ib_solve<KK>, dim3(C), dim3(cluster_threads<KK>()), w.lds_bytes, s, a)
   MC_KSWITCH(L_CLUSTER);
 #undef L_CLUSTER
   e = hipGetLastError();

@@ -119,7 +119,9 @@ struct RodLayoutDev {
 struct CompMeta {
   int n0, nn, e0, ne, b0, nb, nni, nei;  // node / edge / hinge ranges, incidence counts
   int nf, ef, hf;                        // first ids of contiguous ranges (-1: listed)
-  int blob_off, inst, pad0, pad1, pad2;
+  int blob_off, inst;
+  int chain;                             // 1: a chain (the rod one-wavefront path)
+  int pad1, pad2;
 };
 
 struct Actuation {           // rest_b(t) = amp_b * sin(omega_b * t + phase_b)
@@ -188,7 +190,8 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
                       const float* theta_in, const float* rest_len, const float* rest_ang,
                       const float* kb, const Actuation* act, const float* steric,
                       const float* pred_w, float* pos_out, float* vel_out, float* debug_out,
-                      float* theta_out, const LbmArgs* lbm, hipStream_t s);
+                      float* theta_out, const LbmArgs* lbm, hipStream_t s,
+                      bool all_chains = false);
 // Raises the dynamic LDS limit of the rod kernels to lds_max - 1024 bytes.
 hipError_t rod_prepare(int lds_max);
 hipError_t launch_steric_nodes(const Consts& c, const BatchDev& b, int N, const float* steric,
@@ -260,12 +263,17 @@ struct IBWork {
   int *st_sx, *st_sy;          // per node wrapped stencil start cell
   float *st_w;                 // per node K*K weights
   unsigned long long* st_won;  // per node: bit t set when tap t claimed its cell
-  float* st_rho;               // per node tap that claimed its cell: rho of the cell
-  float2* st_u;                // ... and its clamped u (fluid.py:268-276)
   float *vib;                  // per node (2) IB target velocity
   int *owner, *cell_seg;       // per lattice cell: claiming component (-1 idle), segment
   int2* links;                 // component pairs whose stencils share a cell
-  int* nlinks;                 // number of links (0 between calls)
+  int* nlinks;                 // links per bucket (0 between calls)
+  // bucketed: one link bucket per instance (capacity: its nodes x K2, from
+  // link_off[inst]) and membership by a union-find per solve workgroup; else
+  // one list and the single-workgroup cluster build
+  bool bucketed;
+  int nbuckets;
+  int* link_off;
+  struct IbCtl* ictl;
   int *uf_parent, *label;      // per component: union-find parent (global fallback), cluster root
   int *cl_cnt, *cl_node_off, *cl_mem_cnt, *cl_mem_off, *cl_members;  // per cluster root
   unsigned char* cl_scratch;   // cl_bpn bytes per node, cluster-major
@@ -284,6 +292,18 @@ struct IBWork {
   int* wide_count;             // per lattice cell entry counter (0 between calls)
   struct WideCtl* wide_ctl;
 };
+// Per-step counters of the cluster solve: the scratch allocator (nodes
+// taken this step; cleared by the next claim launch) and the count of solve
+// workgroups done with the link buckets (the last one clears the buckets).
+struct IbCtl {
+  int scratch_top;
+  int done;
+};
+// Instances of at most this many filament components take the bucketed
+// membership (a union-find over the instance's components in each solve
+// workgroup's LDS).
+constexpr int IB_UF_MAX = 8192;
+
 // Control block of the wide solve: grid-barrier counter, wide-cluster and
 // wide-node counts, segment and node counters, the per-workgroup partial sums
 // of the grid scans, and the barrier-timeout record.  A barrier wait gives up

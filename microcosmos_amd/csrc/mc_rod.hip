@@ -394,9 +394,203 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
   for (int e = threadIdx.x; e < ne; e += blockDim.x) a.theta_out[L.comp_edges[e0 + e]] = TH[e];
 }
 
+
+// DPP wavefront shifts (GFX9 wave_shl:1 / wave_shr:1, every lane active): lane
+// i receives the value of lane i + 1 / i - 1; the end lane receives 0.
+__device__ __forceinline__ float from_next(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_prev(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+// Rod solve of a chain component (a line filament of at most 64 nodes whose
+// node, edge and hinge ids are contiguous, edge e = (node e, node e + 1),
+// hinge b = (edge b, edge b + 1)): one wavefront, lane i holding node i,
+// edge i and hinge i in registers.  The incidence lists of a chain are fixed:
+// edge e takes +corr(e) (as e_in of hinge e) then -corr(e - 1) (as e_out of
+// hinge e - 1); node i takes +F(i) (as src of edge i) then -F(i - 1) (as tgt
+// of edge i - 1).  The neighbour values come from the adjacent lanes by DPP
+// shifts, so a cycle needs no LDS traffic and no barrier.  Every result is
+// the general path's, operation for operation (a division by a degree of 1
+// or 2 is the exact multiplication by 1 or 0.5).
+__device__ __forceinline__ void rod_chain(const RodArgs& a, const CompMeta& m, float* smem) {
+  const Consts& c = a.c;
+  const InstGeo& gi = a.bd.geo[m.inst];
+  const int lane = threadIdx.x;
+  const int nn = m.nn, ne = m.ne, nb = m.nb;
+  const bool isn = lane < nn, ise = lane < ne, isb = lane < nb;
+  const int g = m.nf + lane, ge = m.ef + lane, gb = m.hf + lane;
+  const bool is_cos = c.solver == 1;
+  const int nsk = c.steric_nskip;
+  const bool stage_pw = c.enable_steric && nsk <= ROD_WIN_MAX;
+  const int pw0 = m.nf - nsk;
+  // every independent global load first
+  float2 pwv[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int i = lane + 64 * h, j = pw0 + i;
+    pwv[h] = (stage_pw && i < nn + 2 * nsk && j >= gi.node0 && j < gi.node0 + gi.nodes)
+                 ? *reinterpret_cast<const float2*>(a.pred_w + 2 * (size_t)j)
+                 : make_float2(0.0f, 0.0f);
+  }
+  float2 q = make_float2(0.0f, 0.0f), v = make_float2(0.0f, 0.0f);
+  if (isn) {
+    q = *reinterpret_cast<const float2*>(a.pos_in + 2 * (size_t)g);
+    v = *reinterpret_cast<const float2*>(a.vel_in + 2 * (size_t)g);
+  }
+  float th = 0.0f, l = 1.0f;
+  if (ise) {
+    th = a.theta_in[ge];
+    l = a.rest_len[ge];
+  }
+  float am = 0.0f, om = 0.0f, ph = 0.0f, rs = 0.0f, kb = 0.0f;
+  if (isb) {
+    if (a.has_act) {
+      am = a.act.amp[gb];
+      om = a.act.omega[gb];
+      ph = a.act.phase[gb];
+    } else {
+      rs = a.rest_ang[gb];
+    }
+    kb = a.kb[gb];
+  }
+  const double tact =
+      a.has_act ? (double)(*a.act.t + (a.act.t_inst ? a.act.t_inst[m.inst] : 0LL)) : 0.0;
+  float2* pw = reinterpret_cast<float2*>(smem);
+  if (stage_pw) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      if (lane + 64 * h < nn + 2 * nsk) pw[lane + 64 * h] = pwv[h];
+    __syncthreads();
+  }
+  // prologue: damping, prediction, steric push, Cosserat target (simulate.py:41-72)
+  float X = 0.0f, Y = 0.0f, TX = 0.0f, TY = 0.0f;
+  if (isn) {
+    const float dvx = v.x * c.damping, dvy = v.y * c.damping;
+    X = q.x + dvx * c.dt;
+    Y = q.y + dvy * c.dt;
+    if (c.enable_steric) {
+      float fx, fy;
+      steric_force(c, gi, a.steric, a.pred_w, g, fx, fy, stage_pw ? pw : nullptr, pw0);
+      X = X + (fx * c.steric_strength) * c.dt;
+      Y = Y + (fy * c.steric_strength) * c.dt;
+    }
+    TX = c.cos_target ? X : 0.0f;
+    TY = c.cos_target ? Y : 0.0f;
+    a.debug_out[2 * (size_t)g] = TX;
+    a.debug_out[2 * (size_t)g + 1] = TY;
+  }
+  if (isb && a.has_act) rs = (float)((double)am * sin((double)om * tact + (double)ph));
+  // per-edge constants: degrees of the edge and of the node, the Cosserat
+  // stiffness terms (stable_cosserat.py:186-195)
+  const float deg_e = (lane < nb && lane >= 1) ? 0.5f : 1.0f;
+  const float deg_n = (lane < ne && lane >= 1) ? 0.5f : 1.0f;
+  const float kl = c.k_st / l, he = ise ? c.k_st / (l * l) : 0.0f;
+  const float Wd = gi.dom_wf, Hd = gi.dom_hf;
+  for (int cyc = 0; cyc < c.cycles; ++cyc) {
+    // theta pass: hinge corrections (pbd.py:32-40)
+    const float th_next = from_next(th);
+    float corr = 0.0f;
+    if (isb) {
+      const float diff = wrap_angle(th_next - th);
+      corr = ((diff - rs) * kb) * 0.5f;
+    }
+    const float corr_prev = from_prev(corr);
+    const float xn = from_next(X), yn = from_next(Y);
+    float fex = 0.0f, fey = 0.0f;
+    if (ise) {
+      float d = 0.0f;
+      if (lane < nb) d = d + corr;
+      if (lane >= 1) d = d + (-corr_prev);
+      const float rx = min_image(xn, X, Wd);
+      const float ry = min_image(yn, Y, Hd);
+      if (!is_cos) {
+        th = th + (d * deg_e) * 1.8f;
+      } else {
+        th = th + d * deg_e;
+        const float alpha = datan2(ry, rx);
+        const float pull = wrap_angle(alpha - th);
+        th = th + pull * 0.05f;
+      }
+      double sd, cd;
+      sincos((double)th, &sd, &cd);
+      const float dx = (float)cd, dy = (float)sd;
+      if (!is_cos) {
+        const float tx = dx * l, ty = dy * l;
+        const float len = sqrtf(rx * rx + ry * ry);
+        const float md = fmax_nan(len, 1e-8f);
+        const float cdx = rx / md, cdy = ry / md;
+        const float stx = cdx * (len - l), sty = cdy * (len - l);
+        const float shx = (rx - tx) - stx, shy = (ry - ty) - sty;
+        fex = ((stx + shx * c.k_sh) * c.k_st) * 0.5f;
+        fey = ((sty + shy * c.k_sh) * c.k_st) * 0.5f;
+      } else {
+        fex = kl * (rx / l - dx);
+        fey = kl * (ry / l - dy);
+      }
+    }
+    // per node: +F(i) as src, then -F(i - 1) as tgt (pbd.py:84-89 /
+    // stable_cosserat.py:197-207)
+    const float fpx = from_prev(fex), fpy = from_prev(fey);
+    const float hp = from_prev(he);
+    if (isn) {
+      if (!is_cos) {
+        float ux = 0.0f, uy = 0.0f;
+        if (lane < ne) {
+          ux = ux + fex;
+          uy = uy + fey;
+        }
+        if (lane >= 1) {
+          ux = ux + (-fpx);
+          uy = uy + (-fpy);
+        }
+        X = X + ux * deg_n;
+        Y = Y + uy * deg_n;
+      } else {
+        const float w = c.cos_w;
+        float fx = w * (TX - X), fy = w * (TY - Y);
+        float h = w;
+        if (lane < ne) {
+          fx = fx + fex;
+          fy = fy + fey;
+        }
+        if (lane >= 1) {
+          fx = fx + (-fpx);
+          fy = fy + (-fpy);
+        }
+        if (lane < ne) h = h + he;
+        if (lane >= 1) h = h + hp;
+        X = X + fx / h;
+        Y = Y + fy / h;
+      }
+    }
+  }
+  // epilogue: v = (x - x_prev) / dt (simulate.py:83-84); unwrapped x feeds the
+  // immersed boundary, or is wrapped here when the fluid is off
+  if (isn) {
+    const float2 vo = make_float2((X - q.x) / c.dt, (Y - q.y) / c.dt);
+    *reinterpret_cast<float2*>(a.vel_out + 2 * (size_t)g) = vo;
+    const float2 po = c.enable_fluid ? make_float2(X, Y)
+                                     : make_float2(fmodf_floor(X, Wd), fmodf_floor(Y, Hd));
+    *reinterpret_cast<float2*>(a.pos_out + 2 * (size_t)g) = po;
+  }
+  if (ise) a.theta_out[ge] = th;
+}
+
+// The chain path runs the whole node phase of a step (mode 0).  CHAINS: every
+// component is a chain, so the general path (and its registers) is left out.
+template <bool CHAINS>
+__device__ __forceinline__ void rod_dispatch(const RodArgs& a, int comp, float* smem) {
+  const CompMeta m = a.L.meta[comp];
+  if (CHAINS || (m.chain && a.mode == 0 && a.use_lds)) rod_chain(a, m, smem);
+  else rod_component(a, comp, smem);
+}
+
+template <bool CHAINS>
 __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
   extern __shared__ float smem[];
-  rod_component(a, blockIdx.x, smem);
+  rod_dispatch<CHAINS>(a, blockIdx.x, smem);
 }
 
 // Rod components in workgroups [0, rod_blocks), then an LBM pass in 64-cell
@@ -404,11 +598,12 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
 // U-mask words stay within a wavefront).  rod_blocks is a multiple of 8:
 // workgroup i runs on XCD i % 8, and the LBM part keeps the standalone
 // kernel's XCD-banded order.
+template <bool CHAINS>
 __global__ void __launch_bounds__(256) k_rod_lbm(RodArgs a, LbmArgs l, int rod_blocks) {
   extern __shared__ float smem[];
   const int bid = blockIdx.x;
   if (bid < rod_blocks) {
-    if (bid < a.L.num_comp) rod_component(a, bid, smem);
+    if (bid < a.L.num_comp) rod_dispatch<CHAINS>(a, bid, smem);
     return;
   }
   int v = bid - rod_blocks;
@@ -463,12 +658,15 @@ hipError_t launch_set_tick(long long* t, long long value, hipStream_t s) {
 hipError_t rod_prepare(int lds_max) {
   // components beyond 64 KB of LDS (about 2000 nodes) need the raised limit
   const int lds = lds_max - 1024;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rod),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rod_lbm),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  return e;
+  const void* fns[] = {reinterpret_cast<const void*>(&k_rod<false>),
+                       reinterpret_cast<const void*>(&k_rod<true>),
+                       reinterpret_cast<const void*>(&k_rod_lbm<false>),
+                       reinterpret_cast<const void*>(&k_rod_lbm<true>)};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L, int N,
@@ -476,7 +674,8 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
                       const float* y_in, const float* theta_in, const float* rest_len,
                       const float* rest_ang, const float* kb, const Actuation* act,
                       const float* steric, const float* pred_w, float* pos_out, float* vel_out,
-                      float* debug_out, float* theta_out, const LbmArgs* lbm, hipStream_t s) {
+                      float* debug_out, float* theta_out, const LbmArgs* lbm, hipStream_t s,
+                      bool all_chains) {
   if (L.num_comp == 0 && !lbm) return hipSuccess;
   RodArgs a;
   a.c = c;
@@ -502,11 +701,18 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
   a.debug_out = debug_out;
   a.theta_out = theta_out;
   const size_t lds = use_lds ? lds_bytes : 0;
+  const bool chains = all_chains && mode == 0 && use_lds;
   if (lbm) {
     const int rb = (L.num_comp + 7) & ~7;
-    hipLaunchKernelGGL(k_rod_lbm, dim3(rb + 4 * b.lbm_blocks), dim3(64), lds, s, a, *lbm, rb);
+    if (chains)
+      hipLaunchKernelGGL(k_rod_lbm<true>, dim3(rb + 4 * b.lbm_blocks), dim3(64), lds, s, a, *lbm, rb);
+    else
+      hipLaunchKernelGGL(k_rod_lbm<false>, dim3(rb + 4 * b.lbm_blocks), dim3(64), lds, s, a, *lbm, rb);
   } else {
-    hipLaunchKernelGGL(k_rod, dim3(L.num_comp), dim3(64), lds, s, a);
+    if (chains)
+      hipLaunchKernelGGL(k_rod<true>, dim3(L.num_comp), dim3(64), lds, s, a);
+    else
+      hipLaunchKernelGGL(k_rod<false>, dim3(L.num_comp), dim3(64), lds, s, a);
   }
   return hipGetLastError();
 }
