@@ -386,10 +386,11 @@ __device__ __forceinline__ int cell_slot(int cell, int bits) {
   return (int)(((unsigned)cell * 2654435761u) >> (32 - bits));
 }
 
-// Table bits for ns segments: the smallest power of two >= 2 ns, at least 64.
+// Table bits for ns segments: the smallest power of two with a load factor of
+// at most 0.8, at least 64 slots.
 __device__ __forceinline__ int table_bits(int ns) {
   int b = 6;
-  while ((1 << b) < 2 * ns) ++b;
+  while ((4 << b) < 5 * ns) ++b;
   return b;
 }
 
@@ -416,13 +417,16 @@ __device__ __forceinline__ void load_node(const ClusterArgs& a, int n, NodeIn<K2
 }
 
 // LDS bytes of the small solve: the cell -> segment table (2^bits keys and
-// values), per node its stencil start and won mask (16 B) and correction (8
-// B), per segment its entry start, cursor, cell, 1/kw, rho (4 B each) and u0,
-// u (8 B each), per entry its id, weight and segment (4 B each).
+// values), which the entries grouped by segment replace once every tap knows
+// its segment; per node its stencil start, segment base and won mask (16 B)
+// and correction (8 B); per segment its entry start, cursor, cell, 1/kw, rho
+// (4 B each) and u0, u (8 B each); per entry its weight and segment (4 B
+// each, tap-major: entry (r, t) at t nq + r).
 __device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns, int bits) {
-  return 2 * al16(4 * ((size_t)1 << bits)) + al16(16 * (size_t)nq) + al16(8 * (size_t)nq) +
+  const size_t table = 8 * ((size_t)1 << bits), grouped = 4 * (size_t)ne;
+  return al16(table > grouped ? table : grouped) + al16(16 * (size_t)nq) + al16(8 * (size_t)nq) +
          al16(4 * ((size_t)ns + 1)) + 4 * al16(4 * (size_t)ns) + 2 * al16(8 * (size_t)ns) +
-         3 * al16(4 * (size_t)ne);
+         2 * al16(4 * (size_t)ne);
 }
 
 // Small-cluster solve: one node per thread for the node phases (thread r
@@ -458,8 +462,11 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
     off += al16(b);
     return p;
   };
-  int* hkey = (int*)take(4 * (size_t)T);
-  int* hval = (int*)take(4 * (size_t)T);
+  const size_t table = 8 * (size_t)T, grouped = 4 * (size_t)ne;
+  unsigned char* hreg = take(table > grouped ? table : grouped);
+  int* hkey = (int*)hreg;
+  int* hval = hkey + T;
+  int* arr = (int*)hreg;  // replaces the table once every tap knows its segment
   int4* nd = (int4*)take(16 * (size_t)nq);  // per node: sx | sy << 16, seg0, won (low, high)
   float2* corr = (float2*)take(8 * (size_t)nq);
   int* sstart = (int*)take(4 * ((size_t)ns + 1));
@@ -469,9 +476,9 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
   float* srho = (float*)take(4 * (size_t)ns);
   float2* su0 = (float2*)take(8 * (size_t)ns);
   float2* suc = (float2*)take(8 * (size_t)ns);
-  int* arr = (int*)take(4 * (size_t)ne);
-  float* wt = (float*)take(4 * (size_t)ne);
+  float* wt = (float*)take(4 * (size_t)ne);   // tap-major: entry (r, t) at t nq + r
   int* eseg = (int*)take(4 * (size_t)ne);
+  auto tix = [&](int e) { return (e % K2) * nq + e / K2; };
 
   const bool mine = tid < nq;
   const int r = tid;
@@ -480,17 +487,17 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
   if (mine) {
     nd[r] = make_int4(d.sx | (d.sy << 16), seg0, (int)(unsigned)d.won, (int)(unsigned)(d.won >> 32));
 #pragma unroll
-    for (int t = 0; t < K2; ++t) wt[r * K2 + t] = d.w[t];
+    for (int t = 0; t < K2; ++t) wt[t * nq + r] = d.w[t];
   }
   __syncthreads();
   auto ecell = [&](const int4& q, int t) {
     const int sx = q.x & 0xffff, sy = (int)((unsigned)q.x >> 16);
     return wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W);
   };
-  // won taps (one entry per thread) enter their cell in the table: tap t of
-  // node r is segment seg0(r) + (won taps of r before t)
-  for (int e = tid; e < ne; e += bs) {
-    const int rr = e / K2, t = e - rr * K2;
+  // won taps (one entry per thread, tap-major) enter their cell in the table:
+  // tap t of node r is segment seg0(r) + (won taps of r before t)
+  for (int i = tid; i < ne; i += bs) {
+    const int t = i / nq, rr = i - t * nq;
     const int4 q = nd[rr];
     const unsigned long long wm = ((unsigned long long)(unsigned)q.w << 32) | (unsigned)q.z;
     if ((wm >> t) & 1ull) {
@@ -516,13 +523,13 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
   }
   // every entry's segment (its cell's table entry; a cell is in the table
   // exactly once) and the segments' entry counts
-  for (int e = tid; e < ne; e += bs) {
-    const int rr = e / K2, t = e - rr * K2;
+  for (int i = tid; i < ne; i += bs) {
+    const int t = i / nq, rr = i - t * nq;
     const int g = ecell(nd[rr], t);
     int h = cell_slot(g, bits);
     for (int k = 0; k < T && hkey[h] != g; ++k) h = (h + 1) & (T - 1);
     const int sg = hkey[h] == g ? hval[h] : 0;
-    eseg[e] = sg;
+    eseg[i] = sg;
     atomicAdd(&sstart[sg], 1);
   }
   // the cells' rho and clamped u (fluid.py:268-276)
@@ -565,12 +572,16 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
     if (tid == 0) sstart[ns] = carry;  // == ne
   }
   __syncthreads();
-  // entries to their segments (arrival order), and each node's segments
-  for (int e = tid; e < ne; e += bs) arr[atomicAdd(&scur[eseg[e]], 1)] = e;
+  // entries to their segments (arrival order; the table is dead), and each
+  // node's segments
+  for (int i = tid; i < ne; i += bs) {
+    const int t = i / nq, rr = i - t * nq;
+    arr[atomicAdd(&scur[eseg[i]], 1)] = rr * K2 + t;
+  }
   int seg[K2];
   if (mine) {
 #pragma unroll
-    for (int t = 0; t < K2; ++t) seg[t] = eseg[r * K2 + t];
+    for (int t = 0; t < K2; ++t) seg[t] = eseg[t * nq + r];
   }
   __syncthreads();
   // each segment's entries ascending by entry id, then the kernel-weight fold
@@ -587,7 +598,7 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
       for (int k = 0; k < 8; ++k)
         if (k < len) {
           arr[lo + k] = v[k];
-          kw = kw + wt[v[k]];
+          kw = kw + wt[tix(v[k])];
         }
     } else if (len <= 16) {
       int v[16];
@@ -598,7 +609,7 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
       for (int k = 0; k < 16; ++k)
         if (k < len) {
           arr[lo + k] = v[k];
-          kw = kw + wt[v[k]];
+          kw = kw + wt[tix(v[k])];
         }
     } else {
       for (int j = lo + 1; j < hi; ++j) {
@@ -610,7 +621,7 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
         }
         arr[k + 1] = v;
       }
-      for (int j = lo; j < hi; ++j) kw = kw + wt[arr[j]];
+      for (int j = lo; j < hi; ++j) kw = kw + wt[tix(arr[j])];
     }
     sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
   }
@@ -630,7 +641,7 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
     for (int q = 0; q < 8; ++q) {
       const int e = q < len && len <= 8 ? arr[lo + q] : 0;
       cr[k][q] = e / K2;
-      cw[k][q] = wt[e];
+      cw[k][q] = wt[tix(e)];
     }
   }
   // multi-direct forcing iterations (fluid.py:292-316)
@@ -641,7 +652,7 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
 #pragma unroll
       for (int t = 0; t < K2; ++t) {
         u[t] = suc[seg[t]];
-        w[t] = wt[r * K2 + t];
+        w[t] = wt[t * nq + r];
       }
       float ux = 0.0f, uy = 0.0f;
 #pragma unroll
@@ -688,7 +699,7 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
         for (int k = 0; k < 8; ++k) e[k] = j0 + k < hi ? arr[j0 + k] : 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-          wv[k] = wt[e[k]];
+          wv[k] = wt[tix(e[k])];
           cv[k] = corr[e[k] / K2];
         }
 #pragma unroll
@@ -737,7 +748,7 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
 #pragma unroll
     for (int t = 0; t < K2; ++t) {
       F[t] = suc[seg[t]];
-      w[t] = wt[r * K2 + t];
+      w[t] = wt[t * nq + r];
     }
     float fx = 0.0f, fy = 0.0f;
 #pragma unroll

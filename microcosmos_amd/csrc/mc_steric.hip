@@ -783,15 +783,32 @@ __global__ void __launch_bounds__(256) k_rows_conv(BatchDev bd, const float* __r
       for (int i = threadIdx.x; i < W; i += blockDim.x) density_out[g0 + i] = a[i].x;
   }
   __syncthreads();
-  for (int x = threadIdx.x; x < W; x += blockDim.x) {
-    float acc = 0.0f;
-    int i = x + lo;
-    if (i < 0) i += W;
-    for (int t = 0; t < nt; ++t) {
-      acc = acc + tw[t] * a[i].x;
-      i = (i + 1 == W) ? 0 : i + 1;
+  // four consecutive outputs per thread share one input window; taps in
+  // chunks of eight whose loads are issued together, each output summed tap
+  // by tap in order
+  for (int x0 = 4 * threadIdx.x; x0 < W; x0 += 4 * blockDim.x) {
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int i0 = x0 + lo;  // >= -W
+    if (i0 < 0) i0 += W;
+    for (int t0 = 0; t0 < nt; t0 += 8) {
+      float win[11], tv[8];
+#pragma unroll
+      for (int m = 0; m < 11; ++m) {
+        int i = i0 + t0 + m;
+        while (i >= W) i -= W;
+        win[m] = a[i].x;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) tv[j] = tw[t0 + j < nt ? t0 + j : 0];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (t0 + j < nt)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) acc[k] = acc[k] + tv[j] * win[k + j];
     }
-    tmp[g0 + x] = acc;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (x0 + k < W) tmp[g0 + x0 + k] = acc[k];
   }
 }
 
@@ -840,13 +857,33 @@ __global__ void __launch_bounds__(256) k_cols_conv_tiled(BatchDev bd, const floa
       if (r0 + k * RS < nin) tile[(r0 + k * RS) * FILTER_TILE_W + c] = v[k];
   }
   __syncthreads();
+  // ROWS consecutive outputs per thread share one input window; taps in
+  // chunks of eight whose loads are issued together, each output summed tap
+  // by tap in order
   constexpr int ROWS = FILTER_TILE_H / (256 / FILTER_TILE_W);
+  const int orow0 = rq * ROWS;
+  float acc[ROWS];
+#pragma unroll
+  for (int k = 0; k < ROWS; ++k) acc[k] = 0.0f;
+  for (int t0 = 0; t0 < nt; t0 += 8) {
+    float win[ROWS + 7], tv[8];
+#pragma unroll
+    for (int m = 0; m < ROWS + 7; ++m) {
+      const int r = orow0 + t0 + m;
+      win[m] = r < nin ? tile[r * FILTER_TILE_W + c] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) tv[j] = ty[t0 + j < nt ? t0 + j : 0];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (t0 + j < nt)
+#pragma unroll
+        for (int k = 0; k < ROWS; ++k) acc[k] = acc[k] + tv[j] * win[k + j];
+  }
+#pragma unroll
   for (int k = 0; k < ROWS; ++k) {
-    const int orow = rq * ROWS + k;
-    float acc = 0.0f;
-    for (int t = 0; t < nt; ++t) acc = acc + ty[t] * tile[(orow + t) * FILTER_TILE_W + c];
-    const int y = ty0 + orow;
-    if (x < W && y < H) out[gi.grid_off + (size_t)y * W + x] = acc;
+    const int y = ty0 + orow0 + k;
+    if (x < W && y < H) out[gi.grid_off + (size_t)y * W + x] = acc[k];
   }
 }
 
