@@ -322,6 +322,20 @@ mc_status mc_point_loss(int32_t kind, const float* particles, int64_t n, const f
                         int64_t m, double alpha, float* loss, float* grad, void* workspace,
                         void* stream);
 
+/* mc_point_loss without a workspace argument and without allocating: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace is a static device array that each device
+ * holds until the process ends (2.25 MiB); n and m are each at most
+ * MC_POINT_LOSS_STATIC_MAX, larger counts return MC_ERR_BAD_SHAPE.  Every call
+ * on one device shares that device's array: calls that overlap on one device
+ * (other streams, other threads) race, so order them on one stream.  Results
+ * are bit-identical to mc_point_loss's. */
+#define MC_POINT_LOSS_STATIC_MAX 65536
+mc_status mc_point_loss_static(int32_t kind, const float* particles, int64_t n,
+                               const float* targets, int64_t m, double alpha, float* loss,
+                               float* grad, void* stream);
+
 /* Stability term of the folding loss (experiments/filament_folding/
  * experiment.py:878-893, `_frame_stab` averaged over the stability frames):
  *   loss = mean_s [ mean(d(frames[s], main)^2) + mean(d(frames[s], anchor)^2) ]

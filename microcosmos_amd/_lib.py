@@ -112,6 +112,8 @@ SIGNATURES = {
     "mc_point_loss_workspace": (c_size_t, [c_int64, c_int64]),
     "mc_point_loss": (c_int32, [c_int32, c_void_p, c_int64, c_void_p, c_int64, c_double, c_void_p,
                                 c_void_p, c_void_p, c_void_p]),
+    "mc_point_loss_static": (c_int32, [c_int32, c_void_p, c_int64, c_void_p, c_int64, c_double,
+                                       c_void_p, c_void_p, c_void_p]),
     "mc_frame_stability": (c_int32, [c_void_p, c_int32, c_int64, c_void_p, c_void_p, c_int32,
                                      c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 }

@@ -1957,19 +1957,44 @@ size_t mc_point_loss_workspace(int64_t n, int64_t m) {
   return n < 0 || m < 0 ? 0 : point_loss_workspace(n, m);
 }
 
-mc_status mc_point_loss(int32_t kind, const float* particles, int64_t n, const float* targets,
-                        int64_t m, double alpha, float* loss, float* grad, void* workspace,
-                        void* stream) {
-  g_err.clear();
+static mc_status check_point_loss(int32_t kind, const float* particles, int64_t n,
+                                  const float* targets, int64_t m, double alpha, const float* loss) {
   if (kind != MC_LOSS_CHAMFER && kind != MC_LOSS_DCD && kind != MC_LOSS_SMOOTH_MAX)
     return fail(MC_ERR_INVALID, "unknown loss kind %d", kind);
   if (n < 1 || m < 1 || n > (1 << 24) || m > (1 << 24))
     return fail(MC_ERR_BAD_SHAPE, "particle and target counts must be in 1 .. 2^24");
-  if (!particles || !targets || !loss || !workspace) return fail(MC_ERR_INVALID, "NULL argument");
+  if (!particles || !targets || !loss) return fail(MC_ERR_INVALID, "NULL argument");
   if (kind == MC_LOSS_SMOOTH_MAX && !(alpha > 0.0))
     return fail(MC_ERR_BAD_SHAPE, "alpha must be positive");
+  return MC_OK;
+}
+
+mc_status mc_point_loss(int32_t kind, const float* particles, int64_t n, const float* targets,
+                        int64_t m, double alpha, float* loss, float* grad, void* workspace,
+                        void* stream) {
+  g_err.clear();
+  const mc_status st = check_point_loss(kind, particles, n, targets, m, alpha, loss);
+  if (st != MC_OK) return st;
+  if (!workspace) return fail(MC_ERR_INVALID, "NULL argument");
   HIPCHK(launch_point_loss(kind, particles, (int)n, targets, (int)m, (float)alpha, loss, grad,
                            workspace, (hipStream_t)stream));
+  return MC_OK;
+}
+
+mc_status mc_point_loss_static(int32_t kind, const float* particles, int64_t n,
+                               const float* targets, int64_t m, double alpha, float* loss,
+                               float* grad, void* stream) {
+  g_err.clear();
+  const mc_status st = check_point_loss(kind, particles, n, targets, m, alpha, loss);
+  if (st != MC_OK) return st;
+  if (n > MC_POINT_LOSS_STATIC_MAX || m > MC_POINT_LOSS_STATIC_MAX)
+    return fail(MC_ERR_BAD_SHAPE,
+                "%lld particles, %lld targets: the static workspace holds at most %d of each",
+                (long long)n, (long long)m, MC_POINT_LOSS_STATIC_MAX);
+  void* ws = nullptr;
+  HIPCHK(point_loss_scratch(&ws));
+  HIPCHK(launch_point_loss(kind, particles, (int)n, targets, (int)m, (float)alpha, loss, grad, ws,
+                           (hipStream_t)stream));
   return MC_OK;
 }
 

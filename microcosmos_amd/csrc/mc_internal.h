@@ -245,6 +245,10 @@ hipError_t launch_steric_adj_nodes(const Consts& c, const BatchDev& b, int N, co
 // Folding losses (mc_losses.hip): kind = mc_loss_kind; the workspace holds
 // point_loss_workspace(n, m) bytes; grad may be NULL.
 size_t point_loss_workspace(long long n, long long m);
+// The current device's address of the library's static point-loss workspace
+// (point_loss_workspace(MC_POINT_LOSS_STATIC_MAX, MC_POINT_LOSS_STATIC_MAX)
+// bytes); resolved once per device, allocates nothing.
+hipError_t point_loss_scratch(void** out);
 hipError_t launch_point_loss(int kind, const float* particles, int n, const float* targets, int m,
                              float alpha, float* loss, float* grad, void* workspace,
                              hipStream_t s);

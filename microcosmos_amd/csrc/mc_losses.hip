@@ -20,6 +20,8 @@
 //                equally), in target order.
 // Squared distances are dx * dx + dy * dy in float32 (contraction off), the
 // same expression in every pass, so the tie tests are exact.
+#include <mutex>
+
 #include "mc_common.h"
 #include "mc_internal.h"
 #include "../../include/microcosmos_hip.h"
@@ -317,6 +319,36 @@ hipError_t launch_stability(const float* frames, int nf, int n, const float* mai
 
 size_t point_loss_workspace(long long n, long long m) {
   return (size_t)(5 * n + 4 * m + 4) * sizeof(float);
+}
+
+// The workspace of mc_point_loss_static: one array per device, large enough
+// for MC_POINT_LOSS_STATIC_MAX particles and as many targets (2.25 MiB).  The
+// kernels write every word they read, so nothing needs clearing between
+// calls.  Host code uses only the device address that hipGetSymbolAddress
+// returns for it (the name itself is a host-side stand-in).
+constexpr long long SCRATCH_POINTS = MC_POINT_LOSS_STATIC_MAX;
+static __device__ __align__(256) float scratch[5 * SCRATCH_POINTS + 4 * SCRATCH_POINTS + 4];
+
+hipError_t point_loss_scratch(void** out) {
+  constexpr int MAX_DEVICES = 64;
+  static std::mutex mu;
+  static void* addr[MAX_DEVICES] = {};
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    // once per device, with that device current: each device holds its own
+    // copy of the array at its own address
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return hipSuccess;
 }
 
 hipError_t launch_point_loss(int kind, const float* particles, int n, const float* targets, int m,
