@@ -6,6 +6,7 @@ import torch
 
 from .structs import Fields
 from .topology import compute_bending_pairs, line_instances, make_edges  # noqa: F401
+from .graph_builder import GraphBuilder, initialize_positions, parse_graph_json  # noqa: F401
 
 _E = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, 1], [-1, -1], [1, -1]])
 _W = np.array([4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 36, 1 / 36])
