@@ -202,6 +202,26 @@ mc_status mc_batch_rollout(mc_plan* plan, const mc_params* params, const mc_stat
                            const mc_edge_params* edges, const mc_actuation* act, double dt,
                            int64_t num_steps, float* displacement_out, void* stream);
 
+/* Per-step actions of the gym step (gym/base.py:163-176): before step k of
+ * the rollout, rest_lengths = edges->rest_lengths + d_rest_length[k] and
+ * bending_rest_angles = edges->bending_rest_angles + d_bending_angle[k]
+ * (float32 adds), the rewrite the reference's env step makes from the base
+ * values and an arbitrary controller's action.  Rows are device arrays
+ * (num_rows, E) and (num_rows, B) over the plan's whole edge and hinge
+ * numbering (every instance of a batched plan); either may be NULL (a zero
+ * action); num_rows >= num_steps. */
+typedef struct mc_actions {
+  const float* d_rest_length;
+  const float* d_bending_angle;
+  int64_t num_rows;
+} mc_actions;
+/* num_steps coupled steps in place on every instance of the plan, graph-
+ * replayed, step k taking action row k; displacement_out as in
+ * mc_batch_rollout (may be NULL).  Stream-ordered. */
+mc_status mc_rollout_actions(mc_plan* plan, const mc_params* params, const mc_state* state,
+                             const mc_edge_params* edges, const mc_actions* actions, double dt,
+                             int64_t num_steps, float* displacement_out, void* stream);
+
 /* Continuous batching: mc_batch_rollout for instances that each run on their
  * own clock.  Instance i's actuation uses t = act->t0 + instance_t0[i] + step
  * index (instance_t0: host, num_instances entries, may be NULL), so

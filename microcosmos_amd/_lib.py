@@ -53,6 +53,10 @@ class McActuation(ctypes.Structure):
     _fields_ = [("amp", c_void_p), ("omega", c_void_p), ("phase", c_void_p), ("t0", c_int64)]
 
 
+class McActions(ctypes.Structure):
+    _fields_ = [("d_rest_length", c_void_p), ("d_bending_angle", c_void_p), ("num_rows", c_int64)]
+
+
 # exported symbol -> (restype, argtypes)
 SIGNATURES = {
     "mc_version": (ctypes.c_char_p, []),
@@ -84,6 +88,9 @@ SIGNATURES = {
     "mc_batch_rollout": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
                                    POINTER(McEdgeParams), POINTER(McActuation), c_double, c_int64,
                                    c_void_p, c_void_p]),
+    "mc_rollout_actions": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
+                                     POINTER(McEdgeParams), POINTER(McActions), c_double, c_int64,
+                                     c_void_p, c_void_p]),
     "mc_batch_advance": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
                                    POINTER(McEdgeParams), POINTER(McActuation), c_void_p,
                                    c_double, c_int64, c_void_p, c_void_p]),

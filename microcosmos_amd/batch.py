@@ -31,6 +31,29 @@ class _nullctx:
         return False
 
 
+def _action_rows(d_rest_length, d_bending_angle, num_edges: int, num_hinges: int, dev):
+    """The mc_actions of per-step action arrays (T, E) / (T, B), either None;
+    returns it with the device tensors it points at."""
+    arrs = []
+    rows = None
+    for a, width, name in ((d_rest_length, num_edges, "d_rest_length"),
+                           (d_bending_angle, num_hinges, "d_bending_angle")):
+        if a is None:
+            arrs.append(None)
+            continue
+        t = as_f32(a, dev).contiguous()
+        if t.dim() != 2 or t.shape[1] != width:
+            raise ValueError(f"{name} must be (steps, {width}), got {tuple(t.shape)}")
+        if rows is not None and t.shape[0] != rows:
+            raise ValueError("d_rest_length and d_bending_angle have different step counts")
+        rows = t.shape[0]
+        arrs.append(t)
+    if rows is None:
+        raise ValueError("no action arrays given")
+    m = _lib.McActions(ptr(arrs[0]), ptr(arrs[1]), int(rows))
+    return m, arrs
+
+
 class BatchPlan:
     """Owns one batched mc_plan handle."""
 
@@ -163,6 +186,27 @@ class Batch:
                                    ctypes.byref(ep), ctypes.byref(act) if act else None, self.dt,
                                    int(num_steps), ptr(disp), self._stream()))
         self.t += int(num_steps)
+        return disp
+
+    def run_actions(self, d_rest_length=None, d_bending_angle=None, track: bool = True):
+        """One graph-replayed step per action row (gym/base.py:163-176): before
+        step k every instance's rest lengths and rest angles are its base
+        values plus row k of ``d_rest_length`` (T, total edges) and
+        ``d_bending_angle`` (T, total hinges), each float32 on the device or
+        None (a zero action), in the batch's concatenated numbering
+        (``edge_off`` / ``hinge_off``).  Returns the (n, 2) displacement as
+        ``run`` does."""
+        rows, keep = _action_rows(d_rest_length, d_bending_angle, int(self.edge_off[-1]),
+                                  int(self.hinge_off[-1]), self.dev)
+        lib = _lib.load()
+        st = self._state()
+        ep = _lib.McEdgeParams(ptr(self.rest_lengths), ptr(self.rest_angles), ptr(self.stiffness))
+        disp = torch.empty((self.n, 2), dtype=torch.float32, device=self.dev) if track else None
+        check(lib.mc_rollout_actions(self.plan.handle, ctypes.byref(self.params),
+                                     ctypes.byref(st), ctypes.byref(ep), ctypes.byref(rows),
+                                     self.dt, int(rows.num_rows), ptr(disp), self._stream()))
+        self._action_keep = keep  # read by the enqueued steps
+        self.t += int(rows.num_rows)
         return disp
 
     def advance(self, num_steps: int, instance_t0, displacement: torch.Tensor):

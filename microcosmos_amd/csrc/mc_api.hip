@@ -1226,7 +1226,7 @@ mc_status drop_graph(mc_plan* p, int slot) {
 mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const mc_edge_params* ed,
                        const mc_actuation* act, const long long* t_inst, double dt,
                        int64_t num_steps, float* rec_pos, float* rec_vel, int32_t use_graph,
-                       void* stream) {
+                       void* stream, const mc_actions* rows = nullptr) {
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   MCCHK(check_params(p, q, true, true));
   MCCHK(check_state(p, st, "state"));
@@ -1235,6 +1235,10 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
   if (act && p->B > 0 && (!act->amp || !act->omega || !act->phase))
     return fail(MC_ERR_INVALID, "actuation arrays are NULL");
+  if (act && rows) return fail(MC_ERR_INVALID, "actuation and action rows are exclusive");
+  if (rows && rows->num_rows < num_steps)
+    return fail(MC_ERR_BAD_SHAPE, "%lld action rows for %lld steps", (long long)rows->num_rows,
+                (long long)num_steps);
   MCCHK(check_async(p, (hipStream_t)stream));
   if (num_steps == 0) return MC_OK;
   hipStream_t us = (hipStream_t)stream;
@@ -1248,6 +1252,12 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
     A = Actuation{act->amp, act->omega, act->phase, p->tick, t_inst};
     ap = &A;
     HIPCHK(launch_set_tick(p->tick, (long long)act->t0, us));
+  } else if (rows) {
+    // step k of this call reads row k: the counter starts at 0
+    A = Actuation{nullptr, nullptr, nullptr, p->tick, nullptr, rows->d_rest_length,
+                  rows->d_bending_angle, (long long)p->E, (long long)p->B, 1, 0};
+    ap = &A;
+    HIPCHK(launch_set_tick(p->tick, 0LL, us));
   }
   // ping-pong: even steps read state->f_grid, odd steps read f_tmp
   mc_state sa = *st, sb = *st;
@@ -1576,7 +1586,8 @@ mc_status ensure_inst_track(mc_plan* p) {
 mc_status tracked_rollout(mc_plan* p, const mc_params* q, const mc_state* st,
                           const mc_edge_params* ed, const mc_actuation* act,
                           const long long* t_inst, double dt, int64_t num_steps,
-                          const float* disp_in, float* disp_out, hipStream_t s) {
+                          const float* disp_in, float* disp_out, hipStream_t s,
+                          const mc_actions* rows = nullptr) {
   if (disp_out) {
     if (!st || (p->N > 0 && !st->position)) return fail(MC_ERR_INVALID, "state is NULL");
     MCCHK(ensure_inst_track(p));
@@ -1587,7 +1598,8 @@ mc_status tracked_rollout(mc_plan* p, const mc_params* q, const mc_state* st,
   }
   const TrackWork keep = p->track;
   if (disp_out) p->track = p->inst_track;
-  const mc_status r = rollout_impl(p, q, st, ed, act, t_inst, dt, num_steps, nullptr, nullptr, 1, s);
+  const mc_status r =
+      rollout_impl(p, q, st, ed, act, t_inst, dt, num_steps, nullptr, nullptr, 1, s, rows);
   p->track = keep;
   if (r != MC_OK) return r;
   if (disp_out)
@@ -1607,6 +1619,16 @@ mc_status mc_batch_rollout(mc_plan* p, const mc_params* q, const mc_state* st,
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   return tracked_rollout(p, q, st, ed, act, nullptr, dt, num_steps, nullptr, displacement_out,
                          (hipStream_t)stream);
+}
+
+mc_status mc_rollout_actions(mc_plan* p, const mc_params* q, const mc_state* st,
+                             const mc_edge_params* ed, const mc_actions* actions, double dt,
+                             int64_t num_steps, float* displacement_out, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  if (!actions) return fail(MC_ERR_INVALID, "actions is NULL");
+  return tracked_rollout(p, q, st, ed, nullptr, nullptr, dt, num_steps, nullptr, displacement_out,
+                         (hipStream_t)stream, actions);
 }
 
 mc_status mc_batch_advance(mc_plan* p, const mc_params* q, const mc_state* st,

@@ -124,10 +124,18 @@ struct CompMeta {
   int pad1, pad2;
 };
 
-struct Actuation {           // rest_b(t) = amp_b * sin(omega_b * t + phase_b)
+// Per-step rewrite of the rest values inside the rod launch.  kind 0:
+// rest_b(t) = amp_b * sin(omega_b * t + phase_b); kind 1: rest lengths and
+// rest angles = base + row t of the action arrays (a NULL array adds 0).
+struct Actuation {
   const float *amp, *omega, *phase;
   const long long* t;        // device step counter (advanced by launch_tick)
   const long long* t_inst;   // per-instance offset added to *t (NULL: none)
+  const float* d_rest;       // kind 1: (rows, e_stride) rest-length offsets or NULL
+  const float* d_ang;        // kind 1: (rows, b_stride) rest-angle offsets or NULL
+  long long e_stride, b_stride;
+  int kind;
+  int pad;                   // no tail padding: graph keys compare bytes
 };
 
 // Arguments of one LBM pass (fused collide-stream of every instance).

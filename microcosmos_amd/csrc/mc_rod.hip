@@ -30,11 +30,26 @@ struct RodArgs {
   float* gscratch;
   const float *pos_in, *vel_in, *y_in, *theta_in, *rest_len, *rest_ang, *kb;
   Actuation act;
-  int has_act;
+  int has_act;   // sine actuation of the rest angles (act.kind 0)
+  int has_rows;  // per-step action rows added to the rest values (act.kind 1)
   const float* steric;
   const float* pred_w;
   float *pos_out, *vel_out, *debug_out, *theta_out;
 };
+
+// The step's action row of an instance (kind 1 actuation), and the rest
+// values with that row added: base + action in float32 (gym/base.py:169-173).
+__device__ __forceinline__ long long action_row(const RodArgs& a, int inst) {
+  return a.has_rows ? *a.act.t + (a.act.t_inst ? a.act.t_inst[inst] : 0LL) : 0LL;
+}
+__device__ __forceinline__ float rest_length(const RodArgs& a, int ge, long long row) {
+  const float l = a.rest_len[ge];
+  return (a.has_rows && a.act.d_rest) ? l + a.act.d_rest[row * a.act.e_stride + ge] : l;
+}
+__device__ __forceinline__ float rest_angle(const RodArgs& a, int gb, long long row) {
+  const float r = a.rest_ang[gb];
+  return (a.has_rows && a.act.d_ang) ? r + a.act.d_ang[row * a.act.b_stride + gb] : r;
+}
 
 __device__ __forceinline__ float sample_bilinear_wrap(const float* V, int H, int W, float yy, float xx) {
   const float ly = floorf(yy), lx = floorf(xx);
@@ -137,6 +152,7 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
   }
   const double tact =
       a.has_act ? (double)(*a.act.t + (a.act.t_inst ? a.act.t_inst[m.inst] : 0LL)) : 0.0;
+  const long long arow = action_row(a, m.inst);
   int g = -1;
   float px = 0.0f, py = 0.0f, vx = 0.0f, vy = 0.0f, yx = 0.0f, yy = 0.0f;
   if (tid < nn) {
@@ -155,7 +171,7 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
   if (tid < ne) {
     const int ge = edge_id(tid);
     th0 = a.theta_in[ge];
-    le0 = a.rest_len[ge];
+    le0 = rest_length(a, ge, arow);
   }
   float am = 0.0f, om = 0.0f, ph = 0.0f, rs0 = 0.0f, kb0 = 0.0f;
   if (tid < nb) {
@@ -165,7 +181,7 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
       om = a.act.omega[gb];
       ph = a.act.phase[gb];
     } else {
-      rs0 = a.rest_ang[gb];
+      rs0 = rest_angle(a, gb, arow);
     }
     kb0 = a.kb[gb];
   }
@@ -269,12 +285,12 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
   for (int e = tid + bs; e < ne; e += bs) {
     const int ge = edge_id(e);
     TH[e] = a.theta_in[ge];
-    Le[e] = a.rest_len[ge];
+    Le[e] = rest_length(a, ge, arow);
   }
   for (int b = tid + bs; b < nb; b += bs) {
     const int gb = hinge_id(b);
     RS[b] = a.has_act ? hinge_rest(a.act.amp[gb], a.act.omega[gb], a.act.phase[gb], 0.0f)
-                      : a.rest_ang[gb];
+                      : rest_angle(a, gb, arow);
     KB[b] = a.kb[gb];
   }
   __syncthreads();
@@ -439,10 +455,11 @@ __device__ __forceinline__ void rod_chain(const RodArgs& a, const CompMeta& m, f
     q = *reinterpret_cast<const float2*>(a.pos_in + 2 * (size_t)g);
     v = *reinterpret_cast<const float2*>(a.vel_in + 2 * (size_t)g);
   }
+  const long long arow = action_row(a, m.inst);
   float th = 0.0f, l = 1.0f;
   if (ise) {
     th = a.theta_in[ge];
-    l = a.rest_len[ge];
+    l = rest_length(a, ge, arow);
   }
   float am = 0.0f, om = 0.0f, ph = 0.0f, rs = 0.0f, kb = 0.0f;
   if (isb) {
@@ -451,7 +468,7 @@ __device__ __forceinline__ void rod_chain(const RodArgs& a, const CompMeta& m, f
       om = a.act.omega[gb];
       ph = a.act.phase[gb];
     } else {
-      rs = a.rest_ang[gb];
+      rs = rest_angle(a, gb, arow);
     }
     kb = a.kb[gb];
   }
@@ -692,8 +709,9 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
   a.rest_len = rest_len;
   a.rest_ang = rest_ang;
   a.kb = kb;
-  a.has_act = act != nullptr;
-  if (act) a.act = *act; else a.act = Actuation{nullptr, nullptr, nullptr, nullptr, nullptr};
+  a.has_act = act != nullptr && act->kind == 0;
+  a.has_rows = act != nullptr && act->kind == 1;
+  if (act) a.act = *act; else a.act = Actuation{};
   a.steric = steric;
   a.pred_w = pred_w;
   a.pos_out = pos_out;

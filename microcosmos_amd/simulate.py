@@ -138,6 +138,22 @@ class Rollout:
                              stream_handle(self.dev)))
         self.t += int(num_steps)
 
+    def run_actions(self, d_rest_length=None, d_bending_angle=None):
+        """One graph-replayed step per action row (gym/base.py:163-176): before
+        step k the rest lengths and rest angles are the edges' base values
+        plus row k of ``d_rest_length`` (T, E) and ``d_bending_angle`` (T, B)
+        (float32, either None for a zero action)."""
+        from .batch import _action_rows
+        rows, keep = _action_rows(d_rest_length, d_bending_angle, int(self.edges.pairs.shape[0]),
+                                  int(self.edges.bending_pairs.shape[0]), self.dev)
+        lib = _lib.load()
+        st = self.state()
+        check(lib.mc_rollout_actions(self.plan.handle, ctypes.byref(self.params),
+                                     ctypes.byref(st), ctypes.byref(self.ep), ctypes.byref(rows),
+                                     self.dt, int(rows.num_rows), None, stream_handle(self.dev)))
+        self._action_keep = keep  # read by the enqueued steps
+        self.t += int(rows.num_rows)
+
     def profile(self, num_steps: int):
         """num_steps steps with HIP events around each fused LBM launch:
         returns (average LBM kernel ms, average step ms)."""
