@@ -202,6 +202,29 @@ mc_status mc_batch_rollout(mc_plan* plan, const mc_params* params, const mc_stat
                            const mc_edge_params* edges, const mc_actuation* act, double dt,
                            int64_t num_steps, float* displacement_out, void* stream);
 
+/* Trajectory of simulate() (simulate.py:15-35), recorded on the device:
+ * after every stride-th step one frame of each non-NULL array, frame
+ * f = (step + 1) / stride - 1, F = num_steps / stride frames:
+ *   position, velocity, debug_vector  (F, N, 2)
+ *   steric                            (F, H, W)
+ *   fluid_velocity                    (F, 2, H, W): the lattice velocity,
+ *       linearly resized to the grid as jax.image.resize(..., "linear")
+ *       does when the lattice and the grid differ (simulate.py:24-28).
+ * The frame index is a device counter, so the recording is captured inside
+ * the replayed rollout graphs (use_graph != 0).  Single-instance plans. */
+typedef struct mc_record {
+  float* position;
+  float* velocity;
+  float* debug_vector;
+  float* steric;
+  float* fluid_velocity;
+  int64_t stride;
+} mc_record;
+mc_status mc_rollout_record(mc_plan* plan, const mc_params* params, const mc_state* state,
+                            const mc_edge_params* edges, const mc_actuation* act, double dt,
+                            int64_t num_steps, const mc_record* record, int32_t use_graph,
+                            void* stream);
+
 /* Per-step actions of the gym step (gym/base.py:163-176): before step k of
  * the rollout, rest_lengths = edges->rest_lengths + d_rest_length[k] and
  * bending_rest_angles = edges->bending_rest_angles + d_bending_angle[k]
@@ -321,6 +344,14 @@ mc_status mc_rod_adjoint_backward_params(mc_plan* plan, const mc_params* params,
  * and writes the first offset to *lo and up to `capacity` float32 taps, 0
  * when the axis needs the FFT (more than 129 taps), -1 on a bad argument. */
 int32_t mc_filter_taps(int32_t n, double sigma, int32_t* lo, float* taps, int32_t capacity);
+
+/* The linear-resize taps of mc_rollout_record along one axis (host only, no
+ * device work; no counterpart in the reference -- it exposes the tabulation
+ * of jax.image.resize's linear weights, simulate.py:24-28, for checking):
+ * writes n_out + 1 CSR offsets and up to `capacity` (input index, weight)
+ * taps; returns the tap count, -1 on a bad argument. */
+int32_t mc_resize_taps(int32_t n_in, int32_t n_out, int32_t* offsets, int32_t* index,
+                       float* weight, int32_t capacity);
 
 /* Folding losses of the filament-folding experiment between particles (n, 2)
  * and targets (m, 2), float32 device arrays, and their gradient with respect

@@ -53,6 +53,11 @@ class McActuation(ctypes.Structure):
     _fields_ = [("amp", c_void_p), ("omega", c_void_p), ("phase", c_void_p), ("t0", c_int64)]
 
 
+class McRecord(ctypes.Structure):
+    _fields_ = [("position", c_void_p), ("velocity", c_void_p), ("debug_vector", c_void_p),
+                ("steric", c_void_p), ("fluid_velocity", c_void_p), ("stride", c_int64)]
+
+
 class McActions(ctypes.Structure):
     _fields_ = [("d_rest_length", c_void_p), ("d_bending_angle", c_void_p), ("num_rows", c_int64)]
 
@@ -88,6 +93,9 @@ SIGNATURES = {
     "mc_batch_rollout": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
                                    POINTER(McEdgeParams), POINTER(McActuation), c_double, c_int64,
                                    c_void_p, c_void_p]),
+    "mc_rollout_record": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
+                                    POINTER(McEdgeParams), POINTER(McActuation), c_double,
+                                    c_int64, POINTER(McRecord), c_int32, c_void_p]),
     "mc_rollout_actions": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
                                      POINTER(McEdgeParams), POINTER(McActions), c_double, c_int64,
                                      c_void_p, c_void_p]),
@@ -116,6 +124,7 @@ SIGNATURES = {
     "mc_track_reset": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "mc_track_read": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mc_filter_taps": (c_int32, [c_int32, c_double, c_void_p, c_void_p, c_int32]),
+    "mc_resize_taps": (c_int32, [c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32]),
     "mc_point_loss_workspace": (c_size_t, [c_int64, c_int64]),
     "mc_point_loss": (c_int32, [c_int32, c_void_p, c_int64, c_void_p, c_int64, c_double, c_void_p,
                                 c_void_p, c_void_p, c_void_p]),

@@ -115,6 +115,8 @@ struct GraphKey {
   mc_edge_params ed;
   Actuation act;
   int has_act;
+  int has_rec;
+  RecordDev rec;
   int64_t steps;
   TrackWork track;
 };
@@ -198,6 +200,12 @@ struct mc_plan {
   double adj_dt = 0.0;
   mc_params adj_q;
   mc_edge_params adj_ed;
+  // trajectory recording (mc_rollout_record): frame counter and the linear
+  // resize taps of the fluid velocity, lattice -> grid
+  long long* rec_tick = nullptr;
+  int* rec_idx = nullptr;       // ry_off (H+1), ry_idx, rx_off (W+1), rx_idx
+  float* rec_w = nullptr;       // ry_w, rx_w
+  int rec_ny = 0, rec_nx = 0;   // tap counts
   int* host_err = nullptr;      // pinned: set by the wide IB solve when a barrier wait gave up
   TrackWork track{};            // displacement tracking (off: n_groups == 0)
   TrackWork inst_track{};       // per-instance tracker of mc_batch_rollout
@@ -1223,10 +1231,70 @@ mc_status drop_graph(mc_plan* p, int slot) {
 }
 
 // mc_rollout with optional per-instance actuation offsets t_inst (device).
+// The recording taps of a single-instance plan (built on first use).
+mc_status ensure_record(mc_plan* p) {
+  if (p->rec_tick) return MC_OK;
+  const InstHost& h = p->inst[0];
+  std::vector<int> yo, yi, xo, xi;
+  std::vector<float> yw, xw;
+  resize_taps(h.Hf, h.H, yo, yi, yw);
+  resize_taps(h.Wf, h.W, xo, xi, xw);
+  p->rec_ny = (int)yi.size();
+  p->rec_nx = (int)xi.size();
+  std::vector<int> idx;
+  idx.insert(idx.end(), yo.begin(), yo.end());
+  idx.insert(idx.end(), yi.begin(), yi.end());
+  idx.insert(idx.end(), xo.begin(), xo.end());
+  idx.insert(idx.end(), xi.begin(), xi.end());
+  std::vector<float> w(yw);
+  w.insert(w.end(), xw.begin(), xw.end());
+  long long* t = nullptr;
+  MCCHK(p->alloc(&p->rec_idx, idx.size(), "resize taps"));
+  MCCHK(p->alloc(&p->rec_w, w.size(), "resize weights"));
+  HIPCHK(hipMemcpy(p->rec_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!w.empty())
+    HIPCHK(hipMemcpy(p->rec_w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+  MCCHK(p->alloc(&t, 1, "frame counter"));
+  p->rec_tick = t;
+  return MC_OK;
+}
+
+RecordDev record_dev(const mc_plan* p, const mc_record* r, const mc_state* st) {
+  const InstHost& h = p->inst[0];
+  RecordDev d;
+  memset(&d, 0, sizeof(d));
+  d.pos = r->position;
+  d.vel = r->velocity;
+  d.dbg = r->debug_vector;
+  d.steric = r->steric;
+  d.fv = r->fluid_velocity;
+  d.s_pos = st->position;
+  d.s_vel = st->velocity;
+  d.s_dbg = st->debug_vector;
+  d.s_steric = st->steric;
+  d.s_fv = st->fluid_velocity;
+  d.counter = p->rec_tick;
+  d.ry_off = p->rec_idx;
+  d.ry_idx = d.ry_off + (h.H + 1);
+  d.rx_off = d.ry_idx + p->rec_ny;
+  d.rx_idx = d.rx_off + (h.W + 1);
+  d.ry_w = p->rec_w;
+  d.rx_w = p->rec_w + p->rec_ny;
+  d.stride = (long long)r->stride;
+  d.N = p->N;
+  d.H = h.H;
+  d.W = h.W;
+  d.Hf = h.Hf;
+  d.Wf = h.Wf;
+  d.resize = (h.H != h.Hf || h.W != h.Wf) ? 1 : 0;
+  return d;
+}
+
 mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const mc_edge_params* ed,
                        const mc_actuation* act, const long long* t_inst, double dt,
                        int64_t num_steps, float* rec_pos, float* rec_vel, int32_t use_graph,
-                       void* stream, const mc_actions* rows = nullptr) {
+                       void* stream, const mc_actions* rows = nullptr,
+                       const mc_record* frames = nullptr) {
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   MCCHK(check_params(p, q, true, true));
   MCCHK(check_state(p, st, "state"));
@@ -1259,6 +1327,19 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
     ap = &A;
     HIPCHK(launch_set_tick(p->tick, 0LL, us));
   }
+  RecordDev R;
+  memset(&R, 0, sizeof(R));
+  if (frames) {
+    MCCHK(ensure_record(p));
+    R = record_dev(p, frames, st);
+    HIPCHK(launch_set_tick(p->rec_tick, 0LL, us));
+  }
+  auto frame = [&](hipStream_t s) -> mc_status {
+    if (!frames) return MC_OK;
+    HIPCHK(launch_record(R, s));
+    HIPCHK(launch_tick(p->rec_tick, s));
+    return MC_OK;
+  };
   // ping-pong: even steps read state->f_grid, odd steps read f_tmp
   mc_state sa = *st, sb = *st;
   sb.f_grid = p->f_tmp;
@@ -1278,6 +1359,7 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
       const int par = (int)(k & 1);
       MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, us));
       MCCHK(record(k, us));
+      MCCHK(frame(us));
     }
   } else {
     // The bulk of the rollout replays one graph of ROLLOUT_GRAPH_STEPS steps
@@ -1299,6 +1381,8 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
       key.ed = *ed;
       if (ap) key.act = A;
       key.has_act = ap != nullptr;
+      key.has_rec = frames != nullptr;
+      key.rec = R;
       key.steps = len;
       key.track = p->track;
       if (p->graph_exec[slot] && memcmp(&key, &p->graph_key[slot], sizeof(key)) != 0)
@@ -1314,9 +1398,12 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
       const LbmArgs* fuse = nullptr;
       for (int64_t k = 0; k < len && r == MC_OK; ++k) {
         const int par = (int)(k & 1);
-        LbmArgs* defer = (q->enable_fluid && p->rod_fuse && k + 1 < len) ? &pend[par] : nullptr;
+        // a recorded frame needs the step's lattice pass done: no deferral
+        LbmArgs* defer = (q->enable_fluid && p->rod_fuse && k + 1 < len && !frames) ? &pend[par]
+                                                                                    : nullptr;
         r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, StepSync(), fuse,
                          defer);
+        if (r == MC_OK) r = frame(cs);
         fuse = defer;
       }
       hipError_t ec = hipStreamEndCapture(cs, &g);
@@ -1619,6 +1706,24 @@ mc_status mc_batch_rollout(mc_plan* p, const mc_params* q, const mc_state* st,
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   return tracked_rollout(p, q, st, ed, act, nullptr, dt, num_steps, nullptr, displacement_out,
                          (hipStream_t)stream);
+}
+
+mc_status mc_rollout_record(mc_plan* p, const mc_params* q, const mc_state* st,
+                            const mc_edge_params* ed, const mc_actuation* act, double dt,
+                            int64_t num_steps, const mc_record* rec, int32_t use_graph,
+                            void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  if (!rec) return fail(MC_ERR_INVALID, "record is NULL");
+  if (p->n_inst != 1) return fail(MC_ERR_UNSUPPORTED, "recording needs a single-instance plan");
+  if (rec->stride < 1) return fail(MC_ERR_BAD_SHAPE, "stride must be at least 1");
+  if (!st) return fail(MC_ERR_INVALID, "state is NULL");
+  if ((rec->position && !st->position) || (rec->velocity && !st->velocity) ||
+      (rec->debug_vector && !st->debug_vector) || (rec->steric && !st->steric) ||
+      (rec->fluid_velocity && !st->fluid_velocity))
+    return fail(MC_ERR_INVALID, "a recorded state array is NULL");
+  return rollout_impl(p, q, st, ed, act, nullptr, dt, num_steps, nullptr, nullptr, use_graph,
+                      stream, nullptr, rec);
 }
 
 mc_status mc_rollout_actions(mc_plan* p, const mc_params* q, const mc_state* st,
@@ -1973,6 +2078,20 @@ int32_t mc_filter_taps(int32_t n, double sigma, int32_t* lo, float* taps, int32_
   *lo = l;
   for (int i = 0; i < (int)t.size() && i < capacity; ++i) taps[i] = t[i];
   return (int32_t)t.size();
+}
+
+int32_t mc_resize_taps(int32_t n_in, int32_t n_out, int32_t* offsets, int32_t* index,
+                       float* weight, int32_t capacity) {
+  if (n_in < 1 || n_out < 1 || !offsets) return -1;
+  std::vector<int> off, idx;
+  std::vector<float> w;
+  resize_taps(n_in, n_out, off, idx, w);
+  for (int i = 0; i <= n_out; ++i) offsets[i] = off[i];
+  for (int i = 0; i < (int)idx.size() && i < capacity; ++i) {
+    if (index) index[i] = idx[i];
+    if (weight) weight[i] = w[i];
+  }
+  return (int32_t)idx.size();
 }
 
 size_t mc_point_loss_workspace(int64_t n, int64_t m) {

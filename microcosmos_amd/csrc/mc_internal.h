@@ -1,6 +1,8 @@
 // Internal (non-exported) declarations shared by the kernel translation units
 // and the C-ABI layer.
 #pragma once
+
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -204,6 +206,23 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
 hipError_t rod_prepare(int lds_max);
 hipError_t launch_steric_nodes(const Consts& c, const BatchDev& b, int N, const float* steric,
                                const float* pred_w, float* force_out, hipStream_t s);
+// One recorded frame of a rollout (launch_record): the frame arrays (NULL:
+// not recorded), the live state, the step counter and the stride, the
+// linear-resize taps of the fluid velocity (CSR per output row / column).
+struct RecordDev {
+  float *pos, *vel, *dbg, *steric, *fv;
+  const float *s_pos, *s_vel, *s_dbg, *s_steric, *s_fv;
+  const long long* counter;
+  const int *ry_off, *ry_idx, *rx_off, *rx_idx;
+  const float *ry_w, *rx_w;
+  long long stride;
+  int N, H, W, Hf, Wf, resize;
+};
+// Writes frame (counter + 1) / stride - 1 when (counter + 1) % stride == 0.
+hipError_t launch_record(const RecordDev& r, hipStream_t s);
+// jax.image.resize's linear weights from `in` to `out` samples along one axis.
+void resize_taps(int in, int out, std::vector<int>& off, std::vector<int>& idx,
+                 std::vector<float>& w);
 hipError_t launch_tick(long long* t, hipStream_t s);
 hipError_t launch_set_tick(long long* t, long long value, hipStream_t s);
 size_t rod_scratch_floats(int nn, int ne, int nb);

@@ -61,26 +61,26 @@ def step(nodes: Nodes, edges: Edges, fields: Fields, dt: float, solver_config):
 
 
 def simulate(nodes: Nodes, edges: Edges, fields: Fields, dt: float, num_steps: int,
-             solver_config):
+             solver_config, *, record_every: int = 1):
     """simulate.py:15-35 -- returns (nodes_timeseries, fields_timeseries), each a
-    pytree of arrays stacked over steps; f_grid is dropped from the history."""
-    H, W = fields.grid_shape
-    if fields.f_grid is not None and tuple(fields.f_grid.shape[1:]) != (H, W):
-        raise NotImplementedError("simulate() with a lattice resolution different from "
-                                  "grid_shape (linear resize of the recorded velocity)")
-    pos, vel, dbg, cb, st, fv = [], [], [], [], [], []
-    for _ in range(int(num_steps)):
-        nodes, edges, fields = step(nodes, edges, fields, dt, solver_config)
-        pos.append(nodes.position)
-        vel.append(nodes.velocity)
-        dbg.append(nodes.debug_vector)
-        cb.append(nodes.color_bend)
-        st.append(fields.steric)
-        fv.append(fields.fluid_velocity)
-    nts = Nodes(position=torch.stack(pos), velocity=torch.stack(vel),
-                color_bend=torch.stack([torch.as_tensor(c) for c in cb]),
-                debug_vector=torch.stack(dbg))
-    fts = replace(fields, steric=torch.stack(st), fluid_velocity=torch.stack(fv), f_grid=None)
+    pytree of arrays stacked over steps; f_grid is dropped from the history and
+    the fluid velocity is recorded at the grid's resolution (linearly resized
+    from the lattice when they differ).  The steps run as graph-replayed
+    device rollouts (mc_rollout_record) with the frames written on the device;
+    ``record_every`` > 1 keeps every k-th step's frame only."""
+    num_steps = int(num_steps)
+    k = int(record_every)
+    if k < 1:
+        raise ValueError("record_every must be at least 1")
+    ro = Rollout(nodes, edges, fields, solver_config, dt)
+    frames = ro.record(num_steps, stride=k)
+    n_frames = num_steps // k
+    cb = torch.as_tensor(nodes.color_bend)
+    nts = Nodes(position=frames["position"], velocity=frames["velocity"],
+                color_bend=cb.unsqueeze(0).expand(n_frames, *cb.shape).clone(),
+                debug_vector=frames["debug_vector"])
+    fts = replace(fields, steric=frames["steric"], fluid_velocity=frames["fluid_velocity"],
+                  f_grid=None)
     return nts, fts
 
 
@@ -153,6 +153,40 @@ class Rollout:
                                      self.dt, int(rows.num_rows), None, stream_handle(self.dev)))
         self._action_keep = keep  # read by the enqueued steps
         self.t += int(rows.num_rows)
+
+    def record(self, num_steps: int, stride: int = 1, use_graph: bool = True):
+        """num_steps steps with a frame of the node arrays, the steric field
+        and the grid-resolution fluid velocity after every stride-th step,
+        written on the device inside the step graphs (mc_rollout_record).
+        Returns a dict of (frames, ...) device tensors: position, velocity,
+        debug_vector (F, N, 2), steric (F, H, W), fluid_velocity
+        (F, 2, H, W), F = num_steps // stride."""
+        lib = _lib.load()
+        d = self.dev
+        F = int(num_steps) // int(stride)
+        N = self.position.shape[0]
+        H, W = self.grid_shape
+        out = {
+            "position": torch.empty((F, N, 2), dtype=torch.float32, device=d),
+            "velocity": torch.empty((F, N, 2), dtype=torch.float32, device=d),
+            "debug_vector": torch.empty((F, N, 2), dtype=torch.float32, device=d),
+            "steric": torch.empty((F, H, W), dtype=torch.float32, device=d),
+            "fluid_velocity": torch.empty((F, 2, H, W), dtype=torch.float32, device=d),
+        }
+        rec = _lib.McRecord(ptr(out["position"]), ptr(out["velocity"]),
+                            ptr(out["debug_vector"]), ptr(out["steric"]),
+                            ptr(out["fluid_velocity"]), int(stride))
+        st = self.state()
+        act = None
+        if self.act is not None:
+            a, o, p = self.act
+            act = McActuation(ptr(a), ptr(o), ptr(p), int(self.t))
+        check(lib.mc_rollout_record(self.plan.handle, ctypes.byref(self.params), ctypes.byref(st),
+                                    ctypes.byref(self.ep), ctypes.byref(act) if act else None,
+                                    self.dt, int(num_steps), ctypes.byref(rec),
+                                    int(bool(use_graph)), stream_handle(d)))
+        self.t += int(num_steps)
+        return out
 
     def profile(self, num_steps: int):
         """num_steps steps with HIP events around each fused LBM launch:

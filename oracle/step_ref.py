@@ -54,6 +54,9 @@ Parity status per function (how each is pinned; see tests/test_oracle_*.py):
                             point, exact streaming permutation, Guo momentum input,
                             Taylor-Green viscous decay exp(-2 nu k^2 t)
   step                   -- reference worm behaviour (tests/test_worm.py:15-152)
+  resize_linear          -- identity at equal sizes, partition of unity, exact
+                            linear interpolation of a ramp when upsampling,
+                            closed-form 2x downsampling taps (1, 3, 3, 1) / 8
   group_com / swim tracker -- float64 mean, translation by a lattice period,
                             a blob straddling the boundary, brute-force
                             per-step cumulative sum
@@ -831,6 +834,42 @@ def step(s: State, dt: float, p: Params) -> State:
             s.position, s.velocity, s.grid_shape, s.f_grid, dt, p)
     s.position = periodic_boundary(s.grid_shape, s.position)
     return s
+
+
+def resize_weights(n_in, n_out, dtype=np.float64):
+    """The (n_in, n_out) weight matrix of jax.image.resize(..., method="linear")
+    along one axis (the published triangle-kernel scale-and-translate with
+    antialiasing that simulate.py:24-28 calls; jax 0.8.1, not vendored): the
+    sample position of output o is (o + 0.5) / scale - 0.5 with scale =
+    n_out / n_in; input i weighs max(0, 1 - |sample - i| / max(1 / scale, 1));
+    each output's weights are divided by their sum (when it exceeds 1000 eps)
+    and zeroed when the sample lies outside [-0.5, n_in - 0.5]."""
+    dt = np.dtype(dtype).type
+    scale = n_out / n_in
+    inv = 1.0 / scale
+    ks = max(inv, 1.0)
+    sf = (np.arange(n_out, dtype=dt) + dt(0.5)) * dt(inv) - dt(0.5)
+    x = np.abs(sf[None, :] - np.arange(n_in, dtype=dt)[:, None]) / dt(ks)
+    w = np.maximum(dt(0.0), dt(1.0) - np.abs(x)).astype(dt)
+    tot = np.zeros((1, n_out), dt)
+    for i in range(n_in):  # in input order
+        tot = tot + w[i:i + 1]
+    w = np.where(np.abs(tot) > dt(1000.0) * EPSIL, w / np.where(tot != 0, tot, dt(1)), dt(0.0))
+    inside = (sf >= dt(-0.5)) & (sf <= dt(n_in - 0.5))
+    return np.where(inside[None, :], w, dt(0.0)).astype(dt)
+
+
+def resize_linear(u, shape):
+    """simulate.py:24-28 -- jax.image.resize(u, shape, method="linear") of a
+    (C, H, W) array to (C, H', W'), separable: rows, then columns."""
+    c, h, w = u.shape
+    _, h2, w2 = shape
+    out = np.asarray(u)
+    if (h2, w2) == (h, w):
+        return out.copy()
+    wy = resize_weights(h, h2, np.float64) if h2 != h else np.eye(h)
+    wx = resize_weights(w, w2, np.float64) if w2 != w else np.eye(w)
+    return np.einsum("chw,hy,wx->cyx", out.astype(np.float64), wy, wx)
 
 
 def simulate(s: State, dt: float, num_steps: int, p: Params, actuation=None):
