@@ -381,19 +381,6 @@ __device__ __forceinline__ void sort_regs(int (&v)[M]) {
     }
 }
 
-// Slot of lattice cell `cell` in a table of 2^bits slots (Fibonacci hashing).
-__device__ __forceinline__ int cell_slot(int cell, int bits) {
-  return (int)(((unsigned)cell * 2654435761u) >> (32 - bits));
-}
-
-// Table bits for ns segments: the smallest power of two with a load factor of
-// at most 0.8, at least 64 slots.
-__device__ __forceinline__ int table_bits(int ns) {
-  int b = 6;
-  while ((4 << b) < 5 * ns) ++b;
-  return b;
-}
-
 // Node data of the small solve, loaded by the thread of the node's rank.
 template <int K2>
 struct NodeIn {
@@ -416,15 +403,14 @@ __device__ __forceinline__ void load_node(const ClusterArgs& a, int n, NodeIn<K2
   d.v0 = *reinterpret_cast<const float2*>(a.vel_io + 2 * n);
 }
 
-// LDS bytes of the small solve: the cell -> segment table (2^bits keys and
-// values), which the entries grouped by segment replace once every tap knows
-// its segment; per node its stencil start, segment base and won mask (16 B)
-// and correction (8 B); per segment its entry start, cursor, cell, 1/kw, rho
-// (4 B each) and u0, u (8 B each); per entry its weight and segment (4 B
-// each, tap-major: entry (r, t) at t nq + r).
-__device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns, int bits) {
-  const size_t table = 8 * ((size_t)1 << bits), grouped = 4 * (size_t)ne;
-  return al16(table > grouped ? table : grouped) + al16(16 * (size_t)nq) + al16(8 * (size_t)nq) +
+// LDS bytes of the small solve: per entry the grouped entry list (4 B); per
+// node its stencil start, segment base and won mask (16 B) and correction
+// (8 B); per segment its entry start, cursor, cell, 1/kw, rho (4 B each) and
+// u0, u (8 B each); per entry its weight and segment (4 B each, tap-major:
+// entry (r, t) at t nq + r; the segments' slots later hold the grouped
+// entries' weights).
+__device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns) {
+  return al16(4 * (size_t)ne) + al16(16 * (size_t)nq) + al16(8 * (size_t)nq) +
          al16(4 * ((size_t)ns + 1)) + 4 * al16(4 * (size_t)ns) + 2 * al16(8 * (size_t)ns) +
          2 * al16(4 * (size_t)ne);
 }
@@ -434,7 +420,9 @@ __device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns, int bits) 
 // (node, tap) per thread for the entry phases, every array in LDS.  The
 // segments are the cells the cluster's won taps claimed, one per cell, node
 // r's won taps numbered from seg0 (the exclusive prefix of the won counts) in
-// tap order; a cell -> segment hash table in LDS gives every tap its segment.
+// tap order; the winning tap writes its segment into the lattice's cell ->
+// segment map, where every tap of the cluster reads it (a cell's taps all
+// belong to the cluster that claimed it).
 // A segment's entries (entry id e = r K2 + t) are placed by arrival, then
 // sorted by e -- by node rank, i.e. ascending node id -- by the segment's own
 // thread, which folds the kernel weight in that order.  Global memory is
@@ -443,7 +431,7 @@ __device__ __forceinline__ size_t small_bytes(int nq, int ne, int ns, int bits) 
 template <int K>
 __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo& gi,
                                             unsigned char* base, const NodeIn<K * K>& d, int nq,
-                                            int ns, int seg0, int bits, int* wsum) {
+                                            int ns, int seg0, int* wsum) {
   constexpr int K2 = K * K;
   const Consts& c = a.c;
   const int ne = nq * K2;
@@ -455,18 +443,14 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
   int* own = a.owner + gi.fl_off;
   const float* fin = a.f_in + 9 * gi.fl_off;
   const int tid = threadIdx.x, bs = blockDim.x;
-  const int T = 1 << bits;
+  int* cs = a.cell_seg + gi.fl_off;
   size_t off = 0;
   auto take = [&](size_t b) {
     unsigned char* p = base + off;
     off += al16(b);
     return p;
   };
-  const size_t table = 8 * (size_t)T, grouped = 4 * (size_t)ne;
-  unsigned char* hreg = take(table > grouped ? table : grouped);
-  int* hkey = (int*)hreg;
-  int* hval = hkey + T;
-  int* arr = (int*)hreg;  // replaces the table once every tap knows its segment
+  int* arr = (int*)take(4 * (size_t)ne);  // grouped entries: ids, then node ranks
   int4* nd = (int4*)take(16 * (size_t)nq);  // per node: sx | sy << 16, seg0, won (low, high)
   float2* corr = (float2*)take(8 * (size_t)nq);
   int* sstart = (int*)take(4 * ((size_t)ns + 1));
@@ -482,7 +466,6 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
 
   const bool mine = tid < nq;
   const int r = tid;
-  for (int i = tid; i < T; i += bs) hkey[i] = -1;
   for (int s = tid; s <= ns; s += bs) sstart[s] = 0;
   if (mine) {
     nd[r] = make_int4(d.sx | (d.sy << 16), seg0, (int)(unsigned)d.won, (int)(unsigned)(d.won >> 32));
@@ -494,8 +477,8 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
     const int sx = q.x & 0xffff, sy = (int)((unsigned)q.x >> 16);
     return wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W);
   };
-  // won taps (one entry per thread, tap-major) enter their cell in the table:
-  // tap t of node r is segment seg0(r) + (won taps of r before t)
+  // won taps (one entry per thread, tap-major) enter their cell in the cell ->
+  // segment map: tap t of node r is segment seg0(r) + (won taps of r before t)
   for (int i = tid; i < ne; i += bs) {
     const int t = i / nq, rr = i - t * nq;
     const int4 q = nd[rr];
@@ -503,12 +486,13 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
     if ((wm >> t) & 1ull) {
       const int g = ecell(q, t);
       const int sg = q.y + __popcll(wm & ((1ull << t) - 1ull));
-      int h = cell_slot(g, bits);
-      for (int k = 0; k < T && atomicCAS(&hkey[h], -1, g) != -1; ++k) h = (h + 1) & (T - 1);
-      hval[h] = sg;
+      cs[g] = sg;
       scell[sg] = g;
     }
   }
+  // the map's stores complete (at L2) before any wave of the workgroup reads
+  // them back, past its L1
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   // per segment (at most two per thread here, more in a loop): the cell's
   // populations, in flight while the taps look their segments up
@@ -521,16 +505,29 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
 #pragma unroll
     for (int i = 0; i < 9; ++i) fp[k][i] = s < ns ? fin[i * HW + cell] : 1.0f;
   }
-  // every entry's segment (its cell's table entry; a cell is in the table
-  // exactly once) and the segments' entry counts
-  for (int i = tid; i < ne; i += bs) {
-    const int t = i / nq, rr = i - t * nq;
-    const int g = ecell(nd[rr], t);
-    int h = cell_slot(g, bits);
-    for (int k = 0; k < T && hkey[h] != g; ++k) h = (h + 1) & (T - 1);
-    const int sg = hkey[h] == g ? hval[h] : 0;
-    eseg[i] = sg;
-    atomicAdd(&sstart[sg], 1);
+  // every entry's segment (its cell's map entry, written above by the one
+  // tap that claimed the cell) and the segments' entry counts; eight loads
+  // in flight per thread
+  for (int i0 = tid; i0 < ne; i0 += 8 * bs) {
+    int sg[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = i0 + k * bs;
+      sg[k] = 0;
+      if (i < ne) {
+        const int t = i / nq, rr = i - t * nq;
+        sg[k] = __hip_atomic_load(&cs[ecell(nd[rr], t)], __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = i0 + k * bs;
+      if (i < ne) {
+        eseg[i] = sg[k];
+        atomicAdd(&sstart[sg[k]], 1);
+      }
+    }
   }
   // the cells' rho and clamped u (fluid.py:268-276)
 #pragma unroll
@@ -572,8 +569,7 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
     if (tid == 0) sstart[ns] = carry;  // == ne
   }
   __syncthreads();
-  // entries to their segments (arrival order; the table is dead), and each
-  // node's segments
+  // entries to their segments (arrival order), and each node's segments
   for (int i = tid; i < ne; i += bs) {
     const int t = i / nq, rr = i - t * nq;
     arr[atomicAdd(&scur[eseg[i]], 1)] = rr * K2 + t;
@@ -585,7 +581,10 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
   }
   __syncthreads();
   // each segment's entries ascending by entry id, then the kernel-weight fold
-  // in that order and 1/max(kw, 1) (fluid.py:123-133,289-290)
+  // in that order and 1/max(kw, 1) (fluid.py:123-133,289-290); the grouped
+  // list then holds each entry's node rank, and gw (the dead segment slots)
+  // its weight, for the spreads
+  float* gw = reinterpret_cast<float*>(eseg);
   for (int s = tid; s < ns; s += bs) {
     const int lo = sstart[s], hi = sstart[s + 1], len = hi - lo;
     float kw = 0.0f;
@@ -597,8 +596,10 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (k < len) {
-          arr[lo + k] = v[k];
-          kw = kw + wt[tix(v[k])];
+          const float w = wt[tix(v[k])];
+          arr[lo + k] = v[k] / K2;
+          gw[lo + k] = w;
+          kw = kw + w;
         }
     } else if (len <= 16) {
       int v[16];
@@ -608,8 +609,10 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
 #pragma unroll
       for (int k = 0; k < 16; ++k)
         if (k < len) {
-          arr[lo + k] = v[k];
-          kw = kw + wt[tix(v[k])];
+          const float w = wt[tix(v[k])];
+          arr[lo + k] = v[k] / K2;
+          gw[lo + k] = w;
+          kw = kw + w;
         }
     } else {
       for (int j = lo + 1; j < hi; ++j) {
@@ -621,7 +624,13 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
         }
         arr[k + 1] = v;
       }
-      for (int j = lo; j < hi; ++j) kw = kw + wt[tix(arr[j])];
+      for (int j = lo; j < hi; ++j) {
+        const int e = arr[j];
+        const float w = wt[tix(e)];
+        arr[j] = e / K2;
+        gw[j] = w;
+        kw = kw + w;
+      }
     }
     sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
   }
@@ -639,9 +648,9 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
     clen[k] = len <= 8 ? len : -1;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const int e = q < len && len <= 8 ? arr[lo + q] : 0;
-      cr[k][q] = e / K2;
-      cw[k][q] = wt[tix(e)];
+      const bool in = q < len && len <= 8;
+      cr[k][q] = in ? arr[lo + q] : 0;
+      cw[k][q] = in ? gw[lo + q] : 0.0f;
     }
   }
   // multi-direct forcing iterations (fluid.py:292-316)
@@ -696,12 +705,12 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
         float wv[8];
         float2 cv[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) e[k] = j0 + k < hi ? arr[j0 + k] : 0;
-#pragma unroll
         for (int k = 0; k < 8; ++k) {
-          wv[k] = wt[tix(e[k])];
-          cv[k] = corr[e[k] / K2];
+          e[k] = j0 + k < hi ? arr[j0 + k] : 0;
+          wv[k] = j0 + k < hi ? gw[j0 + k] : 0.0f;
         }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cv[k] = corr[e[k]];
 #pragma unroll
         for (int k = 0; k < 8; ++k)
           if (j0 + k < hi) {
@@ -1574,9 +1583,8 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_solve(ClusterArgs a
     const int cnt = tid < nq ? __popcll(d.won) : 0;
     int ns;
     const int inc = block_inclusive_scan(cnt, wsum, &ns);
-    const int bits = table_bits(ns);
-    if (small_bytes(nq, ne, ns, bits) <= avail) {
-      solve_small<K>(a, gi, smem, d, nq, ns, inc - cnt, bits, wsum);
+    if (small_bytes(nq, ne, ns) <= avail) {
+      solve_small<K>(a, gi, smem, d, nq, ns, inc - cnt, wsum);
       if (a.bucketed) {
         write_labels(a, gi, root, mbits, nq);
         links_done(a);

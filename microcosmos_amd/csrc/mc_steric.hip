@@ -16,6 +16,8 @@
 // The FFT is a Stockham autosort radix-4/2/3 (+ generic radix <= 16) network
 // in LDS with a double-rounded twiddle table.
 #include "mc_bin.h"
+#include <type_traits>
+
 #include "mc_common.h"
 #include "mc_internal.h"
 
@@ -785,31 +787,40 @@ __global__ void __launch_bounds__(256) k_rows_conv(BatchDev bd, const float* __r
   __syncthreads();
   // four consecutive outputs per thread share one input window; taps in
   // chunks of eight whose loads are issued together, each output summed tap
-  // by tap in order
-  for (int x0 = 4 * threadIdx.x; x0 < W; x0 += 4 * blockDim.x) {
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    int i0 = x0 + lo;  // >= -W
-    if (i0 < 0) i0 += W;
-    for (int t0 = 0; t0 < nt; t0 += 8) {
-      float win[11], tv[8];
+  // by tap in order.  A row at least nt + 11 wide wraps a window index at most
+  // once (a select); narrower rows take the remainder.
+  auto row_pass = [&](auto wide_row) {
+    constexpr bool WIDE = decltype(wide_row)::value;
+    for (int x0 = 4 * threadIdx.x; x0 < W; x0 += 4 * blockDim.x) {
+      float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      int i0 = x0 + lo;  // >= -W
+      if (i0 < 0) i0 += W;
+      for (int t0 = 0; t0 < nt; t0 += 8) {
+        float win[11], tv[8];
+        int ib = i0 + t0;
+        if (WIDE) ib = ib >= W ? ib - W : ib;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) {
-        int i = i0 + t0 + m;
-        while (i >= W) i -= W;
-        win[m] = a[i].x;
+        for (int m = 0; m < 11; ++m) {
+          int i = ib + m;
+          if (WIDE) i = i >= W ? i - W : i;
+          else i = i % W;
+          win[m] = a[i].x;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) tv[j] = tw[t0 + j < nt ? t0 + j : 0];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (t0 + j < nt)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] = acc[k] + tv[j] * win[k + j];
       }
 #pragma unroll
-      for (int j = 0; j < 8; ++j) tv[j] = tw[t0 + j < nt ? t0 + j : 0];
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (t0 + j < nt)
-#pragma unroll
-          for (int k = 0; k < 4; ++k) acc[k] = acc[k] + tv[j] * win[k + j];
+      for (int k = 0; k < 4; ++k)
+        if (x0 + k < W) tmp[g0 + x0 + k] = acc[k];
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (x0 + k < W) tmp[g0 + x0 + k] = acc[k];
-  }
+  };
+  if (W >= nt + 11) row_pass(std::true_type{});
+  else row_pass(std::false_type{});
 }
 
 // Column pass by tiles: a workgroup per FILTER_TILE_W x FILTER_TILE_H output
@@ -843,13 +854,14 @@ __global__ void __launch_bounds__(256) k_cols_conv_tiled(BatchDev bd, const floa
   if (yb < 0) yb += H;
   // the tile's input rows, 8 loads in flight per thread before their stores
   constexpr int RS = 256 / FILTER_TILE_W, LB = 8;
+  const bool tall = H >= nin;  // a tile row wraps at most once
   for (int r0 = rq; r0 < nin; r0 += RS * LB) {
     float v[LB];
 #pragma unroll
     for (int k = 0; k < LB; ++k) {
       const int r = r0 + k * RS;
       int yy = yb + r;
-      while (yy >= H) yy -= H;
+      yy = tall ? (yy >= H ? yy - H : yy) : yy % H;
       v[k] = (r < nin && x < W) ? src[(size_t)yy * W + x] : 0.0f;
     }
 #pragma unroll
