@@ -169,6 +169,7 @@ struct mc_plan {
   double kf_sigma = -1.0;       // sigma the table holds (-1: none)
   bool filter_direct = false;   // some shape's table holds direct-filter taps
   bool filter_fft = false;      // some shape's table holds a transfer function
+  bool filter_fused = false;    // every direct shape's kernels fit the one-launch 2-D filter
   hipStream_t cap_stream = nullptr;
   // the instantiated S-step rollout graph of the last mc_rollout call, reused
   // while its key matches: slot 0 the ROLLOUT_GRAPH_STEPS-step graph
@@ -788,6 +789,7 @@ mc_status ensure_kf(mc_plan* p, double sigma, hipStream_t s) {
   if (!p->fft_ok || p->kf_sigma == sigma) return MC_OK;
   const float coef = (float)(-2.0 * (M_PI * M_PI) * (sigma * sigma));  // utils.py:47
   bool any_direct = false, any_fft = false;
+  size_t max_taps = 0;
   std::vector<float> tab;
   for (size_t i = 0; i < p->shapes.size(); ++i) {
     const int H = p->shapes[i].x, W = p->shapes[i].y;
@@ -807,6 +809,7 @@ mc_status ensure_kf(mc_plan* p, double sigma, hipStream_t s) {
       continue;
     }
     any_direct = true;
+    max_taps = std::max(max_taps, std::max(tx.size(), ty.size()));
     const int hd[4] = {lx, (int)tx.size(), ly, (int)ty.size()};
     tab.assign(4, 0.0f);
     memcpy(tab.data(), hd, sizeof(hd));
@@ -820,6 +823,7 @@ mc_status ensure_kf(mc_plan* p, double sigma, hipStream_t s) {
   }
   p->filter_direct = any_direct;
   p->filter_fft = any_fft;
+  p->filter_fused = any_direct && max_taps <= 33 && !getenv("MC_FILTER_TWO_PASS");
   p->kf_sigma = sigma;
   return MC_OK;
 }
@@ -832,7 +836,7 @@ hipError_t run_filter(mc_plan* p, const float* real_in, const StericDep* dep, fl
                       float* out, hipStream_t s) {
   if (p->filter_direct) {
     const hipError_t e = launch_steric_conv(p->bd, real_in, dep, reinterpret_cast<float*>(p->spec),
-                                            p->kf, density_out, out, s);
+                                            p->kf, density_out, out, p->filter_fused, s);
     if (e != hipSuccess) return e;
   }
   if (p->filter_fft)

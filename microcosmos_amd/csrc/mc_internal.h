@@ -184,9 +184,11 @@ hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const St
 // (int lo_x, n_x, lo_y, n_y), the x taps, the y taps; tmp is grid-sized
 // scratch.
 constexpr int FILTER_TAPS_MAX = 129;
+// fused: every direct shape's kernels have at most 33 taps per axis; the
+// deposit then goes to density_out (or tmp) and one launch filters both axes.
 hipError_t launch_steric_conv(const BatchDev& b, const float* real_in, const StericDep* dep,
                               float* tmp, const float* taps, float* density_out, float* out,
-                              hipStream_t s);
+                              bool fused, hipStream_t s);
 size_t fft_rows_lds_bytes(int max_w);
 size_t fft_cols_lds_bytes(int max_h);
 // rod

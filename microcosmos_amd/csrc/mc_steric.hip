@@ -761,7 +761,8 @@ __global__ void __launch_bounds__(256) k_rows_inv_T(BatchDev bd, const float2* _
 __global__ void __launch_bounds__(256) k_rows_conv(BatchDev bd, const float* __restrict__ real_in,
                                                    DepArgs dep, const float* __restrict__ taps,
                                                    float* __restrict__ tmp,
-                                                   float* __restrict__ density_out) {
+                                                   float* __restrict__ density_out,
+                                                   int filter_rows) {
   extern __shared__ float2 lds[];
   __shared__ int wsum[32];
   __shared__ int s_cnt[2];
@@ -784,6 +785,7 @@ __global__ void __launch_bounds__(256) k_rows_conv(BatchDev bd, const float* __r
     if (density_out)
       for (int i = threadIdx.x; i < W; i += blockDim.x) density_out[g0 + i] = a[i].x;
   }
+  if (!filter_rows) return;  // the deposit only: the 2-D pass filters it
   __syncthreads();
   // four consecutive outputs per thread share one input window; taps in
   // chunks of eight whose loads are issued together, each output summed tap
@@ -892,6 +894,130 @@ __global__ void __launch_bounds__(256) k_cols_conv_tiled(BatchDev bd, const floa
 #pragma unroll
         for (int k = 0; k < ROWS; ++k) acc[k] = acc[k] + tv[j] * win[k + j];
   }
+#pragma unroll
+  for (int k = 0; k < ROWS; ++k) {
+    const int y = ty0 + orow0 + k;
+    if (x < W && y < H) out[gi.grid_off + (size_t)y * W + x] = acc[k];
+  }
+}
+
+// Both passes in one launch, for kernels of at most F2_TAPS taps per axis: a
+// workgroup per FILTER_TILE_W x FILTER_TILE_H output tile (the column pass's
+// tile table) stages the tile's input window (its rows and columns plus the
+// kernels' reach, wrapped) in LDS, filters the window's rows into a second
+// LDS array, then its columns, each output summed tap by tap in order from
+// the first offset exactly as the row and column launches do, so the result
+// is the same bit for bit without the grid-sized intermediate.
+constexpr int F2_TAPS = 33;
+constexpr int F2_THREADS = 512;
+constexpr int F2_RIN = FILTER_TILE_H + F2_TAPS - 1;          // input rows at most
+constexpr int F2_CS = FILTER_TILE_W + F2_TAPS - 1 + 4;       // input row stride (floats, 16 B)
+__global__ void __launch_bounds__(F2_THREADS) k_filter2d(BatchDev bd, const float* __restrict__ taps,
+                                                         const float* __restrict__ src,
+                                                         float* __restrict__ out) {
+  __shared__ __align__(16) float in[F2_RIN * F2_CS];
+  __shared__ __align__(16) float rt[F2_RIN * FILTER_TILE_W];
+  __shared__ int rowy[F2_RIN], colx[F2_CS];
+  const int b = xcd_group(blockIdx.x, gridDim.x);
+  const int inst = find_inst(bd.ctile, bd.n_inst, b);
+  const InstGeo& gi = bd.geo[inst];
+  if (!filter_direct(taps, gi)) return;
+  const int W = gi.W, H = gi.H, tid = threadIdx.x;
+  const int txn = (W + FILTER_TILE_W - 1) / FILTER_TILE_W;
+  const int local = b - bd.ctile[inst];
+  const int ty0 = (local / txn) * FILTER_TILE_H, tx0 = (local % txn) * FILTER_TILE_W;
+  const int* hd = reinterpret_cast<const int*>(taps + gi.kf_off);
+  const int lox = hd[0], nx = hd[1], loy = hd[2], ny = hd[3];
+  // the taps are read with uniform addresses (scalar loads) in the passes
+  const float* tx = taps + gi.kf_off + 4;
+  const float* ty = tx + nx;
+  const int rin = FILTER_TILE_H + ny - 1, cin = FILTER_TILE_W + nx - 1;
+  int yb = (ty0 + loy) % H, xb = (tx0 + lox) % W;  // lo > -side
+  if (yb < 0) yb += H;
+  if (xb < 0) xb += W;
+  const float* g = src + gi.grid_off;
+  // the input window, every load of a thread in flight at once: the wrapped
+  // row and column indices tabulated first, then element tid + F2_THREADS k
+  if (tid < rin) rowy[tid] = (yb + tid) % H;
+  if (tid < cin) colx[tid] = (xb + tid) % W;
+  __syncthreads();
+  {
+    constexpr int PER = (F2_RIN * (F2_CS - 4) + F2_THREADS - 1) / F2_THREADS;
+    const int dr = F2_THREADS / cin, dc = F2_THREADS - dr * cin;
+    int r = tid / cin, c = tid - r * cin;
+    float v[PER];
+    int dst[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const bool ok = r < rin;
+      v[k] = ok ? g[(size_t)rowy[ok ? r : 0] * W + colx[c]] : 0.0f;
+      dst[k] = ok ? r * F2_CS + c : -1;
+      c += dc;
+      r += dr;
+      if (c >= cin) {
+        c -= cin;
+        ++r;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+      if (dst[k] >= 0) in[dst[k]] = v[k];
+  }
+  __syncthreads();
+  // rows: four consecutive outputs per thread from one window, taps in
+  // chunks of eight (three 16-byte loads per chunk)
+  {
+    const int x0 = 4 * (tid & 15);
+    for (int r = tid >> 4; r < rin; r += F2_THREADS / 16) {
+      float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      const float* rowp = in + r * F2_CS + x0;
+#pragma unroll
+      for (int t0 = 0; t0 < F2_TAPS; t0 += 8) {
+        if (t0 >= nx) break;
+        float win[12], tv[8];
+        const float4 w0 = *reinterpret_cast<const float4*>(rowp + t0);
+        const float4 w1 = *reinterpret_cast<const float4*>(rowp + t0 + 4);
+        const float4 w2 = *reinterpret_cast<const float4*>(rowp + t0 + 8);
+        win[0] = w0.x; win[1] = w0.y; win[2] = w0.z; win[3] = w0.w;
+        win[4] = w1.x; win[5] = w1.y; win[6] = w1.z; win[7] = w1.w;
+        win[8] = w2.x; win[9] = w2.y; win[10] = w2.z; win[11] = w2.w;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) tv[j] = tx[t0 + j < nx ? t0 + j : 0];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (t0 + j < nx)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] = acc[k] + tv[j] * win[k + j];
+      }
+      *reinterpret_cast<float4*>(rt + r * FILTER_TILE_W + x0) =
+          make_float4(acc[0], acc[1], acc[2], acc[3]);
+    }
+  }
+  __syncthreads();
+  // columns: ROWS consecutive outputs per thread from one window
+  constexpr int ROWS = FILTER_TILE_H / (F2_THREADS / FILTER_TILE_W);
+  const int c = tid & (FILTER_TILE_W - 1), orow0 = (tid / FILTER_TILE_W) * ROWS;
+  float acc[ROWS];
+#pragma unroll
+  for (int k = 0; k < ROWS; ++k) acc[k] = 0.0f;
+#pragma unroll
+  for (int t0 = 0; t0 < F2_TAPS; t0 += 8) {
+    if (t0 >= ny) break;
+    float win[ROWS + 7], tv[8];
+#pragma unroll
+    for (int m = 0; m < ROWS + 7; ++m) {
+      const int r = orow0 + t0 + m;
+      win[m] = r < rin ? rt[r * FILTER_TILE_W + c] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) tv[j] = ty[t0 + j < ny ? t0 + j : 0];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (t0 + j < ny)
+#pragma unroll
+        for (int k = 0; k < ROWS; ++k) acc[k] = acc[k] + tv[j] * win[k + j];
+  }
+  const int x = tx0 + c;
 #pragma unroll
   for (int k = 0; k < ROWS; ++k) {
     const int y = ty0 + orow0 + k;
@@ -1012,7 +1138,7 @@ hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const St
 
 hipError_t launch_steric_conv(const BatchDev& b, const float* real_in, const StericDep* d,
                               float* tmp, const float* taps, float* density_out, float* out,
-                              hipStream_t s) {
+                              bool fused, hipStream_t s) {
   const int tr = fft_threads(b.max_w);
   const RowsLayout rl = rows_layout(b.max_w);
   DepArgs dep{};
@@ -1032,8 +1158,21 @@ hipError_t launch_steric_conv(const BatchDev& b, const float* real_in, const Ste
   dep.tw_off = rl.tw_off;
   // the FFT pass's layout without its twiddles: the taps need FILTER_TAPS_MAX floats
   const size_t lds = (size_t)(rl.tw_off + (FILTER_TAPS_MAX + 1) / 2) * sizeof(float2);
+  if (fused) {
+    // the deposit (unless the input is given) into density_out or tmp, then
+    // both filter passes in one launch
+    const float* src = real_in;
+    if (!real_in) {
+      float* dens = density_out ? density_out : tmp;
+      hipLaunchKernelGGL(k_rows_conv, dim3(b.rows), dim3(tr), lds, s, b, nullptr, dep, taps, tmp,
+                         dens, 0);
+      src = dens;
+    }
+    hipLaunchKernelGGL(k_filter2d, dim3(b.ctiles), dim3(F2_THREADS), 0, s, b, taps, src, out);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_rows_conv, dim3(b.rows), dim3(tr), lds, s, b,
-                     real_in, dep, taps, tmp, density_out);
+                     real_in, dep, taps, tmp, density_out, 1);
   hipLaunchKernelGGL(k_cols_conv_tiled, dim3(b.ctiles), dim3(256), 0, s, b, taps, tmp, out);
   return hipGetLastError();
 }
