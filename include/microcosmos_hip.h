@@ -263,10 +263,10 @@ mc_status mc_batch_advance(mc_plan* plan, const mc_params* params, const mc_stat
                            const int64_t* instance_t0, double dt, int64_t num_steps,
                            float* displacement_io, void* stream);
 
-/* Differentiation of the rod-only step (PBD constraint, enable_fluid = 0,
- * enable_steric = 0): the configuration the reference differentiates with
- * jax.value_and_grad through lax.scan of simulate.step
- * (experiments/filament_folding/experiment.py:985-1018).
+/* Differentiation of the step: the configurations the reference
+ * differentiates with jax.value_and_grad through lax.scan of simulate.step
+ * (experiments/filament_folding/experiment.py:985-1018: the rod-only step;
+ * tests/test_differentiability.py:51-71: PBD_SCHEME, the fluid-coupled step).
  *
  * mc_rod_adjoint_forward runs num_steps steps in place on `state` (as
  * mc_rollout without actuation: the rest angles edges->bending_rest_angles
@@ -277,8 +277,18 @@ mc_status mc_batch_advance(mc_plan* plan, const mc_params* params, const mc_stat
  * last step (N, 2) and velocity_bar = dL/d velocity (may be NULL: zero), and
  * writes rest_angle_bar (B) = dL/d bending_rest_angles, plus (each may be
  * NULL) the gradient with respect to the initial position, velocity and
- * theta.  params->enable_fluid must be 0; the PBD and stable Cosserat
- * constraints are covered, with or without the steric term.  Without it the
+ * theta.  The PBD and stable Cosserat constraints are covered, with or
+ * without the steric term, and with or without the fluid.  With the fluid
+ * (single-instance plans; MC_ERR_UNSUPPORTED for batched ones), as with the
+ * steric term, every step's state is kept, and its input lattice too (9 Hf Wf
+ * floats per step); the forward runs the coupled steps as mc_rollout does
+ * (bit-identical), and each reverse step recomputes the rod half from its
+ * checkpoint, then runs the adjoint of the TRT/Guo collide-and-stream, the
+ * drag, the force and post clamps and the multi-direct forcing iterations
+ * (the IB positions held constant: the reference's stop_gradient,
+ * fluid.py:239; the node velocities carry gradient), giving the cotangents
+ * of the step's input lattice and of the rod's output velocities, then the
+ * rod half's adjoint.  Without it the
  * filaments are independent: one workgroup per filament runs a whole
  * checkpoint segment per launch, forward (bit-identical to mc_rollout) or
  * reverse (the segment recomputed from its checkpoint, then the hand-written

@@ -17,7 +17,8 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIBNAME = "libmicrocosmos_hip.so"
 SOURCES = ["mc_api.hip", "mc_lbm.hip", "mc_ib.hip", "mc_rod.hip", "mc_steric.hip", "mc_track.hip",
-           "mc_adjoint.hip", "mc_losses.hip", "mc_record.hip"]
+           "mc_adjoint.hip", "mc_losses.hip", "mc_record.hip",
+           "mc_fluid_adj.hip"]
 ARCH = os.environ.get("MC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

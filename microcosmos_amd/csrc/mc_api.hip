@@ -192,6 +192,15 @@ struct mc_plan {
   size_t adj_utape_cap = 0;
   float* adj_work = nullptr;
   float* adj_steric = nullptr;  // steric term: force (2N), pb1 (2N), V / Vbar / Dbar grids
+  // fluid term: the lattice before every step of the recorded rollout, the
+  // rod's recomputed outputs (x', v', debug, theta), the lattice cotangents
+  // (two buffers) and the fluid adjoint's work arrays
+  float* adj_ftape = nullptr;
+  size_t adj_ftape_cap = 0;
+  float* adj_fx = nullptr;
+  float* adj_fb = nullptr;
+  float* adj_fw = nullptr;
+  int* adj_iw = nullptr;
   float* adj_stape = nullptr;   // steric term, per step: field, force, wrapped prediction
   size_t adj_stape_cap = 0;
   bool adj_stored = false;      // the last forward kept the steric tape
@@ -1771,10 +1780,8 @@ namespace {
 
 mc_status check_rod_adjoint(const mc_params* q) {
   if (!q) return fail(MC_ERR_INVALID, "params is NULL");
-  if ((q->solver_type != MC_SOLVER_PBD && q->solver_type != MC_SOLVER_COSSERAT) ||
-      q->enable_fluid)
-    return fail(MC_ERR_UNSUPPORTED,
-                "the rod adjoint covers the step without the fluid (enable_fluid = 0)");
+  if (q->solver_type != MC_SOLVER_PBD && q->solver_type != MC_SOLVER_COSSERAT)
+    return fail(MC_ERR_UNSUPPORTED, "solver_type %d", q->solver_type);
   const int kern = q->bending_constraint == MC_CONSTRAINT_AUTO
                        ? q->solver_type
                        : (q->bending_constraint == MC_CONSTRAINT_COSSERAT ? 1 : 0);
@@ -1790,6 +1797,22 @@ mc_status check_rod_adjoint(const mc_params* q) {
 constexpr size_t ADJ_STERIC_TAPE_MAX = (size_t)2 << 30;
 
 // Steric-term scratch of the adjoint: per-node force and pb1, three grids.
+// The fluid adjoint's buffers of a single-instance plan (allocated once).
+mc_status ensure_adj_fluid(mc_plan* p, const mc_params* q) {
+  if (p->n_inst != 1)
+    return fail(MC_ERR_UNSUPPORTED, "the adjoint of the fluid-coupled step takes single-instance "
+                "plans");
+  if (p->adj_fw) return MC_OK;
+  const long long HW = p->lat_cells;
+  const int kmax = p->kmax;
+  MCCHK(p->alloc(&p->adj_fx, 6 * (size_t)p->N + (size_t)p->E + 1, "fluid adjoint rod outputs"));
+  MCCHK(p->alloc(&p->adj_fb, 18 * (size_t)HW, "lattice cotangents"));
+  MCCHK(p->alloc(&p->adj_fw, fluid_adj_floats(p->N, kmax, std::max(q->ibm_iterations, 1), HW),
+                 "fluid adjoint work"));
+  MCCHK(p->alloc(&p->adj_iw, fluid_adj_ints(p->N, kmax, HW), "fluid adjoint lists"));
+  return MC_OK;
+}
+
 mc_status ensure_adj_steric(mc_plan* p) {
   if (!p->adj_steric)
     MCCHK(p->alloc(&p->adj_steric, 4 * (size_t)p->N + 3 * (size_t)p->grid_cells,
@@ -1813,9 +1836,12 @@ mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state*
   if (num_steps < 1) return fail(MC_ERR_BAD_SHAPE, "num_steps < 1");
   if (checkpoint_every < 1) return fail(MC_ERR_BAD_SHAPE, "checkpoint_every < 1");
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
-  // the steric term couples all filaments every step: every step's state is kept
+  // the steric term and the fluid couple all filaments every step: every
+  // step's state is kept
   const bool steric = q->enable_steric != 0;
-  if (steric) checkpoint_every = 1;
+  const bool fluid = q->enable_fluid != 0;
+  if (steric || fluid) checkpoint_every = 1;
+  if (fluid) MCCHK(ensure_adj_fluid(p, q));
   const size_t lds = rod_segment_lds_bytes(q->cycles_per_step, p->comp_max_nn, p->comp_max_ne,
                                            p->comp_max_nb, true);
   if (lds + 1024 > (size_t)p->lds_max)
@@ -1849,7 +1875,7 @@ mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state*
   size_t tape_max = ADJ_STERIC_TAPE_MAX;
   if (const char* env = getenv("MC_ADJ_STERIC_TAPE_MAX"))  // bytes; 0 always recomputes
     tape_max = (size_t)strtoull(env, nullptr, 10);
-  const bool store = steric && (size_t)num_steps * slot * sizeof(float) <= tape_max;
+  const bool store = steric && !fluid && (size_t)num_steps * slot * sizeof(float) <= tape_max;
   if (steric) {
     MCCHK(ensure_geo(p, q));
     MCCHK(ensure_kf(p, q->steric_sigma, s));
@@ -1860,7 +1886,38 @@ mc_status mc_rod_adjoint_forward(mc_plan* p, const mc_params* q, const mc_state*
       p->adj_stape_cap = (size_t)num_steps * slot;
     }
   }
-  for (int64_t seg = 0; seg < nck; ++seg) {
+  if (fluid) {
+    // coupled steps as mc_rollout takes them (unfused), each after a copy of
+    // its input state to the checkpoint slot and of its lattice to the tape
+    MCCHK(ensure_geo(p, q));
+    MCCHK(check_async(p, s));
+    const size_t hw9 = 9 * (size_t)p->lat_cells;
+    if (p->adj_ftape_cap < (size_t)num_steps * hw9) {
+      MCCHK(p->alloc(&p->adj_ftape, (size_t)num_steps * hw9, "adjoint lattice tape"));
+      p->adj_ftape_cap = (size_t)num_steps * hw9;
+    }
+    mc_state sa = *st, sb = *st;
+    sb.f_grid = p->f_tmp;
+    const size_t n2 = 2 * (size_t)p->N * sizeof(float);
+    for (int64_t k = 0; k < num_steps; ++k) {
+      const mc_state* in = (k & 1) ? &sb : &sa;
+      const mc_state* out = (k & 1) ? &sa : &sb;
+      float* ck = p->adj_ckpt + (size_t)k * S;
+      if (p->N) {
+        HIPCHK(hipMemcpyAsync(ck, in->position, n2, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ck + 2 * (size_t)p->N, in->velocity, n2, hipMemcpyDeviceToDevice, s));
+      }
+      if (p->E)
+        HIPCHK(hipMemcpyAsync(ck + 4 * (size_t)p->N, in->theta, (size_t)p->E * sizeof(float),
+                              hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemcpyAsync(p->adj_ftape + (size_t)k * hw9, in->f_grid, hw9 * sizeof(float),
+                            hipMemcpyDeviceToDevice, s));
+      MCCHK(enqueue_step(p, q, a.c, in, out, ed, nullptr, s));
+    }
+    if (num_steps & 1)
+      HIPCHK(hipMemcpyAsync(st->f_grid, p->f_tmp, hw9 * sizeof(float), hipMemcpyDeviceToDevice, s));
+  }
+  for (int64_t seg = 0; seg < (fluid ? 0 : nck); ++seg) {
     const int len = (int)std::min<int64_t>(checkpoint_every, num_steps - seg * checkpoint_every);
     if (store) {
       // the step's field, force and prediction, kept for the reverse pass
@@ -1974,6 +2031,16 @@ mc_status adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params*
     a.fst = fst;
     a.pb1 = pb1;
   }
+  const bool fluid = q->enable_fluid != 0;
+  float *fbn = nullptr, *fbo = nullptr;
+  if (fluid) {
+    MCCHK(ensure_adj_fluid(p, q));
+    MCCHK(ensure_geo(p, q));
+    // the final lattice carries no cotangent (the losses score node states)
+    fbn = p->adj_fb;
+    fbo = fbn + 9 * (size_t)p->lat_cells;
+    HIPCHK(hipMemsetAsync(fbn, 0, 9 * (size_t)p->lat_cells * sizeof(float), s));
+  }
   for (int64_t seg = nck - 1; seg >= 0; --seg) {
     const int len = (int)std::min<int64_t>(every, T - seg * every);
     // the cotangents of the frames at the state this segment ends in
@@ -1981,6 +2048,26 @@ mc_status adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params*
       if (frame_steps[f] == seg * every + len)
         HIPCHK(launch_add(xb, frame_bars + (size_t)f * 2 * N, 2 * (size_t)N, s));
     const float* ck = p->adj_ckpt + (size_t)seg * S;
+    if (fluid) {
+      // the step's rod half again from its checkpoint (positions before the
+      // wrap, velocities before the drag), then the fluid half's reverse:
+      // vb becomes the cotangent of those velocities, fbn that of the step's
+      // input lattice
+      float* x1 = p->adj_fx;
+      float* v1 = x1 + 2 * (size_t)N;
+      float* d1 = v1 + 2 * (size_t)N;
+      float* t1 = d1 + 2 * (size_t)N;
+      MCCHK(run_steric(p, a.c, ck, ck + 2 * (size_t)N, steric ? vg : nullptr, nullptr, s));
+      HIPCHK(launch_rod(a.c, p->bd, p->L, N, p->rod_lds_bytes, p->rod_use_lds, p->rod_gscratch, 0,
+                        ck, ck + 2 * (size_t)N, nullptr, ck + 4 * (size_t)N, ed->rest_lengths,
+                        ed->bending_rest_angles, ed->bending_stiffness, nullptr,
+                        steric ? vg : nullptr, p->pred_w, x1, v1, d1, t1, nullptr, s,
+                        p->all_chains));
+      HIPCHK(fluid_adjoint_step(a.c, p->geo_h[0], N, p->adj_fw, p->adj_iw,
+                                p->adj_ftape + (size_t)seg * 9 * (size_t)p->lat_cells, x1, v1, vb,
+                                fbn, fbo, s));
+      std::swap(fbn, fbo);
+    }
     const float *vt = vg, *pt = p->pred_w;
     if (steric && p->adj_stored) {
       // the step's field, force and prediction from the forward's tape
@@ -1990,7 +2077,7 @@ mc_status adjoint_backward(mc_plan* p, const mc_params* q, const mc_edge_params*
       pt = a.fst + 2 * (size_t)N;
     } else if (steric) {
       // recompute the step's prediction, field and force from its state
-      MCCHK(run_steric(p, a.c, ck, ck + 2 * (size_t)N, vg, nullptr, s));
+      if (!fluid) MCCHK(run_steric(p, a.c, ck, ck + 2 * (size_t)N, vg, nullptr, s));
       HIPCHK(launch_steric_nodes(a.c, p->bd, N, vg, p->pred_w, fst, s));
     }
     HIPCHK(launch_rod_segment(a, true, len, const_cast<float*>(ck), p->adj_tape, lds, s));

@@ -225,6 +225,16 @@ hipError_t launch_record(const RecordDev& r, hipStream_t s);
 // jax.image.resize's linear weights from `in` to `out` samples along one axis.
 void resize_taps(int in, int out, std::vector<int>& off, std::vector<int>& idx,
                  std::vector<float>& w);
+// Reverse of the fluid half of one coupled step of a single-instance plan:
+// from the step's input lattice fk, the rod's outputs x1 (before the wrap)
+// and v1 (before the drag), and the cotangents vb (of the step's output
+// velocities; replaced by that of v1) and fbn (of the output lattice), the
+// cotangent fb_out of fk.  fw / iw: fluid_adj_floats / fluid_adj_ints words.
+size_t fluid_adj_floats(int N, int K, int iters, long long HW);
+size_t fluid_adj_ints(int N, int K, long long HW);
+hipError_t fluid_adjoint_step(const Consts& c, const InstGeo& g, int N, float* fw, int* iw,
+                              const float* fk, const float* x1, const float* v1, float* vb,
+                              const float* fbn, float* fb_out, hipStream_t s);
 hipError_t launch_tick(long long* t, hipStream_t s);
 hipError_t launch_set_tick(long long* t, long long value, hipStream_t s);
 size_t rod_scratch_floats(int nn, int ne, int nb);

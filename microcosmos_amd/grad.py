@@ -55,7 +55,10 @@ def min_image(grid_shape, a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
 
 
 class RodAdjoint:
-    """A rollout of the rod-only step that can be differentiated.
+    """A rollout of the step that can be differentiated: the rod-only step, or
+    with ``enable_fluid`` the fluid-coupled step (immersed boundary with the
+    node positions held constant, as the reference's stop_gradient does, and
+    the TRT/Guo lattice; single-instance plans, a checkpoint every step).
 
     ``forward(num_steps)`` advances a private copy of the state and records
     checkpoints; ``backward(position_bar)`` returns d loss / d rest angles
@@ -64,10 +67,8 @@ class RodAdjoint:
     """
 
     def __init__(self, nodes: Nodes, edges: Edges, fields: Fields, solver_config, dt: float):
-        if solver_config.enable_fluid or \
-                getattr(solver_config, "solver_type", "pbd") not in ("pbd", "cosserat"):
-            raise NotImplementedError("rod adjoint: PBD or stable Cosserat constraint with "
-                                      "enable_fluid=False")
+        if getattr(solver_config, "solver_type", "pbd") not in ("pbd", "cosserat"):
+            raise NotImplementedError("rod adjoint: PBD or stable Cosserat constraint")
         d = nodes.position.device
         if d.type != "cuda":
             raise RuntimeError("microcosmos_amd runs on the GPU only")
@@ -80,8 +81,9 @@ class RodAdjoint:
         self.theta = as_f32(edges.theta, d).clone()
         self.grid_shape = tuple(fields.grid_shape)
         self.steric = as_f32(fields.steric, d).clone()
-        self.fluid_velocity = as_f32(fields.fluid_velocity, d)
-        self.f_grid = as_f32(fields.f_grid, d)
+        # with the fluid the forward advances the lattice: private copies
+        self.fluid_velocity = as_f32(fields.fluid_velocity, d).clone()
+        self.f_grid = as_f32(fields.f_grid, d).clone()
         from ._runtime import _host_topology
         pa = _host_topology(edges.pairs)[0]
         bp = _host_topology(edges.bending_pairs)[0]

@@ -808,8 +808,14 @@ def immersed_boundary_interaction(position, velocity, grid_shape, f_grid, dt, p:
 # --------------------------------------------------------------------------
 # The step (simulate.py)
 # --------------------------------------------------------------------------
-def step(s: State, dt: float, p: Params) -> State:
-    """simulate.py:37-100 -- one coupled step; returns a new State."""
+def step(s: State, dt: float, p: Params, ib_position=None, ib_trace=None) -> State:
+    """simulate.py:37-100 -- one coupled step; returns a new State.
+
+    For checking gradients, which the reference takes with the IB positions
+    stop_gradient'ed (fluid.py:239): ``ib_trace`` (a list) receives the
+    positions the IB stencil is built from, and ``ib_position`` replaces
+    them, so that a finite difference with the recorded positions held fixed
+    is the derivative the reference's jax.grad computes."""
     s = s.copy()
     s.debug_vector = np.zeros_like(s.position)
     prev = s.position
@@ -830,8 +836,11 @@ def step(s: State, dt: float, p: Params) -> State:
         fn(s, p)
     s.velocity = (s.position - prev) / dt
     if p.enable_fluid:
+        if ib_trace is not None:
+            ib_trace.append(s.position.copy())
         s.velocity, s.fluid_velocity, s.f_grid = immersed_boundary_interaction(
-            s.position, s.velocity, s.grid_shape, s.f_grid, dt, p)
+            s.position if ib_position is None else ib_position, s.velocity, s.grid_shape,
+            s.f_grid, dt, p)
     s.position = periodic_boundary(s.grid_shape, s.position)
     return s
 
