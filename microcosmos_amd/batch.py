@@ -188,6 +188,17 @@ class Batch:
         self.t += int(num_steps)
         return disp
 
+    def autotune(self, steps: int = 8) -> dict:
+        """mc_plan_autotune on the batch's plan (a copy of its state is
+        stepped); returns the chosen launch knobs."""
+        from .simulate import tuning
+        lib = _lib.load()
+        st = self._state()
+        ep = _lib.McEdgeParams(ptr(self.rest_lengths), ptr(self.rest_angles), ptr(self.stiffness))
+        check(lib.mc_plan_autotune(self.plan.handle, ctypes.byref(self.params), ctypes.byref(st),
+                                   ctypes.byref(ep), self.dt, int(steps), self._stream()))
+        return tuning(self.plan.handle)
+
     def run_actions(self, d_rest_length=None, d_bending_angle=None, track: bool = True):
         """One graph-replayed step per action row (gym/base.py:163-176): before
         step k every instance's rest lengths and rest angles are its base

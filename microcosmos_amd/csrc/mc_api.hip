@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -145,6 +146,8 @@ struct mc_plan {
   size_t rod_lds_bytes = 0;
   int rod_use_lds = 1;
   bool rod_fuse = false;        // rollouts may run an LBM pass inside the rod launch
+  bool rod_fuse_ok = false;     // the rod launch has room for it (rod_fuse may be tuned off)
+  bool tuned = false;           // mc_plan_autotune chose the knobs
   bool all_chains = false;      // every component is a chain (rod: one-wavefront path only)
   float* rod_gscratch = nullptr;
   // steric
@@ -220,6 +223,53 @@ struct mc_plan {
   TrackWork track{};            // displacement tracking (off: n_groups == 0)
   TrackWork inst_track{};       // per-instance tracker of mc_batch_rollout
 
+  // While a plan is being built (`deferring`), pooled allocations, their
+  // zero fills and the uploads are all queued on the pool's stream and the
+  // host waits once, at the end (settle); the uploads' host data is staged
+  // in `staged` until then.  Later allocations complete before they return.
+  bool deferring = false;
+  bool pool_pending = false;
+  std::vector<std::vector<unsigned char>> staged;
+
+  mc_status settle() {
+    if (pool_pending) {
+      DevicePool* dp = device_pool(dev);
+      const hipError_t e = hipStreamSynchronize(dp->stream);
+      if (e != hipSuccess) return fail(MC_ERR_HIP, "plan set-up: %s", hipGetErrorString(e));
+      pool_pending = false;
+    }
+    staged.clear();
+    return MC_OK;
+  }
+
+  // Host bytes to device memory allocated by `alloc`: queued behind the
+  // zero fills while deferring, else a synchronous copy.
+  mc_status put(void* dst, const void* src, size_t nb, const char* what) {
+    if (nb == 0) return MC_OK;
+    if (deferring && pooled) {
+      staged.emplace_back(static_cast<const unsigned char*>(src),
+                          static_cast<const unsigned char*>(src) + nb);
+      const hipError_t e = hipMemcpyAsync(dst, staged.back().data(), nb, hipMemcpyHostToDevice,
+                                          device_pool(dev)->stream);
+      if (e != hipSuccess) return fail(MC_ERR_HIP, "upload (%s): %s", what, hipGetErrorString(e));
+      pool_pending = true;
+      return MC_OK;
+    }
+    const hipError_t e = hipMemcpy(dst, src, nb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(MC_ERR_HIP, "upload (%s): %s", what, hipGetErrorString(e));
+    return MC_OK;
+  }
+
+  // A byte fill of memory allocated by `alloc`, ordered like `put`.
+  mc_status fill(void* dst, int value, size_t nb, const char* what) {
+    const hipError_t e = (deferring && pooled)
+                             ? hipMemsetAsync(dst, value, nb, device_pool(dev)->stream)
+                             : hipMemset(dst, value, nb);
+    if (e != hipSuccess) return fail(MC_ERR_HIP, "fill (%s): %s", what, hipGetErrorString(e));
+    if (deferring && pooled) pool_pending = true;
+    return MC_OK;
+  }
+
   template <class T>
   mc_status alloc(T** p, size_t count, const char* what) {
     *p = nullptr;
@@ -229,14 +279,16 @@ struct mc_plan {
     DevicePool* dp = device_pool(dev);
     if (allocs.empty()) pooled = dp != nullptr;
     if (pooled) {
-      // zero-filled and complete before return: later synchronous uploads on
-      // the null stream are ordered after the fill
+      // zero-filled on the pool's stream; complete before return (or, while
+      // the plan is being built, before it is returned): later synchronous
+      // uploads on the null stream are ordered after the fill
       hipError_t e = hipMallocFromPoolAsync(&q, nb, dp->pool, dp->stream);
       if (e != hipSuccess) return fail(MC_ERR_OOM, "pool allocation (%s, %zu B): %s", what, nb, hipGetErrorString(e));
       allocs.push_back(q);
       e = hipMemsetAsync(q, 0, nb, dp->stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(dp->stream);
+      if (e == hipSuccess && !deferring) e = hipStreamSynchronize(dp->stream);
       if (e != hipSuccess) return fail(MC_ERR_HIP, "zero fill (%s): %s", what, hipGetErrorString(e));
+      if (deferring) pool_pending = true;
     } else {
       hipError_t e = hipMalloc(&q, nb);
       if (e != hipSuccess) return fail(MC_ERR_OOM, "hipMalloc(%s, %zu B): %s", what, nb, hipGetErrorString(e));
@@ -277,8 +329,7 @@ void release_plan(mc_plan* p) {
 template <class T>
 mc_status upload(mc_plan* p, T** dst, const std::vector<T>& v, const char* what) {
   MCCHK(p->alloc(dst, v.size(), what));
-  if (!v.empty()) HIPCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  return MC_OK;
+  return p->put(*dst, v.data(), v.size() * sizeof(T), what);
 }
 
 // Connected components of the edge graph (union by smaller root id), then
@@ -388,6 +439,7 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
   // the fused launch gives every workgroup the rod's LDS: keep it small enough
   // for the LBM workgroups to reach about 20 wavefronts per CU
   p->rod_fuse = !p->rod_use_lds || p->rod_lds_bytes <= 8192;
+  p->rod_fuse_ok = p->rod_fuse;
   // per-component topology blocks and contiguous id ranges
   std::vector<int> blob, blob_off(1, 0), nfirst(C), efirst(C), hfirst(C);
   auto first_of = [](const std::vector<int>& off, const std::vector<int>& order, int c) {
@@ -996,13 +1048,15 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
     return st;
   };
   mc_status st;
+  // the set-up's device work is queued and waited for once, at the end
+  p->deferring = true;
   if ((st = build_instances(p)) != MC_OK) return bail(st);
   if ((st = build_layout(p, pairs_host, bending_pairs_host)) != MC_OK) return bail(st);
   if ((st = build_fft(p)) != MC_OK) return bail(st);
   // geometry without u_max (set by ensure_geo from the parameters of a call)
-  if (hipMemcpy(p->geo_d, p->geo_h.data(), p->geo_h.size() * sizeof(InstGeo),
-                hipMemcpyHostToDevice) != hipSuccess)
-    return bail(fail(MC_ERR_HIP, "upload of the instance geometry failed"));
+  if ((st = p->put(p->geo_d, p->geo_h.data(), p->geo_h.size() * sizeof(InstGeo),
+                   "instance geometry")) != MC_OK)
+    return bail(st);
   const int N = p->N;
   const size_t hwf = (size_t)p->lat_cells;
   if ((st = p->alloc(&p->pred_w, 2 * (size_t)N, "pred")) != MC_OK) return bail(st);
@@ -1066,8 +1120,7 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
     std::vector<int> ident(C);
     for (size_t i = 0; i < C; ++i) ident[i] = (int)i;
     if ((st = upload(p, &w.uf_parent, ident, "union-find")) != MC_OK) return bail(st);
-    if (hipMemset(w.owner, 0xff, hwf * sizeof(int)) != hipSuccess)
-      return bail(fail(MC_ERR_HIP, "hipMemset(cell owner) failed"));
+    if ((st = p->fill(w.owner, 0xff, hwf * sizeof(int), "cell owner")) != MC_OK) return bail(st);
   }
   if (ib_prepare(p->lds_max) != hipSuccess)
     return bail(fail(MC_ERR_HIP, "cluster solve: %zu B of dynamic LDS refused", w.lds_bytes));
@@ -1118,13 +1171,14 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
     WideCtl init{};
     init.poll_limit = limit;
     init.host_err = dev_flag;
-    if (hipMemcpy(w.wide_ctl, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess)
-      return bail(fail(MC_ERR_HIP, "wide control upload"));
+    if ((st = p->put(w.wide_ctl, &init, sizeof(init), "wide control")) != MC_OK) return bail(st);
   }
   if ((st = p->alloc(&w.force, 2 * hwf, "force U")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.mask, (size_t)p->mask_words, "U mask")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->f_tmp, 9 * hwf, "f ping-pong")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->tick, 1, "tick")) != MC_OK) return bail(st);
+  p->deferring = false;
+  if ((st = p->settle()) != MC_OK) return bail(st);
   *out = p;
   return MC_OK;
 }
@@ -1202,6 +1256,197 @@ mc_status mc_plan_destroy_after(mc_plan* p, void* stream) {
     return fail(MC_ERR_HIP,
                 "immersed-boundary wide solve: a grid barrier wait gave up (its workgroups were "
                 "not all resident on the device); the plan's last results are invalid");
+  return MC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+mc_status drop_graph(mc_plan* p, int slot);
+
+// The tuned knobs of plans with the same instances and physics, shared by
+// every plan of the process (a re-formed batch of a known shape reuses them).
+struct TuneChoice {
+  int wide_min;
+  bool rod_fuse;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneChoice> g_tune;
+
+std::string tune_key(const mc_plan* p, const mc_params* q, double dt) {
+  std::string k;
+  auto add = [&](long long v) {
+    k += std::to_string(v);
+    k += ',';
+  };
+  add(p->dev);
+  add(p->n_inst);
+  for (const InstHost& h : p->inst) {
+    add(h.H);
+    add(h.W);
+    add(h.Hf);
+    add(h.Wf);
+    add(h.nodes);
+  }
+  add(p->E);
+  add(p->B);
+  add(q->ibm_kernel_size);
+  add(q->ibm_iterations);
+  add(q->cycles_per_step);
+  add(q->enable_fluid);
+  add(q->enable_steric);
+  add(q->solver_type);
+  add(q->bending_constraint);
+  add((long long)(dt * 1e9));
+  return k;
+}
+
+// `steps` coupled steps of the rollout graphs' shape on `s`, eagerly: the
+// LBM pass of each step inside the next step's rod launch when fused.
+mc_status enqueue_steps(mc_plan* p, const mc_params* q, const Consts& c, const mc_state& sa,
+                        const mc_state& sb, const mc_edge_params* ed, int steps, hipStream_t s) {
+  LbmArgs pend[2];
+  const LbmArgs* fuse = nullptr;
+  for (int k = 0; k < steps; ++k) {
+    const int par = k & 1;
+    LbmArgs* defer = (q->enable_fluid && p->rod_fuse && k + 1 < steps) ? &pend[par] : nullptr;
+    MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, nullptr, s, StepSync(), fuse,
+                       defer));
+    fuse = defer;
+  }
+  return MC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
+                           const mc_edge_params* ed, double dt, int32_t steps, void* stream) {
+  g_err.clear();
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  MCCHK(check_params(p, q, true, true));
+  MCCHK(check_state(p, st, "state"));
+  MCCHK(check_edges(p, ed));
+  if (steps < 1) return fail(MC_ERR_BAD_SHAPE, "steps < 1");
+  if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  const std::string key = tune_key(p, q, dt);
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      const int wm = p->ib.wide_wg > 0 ? it->second.wide_min : p->ib.wide_min;
+      const bool fz = it->second.rod_fuse && p->rod_fuse_ok;
+      const bool changed = wm != p->ib.wide_min || fz != p->rod_fuse;
+      p->ib.wide_min = wm;
+      p->rod_fuse = fz;
+      p->tuned = true;
+      if (changed)  // the graphs captured with the old knobs are stale
+        for (int slot = 0; slot < 2; ++slot) MCCHK(drop_graph(p, slot));
+      return MC_OK;
+    }
+  }
+  MCCHK(check_async(p, s));
+  const Consts c = derive(p, q, dt);
+  MCCHK(ensure_geo(p, q));
+  if (q->enable_steric) MCCHK(ensure_kf(p, q->steric_sigma, s));
+  // a scratch copy of the state: the candidates step it, not the caller's
+  const size_t N2 = 2 * (size_t)p->N, E = (size_t)p->E, G = (size_t)p->grid_cells,
+               L = (size_t)p->lat_cells;
+  const size_t words = 3 * N2 + E + G + 2 * L + 18 * L + 8;
+  float* scratch = nullptr;
+  HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&scratch), words * sizeof(float), s));
+  mc_state sa;
+  sa.position = scratch;
+  sa.velocity = sa.position + N2;
+  sa.debug_vector = sa.velocity + N2;
+  sa.theta = sa.debug_vector + N2;
+  sa.steric = sa.theta + E;
+  sa.fluid_velocity = sa.steric + G;
+  sa.f_grid = sa.fluid_velocity + 2 * L;
+  mc_state sb = sa;
+  sb.f_grid = sa.f_grid + 9 * L;
+  auto copy = [&](float* dst, const float* src, size_t n) -> mc_status {
+    if (n && src) HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    return MC_OK;
+  };
+  mc_status r = MC_OK;
+  if (r == MC_OK) r = copy(sa.position, st->position, N2);
+  if (r == MC_OK) r = copy(sa.velocity, st->velocity, N2);
+  if (r == MC_OK) r = copy(sa.debug_vector, st->debug_vector, N2);
+  if (r == MC_OK) r = copy(sa.theta, st->theta, E);
+  if (r == MC_OK) r = copy(sa.steric, st->steric, G);
+  if (r == MC_OK) r = copy(sa.fluid_velocity, st->fluid_velocity, 2 * L);
+  if (r == MC_OK) r = copy(sa.f_grid, st->f_grid, 9 * L);
+  // the candidates: the cluster size above which the wide solve takes over
+  // (when the plan has one) and the LBM pass inside the rod launch or not;
+  // every choice computes the same result bit for bit
+  std::vector<int> wides;
+  if (p->ib.wide_wg > 0) {
+    for (int v : {384, 512, IB_WIDE_MIN, IB_CLUSTER_MAX})
+      if (v >= 64 && v <= IB_CLUSTER_MAX) wides.push_back(v);
+  } else {
+    wides.push_back(p->ib.wide_min);
+  }
+  std::vector<bool> fuses = {p->rod_fuse_ok};
+  if (p->rod_fuse_ok && q->enable_fluid) fuses.push_back(false);
+  const TrackWork keep_track = p->track;
+  p->track.n_groups = 0;  // the scratch steps feed no tracker
+  const int keep_wide = p->ib.wide_min;
+  const bool keep_fuse = p->rod_fuse;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float best = 0.0f;
+  TuneChoice choice{keep_wide, keep_fuse};
+  if (r == MC_OK && hipEventCreate(&e0) != hipSuccess) r = fail(MC_ERR_HIP, "event");
+  if (r == MC_OK && hipEventCreate(&e1) != hipSuccess) r = fail(MC_ERR_HIP, "event");
+  for (int wm : wides) {
+    for (bool fz : fuses) {
+      if (r != MC_OK) break;
+      p->ib.wide_min = wm;
+      p->rod_fuse = fz;
+      // two untimed steps, then the timed ones (an even count each: the
+      // scratch lattice ends in its first buffer)
+      r = enqueue_steps(p, q, c, sa, sb, ed, 2, s);
+      if (r == MC_OK && hipEventRecord(e0, s) != hipSuccess) r = fail(MC_ERR_HIP, "event record");
+      if (r == MC_OK) r = enqueue_steps(p, q, c, sa, sb, ed, 2 * ((steps + 1) / 2), s);
+      if (r == MC_OK && hipEventRecord(e1, s) != hipSuccess) r = fail(MC_ERR_HIP, "event record");
+      float ms = 0.0f;
+      if (r == MC_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        r = fail(MC_ERR_HIP, "timing");
+      if (r == MC_OK) r = check_async(p, s);
+      if (r == MC_OK && (best == 0.0f || ms < best)) {
+        best = ms;
+        choice = TuneChoice{wm, fz};
+      }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  p->track = keep_track;
+  (void)hipFreeAsync(scratch, s);
+  if (r != MC_OK) {
+    p->ib.wide_min = keep_wide;
+    p->rod_fuse = keep_fuse;
+    return r;
+  }
+  p->ib.wide_min = choice.wide_min;
+  p->rod_fuse = choice.rod_fuse;
+  p->tuned = true;
+  // the graphs captured with the old knobs are stale
+  for (int slot = 0; slot < 2; ++slot) MCCHK(drop_graph(p, slot));
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune[key] = choice;
+  return MC_OK;
+}
+
+mc_status mc_plan_tuning(const mc_plan* p, int32_t* wide_min, int32_t* rod_fuse) {
+  g_err.clear();
+  if (!p || !wide_min || !rod_fuse) return fail(MC_ERR_INVALID, "NULL argument");
+  *wide_min = p->ib.wide_min;
+  *rod_fuse = p->rod_fuse ? 1 : 0;
   return MC_OK;
 }
 

@@ -84,6 +84,14 @@ def simulate(nodes: Nodes, edges: Edges, fields: Fields, dt: float, num_steps: i
     return nts, fts
 
 
+def tuning(plan_handle) -> dict:
+    """A plan's launch choices (mc_plan_tuning)."""
+    lib = _lib.load()
+    wm, fz = ctypes.c_int32(), ctypes.c_int32()
+    check(lib.mc_plan_tuning(plan_handle, ctypes.byref(wm), ctypes.byref(fz)))
+    return {"wide_min": int(wm.value), "lbm_in_rod_launch": bool(fz.value)}
+
+
 class Rollout:
     """Device-resident, in-place rollout state for long runs (mc_rollout).
 
@@ -153,6 +161,17 @@ class Rollout:
                                      self.dt, int(rows.num_rows), None, stream_handle(self.dev)))
         self._action_keep = keep  # read by the enqueued steps
         self.t += int(rows.num_rows)
+
+    def autotune(self, steps: int = 8) -> dict:
+        """Times the plan's launch choices on a copy of the current state and
+        keeps the fastest (mc_plan_autotune); returns them.  Results do not
+        change: every choice computes the same steps bit for bit."""
+        lib = _lib.load()
+        st = self.state()
+        check(lib.mc_plan_autotune(self.plan.handle, ctypes.byref(self.params), ctypes.byref(st),
+                                   ctypes.byref(self.ep), self.dt, int(steps),
+                                   stream_handle(self.dev)))
+        return tuning(self.plan.handle)
 
     def record(self, num_steps: int, stride: int = 1, use_graph: bool = True):
         """num_steps steps with a frame of the node arrays, the steric field
