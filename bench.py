@@ -459,7 +459,7 @@ def main():
     use_graph = not args.no_graph
     # the plan's launch knobs, timed on a copy of the state (results do not
     # depend on them)
-    tuning = ro.autotune(steps=8)
+    tuning = ro.autotune(steps=16)
     ro.run(args.warmup, use_graph=use_graph)
     # one untimed run of the timed length instantiates the graphs the timed
     # run replays (its bulk graph and its remainder graph)

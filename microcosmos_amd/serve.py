@@ -14,7 +14,13 @@ the device until it completes:
 * **Lanes.**  Each lane is a HIP stream running one batch at a time: the
   lane's admitted requests packed into one batched plan, so that every
   kernel launch of a step covers all of them.  The lane advances its batch
-  in chunks (``mc_batch_advance``, graph-replayed steps).
+  in chunks (``mc_batch_advance``, graph-replayed steps).  With two lanes or
+  more, the last lane takes the crowded requests -- those whose filaments
+  cover their lattice densely enough to chain into clusters of more than 768
+  nodes, solved by the multi-workgroup immersed-boundary launch -- and the
+  others the light ones, so that a light request's steps never wait on a
+  crowded one's; a lane with nothing of its own class waiting takes the
+  other class.
 * **Continuous batching.**  At each chunk boundary the lane retires the
   requests that have completed (their final state is copied out), admits
   waiting requests that fit, and re-forms its batch from the survivors'
@@ -79,6 +85,7 @@ class Request:
     rest_angle_grad: Optional[torch.Tensor] = None
     # filled in by the server
     footprint: int = 0
+    crowded: bool = False
     admitted: Optional[float] = None
     finished: Optional[float] = None
     result: Optional[tuple] = None          # (Nodes, Edges, Fields) after `steps` steps
@@ -107,6 +114,7 @@ class _GradLane:
 @dataclass
 class _Lane:
     stream: torch.cuda.Stream
+    crowded: Optional[bool] = None   # the class of requests it takes (None: any)
     slots: List[_Slot] = field(default_factory=list)
     batch: Optional[Batch] = None
     disp: Optional[torch.Tensor] = None
@@ -144,7 +152,9 @@ class Server:
         self.max_batch = int(max_batch)
         if lanes < 1 or self.chunk < 1 or self.max_batch < 1:
             raise ValueError("lanes, chunk and max_batch must be positive")
-        self.lanes = [_Lane(stream=torch.cuda.Stream(device=self.dev)) for _ in range(lanes)]
+        self.lanes = [_Lane(stream=torch.cuda.Stream(device=self.dev),
+                            crowded=None if lanes < 2 else (i == lanes - 1))
+                      for i in range(lanes)]
         if gradient_config is None:
             from .solver.config import PBD_SCHEME_NO_FLUID
             gradient_config = replace(PBD_SCHEME_NO_FLUID, enable_steric=False)
@@ -170,6 +180,9 @@ class Server:
         req.footprint = footprint_bytes(req.fields.grid_shape, fshape, n,
                                         int(req.edges.pairs.shape[0]),
                                         int(req.edges.bending_pairs.shape[0]))
+        # crowded: K^2 stencil cells per node cover at least half the lattice
+        k = int(getattr(self.config, "ibm_kernel_size", 4))
+        req.crowded = n > 768 and n * k * k >= 0.5 * fshape[0] * fshape[1]
         if req.footprint > self.budget:
             raise ValueError(f"request needs ~{req.footprint} B, more than the budget {self.budget} B")
         if req.gradient:
@@ -201,8 +214,6 @@ class Server:
         free = [l for l in self.lanes if l.event is None or l.event.query()]
         if not free:
             return done
-        # waiting requests are shared among the lanes at a boundary now
-        share = max(1, -(-len(self.waiting) // len(free)))
         for lane in free:
             if lane.event is not None:
                 # the chunk has finished: a barrier timeout of its wide IB
@@ -213,7 +224,7 @@ class Server:
                 for sl in lane.slots:
                     sl.done += lane.in_flight
                 lane.in_flight = 0
-            done.extend(self._reform(lane, share))
+            done.extend(self._reform(lane, self._share(lane, free)))
             self._launch(lane)
         return done
 
@@ -265,11 +276,33 @@ class Server:
         g.reqs, g.adj, g.event = take, adj, ev
         return out
 
+    def _takes(self, lane: _Lane, r: Request) -> bool:
+        """Whether ``lane`` admits ``r``: its own class, or the other class
+        when none of its own is waiting and it is idle."""
+        if lane.crowded is None or r.crowded == lane.crowded:
+            return True
+        own = any(w.crowded == lane.crowded for w in self.waiting)
+        return not own and not lane.slots
+
+    def _share(self, lane: _Lane, free) -> int:
+        """Waiting requests this lane may take at this boundary: its class's
+        waiting count split among the free lanes of that class."""
+        mine = [w for w in self.waiting if self._takes(lane, w)]
+        peers = [l for l in free if l.crowded == lane.crowded] or [lane]
+        return max(1, -(-len(mine) // len(peers)))
+
     def _admit(self, lane: _Lane, share: int) -> List[Request]:
         out = []
-        while (self.waiting and len(out) < share and len(lane.slots) + len(out) < self.max_batch
-               and self.used + self.waiting[0].footprint <= self.budget):
-            r = self.waiting.pop(0)
+        i = 0
+        while (i < len(self.waiting) and len(out) < share
+               and len(lane.slots) + len(out) < self.max_batch):
+            r = self.waiting[i]
+            if not self._takes(lane, r):
+                i += 1
+                continue
+            if self.used + r.footprint > self.budget:
+                break  # first come first served within the class
+            self.waiting.pop(i)
             self.used += r.footprint
             self.peak_used = max(self.peak_used, self.used)
             r.admitted = self.clock()
