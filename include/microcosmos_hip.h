@@ -176,15 +176,21 @@ mc_status mc_plan_workspace_bytes(const mc_plan* plan, size_t* bytes);
  * copy of `state` (allocated and freed here; state itself is not changed),
  * for each candidate -- the cluster size above which the multi-workgroup
  * immersed-boundary solve takes over (384, 512, 768, 1024 nodes, when the
- * plan has that solve) and the LBM pass inside the next step's rod launch or
- * as a launch of its own -- and keeps the fastest for the plan's later
+ * plan has that solve) and where each step's LBM pass runs (see
+ * mc_plan_tuning) -- and keeps the fastest for the plan's later
  * rollouts.  Every candidate computes the same result bit for bit.  Choices
  * are cached per (instances, physics) for the process: a later plan of the
  * same shape takes them without timing.  Waits for `stream`. */
 mc_status mc_plan_autotune(mc_plan* plan, const mc_params* params, const mc_state* state,
                            const mc_edge_params* edges, double dt, int32_t steps, void* stream);
-/* The plan's current choices: wide-solve cluster size and LBM fusion (0/1). */
-mc_status mc_plan_tuning(const mc_plan* plan, int32_t* wide_min, int32_t* rod_fuse);
+/* The plan's current choices: the wide-solve cluster size, and where a
+ * rollout runs each step's LBM pass: 0 a launch of its own after the step's
+ * IB, 1 inside the next step's rod launch, 2 on a second stream beside the
+ * next step's steric and rod launches. */
+mc_status mc_plan_tuning(const mc_plan* plan, int32_t* wide_min, int32_t* lbm_mode);
+/* Sets those choices by hand (wide_min 64 .. 1024, taken only when the plan
+ * has the wide solve; lbm_mode 0, 1 or 2).  No counterpart in the reference. */
+mc_status mc_plan_set_tuning(mc_plan* plan, int32_t wide_min, int32_t lbm_mode);
 
 /* One coupled step: simulate.step (simulate.py:37-100).
  * in -> out with dt; f_grid of out must differ from in. */

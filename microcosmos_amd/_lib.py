@@ -96,6 +96,7 @@ SIGNATURES = {
     "mc_plan_autotune": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
                                    POINTER(McEdgeParams), c_double, c_int32, c_void_p]),
     "mc_plan_tuning": (c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
+    "mc_plan_set_tuning": (c_int32, [c_void_p, c_int32, c_int32]),
     "mc_rollout_record": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
                                     POINTER(McEdgeParams), POINTER(McActuation), c_double,
                                     c_int64, POINTER(McRecord), c_int32, c_void_p]),

@@ -147,6 +147,11 @@ struct mc_plan {
   int rod_use_lds = 1;
   bool rod_fuse = false;        // rollouts may run an LBM pass inside the rod launch
   bool rod_fuse_ok = false;     // the rod launch has room for it (rod_fuse may be tuned off)
+  // rollouts may instead run a step's LBM pass on a second stream beside the
+  // next step's steric and rod launches (it joins before that step's IB)
+  bool lbm_side = false;
+  hipStream_t side_stream = nullptr;
+  hipEvent_t side_fork[ROLLOUT_GRAPH_STEPS] = {}, side_join[ROLLOUT_GRAPH_STEPS] = {};
   bool tuned = false;           // mc_plan_autotune chose the knobs
   bool all_chains = false;      // every component is a chain (rod: one-wavefront path only)
   float* rod_gscratch = nullptr;
@@ -323,6 +328,11 @@ void release_plan(mc_plan* p) {
   }
   if (dp) (void)hipStreamSynchronize(dp->stream);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  if (p->side_stream) (void)hipStreamDestroy(p->side_stream);
+  for (int k = 0; k < ROLLOUT_GRAPH_STEPS; ++k) {
+    if (p->side_fork[k]) (void)hipEventDestroy(p->side_fork[k]);
+    if (p->side_join[k]) (void)hipEventDestroy(p->side_join[k]);
+  }
   delete p;
 }
 
@@ -932,16 +942,48 @@ struct StepSync {
   hipEvent_t t0 = nullptr, t1 = nullptr;
 };
 
+// The previous step's LBM pass on a second stream, forked from `s` at the
+// start of a step and joined before its IB (see enqueue_step).
+struct SideLbm {
+  const LbmArgs* lbm = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
+mc_status ensure_side(mc_plan* p) {
+  if (p->side_stream) return MC_OK;
+  HIPCHK(hipStreamCreateWithFlags(&p->side_stream, hipStreamNonBlocking));
+  for (int k = 0; k < ROLLOUT_GRAPH_STEPS; ++k) {
+    HIPCHK(hipEventCreateWithFlags(&p->side_fork[k], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&p->side_join[k], hipEventDisableTiming));
+  }
+  return MC_OK;
+}
+
 // Enqueues one coupled step on `s`.  The LBM pass of the step may be deferred
 // (defer_lbm != NULL: its arguments are returned instead of launched); the
 // next step then runs it inside its rod launch (fuse_lbm), where it fills the
 // CUs the filaments leave idle.  That is valid because the rod and steric
 // work of the next step neither read the lattice nor the IB force, and its IB
 // (which reads the new lattice) is enqueued after the rod launch.
+//
+// With `side`, the previous step's deferred LBM pass runs on a second stream
+// instead, beside this step's steric and rod launches (which read neither the
+// lattice nor the IB force and mask), and joins `s` before this step's IB,
+// which reads the lattice it writes and rewrites the force and mask it reads.
 mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc_state* in,
                        const mc_state* out, const mc_edge_params* ed, const Actuation* act,
                        hipStream_t s, const StepSync& sy = StepSync(),
-                       const LbmArgs* fuse_lbm = nullptr, LbmArgs* defer_lbm = nullptr) {
+                       const LbmArgs* fuse_lbm = nullptr, LbmArgs* defer_lbm = nullptr,
+                       const SideLbm& side = SideLbm()) {
+  if (side.lbm) {
+    HIPCHK(hipEventRecord(side.fork, s));
+    HIPCHK(hipStreamWaitEvent(side.stream, side.fork, 0));
+    const LbmArgs& l = *side.lbm;
+    HIPCHK(launch_lbm(c, p->bd, l.fin, l.fout, l.ustar, l.mask, l.force, side.stream, nullptr,
+                      nullptr));
+    HIPCHK(hipEventRecord(side.join, side.stream));
+  }
   MCCHK(run_steric(p, c, in->position, in->velocity, out->steric, nullptr, s));
   if (!c.enable_steric && out->steric != in->steric)
     HIPCHK(hipMemcpyAsync(out->steric, in->steric, (size_t)p->grid_cells * sizeof(float),
@@ -953,6 +995,7 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
                     p->all_chains));
   const size_t hwf = (size_t)p->lat_cells;
   long long* tick = act ? p->tick : nullptr;
+  if (side.lbm) HIPCHK(hipStreamWaitEvent(s, side.join, 0));
   if (c.enable_fluid) {
     HIPCHK(launch_ib(c, p->bd, p->ib, in->f_grid, out->position, out->velocity, out->position, tick,
                      s));
@@ -1270,6 +1313,7 @@ mc_status drop_graph(mc_plan* p, int slot);
 struct TuneChoice {
   int wide_min;
   bool rod_fuse;
+  bool lbm_side;
 };
 std::mutex g_tune_mu;
 std::map<std::string, TuneChoice> g_tune;
@@ -1308,11 +1352,21 @@ mc_status enqueue_steps(mc_plan* p, const mc_params* q, const Consts& c, const m
                         const mc_state& sb, const mc_edge_params* ed, int steps, hipStream_t s) {
   LbmArgs pend[2];
   const LbmArgs* fuse = nullptr;
+  const bool side_mode = p->lbm_side && q->enable_fluid;
+  if (side_mode) MCCHK(ensure_side(p));
   for (int k = 0; k < steps; ++k) {
     const int par = k & 1;
-    LbmArgs* defer = (q->enable_fluid && p->rod_fuse && k + 1 < steps) ? &pend[par] : nullptr;
-    MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, nullptr, s, StepSync(), fuse,
-                       defer));
+    LbmArgs* defer =
+        (q->enable_fluid && (p->rod_fuse || side_mode) && k + 1 < steps) ? &pend[par] : nullptr;
+    SideLbm sd;
+    if (side_mode && fuse) {
+      sd.lbm = fuse;
+      sd.stream = p->side_stream;
+      sd.fork = p->side_fork[k % ROLLOUT_GRAPH_STEPS];
+      sd.join = p->side_join[k % ROLLOUT_GRAPH_STEPS];
+    }
+    MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, nullptr, s, StepSync(),
+                       side_mode ? nullptr : fuse, defer, sd));
     fuse = defer;
   }
   return MC_OK;
@@ -1339,9 +1393,11 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
     if (it != g_tune.end()) {
       const int wm = p->ib.wide_wg > 0 ? it->second.wide_min : p->ib.wide_min;
       const bool fz = it->second.rod_fuse && p->rod_fuse_ok;
-      const bool changed = wm != p->ib.wide_min || fz != p->rod_fuse;
+      const bool sd = it->second.lbm_side;
+      const bool changed = wm != p->ib.wide_min || fz != p->rod_fuse || sd != p->lbm_side;
       p->ib.wide_min = wm;
       p->rod_fuse = fz;
+      p->lbm_side = sd;
       p->tuned = true;
       if (changed)  // the graphs captured with the old knobs are stale
         for (int slot = 0; slot < 2; ++slot) MCCHK(drop_graph(p, slot));
@@ -1388,22 +1444,29 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
   if (p->ib.wide_wg > 0)
     for (int v : {384, 512, IB_WIDE_MIN, IB_CLUSTER_MAX})
       if (v >= 64 && v <= IB_CLUSTER_MAX && v != p->ib.wide_min) wides.push_back(v);
-  std::vector<bool> fuses = {p->rod_fuse};
-  if (p->rod_fuse_ok && q->enable_fluid) fuses.push_back(!p->rod_fuse);
+  // the LBM pass: 1 inside the next rod launch, 2 on a second stream beside
+  // the next step's steric and rod launches, 0 a launch after the step's IB
+  const int cur_mode = p->lbm_side ? 2 : (p->rod_fuse ? 1 : 0);
+  std::vector<int> modes = {cur_mode};
+  if (q->enable_fluid)
+    for (int m : {1, 2, 0})
+      if (m != cur_mode && (m != 1 || p->rod_fuse_ok)) modes.push_back(m);
   const TrackWork keep_track = p->track;
   p->track.n_groups = 0;  // the scratch steps feed no tracker
   const int keep_wide = p->ib.wide_min;
-  const bool keep_fuse = p->rod_fuse;
+  const bool keep_fuse = p->rod_fuse, keep_side = p->lbm_side;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   float best = 0.0f;
-  TuneChoice choice{keep_wide, keep_fuse};
+  TuneChoice choice{keep_wide, keep_fuse, keep_side};
   if (r == MC_OK && hipEventCreate(&e0) != hipSuccess) r = fail(MC_ERR_HIP, "event");
   if (r == MC_OK && hipEventCreate(&e1) != hipSuccess) r = fail(MC_ERR_HIP, "event");
   for (int wm : wides) {
-    for (bool fz : fuses) {
+    for (int mode : modes) {
       if (r != MC_OK) break;
+      const bool fz = mode == 1, sd = mode == 2;
       p->ib.wide_min = wm;
       p->rod_fuse = fz;
+      p->lbm_side = sd;
       // two untimed steps, then the timed ones twice (an even count each:
       // the scratch lattice ends in its first buffer); the faster repeat
       float ms = 0.0f;
@@ -1421,7 +1484,7 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
       if (r == MC_OK) r = check_async(p, s);
       if (r == MC_OK && (best == 0.0f || ms < 0.97f * best)) {
         best = ms;
-        choice = TuneChoice{wm, fz};
+        choice = TuneChoice{wm, fz, sd};
       }
     }
   }
@@ -1432,10 +1495,12 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
   if (r != MC_OK) {
     p->ib.wide_min = keep_wide;
     p->rod_fuse = keep_fuse;
+    p->lbm_side = keep_side;
     return r;
   }
   p->ib.wide_min = choice.wide_min;
   p->rod_fuse = choice.rod_fuse;
+  p->lbm_side = choice.lbm_side;
   p->tuned = true;
   // the graphs captured with the old knobs are stale
   for (int slot = 0; slot < 2; ++slot) MCCHK(drop_graph(p, slot));
@@ -1444,11 +1509,31 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
   return MC_OK;
 }
 
-mc_status mc_plan_tuning(const mc_plan* p, int32_t* wide_min, int32_t* rod_fuse) {
+mc_status mc_plan_set_tuning(mc_plan* p, int32_t wide_min, int32_t lbm_mode) {
   g_err.clear();
-  if (!p || !wide_min || !rod_fuse) return fail(MC_ERR_INVALID, "NULL argument");
+  if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
+  if (lbm_mode < 0 || lbm_mode > 2) return fail(MC_ERR_INVALID, "lbm_mode %d (0, 1 or 2)", lbm_mode);
+  if (lbm_mode == 1 && !p->rod_fuse_ok)
+    return fail(MC_ERR_UNSUPPORTED, "the rod launch of this plan has no room for the LBM pass");
+  if (wide_min < 64 || wide_min > IB_CLUSTER_MAX)
+    return fail(MC_ERR_BAD_SHAPE, "wide_min %d (64 .. %d)", wide_min, IB_CLUSTER_MAX);
+  // without the wide solve every cluster is solved per workgroup: the size
+  // stays above every instance's node count
+  const int wm = p->ib.wide_wg > 0 ? wide_min : p->ib.wide_min;
+  const bool fz = lbm_mode == 1, sd = lbm_mode == 2;
+  if (wm != p->ib.wide_min || fz != p->rod_fuse || sd != p->lbm_side)
+    for (int slot = 0; slot < 2; ++slot) MCCHK(drop_graph(p, slot));
+  p->ib.wide_min = wm;
+  p->rod_fuse = fz;
+  p->lbm_side = sd;
+  return MC_OK;
+}
+
+mc_status mc_plan_tuning(const mc_plan* p, int32_t* wide_min, int32_t* lbm_mode) {
+  g_err.clear();
+  if (!p || !wide_min || !lbm_mode) return fail(MC_ERR_INVALID, "NULL argument");
   *wide_min = p->ib.wide_min;
-  *rod_fuse = p->rod_fuse ? 1 : 0;
+  *lbm_mode = p->lbm_side ? 2 : (p->rod_fuse ? 1 : 0);
   return MC_OK;
 }
 
@@ -1656,13 +1741,23 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
       HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       LbmArgs pend[2];
       const LbmArgs* fuse = nullptr;
+      const bool side_mode = p->lbm_side && q->enable_fluid && !frames;
+      if (side_mode) r = ensure_side(p);
       for (int64_t k = 0; k < len && r == MC_OK; ++k) {
         const int par = (int)(k & 1);
         // a recorded frame needs the step's lattice pass done: no deferral
-        LbmArgs* defer = (q->enable_fluid && p->rod_fuse && k + 1 < len && !frames) ? &pend[par]
-                                                                                    : nullptr;
-        r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, StepSync(), fuse,
-                         defer);
+        LbmArgs* defer = (q->enable_fluid && (p->rod_fuse || side_mode) && k + 1 < len && !frames)
+                             ? &pend[par]
+                             : nullptr;
+        SideLbm sd;
+        if (side_mode && fuse) {
+          sd.lbm = fuse;
+          sd.stream = p->side_stream;
+          sd.fork = p->side_fork[k % ROLLOUT_GRAPH_STEPS];
+          sd.join = p->side_join[k % ROLLOUT_GRAPH_STEPS];
+        }
+        r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, StepSync(),
+                         side_mode ? nullptr : fuse, defer, sd);
         if (r == MC_OK) r = frame(cs);
         fuse = defer;
       }

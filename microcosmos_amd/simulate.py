@@ -87,9 +87,10 @@ def simulate(nodes: Nodes, edges: Edges, fields: Fields, dt: float, num_steps: i
 def tuning(plan_handle) -> dict:
     """A plan's launch choices (mc_plan_tuning)."""
     lib = _lib.load()
-    wm, fz = ctypes.c_int32(), ctypes.c_int32()
-    check(lib.mc_plan_tuning(plan_handle, ctypes.byref(wm), ctypes.byref(fz)))
-    return {"wide_min": int(wm.value), "lbm_in_rod_launch": bool(fz.value)}
+    wm, mode = ctypes.c_int32(), ctypes.c_int32()
+    check(lib.mc_plan_tuning(plan_handle, ctypes.byref(wm), ctypes.byref(mode)))
+    return {"wide_min": int(wm.value),
+            "lbm_pass": ("own launch", "in next rod launch", "second stream")[int(mode.value)]}
 
 
 class Rollout:
@@ -171,6 +172,13 @@ class Rollout:
         check(lib.mc_plan_autotune(self.plan.handle, ctypes.byref(self.params), ctypes.byref(st),
                                    ctypes.byref(self.ep), self.dt, int(steps),
                                    stream_handle(self.dev)))
+        return tuning(self.plan.handle)
+
+    def set_tuning(self, wide_min: int = 768, lbm_pass: int = 1):
+        """Sets the plan's launch choices by hand (mc_plan_set_tuning):
+        lbm_pass 0 its own launch, 1 in the next rod launch, 2 on a second
+        stream."""
+        check(_lib.load().mc_plan_set_tuning(self.plan.handle, int(wide_min), int(lbm_pass)))
         return tuning(self.plan.handle)
 
     def record(self, num_steps: int, stride: int = 1, use_graph: bool = True):

@@ -60,7 +60,7 @@ def test_tuned_rollout_equals_untuned(dev, case):
             choice = ro.autotune(steps=4)
             torch.cuda.synchronize()
             _say(f"  choice {choice}")
-            assert set(choice) == {"wide_min", "lbm_in_rod_launch"}
+            assert set(choice) == {"wide_min", "lbm_pass"}
             np.testing.assert_array_equal(_np(ro.position), before[0])
             np.testing.assert_array_equal(_np(ro.f_grid), before[1])
         ro.run(20)
@@ -94,3 +94,30 @@ def test_tuned_batch_equals_untuned(dev):
         outs.append((_np(b.position), _np(b.f_grid), _np(disp)))
     for a, c in zip(*outs):
         np.testing.assert_array_equal(a, c)
+
+
+@pytest.mark.parametrize("case", ["plain", "wide"])
+def test_lbm_placements_agree(dev, case):
+    """The three placements of the LBM pass in a graph-replayed rollout (its
+    own launch, inside the next rod launch, on a second stream beside the next
+    steric and rod launches) and the step-by-step rollout agree bit for bit."""
+    from microcosmos_amd import Rollout
+    if case == "wide":
+        wl = W.small(grid=(64, 64), n_fil=48, n_nodes=40, seed=3)
+    else:
+        wl = W.small(grid=(48, 48), n_fil=3, n_nodes=20, seed=23)
+    cfg = device_config(wl)
+    act = (wl.act_amp, wl.act_omega, wl.act_phase)
+    runs = []
+    for mode in (None, 0, 1, 2):
+        _say(f"{case}: lbm placement {mode}")
+        nodes, edges, fields = device_state(wl, dev)
+        ro = _private(Rollout(nodes, edges, fields, cfg, wl.dt, actuation=act))
+        if mode is not None:
+            ro.set_tuning(lbm_pass=mode)
+        ro.run(37, use_graph=mode is not None)
+        torch.cuda.synchronize()
+        runs.append((_np(ro.position), _np(ro.velocity), _np(ro.f_grid), _np(ro.fluid_velocity)))
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            np.testing.assert_array_equal(a, b)
