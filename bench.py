@@ -395,6 +395,9 @@ def main():
                     help="steps of the 128^2 plain case on the CPU restatement (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the plan's launch choices on the initial state first "
+                         "(mc_plan_autotune); the defaults otherwise")
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=512)
@@ -457,9 +460,12 @@ def main():
     ro = Rollout(nodes, edges, fields, cfg, wl.dt,
                  actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
     use_graph = not args.no_graph
-    # the plan's launch knobs, timed on a copy of the state (results do not
-    # depend on them)
-    tuning = ro.autotune(steps=16)
+    if args.autotune:
+        # the plan's launch knobs timed on a copy of the initial state
+        # (results do not depend on them)
+        ro.autotune(steps=32)
+    from microcosmos_amd.simulate import tuning as plan_tuning
+    tuning = plan_tuning(ro.plan.handle)
     ro.run(args.warmup, use_graph=use_graph)
     # one untimed run of the timed length instantiates the graphs the timed
     # run replays (its bulk graph and its remainder graph)

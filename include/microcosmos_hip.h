@@ -175,8 +175,8 @@ mc_status mc_plan_workspace_bytes(const mc_plan* plan, size_t* bytes);
  * jax's compiler picks its own): times `steps` coupled steps, on a scratch
  * copy of `state` (allocated and freed here; state itself is not changed),
  * for each candidate -- the cluster size above which the multi-workgroup
- * immersed-boundary solve takes over (384, 512, 768, 1024 nodes, when the
- * plan has that solve) and where each step's LBM pass runs (see
+ * immersed-boundary solve takes over (768 or 1024 nodes, when the plan has
+ * that solve and a cluster of the timed steps is that large) and where each step's LBM pass runs (see
  * mc_plan_tuning) -- and keeps the fastest for the plan's later
  * rollouts.  Every candidate computes the same result bit for bit.  Choices
  * are cached per (instances, physics) for the process: a later plan of the

@@ -1307,6 +1307,11 @@ mc_status mc_plan_destroy_after(mc_plan* p, void* stream) {
 namespace {
 
 mc_status drop_graph(mc_plan* p, int slot);
+mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const mc_edge_params* ed,
+                       const mc_actuation* act, const long long* t_inst, double dt,
+                       int64_t num_steps, float* rec_pos, float* rec_vel, int32_t use_graph,
+                       void* stream, const mc_actions* rows = nullptr,
+                       const mc_record* frames = nullptr);
 
 // The tuned knobs of plans with the same instances and physics, shared by
 // every plan of the process (a re-formed batch of a known shape reuses them).
@@ -1439,11 +1444,30 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
   // the candidates: the cluster size above which the wide solve takes over
   // (when the plan has one) and the LBM pass inside the rod launch or not;
   // every choice computes the same result bit for bit
-  // (the current choice first: another must beat it by 3 % to replace it)
+  const TrackWork keep_track = p->track;
+  p->track.n_groups = 0;  // the scratch steps feed no tracker
+  // (the current choice first: another must beat it by 5 % to replace it).
+  // Sizes that no cluster of the scratch steps exceeds change nothing there
+  // and are not timed: the largest cluster is read after two steps.
   std::vector<int> wides = {p->ib.wide_min};
-  if (p->ib.wide_wg > 0)
-    for (int v : {384, 512, IB_WIDE_MIN, IB_CLUSTER_MAX})
-      if (v >= 64 && v <= IB_CLUSTER_MAX && v != p->ib.wide_min) wides.push_back(v);
+  if (p->ib.wide_wg > 0 && q->enable_fluid && r == MC_OK) {
+    r = enqueue_steps(p, q, c, sa, sb, ed, 2, s);
+    std::vector<int> cnt(std::max(p->ib.C, 1));
+    if (r == MC_OK && p->ib.C > 0 &&
+        (hipMemcpyAsync(cnt.data(), p->ib.cl_cnt, (size_t)p->ib.C * sizeof(int),
+                        hipMemcpyDeviceToHost, s) != hipSuccess ||
+         hipStreamSynchronize(s) != hipSuccess))
+      r = fail(MC_ERR_HIP, "cluster sizes");
+    int largest = 0;
+    for (int v : cnt) largest = std::max(largest, v);
+    // (sizes below the default put clusters a long rollout keeps forming on
+    // the slower multi-workgroup path: only the default and the largest the
+    // per-cluster solve takes are tried)
+    for (int v : {IB_WIDE_MIN, IB_CLUSTER_MAX})
+      if (v >= 64 && v <= IB_CLUSTER_MAX && v != p->ib.wide_min &&
+          (v < largest || p->ib.wide_min < largest))
+        wides.push_back(v);
+  }
   // the LBM pass: 1 inside the next rod launch, 2 on a second stream beside
   // the next step's steric and rod launches, 0 a launch after the step's IB
   const int cur_mode = p->lbm_side ? 2 : (p->rod_fuse ? 1 : 0);
@@ -1451,8 +1475,6 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
   if (q->enable_fluid)
     for (int m : {1, 2, 0})
       if (m != cur_mode && (m != 1 || p->rod_fuse_ok)) modes.push_back(m);
-  const TrackWork keep_track = p->track;
-  p->track.n_groups = 0;  // the scratch steps feed no tracker
   const int keep_wide = p->ib.wide_min;
   const bool keep_fuse = p->rod_fuse, keep_side = p->lbm_side;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1467,13 +1489,20 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
       p->ib.wide_min = wm;
       p->rod_fuse = fz;
       p->lbm_side = sd;
-      // two untimed steps, then the timed ones twice (an even count each:
-      // the scratch lattice ends in its first buffer); the faster repeat
+      // graph-replayed rollouts of the scratch state, as mc_rollout runs
+      // them: one to capture the graphs, then the timed ones twice; the
+      // faster repeat
+      for (int slot = 0; slot < 2 && r == MC_OK; ++slot) r = drop_graph(p, slot);
+      const int64_t len = (int64_t)ROLLOUT_GRAPH_STEPS *
+                          std::max<int64_t>(1, (steps + ROLLOUT_GRAPH_STEPS - 1) / ROLLOUT_GRAPH_STEPS);
       float ms = 0.0f;
-      r = enqueue_steps(p, q, c, sa, sb, ed, 2, s);
+      if (r == MC_OK)
+        r = rollout_impl(p, q, &sa, ed, nullptr, nullptr, dt, ROLLOUT_GRAPH_STEPS, nullptr,
+                         nullptr, 1, s);
       for (int rep = 0; rep < 2 && r == MC_OK; ++rep) {
         if (hipEventRecord(e0, s) != hipSuccess) r = fail(MC_ERR_HIP, "event record");
-        if (r == MC_OK) r = enqueue_steps(p, q, c, sa, sb, ed, 2 * ((steps + 1) / 2), s);
+        if (r == MC_OK)
+          r = rollout_impl(p, q, &sa, ed, nullptr, nullptr, dt, len, nullptr, nullptr, 1, s);
         if (r == MC_OK && hipEventRecord(e1, s) != hipSuccess) r = fail(MC_ERR_HIP, "event record");
         float t = 0.0f;
         if (r == MC_OK && (hipEventSynchronize(e1) != hipSuccess ||
@@ -1482,7 +1511,7 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
         if (r == MC_OK) ms = rep == 0 ? t : std::min(ms, t);
       }
       if (r == MC_OK) r = check_async(p, s);
-      if (r == MC_OK && (best == 0.0f || ms < 0.97f * best)) {
+      if (r == MC_OK && (best == 0.0f || ms < 0.95f * best)) {
         best = ms;
         choice = TuneChoice{wm, fz, sd};
       }
@@ -1491,6 +1520,11 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   p->track = keep_track;
+  // the graphs captured on the scratch state go with it
+  for (int slot = 0; slot < 2; ++slot) {
+    const mc_status d = drop_graph(p, slot);
+    if (r == MC_OK) r = d;
+  }
   (void)hipFreeAsync(scratch, s);
   if (r != MC_OK) {
     p->ib.wide_min = keep_wide;
@@ -1638,8 +1672,7 @@ RecordDev record_dev(const mc_plan* p, const mc_record* r, const mc_state* st) {
 mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const mc_edge_params* ed,
                        const mc_actuation* act, const long long* t_inst, double dt,
                        int64_t num_steps, float* rec_pos, float* rec_vel, int32_t use_graph,
-                       void* stream, const mc_actions* rows = nullptr,
-                       const mc_record* frames = nullptr) {
+                       void* stream, const mc_actions* rows, const mc_record* frames) {
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   MCCHK(check_params(p, q, true, true));
   MCCHK(check_state(p, st, "state"));
