@@ -443,6 +443,18 @@ mc_status mc_profile_step(mc_plan* plan, const mc_params* params, const mc_state
                           const mc_edge_params* edges, const mc_actuation* act, double dt,
                           int64_t num_steps, double* lbm_ms_avg, double* step_ms_avg, void* stream);
 
+/* Measurement of the launch that runs in the rollout graphs: num_steps (even,
+ * >= 2) coupled steps in place with each step's LBM pass carried by the next
+ * step's rod launch, as in those graphs, and HIP events carried by each such
+ * rod launch; returns the average duration of those launches (filament solve
+ * plus the previous step's LBM pass) and of a step in ms.  Needs a fluid
+ * scenario whose plan fuses the two (MC_ERR_UNSUPPORTED otherwise).  No
+ * counterpart in the reference. */
+mc_status mc_profile_fused_step(mc_plan* plan, const mc_params* params, const mc_state* state,
+                                const mc_edge_params* edges, const mc_actuation* act, double dt,
+                                int64_t num_steps, double* kernel_ms_avg, double* step_ms_avg,
+                                void* stream);
+
 /* Sub-operators of the step, each a drop-in for one reference function. */
 
 /* utils.py:35-49 gaussian_filter on an (H, W) field (H = grid_h, W = grid_w).

@@ -80,6 +80,9 @@ SIGNATURES = {
     "mc_profile_step": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
                                   POINTER(McEdgeParams), POINTER(McActuation), c_double, c_int64,
                                   POINTER(c_double), POINTER(c_double), c_void_p]),
+    "mc_profile_fused_step": (c_int32, [c_void_p, POINTER(McParams), POINTER(McState),
+                                        POINTER(McEdgeParams), POINTER(McActuation), c_double,
+                                        c_int64, POINTER(c_double), POINTER(c_double), c_void_p]),
     "mc_gaussian_filter": (c_int32, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "mc_steric_field": (c_int32, [c_void_p, POINTER(McParams), c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p]),

@@ -940,6 +940,7 @@ mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float*
 // Optional timing events around the LBM launch of one step.
 struct StepSync {
   hipEvent_t t0 = nullptr, t1 = nullptr;
+  bool on_rod = false;  // the events bracket the rod launch (carrying an LBM pass)
 };
 
 // The previous step's LBM pass on a second stream, forked from `s` at the
@@ -992,7 +993,7 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
                     in->position, in->velocity, nullptr, in->theta, ed->rest_lengths,
                     ed->bending_rest_angles, ed->bending_stiffness, act, out->steric, p->pred_w,
                     out->position, out->velocity, out->debug_vector, out->theta, fuse_lbm, s,
-                    p->all_chains));
+                    p->all_chains, sy.on_rod ? sy.t0 : nullptr, sy.on_rod ? sy.t1 : nullptr));
   const size_t hwf = (size_t)p->lat_cells;
   long long* tick = act ? p->tick : nullptr;
   if (side.lbm) HIPCHK(hipStreamWaitEvent(s, side.join, 0));
@@ -1004,7 +1005,7 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
                                  p->ib.force);
     } else {
       HIPCHK(launch_lbm(c, p->bd, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
-                        p->ib.force, s, sy.t0, sy.t1));
+                        p->ib.force, s, sy.on_rod ? nullptr : sy.t0, sy.on_rod ? nullptr : sy.t1));
     }
   } else {
     if (tick) HIPCHK(launch_tick(tick, s));
@@ -1889,6 +1890,67 @@ mc_status mc_profile_step(mc_plan* p, const mc_params* q, const mc_state* st,
     HIPCHK(hipMemcpyAsync(st->f_grid, p->f_tmp, 9 * (size_t)p->lat_cells * sizeof(float),
                           hipMemcpyDeviceToDevice, s));
   *lbm_ms_avg = lbm / (double)num_steps;
+  *step_ms_avg = (double)tot / (double)num_steps;
+  return MC_OK;
+}
+
+mc_status mc_profile_fused_step(mc_plan* p, const mc_params* q, const mc_state* st,
+                                const mc_edge_params* ed, const mc_actuation* act, double dt,
+                                int64_t num_steps, double* kernel_ms_avg, double* step_ms_avg,
+                                void* stream) {
+  g_err.clear();
+  if (!p || !kernel_ms_avg || !step_ms_avg) return fail(MC_ERR_INVALID, "NULL argument");
+  MCCHK(check_params(p, q, true, true));
+  MCCHK(check_state(p, st, "state"));
+  MCCHK(check_edges(p, ed));
+  if (num_steps < 2 || (num_steps & 1))
+    return fail(MC_ERR_BAD_SHAPE, "num_steps must be even and at least 2");
+  if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
+  if (!q->enable_fluid || !p->rod_fuse_ok)
+    return fail(MC_ERR_UNSUPPORTED, "no LBM pass is carried by the rod launch on this plan");
+  hipStream_t s = (hipStream_t)stream;
+  const Consts c = derive(p, q, dt);
+  MCCHK(ensure_geo(p, q));
+  if (q->enable_steric) MCCHK(ensure_kf(p, q->steric_sigma, s));
+  Actuation A{};
+  const Actuation* ap = nullptr;
+  if (act) {
+    A = Actuation{act->amp, act->omega, act->phase, p->tick, nullptr};
+    ap = &A;
+    long long t0 = (long long)act->t0;
+    HIPCHK(hipMemcpyAsync(p->tick, &t0, sizeof(t0), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  mc_state sa = *st, sb = *st;
+  sb.f_grid = p->f_tmp;
+  std::vector<hipEvent_t> ev(2 * (size_t)num_steps + 2);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(ev[0], s));
+  LbmArgs pend[2];
+  const LbmArgs* fuse = nullptr;
+  for (int64_t k = 0; k < num_steps; ++k) {
+    const int par = (int)(k & 1);
+    LbmArgs* defer = k + 1 < num_steps ? &pend[par] : nullptr;
+    StepSync sy;
+    sy.t0 = ev[2 * k + 1];
+    sy.t1 = ev[2 * k + 2];
+    sy.on_rod = true;
+    MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, s, sy, fuse, defer));
+    fuse = defer;
+  }
+  HIPCHK(hipEventRecord(ev[2 * num_steps + 1], s));
+  HIPCHK(hipEventSynchronize(ev[2 * num_steps + 1]));
+  // steps 1 .. n-1 carry the previous step's LBM pass in their rod launch
+  double kern = 0.0;
+  for (int64_t k = 1; k < num_steps; ++k) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev[2 * k + 1], ev[2 * k + 2]));
+    kern += ms;
+  }
+  float tot = 0.0f;
+  HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[2 * num_steps + 1]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *kernel_ms_avg = kern / (double)(num_steps - 1);
   *step_ms_avg = (double)tot / (double)num_steps;
   return MC_OK;
 }

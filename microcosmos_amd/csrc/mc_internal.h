@@ -203,7 +203,8 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
                       const float* kb, const Actuation* act, const float* steric,
                       const float* pred_w, float* pos_out, float* vel_out, float* debug_out,
                       float* theta_out, const LbmArgs* lbm, hipStream_t s,
-                      bool all_chains = false);
+                      bool all_chains = false, hipEvent_t ev0 = nullptr,
+                      hipEvent_t ev1 = nullptr);
 // Raises the dynamic LDS limit of the rod kernels to lds_max - 1024 bytes.
 hipError_t rod_prepare(int lds_max);
 hipError_t launch_steric_nodes(const Consts& c, const BatchDev& b, int N, const float* steric,

@@ -13,6 +13,8 @@
 // The kernel also performs the step's node prologue -- damping, inertial
 // prediction, steric sampling (forces.py:20-81) and the steric push -- and
 // its epilogue, v = (x - x_prev) / dt.
+#include <hip/hip_ext.h>
+
 #include "mc_common.h"
 #include "mc_internal.h"
 #include "mc_lbm.h"
@@ -692,7 +694,7 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
                       const float* rest_ang, const float* kb, const Actuation* act,
                       const float* steric, const float* pred_w, float* pos_out, float* vel_out,
                       float* debug_out, float* theta_out, const LbmArgs* lbm, hipStream_t s,
-                      bool all_chains) {
+                      bool all_chains, hipEvent_t ev0, hipEvent_t ev1) {
   if (L.num_comp == 0 && !lbm) return hipSuccess;
   RodArgs a;
   a.c = c;
@@ -722,10 +724,13 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
   const bool chains = all_chains && mode == 0 && use_lds;
   if (lbm) {
     const int rb = (L.num_comp + 7) & ~7;
+    // events carried by the dispatch itself (measurement) bracket its execution
     if (chains)
-      hipLaunchKernelGGL(k_rod_lbm<true>, dim3(rb + 4 * b.lbm_blocks), dim3(64), lds, s, a, *lbm, rb);
+      hipExtLaunchKernelGGL(k_rod_lbm<true>, dim3(rb + 4 * b.lbm_blocks), dim3(64), lds, s, ev0,
+                            ev1, 0, a, *lbm, rb);
     else
-      hipLaunchKernelGGL(k_rod_lbm<false>, dim3(rb + 4 * b.lbm_blocks), dim3(64), lds, s, a, *lbm, rb);
+      hipExtLaunchKernelGGL(k_rod_lbm<false>, dim3(rb + 4 * b.lbm_blocks), dim3(64), lds, s, ev0,
+                            ev1, 0, a, *lbm, rb);
   } else {
     if (chains)
       hipLaunchKernelGGL(k_rod<true>, dim3(L.num_comp), dim3(64), lds, s, a);

@@ -216,8 +216,9 @@ class Rollout:
         return out
 
     def profile(self, num_steps: int):
-        """num_steps steps with HIP events around each fused LBM launch:
-        returns (average LBM kernel ms, average step ms)."""
+        """num_steps steps (no graph, each LBM pass its own launch) with HIP
+        events carried by each LBM launch: returns (average LBM kernel ms,
+        average step ms)."""
         lib = _lib.load()
         st = self.state()
         act = None
@@ -232,6 +233,26 @@ class Rollout:
                                   stream_handle(self.dev)))
         self.t += int(num_steps)
         return lbm.value, stp.value
+
+    def profile_fused(self, num_steps: int):
+        """num_steps (even) steps with each LBM pass carried by the next
+        step's rod launch, as in the rollout graphs, and HIP events carried by
+        those launches: returns (average rod+LBM launch ms, average step ms)."""
+        lib = _lib.load()
+        st = self.state()
+        act = None
+        if self.act is not None:
+            a, o, p = self.act
+            act = McActuation(ptr(a), ptr(o), ptr(p), int(self.t))
+        kern = ctypes.c_double()
+        stp = ctypes.c_double()
+        check(lib.mc_profile_fused_step(self.plan.handle, ctypes.byref(self.params),
+                                        ctypes.byref(st), ctypes.byref(self.ep),
+                                        ctypes.byref(act) if act else None, self.dt,
+                                        int(num_steps), ctypes.byref(kern), ctypes.byref(stp),
+                                        stream_handle(self.dev)))
+        self.t += int(num_steps)
+        return kern.value, stp.value
 
     def track(self, groups=None, start_step: int = 0):
         """Displacement tracking of swim_base.py:179-209 over this rollout's
