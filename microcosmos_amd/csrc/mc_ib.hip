@@ -6,16 +6,22 @@
 // stencils share cells are merged into clusters; no cell is touched by two
 // clusters, so one workgroup can run a whole cluster's IB -- kernel-weight
 // fold, every iteration, post-clamp, force and node drag -- with workgroup
-// barriers only.  Three launches per step:
-//   1. stencil: per node, wrapped start cell, normalised K*K weights
-//      (fluid.py:40-87) and v_ib; per tap, a compare-and-swap that makes the
-//      first arriving component the cell's owner.  The node records which of
-//      its taps won; a tap that finds another component's claim emits a link;
-//   2. cluster build (one workgroup): union-find over the links (in LDS),
-//      cluster labels, node counts, offsets and member lists;
-//   3. cluster solve (one workgroup per cluster): the cluster's nodes ranked
-//      by id, its covered cells ("segments", one per won tap), each cell's
-//      entry list in ascending node id, then the reference's arithmetic.
+// barriers only.  Launches per step:
+//   1. claim (k_ib_claim): per node, wrapped start cell, normalised K*K
+//      weights (fluid.py:40-87) and v_ib; per tap, a compare-and-swap that
+//      makes the first arriving component the cell's owner.  The node records
+//      which of its taps won; a tap that finds another component's claim
+//      emits a link into its instance's link bucket;
+//   2. solve (k_ib_solve, one workgroup per component): the workgroup of a
+//      cluster's smallest component unions the instance's links in LDS, lists
+//      the cluster's members, ranks its nodes by id, builds its covered cells
+//      ("segments", one per won tap) and each cell's entry list in ascending
+//      node id, then runs the reference's arithmetic; the other members'
+//      workgroups return.  Instances with more components than the LDS
+//      union-find holds take a one-workgroup cluster build
+//      (k_ib_cluster_build) between the two launches instead;
+//   3. wide solve (k_ib_wide): clusters above the per-workgroup size, all of
+//      them together, with grid barriers (returns at once when there is none).
 //
 // The reference scatters node-major, tap-minor (fluid.py:111-118,130-132).  A
 // node's stencil covers each cell at most once (lattice >= K per axis), so a
@@ -25,8 +31,8 @@
 // atomics and is arbitrary; no floating-point result depends on it.
 //
 // Idle state between calls: owner == -1 on every lattice cell and no pending
-// links.  The solve resets the owners of the cells it covered, the build
-// clears the link count.
+// links.  The solves reset the owners of the cells they covered and clear
+// the link counts.
 #include <algorithm>
 
 #include "mc_bin.h"
