@@ -10,11 +10,15 @@ adjoint kernel backwards through it, so the device memory of a gradient is
 (checkpoints + one segment) x state instead of the whole trajectory.
 
 Scope: the step without the fluid (PBD or stable Cosserat constraint,
-``enable_fluid=False``, with or without the steric term) with the bending
-rest angles (and, through ``backward_params``, the rest lengths) as the
-parameters.  The steric term couples every filament each
-step, so with it every step's state is kept (``checkpoint_every`` is 1) and
-each reverse step recomputes that step's steric field.
+``enable_fluid=False``, with or without the steric term), single or batched,
+and the fluid-coupled step (``enable_fluid=True``, single-instance plans:
+the immersed boundary with the node positions held constant, as the
+reference's stop_gradient does at fluid.py:239, and the TRT/Guo lattice),
+with the bending rest angles (and, through ``backward_params``, the rest
+lengths) as the parameters.  The steric term and the fluid couple every
+filament each step, so with either every step's state is kept
+(``checkpoint_every`` is 1; with the fluid the step's input lattice too) and
+each reverse step recomputes that step's rod half.
 """
 from __future__ import annotations
 
