@@ -376,6 +376,15 @@ mc_status mc_rod_adjoint_backward_params(mc_plan* plan, const mc_params* params,
  * when the axis needs the FFT (more than 129 taps), -1 on a bad argument. */
 int32_t mc_filter_taps(int32_t n, double sigma, int32_t* lo, float* taps, int32_t capacity);
 
+/* The kernels' float32 math routines evaluated on the host (no device work;
+ * no counterpart in the reference -- it exposes the routines that replace
+ * jnp.sin / jnp.cos / jnp.exp in the filament cycles (pbd.py:63-64,
+ * stable_cosserat.py:111-112), the IB weights (fluid.py:31-38) and the steric
+ * window (forces.py:72-75), for checking): fn 0 writes sin(x[i]) to out0 and
+ * cos(x[i]) to out1, fn 1 writes exp(x[i]) to out0.  Returns MC_OK or
+ * MC_ERR_INVALID. */
+int32_t mc_math_f32(int32_t fn, const float* x, int64_t n, float* out0, float* out1);
+
 /* The linear-resize taps of mc_rollout_record along one axis (host only, no
  * device work; no counterpart in the reference -- it exposes the tabulation
  * of jax.image.resize's linear weights, simulate.py:24-28, for checking):

@@ -131,6 +131,7 @@ SIGNATURES = {
     "mc_track_reset": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "mc_track_read": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mc_filter_taps": (c_int32, [c_int32, c_double, c_void_p, c_void_p, c_int32]),
+    "mc_math_f32": (c_int32, [c_int32, c_void_p, c_int64, c_void_p, c_void_p]),
     "mc_resize_taps": (c_int32, [c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32]),
     "mc_point_loss_workspace": (c_size_t, [c_int64, c_int64]),
     "mc_point_loss": (c_int32, [c_int32, c_void_p, c_int64, c_void_p, c_int64, c_double, c_void_p,

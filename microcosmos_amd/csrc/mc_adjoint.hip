@@ -151,9 +151,10 @@ __device__ __forceinline__ void seg_step(const Consts& c, SegCtx& s, float4* rec
       }
       s.TH[e] = th;
       const float l = s.LE[e];
-      double sd, cd;
-      sincos((double)th, &sd, &cd);
-      if (rec) rec[(size_t)cyc * ne + e] = make_float4(rx, ry, (float)cd, (float)sd);
+      float sf, cf;
+      fsincos(th, &sf, &cf);
+      const double sd = sf, cd = cf;
+      if (rec) rec[(size_t)cyc * ne + e] = make_float4(rx, ry, cf, sf);
       if (!is_cos) {
         const float tx = (float)cd * l, ty = (float)sd * l;
         const float len = sqrtf(rx * rx + ry * ry);
@@ -663,7 +664,7 @@ __global__ void k_steric_adj_nodes(Consts c, BatchDev bd, int N, const float* __
     if (j < lo || j >= hi) continue;
     const float rx = min_image(px, pred_w[2 * j], gi.dom_wf);
     const float ry = min_image(py, pred_w[2 * j + 1], gi.dom_hf);
-    const float Kv = dexp(-(rx * rx + ry * ry) / c.steric_2s2) / c.steric_norm;
+    const float Kv = fexp(-(rx * rx + ry * ry) / c.steric_2s2) / c.steric_norm;
     const float dgx = pb1[2 * j] * sdt - gx, dgy = pb1[2 * j + 1] * sdt - gy;
     const float rd = (rx * dgx + ry * dgy) * inv_s2;
     wx = wx + Kv * (dgx - rx * rd);

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/microcosmos_hip.h"
+#include "mc_common.h"
 #include "mc_internal.h"
 
 #define MC_VERSION "microcosmos-hip 0.1.0 (gfx950)"
@@ -919,9 +920,10 @@ hipError_t run_filter(mc_plan* p, const float* real_in, const StericDep* dep, fl
 // writes the filtered field to steric_out.
 mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float* vel,
                      float* steric_out, float* density_out, hipStream_t s,
-                     float* pred_out = nullptr) {
-  HIPCHK(launch_predict(c, p->bd, p->L, pos, vel, pred_out ? pred_out : p->pred_w, p->cic_cell,
-                        p->cic_w, p->comp_rows, p->dep_top, s));
+                     float* pred_out = nullptr, bool pred_ready = false) {
+  if (!pred_ready)
+    HIPCHK(launch_predict(c, p->bd, p->L, pos, vel, pred_out ? pred_out : p->pred_w, p->cic_cell,
+                          p->cic_w, p->comp_rows, p->dep_top, s));
   if (!c.enable_steric) return MC_OK;
   StericDep d;
   d.L = p->L;
@@ -961,6 +963,14 @@ mc_status ensure_side(mc_plan* p) {
   return MC_OK;
 }
 
+// Whether the steps of a sequence on one stream can hand the next step's
+// inertial prediction to the IB launch that ends the step before (the fluid
+// runs; MC_PREDICT_LAUNCH keeps the prediction launch of every step).
+bool chain_predict(const mc_plan* p, const Consts& c) {
+  static const bool off = getenv("MC_PREDICT_LAUNCH") != nullptr;
+  return c.enable_fluid && p->N > 0 && !off;
+}
+
 // Enqueues one coupled step on `s`.  The LBM pass of the step may be deferred
 // (defer_lbm != NULL: its arguments are returned instead of launched); the
 // next step then runs it inside its rod launch (fuse_lbm), where it fills the
@@ -972,11 +982,16 @@ mc_status ensure_side(mc_plan* p) {
 // instead, beside this step's steric and rod launches (which read neither the
 // lattice nor the IB force and mask), and joins `s` before this step's IB,
 // which reads the lattice it writes and rewrites the force and mask it reads.
+//
+// pred_ready: the previous step on this stream (whose output is `in`) already
+// made this step's inertial prediction; tail_pred: this step makes the next
+// step's, at the end of its IB (both only where chain_predict holds).
 mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc_state* in,
                        const mc_state* out, const mc_edge_params* ed, const Actuation* act,
                        hipStream_t s, const StepSync& sy = StepSync(),
                        const LbmArgs* fuse_lbm = nullptr, LbmArgs* defer_lbm = nullptr,
-                       const SideLbm& side = SideLbm()) {
+                       const SideLbm& side = SideLbm(), bool pred_ready = false,
+                       bool tail_pred = false) {
   if (side.lbm) {
     HIPCHK(hipEventRecord(side.fork, s));
     HIPCHK(hipStreamWaitEvent(side.stream, side.fork, 0));
@@ -985,7 +1000,8 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
                       nullptr));
     HIPCHK(hipEventRecord(side.join, side.stream));
   }
-  MCCHK(run_steric(p, c, in->position, in->velocity, out->steric, nullptr, s));
+  MCCHK(run_steric(p, c, in->position, in->velocity, out->steric, nullptr, s, nullptr,
+                   pred_ready));
   if (!c.enable_steric && out->steric != in->steric)
     HIPCHK(hipMemcpyAsync(out->steric, in->steric, (size_t)p->grid_cells * sizeof(float),
                           hipMemcpyDeviceToDevice, s));
@@ -998,8 +1014,10 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
   long long* tick = act ? p->tick : nullptr;
   if (side.lbm) HIPCHK(hipStreamWaitEvent(s, side.join, 0));
   if (c.enable_fluid) {
+    PredictArgs pa{c, p->bd, p->L, out->position, out->velocity, p->pred_w, p->cic_cell,
+                   p->cic_w, p->comp_rows, p->dep_top};
     HIPCHK(launch_ib(c, p->bd, p->ib, in->f_grid, out->position, out->velocity, out->position, tick,
-                     s));
+                     s, tail_pred ? &pa : nullptr));
     if (defer_lbm) {
       *defer_lbm = make_lbm_args(c, p->bd, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
                                  p->ib.force);
@@ -1371,8 +1389,9 @@ mc_status enqueue_steps(mc_plan* p, const mc_params* q, const Consts& c, const m
       sd.fork = p->side_fork[k % ROLLOUT_GRAPH_STEPS];
       sd.join = p->side_join[k % ROLLOUT_GRAPH_STEPS];
     }
+    const bool ch = chain_predict(p, c);
     MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, nullptr, s, StepSync(),
-                       side_mode ? nullptr : fuse, defer, sd));
+                       side_mode ? nullptr : fuse, defer, sd, ch && k > 0, ch && k + 1 < steps));
     fuse = defer;
   }
   return MC_OK;
@@ -1734,9 +1753,11 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
   const bool recording = rec_pos || rec_vel;
   if (!use_graph || recording) {
     // step by step on the caller's stream (records between steps)
+    const bool ch = chain_predict(p, c);
     for (int64_t k = 0; k < num_steps; ++k) {
       const int par = (int)(k & 1);
-      MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, us));
+      MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, us, StepSync(),
+                         nullptr, nullptr, SideLbm(), ch && k > 0, ch && k + 1 < num_steps));
       MCCHK(record(k, us));
       MCCHK(frame(us));
     }
@@ -1790,8 +1811,9 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
           sd.fork = p->side_fork[k % ROLLOUT_GRAPH_STEPS];
           sd.join = p->side_join[k % ROLLOUT_GRAPH_STEPS];
         }
+        const bool ch = chain_predict(p, c);
         r = enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, cs, StepSync(),
-                         side_mode ? nullptr : fuse, defer, sd);
+                         side_mode ? nullptr : fuse, defer, sd, ch && k > 0, ch && k + 1 < len);
         if (r == MC_OK) r = frame(cs);
         fuse = defer;
       }
@@ -1935,7 +1957,9 @@ mc_status mc_profile_fused_step(mc_plan* p, const mc_params* q, const mc_state* 
     sy.t0 = ev[2 * k + 1];
     sy.t1 = ev[2 * k + 2];
     sy.on_rod = true;
-    MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, s, sy, fuse, defer));
+    const bool ch = chain_predict(p, c);
+    MCCHK(enqueue_step(p, q, c, par ? &sb : &sa, par ? &sa : &sb, ed, ap, s, sy, fuse, defer,
+                       SideLbm(), ch && k > 0, ch && k + 1 < num_steps));
     fuse = defer;
   }
   HIPCHK(hipEventRecord(ev[2 * num_steps + 1], s));
@@ -2606,6 +2630,17 @@ int32_t mc_filter_taps(int32_t n, double sigma, int32_t* lo, float* taps, int32_
   *lo = l;
   for (int i = 0; i < (int)t.size() && i < capacity; ++i) taps[i] = t[i];
   return (int32_t)t.size();
+}
+
+int32_t mc_math_f32(int32_t fn, const float* x, int64_t n, float* out0, float* out1) {
+  g_err.clear();
+  if (n < 0 || (n > 0 && (!x || !out0 || (fn == 0 && !out1))) || fn < 0 || fn > 1)
+    return fail(MC_ERR_INVALID, "mc_math_f32: bad argument");
+  for (int64_t i = 0; i < n; ++i) {
+    if (fn == 0) mc::fsincos(x[i], &out0[i], &out1[i]);
+    else out0[i] = mc::fexp(x[i]);
+  }
+  return MC_OK;
 }
 
 int32_t mc_resize_taps(int32_t n_in, int32_t n_out, int32_t* offsets, int32_t* index,

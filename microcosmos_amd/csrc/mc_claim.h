@@ -6,10 +6,10 @@
 
 namespace mc {
 
-// fluid.py:31-38 -- Peskin 4-point cosine kernel, evaluated in double.
+// fluid.py:31-38 -- Peskin 4-point cosine kernel (float32 cosine).
 __device__ __forceinline__ float peskin(float r, float h) {
   float ra = fabsf(r) / h;
-  return (ra <= 2.0f) ? 0.25f * (1.0f + dcos((MC_PI_F * ra) / 2.0f)) : 0.0f;
+  return (ra <= 2.0f) ? 0.25f * (1.0f + fcos((MC_PI_F * ra) / 2.0f)) : 0.0f;
 }
 
 // Wavefront-aggregated atomic add: one device-scope atomic per wavefront

@@ -39,6 +39,7 @@
 #include "mc_claim.h"
 #include "mc_common.h"
 #include "mc_internal.h"
+#include "mc_predict.h"
 
 namespace mc {
 namespace {
@@ -1163,14 +1164,13 @@ __device__ __forceinline__ float2 wide_corr(const Consts& c, const float* __rest
 }
 
 template <int K>
-__device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H, int* wsum,
-                                             int* s_off) {
+__device__ __forceinline__ bool cluster_wide(const ClusterArgs& a, int h, int H, int* wsum,
+                                             int* s_off, unsigned int& target) {
   constexpr int K2 = K * K;
   const Consts& c = a.c;
   WideCtl* ctl = a.wctl;
   const int tid = threadIdx.x, bs = blockDim.x;
   const int gt = h * bs + tid, gs = H * bs;
-  unsigned int target = 0;
   auto gsync = [&]() {
     target += H;
     return grid_sync(ctl, target, &s_off[2]);
@@ -1212,7 +1212,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
       }
     }
   }
-  if (!gsync()) return;
+  if (!gsync()) return false;
   const int nq = ld_agent(&ctl->nq);
   const int ns = ld_agent(&ctl->nseg);
   // 2. every tap's segment; entry ranges of the segments
@@ -1231,7 +1231,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
   bool ok = true;
   const int tot = grid_scan(ns, [&](int s) { return ld_agent(&wc[sc[s]]); }, L.sstart, ctl, h, H,
                             target, wsum, s_off, ok);
-  if (!ok) return;
+  if (!ok) return false;
   if (gt == 0) L.sstart[ns] = tot;
   // 3. entries in arrival order; the cell counters return to zero; first
   // gather and correction
@@ -1255,7 +1255,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
     if (c.iters > 0)
       L.corr[n] = wide_corr<K2>(c, w, seg, L.suc, make_float2(a.vib[2 * n], a.vib[2 * n + 1]), gi.umax);
   }
-  if (!gsync()) return;
+  if (!gsync()) return false;
   // 4. each entry's rank among its segment's entries -> ascending entry ids
   for (int q = gt; q < nq * K2; q += gs) {
     const int r = q / K2, t = q - r * K2;
@@ -1266,7 +1266,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
     for (int j = lo; j < hi; ++j) rank += L.eplace[j] < e ? 1 : 0;
     L.esort[lo + rank] = e;
   }
-  if (!gsync()) return;
+  if (!gsync()) return false;
   // 5. kernel-weight fold and 1/max(kw, 1) (fluid.py:123-133,289-290), the
   // first spread; then the remaining iterations (fluid.py:292-316)
   for (int s = gt; s < ns; s += gs) {
@@ -1281,7 +1281,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
       L.suc[s] = u;
     }
   }
-  if (!gsync()) return;
+  if (!gsync()) return false;
   for (int it = 1; it < c.iters; ++it) {
     for (int r = gt; r < nq; r += gs) {
       const int n = L.list[r];
@@ -1295,13 +1295,13 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
       L.corr[n] = wide_corr<K2>(c, w, seg, L.suc, make_float2(a.vib[2 * n], a.vib[2 * n + 1]),
                                 a.bd.geo[a.bd.node_inst[n]].umax);
     }
-    if (!gsync()) return;
+    if (!gsync()) return false;
     for (int s = gt; s < ns; s += gs) {
       float2 u = L.suc[s];
       wide_spread<K2>(L.esort, L.sstart[s], L.sstart[s + 1], wts, L.corr, L.sinv[s], u);
       L.suc[s] = u;
     }
-    if (!gsync()) return;
+    if (!gsync()) return false;
   }
   // 6. post-clamp and force (fluid.py:327-339); F replaces u_corrected
   for (int s = gt; s < ns; s += gs) {
@@ -1331,7 +1331,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
     frc[HW + cell] = fy;
     atomicOr(&a.mask[gi.mask_off + (cell >> 5)], 1u << (cell & 31));
   }
-  if (!gsync()) return;
+  if (!gsync()) return false;
   // node drag (fluid.py:364-382) and the final wrap (simulate.py:97)
   for (int r = gt; r < nq; r += gs) {
     const int n = L.list[r];
@@ -1356,6 +1356,7 @@ __device__ __forceinline__ void cluster_wide(const ClusterArgs& a, int h, int H,
     a.pos_out[2 * n] = fmodf_floor(a.pos[2 * n], gi.dom_wf);
     a.pos_out[2 * n + 1] = fmodf_floor(a.pos[2 * n + 1], gi.dom_hf);
   }
+  return true;
 }
 
 // A solve workgroup has read its instance's link bucket: the last workgroup
@@ -1626,27 +1627,53 @@ constexpr int WIDE_THREADS = 256;  // small workgroups: a launch with nothing to
 // The number of participating workgroups follows the problem: one per
 // IB_WIDE_NODES_PER_WG wide nodes (the others return at once), so small
 // problems pay for few barrier arrivals and large ones get the whole launch.
+//
+// With `predict`, the launch then makes the next step's inertial prediction
+// (one wavefront per filament, predict_component): the filaments of wide
+// clusters by the participating workgroups after a last grid barrier (their
+// nodes' final state is complete), every other filament by any workgroup
+// (its final state came from the cluster solve launch).  The launch then
+// runs even when no instance can form a wide cluster (wide_wg 0).
 template <int K>
-__global__ void __launch_bounds__(WIDE_THREADS) k_ib_wide(ClusterArgs a) {
+__global__ void __launch_bounds__(WIDE_THREADS) k_ib_wide(ClusterArgs a, PredictArgs pr,
+                                                          int predict) {
   __shared__ int wsum[16];
   __shared__ int s_off[3];
-  const int nodes = __hip_atomic_load(&a.wctl->wide_nodes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (nodes == 0) return;
-  const int H = min((int)gridDim.x, max(1, (nodes + IB_WIDE_NODES_PER_WG - 1) / IB_WIDE_NODES_PER_WG));
-  if ((int)blockIdx.x >= H) return;
-  if (__hip_atomic_load(&a.wctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-    // an earlier solve gave up at a barrier: its cell counters are not back
-    // at zero until launch_ib_reset, so only the final wrap of the wide
-    // clusters' nodes is done (simulate.py:97); the host reports the error
-    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += H * blockDim.x) {
-      if (a.cl_cnt[a.label[a.comp_of_node[n]]] <= a.wide_min) continue;
-      const InstGeo& gi = a.bd.geo[a.bd.node_inst[n]];
-      a.pos_out[2 * n] = fmodf_floor(a.pos[2 * n], gi.dom_wf);
-      a.pos_out[2 * n + 1] = fmodf_floor(a.pos[2 * n + 1], gi.dom_hf);
+  const int nodes = a.wide_wg > 0 ? __hip_atomic_load(&a.wctl->wide_nodes, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0;
+  const int H =
+      nodes == 0 ? 0 : min(a.wide_wg, max(1, (nodes + IB_WIDE_NODES_PER_WG - 1) / IB_WIDE_NODES_PER_WG));
+  bool wide_ready = false;  // this workgroup predicts the wide clusters' filaments
+  if ((int)blockIdx.x < H) {
+    if (__hip_atomic_load(&a.wctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      // an earlier solve gave up at a barrier: its cell counters are not back
+      // at zero until launch_ib_reset, so only the final wrap of the wide
+      // clusters' nodes is done (simulate.py:97); the host reports the error
+      for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += H * blockDim.x) {
+        if (a.cl_cnt[a.label[a.comp_of_node[n]]] <= a.wide_min) continue;
+        const InstGeo& gi = a.bd.geo[a.bd.node_inst[n]];
+        a.pos_out[2 * n] = fmodf_floor(a.pos[2 * n], gi.dom_wf);
+        a.pos_out[2 * n + 1] = fmodf_floor(a.pos[2 * n + 1], gi.dom_hf);
+      }
+    } else {
+      unsigned int target = 0;
+      if (cluster_wide<K>(a, blockIdx.x, H, wsum, s_off, target) && predict) {
+        target += H;
+        wide_ready = grid_sync(a.wctl, target, &s_off[2]);
+      }
     }
-    return;
   }
-  cluster_wide<K>(a, blockIdx.x, H, wsum, s_off);
+  if (!predict) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *pr.arena_top = 0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nwv = blockDim.x >> 6;
+  auto is_wide = [&](int f) { return H > 0 && a.cl_cnt[a.label[f]] > a.wide_min; };
+  for (int f = blockIdx.x * nwv + wave; f < a.C; f += gridDim.x * nwv)
+    if (!is_wide(f)) predict_component(pr, f, lane);
+  if (wide_ready)
+    for (int f = blockIdx.x * nwv + wave; f < a.C; f += H * nwv)
+      if (is_wide(f)) predict_component(pr, f, lane);
 }
 
 __global__ void k_ib_reset(long long HW, int C, int* __restrict__ owner, int* __restrict__ parent,
@@ -1721,7 +1748,7 @@ hipError_t launch_ib_reset(const IBWork& w, hipStream_t s) {
 
 hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float* f_in,
                      const float* pos, float* vel_io, float* pos_out, long long* tick,
-                     hipStream_t s) {
+                     hipStream_t s, const PredictArgs* pred) {
   const int N = w.N, C = w.C;
   if (N == 0) {
     // no nodes: no support; the LBM pass takes the clamp-only force everywhere
@@ -1800,9 +1827,14 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   MC_KSWITCH(L_CLUSTER);
 #undef L_CLUSTER
   e = hipGetLastError();
-  if (e == hipSuccess && w.wide_wg > 0) {
+  if (e == hipSuccess && (w.wide_wg > 0 || pred)) {
+    // the wide solve's workgroups, and with `pred` a wavefront per filament
+    constexpr int waves = WIDE_THREADS / 64;
+    const int G = std::max(std::max(w.wide_wg, 1), pred ? (C + waves - 1) / waves : 0);
+    const PredictArgs pa = pred ? *pred : PredictArgs{};
+    const int on = pred ? 1 : 0;
 #define L_WIDE(KK) \
-  hipLaunchKernelGGL(k_ib_wide<KK>, dim3(w.wide_wg), dim3(WIDE_THREADS), 0, s, a)
+  hipLaunchKernelGGL(k_ib_wide<KK>, dim3(G), dim3(WIDE_THREADS), 0, s, a, pa, on)
     MC_KSWITCH(L_WIDE);
 #undef L_WIDE
     e = hipGetLastError();

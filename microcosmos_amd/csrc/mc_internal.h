@@ -161,6 +161,20 @@ hipError_t launch_predict(const Consts& c, const BatchDev& b, const RodLayoutDev
 // Raises the dynamic LDS limit of the steric kernels to lds_max - 1024 bytes.
 hipError_t steric_prepare(int lds_max);
 hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t s);
+// The inertial prediction of a step (launch_predict, or the IB launch that
+// ends the previous step): state in, prediction and CIC set-up out.
+struct PredictArgs {
+  Consts c;
+  BatchDev bd;
+  RodLayoutDev L;
+  const float* pos;
+  const float* vel;
+  float* pred_w;
+  int* cic_cell;
+  float* cic_w;
+  int4* comp_rows;
+  int* arena_top;  // the deposit's overflow arena counter, cleared
+};
 // Inputs of the CIC deposit (launch_predict's outputs and scratch).
 struct StericDep {
   RodLayoutDev L;
@@ -370,10 +384,12 @@ hipError_t ib_prepare(int lds_max);
 // Whole IB of a step: stencil + cluster links, cluster build, cluster solve
 // (support, iterations, force into w.force / w.mask, node drag into vel_io,
 // final wrap of pos into pos_out).  Only the solve reads f_in.  `tick` (may be
-// NULL) is advanced once.
+// NULL) is advanced once.  `pred` (may be NULL): the next step's inertial
+// prediction from the final state (pos_out, vel_io), made by the launch that
+// ends the IB (the wide solve's), so the next step needs no launch for it.
 hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float* f_in,
                      const float* pos, float* vel_io, float* pos_out, long long* tick,
-                     hipStream_t s);
+                     hipStream_t s, const PredictArgs* pred = nullptr);
 // Restores the idle state between calls (cell owners, union-find parents, the
 // link count, the wide solve's cell counters) and clears WideCtl::err.
 hipError_t launch_ib_reset(const IBWork& w, hipStream_t s);

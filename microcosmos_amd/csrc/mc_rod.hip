@@ -53,18 +53,68 @@ __device__ __forceinline__ float rest_angle(const RodArgs& a, int gb, long long 
   return (a.has_rows && a.act.d_ang) ? r + a.act.d_ang[row * a.act.b_stride + gb] : r;
 }
 
+// Bilinear sample with index wrap (map_coordinates order 1, mode "wrap").
+// The sample points are wrapped predictions +- 0.5, so the taps lie in
+// [-1, side + 1] and wrap with selects.
 __device__ __forceinline__ float sample_bilinear_wrap(const float* V, int H, int W, float yy, float xx) {
   const float ly = floorf(yy), lx = floorf(xx);
   const float wyh = yy - ly, wxh = xx - lx;
   const float wyl = 1.0f - wyh, wxl = 1.0f - wxh;
   const int iy = f2i(ly), ix = f2i(lx);
-  const int y0 = imod(iy, H), y1 = imod(iy + 1, H);
-  const int x0 = imod(ix, W), x1 = imod(ix + 1, W);
+  const int y0 = wrap1c(iy, H), y1 = wrap1c(iy + 1, H);
+  const int x0 = wrap1c(ix, W), x1 = wrap1c(ix + 1, W);
   float out = (wyl * wxl) * V[(size_t)y0 * W + x0];
   out = out + (wyl * wxh) * V[(size_t)y0 * W + x1];
   out = out + (wyh * wxl) * V[(size_t)y1 * W + x0];
   out = out + (wyh * wxh) * V[(size_t)y1 * W + x1];
   return out;
+}
+
+// The analytic window term of forces.py:66-81 for node g at (px, py): the
+// (2n+1)-window over the flat node index (within the node's instance), summed
+// in window order.  Its terms are independent (their displacements, kernel
+// values and exponentials overlap, four at a time, with no branch between
+// them): EXACT = false takes the minimum images on their common range and
+// returns whether a lane left it (the caller then repeats with EXACT = true);
+// FROM_LDS reads the window's predictions from pw (staged from node pw0).
+template <bool EXACT, bool FROM_LDS>
+__device__ __forceinline__ bool steric_window(const Consts& c, const InstGeo& gi,
+                                              const float* pred_w, const float2* pw, int pw0,
+                                              int g, float px, float py, float& ax, float& ay) {
+  const int lo = gi.node0, hi = gi.node0 + gi.nodes;
+  const int nsk = c.steric_nskip;
+  bool slow = false;
+  ax = 0.0f;
+  ay = 0.0f;
+  constexpr int WU = 4;
+  for (int k0 = -nsk; k0 <= nsk; k0 += WU) {
+    float rxv[WU], ryv[WU], kvv[WU];
+#pragma unroll
+    for (int u = 0; u < WU; ++u) {
+      const int j = g + min(k0 + u, nsk);  // past the window: a repeat, not summed
+      const float valid = (j >= lo && j < hi) ? 1.0f : 0.0f;
+      const int js = j < lo ? lo : (j > hi - 1 ? hi - 1 : j);
+      const float2 q =
+          FROM_LDS ? pw[js - pw0] : *reinterpret_cast<const float2*>(pred_w + 2 * js);
+      if (EXACT) {
+        rxv[u] = min_image(px, q.x, gi.dom_wf);
+        ryv[u] = min_image(py, q.y, gi.dom_hf);
+      } else {
+        rxv[u] = min_image_fast(px, q.x, gi.dom_wf, slow);
+        ryv[u] = min_image_fast(py, q.y, gi.dom_hf, slow);
+      }
+      const float d2 = rxv[u] * rxv[u] + ryv[u] * ryv[u];
+      const float Kv = fexp(-d2 / c.steric_2s2) / c.steric_norm;
+      kvv[u] = Kv * valid;
+    }
+#pragma unroll
+    for (int u = 0; u < WU; ++u)
+      if (k0 + u <= nsk) {
+        ax = ax + kvv[u] * rxv[u];
+        ay = ay + kvv[u] * ryv[u];
+      }
+  }
+  return slow;
 }
 
 // forces.py:20-81 for one node: -grad V by central differences minus the
@@ -77,27 +127,18 @@ __device__ __forceinline__ void steric_force(const Consts& c, const InstGeo& gi,
                                              const float2* pw = nullptr, int pw0 = 0) {
   const int H = gi.H, W = gi.W;
   const float* V = V_all + gi.grid_off;
-  const int lo = gi.node0, hi = gi.node0 + gi.nodes;
   const float2 pg = *reinterpret_cast<const float2*>(pred_w + 2 * g);
   const float px = pg.x, py = pg.y;
   const float gx = (sample_bilinear_wrap(V, H, W, py, px + 0.5f) -
                     sample_bilinear_wrap(V, H, W, py, px - 0.5f)) / 1.0f;
   const float gy = (sample_bilinear_wrap(V, H, W, py + 0.5f, px) -
                     sample_bilinear_wrap(V, H, W, py - 0.5f, px)) / 1.0f;
-  float ax = 0.0f, ay = 0.0f;
-  const int nsk = c.steric_nskip;
-  for (int k = -nsk; k <= nsk; ++k) {
-    const int j = g + k;
-    const float valid = (j >= lo && j < hi) ? 1.0f : 0.0f;
-    const int js = j < lo ? lo : (j > hi - 1 ? hi - 1 : j);
-    const float2 q = pw ? pw[js - pw0] : *reinterpret_cast<const float2*>(pred_w + 2 * js);
-    const float rx = min_image(px, q.x, gi.dom_wf);
-    const float ry = min_image(py, q.y, gi.dom_hf);
-    const float d2 = rx * rx + ry * ry;
-    const float Kv = dexp(-d2 / c.steric_2s2) / c.steric_norm;
-    const float kv = Kv * valid;
-    ax = ax + kv * rx;
-    ay = ay + kv * ry;
+  float ax, ay;
+  bool slow = pw ? steric_window<false, true>(c, gi, pred_w, pw, pw0, g, px, py, ax, ay)
+                 : steric_window<false, false>(c, gi, pred_w, pw, pw0, g, px, py, ax, ay);
+  if (slow) {
+    if (pw) steric_window<true, true>(c, gi, pred_w, pw, pw0, g, px, py, ax, ay);
+    else steric_window<true, false>(c, gi, pred_w, pw, pw0, g, px, py, ax, ay);
   }
   fx = -gx - c.steric_svs * ax;
   fy = -gy - c.steric_svs * ay;
@@ -331,9 +372,8 @@ __device__ __forceinline__ void rod_component(const RodArgs& a, int comp, float*
       }
       TH[e] = th;
       const float l = Le[e];
-      double sd, cd;
-      sincos((double)th, &sd, &cd);
-      const float dx = (float)cd, dy = (float)sd;
+      float dx, dy;
+      fsincos(th, &dy, &dx);
       if (!is_cos) {
         const float tx = dx * l, ty = dy * l;
         const float len = sqrtf(rx * rx + ry * ry);
@@ -508,22 +548,27 @@ __device__ __forceinline__ void rod_chain(const RodArgs& a, const CompMeta& m, f
   const float kl = c.k_st / l, he = ise ? c.k_st / (l * l) : 0.0f;
   const float Wd = gi.dom_wf, Hd = gi.dom_hf;
   for (int cyc = 0; cyc < c.cycles; ++cyc) {
-    // theta pass: hinge corrections (pbd.py:32-40)
+    // theta pass: hinge corrections (pbd.py:32-40); the cycle's wraps on
+    // their common ranges, one check for the wavefront, exact otherwise
     const float th_next = from_next(th);
-    float corr = 0.0f;
-    if (isb) {
-      const float diff = wrap_angle(th_next - th);
-      corr = ((diff - rs) * kb) * 0.5f;
-    }
-    const float corr_prev = from_prev(corr);
     const float xn = from_next(X), yn = from_next(Y);
+    bool slow_b = false, slow_e = false;
+    float diff = wrap_angle_fast(th_next - th, slow_b);
+    float rx = min_image_fast(xn, X, Wd, slow_e);
+    float ry = min_image_fast(yn, Y, Hd, slow_e);
+    if ((slow_b && isb) || (slow_e && ise)) {
+      diff = wrap_angle(th_next - th);
+      rx = min_image(xn, X, Wd);
+      ry = min_image(yn, Y, Hd);
+    }
+    float corr = 0.0f;
+    if (isb) corr = ((diff - rs) * kb) * 0.5f;
+    const float corr_prev = from_prev(corr);
     float fex = 0.0f, fey = 0.0f;
     if (ise) {
       float d = 0.0f;
       if (lane < nb) d = d + corr;
       if (lane >= 1) d = d + (-corr_prev);
-      const float rx = min_image(xn, X, Wd);
-      const float ry = min_image(yn, Y, Hd);
       if (!is_cos) {
         th = th + (d * deg_e) * 1.8f;
       } else {
@@ -532,9 +577,8 @@ __device__ __forceinline__ void rod_chain(const RodArgs& a, const CompMeta& m, f
         const float pull = wrap_angle(alpha - th);
         th = th + pull * 0.05f;
       }
-      double sd, cd;
-      sincos((double)th, &sd, &cd);
-      const float dx = (float)cd, dy = (float)sd;
+      float dx, dy;
+      fsincos(th, &dy, &dx);
       if (!is_cos) {
         const float tx = dx * l, ty = dy * l;
         const float len = sqrtf(rx * rx + ry * ry);
@@ -722,6 +766,7 @@ hipError_t launch_rod(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
   a.theta_out = theta_out;
   const size_t lds = use_lds ? lds_bytes : 0;
   const bool chains = all_chains && mode == 0 && use_lds;
+
   if (lbm) {
     const int rb = (L.num_comp + 7) & ~7;
     // events carried by the dispatch itself (measurement) bracket its execution

@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "mc_common.h"
+#include "mc_predict.h"
 #include "mc_internal.h"
 
 namespace mc {
@@ -336,30 +337,8 @@ __device__ __forceinline__ float2* fft_any(float2* a, float2* b, int n, const in
 #undef MC_CT_CASE
 }
 
-// Wrapped inertial prediction of node n (simulate.py:46-50, forces.py:95);
-// without vel, the positions as given.
-__device__ __forceinline__ void predict_xy(const Consts& c, const InstGeo& gi,
-                                           const float* __restrict__ pos,
-                                           const float* __restrict__ vel, int n, float& x,
-                                           float& y) {
-  float px = pos[2 * n], py = pos[2 * n + 1];
-  if (vel) {
-    const float vx = vel[2 * n] * c.damping, vy = vel[2 * n + 1] * c.damping;
-    px = px + vx * c.dt;
-    py = py + vy * c.dt;
-  }
-  x = fmodf_floor(px, gi.dom_wf);
-  y = fmodf_floor(py, gi.dom_hf);
-}
-
-// One wavefront per filament (connected component of the edge graph): each
-// node's wrapped prediction, CIC cell and weights (forces.py:95-110), and the
-// band of instance rows the component's CIC entries touch, rows
-// start .. start + len - 1 (mod H), with the component's node count and ids.  A node's row y0 is the first node's row
-// plus a minimum-image offset d in [-H/2, H/2); the band runs from min d to
-// max d + 1, so it holds every y0 and y1 = y0 + 1 of the component whatever
-// its shape (a loose band only costs the row pass extra checks).  Block 0
-// also clears the deposit's overflow arena counter.
+// One wavefront per filament (see predict_component).  Block 0 also clears
+// the deposit's overflow arena counter.
 __global__ void __launch_bounds__(64) k_predict(Consts c, BatchDev bd, RodLayoutDev L,
                                                 const float* __restrict__ pos,
                                                 const float* __restrict__ vel,
@@ -368,54 +347,9 @@ __global__ void __launch_bounds__(64) k_predict(Consts c, BatchDev bd, RodLayout
                                                 float* __restrict__ cic_w,
                                                 int4* __restrict__ comp_rows,
                                                 int* __restrict__ arena_top) {
-  const int comp = blockIdx.x, lane = threadIdx.x;
-  if (comp == 0 && lane == 0) *arena_top = 0;
-  const CompMeta m = L.meta[comp];
-  const InstGeo& gi = bd.geo[m.inst];
-  const int H = gi.H, W = gi.W;
-  auto node_id = [&](int i) { return m.nf >= 0 ? m.nf + i : L.comp_nodes[m.n0 + i]; };
-  int yref = 0;
-  if (c.enable_steric) {
-    float x, y;
-    predict_xy(c, gi, pos, vel, node_id(0), x, y);
-    yref = imod(f2i(floorf(y)), H);
-  }
-  int lo = 0x7fffffff, hi = -0x7fffffff;
-  for (int i = lane; i < m.nn; i += 64) {
-    const int n = node_id(i);
-    float x, y;
-    predict_xy(c, gi, pos, vel, n, x, y);
-    pred_w[2 * n] = x;
-    pred_w[2 * n + 1] = y;
-    if (!c.enable_steric) continue;
-    const float fx = floorf(x), fy = floorf(y);
-    const int x0 = imod(f2i(fx), W), y0 = imod(f2i(fy), H);
-    const int x1 = imod(x0 + 1, W), y1 = imod(y0 + 1, H);
-    const float dx = x - fx, dy = y - fy;
-    const float sv = c.steric_sv;
-    // columns within the instance, rows as global steric rows
-    *reinterpret_cast<int4*>(cic_cell + 4 * (size_t)n) =
-        make_int4(x0, x1, gi.row0 + y0, gi.row0 + y1);
-    *reinterpret_cast<float4*>(cic_w + 4 * (size_t)n) =
-        make_float4(((1.0f - dx) * (1.0f - dy)) * sv, (dx * (1.0f - dy)) * sv,
-                    ((1.0f - dx) * dy) * sv, (dx * dy) * sv);
-    const int d = imod(y0 - yref + H / 2, H) - H / 2;
-    lo = min(lo, d);
-    hi = max(hi, d);
-  }
-  if (!c.enable_steric) return;
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = min(lo, __shfl_xor(lo, o, 64));
-    hi = max(hi, __shfl_xor(hi, o, 64));
-  }
-  if (lane == 0) {
-    const int len = hi + 2 - lo;  // rows lo .. hi + 1 relative to yref
-    // with the component's node count and first id (contiguous ids) or the
-    // complement of its node-list offset, for the row pass
-    const int first = m.nf >= 0 ? m.nf : ~m.n0;
-    comp_rows[comp] = len >= H ? make_int4(0, H, m.nn, first)
-                               : make_int4(imod(yref + lo, H), len, m.nn, first);
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *arena_top = 0;
+  PredictArgs a{c, bd, L, pos, vel, pred_w, cic_cell, cic_w, comp_rows, arena_top};
+  predict_component(a, blockIdx.x, threadIdx.x & 63);
 }
 
 __device__ __forceinline__ int xcd_group(int b, int G) {

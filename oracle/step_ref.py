@@ -20,9 +20,13 @@ Evaluation-order conventions (chosen to be reproducible on the GPU):
     applies updates in index order);
   * every small reduction (9 populations, 16 stencil taps, 2n+1 steric window)
     is a left fold in index order -- numpy's pairwise ``sum`` is never used;
-  * in float32 runs, cos/sin/atan2/exp are evaluated in float64 and rounded to
-    float32 (a correctly rounded result); the device does the same.  XLA-CPU's
-    own float32 polynomials are not reproduced (they differ by <= ~1 ulp);
+  * in float32 runs, the filament angles' sin/cos, the IB weights' cos and the
+    steric window's exp use float32 routines (``sincos_f32``, ``exp_f32``:
+    Cody-Waite reduction and Taylor polynomials, restated operation by
+    operation, within ~2 ulp / 1e-7 absolute of the float64 functions);
+    atan2 and the other exps are evaluated in float64 and rounded to float32.
+    The device does the same.  XLA-CPU's own float32 polynomials are not
+    reproduced (they differ from either by a few ulp);
   * the 2-D FFT is restated as the device's Stockham radix-4/2/3 network
     (rows, then columns, per-axis inverse scaling); it is pinned against
     numpy's pocketfft in float64 and a brute-force DFT.  jnp.fft (ducc, FP32)
@@ -173,6 +177,84 @@ def _tf(fn, x):
     if x.dtype == np.float32:
         return fn(x.astype(np.float64)).astype(np.float32)
     return fn(x)
+
+
+# Float32 sine / cosine / exp of the device's latency-bound chains (the
+# filament cycles, the IB weights, the steric window), restated operation by
+# operation: Cody-Waite reduction by pi/2 (three-part split, exact products for
+# |x| <= 8192) and the Taylor series of sin to x^9, cos to x^10 on
+# [-pi/4, pi/4]; exp by ln 2 reduction (two-part split), the series to x^7 and
+# ldexp.  Every step is one IEEE float32 operation in the device's order, so
+# the device reproduces these bit for bit.  Pinned against the float64 library
+# functions (absolute error, ulp bounds) in the oracle tests.
+_H = float.fromhex
+_F = np.float32
+_TWO_OVER_PI = _F(_H("0x1.45f306p-1"))
+_PIO2 = (_F(_H("0x1.92p+0")), _F(_H("0x1.fb4p-12")), _F(_H("0x1.4442d2p-24")))
+_SIN_C = (_F(_H("-0x1.555556p-3")), _F(_H("0x1.111112p-7")), _F(_H("-0x1.a01a02p-13")),
+          _F(_H("0x1.71de3ap-19")))
+_COS_C = (_F(_H("0x1.555556p-5")), _F(_H("-0x1.6c16c2p-10")), _F(_H("0x1.a01a02p-16")),
+          _F(_H("-0x1.27e4fcp-22")))
+_LOG2E = _F(_H("0x1.715476p+0"))
+_LN2 = (_F(_H("0x1.63p-1")), _F(_H("-0x1.bd0106p-13")))
+_EXP_C = (_F(1.0), _F(0.5), _F(_H("0x1.555556p-3")), _F(_H("0x1.555556p-5")),
+          _F(_H("0x1.111112p-7")), _F(_H("0x1.6c16c2p-10")), _F(_H("0x1.a01a02p-13")))
+_SINCOS_LIMIT = _F(8192.0)
+
+
+def sincos_f32(x):
+    """(sin x, cos x) in float32 by the device's reduction and polynomials;
+    beyond |x| = 8192 the float64 functions rounded to float32."""
+    x = np.asarray(x, dtype=np.float32)
+    with np.errstate(invalid="ignore", over="ignore"):
+        k = np.rint(x * _TWO_OVER_PI)
+        r = ((x - k * _PIO2[0]) - k * _PIO2[1]) - k * _PIO2[2]
+        z = r * r
+        s1, s2, s3, s4 = _SIN_C
+        c2, c3, c4, c5 = _COS_C
+        ps = r + (r * z) * (s1 + z * (s2 + z * (s3 + z * s4)))
+        pc = (_F(1.0) - _F(0.5) * z) + (z * z) * (c2 + z * (c3 + z * (c4 + z * c5)))
+        q = np.where(k == k, k, _F(0.0)).astype(np.int64) & 3
+        sn = np.where(q == 0, ps, np.where(q == 1, pc, np.where(q == 2, -ps, -pc)))
+        cs = np.where(q == 0, pc, np.where(q == 1, -ps, np.where(q == 2, -pc, ps)))
+        big = np.abs(x) > _SINCOS_LIMIT
+        if np.any(big):
+            xd = x.astype(np.float64)
+            sn = np.where(big, np.sin(xd).astype(np.float32), sn)
+            cs = np.where(big, np.cos(xd).astype(np.float32), cs)
+    return sn.astype(np.float32), cs.astype(np.float32)
+
+
+def exp_f32(x):
+    """exp(x) in float32 by the device's reduction, polynomial and ldexp; the
+    argument clamped to [-104, 88.5]."""
+    x = np.asarray(x, dtype=np.float32)
+    x = np.where(x < _F(-104.0), _F(-104.0), np.where(x > _F(88.5), _F(88.5), x)).astype(np.float32)
+    with np.errstate(invalid="ignore"):
+        n = np.rint(x * _LOG2E)
+        r = (x - n * _LN2[0]) - n * _LN2[1]
+        e1, e2, e3, e4, e5, e6, e7 = _EXP_C
+        p = _F(1.0) + r * (e1 + r * (e2 + r * (e3 + r * (e4 + r * (e5 + r * (e6 + r * e7))))))
+        ni = np.where(n == n, n, _F(0.0)).astype(np.int32)
+    return np.ldexp(p, ni).astype(np.float32)
+
+
+def _sincos(x):
+    """(sin, cos) of the filament angles: the float32 routine, or float64."""
+    x = np.asarray(x)
+    if x.dtype == np.float32:
+        return sincos_f32(x)
+    return np.sin(x), np.cos(x)
+
+
+def _cos(x):
+    x = np.asarray(x)
+    return sincos_f32(x)[1] if x.dtype == np.float32 else np.cos(x)
+
+
+def _exp(x):
+    x = np.asarray(x)
+    return exp_f32(x) if x.dtype == np.float32 else np.exp(x)
 
 
 def _tf2(fn, a, b):
@@ -484,7 +566,7 @@ def steric_window_subtraction(pos, grid_shape, sigma, neighbor_skip, scatter_val
     j_safe = np.clip(j_idx, 0, n - 1)
     r = displacement(grid_shape, pos[:, None, :], pos[j_safe])
     d2 = r[..., 0] * r[..., 0] + r[..., 1] * r[..., 1]
-    K = _tf(np.exp, -d2 / (2.0 * sigma * sigma)) / (2.0 * np.pi * sigma * sigma)
+    K = _exp(-d2 / (2.0 * sigma * sigma)) / (2.0 * np.pi * sigma * sigma)
     terms = (K * valid)[..., None] * r
     acc = np.zeros((n, 2), dtype=dtype)
     for k in range(terms.shape[1]):
@@ -543,7 +625,8 @@ def pbd_rod_constraint(s: State, p: Params):
     np.add.at(deg_n, src, 1.0)
     np.add.at(deg_n, tgt, 1.0)
     deg_n = np.maximum(deg_n, 1.0)
-    dirs = np.stack([_tf(np.cos, s.theta), _tf(np.sin, s.theta)], axis=-1)
+    sn, cs = _sincos(s.theta)
+    dirs = np.stack([cs, sn], axis=-1)
     target = dirs * s.rest_lengths[:, None]
     cur = displacement(s.grid_shape, s.position[tgt], s.position[src])
     length = np.sqrt(cur[:, 0:1] * cur[:, 0:1] + cur[:, 1:2] * cur[:, 1:2])
@@ -580,7 +663,8 @@ def stable_cosserat_constraint(s: State, p: Params):
 
     k_s = p.stiffness_stretch
     w = 1.0 / (p.config_dt ** 2)
-    d = np.stack([_tf(np.cos, theta_new), _tf(np.sin, theta_new)], axis=-1)
+    sn, cs = _sincos(theta_new)
+    d = np.stack([cs, sn], axis=-1)
     r = displacement(s.grid_shape, x[tgt], x[src])
     c = r / l[:, None] - d
     f_edge = (k_s / l)[:, None] * c
@@ -600,7 +684,7 @@ def stable_cosserat_constraint(s: State, p: Params):
 def kernel_1d_peskin(r, h):
     """fluid.py:31-38 -- 0.25 (1 + cos(pi |r|/h / 2)) for |r|/h <= 2."""
     r_abs = np.abs(r) / h
-    return np.where(r_abs <= 2.0, 0.25 * (1.0 + _tf(np.cos, np.pi * r_abs / 2.0)), 0.0).astype(r.dtype)
+    return np.where(r_abs <= 2.0, 0.25 * (1.0 + _cos(np.pi * r_abs / 2.0)), 0.0).astype(r.dtype)
 
 
 def ib_stencil(grid_pos, h, w, kernel_size):

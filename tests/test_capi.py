@@ -16,7 +16,7 @@ ROOT = Path(__file__).resolve().parents[1]
 def _header_functions():
     txt = (ROOT / "include" / "microcosmos_hip.h").read_text()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(mc_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(mc_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_header_declares_entry_points():

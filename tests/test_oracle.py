@@ -527,3 +527,80 @@ def test_library_filter_taps_equal_the_restatement(n, sigma):
         return
     assert cnt == ref[1].size and lo.value == ref[0]
     np.testing.assert_array_equal(np.frombuffer(buf, np.float32)[:cnt], ref[1].astype(np.float32))
+
+
+def _f32_math_inputs():
+    rng = np.random.default_rng(11)
+    special = np.array([0.0, -0.0, np.nan, np.inf, -np.inf, 8192.0, -8192.0, 8192.001, 1e-30,
+                        -1e-30, 1e-40, np.pi / 4, -np.pi / 4, np.pi / 2, np.pi, 3 * np.pi / 2,
+                        2 * np.pi, 1e4, -1e6, 3.0e38], dtype=np.float32)
+    return np.concatenate([special, rng.uniform(-8200, 8200, 200_000).astype(np.float32),
+                           rng.uniform(-12, 12, 200_000).astype(np.float32),
+                           np.linspace(-4, 4, 40_001, dtype=np.float32)])
+
+
+def test_sincos_f32_accuracy():
+    """The float32 sine / cosine of the filament cycles and IB weights against
+    float64 numpy: absolute error below 1e-7 for |x| <= 8192 (reduction by a
+    three-part pi/2 split, Taylor series to x^9 / x^10), within 2 ulp where
+    the value is not near a zero of the function, sin(-x) = -sin(x),
+    cos(-x) = cos(x), and the float64 fallback beyond 8192."""
+    x = _f32_math_inputs()
+    x = x[np.isfinite(x)]
+    s, c = R.sincos_f32(x)
+    xd = x.astype(np.float64)
+    ts, tc = np.sin(xd), np.cos(xd)
+    assert np.abs(s - ts).max() < 1e-7 and np.abs(c - tc).max() < 1e-7
+    for got, ref in ((s, ts), (c, tc)):
+        far = (np.abs(ref) > 0.05) & (np.abs(xd) < 16)
+        ulp = np.spacing(np.abs(ref.astype(np.float32))).astype(np.float64)
+        assert (np.abs(got[far] - ref[far]) / ulp[far]).max() <= 2.0
+    s2, c2 = R.sincos_f32(-x)
+    np.testing.assert_array_equal(s2, -s)
+    np.testing.assert_array_equal(c2, c)
+    big = np.abs(x) > 8192
+    np.testing.assert_array_equal(s[big], ts[big].astype(np.float32))
+    # quadrants: sin / cos at the multiples of pi/2 to 1e-7
+    q = np.arange(-8, 9, dtype=np.float32) * np.float32(np.pi / 2)
+    sq, cq = R.sincos_f32(q)
+    np.testing.assert_allclose(sq, np.sin(q.astype(np.float64)), atol=1e-7)
+    np.testing.assert_allclose(cq, np.cos(q.astype(np.float64)), atol=1e-7)
+
+
+def test_exp_f32_accuracy():
+    """The float32 exp of the steric window against float64 numpy: within
+    1.5 ulp for normal results, one rounding of the denormal range, +0 below
+    -104, NaN propagates, exp(0) = 1 exactly."""
+    rng = np.random.default_rng(12)
+    x = np.concatenate([rng.uniform(-110, 0, 400_000), rng.uniform(-8, 8, 200_000),
+                        rng.uniform(0, 88, 50_000), -np.linspace(0, 30, 30_001)]).astype(np.float32)
+    e = R.exp_f32(x)
+    t = np.exp(x.astype(np.float64))
+    normal = t >= np.finfo(np.float32).tiny
+    ulp = np.spacing(t[normal].astype(np.float32)).astype(np.float64)
+    assert (np.abs(e[normal] - t[normal]) / ulp).max() <= 1.5
+    assert (np.abs(e[~normal] - t[~normal]) / 2.0 ** -149).max() <= 1.0
+    assert R.exp_f32(np.float32(0.0)) == 1.0
+    assert np.isnan(R.exp_f32(np.float32(np.nan)))
+    assert R.exp_f32(np.float32(-1e6)) == 0.0 and R.exp_f32(np.float32(-105.0)) == 0.0
+
+
+def test_library_f32_math_equals_the_restatement():
+    """The kernels' float32 sine / cosine / exp (mc_math_f32: the same source
+    evaluated on the host, no device work) equal the restatement bit for bit,
+    special values included."""
+    import ctypes
+    from microcosmos_amd import _lib
+    lib = _lib.load()
+    x = _f32_math_inputs()
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    assert lib.mc_math_f32(0, ptr(x), x.size, ptr(s), ptr(c)) == 0
+    rs, rc = R.sincos_f32(x)
+    np.testing.assert_array_equal(s.view(np.uint32), rs.view(np.uint32))
+    np.testing.assert_array_equal(c.view(np.uint32), rc.view(np.uint32))
+    xe = np.concatenate([x, -np.abs(x), np.linspace(-110, 0, 50_001, dtype=np.float32)])
+    e = np.empty_like(xe)
+    assert lib.mc_math_f32(1, ptr(xe), xe.size, ptr(e), None) == 0
+    np.testing.assert_array_equal(e.view(np.uint32), R.exp_f32(xe).view(np.uint32))
