@@ -161,6 +161,7 @@ struct mc_plan {
   int* cic_cell = nullptr;
   float* cic_w = nullptr;
   int4* comp_rows = nullptr;    // per component: steric row band, node count, ids
+  int2* comp_cols = nullptr;    // per component: steric column band
   int *dep_arena = nullptr, *dep_top = nullptr, *row_sort = nullptr;
   unsigned long long* row_keys = nullptr;
   float2* spec = nullptr;
@@ -179,6 +180,10 @@ struct mc_plan {
   bool filter_direct = false;   // some shape's table holds direct-filter taps
   bool filter_fft = false;      // some shape's table holds a transfer function
   bool filter_fused = false;    // every direct shape's kernels fit the one-launch 2-D filter
+  // the CIC deposit inside the 2-D filter launch (every direct instance wider
+  // and taller than a tile's window and at most one node per 32 cells)
+  bool filter_dep_fused = false;
+  DepEntry* tile_arena = nullptr;  // 2 x 16N entries, allocated on first use
   hipStream_t cap_stream = nullptr;
   // the instantiated S-step rollout graph of the last mc_rollout call, reused
   // while its key matches: slot 0 the ROLLOUT_GRAPH_STEPS-step graph
@@ -896,6 +901,22 @@ mc_status ensure_kf(mc_plan* p, double sigma, hipStream_t s) {
   p->filter_direct = any_direct;
   p->filter_fft = any_fft;
   p->filter_fused = any_direct && max_taps <= 33 && !getenv("MC_FILTER_TWO_PASS");
+  // the deposit joins the 2-D filter launch when each window holds every
+  // grid cell at most once and the nodes are sparse (a window's entries then
+  // fit the LDS); MC_DEPOSIT_ROWS keeps the row deposit launch
+  bool dep_fused = p->filter_fused && !getenv("MC_DEPOSIT_ROWS") && p->N > 0;
+  for (size_t i = 0; i < p->shapes.size() && dep_fused; ++i) {
+    const int H = p->shapes[i].x, W = p->shapes[i].y;
+    int lx, ly;
+    std::vector<float> tx, ty;
+    if (!(gauss_taps(W, sigma, &lx, &tx) && gauss_taps(H, sigma, &ly, &ty))) continue;
+    dep_fused = H >= FILTER_TILE_H + (int)ty.size() - 1 && W >= FILTER_TILE_W + (int)tx.size() - 1;
+  }
+  for (const InstHost& h : p->inst)
+    dep_fused = dep_fused && (long long)h.nodes * 32 <= (long long)h.H * h.W;
+  if (dep_fused && !p->tile_arena)
+    MCCHK(p->alloc(&p->tile_arena, 32 * (size_t)p->N, "fused deposit arena"));
+  p->filter_dep_fused = dep_fused;
   p->kf_sigma = sigma;
   return MC_OK;
 }
@@ -923,17 +944,20 @@ mc_status run_steric(mc_plan* p, const Consts& c, const float* pos, const float*
                      float* pred_out = nullptr, bool pred_ready = false) {
   if (!pred_ready)
     HIPCHK(launch_predict(c, p->bd, p->L, pos, vel, pred_out ? pred_out : p->pred_w, p->cic_cell,
-                          p->cic_w, p->comp_rows, p->dep_top, s));
+                          p->cic_w, p->comp_rows, p->comp_cols, p->dep_top, s));
   if (!c.enable_steric) return MC_OK;
   StericDep d;
   d.L = p->L;
   d.comp_rows = p->comp_rows;
+  d.comp_cols = p->comp_cols;
   d.cic_cell = p->cic_cell;
   d.cic_w = p->cic_w;
   d.arena = p->dep_arena;
   d.arena_top = p->dep_top;
   d.sort_tmp = p->row_sort;
   d.key_tmp = p->row_keys;
+  d.tile_arena = p->filter_dep_fused ? p->tile_arena : nullptr;
+  d.tile_top = p->dep_top + 1;
   d.key_half = (size_t)4 * p->N;
   HIPCHK(run_filter(p, nullptr, &d, density_out, steric_out, s));
   return MC_OK;
@@ -1015,7 +1039,7 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
   if (side.lbm) HIPCHK(hipStreamWaitEvent(s, side.join, 0));
   if (c.enable_fluid) {
     PredictArgs pa{c, p->bd, p->L, out->position, out->velocity, p->pred_w, p->cic_cell,
-                   p->cic_w, p->comp_rows, p->dep_top};
+                   p->cic_w, p->comp_rows, p->comp_cols, p->dep_top};
     HIPCHK(launch_ib(c, p->bd, p->ib, in->f_grid, out->position, out->velocity, out->position, tick,
                      s, tail_pred ? &pa : nullptr));
     if (defer_lbm) {
@@ -1126,8 +1150,9 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
   if ((st = p->alloc(&p->cic_cell, 4 * (size_t)N, "cic cell")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->cic_w, 4 * (size_t)N, "cic w")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->comp_rows, (size_t)p->L.num_comp, "comp rows")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->comp_cols, (size_t)p->L.num_comp, "comp cols")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->dep_arena, 2 * (size_t)N, "deposit arena")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&p->dep_top, 1, "deposit arena top")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&p->dep_top, 2, "deposit arena tops")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_sort, 2 * (size_t)N, "row sort")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->row_keys, 8 * (size_t)N, "row keys")) != MC_OK) return bail(st);
   // immersed boundary

@@ -1665,7 +1665,7 @@ __global__ void __launch_bounds__(WIDE_THREADS) k_ib_wide(ClusterArgs a, Predict
     }
   }
   if (!predict) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *pr.arena_top = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) pr.arena_top[0] = pr.arena_top[1] = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nwv = blockDim.x >> 6;
   auto is_wide = [&](int f) { return H > 0 && a.cl_cnt[a.label[f]] > a.wide_min; };

@@ -157,7 +157,8 @@ struct LbmArgs {
 // of steric rows its CIC entries touch (comp_rows); clears *arena_top.
 hipError_t launch_predict(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
                           const float* pos, const float* vel, float* pred_w, int* cic_cell,
-                          float* cic_w, int4* comp_rows, int* arena_top, hipStream_t s);
+                          float* cic_w, int4* comp_rows, int2* comp_cols, int* arena_top,
+                          hipStream_t s);
 // Raises the dynamic LDS limit of the steric kernels to lds_max - 1024 bytes.
 hipError_t steric_prepare(int lds_max);
 hipError_t launch_gauss_table(int H, int W, float coef, float* kfT, hipStream_t s);
@@ -173,19 +174,32 @@ struct PredictArgs {
   int* cic_cell;
   float* cic_w;
   int4* comp_rows;
-  int* arena_top;  // the deposit's overflow arena counter, cleared
+  int2* comp_cols;  // per component: steric column band (first column, columns)
+  int* arena_top;  // the deposits' overflow counters (rows, tiles), cleared
 };
 // Inputs of the CIC deposit (launch_predict's outputs and scratch).
+// One CIC contribution staged by the fused deposit + filter launch: window
+// cell << 2 | corner, node id, weight.
+struct DepEntry {
+  int pq, n;
+  float w;
+};
 struct StericDep {
   RodLayoutDev L;
   const int4* comp_rows;
+  const int2* comp_cols;
   const int* cic_cell;
   const float* cic_w;
   int* arena;                   // 2N ints
-  int* arena_top;               // 1 int, cleared by launch_predict
+  int* arena_top;               // 2 ints (rows, tiles), cleared by launch_predict
   int* sort_tmp;                // 2N ints
   unsigned long long* key_tmp;  // 8N
   size_t key_half;              // 4N
+  // the fused deposit + filter launch (NULL: the row deposit launch): its
+  // windows beyond the LDS take slices of tile_arena (2 x 16N entries),
+  // counted by tile_top (cleared with arena_top)
+  DepEntry* tile_arena;
+  int* tile_top;
 };
 // Steric filter (FFT path) over every instance whose table holds a transfer
 // function: forward rows (CIC deposit from `dep` when real_in is NULL),

@@ -32,20 +32,22 @@ __device__ __forceinline__ void predict_xy(const Consts& c, const InstGeo& gi,
 // node count and ids.  A node's row y0 is the first node's row plus a
 // minimum-image offset d in [-H/2, H/2); the band runs from min d to max d + 1,
 // so it holds every y0 and y1 = y0 + 1 of the component whatever its shape (a
-// loose band only costs the row pass extra checks).
+// loose band only costs the row pass extra checks); the same for its columns
+// when comp_cols is given.
 __device__ __forceinline__ void predict_component(const PredictArgs& a, int comp, int lane) {
   const Consts& c = a.c;
   const CompMeta m = a.L.meta[comp];
   const InstGeo& gi = a.bd.geo[m.inst];
   const int H = gi.H, W = gi.W;
   auto node_id = [&](int i) { return m.nf >= 0 ? m.nf + i : a.L.comp_nodes[m.n0 + i]; };
-  int yref = 0;
+  int yref = 0, xref = 0;
   if (c.enable_steric) {
     float x, y;
     predict_xy(c, gi, a.pos, a.vel, node_id(0), x, y);
     yref = imod(f2i(floorf(y)), H);
+    xref = imod(f2i(floorf(x)), W);
   }
-  int lo = 0x7fffffff, hi = -0x7fffffff;
+  int lo = 0x7fffffff, hi = -0x7fffffff, clo = 0x7fffffff, chi = -0x7fffffff;
   for (int i = lane; i < m.nn; i += 64) {
     const int n = node_id(i);
     float x, y;
@@ -67,11 +69,16 @@ __device__ __forceinline__ void predict_component(const PredictArgs& a, int comp
     const int d = imod(y0 - yref + H / 2, H) - H / 2;
     lo = min(lo, d);
     hi = max(hi, d);
+    const int dc = imod(x0 - xref + W / 2, W) - W / 2;
+    clo = min(clo, dc);
+    chi = max(chi, dc);
   }
   if (!c.enable_steric) return;
   for (int o = 32; o > 0; o >>= 1) {
     lo = min(lo, __shfl_xor(lo, o, 64));
     hi = max(hi, __shfl_xor(hi, o, 64));
+    clo = min(clo, __shfl_xor(clo, o, 64));
+    chi = max(chi, __shfl_xor(chi, o, 64));
   }
   if (lane == 0) {
     const int len = hi + 2 - lo;  // rows lo .. hi + 1 relative to yref
@@ -80,6 +87,10 @@ __device__ __forceinline__ void predict_component(const PredictArgs& a, int comp
     const int first = m.nf >= 0 ? m.nf : ~m.n0;
     a.comp_rows[comp] = len >= H ? make_int4(0, H, m.nn, first)
                                  : make_int4(imod(yref + lo, H), len, m.nn, first);
+    // the columns its CIC entries touch, the same way (x0 and x0 + 1)
+    const int clen = chi + 2 - clo;
+    if (a.comp_cols)
+      a.comp_cols[comp] = clen >= W ? make_int2(0, W) : make_int2(imod(xref + clo, W), clen);
   }
 }
 

@@ -346,9 +346,10 @@ __global__ void __launch_bounds__(64) k_predict(Consts c, BatchDev bd, RodLayout
                                                 int* __restrict__ cic_cell,
                                                 float* __restrict__ cic_w,
                                                 int4* __restrict__ comp_rows,
+                                                int2* __restrict__ comp_cols,
                                                 int* __restrict__ arena_top) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *arena_top = 0;
-  PredictArgs a{c, bd, L, pos, vel, pred_w, cic_cell, cic_w, comp_rows, arena_top};
+  if (blockIdx.x == 0 && threadIdx.x == 0) arena_top[0] = arena_top[1] = 0;
+  PredictArgs a{c, bd, L, pos, vel, pred_w, cic_cell, cic_w, comp_rows, comp_cols, arena_top};
   predict_component(a, blockIdx.x, threadIdx.x & 63);
 }
 
@@ -376,6 +377,7 @@ struct DepArgs {
   RodLayoutDev L;
   const int4* comp_rows;        // per component: band (first row, rows), node count, first id
                                 // (contiguous ids) or ~(node-list offset)
+  const int2* comp_cols;        // per component: column band (first column, columns)
   const int* cic_cell;
   const float* cic_w;
   int* arena;                   // 2N ids for rows beyond cap
@@ -385,6 +387,8 @@ struct DepArgs {
   size_t key_half;
   int cap;                      // LDS capacity in nodes (a multiple of 4, 16 cap >= 8 blockDim)
   int b_off, tw_off;            // LDS offsets (float2) of the scratch / second buffer and twiddles
+  DepEntry* tile_arena;         // windows of the fused deposit beyond the LDS (2 x 16N entries)
+  int* tile_top;                // tile_arena entries taken this step
 };
 
 // Ids of the nodes whose CIC rows include global row `row`, each once: the
@@ -846,58 +850,16 @@ constexpr int F2_TAPS = 33;
 constexpr int F2_THREADS = 512;
 constexpr int F2_RIN = FILTER_TILE_H + F2_TAPS - 1;          // input rows at most
 constexpr int F2_CS = FILTER_TILE_W + F2_TAPS - 1 + 4;       // input row stride (floats, 16 B)
-__global__ void __launch_bounds__(F2_THREADS) k_filter2d(BatchDev bd, const float* __restrict__ taps,
-                                                         const float* __restrict__ src,
-                                                         float* __restrict__ out) {
-  __shared__ __align__(16) float in[F2_RIN * F2_CS];
-  __shared__ __align__(16) float rt[F2_RIN * FILTER_TILE_W];
-  __shared__ int rowy[F2_RIN], colx[F2_CS];
-  const int b = xcd_group(blockIdx.x, gridDim.x);
-  const int inst = find_inst(bd.ctile, bd.n_inst, b);
-  const InstGeo& gi = bd.geo[inst];
-  if (!filter_direct(taps, gi)) return;
+// The two passes of the 2-D filter over a staged window (`in`, F2_CS floats
+// per row, rin rows): x into `rt`, then y into the tile's outputs.  Each
+// output sums its taps in order from the first offset, as the row and column
+// launches do.
+__device__ __forceinline__ void filter2d_passes(const float* __restrict__ in, float* __restrict__ rt,
+                                                const float* __restrict__ tx,
+                                                const float* __restrict__ ty, int nx, int ny,
+                                                int rin, const InstGeo& gi, int ty0, int tx0,
+                                                float* __restrict__ out) {
   const int W = gi.W, H = gi.H, tid = threadIdx.x;
-  const int txn = (W + FILTER_TILE_W - 1) / FILTER_TILE_W;
-  const int local = b - bd.ctile[inst];
-  const int ty0 = (local / txn) * FILTER_TILE_H, tx0 = (local % txn) * FILTER_TILE_W;
-  const int* hd = reinterpret_cast<const int*>(taps + gi.kf_off);
-  const int lox = hd[0], nx = hd[1], loy = hd[2], ny = hd[3];
-  // the taps are read with uniform addresses (scalar loads) in the passes
-  const float* tx = taps + gi.kf_off + 4;
-  const float* ty = tx + nx;
-  const int rin = FILTER_TILE_H + ny - 1, cin = FILTER_TILE_W + nx - 1;
-  int yb = (ty0 + loy) % H, xb = (tx0 + lox) % W;  // lo > -side
-  if (yb < 0) yb += H;
-  if (xb < 0) xb += W;
-  const float* g = src + gi.grid_off;
-  // the input window, every load of a thread in flight at once: the wrapped
-  // row and column indices tabulated first, then element tid + F2_THREADS k
-  if (tid < rin) rowy[tid] = (yb + tid) % H;
-  if (tid < cin) colx[tid] = (xb + tid) % W;
-  __syncthreads();
-  {
-    constexpr int PER = (F2_RIN * (F2_CS - 4) + F2_THREADS - 1) / F2_THREADS;
-    const int dr = F2_THREADS / cin, dc = F2_THREADS - dr * cin;
-    int r = tid / cin, c = tid - r * cin;
-    float v[PER];
-    int dst[PER];
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const bool ok = r < rin;
-      v[k] = ok ? g[(size_t)rowy[ok ? r : 0] * W + colx[c]] : 0.0f;
-      dst[k] = ok ? r * F2_CS + c : -1;
-      c += dc;
-      r += dr;
-      if (c >= cin) {
-        c -= cin;
-        ++r;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < PER; ++k)
-      if (dst[k] >= 0) in[dst[k]] = v[k];
-  }
-  __syncthreads();
   // rows: four consecutive outputs per thread from one window, taps in
   // chunks of eight (three 16-byte loads per chunk)
   {
@@ -959,6 +921,266 @@ __global__ void __launch_bounds__(F2_THREADS) k_filter2d(BatchDev bd, const floa
   }
 }
 
+__global__ void __launch_bounds__(F2_THREADS) k_filter2d(BatchDev bd, const float* __restrict__ taps,
+                                                         const float* __restrict__ src,
+                                                         float* __restrict__ out) {
+  __shared__ __align__(16) float in[F2_RIN * F2_CS];
+  __shared__ __align__(16) float rt[F2_RIN * FILTER_TILE_W];
+  __shared__ int rowy[F2_RIN], colx[F2_CS];
+  const int b = xcd_group(blockIdx.x, gridDim.x);
+  const int inst = find_inst(bd.ctile, bd.n_inst, b);
+  const InstGeo& gi = bd.geo[inst];
+  if (!filter_direct(taps, gi)) return;
+  const int W = gi.W, H = gi.H, tid = threadIdx.x;
+  const int txn = (W + FILTER_TILE_W - 1) / FILTER_TILE_W;
+  const int local = b - bd.ctile[inst];
+  const int ty0 = (local / txn) * FILTER_TILE_H, tx0 = (local % txn) * FILTER_TILE_W;
+  const int* hd = reinterpret_cast<const int*>(taps + gi.kf_off);
+  const int lox = hd[0], nx = hd[1], loy = hd[2], ny = hd[3];
+  // the taps are read with uniform addresses (scalar loads) in the passes
+  const float* tx = taps + gi.kf_off + 4;
+  const float* ty = tx + nx;
+  const int rin = FILTER_TILE_H + ny - 1, cin = FILTER_TILE_W + nx - 1;
+  int yb = (ty0 + loy) % H, xb = (tx0 + lox) % W;  // lo > -side
+  if (yb < 0) yb += H;
+  if (xb < 0) xb += W;
+  const float* g = src + gi.grid_off;
+  // the input window, every load of a thread in flight at once: the wrapped
+  // row and column indices tabulated first, then element tid + F2_THREADS k
+  if (tid < rin) rowy[tid] = (yb + tid) % H;
+  if (tid < cin) colx[tid] = (xb + tid) % W;
+  __syncthreads();
+  {
+    constexpr int PER = (F2_RIN * (F2_CS - 4) + F2_THREADS - 1) / F2_THREADS;
+    const int dr = F2_THREADS / cin, dc = F2_THREADS - dr * cin;
+    int r = tid / cin, c = tid - r * cin;
+    float v[PER];
+    int dst[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const bool ok = r < rin;
+      v[k] = ok ? g[(size_t)rowy[ok ? r : 0] * W + colx[c]] : 0.0f;
+      dst[k] = ok ? r * F2_CS + c : -1;
+      c += dc;
+      r += dr;
+      if (c >= cin) {
+        c -= cin;
+        ++r;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+      if (dst[k] >= 0) in[dst[k]] = v[k];
+  }
+  __syncthreads();
+  filter2d_passes(in, rt, tx, ty, nx, ny, rin, gi, ty0, tx0, out);
+}
+
+// The CIC deposit (forces.py:100-115) and the 2-D filter in one launch, for
+// instances sparse enough that a tile's window holds few nodes (the launch
+// choice is the plan's).  A workgroup per FILTER_TILE_W x FILTER_TILE_H output
+// tile deposits its input window (the tile's rows and columns plus the
+// kernels' reach, wrapped; each grid cell appears once in it) straight into
+// LDS:
+//   1. the instance's components whose row and column bands meet the window
+//      are compacted; their nodes' CIC corners inside the window become
+//      entries (window cell, corner, node id, weight), counted per window
+//      cell;
+//   2. an exclusive scan of the counts and one placement pass group the
+//      entries by cell;
+//   3. the thread holding a cell's first entry orders the cell's entries by
+//      (corner, node id) -- the reference's scatter order, all w00 by node,
+//      then w10, w01, w11 -- and folds them from zero, exactly as the row
+//      deposit folds them.
+// A window with more entries than the LDS holds redoes steps 1-3 in a slice
+// of tile_arena.  The density is then filtered as k_filter2d does, bit for bit
+// the same as the row deposit followed by k_filter2d.
+constexpr int DF_CELLS = F2_RIN * (FILTER_TILE_W + F2_TAPS - 1);
+constexpr int DF_ENT = (F2_RIN * FILTER_TILE_W * 4) / (int)sizeof(DepEntry);
+__global__ void __launch_bounds__(F2_THREADS) k_dep_filter2d(BatchDev bd,
+                                                             const float* __restrict__ taps,
+                                                             DepArgs dep,
+                                                             float* __restrict__ density_out,
+                                                             float* __restrict__ out) {
+  __shared__ __align__(16) float in[F2_RIN * F2_CS];           // the arrival entries first
+  __shared__ __align__(16) float rt[F2_RIN * FILTER_TILE_W];   // the grouped entries first
+  __shared__ int cnt[DF_CELLS];
+  __shared__ int cand[2 * F2_THREADS];
+  __shared__ int wsum[32];
+  __shared__ int s_m, s_base;
+  const int b = xcd_group(blockIdx.x, gridDim.x);
+  const int inst = find_inst(bd.ctile, bd.n_inst, b);
+  const InstGeo& gi = bd.geo[inst];
+  if (!filter_direct(taps, gi)) return;
+  const int W = gi.W, H = gi.H, tid = threadIdx.x, bs = blockDim.x;
+  const int txn = (W + FILTER_TILE_W - 1) / FILTER_TILE_W;
+  const int local = b - bd.ctile[inst];
+  const int ty0 = (local / txn) * FILTER_TILE_H, tx0 = (local % txn) * FILTER_TILE_W;
+  const int* hd = reinterpret_cast<const int*>(taps + gi.kf_off);
+  const int lox = hd[0], nx = hd[1], loy = hd[2], ny = hd[3];
+  const float* tx = taps + gi.kf_off + 4;
+  const float* ty = tx + nx;
+  const int rin = FILTER_TILE_H + ny - 1, cin = FILTER_TILE_W + nx - 1, ncell = rin * cin;
+  int yb = (ty0 + loy) % H, xb = (tx0 + lox) % W;  // lo > -side
+  if (yb < 0) yb += H;
+  if (xb < 0) xb += W;
+  for (int i = tid; i < rin * F2_CS; i += bs) in[i] = 0.0f;
+  for (int i = tid; i < ncell; i += bs) cnt[i] = 0;
+  if (tid == 0) s_m = 0;
+  __syncthreads();
+  DepEntry* arr = reinterpret_cast<DepEntry*>(in);
+  DepEntry* grp = reinterpret_cast<DepEntry*>(rt);
+  int cap = DF_ENT;
+  int m = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    // 1. entries of the candidate components' nodes, chunk by chunk
+    for (int base = 0; base < gi.ncomp; base += bs) {
+      int is = 0, nn = 0, first = 0;
+      if (base + tid < gi.ncomp) {
+        const int4 r = dep.comp_rows[gi.comp0 + base + tid];
+        const int2 cb = dep.comp_cols[gi.comp0 + base + tid];
+        // bands r.x .. r.x + r.y - 1 and cb.x .. cb.x + cb.y - 1 against the
+        // window's rows yb .. yb + rin - 1 (mod H) and columns xb .. (mod W)
+        if ((r.y >= H || wrap1(r.x - yb, H) < rin || wrap1(yb - r.x, H) < r.y) &&
+            (cb.y >= W || wrap1(cb.x - xb, W) < cin || wrap1(xb - cb.x, W) < cb.y)) {
+          is = 1;
+          nn = r.z;
+          first = r.w;
+        }
+      }
+      int2 tot;
+      const int2 inc = block_inclusive_scan2(make_int2(is, nn), wsum, &tot);
+      if (is) {
+        cand[inc.x - 1] = first;
+        cand[bs + inc.x - 1] = inc.y - nn;
+      }
+      __syncthreads();
+      constexpr int CR = 4;  // candidate nodes per thread with their loads in flight together
+      for (int f0 = tid; f0 < tot.y; f0 += CR * bs) {
+        int nid[CR];
+#pragma unroll
+        for (int u = 0; u < CR; ++u) {
+          const int f = f0 + u * bs;
+          nid[u] = -1;
+          if (f < tot.y) {
+            int lo = 0, hi = tot.x - 1;  // the last candidate whose node offset is <= f
+            while (lo < hi) {
+              const int mid = (lo + hi + 1) >> 1;
+              if (cand[bs + mid] <= f) lo = mid; else hi = mid - 1;
+            }
+            const int loc = f - cand[bs + lo], fc = cand[lo];
+            nid[u] = fc >= 0 ? fc + loc : dep.L.comp_nodes[~fc + loc];
+          }
+        }
+        int4 cc[CR];
+        float4 ww[CR];
+#pragma unroll
+        for (int u = 0; u < CR; ++u) {
+          const size_t q = 4 * (size_t)(nid[u] < 0 ? 0 : nid[u]);
+          cc[u] = *reinterpret_cast<const int4*>(dep.cic_cell + q);
+          ww[u] = *reinterpret_cast<const float4*>(dep.cic_w + q);
+        }
+#pragma unroll
+        for (int u = 0; u < CR; ++u) {
+          if (nid[u] < 0) continue;
+          const float wq[4] = {ww[u].x, ww[u].y, ww[u].z, ww[u].w};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            // corner q: x0 / x1 (q & 1), row y0 / y1 (q & 2); rows are global steric rows
+            const int x = (q & 1) ? cc[u].y : cc[u].x;
+            const int y = ((q & 2) ? cc[u].w : cc[u].z) - gi.row0;
+            const int r = wrap1(y - yb, H), c = wrap1(x - xb, W);
+            if (r >= rin || c >= cin) continue;
+            const int e = atomicAdd(&s_m, 1);
+            if (e >= cap) continue;
+            const int pos = r * cin + c;
+            arr[e] = DepEntry{(pos << 2) | q, nid[u], wq[q]};
+            atomicAdd(&cnt[pos], 1);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    m = s_m;
+    if (m <= cap) break;
+    // 1'. more entries than the LDS holds: all of them again, into a slice
+    // of the arena (arrival and grouped lists of m entries each)
+    __syncthreads();
+    if (tid == 0) {
+      s_base = atomicAdd(dep.tile_top, 2 * m);
+      s_m = 0;
+    }
+    for (int i = tid; i < ncell; i += bs) cnt[i] = 0;
+    for (int i = tid; i < 3 * DF_ENT; i += bs) in[i] = 0.0f;  // the LDS list's partial fill
+    __syncthreads();
+    arr = dep.tile_arena + s_base;
+    grp = arr + m;
+    cap = m;
+  }
+  // 2. exclusive scan of the cells' counts (a run of cells per thread), then
+  // the entries grouped by cell in arrival order; cnt[i] ends as the end of
+  // cell i's entries
+  {
+    const int per = (ncell + bs - 1) / bs;
+    int loc = 0;
+    for (int k = 0; k < per; ++k) {
+      const int i = tid * per + k;
+      if (i < ncell) loc += cnt[i];
+    }
+    int tot;
+    int run = block_inclusive_scan(loc, wsum, &tot) - loc;
+    for (int k = 0; k < per; ++k) {
+      const int i = tid * per + k;
+      if (i < ncell) {
+        const int v = cnt[i];
+        cnt[i] = run;
+        run += v;
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < m; e += bs) {
+    const DepEntry d = arr[e];
+    grp[atomicAdd(&cnt[d.pq >> 2], 1)] = d;
+  }
+  __syncthreads();
+  if (cap == DF_ENT)  // the LDS arrival list is dead: the window back to zero
+    for (int i = tid; i < 3 * m; i += bs) in[i] = 0.0f;
+  __syncthreads();
+  // 3. each cell's fold in (corner, node id) order from zero, by the thread
+  // holding the cell's first entry
+  for (int lo = tid; lo < m; lo += bs) {
+    const int i = grp[lo].pq >> 2;
+    if (lo > 0 && (grp[lo - 1].pq >> 2) == i) continue;
+    int hi = lo + 1;
+    while (hi < m && (grp[hi].pq >> 2) == i) ++hi;
+    for (int j = lo + 1; j < hi; ++j) {
+      const DepEntry v = grp[j];
+      int k = j - 1;
+      while (k >= lo && ((grp[k].pq & 3) > (v.pq & 3) ||
+                         ((grp[k].pq & 3) == (v.pq & 3) && grp[k].n > v.n))) {
+        grp[k + 1] = grp[k];
+        --k;
+      }
+      grp[k + 1] = v;
+    }
+    float acc = 0.0f;
+    for (int j = lo; j < hi; ++j) acc = acc + grp[j].w;
+    in[(i / cin) * F2_CS + (i % cin)] = acc;
+  }
+  __syncthreads();
+  if (density_out) {
+    // the tile's own cells: window rows -loy .., columns -lox ..
+    for (int j = tid; j < FILTER_TILE_H * FILTER_TILE_W; j += bs) {
+      const int oy = j / FILTER_TILE_W, ox = j - oy * FILTER_TILE_W;
+      const int y = ty0 + oy, x = tx0 + ox;
+      if (x < W && y < H)
+        density_out[gi.grid_off + (size_t)y * W + x] = in[(oy - loy) * F2_CS + (ox - lox)];
+    }
+  }
+  filter2d_passes(in, rt, tx, ty, nx, ny, rin, gi, ty0, tx0, out);
+}
+
 // Transposed transfer function kfT[k * H + y] = exp(coef (fr(y)^2 + fc(k)^2)).
 __global__ void k_gauss_table_T(int H, int W, float coef, float* __restrict__ kfT) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -972,10 +1194,11 @@ __global__ void k_gauss_table_T(int H, int W, float coef, float* __restrict__ kf
 
 hipError_t launch_predict(const Consts& c, const BatchDev& b, const RodLayoutDev& L,
                           const float* pos, const float* vel, float* pred_w, int* cic_cell,
-                          float* cic_w, int4* comp_rows, int* arena_top, hipStream_t s) {
+                          float* cic_w, int4* comp_rows, int2* comp_cols, int* arena_top,
+                          hipStream_t s) {
   if (L.num_comp == 0) return hipSuccess;
   hipLaunchKernelGGL(k_predict, dim3(L.num_comp), dim3(64), 0, s, c, b, L, pos, vel, pred_w,
-                     cic_cell, cic_w, comp_rows, arena_top);
+                     cic_cell, cic_w, comp_rows, comp_cols, arena_top);
   return hipGetLastError();
 }
 
@@ -1050,6 +1273,7 @@ hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const St
   if (d) {
     dep.L = d->L;
     dep.comp_rows = d->comp_rows;
+    dep.comp_cols = d->comp_cols;
     dep.cic_cell = d->cic_cell;
     dep.cic_w = d->cic_w;
     dep.arena = d->arena;
@@ -1057,6 +1281,8 @@ hipError_t launch_steric_fft_T(const BatchDev& b, const float* real_in, const St
     dep.sort_tmp = d->sort_tmp;
     dep.key_tmp = d->key_tmp;
     dep.key_half = d->key_half;
+    dep.tile_arena = d->tile_arena;
+    dep.tile_top = d->tile_top;
   }
   dep.cap = rl.cap;
   dep.b_off = rl.b_off;
@@ -1079,6 +1305,7 @@ hipError_t launch_steric_conv(const BatchDev& b, const float* real_in, const Ste
   if (d) {
     dep.L = d->L;
     dep.comp_rows = d->comp_rows;
+    dep.comp_cols = d->comp_cols;
     dep.cic_cell = d->cic_cell;
     dep.cic_w = d->cic_w;
     dep.arena = d->arena;
@@ -1086,12 +1313,20 @@ hipError_t launch_steric_conv(const BatchDev& b, const float* real_in, const Ste
     dep.sort_tmp = d->sort_tmp;
     dep.key_tmp = d->key_tmp;
     dep.key_half = d->key_half;
+    dep.tile_arena = d->tile_arena;
+    dep.tile_top = d->tile_top;
   }
   dep.cap = rl.cap;
   dep.b_off = rl.b_off;
   dep.tw_off = rl.tw_off;
   // the FFT pass's layout without its twiddles: the taps need FILTER_TAPS_MAX floats
   const size_t lds = (size_t)(rl.tw_off + (FILTER_TAPS_MAX + 1) / 2) * sizeof(float2);
+  if (fused && !real_in && d && d->tile_arena) {
+    // the deposit and both filter passes in one launch
+    hipLaunchKernelGGL(k_dep_filter2d, dim3(b.ctiles), dim3(F2_THREADS), 0, s, b, taps, dep,
+                       density_out, out);
+    return hipGetLastError();
+  }
   if (fused) {
     // the deposit (unless the input is given) into density_out or tmp, then
     // both filter passes in one launch

@@ -68,8 +68,9 @@ def test_gaussian_filter(dev, shape, sigma):
 
 
 def test_fused_filter_equals_two_pass(dev, monkeypatch):
-    """The one-launch 2-D filter (kernels of at most 33 taps) and the row and
-    column launches give the same steric field and steps, bit for bit."""
+    """The one-launch 2-D filter (kernels of at most 33 taps; at 1024^2 with
+    the CIC deposit inside it) and the row deposit with the row and column
+    launches give the same steric field and steps, bit for bit."""
     _say("fused 2-D steric filter vs two passes, 6 steps on 64x80 and 1024x1024")
     from microcosmos_amd import Rollout
     from microcosmos_amd._runtime import Plan
@@ -162,6 +163,38 @@ def test_cic_deposit_dense_rows(dev, shape):
     dens = _np(deposit_density(torch.as_tensor(pos, device=dev), shape, 0.7))
     exp_dens = R.compute_field_steric_force_corrected(pos, shape, 2.0, 5, 0.7)[0]
     assert np.array_equal(dens, exp_dens), max_ulp(dens, exp_dens)
+
+
+@pytest.mark.parametrize("shape", [(128, 160), (256, 256)])
+def test_fused_deposit_dense_windows(dev, shape):
+    """The deposit inside the 2-D filter launch (grids wider and taller than
+    a tile's window, at most one node per 32 cells): a patch holding more
+    CIC entries than a window keeps in LDS (600 nodes in 30 x 30 cells) takes
+    the arena path; a stack of nodes on one cell checks the fold order; nodes
+    across the periodic edges and on cell corners.  Density bit-exact against
+    the restatement, the filtered field too (the same taps, the same order)."""
+    _say(f"fused cic deposit dense windows {shape}")
+    from microcosmos_amd.forces import deposit_density, update_steric_potential
+    from microcosmos_amd.structs import Fields, Nodes
+    H, W = shape
+    rng = np.random.default_rng(23)
+    a = rng.uniform(40.0, 70.0, (600, 2))
+    b = np.tile([[11.5, 90.25]], (16, 1)) + rng.normal(0, 1e-3, (16, 2))
+    c = np.array([[0.0, 0.0], [W - 1e-3, H - 1e-3], [W * 0.5, 0.0], [3.0, 7.0]])
+    pos = np.concatenate([a, b, c]).astype(np.float32)
+    rng.shuffle(pos)
+    assert pos.shape[0] * 32 <= H * W
+    dens = _np(deposit_density(torch.as_tensor(pos, device=dev), shape, 0.7))
+    exp_dens, exp_st, _ = R.compute_field_steric_force_corrected(pos, shape, 2.0, 5, 0.7)
+    assert np.array_equal(dens, exp_dens), max_ulp(dens, exp_dens)
+    n = pos.shape[0]
+    z = torch.zeros((n, 2), device=dev)
+    nodes = Nodes(position=torch.as_tensor(pos, device=dev), velocity=z,
+                  color_bend=torch.zeros(n, dtype=torch.int32, device=dev), debug_vector=z)
+    fields = Fields(grid_shape=shape, steric=torch.zeros(shape, device=dev),
+                    fluid_velocity=torch.zeros((2, H, W), device=dev), f_grid=None)
+    st = _np(update_steric_potential(nodes, fields, sigma=2.0, scatter_value=0.7).steric)
+    np.testing.assert_array_equal(st, exp_st)
 
 
 @pytest.mark.parametrize("solver", ["pbd", "cosserat"])
