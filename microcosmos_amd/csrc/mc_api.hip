@@ -172,6 +172,13 @@ struct mc_plan {
   std::string fft_why;
   // immersed boundary + LBM
   IBWork ib{};
+  // the IB over synthetic points (synthetic_node_width > 0), set up on first
+  // use from host tables built with the layout: per node its incident
+  // (src, tgt) pairs in scatter order, and the points' component layout
+  IBWork ib_syn{};
+  bool ib_syn_ready = false;
+  std::vector<int> syn_inc_off_h, syn_inc_h, syn_off_h, syn_pts_h, syn_pinst_h, syn_pcomp_h;
+  std::vector<CompMeta> syn_meta_h;
   float* f_tmp = nullptr;
   float* node_tmp = nullptr;  // (N, 2) scratch
   long long* tick = nullptr;
@@ -507,6 +514,46 @@ mc_status build_layout(mc_plan* p, const int32_t* pairs, const int32_t* bp) {
   }
   p->all_chains = C > 0;
   for (int c = 0; c < C; ++c) p->all_chains = p->all_chains && meta[c].chain;
+  // synthetic IB points (fluid.py:246-263): node n's incident edges as
+  // (src, tgt) pairs, every edge at its source in edge order, then at its
+  // target in edge order; point n < N is node n's + point, N + n its - point;
+  // a component's points are its nodes, then its nodes + N (ascending)
+  {
+    std::vector<int> cnt(N + 1, 0);
+    for (int e = 0; e < E; ++e) {
+      cnt[pairs[2 * e] + 1]++;
+      cnt[pairs[2 * e + 1] + 1]++;
+    }
+    for (int n = 0; n < N; ++n) cnt[n + 1] += cnt[n];
+    p->syn_inc_off_h = cnt;
+    p->syn_inc_h.assign(4 * (size_t)E, 0);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int k = 0; k < 2; ++k)
+      for (int e = 0; e < E; ++e) {
+        const int at = fill[pairs[2 * e + k]]++;
+        p->syn_inc_h[2 * at] = pairs[2 * e];
+        p->syn_inc_h[2 * at + 1] = pairs[2 * e + 1];
+      }
+    p->syn_off_h.assign(C + 1, 0);
+    p->syn_pts_h.clear();
+    p->syn_pts_h.reserve(2 * (size_t)N);
+    p->syn_meta_h = meta;
+    for (int c = 0; c < C; ++c) {
+      for (int q = noff[c]; q < noff[c + 1]; ++q) p->syn_pts_h.push_back(norder[q]);
+      for (int q = noff[c]; q < noff[c + 1]; ++q) p->syn_pts_h.push_back(norder[q] + N);
+      p->syn_off_h[c + 1] = 2 * noff[c + 1];
+      CompMeta& m = p->syn_meta_h[c];
+      m.n0 = 2 * noff[c];
+      m.nn = 2 * (noff[c + 1] - noff[c]);
+      m.nf = -1;
+    }
+    p->syn_pinst_h.assign(2 * (size_t)N, 0);
+    p->syn_pcomp_h.assign(2 * (size_t)N, 0);
+    for (int n = 0; n < N; ++n) {
+      p->syn_pinst_h[n] = p->syn_pinst_h[N + n] = p->node_inst_h[n];
+      p->syn_pcomp_h[n] = p->syn_pcomp_h[N + n] = comp_of_node[n];
+    }
+  }
   CompMeta* md;
   MCCHK(upload(p, &md, meta, "component records"));
   int *bl, *blo, *nf, *ef, *hf;
@@ -573,6 +620,7 @@ mc_status build_instances(mc_plan* p) {
     // fluid.py:225-234 (scale_x uses grid_shape[0], as the reference does)
     g.scale_x = (float)((double)h.Wf / (double)h.H);
     g.scale_y = (float)((double)h.Hf / (double)h.W);
+    g.mean_scale = (float)(0.5 * ((double)h.Wf / (double)h.H + (double)h.Hf / (double)h.W));
     g.grid_off = go;
     g.fl_off = fo;
     g.mask_off = mo;
@@ -735,8 +783,8 @@ mc_status check_params(const mc_plan* p, const mc_params* q, bool need_fluid, bo
     return fail(MC_ERR_UNSUPPORTED, "bending_constraint %d", q->bending_constraint);
   if (q->cycles_per_step < 0) return fail(MC_ERR_BAD_SHAPE, "cycles_per_step < 0");
   if (need_fluid && q->enable_fluid) {
-    if (q->synthetic_node_width > 0.0)
-      return fail(MC_ERR_UNSUPPORTED, "synthetic_node_width > 0 is not implemented on the device path");
+    if (!(q->synthetic_node_width >= 0.0))
+      return fail(MC_ERR_BAD_SHAPE, "synthetic_node_width must be >= 0");
     if (q->ibm_kernel_size < 1 || q->ibm_kernel_size > IB_KMAX || q->ibm_kernel_size > p->kmax)
       return fail(MC_ERR_UNSUPPORTED, "ibm_kernel_size %d (plan supports 1..%d)", q->ibm_kernel_size, p->kmax);
     if (q->ibm_kernel_size > p->min_hf || q->ibm_kernel_size > p->min_wf)
@@ -783,6 +831,8 @@ Consts derive(const mc_plan* p, const mc_params* q, double dt) {
   c.force_clamp = q->enable_force_clamp != 0;
   c.relax = (float)q->ibm_relaxation;
   c.node_mass = (float)q->node_mass;
+  c.syn_d = (float)q->synthetic_node_width;
+  c.synthetic = q->enable_fluid && q->synthetic_node_width > 0.0;
   // fluid.py:155-161,342-344, float32 arithmetic after the double 1/(3 nu + 0.5)
   const float eps = 1.1920928955078125e-07f;
   float op = (float)(1.0 / (3.0 * q->viscosity + 0.5));
@@ -992,7 +1042,16 @@ mc_status ensure_side(mc_plan* p) {
 // runs; MC_PREDICT_LAUNCH keeps the prediction launch of every step).
 bool chain_predict(const mc_plan* p, const Consts& c) {
   static const bool off = getenv("MC_PREDICT_LAUNCH") != nullptr;
-  return c.enable_fluid && p->N > 0 && !off;
+  // synthetic points end the IB with the node pass, not the wide solve launch
+  return c.enable_fluid && p->N > 0 && !off && !c.synthetic;
+}
+
+// The IB work of the call: over the nodes, or over the synthetic points
+// (set up by ensure_synth).
+IBWork& ib_of(mc_plan* p, const Consts& c) {
+  if (!c.synthetic) return p->ib;
+  p->ib_syn.wide_min = p->ib.wide_min;  // tuned on the plan's own IB
+  return p->ib_syn;
 }
 
 // Enqueues one coupled step on `s`.  The LBM pass of the step may be deferred
@@ -1040,8 +1099,10 @@ mc_status enqueue_step(mc_plan* p, const mc_params* q, const Consts& c, const mc
   if (c.enable_fluid) {
     PredictArgs pa{c, p->bd, p->L, out->position, out->velocity, p->pred_w, p->cic_cell,
                    p->cic_w, p->comp_rows, p->comp_cols, p->dep_top};
-    HIPCHK(launch_ib(c, p->bd, p->ib, in->f_grid, out->position, out->velocity, out->position, tick,
-                     s, tail_pred ? &pa : nullptr));
+    if (c.synthetic && !p->ib_syn_ready)
+      return fail(MC_ERR_INVALID, "synthetic IB points are not set up on this plan");
+    HIPCHK(launch_ib(c, p->bd, ib_of(p, c), in->f_grid, out->position, out->velocity, out->position,
+                     tick, s, tail_pred ? &pa : nullptr));
     if (defer_lbm) {
       *defer_lbm = make_lbm_args(c, p->bd, in->f_grid, out->f_grid, out->fluid_velocity, p->ib.mask,
                                  p->ib.force);
@@ -1091,6 +1152,105 @@ const char* mc_last_error(void) { return g_err.c_str(); }
 }  // extern "C"
 
 namespace {
+
+// The wide solve's cell counters and control block (shared by the plan's IB
+// over nodes and over synthetic points), with the barrier-timeout record: a
+// pinned host flag the wide solve sets, and the poll bound of its barrier
+// waits (MC_IB_WIDE_POLL_LIMIT overrides the default 2^22, e.g. to exercise
+// the error path).
+mc_status alloc_wide_ctl(mc_plan* p) {
+  IBWork& w = p->ib;
+  if (w.wide_ctl) return MC_OK;
+  MCCHK(p->alloc(&w.wide_count, (size_t)p->lat_cells, "wide cell counters"));
+  MCCHK(p->alloc(&w.wide_ctl, 1, "wide control"));
+  if (hipHostMalloc(reinterpret_cast<void**>(&p->host_err), sizeof(int)) != hipSuccess)
+    return fail(MC_ERR_OOM, "pinned barrier-timeout flag");
+  *p->host_err = 0;
+  int limit = 1 << 22;
+  if (const char* env = getenv("MC_IB_WIDE_POLL_LIMIT")) {
+    const long v = atol(env);
+    if (v >= 0 && v <= (1L << 30)) limit = (int)v;
+  }
+  int* dev_flag = nullptr;
+  if (hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_flag), p->host_err, 0) != hipSuccess)
+    return fail(MC_ERR_HIP, "device address of the barrier-timeout flag");
+  WideCtl init{};
+  init.poll_limit = limit;
+  init.host_err = dev_flag;
+  return p->put(w.wide_ctl, &init, sizeof(init), "wide control");
+}
+
+// MC_IB_WIDE_WG (1..IB_WIDE_WG) caps the wide solve's workgroup count, for tuning.
+int wide_wg_cap() {
+  const char* env = getenv("MC_IB_WIDE_WG");
+  const int v = env ? atoi(env) : 0;
+  return (v >= 1 && v <= IB_WIDE_WG) ? v : IB_WIDE_WG;
+}
+
+// On the first call with synthetic points, sets up their IB work: the per-point arrays for 2N
+// points, the point layout and the node -> edge table; the lattice-cell
+// arrays, cluster tables and counters are the plan's own IB's (one IB runs
+// at a time on a plan).  The wide solve launches whenever an instance holds
+// more than 64 points (the smallest wide_min), as the tuning may lower it.
+mc_status ensure_synth(mc_plan* p, const mc_params* q) {
+  if (!(q->enable_fluid && q->synthetic_node_width > 0.0) || p->ib_syn_ready) return MC_OK;
+  const IBWork& b = p->ib;
+  IBWork w = b;
+  const int N = p->N;
+  const size_t P = 2 * (size_t)N;
+  const size_t taps = (size_t)p->kmax * p->kmax;
+  w.N = (int)P;
+  w.nodes = N;
+  int *pcomp, *poff, *pts, *ioff, *pinst;
+  MCCHK(upload(p, &pcomp, p->syn_pcomp_h, "point components"));
+  MCCHK(upload(p, &poff, p->syn_off_h, "component point offsets"));
+  MCCHK(upload(p, &pts, p->syn_pts_h, "component points"));
+  w.comp_of_node = pcomp;
+  w.comp_node_off = poff;
+  w.comp_nodes = pts;
+  CompMeta* md;
+  MCCHK(upload(p, &md, p->syn_meta_h, "component point records"));
+  w.meta = md;
+  MCCHK(upload(p, &ioff, p->syn_inc_off_h, "node edge offsets"));
+  w.syn_inc_off = ioff;
+  int* inc;
+  MCCHK(upload(p, &inc, p->syn_inc_h, "node edges"));
+  w.syn_inc = reinterpret_cast<const int2*>(inc);
+  MCCHK(upload(p, &pinst, p->syn_pinst_h, "point instances"));
+  w.point_inst = pinst;
+  MCCHK(p->alloc(&w.syn_force, 2 * P, "point drag"));
+  MCCHK(p->fill(w.syn_force, 0, 2 * P * sizeof(float), "point drag"));
+  MCCHK(p->alloc(&w.st_sx, P, "point st sx"));
+  MCCHK(p->alloc(&w.st_sy, P, "point st sy"));
+  MCCHK(p->alloc(&w.st_w, taps * P, "point st w"));
+  MCCHK(p->alloc(&w.st_won, P, "point st won"));
+  MCCHK(p->alloc(&w.vib, 2 * P, "point vib"));
+  MCCHK(p->alloc(&w.links, taps * P, "point cluster links"));
+  {
+    std::vector<int> loff(p->n_inst);
+    long long acc = 0;
+    for (int i = 0; i < p->n_inst; ++i) {
+      loff[i] = (int)acc;
+      acc += 2LL * p->inst[i].nodes * (long long)taps;
+    }
+    MCCHK(upload(p, &w.link_off, loff, "point link bucket offsets"));
+  }
+  MCCHK(p->alloc(&w.cl_scratch, w.cl_bpn * P, "point cluster scratch"));
+  w.wide_wg = 0;
+  for (const InstHost& h : p->inst)
+    if (2LL * h.nodes > 64) w.wide_wg = b.wide_wg > 0 ? b.wide_wg : wide_wg_cap();
+  w.wide_scratch = nullptr;
+  if (w.wide_wg > 0) {
+    MCCHK(alloc_wide_ctl(p));
+    w.wide_count = p->ib.wide_count;
+    w.wide_ctl = p->ib.wide_ctl;
+    MCCHK(p->alloc(&w.wide_scratch, ib_wide_scratch_bytes((int)P, p->kmax), "point wide scratch"));
+  }
+  w.dirty = false;
+  p->ib_syn = w;
+  p->ib_syn_ready = true;
+  return MC_OK;
+}
 
 mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const int32_t* node_off,
                       const int32_t* pairs_host, int32_t num_edges,
@@ -1239,26 +1399,7 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
     if ((st = p->alloc(&ws, ib_wide_scratch_bytes(N, max_kernel_size), "wide scratch")) != MC_OK)
       return bail(st);
     w.wide_scratch = ws;
-    if ((st = p->alloc(&w.wide_count, hwf, "wide cell counters")) != MC_OK) return bail(st);
-    if ((st = p->alloc(&w.wide_ctl, 1, "wide control")) != MC_OK) return bail(st);
-    // the barrier-timeout record: a pinned host flag the wide solve sets, and
-    // the poll bound of its barrier waits (MC_IB_WIDE_POLL_LIMIT overrides
-    // the default 2^22, e.g. to exercise the error path)
-    if (hipHostMalloc(reinterpret_cast<void**>(&p->host_err), sizeof(int)) != hipSuccess)
-      return bail(fail(MC_ERR_OOM, "pinned barrier-timeout flag"));
-    *p->host_err = 0;
-    int limit = 1 << 22;
-    if (const char* env = getenv("MC_IB_WIDE_POLL_LIMIT")) {
-      const long v = atol(env);
-      if (v >= 0 && v <= (1L << 30)) limit = (int)v;
-    }
-    int* dev_flag = nullptr;
-    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_flag), p->host_err, 0) != hipSuccess)
-      return bail(fail(MC_ERR_HIP, "device address of the barrier-timeout flag"));
-    WideCtl init{};
-    init.poll_limit = limit;
-    init.host_err = dev_flag;
-    if ((st = p->put(w.wide_ctl, &init, sizeof(init), "wide control")) != MC_OK) return bail(st);
+    if ((st = alloc_wide_ctl(p)) != MC_OK) return bail(st);
   }
   if ((st = p->alloc(&w.force, 2 * hwf, "force U")) != MC_OK) return bail(st);
   if ((st = p->alloc(&w.mask, (size_t)p->mask_words, "U mask")) != MC_OK) return bail(st);
@@ -1277,7 +1418,8 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
 mc_status check_async(mc_plan* p, hipStream_t s) {
   if (!p->host_err || __atomic_load_n(p->host_err, __ATOMIC_ACQUIRE) == 0) return MC_OK;
   __atomic_store_n(p->host_err, 0, __ATOMIC_RELEASE);
-  const hipError_t e = launch_ib_reset(p->ib, s);
+  hipError_t e = launch_ib_reset(p->ib, s);
+  if (e == hipSuccess && p->ib_syn_ready) e = launch_ib_reset(p->ib_syn, s);
   if (e != hipSuccess) return fail(MC_ERR_HIP, "IB reset after a barrier timeout: %s", hipGetErrorString(e));
   return fail(MC_ERR_HIP,
               "immersed-boundary wide solve: a grid barrier wait gave up (its workgroups were not "
@@ -1431,6 +1573,7 @@ mc_status mc_plan_autotune(mc_plan* p, const mc_params* q, const mc_state* st,
   g_err.clear();
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   MCCHK(check_params(p, q, true, true));
+  MCCHK(ensure_synth(p, q));
   MCCHK(check_state(p, st, "state"));
   MCCHK(check_edges(p, ed));
   if (steps < 1) return fail(MC_ERR_BAD_SHAPE, "steps < 1");
@@ -1627,6 +1770,7 @@ mc_status mc_step(mc_plan* p, const mc_params* q, const mc_state* in, const mc_s
   g_err.clear();
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   MCCHK(check_params(p, q, true, true));
+  MCCHK(ensure_synth(p, q));
   MCCHK(check_state(p, in, "input"));
   MCCHK(check_state(p, out, "output"));
   MCCHK(check_edges(p, ed));
@@ -1720,6 +1864,7 @@ mc_status rollout_impl(mc_plan* p, const mc_params* q, const mc_state* st, const
                        void* stream, const mc_actions* rows, const mc_record* frames) {
   if (!p) return fail(MC_ERR_INVALID, "plan is NULL");
   MCCHK(check_params(p, q, true, true));
+  MCCHK(ensure_synth(p, q));
   MCCHK(check_state(p, st, "state"));
   MCCHK(check_edges(p, ed));
   if (num_steps < 0) return fail(MC_ERR_BAD_SHAPE, "num_steps < 0");
@@ -1890,6 +2035,7 @@ mc_status mc_profile_step(mc_plan* p, const mc_params* q, const mc_state* st,
   g_err.clear();
   if (!p || !lbm_ms_avg || !step_ms_avg) return fail(MC_ERR_INVALID, "NULL argument");
   MCCHK(check_params(p, q, true, true));
+  MCCHK(ensure_synth(p, q));
   MCCHK(check_state(p, st, "state"));
   MCCHK(check_edges(p, ed));
   if (num_steps < 1) return fail(MC_ERR_BAD_SHAPE, "num_steps < 1");
@@ -1948,6 +2094,7 @@ mc_status mc_profile_fused_step(mc_plan* p, const mc_params* q, const mc_state* 
   g_err.clear();
   if (!p || !kernel_ms_avg || !step_ms_avg) return fail(MC_ERR_INVALID, "NULL argument");
   MCCHK(check_params(p, q, true, true));
+  MCCHK(ensure_synth(p, q));
   MCCHK(check_state(p, st, "state"));
   MCCHK(check_edges(p, ed));
   if (num_steps < 2 || (num_steps & 1))
@@ -2040,6 +2187,7 @@ mc_status mc_ib_lbm_step(mc_plan* p, const mc_params* q, const float* position, 
   mc_params qq = *q;
   qq.enable_fluid = 1;
   MCCHK(check_params(p, &qq, true, false));
+  MCCHK(ensure_synth(p, &qq));
   if (!(dt > 0.0)) return fail(MC_ERR_BAD_SHAPE, "dt must be positive");
   hipStream_t s = (hipStream_t)stream;
   MCCHK(check_async(p, s));
@@ -2047,7 +2195,7 @@ mc_status mc_ib_lbm_step(mc_plan* p, const mc_params* q, const float* position, 
   MCCHK(ensure_geo(p, &qq));
   if (p->N > 0 && velocity_out != velocity)
     HIPCHK(hipMemcpyAsync(velocity_out, velocity, 2 * (size_t)p->N * sizeof(float), hipMemcpyDeviceToDevice, s));
-  HIPCHK(launch_ib(c, p->bd, p->ib, f_in, position, velocity_out, pos_out ? pos_out : p->node_tmp,
+  HIPCHK(launch_ib(c, p->bd, ib_of(p, c), f_in, position, velocity_out, pos_out ? pos_out : p->node_tmp,
                    nullptr, s));
   HIPCHK(launch_lbm(c, p->bd, f_in, f_out, fluid_velocity_out, p->ib.mask, p->ib.force, s));
   return MC_OK;
@@ -2275,6 +2423,9 @@ mc_status check_rod_adjoint(const mc_params* q) {
     return fail(MC_ERR_UNSUPPORTED,
                 "the rod adjoint needs bending_constraint and solver_type to agree "
                 "(a Cosserat constraint with solver_type pbd, or the reverse)");
+  if (q->enable_fluid && q->synthetic_node_width > 0.0)
+    return fail(MC_ERR_UNSUPPORTED, "the adjoint of the fluid-coupled step takes "
+                "synthetic_node_width 0 (IB on the nodes)");
   return MC_OK;
 }
 

@@ -75,13 +75,42 @@ __device__ __forceinline__ void uf_union(int* p, int a, int b) {
   }
 }
 
-// One lane per (node, tap).  Every lane of a node group evaluates the node's
+// The lattice position of synthetic IB point `nd` +/- (fluid.py:246-263):
+// the node's incident edge vectors (tgt - src, minimum image) summed from
+// zero in the reference's scatter order (every edge at its source in edge
+// order, then at its target in edge order), normalised with the EPSIL under
+// the square root, turned by +90 degrees, and scaled by the width and the
+// mean lattice scale; (gx, gy) holds the node's lattice position on entry.
+__device__ __forceinline__ void synthetic_point(const Consts& c, const InstGeo& gi,
+                                                const float* __restrict__ pos,
+                                                const int* __restrict__ inc_off,
+                                                const int2* __restrict__ inc, int nd, bool minus,
+                                                float& gx, float& gy) {
+  float tx = 0.0f, ty = 0.0f;
+  for (int i = inc_off[nd], e = inc_off[nd + 1]; i < e; ++i) {
+    const int2 st = inc[i];
+    const float2 a = *reinterpret_cast<const float2*>(pos + 2 * st.x);
+    const float2 b = *reinterpret_cast<const float2*>(pos + 2 * st.y);
+    tx = tx + min_image(b.x, a.x, gi.dom_wf);
+    ty = ty + min_image(b.y, a.y, gi.dom_hf);
+  }
+  const float nrm = sqrtf((tx * tx + ty * ty) + MC_EPSIL);
+  const float ux = tx / nrm, uy = ty / nrm;
+  const float ox = (c.syn_d * -uy) * gi.mean_scale;
+  const float oy = (c.syn_d * ux) * gi.mean_scale;
+  gx = minus ? gx - ox : gx + ox;
+  gy = minus ? gy - oy : gy + oy;
+}
+
+// One lane per (point, tap).  Every lane of a node group evaluates the node's
 // geometry; lane t owns tap t (dx = t / K, dy = t % K): its weight, the claim
 // of its cell (compare-and-swap: the first arriving component owns it; the
 // node records which of its taps won) and a link when another component holds
 // the cell.  Links go to the node's instance bucket (bucketed) or to one list.
 // Block 0 also clears the wide solve's counters and the solve's scratch
 // allocator (the previous step's launches have completed).
+// The N points are the nodes, or with inc_off the 2 x nodes synthetic points
+// (bd.node_inst and comp_of_node then per point; pos and vel per node).
 template <int K>
 __global__ void __launch_bounds__(256) k_ib_claim(
     Consts c, BatchDev bd, int N, const float* __restrict__ pos, const float* __restrict__ vel,
@@ -89,7 +118,8 @@ __global__ void __launch_bounds__(256) k_ib_claim(
     float* __restrict__ st_w, unsigned long long* __restrict__ st_won, float* __restrict__ vib,
     int* __restrict__ owner, int2* __restrict__ links, int* __restrict__ nlinks,
     const int* __restrict__ link_off, int bucketed, WideCtl* __restrict__ wctl,
-    IbCtl* __restrict__ ictl) {
+    IbCtl* __restrict__ ictl, int nodes, const int* __restrict__ inc_off,
+    const int2* __restrict__ inc) {
   constexpr int K2 = K * K;
   constexpr int L = stencil_lanes<K>();
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -115,8 +145,11 @@ __global__ void __launch_bounds__(256) k_ib_claim(
     inst = bd.node_inst[n];
     const InstGeo* gi = &bd.geo[inst];
     const int W = gi->Wf, H = gi->Hf;
-    const float gx = pos[2 * n] * gi->scale_x;
-    const float gy = pos[2 * n + 1] * gi->scale_y;
+    const bool minus = inc_off && n >= nodes;
+    const int nd = minus ? n - nodes : n;  // the point's node
+    float gx = pos[2 * nd] * gi->scale_x;
+    float gy = pos[2 * nd + 1] * gi->scale_y;
+    if (inc_off) synthetic_point(c, *gi, pos, inc_off, inc, nd, minus, gx, gy);
     const float sxf = floorf(gx) - (float)c.half;
     const float syf = floorf(gy) - (float)c.half;
     const int sx = imod(f2i(sxf), W);
@@ -134,8 +167,8 @@ __global__ void __launch_bounds__(256) k_ib_claim(
     if (t == 0) {
       st_sx[n] = sx;
       st_sy[n] = sy;
-      vib[2 * n] = (vel[2 * n] * gi->scale_x) * c.dt;
-      vib[2 * n + 1] = (vel[2 * n + 1] * gi->scale_y) * c.dt;
+      vib[2 * n] = (vel[2 * nd] * gi->scale_x) * c.dt;
+      vib[2 * n + 1] = (vel[2 * nd + 1] * gi->scale_y) * c.dt;
     }
   }
   // the normalising sum over taps 0..K2-1 in order (every lane of the group)
@@ -269,6 +302,7 @@ struct ClusterArgs {
   const unsigned long long* st_won;
   const float *st_w, *vib, *f_in, *pos;
   float *vel_io, *pos_out, *force;
+  float* syn_force;  // synthetic points: per-point drag out (no node update), else NULL
   int *cell_seg, *owner;
   uint32_t* mask;
   BatchDev bd;
@@ -405,9 +439,12 @@ __device__ __forceinline__ void load_node(const ClusterArgs& a, int n, NodeIn<K2
   d.won = a.st_won[n];
   load_taps<K2>(a.st_w + (size_t)n * K2, d.w);
   d.vb = *reinterpret_cast<const float2*>(a.vib + 2 * n);
-  // the drag's inputs, read now (only this thread writes them, at the end)
-  d.p0 = *reinterpret_cast<const float2*>(a.pos + 2 * n);
-  d.v0 = *reinterpret_cast<const float2*>(a.vel_io + 2 * n);
+  // the drag's inputs, read now (only this thread writes them, at the end);
+  // synthetic points hand their drag to the node pass instead
+  if (!a.syn_force) {
+    d.p0 = *reinterpret_cast<const float2*>(a.pos + 2 * n);
+    d.v0 = *reinterpret_cast<const float2*>(a.vel_io + 2 * n);
+  }
 }
 
 // LDS bytes of the small solve: per entry the grouped entry list (4 B); per
@@ -772,12 +809,16 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
       fx = fx + F[t].x * w[t];
       fy = fy + F[t].y * w[t];
     }
-    const float dvx = (fx / c.node_mass) / (gi.scale_x * c.dt);
-    const float dvy = (fy / c.node_mass) / (gi.scale_y * c.dt);
     const int n = d.n;
-    *reinterpret_cast<float2*>(a.vel_io + 2 * n) = make_float2(d.v0.x - dvx, d.v0.y - dvy);
-    *reinterpret_cast<float2*>(a.pos_out + 2 * n) =
-        make_float2(fmodf_floor(d.p0.x, gi.dom_wf), fmodf_floor(d.p0.y, gi.dom_hf));
+    if (a.syn_force) {
+      *reinterpret_cast<float2*>(a.syn_force + 2 * n) = make_float2(fx, fy);
+    } else {
+      const float dvx = (fx / c.node_mass) / (gi.scale_x * c.dt);
+      const float dvy = (fy / c.node_mass) / (gi.scale_y * c.dt);
+      *reinterpret_cast<float2*>(a.vel_io + 2 * n) = make_float2(d.v0.x - dvx, d.v0.y - dvy);
+      *reinterpret_cast<float2*>(a.pos_out + 2 * n) =
+          make_float2(fmodf_floor(d.p0.x, gi.dom_wf), fmodf_floor(d.p0.y, gi.dom_hf));
+    }
   }
 }
 
@@ -958,6 +999,10 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
       const float2 F = L.suc[L.tapseg[r * K2 + t]];
       fx = fx + F.x * w;
       fy = fy + F.y * w;
+    }
+    if (a.syn_force) {
+      *reinterpret_cast<float2*>(a.syn_force + 2 * n) = make_float2(fx, fy);
+      continue;
     }
     const float dvx = (fx / c.node_mass) / (gi.scale_x * c.dt);
     const float dvy = (fy / c.node_mass) / (gi.scale_y * c.dt);
@@ -1349,6 +1394,10 @@ __device__ __forceinline__ bool cluster_wide(const ClusterArgs& a, int h, int H,
       fx = fx + F[t].x * w[t];
       fy = fy + F[t].y * w[t];
     }
+    if (a.syn_force) {
+      *reinterpret_cast<float2*>(a.syn_force + 2 * n) = make_float2(fx, fy);
+      continue;
+    }
     const float dvx = (fx / c.node_mass) / (gi.scale_x * c.dt);
     const float dvy = (fy / c.node_mass) / (gi.scale_y * c.dt);
     a.vel_io[2 * n] = a.vel_io[2 * n] - dvx;
@@ -1649,8 +1698,10 @@ __global__ void __launch_bounds__(WIDE_THREADS) k_ib_wide(ClusterArgs a, Predict
     if (__hip_atomic_load(&a.wctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
       // an earlier solve gave up at a barrier: its cell counters are not back
       // at zero until launch_ib_reset, so only the final wrap of the wide
-      // clusters' nodes is done (simulate.py:97); the host reports the error
-      for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += H * blockDim.x) {
+      // clusters' nodes is done (simulate.py:97; synthetic points: by the
+      // node pass); the host reports the error
+      const int nw = a.syn_force ? 0 : a.N;
+      for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < nw; n += H * blockDim.x) {
         if (a.cl_cnt[a.label[a.comp_of_node[n]]] <= a.wide_min) continue;
         const InstGeo& gi = a.bd.geo[a.bd.node_inst[n]];
         a.pos_out[2 * n] = fmodf_floor(a.pos[2 * n], gi.dom_wf);
@@ -1674,6 +1725,28 @@ __global__ void __launch_bounds__(WIDE_THREADS) k_ib_wide(ClusterArgs a, Predict
   if (wide_ready)
     for (int f = blockIdx.x * nwv + wave; f < a.C; f += H * nwv)
       if (is_wide(f)) predict_component(pr, f, lane);
+}
+
+// Synthetic points: node n's drag is the mean of the forces at its points n
+// and nodes + n (fluid.py:364-382), then the final wrap (simulate.py:97).
+// bd is the node-level view.
+__global__ void __launch_bounds__(256) k_ib_syn_drag(Consts c, BatchDev bd, int nodes,
+                                                     const float2* __restrict__ F,
+                                                     const float* __restrict__ pos,
+                                                     float* __restrict__ vel_io,
+                                                     float* __restrict__ pos_out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nodes) return;
+  const InstGeo& gi = bd.geo[bd.node_inst[n]];
+  const float2 fa = F[n], fb = F[nodes + n];
+  const float fx = (fa.x + fb.x) * 0.5f, fy = (fa.y + fb.y) * 0.5f;
+  const float dvx = (fx / c.node_mass) / (gi.scale_x * c.dt);
+  const float dvy = (fy / c.node_mass) / (gi.scale_y * c.dt);
+  const float2 p0 = *reinterpret_cast<const float2*>(pos + 2 * n);
+  const float2 v0 = *reinterpret_cast<const float2*>(vel_io + 2 * n);
+  *reinterpret_cast<float2*>(vel_io + 2 * n) = make_float2(v0.x - dvx, v0.y - dvy);
+  *reinterpret_cast<float2*>(pos_out + 2 * n) =
+      make_float2(fmodf_floor(p0.x, gi.dom_wf), fmodf_floor(p0.y, gi.dom_hf));
 }
 
 __global__ void k_ib_reset(long long HW, int C, int* __restrict__ owner, int* __restrict__ parent,
@@ -1762,12 +1835,17 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   }
   w.dirty = true;
   WideCtl* wctl = w.wide_wg > 0 ? w.wide_ctl : nullptr;
+  // synthetic points: the claim and the solves see the per-point instances
+  const bool syn = w.syn_force != nullptr;
+  if (syn && pred) return hipErrorInvalidValue;
+  BatchDev bp = b;
+  if (syn) bp.node_inst = w.point_inst;
 #define L_CLAIM(KK)                                                                               \
   hipLaunchKernelGGL(k_ib_claim<KK>,                                                              \
                      dim3((unsigned)(((long long)N * stencil_lanes<KK>() + 255) / 256)), dim3(256), \
-                     0, s, c, b, N, pos, vel_io, w.comp_of_node, w.st_sx, w.st_sy, w.st_w,         \
+                     0, s, c, bp, N, pos, vel_io, w.comp_of_node, w.st_sx, w.st_sy, w.st_w,        \
                      w.st_won, w.vib, w.owner, w.links, w.nlinks, w.link_off, w.bucketed ? 1 : 0,  \
-                     wctl, w.ictl)
+                     wctl, w.ictl, w.nodes, syn ? w.syn_inc_off : nullptr, w.syn_inc)
   MC_KSWITCH(L_CLAIM);
 #undef L_CLAIM
   hipError_t e = hipGetLastError();
@@ -1801,10 +1879,11 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   a.vel_io = vel_io;
   a.pos_out = pos_out;
   a.force = w.force;
+  a.syn_force = w.syn_force;
   a.cell_seg = w.cell_seg;
   a.owner = w.owner;
   a.mask = w.mask;
-  a.bd = b;
+  a.bd = bp;
   a.gscratch = w.cl_scratch;
   a.bpn = w.cl_bpn;
   a.lds_bytes = w.lds_bytes;
@@ -1837,6 +1916,12 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   hipLaunchKernelGGL(k_ib_wide<KK>, dim3(G), dim3(WIDE_THREADS), 0, s, a, pa, on)
     MC_KSWITCH(L_WIDE);
 #undef L_WIDE
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && syn) {
+    hipLaunchKernelGGL(k_ib_syn_drag, dim3((unsigned)((w.nodes + 255) / 256)), dim3(256), 0, s, c,
+                       b, w.nodes, reinterpret_cast<const float2*>(w.syn_force), pos, vel_io,
+                       pos_out);
     e = hipGetLastError();
   }
   if (e == hipSuccess) w.dirty = false;

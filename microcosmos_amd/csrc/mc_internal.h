@@ -36,6 +36,10 @@ struct Consts {
   int K, half, iters, force_clamp;
   float half_width, relax, node_mass;
   float omega_p, omega_m, cp, cm;
+  // synthetic IB points (fluid.py:246-263): on when synthetic_node_width > 0;
+  // syn_d its float32 value
+  int synthetic;
+  float syn_d;
 };
 
 // One independent instance (lattice + filaments) of a plan.  A plan holds
@@ -50,6 +54,7 @@ struct InstGeo {
   float dom_hf, dom_wf;       // H, W as floats (periodic_boundary sizes)
   float scale_x, scale_y;     // fluid.py:225-234 (scale_x uses grid_shape[0])
   float umax;                 // max_fluid_velocity * mean scale (params-dependent)
+  float mean_scale;           // float32 of 0.5 * (scale_x + scale_y) in double
   long long grid_off;         // first cell of the (H, W) grids
   long long fl_off;           // first cell of the lattice
   long long mask_off;         // first U-mask word (ceil(Hf * Wf / 32) words, flat cell index)
@@ -329,10 +334,23 @@ hipError_t launch_rod_segment(const RodAdjArgs& a, bool reverse, int steps, floa
                               float* tape_or_dbg, size_t lds_bytes, hipStream_t s);
 // immersed boundary (solved per interaction cluster)
 struct IBWork {
-  int N, C;                    // nodes, filament components
-  const int *comp_of_node, *comp_node_off, *comp_nodes;  // component layout (rod layout)
-  const struct CompMeta* meta;                           // per-component record (rod layout)
-  int *st_sx, *st_sy;          // per node wrapped stencil start cell
+  // The IB runs on N points: the nodes, or with synthetic points (a second
+  // IBWork of the plan) 2 x nodes points, point p < nodes the + point of node
+  // p, point nodes + k the - point of node k.  Component layout of the points
+  // (the rod layout, or each component's + points then its - points).
+  int N, C;                    // IB points, filament components
+  const int *comp_of_node, *comp_node_off, *comp_nodes;  // per point / component
+  const struct CompMeta* meta;                           // per-component record (nn, nf, n0)
+  // synthetic points: the node count, the node -> incident edge table (per
+  // node its (src, tgt) pairs: every edge at its source in edge order, then at
+  // its target in edge order), each point's instance, and the per-point drag
+  // the solves leave for the node pass (NULL otherwise)
+  int nodes;
+  const int* syn_inc_off;
+  const int2* syn_inc;
+  const int* point_inst;
+  float* syn_force;
+  int *st_sx, *st_sy;          // per point wrapped stencil start cell
   float *st_w;                 // per node K*K weights
   unsigned long long* st_won;  // per node: bit t set when tap t claimed its cell
   float *vib;                  // per node (2) IB target velocity
@@ -398,7 +416,9 @@ hipError_t ib_prepare(int lds_max);
 // Whole IB of a step: stencil + cluster links, cluster build, cluster solve
 // (support, iterations, force into w.force / w.mask, node drag into vel_io,
 // final wrap of pos into pos_out).  Only the solve reads f_in.  `tick` (may be
-// NULL) is advanced once.  `pred` (may be NULL): the next step's inertial
+// NULL) is advanced once.  With synthetic points (w.syn_force), pos and
+// vel_io are the nodes' and a last launch averages each node's two drag
+// samples.  `pred` (may be NULL; not with synthetic points): the next step's inertial
 // prediction from the final state (pos_out, vel_io), made by the launch that
 // ends the IB (the wide solve's), so the next step needs no launch for it.
 hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float* f_in,

@@ -836,14 +836,36 @@ class IBTrace:
     node_force: np.ndarray = None
 
 
+def synthetic_ib_points(position, pairs, grid_shape, grid_pos, width, mean_scale):
+    """fluid.py:246-263 -- the two synthetic IB points of every node, at
+    +/- width * mean_scale along the chain normal: rows 0..N-1 the + points,
+    N..2N-1 the - points.  The tangent is the sum of the node's incident edge
+    vectors (tgt - src, minimum image), added in the reference's scatter
+    order: every edge at its source in edge order, then at its target in edge
+    order; the normal is the unit tangent turned by +90 degrees."""
+    dtype = grid_pos.dtype
+    pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    src, tgt = pairs[:, 0], pairs[:, 1]
+    ev = displacement(grid_shape, position[tgt], position[src])
+    traw = np.zeros_like(position)
+    np.add.at(traw, src, ev)  # sequential in index order
+    np.add.at(traw, tgt, ev)
+    nrm = np.sqrt((traw[:, 0:1] * traw[:, 0:1] + traw[:, 1:2] * traw[:, 1:2]) + EPSIL)
+    t = traw / nrm
+    perp = np.concatenate([-t[:, 1:2], t[:, 0:1]], axis=1)
+    off = (np.asarray(width, dtype) * perp) * np.asarray(mean_scale, dtype)
+    return np.concatenate([grid_pos + off, grid_pos - off], axis=0)
+
+
 def immersed_boundary_interaction(position, velocity, grid_shape, f_grid, dt, p: Params,
-                                  trace: Optional[IBTrace] = None):
+                                  trace: Optional[IBTrace] = None, pairs=None):
     """fluid.py:202-386 -- IB multi-direct forcing + TRT/Guo collide + stream + drag.
 
+    With synthetic_node_width > 0 the IB runs on the 2N synthetic points of
+    synthetic_ib_points (``pairs`` is then required) and each node's drag is
+    the mean of its two points' forces (fluid.py:364-368).
     Returns (new node velocity, fluid_velocity = u*, f_new).
     """
-    if p.synthetic_node_width > 0.0:
-        raise NotImplementedError("synthetic_node_width > 0 (fluid.py:246-263)")
     dtype = f_grid.dtype
     h, w = f_grid.shape[1], f_grid.shape[2]
     scale_x = w / grid_shape[0]
@@ -853,6 +875,14 @@ def immersed_boundary_interaction(position, velocity, grid_shape, f_grid, dt, p:
     umax = p.max_fluid_velocity * mean_scale
     ibm_vel = velocity * scale * dt
     grid_pos = position * scale
+    n_nodes = position.shape[0]
+    synthetic = p.synthetic_node_width > 0.0
+    if synthetic:
+        if pairs is None:
+            raise ValueError("synthetic_node_width > 0 needs the edge pairs")
+        grid_pos = synthetic_ib_points(position, pairs, grid_shape, grid_pos,
+                                       p.synthetic_node_width, mean_scale)
+        ibm_vel = np.concatenate([ibm_vel, ibm_vel], axis=0)
 
     f = np.maximum(f_grid, EPSIL)
     rho, u = macroscopic(f)
@@ -881,6 +911,8 @@ def immersed_boundary_interaction(position, velocity, grid_shape, f_grid, dt, p:
     f_new = stream(f_post)
 
     fn = ib_interpolate(flat, wts, force)
+    if synthetic:
+        fn = (fn[:n_nodes] + fn[n_nodes:]) * dtype.type(0.5)
     dv = (fn / p.node_mass) / (scale * dt)
     if trace is not None:
         trace.flat, trace.weights, trace.kernel_weight = flat, wts, kw
@@ -924,7 +956,7 @@ def step(s: State, dt: float, p: Params, ib_position=None, ib_trace=None) -> Sta
             ib_trace.append(s.position.copy())
         s.velocity, s.fluid_velocity, s.f_grid = immersed_boundary_interaction(
             s.position if ib_position is None else ib_position, s.velocity, s.grid_shape,
-            s.f_grid, dt, p)
+            s.f_grid, dt, p, pairs=s.pairs)
     s.position = periodic_boundary(s.grid_shape, s.position)
     return s
 

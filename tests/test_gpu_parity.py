@@ -262,6 +262,78 @@ def test_ib_lbm_wide_cluster(dev, K):
     _ib_case(dev, W.small(grid=(64, 64), n_fil=48, n_nodes=40, seed=3), K)
 
 
+def _branch(wl, extra):
+    """The workload with extra edges (pairs of node ids) appended: nodes of
+    degree 3 and more, whose synthetic-point tangents sum several edges in
+    scatter order; filaments joined by an edge become one component."""
+    from dataclasses import replace as _r
+    extra = np.asarray(extra, np.int32).reshape(-1, 2)
+    k = len(extra)
+    return _r(wl, pairs=np.concatenate([wl.pairs, extra]),
+              theta=np.concatenate([wl.theta, np.zeros(k, wl.theta.dtype)]),
+              rest_lengths=np.concatenate([wl.rest_lengths,
+                                           np.ones(k, wl.rest_lengths.dtype)]))
+
+
+def _ib_syn_case(dev, wl, K, width, seed=21):
+    """One IB + LBM call with synthetic points against the restatement: the
+    points (tangent sums, normal, offsets), the IB over 2N points and the
+    averaged drag, bit for bit."""
+    from microcosmos_amd.solver.fluid import immersed_boundary_interaction
+    rng = np.random.default_rng(seed)
+    vel = rng.normal(0, 0.05, wl.position.shape).astype(np.float32)
+    wl.velocity = vel
+    p = oracle_params(wl, ibm_kernel_size=K, synthetic_node_width=width)
+    v_exp, u_exp, f_exp = R.immersed_boundary_interaction(wl.position, vel, wl.grid_shape,
+                                                          wl.f_grid, 1.0, p, pairs=wl.pairs)
+    nodes, edges, fields = device_state(wl, dev)
+    cfg = device_config(wl, ibm_kernel_size=K, synthetic_node_width=width)
+    n2, fl2, f2 = immersed_boundary_interaction(nodes, edges, fields, fields.f_grid, 1.0, cfg)
+    np.testing.assert_array_equal(_np(f2), f_exp)
+    np.testing.assert_array_equal(_np(fl2.fluid_velocity), u_exp)
+    np.testing.assert_array_equal(_np(n2.velocity), v_exp)
+
+
+@pytest.mark.parametrize("K,width", [(4, 0.5), (3, 1.25), (6, 0.3)])
+def test_ib_lbm_synthetic_points(dev, K, width):
+    _say(f"IB+LBM single call, synthetic points K={K} width={width}")
+    _ib_syn_case(dev, W.small(grid=(48, 48), n_fil=3, n_nodes=24, seed=13), K, width)
+
+
+def test_ib_lbm_synthetic_branched(dev):
+    # nodes of degree 3 and 4, two filaments joined, singleton components
+    _say("IB+LBM single call, synthetic points on a branched graph")
+    wl = W.small(grid=(40, 44), seed=5, node_counts=[1, 3, 1, 12, 2, 1, 14])
+    wl = _branch(wl, [[9, 4], [9, 13], [20, 9], [6, 22]])
+    _ib_syn_case(dev, wl, 4, 0.7)
+
+
+def test_ib_lbm_synthetic_wide_cluster(dev):
+    # 1920 nodes = 3840 points on 64^2: one cluster above the wide-solve size
+    _say("IB+LBM single call, synthetic points, one 3840-point cluster (wide solve)")
+    _ib_syn_case(dev, W.small(grid=(64, 64), n_fil=48, n_nodes=40, seed=3), 4, 0.5)
+
+
+def test_step_parity_synthetic_points(dev):
+    """Twelve coupled steps with synthetic IB points (a 16-step graph's
+    remainder graph, and step-by-step): bit-identical to the restatement."""
+    _say("12 coupled steps with synthetic points, 48^2")
+    wl = W.small(grid=(48, 48), n_fil=4, n_nodes=20, seed=9)
+    p = oracle_params(wl, synthetic_node_width=0.6)
+    o = run_oracle(wl, 12, p)
+    from microcosmos_amd import Rollout
+    for graph in (True, False):
+        nodes, edges, fields = device_state(wl, dev)
+        ro = Rollout(nodes, edges, fields, device_config(wl, synthetic_node_width=0.6), wl.dt,
+                     actuation=(wl.act_amp, wl.act_omega, wl.act_phase))
+        ro.run(12, use_graph=graph)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(_np(ro.position), o.position)
+        np.testing.assert_array_equal(_np(ro.velocity), o.velocity)
+        np.testing.assert_array_equal(_np(ro.f_grid), o.f_grid)
+        np.testing.assert_array_equal(_np(ro.fluid_velocity), o.fluid_velocity)
+
+
 def test_step_parity_wide_cluster(dev):
     """Ten coupled steps of a crowded 64^2 case whose filaments form one wide
     cluster: bit-identical to the float32 restatement."""

@@ -259,3 +259,140 @@ def test_dense_fluid_force_carries_rho():
     np.testing.assert_allclose(v_new[0], V - rho * V * g, rtol=1e-12, atol=1e-16)
     np.testing.assert_allclose(ustar.reshape(2, -1)[:, flat], 0.5 * np.outer(V * g / S, w),
                                rtol=1e-12, atol=1e-16)
+
+
+# --------------------------------------------------------------------------
+# Synthetic IB points (fluid.py:246-263, 364-368)
+# --------------------------------------------------------------------------
+def _chain_pairs(n):
+    return np.stack([np.arange(n - 1), np.arange(1, n)], axis=1).astype(np.int32)
+
+
+def _syn_points(pos, pairs, grid, lattice, width, dtype=np.float32):
+    pos = np.asarray(pos, dtype)
+    sx = lattice[1] / grid[0]
+    sy = lattice[0] / grid[1]
+    gp = pos * np.array([sx, sy], dtype)
+    return R.synthetic_ib_points(pos, pairs, grid, gp, width, 0.5 * (sx + sy))
+
+
+@pytest.mark.parametrize("direction", ["+x", "+y", "-x"])
+def test_synthetic_points_straight_chain_closed_form(direction):
+    """A straight chain of spacing 2 (every tangent norm exact): the + point of
+    each node lies at width * mean_scale along the tangent turned by +90
+    degrees, the - point opposite; the halves are + points then - points."""
+    grid, lattice, width = (32, 32), (64, 64), 0.75  # scale 2: offset 1.5 lattice cells
+    n = 5
+    t = {"+x": (1.0, 0.0), "+y": (0.0, 1.0), "-x": (-1.0, 0.0)}[direction]
+    pos = np.array([[8.0 + 2.0 * k * t[0], 9.0 + 2.0 * k * t[1]] for k in range(n)])
+    pts = _syn_points(pos, _chain_pairs(n), grid, lattice, width)
+    normal = np.array([-t[1], t[0]])  # +90 degrees
+    exp = np.concatenate([pos * 2.0 + 1.5 * normal, pos * 2.0 - 1.5 * normal])
+    np.testing.assert_array_equal(pts, exp.astype(np.float32))
+
+
+def test_synthetic_points_across_the_periodic_seam():
+    """Edge vectors are minimum images: a chain crossing x = W has the same
+    normal at every node as one that does not."""
+    grid, lattice, width = (32, 32), (32, 32), 0.5
+    pos = np.array([[28.0, 5.0], [30.0, 5.0], [0.0, 5.0], [2.0, 5.0]])
+    pts = _syn_points(pos, _chain_pairs(4), grid, lattice, width)
+    np.testing.assert_array_equal(pts[:4, 1], np.full(4, 5.5, np.float32))
+    np.testing.assert_array_equal(pts[4:, 1], np.full(4, 4.5, np.float32))
+    np.testing.assert_array_equal(pts[:4, 0], pos[:, 0].astype(np.float32))
+
+
+def test_synthetic_points_invariants_random_graph():
+    """Any graph: each offset has length width * mean_scale (up to the EPSIL in
+    the norm), is normal to the summed incident edge vectors (computed here in
+    float64 from the positions), lies on their left, and the - point mirrors
+    the + point through the node; an isolated node keeps both on the node."""
+    rng = np.random.default_rng(7)
+    grid, lattice, width = (48, 40), (96, 80), 0.6
+    n = 40
+    pos = rng.uniform(0, 40, (n, 2)) * np.array([1.0, 48 / 40])
+    pairs = np.array([[i, (i + 1 + rng.integers(0, 5)) % (n - 1)] for i in range(n - 1)], np.int32)
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    pts = _syn_points(pos, pairs, grid, lattice, width).astype(np.float64)
+    scale = np.array([lattice[1] / grid[0], lattice[0] / grid[1]])
+    ms = 0.5 * scale.sum()
+    gp = pos.astype(np.float32).astype(np.float64) * scale
+    size = np.array([grid[1], grid[0]], np.float64)
+    tan = np.zeros((n, 2))
+    for a, b in pairs:
+        d = pos[b] - pos[a]
+        d = (d + size / 2) % size - size / 2
+        tan[a] += d
+        tan[b] += d
+    off = pts[:n] - gp
+    # (float32 positions near 80: one ulp is 7.6e-6)
+    np.testing.assert_allclose(pts[n:] - gp, -off, atol=3e-5)
+    has = np.linalg.norm(tan, axis=1) > 1e-3
+    assert has.sum() >= n - 2
+    np.testing.assert_allclose(np.linalg.norm(off[has], axis=1), width * ms, atol=3e-5)
+    ut = tan[has] / np.linalg.norm(tan[has], axis=1, keepdims=True)
+    np.testing.assert_allclose((off[has] * ut).sum(1), 0.0, atol=3e-5)
+    cross = ut[:, 0] * off[has, 1] - ut[:, 1] * off[has, 0]
+    assert (cross > 0).all()
+    iso = np.setdiff1d(np.arange(n), pairs.ravel())
+    assert len(iso) > 0
+    np.testing.assert_array_equal(pts[:n][iso], pts[n:][iso])
+
+
+def test_synthetic_vanishing_width_equals_duplicated_nodes():
+    """A width far below the positions' resolution puts both points on the
+    node: the IB then equals the plain IB over the node set given twice, whose
+    two copies feel identical forces, so their mean is either copy's force --
+    bit for bit, lattice and u* included."""
+    grid = lattice = (32, 32)
+    p = _params(ibm_iterations=3)
+    rng = np.random.default_rng(3)
+    pos = np.array([[10.3 + 1.1 * k, 12.7 + 0.4 * k] for k in range(6)], np.float32)
+    vel = rng.normal(0, 0.04, (6, 2)).astype(np.float32)
+    f, _ = _rest_fluid(*lattice)
+    f = f.astype(np.float32)
+    v_s, u_s, f_s = R.immersed_boundary_interaction(
+        pos, vel, grid, f, 1.0, replace(p, synthetic_node_width=1e-30), pairs=_chain_pairs(6))
+    v_d, u_d, f_d = R.immersed_boundary_interaction(
+        np.concatenate([pos, pos]), np.concatenate([vel, vel]), grid, f, 1.0, p)
+    np.testing.assert_array_equal(v_s, v_d[:6])
+    np.testing.assert_array_equal(v_d[:6], v_d[6:])
+    np.testing.assert_array_equal(u_s, u_d)
+    np.testing.assert_array_equal(f_s, f_d)
+
+
+def test_synthetic_drag_is_the_mean_of_the_two_points():
+    """Points far enough apart that the + and - stencils never share a cell:
+    the synthetic IB equals the plain IB over the explicit 2N points (closed
+    form of a straight chain), and node k's drag is the mean of the forces at
+    points k and N + k."""
+    grid, lattice = (32, 32), (32, 32)
+    p = _params(ibm_iterations=4, ibm_relaxation=0.5)
+    width = 3.0  # 6 cells between the + and - rows, K = 4
+    n = 3
+    pos = np.array([[10.0 + 2.0 * k, 14.25] for k in range(n)], np.float32)
+    vel = np.array([[0.02, 0.05], [0.0, 0.04], [-0.01, 0.06]], np.float32)
+    # a shear flow, so that the + and - rows see different fluid
+    yy = np.arange(lattice[0])[:, None] + np.zeros(lattice)
+    u0 = np.stack([0.03 * np.sin(2 * np.pi * yy / lattice[0]), np.zeros(lattice)])
+    f = R.equilibrium(np.ones(lattice), u0).astype(np.float32)
+    v_s, u_s, f_s = R.immersed_boundary_interaction(
+        pos, vel, grid, f, 1.0, replace(p, synthetic_node_width=width), pairs=_chain_pairs(n))
+    pts = np.concatenate([pos + [0.0, 3.0], pos - [0.0, 3.0]]).astype(np.float32)
+    tr = R.IBTrace()
+    _, u_e, f_e = R.immersed_boundary_interaction(pts, np.concatenate([vel, vel]), grid, f, 1.0,
+                                                  p, trace=tr)
+    np.testing.assert_array_equal(u_s, u_e)
+    np.testing.assert_array_equal(f_s, f_e)
+    F = tr.node_force.astype(np.float64)
+    assert np.abs(F[:n] - F[n:]).max() > 1e-6  # the two sides differ: the mean matters
+    v_exp = vel - ((F[:n] + F[n:]) / 2.0) / p.node_mass
+    np.testing.assert_allclose(v_s, v_exp, rtol=0, atol=1e-7)
+
+
+def test_synthetic_needs_pairs():
+    grid = lattice = (16, 16)
+    f, _ = _rest_fluid(*lattice)
+    with pytest.raises(ValueError):
+        R.immersed_boundary_interaction(np.zeros((2, 2)) + 4.0, np.zeros((2, 2)), grid, f, 1.0,
+                                        _params(synthetic_node_width=0.5))
