@@ -66,7 +66,7 @@ typedef struct mc_params {
   double max_fluid_velocity;      /* config.py:25 */
   double config_dt;               /* config.py:32 (Cosserat inertia) */
   int32_t ibm_iterations;         /* config.py:35 */
-  double synthetic_node_width;    /* config.py:36 (only 0.0 supported) */
+  double synthetic_node_width;    /* config.py:36; > 0: synthetic IB points (fluid.py:246-263) */
   int32_t ibm_kernel_size;        /* config.py:37, 1..6 */
   double ibm_relaxation;          /* config.py:38 */
   double node_mass;               /* config.py:45 */
@@ -483,7 +483,9 @@ mc_status mc_steric_field(mc_plan* plan, const mc_params* params, const float* p
 
 /* fluid.py:202-386 immersed_boundary_interaction: returns node velocity,
  * fluid_velocity (u*) and streamed f.  pos_out receives the wrapped positions
- * (simulate.py:97); it may be NULL. */
+ * (simulate.py:97); it may be NULL.  With params->synthetic_node_width > 0 the
+ * IB runs on each node's two synthetic points (fluid.py:246-263) and the drag
+ * is the mean of their forces (fluid.py:364-368). */
 mc_status mc_ib_lbm_step(mc_plan* plan, const mc_params* params, const float* position,
                          const float* velocity, const float* f_in, float* velocity_out,
                          float* fluid_velocity_out, float* f_out, float* pos_out, double dt,
