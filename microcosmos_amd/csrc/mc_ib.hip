@@ -141,35 +141,62 @@ __global__ void __launch_bounds__(256) k_ib_claim(
   const bool tap = node_ok && t < K2;
   float prod = 0.0f;
   int prev = -1, comp = -1, inst = 0;
+  int sx = 0, sy = 0, W = 1, H = 1, g = -1;
+  const int dx = t / K, dy = t - dx * K;
+  float gx = 0.0f, gy = 0.0f, sxf = 0.0f, syf = 0.0f;
+  const InstGeo* gi = nullptr;
   if (node_ok) {
     inst = bd.node_inst[n];
-    const InstGeo* gi = &bd.geo[inst];
-    const int W = gi->Wf, H = gi->Hf;
+    gi = &bd.geo[inst];
+    W = gi->Wf;
+    H = gi->Hf;
     const bool minus = inc_off && n >= nodes;
     const int nd = minus ? n - nodes : n;  // the point's node
-    float gx = pos[2 * nd] * gi->scale_x;
-    float gy = pos[2 * nd + 1] * gi->scale_y;
+    gx = pos[2 * nd] * gi->scale_x;
+    gy = pos[2 * nd + 1] * gi->scale_y;
     if (inc_off) synthetic_point(c, *gi, pos, inc_off, inc, nd, minus, gx, gy);
-    const float sxf = floorf(gx) - (float)c.half;
-    const float syf = floorf(gy) - (float)c.half;
-    const int sx = imod(f2i(sxf), W);
-    const int sy = imod(f2i(syf), H);
+    sxf = floorf(gx) - (float)c.half;
+    syf = floorf(gy) - (float)c.half;
+    sx = imod(f2i(sxf), W);
+    sy = imod(f2i(syf), H);
     comp = comp_of_node[n];
-    if (tap) {
-      const int dx = t / K, dy = t - dx * K;
-      const int g = wrap1(sy + dy, H) * W + wrap1(sx + dx, W);
-      prev = atomicCAS(&owner[gi->fl_off + g], -1, comp);
-      // 1-D weights: tap t uses wx[t / K] and wy[t % K] (fluid.py:53-69)
-      const float wxv = peskin((sxf + (float)dx) - gx, c.half_width);
-      const float wyv = peskin((syf + (float)dy) - gy, c.half_width);
-      prod = wxv * wyv;
-    }
+    if (tap) g = wrap1(sy + dy, H) * W + wrap1(sx + dx, W);
     if (t == 0) {
       st_sx[n] = sx;
       st_sy[n] = sy;
       vib[2 * n] = (vel[2 * nd] * gi->scale_x) * c.dt;
       vib[2 * n + 1] = (vel[2 * nd + 1] * gi->scale_y) * c.dt;
     }
+  }
+  // Neighbouring nodes of a filament cover many of the same cells.  A tap
+  // whose cell is also covered by a tap of the same component in one of the
+  // three node groups before it in the wavefront leaves the claim to that tap
+  // (which claims it, or leaves it to an earlier one in turn): the cell's
+  // owner and links are the same, with fewer contended atomics.  The match is
+  // checked on the cell index itself.
+  bool dup = false;
+  if constexpr (L < 64) {
+    const int grp = lane / L;
+#pragma unroll
+    for (int m = 1; m <= 3 && m < 64 / L; ++m) {
+      const int k = grp - m;
+      const int kb = (k < 0 ? 0 : k) * L;
+      const int sxk = __shfl(sx, kb, 64), syk = __shfl(sy, kb, 64), ck = __shfl(comp, kb, 64);
+      int ddx = sx - sxk, ddy = sy - syk;  // the group's offset, wrapped to the nearer image
+      ddx = ddx > W / 2 ? ddx - W : (ddx < -(W / 2) ? ddx + W : ddx);
+      ddy = ddy > H / 2 ? ddy - H : (ddy < -(H / 2) ? ddy + H : ddy);
+      const int dxk = dx + ddx, dyk = dy + ddy;
+      const bool in = k >= 0 && tap && ck == comp && dxk >= 0 && dxk < K && dyk >= 0 && dyk < K;
+      const int gk = __shfl(g, in ? kb + dxk * K + dyk : lane, 64);
+      dup = dup || (in && gk == g);
+    }
+  }
+  if (tap) {
+    prev = dup ? -2 : atomicCAS(&owner[gi->fl_off + g], -1, comp);
+    // 1-D weights: tap t uses wx[t / K] and wy[t % K] (fluid.py:53-69)
+    const float wxv = peskin((sxf + (float)dx) - gx, c.half_width);
+    const float wyv = peskin((syf + (float)dy) - gy, c.half_width);
+    prod = wxv * wyv;
   }
   // the normalising sum over taps 0..K2-1 in order (every lane of the group)
   float total = 0.0f;
@@ -178,8 +205,9 @@ __global__ void __launch_bounds__(256) k_ib_claim(
   const unsigned long long won_lanes = __ballot(tap && prev == -1);
   const int prev_left = __shfl(prev, t > 0 ? lane - 1 : lane, 64);
   // a cell claimed by another component links the two (a repeat of the left
-  // neighbour tap's claimant is dropped; at most one link per tap)
-  const bool emit = tap && prev != -1 && prev != comp && (t == 0 || prev != prev_left);
+  // neighbour tap's claimant is dropped; at most one link per tap; a tap that
+  // left the claim to another (prev -2) neither wins nor links)
+  const bool emit = tap && prev >= 0 && prev != comp && (t == 0 || prev != prev_left);
   // one counter atomic per wavefront when its nodes share a bucket
   const int key = bucketed ? inst : 0;
   const int k0 = __shfl(key, 0, 64);
@@ -680,16 +708,21 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
   }
   __syncthreads();
   // a thread's first segment, when it has at most 8 entries, keeps their node
-  // ranks and weights in registers for the spreads
+  // ranks and weights in registers for the spreads, with its 1 / max(kw, 1)
+  // and its corrected velocity (only this thread writes it)
   constexpr int CSEG = 1;
   int cr[CSEG][8];
   float cw[CSEG][8];
   int clen[CSEG];
+  float civ[CSEG];
+  float2 cu[CSEG];
 #pragma unroll
   for (int k = 0; k < CSEG; ++k) {
     const int s = tid + k * bs;
     const int lo = s < ns ? sstart[s] : 0, len = s < ns ? sstart[s + 1] - lo : 0;
     clen[k] = len <= 8 ? len : -1;
+    civ[k] = s < ns ? sinv[s] : 0.0f;
+    cu[k] = s < ns ? suc[s] : make_float2(0.0f, 0.0f);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const bool in = q < len && len <= 8;
@@ -697,16 +730,18 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
       cw[k][q] = in ? gw[lo + q] : 0.0f;
     }
   }
+  // the node's tap weights, the same every iteration
+  float w[K2];
+  if (mine) {
+#pragma unroll
+    for (int t = 0; t < K2; ++t) w[t] = wt[t * nq + r];
+  }
   // multi-direct forcing iterations (fluid.py:292-316)
   for (int it = 0; it < c.iters; ++it) {
     if (mine) {
       float2 u[K2];
-      float w[K2];
 #pragma unroll
-      for (int t = 0; t < K2; ++t) {
-        u[t] = suc[seg[t]];
-        w[t] = wt[t * nq + r];
-      }
+      for (int t = 0; t < K2; ++t) u[t] = suc[seg[t]];
       float ux = 0.0f, uy = 0.0f;
 #pragma unroll
       for (int t = 0; t < K2; ++t) {
@@ -734,9 +769,8 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
           ax = ax + cv[q].x * cw[k][q];
           ay = ay + cv[q].y * cw[k][q];
         }
-      const float iv = sinv[s];
-      const float2 u = suc[s];
-      suc[s] = make_float2(u.x + ax * iv, u.y + ay * iv);
+      cu[k] = make_float2(cu[k].x + ax * civ[k], cu[k].y + ay * civ[k]);
+      suc[s] = cu[k];
     }
     for (int s = tid; s < ns; s += bs) {
       const int k = (s - tid) / bs;
@@ -797,12 +831,8 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
   // node drag (fluid.py:364-382) and the final wrap (simulate.py:97)
   if (mine) {
     float2 F[K2];
-    float w[K2];
 #pragma unroll
-    for (int t = 0; t < K2; ++t) {
-      F[t] = suc[seg[t]];
-      w[t] = wt[t * nq + r];
-    }
+    for (int t = 0; t < K2; ++t) F[t] = suc[seg[t]];
     float fx = 0.0f, fy = 0.0f;
 #pragma unroll
     for (int t = 0; t < K2; ++t) {
