@@ -169,27 +169,29 @@ __global__ void __launch_bounds__(256) k_ib_claim(
     }
   }
   // Neighbouring nodes of a filament cover many of the same cells.  A tap
-  // whose cell is also covered by a tap of the same component in one of the
-  // three node groups before it in the wavefront leaves the claim to that tap
+  // whose cell is also covered by a tap of the same component among the three
+  // node groups before it in the workgroup leaves the claim to that tap
   // (which claims it, or leaves it to an earlier one in turn): the cell's
   // owner and links are the same, with fewer contended atomics.  The match is
   // checked on the cell index itself.
+  constexpr int NPB = 256 / L;  // node groups per workgroup
+  __shared__ int4 grp_geo[NPB];
+  const int grp = threadIdx.x / L;
+  if (t == 0) grp_geo[grp] = make_int4(sx, sy, comp, 0);
+  __syncthreads();
   bool dup = false;
-  if constexpr (L < 64) {
-    const int grp = lane / L;
 #pragma unroll
-    for (int m = 1; m <= 3 && m < 64 / L; ++m) {
-      const int k = grp - m;
-      const int kb = (k < 0 ? 0 : k) * L;
-      const int sxk = __shfl(sx, kb, 64), syk = __shfl(sy, kb, 64), ck = __shfl(comp, kb, 64);
-      int ddx = sx - sxk, ddy = sy - syk;  // the group's offset, wrapped to the nearer image
-      ddx = ddx > W / 2 ? ddx - W : (ddx < -(W / 2) ? ddx + W : ddx);
-      ddy = ddy > H / 2 ? ddy - H : (ddy < -(H / 2) ? ddy + H : ddy);
-      const int dxk = dx + ddx, dyk = dy + ddy;
-      const bool in = k >= 0 && tap && ck == comp && dxk >= 0 && dxk < K && dyk >= 0 && dyk < K;
-      const int gk = __shfl(g, in ? kb + dxk * K + dyk : lane, 64);
-      dup = dup || (in && gk == g);
-    }
+  for (int m = 1; m <= 3 && m < NPB; ++m) {
+    const int k = grp - m;
+    if (k < 0 || !tap) break;
+    const int4 q = grp_geo[k];
+    int ddx = sx - q.x, ddy = sy - q.y;  // the group's offset, wrapped to the nearer image
+    ddx = ddx > W / 2 ? ddx - W : (ddx < -(W / 2) ? ddx + W : ddx);
+    ddy = ddy > H / 2 ? ddy - H : (ddy < -(H / 2) ? ddy + H : ddy);
+    const int dxk = dx + ddx, dyk = dy + ddy;
+    if (q.z == comp && dxk >= 0 && dxk < K && dyk >= 0 && dyk < K &&
+        wrap1(q.y + dyk, H) * W + wrap1(q.x + dxk, W) == g)
+      dup = true;
   }
   if (tap) {
     prev = dup ? -2 : atomicCAS(&owner[gi->fl_off + g], -1, comp);
