@@ -135,7 +135,7 @@ struct mc_plan {
   InstGeo* geo_d = nullptr;
   double geo_mfv = 0.0;              // max_fluid_velocity the device table holds
   bool geo_umax_set = false;         // u_max of the device table is valid
-  long long grid_cells = 0, lat_cells = 0, mask_words = 0;
+  long long grid_cells = 0, lat_cells = 0;
   int rows = 0, cols = 0, lbm_blocks = 0, max_w = 0, max_h = 0, min_hf = 0, min_wf = 0;
   BatchDev bd{};
   std::vector<void*> allocs;
@@ -604,7 +604,7 @@ mc_status build_instances(mc_plan* p) {
   p->geo_h.assign(n, InstGeo{});
   p->node_inst_h.assign(p->N, 0);
   std::vector<int> row_inst, col_inst, lbm_blk(n + 1, 0), ctile(n + 1, 0);
-  long long go = 0, fo = 0, mo = 0;
+  long long go = 0, fo = 0;
   int r0 = 0, c0 = 0, b0 = 0;
   p->min_hf = 1 << 30;
   p->min_wf = 1 << 30;
@@ -623,7 +623,6 @@ mc_status build_instances(mc_plan* p) {
     g.mean_scale = (float)(0.5 * ((double)h.Wf / (double)h.H + (double)h.Hf / (double)h.W));
     g.grid_off = go;
     g.fl_off = fo;
-    g.mask_off = mo;
     g.row0 = r0;
     g.col0 = c0;
     g.lbm_blk0 = b0;
@@ -636,7 +635,6 @@ mc_status build_instances(mc_plan* p) {
                  : 0ull;
     go += (long long)h.H * h.W;
     fo += hwf;
-    mo += (hwf + 31) / 32;
     for (int y = 0; y < h.H; ++y) row_inst.push_back(i);
     for (int x = 0; x < h.W; ++x) col_inst.push_back(i);
     r0 += h.H;
@@ -653,7 +651,6 @@ mc_status build_instances(mc_plan* p) {
   }
   p->grid_cells = go;
   p->lat_cells = fo;
-  p->mask_words = mo;
   p->rows = r0;
   p->cols = c0;
   p->lbm_blocks = b0;
@@ -1402,7 +1399,7 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
     if ((st = alloc_wide_ctl(p)) != MC_OK) return bail(st);
   }
   if ((st = p->alloc(&w.force, 2 * hwf, "force U")) != MC_OK) return bail(st);
-  if ((st = p->alloc(&w.mask, (size_t)p->mask_words, "U mask")) != MC_OK) return bail(st);
+  if ((st = p->alloc(&w.mask, (size_t)p->lat_cells, "U flags")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->f_tmp, 9 * hwf, "f ping-pong")) != MC_OK) return bail(st);
   if ((st = p->alloc(&p->tick, 1, "tick")) != MC_OK) return bail(st);
   p->deferring = false;

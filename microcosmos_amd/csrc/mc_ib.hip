@@ -334,7 +334,7 @@ struct ClusterArgs {
   float *vel_io, *pos_out, *force;
   float* syn_force;  // synthetic points: per-point drag out (no node update), else NULL
   int *cell_seg, *owner;
-  uint32_t* mask;
+  uint8_t* mask;
   BatchDev bd;
   unsigned char* gscratch;
   size_t bpn;        // global scratch bytes per node
@@ -513,7 +513,7 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
   const size_t HW = (size_t)H * W;
   const float umax = gi.umax;
   float* frc = a.force + 2 * gi.fl_off;
-  uint32_t* msk = a.mask + gi.mask_off;
+  uint8_t* msk = a.mask + gi.fl_off;
   int* own = a.owner + gi.fl_off;
   const float* fin = a.f_in + 9 * gi.fl_off;
   const int tid = threadIdx.x, bs = blockDim.x;
@@ -826,7 +826,7 @@ __device__ __forceinline__ void solve_small(const ClusterArgs& a, const InstGeo&
     suc[s] = make_float2(fx, fy);
     frc[cell] = fx;
     frc[HW + cell] = fy;
-    atomicOr(&msk[cell >> 5], 1u << (cell & 31));
+    msk[cell] = 1;
     own[cell] = -1;
   }
   __syncthreads();
@@ -869,7 +869,7 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
   const size_t HW = (size_t)H * W;
   const float umax = gi.umax;
   float* frc = a.force + 2 * gi.fl_off;
-  uint32_t* msk = a.mask + gi.mask_off;
+  uint8_t* msk = a.mask + gi.fl_off;
   int* cseg = a.cell_seg + gi.fl_off;
   int* own = a.owner + gi.fl_off;
   const int tid = threadIdx.x, bs = blockDim.x;
@@ -1019,7 +1019,7 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
     L.suc[s] = make_float2(fx, fy);
     frc[cell] = fx;
     frc[HW + cell] = fy;
-    atomicOr(&msk[cell >> 5], 1u << (cell & 31));
+    msk[cell] = 1;
   }
   __syncthreads();
   for (int r = tid; r < nq; r += bs) {
@@ -1406,7 +1406,7 @@ __device__ __forceinline__ bool cluster_wide(const ClusterArgs& a, int h, int H,
     float* frc = a.force + 2 * gi.fl_off;
     frc[cell] = fx;
     frc[HW + cell] = fy;
-    atomicOr(&a.mask[gi.mask_off + (cell >> 5)], 1u << (cell & 31));
+    a.mask[gi.fl_off + cell] = 1;
   }
   if (!gsync()) return false;
   // node drag (fluid.py:364-382) and the final wrap (simulate.py:97)

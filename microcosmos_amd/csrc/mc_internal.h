@@ -57,7 +57,6 @@ struct InstGeo {
   float mean_scale;           // float32 of 0.5 * (scale_x + scale_y) in double
   long long grid_off;         // first cell of the (H, W) grids
   long long fl_off;           // first cell of the lattice
-  long long mask_off;         // first U-mask word (ceil(Hf * Wf / 32) words, flat cell index)
   int row0, col0;             // first global steric row / column
   int lbm_blk0;               // first LBM workgroup
   int node0, nodes;           // node range
@@ -150,7 +149,7 @@ struct LbmArgs {
   BatchDev bd;
   const float* fin;
   float *fout, *ustar;
-  uint32_t* mask;
+  uint8_t* mask;
   const float* force;
   float op, om, cp, cm;
   int force_clamp;
@@ -372,7 +371,7 @@ struct IBWork {
   bool dirty;                  // a call stopped between launches: idle state to restore
   long long cells;             // lattice cells over all instances
   float *force;                // (2, Hf, Wf) per instance: IB force, valid on the support U
-  uint32_t* mask;              // U bits by flat lattice cell index, per instance
+  uint8_t* mask;               // U flag per lattice cell (1: the force is valid), lattice layout
   // wide clusters (more than IB_WIDE_MIN nodes) are solved together by a
   // launch of wide_wg workgroups after the cluster solve (0: no instance is
   // large enough to form one)
@@ -443,10 +442,10 @@ hipError_t launch_com(int n_groups, const int* off, const float* pos, int H, int
 hipError_t launch_track(const TrackWork& t, const float* pos, int mode, hipStream_t s);
 // LBM
 LbmArgs make_lbm_args(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
-                      float* ustar, uint32_t* mask, const float* force);
+                      float* ustar, uint8_t* mask, const float* force);
 // start / stop (optional): events recorded with the dispatch, timing the kernel
 hipError_t launch_lbm(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
-                      float* ustar, uint32_t* mask, const float* force, hipStream_t s,
+                      float* ustar, uint8_t* mask, const float* force, hipStream_t s,
                       hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 }  // namespace mc

@@ -14,7 +14,8 @@
 // streaming pass with no redundant collision.
 //
 // HBM traffic per cell: 36 B read + 36 B written (f) + 8 B written (u*)
-// (+ 1/8 B of the U bitmask, + 8 B only on the ~6 % of cells in U).
+// (+ 1 B of the U flags, + 8 B read and 1 B written only on the ~6 % of
+// cells in U).
 #pragma once
 #include "mc_common.h"
 #include "mc_internal.h"
@@ -65,7 +66,7 @@ __device__ __forceinline__ void lbm_cell(const LbmArgs& a, int b, int t) {
   float* fout = a.fout + 9 * g.fl_off;
   float* ustar = a.ustar + 2 * g.fl_off;
   const float* force = a.force + 2 * g.fl_off;
-  uint32_t* mask = a.mask + g.mask_off;
+  uint8_t* mask = a.mask + g.fl_off;
   const int yp = (y + 1 == H) ? 0 : y + 1;
   const int ym = (y == 0) ? H - 1 : y - 1;
   const size_t cell = (size_t)y * W + x;
@@ -74,10 +75,8 @@ __device__ __forceinline__ void lbm_cell(const LbmArgs& a, int b, int t) {
   float f[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) f[i] = fmax_nan(__builtin_nontemporal_load(&fin[i * HW + cell]), MC_EPSIL);
-  // U bits are indexed by the flat cell index: a 32-cell word lies inside one
-  // wavefront, which reads it before its first lane clears it
-  uint32_t* mw = mask + (ci >> 5);
-  const uint32_t m = *mw;
+  // the cell's U flag (set by the IB solve that wrote its force)
+  const uint8_t m = mask[cell];
   // density and clamped velocity (fluid.py:136-142,268-274); `cl1` tells
   // whether the clamp changed u
   float r = f[0];
@@ -92,7 +91,7 @@ __device__ __forceinline__ void lbm_cell(const LbmArgs& a, int b, int t) {
   uy = uy * cl1;
   float Fx, Fy;
   bool zero_force = false;
-  if ((m >> (ci & 31)) & 1u) {
+  if (m) {
     Fx = force[cell];
     Fy = force[HW + cell];
   } else if (cl1 == 1.0f) {
@@ -115,7 +114,7 @@ __device__ __forceinline__ void lbm_cell(const LbmArgs& a, int b, int t) {
       Fy = Fy * fc;
     }
   }
-  if ((ci & 31) == 0) *mw = 0u;  // consume the U bitmask word (every lane has read it)
+  if (m) mask[cell] = 0;  // consume the U flag
   // u* = u + 0.5 F / rho (fluid.py:346); with F = +0 the quotient is +0
   const float usx = zero_force ? ux + 0.0f : ux + (0.5f * Fx) / rho;
   const float usy = zero_force ? uy + 0.0f : uy + (0.5f * Fy) / rho;

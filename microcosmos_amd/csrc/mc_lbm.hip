@@ -19,7 +19,7 @@ __global__ void __launch_bounds__(LBM_THREADS) k_lbm(LbmArgs a) {
 }  // namespace
 
 LbmArgs make_lbm_args(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
-                      float* ustar, uint32_t* mask, const float* force) {
+                      float* ustar, uint8_t* mask, const float* force) {
   LbmArgs a;
   a.bd = b;
   a.fin = f_in;
@@ -36,7 +36,7 @@ LbmArgs make_lbm_args(const Consts& c, const BatchDev& b, const float* f_in, flo
 }
 
 hipError_t launch_lbm(const Consts& c, const BatchDev& b, const float* f_in, float* f_out,
-                      float* ustar, uint32_t* mask, const float* force, hipStream_t s,
+                      float* ustar, uint8_t* mask, const float* force, hipStream_t s,
                       hipEvent_t start, hipEvent_t stop) {
   const LbmArgs a = make_lbm_args(c, b, f_in, f_out, ustar, mask, force);
   if (start && stop) {
