@@ -657,8 +657,8 @@ __global__ void __launch_bounds__(256) k_rod(RodArgs a) {
 }
 
 // Rod components in workgroups [0, rod_blocks), then an LBM pass in 64-cell
-// workgroups (a quarter of an LBM_THREADS block each, one wavefront, so the
-// U-mask words stay within a wavefront).  rod_blocks is a multiple of 8:
+// workgroups (a quarter of an LBM_THREADS block each, one wavefront; more
+// cells per thread measured slower).  rod_blocks is a multiple of 8:
 // workgroup i runs on XCD i % 8, and the LBM part keeps the standalone
 // kernel's XCD-banded order.
 template <bool CHAINS>
