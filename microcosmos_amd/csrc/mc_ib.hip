@@ -1471,7 +1471,7 @@ template <int K>
 __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_solve(ClusterArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int wsum[32];
-  __shared__ int s_ns, s_seg, s_base;
+  __shared__ int s_ns, s_seg, s_base, s_nm, s_nq;
   constexpr int K2 = K * K;
   const int root = blockIdx.x;
   const int tid = threadIdx.x, bs = blockDim.x;
@@ -1520,24 +1520,47 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_solve(ClusterArgs a
       links_done(a);
       return;
     }
-    // members (ascending component ids: ascending smallest node ids) and their
-    // node count; every member's label and the root's count for the wide solve
-    int carry = 0, nodes = 0;
+    // members (ascending component ids: ascending smallest node ids), each
+    // with its first position in the node list, and the node count: listed by
+    // wavefront 0 alone (ballots and shuffles, no workgroup barrier per chunk)
+    // per member: first position in the node list, written over the node
+    // counts (member k's slot k is at most its component index, whose count
+    // wavefront 0 has read by then)
+    int* moff = cnn;
+    if (tid < 64) {
+      int carry = 0, nodes = 0;
+      for (int b0 = 0; b0 < nc; b0 += 64) {
+        const int i = b0 + tid;
+        const bool in = i < nc && uf_find(par, i) == me;
+        const unsigned long long bal = __ballot(in);
+        const int cnt = in ? cnn[i] : 0;
+        int inc = cnt;  // inclusive prefix of the node counts over the lanes
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const int u = __shfl_up(inc, d, 64);
+          if (tid >= d) inc += u;
+        }
+        if (in) {
+          const int pos = carry + __popcll(bal & ((1ull << tid) - 1ull));
+          mem[pos] = i;
+          moff[pos] = nodes + inc - cnt;
+        }
+        carry += __popcll(bal);
+        nodes += __shfl(inc, 63, 64);
+      }
+      if (tid == 0) {
+        s_nm = carry;
+        s_nq = nodes;
+      }
+    }
+    // this thread's member bits, for the labels every member gets at the end
     for (int b0 = 0, k = 0; b0 < nc; b0 += bs, ++k) {
       const int i = b0 + tid;
-      const bool in = i < nc && uf_find(par, i) == me;
-      const int2 v = make_int2(in ? 1 : 0, in ? cnn[i] : 0);
-      int2 tot;
-      const int2 inc = block_inclusive_scan2(v, wsum, &tot);
-      if (in) {
-        mem[carry + inc.x - 1] = i;
-        mbits |= 1u << k;
-      }
-      carry += tot.x;
-      nodes += tot.y;
+      if (i < nc && uf_find(par, i) == me) mbits |= 1u << k;
     }
-    const int nm = carry;
-    nq = nodes;
+    __syncthreads();
+    const int nm = s_nm;
+    nq = s_nq;
     if (a.wide_wg > 0 && nq > a.wide_min) {
       // solved by the wide workgroups
       write_labels(a, gi, root, mbits, nq);
@@ -1550,30 +1573,21 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_solve(ClusterArgs a
     }
     in_bytes = al16(4 * (size_t)nq);
     stage_lds = 8 * in_bytes <= a.lds_bytes && 16 * (size_t)ncp + 2 * in_bytes <= a.lds_bytes;
-    if (!stage_lds && tid == 0) s_base = atomicAdd(&a.ictl->scratch_top, nq);
-    __syncthreads();
-    if (!stage_lds) glayout(a.gscratch + (size_t)s_base * a.bpn, nq, nq * K2, G, X);
+    if (!stage_lds) {
+      if (tid == 0) s_base = atomicAdd(&a.ictl->scratch_top, nq);
+      __syncthreads();
+      glayout(a.gscratch + (size_t)s_base * a.bpn, nq, nq * K2, G, X);
+    }
     nodes_in = stage_lds ? reinterpret_cast<int*>(smem + a.lds_bytes - in_bytes) : G.tmp;
     ranked = stage_lds ? reinterpret_cast<int*>(smem + a.lds_bytes - 2 * in_bytes) : G.nodes;
-    // members in order: their node ranges concatenated (member k's first
-    // list position into par[k], then one node per thread)
-    carry = 0;
-    for (int b0 = 0; b0 < nm; b0 += bs) {
-      const int i = b0 + tid;
-      const int cnt = i < nm ? cnn[mem[i]] : 0;
-      int tot;
-      const int inc = block_inclusive_scan(cnt, wsum, &tot);
-      if (i < nm) par[i] = carry + inc - cnt;
-      carry += tot;
-    }
-    __syncthreads();
+    // members in order: their node ranges concatenated, one node per thread
     for (int i = tid; i < nq; i += bs) {
       int lo = 0, hi = nm - 1;  // last member whose first position is <= i
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (par[mid] <= i) lo = mid; else hi = mid - 1;
+        if (moff[mid] <= i) lo = mid; else hi = mid - 1;
       }
-      const int f = cnf[mem[lo]], k = i - par[lo];
+      const int f = cnf[mem[lo]], k = i - moff[lo];
       nodes_in[i] = f >= 0 ? f + k : a.comp_nodes[~f + k];
     }
     __syncthreads();
