@@ -127,7 +127,9 @@ __device__ __forceinline__ void steric_force(const Consts& c, const InstGeo& gi,
                                              const float2* pw = nullptr, int pw0 = 0) {
   const int H = gi.H, W = gi.W;
   const float* V = V_all + gi.grid_off;
-  const float2 pg = *reinterpret_cast<const float2*>(pred_w + 2 * g);
+  // the node's own prediction: from the staged window when there is one (an
+  // LDS read instead of a dependent global load before the field samples)
+  const float2 pg = pw ? pw[g - pw0] : *reinterpret_cast<const float2*>(pred_w + 2 * g);
   const float px = pg.x, py = pg.y;
   const float gx = (sample_bilinear_wrap(V, H, W, py, px + 0.5f) -
                     sample_bilinear_wrap(V, H, W, py, px - 0.5f)) / 1.0f;
