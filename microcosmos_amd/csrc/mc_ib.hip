@@ -360,7 +360,7 @@ struct ClusterArgs {
 // cluster's global scratch; scell / srho / su0 always live in global scratch.
 struct CLayout {
   int *nodes, *sstart, *eidx, *tapseg, *tmp;
-  float *wt, *sinv;
+  float *wt, *sinv, *gw;  // gw: the entries' weights in segment order
   float2 *corr, *vib, *suc;
 };
 
@@ -387,6 +387,7 @@ __device__ __forceinline__ void glayout(unsigned char* base, int nq, int ne, CLa
   L.eidx = (int*)take(4 * (size_t)ne);
   L.tapseg = (int*)take(4 * (size_t)ne);
   L.wt = (float*)take(4 * (size_t)ne);
+  L.gw = (float*)take(4 * (size_t)ne);
   L.corr = (float2*)take(8 * (size_t)nq);
   L.vib = (float2*)take(8 * (size_t)nq);
   L.sinv = (float*)take(4 * (size_t)ne);
@@ -406,6 +407,7 @@ __device__ __forceinline__ size_t llayout(unsigned char* base, int nq, int ne, i
   L.eidx = (int*)take(4 * (size_t)ne);
   L.tapseg = (int*)take(4 * (size_t)ne);
   L.wt = (float*)take(4 * (size_t)ne);
+  L.gw = (float*)take(4 * (size_t)ne);
   L.corr = (float2*)take(8 * (size_t)nq);
   L.vib = (float2*)take(8 * (size_t)nq);
   L.sinv = (float*)take(4 * (size_t)ns);
@@ -880,26 +882,50 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
   for (int i = tid; i < nq; i += bs) L.nodes[i] = nodes[i];
   for (int s = tid; s <= ns; s += bs) L.sstart[s] = 0;
   __syncthreads();
-  // won taps claim segments (arbitrary numbering); their cells return to idle
+  // won taps claim segments (arbitrary numbering: a node's won taps take
+  // consecutive ids); their cells return to idle
   for (int r = tid; r < nq; r += bs) {
     const int n = L.nodes[r];
     const unsigned long long won = a.st_won[n];
     const int sx = a.st_sx[n], sy = a.st_sy[n];
+    int seg = won ? atomicAdd(s_seg, __popcll(won)) : 0;
 #pragma unroll
     for (int t = 0; t < K2; ++t) {
       if ((won >> t) & 1ull) {
         const int g = wrap1(sy + t % K, H) * W + wrap1(sx + t / K, W);
-        const int seg = atomicAdd(s_seg, 1);
         cseg[g] = seg;
         X.scell[seg] = g;
         own[g] = -1;
-        // the cell's rho and clamped u (fluid.py:268-276)
-        float rho;
-        float2 u;
-        cell_state(a.f_in, gi, g, rho, u);
-        X.srho[seg] = rho;
-        X.su0[seg] = u;
-        L.suc[seg] = u;
+        ++seg;
+      }
+    }
+  }
+  __syncthreads();
+  // the cells' rho and clamped u (fluid.py:268-276): four segments per thread
+  // with their population loads in flight together
+  {
+    const float* fin = a.f_in + 9 * gi.fl_off;
+    for (int s0 = tid; s0 < ns; s0 += 4 * bs) {
+      int cellv[4];
+      float fp[4][9];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) cellv[k] = s0 + k * bs < ns ? X.scell[s0 + k * bs] : 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) fp[k][i] = s0 + k * bs < ns ? fin[i * HW + cellv[k]] : 1.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int sg = s0 + k * bs;
+        if (sg >= ns) continue;
+        float f[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f[i] = fmax_nan(fp[k][i], MC_EPSIL);
+        float rho, ux, uy;
+        lbm_macro(f, umax, rho, ux, uy);
+        X.srho[sg] = rho;
+        X.su0[sg] = make_float2(ux, uy);
+        L.suc[sg] = make_float2(ux, uy);
       }
     }
   }
@@ -949,19 +975,55 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
     L.vib[r] = vb;
   }
   __syncthreads();
+  // each segment's entries ascending by entry id (by node rank), the
+  // kernel-weight fold in that order and 1/max(kw, 1) (fluid.py:123-133,
+  // 289-290); then eidx holds each entry's node rank and gw its weight, in
+  // segment order, for the spreads
   for (int s = tid; s < ns; s += bs) {
-    const int lo = L.sstart[s], hi = L.sstart[s + 1];
-    for (int j = lo + 1; j < hi; ++j) {
-      const int v = L.eidx[j];
-      int k = j - 1;
-      while (k >= lo && L.eidx[k] > v) {
-        L.eidx[k + 1] = L.eidx[k];
-        --k;
-      }
-      L.eidx[k + 1] = v;
-    }
+    const int lo = L.sstart[s], hi = L.sstart[s + 1], len = hi - lo;
     float kw = 0.0f;
-    for (int j = lo; j < hi; ++j) kw = kw + L.wt[L.eidx[j]];
+    if (len <= 16) {
+      int v[16];
+      float w[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = k < len ? L.eidx[lo + k] : 0x7fffffff;
+      if (len <= 8) {
+        int u[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) u[k] = v[k];
+        sort_regs<8>(u);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = u[k];
+      } else {
+        sort_regs<16>(v);
+      }
+#pragma unroll
+      for (int k = 0; k < 16; ++k) w[k] = k < len ? L.wt[v[k]] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if (k < len) {
+          kw = kw + w[k];
+          L.eidx[lo + k] = v[k] / K2;
+          L.gw[lo + k] = w[k];
+        }
+    } else {
+      for (int j = lo + 1; j < hi; ++j) {
+        const int v = L.eidx[j];
+        int k = j - 1;
+        while (k >= lo && L.eidx[k] > v) {
+          L.eidx[k + 1] = L.eidx[k];
+          --k;
+        }
+        L.eidx[k + 1] = v;
+      }
+      for (int j = lo; j < hi; ++j) {
+        const int e = L.eidx[j];
+        const float w = L.wt[e];
+        kw = kw + w;
+        L.eidx[j] = e / K2;
+        L.gw[j] = w;
+      }
+    }
     L.sinv[s] = 1.0f / fmax_nan(kw, 1.0f);
   }
   __syncthreads();
@@ -985,13 +1047,25 @@ __device__ __forceinline__ void cluster_general(const ClusterArgs& a, const Inst
     __syncthreads();
     for (int s = tid; s < ns; s += bs) {
       float ax = 0.0f, ay = 0.0f;
-      const int hi = L.sstart[s + 1];
-      for (int j = L.sstart[s]; j < hi; ++j) {
-        const int e = L.eidx[j];
-        const float w = L.wt[e];
-        const float2 cc = L.corr[e / K2];
-        ax = ax + cc.x * w;
-        ay = ay + cc.y * w;
+      const int lo = L.sstart[s], hi = L.sstart[s + 1];
+      // eight entries' loads in flight at a time, summed in order
+      for (int j0 = lo; j0 < hi; j0 += 8) {
+        int rk[8];
+        float wv[8];
+        float2 cv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          rk[k] = j0 + k < hi ? L.eidx[j0 + k] : 0;
+          wv[k] = j0 + k < hi ? L.gw[j0 + k] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cv[k] = L.corr[rk[k]];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (j0 + k < hi) {
+            ax = ax + cv[k].x * wv[k];
+            ay = ay + cv[k].y * wv[k];
+          }
       }
       const float iv = L.sinv[s];
       const float2 u = L.suc[s];
@@ -1815,9 +1889,10 @@ __global__ void k_ib_reset(long long HW, int C, int* __restrict__ owner, int* __
 }  // namespace
 
 size_t ib_scratch_bytes_per_node(int kmax) {
-  // glayout(nq, nq * K2): nq * (28 + 40 K2) + 4 + 13 arrays x 15 B of padding, nq >= 1
+  // glayout(nq, nq * K2): nq * (24 + 48 K2) + 4 + 14 arrays x 15 B of padding,
+  // covered per node for nq >= 1
   const size_t K2 = (size_t)kmax * kmax;
-  return 28 + 40 * K2 + 4 + 16 * 16;
+  return 24 + 48 * K2 + 4 + 16 * 16;
 }
 
 size_t ib_wide_scratch_bytes(int N, int kmax) {
