@@ -1323,6 +1323,25 @@ mc_status create_plan(mc_plan** out, int32_t n_inst, const int32_t* shape, const
   w.cells = p->lat_cells;
   w.cl_bpn = ib_scratch_bytes_per_node(max_kernel_size);
   w.lds_bytes = ib_lds_bytes(p->lds_max);
+  w.comp_order = nullptr;
+  if (p->n_inst > 1 && p->L.num_comp > 0) {
+    // the solve workgroups of the densest instances (stencil taps per lattice
+    // cell) are dispatched first: their large clusters then start at once
+    // instead of in the last wave of workgroups
+    std::vector<int> io(p->n_inst);
+    for (int i = 0; i < p->n_inst; ++i) io[i] = i;
+    auto dens = [&](int i) {
+      return (double)p->inst[i].nodes / ((double)p->inst[i].Hf * (double)p->inst[i].Wf);
+    };
+    std::stable_sort(io.begin(), io.end(), [&](int x, int y) { return dens(x) > dens(y); });
+    std::vector<int> order;
+    order.reserve(p->L.num_comp);
+    for (int i : io)
+      for (int c = 0; c < p->geo_h[i].ncomp; ++c) order.push_back(p->geo_h[i].comp0 + c);
+    int* od;
+    if ((st = upload(p, &od, order, "solve order")) != MC_OK) return bail(st);
+    w.comp_order = od;
+  }
   w.dirty = false;
   const size_t taps = (size_t)max_kernel_size * max_kernel_size;
   const size_t C = (size_t)w.C;

@@ -335,6 +335,7 @@ struct ClusterArgs {
   float* syn_force;  // synthetic points: per-point drag out (no node update), else NULL
   int *cell_seg, *owner;
   uint8_t* mask;
+  const int* comp_order;  // workgroup -> component (NULL: identity)
   BatchDev bd;
   unsigned char* gscratch;
   size_t bpn;        // global scratch bytes per node
@@ -1547,7 +1548,7 @@ __global__ void __launch_bounds__(cluster_threads<K>()) k_ib_solve(ClusterArgs a
   __shared__ int wsum[32];
   __shared__ int s_ns, s_seg, s_base, s_nm, s_nq;
   constexpr int K2 = K * K;
-  const int root = blockIdx.x;
+  const int root = a.comp_order ? a.comp_order[blockIdx.x] : (int)blockIdx.x;
   const int tid = threadIdx.x, bs = blockDim.x;
   const int inst = a.bd.comp_inst[root];  // a cluster never spans instances
   const InstGeo& gi = a.bd.geo[inst];
@@ -2004,6 +2005,7 @@ hipError_t launch_ib(const Consts& c, const BatchDev& b, IBWork& w, const float*
   a.cell_seg = w.cell_seg;
   a.owner = w.owner;
   a.mask = w.mask;
+  a.comp_order = w.comp_order;
   a.bd = bp;
   a.gscratch = w.cl_scratch;
   a.bpn = w.cl_bpn;

@@ -368,6 +368,9 @@ struct IBWork {
   unsigned char* cl_scratch;   // cl_bpn bytes per node, cluster-major
   size_t cl_bpn;
   size_t lds_bytes;            // dynamic LDS of the cluster solve
+  // solve workgroup -> component (NULL: identity): the instances with the most
+  // stencil taps per lattice cell, where the large clusters form, first
+  const int* comp_order;
   bool dirty;                  // a call stopped between launches: idle state to restore
   long long cells;             // lattice cells over all instances
   float *force;                // (2, Hf, Wf) per instance: IB force, valid on the support U
