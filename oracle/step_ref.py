@@ -54,6 +54,13 @@ Parity status per function (how each is pinned; see tests/test_oracle_*.py):
                             (1 - r S_M)^n through inv_kw, both clamps through the
                             rank-one recurrence, mass / scale / dt / axis quirk,
                             rho in the force, injected momentum
+  synthetic_ib_points and the 2N-point IB with the averaged drag
+                         -- closed forms of straight chains (+x, +y, -x) and of
+                            a chain across the periodic seam; offset length,
+                            normality and side on a random branched graph; a
+                            vanishing width equal to the plain IB over the nodes
+                            given twice; separated points equal to the plain IB
+                            over the explicit 2N points with the mean drag
   LBM                    -- feq moments (float64), mass conservation, uniform fixed
                             point, exact streaming permutation, Guo momentum input,
                             Taylor-Green viscous decay exp(-2 nu k^2 t)
